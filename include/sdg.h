@@ -1,0 +1,218 @@
+/*
+ * sdg.h -- C-ABI of libsdg_hip.so, the MI355X (gfx950) discontinuous Galerkin residual and
+ * explicit SSP-RK time-stepping engine.
+ *
+ * The reference has no FFI: its operator boundary is the C++ template Solver<SimulationControl>
+ * (src/Solver/SolveControl.cpp:377-435), driven by System<SimulationControl>::solve
+ * (src/Utils/SystemControl.cpp:159-195). Every entry point below replaces one member of that
+ * template (cited per function), so a C++ System<SC> adaptor, a ctypes test harness or a
+ * torch.autograd wrapper can drive the same hot path.
+ *
+ * Conventions
+ *   - every function returns SDG_OK (0) or a negative SDG_E_* code and never throws;
+ *     sdg_last_error() returns the message of the last failure in a context (or globally).
+ *   - the library owns all device memory; callers own host buffers, which are borrowed for the
+ *     duration of the call only.
+ *   - one host thread per context; distinct contexts are independent (no process-global
+ *     physical state, unlike the reference's inline static members, PhysicalModel.cpp:28-29).
+ *   - coefficient arrays in the "reference layout" are, per element, the NV x N_b matrix of
+ *     modal coefficients stored column-major (variable index fastest), elements in mesh order,
+ *     exactly as Eigen stores variable_basis_function_coefficient_ (SolveControl.cpp:47-57).
+ *   - conserved variables are (rho, rho*u, rho*v[, rho*w], rho*E) (VariableConvertor.cpp:28-48);
+ *     primitive variables are (rho, u, v[, w], T) (VariableConvertor.cpp:118-139).
+ */
+#ifndef SDG_H_
+#define SDG_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDG_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------------------------- */
+#define SDG_OK 0
+#define SDG_E_INVALID (-1)     /* bad argument or descriptor                              */
+#define SDG_E_UNSUPPORTED (-2) /* valid request this build does not implement             */
+#define SDG_E_DEVICE (-3)      /* HIP runtime error (no device, launch failure, ...)      */
+#define SDG_E_NOMEM (-4)       /* host or device allocation failed                        */
+#define SDG_E_STATE (-5)       /* call out of order (e.g. step before a state was set)    */
+
+/* ---- enumerations mirroring src/Utils/Enum.cpp:22-233 ----------------------------------- */
+enum sdg_element { SDG_ELEM_LINE = 1, SDG_ELEM_TRIANGLE = 2, SDG_ELEM_QUADRANGLE = 3, SDG_ELEM_HEXAHEDRON = 6 };
+enum sdg_equation { SDG_EQ_COMPRESSIBLE_EULER = 0, SDG_EQ_COMPRESSIBLE_NS = 1 };
+enum sdg_convective_flux {
+  SDG_FLUX_CENTRAL = 0,
+  SDG_FLUX_LAX_FRIEDRICHS = 1,
+  SDG_FLUX_HLLC = 2,
+  SDG_FLUX_ROE = 3,
+  SDG_FLUX_EXACT = 4
+};
+enum sdg_viscous_flux { SDG_VISC_NONE = 0, SDG_VISC_BR1 = 1, SDG_VISC_BR2 = 2 };
+enum sdg_time_integration { SDG_RK_FORWARD_EULER = 0, SDG_RK_HEUN2 = 1, SDG_RK_SSPRK3 = 2 };
+enum sdg_eos { SDG_EOS_IDEAL_GAS = 0, SDG_EOS_WEAK_COMPRESSIBLE = 1 };
+enum sdg_transport { SDG_TRANSPORT_NONE = 0, SDG_TRANSPORT_CONSTANT = 1, SDG_TRANSPORT_SUTHERLAND = 2 };
+enum sdg_precision { SDG_PREC_FP64 = 0, SDG_PREC_MIXED = 1 };
+
+/* Runtime copy of the reference's compile-time SimulationControl
+ * (SimulationControl.cpp:1197-1279) plus the System setters that carry physical constants
+ * (SystemControl.cpp:79-109). gamma = 1.4 and Pr = 0.71 are constants, as in
+ * PhysicalModel.cpp:45,91,107. */
+typedef struct sdg_problem_desc {
+  int abi_version;        /* SDG_ABI_VERSION                                              */
+  int equation;           /* enum sdg_equation                                            */
+  int convective_flux;    /* enum sdg_convective_flux                                     */
+  int viscous_flux;       /* enum sdg_viscous_flux (NONE for Euler)                       */
+  int time_integration;   /* enum sdg_time_integration                                    */
+  int equation_of_state;  /* enum sdg_eos                                                 */
+  int transport;          /* enum sdg_transport (NONE for Euler)                          */
+  int precision;          /* enum sdg_precision                                           */
+  double cp;              /* specific heat at constant pressure                           */
+  double cv;              /* specific heat at constant volume                             */
+  double dynamic_viscosity;
+  double reference_sound_speed; /* weakly-compressible EOS only                           */
+  double reference_density;     /* weakly-compressible EOS only                           */
+} sdg_problem_desc;
+
+/* Structured periodic mesh on an axis-aligned box: the project-side replacement for the gmsh
+ * transfinite + setPeriodic generators of the examples (e.g. periodic_2d_ceuler.cpp:63-94).
+ * element: SDG_ELEM_QUADRANGLE (2-D) or SDG_ELEM_HEXAHEDRON (3-D). Every direction is periodic
+ * and needs at least 2 cells. */
+typedef struct sdg_structured_desc {
+  int element;            /* enum sdg_element                                             */
+  int polynomial_order;   /* P = 1..5                                                     */
+  int n[3];               /* cells per direction (n[2] ignored in 2-D)                    */
+  double lo[3];           /* box lower corner                                             */
+  double hi[3];           /* box upper corner                                             */
+} sdg_structured_desc;
+
+/* Sizes of a mesh / discretisation (ElementTrait, SimulationControl.cpp:1083-1108). */
+typedef struct sdg_mesh_info {
+  int dimension;          /* D                                                            */
+  int element;            /* enum sdg_element                                             */
+  int polynomial_order;   /* P                                                            */
+  int64_t elements;       /* E                                                            */
+  int64_t faces;          /* interior faces F (all faces are interior on a periodic mesh) */
+  int basis;              /* N_b                                                          */
+  int quadrature;         /* N_q   (volume points, Gauss order 2P)                        */
+  int faces_per_element;  /* N_f                                                          */
+  int face_quadrature;    /* N_fq  (points per face, Gauss order 2P+1)                    */
+  int affine;             /* 1 when every element has a constant Jacobian                 */
+} sdg_mesh_info;
+
+/* Operand tables a mesh exports (all FP64 unless noted). Shapes:
+ *   BASIS_VALUE        N_q  x N_b      modal_value_                  BasisFunction.cpp:205-211
+ *   BASIS_GRADIENT     (N_q*D) x N_b   modal_gradient_value_ (row q*D+d)       :218-226
+ *   BASIS_FACE_VALUE   (N_f*N_fq) x N_b modal_adjacency_value_                 :181-186
+ *   BASIS_LSQ_INVERSE  N_b x N_b       modal_least_squares_inverse_            :217
+ *   QUAD_WEIGHT        N_q             ElementQuadrature::weight_  Quadrature.cpp:37-54
+ *   FACE_QUAD_WEIGHT   N_fq
+ *   QUAD_COORD         E x N_q x D     physical volume quadrature points (Geometry.cpp:57)
+ *   JACOBIAN_DET_W     E x N_q         |J| w                                     Geometry.cpp:61
+ *   JACOBIAN_TIW       E x N_q x D*D   (J^T)^-1 |J| w, column-major D x D        Geometry.cpp:63
+ *   MASS_INVERSE       E x N_b x N_b   local_mass_matrix_inverse_                Geometry.cpp:88-100
+ *   MIN_EDGE           E               minimum_edge_                             Geometry.cpp:29-42
+ *   FACE_NORMAL        F x N_fq x D    outward from the left parent              Geometry.cpp:102-168
+ *   FACE_JACOBIAN_W    F x N_fq        |J_f| w                                   Geometry.cpp:69-86
+ *   FACE_COORD         F x N_fq x D    physical face quadrature points (left parent's view)
+ *   FACE_PARENT (int32)     F x 2      left, right element
+ *   FACE_LOCAL  (int32)     F x 2      local face index in left, right parent
+ *   FACE_SEQUENCE (int32)   F x N_fq   right-parent point index of left point j
+ *                                      (getAdjacencyElementQuadratureSequence,
+ *                                       SimulationControl.cpp:381-523)
+ *   FACE_ROTATION (int32)   F          rotation index as defined by Adjacency.cpp:169-171
+ */
+enum sdg_mesh_field {
+  SDG_FIELD_BASIS_VALUE = 0,
+  SDG_FIELD_BASIS_GRADIENT = 1,
+  SDG_FIELD_BASIS_FACE_VALUE = 2,
+  SDG_FIELD_BASIS_LSQ_INVERSE = 3,
+  SDG_FIELD_QUAD_WEIGHT = 4,
+  SDG_FIELD_FACE_QUAD_WEIGHT = 5,
+  SDG_FIELD_QUAD_COORD = 6,
+  SDG_FIELD_JACOBIAN_DET_W = 7,
+  SDG_FIELD_JACOBIAN_TIW = 8,
+  SDG_FIELD_MASS_INVERSE = 9,
+  SDG_FIELD_MIN_EDGE = 10,
+  SDG_FIELD_FACE_NORMAL = 11,
+  SDG_FIELD_FACE_JACOBIAN_W = 12,
+  SDG_FIELD_FACE_COORD = 13,
+  SDG_FIELD_FACE_PARENT = 14,
+  SDG_FIELD_FACE_LOCAL = 15,
+  SDG_FIELD_FACE_SEQUENCE = 16,
+  SDG_FIELD_FACE_ROTATION = 17,
+  SDG_FIELD_QUAD_POINT = 18,      /* N_q x D reference-element volume points */
+  SDG_FIELD_FACE_QUAD_POINT = 19, /* N_fq x (D-1) reference face points      */
+  SDG_FIELD_COUNT_ = 20
+};
+
+typedef struct sdg_mesh sdg_mesh;
+typedef struct sdg_ctx sdg_ctx;
+
+/* ---- library ---------------------------------------------------------------------------- */
+/* ABI version compiled into the library. */
+int sdg_abi_version(void);
+/* Last error message of a context, or of the last failing context-less call when ctx is NULL. */
+const char* sdg_last_error(const sdg_ctx* ctx);
+
+/* ---- mesh and operands (host only; no device work) --------------------------------------- */
+/* Replaces Mesh<SC>::initializeMesh + readMeshElement (ReadControl.cpp:286-335) for structured
+ * periodic meshes: elements, faces with periodic pairing (Adjacency.cpp:185-328), Jacobians,
+ * mass-matrix inverses, normals, basis and quadrature tables. */
+int sdg_mesh_create_structured(const sdg_structured_desc* desc, sdg_mesh** out);
+void sdg_mesh_destroy(sdg_mesh* mesh);
+int sdg_mesh_get_info(const sdg_mesh* mesh, sdg_mesh_info* info);
+/* Number of scalar entries of an exported field (0 for an unknown field). */
+int64_t sdg_mesh_field_size(const sdg_mesh* mesh, int field);
+/* Copy an exported field into out (double* or int32_t* per the table above); `count` is the
+ * capacity of out in entries and must be >= sdg_mesh_field_size. */
+int sdg_mesh_export(const sdg_mesh* mesh, int field, void* out, int64_t count);
+
+/* ---- solver context (device) ------------------------------------------------------------- */
+/* Replaces Solver<SC> construction + the device-resident part of initializeSolver
+ * (InitialCondition.cpp:151-186): uploads operands to HBM in the structure-of-arrays layout.
+ * The mesh may be destroyed after this call. */
+int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh, sdg_ctx** out);
+void sdg_destroy(sdg_ctx* ctx);
+
+/* Initial condition from primitive values at the volume quadrature points (E x N_q x (D+2),
+ * primitive index fastest): conserved = f(primitive) (VariableConvertor.cpp:341-366), then the
+ * unweighted least-squares projection C = U_q V (V^T V)^-1 (InitialCondition.cpp:85-104). */
+int sdg_set_initial_primitive(sdg_ctx* ctx, const double* primitive_at_quadrature);
+/* Set / get the modal coefficients (reference layout, E x N_b x NV with NV fastest). */
+int sdg_set_state(sdg_ctx* ctx, const double* coefficients);
+int sdg_get_state(sdg_ctx* ctx, double* coefficients);
+
+/* CFL time step from the current state: min over elements and points of
+ * CFL * minEdge / ((|u| + c) (P+1)^2) (TimeIntegration.cpp:104-131). */
+int sdg_compute_dt(sdg_ctx* ctx, double cfl, double* dt);
+
+/* One evaluation of the spatial residual R(U) of the current state, before M^-1: the passes
+ * calculateGardientQuadrature .. calculateResidual of stepSolver (TimeIntegration.cpp:340-346).
+ * residual: reference layout (E x N_b x NV). Leaves the state unchanged. */
+int sdg_residual(sdg_ctx* ctx, double* residual);
+/* NS only: the lifted total-gradient coefficients w of the current state, (NV*D) x N_b per
+ * element column-major, row var*D+dim (variable_gradient_basis_function_coefficient_,
+ * TimeIntegration.cpp:200-228). */
+int sdg_get_gradient(sdg_ctx* ctx, double* gradient);
+
+/* Advance nsteps explicit steps of size dt: stepSolver (TimeIntegration.cpp:327-350) repeated,
+ * including the residual monitor calculateRelativeError (TimeIntegration.cpp:279-324).
+ * relative_error (may be NULL): NV doubles, the monitor after the last step taken.
+ * nan_flag (may be NULL): set to 1 if the monitor was all-NaN after some step, in which case
+ * stepping stops after that step (SystemControl.cpp:185-191); steps_taken (may be NULL). */
+int sdg_step(sdg_ctx* ctx, double dt, int nsteps, double* relative_error, int* nan_flag, int* steps_taken);
+
+/* Device timing of the last sdg_step call in milliseconds (HIP events on the context stream). */
+int sdg_last_step_ms(const sdg_ctx* ctx, double* ms);
+/* Device synchronisation of the context stream. */
+int sdg_synchronize(sdg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDG_H_ */

@@ -1,0 +1,833 @@
+// oracle.cpp -- CPU ORACLE (test infrastructure only).
+//
+// A plain C++/OpenMP restatement of the reference's DG residual and explicit time stepping
+// (SubrosaDG src/Solver/*), kept in the reference's own pass structure and per-element
+// array-of-structures layout so that it can serve as the parity checker of the HIP engine and as
+// the timed CPU baseline. Nothing in the product links or calls this file: only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+//
+// Per-element matrices are stored column-major, as the reference's Eigen members are
+// (SolveControl.cpp:45-174); operand tables arrive through oracle_create in the row-major layout
+// sdg_mesh_export writes and are transposed here.
+//
+// Pinning (what makes this restatement trustworthy; the checks live in tests/):
+//   quadrature, basis, geometry    -> closed forms and numpy (tests/test_kit.py)
+//   pointwise physics and fluxes   -> consistency, conservation, rotation invariance, Roe vs an
+//                                     eigen-decomposition |A| built with numpy, supersonic
+//                                     upwinding, stationary contact (tests/test_oracle_flux.py)
+//   whole residual / step          -> free-stream preservation, discrete conservation, exact
+//                                     lifted gradient of continuous polynomials, convergence
+//                                     order on a smooth Euler solution, viscous decay rate,
+//                                     SSP-RK3 on a linear ODE (tests/test_oracle_scheme.py)
+//   parity unpinned                -> the Exact (weakly-compressible) flux: only consistency
+//                                     and antisymmetry are checked; no independent solution
+//                                     of its Riemann problem is committed.
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <vector>
+
+#include "../include/sdg.h"
+
+namespace {
+
+constexpr double kGamma = 1.4;        // PhysicalModel.cpp:45
+constexpr double kPrandtl = 0.71;     // PhysicalModel.cpp:91,107
+constexpr double kSutherland = 110.4 / 273.15;  // PhysicalModel.cpp:109
+
+struct Physics {
+  int eos = SDG_EOS_IDEAL_GAS;
+  int transport = SDG_TRANSPORT_NONE;
+  double cp = 0, cv = 0, mu = 0, kappa = 0, c0 = 0, rho0 = 0, padd = 0;
+
+  // ThermodynamicModel<Constant> (PhysicalModel.cpp:24-38)
+  double internalEnergyFromTemperature(double t) const { return cv * t; }
+  double temperatureFromInternalEnergy(double e) const { return e / cv; }
+  // EquationOfState (PhysicalModel.cpp:40-75)
+  double pressure(double rho, double e) const {
+    if (eos == SDG_EOS_IDEAL_GAS) return (kGamma - 1.0) * rho * e;
+    return c0 * c0 * (rho - rho0) + padd;
+  }
+  double soundSpeed(double rho, double p) const {
+    if (eos == SDG_EOS_IDEAL_GAS) return std::sqrt(kGamma * p / rho);
+    return c0;
+  }
+  // TransportModel (PhysicalModel.cpp:77-122)
+  double sutherlandRatio(double t) const {
+    return std::sqrt(t * t * t) * (1.0 + kSutherland) / (t + kSutherland);
+  }
+  double viscosity(double t) const { return transport == SDG_TRANSPORT_SUTHERLAND ? mu * sutherlandRatio(t) : mu; }
+  double conductivity(double t) const {
+    return transport == SDG_TRANSPORT_SUTHERLAND ? kappa * sutherlandRatio(t) : kappa;
+  }
+};
+
+// column-major helpers: C (m x n) (+)= A (m x k) * B (k x n)
+inline void gemm(double* c, const double* a, const double* b, int m, int k, int n, bool accumulate, double alpha = 1.0) {
+  for (int j = 0; j < n; j++) {
+    double* cj = c + static_cast<size_t>(j) * m;
+    if (!accumulate)
+      for (int i = 0; i < m; i++) cj[i] = 0.0;
+    for (int l = 0; l < k; l++) {
+      const double blj = alpha * b[l + static_cast<size_t>(j) * k];
+      const double* al = a + static_cast<size_t>(l) * m;
+      for (int i = 0; i < m; i++) cj[i] += al[i] * blj;
+    }
+  }
+}
+// C (m x n) (+)= A (m x k) * B^T where B is (n x k)
+inline void gemmBt(double* c, const double* a, const double* b, int m, int k, int n, bool accumulate) {
+  for (int j = 0; j < n; j++) {
+    double* cj = c + static_cast<size_t>(j) * m;
+    if (!accumulate)
+      for (int i = 0; i < m; i++) cj[i] = 0.0;
+    for (int l = 0; l < k; l++) {
+      const double blj = b[j + static_cast<size_t>(l) * n];
+      const double* al = a + static_cast<size_t>(l) * m;
+      for (int i = 0; i < m; i++) cj[i] += al[i] * blj;
+    }
+  }
+}
+
+struct Oracle {
+  // sizes
+  int dim = 0, nv = 0, nb = 0, nq = 0, nf = 0, nfq = 0, naq = 0, order = 0;
+  int64_t ne = 0, nface = 0;
+  int equation = 0, flux = 0, viscous = 0, rk = 0;
+  Physics phys;
+  // operands (column-major like Eigen)
+  std::vector<double> V;    // N_q x N_b
+  std::vector<double> dV;   // (N_q D) x N_b
+  std::vector<double> Vf;   // N_aq x N_b
+  std::vector<double> lsq;  // N_b x N_b
+  std::vector<double> detjw, tiw, minv, min_edge;
+  std::vector<int32_t> fparent, flocal, fseq;
+  std::vector<double> fnormal, fjw;
+  // per-element solver data (SolveControl.cpp:45-174), flattened element-major
+  std::vector<double> coeff, coeff_last, vol_quad, adj_quad, residual;
+  std::vector<double> vg_coeff, vg_quad, vg_adj, vg_res;           // volume gradient
+  std::vector<double> ig_adj, ig_res, ig_coeff, g_coeff;           // interface / total gradient (NS)
+  int nig = 0;  // interface-gradient residual blocks per element: 1 (BR1) or N_f (BR2)
+  std::vector<double> rel_err;
+  bool has_state = false;
+
+  size_t szC() const { return static_cast<size_t>(nv) * nb; }
+  size_t szG() const { return static_cast<size_t>(nv) * dim * nb; }
+  bool ns() const { return equation == SDG_EQ_COMPRESSIBLE_NS; }
+
+  // --- variable conversions (VariableConvertor.cpp:315-339, 574-620) ---
+  // computational: (rho, u.., e, p)
+  void computational(const double* u, double* w) const {
+    const double rho = u[0];
+    w[0] = rho;
+    double v2 = 0.0;
+    for (int d = 0; d < dim; d++) {
+      w[1 + d] = u[1 + d] / rho;
+      v2 += w[1 + d] * w[1 + d];
+    }
+    const double e = u[dim + 1] / rho - v2 / 2.0;
+    w[dim + 1] = e;
+    w[dim + 2] = phys.pressure(rho, e);
+  }
+  double velocitySquared(const double* w) const {
+    double s = 0.0;
+    for (int d = 0; d < dim; d++) s += w[1 + d] * w[1 + d];
+    return s;
+  }
+  // primitive gradient from conserved gradient g (row var*D+dim); out: (NV*D): rho, u (D x D
+  // column-major, (a,b) = du_b/dx_a), T
+  void primitiveGradient(const double* w, const double* u, const double* g, double* pg) const {
+    const double rho = w[0];
+    const double* drho = g;
+    for (int a = 0; a < dim; a++) pg[a] = drho[a];
+    double vg[9];
+    for (int b = 0; b < dim; b++)
+      for (int a = 0; a < dim; a++) {
+        vg[a + dim * b] = (g[(1 + b) * dim + a] - drho[a] * w[1 + b]) / rho;
+        pg[(1 + b) * dim + a] = vg[a + dim * b];
+      }
+    const double etot = u[dim + 1] / rho;
+    for (int a = 0; a < dim; a++) {
+      double vgu = 0.0;
+      for (int b = 0; b < dim; b++) vgu += vg[a + dim * b] * w[1 + b];
+      const double de = (g[(dim + 1) * dim + a] - drho[a] * etot) / rho - vgu;
+      pg[(dim + 1) * dim + a] = phys.temperatureFromInternalEnergy(de);
+    }
+  }
+
+  // --- fluxes (ConvectiveFlux.cpp:27-439, ViscousFlux.cpp:58-170) ---
+  void normalFlux(const double* n, const double* w, double* f) const {  // :59-91
+    const double rho = w[0];
+    double un = 0.0;
+    for (int d = 0; d < dim; d++) un += w[1 + d] * n[d];
+    const double p = w[dim + 2];
+    f[0] = rho * un;
+    for (int d = 0; d < dim; d++) f[1 + d] = rho * un * w[1 + d] + p * n[d];
+    const double et = w[dim + 1] + velocitySquared(w) / 2.0;
+    f[dim + 1] = (rho * et + p) * un;
+  }
+  void convectiveFlux(const double* n, const double* ul, const double* wl, const double* ur, const double* wr,
+                      double* h) const {
+    double fl[5], fr[5];
+    if (flux == SDG_FLUX_CENTRAL) {  // :93-104
+      normalFlux(n, wl, fl);
+      normalFlux(n, wr, fr);
+      for (int v = 0; v < nv; v++) h[v] = (fl[v] + fr[v]) / 2.0;
+    } else if (flux == SDG_FLUX_LAX_FRIEDRICHS) {  // :106-134
+      normalFlux(n, wl, fl);
+      normalFlux(n, wr, fr);
+      double unl = 0, unr = 0;
+      for (int d = 0; d < dim; d++) {
+        unl += wl[1 + d] * n[d];
+        unr += wr[1 + d] * n[d];
+      }
+      const double cl = phys.soundSpeed(wl[0], wl[dim + 2]);
+      const double cr = phys.soundSpeed(wr[0], wr[dim + 2]);
+      const double lam = std::max(std::fabs(unl) + cl, std::fabs(unr) + cr);
+      for (int v = 0; v < nv; v++) h[v] = ((fl[v] + fr[v]) - lam * (ur[v] - ul[v])) / 2.0;
+    } else if (flux == SDG_FLUX_HLLC) {  // :136-238
+      const double rl = wl[0], rr = wr[0], pl = wl[dim + 2], pr = wr[dim + 2];
+      double unl = 0, unr = 0;
+      for (int d = 0; d < dim; d++) {
+        unl += wl[1 + d] * n[d];
+        unr += wr[1 + d] * n[d];
+      }
+      const double cl = phys.soundSpeed(rl, pl), cr = phys.soundSpeed(rr, pr);
+      const double rbar = (rl + rr) / 2.0, cbar = (cl + cr) / 2.0;
+      const double ps = std::max(0.0, (pl + pr) / 2.0 - (unr - unl) * rbar * cbar);
+      const double sl =
+          unl - cl * (ps <= pl ? 1.0 : std::sqrt(1.0 + (kGamma + 1.0) * (ps / pl - 1.0) / 2.0 / kGamma));
+      if (sl >= 0.0) {
+        normalFlux(n, wl, h);
+        return;
+      }
+      const double sr =
+          unr + cr * (ps <= pr ? 1.0 : std::sqrt(1.0 + (kGamma + 1.0) * (ps / pr - 1.0) / 2.0 / kGamma));
+      if (sr <= 0.0) {
+        normalFlux(n, wr, h);
+        return;
+      }
+      const double sm = (pr - pl + rl * unl * (sl - unl) - rr * unr * (sr - unr)) / (rl * (sl - unl) - rr * (sr - unr));
+      double us[5];
+      if (sm >= 0.0) {
+        normalFlux(n, wl, fl);
+        us[0] = rl * (sl - unl) / (sl - sm);
+        for (int d = 0; d < dim; d++) us[1 + d] = ((sl - unl) * rl * wl[1 + d] + (ps - pl) * n[d]) / (sl - sm);
+        us[dim + 1] = ((sl - unl) * rl * (wl[dim + 1] + velocitySquared(wl) / 2.0) - pl * unl + ps * sm) / (sl - sm);
+        for (int v = 0; v < nv; v++) h[v] = fl[v] + sl * (us[v] - ul[v]);
+      } else {
+        normalFlux(n, wr, fr);
+        us[0] = rr * (sr - unr) / (sr - sm);
+        for (int d = 0; d < dim; d++) us[1 + d] = ((sr - unr) * rr * wr[1 + d] + (ps - pr) * n[d]) / (sr - sm);
+        us[dim + 1] = ((sr - unr) * rr * (wr[dim + 1] + velocitySquared(wr) / 2.0) - pr * unr + ps * sm) / (sr - sm);
+        for (int v = 0; v < nv; v++) h[v] = fr[v] + sr * (us[v] - ur[v]);
+      }
+    } else if (flux == SDG_FLUX_ROE) {  // :240-350
+      normalFlux(n, wl, fl);
+      normalFlux(n, wr, fr);
+      const double sql = std::sqrt(wl[0]), sqr = std::sqrt(wr[0]);
+      const double sqs = sql + sqr;
+      const double rho = std::sqrt(wl[0] * wr[0]);
+      double vel[3];
+      for (int d = 0; d < dim; d++) vel[d] = (sql * wl[1 + d] + sqr * wr[1 + d]) / sqs;
+      const double hl = wl[dim + 1] * kGamma + velocitySquared(wl) / 2.0;
+      const double hr = wr[dim + 1] * kGamma + velocitySquared(wr) / 2.0;
+      const double htot = (sql * hl + sqr * hr) / sqs;
+      double v2 = 0.0, un = 0.0;
+      for (int d = 0; d < dim; d++) {
+        v2 += vel[d] * vel[d];
+        un += vel[d] * n[d];
+      }
+      const double e = (htot - v2 / 2.0) / kGamma;
+      const double p = phys.pressure(rho, e);
+      const double c = phys.soundSpeed(rho, p);
+      double dw[8];
+      for (int i = 0; i < dim + 3; i++) dw[i] = wr[i] - wl[i];
+      double dun = 0.0, vdv = 0.0;
+      for (int d = 0; d < dim; d++) {
+        dun += dw[1 + d] * n[d];
+        vdv += vel[d] * dw[1 + d];
+      }
+      const double delta = c / 20.0;
+      const double a1 = std::fabs(un - c) > delta ? std::fabs(un - c) : ((un - c) * (un - c) + delta * delta) / (2.0 * delta);
+      const double a3 = std::fabs(un + c) > delta ? std::fabs(un + c) : ((un + c) * (un + c) + delta * delta) / (2.0 * delta);
+      const double dp = dw[dim + 2], drho = dw[0];
+      double col[5][5] = {};
+      const double s0 = a1 * (dp - rho * c * dun) / (2.0 * c * c);
+      col[0][0] = 1.0 * s0;
+      for (int d = 0; d < dim; d++) col[0][1 + d] = (vel[d] - c * n[d]) * s0;
+      col[0][dim + 1] = (htot - c * un) * s0;
+      const double s1 = std::fabs(un) * (drho - dp / (c * c));
+      col[1][0] = 1.0 * s1;
+      for (int d = 0; d < dim; d++) col[1][1 + d] = vel[d] * s1;
+      col[1][dim + 1] = v2 / 2.0 * s1;
+      if (dim >= 2) {
+        const double s2 = std::fabs(un) * rho;
+        col[2][0] = 0.0 * s2;
+        for (int d = 0; d < dim; d++) col[2][1 + d] = (dw[1 + d] - dun * n[d]) * s2;
+        col[2][dim + 1] = (vdv - un * dun) * s2;
+      }
+      const double s3 = a3 * (dp + rho * c * dun) / (2.0 * c * c);
+      col[dim + 1][0] = 1.0 * s3;
+      for (int d = 0; d < dim; d++) col[dim + 1][1 + d] = (vel[d] + c * n[d]) * s3;
+      col[dim + 1][dim + 1] = (htot + c * un) * s3;
+      for (int v = 0; v < nv; v++) {
+        double s = 0.0;
+        for (int k = 0; k < nv; k++) s += col[k][v];  // rowwise().sum()
+        h[v] = ((fl[v] + fr[v]) - s) / 2.0;
+      }
+    } else {  // Exact (weakly compressible), :352-414
+      const double c = phys.soundSpeed(0.0, 0.0);
+      double unl = 0, unr = 0;
+      for (int d = 0; d < dim; d++) {
+        unl += wl[1 + d] * n[d];
+        unr += wr[1 + d] * n[d];
+      }
+      const double rho = std::sqrt(wl[0] * wr[0] * std::exp((unl - unr) / c));
+      const double un = (unl + unr) / 2.0 + std::log(wl[0] / wr[0]) * c / 2.0;
+      double we[8];
+      const double* ws = un < 0.0 ? wr : wl;
+      const double uns = un < 0.0 ? unr : unl;
+      const double e = ws[dim + 1] * ws[0] / rho;
+      we[0] = rho;
+      for (int d = 0; d < dim; d++) we[1 + d] = ws[1 + d] + (un - uns) * n[d];
+      we[dim + 1] = e;
+      we[dim + 2] = phys.pressure(rho, e);
+      normalFlux(n, we, h);
+    }
+  }
+  // viscous raw flux F_v (D x NV, column-major), ViscousFlux.cpp:58-103
+  void viscousRawFlux(const double* w, const double* pg, double* fv) const {
+    const double t = phys.temperatureFromInternalEnergy(w[dim + 1]);
+    const double mu = phys.viscosity(t);
+    const double* vg = pg + dim;  // D x D column-major, (a,b) = du_b/dx_a
+    double tr = 0.0;
+    for (int a = 0; a < dim; a++) tr += vg[a + dim * a];
+    for (int a = 0; a < dim; a++) fv[a] = 0.0;
+    double tau[9];
+    for (int b = 0; b < dim; b++)
+      for (int a = 0; a < dim; a++)
+        tau[a + dim * b] = mu * (vg[a + dim * b] + vg[b + dim * a]) - (a == b ? 2.0 / 3.0 * mu * tr * 1.0 : 2.0 / 3.0 * mu * tr * 0.0);
+    for (int b = 0; b < dim; b++)
+      for (int a = 0; a < dim; a++) fv[a + dim * (1 + b)] = tau[a + dim * b];
+    const double k = phys.conductivity(t);
+    const double* tg = pg + (dim + 1) * dim;
+    for (int a = 0; a < dim; a++) {
+      double s = 0.0;
+      for (int b = 0; b < dim; b++) s += tau[a + dim * b] * w[1 + b];
+      fv[a + dim * (dim + 1)] = s + k * tg[a];
+    }
+  }
+  void viscousNormalFlux(const double* n, const double* w, const double* pg, double* f) const {  // :115-124
+    double fv[15];
+    viscousRawFlux(w, pg, fv);
+    for (int v = 0; v < nv; v++) {
+      double s = 0.0;
+      for (int a = 0; a < dim; a++) s += fv[a + dim * v] * n[a];
+      f[v] = s;
+    }
+  }
+
+  int accumulate(int s) const { return s * nfq; }  // getElementAccumulateAdjacencyQuadratureNumber
+
+  // --- passes of stepSolver ---
+  void gradientQuadrature() {  // SpatialDiscrete.cpp:294-346
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; e++) {
+      std::vector<double> uq(static_cast<size_t>(nv) * nq);
+      gemmBt(uq.data(), &coeff[e * szC()], V.data(), nv, nb, nq, false);
+      double* out = &vg_quad[static_cast<size_t>(e) * nv * dim * dim * nq];
+      const int rows = nv * dim;
+      for (int q = 0; q < nq; q++) {
+        const double* G = &tiw[(static_cast<size_t>(e) * nq + q) * dim * dim];
+        for (int k = 0; k < nv; k++)
+          for (int b = 0; b < dim; b++)
+            for (int a = 0; a < dim; a++)
+              out[(k * dim + a) + static_cast<size_t>(rows) * (q * dim + b)] = uq[k + static_cast<size_t>(nv) * q] * G[a + dim * b];
+      }
+    }
+  }
+  void faceTraces(int64_t f, double* ul, double* ur) const {
+    const int32_t L = fparent[2 * f], R = fparent[2 * f + 1];
+    const int sl = flocal[2 * f], sr = flocal[2 * f + 1];
+    // NV x N_fq = C * Vf[face rows]^T
+    for (int j = 0; j < nfq; j++)
+      for (int v = 0; v < nv; v++) {
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < nb; i++) {
+          a += coeff[L * szC() + v + nv * i] * Vf[(accumulate(sl) + j) + static_cast<size_t>(naq) * i];
+          b += coeff[R * szC() + v + nv * i] * Vf[(accumulate(sr) + j) + static_cast<size_t>(naq) * i];
+        }
+        ul[v + nv * j] = a;
+        ur[v + nv * j] = b;
+      }
+  }
+  void adjacencyGradientQuadrature() {  // SpatialDiscrete.cpp:844-911
+    const int rows = nv * dim;
+#pragma omp parallel for schedule(static)
+    for (int64_t f = 0; f < nface; f++) {
+      std::vector<double> ul(static_cast<size_t>(nv) * nfq), ur(static_cast<size_t>(nv) * nfq);
+      faceTraces(f, ul.data(), ur.data());
+      const int32_t L = fparent[2 * f], R = fparent[2 * f + 1];
+      const int sl = flocal[2 * f], sr = flocal[2 * f + 1];
+      for (int j = 0; j < nfq; j++) {
+        const int k = fseq[f * nfq + j];
+        const double* n = &fnormal[(static_cast<size_t>(f) * nfq + j) * dim];
+        const double jw = fjw[f * nfq + j];
+        double tmp[15];
+        for (int v = 0; v < nv; v++)
+          for (int a = 0; a < dim; a++) tmp[v * dim + a] = (n[a] * (ul[v + nv * j] + ur[v + nv * k]) / 2.0) * jw;
+        double* cl = &vg_adj[static_cast<size_t>(L) * rows * naq + static_cast<size_t>(rows) * (accumulate(sl) + j)];
+        double* cr = &vg_adj[static_cast<size_t>(R) * rows * naq + static_cast<size_t>(rows) * (accumulate(sr) + k)];
+        for (int r = 0; r < rows; r++) {
+          cl[r] = tmp[r];
+          cr[r] = -tmp[r];
+        }
+        if (ns()) {
+          for (int v = 0; v < nv; v++)
+            for (int a = 0; a < dim; a++) tmp[v * dim + a] = (n[a] * (ur[v + nv * k] - ul[v + nv * j]) / 2.0) * jw;
+          double* il = &ig_adj[static_cast<size_t>(L) * rows * naq + static_cast<size_t>(rows) * (accumulate(sl) + j)];
+          double* ir = &ig_adj[static_cast<size_t>(R) * rows * naq + static_cast<size_t>(rows) * (accumulate(sr) + k)];
+          for (int r = 0; r < rows; r++) {
+            il[r] = tmp[r];
+            ir[r] = tmp[r];
+          }
+        }
+      }
+    }
+  }
+  void gradientResidual() {  // SpatialDiscrete.cpp:1034-1068
+    const int rows = nv * dim;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; e++) {
+      double* res = &vg_res[e * szG()];
+      gemm(res, &vg_adj[static_cast<size_t>(e) * rows * naq], Vf.data(), rows, naq, nb, false);
+      gemm(res, &vg_quad[static_cast<size_t>(e) * rows * dim * nq], dV.data(), rows, dim * nq, nb, true, -1.0);
+      if (ns()) {
+        if (viscous == SDG_VISC_BR1) {
+          gemm(&ig_res[e * szG()], &ig_adj[static_cast<size_t>(e) * rows * naq], Vf.data(), rows, naq, nb, false);
+        } else {
+          for (int s = 0; s < nf; s++) {
+            double* out = &ig_res[(static_cast<size_t>(e) * nf + s) * szG()];
+            for (int j = 0; j < nb; j++)
+              for (int r = 0; r < rows; r++) {
+                double acc = 0.0;
+                for (int p = 0; p < nfq; p++)
+                  acc += ig_adj[static_cast<size_t>(e) * rows * naq + r + static_cast<size_t>(rows) * (accumulate(s) + p)] *
+                         Vf[(accumulate(s) + p) + static_cast<size_t>(naq) * j];
+                out[r + static_cast<size_t>(rows) * j] = acc;
+              }
+          }
+        }
+      }
+    }
+  }
+  void updateGradientCoefficient() {  // TimeIntegration.cpp:200-228
+    const int rows = nv * dim;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; e++) {
+      const double* mi = &minv[static_cast<size_t>(e) * nb * nb];
+      gemm(&vg_coeff[e * szG()], &vg_res[e * szG()], mi, rows, nb, nb, false);
+      if (ns()) {
+        double* g = &g_coeff[e * szG()];
+        std::memcpy(g, &vg_coeff[e * szG()], szG() * sizeof(double));
+        for (int s = 0; s < nig; s++) {
+          double* ic = &ig_coeff[(static_cast<size_t>(e) * nig + s) * szG()];
+          gemm(ic, &ig_res[(static_cast<size_t>(e) * nig + s) * szG()], mi, rows, nb, nb, false);
+          for (size_t i = 0; i < szG(); i++) g[i] += ic[i];
+        }
+      }
+    }
+  }
+  void volumeQuadrature() {  // SpatialDiscrete.cpp:194-266
+    const int rows = nv * dim;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; e++) {
+      std::vector<double> uq(static_cast<size_t>(nv) * nq), gq;
+      gemmBt(uq.data(), &coeff[e * szC()], V.data(), nv, nb, nq, false);
+      if (ns()) {
+        gq.resize(static_cast<size_t>(rows) * nq);
+        gemmBt(gq.data(), &g_coeff[e * szG()], V.data(), rows, nb, nq, false);
+      }
+      double* out = &vol_quad[static_cast<size_t>(e) * nv * dim * nq];
+      for (int q = 0; q < nq; q++) {
+        double w[8], fc[15], fv[15], pg[15];
+        const double* u = &uq[static_cast<size_t>(nv) * q];
+        computational(u, w);
+        // F_c (D x NV column-major), ConvectiveFlux.cpp:27-44
+        const double rho = w[0], p = w[dim + 2];
+        const double et = w[dim + 1] + velocitySquared(w) / 2.0;
+        for (int a = 0; a < dim; a++) {
+          fc[a] = rho * w[1 + a];
+          for (int b = 0; b < dim; b++) fc[a + dim * (1 + b)] = rho * w[1 + a] * w[1 + b] + (a == b ? p * 1.0 : p * 0.0);
+          fc[a + dim * (dim + 1)] = (rho * et + p) * w[1 + a];
+        }
+        if (ns()) {
+          primitiveGradient(w, u, &gq[static_cast<size_t>(rows) * q], pg);
+          viscousRawFlux(w, pg, fv);
+        }
+        const double* G = &tiw[(static_cast<size_t>(e) * nq + q) * dim * dim];
+        for (int v = 0; v < nv; v++)
+          for (int b = 0; b < dim; b++) {
+            double s = 0.0;
+            for (int a = 0; a < dim; a++) {
+              const double f = ns() ? (fc[a + dim * v] - fv[a + dim * v]) : fc[a + dim * v];
+              s += f * G[a + dim * b];
+            }
+            out[v + static_cast<size_t>(nv) * (q * dim + b)] = s;
+          }
+      }
+    }
+  }
+  void faceGradientTrace(int64_t e, int s, double* gt) const {  // VariableConvertor.cpp:640-689
+    const int rows = nv * dim;
+    for (int j = 0; j < nfq; j++)
+      for (int r = 0; r < rows; r++) {
+        double acc = 0.0;
+        for (int i = 0; i < nb; i++) {
+          double c;
+          if (viscous == SDG_VISC_BR2)
+            c = vg_coeff[e * szG() + r + static_cast<size_t>(rows) * i] +
+                ig_coeff[(static_cast<size_t>(e) * nig + s) * szG() + r + static_cast<size_t>(rows) * i];
+          else
+            c = g_coeff[e * szG() + r + static_cast<size_t>(rows) * i];
+          acc += c * Vf[(accumulate(s) + j) + static_cast<size_t>(naq) * i];
+        }
+        gt[r + static_cast<size_t>(rows) * j] = acc;
+      }
+  }
+  void adjacencyQuadrature() {  // SpatialDiscrete.cpp:633-748
+    const int rows = nv * dim;
+#pragma omp parallel for schedule(static)
+    for (int64_t f = 0; f < nface; f++) {
+      std::vector<double> ul(static_cast<size_t>(nv) * nfq), ur(static_cast<size_t>(nv) * nfq);
+      std::vector<double> wl(static_cast<size_t>(dim + 3) * nfq), wr(static_cast<size_t>(dim + 3) * nfq);
+      std::vector<double> gl, gr;
+      faceTraces(f, ul.data(), ur.data());
+      const int32_t L = fparent[2 * f], R = fparent[2 * f + 1];
+      const int sl = flocal[2 * f], sr = flocal[2 * f + 1];
+      for (int j = 0; j < nfq; j++) {
+        computational(&ul[nv * j], &wl[(dim + 3) * j]);
+        computational(&ur[nv * j], &wr[(dim + 3) * j]);
+      }
+      if (ns()) {
+        gl.resize(static_cast<size_t>(rows) * nfq);
+        gr.resize(static_cast<size_t>(rows) * nfq);
+        faceGradientTrace(L, sl, gl.data());
+        faceGradientTrace(R, sr, gr.data());
+      }
+      for (int j = 0; j < nfq; j++) {
+        const int k = fseq[f * nfq + j];
+        const double* n = &fnormal[(static_cast<size_t>(f) * nfq + j) * dim];
+        double h[5], tmp[5];
+        convectiveFlux(n, &ul[nv * j], &wl[(dim + 3) * j], &ur[nv * k], &wr[(dim + 3) * k], h);
+        if (ns()) {
+          double pgl[15], pgr[15], fl[5], fr[5];
+          primitiveGradient(&wl[(dim + 3) * j], &ul[nv * j], &gl[static_cast<size_t>(rows) * j], pgl);
+          primitiveGradient(&wr[(dim + 3) * k], &ur[nv * k], &gr[static_cast<size_t>(rows) * k], pgr);
+          viscousNormalFlux(n, &wl[(dim + 3) * j], pgl, fl);
+          viscousNormalFlux(n, &wr[(dim + 3) * k], pgr, fr);
+          for (int v = 0; v < nv; v++) tmp[v] = (h[v] - (fl[v] + fr[v]) / 2.0) * fjw[f * nfq + j];
+        } else {
+          for (int v = 0; v < nv; v++) tmp[v] = h[v] * fjw[f * nfq + j];
+        }
+        double* cl = &adj_quad[static_cast<size_t>(L) * nv * naq + static_cast<size_t>(nv) * (accumulate(sl) + j)];
+        double* cr = &adj_quad[static_cast<size_t>(R) * nv * naq + static_cast<size_t>(nv) * (accumulate(sr) + k)];
+        for (int v = 0; v < nv; v++) {
+          cl[v] = tmp[v];
+          cr[v] = -tmp[v];
+        }
+      }
+    }
+  }
+  void calcResidual() {  // SpatialDiscrete.cpp:1016-1032
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; e++) {
+      double* res = &residual[e * szC()];
+      gemm(res, &vol_quad[static_cast<size_t>(e) * nv * dim * nq], dV.data(), nv, dim * nq, nb, false);
+      gemm(res, &adj_quad[static_cast<size_t>(e) * nv * naq], Vf.data(), nv, naq, nb, true, -1.0);
+    }
+  }
+  void rkCoefficients(int stage, double c[3]) const {  // TimeIntegration.cpp:45-65
+    if (rk == SDG_RK_FORWARD_EULER) {
+      c[0] = 1.0; c[1] = 0.0; c[2] = 1.0;
+    } else if (rk == SDG_RK_HEUN2) {
+      const double t[2][3] = {{1.0, 0.0, 1.0}, {0.5, 0.5, 0.5}};
+      for (int i = 0; i < 3; i++) c[i] = t[stage][i];
+    } else {
+      const double t[3][3] = {{1.0, 0.0, 1.0}, {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+      for (int i = 0; i < 3; i++) c[i] = t[stage][i];
+    }
+  }
+  int stages() const { return rk == SDG_RK_FORWARD_EULER ? 1 : (rk == SDG_RK_HEUN2 ? 2 : 3); }
+  void updateCoefficient(int stage, double dt) {  // TimeIntegration.cpp:181-198
+    double c[3];
+    rkCoefficients(stage, c);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; e++) {
+      double* u = &coeff[e * szC()];
+      const double* un = &coeff_last[e * szC()];
+      std::vector<double> rm(szC());
+      gemm(rm.data(), &residual[e * szC()], &minv[static_cast<size_t>(e) * nb * nb], nv, nb, nb, false);
+      for (size_t i = 0; i < szC(); i++) u[i] *= c[1];
+      for (size_t i = 0; i < szC(); i++) u[i] += c[0] * un[i];
+      for (size_t i = 0; i < szC(); i++) u[i] += c[2] * dt * rm[i];
+    }
+  }
+  void relativeError() {  // TimeIntegration.cpp:279-324
+    std::vector<double> acc(nv, 0.0);
+    std::vector<double> per(static_cast<size_t>(ne) * nv);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; e++) {
+      std::vector<double> rq(static_cast<size_t>(nv) * nq);
+      gemmBt(rq.data(), &residual[e * szC()], V.data(), nv, nb, nq, false);
+      for (int v = 0; v < nv; v++) {
+        double s = 0.0;
+        for (int q = 0; q < nq; q++) s += std::fabs(rq[v + static_cast<size_t>(nv) * q]);
+        per[e * nv + v] = s / nq;
+      }
+    }
+    for (int64_t e = 0; e < ne; e++)
+      for (int v = 0; v < nv; v++) acc[v] += per[e * nv + v];
+    for (int v = 0; v < nv; v++) rel_err[v] = acc[v] / static_cast<double>(ne);
+  }
+  void spatial() {
+    gradientQuadrature();
+    adjacencyGradientQuadrature();
+    gradientResidual();
+    updateGradientCoefficient();
+    volumeQuadrature();
+    adjacencyQuadrature();
+    calcResidual();
+  }
+  void step(double dt) {  // TimeIntegration.cpp:327-350
+    coeff_last = coeff;
+    for (int s = 0; s < stages(); s++) {
+      spatial();
+      updateCoefficient(s, dt);
+    }
+    relativeError();
+  }
+};
+
+template <typename T>
+void colMajorFromRowMajor(std::vector<T>& dst, const T* src, int rows, int cols) {
+  dst.resize(static_cast<size_t>(rows) * cols);
+  for (int r = 0; r < rows; r++)
+    for (int c = 0; c < cols; c++) dst[r + static_cast<size_t>(rows) * c] = src[static_cast<size_t>(r) * cols + c];
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+}
+int oracle_get_threads(void) { return omp_get_max_threads(); }
+
+// Sizes: dim, nb, nq, nf, nfq, order, ne, nface. Row-major tables as sdg_mesh_export writes them.
+void* oracle_create(const sdg_problem_desc* p, int dim, int nb, int nq, int nf, int nfq, int order, int64_t ne,
+                    int64_t nface, const double* V, const double* dV, const double* Vf, const double* lsq,
+                    const double* detjw, const double* tiw, const double* minv, const double* min_edge,
+                    const int32_t* fparent, const int32_t* flocal, const int32_t* fseq, const double* fnormal,
+                    const double* fjw) {
+  if (!p || dim < 1 || dim > 3) return nullptr;
+  Oracle* o = new (std::nothrow) Oracle();
+  if (!o) return nullptr;
+  o->dim = dim;
+  o->nv = dim + 2;
+  o->nb = nb;
+  o->nq = nq;
+  o->nf = nf;
+  o->nfq = nfq;
+  o->naq = nf * nfq;
+  o->order = order;
+  o->ne = ne;
+  o->nface = nface;
+  o->equation = p->equation;
+  o->flux = p->convective_flux;
+  o->viscous = p->equation == SDG_EQ_COMPRESSIBLE_NS ? p->viscous_flux : SDG_VISC_NONE;
+  o->rk = p->time_integration;
+  o->phys.eos = p->equation_of_state;
+  o->phys.transport = p->transport;
+  o->phys.cp = p->cp;
+  o->phys.cv = p->cv;
+  o->phys.mu = p->dynamic_viscosity;
+  o->phys.kappa = p->cp * p->dynamic_viscosity / kPrandtl;  // PhysicalModel.cpp:149-153
+  o->phys.c0 = p->reference_sound_speed;
+  o->phys.rho0 = p->reference_density;
+  o->phys.padd = 0.01 * p->reference_density * p->reference_sound_speed * p->reference_sound_speed;
+  colMajorFromRowMajor(o->V, V, nq, nb);
+  colMajorFromRowMajor(o->dV, dV, nq * dim, nb);
+  colMajorFromRowMajor(o->Vf, Vf, nf * nfq, nb);
+  colMajorFromRowMajor(o->lsq, lsq, nb, nb);
+  o->detjw.assign(detjw, detjw + ne * nq);
+  o->tiw.assign(tiw, tiw + ne * nq * dim * dim);
+  o->minv.resize(static_cast<size_t>(ne) * nb * nb);
+  for (int64_t e = 0; e < ne; e++)
+    for (int r = 0; r < nb; r++)
+      for (int c = 0; c < nb; c++)
+        o->minv[e * nb * nb + r + static_cast<size_t>(nb) * c] = minv[e * nb * nb + static_cast<size_t>(r) * nb + c];
+  o->min_edge.assign(min_edge, min_edge + ne);
+  o->fparent.assign(fparent, fparent + 2 * nface);
+  o->flocal.assign(flocal, flocal + 2 * nface);
+  o->fseq.assign(fseq, fseq + nface * nfq);
+  o->fnormal.assign(fnormal, fnormal + nface * nfq * dim);
+  o->fjw.assign(fjw, fjw + nface * nfq);
+  const size_t nv = o->nv;
+  o->coeff.assign(ne * nv * nb, 0.0);
+  o->coeff_last.assign(ne * nv * nb, 0.0);
+  o->vol_quad.assign(ne * nv * dim * nq, 0.0);
+  o->adj_quad.assign(ne * nv * o->naq, 0.0);
+  o->residual.assign(ne * nv * nb, 0.0);
+  o->vg_coeff.assign(ne * o->szG(), 0.0);
+  o->vg_quad.assign(ne * nv * dim * dim * nq, 0.0);
+  o->vg_adj.assign(ne * nv * dim * o->naq, 0.0);
+  o->vg_res.assign(ne * o->szG(), 0.0);
+  if (o->ns()) {
+    o->nig = o->viscous == SDG_VISC_BR2 ? nf : 1;
+    o->ig_adj.assign(ne * nv * dim * o->naq, 0.0);
+    o->ig_res.assign(ne * o->nig * o->szG(), 0.0);
+    o->ig_coeff.assign(ne * o->nig * o->szG(), 0.0);
+    o->g_coeff.assign(ne * o->szG(), 0.0);
+  }
+  o->rel_err.assign(nv, 0.0);
+  return o;
+}
+
+void oracle_destroy(void* h) { delete static_cast<Oracle*>(h); }
+
+int oracle_set_state(void* h, const double* c) {
+  Oracle* o = static_cast<Oracle*>(h);
+  std::memcpy(o->coeff.data(), c, o->coeff.size() * sizeof(double));
+  o->coeff_last = o->coeff;
+  o->has_state = true;
+  return 0;
+}
+int oracle_get_state(void* h, double* c) {
+  Oracle* o = static_cast<Oracle*>(h);
+  std::memcpy(c, o->coeff.data(), o->coeff.size() * sizeof(double));
+  return 0;
+}
+// InitialCondition.cpp:85-104: conserved at quadrature points from primitives, then
+// C = U_q V (V^T V)^-1
+int oracle_set_initial_primitive(void* h, const double* prim) {
+  Oracle* o = static_cast<Oracle*>(h);
+  const int nv = o->nv, nq = o->nq, nb = o->nb, dim = o->dim;
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < o->ne; e++) {
+    std::vector<double> uq(static_cast<size_t>(nv) * nq), tmp(static_cast<size_t>(nv) * nb);
+    for (int q = 0; q < nq; q++) {
+      const double* pr = &prim[(static_cast<size_t>(e) * nq + q) * nv];
+      const double rho = pr[0];
+      double v2 = 0.0;
+      uq[static_cast<size_t>(nv) * q] = rho;
+      for (int d = 0; d < dim; d++) {
+        uq[1 + d + static_cast<size_t>(nv) * q] = rho * pr[1 + d];
+        v2 += pr[1 + d] * pr[1 + d];
+      }
+      const double et = o->phys.internalEnergyFromTemperature(pr[dim + 1]) + v2 / 2.0;
+      uq[dim + 1 + static_cast<size_t>(nv) * q] = rho * et;
+    }
+    gemm(tmp.data(), uq.data(), o->V.data(), nv, nq, nb, false);
+    gemm(&o->coeff[e * o->szC()], tmp.data(), o->lsq.data(), nv, nb, nb, false);
+  }
+  o->coeff_last = o->coeff;
+  o->has_state = true;
+  return 0;
+}
+int oracle_residual(void* h, double* r) {
+  Oracle* o = static_cast<Oracle*>(h);
+  o->spatial();
+  if (r) std::memcpy(r, o->residual.data(), o->residual.size() * sizeof(double));
+  return 0;
+}
+int oracle_get_gradient(void* h, double* g) {
+  Oracle* o = static_cast<Oracle*>(h);
+  if (!o->ns()) return -2;
+  std::memcpy(g, o->g_coeff.data(), o->g_coeff.size() * sizeof(double));
+  return 0;
+}
+int oracle_compute_dt(void* h, double cfl, double* dt) {  // TimeIntegration.cpp:104-131
+  Oracle* o = static_cast<Oracle*>(h);
+  const int nv = o->nv, nq = o->nq, nb = o->nb, dim = o->dim;
+  double best = std::numeric_limits<double>::max();
+#pragma omp parallel for schedule(static) reduction(min : best)
+  for (int64_t e = 0; e < o->ne; e++) {
+    std::vector<double> uq(static_cast<size_t>(nv) * nq);
+    gemmBt(uq.data(), &o->coeff[e * o->szC()], o->V.data(), nv, nb, nq, false);
+    for (int q = 0; q < nq; q++) {
+      double w[8];
+      o->computational(&uq[static_cast<size_t>(nv) * q], w);
+      const double c = o->phys.soundSpeed(w[0], w[dim + 2]);
+      const double sr = std::sqrt(o->velocitySquared(w)) + c;
+      const double t = cfl * o->min_edge[e] / (sr * (o->order + 1.0) * (o->order + 1.0));
+      best = std::min(best, t);
+    }
+  }
+  *dt = best;
+  return 0;
+}
+int oracle_step(void* h, double dt, int nsteps, double* rel_err, int* nan_flag, int* steps_taken) {
+  Oracle* o = static_cast<Oracle*>(h);
+  int taken = 0, nanf = 0;
+  for (int i = 0; i < nsteps; i++) {
+    o->step(dt);
+    taken++;
+    bool all_nan = true;
+    for (int v = 0; v < o->nv; v++) all_nan = all_nan && std::isnan(o->rel_err[v]);
+    if (all_nan) {  // SystemControl.cpp:185-191
+      nanf = 1;
+      break;
+    }
+  }
+  if (rel_err) std::memcpy(rel_err, o->rel_err.data(), o->nv * sizeof(double));
+  if (nan_flag) *nan_flag = nanf;
+  if (steps_taken) *steps_taken = taken;
+  return 0;
+}
+// Pointwise physics for unit tests: conserved states (NV each), normal (D) -> flux (NV).
+int oracle_point_flux(const sdg_problem_desc* p, int dim, int flux, const double* n, const double* ul, const double* ur,
+                      double* h) {
+  Oracle o;
+  o.dim = dim;
+  o.nv = dim + 2;
+  o.flux = flux;
+  o.phys.eos = p->equation_of_state;
+  o.phys.cp = p->cp;
+  o.phys.cv = p->cv;
+  o.phys.c0 = p->reference_sound_speed;
+  o.phys.rho0 = p->reference_density;
+  o.phys.padd = 0.01 * p->reference_density * p->reference_sound_speed * p->reference_sound_speed;
+  double wl[8], wr[8];
+  o.computational(ul, wl);
+  o.computational(ur, wr);
+  o.convectiveFlux(n, ul, wl, ur, wr, h);
+  return 0;
+}
+// Viscous normal flux at one point from conserved state and conserved gradient (row var*D+dim).
+int oracle_point_viscous_flux(const sdg_problem_desc* p, int dim, const double* n, const double* u, const double* g,
+                              double* f) {
+  Oracle o;
+  o.dim = dim;
+  o.nv = dim + 2;
+  o.phys.eos = p->equation_of_state;
+  o.phys.transport = p->transport;
+  o.phys.cp = p->cp;
+  o.phys.cv = p->cv;
+  o.phys.mu = p->dynamic_viscosity;
+  o.phys.kappa = p->cp * p->dynamic_viscosity / kPrandtl;
+  double w[8], pg[15];
+  o.computational(u, w);
+  o.primitiveGradient(w, u, g, pg);
+  o.viscousNormalFlux(n, w, pg, f);
+  return 0;
+}
+
+}  // extern "C"

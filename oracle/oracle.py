@@ -1,0 +1,156 @@
+"""ctypes wrapper of the CPU ORACLE (oracle/liboracle.so) -- test infrastructure only.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this module, as the
+checker (or the timed CPU baseline), never as part of the product path. The oracle consumes the
+same operand tables (basis, geometry, faces) that the product library exports for a mesh, so a
+parity test compares the two implementations of the hot path on identical inputs.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, c_double, c_int, c_int64, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+SRC_PATH = os.path.join(HERE, "oracle.cpp")
+
+
+def build(force: bool = False) -> str:
+    """Compile oracle.cpp into liboracle.so with g++ (OpenMP)."""
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(SRC_PATH):
+        cmd = ["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fopenmp", "-fPIC", "-shared",
+               "-o", LIB_PATH, SRC_PATH]
+        subprocess.run(cmd, check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.oracle_create.restype = c_void_p
+        lib.oracle_create.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int64, c_int64] + \
+            [c_void_p] * 13
+        lib.oracle_destroy.argtypes = [c_void_p]
+        for name in ("oracle_set_state", "oracle_get_state", "oracle_set_initial_primitive", "oracle_residual",
+                     "oracle_get_gradient"):
+            getattr(lib, name).argtypes = [c_void_p, c_void_p]
+            getattr(lib, name).restype = c_int
+        lib.oracle_compute_dt.argtypes = [c_void_p, c_double, POINTER(c_double)]
+        lib.oracle_step.argtypes = [c_void_p, c_double, c_int, c_void_p, POINTER(c_int), POINTER(c_int)]
+        lib.oracle_point_flux.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
+        lib.oracle_point_viscous_flux.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
+        lib.oracle_set_threads.argtypes = [c_int]
+        lib.oracle_get_threads.restype = c_int
+        _lib = lib
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data
+
+
+class Oracle:
+    """CPU restatement of Solver<SC> on one mesh."""
+
+    def __init__(self, problem, mesh):
+        self.lib = load()
+        self.mesh = mesh
+        self.dim, self.nv, self.nb, self.nq = mesh.dim, mesh.nv, mesh.nb, mesh.nq
+        self.E = mesh.E
+        self._keep = {k: mesh.field(k) for k in (
+            "basis_value", "basis_gradient", "basis_face_value", "basis_lsq_inverse", "jacobian_det_w",
+            "jacobian_tiw", "mass_inverse", "min_edge", "face_parent", "face_local", "face_sequence",
+            "face_normal", "face_jacobian_w")}
+        k = self._keep
+        self.problem = problem
+        self.h = self.lib.oracle_create(
+            ctypes.byref(problem), mesh.dim, mesh.nb, mesh.nq, mesh.nf, mesh.nfq, mesh.order, mesh.E, mesh.F,
+            _ptr(k["basis_value"]), _ptr(k["basis_gradient"]), _ptr(k["basis_face_value"]),
+            _ptr(k["basis_lsq_inverse"]), _ptr(k["jacobian_det_w"]), _ptr(k["jacobian_tiw"]),
+            _ptr(k["mass_inverse"]), _ptr(k["min_edge"]), _ptr(k["face_parent"]), _ptr(k["face_local"]),
+            _ptr(k["face_sequence"]), _ptr(k["face_normal"]), _ptr(k["face_jacobian_w"]))
+        if not self.h:
+            raise RuntimeError("oracle_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.oracle_destroy(self.h)
+            self.h = None
+
+    def state_shape(self):
+        return (self.E, self.nb, self.nv)
+
+    def set_state(self, c: np.ndarray):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        self.lib.oracle_set_state(self.h, _ptr(c))
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty(self.state_shape())
+        self.lib.oracle_get_state(self.h, _ptr(out))
+        return out
+
+    def set_initial_primitive(self, prim: np.ndarray):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        self.lib.oracle_set_initial_primitive(self.h, _ptr(prim))
+
+    def residual(self) -> np.ndarray:
+        out = np.empty(self.state_shape())
+        self.lib.oracle_residual(self.h, _ptr(out))
+        return out
+
+    def gradient(self) -> np.ndarray:
+        out = np.empty((self.E, self.nb, self.nv * self.dim))
+        rc = self.lib.oracle_get_gradient(self.h, _ptr(out))
+        if rc != 0:
+            raise RuntimeError("gradient is only defined for Navier-Stokes")
+        return out
+
+    def compute_dt(self, cfl: float) -> float:
+        dt = c_double()
+        self.lib.oracle_compute_dt(self.h, cfl, ctypes.byref(dt))
+        return dt.value
+
+    def step(self, dt: float, nsteps: int = 1):
+        rel = np.empty(self.nv)
+        nan = c_int()
+        taken = c_int()
+        self.lib.oracle_step(self.h, dt, nsteps, _ptr(rel), ctypes.byref(nan), ctypes.byref(taken))
+        return rel, nan.value, taken.value
+
+
+def set_threads(n: int):
+    load().oracle_set_threads(n)
+
+
+def get_threads() -> int:
+    return load().oracle_get_threads()
+
+
+def point_flux(problem, dim: int, flux: int, n, ul, ur) -> np.ndarray:
+    lib = load()
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    ul = np.ascontiguousarray(ul, dtype=np.float64)
+    ur = np.ascontiguousarray(ur, dtype=np.float64)
+    h = np.empty(dim + 2)
+    lib.oracle_point_flux(ctypes.byref(problem), dim, flux, _ptr(n), _ptr(ul), _ptr(ur), _ptr(h))
+    return h
+
+
+def point_viscous_flux(problem, dim: int, n, u, g) -> np.ndarray:
+    lib = load()
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    f = np.empty(dim + 2)
+    lib.oracle_point_viscous_flux(ctypes.byref(problem), dim, _ptr(n), _ptr(u), _ptr(g), _ptr(f))
+    return f

@@ -1,0 +1,290 @@
+"""ctypes binding of libsdg_hip.so (the C-ABI declared in include/sdg.h).
+
+The library is built in-tree (see subrosadg_amd/build.py). Loading fails loudly when it is
+missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, Structure, c_char_p, c_double, c_int, c_int32, c_int64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsdg_hip.so")
+
+ABI_VERSION = 1
+
+# status codes
+OK, E_INVALID, E_UNSUPPORTED, E_DEVICE, E_NOMEM, E_STATE = 0, -1, -2, -3, -4, -5
+
+# enums (include/sdg.h)
+ELEM_LINE, ELEM_TRIANGLE, ELEM_QUADRANGLE, ELEM_HEXAHEDRON = 1, 2, 3, 6
+EQ_EULER, EQ_NS = 0, 1
+FLUX_CENTRAL, FLUX_LAX_FRIEDRICHS, FLUX_HLLC, FLUX_ROE, FLUX_EXACT = 0, 1, 2, 3, 4
+VISC_NONE, VISC_BR1, VISC_BR2 = 0, 1, 2
+RK_FORWARD_EULER, RK_HEUN2, RK_SSPRK3 = 0, 1, 2
+EOS_IDEAL_GAS, EOS_WEAK_COMPRESSIBLE = 0, 1
+TRANSPORT_NONE, TRANSPORT_CONSTANT, TRANSPORT_SUTHERLAND = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+
+FIELDS = {
+    "basis_value": (0, np.float64),
+    "basis_gradient": (1, np.float64),
+    "basis_face_value": (2, np.float64),
+    "basis_lsq_inverse": (3, np.float64),
+    "quad_weight": (4, np.float64),
+    "face_quad_weight": (5, np.float64),
+    "quad_coord": (6, np.float64),
+    "jacobian_det_w": (7, np.float64),
+    "jacobian_tiw": (8, np.float64),
+    "mass_inverse": (9, np.float64),
+    "min_edge": (10, np.float64),
+    "face_normal": (11, np.float64),
+    "face_jacobian_w": (12, np.float64),
+    "face_coord": (13, np.float64),
+    "face_parent": (14, np.int32),
+    "face_local": (15, np.int32),
+    "face_sequence": (16, np.int32),
+    "face_rotation": (17, np.int32),
+    "quad_point": (18, np.float64),
+    "face_quad_point": (19, np.float64),
+}
+
+
+class ProblemDesc(Structure):
+    _fields_ = [
+        ("abi_version", c_int),
+        ("equation", c_int),
+        ("convective_flux", c_int),
+        ("viscous_flux", c_int),
+        ("time_integration", c_int),
+        ("equation_of_state", c_int),
+        ("transport", c_int),
+        ("precision", c_int),
+        ("cp", c_double),
+        ("cv", c_double),
+        ("dynamic_viscosity", c_double),
+        ("reference_sound_speed", c_double),
+        ("reference_density", c_double),
+    ]
+
+
+class StructuredDesc(Structure):
+    _fields_ = [
+        ("element", c_int),
+        ("polynomial_order", c_int),
+        ("n", c_int * 3),
+        ("lo", c_double * 3),
+        ("hi", c_double * 3),
+    ]
+
+
+class MeshInfo(Structure):
+    _fields_ = [
+        ("dimension", c_int),
+        ("element", c_int),
+        ("polynomial_order", c_int),
+        ("elements", c_int64),
+        ("faces", c_int64),
+        ("basis", c_int),
+        ("quadrature", c_int),
+        ("faces_per_element", c_int),
+        ("face_quadrature", c_int),
+        ("affine", c_int),
+    ]
+
+
+# every symbol declared in include/sdg.h, with its ctypes signature
+SIGNATURES = {
+    "sdg_abi_version": (c_int, []),
+    "sdg_last_error": (c_char_p, [c_void_p]),
+    "sdg_mesh_create_structured": (c_int, [POINTER(StructuredDesc), POINTER(c_void_p)]),
+    "sdg_mesh_destroy": (None, [c_void_p]),
+    "sdg_mesh_get_info": (c_int, [c_void_p, POINTER(MeshInfo)]),
+    "sdg_mesh_field_size": (c_int64, [c_void_p, c_int]),
+    "sdg_mesh_export": (c_int, [c_void_p, c_int, c_void_p, c_int64]),
+    "sdg_create": (c_int, [POINTER(ProblemDesc), c_void_p, POINTER(c_void_p)]),
+    "sdg_destroy": (None, [c_void_p]),
+    "sdg_set_initial_primitive": (c_int, [c_void_p, c_void_p]),
+    "sdg_set_state": (c_int, [c_void_p, c_void_p]),
+    "sdg_get_state": (c_int, [c_void_p, c_void_p]),
+    "sdg_compute_dt": (c_int, [c_void_p, c_double, POINTER(c_double)]),
+    "sdg_residual": (c_int, [c_void_p, c_void_p]),
+    "sdg_get_gradient": (c_int, [c_void_p, c_void_p]),
+    "sdg_step": (c_int, [c_void_p, c_double, c_int, c_void_p, POINTER(c_int), POINTER(c_int)]),
+    "sdg_last_step_ms": (c_int, [c_void_p, POINTER(c_double)]),
+    "sdg_synchronize": (c_int, [c_void_p]),
+}
+
+_lib = None
+
+
+class SDGError(RuntimeError):
+    pass
+
+
+def load(path: str | None = None):
+    """Load libsdg_hip.so; raise if it was not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise SDGError(
+            f"{p} is missing: build the HIP extension first (python -c 'import __graft_entry__ as g; g.build()')"
+        )
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.sdg_abi_version() != ABI_VERSION:
+        raise SDGError("ABI version mismatch between subrosadg_amd and libsdg_hip.so")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc: int, ctx=None):
+    if rc != OK:
+        lib = load()
+        msg = lib.sdg_last_error(ctx)
+        raise SDGError(f"sdg error {rc}: {msg.decode() if msg else ''}")
+
+
+def problem_desc(**kw) -> ProblemDesc:
+    p = ProblemDesc()
+    p.abi_version = ABI_VERSION
+    p.equation = kw.get("equation", EQ_EULER)
+    p.convective_flux = kw.get("convective_flux", FLUX_ROE)
+    p.viscous_flux = kw.get("viscous_flux", VISC_NONE)
+    p.time_integration = kw.get("time_integration", RK_SSPRK3)
+    p.equation_of_state = kw.get("equation_of_state", EOS_IDEAL_GAS)
+    p.transport = kw.get("transport", TRANSPORT_NONE)
+    p.precision = kw.get("precision", PREC_FP64)
+    p.cp = kw.get("cp", 3.5)
+    p.cv = kw.get("cv", 2.5)
+    p.dynamic_viscosity = kw.get("dynamic_viscosity", 0.0)
+    p.reference_sound_speed = kw.get("reference_sound_speed", 0.0)
+    p.reference_density = kw.get("reference_density", 0.0)
+    return p
+
+
+class Mesh:
+    """Structured periodic mesh plus its operand tables (host only)."""
+
+    def __init__(self, element: int, order: int, n, lo, hi):
+        self.lib = load()
+        d = StructuredDesc()
+        d.element = element
+        d.polynomial_order = order
+        for i in range(3):
+            d.n[i] = int(n[i]) if i < len(n) else 1
+            d.lo[i] = float(lo[i]) if i < len(lo) else 0.0
+            d.hi[i] = float(hi[i]) if i < len(hi) else 1.0
+        h = c_void_p()
+        check(self.lib.sdg_mesh_create_structured(ctypes.byref(d), ctypes.byref(h)))
+        self.handle = h
+        info = MeshInfo()
+        check(self.lib.sdg_mesh_get_info(h, ctypes.byref(info)))
+        self.info = info
+        self.dim = info.dimension
+        self.order = info.polynomial_order
+        self.E = info.elements
+        self.F = info.faces
+        self.nb = info.basis
+        self.nq = info.quadrature
+        self.nf = info.faces_per_element
+        self.nfq = info.face_quadrature
+        self.nv = self.dim + 2
+
+    def field(self, name: str) -> np.ndarray:
+        fid, dt = FIELDS[name]
+        n = self.lib.sdg_mesh_field_size(self.handle, fid)
+        out = np.empty(n, dtype=dt)
+        check(self.lib.sdg_mesh_export(self.handle, fid, out.ctypes.data, n))
+        return out
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self.lib.sdg_mesh_destroy(h)
+            self.handle = None
+
+
+class Solver:
+    """A device solver context (sdg_ctx) on one mesh: the drop-in for Solver<SC>."""
+
+    def __init__(self, problem: ProblemDesc, mesh: Mesh):
+        self.lib = load()
+        self.mesh = mesh
+        self.problem = problem
+        h = c_void_p()
+        rc = self.lib.sdg_create(ctypes.byref(problem), mesh.handle, ctypes.byref(h))
+        check(rc)
+        self.handle = h
+        self.E, self.nb, self.nv, self.dim = mesh.E, mesh.nb, mesh.nv, mesh.dim
+
+    def _check(self, rc):
+        if rc != OK:
+            msg = self.lib.sdg_last_error(self.handle)
+            raise SDGError(f"sdg error {rc}: {msg.decode() if msg else ''}")
+
+    def state_shape(self):
+        return (self.E, self.nb, self.nv)
+
+    def set_state(self, c: np.ndarray):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        assert c.size == self.E * self.nb * self.nv
+        self._check(self.lib.sdg_set_state(self.handle, c.ctypes.data))
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty(self.state_shape())
+        self._check(self.lib.sdg_get_state(self.handle, out.ctypes.data))
+        return out
+
+    def set_initial_primitive(self, prim: np.ndarray):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        assert prim.size == self.E * self.mesh.nq * self.nv
+        self._check(self.lib.sdg_set_initial_primitive(self.handle, prim.ctypes.data))
+
+    def residual(self) -> np.ndarray:
+        out = np.empty(self.state_shape())
+        self._check(self.lib.sdg_residual(self.handle, out.ctypes.data))
+        return out
+
+    def gradient(self) -> np.ndarray:
+        out = np.empty((self.E, self.nb, self.nv * self.dim))
+        self._check(self.lib.sdg_get_gradient(self.handle, out.ctypes.data))
+        return out
+
+    def compute_dt(self, cfl: float) -> float:
+        dt = c_double()
+        self._check(self.lib.sdg_compute_dt(self.handle, cfl, ctypes.byref(dt)))
+        return dt.value
+
+    def step(self, dt: float, nsteps: int = 1):
+        rel = np.empty(self.nv)
+        nan = c_int()
+        taken = c_int()
+        self._check(self.lib.sdg_step(self.handle, dt, nsteps, rel.ctypes.data, ctypes.byref(nan), ctypes.byref(taken)))
+        return rel, nan.value, taken.value
+
+    def last_step_ms(self) -> float:
+        ms = c_double()
+        self._check(self.lib.sdg_last_step_ms(self.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        self._check(self.lib.sdg_synchronize(self.handle))
+
+    def close(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self.lib.sdg_destroy(h)
+            self.handle = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,544 @@
+// kit.cpp -- quadrature, Lobatto modal basis, reference tables and structured periodic meshes.
+// See kit.hpp for the conventions kept from the reference.
+#include "kit.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace sdg {
+
+namespace {
+constexpr double kPi = 3.14159265358979323846;
+
+// gmsh reference vertices (SimulationControl.cpp:218-241)
+const double kQuadVertex[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+const double kHexVertex[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                 {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+// local face vertex lists (SimulationControl.cpp:177-216)
+const int kQuadFace[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+const int kHexFace[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {0, 4, 7, 3}, {1, 2, 6, 5}, {2, 3, 7, 6}, {4, 5, 6, 7}};
+// P1 face element vertices: line (-1),(1); quad (-1,-1),(1,-1),(1,1),(-1,1)
+const double kLineFaceVertex[2] = {-1, 1};
+const double kQuadFaceVertex[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+
+// Tensor-product P1 shape function of a vertex with coordinates v (entries +-1) at xi, and its
+// gradient.
+void p1Shape(int dim, const double* v, const double* xi, double& n, double* grad) {
+  double f[3];
+  for (int d = 0; d < dim; d++) f[d] = 0.5 * (1.0 + v[d] * xi[d]);
+  n = 1.0;
+  for (int d = 0; d < dim; d++) n *= f[d];
+  for (int d = 0; d < dim; d++) {
+    double g = 0.5 * v[d];
+    for (int e = 0; e < dim; e++)
+      if (e != d) g *= f[e];
+    grad[d] = g;
+  }
+}
+
+double det(int dim, const double* a) {  // row-major
+  if (dim == 1) return a[0];
+  if (dim == 2) return a[0] * a[3] - a[1] * a[2];
+  return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+         a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+}  // namespace
+
+void legendre(int n, double x, double& value, double& derivative) {
+  if (n == 0) {
+    value = 1.0;
+    derivative = 0.0;
+    return;
+  }
+  double p0 = 1.0, p1 = x;
+  for (int k = 2; k <= n; k++) {
+    const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+    p0 = p1;
+    p1 = p2;
+  }
+  value = p1;
+  // derivative from the recurrence (1-x^2) L_n' = n (L_{n-1} - x L_n), and the closed form at +-1
+  if (std::fabs(1.0 - x * x) < 1e-300) {
+    const double s = (x > 0 || (n % 2 == 1)) ? 1.0 : -1.0;
+    derivative = s * 0.5 * n * (n + 1.0);
+  } else {
+    derivative = n * (p0 - x * p1) / (1.0 - x * x);
+  }
+}
+
+void gaussLegendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  for (int i = 0; i < n; i++) {
+    // i-th largest root, Newton from the Chebyshev-like guess
+    double r = std::cos(kPi * (i + 0.75) / (n + 0.5));
+    for (int it = 0; it < 100; it++) {
+      double v, d;
+      legendre(n, r, v, d);
+      const double dr = v / d;
+      r -= dr;
+      if (std::fabs(dr) < 1e-17) break;
+    }
+    double v, d;
+    legendre(n, r, v, d);
+    x[n - 1 - i] = r;
+    w[n - 1 - i] = 2.0 / ((1.0 - r * r) * d * d);
+  }
+}
+
+void lobatto(int k, double x, double& value, double& derivative) {
+  if (k == 0) {
+    value = 0.5 * (1.0 - x);
+    derivative = -0.5;
+    return;
+  }
+  if (k == 1) {
+    value = 0.5 * (1.0 + x);
+    derivative = 0.5;
+    return;
+  }
+  double lk, dk, lk2, dk2, lk1, dk1;
+  legendre(k, x, lk, dk);
+  legendre(k - 2, x, lk2, dk2);
+  legendre(k - 1, x, lk1, dk1);
+  const double s = std::sqrt(2.0 * (2.0 * k - 1.0));
+  value = (lk - lk2) / s;
+  derivative = (2.0 * k - 1.0) * lk1 / s;  // L_k' - L_{k-2}' = (2k-1) L_{k-1}
+}
+
+bool invertMatrix(int n, const double* a, double* inverse) {
+  std::vector<double> m(static_cast<size_t>(n) * 2 * n, 0.0);
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j < n; j++) m[i * 2 * n + j] = a[i * n + j];
+    m[i * 2 * n + n + i] = 1.0;
+  }
+  for (int c = 0; c < n; c++) {
+    int piv = c;
+    for (int r = c + 1; r < n; r++)
+      if (std::fabs(m[r * 2 * n + c]) > std::fabs(m[piv * 2 * n + c])) piv = r;
+    if (std::fabs(m[piv * 2 * n + c]) < 1e-300) return false;
+    if (piv != c)
+      for (int j = 0; j < 2 * n; j++) std::swap(m[c * 2 * n + j], m[piv * 2 * n + j]);
+    const double inv = 1.0 / m[c * 2 * n + c];
+    for (int j = 0; j < 2 * n; j++) m[c * 2 * n + j] *= inv;
+    for (int r = 0; r < n; r++) {
+      if (r == c) continue;
+      const double f = m[r * 2 * n + c];
+      if (f == 0.0) continue;
+      for (int j = 0; j < 2 * n; j++) m[r * 2 * n + j] -= f * m[c * 2 * n + j];
+    }
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) inverse[i * n + j] = m[i * 2 * n + n + j];
+  return true;
+}
+
+static void evalBasis(const RefElement& ref, const double* xi, double* value, double* grad /*N_b x D*/) {
+  const int dim = ref.dim;
+  for (int b = 0; b < ref.nb; b++) {
+    double l[3], dl[3];
+    for (int d = 0; d < dim; d++) lobatto(ref.modes[b][d], xi[d], l[d], dl[d]);
+    double v = 1.0;
+    for (int d = 0; d < dim; d++) v *= l[d];
+    value[b] = v;
+    if (grad) {
+      for (int d = 0; d < dim; d++) {
+        double g = dl[d];
+        for (int e = 0; e < dim; e++)
+          if (e != d) g *= l[e];
+        grad[b * dim + d] = g;
+      }
+    }
+  }
+}
+
+bool buildRefElement(int type, int order, RefElement& ref, std::string& error) {
+  if (order < 1 || order > 5) {
+    error = "polynomial order must be 1..5";
+    return false;
+  }
+  ref = RefElement();
+  ref.type = type;
+  ref.order = order;
+  const int n1 = order + 1;  // points per direction: Gauss order 2P -> P+1, order 2P+1 -> P+1
+  std::vector<double> gx, gw;
+  gaussLegendre(n1, gx, gw);
+  if (type == 3) {  // quadrangle
+    ref.dim = 2;
+    ref.nf = 4;
+    ref.nvert = 4;
+    ref.nfvert = 2;
+    for (int j = 0; j <= order; j++)
+      for (int i = 0; i <= order; i++) ref.modes.push_back({i, j, 0});
+    for (int v = 0; v < 4; v++)
+      for (int d = 0; d < 2; d++) ref.vertex.push_back(kQuadVertex[v][d]);
+    for (int f = 0; f < 4; f++)
+      for (int v = 0; v < 2; v++) ref.face_vertex.push_back(kQuadFace[f][v]);
+    for (int b = 0; b < n1; b++)
+      for (int a = 0; a < n1; a++) {
+        ref.qpoint.push_back(gx[a]);
+        ref.qpoint.push_back(gx[b]);
+        ref.qweight.push_back(gw[a] * gw[b]);
+      }
+    for (int a = 0; a < n1; a++) {
+      ref.fpoint.push_back(gx[a]);
+      ref.fweight.push_back(gw[a]);
+    }
+  } else if (type == 6) {  // hexahedron
+    ref.dim = 3;
+    ref.nf = 6;
+    ref.nvert = 8;
+    ref.nfvert = 4;
+    for (int k = 0; k <= order; k++)
+      for (int j = 0; j <= order; j++)
+        for (int i = 0; i <= order; i++) ref.modes.push_back({i, j, k});
+    for (int v = 0; v < 8; v++)
+      for (int d = 0; d < 3; d++) ref.vertex.push_back(kHexVertex[v][d]);
+    for (int f = 0; f < 6; f++)
+      for (int v = 0; v < 4; v++) ref.face_vertex.push_back(kHexFace[f][v]);
+    for (int c = 0; c < n1; c++)
+      for (int b = 0; b < n1; b++)
+        for (int a = 0; a < n1; a++) {
+          ref.qpoint.push_back(gx[a]);
+          ref.qpoint.push_back(gx[b]);
+          ref.qpoint.push_back(gx[c]);
+          ref.qweight.push_back(gw[a] * gw[b] * gw[c]);
+        }
+    // face points with the second face coordinate fastest: with this order the right-parent
+    // permutations equal the rotation tables of SimulationControl.cpp:454-520
+    for (int a = 0; a < n1; a++)
+      for (int b = 0; b < n1; b++) {
+        ref.fpoint.push_back(gx[a]);
+        ref.fpoint.push_back(gx[b]);
+        ref.fweight.push_back(gw[a] * gw[b]);
+      }
+  } else {
+    error = "unsupported element type (quadrangle=3 and hexahedron=6 are implemented)";
+    return false;
+  }
+  // hierarchical ordering: sort modes by (max index, then slowest-to-fastest index) so that the
+  // first N_b(P-1) functions span the order P-1 space
+  std::stable_sort(ref.modes.begin(), ref.modes.end(), [](const std::array<int, 3>& a, const std::array<int, 3>& b) {
+    const int ma = std::max(a[0], std::max(a[1], a[2]));
+    const int mb = std::max(b[0], std::max(b[1], b[2]));
+    if (ma != mb) return ma < mb;
+    if (a[2] != b[2]) return a[2] < b[2];
+    if (a[1] != b[1]) return a[1] < b[1];
+    return a[0] < b[0];
+  });
+  const int dim = ref.dim;
+  ref.nb = static_cast<int>(ref.modes.size());
+  ref.nq = static_cast<int>(ref.qweight.size());
+  ref.nfq = static_cast<int>(ref.fweight.size());
+  const int nb = ref.nb, nq = ref.nq, nf = ref.nf, nfq = ref.nfq;
+  ref.V.assign(static_cast<size_t>(nq) * nb, 0.0);
+  ref.dV.assign(static_cast<size_t>(nq) * dim * nb, 0.0);
+  std::vector<double> val(nb), grad(static_cast<size_t>(nb) * dim);
+  for (int q = 0; q < nq; q++) {
+    evalBasis(ref, &ref.qpoint[static_cast<size_t>(q) * dim], val.data(), grad.data());
+    for (int b = 0; b < nb; b++) {
+      ref.V[static_cast<size_t>(q) * nb + b] = val[b];
+      for (int d = 0; d < dim; d++) ref.dV[(static_cast<size_t>(q) * dim + d) * nb + b] = grad[b * dim + d];
+    }
+  }
+  // face points in parent coordinates: P1 map of the face rule through the face vertices
+  ref.face_param.assign(static_cast<size_t>(nf) * nfq * dim, 0.0);
+  ref.Vf.assign(static_cast<size_t>(nf) * nfq * nb, 0.0);
+  for (int f = 0; f < nf; f++) {
+    for (int j = 0; j < nfq; j++) {
+      double xi[3] = {0, 0, 0};
+      for (int v = 0; v < ref.nfvert; v++) {
+        double n, g[2];
+        if (dim == 2) {
+          const double fv[1] = {kLineFaceVertex[v]};
+          p1Shape(1, fv, &ref.fpoint[j], n, g);
+        } else {
+          p1Shape(2, kQuadFaceVertex[v], &ref.fpoint[static_cast<size_t>(j) * 2], n, g);
+        }
+        const int lv = ref.face_vertex[f * ref.nfvert + v];
+        for (int d = 0; d < dim; d++) xi[d] += n * ref.vertex[lv * dim + d];
+      }
+      for (int d = 0; d < dim; d++) ref.face_param[(static_cast<size_t>(f) * nfq + j) * dim + d] = xi[d];
+      evalBasis(ref, xi, val.data(), nullptr);
+      for (int b = 0; b < nb; b++) ref.Vf[(static_cast<size_t>(f) * nfq + j) * nb + b] = val[b];
+    }
+  }
+  std::vector<double> vtv(static_cast<size_t>(nb) * nb, 0.0);
+  ref.mass.assign(static_cast<size_t>(nb) * nb, 0.0);
+  for (int i = 0; i < nb; i++)
+    for (int j = 0; j < nb; j++) {
+      double s = 0.0, m = 0.0;
+      for (int q = 0; q < nq; q++) {
+        const double p = ref.V[static_cast<size_t>(q) * nb + i] * ref.V[static_cast<size_t>(q) * nb + j];
+        s += p;
+        m += p * ref.qweight[q];
+      }
+      vtv[static_cast<size_t>(i) * nb + j] = s;
+      ref.mass[static_cast<size_t>(i) * nb + j] = m;
+    }
+  ref.lsq_inverse.assign(static_cast<size_t>(nb) * nb, 0.0);
+  ref.mass_inverse.assign(static_cast<size_t>(nb) * nb, 0.0);
+  if (!invertMatrix(nb, vtv.data(), ref.lsq_inverse.data()) ||
+      !invertMatrix(nb, ref.mass.data(), ref.mass_inverse.data())) {
+    error = "singular reference matrix";
+    return false;
+  }
+  return true;
+}
+
+namespace {
+// physical point of reference point xi in element e, and the Jacobian J(l,k) = dx_l/dxi_k
+void mapPoint(const Mesh& mesh, int64_t e, const double* xi, double* x, double* jac) {
+  const RefElement& ref = mesh.ref;
+  const int dim = ref.dim;
+  for (int d = 0; d < dim; d++) x[d] = 0.0;
+  if (jac)
+    for (int i = 0; i < dim * dim; i++) jac[i] = 0.0;
+  for (int v = 0; v < ref.nvert; v++) {
+    double n, g[3];
+    p1Shape(dim, &ref.vertex[static_cast<size_t>(v) * dim], xi, n, g);
+    const double* xv = &mesh.vertex[(static_cast<size_t>(e) * ref.nvert + v) * dim];
+    for (int l = 0; l < dim; l++) {
+      x[l] += n * xv[l];
+      if (jac)
+        for (int k = 0; k < dim; k++) jac[l * dim + k] += xv[l] * g[k];
+    }
+  }
+}
+}  // namespace
+
+bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3], const double hi[3], Mesh& mesh,
+                         std::string& error) {
+  mesh = Mesh();
+  if (!buildRefElement(type, order, mesh.ref, error)) return false;
+  const RefElement& ref = mesh.ref;
+  const int dim = ref.dim;
+  int nc[3] = {n[0], n[1], dim == 3 ? n[2] : 1};
+  for (int d = 0; d < dim; d++) {
+    if (nc[d] < 2) {
+      error = "a periodic direction needs at least 2 cells";
+      return false;
+    }
+    if (!(hi[d] > lo[d])) {
+      error = "box upper corner must exceed the lower corner";
+      return false;
+    }
+  }
+  double h[3], period[3];
+  for (int d = 0; d < dim; d++) {
+    period[d] = hi[d] - lo[d];
+    h[d] = period[d] / nc[d];
+  }
+  const int64_t E = static_cast<int64_t>(nc[0]) * nc[1] * nc[2];
+  if (E > (int64_t(1) << 30)) {
+    error = "too many elements";
+    return false;
+  }
+  mesh.elements = E;
+  mesh.vertex.assign(static_cast<size_t>(E) * ref.nvert * dim, 0.0);
+  auto cellIndex = [&](int i, int j, int k) -> int64_t {
+    i = (i % nc[0] + nc[0]) % nc[0];
+    j = (j % nc[1] + nc[1]) % nc[1];
+    k = (k % nc[2] + nc[2]) % nc[2];
+    return i + static_cast<int64_t>(nc[0]) * (j + static_cast<int64_t>(nc[1]) * k);
+  };
+  auto vertexId = [&](int i, int j, int k) -> int64_t {  // periodic global vertex id
+    return cellIndex(i, j, k);
+  };
+  std::vector<int64_t> vid(static_cast<size_t>(E) * ref.nvert);
+  for (int k = 0; k < nc[2]; k++)
+    for (int j = 0; j < nc[1]; j++)
+      for (int i = 0; i < nc[0]; i++) {
+        const int64_t e = cellIndex(i, j, k);
+        for (int v = 0; v < ref.nvert; v++) {
+          const double* rv = &ref.vertex[static_cast<size_t>(v) * dim];
+          const int off[3] = {rv[0] > 0 ? 1 : 0, rv[1] > 0 ? 1 : 0, dim == 3 && rv[2] > 0 ? 1 : 0};
+          const int idx[3] = {i + off[0], j + off[1], k + off[2]};
+          for (int d = 0; d < dim; d++) mesh.vertex[(static_cast<size_t>(e) * ref.nvert + v) * dim + d] = lo[d] + idx[d] * h[d];
+          vid[static_cast<size_t>(e) * ref.nvert + v] = vertexId(idx[0], idx[1], idx[2]);
+        }
+      }
+  // geometry: affine elements (rectangular cells), Jacobian at the centre
+  mesh.jac.assign(static_cast<size_t>(E) * dim * dim, 0.0);
+  mesh.detj.assign(E, 0.0);
+  mesh.min_edge.assign(E, 0.0);
+  static const int kQuadEdge[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+  static const int kHexEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                      {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+  for (int64_t e = 0; e < E; e++) {
+    const double xi0[3] = {0, 0, 0};
+    double x[3];
+    mapPoint(mesh, e, xi0, x, &mesh.jac[static_cast<size_t>(e) * dim * dim]);
+    mesh.detj[e] = det(dim, &mesh.jac[static_cast<size_t>(e) * dim * dim]);
+    if (!(mesh.detj[e] > 0.0)) {
+      error = "non-positive Jacobian";
+      return false;
+    }
+    double me = std::numeric_limits<double>::max();
+    const int ne = dim == 2 ? 4 : 12;
+    for (int g = 0; g < ne; g++) {
+      const int a = dim == 2 ? kQuadEdge[g][0] : kHexEdge[g][0];
+      const int b = dim == 2 ? kQuadEdge[g][1] : kHexEdge[g][1];
+      double s = 0.0;
+      for (int d = 0; d < dim; d++) {
+        const double t = mesh.vertex[(static_cast<size_t>(e) * ref.nvert + a) * dim + d] -
+                         mesh.vertex[(static_cast<size_t>(e) * ref.nvert + b) * dim + d];
+        s += t * t;
+      }
+      me = std::min(me, std::sqrt(s));
+    }
+    mesh.min_edge[e] = me;
+  }
+  // faces: neighbour across each local face, created on first visit (the visitor is the left
+  // parent), as the reference's first-seen rule (Adjacency.cpp:61-183)
+  static const int kQuadNb[4][4] = {{0, -1, 0, 2}, {1, 0, 0, 3}, {0, 1, 0, 0}, {-1, 0, 0, 1}};  // di,dj,dk,face
+  static const int kHexNb[6][4] = {{0, 0, -1, 5}, {0, -1, 0, 4}, {-1, 0, 0, 3},
+                                   {1, 0, 0, 2},  {0, 1, 0, 1},  {0, 0, 1, 0}};
+  const int nf = ref.nf, nfq = ref.nfq;
+  mesh.elem_face.assign(static_cast<size_t>(E) * nf, -1);
+  mesh.elem_side.assign(static_cast<size_t>(E) * nf, 0);
+  std::vector<double> xl(static_cast<size_t>(nfq) * dim), xr(static_cast<size_t>(nfq) * dim);
+  for (int k = 0; k < nc[2]; k++)
+    for (int j = 0; j < nc[1]; j++)
+      for (int i = 0; i < nc[0]; i++) {
+        const int64_t e = cellIndex(i, j, k);
+        for (int s = 0; s < nf; s++) {
+          if (mesh.elem_face[static_cast<size_t>(e) * nf + s] >= 0) continue;
+          const int* nbr = dim == 2 ? kQuadNb[s] : kHexNb[s];
+          const int64_t r = cellIndex(i + nbr[0], j + nbr[1], k + nbr[2]);
+          const int rs = nbr[3];
+          Face face;
+          face.parent[0] = static_cast<int32_t>(e);
+          face.parent[1] = static_cast<int32_t>(r);
+          face.local[0] = s;
+          face.local[1] = rs;
+          // physical face points from both sides, matched modulo the period
+          for (int p = 0; p < nfq; p++) {
+            mapPoint(mesh, e, &ref.face_param[(static_cast<size_t>(s) * nfq + p) * dim], &xl[p * dim], nullptr);
+            mapPoint(mesh, r, &ref.face_param[(static_cast<size_t>(rs) * nfq + p) * dim], &xr[p * dim], nullptr);
+          }
+          face.seq.assign(nfq, -1);
+          for (int p = 0; p < nfq; p++) {
+            double best = std::numeric_limits<double>::max();
+            int bi = -1;
+            for (int qq = 0; qq < nfq; qq++) {
+              double s2 = 0.0;
+              for (int d = 0; d < dim; d++) {
+                double t = xl[p * dim + d] - xr[qq * dim + d];
+                t -= period[d] * std::round(t / period[d]);
+                s2 += t * t;
+              }
+              if (s2 < best) {
+                best = s2;
+                bi = qq;
+              }
+            }
+            if (best > 1e-20 * (1.0 + h[0] * h[0])) {
+              error = "face points of the two parents do not match";
+              return false;
+            }
+            face.seq[p] = bi;
+          }
+          // rotation: position of the left's first face vertex in the right's face vertex list
+          face.rotation = 0;
+          if (dim == 3) {
+            const int64_t first = vid[static_cast<size_t>(e) * ref.nvert + ref.face_vertex[s * ref.nfvert]];
+            for (int v = 0; v < ref.nfvert; v++)
+              if (vid[static_cast<size_t>(r) * ref.nvert + ref.face_vertex[rs * ref.nfvert + v]] == first) face.rotation = v;
+          }
+          // normals and |J_f| w from the left parent's face map
+          face.normal.assign(static_cast<size_t>(nfq) * dim, 0.0);
+          face.jw.assign(nfq, 0.0);
+          face.coord.assign(xl.begin(), xl.end());
+          const double* J = &mesh.jac[static_cast<size_t>(e) * dim * dim];
+          for (int p = 0; p < nfq; p++) {
+            double ts[3] = {0, 0, 0}, tt[3] = {0, 0, 0};  // d xi / d s, d xi / d t
+            for (int v = 0; v < ref.nfvert; v++) {
+              double nval, g[2];
+              if (dim == 2) {
+                const double fv[1] = {kLineFaceVertex[v]};
+                p1Shape(1, fv, &ref.fpoint[p], nval, g);
+              } else {
+                p1Shape(2, kQuadFaceVertex[v], &ref.fpoint[static_cast<size_t>(p) * 2], nval, g);
+              }
+              const int lv = ref.face_vertex[s * ref.nfvert + v];
+              for (int d = 0; d < dim; d++) {
+                ts[d] += g[0] * ref.vertex[lv * dim + d];
+                if (dim == 3) tt[d] += g[1] * ref.vertex[lv * dim + d];
+              }
+            }
+            double Ts[3] = {0, 0, 0}, Tt[3] = {0, 0, 0};
+            for (int l = 0; l < dim; l++)
+              for (int kk = 0; kk < dim; kk++) {
+                Ts[l] += J[l * dim + kk] * ts[kk];
+                Tt[l] += J[l * dim + kk] * tt[kk];
+              }
+            double nv[3];
+            if (dim == 2) {
+              nv[0] = Ts[1];
+              nv[1] = -Ts[0];
+            } else {
+              nv[0] = Ts[1] * Tt[2] - Ts[2] * Tt[1];
+              nv[1] = Ts[2] * Tt[0] - Ts[0] * Tt[2];
+              nv[2] = Ts[0] * Tt[1] - Ts[1] * Tt[0];
+            }
+            double norm = 0.0;
+            for (int d = 0; d < dim; d++) norm += nv[d] * nv[d];
+            norm = std::sqrt(norm);
+            for (int d = 0; d < dim; d++) face.normal[static_cast<size_t>(p) * dim + d] = nv[d] / norm;
+            face.jw[p] = norm * ref.fweight[p];
+          }
+          const int32_t fid = static_cast<int32_t>(mesh.faces.size());
+          mesh.faces.push_back(std::move(face));
+          mesh.elem_face[static_cast<size_t>(e) * nf + s] = fid;
+          mesh.elem_side[static_cast<size_t>(e) * nf + s] = 0;
+          if (mesh.elem_face[static_cast<size_t>(r) * nf + rs] >= 0) {
+            error = "face recorded more than twice";
+            return false;
+          }
+          mesh.elem_face[static_cast<size_t>(r) * nf + rs] = fid;
+          mesh.elem_side[static_cast<size_t>(r) * nf + rs] = 1;
+        }
+      }
+  mesh.affine = true;
+  return true;
+}
+
+void elementGeometry(const Mesh& mesh, int64_t e, double* detjw, double* tiw, double* coord) {
+  const RefElement& ref = mesh.ref;
+  const int dim = ref.dim;
+  const double* J = &mesh.jac[static_cast<size_t>(e) * dim * dim];
+  // Jt(k,l) = J(l,k); G = inverse(Jt) |J| w, column-major: tiw[a + D b] = inv(Jt)(a,b)
+  double jt[9], jti[9];
+  for (int k = 0; k < dim; k++)
+    for (int l = 0; l < dim; l++) jt[k * dim + l] = J[l * dim + k];
+  invertMatrix(dim, jt, jti);
+  const double dj = mesh.detj[e];
+  for (int q = 0; q < ref.nq; q++) {
+    const double jw = dj * ref.qweight[q];
+    if (detjw) detjw[q] = jw;
+    if (tiw)
+      for (int a = 0; a < dim; a++)
+        for (int b = 0; b < dim; b++) tiw[static_cast<size_t>(q) * dim * dim + a + dim * b] = jti[a * dim + b] * jw;
+    if (coord) mapPoint(mesh, e, &ref.qpoint[static_cast<size_t>(q) * dim], &coord[static_cast<size_t>(q) * dim], nullptr);
+  }
+}
+
+void elementMassInverse(const Mesh& mesh, int64_t e, double* minv) {
+  const RefElement& ref = mesh.ref;
+  const int nb = ref.nb;
+  const double dj = mesh.detj[e];
+  std::vector<double> m(static_cast<size_t>(nb) * nb, 0.0);
+  for (int i = 0; i < nb; i++)
+    for (int j = 0; j < nb; j++) {
+      double s = 0.0;
+      for (int q = 0; q < ref.nq; q++)
+        s += ref.V[static_cast<size_t>(q) * nb + i] * (ref.V[static_cast<size_t>(q) * nb + j] * (dj * ref.qweight[q]));
+      m[static_cast<size_t>(i) * nb + j] = s;
+    }
+  invertMatrix(nb, m.data(), minv);
+}
+
+}  // namespace sdg

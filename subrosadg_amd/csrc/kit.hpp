@@ -1,0 +1,105 @@
+// kit.hpp -- host-side discretisation kit: quadrature rules, the hierarchical Lobatto modal
+// basis, reference-element tables and structured periodic meshes with affine geometry.
+//
+// This replaces the gmsh-owned parts of the reference's setup
+// (src/Mesh/Quadrature.cpp:28-54, src/Mesh/BasisFunction.cpp:136-230, src/Mesh/Geometry.cpp:29-168,
+//  src/Mesh/Adjacency.cpp:37-495) with project-written code. Conventions kept from the reference:
+//   * reference elements: quad [-1,1]^2, hex [-1,1]^3 with gmsh vertex numbering
+//     (SimulationControl.cpp:218-241);
+//   * local face vertex lists (SimulationControl.cpp:177-216);
+//   * face points in parent coordinates = P1 map of the face rule through the face vertices
+//     (BasisFunction.cpp:76-111);
+//   * volume rule of order 2P, face rule of order 2P+1 (SimulationControl.cpp:275-283);
+//   * normals outward from the left (first-seen) parent (Geometry.cpp:102-168);
+//   * right-parent point index = seq[j] (SimulationControl.cpp:381-523), computed here by
+//     matching physical face-point coordinates.
+#ifndef SDG_KIT_HPP_
+#define SDG_KIT_HPP_
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace sdg {
+
+// Gauss-Legendre rule on [-1,1] with n points, ascending abscissae.
+void gaussLegendre(int n, std::vector<double>& x, std::vector<double>& w);
+// Legendre polynomial L_n and its derivative at x.
+void legendre(int n, double x, double& value, double& derivative);
+// Solin's 1-D Lobatto shape function l_k and its derivative on [-1,1]:
+// l0 = (1-x)/2, l1 = (1+x)/2, l_k = (L_k - L_{k-2}) / sqrt(2(2k-1)) for k >= 2.
+void lobatto(int k, double x, double& value, double& derivative);
+
+// Reference-element tables for one (element type, order).
+struct RefElement {
+  int type = 0;       // sdg_element
+  int dim = 0;        // D
+  int order = 0;      // P
+  int nb = 0;         // N_b
+  int nq = 0;         // N_q
+  int nf = 0;         // N_f
+  int nfq = 0;        // N_fq
+  int nvert = 0;      // vertices of the element
+  int nfvert = 0;     // vertices per face
+  std::vector<std::array<int, 3>> modes;   // per basis function: 1-D Lobatto indices (i,j,k)
+  std::vector<double> qpoint;              // N_q x D reference volume points
+  std::vector<double> qweight;             // N_q
+  std::vector<double> fpoint;              // N_fq x (D-1) reference face points
+  std::vector<double> fweight;             // N_fq
+  std::vector<double> vertex;              // nvert x D reference vertices
+  std::vector<int> face_vertex;            // N_f x nfvert local face vertex lists
+  std::vector<double> V;                   // N_q x N_b        basis at volume points
+  std::vector<double> dV;                  // (N_q*D) x N_b    reference gradient, row q*D+d
+  std::vector<double> Vf;                  // (N_f*N_fq) x N_b basis at face points
+  std::vector<double> face_param;          // N_f x N_fq x D   face points in parent coordinates
+  std::vector<double> lsq_inverse;         // N_b x N_b        (V^T V)^-1
+  std::vector<double> mass;                // N_b x N_b        sum_q V_qi V_qj w_q (reference mass)
+  std::vector<double> mass_inverse;        // N_b x N_b
+};
+
+// Build the reference tables; returns false for an unsupported (type, order).
+bool buildRefElement(int type, int order, RefElement& ref, std::string& error);
+
+// Dense inverse by Gauss-Jordan elimination with partial pivoting (row-major n x n).
+bool invertMatrix(int n, const double* a, double* inverse);
+
+struct Face {
+  int32_t parent[2] = {0, 0};   // left, right element
+  int32_t local[2] = {0, 0};    // local face index in left, right
+  int32_t rotation = 0;
+  std::vector<int32_t> seq;     // N_fq: right point index of left point j
+  std::vector<double> normal;   // N_fq x D, outward from the left parent
+  std::vector<double> jw;       // N_fq: |J_f| w
+  std::vector<double> coord;    // N_fq x D physical points (left parent's view)
+};
+
+struct Mesh {
+  RefElement ref;
+  int64_t elements = 0;
+  std::vector<double> vertex;       // E x nvert x D physical vertex coordinates (unwrapped)
+  std::vector<double> jac;          // E x D x D, jac[l*D+k] = d x_l / d xi_k (constant: affine)
+  std::vector<double> detj;         // E
+  std::vector<double> min_edge;     // E
+  std::vector<Face> faces;          // interior faces (periodic mesh: all faces)
+  // element -> face lookup: per element and local face, face index and side (0 left, 1 right)
+  std::vector<int32_t> elem_face;   // E x N_f
+  std::vector<int8_t> elem_side;    // E x N_f
+  bool affine = true;
+};
+
+// Structured periodic mesh; returns false with a message on a bad descriptor.
+bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3], const double hi[3],
+                         Mesh& mesh, std::string& error);
+
+// Geometry derived per element and volume point (affine elements: constant Jacobian).
+//   detjw[q]           = |J| w_q
+//   tiw[q*D*D + a+D*b] = ((J^T)^-1)(a,b) |J| w_q   (column-major D x D)
+//   coord[q*D + d]     = physical point
+void elementGeometry(const Mesh& mesh, int64_t e, double* detjw, double* tiw, double* coord);
+// Inverse of the element mass matrix sum_q V_qi V_qj |J| w_q (row-major N_b x N_b).
+void elementMassInverse(const Mesh& mesh, int64_t e, double* minv);
+
+}  // namespace sdg
+
+#endif  // SDG_KIT_HPP_

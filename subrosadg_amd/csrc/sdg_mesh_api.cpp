@@ -1,0 +1,187 @@
+// sdg_mesh_api.cpp -- mesh and operand-export part of the C-ABI (host only, no device work).
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sdg.h"
+#include "kit.hpp"
+#include "sdg_internal.hpp"
+
+struct sdg_mesh {
+  sdg::Mesh mesh;
+};
+
+namespace sdg {
+
+thread_local std::string g_last_error;
+
+void setGlobalError(const std::string& msg) { g_last_error = msg; }
+const char* globalError() { return g_last_error.c_str(); }
+const Mesh* meshOf(const sdg_mesh* m) { return m ? &m->mesh : nullptr; }
+
+}  // namespace sdg
+
+extern "C" {
+
+int sdg_abi_version(void) { return SDG_ABI_VERSION; }
+
+int sdg_mesh_create_structured(const sdg_structured_desc* desc, sdg_mesh** out) {
+  if (!desc || !out) {
+    sdg::setGlobalError("sdg_mesh_create_structured: null argument");
+    return SDG_E_INVALID;
+  }
+  *out = nullptr;
+  if (desc->element != SDG_ELEM_QUADRANGLE && desc->element != SDG_ELEM_HEXAHEDRON) {
+    sdg::setGlobalError("sdg_mesh_create_structured: element must be quadrangle or hexahedron");
+    return SDG_E_UNSUPPORTED;
+  }
+  sdg_mesh* m = new (std::nothrow) sdg_mesh();
+  if (!m) {
+    sdg::setGlobalError("sdg_mesh_create_structured: out of host memory");
+    return SDG_E_NOMEM;
+  }
+  std::string err;
+  try {
+    if (!sdg::buildStructuredMesh(desc->element, desc->polynomial_order, desc->n, desc->lo, desc->hi, m->mesh, err)) {
+      delete m;
+      sdg::setGlobalError("sdg_mesh_create_structured: " + err);
+      return SDG_E_INVALID;
+    }
+  } catch (const std::bad_alloc&) {
+    delete m;
+    sdg::setGlobalError("sdg_mesh_create_structured: out of host memory");
+    return SDG_E_NOMEM;
+  }
+  *out = m;
+  return SDG_OK;
+}
+
+void sdg_mesh_destroy(sdg_mesh* mesh) { delete mesh; }
+
+int sdg_mesh_get_info(const sdg_mesh* mesh, sdg_mesh_info* info) {
+  if (!mesh || !info) {
+    sdg::setGlobalError("sdg_mesh_get_info: null argument");
+    return SDG_E_INVALID;
+  }
+  const sdg::Mesh& m = mesh->mesh;
+  info->dimension = m.ref.dim;
+  info->element = m.ref.type;
+  info->polynomial_order = m.ref.order;
+  info->elements = m.elements;
+  info->faces = static_cast<int64_t>(m.faces.size());
+  info->basis = m.ref.nb;
+  info->quadrature = m.ref.nq;
+  info->faces_per_element = m.ref.nf;
+  info->face_quadrature = m.ref.nfq;
+  info->affine = m.affine ? 1 : 0;
+  return SDG_OK;
+}
+
+int64_t sdg_mesh_field_size(const sdg_mesh* mesh, int field) {
+  if (!mesh) return 0;
+  const sdg::Mesh& m = mesh->mesh;
+  const int64_t E = m.elements, F = static_cast<int64_t>(m.faces.size());
+  const int64_t D = m.ref.dim, nb = m.ref.nb, nq = m.ref.nq, nf = m.ref.nf, nfq = m.ref.nfq;
+  switch (field) {
+    case SDG_FIELD_BASIS_VALUE: return nq * nb;
+    case SDG_FIELD_BASIS_GRADIENT: return nq * D * nb;
+    case SDG_FIELD_BASIS_FACE_VALUE: return nf * nfq * nb;
+    case SDG_FIELD_BASIS_LSQ_INVERSE: return nb * nb;
+    case SDG_FIELD_QUAD_WEIGHT: return nq;
+    case SDG_FIELD_FACE_QUAD_WEIGHT: return nfq;
+    case SDG_FIELD_QUAD_COORD: return E * nq * D;
+    case SDG_FIELD_JACOBIAN_DET_W: return E * nq;
+    case SDG_FIELD_JACOBIAN_TIW: return E * nq * D * D;
+    case SDG_FIELD_MASS_INVERSE: return E * nb * nb;
+    case SDG_FIELD_MIN_EDGE: return E;
+    case SDG_FIELD_FACE_NORMAL: return F * nfq * D;
+    case SDG_FIELD_FACE_JACOBIAN_W: return F * nfq;
+    case SDG_FIELD_FACE_COORD: return F * nfq * D;
+    case SDG_FIELD_FACE_PARENT: return F * 2;
+    case SDG_FIELD_FACE_LOCAL: return F * 2;
+    case SDG_FIELD_FACE_SEQUENCE: return F * nfq;
+    case SDG_FIELD_FACE_ROTATION: return F;
+    case SDG_FIELD_QUAD_POINT: return nq * D;
+    case SDG_FIELD_FACE_QUAD_POINT: return nfq * (D - 1);
+    default: return 0;
+  }
+}
+
+int sdg_mesh_export(const sdg_mesh* mesh, int field, void* out, int64_t count) {
+  const int64_t need = sdg_mesh_field_size(mesh, field);
+  if (!mesh || !out || need == 0) {
+    sdg::setGlobalError("sdg_mesh_export: null argument or unknown field");
+    return SDG_E_INVALID;
+  }
+  if (count < need) {
+    sdg::setGlobalError("sdg_mesh_export: output buffer too small");
+    return SDG_E_INVALID;
+  }
+  const sdg::Mesh& m = mesh->mesh;
+  const sdg::RefElement& r = m.ref;
+  const int64_t E = m.elements;
+  const int D = r.dim, nb = r.nb, nq = r.nq, nfq = r.nfq;
+  double* od = static_cast<double*>(out);
+  int32_t* oi = static_cast<int32_t*>(out);
+  auto copy = [&](const std::vector<double>& v) { std::memcpy(od, v.data(), v.size() * sizeof(double)); };
+  switch (field) {
+    case SDG_FIELD_BASIS_VALUE: copy(r.V); break;
+    case SDG_FIELD_BASIS_GRADIENT: copy(r.dV); break;
+    case SDG_FIELD_BASIS_FACE_VALUE: copy(r.Vf); break;
+    case SDG_FIELD_BASIS_LSQ_INVERSE: copy(r.lsq_inverse); break;
+    case SDG_FIELD_QUAD_WEIGHT: copy(r.qweight); break;
+    case SDG_FIELD_FACE_QUAD_WEIGHT: copy(r.fweight); break;
+    case SDG_FIELD_QUAD_POINT: copy(r.qpoint); break;
+    case SDG_FIELD_FACE_QUAD_POINT: copy(r.fpoint); break;
+    case SDG_FIELD_QUAD_COORD:
+      for (int64_t e = 0; e < E; e++) sdg::elementGeometry(m, e, nullptr, nullptr, od + e * nq * D);
+      break;
+    case SDG_FIELD_JACOBIAN_DET_W:
+      for (int64_t e = 0; e < E; e++) sdg::elementGeometry(m, e, od + e * nq, nullptr, nullptr);
+      break;
+    case SDG_FIELD_JACOBIAN_TIW:
+      for (int64_t e = 0; e < E; e++) sdg::elementGeometry(m, e, nullptr, od + e * nq * D * D, nullptr);
+      break;
+    case SDG_FIELD_MASS_INVERSE:
+      for (int64_t e = 0; e < E; e++) sdg::elementMassInverse(m, e, od + e * nb * nb);
+      break;
+    case SDG_FIELD_MIN_EDGE: copy(m.min_edge); break;
+    case SDG_FIELD_FACE_NORMAL:
+      for (size_t f = 0; f < m.faces.size(); f++)
+        std::memcpy(od + f * nfq * D, m.faces[f].normal.data(), sizeof(double) * nfq * D);
+      break;
+    case SDG_FIELD_FACE_JACOBIAN_W:
+      for (size_t f = 0; f < m.faces.size(); f++) std::memcpy(od + f * nfq, m.faces[f].jw.data(), sizeof(double) * nfq);
+      break;
+    case SDG_FIELD_FACE_COORD:
+      for (size_t f = 0; f < m.faces.size(); f++)
+        std::memcpy(od + f * nfq * D, m.faces[f].coord.data(), sizeof(double) * nfq * D);
+      break;
+    case SDG_FIELD_FACE_PARENT:
+      for (size_t f = 0; f < m.faces.size(); f++) {
+        oi[2 * f] = m.faces[f].parent[0];
+        oi[2 * f + 1] = m.faces[f].parent[1];
+      }
+      break;
+    case SDG_FIELD_FACE_LOCAL:
+      for (size_t f = 0; f < m.faces.size(); f++) {
+        oi[2 * f] = m.faces[f].local[0];
+        oi[2 * f + 1] = m.faces[f].local[1];
+      }
+      break;
+    case SDG_FIELD_FACE_SEQUENCE:
+      for (size_t f = 0; f < m.faces.size(); f++)
+        std::memcpy(oi + f * nfq, m.faces[f].seq.data(), sizeof(int32_t) * nfq);
+      break;
+    case SDG_FIELD_FACE_ROTATION:
+      for (size_t f = 0; f < m.faces.size(); f++) oi[f] = m.faces[f].rotation;
+      break;
+    default:
+      sdg::setGlobalError("sdg_mesh_export: unknown field");
+      return SDG_E_INVALID;
+  }
+  return SDG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+"""Shared problem set-ups for the tests (synthetic meshes and initial conditions)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from subrosadg_amd import _lib as L
+
+GAMMA = 1.4
+
+
+def isentropic_vortex_primitive(xq: np.ndarray, eps: float = 5.0, u0=1.0, v0=1.0, rinf=1.0):
+    """Shu's isentropic vortex (strength eps, centre 0) on top of the free stream
+    rho = 1, u = v = 1, p = 1 with R = 1: T = 1 - (gamma-1) eps^2/(8 gamma pi^2) e^(1-r^2)."""
+    x, y = xq[..., 0], xq[..., 1]
+    r2 = x * x + y * y
+    f = np.exp((1.0 - r2) / 2.0)
+    T = 1.0 - (GAMMA - 1.0) * eps ** 2 / (8.0 * GAMMA * math.pi ** 2) * np.exp(1.0 - r2)
+    rho = T ** (1.0 / (GAMMA - 1.0))
+    prim = np.zeros(xq.shape[:-1] + (4,))
+    prim[..., 0] = rho
+    prim[..., 1] = u0 - eps / (2.0 * math.pi) * f * y
+    prim[..., 2] = v0 + eps / (2.0 * math.pi) * f * x
+    prim[..., 3] = T
+    return prim
+
+
+def euler_vortex(n=8, order=2, flux=L.FLUX_ROE, rk=L.RK_SSPRK3):
+    """The 2-D isentropic-vortex case on periodic [-5,5]^2 (R = 1: cv = 2.5, cp = 3.5)."""
+    mesh = L.Mesh(L.ELEM_QUADRANGLE, order, (n, n), (-5.0, -5.0), (5.0, 5.0))
+    prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=flux, time_integration=rk, cp=3.5, cv=2.5)
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 2)
+    return mesh, prob, isentropic_vortex_primitive(xq)
+
+
+def tgv_primitive(xq: np.ndarray, mach=0.1, T0=1.0):
+    """Taylor-Green vortex: u = sin x cos y cos z, v = -cos x sin y cos z, w = 0,
+    p = p0 + (cos 2x + cos 2y)(cos 2z + 2)/16 with p0 = 1/(gamma Ma^2), rho = p/(R T0), R = p0."""
+    x, y, z = xq[..., 0], xq[..., 1], xq[..., 2]
+    p0 = 1.0 / (GAMMA * mach ** 2)
+    R = p0
+    p = p0 + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0) / 16.0
+    prim = np.zeros(xq.shape[:-1] + (5,))
+    prim[..., 0] = p / (R * T0)
+    prim[..., 1] = np.sin(x) * np.cos(y) * np.cos(z)
+    prim[..., 2] = -np.cos(x) * np.sin(y) * np.cos(z)
+    prim[..., 3] = 0.0
+    prim[..., 4] = T0
+    return prim
+
+
+def tgv_thermo(mach=0.1):
+    p0 = 1.0 / (GAMMA * mach ** 2)
+    R = p0
+    cv = R / (GAMMA - 1.0)
+    return GAMMA * cv, cv
+
+
+def ns_tgv(n=4, order=3, flux=L.FLUX_HLLC, visc=L.VISC_BR2, reynolds=1600.0, mach=0.1):
+    mesh = L.Mesh(L.ELEM_HEXAHEDRON, order, (n, n, n), (0.0, 0.0, 0.0), (2 * math.pi,) * 3)
+    cp, cv = tgv_thermo(mach)
+    prob = L.problem_desc(equation=L.EQ_NS, convective_flux=flux, viscous_flux=visc, transport=L.TRANSPORT_CONSTANT,
+                          cp=cp, cv=cv, dynamic_viscosity=1.0 / reynolds)
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 3)
+    return mesh, prob, tgv_primitive(xq, mach)
+
+
+def shear_layer_primitive(xq: np.ndarray, mach=0.1):
+    """Double shear layer on [0,1]^2: rho = 1, u = tanh(80(y-1/4)) (y <= 1/2) else tanh(80(3/4-y)),
+    v = 0.05 sin(2 pi x), p = p0 = 1/(gamma Ma^2), R = p0 (T = 1)."""
+    x, y = xq[..., 0], xq[..., 1]
+    prim = np.zeros(xq.shape[:-1] + (4,))
+    prim[..., 0] = 1.0
+    prim[..., 1] = np.where(y <= 0.5, np.tanh(80.0 * (y - 0.25)), np.tanh(80.0 * (0.75 - y)))
+    prim[..., 2] = 0.05 * np.sin(2.0 * math.pi * x)
+    prim[..., 3] = 1.0
+    return prim
+
+
+def ns_shear(n=8, order=3, flux=L.FLUX_ROE, visc=L.VISC_BR2, reynolds=1.0e4, mach=0.1):
+    mesh = L.Mesh(L.ELEM_QUADRANGLE, order, (n, n), (0.0, 0.0), (1.0, 1.0))
+    cp, cv = tgv_thermo(mach)
+    prob = L.problem_desc(equation=L.EQ_NS, convective_flux=flux, viscous_flux=visc, transport=L.TRANSPORT_CONSTANT,
+                          cp=cp, cv=cv, dynamic_viscosity=1.0 / reynolds)
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 2)
+    return mesh, prob, shear_layer_primitive(xq, mach)
+
+
+def rel_err(a: np.ndarray, b: np.ndarray) -> float:
+    """max |a-b| / max |b| (relative to the reference's scale)."""
+    scale = np.abs(b).max()
+    return float(np.abs(a - b).max() / (scale if scale > 0 else 1.0))
+
+
+def rel_err_per_var(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Per-variable max |a-b| / max |b| for arrays whose last axis is the variable index."""
+    a2 = a.reshape(-1, a.shape[-1])
+    b2 = b.reshape(-1, b.shape[-1])
+    scale = np.abs(b2).max(axis=0)
+    scale = np.where(scale > 0, scale, 1.0)
+    return np.abs(a2 - b2).max(axis=0) / scale
+
+
+def parity_error(a: np.ndarray, b: np.ndarray, state: np.ndarray, dim: int, floor: float = 1e-3) -> float:
+    """Parity metric used for the 1e-12 FP64 bar.
+
+    Variables are grouped physically -- density, the momentum vector, total energy -- and the
+    error of a group is max |a - b| over the group divided by the group's scale. The scale is the
+    group's magnitude in b, measured in units of the state's magnitude of the same group (so a
+    residual or gradient of rho*E is compared with rho*E, not with rho). A group whose
+    nondimensional magnitude is below `floor` times the largest one (a field that is zero
+    analytically, e.g. the mass residual of a uniform-density flow) is measured against that
+    floor instead of its own rounding noise. a, b: (..., NV) or, for gradients, (..., NV*D)
+    with variable index = column // D; state: (..., NV).
+    """
+    nv = dim + 2
+    groups = [[0], list(range(1, dim + 1)), [dim + 1]]
+    ncol = a.shape[-1]
+    per = ncol // nv  # 1 for states/residuals, D for gradients
+    a2 = a.reshape(-1, ncol)
+    b2 = b.reshape(-1, ncol)
+    s2 = state.reshape(-1, nv)
+    S, nd, diff = [], [], []
+    for g in groups:
+        cols = [v * per + k for v in g for k in range(per)]
+        sg = np.abs(s2[:, g]).max()
+        sg = sg if sg > 0 else 1.0
+        S.append(sg)
+        nd.append(np.abs(b2[:, cols]).max() / sg)
+        diff.append(np.abs(a2[:, cols] - b2[:, cols]).max())
+    fl = floor * max(nd)
+    errs = [diff[i] / (S[i] * max(nd[i], fl)) if max(nd[i], fl) > 0 else diff[i] for i in range(3)]
+    return float(max(errs))
