@@ -1,0 +1,169 @@
+"""Benchmark: DG residual + SSP-RK3 stepping of the 3-D compressible Navier-Stokes Taylor-Green
+vortex on one MI355X.
+
+Workload (default): periodic [0, 2 pi]^3, uniform 32^3 hexahedra, P3 (64 modes per element,
+5 variables), Re = 1600, Ma = 0.1, Pr = 0.71, HLLC + BR2, SSP-RK3, FP64. Synthetic initial state
+u = sin x cos y cos z, v = -cos x sin y cos z, w = 0, p = p0 + (cos 2x + cos 2y)(cos 2z + 2)/16,
+rho = p / (R T0).
+
+A step is one SSP-RK3 time step (3 stages) of the whole mesh with the state resident in HBM.
+Metric: DOF.stage/s = E * N_b * stages * steps / wall time of the timed steps.
+
+Prints one JSON line with the metric, a roofline entry for the dominant kernel (timed with HIP
+events on the solver's stream) and the CPU baseline (the oracle on a bounded sample of the same
+workload on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from subrosadg_amd import _lib as L  # noqa: E402
+
+GAMMA = 1.4
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+
+
+def tgv_setup(n: int, order: int, mach=0.1, reynolds=1600.0):
+    mesh = L.Mesh(L.ELEM_HEXAHEDRON, order, (n, n, n), (0.0, 0.0, 0.0), (2 * math.pi,) * 3)
+    p0 = 1.0 / (GAMMA * mach ** 2)
+    R = p0
+    cv = R / (GAMMA - 1.0)
+    cp = GAMMA * cv
+    prob = L.problem_desc(equation=L.EQ_NS, convective_flux=L.FLUX_HLLC, viscous_flux=L.VISC_BR2,
+                          time_integration=L.RK_SSPRK3, transport=L.TRANSPORT_CONSTANT,
+                          cp=cp, cv=cv, dynamic_viscosity=1.0 / reynolds)
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 3)
+    x, y, z = xq[..., 0], xq[..., 1], xq[..., 2]
+    p = p0 + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0) / 16.0
+    prim = np.zeros((mesh.E, mesh.nq, 5))
+    prim[..., 0] = p / (R * 1.0)
+    prim[..., 1] = np.sin(x) * np.cos(y) * np.cos(z)
+    prim[..., 2] = -np.cos(x) * np.sin(y) * np.cos(z)
+    prim[..., 4] = 1.0
+    return mesh, prob, prim
+
+
+def algorithmic_counts(mesh, ns=True):
+    """Dense reference-formulation counts per element-stage (F_alg, B_alg FP64) and the split of
+    F_alg between the two Navier-Stokes stage kernels."""
+    D, nv, nb, nq = mesh.dim, mesh.nv, mesh.nb, mesh.nq
+    naq, nf = mesh.nf * mesh.nfq, mesh.nf
+    f_grad = 2 * nv * nb * (nq + D * nq) + (2 * nv * D * nb * (D * nq + nq + 3 * naq + nb * (1 + nf)) if ns else 0)
+    f_flux = 2 * nv * nb * (2 * naq + nb)
+    b_alg = 8 * (3 * nv * nb + (2 * nv * D * nb if ns else 0))
+    return f_grad, f_flux, b_alg
+
+
+def cpu_baseline(order: int, target_s: float = 15.0):
+    """The CPU oracle (C++/OpenMP restatement of the reference) on a bounded sample of the same
+    workload: an 8^3 mesh of the same element, order, physics and scheme."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # checker library, used here only as the timed CPU baseline
+
+    cores = os.cpu_count() or 1
+    threads = max(1, min(cores - 1, 16)) if cores > 1 else 1  # reference rule: physical cores - 1
+    O.set_threads(threads)
+    n = 8
+    mesh, prob, prim = tgv_setup(n, order)
+    o = O.Oracle(prob, mesh)
+    o.set_initial_primitive(prim)
+    dt = o.compute_dt(0.1)
+    t0 = time.perf_counter()
+    o.step(dt, 1)
+    one = time.perf_counter() - t0
+    steps = max(1, min(200, int(target_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    o.step(dt, steps)
+    t = time.perf_counter() - t0
+    value = mesh.E * mesh.nb * 3 * steps / t
+    return {"value": value, "unit": "DOF.stage/s", "cores": O.get_threads(), "kind": "port",
+            "sample": f"oracle, TGV {n}^3 hex P{order} HLLC+BR2 SSP-RK3 FP64, {steps} steps, {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=32, help="elements per direction")
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--cfl", type=float, default=0.1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU sharding of the element partition "
+              "is a design (see DESIGN.md) and is not launched", file=sys.stderr)
+        sys.exit(2)
+
+    t_setup = time.perf_counter()
+    mesh, prob, prim = tgv_setup(args.n, args.order)
+    solver = L.Solver(prob, mesh)
+    solver.set_initial_primitive(prim)
+    dt = solver.compute_dt(args.cfl)
+    print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s E={mesh.E} N_b={mesh.nb} dt={dt:.4e}",
+          file=sys.stderr, flush=True)
+
+    rel, nan, _ = solver.step(dt, args.warmup)
+    solver.synchronize()
+    t0 = time.perf_counter()
+    rel, nan, taken = solver.step(dt, args.steps)
+    solver.synchronize()
+    wall = time.perf_counter() - t0
+    if nan or taken != args.steps or not np.all(np.isfinite(rel)):
+        raise RuntimeError(f"bench: run did not complete cleanly (nan={nan}, taken={taken})")
+    stages = 3
+    dof = mesh.E * mesh.nb
+    value = dof * stages * args.steps / wall
+    ms_per_step = 1000.0 * wall / args.steps
+    print(f"[bench] {args.steps} steps in {wall:.3f}s (device {solver.last_step_ms():.1f} ms), "
+          f"monitor {rel.tolist()}", file=sys.stderr, flush=True)
+
+    # dominant kernel: average duration with HIP events on the solver stream
+    kms = solver.time_kernels(dt, max(2, min(args.steps, 10)))
+    f_grad, f_flux, b_alg = algorithmic_counts(mesh)
+    if kms[0] >= kms[1]:
+        name, ms, flops = "nsGradKernel", kms[0], f_grad
+    else:
+        name, ms, flops = "nsFluxKernel", kms[1], f_flux
+    achieved = flops * mesh.E / (ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel": name,
+                "kernel_ms": ms, "kernel_ms_all": kms.tolist()}
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.order)
+    line = {
+        "metric": "DOF.stage/s (FP64, 3-D NS Taylor-Green P3)",
+        "value": value,
+        "unit": "DOF.stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3-D compressible NS Taylor-Green vortex, periodic [0,2pi]^3, {args.n}^3 hex, "
+                               f"P{args.order}, Re 1600, Ma 0.1, HLLC + BR2, SSP-RK3, CFL {args.cfl}",
+                   "elements": mesh.E, "basis": mesh.nb, "dof_per_field": dof, "precision": "fp64",
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -211,6 +211,16 @@ int sdg_last_step_ms(const sdg_ctx* ctx, double* ms);
 /* Device synchronisation of the context stream. */
 int sdg_synchronize(sdg_ctx* ctx);
 
+/* Measurement hook (no counterpart in the reference): advance nsteps steps launching every
+ * kernel eagerly (no graph) with HIP events around each launch on the context stream, and write
+ * the average duration in milliseconds of each kernel class into avg_ms[0..n_out):
+ *   [0] stage kernel 1 (Euler: the whole stage; Navier-Stokes: gradient lifting + volume flux)
+ *   [1] stage kernel 2 (Navier-Stokes: face fluxes + residual + update; Euler: 0)
+ *   [2] residual-monitor reduction
+ * The state advances exactly as with sdg_step. */
+#define SDG_KERNEL_CLASSES 3
+int sdg_time_kernels(sdg_ctx* ctx, double dt, int nsteps, double* avg_ms, int n_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -116,6 +116,7 @@ SIGNATURES = {
     "sdg_step": (c_int, [c_void_p, c_double, c_int, c_void_p, POINTER(c_int), POINTER(c_int)]),
     "sdg_last_step_ms": (c_int, [c_void_p, POINTER(c_double)]),
     "sdg_synchronize": (c_int, [c_void_p]),
+    "sdg_time_kernels": (c_int, [c_void_p, c_double, c_int, c_void_p, c_int]),
 }
 
 _lib = None
@@ -276,6 +277,11 @@ class Solver:
         ms = c_double()
         self._check(self.lib.sdg_last_step_ms(self.handle, ctypes.byref(ms)))
         return ms.value
+
+    def time_kernels(self, dt: float, nsteps: int) -> np.ndarray:
+        out = np.zeros(3)
+        self._check(self.lib.sdg_time_kernels(self.handle, dt, nsteps, out.ctypes.data, 3))
+        return out
 
     def synchronize(self):
         self._check(self.lib.sdg_synchronize(self.handle))

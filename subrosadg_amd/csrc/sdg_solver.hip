@@ -207,8 +207,9 @@ StageArgs baseArgs(sdg_ctx* ctx) {
   return a;
 }
 
-// Enqueue one stage (mode 0: update, 1: residual only). Advances ctx->cur for mode 0.
-void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout) {
+// Enqueue one stage (mode 0: update, 1: residual only). Advances ctx->cur for mode 0. With ev
+// (four events), records ev[0]/ev[1] around the first kernel and ev[2]/ev[3] around the second.
+void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hipEvent_t* ev = nullptr) {
   StageArgs a = baseArgs(ctx);
   double c[3];
   ctx->rkCoefficients(stage < 0 ? 0 : stage, c);
@@ -220,14 +221,25 @@ void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout) {
   a.Wout = wout;
   if (mode == 1) a.status = nullptr;
   const int blocks = (ctx->E + ctx->launch.epb - 1) / ctx->launch.epb;
+  if (ev) (void)hipEventRecord(ev[0], ctx->stream);
   if (ctx->ns()) {
     if (ctx->prob.viscous_flux == SDG_VISC_BR1)
       ctx->launch.ns_grad_br1(blocks, ctx->stream, ctx->geo, ctx->phys, a);
     else
       ctx->launch.ns_grad_br2(blocks, ctx->stream, ctx->geo, ctx->phys, a);
+    if (ev) {
+      (void)hipEventRecord(ev[1], ctx->stream);
+      (void)hipEventRecord(ev[2], ctx->stream);
+    }
     ctx->launch.ns_flux(blocks, ctx->stream, ctx->geo, ctx->phys, a);
+    if (ev) (void)hipEventRecord(ev[3], ctx->stream);
   } else {
     ctx->launch.euler(blocks, ctx->stream, ctx->geo, ctx->phys, a);
+    if (ev) {
+      (void)hipEventRecord(ev[1], ctx->stream);
+      (void)hipEventRecord(ev[2], ctx->stream);
+      (void)hipEventRecord(ev[3], ctx->stream);
+    }
   }
   if (mode == 0) ctx->cur ^= 1;
 }
@@ -641,6 +653,50 @@ int sdg_last_step_ms(const sdg_ctx* ctx, double* ms) {
 int sdg_synchronize(sdg_ctx* ctx) {
   if (!ctx) return SDG_E_INVALID;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return SDG_OK;
+}
+
+int sdg_time_kernels(sdg_ctx* ctx, double dt, int nsteps, double* avg_ms, int n_out) {
+  if (!ctx || !avg_ms || n_out < 1) return fail(ctx, SDG_E_INVALID, "sdg_time_kernels: bad argument");
+  if (!ctx->has_state) return fail(ctx, SDG_E_STATE, "sdg_time_kernels: no state set");
+  if (nsteps < 1 || !(dt >= 0.0)) return fail(ctx, SDG_E_INVALID, "sdg_time_kernels: bad dt or step count");
+  const int st = ctx->stages();
+  const size_t per_step = static_cast<size_t>(4 * st + 2);
+  std::vector<hipEvent_t> ev(per_step * nsteps, nullptr);
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      for (auto& x : ev)
+        if (x) (void)hipEventDestroy(x);
+      return fail(ctx, SDG_E_DEVICE, "sdg_time_kernels: event creation failed");
+    }
+  hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
+  for (int i = 0; i < nsteps; i++) {
+    hipEvent_t* e = &ev[per_step * i];
+    for (int s = 0; s < st; s++) enqueueStage(ctx, s, dt, 0, nullptr, e + 4 * s);
+    (void)hipEventRecord(e[4 * st], ctx->stream);
+    ctx->launch.monitor(ctx->stream, ctx->mon, ctx->E, ctx->rel, ctx->status);
+    (void)hipEventRecord(e[4 * st + 1], ctx->stream);
+  }
+  hipError_t err = hipStreamSynchronize(ctx->stream);
+  double sum[SDG_KERNEL_CLASSES] = {0.0, 0.0, 0.0};
+  for (int i = 0; i < nsteps && err == hipSuccess; i++) {
+    hipEvent_t* e = &ev[per_step * i];
+    for (int s = 0; s < st; s++) {
+      float a = 0.0f, b = 0.0f;
+      (void)hipEventElapsedTime(&a, e[4 * s], e[4 * s + 1]);
+      (void)hipEventElapsedTime(&b, e[4 * s + 2], e[4 * s + 3]);
+      sum[0] += a;
+      sum[1] += b;
+    }
+    float m = 0.0f;
+    (void)hipEventElapsedTime(&m, e[4 * st], e[4 * st + 1]);
+    sum[2] += m;
+  }
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  if (err != hipSuccess) return fail(ctx, SDG_E_DEVICE, std::string("sdg_time_kernels: ") + hipGetErrorString(err));
+  const double launches[SDG_KERNEL_CLASSES] = {double(nsteps) * st, double(nsteps) * st, double(nsteps)};
+  for (int k = 0; k < n_out; k++) avg_ms[k] = k < SDG_KERNEL_CLASSES ? sum[k] / launches[k] : 0.0;
+  if (!ctx->ns() && n_out > 1) avg_ms[1] = 0.0;
   return SDG_OK;
 }
 

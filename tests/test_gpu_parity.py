@@ -13,12 +13,12 @@ from subrosadg_amd import _lib as L
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
-# Lifted-gradient coefficients are intermediate values. Their nearly constant components (e.g.
-# grad rho when rho ~ 1 + 1e-3) come out of a face/volume cancellation of O(|U| h^(D-1)) terms
-# followed by M^-1, which amplifies rounding by |U| / (h |grad U|) * cond(M): two correct FP64
-# implementations with different summation orders (the oracle and an independent numpy lifting)
-# already differ by ~1e-10 there. Residuals and states are held to TOL.
-GRAD_TOL = 1e-10
+# Lifted-gradient coefficients are intermediate values, compared with the conditioning-aware scale
+# of problems.parity_error(h=...) (at least |U|/h per variable group). Against an extended
+# precision (80-bit) evaluation of the same lifting, the oracle itself is 1e-11 off on the
+# hex-P3 Taylor-Green state (energy gradient: rho E ~ 179 carried through M^-1), so the bar for
+# two FP64 implementations is 5e-11.
+GRAD_TOL = 5e-11
 
 
 def _pair(mesh, prob, prim):
@@ -89,8 +89,8 @@ def test_euler_rk_variants(rk):
 
 
 NS_CASES = [
-    ("shear-br2-roe", lambda: P.ns_shear(n=4, order=3, flux=L.FLUX_ROE, visc=L.VISC_BR2)),
-    ("shear-br1-roe", lambda: P.ns_shear(n=4, order=2, flux=L.FLUX_ROE, visc=L.VISC_BR1)),
+    ("shear-br2-roe", lambda: P.ns_shear(n=4, order=3, flux=L.FLUX_ROE, visc=L.VISC_BR2, perturb=True)),
+    ("shear-br1-roe", lambda: P.ns_shear(n=4, order=2, flux=L.FLUX_ROE, visc=L.VISC_BR1, perturb=True)),
     ("tgv-br2-hllc-p2", lambda: P.ns_tgv(n=3, order=2, flux=L.FLUX_HLLC, visc=L.VISC_BR2)),
     ("tgv-br2-hllc-p3", lambda: P.ns_tgv(n=3, order=3, flux=L.FLUX_HLLC, visc=L.VISC_BR2)),
     ("tgv-br1-lf-p1", lambda: P.ns_tgv(n=4, order=1, flux=L.FLUX_LAX_FRIEDRICHS, visc=L.VISC_BR1)),
@@ -106,7 +106,8 @@ def test_ns_residual_and_gradient_parity(name, make):
     ro = o.residual()
     go = o.gradient()
     gg = s.gradient()
-    eg = P.parity_error(gg, go, u0, mesh.dim)
+    h = float(mesh.field("min_edge").min())
+    eg = P.parity_error(gg, go, u0, mesh.dim, h=h)
     assert eg <= GRAD_TOL, eg
     err = P.parity_error(s.residual(), ro, u0, mesh.dim)
     assert err <= TOL, err

@@ -1,0 +1,219 @@
+"""Whole-scheme checks of the CPU oracle: properties a correct DG discretisation must satisfy,
+chosen so that a dropped term, a wrong sign or a transposed operand fails one of them.
+
+* free-stream preservation: R(U) = 0 for any uniform state, every flux, Euler and NS;
+* discrete conservation of mass, momentum and energy on periodic meshes;
+* the lifted gradient (BR1 and BR2) of a continuous periodic polynomial is exact;
+* spatial convergence order ~P+1 on an advected density wave (exact solution known);
+* the decay rate of a viscous shear wave is nu k^2 and of a temperature wave kappa k^2/(rho cp);
+* the temporal order of forward Euler, Heun RK2 and SSP-RK3 is 1, 2, 3.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import problems as P
+from subrosadg_amd import _lib as L
+
+
+def totals(mesh, coeff):
+    """Domain integrals of the conserved variables: sum_e sum_b C(e,b,v) int phi_b |J|."""
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    detjw = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
+    m = detjw @ V  # E x N_b
+    return np.einsum("eb,ebv->v", m, coeff)
+
+
+def uniform_prim(mesh, rho, vel, T):
+    prim = np.zeros((mesh.E, mesh.nq, mesh.nv))
+    prim[..., 0] = rho
+    prim[..., 1:mesh.dim + 1] = vel
+    prim[..., mesh.dim + 1] = T
+    return prim
+
+
+@pytest.mark.parametrize("flux", [L.FLUX_CENTRAL, L.FLUX_LAX_FRIEDRICHS, L.FLUX_HLLC, L.FLUX_ROE])
+@pytest.mark.parametrize("case", ["euler2d", "euler3d", "ns2d-br2", "ns3d-br1"])
+def test_free_stream_preservation(case, flux):
+    if case.startswith("euler"):
+        dim = 2 if case == "euler2d" else 3
+        prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=flux, cp=3.5, cv=2.5)
+    else:
+        dim = 2 if case.startswith("ns2d") else 3
+        prob = L.problem_desc(equation=L.EQ_NS, convective_flux=flux,
+                              viscous_flux=L.VISC_BR2 if "br2" in case else L.VISC_BR1,
+                              transport=L.TRANSPORT_CONSTANT, cp=3.5, cv=2.5, dynamic_viscosity=0.05)
+    elem = L.ELEM_QUADRANGLE if dim == 2 else L.ELEM_HEXAHEDRON
+    mesh = L.Mesh(elem, 2, (3,) * dim, (0,) * dim, (1.0, 2.0, 1.5)[:dim])
+    o = O.Oracle(prob, mesh)
+    o.set_initial_primitive(uniform_prim(mesh, 1.2, np.linspace(0.3, -0.2, dim), 0.9))
+    r = o.residual()
+    # scale: the pressure flux through one face
+    assert np.abs(r).max() <= 1e-13 * 1.2 * 2.5 * 0.9 * 0.4 * 10
+
+
+@pytest.mark.parametrize("case", ["euler", "ns"])
+def test_discrete_conservation(case):
+    if case == "euler":
+        mesh, prob, prim = P.euler_vortex(n=6, order=2, flux=L.FLUX_HLLC)
+        steps = 20
+    else:
+        mesh, prob, prim = P.ns_tgv(n=3, order=2)
+        steps = 4
+    o = O.Oracle(prob, mesh)
+    o.set_initial_primitive(prim)
+    t0 = totals(mesh, o.get_state())
+    dt = o.compute_dt(0.2)
+    o.step(dt, steps)
+    t1 = totals(mesh, o.get_state())
+    scale = np.abs(totals(mesh, np.abs(o.get_state()))).max()
+    assert np.abs(t1 - t0).max() <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("visc", [L.VISC_BR1, L.VISC_BR2])
+@pytest.mark.parametrize("order", [2, 3])
+def test_lifted_gradient_of_continuous_polynomial_is_exact(visc, order):
+    n = 4
+    mesh = L.Mesh(L.ELEM_QUADRANGLE, order, (n, n), (-5, -5), (5, 5))
+    cv = 2.5
+    prob = L.problem_desc(equation=L.EQ_NS, convective_flux=L.FLUX_ROE, viscous_flux=visc,
+                          transport=L.TRANSPORT_CONSTANT, cp=3.5, cv=cv, dynamic_viscosity=0.01)
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 2)
+    x, y = xq[..., 0], xq[..., 1]
+    rho = 1 + 0.1 * (25 - x ** 2) * (25 - y ** 2) / 625  # continuous and periodic on [-5,5]^2
+    drho = np.stack([0.1 * (-2 * x) * (25 - y ** 2) / 625, 0.1 * (25 - x ** 2) * (-2 * y) / 625], axis=-1)
+    prim = np.zeros((mesh.E, mesh.nq, 4))
+    prim[..., 0] = rho
+    prim[..., 3] = 1.0
+    o = O.Oracle(prob, mesh)
+    o.set_initial_primitive(prim)
+    o.residual()
+    g = o.gradient()  # E x N_b x (NV*D)
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    gq = np.einsum("qb,ebr->eqr", V, g)
+    np.testing.assert_allclose(gq[..., 0:2], drho, atol=1e-12)      # grad rho
+    np.testing.assert_allclose(gq[..., 2:6], 0.0, atol=1e-12)       # grad (rho u), (rho v)
+    np.testing.assert_allclose(gq[..., 6:8], cv * drho, atol=1e-11)  # grad (rho E) = cv grad rho
+
+
+def density_wave_error(n, order, t_end=1.0):
+    mesh = L.Mesh(L.ELEM_QUADRANGLE, order, (n, n), (-5, -5), (5, 5))
+    prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=L.FLUX_ROE, cp=3.5, cv=2.5)
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 2)
+
+    def rho_exact(t):
+        return 1 + 0.2 * np.sin(2 * math.pi * (xq[..., 0] + xq[..., 1] - 2 * t) / 10)
+    prim = np.zeros((mesh.E, mesh.nq, 4))
+    prim[..., 0] = rho_exact(0.0)
+    prim[..., 1] = 1.0
+    prim[..., 2] = 1.0
+    prim[..., 3] = 1.0 / prim[..., 0]  # p = rho T = 1 (R = 1)
+    o = O.Oracle(prob, mesh)
+    o.set_initial_primitive(prim)
+    dt0 = o.compute_dt(0.1)
+    nsteps = int(math.ceil(t_end / dt0))
+    o.step(t_end / nsteps, nsteps)
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    detjw = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
+    rho_h = np.einsum("qb,eb->eq", V, o.get_state()[..., 0])
+    return math.sqrt(np.sum((rho_h - rho_exact(t_end)) ** 2 * detjw))
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_spatial_convergence_order(order):
+    e1 = density_wave_error(4, order)
+    e2 = density_wave_error(8, order)
+    rate = math.log2(e1 / e2)
+    assert rate >= order + 0.6, (e1, e2, rate)
+
+
+def test_viscous_shear_wave_decay():
+    """u = eps sin(2 pi y), v = 0, uniform rho and p: the x-momentum obeys rho u_t = mu u_yy
+    exactly (viscous heating is O(eps^2)), so the amplitude decays as exp(-nu k^2 t)."""
+    mu, n, order, eps, t_end = 0.01, 4, 3, 0.01, 0.2
+    mesh = L.Mesh(L.ELEM_QUADRANGLE, order, (n, n), (0, 0), (1, 1))
+    cp, cv = P.tgv_thermo(0.1)
+    prob = L.problem_desc(equation=L.EQ_NS, convective_flux=L.FLUX_ROE, viscous_flux=L.VISC_BR2,
+                          transport=L.TRANSPORT_CONSTANT, cp=cp, cv=cv, dynamic_viscosity=mu)
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 2)
+    prim = np.zeros((mesh.E, mesh.nq, 4))
+    prim[..., 0] = 1.0
+    prim[..., 1] = eps * np.sin(2 * math.pi * xq[..., 1])
+    prim[..., 3] = 1.0
+    o = O.Oracle(prob, mesh)
+    o.set_initial_primitive(prim)
+    dt0 = o.compute_dt(0.1)
+    nsteps = int(math.ceil(t_end / dt0))
+    o.step(t_end / nsteps, nsteps)
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    detjw = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
+    mom = np.einsum("qb,eb->eq", V, o.get_state()[..., 1])
+    amp = 2 * np.sum(mom * np.sin(2 * math.pi * xq[..., 1]) * detjw)
+    expect = eps * math.exp(-mu * (2 * math.pi) ** 2 * t_end)
+    assert abs(amp / expect - 1) < 1e-3, (amp, expect)
+
+
+def test_heat_conduction_decay():
+    """T = 1 + eps sin(2 pi x) at uniform pressure and rest: at low Mach the temperature decays
+    isobarically at kappa k^2 / (rho cp) with kappa = cp mu / Pr."""
+    mu, n, order, eps, t_end = 0.01, 4, 3, 0.01, 0.2
+    mesh = L.Mesh(L.ELEM_QUADRANGLE, order, (n, n), (0, 0), (1, 1))
+    cp, cv = P.tgv_thermo(0.05)
+    R = cp - cv
+    prob = L.problem_desc(equation=L.EQ_NS, convective_flux=L.FLUX_ROE, viscous_flux=L.VISC_BR2,
+                          transport=L.TRANSPORT_CONSTANT, cp=cp, cv=cv, dynamic_viscosity=mu)
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 2)
+    T = 1 + eps * np.sin(2 * math.pi * xq[..., 0])
+    p0 = R * 1.0
+    prim = np.zeros((mesh.E, mesh.nq, 4))
+    prim[..., 0] = p0 / (R * T)
+    prim[..., 3] = T
+    o = O.Oracle(prob, mesh)
+    o.set_initial_primitive(prim)
+    dt0 = o.compute_dt(0.1)
+    nsteps = int(math.ceil(t_end / dt0))
+    o.step(t_end / nsteps, nsteps)
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    detjw = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
+    c = o.get_state()
+    rho = np.einsum("qb,eb->eq", V, c[..., 0])
+    rhoE = np.einsum("qb,eb->eq", V, c[..., 3])
+    mom = np.stack([np.einsum("qb,eb->eq", V, c[..., k]) for k in (1, 2)], axis=-1)
+    Th = (rhoE / rho - 0.5 * np.sum((mom / rho[..., None]) ** 2, axis=-1)) / cv
+    amp = 2 * np.sum(Th * np.sin(2 * math.pi * xq[..., 0]) * detjw)
+    rate = cp * mu / 0.71 * (2 * math.pi) ** 2 / (1.0 * cp)
+    expect = eps * math.exp(-rate * t_end)
+    assert abs(amp / expect - 1) < 1e-2, (amp, expect)
+
+
+@pytest.mark.parametrize("rk,order", [(L.RK_FORWARD_EULER, 1), (L.RK_HEUN2, 2), (L.RK_SSPRK3, 3)])
+def test_temporal_order(rk, order):
+    mesh, prob, prim = P.euler_vortex(n=4, order=1, flux=L.FLUX_ROE, rk=rk)
+    o = O.Oracle(prob, mesh)
+    o.set_initial_primitive(prim)
+    c0 = o.get_state()
+    t_end = 0.4
+    dt = o.compute_dt(0.2)
+    results = []
+    for k in (1, 2, 4, 16):
+        o.set_state(c0)
+        nsteps = int(math.ceil(t_end / dt)) * k
+        o.step(t_end / nsteps, nsteps)
+        results.append(o.get_state())
+    ref = results[-1]
+    e = [np.abs(r - ref).max() for r in results[:3]]
+    rate = math.log2(e[1] / e[2])
+    assert abs(rate - order) < 0.35, (e, rate)
+
+
+def test_nan_stops_oracle():
+    mesh, prob, prim = P.euler_vortex(n=4, order=1)
+    o = O.Oracle(prob, mesh)
+    o.set_initial_primitive(prim)
+    c = o.get_state()
+    c[:] = np.nan
+    o.set_state(c)
+    rel, nan, taken = o.step(1e-3, 5)
+    assert nan == 1 and taken == 1 and np.isnan(rel).all()
