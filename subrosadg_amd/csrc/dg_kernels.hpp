@@ -24,6 +24,20 @@ namespace sdg {
 
 constexpr int kThreads = 256;
 
+constexpr int r16(int x) { return (x + 15) / 16 * 16; }
+constexpr int r4(int x) { return (x + 3) / 4 * 4; }
+
+// Padded, transposed operator tables T[K4][M16] for the matrix-core kernels (zero outside the
+// operator): Out[m][n] = sum_k T[k][m] X[k][n].
+struct MOps {
+  const double* interp;  // K = N_b,   M = N_q   : basis values          (U -> U_q, w -> grad U_q, R -> R_q)
+  const double* grad;    // K = N_q D, M = N_b   : basis gradients       (weak divergence, volume gradient term)
+  const double* face;    // K = N_aq,  M = N_b   : face values           (face terms of residuals)
+  const double* minv;    // K = N_b,   M = N_b   : reference mass inverse
+  const double* trace;   // K = N_b,   M = N_aq  : face values           (coefficients -> face traces)
+  const double* lift;    // K = N_aq,  M = N_aq  : block-diagonal Vf_s Mref^-1 Vf_s^T (BR2 face lift)
+};
+
 struct Geo {
   const double* Vt;    // N_b x N_q        basis at volume points, transposed
   const double* dV;    // (N_q D) x N_b    reference basis gradient, row q*D+d
@@ -43,6 +57,7 @@ struct Geo {
   const int* enbf;        // E x N_f        neighbour's local face
   const int* eside;       // E x N_f        0: this element is the face's left parent
   int E;
+  MOps op;
 };
 
 struct StageArgs {

@@ -16,6 +16,7 @@
 
 #include "../../include/sdg.h"
 #include "dg_kernels.hpp"
+#include "dg_mfma.hpp"
 #include "kit.hpp"
 #include "sdg_internal.hpp"
 
@@ -25,13 +26,13 @@ using sdg::StageArgs;
 
 namespace {
 
-using StageFn = void (*)(int blocks, hipStream_t, const Geo&, const Phys&, const StageArgs&);
+using StageFn = void (*)(hipStream_t, const Geo&, const Phys&, const StageArgs&);
 using TraceFn = void (*)(hipStream_t, const Geo&, const double*, double*);
 using DtFn = void (*)(hipStream_t, const Geo&, const Phys&, const double*, const double*, double, int, double*);
 using MonFn = void (*)(hipStream_t, const double*, int, double*, int*);
 
 struct Launchers {
-  int epb = 1;
+  bool mfma = false;
   StageFn euler = nullptr;
   StageFn ns_grad_br1 = nullptr;
   StageFn ns_grad_br2 = nullptr;
@@ -41,28 +42,52 @@ struct Launchers {
   MonFn monitor = nullptr;
 };
 
-template <int D, int NB, int NQ, int NF, int NFQ>
+inline int blocksFor(int E, int epb) { return (E + epb - 1) / epb; }
+
+// VALU kernels: any size; MFMA kernels: bases that fill 16x16 tiles
+template <int D, int NB, int NQ, int NF, int NFQ, bool MFMA>
 Launchers makeLaunchers() {
   using S = sdg::Sizes<D, NB, NQ, NF, NFQ>;
   constexpr int EPB = S::EPB;
+  constexpr int NV = D + 2;
   Launchers l;
-  l.epb = EPB;
-  l.euler = [](int blocks, hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-    hipLaunchKernelGGL((sdg::eulerStageKernel<D, NB, NQ, NF, NFQ, EPB>), dim3(blocks), dim3(sdg::kThreads), 0, st, g,
-                       ph, a);
-  };
-  l.ns_grad_br1 = [](int blocks, hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-    hipLaunchKernelGGL((sdg::nsGradKernel<D, NB, NQ, NF, NFQ, EPB, 1>), dim3(blocks), dim3(sdg::kThreads), 0, st, g,
-                       ph, a);
-  };
-  l.ns_grad_br2 = [](int blocks, hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-    hipLaunchKernelGGL((sdg::nsGradKernel<D, NB, NQ, NF, NFQ, EPB, 2>), dim3(blocks), dim3(sdg::kThreads), 0, st, g,
-                       ph, a);
-  };
-  l.ns_flux = [](int blocks, hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-    hipLaunchKernelGGL((sdg::nsFluxKernel<D, NB, NQ, NF, NFQ, EPB>), dim3(blocks), dim3(sdg::kThreads), 0, st, g, ph,
-                       a);
-  };
+  l.mfma = MFMA;
+  if constexpr (MFMA) {
+    constexpr int EPB_S = 16 / NV, EPB_G = 16 / (NV * D);
+    l.euler = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+      hipLaunchKernelGGL((sdg::eulerStageMfma<D, NB, NQ, NF, NFQ>), dim3(blocksFor(g.E, EPB_S)),
+                         dim3(sdg::kThreads), 0, st, g, ph, a);
+    };
+    l.ns_grad_br1 = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+      hipLaunchKernelGGL((sdg::nsGradMfma<D, NB, NQ, NF, NFQ, 1>), dim3(blocksFor(g.E, EPB_G)), dim3(sdg::kThreads),
+                         0, st, g, ph, a);
+    };
+    l.ns_grad_br2 = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+      hipLaunchKernelGGL((sdg::nsGradMfma<D, NB, NQ, NF, NFQ, 2>), dim3(blocksFor(g.E, EPB_G)), dim3(sdg::kThreads),
+                         0, st, g, ph, a);
+    };
+    l.ns_flux = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+      hipLaunchKernelGGL((sdg::nsFluxMfma<D, NB, NQ, NF, NFQ>), dim3(blocksFor(g.E, EPB_S)), dim3(sdg::kThreads), 0,
+                         st, g, ph, a);
+    };
+  } else {
+    l.euler = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+      hipLaunchKernelGGL((sdg::eulerStageKernel<D, NB, NQ, NF, NFQ, EPB>), dim3(blocksFor(g.E, EPB)),
+                         dim3(sdg::kThreads), 0, st, g, ph, a);
+    };
+    l.ns_grad_br1 = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+      hipLaunchKernelGGL((sdg::nsGradKernel<D, NB, NQ, NF, NFQ, EPB, 1>), dim3(blocksFor(g.E, EPB)),
+                         dim3(sdg::kThreads), 0, st, g, ph, a);
+    };
+    l.ns_grad_br2 = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+      hipLaunchKernelGGL((sdg::nsGradKernel<D, NB, NQ, NF, NFQ, EPB, 2>), dim3(blocksFor(g.E, EPB)),
+                         dim3(sdg::kThreads), 0, st, g, ph, a);
+    };
+    l.ns_flux = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+      hipLaunchKernelGGL((sdg::nsFluxKernel<D, NB, NQ, NF, NFQ, EPB>), dim3(blocksFor(g.E, EPB)), dim3(sdg::kThreads),
+                         0, st, g, ph, a);
+    };
+  }
   l.trace = [](hipStream_t st, const Geo& g, const double* U, double* Uf) {
     const int64_t n = static_cast<int64_t>(g.E) * (D + 2) * NF * NFQ;
     hipLaunchKernelGGL((sdg::traceKernel<D, NB, NQ, NF, NFQ>), dim3((n + sdg::kThreads - 1) / sdg::kThreads),
@@ -79,27 +104,49 @@ Launchers makeLaunchers() {
   return l;
 }
 
-bool selectLaunchers(int type, int order, Launchers& l) {
+bool selectLaunchers(int type, int order, bool force_valu, Launchers& l) {
   if (type == SDG_ELEM_QUADRANGLE) {
     switch (order) {
-      case 1: l = makeLaunchers<2, 4, 4, 4, 2>(); return true;
-      case 2: l = makeLaunchers<2, 9, 9, 4, 3>(); return true;
-      case 3: l = makeLaunchers<2, 16, 16, 4, 4>(); return true;
-      case 4: l = makeLaunchers<2, 25, 25, 4, 5>(); return true;
-      case 5: l = makeLaunchers<2, 36, 36, 4, 6>(); return true;
+      case 1: l = makeLaunchers<2, 4, 4, 4, 2, false>(); return true;
+      case 2: l = makeLaunchers<2, 9, 9, 4, 3, false>(); return true;
+      case 3:
+        l = force_valu ? makeLaunchers<2, 16, 16, 4, 4, false>() : makeLaunchers<2, 16, 16, 4, 4, true>();
+        return true;
+      case 4:
+        l = force_valu ? makeLaunchers<2, 25, 25, 4, 5, false>() : makeLaunchers<2, 25, 25, 4, 5, true>();
+        return true;
+      case 5:
+        l = force_valu ? makeLaunchers<2, 36, 36, 4, 6, false>() : makeLaunchers<2, 36, 36, 4, 6, true>();
+        return true;
       default: return false;
     }
   }
   if (type == SDG_ELEM_HEXAHEDRON) {
     switch (order) {
-      case 1: l = makeLaunchers<3, 8, 8, 6, 4>(); return true;
-      case 2: l = makeLaunchers<3, 27, 27, 6, 9>(); return true;
-      case 3: l = makeLaunchers<3, 64, 64, 6, 16>(); return true;
-      case 4: l = makeLaunchers<3, 125, 125, 6, 25>(); return true;
+      case 1: l = makeLaunchers<3, 8, 8, 6, 4, false>(); return true;
+      case 2:
+        l = force_valu ? makeLaunchers<3, 27, 27, 6, 9, false>() : makeLaunchers<3, 27, 27, 6, 9, true>();
+        return true;
+      case 3:
+        l = force_valu ? makeLaunchers<3, 64, 64, 6, 16, false>() : makeLaunchers<3, 64, 64, 6, 16, true>();
+        return true;
+      case 4:
+        l = force_valu ? makeLaunchers<3, 125, 125, 6, 25, false>() : makeLaunchers<3, 125, 125, 6, 25, true>();
+        return true;
       default: return false;
     }
   }
   return false;
+}
+
+// Zero-padded transposed operator T[r4(K)][r16(M)] with T[k][m] = src(m, k).
+template <typename F>
+std::vector<double> paddedOperator(int K, int M, F src) {
+  const int Kp = sdg::r4(K), Mp = sdg::r16(M);
+  std::vector<double> t(static_cast<size_t>(Kp) * Mp, 0.0);
+  for (int k = 0; k < K; k++)
+    for (int m = 0; m < M; m++) t[static_cast<size_t>(k) * Mp + m] = src(m, k);
+  return t;
 }
 
 }  // namespace
@@ -220,21 +267,20 @@ void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hi
   a.mode = mode;
   a.Wout = wout;
   if (mode == 1) a.status = nullptr;
-  const int blocks = (ctx->E + ctx->launch.epb - 1) / ctx->launch.epb;
   if (ev) (void)hipEventRecord(ev[0], ctx->stream);
   if (ctx->ns()) {
     if (ctx->prob.viscous_flux == SDG_VISC_BR1)
-      ctx->launch.ns_grad_br1(blocks, ctx->stream, ctx->geo, ctx->phys, a);
+      ctx->launch.ns_grad_br1(ctx->stream, ctx->geo, ctx->phys, a);
     else
-      ctx->launch.ns_grad_br2(blocks, ctx->stream, ctx->geo, ctx->phys, a);
+      ctx->launch.ns_grad_br2(ctx->stream, ctx->geo, ctx->phys, a);
     if (ev) {
       (void)hipEventRecord(ev[1], ctx->stream);
       (void)hipEventRecord(ev[2], ctx->stream);
     }
-    ctx->launch.ns_flux(blocks, ctx->stream, ctx->geo, ctx->phys, a);
+    ctx->launch.ns_flux(ctx->stream, ctx->geo, ctx->phys, a);
     if (ev) (void)hipEventRecord(ev[3], ctx->stream);
   } else {
-    ctx->launch.euler(blocks, ctx->stream, ctx->geo, ctx->phys, a);
+    ctx->launch.euler(ctx->stream, ctx->geo, ctx->phys, a);
     if (ev) {
       (void)hipEventRecord(ev[1], ctx->stream);
       (void)hipEventRecord(ev[2], ctx->stream);
@@ -336,7 +382,8 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   ctx->type = r.type;
   ctx->E = static_cast<int>(mesh.elements);
   ctx->F = static_cast<int64_t>(mesh.faces.size());
-  if (!selectLaunchers(r.type, r.order, ctx->launch)) {
+  const char* fv = std::getenv("SDG_FORCE_VALU");
+  if (!selectLaunchers(r.type, r.order, fv && fv[0] == '1', ctx->launch)) {
     delete ctx;
     return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: no kernel instantiation for this element type and order");
   }
@@ -456,6 +503,30 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   g.enbf = denbf;
   g.eside = deside;
   g.E = ctx->E;
+  {
+    const int nfq_ = nfq;
+    auto V = [&](int q, int b) { return r.V[static_cast<size_t>(q) * nb + b]; };
+    auto Vfv = [&](int sl, int b) { return r.Vf[static_cast<size_t>(sl) * nb + b]; };
+    std::vector<double> t_interp = paddedOperator(nb, nq, [&](int m, int k) { return V(m, k); });
+    std::vector<double> t_grad = paddedOperator(nq * D, nb, [&](int m, int k) { return r.dV[static_cast<size_t>(k) * nb + m]; });
+    std::vector<double> t_face = paddedOperator(naq, nb, [&](int m, int k) { return Vfv(k, m); });
+    std::vector<double> t_minv = paddedOperator(nb, nb, [&](int m, int k) { return r.mass_inverse[static_cast<size_t>(k) * nb + m]; });
+    std::vector<double> t_trace = paddedOperator(nb, naq, [&](int m, int k) { return Vfv(m, k); });
+    std::vector<double> t_lift = paddedOperator(naq, naq, [&](int m, int k) {
+      const int sm = m / nfq_, sk = k / nfq_;
+      return sm == sk ? Lf[(static_cast<size_t>(sm) * nfq_ + k % nfq_) * nfq_ + m % nfq_] : 0.0;
+    });
+    double *pi, *pg, *pf, *pm, *pt, *pl;
+    if (upload(ctx, t_interp, &pi) || upload(ctx, t_grad, &pg) || upload(ctx, t_face, &pf) || upload(ctx, t_minv, &pm) ||
+        upload(ctx, t_trace, &pt) || upload(ctx, t_lift, &pl))
+      return bail(SDG_E_NOMEM);
+    g.op.interp = pi;
+    g.op.grad = pg;
+    g.op.face = pf;
+    g.op.minv = pm;
+    g.op.trace = pt;
+    g.op.lift = pl;
+  }
   ctx->minedge = dme;
   ctx->h_lsq = r.lsq_inverse;
   ctx->h_V = r.V;
@@ -583,11 +654,10 @@ int sdg_get_gradient(sdg_ctx* ctx, double* gradient) {
   a.mode = 1;
   a.status = nullptr;
   a.Wout = ctx->Wout;
-  const int blocks = (ctx->E + ctx->launch.epb - 1) / ctx->launch.epb;
   if (ctx->prob.viscous_flux == SDG_VISC_BR1)
-    ctx->launch.ns_grad_br1(blocks, ctx->stream, ctx->geo, ctx->phys, a);
+    ctx->launch.ns_grad_br1(ctx->stream, ctx->geo, ctx->phys, a);
   else
-    ctx->launch.ns_grad_br2(blocks, ctx->stream, ctx->geo, ctx->phys, a);
+    ctx->launch.ns_grad_br2(ctx->stream, ctx->geo, ctx->phys, a);
   HIP_TRY(ctx, hipGetLastError());
   const size_t n = static_cast<size_t>(ctx->E) * ctx->ng * ctx->nb;
   HIP_TRY(ctx, hipMemcpyAsync(gradient, ctx->Wout, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

@@ -143,3 +143,23 @@ def test_nan_stops_stepping():
     s.set_initial_primitive(prim)
     rel, nan, taken = s.step(1e-3, 3)
     assert nan == 0 and taken == 3
+
+
+@pytest.mark.parametrize("make", [lambda: P.ns_tgv(n=3, order=3), lambda: P.ns_shear(n=4, order=3, perturb=True),
+                                  lambda: P.euler_vortex(n=6, order=3)])
+def test_matrix_core_and_vector_paths_agree(make, monkeypatch):
+    """The MFMA (matrix-core) stage kernels and the VALU kernels give the same residual and steps."""
+    mesh, prob, prim = make()
+    s_mfma = L.Solver(prob, mesh)
+    s_mfma.set_initial_primitive(prim)
+    monkeypatch.setenv("SDG_FORCE_VALU", "1")
+    s_valu = L.Solver(prob, mesh)
+    monkeypatch.delenv("SDG_FORCE_VALU")
+    s_valu.set_state(s_mfma.get_state())
+    u0 = s_mfma.get_state()
+    assert P.parity_error(s_mfma.residual(), s_valu.residual(), u0, mesh.dim) <= TOL
+    dt = s_mfma.compute_dt(0.1)
+    s_mfma.step(dt, 3)
+    s_valu.step(dt, 3)
+    u1 = s_valu.get_state()
+    assert P.parity_error(s_mfma.get_state(), u1, u1, mesh.dim) <= TOL
