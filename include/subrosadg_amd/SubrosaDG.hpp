@@ -1,0 +1,263 @@
+// SubrosaDG.hpp -- C++ drop-in for the reference's compile-time configuration and System driver
+// (src/Solver/SimulationControl.cpp:1197-1279, src/Utils/SystemControl.cpp:42-236), running the
+// hot path on the GPU through the C-ABI of libsdg_hip.so (sdg.h).
+//
+// Kept from the reference: the enum vocabulary (src/Utils/Enum.cpp:22-233), the
+// SimulationControl<SolveControl<...>, NumericalControl<...>, *Variable<...>> composition, the
+// System setters and solve() loop with the all-NaN stop (SystemControl.cpp:159-195), and the
+// user-specialised InitialCondition<SC>::calculatePrimitiveFromCoordinate.
+// Changed: coordinates and primitive states are std::array instead of Eigen vectors; meshes are
+// structured periodic boxes (setMesh(StructuredMesh)) instead of gmsh files; raw-binary and VTU
+// output are not produced (the monitor history is kept in memory and can be streamed).
+#ifndef SUBROSA_DG_AMD_HPP_
+#define SUBROSA_DG_AMD_HPP_
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../sdg.h"
+
+namespace SubrosaDG {
+
+using Real = double;
+using Isize = int;
+
+enum class DimensionEnum { D1 = 1, D2, D3 };
+enum class PolynomialOrderEnum { P1 = 1, P2, P3, P4, P5 };
+enum class MeshModelEnum {
+  Line, Triangle, Quadrangle, TriangleQuadrangle, Tetrahedron, Hexahedron, TetrahedronPyramidHexahedron
+};
+enum class EquationModelEnum { CompresibleEuler, CompresibleNS, IncompresibleEuler, IncompresibleNS, CompresibleRANS };
+enum class SourceTermEnum { None, Boussinesq };
+enum class ShockCapturingEnum { None, ArtificialViscosity };
+enum class LimiterEnum { None, PositivityPreserving };
+enum class InitialConditionEnum { Function, SpecificFile, LastStep };
+enum class BoundaryConditionEnum {
+  RiemannFarfield, VelocityInflow, PressureOutflow, IsoThermalNonSlipWall, AdiabaticSlipWall, AdiabaticNonSlipWall,
+  Periodic
+};
+enum class BoundaryTimeEnum { Steady, TimeVarying };
+enum class ConvectiveFluxEnum { Central, LaxFriedrichs, HLLC, Roe, Exact };
+enum class ViscousFluxEnum { None, BR1, BR2 };
+enum class ThermodynamicModelEnum { Constant };
+enum class EquationOfStateEnum { IdealGas, WeakCompressibleFluid };
+enum class TransportModelEnum { None, Constant, Sutherland };
+enum class TimeIntegrationEnum { ForwardEuler, HeunRK2, SSPRK3 };
+
+template <DimensionEnum Dimension, PolynomialOrderEnum PolynomialOrder, BoundaryTimeEnum BoundaryTimeType,
+          SourceTermEnum SourceTermType>
+struct SolveControl {
+  inline static constexpr int kDimension{static_cast<int>(Dimension)};
+  inline static constexpr int kPolynomialOrder{static_cast<int>(PolynomialOrder)};
+  inline static constexpr BoundaryTimeEnum kBoundaryTime{BoundaryTimeType};
+  inline static constexpr SourceTermEnum kSourceTerm{SourceTermType};
+};
+
+template <MeshModelEnum MeshModelType, ShockCapturingEnum ShockCapturingType, LimiterEnum LimiterType,
+          InitialConditionEnum InitialConditionType, TimeIntegrationEnum TimeIntegrationType>
+struct NumericalControl {
+  inline static constexpr MeshModelEnum kMeshModel{MeshModelType};
+  inline static constexpr InitialConditionEnum kInitialCondition{InitialConditionType};
+  inline static constexpr ShockCapturingEnum kShockCapturing{ShockCapturingType};
+  inline static constexpr LimiterEnum kLimiter{LimiterType};
+  inline static constexpr TimeIntegrationEnum kTimeIntegration{TimeIntegrationType};
+};
+
+template <ThermodynamicModelEnum ThermodynamicModelType, EquationOfStateEnum EquationOfStateType,
+          ConvectiveFluxEnum ConvectiveFluxType>
+struct CompresibleEulerVariable {
+  inline static constexpr EquationModelEnum kEquationModel{EquationModelEnum::CompresibleEuler};
+  inline static constexpr ThermodynamicModelEnum kThermodynamicModel{ThermodynamicModelType};
+  inline static constexpr EquationOfStateEnum kEquationOfState{EquationOfStateType};
+  inline static constexpr TransportModelEnum kTransportModel{TransportModelEnum::None};
+  inline static constexpr ConvectiveFluxEnum kConvectiveFlux{ConvectiveFluxType};
+  inline static constexpr ViscousFluxEnum kViscousFlux{ViscousFluxEnum::None};
+};
+
+template <ThermodynamicModelEnum ThermodynamicModelType, EquationOfStateEnum EquationOfStateType,
+          TransportModelEnum TransportModelType, ConvectiveFluxEnum ConvectiveFluxType, ViscousFluxEnum ViscousFluxType>
+struct CompresibleNSVariable {
+  inline static constexpr EquationModelEnum kEquationModel{EquationModelEnum::CompresibleNS};
+  inline static constexpr ThermodynamicModelEnum kThermodynamicModel{ThermodynamicModelType};
+  inline static constexpr EquationOfStateEnum kEquationOfState{EquationOfStateType};
+  inline static constexpr TransportModelEnum kTransportModel{TransportModelType};
+  inline static constexpr ConvectiveFluxEnum kConvectiveFlux{ConvectiveFluxType};
+  inline static constexpr ViscousFluxEnum kViscousFlux{ViscousFluxType};
+};
+
+template <typename SolveControlT, typename NumericalControlT, typename EquationVariable>
+struct SimulationControl : SolveControlT, NumericalControlT, EquationVariable {
+  inline static constexpr int kConservedVariableNumber{SolveControlT::kDimension + 2};
+  inline static constexpr int kComputationalVariableNumber{SolveControlT::kDimension + 3};
+  inline static constexpr int kPrimitiveVariableNumber{SolveControlT::kDimension + 2};
+};
+
+// User hook, specialised per example like the reference's InitialCondition<SC>
+// (InitialCondition.cpp:85-116): primitive (rho, u, v[, w], T) at a physical point.
+template <typename SC>
+struct InitialCondition {
+  std::array<Real, SC::kPrimitiveVariableNumber> calculatePrimitiveFromCoordinate(
+      const std::array<Real, SC::kDimension>& coordinate) const;
+};
+
+// Structured periodic box (replaces the gmsh generate-mesh callback of the examples).
+struct StructuredMesh {
+  std::array<int, 3> cells{2, 2, 2};
+  std::array<Real, 3> lower{0.0, 0.0, 0.0};
+  std::array<Real, 3> upper{1.0, 1.0, 1.0};
+};
+
+template <typename SC>
+struct System {
+  static_assert(SC::kEquationModel == EquationModelEnum::CompresibleEuler ||
+                    SC::kEquationModel == EquationModelEnum::CompresibleNS,
+                "the GPU engine implements the compressible Euler and Navier-Stokes models");
+  static_assert(SC::kShockCapturing == ShockCapturingEnum::None, "artificial viscosity is not implemented");
+  static_assert(SC::kSourceTerm == SourceTermEnum::None, "source terms are not implemented");
+  static_assert(SC::kInitialCondition == InitialConditionEnum::Function,
+                "initial conditions come from InitialCondition<SC>::calculatePrimitiveFromCoordinate");
+
+  InitialCondition<SC> initial_condition_;
+  StructuredMesh mesh_desc_;
+  bool mesh_set_ = false;
+  sdg_problem_desc problem_{};
+  sdg_mesh* mesh_ = nullptr;
+  sdg_ctx* ctx_ = nullptr;
+  int iteration_start_ = 0, iteration_end_ = 0;
+  Real cfl_ = 0.0, delta_time_ = 0.0;
+  int io_interval_ = 0;
+  std::vector<std::array<Real, SC::kConservedVariableNumber>> relative_error_history_;
+  std::function<void(int, const std::array<Real, SC::kConservedVariableNumber>&)> on_step_;
+
+  System() {
+    problem_.abi_version = SDG_ABI_VERSION;
+    problem_.equation = SC::kEquationModel == EquationModelEnum::CompresibleNS ? SDG_EQ_COMPRESSIBLE_NS
+                                                                                : SDG_EQ_COMPRESSIBLE_EULER;
+    problem_.convective_flux = static_cast<int>(SC::kConvectiveFlux);
+    problem_.viscous_flux = static_cast<int>(SC::kViscousFlux);
+    problem_.time_integration = static_cast<int>(SC::kTimeIntegration);
+    problem_.equation_of_state = static_cast<int>(SC::kEquationOfState);
+    problem_.transport = static_cast<int>(SC::kTransportModel);
+    problem_.precision = SDG_PREC_FP64;
+  }
+  ~System() {
+    if (ctx_) sdg_destroy(ctx_);
+    if (mesh_) sdg_mesh_destroy(mesh_);
+  }
+  System(const System&) = delete;
+  System& operator=(const System&) = delete;
+
+  static void check(int rc, const sdg_ctx* ctx) {
+    if (rc != SDG_OK) throw std::runtime_error(std::string("SubrosaDG: ") + sdg_last_error(ctx));
+  }
+
+  // SystemControl.cpp:55-61 (structured periodic box instead of a gmsh file)
+  void setMesh(const StructuredMesh& mesh) {
+    mesh_desc_ = mesh;
+    mesh_set_ = true;
+  }
+  // SystemControl.cpp:76-80: only periodic boundaries exist on a structured periodic box
+  template <BoundaryConditionEnum BoundaryConditionType>
+  void addBoundaryCondition([[maybe_unused]] const Isize physical_index) {
+    static_assert(BoundaryConditionType == BoundaryConditionEnum::Periodic,
+                  "structured meshes are periodic; wall and far-field boundaries are not implemented");
+  }
+  // SystemControl.cpp:82-88
+  template <ThermodynamicModelEnum ThermodynamicModelType>
+  void setThermodynamicModel(const Real specific_heat_constant_pressure, const Real specific_heat_constant_volume) {
+    problem_.cp = specific_heat_constant_pressure;
+    problem_.cv = specific_heat_constant_volume;
+  }
+  // SystemControl.cpp:90-96
+  template <EquationOfStateEnum EquationOfStateType>
+  void setEquationOfState(const Real reference_sound_speed, const Real reference_density) {
+    problem_.reference_sound_speed = reference_sound_speed;
+    problem_.reference_density = reference_density;
+  }
+  // SystemControl.cpp:98-103 (thermal conductivity cp mu / Pr is derived inside the engine)
+  template <TransportModelEnum TransportModelType>
+  void setTransportModel(const Real dynamic_viscosity) {
+    problem_.dynamic_viscosity = dynamic_viscosity;
+  }
+  // SystemControl.cpp:110-121 (no interactive prompt: the range is required)
+  void setTimeIntegration(const Real courant_friedrichs_lewy_number, const std::pair<int, int> iteration_range) {
+    cfl_ = courant_friedrichs_lewy_number;
+    iteration_start_ = iteration_range.first;
+    iteration_end_ = iteration_range.second;
+  }
+  void setDeltaTime(const Real delta_time) { delta_time_ = delta_time; }
+  // SystemControl.cpp:125-140: only the interval is kept (no files are written)
+  void setViewConfig(const std::string& /*output_directory*/, const std::string& /*prefix*/, const int io_interval = -1) {
+    io_interval_ = io_interval;
+  }
+  // Called after every step with the step number and the residual monitor.
+  void setStepCallback(std::function<void(int, const std::array<Real, SC::kConservedVariableNumber>&)> f) {
+    on_step_ = std::move(f);
+  }
+
+  // SystemControl.cpp:146-157: build the mesh and the device context
+  void synchronize() {
+    if (!mesh_set_) throw std::runtime_error("SubrosaDG: setMesh was not called");
+    sdg_structured_desc d{};
+    d.element = SC::kDimension == 2 ? SDG_ELEM_QUADRANGLE : SDG_ELEM_HEXAHEDRON;
+    d.polynomial_order = SC::kPolynomialOrder;
+    for (int i = 0; i < 3; i++) {
+      d.n[i] = mesh_desc_.cells[i];
+      d.lo[i] = mesh_desc_.lower[i];
+      d.hi[i] = mesh_desc_.upper[i];
+    }
+    check(sdg_mesh_create_structured(&d, &mesh_), nullptr);
+    check(sdg_create(&problem_, mesh_, &ctx_), nullptr);
+  }
+
+  // SystemControl.cpp:159-195
+  void solve() {
+    if (!ctx_) synchronize();
+    constexpr int D = SC::kDimension, NV = SC::kConservedVariableNumber;
+    sdg_mesh_info info{};
+    check(sdg_mesh_get_info(mesh_, &info), nullptr);
+    const int64_t nq = info.quadrature;
+    std::vector<double> xq(static_cast<size_t>(sdg_mesh_field_size(mesh_, SDG_FIELD_QUAD_COORD)));
+    check(sdg_mesh_export(mesh_, SDG_FIELD_QUAD_COORD, xq.data(), static_cast<int64_t>(xq.size())), nullptr);
+    std::vector<double> prim(static_cast<size_t>(info.elements * nq * NV));
+    for (int64_t p = 0; p < info.elements * nq; p++) {
+      std::array<Real, D> x;
+      for (int d = 0; d < D; d++) x[d] = xq[p * D + d];
+      const auto v = initial_condition_.calculatePrimitiveFromCoordinate(x);
+      for (int k = 0; k < NV; k++) prim[p * NV + k] = v[k];
+    }
+    check(sdg_set_initial_primitive(ctx_, prim.data()), ctx_);
+    if (delta_time_ == 0.0) check(sdg_compute_dt(ctx_, cfl_, &delta_time_), ctx_);
+    for (int i = iteration_start_ + 1; i <= iteration_end_; i++) {
+      std::array<Real, NV> rel{};
+      int nan = 0, taken = 0;
+      check(sdg_step(ctx_, delta_time_, 1, rel.data(), &nan, &taken), ctx_);
+      relative_error_history_.push_back(rel);
+      if (on_step_) on_step_(i, rel);
+      if (nan) {  // all-NaN monitor: truncate the run (SystemControl.cpp:185-191)
+        iteration_end_ = i;
+        break;
+      }
+    }
+  }
+
+  // Modal coefficients, per element N_b x NV with NV fastest (the reference's Eigen layout).
+  std::vector<Real> state() const {
+    sdg_mesh_info info{};
+    check(sdg_mesh_get_info(mesh_, &info), nullptr);
+    std::vector<Real> c(static_cast<size_t>(info.elements * info.basis * SC::kConservedVariableNumber));
+    check(sdg_get_state(ctx_, c.data()), ctx_);
+    return c;
+  }
+};
+
+}  // namespace SubrosaDG
+
+#endif  // SUBROSA_DG_AMD_HPP_

@@ -17,6 +17,7 @@ BUILD_DIR = os.path.join(HERE, "_build")
 
 HOST_SOURCES = ["kit.cpp", "sdg_mesh_api.cpp"]
 HIP_SOURCES = ["sdg_solver.hip"]
+EXAMPLES = ["taylor_green_3d_cns"]
 ARCH = os.environ.get("SDG_OFFLOAD_ARCH", "gfx950")
 
 # Packed-FP32 VALU instructions are disabled for all device code; hipcc's host pass warns that it
@@ -57,6 +58,13 @@ def build(verbose: bool = True) -> str:
     tmp = OUT + ".tmp"
     _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, OUT)
+    # C++ drop-in driver example (include/subrosadg_amd/SubrosaDG.hpp) linked against the library
+    root = os.path.dirname(HERE)
+    exe_dir = os.path.join(root, "examples", "bin")
+    os.makedirs(exe_dir, exist_ok=True)
+    for ex in EXAMPLES:
+        _run(["g++", "-std=c++17", "-O2", "-I", os.path.join(root, "include"), os.path.join(root, "examples", ex + ".cpp"),
+              "-L", HERE, "-lsdg_hip", "-Wl,-rpath,$ORIGIN/../../subrosadg_amd", "-o", os.path.join(exe_dir, ex)], verbose)
     if verbose:
         print(f"built {OUT} in {time.time() - t0:.1f}s", flush=True)
     return OUT

@@ -56,7 +56,10 @@ struct Geo {
   const int* enb;         // E x N_f        neighbour element
   const int* enbf;        // E x N_f        neighbour's local face
   const int* eside;       // E x N_f        0: this element is the face's left parent
+  const int* fpar;        // F x 2          left, right parent element
+  const int* floc;        // F x 2          local face index in left, right parent
   int E;
+  int F;
   MOps op;
 };
 
@@ -69,6 +72,7 @@ struct StageArgs {
   double* Rvol;
   double* Rout;   // residual output (mode 1)
   double* Wout;   // lifted total gradient output (optional)
+  double* Hs;     // E x NV x N_aq signed face-flux slots (Navier-Stokes, matrix-core path)
   double* mon;    // E x NV monitor partials (last stage)
   int* status;    // [0] NaN stop flag, [1] steps taken; nullptr: no check
   double c0, c1, c2dt;  // U <- c1 U + c0 Un + c2dt R Minv (TimeIntegration.cpp:181-198)

@@ -504,6 +504,105 @@ __global__ __launch_bounds__(kThreads) void nsFluxMfma(Geo g, Phys ph, StageArgs
   mfmaStageTail<D, NB, NQ, NF, NFQ, EPB>(g, a, e0, sR, sU, sRed);
 }
 
+// ---- Navier-Stokes face fluxes, one thread per face point ---------------------------------------
+// calculateInteriorAdjacencyElementQuadrature (SpatialDiscrete.cpp:633-748): each face point is
+// evaluated once with its left/right roles; t = (H_c - H_v) |J_f| w goes to the left parent's slot
+// as -t and to the right parent's slot (point seq[j]) as +t, the signs with which the face term
+// enters R = R_vol + sum_slots Phi_f s.
+template <int D, int NF, int NFQ>
+__global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, StageArgs a) {
+  constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
+  if (a.status && a.status[0]) return;
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (idx >= static_cast<int64_t>(g.F) * NFQ) return;
+  const int64_t f = idx / NFQ;
+  const int j = static_cast<int>(idx % NFQ);
+  const int k = g.fseq[f * NFQ + j];
+  const int64_t L = g.fpar[2 * f], R = g.fpar[2 * f + 1];
+  const int slL = g.floc[2 * f] * NFQ + j, slR = g.floc[2 * f + 1] * NFQ + k;
+  double ul[NV], ur[NV], gl[NG], gr[NG];
+  for (int v = 0; v < NV; v++) {
+    ul[v] = a.Uf_in[(L * NV + v) * NAQ + slL];
+    ur[v] = a.Uf_in[(R * NV + v) * NAQ + slR];
+  }
+  for (int r = 0; r < NG; r++) {
+    gl[r] = a.Gf[(L * NG + r) * NAQ + slL];
+    gr[r] = a.Gf[(R * NG + r) * NAQ + slR];
+  }
+  double wl[D + 3], wr[D + 3], n[D], h[NV], pgl[NG], pgr[NG], fvl[D * NV], fvr[D * NV];
+  computational<D>(ph, ul, wl);
+  computational<D>(ph, ur, wr);
+  for (int d = 0; d < D; d++) n[d] = g.fnormal[(f * NFQ + j) * D + d];
+  convectiveFlux<D>(ph, n, ul, wl, ur, wr, h);
+  primitiveGradient<D>(ph, wl, ul, gl, pgl);
+  primitiveGradient<D>(ph, wr, ur, gr, pgr);
+  viscousRawFlux<D>(ph, wl, pgl, fvl);
+  viscousRawFlux<D>(ph, wr, pgr, fvr);
+  const double jw = g.fjw[f * NFQ + j];
+  for (int v = 0; v < NV; v++) {
+    double vl = 0.0, vr = 0.0;
+    for (int d = 0; d < D; d++) {
+      vl += fvl[d + D * v] * n[d];
+      vr += fvr[d + D * v] * n[d];
+    }
+    const double t = (h[v] - (vl + vr) / 2.0) * jw;
+    a.Hs[(L * NV + v) * NAQ + slL] = -t;
+    a.Hs[(R * NV + v) * NAQ + slR] = t;
+  }
+}
+
+// ---- Navier-Stokes residual + update + next traces (MFMA), after nsFaceFluxKernel ---------------
+// calculateResidual + updateBasisFunctionCoefficient (SpatialDiscrete.cpp:1016-1032,
+// TimeIntegration.cpp:181-198)
+template <int D, int NB, int NQ, int NF, int NFQ>
+__global__ __launch_bounds__(kThreads) void nsResidualMfma(Geo g, Phys ph, StageArgs a) {
+  const MOps& op = g.op;
+  constexpr int NV = D + 2, NAQ = NF * NFQ;
+  constexpr int EPB = 16 / NV;
+  constexpr int NBk = r4(NB), NBm = r16(NB), NAQk = r4(NAQ);
+  constexpr int NBr = NBk > NBm ? NBk : NBm;
+  __shared__ double sU[NBr * 16];
+  __shared__ double sH[NAQk * 16];
+  __shared__ double sR[NBr * 16];
+  __shared__ double sRed[64];
+  if (a.status && a.status[0]) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c = lane & 15, rr = lane >> 4;
+  const int e0 = blockIdx.x * EPB;
+  for (int idx = tid; idx < NBr * 16; idx += kThreads) {
+    const int b = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
+    sU[idx] = (b < NB && cc < EPB * NV && e < g.E) ? a.U[(static_cast<int64_t>(e) * NV + v) * NB + b] : 0.0;
+  }
+  for (int idx = tid; idx < NAQk * 16; idx += kThreads) {
+    const int sl = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
+    sH[idx] = (sl < NAQ && cc < EPB * NV && e < g.E) ? a.Hs[(static_cast<int64_t>(e) * NV + v) * NAQ + sl] : 0.0;
+  }
+  __syncthreads();
+  {
+    const int le = c / NV, v = c % NV, e = e0 + le;
+    const bool ok = c < EPB * NV && e < g.E;
+    for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
+      d4 acc;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int bi = m0 + rr + 4 * i;
+        acc[i] = (ok && bi < NB) ? a.Rvol[(static_cast<int64_t>(e) * NV + v) * NB + bi] : 0.0;
+      }
+      tileAcc<NBm>(acc, op.face, m0, sH, 0, NAQk, lane);
+      tileStore(acc, sR, m0, lane);
+    }
+  }
+  __syncthreads();
+  if (a.mode == 1) {
+    for (int idx = tid; idx < NB * 16; idx += kThreads) {
+      const int b = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
+      if (cc < EPB * NV && e < g.E) a.Rout[(static_cast<int64_t>(e) * NV + v) * NB + b] = sR[idx];
+    }
+    return;
+  }
+  mfmaStageTail<D, NB, NQ, NF, NFQ, EPB>(g, a, e0, sR, sU, sRed);
+}
+
 }  // namespace sdg
 
 #endif  // SDG_DG_MFMA_HPP_

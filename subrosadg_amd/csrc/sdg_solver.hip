@@ -37,6 +37,7 @@ struct Launchers {
   StageFn ns_grad_br1 = nullptr;
   StageFn ns_grad_br2 = nullptr;
   StageFn ns_flux = nullptr;
+  StageFn ns_face = nullptr;  // matrix-core path: face fluxes before ns_flux
   TraceFn trace = nullptr;
   DtFn dt = nullptr;
   MonFn monitor = nullptr;
@@ -66,9 +67,13 @@ Launchers makeLaunchers() {
       hipLaunchKernelGGL((sdg::nsGradMfma<D, NB, NQ, NF, NFQ, 2>), dim3(blocksFor(g.E, EPB_G)), dim3(sdg::kThreads),
                          0, st, g, ph, a);
     };
+    l.ns_face = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+      hipLaunchKernelGGL((sdg::nsFaceFluxKernel<D, NF, NFQ>), dim3(blocksFor(g.F * NFQ, sdg::kThreads)),
+                         dim3(sdg::kThreads), 0, st, g, ph, a);
+    };
     l.ns_flux = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-      hipLaunchKernelGGL((sdg::nsFluxMfma<D, NB, NQ, NF, NFQ>), dim3(blocksFor(g.E, EPB_S)), dim3(sdg::kThreads), 0,
-                         st, g, ph, a);
+      hipLaunchKernelGGL((sdg::nsResidualMfma<D, NB, NQ, NF, NFQ>), dim3(blocksFor(g.E, EPB_S)), dim3(sdg::kThreads),
+                         0, st, g, ph, a);
     };
   } else {
     l.euler = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
@@ -166,7 +171,7 @@ struct sdg_ctx {
   double* minedge = nullptr;
   double* lsq = nullptr;  // host copy
   std::vector<double> h_lsq, h_V;
-  double *U = nullptr, *Un = nullptr, *Uf[2] = {nullptr, nullptr}, *Gf = nullptr, *Rvol = nullptr;
+  double *U = nullptr, *Un = nullptr, *Uf[2] = {nullptr, nullptr}, *Gf = nullptr, *Rvol = nullptr, *Hs = nullptr;
   double *Rout = nullptr, *Wout = nullptr, *mon = nullptr, *rel = nullptr, *dtbuf = nullptr;
   int* status = nullptr;
   int cur = 0;
@@ -249,6 +254,7 @@ StageArgs baseArgs(sdg_ctx* ctx) {
   a.Rvol = ctx->Rvol;
   a.Rout = ctx->Rout;
   a.Wout = nullptr;
+  a.Hs = ctx->Hs;
   a.mon = ctx->mon;
   a.status = ctx->status;
   return a;
@@ -277,6 +283,7 @@ void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hi
       (void)hipEventRecord(ev[1], ctx->stream);
       (void)hipEventRecord(ev[2], ctx->stream);
     }
+    if (ctx->launch.ns_face) ctx->launch.ns_face(ctx->stream, ctx->geo, ctx->phys, a);
     ctx->launch.ns_flux(ctx->stream, ctx->geo, ctx->phys, a);
     if (ev) (void)hipEventRecord(ev[3], ctx->stream);
   } else {
@@ -459,6 +466,13 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
       fseqinv[static_cast<size_t>(f) * nfq + fc.seq[j]] = j;
     }
   }
+  std::vector<int> fpar(static_cast<size_t>(F) * 2), floc(static_cast<size_t>(F) * 2);
+  for (int64_t f = 0; f < F; f++) {
+    fpar[2 * f] = mesh.faces[f].parent[0];
+    fpar[2 * f + 1] = mesh.faces[f].parent[1];
+    floc[2 * f] = mesh.faces[f].local[0];
+    floc[2 * f + 1] = mesh.faces[f].local[1];
+  }
   std::vector<int> efid(static_cast<size_t>(E) * nf), enb(static_cast<size_t>(E) * nf), enbf(static_cast<size_t>(E) * nf),
       eside(static_cast<size_t>(E) * nf);
   for (int64_t e = 0; e < E; e++)
@@ -477,13 +491,14 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     }
   Geo& g = ctx->geo;
   double *dVt, *ddV, *dVft, *dVf, *dMinv, *dLf, *dwq, *dginv, *ddetj, *dnormal, *djw, *dme;
-  int *dseq, *dseqinv, *defid, *denb, *denbf, *deside;
+  int *dseq, *dseqinv, *defid, *denb, *denbf, *deside, *dfpar, *dfloc;
   if (upload(ctx, Vt, &dVt) || upload(ctx, r.dV, &ddV) || upload(ctx, Vft, &dVft) || upload(ctx, r.Vf, &dVf) ||
       upload(ctx, r.mass_inverse, &dMinv) || upload(ctx, Lf, &dLf) || upload(ctx, r.qweight, &dwq) ||
       upload(ctx, ginv, &dginv) || upload(ctx, detj, &ddetj) || upload(ctx, fnormal, &dnormal) ||
       upload(ctx, fjw, &djw) || upload(ctx, fseq, &dseq) || upload(ctx, fseqinv, &dseqinv) ||
       upload(ctx, efid, &defid) || upload(ctx, enb, &denb) || upload(ctx, enbf, &denbf) ||
-      upload(ctx, eside, &deside) || upload(ctx, mesh.min_edge, &dme))
+      upload(ctx, eside, &deside) || upload(ctx, mesh.min_edge, &dme) || upload(ctx, fpar, &dfpar) ||
+      upload(ctx, floc, &dfloc))
     return bail(SDG_E_NOMEM);
   g.Vt = dVt;
   g.dV = ddV;
@@ -502,7 +517,10 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   g.enb = denb;
   g.enbf = denbf;
   g.eside = deside;
+  g.fpar = dfpar;
+  g.floc = dfloc;
   g.E = ctx->E;
+  g.F = static_cast<int>(F);
   {
     const int nfq_ = nfq;
     auto V = [&](int q, int b) { return r.V[static_cast<size_t>(q) * nb + b]; };
@@ -537,7 +555,8 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
       allocate(ctx, static_cast<size_t>(E), &ctx->dtbuf) || allocate(ctx, size_t(2), &ctx->status))
     return bail(SDG_E_NOMEM);
   if (ctx->ns()) {
-    if (allocate(ctx, static_cast<size_t>(E) * ctx->ng * naq, &ctx->Gf) || allocate(ctx, nU, &ctx->Rvol))
+    if (allocate(ctx, static_cast<size_t>(E) * ctx->ng * naq, &ctx->Gf) || allocate(ctx, nU, &ctx->Rvol) ||
+        allocate(ctx, nUf, &ctx->Hs))
       return bail(SDG_E_NOMEM);
   }
   hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
