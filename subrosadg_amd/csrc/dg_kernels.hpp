@@ -36,6 +36,8 @@ struct MOps {
   const double* minv;    // K = N_b,   M = N_b   : reference mass inverse
   const double* trace;   // K = N_b,   M = N_aq  : face values           (coefficients -> face traces)
   const double* lift;    // K = N_aq,  M = N_aq  : block-diagonal Vf_s Mref^-1 Vf_s^T (BR2 face lift)
+  const double* pinterp; // K = N_b,   M = N_q   : Mref^-1 V^T   (R -> gradient at volume points)
+  const double* ptrace;  // K = N_b,   M = N_aq  : Mref^-1 Vf^T  (R -> gradient traces at face points)
 };
 
 struct Geo {
@@ -56,6 +58,8 @@ struct Geo {
   const int* enb;         // E x N_f        neighbour element
   const int* enbf;        // E x N_f        neighbour's local face
   const int* eside;       // E x N_f        0: this element is the face's left parent
+  const int* snb;         // E x N_aq       neighbour slot: neighbour element * N_aq + its slot
+  const int* sfp;         // E x N_aq       2 (f N_fq + p) + side: face point of the slot and side
   const int* fpar;        // F x 2          left, right parent element
   const int* floc;        // F x 2          local face index in left, right parent
   int E;
@@ -73,6 +77,7 @@ struct StageArgs {
   double* Rout;   // residual output (mode 1)
   double* Wout;   // lifted total gradient output (optional)
   double* Hs;     // E x NV x N_aq signed face-flux slots (Navier-Stokes, matrix-core path)
+  double* Fv;     // E x (N_q D) x NV contravariant volume flux (Navier-Stokes, matrix-core path)
   double* mon;    // E x NV monitor partials (last stage)
   int* status;    // [0] NaN stop flag, [1] steps taken; nullptr: no check
   double c0, c1, c2dt;  // U <- c1 U + c0 Un + c2dt R Minv (TimeIntegration.cpp:181-198)

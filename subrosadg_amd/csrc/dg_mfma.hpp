@@ -31,7 +31,7 @@ template <int MP>
 __device__ __forceinline__ void tileAcc(d4& acc, const double* __restrict__ T, int m0, const double* X, int k0, int k1,
                                         int lane) {
   const int c = lane & 15, kr = lane >> 4;
-#pragma unroll 4
+#pragma unroll 8
   for (int k = k0; k < k1; k += 4) acc = mfma64(T[(k + kr) * MP + m0 + c], X[(k + kr) * 16 + c], acc);
 }
 
@@ -223,6 +223,9 @@ __global__ __launch_bounds__(kThreads) void eulerStageMfma(Geo g, Phys ph, Stage
 
 // ---- Navier-Stokes kernel 1 (MFMA): gradient lifting + volume flux -------------------------------
 // One element per workgroup in 3-D (15 gradient columns), two in 2-D (2 x 8 columns).
+// The lifted coefficients w = R Mref^-1 / |J| are only ever evaluated at points, so Mref^-1 is folded
+// into the evaluation operators: grad U_q = R_t (Mref^-1 V^T) / |J| and the BR2 face trace
+// R_w (Mref^-1 Vf^T) / |J| + J Lf / |J| (BR1: R_t (Mref^-1 Vf^T) / |J|).
 template <int D, int NB, int NQ, int NF, int NFQ, int VISC>
 __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs a) {
   const MOps& op = g.op;
@@ -230,27 +233,21 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
   constexpr int EPB = 16 / NG;
   constexpr int NBk = r4(NB), NBm = r16(NB), NQm = r16(NQ), NAQk = r4(NAQ), NAQm = r16(NAQ), NDQk = r4(NQ * D);
   constexpr int NBr = NBk > NBm ? NBk : NBm;
-  constexpr int AJ = 2 * NAQk * 16 > NDQk * 16 ? 2 * NAQk * 16 : NDQk * 16;
-  constexpr int RR = 2 * NBr * 16 > NQm * 16 ? 2 * NBr * 16 : NQm * 16;
-  __shared__ double sU[NBr * 16];   // state panel, columns le*NV + v
-  __shared__ double sUq[NQm * 16];  // state at volume points
-  __shared__ double sAJ[AJ];        // face terms A | J ([slot][le*NG + r]); later the volume flux
-  __shared__ double sRR[RR];        // R_w | R_t; later the gradient at volume points
-  __shared__ double sW[2 * NBr * 16];  // w_vol | w
+  constexpr int AJ = 2 * NAQk * 16 > NQm * 16 ? 2 * NAQk * 16 : NQm * 16;
+  __shared__ double sUq[NQm * 16];    // state at volume points
+  __shared__ double sAJ[AJ];          // face terms A | J ([slot][le*NG + r]); later the gradient at points
+  __shared__ double sRR[2 * NBr * 16];  // state panel; then R_w | R_t
+  double* sU = sRR;
   double* sA = sAJ;
   double* sJ = sAJ + NAQk * 16;
   double* sRw = sRR;
   double* sRt = sRR + NBr * 16;
-  double* sGq = sRR;
-  double* sWv = sW;
-  double* sWt = sW + NBr * 16;
-  double* sF = sAJ;
+  double* sGq = sAJ;
   if (a.status && a.status[0]) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int c = lane & 15, rr = lane >> 4;
   const int e0 = blockIdx.x * EPB;
   zeroLds(sAJ, AJ, tid);
-  zeroLds(sW, 2 * NBr * 16, tid);
   for (int idx = tid; idx < NBr * 16; idx += kThreads) {
     const int b = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
     sU[idx] = (b < NB && cc < EPB * NV && e < g.E) ? a.U[(static_cast<int64_t>(e) * NV + v) * NB + b] : 0.0;
@@ -260,16 +257,16 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
   for (int idx = tid; idx < EPB * NAQ * NV; idx += kThreads) {
     const int le = idx / (NAQ * NV), rem = idx % (NAQ * NV), sl = rem / NV, v = rem % NV, e = e0 + le;
     if (e >= g.E) continue;
-    const int s = sl / NFQ, j = sl % NFQ;
-    int f, p, side;
-    int64_t nbslot;
-    slotRoles<NF, NFQ>(g, e, s, j, f, p, nbslot, side);
+    const int64_t es = static_cast<int64_t>(e) * NAQ + sl;
+    const int nbs = g.snb[es], fps = g.sfp[es];
+    const int side = fps & 1;
+    const int64_t fp = fps >> 1;
     const double uo = a.Uf_in[(static_cast<int64_t>(e) * NV + v) * NAQ + sl];
-    const double un = a.Uf_in[((nbslot / NAQ) * NV + v) * NAQ + nbslot % NAQ];
+    const double un = a.Uf_in[(static_cast<int64_t>(nbs / NAQ) * NV + v) * NAQ + nbs % NAQ];
     const double ul = side == 0 ? uo : un, ur = side == 0 ? un : uo;
-    const double jw = g.fjw[static_cast<int64_t>(f) * NFQ + p];
+    const double jw = g.fjw[fp];
     for (int d = 0; d < D; d++) {
-      const double n = g.fnormal[(static_cast<int64_t>(f) * NFQ + p) * D + d];
+      const double n = g.fnormal[fp * D + d];
       const double avg = (n * (ul + ur) / 2.0) * jw;
       sA[sl * 16 + le * NG + v * D + d] = side == 0 ? avg : -avg;
       sJ[sl * 16 + le * NG + v * D + d] = (n * (ur - ul) / 2.0) * jw;
@@ -284,27 +281,31 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
   __syncthreads();
   // gradient residuals: R_w = A Phi_f - Q Phi_grad, R_t = R_w + J Phi_f, with
   // Q[(le,k,a)][(q,b)] = u_k(q) (J^T)^-1(a,b) |J| w_q generated in the B fragment
+  const int gle = c / NG, ge = e0 + gle;
+  const bool gok = c < EPB * NG && ge < g.E;
+  const int gec = gok ? ge : 0;
+  const double dj = g.detj[gec];
   {
-    const int gcol_le = c / NG, gk = (c % NG) / D, ga = (c % NG) % D;
-    const int ge = e0 + gcol_le;
-    const bool gok = c < EPB * NG && ge < g.E;
-    const int gec = gok ? ge : 0;
-    const double dj = g.detj[gec];
+    const int gk = (c % NG) / D, ga = (c % NG) % D;
+    double gi[D];
+#pragma unroll
+    for (int b = 0; b < D; b++) gi[b] = g.ginv[static_cast<int64_t>(gec) * D * D + ga + D * b];
     for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
       d4 aw = {0.0, 0.0, 0.0, 0.0}, aj = {0.0, 0.0, 0.0, 0.0};
       tileAcc<NBm>(aw, op.face, m0, sA, 0, NAQk, lane);
       tileAcc<NBm>(aj, op.face, m0, sJ, 0, NAQk, lane);
-#pragma unroll 2
+#pragma unroll 4
       for (int k = 0; k < NDQk; k += 4) {
         const int kk = k + rr;
         double bq = 0.0;
         if (gok && kk < NQ * D) {
           const int q = kk / D, b = kk % D;
-          const double G = g.ginv[static_cast<int64_t>(gec) * D * D + ga + D * b] * (dj * g.wq[q]);
-          bq = -(sUq[q * 16 + gcol_le * NV + gk] * G);
+          const double G = gi[b] * (dj * g.wq[q]);
+          bq = -(sUq[q * 16 + gle * NV + gk] * G);
         }
         aw = mfma64(op.grad[kk * NBm + m0 + c], bq, aw);
       }
+      // the state panel (aliased by R) was last read before the barrier above
       tileStore(aw, sRw, m0, lane);
       d4 at = aw + aj;
       tileStore(at, sRt, m0, lane);
@@ -312,63 +313,50 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
   }
   for (int idx = NBm * 16 + tid; idx < NBr * 16; idx += kThreads) sRw[idx] = sRt[idx] = 0.0;
   __syncthreads();
-  // lifted coefficients: w_vol = R_w Mref^-1 / |J|, w = R_t Mref^-1 / |J|
-  for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
-    d4 wv = {0.0, 0.0, 0.0, 0.0}, wt = {0.0, 0.0, 0.0, 0.0};
-    tileAcc<NBm>(wv, op.minv, m0, sRw, 0, NBk, lane);
-    tileAcc<NBm>(wt, op.minv, m0, sRt, 0, NBk, lane);
-    const int le = c / NG, e = e0 + le;
-    const bool ok = c < EPB * NG && e < g.E;
-    const double idj = 1.0 / g.detj[ok ? e : 0];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int bi = m0 + rr + 4 * i;
-      const double x = (ok && bi < NB) ? wv[i] * idj : 0.0;
-      const double y = (ok && bi < NB) ? wt[i] * idj : 0.0;
-      if (bi < NBr) {
-        sWv[bi * 16 + c] = x;
-        sWt[bi * 16 + c] = y;
-      }
-      if (a.Wout && ok && bi < NB) a.Wout[(static_cast<int64_t>(e) * NB + bi) * NG + (c % NG)] = y;
-    }
-  }
-  __syncthreads();
-  // total gradient at the volume points; lifted gradient traces at the face points -> HBM
-  for (int m0 = wave * 16; m0 < NQm; m0 += 64) {
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
-    tileAcc<NQm>(acc, op.interp, m0, sWt, 0, NBk, lane);
-    tileStore(acc, sGq, m0, lane);
-  }
-  {
-    const int le = c / NG, e = e0 + le;
-    const bool ok = c < EPB * NG && e < g.E;
-    const double idj = 1.0 / g.detj[ok ? e : 0];
-    for (int m0 = wave * 16; m0 < NAQm; m0 += 64) {
-      d4 acc = {0.0, 0.0, 0.0, 0.0};
-      if (VISC == 2) {
-        tileAcc<NAQm>(acc, op.trace, m0, sWv, 0, NBk, lane);
-        // block-diagonal BR2 lift: only the faces this row tile touches
-        d4 lf = {0.0, 0.0, 0.0, 0.0};
-        const int s_lo = m0 / NFQ;
-        const int s_hi = (m0 + 15 < NAQ ? m0 + 15 : NAQ - 1) / NFQ;
-        const int k0 = (s_lo * NFQ) / 4 * 4;
-        const int k1 = r4((s_hi + 1) * NFQ);
-        tileAcc<NAQm>(lf, op.lift, m0, sJ, k0, k1 < NAQk ? k1 : NAQk, lane);
-        acc += lf * idj;
-      } else {
-        tileAcc<NAQm>(acc, op.trace, m0, sWt, 0, NBk, lane);
-      }
+  const double idj = 1.0 / dj;
+  // optional output of the lifted total-gradient coefficients w = R_t Mref^-1 / |J|
+  if (a.Wout) {
+    for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
+      d4 wt = {0.0, 0.0, 0.0, 0.0};
+      tileAcc<NBm>(wt, op.minv, m0, sRt, 0, NBk, lane);
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int sl = m0 + rr + 4 * i;
-        if (ok && sl < NAQ) a.Gf[(static_cast<int64_t>(e) * NG + (c % NG)) * NAQ + sl] = acc[i];
+        const int bi = m0 + rr + 4 * i;
+        if (gok && bi < NB) a.Wout[(static_cast<int64_t>(ge) * NB + bi) * NG + (c % NG)] = wt[i] * idj;
       }
     }
   }
+  // lifted gradient traces at the face points -> HBM
+  for (int m0 = wave * 16; m0 < NAQm; m0 += 64) {
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    if (VISC == 2) {
+      tileAcc<NAQm>(acc, op.ptrace, m0, sRw, 0, NBk, lane);
+      // block-diagonal BR2 lift: only the faces this row tile touches
+      const int s_lo = m0 / NFQ;
+      const int s_hi = (m0 + 15 < NAQ ? m0 + 15 : NAQ - 1) / NFQ;
+      const int k0 = (s_lo * NFQ) / 4 * 4;
+      const int k1 = r4((s_hi + 1) * NFQ);
+      tileAcc<NAQm>(acc, op.lift, m0, sJ, k0, k1 < NAQk ? k1 : NAQk, lane);
+    } else {
+      tileAcc<NAQm>(acc, op.ptrace, m0, sRt, 0, NBk, lane);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int sl = m0 + rr + 4 * i;
+      if (gok && sl < NAQ) a.Gf[(static_cast<int64_t>(ge) * NG + (c % NG)) * NAQ + sl] = acc[i] * idj;
+    }
+  }
   __syncthreads();
-  // volume flux (F_c - F_v)^T (J^T)^-1 |J| w at the points -> sF[(q D + b)][le NV + v]
-  zeroLds(sF, NDQk * 16, tid);
+  // total gradient at the volume points (into the face-term buffer, no longer needed)
+  for (int m0 = wave * 16; m0 < NQm; m0 += 64) {
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    tileAcc<NQm>(acc, op.pinterp, m0, sRt, 0, NBk, lane);
+    acc *= idj;
+    tileStore(acc, sGq, m0, lane);
+  }
   __syncthreads();
+  // volume flux (F_c - F_v)^T (J^T)^-1 |J| w at the points -> HBM Fv[e][(q D + b)][v]; the weak
+  // divergence F Phi_grad is taken in nsResidualMfma, which packs three elements per 16 columns
   for (int idx = tid; idx < EPB * NQ; idx += kThreads) {
     const int le = idx / NQ, q = idx % NQ, e = e0 + le;
     if (e >= g.E) continue;
@@ -389,119 +377,14 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
     const double jw = g.detj[e] * g.wq[q];
     double G[D * D];
     for (int i = 0; i < D * D; i++) G[i] = g.ginv[static_cast<int64_t>(e) * D * D + i] * jw;
-    for (int v = 0; v < NV; v++)
-      for (int b = 0; b < D; b++) {
+    double* out = a.Fv + (static_cast<int64_t>(e) * NQ * D + q * D) * NV;
+    for (int b = 0; b < D; b++)
+      for (int v = 0; v < NV; v++) {
         double s = 0.0;
         for (int aa = 0; aa < D; aa++) s += (fc[aa + D * v] - fv[aa + D * v]) * G[aa + D * b];
-        sF[(q * D + b) * 16 + le * NV + v] = s;
+        out[b * NV + v] = s;
       }
   }
-  __syncthreads();
-  // volume residual R_vol = F Phi_grad -> HBM
-  {
-    const int le = c / NV, v = c % NV, e = e0 + le;
-    const bool ok = c < EPB * NV && e < g.E;
-    for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
-      d4 acc = {0.0, 0.0, 0.0, 0.0};
-      tileAcc<NBm>(acc, op.grad, m0, sF, 0, NDQk, lane);
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int bi = m0 + rr + 4 * i;
-        if (ok && bi < NB) a.Rvol[(static_cast<int64_t>(e) * NV + v) * NB + bi] = acc[i];
-      }
-    }
-  }
-}
-
-// ---- Navier-Stokes kernel 2 (MFMA): face fluxes + residual + update + next traces ---------------
-template <int D, int NB, int NQ, int NF, int NFQ>
-__global__ __launch_bounds__(kThreads) void nsFluxMfma(Geo g, Phys ph, StageArgs a) {
-  const MOps& op = g.op;
-  constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
-  constexpr int EPB = 16 / NV;
-  constexpr int NBk = r4(NB), NBm = r16(NB), NAQk = r4(NAQ);
-  constexpr int NBr = NBk > NBm ? NBk : NBm;
-  __shared__ double sU[NBr * 16];
-  __shared__ double sH[NAQk * 16];
-  __shared__ double sR[NBr * 16];
-  __shared__ double sRed[64];
-  if (a.status && a.status[0]) return;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int c = lane & 15, rr = lane >> 4;
-  const int e0 = blockIdx.x * EPB;
-  zeroLds(sH, NAQk * 16, tid);
-  for (int idx = tid; idx < NBr * 16; idx += kThreads) {
-    const int b = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
-    sU[idx] = (b < NB && cc < EPB * NV && e < g.E) ? a.U[(static_cast<int64_t>(e) * NV + v) * NB + b] : 0.0;
-  }
-  __syncthreads();
-  for (int idx = tid; idx < EPB * NAQ; idx += kThreads) {
-    const int le = idx / NAQ, sl = idx % NAQ, e = e0 + le;
-    if (e >= g.E) continue;
-    const int s = sl / NFQ, j = sl % NFQ;
-    int f, p, side;
-    int64_t nbslot;
-    slotRoles<NF, NFQ>(g, e, s, j, f, p, nbslot, side);
-    const int64_t nbel = nbslot / NAQ, nbsl = nbslot % NAQ;
-    double uo[NV], un[NV], go[NG], gn[NG];
-    for (int v = 0; v < NV; v++) {
-      uo[v] = a.Uf_in[(static_cast<int64_t>(e) * NV + v) * NAQ + sl];
-      un[v] = a.Uf_in[(nbel * NV + v) * NAQ + nbsl];
-    }
-    for (int r = 0; r < NG; r++) {
-      go[r] = a.Gf[(static_cast<int64_t>(e) * NG + r) * NAQ + sl];
-      gn[r] = a.Gf[(nbel * NG + r) * NAQ + nbsl];
-    }
-    const double* ul = side == 0 ? uo : un;
-    const double* ur = side == 0 ? un : uo;
-    const double* gl = side == 0 ? go : gn;
-    const double* gr = side == 0 ? gn : go;
-    double wl[D + 3], wr[D + 3], n[D], h[NV], pgl[NG], pgr[NG], fvl[D * NV], fvr[D * NV];
-    computational<D>(ph, ul, wl);
-    computational<D>(ph, ur, wr);
-    for (int d = 0; d < D; d++) n[d] = g.fnormal[(static_cast<int64_t>(f) * NFQ + p) * D + d];
-    convectiveFlux<D>(ph, n, ul, wl, ur, wr, h);
-    primitiveGradient<D>(ph, wl, ul, gl, pgl);
-    primitiveGradient<D>(ph, wr, ur, gr, pgr);
-    viscousRawFlux<D>(ph, wl, pgl, fvl);
-    viscousRawFlux<D>(ph, wr, pgr, fvr);
-    const double jw = g.fjw[static_cast<int64_t>(f) * NFQ + p];
-    for (int v = 0; v < NV; v++) {
-      double vl = 0.0, vr = 0.0;
-      for (int d = 0; d < D; d++) {
-        vl += fvl[d + D * v] * n[d];
-        vr += fvr[d + D * v] * n[d];
-      }
-      const double t = (h[v] - (vl + vr) / 2.0) * jw;
-      sH[sl * 16 + le * NV + v] = side == 0 ? -t : t;
-    }
-  }
-  __syncthreads();
-  // R = R_vol - H Phi_face
-  {
-    const int le = c / NV, v = c % NV, e = e0 + le;
-    const bool ok = c < EPB * NV && e < g.E;
-    for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
-      d4 acc;
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int bi = m0 + rr + 4 * i;
-        acc[i] = (ok && bi < NB) ? a.Rvol[(static_cast<int64_t>(e) * NV + v) * NB + bi] : 0.0;
-      }
-      tileAcc<NBm>(acc, op.face, m0, sH, 0, NAQk, lane);
-      tileStore(acc, sR, m0, lane);
-    }
-  }
-  for (int idx = NBm * 16 + tid; idx < NBr * 16; idx += kThreads) sR[idx] = 0.0;
-  __syncthreads();
-  if (a.mode == 1) {
-    for (int idx = tid; idx < NB * 16; idx += kThreads) {
-      const int b = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
-      if (cc < EPB * NV && e < g.E) a.Rout[(static_cast<int64_t>(e) * NV + v) * NB + b] = sR[idx];
-    }
-    return;
-  }
-  mfmaStageTail<D, NB, NQ, NF, NFQ, EPB>(g, a, e0, sR, sU, sRed);
 }
 
 // ---- Navier-Stokes face fluxes, one thread per face point ---------------------------------------
@@ -561,13 +444,14 @@ __global__ __launch_bounds__(kThreads) void nsResidualMfma(Geo g, Phys ph, Stage
   constexpr int EPB = 16 / NV;
   constexpr int NBk = r4(NB), NBm = r16(NB), NAQk = r4(NAQ);
   constexpr int NBr = NBk > NBm ? NBk : NBm;
+  constexpr int NDQ = NQ * D, NDQk = r4(NDQ);
   __shared__ double sU[NBr * 16];
   __shared__ double sH[NAQk * 16];
+  __shared__ double sF[NDQk * 16];
   __shared__ double sR[NBr * 16];
   __shared__ double sRed[64];
   if (a.status && a.status[0]) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int c = lane & 15, rr = lane >> 4;
   const int e0 = blockIdx.x * EPB;
   for (int idx = tid; idx < NBr * 16; idx += kThreads) {
     const int b = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
@@ -577,20 +461,16 @@ __global__ __launch_bounds__(kThreads) void nsResidualMfma(Geo g, Phys ph, Stage
     const int sl = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
     sH[idx] = (sl < NAQ && cc < EPB * NV && e < g.E) ? a.Hs[(static_cast<int64_t>(e) * NV + v) * NAQ + sl] : 0.0;
   }
+  for (int idx = tid; idx < NDQk * 16; idx += kThreads) {
+    const int k = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
+    sF[idx] = (k < NDQ && cc < EPB * NV && e < g.E) ? a.Fv[(static_cast<int64_t>(e) * NDQ + k) * NV + v] : 0.0;
+  }
   __syncthreads();
-  {
-    const int le = c / NV, v = c % NV, e = e0 + le;
-    const bool ok = c < EPB * NV && e < g.E;
-    for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
-      d4 acc;
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int bi = m0 + rr + 4 * i;
-        acc[i] = (ok && bi < NB) ? a.Rvol[(static_cast<int64_t>(e) * NV + v) * NB + bi] : 0.0;
-      }
-      tileAcc<NBm>(acc, op.face, m0, sH, 0, NAQk, lane);
-      tileStore(acc, sR, m0, lane);
-    }
+  for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    tileAcc<NBm>(acc, op.grad, m0, sF, 0, NDQk, lane);
+    tileAcc<NBm>(acc, op.face, m0, sH, 0, NAQk, lane);
+    tileStore(acc, sR, m0, lane);
   }
   __syncthreads();
   if (a.mode == 1) {

@@ -171,7 +171,7 @@ struct sdg_ctx {
   double* minedge = nullptr;
   double* lsq = nullptr;  // host copy
   std::vector<double> h_lsq, h_V;
-  double *U = nullptr, *Un = nullptr, *Uf[2] = {nullptr, nullptr}, *Gf = nullptr, *Rvol = nullptr, *Hs = nullptr;
+  double *U = nullptr, *Un = nullptr, *Uf[2] = {nullptr, nullptr}, *Gf = nullptr, *Rvol = nullptr, *Hs = nullptr, *Fv = nullptr;
   double *Rout = nullptr, *Wout = nullptr, *mon = nullptr, *rel = nullptr, *dtbuf = nullptr;
   int* status = nullptr;
   int cur = 0;
@@ -255,6 +255,7 @@ StageArgs baseArgs(sdg_ctx* ctx) {
   a.Rout = ctx->Rout;
   a.Wout = nullptr;
   a.Hs = ctx->Hs;
+  a.Fv = ctx->Fv;
   a.mon = ctx->mon;
   a.status = ctx->status;
   return a;
@@ -489,16 +490,34 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
       enb[static_cast<size_t>(e) * nf + s] = fc.parent[side ^ 1];
       enbf[static_cast<size_t>(e) * nf + s] = fc.local[side ^ 1];
     }
+  // per element slot: neighbour slot and (face point, side), so a kernel reaches the neighbour's
+  // trace with one indexed load (no face -> permutation -> slot chain)
+  if (static_cast<int64_t>(E) * naq >= (int64_t(1) << 31) || F * nfq >= (int64_t(1) << 30)) {
+    ctx->err = "sdg_create: mesh too large for 32-bit slot indices";
+    return bail(SDG_E_UNSUPPORTED);
+  }
+  std::vector<int> snb(static_cast<size_t>(E) * naq), sfp(static_cast<size_t>(E) * naq);
+  for (int64_t e = 0; e < E; e++)
+    for (int s = 0; s < nf; s++) {
+      const int fid = efid[static_cast<size_t>(e) * nf + s];
+      const int side = eside[static_cast<size_t>(e) * nf + s];
+      for (int j = 0; j < nfq; j++) {
+        const int p = side == 0 ? j : fseqinv[static_cast<size_t>(fid) * nfq + j];
+        const int k = side == 0 ? fseq[static_cast<size_t>(fid) * nfq + j] : p;
+        snb[static_cast<size_t>(e) * naq + s * nfq + j] = enb[static_cast<size_t>(e) * nf + s] * naq + enbf[static_cast<size_t>(e) * nf + s] * nfq + k;
+        sfp[static_cast<size_t>(e) * naq + s * nfq + j] = (fid * nfq + p) * 2 + side;
+      }
+    }
   Geo& g = ctx->geo;
   double *dVt, *ddV, *dVft, *dVf, *dMinv, *dLf, *dwq, *dginv, *ddetj, *dnormal, *djw, *dme;
-  int *dseq, *dseqinv, *defid, *denb, *denbf, *deside, *dfpar, *dfloc;
+  int *dseq, *dseqinv, *defid, *denb, *denbf, *deside, *dfpar, *dfloc, *dsnb, *dsfp;
   if (upload(ctx, Vt, &dVt) || upload(ctx, r.dV, &ddV) || upload(ctx, Vft, &dVft) || upload(ctx, r.Vf, &dVf) ||
       upload(ctx, r.mass_inverse, &dMinv) || upload(ctx, Lf, &dLf) || upload(ctx, r.qweight, &dwq) ||
       upload(ctx, ginv, &dginv) || upload(ctx, detj, &ddetj) || upload(ctx, fnormal, &dnormal) ||
       upload(ctx, fjw, &djw) || upload(ctx, fseq, &dseq) || upload(ctx, fseqinv, &dseqinv) ||
       upload(ctx, efid, &defid) || upload(ctx, enb, &denb) || upload(ctx, enbf, &denbf) ||
       upload(ctx, eside, &deside) || upload(ctx, mesh.min_edge, &dme) || upload(ctx, fpar, &dfpar) ||
-      upload(ctx, floc, &dfloc))
+      upload(ctx, floc, &dfloc) || upload(ctx, snb, &dsnb) || upload(ctx, sfp, &dsfp))
     return bail(SDG_E_NOMEM);
   g.Vt = dVt;
   g.dV = ddV;
@@ -517,6 +536,8 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   g.enb = denb;
   g.enbf = denbf;
   g.eside = deside;
+  g.snb = dsnb;
+  g.sfp = dsfp;
   g.fpar = dfpar;
   g.floc = dfloc;
   g.E = ctx->E;
@@ -534,10 +555,23 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
       const int sm = m / nfq_, sk = k / nfq_;
       return sm == sk ? Lf[(static_cast<size_t>(sm) * nfq_ + k % nfq_) * nfq_ + m % nfq_] : 0.0;
     });
-    double *pi, *pg, *pf, *pm, *pt, *pl;
+    // Mref^-1 folded into the evaluation operators: (Mref^-1 V^T)(b, q), (Mref^-1 Vf^T)(b, sl)
+    std::vector<double> pinterp = paddedOperator(nb, nq, [&](int m, int k) {
+      double acc = 0.0;
+      for (int i = 0; i < nb; i++) acc += r.mass_inverse[static_cast<size_t>(k) * nb + i] * V(m, i);
+      return acc;
+    });
+    std::vector<double> ptrace = paddedOperator(nb, naq, [&](int m, int k) {
+      double acc = 0.0;
+      for (int i = 0; i < nb; i++) acc += r.mass_inverse[static_cast<size_t>(k) * nb + i] * Vfv(m, i);
+      return acc;
+    });
+    double *pi, *pg, *pf, *pm, *pt, *pl, *ppi, *ppt;
     if (upload(ctx, t_interp, &pi) || upload(ctx, t_grad, &pg) || upload(ctx, t_face, &pf) || upload(ctx, t_minv, &pm) ||
-        upload(ctx, t_trace, &pt) || upload(ctx, t_lift, &pl))
+        upload(ctx, t_trace, &pt) || upload(ctx, t_lift, &pl) || upload(ctx, pinterp, &ppi) || upload(ctx, ptrace, &ppt))
       return bail(SDG_E_NOMEM);
+    g.op.pinterp = ppi;
+    g.op.ptrace = ppt;
     g.op.interp = pi;
     g.op.grad = pg;
     g.op.face = pf;
@@ -558,6 +592,7 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     if (allocate(ctx, static_cast<size_t>(E) * ctx->ng * naq, &ctx->Gf) || allocate(ctx, nU, &ctx->Rvol) ||
         allocate(ctx, nUf, &ctx->Hs))
       return bail(SDG_E_NOMEM);
+    if (ctx->launch.mfma && allocate(ctx, static_cast<size_t>(E) * nq * D * ctx->nv, &ctx->Fv)) return bail(SDG_E_NOMEM);
   }
   hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
