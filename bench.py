@@ -107,8 +107,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding of the element partition "
-              "is a design (see DESIGN.md) and is not launched", file=sys.stderr)
+        print("bench.py: runs on one GPU only; sharding the element partition over several GPUs "
+              "(face-trace halo exchange per stage) is not implemented", file=sys.stderr)
         sys.exit(2)
 
     t_setup = time.perf_counter()
