@@ -33,6 +33,11 @@ GAMMA = 1.4
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled
+# by a copy kernel of known size (fetch x2.000, write x1.000); kernel summary in
+# profiles/ns_tgv32_p3_fp64_mfma_kernels.md. Keyed by (kernel, cells per direction, order).
+PROFILED_TRAFFIC = {("nsGradMfma", 32, 3): (190387.6 * 2.0 + 614400.2 * 1.0) * 1024}
+
 
 def tgv_setup(n: int, order: int, mach=0.1, reynolds=1600.0):
     mesh = L.Mesh(L.ELEM_HEXAHEDRON, order, (n, n, n), (0.0, 0.0, 0.0), (2 * math.pi,) * 3)
@@ -96,6 +101,21 @@ def cpu_baseline(order: int, target_s: float = 15.0):
             "sample": f"oracle, TGV {n}^3 hex P{order} HLLC+BR2 SSP-RK3 FP64, {steps} steps, {t:.1f} s"}
 
 
+def copy_probe(mib: int, reps: int = 5):
+    """Calibration workload for the HBM byte counters: an elementwise copy of `mib` MiB (read mib,
+    write mib per launch), run under `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` beside the solver
+    so that the counters' scale can be read off a kernel of known traffic."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    src = torch.rand(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "reps": reps, "bytes_read": n * 8, "bytes_written": n * 8}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,7 +125,12 @@ def main():
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--cfl", type=float, default=0.1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run the HBM-counter calibration copy of this many MiB")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.gpus != 1:
         print("bench.py: runs on one GPU only; sharding the element partition over several GPUs "
               "(face-trace halo exchange per stage) is not implemented", file=sys.stderr)
@@ -143,7 +168,8 @@ def main():
         name, ms, flops = "nsFaceFluxKernel+nsResidualMfma", kms[1], f_res
     achieved = flops * mesh.E / (ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel": name,
+                "frac": achieved / PEAK_FP64_TFLOPS,
+                "traffic": PROFILED_TRAFFIC.get((name, args.n, args.order)), "kernel": name,
                 "kernel_ms": ms, "kernel_ms_all": kms.tolist()}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.order)
