@@ -1,17 +1,20 @@
 """Benchmark: DG residual + SSP-RK3 stepping of the 3-D compressible Navier-Stokes Taylor-Green
-vortex on one MI355X.
+vortex on one MI355X, in the FP64 mode and in the mixed-precision mode.
 
 Workload (default): periodic [0, 2 pi]^3, uniform 32^3 hexahedra, P3 (64 modes per element,
-5 variables), Re = 1600, Ma = 0.1, Pr = 0.71, HLLC + BR2, SSP-RK3, FP64. Synthetic initial state
-u = sin x cos y cos z, v = -cos x sin y cos z, w = 0, p = p0 + (cos 2x + cos 2y)(cos 2z + 2)/16,
-rho = p / (R T0).
+5 variables), Re = 1600, Ma = 0.1, Pr = 0.71, HLLC + BR2, SSP-RK3, 500 timed steps. Synthetic
+initial state u = sin x cos y cos z, v = -cos x sin y cos z, w = 0,
+p = p0 + (cos 2x + cos 2y)(cos 2z + 2)/16, rho = p / (R T0).
 
 A step is one SSP-RK3 time step (3 stages) of the whole mesh with the state resident in HBM.
-Metric: DOF.stage/s = E * N_b * stages * steps / wall time of the timed steps.
+Metric: DOF.stage/s = E * N_b * stages * steps / wall time of the timed steps. `value` is the
+mixed-precision rate (FP32 matrix-core contractions, FP64 state, geometry, pointwise physics and RK
+accumulation); the line also carries the FP64 rate, their ratio and the relative L2 deviation of
+the mixed state from the FP64 state after the same steps (per variable group, the largest).
 
-Prints one JSON line with the metric, a roofline entry for the dominant kernel (timed with HIP
-events on the solver's stream) and the CPU baseline (the oracle on a bounded sample of the same
-workload on the host cores).
+Also printed: a roofline entry for the dominant kernel of each mode (HIP events on the solver's
+stream) and the CPU baseline (the oracle on a bounded sample of the same workload on the host
+cores).
 """
 from __future__ import annotations
 
@@ -30,13 +33,15 @@ sys.path.insert(0, ROOT)
 from subrosadg_amd import _lib as L  # noqa: E402
 
 GAMMA = 1.4
-PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+PEAK_FP64_TFLOPS = 78.6        # MI355X dense FP64 (vector and matrix)
+PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_HBM_GBS = 8000.0
 
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled
 # by a copy kernel of known size (fetch x2.000, write x1.000); kernel summary in
-# profiles/ns_tgv32_p3_fp64_mfma_kernels.md. Keyed by (kernel, cells per direction, order).
-PROFILED_TRAFFIC = {("nsGradMfma", 32, 3): (190387.6 * 2.0 + 614400.2 * 1.0) * 1024}
+# profiles/ns_tgv32_p3_fp64_mfma_kernels.md. Keyed by (kernel, precision mode, cells per direction,
+# order).
+PROFILED_TRAFFIC = {("nsGradMfma", "fp64", 32, 3): (190387.6 * 2.0 + 614400.2 * 1.0) * 1024}
 
 
 def tgv_setup(n: int, order: int, mach=0.1, reynolds=1600.0):
@@ -119,12 +124,13 @@ def copy_probe(mib: int, reps: int = 5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=32, help="elements per direction")
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--cfl", type=float, default=0.1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the HBM-counter calibration copy of this many MiB")
     args = ap.parse_args()
@@ -138,12 +144,138 @@ def main():
 
     t_setup = time.perf_counter()
     mesh, prob, prim = tgv_setup(args.n, args.order)
-    solver = L.Solver(prob, mesh)
+    prob_mixed = L.ProblemDesc.from_buffer_copy(prob)
+    prob_mixed.precision = L.PREC_MIXED
+    runs = {}
+    for mode, pd in (("fp64", prob), ("mixed", prob_mixed)):
+        runs[mode] = timed_run(mode, pd, mesh, prim, args)
+    print(f"[bench] setup+runs {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
+    # deviation of the mixed-precision state from the FP64 state after the same warmup + timed steps
+    # from the same initial coefficients (the FP64 engine matches the oracle to 1e-12)
+    dev = group_l2(runs["mixed"]["state"], runs["fp64"]["state"], mesh.dim)
+    dof = mesh.E * mesh.nb
+    f_grad, f_res, _ = algorithmic_counts(mesh)
+    roof = {mode: roofline(r["kms"], f_grad, f_res, mesh, args, mode) for mode, r in runs.items()}
+    head = runs["mixed"]
+
+    adjoint = None if args.no_adjoint else adjoint_leg(args)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.order)
+    line = {
+        "metric": "DOF.stage/s mixed vs FP64 at fixed L2 error (3-D NS Taylor-Green P3)",
+        "value": head["value"],
+        "unit": "DOF.stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 matrix-core contractions, f64 state/geometry/physics/RK",
+        "data": "synthetic",
+        "config": {"workload": f"3-D compressible NS Taylor-Green vortex, periodic [0,2pi]^3, {args.n}^3 hex, "
+                               f"P{args.order}, Re 1600, Ma 0.1, HLLC + BR2, SSP-RK3, CFL {args.cfl}",
+                   "elements": mesh.E, "basis": mesh.nb, "dof_per_field": dof, "precision": "mixed",
+                   "parallelism": "single GPU"},
+        "fp64": {"value": runs["fp64"]["value"], "ms_per_step": runs["fp64"]["ms_per_step"], "dtype": "f64"},
+        "mixed_over_fp64": head["value"] / runs["fp64"]["value"],
+        "mixed_l2_rel_dev_vs_fp64": dev,
+        "l2_dev_after_steps": args.warmup + args.steps,
+        "roofline": roof["mixed"],
+        "roofline_fp64": roof["fp64"],
+        "adjoint": adjoint,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def adjoint_leg(args, n=64, order=3, steps=400, cfl=0.1):
+    """Adjoint/forward time ratio on the double shear layer (periodic [0,1]^2, 64^2 quads, P3,
+    Roe + BR2, Re 1e4, Ma 0.1, 400 SSP-RK3 steps, FP64): forward = the plain 400-step run;
+    adjoint = the checkpointed forward (one snapshot per step) plus the reverse sweep that turns
+    dJ/dU(t_N) of the final kinetic energy J into dJ/dU(t_0). One gradient direction is checked
+    against central differences of J."""
+    gamma, mach = 1.4, 0.1
+    mesh = L.Mesh(L.ELEM_QUADRANGLE, order, (n, n), (0.0, 0.0), (1.0, 1.0))
+    p0 = 1.0 / (gamma * mach ** 2)
+    cv = p0 / (gamma - 1.0)
+    prob = L.problem_desc(equation=L.EQ_NS, convective_flux=L.FLUX_ROE, viscous_flux=L.VISC_BR2,
+                          time_integration=L.RK_SSPRK3, transport=L.TRANSPORT_CONSTANT, cp=gamma * cv, cv=cv,
+                          dynamic_viscosity=1.0e-4)
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 2)
+    x, y = xq[..., 0], xq[..., 1]
+    prim = np.zeros((mesh.E, mesh.nq, 4))
+    prim[..., 0] = 1.0
+    prim[..., 1] = np.where(y <= 0.5, np.tanh(80.0 * (y - 0.25)), np.tanh(80.0 * (0.75 - y)))
+    prim[..., 2] = 0.05 * np.sin(2.0 * math.pi * x)
+    prim[..., 3] = 1.0
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    jw = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
+    lsq = mesh.field("basis_lsq_inverse").reshape(mesh.nb, mesh.nb)
+
+    def ke(U):
+        uq = np.einsum("qb,ebv->eqv", V, U)
+        m2 = uq[..., 1] ** 2 + uq[..., 2] ** 2
+        dj = np.zeros_like(uq)
+        dj[..., 0] = -0.5 * m2 / uq[..., 0] ** 2
+        dj[..., 1:3] = uq[..., 1:3] / uq[..., 0:1]
+        return float((jw * 0.5 * m2 / uq[..., 0]).sum()), np.einsum("eq,qb,eqv->ebv", jw, V, dj)
+
+    s = L.Solver(prob, mesh)
+    s.set_initial_primitive(prim)
+    dt = s.compute_dt(cfl)
+    s.step(dt, 5)  # warm-up (graph capture, clocks)
+    s.set_initial_primitive(prim)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(dt, steps)
+    s.synchronize()
+    t_fwd = time.perf_counter() - t0
+    s.set_initial_primitive(prim)
+    t0 = time.perf_counter()
+    s.adjoint_forward(dt, steps, steps)
+    t_rec = time.perf_counter() - t0
+    J, dJ = ke(s.get_state())
+    t0 = time.perf_counter()
+    lam0 = s.adjoint_backward(dJ)
+    t_bwd = time.perf_counter() - t0
+    # one finite-difference check along a smooth v perturbation
+    dv = np.sin(2 * math.pi * x) * np.cos(2 * math.pi * y)
+    du = np.zeros(prim.shape)
+    du[..., 2] = prim[..., 0] * dv
+    du[..., 3] = prim[..., 0] * prim[..., 2] * dv
+    g_adj = float((lam0 * np.einsum("eqv,qc,cb->ebv", du, V, lsq)).sum())
+    eps = 1e-6
+    js = []
+    for sign in (1.0, -1.0):
+        pp = prim.copy()
+        pp[..., 2] += sign * eps * dv
+        s.set_initial_primitive(pp)
+        s.step(dt, steps)
+        js.append(ke(s.get_state())[0])
+    g_fd = (js[0] - js[1]) / (2 * eps)
+    s.close()
+    out = {"workload": f"2-D NS double shear layer, {n}^2 quads, P{order}, Roe + BR2, Re 1e4, Ma 0.1, "
+                       f"{steps} SSP-RK3 steps, FP64, d(final kinetic energy)/d(initial state)",
+           "forward_s": t_fwd, "checkpointed_forward_s": t_rec, "reverse_s": t_bwd,
+           "adjoint_over_forward": (t_rec + t_bwd) / t_fwd, "reverse_over_forward": t_bwd / t_fwd,
+           "snapshots": steps, "fd_check_rel_err": abs(g_adj - g_fd) / abs(g_fd)}
+    print(f"[bench] adjoint leg: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def group_l2(a, b, dim):
+    """max over variable groups (density, momentum vector, energy) of ||a - b||_2 / ||b||_2."""
+    nv = dim + 2
+    a2, b2 = a.reshape(-1, nv), b.reshape(-1, nv)
+    return max(float(np.linalg.norm(a2[:, g] - b2[:, g]) / np.linalg.norm(b2[:, g]))
+               for g in ([0], list(range(1, dim + 1)), [dim + 1]))
+
+
+def timed_run(mode, pd, mesh, prim, args):
+    solver = L.Solver(pd, mesh)
     solver.set_initial_primitive(prim)
     dt = solver.compute_dt(args.cfl)
-    print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s E={mesh.E} N_b={mesh.nb} dt={dt:.4e}",
-          file=sys.stderr, flush=True)
-
     rel, nan, _ = solver.step(dt, args.warmup)
     solver.synchronize()
     t0 = time.perf_counter()
@@ -151,49 +283,28 @@ def main():
     solver.synchronize()
     wall = time.perf_counter() - t0
     if nan or taken != args.steps or not np.all(np.isfinite(rel)):
-        raise RuntimeError(f"bench: run did not complete cleanly (nan={nan}, taken={taken})")
-    stages = 3
-    dof = mesh.E * mesh.nb
-    value = dof * stages * args.steps / wall
-    ms_per_step = 1000.0 * wall / args.steps
-    print(f"[bench] {args.steps} steps in {wall:.3f}s (device {solver.last_step_ms():.1f} ms), "
-          f"monitor {rel.tolist()}", file=sys.stderr, flush=True)
-
-    # dominant kernel: average duration with HIP events on the solver stream
+        raise RuntimeError(f"bench: {mode} run did not complete cleanly (nan={nan}, taken={taken})")
+    value = mesh.E * mesh.nb * 3 * args.steps / wall
+    print(f"[bench] {mode}: dt={dt:.4e} {args.steps} steps in {wall:.3f}s ({1000 * wall / args.steps:.3f} ms/step, "
+          f"{value:.4e} DOF.stage/s), monitor {rel.tolist()}", file=sys.stderr, flush=True)
+    state = solver.get_state()
+    # per-kernel averages with HIP events on the solver stream (advances the state further)
     kms = solver.time_kernels(dt, max(2, min(args.steps, 10)))
-    f_grad, f_res, b_alg = algorithmic_counts(mesh)
+    solver.close()
+    return {"value": value, "ms_per_step": 1000.0 * wall / args.steps, "state": state, "kms": kms}
+
+
+def roofline(kms, f_grad, f_res, mesh, args, mode):
+    """Roofline entry of the dominant kernel: algorithmic flops per launch / HIP-event duration."""
+    peak = PEAK_FP64_TFLOPS if mode == "fp64" else PEAK_FP32_MFMA_TFLOPS
     if kms[0] >= kms[1]:
         name, ms, flops = "nsGradMfma", kms[0], f_grad
     else:
         name, ms, flops = "nsFaceFluxKernel+nsResidualMfma", kms[1], f_res
     achieved = flops * mesh.E / (ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS,
-                "traffic": PROFILED_TRAFFIC.get((name, args.n, args.order)), "kernel": name,
-                "kernel_ms": ms, "kernel_ms_all": kms.tolist()}
-
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args.order)
-    line = {
-        "metric": "DOF.stage/s (FP64, 3-D NS Taylor-Green P3)",
-        "value": value,
-        "unit": "DOF.stage/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"3-D compressible NS Taylor-Green vortex, periodic [0,2pi]^3, {args.n}^3 hex, "
-                               f"P{args.order}, Re 1600, Ma 0.1, HLLC + BR2, SSP-RK3, CFL {args.cfl}",
-                   "elements": mesh.E, "basis": mesh.nb, "dof_per_field": dof, "precision": "fp64",
-                   "parallelism": "single GPU"},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
+    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": PROFILED_TRAFFIC.get((name, mode, args.n, args.order)), "kernel": name,
+            "kernel_ms": ms, "kernel_ms_all": kms.tolist()}
 
 
 if __name__ == "__main__":

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SDG_ABI_VERSION 1
+#define SDG_ABI_VERSION 2
 
 /* ---- status codes ---------------------------------------------------------------------- */
 #define SDG_OK 0
@@ -56,6 +56,16 @@ enum sdg_time_integration { SDG_RK_FORWARD_EULER = 0, SDG_RK_HEUN2 = 1, SDG_RK_S
 enum sdg_eos { SDG_EOS_IDEAL_GAS = 0, SDG_EOS_WEAK_COMPRESSIBLE = 1 };
 enum sdg_transport { SDG_TRANSPORT_NONE = 0, SDG_TRANSPORT_CONSTANT = 1, SDG_TRANSPORT_SUTHERLAND = 2 };
 enum sdg_precision { SDG_PREC_FP64 = 0, SDG_PREC_MIXED = 1 };
+/* BoundaryConditionEnum (Enum.cpp:92-100), same order */
+enum sdg_boundary {
+  SDG_BC_RIEMANN_FARFIELD = 0,
+  SDG_BC_VELOCITY_INFLOW = 1,
+  SDG_BC_PRESSURE_OUTFLOW = 2,
+  SDG_BC_ISOTHERMAL_NONSLIP_WALL = 3,
+  SDG_BC_ADIABATIC_SLIP_WALL = 4,
+  SDG_BC_ADIABATIC_NONSLIP_WALL = 5,
+  SDG_BC_PERIODIC = 6
+};
 
 /* Runtime copy of the reference's compile-time SimulationControl
  * (SimulationControl.cpp:1197-1279) plus the System setters that carry physical constants
@@ -77,16 +87,20 @@ typedef struct sdg_problem_desc {
   double reference_density;     /* weakly-compressible EOS only                           */
 } sdg_problem_desc;
 
-/* Structured periodic mesh on an axis-aligned box: the project-side replacement for the gmsh
- * transfinite + setPeriodic generators of the examples (e.g. periodic_2d_ceuler.cpp:63-94).
- * element: SDG_ELEM_QUADRANGLE (2-D) or SDG_ELEM_HEXAHEDRON (3-D). Every direction is periodic
- * and needs at least 2 cells. */
+/* Structured mesh on an axis-aligned box: the project-side replacement for the gmsh
+ * transfinite (+ setPeriodic) generators of the examples (e.g. periodic_2d_ceuler.cpp:63-94).
+ * element: SDG_ELEM_QUADRANGLE (2-D) or SDG_ELEM_HEXAHEDRON (3-D). A direction is periodic
+ * (needs at least 2 cells) unless bounded[d] != 0; the sides of a bounded direction are boundary
+ * faces of type boundary[2 d] (low side) and boundary[2 d + 1] (high side), enum sdg_boundary
+ * without SDG_BC_PERIODIC. A zero-initialised descriptor tail means fully periodic. */
 typedef struct sdg_structured_desc {
   int element;            /* enum sdg_element                                             */
   int polynomial_order;   /* P = 1..5                                                     */
   int n[3];               /* cells per direction (n[2] ignored in 2-D)                    */
   double lo[3];           /* box lower corner                                             */
   double hi[3];           /* box upper corner                                             */
+  int bounded[3];         /* 0: periodic direction                                        */
+  int boundary[6];        /* enum sdg_boundary per bounded side                           */
 } sdg_structured_desc;
 
 /* Sizes of a mesh / discretisation (ElementTrait, SimulationControl.cpp:1083-1108). */
@@ -101,6 +115,7 @@ typedef struct sdg_mesh_info {
   int faces_per_element;  /* N_f                                                          */
   int face_quadrature;    /* N_fq  (points per face, Gauss order 2P+1)                    */
   int affine;             /* 1 when every element has a constant Jacobian                 */
+  int64_t boundary_faces; /* boundary faces F_b                                           */
 } sdg_mesh_info;
 
 /* Operand tables a mesh exports (all FP64 unless noted). Shapes:
@@ -124,6 +139,11 @@ typedef struct sdg_mesh_info {
  *                                      (getAdjacencyElementQuadratureSequence,
  *                                       SimulationControl.cpp:381-523)
  *   FACE_ROTATION (int32)   F          rotation index as defined by Adjacency.cpp:169-171
+ *   BOUNDARY_FACE_COORD     F_b x N_fq x D   physical boundary face points
+ *   BOUNDARY_FACE_NORMAL    F_b x N_fq x D   outward from the element
+ *   BOUNDARY_FACE_JACOBIAN_W F_b x N_fq      |J_f| w
+ *   BOUNDARY_FACE_PARENT (int32) F_b x 2     element, its local face index
+ *   BOUNDARY_FACE_TYPE (int32)   F_b         enum sdg_boundary
  */
 enum sdg_mesh_field {
   SDG_FIELD_BASIS_VALUE = 0,
@@ -146,7 +166,12 @@ enum sdg_mesh_field {
   SDG_FIELD_FACE_ROTATION = 17,
   SDG_FIELD_QUAD_POINT = 18,      /* N_q x D reference-element volume points */
   SDG_FIELD_FACE_QUAD_POINT = 19, /* N_fq x (D-1) reference face points      */
-  SDG_FIELD_COUNT_ = 20
+  SDG_FIELD_BOUNDARY_FACE_COORD = 20,
+  SDG_FIELD_BOUNDARY_FACE_NORMAL = 21,
+  SDG_FIELD_BOUNDARY_FACE_JACOBIAN_W = 22,
+  SDG_FIELD_BOUNDARY_FACE_PARENT = 23,
+  SDG_FIELD_BOUNDARY_FACE_TYPE = 24,
+  SDG_FIELD_COUNT_ = 25
 };
 
 typedef struct sdg_mesh sdg_mesh;
@@ -194,6 +219,13 @@ int sdg_compute_dt(sdg_ctx* ctx, double cfl, double* dt);
  * calculateGardientQuadrature .. calculateResidual of stepSolver (TimeIntegration.cpp:340-346).
  * residual: reference layout (E x N_b x NV). Leaves the state unchanged. */
 int sdg_residual(sdg_ctx* ctx, double* residual);
+
+/* Boundary data of a mesh with boundary faces: the primitive state (rho, u, v[, w], T) that the
+ * user's BoundaryCondition::calculatePrimitiveFromCoordinate gives at every boundary face point,
+ * F_b x N_fq x NV (points from SDG_FIELD_BOUNDARY_FACE_COORD). Converted like the reference's
+ * boundary_dummy_variable_ (BoundaryCondition.cpp:29-50); call it before stepping, and again
+ * between steps for time-varying boundary data. */
+int sdg_set_boundary_primitive(sdg_ctx* ctx, const double* primitive_at_boundary_points);
 /* NS only: the lifted total-gradient coefficients w of the current state, (NV*D) x N_b per
  * element column-major, row var*D+dim (variable_gradient_basis_function_coefficient_,
  * TimeIntegration.cpp:200-228). */
@@ -220,6 +252,27 @@ int sdg_synchronize(sdg_ctx* ctx);
  * The state advances exactly as with sdg_step. */
 #define SDG_KERNEL_CLASSES 3
 int sdg_time_kernels(sdg_ctx* ctx, double dt, int nsteps, double* avg_ms, int n_out);
+
+/* ---- discrete adjoint (Navier-Stokes, FP64, matrix-core element sizes) -----------------------
+ * The reference has no adjoint; these entry points transpose its residual (R of sdg_residual,
+ * SpatialDiscrete.cpp:194-1116) and its RK stage update (TimeIntegration.cpp:181-198).
+ * All arrays are in the reference layout (E x N_b x NV, NV fastest). */
+
+/* out = (dR/dU)^T lambda at the current state. */
+int sdg_vjp_residual(sdg_ctx* ctx, const double* lambda, double* out);
+
+/* Advance nsteps steps of size dt from the current state, like sdg_step, while recording
+ * checkpoints for sdg_adjoint_backward: at most `snapshots` states (>= 1; the initial state is
+ * one of them) at the positions of a binomial checkpointing schedule (Griewank's revolve bound:
+ * with s snapshots, n steps are reversed with r repetitions where C(s+r, s) >= n). With
+ * snapshots >= nsteps every step state is kept and nothing is recomputed twice. Fails with
+ * SDG_E_STATE if the monitor turns NaN. */
+int sdg_adjoint_forward(sdg_ctx* ctx, double dt, int nsteps, int snapshots);
+
+/* Reverse sweep of the last sdg_adjoint_forward: given lambda_final = dJ/dU(t_N), writes
+ * lambda_initial = dJ/dU(t_0). Recomputes stage states from the checkpoints; leaves the context
+ * state at the recorded initial state. */
+int sdg_adjoint_backward(sdg_ctx* ctx, const double* lambda_final, double* lambda_initial);
 
 #ifdef __cplusplus
 }

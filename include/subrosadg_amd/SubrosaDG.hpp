@@ -7,8 +7,12 @@
 // System setters and solve() loop with the all-NaN stop (SystemControl.cpp:159-195), and the
 // user-specialised InitialCondition<SC>::calculatePrimitiveFromCoordinate.
 // Changed: coordinates and primitive states are std::array instead of Eigen vectors; meshes are
-// structured periodic boxes (setMesh(StructuredMesh)) instead of gmsh files; raw-binary and VTU
-// output are not produced (the monitor history is kept in memory and can be streamed).
+// structured boxes (setMesh(StructuredMesh)) instead of gmsh files, each box side carrying a
+// physical index (or -1: periodic direction); raw-binary and VTU output are not produced (the
+// monitor history is kept in memory and can be streamed). Boundary data come from the
+// user-specialised BoundaryCondition<SC>::calculatePrimitiveFromCoordinate(x, t, physical_index),
+// evaluated at the boundary face points at start-up and, for BoundaryTimeEnum::TimeVarying,
+// before every step (BoundaryCondition.cpp:29-50).
 #ifndef SUBROSA_DG_AMD_HPP_
 #define SUBROSA_DG_AMD_HPP_
 
@@ -107,11 +111,25 @@ struct InitialCondition {
       const std::array<Real, SC::kDimension>& coordinate) const;
 };
 
-// Structured periodic box (replaces the gmsh generate-mesh callback of the examples).
+// Specialise calculatePrimitiveFromCoordinate for meshes with boundary sides; the default is only
+// reached when a boundary exists without one.
+template <typename SC>
+struct BoundaryCondition {
+  std::array<Real, SC::kPrimitiveVariableNumber> calculatePrimitiveFromCoordinate(
+      const std::array<Real, SC::kDimension>& /*coordinate*/, Real /*time*/, Isize physical_index) const {
+    throw std::runtime_error("SubrosaDG: BoundaryCondition<SC>::calculatePrimitiveFromCoordinate is not specialised "
+                             "(boundary physical index " + std::to_string(physical_index) + ")");
+  }
+};
+
+// Structured box (replaces the gmsh generate-mesh callback of the examples). physical_index gives
+// the boundary physical index of each side (x low, x high, y low, y high, z low, z high); a
+// direction whose two sides are -1 is periodic.
 struct StructuredMesh {
   std::array<int, 3> cells{2, 2, 2};
   std::array<Real, 3> lower{0.0, 0.0, 0.0};
   std::array<Real, 3> upper{1.0, 1.0, 1.0};
+  std::array<Isize, 6> physical_index{-1, -1, -1, -1, -1, -1};
 };
 
 template <typename SC>
@@ -125,6 +143,8 @@ struct System {
                 "initial conditions come from InitialCondition<SC>::calculatePrimitiveFromCoordinate");
 
   InitialCondition<SC> initial_condition_;
+  BoundaryCondition<SC> boundary_condition_;
+  std::vector<std::pair<Isize, BoundaryConditionEnum>> boundary_types_;
   StructuredMesh mesh_desc_;
   bool mesh_set_ = false;
   sdg_problem_desc problem_{};
@@ -163,11 +183,12 @@ struct System {
     mesh_desc_ = mesh;
     mesh_set_ = true;
   }
-  // SystemControl.cpp:76-80: only periodic boundaries exist on a structured periodic box
+  // SystemControl.cpp:76-80: boundary type of a physical index (Periodic: nothing to record,
+  // periodic directions are the ones whose sides carry index -1)
   template <BoundaryConditionEnum BoundaryConditionType>
-  void addBoundaryCondition([[maybe_unused]] const Isize physical_index) {
-    static_assert(BoundaryConditionType == BoundaryConditionEnum::Periodic,
-                  "structured meshes are periodic; wall and far-field boundaries are not implemented");
+  void addBoundaryCondition(const Isize physical_index) {
+    if constexpr (BoundaryConditionType != BoundaryConditionEnum::Periodic)
+      boundary_types_.emplace_back(physical_index, BoundaryConditionType);
   }
   // SystemControl.cpp:82-88
   template <ThermodynamicModelEnum ThermodynamicModelType>
@@ -213,8 +234,45 @@ struct System {
       d.lo[i] = mesh_desc_.lower[i];
       d.hi[i] = mesh_desc_.upper[i];
     }
+    for (int dir = 0; dir < SC::kDimension; dir++) {
+      if (mesh_desc_.physical_index[2 * dir] < 0 && mesh_desc_.physical_index[2 * dir + 1] < 0) continue;
+      d.bounded[dir] = 1;
+      for (int side = 0; side < 2; side++) d.boundary[2 * dir + side] = boundaryType(mesh_desc_.physical_index[2 * dir + side]);
+    }
     check(sdg_mesh_create_structured(&d, &mesh_), nullptr);
     check(sdg_create(&problem_, mesh_, &ctx_), nullptr);
+  }
+
+  int boundaryType(const Isize physical_index) const {
+    for (const auto& b : boundary_types_)
+      if (b.first == physical_index) return static_cast<int>(b.second);
+    throw std::runtime_error("SubrosaDG: no boundary condition added for physical index " +
+                             std::to_string(physical_index));
+  }
+
+  // boundary data at every boundary face point for time t (BoundaryCondition.cpp:29-50); the
+  // side of an axis-aligned box face, hence its physical index, follows from its normal
+  void updateBoundary(const Real time) {
+    constexpr int D = SC::kDimension, NV = SC::kConservedVariableNumber;
+    sdg_mesh_info info{};
+    check(sdg_mesh_get_info(mesh_, &info), nullptr);
+    if (info.boundary_faces == 0) return;
+    const int64_t npts = info.boundary_faces * info.face_quadrature;
+    std::vector<double> xb(static_cast<size_t>(npts * D)), nb(static_cast<size_t>(npts * D)),
+        prim(static_cast<size_t>(npts * NV));
+    check(sdg_mesh_export(mesh_, SDG_FIELD_BOUNDARY_FACE_COORD, xb.data(), npts * D), nullptr);
+    check(sdg_mesh_export(mesh_, SDG_FIELD_BOUNDARY_FACE_NORMAL, nb.data(), npts * D), nullptr);
+    for (int64_t p = 0; p < npts; p++) {
+      std::array<Real, D> x;
+      int side = 0;
+      for (int dd = 0; dd < D; dd++) {
+        x[dd] = xb[p * D + dd];
+        if (std::fabs(nb[p * D + dd]) > 0.5) side = 2 * dd + (nb[p * D + dd] > 0.0 ? 1 : 0);
+      }
+      const auto v = boundary_condition_.calculatePrimitiveFromCoordinate(x, time, mesh_desc_.physical_index[side]);
+      for (int k = 0; k < NV; k++) prim[p * NV + k] = v[k];
+    }
+    check(sdg_set_boundary_primitive(ctx_, prim.data()), ctx_);
   }
 
   // SystemControl.cpp:159-195
@@ -234,8 +292,10 @@ struct System {
       for (int k = 0; k < NV; k++) prim[p * NV + k] = v[k];
     }
     check(sdg_set_initial_primitive(ctx_, prim.data()), ctx_);
+    updateBoundary(0.0);
     if (delta_time_ == 0.0) check(sdg_compute_dt(ctx_, cfl_, &delta_time_), ctx_);
     for (int i = iteration_start_ + 1; i <= iteration_end_; i++) {
+      if (SC::kBoundaryTime == BoundaryTimeEnum::TimeVarying) updateBoundary(i * delta_time_);
       std::array<Real, NV> rel{};
       int nan = 0, taken = 0;
       check(sdg_step(ctx_, delta_time_, 1, rel.data(), &nan, &taken), ctx_);

@@ -115,6 +115,11 @@ struct Oracle {
   int nig = 0;  // interface-gradient residual blocks per element: 1 (BR1) or N_f (BR2)
   std::vector<double> rel_err;
   bool has_state = false;
+  // boundary faces (the element is the left parent): element, local face, type; per point the
+  // outward normal, |J_f| w and the boundary data (boundary_dummy_variable_, computational)
+  int64_t nbface = 0;
+  std::vector<int32_t> bparent, blocal, btype;
+  std::vector<double> bnormal, bjw, bdummy;
 
   size_t szC() const { return static_cast<size_t>(nv) * nb; }
   size_t szG() const { return static_cast<size_t>(nv) * dim * nb; }
@@ -334,6 +339,154 @@ struct Oracle {
   }
 
   int accumulate(int s) const { return s * nfq; }  // getElementAccumulateAdjacencyQuadratureNumber
+
+  // --- boundary conditions (BoundaryCondition.cpp:79-547), computational variables ---
+  // calculateBoundaryVariable: boundary state wb from the interior state wl and boundary data wd
+  void boundaryVariable(int type, const double* n, const double* wl, const double* wd, double* wb) const {
+    const int nw = dim + 3;
+    double unl = 0.0;
+    for (int d = 0; d < dim; d++) unl += wl[1 + d] * n[d];
+    const double mach = unl / phys.soundSpeed(wl[0], wl[dim + 2]);
+    switch (type) {
+      case 0: {  // RiemannFarfield :79-247
+        if (std::fabs(mach) > 1.0) {
+          const double* src = mach < 0.0 ? wd : wl;  // supersonic inflow / outflow
+          for (int i = 0; i < nw; i++) wb[i] = src[i];
+          break;
+        }
+        const double gm1 = kGamma - 1.0;
+        double und = 0.0;
+        for (int d = 0; d < dim; d++) und += wd[1 + d] * n[d];
+        const double left_toward = und - 2.0 * phys.soundSpeed(wd[0], wd[dim + 2]) / gm1;
+        const double right_toward = unl + 2.0 * phys.soundSpeed(wl[0], wl[dim + 2]) / gm1;
+        const double unb = (left_toward + right_toward) / 2.0;
+        const double* ref = mach < 0.0 ? wd : wl;  // subsonic inflow: outside, outflow: inside
+        double unr = 0.0;
+        for (int d = 0; d < dim; d++) unr += ref[1 + d] * n[d];
+        for (int d = 0; d < dim; d++) wb[1 + d] = ref[1 + d] + (unb - unr) * n[d];
+        const double c = gm1 * (right_toward - left_toward) / 4.0;
+        const double entropy = ref[dim + 2] / std::pow(ref[0], kGamma);
+        const double rho = std::pow(c * c / (kGamma * entropy), 1.0 / gm1);
+        const double p = rho * c * c / kGamma;
+        wb[0] = rho;
+        wb[dim + 1] = p / (gm1 * rho);
+        wb[dim + 2] = p;
+        break;
+      }
+      case 1:  // VelocityInflow :309-333
+        for (int i = 0; i < nw; i++) wb[i] = wd[i];
+        if (mach > -1.0) wb[dim + 2] = wl[dim + 2];
+        break;
+      case 2:  // PressureOutflow :362-386
+        for (int i = 0; i < nw; i++) wb[i] = wl[i];
+        if (mach < 1.0) wb[dim + 2] = wd[dim + 2];
+        break;
+      case 3:  // IsoThermalNonSlipWall :416-434
+        wb[0] = wl[0];
+        for (int d = 0; d < dim; d++) wb[1 + d] = wd[1 + d];
+        wb[dim + 1] = wd[dim + 1];
+        wb[dim + 2] = phys.pressure(wl[0], wd[dim + 1]);
+        break;
+      case 4:  // AdiabaticSlipWall :474-488
+        for (int i = 0; i < nw; i++) wb[i] = wl[i];
+        for (int d = 0; d < dim; d++) wb[1 + d] = wl[1 + d] - unl * n[d];
+        break;
+      default:  // AdiabaticNonSlipWall :521-533
+        for (int i = 0; i < nw; i++) wb[i] = wl[i];
+        for (int d = 0; d < dim; d++) wb[1 + d] = wd[1 + d];
+        break;
+    }
+  }
+  void conservedFromComputational(const double* w, double* u) const {
+    u[0] = w[0];
+    for (int d = 0; d < dim; d++) u[1 + d] = w[0] * w[1 + d];
+    u[dim + 1] = w[0] * (w[dim + 1] + velocitySquared(w) / 2.0);
+  }
+  void boundaryTraces(int64_t b, double* ul) const {  // NV x N_fq of the boundary face's element
+    const int64_t e = bparent[b];
+    const int s = blocal[b];
+    for (int j = 0; j < nfq; j++)
+      for (int v = 0; v < nv; v++) {
+        double a = 0.0;
+        for (int i = 0; i < nb; i++) a += coeff[e * szC() + v + nv * i] * Vf[(accumulate(s) + j) + static_cast<size_t>(naq) * i];
+        ul[v + nv * j] = a;
+      }
+  }
+  void boundaryGradientQuadrature() {  // SpatialDiscrete.cpp:913-968 with calculateBoundaryGradientVariable
+    const int rows = nv * dim;
+#pragma omp parallel for schedule(static)
+    for (int64_t b = 0; b < nbface; b++) {
+      std::vector<double> ul(static_cast<size_t>(nv) * nfq);
+      boundaryTraces(b, ul.data());
+      const int64_t e = bparent[b];
+      const int s = blocal[b];
+      for (int j = 0; j < nfq; j++) {
+        const size_t bp = static_cast<size_t>(b) * nfq + j;
+        const double* n = &bnormal[bp * dim];
+        const double jw = bjw[bp];
+        const double* u = &ul[nv * j];
+        double vg[5], ig[5];
+        if (btype[b] <= 2) {  // farfield, inflow, outflow: interior state, no interface term
+          for (int v = 0; v < nv; v++) {
+            vg[v] = u[v];
+            ig[v] = 0.0;
+          }
+        } else {  // walls: conserved boundary state and its jump
+          double wl[8], wb[8];
+          computational(u, wl);
+          boundaryVariable(btype[b], n, wl, &bdummy[bp * (dim + 3)], wb);
+          conservedFromComputational(wb, vg);
+          for (int v = 0; v < nv; v++) ig[v] = vg[v] - u[v];
+        }
+        double* cl = &vg_adj[static_cast<size_t>(e) * rows * naq + static_cast<size_t>(rows) * (accumulate(s) + j)];
+        for (int v = 0; v < nv; v++)
+          for (int a = 0; a < dim; a++) cl[v * dim + a] = (n[a] * vg[v]) * jw;
+        if (ns()) {
+          double* il = &ig_adj[static_cast<size_t>(e) * rows * naq + static_cast<size_t>(rows) * (accumulate(s) + j)];
+          for (int v = 0; v < nv; v++)
+            for (int a = 0; a < dim; a++) il[v * dim + a] = (n[a] * ig[v]) * jw;
+        }
+      }
+    }
+  }
+  void boundaryQuadrature() {  // SpatialDiscrete.cpp:750-842
+    const int rows = nv * dim;
+#pragma omp parallel for schedule(static)
+    for (int64_t b = 0; b < nbface; b++) {
+      std::vector<double> ul(static_cast<size_t>(nv) * nfq), gl;
+      boundaryTraces(b, ul.data());
+      const int64_t e = bparent[b];
+      const int s = blocal[b];
+      if (ns()) {
+        gl.resize(static_cast<size_t>(rows) * nfq);
+        faceGradientTrace(e, s, gl.data());
+      }
+      for (int j = 0; j < nfq; j++) {
+        const size_t bp = static_cast<size_t>(b) * nfq + j;
+        const double* n = &bnormal[bp * dim];
+        const double* u = &ul[nv * j];
+        double wl[8], wb[8], h[5], tmp[5];
+        computational(u, wl);
+        boundaryVariable(btype[b], n, wl, &bdummy[bp * (dim + 3)], wb);
+        normalFlux(n, wb, h);  // calculateConvectiveNormalFlux of the boundary state
+        if (ns()) {
+          double pgl[15], pgb[15], fl[5], fb[5], wm[8];
+          primitiveGradient(wl, u, &gl[static_cast<size_t>(rows) * j], pgl);
+          for (int r = 0; r < rows; r++) pgb[r] = pgl[r];
+          if (btype[b] == 4 || btype[b] == 5)  // adiabatic walls: no temperature gradient
+            for (int a = 0; a < dim; a++) pgb[(dim + 1) * dim + a] = 0.0;
+          for (int i = 0; i < dim + 3; i++) wm[i] = btype[b] >= 3 ? wb[i] : wl[i];  // modifyBoundaryVariable
+          viscousNormalFlux(n, wm, pgl, fl);
+          viscousNormalFlux(n, wb, pgb, fb);
+          for (int v = 0; v < nv; v++) tmp[v] = (h[v] - (fl[v] + fb[v]) / 2.0) * bjw[bp];
+        } else {
+          for (int v = 0; v < nv; v++) tmp[v] = h[v] * bjw[bp];
+        }
+        double* cl = &adj_quad[static_cast<size_t>(e) * nv * naq + static_cast<size_t>(nv) * (accumulate(s) + j)];
+        for (int v = 0; v < nv; v++) cl[v] = tmp[v];
+      }
+    }
+  }
 
   // --- passes of stepSolver ---
   void gradientQuadrature() {  // SpatialDiscrete.cpp:294-346
@@ -599,10 +752,12 @@ struct Oracle {
   void spatial() {
     gradientQuadrature();
     adjacencyGradientQuadrature();
+    boundaryGradientQuadrature();
     gradientResidual();
     updateGradientCoefficient();
     volumeQuadrature();
     adjacencyQuadrature();
+    boundaryQuadrature();
     calcResidual();
   }
   void step(double dt) {  // TimeIntegration.cpp:327-350
@@ -702,6 +857,40 @@ void* oracle_create(const sdg_problem_desc* p, int dim, int nb, int nq, int nf, 
 }
 
 void oracle_destroy(void* h) { delete static_cast<Oracle*>(h); }
+
+// Boundary faces: parent_local (F_b x 2: element, local face), type (F_b), normal
+// (F_b x N_fq x D), jw (F_b x N_fq).
+int oracle_set_boundary(void* h, int64_t nbface, const int32_t* parent_local, const int32_t* type, const double* normal,
+                        const double* jw) {
+  Oracle* o = static_cast<Oracle*>(h);
+  o->nbface = nbface;
+  o->bparent.resize(nbface);
+  o->blocal.resize(nbface);
+  o->btype.assign(type, type + nbface);
+  for (int64_t b = 0; b < nbface; b++) {
+    o->bparent[b] = parent_local[2 * b];
+    o->blocal[b] = parent_local[2 * b + 1];
+  }
+  o->bnormal.assign(normal, normal + nbface * o->nfq * o->dim);
+  o->bjw.assign(jw, jw + nbface * o->nfq);
+  o->bdummy.assign(static_cast<size_t>(nbface) * o->nfq * (o->dim + 3), 0.0);
+  return 0;
+}
+// Boundary data: primitive (rho, u.., T) per boundary point -> computational
+// (VariableConvertor.cpp:118-139)
+int oracle_set_boundary_primitive(void* h, const double* prim) {
+  Oracle* o = static_cast<Oracle*>(h);
+  const int dim = o->dim, nv = o->nv;
+  for (size_t i = 0; i < static_cast<size_t>(o->nbface) * o->nfq; i++) {
+    const double* pr = prim + i * nv;
+    double* w = &o->bdummy[i * (dim + 3)];
+    w[0] = pr[0];
+    for (int d = 0; d < dim; d++) w[1 + d] = pr[1 + d];
+    w[dim + 1] = o->phys.internalEnergyFromTemperature(pr[dim + 1]);
+    w[dim + 2] = o->phys.pressure(w[0], w[dim + 1]);
+  }
+  return 0;
+}
 
 int oracle_set_state(void* h, const double* c) {
   Oracle* o = static_cast<Oracle*>(h);
@@ -809,6 +998,28 @@ int oracle_point_flux(const sdg_problem_desc* p, int dim, int flux, const double
   o.computational(ul, wl);
   o.computational(ur, wr);
   o.convectiveFlux(n, ul, wl, ur, wr, h);
+  return 0;
+}
+// Boundary state of calculateBoundaryVariable at one point: conserved interior state ul,
+// primitive boundary data prim (rho, u.., T); wb: computational (rho, u.., e, p).
+int oracle_point_boundary_state(const sdg_problem_desc* p, int dim, int type, const double* n, const double* ul,
+                                const double* prim, double* wb) {
+  Oracle o;
+  o.dim = dim;
+  o.nv = dim + 2;
+  o.phys.eos = p->equation_of_state;
+  o.phys.cp = p->cp;
+  o.phys.cv = p->cv;
+  o.phys.c0 = p->reference_sound_speed;
+  o.phys.rho0 = p->reference_density;
+  o.phys.padd = 0.01 * p->reference_density * p->reference_sound_speed * p->reference_sound_speed;
+  double wl[8], wd[8];
+  o.computational(ul, wl);
+  wd[0] = prim[0];
+  for (int d = 0; d < dim; d++) wd[1 + d] = prim[1 + d];
+  wd[dim + 1] = o.phys.internalEnergyFromTemperature(prim[dim + 1]);
+  wd[dim + 2] = o.phys.pressure(wd[0], wd[dim + 1]);
+  o.boundaryVariable(type, n, wl, wd, wb);
   return 0;
 }
 // Viscous normal flux at one point from conserved state and conserved gradient (row var*D+dim).

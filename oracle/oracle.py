@@ -49,6 +49,9 @@ def load():
         lib.oracle_step.argtypes = [c_void_p, c_double, c_int, c_void_p, POINTER(c_int), POINTER(c_int)]
         lib.oracle_point_flux.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
         lib.oracle_point_viscous_flux.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
+        lib.oracle_point_boundary_state.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
+        lib.oracle_set_boundary.argtypes = [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]
+        lib.oracle_set_boundary_primitive.argtypes = [c_void_p, c_void_p]
         lib.oracle_set_threads.argtypes = [c_int]
         lib.oracle_get_threads.restype = c_int
         _lib = lib
@@ -81,6 +84,19 @@ class Oracle:
             _ptr(k["face_sequence"]), _ptr(k["face_normal"]), _ptr(k["face_jacobian_w"]))
         if not self.h:
             raise RuntimeError("oracle_create failed")
+        self.Fb = getattr(mesh, "Fb", 0)
+        if self.Fb:
+            for key in ("boundary_face_parent", "boundary_face_type", "boundary_face_normal",
+                        "boundary_face_jacobian_w"):
+                k[key] = mesh.field(key)
+            self.lib.oracle_set_boundary(self.h, self.Fb, _ptr(k["boundary_face_parent"]),
+                                         _ptr(k["boundary_face_type"]), _ptr(k["boundary_face_normal"]),
+                                         _ptr(k["boundary_face_jacobian_w"]))
+
+    def set_boundary_primitive(self, prim: np.ndarray):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        assert prim.size == self.Fb * self.mesh.nfq * self.nv
+        self.lib.oracle_set_boundary_primitive(self.h, _ptr(prim))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -154,3 +170,15 @@ def point_viscous_flux(problem, dim: int, n, u, g) -> np.ndarray:
     f = np.empty(dim + 2)
     lib.oracle_point_viscous_flux(ctypes.byref(problem), dim, _ptr(n), _ptr(u), _ptr(g), _ptr(f))
     return f
+
+
+def point_boundary_state(problem, dim: int, bc: int, n, ul, prim) -> np.ndarray:
+    """Boundary state (rho, u.., e, p) of boundary type bc from the conserved interior state ul and
+    the primitive boundary data prim."""
+    lib = load()
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    ul = np.ascontiguousarray(ul, dtype=np.float64)
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    wb = np.empty(dim + 3)
+    lib.oracle_point_boundary_state(ctypes.byref(problem), dim, bc, _ptr(n), _ptr(ul), _ptr(prim), _ptr(wb))
+    return wb

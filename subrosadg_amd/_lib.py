@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsdg_hip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # status codes
 OK, E_INVALID, E_UNSUPPORTED, E_DEVICE, E_NOMEM, E_STATE = 0, -1, -2, -3, -4, -5
@@ -28,6 +28,8 @@ RK_FORWARD_EULER, RK_HEUN2, RK_SSPRK3 = 0, 1, 2
 EOS_IDEAL_GAS, EOS_WEAK_COMPRESSIBLE = 0, 1
 TRANSPORT_NONE, TRANSPORT_CONSTANT, TRANSPORT_SUTHERLAND = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
+(BC_RIEMANN_FARFIELD, BC_VELOCITY_INFLOW, BC_PRESSURE_OUTFLOW, BC_ISOTHERMAL_NONSLIP_WALL, BC_ADIABATIC_SLIP_WALL,
+ BC_ADIABATIC_NONSLIP_WALL, BC_PERIODIC) = range(7)
 
 FIELDS = {
     "basis_value": (0, np.float64),
@@ -50,6 +52,11 @@ FIELDS = {
     "face_rotation": (17, np.int32),
     "quad_point": (18, np.float64),
     "face_quad_point": (19, np.float64),
+    "boundary_face_coord": (20, np.float64),
+    "boundary_face_normal": (21, np.float64),
+    "boundary_face_jacobian_w": (22, np.float64),
+    "boundary_face_parent": (23, np.int32),
+    "boundary_face_type": (24, np.int32),
 }
 
 
@@ -78,6 +85,8 @@ class StructuredDesc(Structure):
         ("n", c_int * 3),
         ("lo", c_double * 3),
         ("hi", c_double * 3),
+        ("bounded", c_int * 3),
+        ("boundary", c_int * 6),
     ]
 
 
@@ -93,6 +102,7 @@ class MeshInfo(Structure):
         ("faces_per_element", c_int),
         ("face_quadrature", c_int),
         ("affine", c_int),
+        ("boundary_faces", c_int64),
     ]
 
 
@@ -112,11 +122,15 @@ SIGNATURES = {
     "sdg_get_state": (c_int, [c_void_p, c_void_p]),
     "sdg_compute_dt": (c_int, [c_void_p, c_double, POINTER(c_double)]),
     "sdg_residual": (c_int, [c_void_p, c_void_p]),
+    "sdg_set_boundary_primitive": (c_int, [c_void_p, c_void_p]),
     "sdg_get_gradient": (c_int, [c_void_p, c_void_p]),
     "sdg_step": (c_int, [c_void_p, c_double, c_int, c_void_p, POINTER(c_int), POINTER(c_int)]),
     "sdg_last_step_ms": (c_int, [c_void_p, POINTER(c_double)]),
     "sdg_synchronize": (c_int, [c_void_p]),
     "sdg_time_kernels": (c_int, [c_void_p, c_double, c_int, c_void_p, c_int]),
+    "sdg_vjp_residual": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "sdg_adjoint_forward": (c_int, [c_void_p, c_double, c_int, c_int]),
+    "sdg_adjoint_backward": (c_int, [c_void_p, c_void_p, c_void_p]),
 }
 
 _lib = None
@@ -174,9 +188,11 @@ def problem_desc(**kw) -> ProblemDesc:
 
 
 class Mesh:
-    """Structured periodic mesh plus its operand tables (host only)."""
+    """Structured mesh on a box plus its operand tables (host only). Periodic in every direction
+    unless `boundary` gives, per direction, the (low, high) boundary types (BC_* constants) or None
+    for a periodic direction."""
 
-    def __init__(self, element: int, order: int, n, lo, hi):
+    def __init__(self, element: int, order: int, n, lo, hi, boundary=None):
         self.lib = load()
         d = StructuredDesc()
         d.element = element
@@ -185,6 +201,12 @@ class Mesh:
             d.n[i] = int(n[i]) if i < len(n) else 1
             d.lo[i] = float(lo[i]) if i < len(lo) else 0.0
             d.hi[i] = float(hi[i]) if i < len(hi) else 1.0
+        if boundary is not None:
+            for i, sides in enumerate(boundary):
+                if sides is not None:
+                    d.bounded[i] = 1
+                    d.boundary[2 * i] = int(sides[0])
+                    d.boundary[2 * i + 1] = int(sides[1])
         h = c_void_p()
         check(self.lib.sdg_mesh_create_structured(ctypes.byref(d), ctypes.byref(h)))
         self.handle = h
@@ -199,6 +221,7 @@ class Mesh:
         self.nq = info.quadrature
         self.nf = info.faces_per_element
         self.nfq = info.face_quadrature
+        self.Fb = info.boundary_faces
         self.nv = self.dim + 2
 
     def field(self, name: str) -> np.ndarray:
@@ -259,6 +282,30 @@ class Solver:
     def gradient(self) -> np.ndarray:
         out = np.empty((self.E, self.nb, self.nv * self.dim))
         self._check(self.lib.sdg_get_gradient(self.handle, out.ctypes.data))
+        return out
+
+    def set_boundary_primitive(self, prim: np.ndarray):
+        """Primitive boundary data (rho, u.., T) at every boundary face point (F_b, N_fq, NV)."""
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        assert prim.size == self.mesh.Fb * self.mesh.nfq * self.nv
+        self._check(self.lib.sdg_set_boundary_primitive(self.handle, prim.ctypes.data if prim.size else None))
+
+    def vjp_residual(self, lam: np.ndarray) -> np.ndarray:
+        """(dR/dU)^T lam at the current state (reference layout)."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(self.state_shape())
+        out = np.empty(self.state_shape())
+        self._check(self.lib.sdg_vjp_residual(self.handle, lam.ctypes.data, out.ctypes.data))
+        return out
+
+    def adjoint_forward(self, dt: float, nsteps: int, snapshots: int):
+        """Advance nsteps steps recording binomial checkpoints (at most `snapshots` states)."""
+        self._check(self.lib.sdg_adjoint_forward(self.handle, dt, nsteps, snapshots))
+
+    def adjoint_backward(self, lam_final: np.ndarray) -> np.ndarray:
+        """dJ/dU(t_0) from dJ/dU(t_N) for the last adjoint_forward run."""
+        lam = np.ascontiguousarray(lam_final, dtype=np.float64).reshape(self.state_shape())
+        out = np.empty(self.state_shape())
+        self._check(self.lib.sdg_adjoint_backward(self.handle, lam.ctypes.data, out.ctypes.data))
         return out
 
     def compute_dt(self, cfl: float) -> float:

@@ -17,7 +17,7 @@ BUILD_DIR = os.path.join(HERE, "_build")
 
 HOST_SOURCES = ["kit.cpp", "sdg_mesh_api.cpp"]
 HIP_SOURCES = ["sdg_solver.hip"]
-EXAMPLES = ["taylor_green_3d_cns"]
+EXAMPLES = ["taylor_green_3d_cns", "lid_driven_cavity_2d_cns"]
 ARCH = os.environ.get("SDG_OFFLOAD_ARCH", "gfx950")
 
 # Packed-FP32 VALU instructions are disabled for all device code; hipcc's host pass warns that it

@@ -1,0 +1,487 @@
+// dg_adjoint.hpp -- discrete adjoint (vector-Jacobian product) of the Navier-Stokes residual for the
+// matrix-core element sizes, FP64.
+//
+// Forward residual of one element (the MFMA stage kernels):
+//   u_q = V U, traces u_s = Vf U;  A_s = +-(u_L + u_R)/2 (x) n |J_f| w,  J_s = (u_R - u_L)/2 (x) n |J_f| w
+//   R_w = A Phi_f - Q Phi_grad (Q = u_q (x) (J^T)^-1 |J| w),  R_t = R_w + J Phi_f
+//   grad u_q = R_t (Mref^-1 V^T) / |J|,  G_s = R_w (Mref^-1 Vf^T) / |J| + J_s Lf_s / |J| (BR2)
+//   F_q = (F_c - F_v)(u_q, grad u_q) (J^T)^-1 |J| w,  t = (H_c - <F_v> . n) |J_f| w -> slots -t | +t
+//   R = F Phi_grad + H Phi_f
+// Its transpose, given mu (the adjoint of R), runs as four kernels:
+//   nsAdjVolumeMfma  recompute the lifting (and Gf), H-bar = Vf mu, F-bar = Phi_grad mu, the
+//                    pointwise volume-flux VJP -> (u-bar_q, grad-bar_q), R-bar_t(vol) =
+//                    (Mref^-1 V^T)^T grad-bar / |J|, U-bar += V^T u-bar_q
+//   nsAdjFaceKernel  per face point: t-bar = H-bar_R - H-bar_L, face-flux VJP -> trace and
+//                    face-gradient adjoints of both parents' slots
+//   nsAdjLiftMfma    R-bar_w, R-bar_t from the face-gradient adjoint, A-bar, J-bar, Q-bar, and
+//                    their images on u_q and on both parents' traces
+//   nsAdjGatherMfma  U-bar += Vf^T (trace adjoints of own slots and of the neighbours' lifting terms)
+// Pointwise Jacobians come from forward-mode dual numbers (dual.hpp) for the state dependence and
+// from the linearity of the viscous flux in the gradient for the gradient dependence.
+#ifndef SDG_DG_ADJOINT_HPP_
+#define SDG_DG_ADJOINT_HPP_
+
+#include <hip/hip_runtime.h>
+
+#include "dg_mfma.hpp"
+#include "dual.hpp"
+#include "physics.hpp"
+
+namespace sdg {
+
+// transposed operator tables of the adjoint (padded T[r4(K)][r16(M)] like MOps)
+struct AdjOps {
+  const double* gradT;     // K = N_b,   M = N_q D : Phi_grad^T
+  const double* interpT;   // K = N_q,   M = N_b   : V^T
+  const double* pinterpT;  // K = N_q,   M = N_b   : (Mref^-1 V^T)^T
+  const double* ptraceT;   // K = N_aq,  M = N_b   : (Mref^-1 Vf^T)^T
+  const double* liftT;     // K = N_aq,  M = N_aq  : block-diagonal Lf_s^T
+};
+
+struct AdjArgs {
+  AdjOps op;
+  const double* lam;  // E x NV x N_b  adjoint seed
+  int apply_minv;     // 1: lam is the adjoint of M^-1 R (mu = Mref^-1 lam / |J|); 0: of R (mu = lam)
+  double* Hb;         // E x NV x N_aq  mu traced to the face slots
+  double* RtV;        // E x NG x N_b   volume part of the adjoint of R_t
+  double* Ua;         // E x NV x N_b   partial result
+  double* UFf;        // E x NV x N_aq  face-flux adjoint with respect to the own traces
+  double* GFb;        // E x NG x N_aq  adjoint of the lifted gradient traces
+  double* Pown;       // E x NV x N_aq  lifting-face adjoint to the own trace
+  double* Qnb;        // E x NV x N_aq  lifting-face adjoint to the neighbour's trace of the same point
+  double* out;        // E x NV x N_b   (dR/dU)^T mu
+};
+
+// ---- pointwise VJPs --------------------------------------------------------------------------
+// raw volume flux F_c - F_v (D x NV column-major) in the FP64 operation order of the stage kernels
+template <int D, typename T, bool VISC>
+__device__ __forceinline__ void rawVolumeFlux(const Phys& ph, const T* u, const T* g, T* f) {
+  constexpr int NV = D + 2, NG = NV * D;
+  T w[D + 3];
+  computational<D>(ph, u, w);
+  const T rho = w[0], p = w[D + 2];
+  const T et = w[D + 1] + velocitySquared<D>(w) / T(2.0);
+  for (int aa = 0; aa < D; aa++) {
+    f[aa] = rho * w[1 + aa];
+    for (int b = 0; b < D; b++) f[aa + D * (1 + b)] = rho * w[1 + aa] * w[1 + b] + (aa == b ? p * T(1.0) : p * T(0.0));
+    f[aa + D * (D + 1)] = (rho * et + p) * w[1 + aa];
+  }
+  if constexpr (VISC) {
+    T pg[NG], fv[D * NV];
+    primitiveGradient<D>(ph, w, u, g, pg);
+    viscousRawFlux<D>(ph, w, pg, fv);
+    for (int i = 0; i < D * NV; i++) f[i] = f[i] - fv[i];
+  }
+}
+
+// sum_{a,v} Fb[a + D v] * (dF_v[a][v] / dg_j) for every gradient component j (F_v is linear in g)
+template <int D>
+__device__ __forceinline__ void viscousGradientVjp(const Phys& ph, const double* u, const double* Fb, double* gb) {
+  constexpr int NV = D + 2, NG = NV * D;
+  double w[D + 3];
+  computational<D>(ph, u, w);
+  for (int jg = 0; jg < NG; jg++) {
+    double e[NG], pg[NG], fv[D * NV];
+    for (int r = 0; r < NG; r++) e[r] = r == jg ? 1.0 : 0.0;
+    primitiveGradient<D>(ph, w, u, e, pg);
+    viscousRawFlux<D>(ph, w, pg, fv);
+    double s = 0.0;
+    for (int i = 0; i < D * NV; i++) s += Fb[i] * fv[i];
+    gb[jg] = s;
+  }
+}
+
+// volume flux out[b NV + v] = sum_a (F_c - F_v)[a][v](u, g) G[a + D b]: given fbar (same layout),
+// ub = (d out / d u)^T fbar and gb = (d out / d g)^T fbar
+template <int D>
+__device__ __forceinline__ void volumeFluxVjp(const Phys& ph, const double* u, const double* g, const double* G,
+                                              const double* fbar, double* ub, double* gb) {
+  constexpr int NV = D + 2, NG = NV * D;
+  using DT = Dual<NV>;
+  double Fb[D * NV];
+  for (int aa = 0; aa < D; aa++)
+    for (int v = 0; v < NV; v++) {
+      double s = 0.0;
+      for (int b = 0; b < D; b++) s += fbar[b * NV + v] * G[aa + D * b];
+      Fb[aa + D * v] = s;
+    }
+  DT uD[NV], gD[NG], fD[D * NV];
+  for (int k = 0; k < NV; k++) uD[k] = seed<NV>(u[k], k);
+  for (int r = 0; r < NG; r++) gD[r] = DT(g[r]);
+  rawVolumeFlux<D, DT, true>(ph, uD, gD, fD);
+  for (int k = 0; k < NV; k++) {
+    double s = 0.0;
+    for (int i = 0; i < D * NV; i++) s += Fb[i] * fD[i].d[k];
+    ub[k] = s;
+  }
+  viscousGradientVjp<D>(ph, u, Fb, gb);
+  for (int r = 0; r < NG; r++) gb[r] = -gb[r];
+}
+
+// One side of the face-flux VJP: derivatives with respect to the state of side SIDE (0: left,
+// 1: right) by dual numbers, and with respect to that side's gradient by linearity.
+template <int D, int SIDE>
+__device__ __forceinline__ void faceFluxVjpSide(const Phys& ph, const double* n, const double* tb, const double* ul,
+                                                const double* ur, const double* gs, double* ub, double* gb) {
+  constexpr int NV = D + 2, NG = NV * D;
+  using DT = Dual<NV>;
+  DT nD[D], uL[NV], uR[NV], wL[D + 3], wR[D + 3], h[NV];
+  for (int d = 0; d < D; d++) nD[d] = DT(n[d]);
+  for (int k = 0; k < NV; k++) {
+    uL[k] = SIDE == 0 ? seed<NV>(ul[k], k) : DT(ul[k]);
+    uR[k] = SIDE == 1 ? seed<NV>(ur[k], k) : DT(ur[k]);
+  }
+  computational<D>(ph, uL, wL);
+  computational<D>(ph, uR, wR);
+  convectiveFlux<D>(ph, nD, uL, wL, uR, wR, h);
+  DT gD[NG], pg[NG], fv[D * NV];
+  for (int r = 0; r < NG; r++) gD[r] = DT(gs[r]);
+  if (SIDE == 0) {
+    primitiveGradient<D>(ph, wL, uL, gD, pg);
+    viscousRawFlux<D>(ph, wL, pg, fv);
+  } else {
+    primitiveGradient<D>(ph, wR, uR, gD, pg);
+    viscousRawFlux<D>(ph, wR, pg, fv);
+  }
+  for (int k = 0; k < NV; k++) {
+    double s = 0.0;
+    for (int v = 0; v < NV; v++) {
+      double dv = 0.0;
+      for (int d = 0; d < D; d++) dv += fv[d + D * v].d[k] * n[d];
+      s += tb[v] * (h[v].d[k] - dv / 2.0);
+    }
+    ub[k] = s;
+  }
+  // gradient part: -(1/2) sum_{v,a} tb[v] n_a dF_v[a][v]/dg
+  double Fb[D * NV];
+  for (int v = 0; v < NV; v++)
+    for (int d = 0; d < D; d++) Fb[d + D * v] = -0.5 * tb[v] * n[d];
+  viscousGradientVjp<D>(ph, SIDE == 0 ? ul : ur, Fb, gb);
+}
+
+// face flux t = (H_c(ul, ur; n) - (F_v(ul, gl) + F_v(ur, gr)) . n / 2) jw: given tbar, the
+// adjoints of ul, ur, gl, gr
+template <int D>
+__device__ __forceinline__ void faceFluxVjp(const Phys& ph, const double* n, double jw, const double* ul,
+                                            const double* ur, const double* gl, const double* gr, const double* tbar,
+                                            double* ubl, double* ubr, double* gbl, double* gbr) {
+  constexpr int NV = D + 2;
+  double tb[NV];
+  for (int v = 0; v < NV; v++) tb[v] = tbar[v] * jw;
+  faceFluxVjpSide<D, 0>(ph, n, tb, ul, ur, gl, ubl, gbl);
+  faceFluxVjpSide<D, 1>(ph, n, tb, ul, ur, gr, ubr, gbr);
+}
+
+// ---- kernel 1: volume part ----------------------------------------------------------------------
+template <int D, int NB, int NQ, int NF, int NFQ, int VISC>
+__global__ __launch_bounds__(kThreads) void nsAdjVolumeMfma(Geo g, Phys ph, StageArgs a, AdjArgs j) {
+  using LD = GradLds<D, NB, NQ, NF, NFQ>;
+  constexpr int NV = LD::NV, NG = LD::NG, EPB = LD::EPB, NAQ = LD::NAQ;
+  constexpr int NBk = LD::NBk, NBm = LD::NBm, NQm = LD::NQm, NQk = r4(NQ), NAQm = LD::NAQm;
+  constexpr int NBr = LD::NBr, NDQm = r16(NQ * D);
+  __shared__ double sUq[NQm * 16];
+  __shared__ double sAJ[LD::AJ];
+  __shared__ double sRR[2 * NBr * 16];
+  __shared__ double sFb[NDQm * 16];
+  const MOps& op = g.op;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c = lane & 15, rr = lane >> 4;
+  const int e0 = blockIdx.x * EPB;
+  gradLift<double, D, NB, NQ, NF, NFQ, VISC>(g, ph, a, e0, sUq, sAJ, sRR);
+  double* sGq = sAJ;
+  double* sLam = sRR;
+  double* sMu = sRR + NBr * 16;
+  for (int idx = tid; idx < NBr * 16; idx += kThreads) {
+    const int b = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
+    sLam[idx] = (b < NB && cc < EPB * NV && e < g.E) ? j.lam[(static_cast<int64_t>(e) * NV + v) * NB + b] : 0.0;
+  }
+  __syncthreads();
+  const int le_c = c / NV, v_c = c % NV, e_c = e0 + le_c;
+  const bool col_ok = c < EPB * NV && e_c < g.E;
+  // mu = Mref^-1 lam / |J| (adjoint of M^-1 R) or lam
+#pragma unroll 1
+  for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
+    acc_t<double> acc = zeroAcc<double>();
+    if (j.apply_minv) {
+      tileAcc<double, NBm>(acc, op.minv, m0, sLam, 0, NBk, lane);
+      const double idj = col_ok ? 1.0 / g.detj[e_c] : 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; i++) acc[i] *= idj;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; i++) acc[i] = sLam[(m0 + Mfma<double>::row(rr, i)) * 16 + c];
+    }
+    tileStore<double>(acc, sMu, m0, lane);
+  }
+  __syncthreads();
+  // H-bar = Vf mu at the face slots -> HBM
+#pragma unroll 1
+  for (int m0 = wave * 16; m0 < NAQm; m0 += 64) {
+    acc_t<double> acc = zeroAcc<double>();
+    tileAcc<double, NAQm>(acc, op.trace, m0, sMu, 0, NBk, lane);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int sl = m0 + Mfma<double>::row(rr, i);
+      if (col_ok && sl < NAQ) j.Hb[(static_cast<int64_t>(e_c) * NV + v_c) * NAQ + sl] = acc[i];
+    }
+  }
+  // F-bar = Phi_grad mu at the (point, direction) rows
+#pragma unroll 1
+  for (int m0 = wave * 16; m0 < NDQm; m0 += 64) {
+    acc_t<double> acc = zeroAcc<double>();
+    tileAcc<double, NDQm>(acc, j.op.gradT, m0, sMu, 0, NBk, lane);
+    tileStore<double>(acc, sFb, m0, lane);
+  }
+  __syncthreads();
+  // pointwise VJP of the volume flux; u-bar_q and grad-bar_q overwrite u_q and grad u_q in place
+  for (int idx = tid; idx < EPB * NQ; idx += kThreads) {
+    const int le = idx / NQ, q = idx % NQ, e = e0 + le;
+    if (e >= g.E) continue;
+    double u[NV], gq[NG], G[D * D], fb[D * NV], ub[NV], gb[NG];
+    for (int v = 0; v < NV; v++) u[v] = sUq[q * 16 + le * NV + v];
+    for (int r = 0; r < NG; r++) gq[r] = sGq[q * 16 + le * NG + r];
+    const double jw = g.detj[e] * g.wq[q];
+    for (int i = 0; i < D * D; i++) G[i] = g.ginv[static_cast<int64_t>(e) * D * D + i] * jw;
+    for (int b = 0; b < D; b++)
+      for (int v = 0; v < NV; v++) fb[b * NV + v] = sFb[(q * D + b) * 16 + le * NV + v];
+    volumeFluxVjp<D>(ph, u, gq, G, fb, ub, gb);
+    for (int v = 0; v < NV; v++) sUq[q * 16 + le * NV + v] = ub[v];
+    for (int r = 0; r < NG; r++) sGq[q * 16 + le * NG + r] = gb[r];
+  }
+  __syncthreads();
+  // R-bar_t (volume part) = (Mref^-1 V^T)^T grad-bar / |J| -> HBM
+  const int gle = c / NG, ge = e0 + gle;
+  const bool gok = c < EPB * NG && ge < g.E;
+  const double idj = gok ? 1.0 / g.detj[ge] : 0.0;
+#pragma unroll 1
+  for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
+    acc_t<double> acc = zeroAcc<double>();
+    tileAcc<double, NBm>(acc, j.op.pinterpT, m0, sGq, 0, NQk, lane);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int b = m0 + Mfma<double>::row(rr, i);
+      if (gok && b < NB) j.RtV[(static_cast<int64_t>(ge) * NG + c % NG) * NB + b] = acc[i] * idj;
+    }
+  }
+  // U-bar (partial) = V^T u-bar_q -> HBM
+#pragma unroll 1
+  for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
+    acc_t<double> acc = zeroAcc<double>();
+    tileAcc<double, NBm>(acc, j.op.interpT, m0, sUq, 0, NQk, lane);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int b = m0 + Mfma<double>::row(rr, i);
+      if (col_ok && b < NB) j.Ua[(static_cast<int64_t>(e_c) * NV + v_c) * NB + b] = acc[i];
+    }
+  }
+}
+
+// ---- kernel 2: face points --------------------------------------------------------------------
+template <int D, int NF, int NFQ>
+__global__ __launch_bounds__(kThreads) void nsAdjFaceKernel(Geo g, Phys ph, StageArgs a, AdjArgs j) {
+  constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (idx >= static_cast<int64_t>(g.F) * NFQ) return;
+  const int64_t f = idx / NFQ;
+  const int jq = static_cast<int>(idx % NFQ);
+  const int k = g.fseq[f * NFQ + jq];
+  const int64_t L = g.fpar[2 * f], R = g.fpar[2 * f + 1];
+  const int slL = g.floc[2 * f] * NFQ + jq, slR = g.floc[2 * f + 1] * NFQ + k;
+  double ul[NV], ur[NV], gl[NG], gr[NG], tb[NV], n[D];
+  for (int v = 0; v < NV; v++) {
+    ul[v] = a.Uf_in[(L * NV + v) * NAQ + slL];
+    ur[v] = a.Uf_in[(R * NV + v) * NAQ + slR];
+    tb[v] = j.Hb[(R * NV + v) * NAQ + slR] - j.Hb[(L * NV + v) * NAQ + slL];
+  }
+  for (int r = 0; r < NG; r++) {
+    gl[r] = a.Gf[(L * NG + r) * NAQ + slL];
+    gr[r] = a.Gf[(R * NG + r) * NAQ + slR];
+  }
+  for (int d = 0; d < D; d++) n[d] = g.fnormal[(f * NFQ + jq) * D + d];
+  double ubl[NV], ubr[NV], gbl[NG], gbr[NG];
+  faceFluxVjp<D>(ph, n, g.fjw[f * NFQ + jq], ul, ur, gl, gr, tb, ubl, ubr, gbl, gbr);
+  for (int v = 0; v < NV; v++) {
+    j.UFf[(L * NV + v) * NAQ + slL] = ubl[v];
+    j.UFf[(R * NV + v) * NAQ + slR] = ubr[v];
+  }
+  for (int r = 0; r < NG; r++) {
+    j.GFb[(L * NG + r) * NAQ + slL] = gbl[r];
+    j.GFb[(R * NG + r) * NAQ + slR] = gbr[r];
+  }
+}
+
+// ---- kernel 3: lifting terms --------------------------------------------------------------------
+template <int D, int NB, int NQ, int NF, int NFQ, int VISC>
+__global__ __launch_bounds__(kThreads) void nsAdjLiftMfma(Geo g, Phys ph, StageArgs a, AdjArgs j) {
+  using LD = GradLds<D, NB, NQ, NF, NFQ>;
+  constexpr int NV = LD::NV, NG = LD::NG, EPB = LD::EPB, NAQ = LD::NAQ;
+  constexpr int NBk = LD::NBk, NBm = LD::NBm, NQm = LD::NQm, NQk = r4(NQ), NAQk = LD::NAQk, NAQm = LD::NAQm;
+  constexpr int NBr = LD::NBr, NDQm = r16(NQ * D);
+  __shared__ double sRt[NBr * 16];
+  __shared__ double sRw[NBr * 16];
+  __shared__ double sGb[NAQm * 16];
+  __shared__ double sAb[NAQm * 16];
+  __shared__ double sJb[NAQm * 16];
+  __shared__ double sQb[NDQm * 16];
+  __shared__ double sUQ[NQm * 16];
+  const MOps& op = g.op;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c = lane & 15, rr = lane >> 4;
+  const int e0 = blockIdx.x * EPB;
+  zeroLds(sUQ, NQm * 16, tid);
+  for (int idx = tid; idx < NBr * 16; idx += kThreads) {
+    const int b = idx / 16, cc = idx % 16, le = cc / NG, r = cc % NG, e = e0 + le;
+    sRt[idx] = (b < NB && cc < EPB * NG && e < g.E) ? j.RtV[(static_cast<int64_t>(e) * NG + r) * NB + b] : 0.0;
+  }
+  for (int idx = tid; idx < NAQm * 16; idx += kThreads) {
+    const int sl = idx / 16, cc = idx % 16, le = cc / NG, r = cc % NG, e = e0 + le;
+    sGb[idx] = (sl < NAQ && cc < EPB * NG && e < g.E) ? j.GFb[(static_cast<int64_t>(e) * NG + r) * NAQ + sl] : 0.0;
+  }
+  __syncthreads();
+  const int gle = c / NG, ge = e0 + gle;
+  const bool gok = c < EPB * NG && ge < g.E;
+  const double idj = gok ? 1.0 / g.detj[ge] : 0.0;
+  // face-gradient adjoint -> R-bar_w (BR2) or R-bar_t (BR1); each lane updates its own entries
+#pragma unroll 1
+  for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
+    acc_t<double> acc = zeroAcc<double>();
+    tileAcc<double, NBm>(acc, j.op.ptraceT, m0, sGb, 0, NAQk, lane);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int b = m0 + Mfma<double>::row(rr, i);
+      const double t = sRt[b * 16 + c];
+      if (VISC == 2) {
+        sRw[b * 16 + c] = t + acc[i] * idj;
+      } else {
+        sRt[b * 16 + c] = t + acc[i] * idj;
+        sRw[b * 16 + c] = t + acc[i] * idj;
+      }
+    }
+  }
+  __syncthreads();
+  // J-bar = Vf R-bar_t (+ Lf^T G-bar / |J|), A-bar = Vf R-bar_w
+#pragma unroll 1
+  for (int m0 = wave * 16; m0 < NAQm; m0 += 64) {
+    acc_t<double> aj = zeroAcc<double>(), aa = zeroAcc<double>();
+    tileAcc<double, NAQm>(aj, op.trace, m0, sRt, 0, NBk, lane);
+    if (VISC == 2) {
+      acc_t<double> al = zeroAcc<double>();
+      const int s_lo = m0 / NFQ;
+      const int s_hi = (m0 + 15 < NAQ ? m0 + 15 : NAQ - 1) / NFQ;
+      const int k0 = (s_lo * NFQ) / 4 * 4;
+      const int k1 = r4((s_hi + 1) * NFQ);
+      tileAcc<double, NAQm>(al, j.op.liftT, m0, sGb, k0, k1 < NAQk ? k1 : NAQk, lane);
+#pragma unroll
+      for (int i = 0; i < 4; i++) aj[i] += al[i] * idj;
+    }
+    tileStore<double>(aj, sJb, m0, lane);
+    tileAcc<double, NAQm>(aa, op.trace, m0, sRw, 0, NBk, lane);
+    tileStore<double>(aa, sAb, m0, lane);
+  }
+  // Q-bar = -Phi_grad R-bar_w
+#pragma unroll 1
+  for (int m0 = wave * 16; m0 < NDQm; m0 += 64) {
+    acc_t<double> acc = zeroAcc<double>();
+    tileAcc<double, NDQm>(acc, j.op.gradT, m0, sRw, 0, NBk, lane);
+#pragma unroll
+    for (int i = 0; i < 4; i++) acc[i] = -acc[i];
+    tileStore<double>(acc, sQb, m0, lane);
+  }
+  __syncthreads();
+  // Q = u_q (x) (J^T)^-1 |J| w: u-bar_q[k] = sum_{a,b} Q-bar[(q,b)][(k,a)] G[a][b]
+  for (int idx = tid; idx < EPB * NQ * NV; idx += kThreads) {
+    const int le = idx / (NQ * NV), rem = idx % (NQ * NV), q = rem / NV, kv = rem % NV, e = e0 + le;
+    if (e >= g.E) continue;
+    const double jw = g.detj[e] * g.wq[q];
+    double s = 0.0;
+    for (int b = 0; b < D; b++)
+      for (int aa = 0; aa < D; aa++)
+        s += sQb[(q * D + b) * 16 + le * NG + kv * D + aa] * (g.ginv[static_cast<int64_t>(e) * D * D + aa + D * b] * jw);
+    sUQ[q * 16 + le * NV + kv] = s;
+  }
+  // A and J face terms -> adjoints of the own trace (Pown) and of the neighbour's trace (Qnb)
+  for (int idx = tid; idx < EPB * NAQ * NV; idx += kThreads) {
+    const int le = idx / (NAQ * NV), rem = idx % (NAQ * NV), sl = rem / NV, kv = rem % NV, e = e0 + le;
+    if (e >= g.E) continue;
+    const int64_t es = static_cast<int64_t>(e) * NAQ + sl;
+    const int fps = g.sfp[es];
+    const int side = fps & 1;
+    const int64_t fp = fps >> 1;
+    const double jw = g.fjw[fp];
+    double ca = 0.0, cj = 0.0;
+    for (int d = 0; d < D; d++) {
+      const double n = g.fnormal[fp * D + d];
+      ca += sAb[sl * 16 + le * NG + kv * D + d] * n;
+      cj += sJb[sl * 16 + le * NG + kv * D + d] * n;
+    }
+    ca = (side == 0 ? ca : -ca) * jw / 2.0;
+    cj = cj * jw / 2.0;
+    const double sg = side == 0 ? -1.0 : 1.0;
+    const int64_t o = (static_cast<int64_t>(e) * NV + kv) * NAQ + sl;
+    j.Pown[o] = ca + sg * cj;
+    j.Qnb[o] = ca - sg * cj;
+  }
+  __syncthreads();
+  // U-bar (partial) += V^T u-bar_q
+  const int le_c = c / NV, v_c = c % NV, e_c = e0 + le_c;
+  const bool col_ok = c < EPB * NV && e_c < g.E;
+#pragma unroll 1
+  for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
+    acc_t<double> acc = zeroAcc<double>();
+    tileAcc<double, NBm>(acc, j.op.interpT, m0, sUQ, 0, NQk, lane);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int b = m0 + Mfma<double>::row(rr, i);
+      if (col_ok && b < NB) j.Ua[(static_cast<int64_t>(e_c) * NV + v_c) * NB + b] += acc[i];
+    }
+  }
+}
+
+// ---- kernel 4: gather the trace adjoints and apply Vf^T ---------------------------------------
+template <int D, int NB, int NQ, int NF, int NFQ>
+__global__ __launch_bounds__(kThreads) void nsAdjGatherMfma(Geo g, Phys ph, StageArgs a, AdjArgs j) {
+  constexpr int NV = D + 2, NAQ = NF * NFQ, EPB = 16 / NV;
+  constexpr int NBm = r16(NB), NAQk = r4(NAQ);
+  __shared__ double sUf[NAQk * 16];
+  const MOps& op = g.op;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c = lane & 15, rr = lane >> 4;
+  const int e0 = blockIdx.x * EPB;
+  for (int idx = tid; idx < NAQk * 16; idx += kThreads) {
+    const int sl = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
+    double x = 0.0;
+    if (sl < NAQ && cc < EPB * NV && e < g.E) {
+      const int64_t es = static_cast<int64_t>(e) * NAQ + sl;
+      const int nbs = g.snb[es];
+      const int64_t o = (static_cast<int64_t>(e) * NV + v) * NAQ + sl;
+      x = j.UFf[o] + j.Pown[o] + j.Qnb[(static_cast<int64_t>(nbs / NAQ) * NV + v) * NAQ + nbs % NAQ];
+    }
+    sUf[idx] = x;
+  }
+  __syncthreads();
+  const int le_c = c / NV, v_c = c % NV, e_c = e0 + le_c;
+  const bool col_ok = c < EPB * NV && e_c < g.E;
+#pragma unroll 1
+  for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
+    acc_t<double> acc = zeroAcc<double>();
+    tileAcc<double, NBm>(acc, op.face, m0, sUf, 0, NAQk, lane);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int b = m0 + Mfma<double>::row(rr, i);
+      if (col_ok && b < NB) {
+        const int64_t o = (static_cast<int64_t>(e_c) * NV + v_c) * NB + b;
+        j.out[o] = j.Ua[o] + acc[i];
+      }
+    }
+  }
+}
+
+// y = alpha x + beta y (n doubles)
+__global__ __launch_bounds__(kThreads) void axpbyKernel(double* y, const double* x, double alpha, double beta, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < n) y[i] = alpha * x[i] + beta * y[i];
+}
+
+}  // namespace sdg
+
+#endif  // SDG_DG_ADJOINT_HPP_

@@ -29,16 +29,18 @@ constexpr int r4(int x) { return (x + 3) / 4 * 4; }
 
 // Padded, transposed operator tables T[K4][M16] for the matrix-core kernels (zero outside the
 // operator): Out[m][n] = sum_k T[k][m] X[k][n].
-struct MOps {
-  const double* interp;  // K = N_b,   M = N_q   : basis values          (U -> U_q, w -> grad U_q, R -> R_q)
-  const double* grad;    // K = N_q D, M = N_b   : basis gradients       (weak divergence, volume gradient term)
-  const double* face;    // K = N_aq,  M = N_b   : face values           (face terms of residuals)
-  const double* minv;    // K = N_b,   M = N_b   : reference mass inverse
-  const double* trace;   // K = N_b,   M = N_aq  : face values           (coefficients -> face traces)
-  const double* lift;    // K = N_aq,  M = N_aq  : block-diagonal Vf_s Mref^-1 Vf_s^T (BR2 face lift)
-  const double* pinterp; // K = N_b,   M = N_q   : Mref^-1 V^T   (R -> gradient at volume points)
-  const double* ptrace;  // K = N_b,   M = N_aq  : Mref^-1 Vf^T  (R -> gradient traces at face points)
+template <typename TC>
+struct MOpsT {
+  const TC* interp;  // K = N_b,   M = N_q   : basis values          (U -> U_q, w -> grad U_q, R -> R_q)
+  const TC* grad;    // K = N_q D, M = N_b   : basis gradients       (weak divergence, volume gradient term)
+  const TC* face;    // K = N_aq,  M = N_b   : face values           (face terms of residuals)
+  const TC* minv;    // K = N_b,   M = N_b   : reference mass inverse
+  const TC* trace;   // K = N_b,   M = N_aq  : face values           (coefficients -> face traces)
+  const TC* lift;    // K = N_aq,  M = N_aq  : block-diagonal Vf_s Mref^-1 Vf_s^T (BR2 face lift)
+  const TC* pinterp; // K = N_b,   M = N_q   : Mref^-1 V^T   (R -> gradient at volume points)
+  const TC* ptrace;  // K = N_b,   M = N_aq  : Mref^-1 Vf^T  (R -> gradient traces at face points)
 };
+using MOps = MOpsT<double>;
 
 struct Geo {
   const double* Vt;    // N_b x N_q        basis at volume points, transposed
@@ -54,17 +56,46 @@ struct Geo {
   const double* fjw;      // F x N_fq       |J_f| w
   const int* fseq;        // F x N_fq       right point of left point j
   const int* fseqinv;     // F x N_fq       left point of right point k
-  const int* efid;        // E x N_f        face id
-  const int* enb;         // E x N_f        neighbour element
-  const int* enbf;        // E x N_f        neighbour's local face
-  const int* eside;       // E x N_f        0: this element is the face's left parent
   const int* snb;         // E x N_aq       neighbour slot: neighbour element * N_aq + its slot
   const int* sfp;         // E x N_aq       2 (f N_fq + p) + side: face point of the slot and side
   const int* fpar;        // F x 2          left, right parent element
   const int* floc;        // F x 2          local face index in left, right parent
+  // boundary face points bp = b N_fq + p (a slot on a boundary face has snb = -1 - bp)
+  const double* bnormal;  // F_b N_fq x D    outward normal
+  const double* bjw;      // F_b N_fq        |J_f| w
+  const int* btype;       // F_b N_fq        sdg_boundary type
+  const int* bslot;       // F_b N_fq        own slot: element * N_aq + local slot
+  const double* bdata;    // F_b N_fq x (D+3) boundary data, computational variables
   int E;
   int F;
-  MOps op;
+  int Fb;
+  MOps op;                // FP64 operator tables
+  MOpsT<float> opf;       // the same tables rounded to FP32 (mixed-precision mode)
+};
+
+template <typename TC>
+__device__ __forceinline__ const MOpsT<TC>& opsOf(const Geo& g) {
+  if constexpr (sizeof(TC) == 8)
+    return g.op;
+  else
+    return g.opf;
+}
+
+// Mixed-precision mode: the contractions work on the deviation dU = U - uref[v] one[b] from a
+// constant reference state (one[] = modal coefficients of the constant function 1), which the DG
+// operators annihilate exactly (the lifted gradient of a constant is zero, and the weak divergence
+// of a constant flux is zero on affine elements with exact quadrature). For the same reason the
+// reference pressure pref is taken out of the momentum fluxes, and the energy fluxes carry
+// F_E - h0 F_rho with the reference total enthalpy h0: the residual is linear in the fluxes, so
+// R_E = R'_E + h0 R_rho is restored in FP64 after the contractions. Without this, FP32 would round
+// the O(p0) = O(1/Ma^2) parts of the momentum and energy fluxes, whose divergence nearly cancels
+// at low Mach number. Unused (on = 0) in FP64 mode.
+struct MixRef {
+  double uref[5];
+  double pref;
+  double h0;
+  const double* one;
+  int on;
 };
 
 struct StageArgs {
@@ -78,12 +109,97 @@ struct StageArgs {
   double* Wout;   // lifted total gradient output (optional)
   double* Hs;     // E x NV x N_aq signed face-flux slots (Navier-Stokes, matrix-core path)
   double* Fv;     // E x (N_q D) x NV contravariant volume flux (Navier-Stokes, matrix-core path)
+  // FP32 versions of the work buffers above (mixed-precision mode; traces hold the deviation)
+  const float* Uf_in32;
+  float* Uf_out32;
+  float* Gf32;
+  float* Hs32;
+  float* Fv32;
   double* mon;    // E x NV monitor partials (last stage)
   int* status;    // [0] NaN stop flag, [1] steps taken; nullptr: no check
   double c0, c1, c2dt;  // U <- c1 U + c0 Un + c2dt R Minv (TimeIntegration.cpp:181-198)
   int last_stage;
   int mode;  // 0: stage update, 1: residual only
+  MixRef mix;
 };
+
+// the work buffer of contraction type TC
+template <typename TC>
+__device__ __forceinline__ TC* pick(double* d, float* f) {
+  if constexpr (sizeof(TC) == 8)
+    return d;
+  else
+    return f;
+}
+template <typename TC>
+__device__ __forceinline__ const TC* pick(const double* d, const float* f) {
+  if constexpr (sizeof(TC) == 8)
+    return d;
+  else
+    return f;
+}
+
+// Navier-Stokes / Euler flux t at boundary point bp from the interior trace ul (conserved) and, for
+// Navier-Stokes, its lifted gradient gl: calculateBoundaryAdjacencyElementQuadrature
+// (SpatialDiscrete.cpp:750-842): the physical flux of the boundary state (no Riemann solver), and
+// the central viscous flux of the (wall-modified) interior state and the boundary state, whose
+// primitive gradient is the interior one with zero temperature gradient on adiabatic walls. With
+// mixed = true the numerical flux is reduced as in the mixed-precision interior faces.
+template <int D, bool VISC>
+__device__ __forceinline__ void boundaryFlux(const Phys& ph, const Geo& g, int bp, const double* ul, const double* gl,
+                                             bool mixed, double pref, double h0, double* t) {
+  constexpr int NV = D + 2, NG = NV * D;
+  const int bc = g.btype[bp];
+  double n[D], wl[D + 3], wb[D + 3], wd[D + 3], h[NV];
+  for (int d = 0; d < D; d++) n[d] = g.bnormal[static_cast<int64_t>(bp) * D + d];
+  for (int i = 0; i < D + 3; i++) wd[i] = g.bdata[static_cast<int64_t>(bp) * (D + 3) + i];
+  computational<D>(ph, ul, wl);
+  boundaryState<D>(ph, bc, n, wl, wd, wb);
+  normalFlux<D>(n, wb, h);
+  if (mixed) {
+    for (int d = 0; d < D; d++) h[1 + d] -= pref * n[d];
+    h[D + 1] -= h0 * h[0];
+  }
+  const double jw = g.bjw[bp];
+  if constexpr (VISC) {
+    double pgl[NG], pgb[NG], wm[D + 3], fvl[D * NV], fvb[D * NV];
+    primitiveGradient<D>(ph, wl, ul, gl, pgl);
+    for (int r = 0; r < NG; r++) pgb[r] = pgl[r];
+    if (bc == 4 || bc == 5)
+      for (int aa = 0; aa < D; aa++) pgb[(D + 1) * D + aa] = 0.0;
+    for (int i = 0; i < D + 3; i++) wm[i] = bc >= 3 ? wb[i] : wl[i];  // modifyBoundaryVariable
+    viscousRawFlux<D>(ph, wm, pgl, fvl);
+    viscousRawFlux<D>(ph, wb, pgb, fvb);
+    for (int v = 0; v < NV; v++) {
+      double vl = 0.0, vr = 0.0;
+      for (int d = 0; d < D; d++) {
+        vl += fvl[d + D * v] * n[d];
+        vr += fvb[d + D * v] * n[d];
+      }
+      t[v] = (h[v] - (vl + vr) / 2.0) * jw;
+    }
+  } else {
+    for (int v = 0; v < NV; v++) t[v] = h[v] * jw;
+  }
+}
+
+// Gradient-pass face terms at boundary point bp (calculateBoundaryAdjacencyElementGardientQuadrature,
+// SpatialDiscrete.cpp:913-968): A = n (x) (u_vg - shift) |J_f| w, J = n (x) u_ig |J_f| w, component v
+// (shift: the reference state of the mixed-precision deviation form, else 0).
+template <int D>
+__device__ __forceinline__ void boundaryGradientTerms(const Phys& ph, const Geo& g, int bp, const double* ul, int v,
+                                                      double shift, double* A, double* J) {
+  constexpr int NV = D + 2;
+  double n[D], wd[D + 3], uvg[NV], uig[NV];
+  for (int d = 0; d < D; d++) n[d] = g.bnormal[static_cast<int64_t>(bp) * D + d];
+  for (int i = 0; i < D + 3; i++) wd[i] = g.bdata[static_cast<int64_t>(bp) * (D + 3) + i];
+  boundaryGradientStates<D>(ph, g.btype[bp], n, ul, wd, uvg, uig);
+  const double jw = g.bjw[bp];
+  for (int d = 0; d < D; d++) {
+    A[d] = (n[d] * (uvg[v] - shift)) * jw;
+    J[d] = (n[d] * uig[v]) * jw;
+  }
+}
 
 template <int D, int NB, int NQ, int NF, int NFQ>
 struct Sizes {
@@ -95,27 +211,6 @@ struct Sizes {
   static constexpr int EPB_RAW = kThreads / (NV * MAXN);
   static constexpr int EPB = EPB_RAW < 1 ? 1 : (EPB_RAW > 16 ? 16 : EPB_RAW);
 };
-
-// Face point roles of slot (s, j) of element e: face id, face point p (left ordering), neighbour
-// slot, and side (0 left, 1 right).
-template <int NF, int NFQ>
-__device__ __forceinline__ void slotRoles(const Geo& g, int e, int s, int j, int& f, int& p, int64_t& nbslot,
-                                          int& side) {
-  constexpr int NAQ = NF * NFQ;
-  f = g.efid[e * NF + s];
-  side = g.eside[e * NF + s];
-  const int nb = g.enb[e * NF + s];
-  const int nbs = g.enbf[e * NF + s];
-  int k;
-  if (side == 0) {
-    p = j;
-    k = g.fseq[f * NFQ + j];
-  } else {
-    p = g.fseqinv[f * NFQ + j];
-    k = p;
-  }
-  nbslot = static_cast<int64_t>(nb) * NAQ + nbs * NFQ + k;
-}
 
 // ---- monitor, update and next-stage traces (shared tail of the last kernel of a stage) ----
 template <int D, int NB, int NQ, int NF, int NFQ, int EPB>
@@ -238,25 +333,34 @@ __global__ __launch_bounds__(kThreads) void eulerStageKernel(Geo g, Phys ph, Sta
     const int le = idx / NAQ, sl = idx % NAQ;
     const int e = e0 + le;
     if (e >= g.E) continue;
-    const int s = sl / NFQ, j = sl % NFQ;
-    int f, p, side;
-    int64_t nbslot;
-    slotRoles<NF, NFQ>(g, e, s, j, f, p, nbslot, side);
-    double uo[NV], un[NV];
     const int64_t own = static_cast<int64_t>(e) * NV * NAQ + sl;
-    const int64_t nbe = (nbslot / NAQ) * NV * NAQ + nbslot % NAQ;
-    for (int v = 0; v < NV; v++) {
-      uo[v] = a.Uf_in[own + v * NAQ];
-      un[v] = a.Uf_in[nbe + v * NAQ];
+    const int64_t es = static_cast<int64_t>(e) * NAQ + sl;
+    const int nbs = g.snb[es];
+    if (nbs < 0) {  // boundary face point
+      double uo[NV], t[NV];
+      for (int v = 0; v < NV; v++) uo[v] = a.Uf_in[own + v * NAQ];
+      boundaryFlux<D, false>(ph, g, -1 - nbs, uo, nullptr, false, 0.0, 0.0, t);
+      for (int v = 0; v < NV; v++) sH[le][v][sl] = t[v];
+      continue;
     }
-    const double* ul = side == 0 ? uo : un;
-    const double* ur = side == 0 ? un : uo;
+    const int fps = g.sfp[es];
+    const int side = fps & 1;
+    const int64_t fp = fps >> 1;
+    // select the left/right index rather than a pointer to a private array (which would put the
+    // arrays in scratch memory)
+    const int64_t nbe = static_cast<int64_t>(nbs / NAQ) * NV * NAQ + nbs % NAQ;
+    const int64_t li = side == 0 ? own : nbe, ri = side == 0 ? nbe : own;
+    double ul[NV], ur[NV];
+    for (int v = 0; v < NV; v++) {
+      ul[v] = a.Uf_in[li + v * NAQ];
+      ur[v] = a.Uf_in[ri + v * NAQ];
+    }
     double wl[D + 3], wr[D + 3], n[D], h[NV];
     computational<D>(ph, ul, wl);
     computational<D>(ph, ur, wr);
-    for (int d = 0; d < D; d++) n[d] = g.fnormal[(static_cast<int64_t>(f) * NFQ + p) * D + d];
+    for (int d = 0; d < D; d++) n[d] = g.fnormal[fp * D + d];
     convectiveFlux<D>(ph, n, ul, wl, ur, wr, h);
-    const double jw = g.fjw[static_cast<int64_t>(f) * NFQ + p];
+    const double jw = g.fjw[fp];
     for (int v = 0; v < NV; v++) {
       const double t = h[v] * jw;
       sH[le][v][sl] = side == 0 ? t : -t;
@@ -316,15 +420,29 @@ __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageAr
       for (int r = 0; r < NG; r++) sA[le][r][sl] = sJ[le][r][sl] = 0.0;
       continue;
     }
-    const int s = sl / NFQ, j = sl % NFQ;
-    int f, p, side;
-    int64_t nbslot;
-    slotRoles<NF, NFQ>(g, e, s, j, f, p, nbslot, side);
     const int64_t own = static_cast<int64_t>(e) * NV * NAQ + sl;
-    const int64_t nbe = (nbslot / NAQ) * NV * NAQ + nbslot % NAQ;
+    const int64_t es = static_cast<int64_t>(e) * NAQ + sl;
+    const int nbs = g.snb[es];
+    if (nbs < 0) {  // boundary face point
+      double uo[NV];
+      for (int v = 0; v < NV; v++) uo[v] = a.Uf_in[own + v * NAQ];
+      for (int v = 0; v < NV; v++) {
+        double Ab[D], Jb[D];
+        boundaryGradientTerms<D>(ph, g, -1 - nbs, uo, v, 0.0, Ab, Jb);
+        for (int d = 0; d < D; d++) {
+          sA[le][v * D + d][sl] = Ab[d];
+          sJ[le][v * D + d][sl] = Jb[d];
+        }
+      }
+      continue;
+    }
+    const int fps = g.sfp[es];
+    const int side = fps & 1;
+    const int64_t fp = fps >> 1;
+    const int64_t nbe = static_cast<int64_t>(nbs / NAQ) * NV * NAQ + nbs % NAQ;
     double n[D];
-    for (int d = 0; d < D; d++) n[d] = g.fnormal[(static_cast<int64_t>(f) * NFQ + p) * D + d];
-    const double jw = g.fjw[static_cast<int64_t>(f) * NFQ + p];
+    for (int d = 0; d < D; d++) n[d] = g.fnormal[fp * D + d];
+    const double jw = g.fjw[fp];
     for (int v = 0; v < NV; v++) {
       const double uo = a.Uf_in[own + v * NAQ], un = a.Uf_in[nbe + v * NAQ];
       const double ul = side == 0 ? uo : un, ur = side == 0 ? un : uo;
@@ -480,34 +598,41 @@ __global__ __launch_bounds__(kThreads) void nsFluxKernel(Geo g, Phys ph, StageAr
       for (int v = 0; v < NV; v++) sH[le][v][sl] = 0.0;
       continue;
     }
-    const int s = sl / NFQ, j = sl % NFQ;
-    int f, p, side;
-    int64_t nbslot;
-    slotRoles<NF, NFQ>(g, e, s, j, f, p, nbslot, side);
-    const int64_t nbel = nbslot / NAQ, nbsl = nbslot % NAQ;
-    double uo[NV], un[NV], go[NG], gn[NG];
+    const int64_t es = static_cast<int64_t>(e) * NAQ + sl;
+    const int nbs = g.snb[es];
+    if (nbs < 0) {  // boundary face point
+      double uo[NV], go[NG], t[NV];
+      for (int v = 0; v < NV; v++) uo[v] = a.Uf_in[static_cast<int64_t>(e) * NV * NAQ + v * NAQ + sl];
+      for (int r = 0; r < NG; r++) go[r] = a.Gf[static_cast<int64_t>(e) * NG * NAQ + r * NAQ + sl];
+      boundaryFlux<D, true>(ph, g, -1 - nbs, uo, go, false, 0.0, 0.0, t);
+      for (int v = 0; v < NV; v++) sH[le][v][sl] = t[v];
+      continue;
+    }
+    const int fps = g.sfp[es];
+    const int side = fps & 1;
+    const int64_t fp = fps >> 1;
+    const int64_t nbel = nbs / NAQ, nbsl = nbs % NAQ;
+    const int64_t lo = side == 0 ? static_cast<int64_t>(e) : nbel, ro = side == 0 ? nbel : static_cast<int64_t>(e);
+    const int lsl = side == 0 ? sl : nbsl, rsl = side == 0 ? nbsl : sl;
+    double ul[NV], ur[NV], gl[NG], gr[NG];
     for (int v = 0; v < NV; v++) {
-      uo[v] = a.Uf_in[static_cast<int64_t>(e) * NV * NAQ + v * NAQ + sl];
-      un[v] = a.Uf_in[nbel * NV * NAQ + v * NAQ + nbsl];
+      ul[v] = a.Uf_in[lo * NV * NAQ + v * NAQ + lsl];
+      ur[v] = a.Uf_in[ro * NV * NAQ + v * NAQ + rsl];
     }
     for (int r = 0; r < NG; r++) {
-      go[r] = a.Gf[static_cast<int64_t>(e) * NG * NAQ + r * NAQ + sl];
-      gn[r] = a.Gf[nbel * NG * NAQ + r * NAQ + nbsl];
+      gl[r] = a.Gf[lo * NG * NAQ + r * NAQ + lsl];
+      gr[r] = a.Gf[ro * NG * NAQ + r * NAQ + rsl];
     }
-    const double* ul = side == 0 ? uo : un;
-    const double* ur = side == 0 ? un : uo;
-    const double* gl = side == 0 ? go : gn;
-    const double* gr = side == 0 ? gn : go;
     double wl[D + 3], wr[D + 3], n[D], h[NV], pgl[NG], pgr[NG], fvl[D * NV], fvr[D * NV];
     computational<D>(ph, ul, wl);
     computational<D>(ph, ur, wr);
-    for (int d = 0; d < D; d++) n[d] = g.fnormal[(static_cast<int64_t>(f) * NFQ + p) * D + d];
+    for (int d = 0; d < D; d++) n[d] = g.fnormal[fp * D + d];
     convectiveFlux<D>(ph, n, ul, wl, ur, wr, h);
     primitiveGradient<D>(ph, wl, ul, gl, pgl);
     primitiveGradient<D>(ph, wr, ur, gr, pgr);
     viscousRawFlux<D>(ph, wl, pgl, fvl);
     viscousRawFlux<D>(ph, wr, pgr, fvr);
-    const double jw = g.fjw[static_cast<int64_t>(f) * NFQ + p];
+    const double jw = g.fjw[fp];
     for (int v = 0; v < NV; v++) {
       double vl = 0.0, vr = 0.0;
       for (int d = 0; d < D; d++) {
@@ -534,7 +659,7 @@ __global__ __launch_bounds__(kThreads) void nsFluxKernel(Geo g, Phys ph, StageAr
 
 // ---- auxiliary kernels ---------------------------------------------------------------------
 template <int D, int NB, int NQ, int NF, int NFQ>
-__global__ __launch_bounds__(kThreads) void traceKernel(Geo g, const double* U, double* Uf) {
+__global__ __launch_bounds__(kThreads) void traceKernel(Geo g, const double* U, void* Uf_out, MixRef mix) {
   constexpr int NV = D + 2, NAQ = NF * NFQ;
   const int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (idx >= static_cast<int64_t>(g.E) * NV * NAQ) return;
@@ -542,8 +667,14 @@ __global__ __launch_bounds__(kThreads) void traceKernel(Geo g, const double* U, 
   const int rem = static_cast<int>(idx % (NV * NAQ));
   const int v = rem / NAQ, sl = rem % NAQ;
   double t = 0.0;
-  for (int b = 0; b < NB; b++) t += U[e * NV * NB + v * NB + b] * g.Vft[b * NAQ + sl];
-  Uf[idx] = t;
+  if (mix.on) {
+    for (int b = 0; b < NB; b++)
+      t += (U[e * NV * NB + v * NB + b] - mix.uref[v] * mix.one[b]) * g.Vft[b * NAQ + sl];
+    static_cast<float*>(Uf_out)[idx] = static_cast<float>(t);
+  } else {
+    for (int b = 0; b < NB; b++) t += U[e * NV * NB + v * NB + b] * g.Vft[b * NAQ + sl];
+    static_cast<double*>(Uf_out)[idx] = t;
+  }
 }
 
 // CFL time step per element: min_q CFL minEdge / ((|u| + c)(P+1)^2) (TimeIntegration.cpp:104-131)

@@ -21,39 +21,116 @@
 namespace sdg {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ d4 mfma64(double a, double b, d4 c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+// One 16x16x4 matrix-core step per contraction type. Inputs: lane l holds A[l%16][l/16] and
+// B[l/16][l%16] for both types; the accumulator of lane l holds column l%16 and rows
+// row(l/16, i), i = 0..3, which differ between the FP64 and FP32 instructions.
+template <typename TC>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using acc_t = d4;
+  static __device__ __forceinline__ d4 mma(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int rr, int i) { return rr + 4 * i; }
+};
+template <>
+struct Mfma<float> {
+  using acc_t = f4;
+  static __device__ __forceinline__ f4 mma(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int rr, int i) { return 4 * rr + i; }
+};
+template <typename TC>
+using acc_t = typename Mfma<TC>::acc_t;
+
+template <typename TC>
+__device__ __forceinline__ acc_t<TC> zeroAcc() {
+  acc_t<TC> z = {TC(0), TC(0), TC(0), TC(0)};
+  return z;
 }
 
 // acc += T[k][m0 + 0..16) * X[k][0..16) for k in [k0, k1), with X in LDS ([k][16])
-template <int MP>
-__device__ __forceinline__ void tileAcc(d4& acc, const double* __restrict__ T, int m0, const double* X, int k0, int k1,
+template <typename TC, int MP>
+__device__ __forceinline__ void tileAcc(acc_t<TC>& acc, const TC* __restrict__ T, int m0, const TC* X, int k0, int k1,
                                         int lane) {
   const int c = lane & 15, kr = lane >> 4;
 #pragma unroll 8
-  for (int k = k0; k < k1; k += 4) acc = mfma64(T[(k + kr) * MP + m0 + c], X[(k + kr) * 16 + c], acc);
+  for (int k = k0; k < k1; k += 4) acc = Mfma<TC>::mma(T[(k + kr) * MP + m0 + c], X[(k + kr) * 16 + c], acc);
 }
 
-// Y[m0 + r][c] = acc (Y in LDS, [M][16])
-__device__ __forceinline__ void tileStore(const d4& acc, double* Y, int m0, int lane) {
+// Y[m0 + row][c] = acc (Y in LDS, [M][16])
+template <typename TC>
+__device__ __forceinline__ void tileStore(const acc_t<TC>& acc, TC* Y, int m0, int lane) {
   const int c = lane & 15, r = lane >> 4;
 #pragma unroll
-  for (int i = 0; i < 4; i++) Y[(m0 + r + 4 * i) * 16 + c] = acc[i];
+  for (int i = 0; i < 4; i++) Y[(m0 + Mfma<TC>::row(r, i)) * 16 + c] = acc[i];
 }
 
-__device__ __forceinline__ void zeroLds(double* p, int n, int tid) {
-  for (int i = tid; i < n; i += kThreads) p[i] = 0.0;
+template <typename TC>
+__device__ __forceinline__ void zeroLds(TC* p, int n, int tid) {
+  for (int i = tid; i < n; i += kThreads) p[i] = TC(0);
+}
+
+// Coefficient b of variable v of element e as a contraction operand: the FP64 state itself, or in
+// the mixed-precision mode its deviation from the reference state, rounded to FP32.
+template <typename TC>
+__device__ __forceinline__ TC stateOperand(const StageArgs& a, int64_t e, int v, int b, int NV, int NB) {
+  const double u = a.U[(e * NV + v) * NB + b];
+  if constexpr (sizeof(TC) == 8)
+    return u;
+  else
+    return static_cast<float>(u - a.mix.uref[v] * a.mix.one[b]);
+}
+
+// Conserved state at a point from a contraction result (FP64 in both modes).
+template <typename TC>
+__device__ __forceinline__ double pointState(const StageArgs& a, TC x, int v) {
+  if constexpr (sizeof(TC) == 8)
+    return x;
+  else
+    return a.mix.uref[v] + static_cast<double>(x);
+}
+
+// Pressure entering the momentum flux: p itself, or p - pref in the mixed-precision mode.
+template <typename TC>
+__device__ __forceinline__ double fluxPressure(const StageArgs& a, double p) {
+  if constexpr (sizeof(TC) == 8)
+    return p;
+  else
+    return p - a.mix.pref;
+}
+
+// Mixed-precision mode: the numerical flux minus pref n in the momentum and h0 H_rho in the energy.
+template <int D>
+__device__ __forceinline__ void reduceFaceFlux(const StageArgs& a, const double* n, double* h) {
+  for (int d = 0; d < D; d++) h[1 + d] -= a.mix.pref * n[d];
+  h[D + 1] -= a.mix.h0 * h[0];
+}
+
+// Mixed-precision mode: value x of column (element, variable v) of a residual-derived tile, with
+// h0 times the density column of the same element added back to the energy column. All lanes
+// must call it (cross-lane exchange).
+template <int NV>
+__device__ __forceinline__ double restoreEnergy(const StageArgs& a, double x, int v, int lane) {
+  const int src = v == NV - 1 ? lane - (NV - 1) : lane;
+  const double xr = __shfl(x, src);
+  return v == NV - 1 ? x + a.mix.h0 * xr : x;
 }
 
 // ---- shared tail: residual -> (monitor) -> update -> next traces --------------------------------
-// sR: [NBm][16] residual panel (row basis, column le*NV+v), sU: [NBk][16] current state panel.
-template <int D, int NB, int NQ, int NF, int NFQ, int EPB>
-__device__ __forceinline__ void mfmaStageTail(const Geo& g, const StageArgs& a, int e0, double* sR,
-                                              double* sU, double* sRed) {
+// sR: [NBm][16] residual panel (row basis, column le*NV+v), sU: [NBk][16] state panel (FP64 mode:
+// the current state; mixed mode: scratch, the update reads the FP64 state from HBM).
+template <typename TC, int D, int NB, int NQ, int NF, int NFQ, int EPB>
+__device__ __forceinline__ void mfmaStageTail(const Geo& g, const StageArgs& a, int e0, TC* sR, TC* sU,
+                                              double* sRed) {
   constexpr int NV = D + 2, NAQ = NF * NFQ;
   constexpr int NBk = r4(NB), NBm = r16(NB), NQm = r16(NQ), NAQm = r16(NAQ);
-  const MOps& op = g.op;
+  constexpr bool MIX = sizeof(TC) == 4;
+  const MOpsT<TC>& op = opsOf<TC>(g);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int c = lane & 15, rr = lane >> 4;
   const int le = c / NV, v = c % NV;
@@ -61,12 +138,16 @@ __device__ __forceinline__ void mfmaStageTail(const Geo& g, const StageArgs& a, 
   const bool col_ok = c < EPB * NV && e < g.E;
   if (a.last_stage) {  // residual monitor: mean_q |R V^T| (TimeIntegration.cpp:279-298)
     double part = 0.0;
+    #pragma unroll 1
     for (int m0 = wave * 16; m0 < NQm; m0 += 64) {
-      d4 acc = {0.0, 0.0, 0.0, 0.0};
-      tileAcc<NQm>(acc, op.interp, m0, sR, 0, NBk, lane);
+      acc_t<TC> acc = zeroAcc<TC>();
+      tileAcc<TC, NQm>(acc, op.interp, m0, sR, 0, NBk, lane);
 #pragma unroll
-      for (int i = 0; i < 4; i++)
-        if (m0 + rr + 4 * i < NQ) part += fabs(acc[i]);
+      for (int i = 0; i < 4; i++) {
+        double x = static_cast<double>(acc[i]);
+        if constexpr (MIX) x = restoreEnergy<NV>(a, x, v, lane);
+        if (m0 + Mfma<TC>::row(rr, i) < NQ) part += fabs(x);
+      }
     }
     part += __shfl_xor(part, 16);
     part += __shfl_xor(part, 32);
@@ -77,22 +158,34 @@ __device__ __forceinline__ void mfmaStageTail(const Geo& g, const StageArgs& a, 
       a.mon[static_cast<int64_t>(e0 + tid / NV) * NV + tid % NV] = s / NQ;
     }
   }
-  // update: U <- c1 U + c0 Un + c2dt (R Mref^-1) / |J|
+  // update: U <- c1 U + c0 Un + c2dt (R Mref^-1) / |J|, in FP64 in both modes
+  #pragma unroll 1
   for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
-    tileAcc<NBm>(acc, op.minv, m0, sR, 0, NBk, lane);
+    acc_t<TC> acc = zeroAcc<TC>();
+    tileAcc<TC, NBm>(acc, op.minv, m0, sR, 0, NBk, lane);
     const double idj = col_ok ? g.detj[e] : 1.0;
+    double un[4], rm[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const int bi = m0 + rr + 4 * i;
+      rm[i] = static_cast<double>(acc[i]);
+      if constexpr (MIX) rm[i] = restoreEnergy<NV>(a, rm[i], v, lane);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int bi = m0 + Mfma<TC>::row(rr, i);
+      un[i] = 0.0;
       if (col_ok && bi < NB) {
         const int64_t gi = (static_cast<int64_t>(e) * NV + v) * NB + bi;
-        double u = sU[bi * 16 + c] * a.c1;
+        double u;
+        if constexpr (MIX)
+          u = a.U[gi] * a.c1;
+        else
+          u = sU[bi * 16 + c] * a.c1;
         u += a.c0 * a.Un[gi];
-        u += a.c2dt * (acc[i] / idj);
+        u += a.c2dt * (rm[i] / idj);
         a.U[gi] = u;
         if (a.last_stage) a.Un[gi] = u;
-        acc[i] = u;
+        un[i] = u;
       }
     }
     // rows of this tile are read and written by this wave only (the contraction above reads sR),
@@ -100,52 +193,98 @@ __device__ __forceinline__ void mfmaStageTail(const Geo& g, const StageArgs& a, 
     if (m0 < NBk) {
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int bi = m0 + rr + 4 * i;
-        if (bi < NBk) sU[bi * 16 + c] = (col_ok && bi < NB) ? acc[i] : 0.0;
+        const int bi = m0 + Mfma<TC>::row(rr, i);
+        if (bi < NBk) {
+          TC x = TC(0);
+          if (col_ok && bi < NB) {
+            if constexpr (MIX)
+              x = static_cast<float>(un[i] - a.mix.uref[v] * a.mix.one[bi]);
+            else
+              x = un[i];
+          }
+          sU[bi * 16 + c] = x;
+        }
       }
     }
   }
   __syncthreads();
   // traces of the new state
+  TC* Uf_out = pick<TC>(a.Uf_out, a.Uf_out32);
+  #pragma unroll 1
   for (int m0 = wave * 16; m0 < NAQm; m0 += 64) {
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
-    tileAcc<NAQm>(acc, op.trace, m0, sU, 0, NBk, lane);
+    acc_t<TC> acc = zeroAcc<TC>();
+    tileAcc<TC, NAQm>(acc, op.trace, m0, sU, 0, NBk, lane);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const int sl = m0 + rr + 4 * i;
-      if (col_ok && sl < NAQ) a.Uf_out[(static_cast<int64_t>(e) * NV + v) * NAQ + sl] = acc[i];
+      const int sl = m0 + Mfma<TC>::row(rr, i);
+      if (col_ok && sl < NAQ) Uf_out[(static_cast<int64_t>(e) * NV + v) * NAQ + sl] = acc[i];
     }
   }
 }
 
+// volume convective flux minus viscous flux, contracted with (J^T)^-1 |J| w:
+// out[b * NV + v] = sum_a (F_c - F_v)[a][v] G[a][b] (ConvectiveFlux.cpp:27-57, ViscousFlux.cpp:58-103)
+template <typename TC, int D, bool VISC>
+__device__ __forceinline__ void volumeFlux(const StageArgs& a, const double* w, const double* fv, const double* G,
+                                           double* out) {
+  constexpr int NV = D + 2;
+  const double rho = w[0], p = w[D + 2];
+  const double et = w[D + 1] + velocitySquared<D>(w) / 2.0;
+  double fc[D * NV];
+  for (int aa = 0; aa < D; aa++) {
+    fc[aa] = rho * w[1 + aa];
+    if constexpr (sizeof(TC) == 8) {
+      for (int b = 0; b < D; b++) fc[aa + D * (1 + b)] = rho * w[1 + aa] * w[1 + b] + (aa == b ? p * 1.0 : p * 0.0);
+    } else {
+      const double pf = fluxPressure<TC>(a, p);
+      for (int b = 0; b < D; b++) fc[aa + D * (1 + b)] = rho * w[1 + aa] * w[1 + b] + (aa == b ? pf : 0.0);
+    }
+    if constexpr (sizeof(TC) == 8)
+      fc[aa + D * (D + 1)] = (rho * et + p) * w[1 + aa];
+    else
+      fc[aa + D * (D + 1)] = (rho * et + p - a.mix.h0 * rho) * w[1 + aa];
+  }
+  for (int b = 0; b < D; b++)
+    for (int v = 0; v < NV; v++) {
+      double s = 0.0;
+      for (int aa = 0; aa < D; aa++) {
+        const double f = VISC ? fc[aa + D * v] - fv[aa + D * v] : fc[aa + D * v];
+        s += f * G[aa + D * b];
+      }
+      out[b * NV + v] = s;
+    }
+}
+
 // ---- Euler stage (MFMA) -------------------------------------------------------------------------
-template <int D, int NB, int NQ, int NF, int NFQ>
+template <typename TC, int D, int NB, int NQ, int NF, int NFQ>
 __global__ __launch_bounds__(kThreads) void eulerStageMfma(Geo g, Phys ph, StageArgs a) {
-  const MOps& op = g.op;
+  const MOpsT<TC>& op = opsOf<TC>(g);
   constexpr int NV = D + 2, NAQ = NF * NFQ;
   constexpr int EPB = 16 / NV;
   constexpr int NBk = r4(NB), NBm = r16(NB), NQm = r16(NQ), NAQk = r4(NAQ), NDQk = r4(NQ * D);
   constexpr int NBr = NBk > NBm ? NBk : NBm;
-  __shared__ double sU[NBr * 16];
-  __shared__ double sUq[NQm * 16];
-  __shared__ double sF[NDQk * 16];
-  __shared__ double sH[NAQk * 16];
-  __shared__ double sR[NBr * 16];
+  __shared__ TC sU[NBr * 16];
+  __shared__ TC sUq[NQm * 16];
+  __shared__ TC sF[NDQk * 16];
+  __shared__ TC sH[NAQk * 16];
+  __shared__ TC sR[NBr * 16];
   __shared__ double sRed[64];
   if (a.status && a.status[0]) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int e0 = blockIdx.x * EPB;
+  const TC* Uf_in = pick<TC>(a.Uf_in, a.Uf_in32);
   zeroLds(sF, NDQk * 16, tid);
   zeroLds(sH, NAQk * 16, tid);
   for (int idx = tid; idx < NBr * 16; idx += kThreads) {
     const int b = idx / 16, c = idx % 16, le = c / NV, v = c % NV, e = e0 + le;
-    sU[idx] = (b < NB && c < EPB * NV && e < g.E) ? a.U[(static_cast<int64_t>(e) * NV + v) * NB + b] : 0.0;
+    sU[idx] = (b < NB && c < EPB * NV && e < g.E) ? stateOperand<TC>(a, e, v, b, NV, NB) : TC(0);
   }
   __syncthreads();
+  #pragma unroll 1
   for (int m0 = wave * 16; m0 < NQm; m0 += 64) {
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
-    tileAcc<NQm>(acc, op.interp, m0, sU, 0, NBk, lane);
-    tileStore(acc, sUq, m0, lane);
+    acc_t<TC> acc = zeroAcc<TC>();
+    tileAcc<TC, NQm>(acc, op.interp, m0, sU, 0, NBk, lane);
+    tileStore<TC>(acc, sUq, m0, lane);
   }
   __syncthreads();
   // volume flux at the points: (F^T (J^T)^-1 |J| w) -> sF[(q D + b)][le NV + v]
@@ -153,72 +292,77 @@ __global__ __launch_bounds__(kThreads) void eulerStageMfma(Geo g, Phys ph, Stage
     const int le = idx / NQ, q = idx % NQ, e = e0 + le;
     if (e >= g.E) continue;
     double u[NV], w[D + 3];
-    for (int v = 0; v < NV; v++) u[v] = sUq[q * 16 + le * NV + v];
+    for (int v = 0; v < NV; v++) u[v] = pointState<TC>(a, sUq[q * 16 + le * NV + v], v);
     computational<D>(ph, u, w);
-    const double rho = w[0], p = w[D + 2];
-    const double et = w[D + 1] + velocitySquared<D>(w) / 2.0;
-    double fc[D * NV];
-    for (int aa = 0; aa < D; aa++) {
-      fc[aa] = rho * w[1 + aa];
-      for (int b = 0; b < D; b++) fc[aa + D * (1 + b)] = rho * w[1 + aa] * w[1 + b] + (aa == b ? p * 1.0 : p * 0.0);
-      fc[aa + D * (D + 1)] = (rho * et + p) * w[1 + aa];
-    }
     const double jw = g.detj[e] * g.wq[q];
-    double G[D * D];
+    double G[D * D], out[D * NV];
     for (int i = 0; i < D * D; i++) G[i] = g.ginv[static_cast<int64_t>(e) * D * D + i] * jw;
-    for (int v = 0; v < NV; v++)
-      for (int b = 0; b < D; b++) {
-        double s = 0.0;
-        for (int aa = 0; aa < D; aa++) s += fc[aa + D * v] * G[aa + D * b];
-        sF[(q * D + b) * 16 + le * NV + v] = s;
-      }
+    volumeFlux<TC, D, false>(a, w, nullptr, G, out);
+    for (int b = 0; b < D; b++)
+      for (int v = 0; v < NV; v++) sF[(q * D + b) * 16 + le * NV + v] = static_cast<TC>(out[b * NV + v]);
   }
   // face fluxes -> sH[slot][le NV + v] (negated: the face term enters the residual with -)
   for (int idx = tid; idx < EPB * NAQ; idx += kThreads) {
     const int le = idx / NAQ, sl = idx % NAQ, e = e0 + le;
     if (e >= g.E) continue;
-    const int s = sl / NFQ, j = sl % NFQ;
-    int f, p, side;
-    int64_t nbslot;
-    slotRoles<NF, NFQ>(g, e, s, j, f, p, nbslot, side);
+    const int64_t es = static_cast<int64_t>(e) * NAQ + sl;
+    const int nbs = g.snb[es], fps = g.sfp[es];
+    if (nbs < 0) {  // boundary face point
+      double ub[NV], t[NV];
+      for (int v = 0; v < NV; v++) ub[v] = pointState<TC>(a, Uf_in[static_cast<int64_t>(e) * NV * NAQ + sl + v * NAQ], v);
+      boundaryFlux<D, false>(ph, g, -1 - nbs, ub, nullptr, sizeof(TC) == 4, a.mix.pref, a.mix.h0, t);
+      for (int v = 0; v < NV; v++) sH[sl * 16 + le * NV + v] = static_cast<TC>(-t[v]);
+      continue;
+    }
+    const int side = fps & 1;
+    const int64_t fp = fps >> 1;
     double uo[NV], un[NV];
     const int64_t own = static_cast<int64_t>(e) * NV * NAQ + sl;
-    const int64_t nbe = (nbslot / NAQ) * NV * NAQ + nbslot % NAQ;
+    const int64_t nbe = static_cast<int64_t>(nbs / NAQ) * NV * NAQ + nbs % NAQ;
     for (int v = 0; v < NV; v++) {
-      uo[v] = a.Uf_in[own + v * NAQ];
-      un[v] = a.Uf_in[nbe + v * NAQ];
+      uo[v] = pointState<TC>(a, Uf_in[own + v * NAQ], v);
+      un[v] = pointState<TC>(a, Uf_in[nbe + v * NAQ], v);
     }
-    const double* ul = side == 0 ? uo : un;
-    const double* ur = side == 0 ? un : uo;
+    double ul[NV], ur[NV];  // values, not pointers: keeps the arrays in registers
+    for (int v = 0; v < NV; v++) {
+      ul[v] = side == 0 ? uo[v] : un[v];
+      ur[v] = side == 0 ? un[v] : uo[v];
+    }
     double wl[D + 3], wr[D + 3], n[D], h[NV];
     computational<D>(ph, ul, wl);
     computational<D>(ph, ur, wr);
-    for (int d = 0; d < D; d++) n[d] = g.fnormal[(static_cast<int64_t>(f) * NFQ + p) * D + d];
+    for (int d = 0; d < D; d++) n[d] = g.fnormal[fp * D + d];
     convectiveFlux<D>(ph, n, ul, wl, ur, wr, h);
-    const double jw = g.fjw[static_cast<int64_t>(f) * NFQ + p];
+    if constexpr (sizeof(TC) == 4) reduceFaceFlux<D>(a, n, h);
+    const double jw = g.fjw[fp];
     for (int v = 0; v < NV; v++) {
       const double t = h[v] * jw;
-      sH[sl * 16 + le * NV + v] = side == 0 ? -t : t;
+      sH[sl * 16 + le * NV + v] = static_cast<TC>(side == 0 ? -t : t);
     }
   }
   __syncthreads();
   // residual R = F Phi_grad - H Phi_face
+  #pragma unroll 1
   for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
-    tileAcc<NBm>(acc, op.grad, m0, sF, 0, NDQk, lane);
-    tileAcc<NBm>(acc, op.face, m0, sH, 0, NAQk, lane);
-    tileStore(acc, sR, m0, lane);
+    acc_t<TC> acc = zeroAcc<TC>();
+    tileAcc<TC, NBm>(acc, op.grad, m0, sF, 0, NDQk, lane);
+    tileAcc<TC, NBm>(acc, op.face, m0, sH, 0, NAQk, lane);
+    tileStore<TC>(acc, sR, m0, lane);
   }
-  for (int idx = NBm * 16 + tid; idx < NBr * 16; idx += kThreads) sR[idx] = 0.0;
+  for (int idx = NBm * 16 + tid; idx < NBr * 16; idx += kThreads) sR[idx] = TC(0);
   __syncthreads();
   if (a.mode == 1) {
     for (int idx = tid; idx < NB * 16; idx += kThreads) {
       const int b = idx / 16, c = idx % 16, le = c / NV, v = c % NV, e = e0 + le;
-      if (c < EPB * NV && e < g.E) a.Rout[(static_cast<int64_t>(e) * NV + v) * NB + b] = sR[idx];
+      if (c < EPB * NV && e < g.E) {
+        double r = sR[idx];
+        if (sizeof(TC) == 4 && v == NV - 1) r += a.mix.h0 * static_cast<double>(sR[b * 16 + le * NV]);
+        a.Rout[(static_cast<int64_t>(e) * NV + v) * NB + b] = r;
+      }
     }
     return;
   }
-  mfmaStageTail<D, NB, NQ, NF, NFQ, EPB>(g, a, e0, sR, sU, sRed);
+  mfmaStageTail<TC, D, NB, NQ, NF, NFQ, EPB>(g, a, e0, sR, sU, sRed);
 }
 
 // ---- Navier-Stokes kernel 1 (MFMA): gradient lifting + volume flux -------------------------------
@@ -226,31 +370,45 @@ __global__ __launch_bounds__(kThreads) void eulerStageMfma(Geo g, Phys ph, Stage
 // The lifted coefficients w = R Mref^-1 / |J| are only ever evaluated at points, so Mref^-1 is folded
 // into the evaluation operators: grad U_q = R_t (Mref^-1 V^T) / |J| and the BR2 face trace
 // R_w (Mref^-1 Vf^T) / |J| + J Lf / |J| (BR1: R_t (Mref^-1 Vf^T) / |J|).
-template <int D, int NB, int NQ, int NF, int NFQ, int VISC>
-__global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs a) {
-  const MOps& op = g.op;
-  constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
-  constexpr int EPB = 16 / NG;
-  constexpr int NBk = r4(NB), NBm = r16(NB), NQm = r16(NQ), NAQk = r4(NAQ), NAQm = r16(NAQ), NDQk = r4(NQ * D);
-  constexpr int NBr = NBk > NBm ? NBk : NBm;
-  constexpr int AJ = 2 * NAQk * 16 > NQm * 16 ? 2 * NAQk * 16 : NQm * 16;
-  __shared__ double sUq[NQm * 16];    // state at volume points
-  __shared__ double sAJ[AJ];          // face terms A | J ([slot][le*NG + r]); later the gradient at points
-  __shared__ double sRR[2 * NBr * 16];  // state panel; then R_w | R_t
-  double* sU = sRR;
-  double* sA = sAJ;
-  double* sJ = sAJ + NAQk * 16;
-  double* sRw = sRR;
-  double* sRt = sRR + NBr * 16;
-  double* sGq = sAJ;
-  if (a.status && a.status[0]) return;
+// In the mixed-precision mode the lifting acts on the deviation traces (the lifted gradient of the
+// constant reference state is zero).
+// LDS layout shared by the gradient-lifting kernels (forward and adjoint)
+template <int D, int NB, int NQ, int NF, int NFQ>
+struct GradLds {
+  static constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
+  static constexpr int EPB = 16 / NG;
+  static constexpr int NBk = r4(NB), NBm = r16(NB), NQm = r16(NQ), NAQk = r4(NAQ), NAQm = r16(NAQ), NDQk = r4(NQ * D);
+  static constexpr int NBr = NBk > NBm ? NBk : NBm;
+  static constexpr int AJ = 2 * NAQk * 16 > NQm * 16 ? 2 * NAQk * 16 : NQm * 16;
+};
+
+// Gradient lifting (BR1 / BR2) of the workgroup's EPB elements starting at e0. On return:
+//   sUq[q][le*NV + v]   state at the volume points,
+//   sAJ[q][le*NG + r]   total (lifted) gradient at the volume points (first NQm*16 entries),
+//   Gf (HBM)            lifted gradient traces at the element's face points, and, if a.Wout is
+//                       set, the lifted total-gradient coefficients;
+// sRR is free again. Ends with a barrier.
+template <typename TC, int D, int NB, int NQ, int NF, int NFQ, int VISC>
+__device__ __forceinline__ void gradLift(const Geo& g, const Phys& ph, const StageArgs& a, int e0, TC* sUq, TC* sAJ,
+                                         TC* sRR) {
+  using LD = GradLds<D, NB, NQ, NF, NFQ>;
+  const MOpsT<TC>& op = opsOf<TC>(g);
+  constexpr int NV = LD::NV, NG = LD::NG, NAQ = LD::NAQ, EPB = LD::EPB;
+  constexpr int NBk = LD::NBk, NBm = LD::NBm, NQm = LD::NQm, NAQk = LD::NAQk, NAQm = LD::NAQm, NDQk = LD::NDQk;
+  constexpr int NBr = LD::NBr, AJ = LD::AJ;
+  TC* sU = sRR;
+  TC* sA = sAJ;
+  TC* sJ = sAJ + NAQk * 16;
+  TC* sRw = sRR;
+  TC* sRt = sRR + NBr * 16;
+  TC* sGq = sAJ;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int c = lane & 15, rr = lane >> 4;
-  const int e0 = blockIdx.x * EPB;
+  const TC* Uf_in = pick<TC>(a.Uf_in, a.Uf_in32);
   zeroLds(sAJ, AJ, tid);
   for (int idx = tid; idx < NBr * 16; idx += kThreads) {
     const int b = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
-    sU[idx] = (b < NB && cc < EPB * NV && e < g.E) ? a.U[(static_cast<int64_t>(e) * NV + v) * NB + b] : 0.0;
+    sU[idx] = (b < NB && cc < EPB * NV && e < g.E) ? stateOperand<TC>(a, e, v, b, NV, NB) : TC(0);
   }
   __syncthreads();
   // face terms: (u- + u+)/2 (x) n |J_f| w (signed by side) and (u+ - u-)/2 (x) n |J_f| w
@@ -259,24 +417,35 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
     if (e >= g.E) continue;
     const int64_t es = static_cast<int64_t>(e) * NAQ + sl;
     const int nbs = g.snb[es], fps = g.sfp[es];
+    if (nbs < 0) {  // boundary face point: terms from the boundary-condition states
+      double uo[NV], Ab[D], Jb[D];
+      for (int k = 0; k < NV; k++) uo[k] = pointState<TC>(a, Uf_in[(static_cast<int64_t>(e) * NV + k) * NAQ + sl], k);
+      boundaryGradientTerms<D>(ph, g, -1 - nbs, uo, v, sizeof(TC) == 4 ? a.mix.uref[v] : 0.0, Ab, Jb);
+      for (int d = 0; d < D; d++) {
+        sA[sl * 16 + le * NG + v * D + d] = static_cast<TC>(Ab[d]);
+        sJ[sl * 16 + le * NG + v * D + d] = static_cast<TC>(Jb[d]);
+      }
+      continue;
+    }
     const int side = fps & 1;
     const int64_t fp = fps >> 1;
-    const double uo = a.Uf_in[(static_cast<int64_t>(e) * NV + v) * NAQ + sl];
-    const double un = a.Uf_in[(static_cast<int64_t>(nbs / NAQ) * NV + v) * NAQ + nbs % NAQ];
+    const double uo = Uf_in[(static_cast<int64_t>(e) * NV + v) * NAQ + sl];
+    const double un = Uf_in[(static_cast<int64_t>(nbs / NAQ) * NV + v) * NAQ + nbs % NAQ];
     const double ul = side == 0 ? uo : un, ur = side == 0 ? un : uo;
     const double jw = g.fjw[fp];
     for (int d = 0; d < D; d++) {
       const double n = g.fnormal[fp * D + d];
       const double avg = (n * (ul + ur) / 2.0) * jw;
-      sA[sl * 16 + le * NG + v * D + d] = side == 0 ? avg : -avg;
-      sJ[sl * 16 + le * NG + v * D + d] = (n * (ur - ul) / 2.0) * jw;
+      sA[sl * 16 + le * NG + v * D + d] = static_cast<TC>(side == 0 ? avg : -avg);
+      sJ[sl * 16 + le * NG + v * D + d] = static_cast<TC>((n * (ur - ul) / 2.0) * jw);
     }
   }
   // state at the volume points
+  #pragma unroll 1
   for (int m0 = wave * 16; m0 < NQm; m0 += 64) {
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
-    tileAcc<NQm>(acc, op.interp, m0, sU, 0, NBk, lane);
-    tileStore(acc, sUq, m0, lane);
+    acc_t<TC> acc = zeroAcc<TC>();
+    tileAcc<TC, NQm>(acc, op.interp, m0, sU, 0, NBk, lane);
+    tileStore<TC>(acc, sUq, m0, lane);
   }
   __syncthreads();
   // gradient residuals: R_w = A Phi_f - Q Phi_grad, R_t = R_w + J Phi_f, with
@@ -290,100 +459,110 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
     double gi[D];
 #pragma unroll
     for (int b = 0; b < D; b++) gi[b] = g.ginv[static_cast<int64_t>(gec) * D * D + ga + D * b];
+    #pragma unroll 1
     for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
-      d4 aw = {0.0, 0.0, 0.0, 0.0}, aj = {0.0, 0.0, 0.0, 0.0};
-      tileAcc<NBm>(aw, op.face, m0, sA, 0, NAQk, lane);
-      tileAcc<NBm>(aj, op.face, m0, sJ, 0, NAQk, lane);
+      acc_t<TC> aw = zeroAcc<TC>(), aj = zeroAcc<TC>();
+      tileAcc<TC, NBm>(aw, op.face, m0, sA, 0, NAQk, lane);
+      tileAcc<TC, NBm>(aj, op.face, m0, sJ, 0, NAQk, lane);
 #pragma unroll 4
       for (int k = 0; k < NDQk; k += 4) {
         const int kk = k + rr;
-        double bq = 0.0;
+        TC bq = TC(0);
         if (gok && kk < NQ * D) {
           const int q = kk / D, b = kk % D;
           const double G = gi[b] * (dj * g.wq[q]);
-          bq = -(sUq[q * 16 + gle * NV + gk] * G);
+          bq = static_cast<TC>(-(static_cast<double>(sUq[q * 16 + gle * NV + gk]) * G));
         }
-        aw = mfma64(op.grad[kk * NBm + m0 + c], bq, aw);
+        aw = Mfma<TC>::mma(op.grad[kk * NBm + m0 + c], bq, aw);
       }
       // the state panel (aliased by R) was last read before the barrier above
-      tileStore(aw, sRw, m0, lane);
-      d4 at = aw + aj;
-      tileStore(at, sRt, m0, lane);
+      tileStore<TC>(aw, sRw, m0, lane);
+      acc_t<TC> at = aw + aj;
+      tileStore<TC>(at, sRt, m0, lane);
     }
   }
-  for (int idx = NBm * 16 + tid; idx < NBr * 16; idx += kThreads) sRw[idx] = sRt[idx] = 0.0;
+  for (int idx = NBm * 16 + tid; idx < NBr * 16; idx += kThreads) sRw[idx] = sRt[idx] = TC(0);
   __syncthreads();
   const double idj = 1.0 / dj;
   // optional output of the lifted total-gradient coefficients w = R_t Mref^-1 / |J|
   if (a.Wout) {
+    #pragma unroll 1
     for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
-      d4 wt = {0.0, 0.0, 0.0, 0.0};
-      tileAcc<NBm>(wt, op.minv, m0, sRt, 0, NBk, lane);
+      acc_t<TC> wt = zeroAcc<TC>();
+      tileAcc<TC, NBm>(wt, op.minv, m0, sRt, 0, NBk, lane);
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int bi = m0 + rr + 4 * i;
-        if (gok && bi < NB) a.Wout[(static_cast<int64_t>(ge) * NB + bi) * NG + (c % NG)] = wt[i] * idj;
+        const int bi = m0 + Mfma<TC>::row(rr, i);
+        if (gok && bi < NB) a.Wout[(static_cast<int64_t>(ge) * NB + bi) * NG + (c % NG)] = static_cast<double>(wt[i]) * idj;
       }
     }
   }
   // lifted gradient traces at the face points -> HBM
+  TC* Gf = pick<TC>(a.Gf, a.Gf32);
+  #pragma unroll 1
   for (int m0 = wave * 16; m0 < NAQm; m0 += 64) {
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    acc_t<TC> acc = zeroAcc<TC>();
     if (VISC == 2) {
-      tileAcc<NAQm>(acc, op.ptrace, m0, sRw, 0, NBk, lane);
+      tileAcc<TC, NAQm>(acc, op.ptrace, m0, sRw, 0, NBk, lane);
       // block-diagonal BR2 lift: only the faces this row tile touches
       const int s_lo = m0 / NFQ;
       const int s_hi = (m0 + 15 < NAQ ? m0 + 15 : NAQ - 1) / NFQ;
       const int k0 = (s_lo * NFQ) / 4 * 4;
       const int k1 = r4((s_hi + 1) * NFQ);
-      tileAcc<NAQm>(acc, op.lift, m0, sJ, k0, k1 < NAQk ? k1 : NAQk, lane);
+      tileAcc<TC, NAQm>(acc, op.lift, m0, sJ, k0, k1 < NAQk ? k1 : NAQk, lane);
     } else {
-      tileAcc<NAQm>(acc, op.ptrace, m0, sRt, 0, NBk, lane);
+      tileAcc<TC, NAQm>(acc, op.ptrace, m0, sRt, 0, NBk, lane);
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const int sl = m0 + rr + 4 * i;
-      if (gok && sl < NAQ) a.Gf[(static_cast<int64_t>(ge) * NG + (c % NG)) * NAQ + sl] = acc[i] * idj;
+      const int sl = m0 + Mfma<TC>::row(rr, i);
+      if (gok && sl < NAQ)
+        Gf[(static_cast<int64_t>(ge) * NG + (c % NG)) * NAQ + sl] = static_cast<TC>(static_cast<double>(acc[i]) * idj);
     }
   }
   __syncthreads();
   // total gradient at the volume points (into the face-term buffer, no longer needed)
+  #pragma unroll 1
   for (int m0 = wave * 16; m0 < NQm; m0 += 64) {
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
-    tileAcc<NQm>(acc, op.pinterp, m0, sRt, 0, NBk, lane);
-    acc *= idj;
-    tileStore(acc, sGq, m0, lane);
+    acc_t<TC> acc = zeroAcc<TC>();
+    tileAcc<TC, NQm>(acc, op.pinterp, m0, sRt, 0, NBk, lane);
+#pragma unroll
+    for (int i = 0; i < 4; i++) acc[i] = static_cast<TC>(static_cast<double>(acc[i]) * idj);
+    tileStore<TC>(acc, sGq, m0, lane);
   }
   __syncthreads();
+}
+
+template <typename TC, int D, int NB, int NQ, int NF, int NFQ, int VISC>
+__global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs a) {
+  using LD = GradLds<D, NB, NQ, NF, NFQ>;
+  constexpr int NV = LD::NV, NG = LD::NG, EPB = LD::EPB;
+  __shared__ TC sUq[LD::NQm * 16];      // state at volume points
+  __shared__ TC sAJ[LD::AJ];            // face terms A | J ([slot][le*NG + r]); later the gradient at points
+  __shared__ TC sRR[2 * LD::NBr * 16];  // state panel; then R_w | R_t
+  if (a.status && a.status[0]) return;
+  const int tid = threadIdx.x;
+  const int e0 = blockIdx.x * EPB;
+  gradLift<TC, D, NB, NQ, NF, NFQ, VISC>(g, ph, a, e0, sUq, sAJ, sRR);
+  const TC* sGq = sAJ;
   // volume flux (F_c - F_v)^T (J^T)^-1 |J| w at the points -> HBM Fv[e][(q D + b)][v]; the weak
   // divergence F Phi_grad is taken in nsResidualMfma, which packs three elements per 16 columns
+  TC* Fv = pick<TC>(a.Fv, a.Fv32);
   for (int idx = tid; idx < EPB * NQ; idx += kThreads) {
     const int le = idx / NQ, q = idx % NQ, e = e0 + le;
     if (e >= g.E) continue;
     double u[NV], w[D + 3], gq[NG], pg[NG], fv[D * NV];
-    for (int v = 0; v < NV; v++) u[v] = sUq[q * 16 + le * NV + v];
+    for (int v = 0; v < NV; v++) u[v] = pointState<TC>(a, sUq[q * 16 + le * NV + v], v);
     for (int r = 0; r < NG; r++) gq[r] = sGq[q * 16 + le * NG + r];
     computational<D>(ph, u, w);
     primitiveGradient<D>(ph, w, u, gq, pg);
     viscousRawFlux<D>(ph, w, pg, fv);
-    const double rho = w[0], p = w[D + 2];
-    const double et = w[D + 1] + velocitySquared<D>(w) / 2.0;
-    double fc[D * NV];
-    for (int aa = 0; aa < D; aa++) {
-      fc[aa] = rho * w[1 + aa];
-      for (int b = 0; b < D; b++) fc[aa + D * (1 + b)] = rho * w[1 + aa] * w[1 + b] + (aa == b ? p * 1.0 : p * 0.0);
-      fc[aa + D * (D + 1)] = (rho * et + p) * w[1 + aa];
-    }
     const double jw = g.detj[e] * g.wq[q];
-    double G[D * D];
+    double G[D * D], out[D * NV];
     for (int i = 0; i < D * D; i++) G[i] = g.ginv[static_cast<int64_t>(e) * D * D + i] * jw;
-    double* out = a.Fv + (static_cast<int64_t>(e) * NQ * D + q * D) * NV;
-    for (int b = 0; b < D; b++)
-      for (int v = 0; v < NV; v++) {
-        double s = 0.0;
-        for (int aa = 0; aa < D; aa++) s += (fc[aa + D * v] - fv[aa + D * v]) * G[aa + D * b];
-        out[b * NV + v] = s;
-      }
+    volumeFlux<TC, D, true>(a, w, fv, G, out);
+    TC* o = Fv + (static_cast<int64_t>(e) * NQ * D + q * D) * NV;
+    for (int i = 0; i < D * NV; i++) o[i] = static_cast<TC>(out[i]);
   }
 }
 
@@ -392,12 +571,27 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
 // evaluated once with its left/right roles; t = (H_c - H_v) |J_f| w goes to the left parent's slot
 // as -t and to the right parent's slot (point seq[j]) as +t, the signs with which the face term
 // enters R = R_vol + sum_slots Phi_f s.
-template <int D, int NF, int NFQ>
+template <typename TC, int D, int NF, int NFQ>
 __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, StageArgs a) {
   constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
   if (a.status && a.status[0]) return;
   const int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (idx >= static_cast<int64_t>(g.F) * NFQ) return;
+  if (idx >= static_cast<int64_t>(g.F + g.Fb) * NFQ) return;
+  const TC* Uf_in = pick<TC>(a.Uf_in, a.Uf_in32);
+  const TC* Gf = pick<TC>(static_cast<const double*>(a.Gf), static_cast<const float*>(a.Gf32));
+  TC* Hs = pick<TC>(a.Hs, a.Hs32);
+  if (idx >= static_cast<int64_t>(g.F) * NFQ) {  // boundary face point
+    const int bp = static_cast<int>(idx - static_cast<int64_t>(g.F) * NFQ);
+    const int64_t os = g.bslot[bp];
+    const int64_t e = os / NAQ;
+    const int sl = static_cast<int>(os % NAQ);
+    double ub[NV], gb[NG], t[NV];
+    for (int v = 0; v < NV; v++) ub[v] = pointState<TC>(a, Uf_in[(e * NV + v) * NAQ + sl], v);
+    for (int r = 0; r < NG; r++) gb[r] = Gf[(e * NG + r) * NAQ + sl];
+    boundaryFlux<D, true>(ph, g, bp, ub, gb, sizeof(TC) == 4, a.mix.pref, a.mix.h0, t);
+    for (int v = 0; v < NV; v++) Hs[(e * NV + v) * NAQ + sl] = static_cast<TC>(-t[v]);
+    return;
+  }
   const int64_t f = idx / NFQ;
   const int j = static_cast<int>(idx % NFQ);
   const int k = g.fseq[f * NFQ + j];
@@ -405,18 +599,19 @@ __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, Sta
   const int slL = g.floc[2 * f] * NFQ + j, slR = g.floc[2 * f + 1] * NFQ + k;
   double ul[NV], ur[NV], gl[NG], gr[NG];
   for (int v = 0; v < NV; v++) {
-    ul[v] = a.Uf_in[(L * NV + v) * NAQ + slL];
-    ur[v] = a.Uf_in[(R * NV + v) * NAQ + slR];
+    ul[v] = pointState<TC>(a, Uf_in[(L * NV + v) * NAQ + slL], v);
+    ur[v] = pointState<TC>(a, Uf_in[(R * NV + v) * NAQ + slR], v);
   }
   for (int r = 0; r < NG; r++) {
-    gl[r] = a.Gf[(L * NG + r) * NAQ + slL];
-    gr[r] = a.Gf[(R * NG + r) * NAQ + slR];
+    gl[r] = Gf[(L * NG + r) * NAQ + slL];
+    gr[r] = Gf[(R * NG + r) * NAQ + slR];
   }
   double wl[D + 3], wr[D + 3], n[D], h[NV], pgl[NG], pgr[NG], fvl[D * NV], fvr[D * NV];
   computational<D>(ph, ul, wl);
   computational<D>(ph, ur, wr);
   for (int d = 0; d < D; d++) n[d] = g.fnormal[(f * NFQ + j) * D + d];
   convectiveFlux<D>(ph, n, ul, wl, ur, wr, h);
+  if constexpr (sizeof(TC) == 4) reduceFaceFlux<D>(a, n, h);
   primitiveGradient<D>(ph, wl, ul, gl, pgl);
   primitiveGradient<D>(ph, wr, ur, gr, pgr);
   viscousRawFlux<D>(ph, wl, pgl, fvl);
@@ -429,58 +624,67 @@ __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, Sta
       vr += fvr[d + D * v] * n[d];
     }
     const double t = (h[v] - (vl + vr) / 2.0) * jw;
-    a.Hs[(L * NV + v) * NAQ + slL] = -t;
-    a.Hs[(R * NV + v) * NAQ + slR] = t;
+    Hs[(L * NV + v) * NAQ + slL] = static_cast<TC>(-t);
+    Hs[(R * NV + v) * NAQ + slR] = static_cast<TC>(t);
   }
 }
 
 // ---- Navier-Stokes residual + update + next traces (MFMA), after nsFaceFluxKernel ---------------
 // calculateResidual + updateBasisFunctionCoefficient (SpatialDiscrete.cpp:1016-1032,
 // TimeIntegration.cpp:181-198)
-template <int D, int NB, int NQ, int NF, int NFQ>
+template <typename TC, int D, int NB, int NQ, int NF, int NFQ>
 __global__ __launch_bounds__(kThreads) void nsResidualMfma(Geo g, Phys ph, StageArgs a) {
-  const MOps& op = g.op;
+  const MOpsT<TC>& op = opsOf<TC>(g);
   constexpr int NV = D + 2, NAQ = NF * NFQ;
   constexpr int EPB = 16 / NV;
   constexpr int NBk = r4(NB), NBm = r16(NB), NAQk = r4(NAQ);
   constexpr int NBr = NBk > NBm ? NBk : NBm;
   constexpr int NDQ = NQ * D, NDQk = r4(NDQ);
-  __shared__ double sU[NBr * 16];
-  __shared__ double sH[NAQk * 16];
-  __shared__ double sF[NDQk * 16];
-  __shared__ double sR[NBr * 16];
+  __shared__ TC sU[NBr * 16];
+  __shared__ TC sH[NAQk * 16];
+  __shared__ TC sF[NDQk * 16];
+  __shared__ TC sR[NBr * 16];
   __shared__ double sRed[64];
   if (a.status && a.status[0]) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int e0 = blockIdx.x * EPB;
-  for (int idx = tid; idx < NBr * 16; idx += kThreads) {
-    const int b = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
-    sU[idx] = (b < NB && cc < EPB * NV && e < g.E) ? a.U[(static_cast<int64_t>(e) * NV + v) * NB + b] : 0.0;
+  const TC* Hs = pick<TC>(static_cast<const double*>(a.Hs), static_cast<const float*>(a.Hs32));
+  const TC* Fv = pick<TC>(static_cast<const double*>(a.Fv), static_cast<const float*>(a.Fv32));
+  if constexpr (sizeof(TC) == 8) {  // the mixed-precision tail reads the FP64 state from HBM
+    for (int idx = tid; idx < NBr * 16; idx += kThreads) {
+      const int b = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
+      sU[idx] = (b < NB && cc < EPB * NV && e < g.E) ? a.U[(static_cast<int64_t>(e) * NV + v) * NB + b] : 0.0;
+    }
   }
   for (int idx = tid; idx < NAQk * 16; idx += kThreads) {
     const int sl = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
-    sH[idx] = (sl < NAQ && cc < EPB * NV && e < g.E) ? a.Hs[(static_cast<int64_t>(e) * NV + v) * NAQ + sl] : 0.0;
+    sH[idx] = (sl < NAQ && cc < EPB * NV && e < g.E) ? Hs[(static_cast<int64_t>(e) * NV + v) * NAQ + sl] : TC(0);
   }
   for (int idx = tid; idx < NDQk * 16; idx += kThreads) {
     const int k = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
-    sF[idx] = (k < NDQ && cc < EPB * NV && e < g.E) ? a.Fv[(static_cast<int64_t>(e) * NDQ + k) * NV + v] : 0.0;
+    sF[idx] = (k < NDQ && cc < EPB * NV && e < g.E) ? Fv[(static_cast<int64_t>(e) * NDQ + k) * NV + v] : TC(0);
   }
   __syncthreads();
+  #pragma unroll 1
   for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
-    tileAcc<NBm>(acc, op.grad, m0, sF, 0, NDQk, lane);
-    tileAcc<NBm>(acc, op.face, m0, sH, 0, NAQk, lane);
-    tileStore(acc, sR, m0, lane);
+    acc_t<TC> acc = zeroAcc<TC>();
+    tileAcc<TC, NBm>(acc, op.grad, m0, sF, 0, NDQk, lane);
+    tileAcc<TC, NBm>(acc, op.face, m0, sH, 0, NAQk, lane);
+    tileStore<TC>(acc, sR, m0, lane);
   }
   __syncthreads();
   if (a.mode == 1) {
     for (int idx = tid; idx < NB * 16; idx += kThreads) {
       const int b = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
-      if (cc < EPB * NV && e < g.E) a.Rout[(static_cast<int64_t>(e) * NV + v) * NB + b] = sR[idx];
+      if (cc < EPB * NV && e < g.E) {
+        double r = sR[idx];
+        if (sizeof(TC) == 4 && v == NV - 1) r += a.mix.h0 * static_cast<double>(sR[b * 16 + le * NV]);
+        a.Rout[(static_cast<int64_t>(e) * NV + v) * NB + b] = r;
+      }
     }
     return;
   }
-  mfmaStageTail<D, NB, NQ, NF, NFQ, EPB>(g, a, e0, sR, sU, sRed);
+  mfmaStageTail<TC, D, NB, NQ, NF, NFQ, EPB>(g, a, e0, sR, sU, sRed);
 }
 
 }  // namespace sdg

@@ -307,18 +307,68 @@ void mapPoint(const Mesh& mesh, int64_t e, const double* xi, double* x, double* 
     }
   }
 }
+// normals (outward from element e through its local face s) and |J_f| w at the face points, from
+// the element's affine face map (Geometry.cpp:69-86,102-168)
+void faceNormals(const Mesh& mesh, int64_t e, int s, Face& face) {
+  const RefElement& ref = mesh.ref;
+  const int dim = ref.dim, nfq = ref.nfq;
+  face.normal.assign(static_cast<size_t>(nfq) * dim, 0.0);
+  face.jw.assign(nfq, 0.0);
+  const double* J = &mesh.jac[static_cast<size_t>(e) * dim * dim];
+  for (int p = 0; p < nfq; p++) {
+    double ts[3] = {0, 0, 0}, tt[3] = {0, 0, 0};  // d xi / d s, d xi / d t
+    for (int v = 0; v < ref.nfvert; v++) {
+      double nval, g[2];
+      if (dim == 2) {
+        const double fv[1] = {kLineFaceVertex[v]};
+        p1Shape(1, fv, &ref.fpoint[p], nval, g);
+      } else {
+        p1Shape(2, kQuadFaceVertex[v], &ref.fpoint[static_cast<size_t>(p) * 2], nval, g);
+      }
+      const int lv = ref.face_vertex[s * ref.nfvert + v];
+      for (int d = 0; d < dim; d++) {
+        ts[d] += g[0] * ref.vertex[lv * dim + d];
+        if (dim == 3) tt[d] += g[1] * ref.vertex[lv * dim + d];
+      }
+    }
+    double Ts[3] = {0, 0, 0}, Tt[3] = {0, 0, 0};
+    for (int l = 0; l < dim; l++)
+      for (int kk = 0; kk < dim; kk++) {
+        Ts[l] += J[l * dim + kk] * ts[kk];
+        Tt[l] += J[l * dim + kk] * tt[kk];
+      }
+    double nv[3];
+    if (dim == 2) {
+      nv[0] = Ts[1];
+      nv[1] = -Ts[0];
+    } else {
+      nv[0] = Ts[1] * Tt[2] - Ts[2] * Tt[1];
+      nv[1] = Ts[2] * Tt[0] - Ts[0] * Tt[2];
+      nv[2] = Ts[0] * Tt[1] - Ts[1] * Tt[0];
+    }
+    double norm = 0.0;
+    for (int d = 0; d < dim; d++) norm += nv[d] * nv[d];
+    norm = std::sqrt(norm);
+    for (int d = 0; d < dim; d++) face.normal[static_cast<size_t>(p) * dim + d] = nv[d] / norm;
+    face.jw[p] = norm * ref.fweight[p];
+  }
+}
 }  // namespace
 
-bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3], const double hi[3], Mesh& mesh,
-                         std::string& error) {
+bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3], const double hi[3],
+                         const int periodic[3], const int bc[6], Mesh& mesh, std::string& error) {
   mesh = Mesh();
   if (!buildRefElement(type, order, mesh.ref, error)) return false;
   const RefElement& ref = mesh.ref;
   const int dim = ref.dim;
   int nc[3] = {n[0], n[1], dim == 3 ? n[2] : 1};
   for (int d = 0; d < dim; d++) {
-    if (nc[d] < 2) {
-      error = "a periodic direction needs at least 2 cells";
+    if (nc[d] < (periodic[d] ? 2 : 1)) {
+      error = "a periodic direction needs at least 2 cells, a bounded one at least 1";
+      return false;
+    }
+    if (!periodic[d] && (bc[2 * d] < 0 || bc[2 * d] > 5 || bc[2 * d + 1] < 0 || bc[2 * d + 1] > 5)) {
+      error = "a non-periodic side needs a boundary condition type (0..5)";
       return false;
     }
     if (!(hi[d] > lo[d])) {
@@ -405,9 +455,32 @@ bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3]
       for (int i = 0; i < nc[0]; i++) {
         const int64_t e = cellIndex(i, j, k);
         for (int s = 0; s < nf; s++) {
-          if (mesh.elem_face[static_cast<size_t>(e) * nf + s] >= 0) continue;
+          if (mesh.elem_face[static_cast<size_t>(e) * nf + s] != -1) continue;
           const int* nbr = dim == 2 ? kQuadNb[s] : kHexNb[s];
-          const int64_t r = cellIndex(i + nbr[0], j + nbr[1], k + nbr[2]);
+          const int ijk[3] = {i + nbr[0], j + nbr[1], k + nbr[2]};
+          int side_out = -1;  // boundary side 2 d + (0 low, 1 high) when the neighbour is outside
+          for (int d = 0; d < dim; d++)
+            if (!periodic[d] && (ijk[d] < 0 || ijk[d] >= nc[d])) side_out = 2 * d + (ijk[d] < 0 ? 0 : 1);
+          if (side_out >= 0) {
+            Face face;
+            face.parent[0] = static_cast<int32_t>(e);
+            face.parent[1] = -1;
+            face.local[0] = s;
+            face.local[1] = -1;
+            face.bc = bc[side_out];
+            face.seq.resize(nfq);
+            for (int p = 0; p < nfq; p++) {
+              face.seq[p] = p;
+              mapPoint(mesh, e, &ref.face_param[(static_cast<size_t>(s) * nfq + p) * dim], &xl[p * dim], nullptr);
+            }
+            face.coord.assign(xl.begin(), xl.end());
+            faceNormals(mesh, e, s, face);
+            mesh.elem_face[static_cast<size_t>(e) * nf + s] = -2 - static_cast<int32_t>(mesh.bfaces.size());
+            mesh.elem_side[static_cast<size_t>(e) * nf + s] = 0;
+            mesh.bfaces.push_back(std::move(face));
+            continue;
+          }
+          const int64_t r = cellIndex(ijk[0], ijk[1], ijk[2]);
           const int rs = nbr[3];
           Face face;
           face.parent[0] = static_cast<int32_t>(e);
@@ -448,53 +521,13 @@ bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3]
             for (int v = 0; v < ref.nfvert; v++)
               if (vid[static_cast<size_t>(r) * ref.nvert + ref.face_vertex[rs * ref.nfvert + v]] == first) face.rotation = v;
           }
-          // normals and |J_f| w from the left parent's face map
-          face.normal.assign(static_cast<size_t>(nfq) * dim, 0.0);
-          face.jw.assign(nfq, 0.0);
           face.coord.assign(xl.begin(), xl.end());
-          const double* J = &mesh.jac[static_cast<size_t>(e) * dim * dim];
-          for (int p = 0; p < nfq; p++) {
-            double ts[3] = {0, 0, 0}, tt[3] = {0, 0, 0};  // d xi / d s, d xi / d t
-            for (int v = 0; v < ref.nfvert; v++) {
-              double nval, g[2];
-              if (dim == 2) {
-                const double fv[1] = {kLineFaceVertex[v]};
-                p1Shape(1, fv, &ref.fpoint[p], nval, g);
-              } else {
-                p1Shape(2, kQuadFaceVertex[v], &ref.fpoint[static_cast<size_t>(p) * 2], nval, g);
-              }
-              const int lv = ref.face_vertex[s * ref.nfvert + v];
-              for (int d = 0; d < dim; d++) {
-                ts[d] += g[0] * ref.vertex[lv * dim + d];
-                if (dim == 3) tt[d] += g[1] * ref.vertex[lv * dim + d];
-              }
-            }
-            double Ts[3] = {0, 0, 0}, Tt[3] = {0, 0, 0};
-            for (int l = 0; l < dim; l++)
-              for (int kk = 0; kk < dim; kk++) {
-                Ts[l] += J[l * dim + kk] * ts[kk];
-                Tt[l] += J[l * dim + kk] * tt[kk];
-              }
-            double nv[3];
-            if (dim == 2) {
-              nv[0] = Ts[1];
-              nv[1] = -Ts[0];
-            } else {
-              nv[0] = Ts[1] * Tt[2] - Ts[2] * Tt[1];
-              nv[1] = Ts[2] * Tt[0] - Ts[0] * Tt[2];
-              nv[2] = Ts[0] * Tt[1] - Ts[1] * Tt[0];
-            }
-            double norm = 0.0;
-            for (int d = 0; d < dim; d++) norm += nv[d] * nv[d];
-            norm = std::sqrt(norm);
-            for (int d = 0; d < dim; d++) face.normal[static_cast<size_t>(p) * dim + d] = nv[d] / norm;
-            face.jw[p] = norm * ref.fweight[p];
-          }
+          faceNormals(mesh, e, s, face);
           const int32_t fid = static_cast<int32_t>(mesh.faces.size());
           mesh.faces.push_back(std::move(face));
           mesh.elem_face[static_cast<size_t>(e) * nf + s] = fid;
           mesh.elem_side[static_cast<size_t>(e) * nf + s] = 0;
-          if (mesh.elem_face[static_cast<size_t>(r) * nf + rs] >= 0) {
+          if (mesh.elem_face[static_cast<size_t>(r) * nf + rs] != -1) {
             error = "face recorded more than twice";
             return false;
           }

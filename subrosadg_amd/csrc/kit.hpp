@@ -65,9 +65,10 @@ bool buildRefElement(int type, int order, RefElement& ref, std::string& error);
 bool invertMatrix(int n, const double* a, double* inverse);
 
 struct Face {
-  int32_t parent[2] = {0, 0};   // left, right element
+  int32_t parent[2] = {0, 0};   // left, right element (boundary face: right = -1)
   int32_t local[2] = {0, 0};    // local face index in left, right
   int32_t rotation = 0;
+  int32_t bc = -1;              // boundary faces: sdg_boundary type
   std::vector<int32_t> seq;     // N_fq: right point index of left point j
   std::vector<double> normal;   // N_fq x D, outward from the left parent
   std::vector<double> jw;       // N_fq: |J_f| w
@@ -81,16 +82,20 @@ struct Mesh {
   std::vector<double> jac;          // E x D x D, jac[l*D+k] = d x_l / d xi_k (constant: affine)
   std::vector<double> detj;         // E
   std::vector<double> min_edge;     // E
-  std::vector<Face> faces;          // interior faces (periodic mesh: all faces)
-  // element -> face lookup: per element and local face, face index and side (0 left, 1 right)
+  std::vector<Face> faces;          // interior faces
+  std::vector<Face> bfaces;         // boundary faces (the element is the left parent)
+  // element -> face lookup: per element and local face, face index and side (0 left, 1 right);
+  // a boundary face b is stored as -2 - b
   std::vector<int32_t> elem_face;   // E x N_f
   std::vector<int8_t> elem_side;    // E x N_f
   bool affine = true;
 };
 
-// Structured periodic mesh; returns false with a message on a bad descriptor.
+// Structured mesh on an axis-aligned box; periodic[d] != 0 makes direction d periodic, otherwise
+// its two sides are boundaries of type bc[2 d] (low) and bc[2 d + 1] (high). Returns false with
+// a message on a bad descriptor.
 bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3], const double hi[3],
-                         Mesh& mesh, std::string& error);
+                         const int periodic[3], const int bc[6], Mesh& mesh, std::string& error);
 
 // Geometry derived per element and volume point (affine elements: constant Jacobian).
 //   detjw[q]           = |J| w_q

@@ -204,40 +204,36 @@ __device__ void convectiveFlux(const Phys& ph, const T* n, const T* ul, const T*
     const T a1 = fabs(un - c) > delta ? fabs(un - c) : ((un - c) * (un - c) + delta * delta) / (T(2.0) * delta);
     const T a3 = fabs(un + c) > delta ? fabs(un + c) : ((un + c) * (un + c) + delta * delta) / (T(2.0) * delta);
     const T dp = dw[D + 2], drho = dw[0];
-    T col[NV][NV];
+    // dissipation sum over the wave columns, accumulated column by column (the same per-variable
+    // addition order as summing the NV x NV column matrix row-wise, without materialising it)
+    T sum[NV];
 #pragma unroll
-    for (int k = 0; k < NV; k++)
-#pragma unroll
-      for (int v = 0; v < NV; v++) col[k][v] = T(0);
+    for (int v = 0; v < NV; v++) sum[v] = T(0);
     const T s0 = a1 * (dp - rho * c * dun) / (T(2.0) * c * c);
-    col[0][0] = T(1.0) * s0;
+    sum[0] += T(1.0) * s0;
 #pragma unroll
-    for (int d = 0; d < D; d++) col[0][1 + d] = (vel[d] - c * n[d]) * s0;
-    col[0][D + 1] = (htot - c * un) * s0;
+    for (int d = 0; d < D; d++) sum[1 + d] += (vel[d] - c * n[d]) * s0;
+    sum[D + 1] += (htot - c * un) * s0;
     const T s1 = fabs(un) * (drho - dp / (c * c));
-    col[1][0] = T(1.0) * s1;
+    sum[0] += T(1.0) * s1;
 #pragma unroll
-    for (int d = 0; d < D; d++) col[1][1 + d] = vel[d] * s1;
-    col[1][D + 1] = v2 / T(2.0) * s1;
+    for (int d = 0; d < D; d++) sum[1 + d] += vel[d] * s1;
+    sum[D + 1] += v2 / T(2.0) * s1;
     if (D >= 2) {
       const T s2 = fabs(un) * rho;
-      col[2][0] = T(0.0) * s2;
+      sum[0] += T(0.0) * s2;
 #pragma unroll
-      for (int d = 0; d < D; d++) col[2][1 + d] = (dw[1 + d] - dun * n[d]) * s2;
-      col[2][D + 1] = (vdv - un * dun) * s2;
+      for (int d = 0; d < D; d++) sum[1 + d] += (dw[1 + d] - dun * n[d]) * s2;
+      sum[D + 1] += (vdv - un * dun) * s2;
     }
+    // in 3-D the fourth column stays zero (ConvectiveFlux.cpp:240-350); adding it changes nothing
     const T s3 = a3 * (dp + rho * c * dun) / (T(2.0) * c * c);
-    col[D + 1][0] = T(1.0) * s3;
+    sum[0] += T(1.0) * s3;
 #pragma unroll
-    for (int d = 0; d < D; d++) col[D + 1][1 + d] = (vel[d] + c * n[d]) * s3;
-    col[D + 1][D + 1] = (htot + c * un) * s3;
+    for (int d = 0; d < D; d++) sum[1 + d] += (vel[d] + c * n[d]) * s3;
+    sum[D + 1] += (htot + c * un) * s3;
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-      T s = T(0);
-#pragma unroll
-      for (int k = 0; k < NV; k++) s += col[k][v];
-      h[v] = ((fl[v] + fr[v]) - s) / T(2.0);
-    }
+    for (int v = 0; v < NV; v++) h[v] = ((fl[v] + fr[v]) - sum[v]) / T(2.0);
   } else {  // Exact flux of the weakly-compressible model :352-414
     const T c = soundSpeedOf(ph, T(0), T(0));
     T unl = T(0), unr = T(0);
@@ -290,6 +286,110 @@ __device__ __forceinline__ void viscousRawFlux(const Phys& ph, const T* w, const
 #pragma unroll
     for (int b = 0; b < D; b++) s += tau[a + D * b] * w[1 + b];
     fv[a + D * (D + 1)] = s + kap * tg[a];
+  }
+}
+
+// ---- boundary conditions (BoundaryCondition.cpp:79-547) ---------------------------------------
+// Computational variables w = (rho, u.., e, p). wl: interior state at the boundary point, wd: the
+// boundary data (boundary_dummy_variable_), n: outward normal. Returns the boundary state wb of
+// calculateBoundaryVariable for type bc (sdg_boundary order).
+template <int D>
+__device__ __forceinline__ void boundaryState(const Phys& ph, int bc, const double* n, const double* wl,
+                                              const double* wd, double* wb) {
+  constexpr int NW = D + 3;
+  double unl = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; d++) unl += wl[1 + d] * n[d];
+  const double cl = soundSpeedOf(ph, wl[0], wl[D + 2]);
+  const double mn = unl / cl;
+  if (bc == 0) {  // Riemann-invariant farfield :79-247 (compressible branch)
+    if (fabs(mn) > 1.0) {  // supersonic: all characteristics from one side
+      const bool in = mn < 0.0;
+#pragma unroll
+      for (int i = 0; i < NW; i++) wb[i] = in ? wd[i] : wl[i];
+      return;
+    }
+    const double g = kGamma;
+    double und = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) und += wd[1 + d] * n[d];
+    const double cd = soundSpeedOf(ph, wd[0], wd[D + 2]);
+    const double rm = und - 2.0 * cd / (g - 1.0);  // invariant carried in from outside
+    const double rp = unl + 2.0 * cl / (g - 1.0);  // invariant carried out from inside
+    const double unb = (rm + rp) / 2.0;
+    const bool in = mn < 0.0;  // subsonic inflow: tangential velocity and entropy from outside
+    double uns = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) uns += (in ? wd[1 + d] : wl[1 + d]) * n[d];
+#pragma unroll
+    for (int d = 0; d < D; d++) wb[1 + d] = (in ? wd[1 + d] : wl[1 + d]) + (unb - uns) * n[d];
+    const double cb = (g - 1.0) * (rp - rm) / 4.0;
+    const double rs = in ? wd[0] : wl[0], ps = in ? wd[D + 2] : wl[D + 2];
+    const double sb = ps / pow(rs, g);  // calculateEntropyFromDensityPressure (PhysicalModel.cpp:148-150)
+    const double rb = pow(cb * cb / (g * sb), 1.0 / (g - 1.0));
+    const double pb = rb * cb * cb / g;
+    wb[0] = rb;
+    wb[D + 1] = pb / ((g - 1.0) * rb);
+    wb[D + 2] = pb;
+  } else if (bc == 1) {  // velocity inflow :309-333
+#pragma unroll
+    for (int i = 0; i < NW; i++) wb[i] = wd[i];
+    if (mn > -1.0) wb[D + 2] = wl[D + 2];
+  } else if (bc == 2) {  // pressure outflow :362-386
+#pragma unroll
+    for (int i = 0; i < NW; i++) wb[i] = wl[i];
+    if (mn < 1.0) wb[D + 2] = wd[D + 2];
+  } else if (bc == 3) {  // isothermal no-slip wall :416-434
+    wb[0] = wl[0];
+#pragma unroll
+    for (int d = 0; d < D; d++) wb[1 + d] = wd[1 + d];
+    wb[D + 1] = wd[D + 1];
+    wb[D + 2] = pressureOf(ph, wl[0], wd[D + 1]);
+  } else if (bc == 4) {  // adiabatic slip wall :474-488
+#pragma unroll
+    for (int i = 0; i < NW; i++) wb[i] = wl[i];
+#pragma unroll
+    for (int d = 0; d < D; d++) wb[1 + d] = wl[1 + d] - unl * n[d];
+  } else {  // adiabatic no-slip wall :521-533
+#pragma unroll
+    for (int i = 0; i < NW; i++) wb[i] = wl[i];
+#pragma unroll
+    for (int d = 0; d < D; d++) wb[1 + d] = wd[1 + d];
+  }
+}
+
+// computational -> conserved (VariableConvertor.cpp:341-366 on computational input)
+template <int D>
+__device__ __forceinline__ void conservedOf(const double* w, double* u) {
+  u[0] = w[0];
+#pragma unroll
+  for (int d = 0; d < D; d++) u[1 + d] = w[0] * w[1 + d];
+  u[D + 1] = w[0] * (w[D + 1] + velocitySquared<D>(w) / 2.0);
+}
+
+// States of the gradient pass at a boundary point (calculateBoundaryGradientVariable): the value
+// entering the volume-gradient face term and the jump entering the interface term. Farfield,
+// inflow and outflow: (u-, 0); walls: (u_b, u_b - u-) with u_b the conserved boundary state.
+template <int D>
+__device__ __forceinline__ void boundaryGradientStates(const Phys& ph, int bc, const double* n, const double* ul,
+                                                       const double* wd, double* uvg, double* uig) {
+  constexpr int NV = D + 2;
+  if (bc <= 2) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      uvg[v] = ul[v];
+      uig[v] = 0.0;
+    }
+    return;
+  }
+  double wl[D + 3], wb[D + 3], ub[NV];
+  computational<D>(ph, ul, wl);
+  boundaryState<D>(ph, bc, n, wl, wd, wb);
+  conservedOf<D>(wb, ub);
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    uvg[v] = ub[v];
+    uig[v] = ub[v] - ul[v];
   }
 }
 

@@ -43,7 +43,11 @@ int sdg_mesh_create_structured(const sdg_structured_desc* desc, sdg_mesh** out) 
   }
   std::string err;
   try {
-    if (!sdg::buildStructuredMesh(desc->element, desc->polynomial_order, desc->n, desc->lo, desc->hi, m->mesh, err)) {
+    int periodic[3], bc[6];
+    for (int d = 0; d < 3; d++) periodic[d] = desc->bounded[d] ? 0 : 1;
+    for (int i = 0; i < 6; i++) bc[i] = desc->boundary[i];
+    if (!sdg::buildStructuredMesh(desc->element, desc->polynomial_order, desc->n, desc->lo, desc->hi, periodic, bc,
+                                  m->mesh, err)) {
       delete m;
       sdg::setGlobalError("sdg_mesh_create_structured: " + err);
       return SDG_E_INVALID;
@@ -75,13 +79,14 @@ int sdg_mesh_get_info(const sdg_mesh* mesh, sdg_mesh_info* info) {
   info->faces_per_element = m.ref.nf;
   info->face_quadrature = m.ref.nfq;
   info->affine = m.affine ? 1 : 0;
+  info->boundary_faces = static_cast<int64_t>(m.bfaces.size());
   return SDG_OK;
 }
 
 int64_t sdg_mesh_field_size(const sdg_mesh* mesh, int field) {
   if (!mesh) return 0;
   const sdg::Mesh& m = mesh->mesh;
-  const int64_t E = m.elements, F = static_cast<int64_t>(m.faces.size());
+  const int64_t E = m.elements, F = static_cast<int64_t>(m.faces.size()), Fb = static_cast<int64_t>(m.bfaces.size());
   const int64_t D = m.ref.dim, nb = m.ref.nb, nq = m.ref.nq, nf = m.ref.nf, nfq = m.ref.nfq;
   switch (field) {
     case SDG_FIELD_BASIS_VALUE: return nq * nb;
@@ -104,12 +109,19 @@ int64_t sdg_mesh_field_size(const sdg_mesh* mesh, int field) {
     case SDG_FIELD_FACE_ROTATION: return F;
     case SDG_FIELD_QUAD_POINT: return nq * D;
     case SDG_FIELD_FACE_QUAD_POINT: return nfq * (D - 1);
+    case SDG_FIELD_BOUNDARY_FACE_COORD: return Fb * nfq * D;
+    case SDG_FIELD_BOUNDARY_FACE_NORMAL: return Fb * nfq * D;
+    case SDG_FIELD_BOUNDARY_FACE_JACOBIAN_W: return Fb * nfq;
+    case SDG_FIELD_BOUNDARY_FACE_PARENT: return Fb * 2;
+    case SDG_FIELD_BOUNDARY_FACE_TYPE: return Fb;
     default: return 0;
   }
 }
 
 int sdg_mesh_export(const sdg_mesh* mesh, int field, void* out, int64_t count) {
   const int64_t need = sdg_mesh_field_size(mesh, field);
+  const bool boundary_field = field >= SDG_FIELD_BOUNDARY_FACE_COORD && field <= SDG_FIELD_BOUNDARY_FACE_TYPE;
+  if (mesh && boundary_field && need == 0) return SDG_OK;  // no boundary faces
   if (!mesh || !out || need == 0) {
     sdg::setGlobalError("sdg_mesh_export: null argument or unknown field");
     return SDG_E_INVALID;
@@ -176,6 +188,26 @@ int sdg_mesh_export(const sdg_mesh* mesh, int field, void* out, int64_t count) {
       break;
     case SDG_FIELD_FACE_ROTATION:
       for (size_t f = 0; f < m.faces.size(); f++) oi[f] = m.faces[f].rotation;
+      break;
+    case SDG_FIELD_BOUNDARY_FACE_COORD:
+      for (size_t f = 0; f < m.bfaces.size(); f++)
+        std::memcpy(od + f * nfq * D, m.bfaces[f].coord.data(), sizeof(double) * nfq * D);
+      break;
+    case SDG_FIELD_BOUNDARY_FACE_NORMAL:
+      for (size_t f = 0; f < m.bfaces.size(); f++)
+        std::memcpy(od + f * nfq * D, m.bfaces[f].normal.data(), sizeof(double) * nfq * D);
+      break;
+    case SDG_FIELD_BOUNDARY_FACE_JACOBIAN_W:
+      for (size_t f = 0; f < m.bfaces.size(); f++) std::memcpy(od + f * nfq, m.bfaces[f].jw.data(), sizeof(double) * nfq);
+      break;
+    case SDG_FIELD_BOUNDARY_FACE_PARENT:
+      for (size_t f = 0; f < m.bfaces.size(); f++) {
+        oi[2 * f] = m.bfaces[f].parent[0];
+        oi[2 * f + 1] = m.bfaces[f].local[0];
+      }
+      break;
+    case SDG_FIELD_BOUNDARY_FACE_TYPE:
+      for (size_t f = 0; f < m.bfaces.size(); f++) oi[f] = m.bfaces[f].bc;
       break;
     default:
       sdg::setGlobalError("sdg_mesh_export: unknown field");

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/sdg.h"
+#include "dg_adjoint.hpp"
 #include "dg_kernels.hpp"
 #include "dg_mfma.hpp"
 #include "kit.hpp"
@@ -27,9 +28,10 @@ using sdg::StageArgs;
 namespace {
 
 using StageFn = void (*)(hipStream_t, const Geo&, const Phys&, const StageArgs&);
-using TraceFn = void (*)(hipStream_t, const Geo&, const double*, double*);
+using TraceFn = void (*)(hipStream_t, const Geo&, const double*, void*, const sdg::MixRef&);
 using DtFn = void (*)(hipStream_t, const Geo&, const Phys&, const double*, const double*, double, int, double*);
 using MonFn = void (*)(hipStream_t, const double*, int, double*, int*);
+using AdjFn = void (*)(hipStream_t, const Geo&, const Phys&, const StageArgs&, const sdg::AdjArgs&);
 
 struct Launchers {
   bool mfma = false;
@@ -41,12 +43,18 @@ struct Launchers {
   TraceFn trace = nullptr;
   DtFn dt = nullptr;
   MonFn monitor = nullptr;
+  // discrete adjoint (Navier-Stokes, FP64 matrix-core sizes only)
+  AdjFn adj_volume[2] = {nullptr, nullptr};  // BR1, BR2
+  AdjFn adj_face = nullptr;
+  AdjFn adj_lift[2] = {nullptr, nullptr};
+  AdjFn adj_gather = nullptr;
 };
 
 inline int blocksFor(int E, int epb) { return (E + epb - 1) / epb; }
 
-// VALU kernels: any size; MFMA kernels: bases that fill 16x16 tiles
-template <int D, int NB, int NQ, int NF, int NFQ, bool MFMA>
+// VALU kernels: any size; MFMA kernels: bases that fill 16x16 tiles. TC is the contraction type of
+// the matrix-core kernels: double (FP64 mode) or float (mixed-precision mode).
+template <int D, int NB, int NQ, int NF, int NFQ, bool MFMA, typename TC = double>
 Launchers makeLaunchers() {
   using S = sdg::Sizes<D, NB, NQ, NF, NFQ>;
   constexpr int EPB = S::EPB;
@@ -56,25 +64,51 @@ Launchers makeLaunchers() {
   if constexpr (MFMA) {
     constexpr int EPB_S = 16 / NV, EPB_G = 16 / (NV * D);
     l.euler = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-      hipLaunchKernelGGL((sdg::eulerStageMfma<D, NB, NQ, NF, NFQ>), dim3(blocksFor(g.E, EPB_S)),
+      hipLaunchKernelGGL((sdg::eulerStageMfma<TC, D, NB, NQ, NF, NFQ>), dim3(blocksFor(g.E, EPB_S)),
                          dim3(sdg::kThreads), 0, st, g, ph, a);
     };
     l.ns_grad_br1 = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-      hipLaunchKernelGGL((sdg::nsGradMfma<D, NB, NQ, NF, NFQ, 1>), dim3(blocksFor(g.E, EPB_G)), dim3(sdg::kThreads),
+      hipLaunchKernelGGL((sdg::nsGradMfma<TC, D, NB, NQ, NF, NFQ, 1>), dim3(blocksFor(g.E, EPB_G)), dim3(sdg::kThreads),
                          0, st, g, ph, a);
     };
     l.ns_grad_br2 = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-      hipLaunchKernelGGL((sdg::nsGradMfma<D, NB, NQ, NF, NFQ, 2>), dim3(blocksFor(g.E, EPB_G)), dim3(sdg::kThreads),
+      hipLaunchKernelGGL((sdg::nsGradMfma<TC, D, NB, NQ, NF, NFQ, 2>), dim3(blocksFor(g.E, EPB_G)), dim3(sdg::kThreads),
                          0, st, g, ph, a);
     };
     l.ns_face = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-      hipLaunchKernelGGL((sdg::nsFaceFluxKernel<D, NF, NFQ>), dim3(blocksFor(g.F * NFQ, sdg::kThreads)),
+      hipLaunchKernelGGL((sdg::nsFaceFluxKernel<TC, D, NF, NFQ>), dim3(blocksFor((g.F + g.Fb) * NFQ, sdg::kThreads)),
                          dim3(sdg::kThreads), 0, st, g, ph, a);
     };
     l.ns_flux = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-      hipLaunchKernelGGL((sdg::nsResidualMfma<D, NB, NQ, NF, NFQ>), dim3(blocksFor(g.E, EPB_S)), dim3(sdg::kThreads),
+      hipLaunchKernelGGL((sdg::nsResidualMfma<TC, D, NB, NQ, NF, NFQ>), dim3(blocksFor(g.E, EPB_S)), dim3(sdg::kThreads),
                          0, st, g, ph, a);
     };
+    if constexpr (sizeof(TC) == 8) {
+      l.adj_volume[0] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+        hipLaunchKernelGGL((sdg::nsAdjVolumeMfma<D, NB, NQ, NF, NFQ, 1>), dim3(blocksFor(g.E, EPB_G)),
+                           dim3(sdg::kThreads), 0, st, g, ph, a, j);
+      };
+      l.adj_volume[1] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+        hipLaunchKernelGGL((sdg::nsAdjVolumeMfma<D, NB, NQ, NF, NFQ, 2>), dim3(blocksFor(g.E, EPB_G)),
+                           dim3(sdg::kThreads), 0, st, g, ph, a, j);
+      };
+      l.adj_face = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+        hipLaunchKernelGGL((sdg::nsAdjFaceKernel<D, NF, NFQ>), dim3(blocksFor(g.F * NFQ, sdg::kThreads)),
+                           dim3(sdg::kThreads), 0, st, g, ph, a, j);
+      };
+      l.adj_lift[0] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+        hipLaunchKernelGGL((sdg::nsAdjLiftMfma<D, NB, NQ, NF, NFQ, 1>), dim3(blocksFor(g.E, EPB_G)),
+                           dim3(sdg::kThreads), 0, st, g, ph, a, j);
+      };
+      l.adj_lift[1] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+        hipLaunchKernelGGL((sdg::nsAdjLiftMfma<D, NB, NQ, NF, NFQ, 2>), dim3(blocksFor(g.E, EPB_G)),
+                           dim3(sdg::kThreads), 0, st, g, ph, a, j);
+      };
+      l.adj_gather = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+        hipLaunchKernelGGL((sdg::nsAdjGatherMfma<D, NB, NQ, NF, NFQ>), dim3(blocksFor(g.E, EPB_S)),
+                           dim3(sdg::kThreads), 0, st, g, ph, a, j);
+      };
+    }
   } else {
     l.euler = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
       hipLaunchKernelGGL((sdg::eulerStageKernel<D, NB, NQ, NF, NFQ, EPB>), dim3(blocksFor(g.E, EPB)),
@@ -93,10 +127,10 @@ Launchers makeLaunchers() {
                          0, st, g, ph, a);
     };
   }
-  l.trace = [](hipStream_t st, const Geo& g, const double* U, double* Uf) {
+  l.trace = [](hipStream_t st, const Geo& g, const double* U, void* Uf, const sdg::MixRef& mix) {
     const int64_t n = static_cast<int64_t>(g.E) * (D + 2) * NF * NFQ;
     hipLaunchKernelGGL((sdg::traceKernel<D, NB, NQ, NF, NFQ>), dim3((n + sdg::kThreads - 1) / sdg::kThreads),
-                       dim3(sdg::kThreads), 0, st, g, U, Uf);
+                       dim3(sdg::kThreads), 0, st, g, U, Uf, mix);
   };
   l.dt = [](hipStream_t st, const Geo& g, const Phys& ph, const double* U, const double* me, double cfl, int order,
             double* out) {
@@ -109,35 +143,39 @@ Launchers makeLaunchers() {
   return l;
 }
 
-bool selectLaunchers(int type, int order, bool force_valu, Launchers& l) {
+// Kernel set for (element type, order). Mixed precision runs on the matrix-core kernels only.
+template <int D, int NB, int NQ, int NF, int NFQ>
+bool pickMfma(bool force_valu, bool mixed, Launchers& l) {
+  if (mixed)
+    l = makeLaunchers<D, NB, NQ, NF, NFQ, true, float>();
+  else
+    l = force_valu ? makeLaunchers<D, NB, NQ, NF, NFQ, false>() : makeLaunchers<D, NB, NQ, NF, NFQ, true>();
+  return true;
+}
+template <int D, int NB, int NQ, int NF, int NFQ>
+bool pickValu(bool mixed, Launchers& l) {
+  if (mixed) return false;
+  l = makeLaunchers<D, NB, NQ, NF, NFQ, false>();
+  return true;
+}
+
+bool selectLaunchers(int type, int order, bool force_valu, bool mixed, Launchers& l) {
   if (type == SDG_ELEM_QUADRANGLE) {
     switch (order) {
-      case 1: l = makeLaunchers<2, 4, 4, 4, 2, false>(); return true;
-      case 2: l = makeLaunchers<2, 9, 9, 4, 3, false>(); return true;
-      case 3:
-        l = force_valu ? makeLaunchers<2, 16, 16, 4, 4, false>() : makeLaunchers<2, 16, 16, 4, 4, true>();
-        return true;
-      case 4:
-        l = force_valu ? makeLaunchers<2, 25, 25, 4, 5, false>() : makeLaunchers<2, 25, 25, 4, 5, true>();
-        return true;
-      case 5:
-        l = force_valu ? makeLaunchers<2, 36, 36, 4, 6, false>() : makeLaunchers<2, 36, 36, 4, 6, true>();
-        return true;
+      case 1: return pickValu<2, 4, 4, 4, 2>(mixed, l);
+      case 2: return pickValu<2, 9, 9, 4, 3>(mixed, l);
+      case 3: return pickMfma<2, 16, 16, 4, 4>(force_valu, mixed, l);
+      case 4: return pickMfma<2, 25, 25, 4, 5>(force_valu, mixed, l);
+      case 5: return pickMfma<2, 36, 36, 4, 6>(force_valu, mixed, l);
       default: return false;
     }
   }
   if (type == SDG_ELEM_HEXAHEDRON) {
     switch (order) {
-      case 1: l = makeLaunchers<3, 8, 8, 6, 4, false>(); return true;
-      case 2:
-        l = force_valu ? makeLaunchers<3, 27, 27, 6, 9, false>() : makeLaunchers<3, 27, 27, 6, 9, true>();
-        return true;
-      case 3:
-        l = force_valu ? makeLaunchers<3, 64, 64, 6, 16, false>() : makeLaunchers<3, 64, 64, 6, 16, true>();
-        return true;
-      case 4:
-        l = force_valu ? makeLaunchers<3, 125, 125, 6, 25, false>() : makeLaunchers<3, 125, 125, 6, 25, true>();
-        return true;
+      case 1: return pickValu<3, 8, 8, 6, 4>(mixed, l);
+      case 2: return pickMfma<3, 27, 27, 6, 9>(force_valu, mixed, l);
+      case 3: return pickMfma<3, 64, 64, 6, 16>(force_valu, mixed, l);
+      case 4: return pickMfma<3, 125, 125, 6, 25>(force_valu, mixed, l);
       default: return false;
     }
   }
@@ -174,14 +212,35 @@ struct sdg_ctx {
   double *U = nullptr, *Un = nullptr, *Uf[2] = {nullptr, nullptr}, *Gf = nullptr, *Rvol = nullptr, *Hs = nullptr, *Fv = nullptr;
   double *Rout = nullptr, *Wout = nullptr, *mon = nullptr, *rel = nullptr, *dtbuf = nullptr;
   int* status = nullptr;
+  // boundary faces: count and boundary data (computational variables per boundary face point)
+  int64_t Fb = 0;
+  double* bdata = nullptr;
+  bool has_bdata = false;
+  // mixed-precision mode: FP32 work buffers and the reference state of the deviation form
+  bool mixed = false;
+  float *Uf32[2] = {nullptr, nullptr}, *Gf32 = nullptr, *Hs32 = nullptr, *Fv32 = nullptr;
+  double* one = nullptr;            // device: modal coefficients of the constant function 1
+  std::vector<double> h_one, h_mean;  // host: the same; reference-element mean of each basis function
+  sdg::MixRef mix{};
   int cur = 0;
   bool has_state = false;
+  // discrete adjoint: transposed operator tables and lazily allocated work buffers
+  sdg::AdjOps adj_ops{};
+  bool adj_ready = false;
+  double *adj_lam = nullptr, *adj_Hb = nullptr, *adj_RtV = nullptr, *adj_Ua = nullptr, *adj_UFf = nullptr;
+  double *adj_GFb = nullptr, *adj_Pown = nullptr, *adj_Qnb = nullptr, *adj_out = nullptr;
+  double *adj_S[3] = {nullptr, nullptr, nullptr}, *adj_acc = nullptr, *adj_tmp = nullptr;
+  std::vector<double*> snaps;   // checkpoint states (device layout)
+  std::vector<int> snap_step;   // step index held by each snapshot, -1 when free
+  int adj_nsteps = -1, adj_nslots = 0;
+  double adj_dt = 0.0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   double graph_dt[2] = {0.0, 0.0};
   bool use_graph = true;
   float last_ms = 0.0f;
 
   bool ns() const { return prob.equation == SDG_EQ_COMPRESSIBLE_NS; }
+  void* traces(int i) const { return mixed ? static_cast<void*>(Uf32[i]) : static_cast<void*>(Uf[i]); }
   int stages() const {
     return prob.time_integration == SDG_RK_FORWARD_EULER ? 1 : (prob.time_integration == SDG_RK_HEUN2 ? 2 : 3);
   }
@@ -256,6 +315,12 @@ StageArgs baseArgs(sdg_ctx* ctx) {
   a.Wout = nullptr;
   a.Hs = ctx->Hs;
   a.Fv = ctx->Fv;
+  a.Uf_in32 = ctx->Uf32[ctx->cur];
+  a.Uf_out32 = ctx->Uf32[ctx->cur ^ 1];
+  a.Gf32 = ctx->Gf32;
+  a.Hs32 = ctx->Hs32;
+  a.Fv32 = ctx->Fv32;
+  a.mix = ctx->mix;
   a.mon = ctx->mon;
   a.status = ctx->status;
   return a;
@@ -318,6 +383,30 @@ void toReference(const sdg_ctx* ctx, const std::vector<double>& dev, double* ref
       for (int v = 0; v < nv; v++) ref[(e * nb + b) * nv + v] = dev[(e * nv + v) * nb + b];
 }
 
+// Mixed-precision reference state: the domain mean of the conserved variables (mean over elements
+// of the element means) and its pressure.
+void updateReference(sdg_ctx* ctx, const double* ref) {
+  sdg::MixRef& m = ctx->mix;
+  m = sdg::MixRef{};
+  m.on = ctx->mixed ? 1 : 0;
+  m.one = ctx->one;
+  if (!ctx->mixed) return;
+  const int nv = ctx->nv, nb = ctx->nb, D = ctx->dim;
+  for (int v = 0; v < nv; v++) {
+    double s = 0.0;
+    for (int64_t e = 0; e < ctx->E; e++)
+      for (int b = 0; b < nb; b++) s += ref[(e * nb + b) * nv + v] * ctx->h_mean[b];
+    m.uref[v] = s / static_cast<double>(ctx->E);
+  }
+  const double rho = m.uref[0];
+  double v2 = 0.0;
+  for (int d = 0; d < D; d++) v2 += (m.uref[1 + d] / rho) * (m.uref[1 + d] / rho);
+  const double e = m.uref[D + 1] / rho - v2 / 2.0;
+  const sdg::Phys& ph = ctx->phys;
+  m.pref = ph.eos == SDG_EOS_IDEAL_GAS ? (sdg::kGamma - 1.0) * rho * e : ph.c0 * ph.c0 * (rho - ph.rho0) + ph.padd;
+  m.h0 = (m.uref[D + 1] + m.pref) / rho;
+}
+
 int validateProblem(const sdg_problem_desc* p, std::string& msg) {
   if (p->abi_version != SDG_ABI_VERSION) {
     msg = "abi_version mismatch";
@@ -352,9 +441,9 @@ int validateProblem(const sdg_problem_desc* p, std::string& msg) {
     msg = "specific heats must be positive";
     return SDG_E_INVALID;
   }
-  if (p->precision != SDG_PREC_FP64) {
-    msg = "only the FP64 precision mode is implemented in this build";
-    return SDG_E_UNSUPPORTED;
+  if (p->precision != SDG_PREC_FP64 && p->precision != SDG_PREC_MIXED) {
+    msg = "unknown precision mode";
+    return SDG_E_INVALID;
   }
   return SDG_OK;
 }
@@ -391,9 +480,14 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   ctx->E = static_cast<int>(mesh.elements);
   ctx->F = static_cast<int64_t>(mesh.faces.size());
   const char* fv = std::getenv("SDG_FORCE_VALU");
-  if (!selectLaunchers(r.type, r.order, fv && fv[0] == '1', ctx->launch)) {
+  const bool mixed = problem->precision == SDG_PREC_MIXED;
+  ctx->mixed = mixed;
+  if (!selectLaunchers(r.type, r.order, fv && fv[0] == '1', mixed, ctx->launch)) {
     delete ctx;
-    return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: no kernel instantiation for this element type and order");
+    return fail(nullptr, SDG_E_UNSUPPORTED,
+                mixed ? "sdg_create: the mixed-precision mode needs a matrix-core kernel set (quadrangle P3-P5, "
+                             "hexahedron P2-P4)"
+                           : "sdg_create: no kernel instantiation for this element type and order");
   }
   Phys& ph = ctx->phys;
   ph.eos = problem->equation_of_state;
@@ -474,50 +568,63 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     floc[2 * f] = mesh.faces[f].local[0];
     floc[2 * f + 1] = mesh.faces[f].local[1];
   }
-  std::vector<int> efid(static_cast<size_t>(E) * nf), enb(static_cast<size_t>(E) * nf), enbf(static_cast<size_t>(E) * nf),
-      eside(static_cast<size_t>(E) * nf);
-  for (int64_t e = 0; e < E; e++)
-    for (int s = 0; s < nf; s++) {
-      const int fid = mesh.elem_face[static_cast<size_t>(e) * nf + s];
-      const int side = mesh.elem_side[static_cast<size_t>(e) * nf + s];
-      if (fid < 0) {
-        ctx->err = "sdg_create: element face without a neighbour (boundary faces are not supported)";
-        return bail(SDG_E_UNSUPPORTED);
-      }
-      const sdg::Face& fc = mesh.faces[fid];
-      efid[static_cast<size_t>(e) * nf + s] = fid;
-      eside[static_cast<size_t>(e) * nf + s] = side;
-      enb[static_cast<size_t>(e) * nf + s] = fc.parent[side ^ 1];
-      enbf[static_cast<size_t>(e) * nf + s] = fc.local[side ^ 1];
-    }
   // per element slot: neighbour slot and (face point, side), so a kernel reaches the neighbour's
-  // trace with one indexed load (no face -> permutation -> slot chain)
-  if (static_cast<int64_t>(E) * naq >= (int64_t(1) << 31) || F * nfq >= (int64_t(1) << 30)) {
+  // trace with one indexed load (no face -> permutation -> slot chain); a slot on boundary face b,
+  // point j has snb = -1 - (b N_fq + j)
+  const int64_t Fb = static_cast<int64_t>(mesh.bfaces.size());
+  if (static_cast<int64_t>(E) * naq >= (int64_t(1) << 31) || (F + Fb) * nfq >= (int64_t(1) << 30)) {
     ctx->err = "sdg_create: mesh too large for 32-bit slot indices";
     return bail(SDG_E_UNSUPPORTED);
   }
   std::vector<int> snb(static_cast<size_t>(E) * naq), sfp(static_cast<size_t>(E) * naq);
   for (int64_t e = 0; e < E; e++)
     for (int s = 0; s < nf; s++) {
-      const int fid = efid[static_cast<size_t>(e) * nf + s];
-      const int side = eside[static_cast<size_t>(e) * nf + s];
+      const int fid = mesh.elem_face[static_cast<size_t>(e) * nf + s];
+      const int side = mesh.elem_side[static_cast<size_t>(e) * nf + s];
+      if (fid == -1) {
+        ctx->err = "sdg_create: element face without a neighbour or boundary condition";
+        return bail(SDG_E_INVALID);
+      }
       for (int j = 0; j < nfq; j++) {
+        const size_t o = static_cast<size_t>(e) * naq + s * nfq + j;
+        if (fid <= -2) {
+          snb[o] = -1 - ((-2 - fid) * nfq + j);
+          sfp[o] = 0;
+          continue;
+        }
+        const sdg::Face& fc = mesh.faces[fid];
         const int p = side == 0 ? j : fseqinv[static_cast<size_t>(fid) * nfq + j];
         const int k = side == 0 ? fseq[static_cast<size_t>(fid) * nfq + j] : p;
-        snb[static_cast<size_t>(e) * naq + s * nfq + j] = enb[static_cast<size_t>(e) * nf + s] * naq + enbf[static_cast<size_t>(e) * nf + s] * nfq + k;
-        sfp[static_cast<size_t>(e) * naq + s * nfq + j] = (fid * nfq + p) * 2 + side;
+        snb[o] = fc.parent[side ^ 1] * naq + fc.local[side ^ 1] * nfq + k;
+        sfp[o] = (fid * nfq + p) * 2 + side;
       }
     }
+  // boundary face points
+  std::vector<double> bnormal(static_cast<size_t>(std::max<int64_t>(Fb, 1)) * nfq * D), bjw(static_cast<size_t>(std::max<int64_t>(Fb, 1)) * nfq);
+  std::vector<int> btype(bjw.size()), bslot(bjw.size());
+  for (int64_t b = 0; b < Fb; b++) {
+    const sdg::Face& fc = mesh.bfaces[b];
+    for (int j = 0; j < nfq; j++) {
+      const size_t bp = static_cast<size_t>(b) * nfq + j;
+      for (int d = 0; d < D; d++) bnormal[bp * D + d] = fc.normal[static_cast<size_t>(j) * D + d];
+      bjw[bp] = fc.jw[j];
+      btype[bp] = fc.bc;
+      bslot[bp] = fc.parent[0] * naq + fc.local[0] * nfq + j;
+    }
+  }
+  ctx->Fb = Fb;
   Geo& g = ctx->geo;
   double *dVt, *ddV, *dVft, *dVf, *dMinv, *dLf, *dwq, *dginv, *ddetj, *dnormal, *djw, *dme;
-  int *dseq, *dseqinv, *defid, *denb, *denbf, *deside, *dfpar, *dfloc, *dsnb, *dsfp;
+  int *dseq, *dseqinv, *dfpar, *dfloc, *dsnb, *dsfp, *dbtype, *dbslot;
+  double *dbnormal, *dbjw;
   if (upload(ctx, Vt, &dVt) || upload(ctx, r.dV, &ddV) || upload(ctx, Vft, &dVft) || upload(ctx, r.Vf, &dVf) ||
       upload(ctx, r.mass_inverse, &dMinv) || upload(ctx, Lf, &dLf) || upload(ctx, r.qweight, &dwq) ||
       upload(ctx, ginv, &dginv) || upload(ctx, detj, &ddetj) || upload(ctx, fnormal, &dnormal) ||
       upload(ctx, fjw, &djw) || upload(ctx, fseq, &dseq) || upload(ctx, fseqinv, &dseqinv) ||
-      upload(ctx, efid, &defid) || upload(ctx, enb, &denb) || upload(ctx, enbf, &denbf) ||
-      upload(ctx, eside, &deside) || upload(ctx, mesh.min_edge, &dme) || upload(ctx, fpar, &dfpar) ||
-      upload(ctx, floc, &dfloc) || upload(ctx, snb, &dsnb) || upload(ctx, sfp, &dsfp))
+      upload(ctx, mesh.min_edge, &dme) || upload(ctx, fpar, &dfpar) || upload(ctx, floc, &dfloc) ||
+      upload(ctx, snb, &dsnb) || upload(ctx, sfp, &dsfp) || upload(ctx, bnormal, &dbnormal) || upload(ctx, bjw, &dbjw) ||
+      upload(ctx, btype, &dbtype) || upload(ctx, bslot, &dbslot) ||
+      allocate(ctx, bjw.size() * (D + 3), &ctx->bdata))
     return bail(SDG_E_NOMEM);
   g.Vt = dVt;
   g.dV = ddV;
@@ -532,11 +639,13 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   g.fjw = djw;
   g.fseq = dseq;
   g.fseqinv = dseqinv;
-  g.efid = defid;
-  g.enb = denb;
-  g.enbf = denbf;
-  g.eside = deside;
   g.snb = dsnb;
+  g.bnormal = dbnormal;
+  g.bjw = dbjw;
+  g.btype = dbtype;
+  g.bslot = dbslot;
+  g.bdata = ctx->bdata;
+  g.Fb = static_cast<int>(Fb);
   g.sfp = dsfp;
   g.fpar = dfpar;
   g.floc = dfloc;
@@ -570,6 +679,39 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     if (upload(ctx, t_interp, &pi) || upload(ctx, t_grad, &pg) || upload(ctx, t_face, &pf) || upload(ctx, t_minv, &pm) ||
         upload(ctx, t_trace, &pt) || upload(ctx, t_lift, &pl) || upload(ctx, pinterp, &ppi) || upload(ctx, ptrace, &ppt))
       return bail(SDG_E_NOMEM);
+    if (ctx->mixed) {
+      auto f32 = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
+      float *fi, *fg, *ff, *fm, *ft, *fl, *fpi, *fpt;
+      if (upload(ctx, f32(t_interp), &fi) || upload(ctx, f32(t_grad), &fg) || upload(ctx, f32(t_face), &ff) ||
+          upload(ctx, f32(t_minv), &fm) || upload(ctx, f32(t_trace), &ft) || upload(ctx, f32(t_lift), &fl) ||
+          upload(ctx, f32(pinterp), &fpi) || upload(ctx, f32(ptrace), &fpt))
+        return bail(SDG_E_NOMEM);
+      g.opf = {fi, fg, ff, fm, ft, fl, fpi, fpt};
+    }
+    if (ctx->launch.adj_volume[0]) {
+      // transposes for the discrete adjoint
+      std::vector<double> gradT = paddedOperator(nb, nq * D, [&](int m, int k) { return r.dV[static_cast<size_t>(m) * nb + k]; });
+      std::vector<double> interpT = paddedOperator(nq, nb, [&](int m, int k) { return V(k, m); });
+      std::vector<double> pinterpT = paddedOperator(nq, nb, [&](int m, int k) {
+        double acc = 0.0;
+        for (int i = 0; i < nb; i++) acc += r.mass_inverse[static_cast<size_t>(m) * nb + i] * V(k, i);
+        return acc;
+      });
+      std::vector<double> ptraceT = paddedOperator(naq, nb, [&](int m, int k) {
+        double acc = 0.0;
+        for (int i = 0; i < nb; i++) acc += r.mass_inverse[static_cast<size_t>(m) * nb + i] * Vfv(k, i);
+        return acc;
+      });
+      std::vector<double> liftT = paddedOperator(naq, naq, [&](int m, int k) {
+        const int sm = m / nfq_, sk = k / nfq_;
+        return sm == sk ? Lf[(static_cast<size_t>(sm) * nfq_ + m % nfq_) * nfq_ + k % nfq_] : 0.0;
+      });
+      double *ag, *ai, *api, *apt, *al;
+      if (upload(ctx, gradT, &ag) || upload(ctx, interpT, &ai) || upload(ctx, pinterpT, &api) ||
+          upload(ctx, ptraceT, &apt) || upload(ctx, liftT, &al))
+        return bail(SDG_E_NOMEM);
+      ctx->adj_ops = {ag, ai, api, apt, al};
+    }
     g.op.pinterp = ppi;
     g.op.ptrace = ppt;
     g.op.interp = pi;
@@ -582,17 +724,48 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   ctx->minedge = dme;
   ctx->h_lsq = r.lsq_inverse;
   ctx->h_V = r.V;
+  {
+    // coefficients of the constant 1 by the initial-condition projection (exactly 1 on the vertex
+    // modes and 0 elsewhere for the Lobatto basis; rounding residue snapped), and the mean of each
+    // basis function over the reference element
+    std::vector<double> colsum(nb, 0.0);
+    double wsum = 0.0;
+    ctx->h_mean.assign(nb, 0.0);
+    for (int q = 0; q < nq; q++) {
+      wsum += r.qweight[q];
+      for (int b = 0; b < nb; b++) {
+        colsum[b] += r.V[static_cast<size_t>(q) * nb + b];
+        ctx->h_mean[b] += r.qweight[q] * r.V[static_cast<size_t>(q) * nb + b];
+      }
+    }
+    for (int b = 0; b < nb; b++) ctx->h_mean[b] /= wsum;
+    ctx->h_one.assign(nb, 0.0);
+    for (int b = 0; b < nb; b++) {
+      double x = 0.0;
+      for (int c = 0; c < nb; c++) x += colsum[c] * r.lsq_inverse[static_cast<size_t>(c) * nb + b];
+      if (std::fabs(x) < 1e-12) x = 0.0;
+      if (std::fabs(x - 1.0) < 1e-12) x = 1.0;
+      ctx->h_one[b] = x;
+    }
+    if (upload(ctx, ctx->h_one, &ctx->one)) return bail(SDG_E_NOMEM);
+  }
   const size_t nU = static_cast<size_t>(E) * ctx->nv * nb, nUf = static_cast<size_t>(E) * ctx->nv * naq;
-  if (allocate(ctx, nU, &ctx->U) || allocate(ctx, nU, &ctx->Un) || allocate(ctx, nUf, &ctx->Uf[0]) ||
-      allocate(ctx, nUf, &ctx->Uf[1]) || allocate(ctx, nU, &ctx->Rout) ||
+  if (ctx->mixed ? (allocate(ctx, nUf, &ctx->Uf32[0]) || allocate(ctx, nUf, &ctx->Uf32[1]))
+                 : (allocate(ctx, nUf, &ctx->Uf[0]) || allocate(ctx, nUf, &ctx->Uf[1])))
+    return bail(SDG_E_NOMEM);
+  if (allocate(ctx, nU, &ctx->U) || allocate(ctx, nU, &ctx->Un) || allocate(ctx, nU, &ctx->Rout) ||
       allocate(ctx, static_cast<size_t>(E) * ctx->nv, &ctx->mon) || allocate(ctx, static_cast<size_t>(ctx->nv), &ctx->rel) ||
       allocate(ctx, static_cast<size_t>(E), &ctx->dtbuf) || allocate(ctx, size_t(2), &ctx->status))
     return bail(SDG_E_NOMEM);
   if (ctx->ns()) {
-    if (allocate(ctx, static_cast<size_t>(E) * ctx->ng * naq, &ctx->Gf) || allocate(ctx, nU, &ctx->Rvol) ||
-        allocate(ctx, nUf, &ctx->Hs))
-      return bail(SDG_E_NOMEM);
-    if (ctx->launch.mfma && allocate(ctx, static_cast<size_t>(E) * nq * D * ctx->nv, &ctx->Fv)) return bail(SDG_E_NOMEM);
+    const size_t nGf = static_cast<size_t>(E) * ctx->ng * naq, nFv = static_cast<size_t>(E) * nq * D * ctx->nv;
+    if (ctx->mixed) {
+      if (allocate(ctx, nGf, &ctx->Gf32) || allocate(ctx, nUf, &ctx->Hs32) || allocate(ctx, nFv, &ctx->Fv32))
+        return bail(SDG_E_NOMEM);
+    } else {
+      if (allocate(ctx, nGf, &ctx->Gf) || allocate(ctx, nUf, &ctx->Hs)) return bail(SDG_E_NOMEM);
+      if (ctx->launch.mfma ? allocate(ctx, nFv, &ctx->Fv) : allocate(ctx, nU, &ctx->Rvol)) return bail(SDG_E_NOMEM);
+    }
   }
   hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
@@ -621,7 +794,8 @@ int sdg_set_state(sdg_ctx* ctx, const double* coefficients) {
   HIP_TRY(ctx, hipMemcpyAsync(ctx->U, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   HIP_TRY(ctx, hipMemcpyAsync(ctx->Un, ctx->U, dev.size() * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
   ctx->cur = 0;
-  ctx->launch.trace(ctx->stream, ctx->geo, ctx->U, ctx->Uf[0]);
+  updateReference(ctx, coefficients);
+  ctx->launch.trace(ctx->stream, ctx->geo, ctx->U, ctx->traces(0), ctx->mix);
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   destroyGraphs(ctx);
@@ -673,6 +847,29 @@ int sdg_set_initial_primitive(sdg_ctx* ctx, const double* prim) {
   return sdg_set_state(ctx, coeff.data());
 }
 
+int sdg_set_boundary_primitive(sdg_ctx* ctx, const double* prim) {
+  if (!ctx || (!prim && ctx->Fb > 0)) return fail(ctx, SDG_E_INVALID, "sdg_set_boundary_primitive: null argument");
+  const int D = ctx->dim, nv = ctx->nv, nfq = ctx->nfq;
+  const size_t npts = static_cast<size_t>(ctx->Fb) * nfq;
+  std::vector<double> w(npts * (D + 3));
+  const sdg::Phys& ph = ctx->phys;
+  for (size_t i = 0; i < npts; i++) {  // primitive -> computational (VariableConvertor.cpp:118-139,341-366)
+    const double* pr = prim + i * nv;
+    double* o = &w[i * (D + 3)];
+    o[0] = pr[0];
+    for (int d = 0; d < D; d++) o[1 + d] = pr[1 + d];
+    o[D + 1] = ph.cv * pr[D + 1];
+    o[D + 2] = ph.eos == SDG_EOS_IDEAL_GAS ? (sdg::kGamma - 1.0) * pr[0] * o[D + 1]
+                                           : ph.c0 * ph.c0 * (pr[0] - ph.rho0) + ph.padd;
+  }
+  if (npts) {
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->bdata, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  ctx->has_bdata = true;
+  return SDG_OK;
+}
+
 int sdg_compute_dt(sdg_ctx* ctx, double cfl, double* dt) {
   if (!ctx || !dt) return fail(ctx, SDG_E_INVALID, "sdg_compute_dt: null argument");
   if (!ctx->has_state) return fail(ctx, SDG_E_STATE, "sdg_compute_dt: no state set");
@@ -690,6 +887,8 @@ int sdg_compute_dt(sdg_ctx* ctx, double cfl, double* dt) {
 int sdg_residual(sdg_ctx* ctx, double* residual) {
   if (!ctx || !residual) return fail(ctx, SDG_E_INVALID, "sdg_residual: null argument");
   if (!ctx->has_state) return fail(ctx, SDG_E_STATE, "sdg_residual: no state set");
+  if (ctx->Fb > 0 && !ctx->has_bdata)
+    return fail(ctx, SDG_E_STATE, "sdg_residual: boundary data not set (sdg_set_boundary_primitive)");
   enqueueStage(ctx, -1, 0.0, 1, nullptr);
   HIP_TRY(ctx, hipGetLastError());
   std::vector<double> dev(static_cast<size_t>(ctx->E) * ctx->nv * ctx->nb);
@@ -703,6 +902,8 @@ int sdg_get_gradient(sdg_ctx* ctx, double* gradient) {
   if (!ctx || !gradient) return fail(ctx, SDG_E_INVALID, "sdg_get_gradient: null argument");
   if (!ctx->ns()) return fail(ctx, SDG_E_UNSUPPORTED, "sdg_get_gradient: only defined for Navier-Stokes");
   if (!ctx->has_state) return fail(ctx, SDG_E_STATE, "sdg_get_gradient: no state set");
+  if (ctx->Fb > 0 && !ctx->has_bdata)
+    return fail(ctx, SDG_E_STATE, "sdg_get_gradient: boundary data not set (sdg_set_boundary_primitive)");
   if (!ctx->Wout && allocate(ctx, static_cast<size_t>(ctx->E) * ctx->ng * ctx->nb, &ctx->Wout)) return SDG_E_NOMEM;
   StageArgs a = baseArgs(ctx);
   a.mode = 1;
@@ -722,6 +923,8 @@ int sdg_get_gradient(sdg_ctx* ctx, double* gradient) {
 int sdg_step(sdg_ctx* ctx, double dt, int nsteps, double* relative_error, int* nan_flag, int* steps_taken) {
   if (!ctx) return fail(ctx, SDG_E_INVALID, "sdg_step: null context");
   if (!ctx->has_state) return fail(ctx, SDG_E_STATE, "sdg_step: no state set");
+  if (ctx->Fb > 0 && !ctx->has_bdata)
+    return fail(ctx, SDG_E_STATE, "sdg_step: boundary data not set (sdg_set_boundary_primitive)");
   if (nsteps < 0 || !(dt >= 0.0)) return fail(ctx, SDG_E_INVALID, "sdg_step: bad dt or step count");
   hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -759,7 +962,7 @@ int sdg_step(sdg_ctx* ctx, double dt, int nsteps, double* relative_error, int* n
   // the host counter, so recompute the traces from the stopped state.
   if (st[0] && st[1] < nsteps) {
     ctx->cur = 0;
-    ctx->launch.trace(ctx->stream, ctx->geo, ctx->U, ctx->Uf[0]);
+    ctx->launch.trace(ctx->stream, ctx->geo, ctx->U, ctx->traces(0), ctx->mix);
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   }
   if (relative_error) std::memcpy(relative_error, rel.data(), rel.size() * sizeof(double));
@@ -783,6 +986,8 @@ int sdg_synchronize(sdg_ctx* ctx) {
 int sdg_time_kernels(sdg_ctx* ctx, double dt, int nsteps, double* avg_ms, int n_out) {
   if (!ctx || !avg_ms || n_out < 1) return fail(ctx, SDG_E_INVALID, "sdg_time_kernels: bad argument");
   if (!ctx->has_state) return fail(ctx, SDG_E_STATE, "sdg_time_kernels: no state set");
+  if (ctx->Fb > 0 && !ctx->has_bdata)
+    return fail(ctx, SDG_E_STATE, "sdg_time_kernels: boundary data not set (sdg_set_boundary_primitive)");
   if (nsteps < 1 || !(dt >= 0.0)) return fail(ctx, SDG_E_INVALID, "sdg_time_kernels: bad dt or step count");
   const int st = ctx->stages();
   const size_t per_step = static_cast<size_t>(4 * st + 2);
@@ -821,6 +1026,231 @@ int sdg_time_kernels(sdg_ctx* ctx, double dt, int nsteps, double* avg_ms, int n_
   const double launches[SDG_KERNEL_CLASSES] = {double(nsteps) * st, double(nsteps) * st, double(nsteps)};
   for (int k = 0; k < n_out; k++) avg_ms[k] = k < SDG_KERNEL_CLASSES ? sum[k] / launches[k] : 0.0;
   if (!ctx->ns() && n_out > 1) avg_ms[1] = 0.0;
+  return SDG_OK;
+}
+
+}  // extern "C"
+
+// ---- discrete adjoint ---------------------------------------------------------------------------
+namespace {
+
+int ensureAdjoint(sdg_ctx* ctx) {
+  if (ctx->adj_ready) return SDG_OK;
+  if (!ctx->ns() || ctx->mixed || !ctx->launch.adj_volume[0] || ctx->Fb > 0)
+    return fail(ctx, SDG_E_UNSUPPORTED,
+                "adjoint: implemented for Navier-Stokes in FP64 on periodic meshes of the matrix-core element sizes "
+                "(quadrangle P3-P5, hexahedron P2-P4)");
+  const size_t E = static_cast<size_t>(ctx->E);
+  const size_t nU = E * ctx->nv * ctx->nb, nS = E * ctx->nv * ctx->naq, nG = E * ctx->ng * ctx->nb,
+               nGS = E * ctx->ng * ctx->naq;
+  if (allocate(ctx, nU, &ctx->adj_lam) || allocate(ctx, nS, &ctx->adj_Hb) || allocate(ctx, nG, &ctx->adj_RtV) ||
+      allocate(ctx, nU, &ctx->adj_Ua) || allocate(ctx, nS, &ctx->adj_UFf) || allocate(ctx, nGS, &ctx->adj_GFb) ||
+      allocate(ctx, nS, &ctx->adj_Pown) || allocate(ctx, nS, &ctx->adj_Qnb) || allocate(ctx, nU, &ctx->adj_out) ||
+      allocate(ctx, nU, &ctx->adj_S[1]) || allocate(ctx, nU, &ctx->adj_S[2]) || allocate(ctx, nU, &ctx->adj_acc) ||
+      allocate(ctx, nU, &ctx->adj_tmp))
+    return SDG_E_NOMEM;
+  ctx->adj_ready = true;
+  return SDG_OK;
+}
+
+size_t stateCount(const sdg_ctx* ctx) { return static_cast<size_t>(ctx->E) * ctx->nv * ctx->nb; }
+
+void axpby(sdg_ctx* ctx, double* y, const double* x, double alpha, double beta) {
+  const int64_t n = static_cast<int64_t>(stateCount(ctx));
+  hipLaunchKernelGGL(sdg::axpbyKernel, dim3((n + sdg::kThreads - 1) / sdg::kThreads), dim3(sdg::kThreads), 0,
+                     ctx->stream, y, x, alpha, beta, n);
+}
+
+// out = (dR/dU)^T mu at state U (device layout), mu = lam, or Mref^-1 lam / |J| with apply_minv
+int vjpAt(sdg_ctx* ctx, const double* U, const double* lam, int apply_minv, double* out) {
+  double* Uf = ctx->Uf[ctx->cur ^ 1];  // scratch: the next stage rewrites this trace buffer
+  ctx->launch.trace(ctx->stream, ctx->geo, U, Uf, ctx->mix);
+  StageArgs a = baseArgs(ctx);
+  a.U = const_cast<double*>(U);
+  a.Uf_in = Uf;
+  a.Wout = nullptr;
+  a.status = nullptr;
+  a.mode = 1;
+  sdg::AdjArgs j{};
+  j.op = ctx->adj_ops;
+  j.lam = lam;
+  j.apply_minv = apply_minv;
+  j.Hb = ctx->adj_Hb;
+  j.RtV = ctx->adj_RtV;
+  j.Ua = ctx->adj_Ua;
+  j.UFf = ctx->adj_UFf;
+  j.GFb = ctx->adj_GFb;
+  j.Pown = ctx->adj_Pown;
+  j.Qnb = ctx->adj_Qnb;
+  j.out = out;
+  const int vi = ctx->prob.viscous_flux == SDG_VISC_BR1 ? 0 : 1;
+  ctx->launch.adj_volume[vi](ctx->stream, ctx->geo, ctx->phys, a, j);
+  ctx->launch.adj_face(ctx->stream, ctx->geo, ctx->phys, a, j);
+  ctx->launch.adj_lift[vi](ctx->stream, ctx->geo, ctx->phys, a, j);
+  ctx->launch.adj_gather(ctx->stream, ctx->geo, ctx->phys, a, j);
+  HIP_TRY(ctx, hipGetLastError());
+  return SDG_OK;
+}
+
+// state <- src (device layout), with its traces
+int loadState(sdg_ctx* ctx, const double* src) {
+  const size_t bytes = stateCount(ctx) * sizeof(double);
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->U, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->Un, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  ctx->launch.trace(ctx->stream, ctx->geo, ctx->U, ctx->traces(ctx->cur), ctx->mix);
+  HIP_TRY(ctx, hipGetLastError());
+  return SDG_OK;
+}
+
+int advance(sdg_ctx* ctx, int k) {
+  if (k <= 0) return SDG_OK;
+  int nan = 0, taken = 0;
+  const int rc = sdg_step(ctx, ctx->adj_dt, k, nullptr, &nan, &taken);
+  if (rc != SDG_OK) return rc;
+  if (nan) return fail(ctx, SDG_E_STATE, "adjoint: the residual monitor turned NaN during the forward run");
+  return SDG_OK;
+}
+
+int saveState(sdg_ctx* ctx, double* dst) {
+  HIP_TRY(ctx, hipMemcpyAsync(dst, ctx->U, stateCount(ctx) * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  return SDG_OK;
+}
+
+// adj_lam: adjoint of U_{n+1} -> adjoint of U_n, for the step that starts from U0 (device layout).
+// Stage s maps U_s to U_{s+1} = c0 U_0 + c1 U_s + c2 dt L(U_s) (TimeIntegration.cpp:45-65,181-198),
+// so its transpose is lam_s = c1 lam_{s+1} + c2 dt L'(U_s)^T lam_{s+1}, plus c0 lam_{s+1} to U_0.
+int stepAdjoint(sdg_ctx* ctx, const double* U0) {
+  const int S = ctx->stages();
+  int rc = loadState(ctx, U0);
+  if (rc) return rc;
+  hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
+  for (int s = 0; s + 1 < S; s++) {
+    enqueueStage(ctx, s, ctx->adj_dt, 0, nullptr);
+    if ((rc = saveState(ctx, ctx->adj_S[s + 1]))) return rc;
+  }
+  HIP_TRY(ctx, hipMemsetAsync(ctx->adj_acc, 0, stateCount(ctx) * sizeof(double), ctx->stream));
+  for (int s = S - 1; s >= 0; s--) {
+    double c[3];
+    ctx->rkCoefficients(s, c);
+    const double* Us = s == 0 ? U0 : ctx->adj_S[s];
+    if ((rc = vjpAt(ctx, Us, ctx->adj_lam, 1, ctx->adj_out))) return rc;
+    axpby(ctx, ctx->adj_acc, ctx->adj_lam, c[0], 1.0);
+    axpby(ctx, ctx->adj_lam, ctx->adj_out, c[2] * ctx->adj_dt, c[1]);
+  }
+  axpby(ctx, ctx->adj_lam, ctx->adj_acc, 1.0, 1.0);
+  HIP_TRY(ctx, hipGetLastError());
+  return SDG_OK;
+}
+
+// first checkpoint position of a binomial schedule for n steps with s free snapshots: with r the
+// least repetition count such that C(s + r, s) >= n, the right part n - m must be reversible with
+// s - 1 snapshots (n - m <= C(s - 1 + r, s - 1)) and the left part with r - 1 repetitions
+double binom(int n, int k) {
+  double b = 1.0;
+  for (int i = 1; i <= k; i++) b = b * (n - k + i) / i;
+  return b;
+}
+int splitBinomial(int n, int s) {
+  int r = 0;
+  while (binom(s + r, s) < n) r++;
+  int m = n - static_cast<int>(binom(s - 1 + r, s - 1));
+  if (m < 1) m = 1;
+  if (m > n - 1) m = n - 1;
+  return m;
+}
+
+// Reverse steps [i0, i0 + n): snapshot `slot` holds U_{i0}, slots slot+1 .. slot+s are free (or
+// hold the states the forward run placed there); adj_lam holds the adjoint of U_{i0+n}.
+int reverseSweep(sdg_ctx* ctx, int i0, int n, int s, int slot) {
+  int rc;
+  if (n <= 0) return SDG_OK;
+  if (n == 1) return stepAdjoint(ctx, ctx->snaps[slot]);
+  if (s == 0) {
+    for (int jj = n - 1; jj >= 0; jj--) {
+      if ((rc = loadState(ctx, ctx->snaps[slot])) || (rc = advance(ctx, jj)) || (rc = saveState(ctx, ctx->adj_tmp)) ||
+          (rc = stepAdjoint(ctx, ctx->adj_tmp)))
+        return rc;
+    }
+    return SDG_OK;
+  }
+  const int m = splitBinomial(n, s);
+  if (ctx->snap_step[slot + 1] != i0 + m) {
+    if ((rc = loadState(ctx, ctx->snaps[slot])) || (rc = advance(ctx, m)) || (rc = saveState(ctx, ctx->snaps[slot + 1])))
+      return rc;
+    ctx->snap_step[slot + 1] = i0 + m;
+  }
+  if ((rc = reverseSweep(ctx, i0 + m, n - m, s - 1, slot + 1))) return rc;
+  ctx->snap_step[slot + 1] = -1;
+  return reverseSweep(ctx, i0, m, s, slot);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sdg_vjp_residual(sdg_ctx* ctx, const double* lambda, double* out) {
+  if (!ctx || !lambda || !out) return fail(ctx, SDG_E_INVALID, "sdg_vjp_residual: null argument");
+  if (!ctx->has_state) return fail(ctx, SDG_E_STATE, "sdg_vjp_residual: no state set");
+  int rc = ensureAdjoint(ctx);
+  if (rc) return rc;
+  std::vector<double> dev;
+  toDevice(ctx, lambda, dev);
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_lam, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = vjpAt(ctx, ctx->U, ctx->adj_lam, 0, ctx->adj_out))) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(dev.data(), ctx->adj_out, dev.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  toReference(ctx, dev, out);
+  return SDG_OK;
+}
+
+int sdg_adjoint_forward(sdg_ctx* ctx, double dt, int nsteps, int snapshots) {
+  if (!ctx) return fail(ctx, SDG_E_INVALID, "sdg_adjoint_forward: null context");
+  if (!ctx->has_state) return fail(ctx, SDG_E_STATE, "sdg_adjoint_forward: no state set");
+  if (nsteps < 0 || snapshots < 1 || !(dt >= 0.0)) return fail(ctx, SDG_E_INVALID, "sdg_adjoint_forward: bad argument");
+  int rc = ensureAdjoint(ctx);
+  if (rc) return rc;
+  const int nslots = std::max(1, std::min(snapshots, nsteps));
+  while (static_cast<int>(ctx->snaps.size()) < nslots) {
+    double* p = nullptr;
+    if (allocate(ctx, stateCount(ctx), &p)) return SDG_E_NOMEM;
+    ctx->snaps.push_back(p);
+  }
+  ctx->snap_step.assign(ctx->snaps.size(), -1);
+  ctx->adj_dt = dt;
+  ctx->adj_nsteps = -1;
+  if ((rc = saveState(ctx, ctx->snaps[0]))) return rc;
+  ctx->snap_step[0] = 0;
+  // the forward run places the checkpoints of the schedule's first descent
+  int i0 = 0, n = nsteps, s = nslots - 1, slot = 0;
+  while (n > 1 && s > 0) {
+    const int m = splitBinomial(n, s);
+    if ((rc = advance(ctx, m))) return rc;
+    slot++;
+    if ((rc = saveState(ctx, ctx->snaps[slot]))) return rc;
+    ctx->snap_step[slot] = i0 + m;
+    i0 += m;
+    n -= m;
+    s--;
+  }
+  if ((rc = advance(ctx, n))) return rc;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->adj_nsteps = nsteps;
+  ctx->adj_nslots = nslots;
+  return SDG_OK;
+}
+
+int sdg_adjoint_backward(sdg_ctx* ctx, const double* lambda_final, double* lambda_initial) {
+  if (!ctx || !lambda_final || !lambda_initial) return fail(ctx, SDG_E_INVALID, "sdg_adjoint_backward: null argument");
+  if (ctx->adj_nsteps < 0) return fail(ctx, SDG_E_STATE, "sdg_adjoint_backward: no recorded sdg_adjoint_forward run");
+  std::vector<double> dev;
+  toDevice(ctx, lambda_final, dev);
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_lam, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  int rc = reverseSweep(ctx, 0, ctx->adj_nsteps, ctx->adj_nslots - 1, 0);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(dev.data(), ctx->adj_lam, dev.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if ((rc = loadState(ctx, ctx->snaps[0]))) return rc;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  toReference(ctx, dev, lambda_initial);
   return SDG_OK;
 }
 

@@ -108,7 +108,7 @@ def rel_err_per_var(a: np.ndarray, b: np.ndarray) -> np.ndarray:
 
 
 def parity_error(a: np.ndarray, b: np.ndarray, state: np.ndarray, dim: int, floor: float = 1e-3,
-                 h: float | None = None) -> float:
+                 h: float | None = None, per_group: bool = False):
     """Parity metric used for the 1e-12 FP64 bar.
 
     Variables are grouped physically -- density, the momentum vector, total energy -- and the
@@ -144,4 +144,46 @@ def parity_error(a: np.ndarray, b: np.ndarray, state: np.ndarray, dim: int, floo
     if h is not None:
         fl = max(fl, 1.0 / h)
     errs = [diff[i] / (S[i] * max(nd[i], fl)) if max(nd[i], fl) > 0 else diff[i] for i in range(3)]
+    if per_group:
+        return [float(x) for x in errs]
     return float(max(errs))
+
+
+def kinetic_energy(mesh, U: np.ndarray):
+    """Domain-integrated kinetic energy J = sum_e sum_q |J|w 1/2 |m|^2 / rho of modal coefficients
+    U (E, N_b, NV) and its gradient dJ/dU (same shape)."""
+    D, nv = mesh.dim, mesh.nv
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    jw = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
+    uq = np.einsum("qb,ebv->eqv", V, U)
+    rho, m = uq[..., 0], uq[..., 1:1 + D]
+    m2 = (m * m).sum(-1)
+    J = float((jw * 0.5 * m2 / rho).sum())
+    dj = np.zeros_like(uq)
+    dj[..., 0] = -0.5 * m2 / rho ** 2
+    dj[..., 1:1 + D] = m / rho[..., None]
+    dJ = np.einsum("eq,qb,eqv->ebv", jw, V, dj)
+    return J, dJ
+
+
+def project_conserved(mesh, uq: np.ndarray) -> np.ndarray:
+    """Unweighted least-squares projection of point values (E, N_q, NV) onto modal coefficients
+    (InitialCondition.cpp:99-101): C = U_q V (V^T V)^-1."""
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    lsq = mesh.field("basis_lsq_inverse").reshape(mesh.nb, mesh.nb)
+    return np.einsum("eqv,qc,cb->ebv", uq, V, lsq)
+
+
+def euler_box(n=6, order=2, flux=L.FLUX_ROE, boundary=((L.BC_RIEMANN_FARFIELD,) * 2, (L.BC_RIEMANN_FARFIELD,) * 2),
+              state=(1.0, 0.3, 0.2, 1.0)):
+    """2-D Euler on [0,1]^2 with boundary types per direction (None = periodic) and a uniform
+    primitive state (rho, u, v, T) as initial condition and as boundary data (R = 1)."""
+    mesh = L.Mesh(L.ELEM_QUADRANGLE, order, (n, n), (0.0, 0.0), (1.0, 1.0), boundary=boundary)
+    prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=flux, cp=3.5, cv=2.5)
+    prim = np.tile(np.asarray(state, dtype=np.float64), (mesh.E, mesh.nq, 1))
+    bprim = np.tile(np.asarray(state, dtype=np.float64), (mesh.Fb, mesh.nfq, 1))
+    return mesh, prob, prim, bprim
+
+
+def boundary_points(mesh):
+    return mesh.field("boundary_face_coord").reshape(mesh.Fb, mesh.nfq, mesh.dim)

@@ -1,0 +1,124 @@
+"""GPU parity with boundary conditions: every boundary type of BoundaryCondition.cpp on the
+boundary faces of non-periodic boxes, Euler and Navier-Stokes, vector and matrix-core kernels,
+against the oracle at the FP64 bar; mixed precision against the FP64 oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import problems as P
+from subrosadg_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+GRAD_TOL = 5e-11
+
+BCS = [
+    ("farfield", ((L.BC_RIEMANN_FARFIELD,) * 2, (L.BC_RIEMANN_FARFIELD,) * 2)),
+    ("inflow-outflow+slip", ((L.BC_VELOCITY_INFLOW, L.BC_PRESSURE_OUTFLOW), (L.BC_ADIABATIC_SLIP_WALL,) * 2)),
+    ("isothermal-walls", (None, (L.BC_ISOTHERMAL_NONSLIP_WALL,) * 2)),
+    ("adiabatic-walls", ((L.BC_ADIABATIC_NONSLIP_WALL, L.BC_ADIABATIC_SLIP_WALL), None)),
+]
+
+
+def _state(xq, dim):
+    """Smooth non-uniform subsonic state (rho, u.., T) around rho = 1, u = (0.3, 0.2), T = 1."""
+    x = xq[..., 0]
+    y = xq[..., 1]
+    prim = np.zeros(xq.shape[:-1] + (dim + 2,))
+    prim[..., 0] = 1.0 + 0.1 * np.sin(2 * math.pi * x) * np.cos(2 * math.pi * y)
+    prim[..., 1] = 0.3 + 0.05 * np.cos(2 * math.pi * y)
+    prim[..., 2] = 0.2 + 0.05 * np.sin(2 * math.pi * x)
+    if dim == 3:
+        prim[..., 3] = 0.05 * np.sin(2 * math.pi * xq[..., 2])
+    prim[..., -1] = 1.0 + 0.05 * np.cos(2 * math.pi * (x + y))
+    return prim
+
+
+def _setup(eq, element, order, bc, n=4, flux=L.FLUX_ROE, visc=L.VISC_BR2, precision=L.PREC_FP64):
+    dim = 2 if element == L.ELEM_QUADRANGLE else 3
+    cells = (n,) * dim
+    bcs = tuple(bc) + ((None,) if dim == 3 else ())
+    mesh = L.Mesh(element, order, cells, (0.0,) * dim, (1.0,) * dim, boundary=bcs)
+    if eq == L.EQ_EULER:
+        prob = L.problem_desc(equation=eq, convective_flux=flux, cp=3.5, cv=2.5, precision=precision)
+    else:
+        prob = L.problem_desc(equation=eq, convective_flux=flux, viscous_flux=visc, transport=L.TRANSPORT_CONSTANT,
+                              cp=3.5, cv=2.5, dynamic_viscosity=2e-3, precision=precision)
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, dim)
+    prim = _state(xq, dim)
+    bprim = np.zeros((mesh.Fb, mesh.nfq, dim + 2))
+    bprim[..., 0] = 1.0
+    bprim[..., 1] = 0.3
+    bprim[..., 2] = 0.2
+    bprim[..., -1] = 1.0
+    bx = P.boundary_points(mesh)
+    bprim[..., -1] += 0.02 * np.sin(2 * math.pi * bx[..., 0])  # wall / inflow temperature profile
+    return mesh, prob, prim, bprim
+
+
+def _pair(mesh, prob, prim, bprim):
+    s = L.Solver(prob, mesh)
+    s.set_initial_primitive(prim)
+    s.set_boundary_primitive(bprim)
+    o = O.Oracle(prob, mesh)
+    o.set_state(s.get_state())
+    o.set_boundary_primitive(bprim)
+    return o, s
+
+
+CASES = [(eq, el, order, name, bc) for name, bc in BCS for (eq, el, order) in
+         [(L.EQ_EULER, L.ELEM_QUADRANGLE, 2), (L.EQ_EULER, L.ELEM_QUADRANGLE, 3),
+          (L.EQ_NS, L.ELEM_QUADRANGLE, 2), (L.EQ_NS, L.ELEM_QUADRANGLE, 3), (L.EQ_NS, L.ELEM_HEXAHEDRON, 2)]]
+
+
+@pytest.mark.parametrize("eq,el,order,name,bc", CASES,
+                         ids=[f"{'ns' if c[0] else 'euler'}-{'hex' if c[1] == 6 else 'quad'}-p{c[2]}-{c[3]}" for c in CASES])
+def test_boundary_residual_and_step_parity(eq, el, order, name, bc):
+    print(f"bc parity eq={eq} el={el} P{order} {name}", flush=True)
+    mesh, prob, prim, bprim = _setup(eq, el, order, bc, n=3 if el == L.ELEM_HEXAHEDRON else 4)
+    o, s = _pair(mesh, prob, prim, bprim)
+    u0 = o.get_state()
+    if eq == L.EQ_NS:
+        h = float(mesh.field("min_edge").min())
+        eg = P.parity_error(s.gradient(), o.gradient(), u0, mesh.dim, h=h)
+        assert eg <= GRAD_TOL, eg
+    err = P.parity_error(s.residual(), o.residual(), u0, mesh.dim)
+    assert err <= TOL, err
+    dt = o.compute_dt(0.1)
+    o.step(dt, 5)
+    _, nan, taken = s.step(dt, 5)
+    assert nan == 0 and taken == 5
+    uo = o.get_state()
+    es = P.parity_error(s.get_state(), uo, uo, mesh.dim)
+    assert es <= TOL, es
+
+
+@pytest.mark.parametrize("name,bc", BCS, ids=[b[0] for b in BCS])
+def test_boundary_mixed_precision(name, bc):
+    mesh, prob, prim, bprim = _setup(L.EQ_NS, L.ELEM_QUADRANGLE, 3, bc, precision=L.PREC_MIXED)
+    s = L.Solver(prob, mesh)
+    s.set_initial_primitive(prim)
+    s.set_boundary_primitive(bprim)
+    p64 = L.ProblemDesc.from_buffer_copy(prob)
+    p64.precision = L.PREC_FP64
+    o = O.Oracle(p64, mesh)
+    o.set_state(s.get_state())
+    o.set_boundary_primitive(bprim)
+    dt = o.compute_dt(0.1)
+    o.step(dt, 20)
+    s.step(dt, 20)
+    a, b = s.get_state().reshape(-1, 4), o.get_state().reshape(-1, 4)
+    dev = max(np.linalg.norm(a[:, g] - b[:, g]) / np.linalg.norm(b[:, g]) for g in ([0], [1, 2], [3]))
+    print(f"  mixed L2 deviation {dev:.3e}", flush=True)
+    assert dev <= 1e-6, dev
+
+
+def test_missing_boundary_data_is_an_error():
+    mesh, prob, prim, _ = _setup(L.EQ_EULER, L.ELEM_QUADRANGLE, 2, BCS[0][1])
+    s = L.Solver(prob, mesh)
+    s.set_initial_primitive(prim)
+    with pytest.raises(L.SDGError):
+        s.residual()
