@@ -239,8 +239,16 @@ def adjoint_leg(args, n=64, order=3, steps=400, cfl=0.1):
     t0 = time.perf_counter()
     lam0 = s.adjoint_backward(dJ)
     t_bwd = time.perf_counter() - t0
-    # one finite-difference check along a smooth v perturbation
-    dv = np.sin(2 * math.pi * x) * np.cos(2 * math.pi * y)
+    # one finite-difference check along a seeded random smooth v perturbation with the L2 norm of
+    # the initial v field (Fourier modes 0 <= kx, ky <= 3), central step 1e-6 of that norm
+    rng = np.random.default_rng(20261202)
+    dv = np.zeros_like(x)
+    for kx in range(4):
+        for ky in range(4):
+            if kx or ky:
+                a, b = rng.standard_normal(2)
+                dv += a * np.cos(2 * math.pi * (kx * x + ky * y)) + b * np.sin(2 * math.pi * (kx * x + ky * y))
+    dv *= np.sqrt((jw * prim[..., 2] ** 2).sum() / (jw * dv ** 2).sum())
     du = np.zeros(prim.shape)
     du[..., 2] = prim[..., 0] * dv
     du[..., 3] = prim[..., 0] * prim[..., 2] * dv

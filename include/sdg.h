@@ -42,7 +42,13 @@ extern "C" {
 #define SDG_E_STATE (-5)       /* call out of order (e.g. step before a state was set)    */
 
 /* ---- enumerations mirroring src/Utils/Enum.cpp:22-233 ----------------------------------- */
-enum sdg_element { SDG_ELEM_LINE = 1, SDG_ELEM_TRIANGLE = 2, SDG_ELEM_QUADRANGLE = 3, SDG_ELEM_HEXAHEDRON = 6 };
+enum sdg_element {
+  SDG_ELEM_LINE = 1,
+  SDG_ELEM_TRIANGLE = 2,
+  SDG_ELEM_QUADRANGLE = 3,
+  SDG_ELEM_HEXAHEDRON = 6,
+  SDG_ELEM_MIXED_TRIANGLE_QUADRANGLE = 100 /* mesh info of a split grid holding both types */
+};
 enum sdg_equation { SDG_EQ_COMPRESSIBLE_EULER = 0, SDG_EQ_COMPRESSIBLE_NS = 1 };
 enum sdg_convective_flux {
   SDG_FLUX_CENTRAL = 0,
@@ -171,11 +177,29 @@ enum sdg_mesh_field {
   SDG_FIELD_BOUNDARY_FACE_JACOBIAN_W = 22,
   SDG_FIELD_BOUNDARY_FACE_PARENT = 23,
   SDG_FIELD_BOUNDARY_FACE_TYPE = 24,
-  SDG_FIELD_COUNT_ = 25
+  SDG_FIELD_ELEMENT_TYPE = 25,    /* (int32) E: enum sdg_element per element */
+  SDG_FIELD_COUNT_ = 26
 };
+
+/* Split grid (MeshModelEnum::TriangleQuadrangle meshes of the reference): a periodic structured
+ * n[0] x n[1] grid on [lo, hi] whose cells are quadrangles or, each with probability
+ * split_probability (a seeded splitmix64 Bernoulli draw per cell, cells x fastest), two triangles
+ * along the diagonal from (lo corner) to (hi corner) of the cell. A grid of one element type is an
+ * ordinary mesh for sdg_create; a mixed one runs through the batched solver. For mixed grids the
+ * exported QUAD_COORD / JACOBIAN_DET_W list every element's own quadrature points in element order
+ * and ELEMENT_TYPE gives the element types; other fields are not available. */
+typedef struct sdg_split_desc {
+  int polynomial_order;   /* P = 1..4                                                     */
+  int n[2];               /* cells per direction (>= 2)                                   */
+  double lo[2];
+  double hi[2];
+  double split_probability;
+  uint64_t seed;
+} sdg_split_desc;
 
 typedef struct sdg_mesh sdg_mesh;
 typedef struct sdg_ctx sdg_ctx;
+typedef struct sdg_batch sdg_batch;
 
 /* ---- library ---------------------------------------------------------------------------- */
 /* ABI version compiled into the library. */
@@ -188,6 +212,7 @@ const char* sdg_last_error(const sdg_ctx* ctx);
  * periodic meshes: elements, faces with periodic pairing (Adjacency.cpp:185-328), Jacobians,
  * mass-matrix inverses, normals, basis and quadrature tables. */
 int sdg_mesh_create_structured(const sdg_structured_desc* desc, sdg_mesh** out);
+int sdg_mesh_create_split(const sdg_split_desc* desc, sdg_mesh** out);
 void sdg_mesh_destroy(sdg_mesh* mesh);
 int sdg_mesh_get_info(const sdg_mesh* mesh, sdg_mesh_info* info);
 /* Number of scalar entries of an exported field (0 for an unknown field). */
@@ -273,6 +298,33 @@ int sdg_adjoint_forward(sdg_ctx* ctx, double dt, int nsteps, int snapshots);
  * lambda_initial = dJ/dU(t_0). Recomputes stage states from the checkpoints; leaves the context
  * state at the recorded initial state. */
 int sdg_adjoint_backward(sdg_ctx* ctx, const double* lambda_final, double* lambda_initial);
+
+/* ---- batched solver (no counterpart in the reference: it runs one problem per process) -------
+ * Many independent 2-D compressible Euler problems on split grids (triangles and quadrangles,
+ * orders 1..4), advanced together: every RK stage of the whole batch is one persistent launch,
+ * steps are replayed from hipGraphs. Per problem the arithmetic is the single-problem engine's
+ * (same kernels' formulas, reference layouts). All problems share the problem descriptor
+ * (physics, flux, RK scheme, precision); each has its own mesh, order, state and time step. */
+int sdg_batch_create(const sdg_problem_desc* problem, const sdg_mesh* const* meshes, int count, sdg_batch** out);
+void sdg_batch_destroy(sdg_batch* batch);
+const char* sdg_batch_last_error(const sdg_batch* batch);
+/* primitive state at the mesh's QUAD_COORD points -> projected coefficients (InitialCondition.cpp:85-104) */
+int sdg_batch_set_initial_primitive(sdg_batch* batch, int problem, const double* primitive);
+/* coefficients per element in mesh order, N_b(element) x NV, NV fastest */
+int sdg_batch_set_state(sdg_batch* batch, int problem, const double* coefficients);
+int sdg_batch_get_state(sdg_batch* batch, int problem, double* coefficients);
+/* per problem: min over its elements of the CFL time step (TimeIntegration.cpp:104-131) */
+int sdg_batch_compute_dt(sdg_batch* batch, double cfl, double* dt);
+/* nsteps RK steps, problem i with time step dt[i] */
+int sdg_batch_step(sdg_batch* batch, const double* dt, int nsteps);
+int sdg_batch_last_step_ms(const sdg_batch* batch, double* ms);
+/* Autotuner: picks, per (element type, order) variant, the work-item size in element tiles and the
+ * persistent grid size by timing stages on the current state (restored afterwards); stage_ms
+ * (may be NULL) receives the best full-stage time. */
+int sdg_batch_tune(sdg_batch* batch, double* stage_ms);
+/* Current configuration: work-item tiles and elements per tile per variant (code = 4 * (quadrangle)
+ * + P - 1; 0 where unused), and the grid size. */
+int sdg_batch_get_config(const sdg_batch* batch, int* chunk_tiles, int* elements_per_tile, int* grid);
 
 #ifdef __cplusplus
 }

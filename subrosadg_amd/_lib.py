@@ -21,6 +21,7 @@ OK, E_INVALID, E_UNSUPPORTED, E_DEVICE, E_NOMEM, E_STATE = 0, -1, -2, -3, -4, -5
 
 # enums (include/sdg.h)
 ELEM_LINE, ELEM_TRIANGLE, ELEM_QUADRANGLE, ELEM_HEXAHEDRON = 1, 2, 3, 6
+ELEM_MIXED_TRIANGLE_QUADRANGLE = 100
 EQ_EULER, EQ_NS = 0, 1
 FLUX_CENTRAL, FLUX_LAX_FRIEDRICHS, FLUX_HLLC, FLUX_ROE, FLUX_EXACT = 0, 1, 2, 3, 4
 VISC_NONE, VISC_BR1, VISC_BR2 = 0, 1, 2
@@ -57,6 +58,7 @@ FIELDS = {
     "boundary_face_jacobian_w": (22, np.float64),
     "boundary_face_parent": (23, np.int32),
     "boundary_face_type": (24, np.int32),
+    "element_type": (25, np.int32),
 }
 
 
@@ -87,6 +89,17 @@ class StructuredDesc(Structure):
         ("hi", c_double * 3),
         ("bounded", c_int * 3),
         ("boundary", c_int * 6),
+    ]
+
+
+class SplitDesc(Structure):
+    _fields_ = [
+        ("polynomial_order", c_int),
+        ("n", c_int * 2),
+        ("lo", c_double * 2),
+        ("hi", c_double * 2),
+        ("split_probability", c_double),
+        ("seed", ctypes.c_uint64),
     ]
 
 
@@ -129,6 +142,18 @@ SIGNATURES = {
     "sdg_synchronize": (c_int, [c_void_p]),
     "sdg_time_kernels": (c_int, [c_void_p, c_double, c_int, c_void_p, c_int]),
     "sdg_vjp_residual": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "sdg_mesh_create_split": (c_int, [POINTER(SplitDesc), POINTER(c_void_p)]),
+    "sdg_batch_create": (c_int, [POINTER(ProblemDesc), c_void_p, c_int, POINTER(c_void_p)]),
+    "sdg_batch_destroy": (None, [c_void_p]),
+    "sdg_batch_last_error": (c_char_p, [c_void_p]),
+    "sdg_batch_set_initial_primitive": (c_int, [c_void_p, c_int, c_void_p]),
+    "sdg_batch_set_state": (c_int, [c_void_p, c_int, c_void_p]),
+    "sdg_batch_get_state": (c_int, [c_void_p, c_int, c_void_p]),
+    "sdg_batch_compute_dt": (c_int, [c_void_p, c_double, c_void_p]),
+    "sdg_batch_step": (c_int, [c_void_p, c_void_p, c_int]),
+    "sdg_batch_last_step_ms": (c_int, [c_void_p, POINTER(c_double)]),
+    "sdg_batch_tune": (c_int, [c_void_p, POINTER(c_double)]),
+    "sdg_batch_get_config": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(c_int)]),
     "sdg_adjoint_forward": (c_int, [c_void_p, c_double, c_int, c_int]),
     "sdg_adjoint_backward": (c_int, [c_void_p, c_void_p, c_void_p]),
 }
@@ -192,8 +217,11 @@ class Mesh:
     unless `boundary` gives, per direction, the (low, high) boundary types (BC_* constants) or None
     for a periodic direction."""
 
-    def __init__(self, element: int, order: int, n, lo, hi, boundary=None):
+    def __init__(self, element: int, order: int, n, lo, hi, boundary=None, _handle=None):
         self.lib = load()
+        if _handle is not None:
+            self._attach(_handle)
+            return
         d = StructuredDesc()
         d.element = element
         d.polynomial_order = order
@@ -209,6 +237,26 @@ class Mesh:
                     d.boundary[2 * i + 1] = int(sides[1])
         h = c_void_p()
         check(self.lib.sdg_mesh_create_structured(ctypes.byref(d), ctypes.byref(h)))
+        self._attach(h)
+
+    @classmethod
+    def split(cls, order: int, n, lo, hi, split_probability: float, seed: int):
+        """Periodic split grid: each cell a quadrangle or (with the given probability, seeded) two
+        triangles."""
+        lib = load()
+        d = SplitDesc()
+        d.polynomial_order = order
+        for i in range(2):
+            d.n[i] = int(n[i])
+            d.lo[i] = float(lo[i])
+            d.hi[i] = float(hi[i])
+        d.split_probability = float(split_probability)
+        d.seed = int(seed)
+        h = c_void_p()
+        check(lib.sdg_mesh_create_split(ctypes.byref(d), ctypes.byref(h)))
+        return cls(0, order, (), (), (), _handle=h)
+
+    def _attach(self, h):
         self.handle = h
         info = MeshInfo()
         check(self.lib.sdg_mesh_get_info(h, ctypes.byref(info)))
@@ -223,6 +271,7 @@ class Mesh:
         self.nfq = info.face_quadrature
         self.Fb = info.boundary_faces
         self.nv = self.dim + 2
+        self.mixed = info.element == ELEM_MIXED_TRIANGLE_QUADRANGLE
 
     def field(self, name: str) -> np.ndarray:
         fid, dt = FIELDS[name]
@@ -337,6 +386,81 @@ class Solver:
         h = getattr(self, "handle", None)
         if h is not None and h.value:
             self.lib.sdg_destroy(h)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+class Batch:
+    """Batched solver (sdg_batch): independent 2-D Euler problems on split grids, one persistent
+    launch per RK stage for all of them."""
+
+    def __init__(self, problem: ProblemDesc, meshes):
+        self.lib = load()
+        self.meshes = list(meshes)
+        self.problem = problem
+        arr = (c_void_p * len(self.meshes))(*[m.handle.value for m in self.meshes])
+        h = c_void_p()
+        rc = self.lib.sdg_batch_create(ctypes.byref(problem), ctypes.cast(arr, c_void_p), len(self.meshes),
+                                       ctypes.byref(h))
+        check(rc)
+        self.handle = h
+        self.sizes = []
+        for m in self.meshes:
+            et = m.field("element_type")
+            nb = {ELEM_TRIANGLE: (m.order + 1) * (m.order + 2) // 2, ELEM_QUADRANGLE: (m.order + 1) ** 2}
+            self.sizes.append(int(sum(nb[int(t)] for t in et)) * 4)
+
+    def _check(self, rc):
+        if rc != OK:
+            msg = self.lib.sdg_batch_last_error(self.handle)
+            raise SDGError(f"sdg error {rc}: {msg.decode() if msg else ''}")
+
+    def set_initial_primitive(self, i: int, prim: np.ndarray):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        self._check(self.lib.sdg_batch_set_initial_primitive(self.handle, i, prim.ctypes.data))
+
+    def set_state(self, i: int, c: np.ndarray):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        assert c.size == self.sizes[i]
+        self._check(self.lib.sdg_batch_set_state(self.handle, i, c.ctypes.data))
+
+    def get_state(self, i: int) -> np.ndarray:
+        out = np.empty(self.sizes[i])
+        self._check(self.lib.sdg_batch_get_state(self.handle, i, out.ctypes.data))
+        return out
+
+    def compute_dt(self, cfl: float) -> np.ndarray:
+        out = np.empty(len(self.meshes))
+        self._check(self.lib.sdg_batch_compute_dt(self.handle, cfl, out.ctypes.data))
+        return out
+
+    def step(self, dt, nsteps: int):
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        self._check(self.lib.sdg_batch_step(self.handle, dt.ctypes.data, nsteps))
+
+    def last_step_ms(self) -> float:
+        ms = c_double()
+        self._check(self.lib.sdg_batch_last_step_ms(self.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def tune(self) -> float:
+        ms = c_double()
+        self._check(self.lib.sdg_batch_tune(self.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def config(self):
+        chunk = np.zeros(8, dtype=np.int32)
+        epb = np.zeros(8, dtype=np.int32)
+        grid = c_int()
+        self._check(self.lib.sdg_batch_get_config(self.handle, chunk.ctypes.data, epb.ctypes.data, ctypes.byref(grid)))
+        return chunk, epb, grid.value
+
+    def close(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self.lib.sdg_batch_destroy(h)
             self.handle = None
 
     def __del__(self):

@@ -16,7 +16,7 @@ OUT = os.path.join(HERE, "libsdg_hip.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
 HOST_SOURCES = ["kit.cpp", "sdg_mesh_api.cpp"]
-HIP_SOURCES = ["sdg_solver.hip"]
+HIP_SOURCES = ["sdg_solver.hip", "sdg_batch.hip"]
 EXAMPLES = ["taylor_green_3d_cns", "lid_driven_cavity_2d_cns"]
 ARCH = os.environ.get("SDG_OFFLOAD_ARCH", "gfx950")
 

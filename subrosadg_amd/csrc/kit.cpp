@@ -4,8 +4,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <limits>
+#include <unordered_map>
+#include <utility>
 
 namespace sdg {
 
@@ -35,6 +38,97 @@ void p1Shape(int dim, const double* v, const double* xi, double& n, double* grad
     for (int e = 0; e < dim; e++)
       if (e != d) g *= f[e];
     grad[d] = g;
+  }
+}
+
+// Symmetric quadrature rules on the reference triangle (0,0),(1,0),(0,1), weights summing to its
+// measure 1/2, exact for polynomials of degree 2, 4, 6, 8 with 3, 6, 12, 16 points (the point
+// counts of SimulationControl.cpp:275 for P = 1..4). Orbits in barycentric coordinates: S3 the
+// centroid, S21(a) the 3 permutations of (a, a, 1-2a), S111(a, b) the 6 permutations of
+// (a, b, 1-a-b). Parameters solved from the moment equations (Dunavant's orbit structure);
+// exactness is checked in tests/test_kit.py.
+struct TriOrbit {
+  int kind;  // 0: S3, 1: S21, 2: S111
+  double a, b, w;
+};
+const TriOrbit kTriRule2[] = {{1, 0.16666666666666666, 0.0, 0.16666666666666666}};
+const TriOrbit kTriRule4[] = {{1, 0.44594849091596483, 0.0, 0.11169079483900568},
+                              {1, 0.09157621350977076, 0.0, 0.05497587182766098}};
+const TriOrbit kTriRule6[] = {{1, 0.24928674517091226, 0.0, 0.05839313786318872},
+                              {1, 0.06308901449150205, 0.0, 0.025422453185103156},
+                              {2, 0.05314504984481764, 0.3103524510337826, 0.041425537809187396}};
+const TriOrbit kTriRule8[] = {{0, 0.0, 0.0, 0.0721578038388935},
+                              {1, 0.459292588292723, 0.0, 0.0475458171336425},
+                              {1, 0.17056930775176, 0.0, 0.051608685267359},
+                              {1, 0.050547228317031, 0.0, 0.016229248811599},
+                              {2, 0.008394777409958, 0.263112829634638, 0.0136151570872175}};
+
+void triangleRule(int order, std::vector<double>& pts, std::vector<double>& w) {
+  const TriOrbit* r = order == 1 ? kTriRule2 : order == 2 ? kTriRule4 : order == 3 ? kTriRule6 : kTriRule8;
+  const int n = order == 1 ? 1 : order == 2 ? 2 : order == 3 ? 3 : 5;
+  pts.clear();
+  w.clear();
+  auto add = [&](double l1, double l2, double wt) {  // (xi, eta) = (lambda_1, lambda_2)
+    pts.push_back(l1);
+    pts.push_back(l2);
+    w.push_back(wt);
+  };
+  for (int i = 0; i < n; i++) {
+    const TriOrbit& o = r[i];
+    if (o.kind == 0) {
+      add(1.0 / 3.0, 1.0 / 3.0, o.w);
+    } else if (o.kind == 1) {
+      const double c = 1.0 - 2.0 * o.a;
+      add(o.a, c, o.w);
+      add(c, o.a, o.w);
+      add(o.a, o.a, o.w);
+    } else {
+      const double c = 1.0 - o.a - o.b;
+      const double t[6][2] = {{o.b, c}, {c, o.b}, {o.a, c}, {c, o.a}, {o.a, o.b}, {o.b, o.a}};
+      for (int k = 0; k < 6; k++) add(t[k][0], t[k][1], o.w);
+    }
+  }
+}
+
+// Legendre polynomials L_0..L_n and derivatives at x (any arithmetic type)
+template <typename T>
+void legendreSeries(int n, T x, T* L, T* dL) {
+  L[0] = T(1.0);
+  dL[0] = T(0.0);
+  if (n == 0) return;
+  L[1] = x;
+  dL[1] = T(1.0);
+  for (int k = 2; k <= n; k++) {
+    L[k] = (T(2.0 * k - 1.0) * x * L[k - 1] - T(k - 1.0) * L[k - 2]) / T(double(k));
+    dL[k] = dL[k - 2] + T(2.0 * k - 1.0) * L[k - 1];
+  }
+}
+
+// Hierarchic modal basis of order P on the reference triangle (Solin's H1 construction spanning
+// P_P, the space of BasisFunction.cpp's triangle basis): the barycentric vertex functions, then per
+// degree k = 2..P the three edge functions lambda_a lambda_b kappa_{k-2}(lambda_b - lambda_a) with
+// Solin's kernel kappa_{k-2}(x) = 4 l_k(x) / (1 - x^2), and the bubbles
+// lambda_0 lambda_1 lambda_2 L_i(lambda_1 - lambda_0) L_j(lambda_2 - lambda_1), i + j = k - 3.
+template <typename T>
+void triangleBasis(int order, T xi, T eta, T* value) {
+  const T lam[3] = {T(1.0) - xi - eta, xi, eta};
+  static const int kEdge[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+  T L[8], dL[8], L2[8], dL2[8];
+  int b = 0;
+  for (int v = 0; v < 3; v++) value[b++] = lam[v];
+  for (int k = 2; k <= order; k++) {
+    for (int e = 0; e < 3; e++) {
+      const T l0 = lam[kEdge[e][0]], l1 = lam[kEdge[e][1]];
+      legendreSeries(k - 1, l1 - l0, L, dL);
+      // l_k = (L_k - L_{k-2}) / sqrt(2(2k-1)) = (2k-1)(x^2-1) L'_{k-1} / (k(k-1) sqrt(2(2k-1)))
+      const T kappa = T(-4.0 * (2.0 * k - 1.0) / (k * (k - 1.0) * std::sqrt(2.0 * (2.0 * k - 1.0)))) * dL[k - 1];
+      value[b++] = l0 * l1 * kappa;
+    }
+    if (k >= 3) {
+      legendreSeries(k - 3, lam[1] - lam[0], L, dL);
+      legendreSeries(k - 3, lam[2] - lam[1], L2, dL2);
+      for (int i = 0; i <= k - 3; i++) value[b++] = lam[0] * lam[1] * lam[2] * L[i] * L2[k - 3 - i];
+    }
   }
 }
 
@@ -137,6 +231,19 @@ bool invertMatrix(int n, const double* a, double* inverse) {
 
 static void evalBasis(const RefElement& ref, const double* xi, double* value, double* grad /*N_b x D*/) {
   const int dim = ref.dim;
+  if (ref.type == 2) {  // triangle: values, and gradients by complex-step differentiation
+    triangleBasis<double>(ref.order, xi[0], xi[1], value);
+    if (grad) {
+      const double h = 1e-30;
+      std::vector<std::complex<double>> cv(ref.nb);
+      for (int d = 0; d < 2; d++) {
+        const std::complex<double> x(xi[0], d == 0 ? h : 0.0), y(xi[1], d == 1 ? h : 0.0);
+        triangleBasis<std::complex<double>>(ref.order, x, y, cv.data());
+        for (int b = 0; b < ref.nb; b++) grad[b * 2 + d] = cv[b].imag() / h;
+      }
+    }
+    return;
+  }
   for (int b = 0; b < ref.nb; b++) {
     double l[3], dl[3];
     for (int d = 0; d < dim; d++) lobatto(ref.modes[b][d], xi[d], l[d], dl[d]);
@@ -214,12 +321,34 @@ bool buildRefElement(int type, int order, RefElement& ref, std::string& error) {
         ref.fpoint.push_back(gx[b]);
         ref.fweight.push_back(gw[a] * gw[b]);
       }
+  } else if (type == 2) {  // triangle
+    if (order > 4) {
+      error = "triangles are implemented for P = 1..4";
+      return false;
+    }
+    ref.dim = 2;
+    ref.nf = 3;
+    ref.nvert = 3;
+    ref.nfvert = 2;
+    const double tv[3][2] = {{0, 0}, {1, 0}, {0, 1}};  // SimulationControl.cpp:218-241
+    const int tf[3][2] = {{0, 1}, {1, 2}, {2, 0}};      // SimulationControl.cpp:182-186
+    for (int v = 0; v < 3; v++)
+      for (int d = 0; d < 2; d++) ref.vertex.push_back(tv[v][d]);
+    for (int f = 0; f < 3; f++)
+      for (int v = 0; v < 2; v++) ref.face_vertex.push_back(tf[f][v]);
+    triangleRule(order, ref.qpoint, ref.qweight);
+    for (int a = 0; a < n1; a++) {
+      ref.fpoint.push_back(gx[a]);
+      ref.fweight.push_back(gw[a]);
+    }
+    ref.modes.assign(static_cast<size_t>((order + 1) * (order + 2) / 2), {0, 0, 0});
   } else {
-    error = "unsupported element type (quadrangle=3 and hexahedron=6 are implemented)";
+    error = "unsupported element type (triangle=2, quadrangle=3 and hexahedron=6 are implemented)";
     return false;
   }
   // hierarchical ordering: sort modes by (max index, then slowest-to-fastest index) so that the
   // first N_b(P-1) functions span the order P-1 space
+  if (type != 2)
   std::stable_sort(ref.modes.begin(), ref.modes.end(), [](const std::array<int, 3>& a, const std::array<int, 3>& b) {
     const int ma = std::max(a[0], std::max(a[1], a[2]));
     const int mb = std::max(b[0], std::max(b[1], b[2]));
@@ -290,16 +419,25 @@ bool buildRefElement(int type, int order, RefElement& ref, std::string& error) {
 
 namespace {
 // physical point of reference point xi in element e, and the Jacobian J(l,k) = dx_l/dxi_k
-void mapPoint(const Mesh& mesh, int64_t e, const double* xi, double* x, double* jac) {
-  const RefElement& ref = mesh.ref;
+// physical point of reference point xi for an element of type ref with vertices xv (nvert x D),
+// and the Jacobian J(l,k) = dx_l/dxi_k
+void mapPointRef(const RefElement& ref, const double* xv_all, const double* xi, double* x, double* jac) {
   const int dim = ref.dim;
   for (int d = 0; d < dim; d++) x[d] = 0.0;
   if (jac)
     for (int i = 0; i < dim * dim; i++) jac[i] = 0.0;
   for (int v = 0; v < ref.nvert; v++) {
     double n, g[3];
-    p1Shape(dim, &ref.vertex[static_cast<size_t>(v) * dim], xi, n, g);
-    const double* xv = &mesh.vertex[(static_cast<size_t>(e) * ref.nvert + v) * dim];
+    if (ref.type == 2) {  // barycentric P1 shape functions
+      const double N[3] = {1.0 - xi[0] - xi[1], xi[0], xi[1]};
+      const double G[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
+      n = N[v];
+      g[0] = G[v][0];
+      g[1] = G[v][1];
+    } else {
+      p1Shape(dim, &ref.vertex[static_cast<size_t>(v) * dim], xi, n, g);
+    }
+    const double* xv = xv_all + static_cast<size_t>(v) * dim;
     for (int l = 0; l < dim; l++) {
       x[l] += n * xv[l];
       if (jac)
@@ -307,14 +445,15 @@ void mapPoint(const Mesh& mesh, int64_t e, const double* xi, double* x, double* 
     }
   }
 }
+void mapPoint(const Mesh& mesh, int64_t e, const double* xi, double* x, double* jac) {
+  mapPointRef(mesh.ref, &mesh.vertex[static_cast<size_t>(e) * mesh.ref.nvert * mesh.ref.dim], xi, x, jac);
+}
 // normals (outward from element e through its local face s) and |J_f| w at the face points, from
 // the element's affine face map (Geometry.cpp:69-86,102-168)
-void faceNormals(const Mesh& mesh, int64_t e, int s, Face& face) {
-  const RefElement& ref = mesh.ref;
+void faceNormalsRef(const RefElement& ref, const double* J, int s, Face& face) {
   const int dim = ref.dim, nfq = ref.nfq;
   face.normal.assign(static_cast<size_t>(nfq) * dim, 0.0);
   face.jw.assign(nfq, 0.0);
-  const double* J = &mesh.jac[static_cast<size_t>(e) * dim * dim];
   for (int p = 0; p < nfq; p++) {
     double ts[3] = {0, 0, 0}, tt[3] = {0, 0, 0};  // d xi / d s, d xi / d t
     for (int v = 0; v < ref.nfvert; v++) {
@@ -352,6 +491,9 @@ void faceNormals(const Mesh& mesh, int64_t e, int s, Face& face) {
     for (int d = 0; d < dim; d++) face.normal[static_cast<size_t>(p) * dim + d] = nv[d] / norm;
     face.jw[p] = norm * ref.fweight[p];
   }
+}
+void faceNormals(const Mesh& mesh, int64_t e, int s, Face& face) {
+  faceNormalsRef(mesh.ref, &mesh.jac[static_cast<size_t>(e) * mesh.ref.dim * mesh.ref.dim], s, face);
 }
 }  // namespace
 
@@ -536,6 +678,182 @@ bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3]
         }
       }
   mesh.affine = true;
+  return true;
+}
+
+bool buildSplitMesh(int order, const int n[2], const double lo[2], const double hi[2], const std::vector<uint8_t>& split,
+                    MixedMesh& mesh, std::string& error) {
+  mesh = MixedMesh();
+  mesh.order = order;
+  if (!buildRefElement(2, order, mesh.ref[0], error) || !buildRefElement(3, order, mesh.ref[1], error)) return false;
+  for (int d = 0; d < 2; d++) {
+    if (n[d] < 2) {
+      error = "a periodic direction needs at least 2 cells";
+      return false;
+    }
+    if (!(hi[d] > lo[d])) {
+      error = "box upper corner must exceed the lower corner";
+      return false;
+    }
+  }
+  if (split.size() != static_cast<size_t>(n[0]) * n[1]) {
+    error = "one split flag per cell is required";
+    return false;
+  }
+  const double period[2] = {hi[0] - lo[0], hi[1] - lo[1]};
+  const double h[2] = {period[0] / n[0], period[1] / n[1]};
+  // elements
+  for (int j = 0; j < n[1]; j++)
+    for (int i = 0; i < n[0]; i++) {
+      const double A[2] = {lo[0] + i * h[0], lo[1] + j * h[1]}, B[2] = {A[0] + h[0], A[1]},
+                   C[2] = {A[0] + h[0], A[1] + h[1]}, Dv[2] = {A[0], A[1] + h[1]};
+      auto push = [&](int8_t t, std::initializer_list<const double*> vs) {
+        mesh.etype.push_back(t);
+        int k = 0;
+        for (const double* v : vs) {
+          mesh.vertex.push_back(v[0]);
+          mesh.vertex.push_back(v[1]);
+          k++;
+        }
+        for (; k < 4; k++) {
+          mesh.vertex.push_back(0.0);
+          mesh.vertex.push_back(0.0);
+        }
+      };
+      if (split[static_cast<size_t>(j) * n[0] + i]) {
+        push(0, {A, B, C});
+        push(0, {A, C, Dv});
+      } else {
+        push(1, {A, B, C, Dv});
+      }
+    }
+  const int64_t E = static_cast<int64_t>(mesh.etype.size());
+  mesh.elements = E;
+  mesh.jac.assign(static_cast<size_t>(E) * 4, 0.0);
+  mesh.detj.assign(E, 0.0);
+  mesh.min_edge.assign(E, 0.0);
+  for (int64_t e = 0; e < E; e++) {
+    const RefElement& ref = mesh.refOf(e);
+    const double xi0[2] = {ref.type == 2 ? 1.0 / 3.0 : 0.0, ref.type == 2 ? 1.0 / 3.0 : 0.0};
+    double x[2];
+    mapPointRef(ref, &mesh.vertex[static_cast<size_t>(e) * 8], xi0, x, &mesh.jac[static_cast<size_t>(e) * 4]);
+    mesh.detj[e] = det(2, &mesh.jac[static_cast<size_t>(e) * 4]);
+    if (!(mesh.detj[e] > 0.0)) {
+      error = "non-positive Jacobian";
+      return false;
+    }
+    double me = std::numeric_limits<double>::max();
+    for (int a = 0; a < ref.nvert; a++) {
+      const int b = (a + 1) % ref.nvert;
+      const double dx = mesh.vertex[static_cast<size_t>(e) * 8 + 2 * a] - mesh.vertex[static_cast<size_t>(e) * 8 + 2 * b];
+      const double dy = mesh.vertex[static_cast<size_t>(e) * 8 + 2 * a + 1] - mesh.vertex[static_cast<size_t>(e) * 8 + 2 * b + 1];
+      me = std::min(me, std::sqrt(dx * dx + dy * dy));
+    }
+    mesh.min_edge[e] = me;
+  }
+  // faces: edges keyed by their midpoint modulo the period (on a grid of quarter cells)
+  auto key = [&](int64_t e, int s) {
+    const RefElement& ref = mesh.refOf(e);
+    const int a = ref.face_vertex[s * 2], b = ref.face_vertex[s * 2 + 1];
+    int64_t k[2];
+    for (int d = 0; d < 2; d++) {
+      const double m = 0.5 * (mesh.vertex[static_cast<size_t>(e) * 8 + 2 * a + d] + mesh.vertex[static_cast<size_t>(e) * 8 + 2 * b + d]);
+      const int64_t q = static_cast<int64_t>(std::llround(4.0 * (m - lo[d]) / h[d]));
+      const int64_t per = 4 * static_cast<int64_t>(n[d]);
+      k[d] = ((q % per) + per) % per;
+    }
+    return k[0] + k[1] * 4 * static_cast<int64_t>(n[0]);
+  };
+  std::unordered_map<int64_t, std::vector<std::pair<int64_t, int>>> edges;
+  for (int64_t e = 0; e < E; e++)
+    for (int s = 0; s < mesh.refOf(e).nf; s++) edges[key(e, s)].push_back({e, s});
+  mesh.elem_face.assign(static_cast<size_t>(E) * 4, -1);
+  mesh.elem_side.assign(static_cast<size_t>(E) * 4, 0);
+  const int nfq = order + 1;
+  std::vector<double> xl(static_cast<size_t>(nfq) * 2), xr(static_cast<size_t>(nfq) * 2);
+  for (int64_t e = 0; e < E; e++) {
+    const RefElement& ref = mesh.refOf(e);
+    for (int s = 0; s < ref.nf; s++) {
+      if (mesh.elem_face[static_cast<size_t>(e) * 4 + s] >= 0) continue;
+      const auto& list = edges[key(e, s)];
+      if (list.size() != 2) {
+        error = "an edge is not shared by exactly two elements";
+        return false;
+      }
+      const auto other = list[0].first == e && list[0].second == s ? list[1] : list[0];
+      const int64_t r = other.first;
+      const int rs = other.second;
+      const RefElement& rref = mesh.refOf(r);
+      Face face;
+      face.parent[0] = static_cast<int32_t>(e);
+      face.parent[1] = static_cast<int32_t>(r);
+      face.local[0] = s;
+      face.local[1] = rs;
+      for (int p = 0; p < nfq; p++) {
+        mapPointRef(ref, &mesh.vertex[static_cast<size_t>(e) * 8], &ref.face_param[(static_cast<size_t>(s) * nfq + p) * 2],
+                    &xl[p * 2], nullptr);
+        mapPointRef(rref, &mesh.vertex[static_cast<size_t>(r) * 8], &rref.face_param[(static_cast<size_t>(rs) * nfq + p) * 2],
+                    &xr[p * 2], nullptr);
+      }
+      face.seq.assign(nfq, -1);
+      for (int p = 0; p < nfq; p++) {
+        double best = std::numeric_limits<double>::max();
+        int bi = -1;
+        for (int qq = 0; qq < nfq; qq++) {
+          double s2 = 0.0;
+          for (int d = 0; d < 2; d++) {
+            double t = xl[p * 2 + d] - xr[qq * 2 + d];
+            t -= period[d] * std::round(t / period[d]);
+            s2 += t * t;
+          }
+          if (s2 < best) {
+            best = s2;
+            bi = qq;
+          }
+        }
+        if (best > 1e-20 * (1.0 + h[0] * h[0])) {
+          error = "face points of the two parents do not match";
+          return false;
+        }
+        face.seq[p] = bi;
+      }
+      face.coord.assign(xl.begin(), xl.end());
+      faceNormalsRef(ref, &mesh.jac[static_cast<size_t>(e) * 4], s, face);
+      const int32_t fid = static_cast<int32_t>(mesh.faces.size());
+      mesh.faces.push_back(std::move(face));
+      mesh.elem_face[static_cast<size_t>(e) * 4 + s] = fid;
+      mesh.elem_side[static_cast<size_t>(e) * 4 + s] = 0;
+      mesh.elem_face[static_cast<size_t>(r) * 4 + rs] = fid;
+      mesh.elem_side[static_cast<size_t>(r) * 4 + rs] = 1;
+    }
+  }
+  return true;
+}
+
+bool singleTypeView(const MixedMesh& mm, Mesh& out) {
+  const int64_t E = mm.elements;
+  if (E == 0) return false;
+  for (int64_t e = 1; e < E; e++)
+    if (mm.etype[e] != mm.etype[0]) return false;
+  out = Mesh();
+  out.ref = mm.ref[mm.etype[0]];
+  const int nv = out.ref.nvert, nf = out.ref.nf;
+  out.elements = E;
+  out.vertex.resize(static_cast<size_t>(E) * nv * 2);
+  for (int64_t e = 0; e < E; e++)
+    for (int i = 0; i < nv * 2; i++) out.vertex[static_cast<size_t>(e) * nv * 2 + i] = mm.vertex[static_cast<size_t>(e) * 8 + i];
+  out.jac = mm.jac;
+  out.detj = mm.detj;
+  out.min_edge = mm.min_edge;
+  out.faces = mm.faces;
+  out.elem_face.resize(static_cast<size_t>(E) * nf);
+  out.elem_side.resize(static_cast<size_t>(E) * nf);
+  for (int64_t e = 0; e < E; e++)
+    for (int s = 0; s < nf; s++) {
+      out.elem_face[static_cast<size_t>(e) * nf + s] = mm.elem_face[static_cast<size_t>(e) * 4 + s];
+      out.elem_side[static_cast<size_t>(e) * nf + s] = mm.elem_side[static_cast<size_t>(e) * 4 + s];
+    }
+  out.affine = true;
   return true;
 }
 

@@ -97,6 +97,30 @@ struct Mesh {
 bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3], const double hi[3],
                          const int periodic[3], const int bc[6], Mesh& mesh, std::string& error);
 
+// Triangles and quadrangles on a periodic structured n[0] x n[1] grid: cell (i, j) (i fastest) is a
+// quadrangle, or, when split[cell] != 0, the two triangles (A, B, C), (A, C, D) of its corners
+// A = (i, j), B = (i+1, j), C = (i+1, j+1), D = (i, j+1). Faces are matched geometrically
+// (modulo the period) and created at their first visit, whose element is the left parent.
+struct MixedMesh {
+  int order = 0;
+  RefElement ref[2];               // [0] triangle, [1] quadrangle
+  int64_t elements = 0;
+  std::vector<int8_t> etype;       // per element: 0 triangle, 1 quadrangle
+  std::vector<double> vertex;      // E x 4 x 2 (a triangle uses the first 3)
+  std::vector<double> jac;         // E x 2 x 2, jac[l*2+k] = d x_l / d xi_k
+  std::vector<double> detj;        // E
+  std::vector<double> min_edge;    // E
+  std::vector<Face> faces;         // interior faces (all faces: the grid is periodic)
+  std::vector<int32_t> elem_face;  // E x 4 (-1 past the element's face count)
+  std::vector<int8_t> elem_side;   // E x 4
+  const RefElement& refOf(int64_t e) const { return ref[etype[e]]; }
+};
+
+bool buildSplitMesh(int order, const int n[2], const double lo[2], const double hi[2], const std::vector<uint8_t>& split,
+                    MixedMesh& mesh, std::string& error);
+// The mesh as a single-type Mesh; false when it holds both element types.
+bool singleTypeView(const MixedMesh& mm, Mesh& out);
+
 // Geometry derived per element and volume point (affine elements: constant Jacobian).
 //   detjw[q]           = |J| w_q
 //   tiw[q*D*D + a+D*b] = ((J^T)^-1)(a,b) |J| w_q   (column-major D x D)

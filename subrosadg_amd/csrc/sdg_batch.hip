@@ -1,0 +1,1012 @@
+// sdg_batch.hip -- batched solver: many independent 2-D compressible Euler problems on meshes of
+// mixed triangles and quadrangles, orders 1..4, advanced together. Every RK stage of the whole
+// batch is ONE persistent launch: workgroups stride over a list of work items (element-type and
+// order variant, first element, count), each handled by the variant's element tile with the
+// tile's shape fixed at compile time. Whole steps are captured in hipGraphs. The work-item size
+// per variant and the grid size are picked by a measuring autotuner (sdg_batch_tune).
+//
+// Per stage and element this is the reference's stepSolver pass chain for the Euler model
+// (TimeIntegration.cpp:327-350, SpatialDiscrete.cpp:194-266,633-748,1016-1032): volume flux,
+// numerical face flux (evaluated element-centrically with the face's own left/right roles, so both
+// parents get bit-identical +-H), residual, RK update; the traces of the new state are written for
+// the next stage. In the mixed-precision mode the traces and contractions are FP32 on deviations
+// from each problem's reference state (as in the single-problem engine); state, geometry,
+// pointwise physics and the RK accumulation stay FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdint>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sdg.h"
+#include "kit.hpp"
+#include "physics.hpp"
+#include "sdg_internal.hpp"
+
+namespace sdg {
+namespace batch {
+
+constexpr int kThreads = 256;
+constexpr int kVariants = 8;  // code = (quadrangle ? 4 : 0) + P - 1
+constexpr int NV = 4, D = 2;
+
+struct VariantDev {
+  const double* Vt;    // NB x NQ
+  const double* dV;    // (NQ D) x NB
+  const double* Vft;   // NB x NAQ
+  const double* Vf;    // NAQ x NB
+  const double* Minv;  // NB x NB reference mass inverse
+  const double* one;   // NB coefficients of the constant 1
+  const double* wq;    // NQ
+  int64_t E, u_base, t_base, s_base, e_base;
+};
+
+struct WorkItem {
+  int code;
+  int count;
+  int64_t first;
+};
+
+template <typename TC>
+struct Args {
+  double* U;
+  double* Un;
+  const TC* Tin;
+  TC* Tout;
+  const double* ginv;   // per element: (J^T)^-1 column-major 2 x 2
+  const double* detj;   // per element
+  const int* pid;       // per element: problem
+  const int64_t* nbt;   // per slot: neighbour's trace index of variable 0
+  const int* nbs;       // per slot: neighbour's trace stride (its N_aq)
+  const double* snrm;   // per slot: normal of the face (outward from its left parent)
+  const double* sjw;    // per slot: |J_f| w
+  const int8_t* sside;  // per slot: 0 left parent, 1 right parent
+  const double* dt;     // per problem
+  const double* mix;    // per problem: uref[4], pref, h0
+  const VariantDev* var;
+  const WorkItem* items;
+  int nitems;
+  int last_stage;
+  double c0, c1, c2;
+  Phys ph;
+};
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+template <int NB, int NQ, int NF, int NFQ>
+struct Shape {
+  static constexpr int NAQ = NF * NFQ;
+  static constexpr int MAXN = cmax(NB, cmax(NQ, NAQ));
+  static constexpr int EPB_RAW = kThreads / (NV * MAXN);
+  static constexpr int EPB = EPB_RAW < 1 ? 1 : (EPB_RAW > 16 ? 16 : EPB_RAW);
+  static constexpr int LDS_ELEMS = EPB * NV * (NB + NQ + NQ * D + NAQ + NB);
+};
+
+// numerical flux at one face point; kept out of line so that the eight tile variants share one copy
+__device__ __noinline__ void faceFluxPoint(const Phys& ph, const double* n, const double* ul, const double* ur, double* h) {
+  double wl[D + 3], wr[D + 3];
+  computational<D>(ph, ul, wl);
+  computational<D>(ph, ur, wr);
+  convectiveFlux<D>(ph, n, ul, wl, ur, wr, h);
+}
+
+template <typename TC>
+__device__ __forceinline__ double pointValue(const Args<TC>& a, int p, TC x, int v) {
+  if constexpr (sizeof(TC) == 8)
+    return x;
+  else
+    return a.mix[p * 6 + v] + static_cast<double>(x);
+}
+
+// One variant's elements [first, first + count), EPB at a time (out of line: each variant keeps
+// its own register allocation instead of the union of all eight).
+template <typename TC, int NB, int NQ, int NF, int NFQ>
+__device__ __noinline__ void eulerTile(const Args<TC>& a, const VariantDev& vd, int64_t first, int count, TC* smem) {
+  using S = Shape<NB, NQ, NF, NFQ>;
+  constexpr int NAQ = S::NAQ, EPB = S::EPB;
+  constexpr bool MIX = sizeof(TC) == 4;
+  TC(*sU)[NV][NB] = reinterpret_cast<TC(*)[NV][NB]>(smem);
+  TC(*sUq)[NV][NQ] = reinterpret_cast<TC(*)[NV][NQ]>(smem + EPB * NV * NB);
+  TC(*sF)[NV][NQ * D] = reinterpret_cast<TC(*)[NV][NQ * D]>(smem + EPB * NV * (NB + NQ));
+  TC(*sH)[NV][NAQ] = reinterpret_cast<TC(*)[NV][NAQ]>(smem + EPB * NV * (NB + NQ + NQ * D));
+  TC(*sR)[NV][NB] = reinterpret_cast<TC(*)[NV][NB]>(smem + EPB * NV * (NB + NQ + NQ * D + NAQ));
+  const int tid = threadIdx.x;
+  for (int64_t c0 = first; c0 < first + count; c0 += EPB) {
+    const int ne = static_cast<int>(first + count - c0 < EPB ? first + count - c0 : EPB);
+    for (int idx = tid; idx < EPB * NV * NB; idx += kThreads) {
+      const int le = idx / (NV * NB), rem = idx % (NV * NB), v = rem / NB, b = rem % NB;
+      TC x = TC(0);
+      if (le < ne) {
+        const int64_t e = c0 + le;
+        double u = a.U[vd.u_base + e * NV * NB + rem];
+        if constexpr (MIX) u -= a.mix[a.pid[vd.e_base + e] * 6 + v] * vd.one[b];
+        x = static_cast<TC>(u);
+      }
+      sU[le][v][b] = x;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < EPB * NV * NQ; idx += kThreads) {
+      const int le = idx / (NV * NQ), rem = idx % (NV * NQ), v = rem / NQ, q = rem % NQ;
+      TC s = TC(0);
+      #pragma unroll 4
+      for (int b = 0; b < NB; b++) s += sU[le][v][b] * static_cast<TC>(vd.Vt[b * NQ + q]);
+      sUq[le][v][q] = s;
+    }
+    __syncthreads();
+    // volume flux (F^T (J^T)^-1 |J| w), pointwise in FP64
+    for (int idx = tid; idx < EPB * NQ; idx += kThreads) {
+      const int le = idx / NQ, q = idx % NQ;
+      if (le >= ne) {
+        for (int v = 0; v < NV; v++)
+          for (int b = 0; b < D; b++) sF[le][v][q * D + b] = TC(0);
+        continue;
+      }
+      const int64_t eg = vd.e_base + c0 + le;
+      const int p = a.pid[eg];
+      double u[NV], w[D + 3];
+      for (int v = 0; v < NV; v++) u[v] = pointValue<TC>(a, p, sUq[le][v][q], v);
+      computational<D>(a.ph, u, w);
+      const double rho = w[0], pr = w[D + 2];
+      const double et = w[D + 1] + velocitySquared<D>(w) / 2.0;
+      double fc[D * NV];
+      for (int aa = 0; aa < D; aa++) {
+        fc[aa] = rho * w[1 + aa];
+        for (int b = 0; b < D; b++) {
+          double pd;
+          if constexpr (MIX)
+            pd = aa == b ? pr - a.mix[p * 6 + 4] : 0.0;
+          else
+            pd = aa == b ? pr * 1.0 : pr * 0.0;
+          fc[aa + D * (1 + b)] = rho * w[1 + aa] * w[1 + b] + pd;
+        }
+        if constexpr (MIX)
+          fc[aa + D * (D + 1)] = (rho * et + pr - a.mix[p * 6 + 5] * rho) * w[1 + aa];
+        else
+          fc[aa + D * (D + 1)] = (rho * et + pr) * w[1 + aa];
+      }
+      const double jw = a.detj[eg] * vd.wq[q];
+      double G[D * D];
+      for (int i = 0; i < D * D; i++) G[i] = a.ginv[eg * D * D + i] * jw;
+      for (int v = 0; v < NV; v++)
+        for (int b = 0; b < D; b++) {
+          double s = 0.0;
+          for (int aa = 0; aa < D; aa++) s += fc[aa + D * v] * G[aa + D * b];
+          sF[le][v][q * D + b] = static_cast<TC>(s);
+        }
+    }
+    // numerical face fluxes at the element's face points
+    for (int idx = tid; idx < EPB * NAQ; idx += kThreads) {
+      const int le = idx / NAQ, sl = idx % NAQ;
+      if (le >= ne) {
+        for (int v = 0; v < NV; v++) sH[le][v][sl] = TC(0);
+        continue;
+      }
+      const int64_t e = c0 + le;
+      const int p = a.pid[vd.e_base + e];
+      const int64_t gs = vd.s_base + e * NAQ + sl;
+      const int64_t own = vd.t_base + e * NV * NAQ + sl, nb = a.nbt[gs];
+      const int st = a.nbs[gs], side = a.sside[gs];
+      double ul[NV], ur[NV];
+      for (int v = 0; v < NV; v++) {
+        const double uo = pointValue<TC>(a, p, a.Tin[own + v * NAQ], v);
+        const double un = pointValue<TC>(a, p, a.Tin[nb + static_cast<int64_t>(v) * st], v);
+        ul[v] = side == 0 ? uo : un;
+        ur[v] = side == 0 ? un : uo;
+      }
+      double n[D], h[NV];
+      n[0] = a.snrm[gs * 2];
+      n[1] = a.snrm[gs * 2 + 1];
+      faceFluxPoint(a.ph, n, ul, ur, h);
+      if constexpr (MIX) {
+        h[1] -= a.mix[p * 6 + 4] * n[0];
+        h[2] -= a.mix[p * 6 + 4] * n[1];
+        h[3] -= a.mix[p * 6 + 5] * h[0];
+      }
+      const double jw = a.sjw[gs];
+      for (int v = 0; v < NV; v++) {
+        const double t = h[v] * jw;
+        sH[le][v][sl] = static_cast<TC>(side == 0 ? t : -t);
+      }
+    }
+    __syncthreads();
+    // residual R = F Phi_grad - H Phi_face
+    for (int idx = tid; idx < EPB * NV * NB; idx += kThreads) {
+      const int le = idx / (NV * NB), rem = idx % (NV * NB), v = rem / NB, i = rem % NB;
+      TC r = TC(0), hsum = TC(0);
+      #pragma unroll 4
+      for (int k = 0; k < D * NQ; k++) r += sF[le][v][k] * static_cast<TC>(vd.dV[k * NB + i]);
+      #pragma unroll 4
+      for (int k = 0; k < NAQ; k++) hsum += sH[le][v][k] * static_cast<TC>(vd.Vf[k * NB + i]);
+      sR[le][v][i] = r - hsum;
+    }
+    __syncthreads();
+    // RK update in FP64: U <- c1 U + c0 Un + c2 dt (R Mref^-1) / |J| (TimeIntegration.cpp:181-198)
+    for (int idx = tid; idx < EPB * NV * NB; idx += kThreads) {
+      const int le = idx / (NV * NB), rem = idx % (NV * NB), v = rem / NB, i = rem % NB;
+      if (le >= ne) continue;
+      const int64_t e = c0 + le, eg = vd.e_base + e;
+      const int p = a.pid[eg];
+      double rm = 0.0;
+      #pragma unroll 4
+      for (int b = 0; b < NB; b++) rm += static_cast<double>(sR[le][v][b]) * vd.Minv[b * NB + i];
+      if constexpr (MIX) {
+        if (v == NV - 1) {  // energy: add back h0 R_rho
+          double rr = 0.0;
+          #pragma unroll 4
+          for (int b = 0; b < NB; b++) rr += static_cast<double>(sR[le][0][b]) * vd.Minv[b * NB + i];
+          rm += a.mix[p * 6 + 5] * rr;
+        }
+      }
+      rm = rm / a.detj[eg];
+      const int64_t gi = vd.u_base + e * NV * NB + rem;
+      double u = a.U[gi] * a.c1;
+      u += a.c0 * a.Un[gi];
+      u += a.c2 * a.dt[p] * rm;
+      a.U[gi] = u;
+      if (a.last_stage) a.Un[gi] = u;
+      if constexpr (MIX) u -= a.mix[p * 6 + v] * vd.one[i];
+      sU[le][v][i] = static_cast<TC>(u);
+    }
+    __syncthreads();
+    // traces of the new state
+    for (int idx = tid; idx < EPB * NV * NAQ; idx += kThreads) {
+      const int le = idx / (NV * NAQ), rem = idx % (NV * NAQ), v = rem / NAQ, sl = rem % NAQ;
+      if (le >= ne) continue;
+      TC t = TC(0);
+      #pragma unroll 4
+      for (int b = 0; b < NB; b++) t += sU[le][v][b] * static_cast<TC>(vd.Vft[b * NAQ + sl]);
+      a.Tout[vd.t_base + (c0 + le) * NV * NAQ + rem] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// variant table: (NB, NQ, NF, NFQ) of triangles and quadrangles P1..P4
+#define SDG_BATCH_VARIANTS(X) \
+  X(0, 3, 3, 3, 2)            \
+  X(1, 6, 6, 3, 3)            \
+  X(2, 10, 12, 3, 4)          \
+  X(3, 15, 16, 3, 5)          \
+  X(4, 4, 4, 4, 2)            \
+  X(5, 9, 9, 4, 3)            \
+  X(6, 16, 16, 4, 4)          \
+  X(7, 25, 25, 4, 5)
+
+#define SDG_BATCH_LDS(c, nb, nq, nf, nfq) cmax(Shape<nb, nq, nf, nfq>::LDS_ELEMS,
+constexpr int kMaxLdsElems = SDG_BATCH_VARIANTS(SDG_BATCH_LDS) 0 ) ) ) ) ) ) ) );
+#undef SDG_BATCH_LDS
+
+#define SDG_BATCH_EPB(c, nb, nq, nf, nfq) Shape<nb, nq, nf, nfq>::EPB,
+constexpr int kEpb[kVariants] = {SDG_BATCH_VARIANTS(SDG_BATCH_EPB)};
+#undef SDG_BATCH_EPB
+
+// One RK stage of the whole batch: a persistent grid striding over the work items.
+template <typename TC>
+__global__ __launch_bounds__(kThreads) void batchStageKernel(Args<TC> a) {
+  __shared__ TC smem[kMaxLdsElems];
+  for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
+    const WorkItem w = a.items[it];
+    const VariantDev& vd = a.var[w.code];
+    switch (w.code) {
+#define SDG_BATCH_CASE(c, nb, nq, nf, nfq) \
+  case c:                                  \
+    eulerTile<TC, nb, nq, nf, nfq>(a, vd, w.first, w.count, smem); \
+    break;
+      SDG_BATCH_VARIANTS(SDG_BATCH_CASE)
+#undef SDG_BATCH_CASE
+      default:
+        break;
+    }
+  }
+}
+
+// traces of the state (initial state, or after host writes)
+template <typename TC>
+__global__ __launch_bounds__(kThreads) void batchTraceKernel(Args<TC> a, int code, int nb_, int naq) {
+  const VariantDev& vd = a.var[code];
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (idx >= vd.E * NV * naq) return;
+  const int64_t e = idx / (NV * naq);
+  const int rem = static_cast<int>(idx % (NV * naq)), v = rem / naq, sl = rem % naq;
+  const int p = a.pid[vd.e_base + e];
+  double t = 0.0;
+  for (int b = 0; b < nb_; b++) {
+    double u = a.U[vd.u_base + (e * NV + v) * nb_ + b];
+    if constexpr (sizeof(TC) == 4) u -= a.mix[p * 6 + v] * vd.one[b];
+    t += u * vd.Vft[b * naq + sl];
+  }
+  a.Tout[vd.t_base + idx] = static_cast<TC>(t);
+}
+
+// CFL time step per element (TimeIntegration.cpp:104-131)
+__global__ __launch_bounds__(kThreads) void batchDtKernel(const double* U, const VariantDev* var, int code, int nb_,
+                                                          int nq, int order, Phys ph, const double* min_edge, double cfl,
+                                                          double* out) {
+  const VariantDev& vd = var[code];
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (e >= vd.E) return;
+  double best = 1.0e308;
+  for (int q = 0; q < nq; q++) {
+    double u[NV], w[D + 3];
+    for (int v = 0; v < NV; v++) {
+      double s = 0.0;
+      for (int b = 0; b < nb_; b++) s += U[vd.u_base + (e * NV + v) * nb_ + b] * vd.Vt[b * nq + q];
+      u[v] = s;
+    }
+    computational<D>(ph, u, w);
+    const double c = soundSpeedOf(ph, w[0], w[D + 2]);
+    const double sr = sqrt(velocitySquared<D>(w)) + c;
+    best = fmin(best, cfl * min_edge[vd.e_base + e] / (sr * (order + 1.0) * (order + 1.0)));
+  }
+  out[vd.e_base + e] = best;
+}
+
+}  // namespace batch
+}  // namespace sdg
+
+// ---- host side ------------------------------------------------------------------------------------
+struct sdg_batch {
+  std::string err;
+  sdg_problem_desc prob{};
+  sdg::Phys phys{};
+  bool mixed = false;
+  int nprob = 0;
+  std::vector<const sdg::MixedMesh*> meshes;
+  struct Var {
+    bool used = false;
+    int type = 0, order = 0, nb = 0, nq = 0, nf = 0, nfq = 0, naq = 0;
+    int64_t E = 0, u_base = 0, t_base = 0, s_base = 0, e_base = 0;
+    sdg::RefElement ref;
+    std::vector<double> one, mean;
+  } var[sdg::batch::kVariants];
+  std::vector<std::vector<std::pair<int, int64_t>>> emap;  // problem, element -> (variant, index)
+  std::vector<std::vector<int64_t>> coeff_off;             // problem, element -> offset in reference layout
+  int64_t totalE = 0, totalU = 0, totalT = 0, totalS = 0;
+  double *U = nullptr, *Un = nullptr, *ginv = nullptr, *detj = nullptr, *snrm = nullptr, *sjw = nullptr, *dt = nullptr,
+         *mix = nullptr, *min_edge = nullptr, *dtbuf = nullptr;
+  int* pid = nullptr;
+  int64_t* nbt = nullptr;
+  int* nbs = nullptr;
+  int8_t* sside = nullptr;
+  void* T[2] = {nullptr, nullptr};
+  sdg::batch::VariantDev* dvar = nullptr;
+  sdg::batch::WorkItem* ditems = nullptr;
+  int ditems_cap = 0;
+  std::vector<sdg::batch::WorkItem> items;
+  int chunk[sdg::batch::kVariants] = {1, 1, 1, 1, 1, 1, 1, 1};  // work-item size in tiles
+  int grid = 0;
+  std::vector<double> mixh;
+  std::vector<double> host_dt;
+  std::vector<char> has_state;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int cur = 0;
+  float last_ms = 0.0f;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+using sdg::batch::Args;
+using sdg::batch::VariantDev;
+using sdg::batch::WorkItem;
+
+int bfail(sdg_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  sdg::setGlobalError(msg);
+  return code;
+}
+
+#define BHIP(b, call)                                                                                     \
+  do {                                                                                                    \
+    hipError_t e_ = (call);                                                                               \
+    if (e_ != hipSuccess) return bfail((b), SDG_E_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int balloc(sdg_batch* b, size_t count, T** out) {
+  void* p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return bfail(b, SDG_E_NOMEM, "hipMalloc failed");
+  b->allocs.push_back(p);
+  *out = static_cast<T*>(p);
+  return SDG_OK;
+}
+template <typename T>
+int bupload(sdg_batch* b, const std::vector<T>& h, T** out) {
+  int rc = balloc(b, h.size(), out);
+  if (rc) return rc;
+  if (!h.empty() && hipMemcpy(*out, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+    return bfail(b, SDG_E_DEVICE, "hipMemcpy failed");
+  return SDG_OK;
+}
+
+int variantCode(int type, int order) { return (type == SDG_ELEM_QUADRANGLE ? 4 : 0) + order - 1; }
+
+template <typename TC>
+Args<TC> makeArgs(sdg_batch* b, int parity) {
+  Args<TC> a{};
+  a.U = b->U;
+  a.Un = b->Un;
+  a.Tin = static_cast<const TC*>(b->T[parity]);
+  a.Tout = static_cast<TC*>(b->T[parity ^ 1]);
+  a.ginv = b->ginv;
+  a.detj = b->detj;
+  a.pid = b->pid;
+  a.nbt = b->nbt;
+  a.nbs = b->nbs;
+  a.snrm = b->snrm;
+  a.sjw = b->sjw;
+  a.sside = b->sside;
+  a.dt = b->dt;
+  a.mix = b->mix;
+  a.var = b->dvar;
+  a.items = b->ditems;
+  a.nitems = static_cast<int>(b->items.size());
+  a.ph = b->phys;
+  return a;
+}
+
+void rkCoeff(const sdg_batch* b, int stage, double c[3]) {  // TimeIntegration.cpp:45-65
+  const int rk = b->prob.time_integration;
+  if (rk == SDG_RK_FORWARD_EULER) {
+    c[0] = 1.0; c[1] = 0.0; c[2] = 1.0;
+  } else if (rk == SDG_RK_HEUN2) {
+    const double t[2][3] = {{1.0, 0.0, 1.0}, {0.5, 0.5, 0.5}};
+    for (int i = 0; i < 3; i++) c[i] = t[stage][i];
+  } else {
+    const double t[3][3] = {{1.0, 0.0, 1.0}, {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    for (int i = 0; i < 3; i++) c[i] = t[stage][i];
+  }
+}
+int stagesOf(const sdg_batch* b) {
+  return b->prob.time_integration == SDG_RK_FORWARD_EULER ? 1 : (b->prob.time_integration == SDG_RK_HEUN2 ? 2 : 3);
+}
+
+// work items: per variant, chunks of chunk[v] tiles
+int buildItems(sdg_batch* b, const int* only = nullptr) {
+  b->items.clear();
+  for (int v = 0; v < sdg::batch::kVariants; v++) {
+    const auto& V = b->var[v];
+    if (!V.used || (only && v != *only)) continue;
+    const int64_t step = static_cast<int64_t>(b->chunk[v]) * sdg::batch::kEpb[v];
+    for (int64_t f = 0; f < V.E; f += step) b->items.push_back({v, static_cast<int>(std::min<int64_t>(step, V.E - f)), f});
+  }
+  if (static_cast<int>(b->items.size()) > b->ditems_cap) {
+    sdg::batch::WorkItem* p = nullptr;
+    if (balloc(b, b->items.size(), &p)) return SDG_E_NOMEM;
+    b->ditems = p;
+    b->ditems_cap = static_cast<int>(b->items.size());
+  }
+  if (!b->items.empty())
+    BHIP(b, hipMemcpy(b->ditems, b->items.data(), b->items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+  for (int i = 0; i < 2; i++) {
+    if (b->graph[i]) (void)hipGraphExecDestroy(b->graph[i]);
+    b->graph[i] = nullptr;
+  }
+  return SDG_OK;
+}
+
+void launchStage(sdg_batch* b, int stage, int parity) {
+  double c[3];
+  rkCoeff(b, stage, c);
+  const int grid = std::max(1, std::min<int>(b->grid, static_cast<int>(b->items.size())));
+  if (b->mixed) {
+    Args<float> a = makeArgs<float>(b, parity);
+    a.c0 = c[0];
+    a.c1 = c[1];
+    a.c2 = c[2];
+    a.last_stage = stage == stagesOf(b) - 1;
+    hipLaunchKernelGGL(sdg::batch::batchStageKernel<float>, dim3(grid), dim3(sdg::batch::kThreads), 0, b->stream, a);
+  } else {
+    Args<double> a = makeArgs<double>(b, parity);
+    a.c0 = c[0];
+    a.c1 = c[1];
+    a.c2 = c[2];
+    a.last_stage = stage == stagesOf(b) - 1;
+    hipLaunchKernelGGL(sdg::batch::batchStageKernel<double>, dim3(grid), dim3(sdg::batch::kThreads), 0, b->stream, a);
+  }
+}
+
+void launchTraces(sdg_batch* b, int parity) {
+  for (int v = 0; v < sdg::batch::kVariants; v++) {
+    const auto& V = b->var[v];
+    if (!V.used) continue;
+    const int64_t n = V.E * sdg::batch::NV * V.naq;
+    const dim3 grid(static_cast<unsigned>((n + sdg::batch::kThreads - 1) / sdg::batch::kThreads));
+    if (b->mixed) {
+      Args<float> a = makeArgs<float>(b, parity ^ 1);  // writes T[parity]
+      hipLaunchKernelGGL(sdg::batch::batchTraceKernel<float>, grid, dim3(sdg::batch::kThreads), 0, b->stream, a, v, V.nb, V.naq);
+    } else {
+      Args<double> a = makeArgs<double>(b, parity ^ 1);
+      hipLaunchKernelGGL(sdg::batch::batchTraceKernel<double>, grid, dim3(sdg::batch::kThreads), 0, b->stream, a, v, V.nb, V.naq);
+    }
+  }
+}
+
+// mixed-precision reference state of problem p from its coefficients (reference layout)
+void updateMix(sdg_batch* b, int p, const double* coeff) {
+  const sdg::MixedMesh& m = *b->meshes[p];
+  double u[4] = {0, 0, 0, 0};
+  for (int64_t e = 0; e < m.elements; e++) {
+    const auto& V = b->var[b->emap[p][e].first];
+    const double* c = coeff + b->coeff_off[p][e];
+    for (int v = 0; v < 4; v++)
+      for (int k = 0; k < V.nb; k++) u[v] += c[k * 4 + v] * V.mean[k];
+  }
+  for (int v = 0; v < 4; v++) u[v] /= static_cast<double>(m.elements);
+  double* o = &b->mixh[static_cast<size_t>(p) * 6];
+  for (int v = 0; v < 4; v++) o[v] = u[v];
+  const double rho = u[0], vx = u[1] / rho, vy = u[2] / rho;
+  const double e = u[3] / rho - (vx * vx + vy * vy) / 2.0;
+  const sdg::Phys& ph = b->phys;
+  o[4] = ph.eos == SDG_EOS_IDEAL_GAS ? (sdg::kGamma - 1.0) * rho * e : ph.c0 * ph.c0 * (rho - ph.rho0) + ph.padd;
+  o[5] = (u[3] + o[4]) / rho;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sdg_batch_last_error(const sdg_batch* b) { return b ? b->err.c_str() : sdg::globalError(); }
+
+int sdg_batch_create(const sdg_problem_desc* problem, const sdg_mesh* const* meshes, int count, sdg_batch** out) {
+  if (!problem || !meshes || !out || count < 1) return bfail(nullptr, SDG_E_INVALID, "sdg_batch_create: bad argument");
+  *out = nullptr;
+  if (problem->abi_version != SDG_ABI_VERSION) return bfail(nullptr, SDG_E_INVALID, "sdg_batch_create: abi_version mismatch");
+  if (problem->equation != SDG_EQ_COMPRESSIBLE_EULER)
+    return bfail(nullptr, SDG_E_UNSUPPORTED, "sdg_batch_create: the batched solver runs the compressible Euler model");
+  if (problem->convective_flux < SDG_FLUX_CENTRAL || problem->convective_flux > SDG_FLUX_EXACT ||
+      problem->time_integration < SDG_RK_FORWARD_EULER || problem->time_integration > SDG_RK_SSPRK3 ||
+      !(problem->cv > 0.0) || !(problem->cp > 0.0))
+    return bfail(nullptr, SDG_E_INVALID, "sdg_batch_create: bad problem descriptor");
+  sdg_batch* b = new (std::nothrow) sdg_batch();
+  if (!b) return bfail(nullptr, SDG_E_NOMEM, "sdg_batch_create: out of host memory");
+  auto bail = [&](int code) {
+    std::string m = b->err;
+    sdg_batch_destroy(b);
+    sdg::setGlobalError(m);
+    return code;
+  };
+  b->prob = *problem;
+  b->mixed = problem->precision == SDG_PREC_MIXED;
+  b->nprob = count;
+  sdg::Phys& ph = b->phys;
+  ph.eos = problem->equation_of_state;
+  ph.transport = SDG_TRANSPORT_NONE;
+  ph.flux = problem->convective_flux;
+  ph.cp = problem->cp;
+  ph.cv = problem->cv;
+  ph.c0 = problem->reference_sound_speed;
+  ph.rho0 = problem->reference_density;
+  ph.padd = 0.01 * problem->reference_density * problem->reference_sound_speed * problem->reference_sound_speed;
+  for (int p = 0; p < count; p++) {
+    const sdg::MixedMesh* m = sdg::mixedOf(meshes[p]);
+    if (!m) {
+      b->err = "sdg_batch_create: meshes must come from sdg_mesh_create_split";
+      return bail(SDG_E_UNSUPPORTED);
+    }
+    if (m->order < 1 || m->order > 4) {
+      b->err = "sdg_batch_create: orders 1..4 are supported";
+      return bail(SDG_E_UNSUPPORTED);
+    }
+    b->meshes.push_back(m);
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+    b->err = "sdg_batch_create: no HIP device / stream";
+    return bail(SDG_E_DEVICE);
+  }
+  // variants present and their element ranges (problem-major within a variant)
+  b->emap.resize(count);
+  b->coeff_off.resize(count);
+  for (int p = 0; p < count; p++) {
+    const sdg::MixedMesh& m = *b->meshes[p];
+    b->emap[p].resize(m.elements);
+    b->coeff_off[p].resize(m.elements);
+    int64_t off = 0;
+    for (int64_t e = 0; e < m.elements; e++) {
+      const sdg::RefElement& r = m.refOf(e);
+      const int v = variantCode(r.type, r.order);
+      auto& V = b->var[v];
+      if (!V.used) {
+        V.used = true;
+        V.type = r.type;
+        V.order = r.order;
+        V.ref = r;
+        V.nb = r.nb;
+        V.nq = r.nq;
+        V.nf = r.nf;
+        V.nfq = r.nfq;
+        V.naq = r.nf * r.nfq;
+      }
+      b->emap[p][e] = {v, V.E++};
+      b->coeff_off[p][e] = off;
+      off += static_cast<int64_t>(r.nb) * 4;
+    }
+  }
+  // re-index variant elements problem-major: index = (running count at creation) already is
+  int64_t eb = 0, ub = 0, tb = 0, sb = 0;
+  for (auto& V : b->var) {
+    if (!V.used) continue;
+    V.e_base = eb;
+    V.u_base = ub;
+    V.t_base = tb;
+    V.s_base = sb;
+    eb += V.E;
+    ub += V.E * 4 * V.nb;
+    tb += V.E * 4 * V.naq;
+    sb += V.E * V.naq;
+    // constant-function coefficients and basis means (as in the single-problem engine)
+    const sdg::RefElement& r = V.ref;
+    std::vector<double> colsum(r.nb, 0.0);
+    double wsum = 0.0;
+    V.mean.assign(r.nb, 0.0);
+    for (int q = 0; q < r.nq; q++) {
+      wsum += r.qweight[q];
+      for (int k = 0; k < r.nb; k++) {
+        colsum[k] += r.V[static_cast<size_t>(q) * r.nb + k];
+        V.mean[k] += r.qweight[q] * r.V[static_cast<size_t>(q) * r.nb + k];
+      }
+    }
+    for (int k = 0; k < r.nb; k++) V.mean[k] /= wsum;
+    V.one.assign(r.nb, 0.0);
+    for (int k = 0; k < r.nb; k++) {
+      double x = 0.0;
+      for (int c = 0; c < r.nb; c++) x += colsum[c] * r.lsq_inverse[static_cast<size_t>(c) * r.nb + k];
+      if (std::fabs(x) < 1e-12) x = 0.0;
+      if (std::fabs(x - 1.0) < 1e-12) x = 1.0;
+      V.one[k] = x;
+    }
+  }
+  b->totalE = eb;
+  b->totalU = ub;
+  b->totalT = tb;
+  b->totalS = sb;
+  // per element and per slot tables
+  std::vector<double> ginv(static_cast<size_t>(eb) * 4), detj(eb), me(eb), snrm(static_cast<size_t>(sb) * 2), sjw(sb);
+  std::vector<int> pid(eb), nbs(sb);
+  std::vector<int64_t> nbt(sb);
+  std::vector<int8_t> sside(sb);
+  for (int p = 0; p < count; p++) {
+    const sdg::MixedMesh& m = *b->meshes[p];
+    for (int64_t e = 0; e < m.elements; e++) {
+      const auto [v, k] = b->emap[p][e];
+      const auto& V = b->var[v];
+      const int64_t ge = V.e_base + k;
+      const double* J = &m.jac[static_cast<size_t>(e) * 4];
+      const double jt[4] = {J[0], J[2], J[1], J[3]};
+      double jti[4];
+      sdg::invertMatrix(2, jt, jti);
+      for (int aa = 0; aa < 2; aa++)
+        for (int bb = 0; bb < 2; bb++) ginv[static_cast<size_t>(ge) * 4 + aa + 2 * bb] = jti[aa * 2 + bb];
+      detj[ge] = m.detj[e];
+      me[ge] = m.min_edge[e];
+      pid[ge] = p;
+      for (int s = 0; s < V.nf; s++) {
+        const int fid = m.elem_face[static_cast<size_t>(e) * 4 + s];
+        const int side = m.elem_side[static_cast<size_t>(e) * 4 + s];
+        const sdg::Face& f = m.faces[fid];
+        const int64_t r = f.parent[side ^ 1];
+        const int rs = f.local[side ^ 1];
+        const auto [rv, rk] = b->emap[p][r];
+        const auto& RV = b->var[rv];
+        for (int j = 0; j < V.nfq; j++) {
+          int pl = j, pr = f.seq[j];  // left point index and the right's index of that point
+          if (side == 1) {
+            for (int q = 0; q < V.nfq; q++)
+              if (f.seq[q] == j) pl = q;
+            pr = j;
+          }
+          const int kpt = side == 0 ? pr : pl;  // the neighbour's point index
+          const int64_t gs = V.s_base + k * V.naq + s * V.nfq + j;
+          nbt[gs] = RV.t_base + rk * 4 * RV.naq + rs * RV.nfq + kpt;
+          nbs[gs] = RV.naq;
+          snrm[gs * 2] = f.normal[static_cast<size_t>(pl) * 2];
+          snrm[gs * 2 + 1] = f.normal[static_cast<size_t>(pl) * 2 + 1];
+          sjw[gs] = f.jw[pl];
+          sside[gs] = static_cast<int8_t>(side);
+        }
+      }
+    }
+  }
+  std::vector<VariantDev> dv(sdg::batch::kVariants);
+  for (int v = 0; v < sdg::batch::kVariants; v++) {
+    auto& V = b->var[v];
+    VariantDev& d = dv[v];
+    d = VariantDev{};
+    if (!V.used) continue;
+    const sdg::RefElement& r = V.ref;
+    std::vector<double> Vt(static_cast<size_t>(r.nb) * r.nq), Vft(static_cast<size_t>(r.nb) * V.naq);
+    for (int q = 0; q < r.nq; q++)
+      for (int k = 0; k < r.nb; k++) Vt[static_cast<size_t>(k) * r.nq + q] = r.V[static_cast<size_t>(q) * r.nb + k];
+    for (int s = 0; s < V.naq; s++)
+      for (int k = 0; k < r.nb; k++) Vft[static_cast<size_t>(k) * V.naq + s] = r.Vf[static_cast<size_t>(s) * r.nb + k];
+    double *pVt, *pdV, *pVft, *pVf, *pM, *pone, *pw;
+    if (bupload(b, Vt, &pVt) || bupload(b, r.dV, &pdV) || bupload(b, Vft, &pVft) || bupload(b, r.Vf, &pVf) ||
+        bupload(b, r.mass_inverse, &pM) || bupload(b, V.one, &pone) || bupload(b, r.qweight, &pw))
+      return bail(SDG_E_NOMEM);
+    d = VariantDev{pVt, pdV, pVft, pVf, pM, pone, pw, V.E, V.u_base, V.t_base, V.s_base, V.e_base};
+  }
+  const size_t tbytes = b->mixed ? sizeof(float) : sizeof(double);
+  void *t0 = nullptr, *t1 = nullptr;
+  if (bupload(b, dv, &b->dvar) || bupload(b, ginv, &b->ginv) || bupload(b, detj, &b->detj) || bupload(b, me, &b->min_edge) ||
+      bupload(b, pid, &b->pid) || bupload(b, nbt, &b->nbt) || bupload(b, nbs, &b->nbs) || bupload(b, snrm, &b->snrm) ||
+      bupload(b, sjw, &b->sjw) || bupload(b, sside, &b->sside) || balloc(b, static_cast<size_t>(ub), &b->U) ||
+      balloc(b, static_cast<size_t>(ub), &b->Un) || balloc(b, static_cast<size_t>(count), &b->dt) ||
+      balloc(b, static_cast<size_t>(count) * 6, &b->mix) || balloc(b, static_cast<size_t>(eb), &b->dtbuf) ||
+      balloc(b, static_cast<size_t>(tb) * tbytes, reinterpret_cast<char**>(&t0)) ||
+      balloc(b, static_cast<size_t>(tb) * tbytes, reinterpret_cast<char**>(&t1)))
+    return bail(SDG_E_NOMEM);
+  b->T[0] = t0;
+  b->T[1] = t1;
+  if (hipMemset(b->dt, 0, sizeof(double) * count) != hipSuccess || hipMemset(b->mix, 0, sizeof(double) * 6 * count) != hipSuccess) {
+    b->err = "sdg_batch_create: device initialisation failed";
+    return bail(SDG_E_DEVICE);
+  }
+  b->mixh.assign(static_cast<size_t>(count) * 6, 0.0);
+  b->host_dt.assign(count, 0.0);
+  b->has_state.assign(count, 0);
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  b->grid = cus * 4;
+  for (int v = 0; v < sdg::batch::kVariants; v++) b->chunk[v] = 4;
+  if (buildItems(b)) return bail(SDG_E_DEVICE);
+  *out = b;
+  return SDG_OK;
+}
+
+void sdg_batch_destroy(sdg_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (int i = 0; i < 2; i++)
+    if (b->graph[i]) (void)hipGraphExecDestroy(b->graph[i]);
+  for (void* p : b->allocs) (void)hipFree(p);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int sdg_batch_set_state(sdg_batch* b, int p, const double* coeff) {
+  if (!b || !coeff || p < 0 || p >= b->nprob) return bfail(b, SDG_E_INVALID, "sdg_batch_set_state: bad argument");
+  const sdg::MixedMesh& m = *b->meshes[p];
+  updateMix(b, p, coeff);
+  // scatter into the device layout of each variant: the problem's elements of one variant are
+  // contiguous there, so one copy per variant
+  int64_t lo[sdg::batch::kVariants], hi[sdg::batch::kVariants];
+  for (int v = 0; v < sdg::batch::kVariants; v++) {
+    lo[v] = INT64_MAX;
+    hi[v] = -1;
+  }
+  for (int64_t e = 0; e < m.elements; e++) {
+    const auto [v, k] = b->emap[p][e];
+    lo[v] = std::min(lo[v], k);
+    hi[v] = std::max(hi[v], k);
+  }
+  std::vector<std::vector<double>> host(sdg::batch::kVariants);
+  for (int v = 0; v < sdg::batch::kVariants; v++)
+    if (hi[v] >= 0) host[v].assign(static_cast<size_t>(hi[v] - lo[v] + 1) * 4 * b->var[v].nb, 0.0);
+  for (int64_t e = 0; e < m.elements; e++) {
+    const auto [v, k] = b->emap[p][e];
+    const auto& V = b->var[v];
+    double* panel = &host[v][static_cast<size_t>(k - lo[v]) * 4 * V.nb];
+    const double* c = coeff + b->coeff_off[p][e];
+    for (int vv = 0; vv < 4; vv++)
+      for (int i = 0; i < V.nb; i++) panel[static_cast<size_t>(vv) * V.nb + i] = c[i * 4 + vv];
+  }
+  for (int v = 0; v < sdg::batch::kVariants; v++) {
+    if (hi[v] < 0) continue;
+    const size_t off = static_cast<size_t>(b->var[v].u_base + lo[v] * 4 * b->var[v].nb);
+    BHIP(b, hipMemcpyAsync(b->U + off, host[v].data(), host[v].size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->Un + off, host[v].data(), host[v].size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  }
+  BHIP(b, hipStreamSynchronize(b->stream));
+  BHIP(b, hipMemcpyAsync(b->mix + static_cast<size_t>(p) * 6, &b->mixh[static_cast<size_t>(p) * 6], 6 * sizeof(double),
+                         hipMemcpyHostToDevice, b->stream));
+  b->has_state[p] = 1;
+  b->cur = 0;
+  launchTraces(b, 0);
+  BHIP(b, hipGetLastError());
+  BHIP(b, hipStreamSynchronize(b->stream));
+  for (int i = 0; i < 2; i++) {
+    if (b->graph[i]) (void)hipGraphExecDestroy(b->graph[i]);
+    b->graph[i] = nullptr;
+  }
+  return SDG_OK;
+}
+
+int sdg_batch_get_state(sdg_batch* b, int p, double* coeff) {
+  if (!b || !coeff || p < 0 || p >= b->nprob) return bfail(b, SDG_E_INVALID, "sdg_batch_get_state: bad argument");
+  const sdg::MixedMesh& m = *b->meshes[p];
+  std::vector<double> host(static_cast<size_t>(b->totalU));
+  BHIP(b, hipMemcpyAsync(host.data(), b->U, host.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  for (int64_t e = 0; e < m.elements; e++) {
+    const auto [v, k] = b->emap[p][e];
+    const auto& V = b->var[v];
+    const double* panel = &host[static_cast<size_t>(V.u_base + k * 4 * V.nb)];
+    double* c = coeff + b->coeff_off[p][e];
+    for (int vv = 0; vv < 4; vv++)
+      for (int i = 0; i < V.nb; i++) c[i * 4 + vv] = panel[static_cast<size_t>(vv) * V.nb + i];
+  }
+  return SDG_OK;
+}
+
+int sdg_batch_set_initial_primitive(sdg_batch* b, int p, const double* prim) {
+  if (!b || !prim || p < 0 || p >= b->nprob) return bfail(b, SDG_E_INVALID, "sdg_batch_set_initial_primitive: bad argument");
+  const sdg::MixedMesh& m = *b->meshes[p];
+  std::vector<double> coeff(b->coeff_off[p].empty() ? 0 : static_cast<size_t>(b->coeff_off[p].back()) +
+                                                              4 * static_cast<size_t>(m.refOf(m.elements - 1).nb));
+  int64_t qoff = 0;
+  for (int64_t e = 0; e < m.elements; e++) {  // InitialCondition.cpp:85-104
+    const sdg::RefElement& r = m.refOf(e);
+    std::vector<double> uq(static_cast<size_t>(r.nq) * 4), t(static_cast<size_t>(4) * r.nb);
+    for (int q = 0; q < r.nq; q++) {
+      const double* pr = prim + (qoff + q) * 4;
+      const double rho = pr[0];
+      uq[q * 4] = rho;
+      uq[q * 4 + 1] = rho * pr[1];
+      uq[q * 4 + 2] = rho * pr[2];
+      uq[q * 4 + 3] = rho * (b->phys.cv * pr[3] + (pr[1] * pr[1] + pr[2] * pr[2]) / 2.0);
+    }
+    qoff += r.nq;
+    for (int v = 0; v < 4; v++)
+      for (int k = 0; k < r.nb; k++) {
+        double s = 0.0;
+        for (int q = 0; q < r.nq; q++) s += uq[q * 4 + v] * r.V[static_cast<size_t>(q) * r.nb + k];
+        t[static_cast<size_t>(v) * r.nb + k] = s;
+      }
+    double* c = &coeff[static_cast<size_t>(b->coeff_off[p][e])];
+    for (int v = 0; v < 4; v++)
+      for (int k = 0; k < r.nb; k++) {
+        double s = 0.0;
+        for (int cc = 0; cc < r.nb; cc++) s += t[static_cast<size_t>(v) * r.nb + cc] * r.lsq_inverse[static_cast<size_t>(cc) * r.nb + k];
+        c[k * 4 + v] = s;
+      }
+  }
+  return sdg_batch_set_state(b, p, coeff.data());
+}
+
+int sdg_batch_compute_dt(sdg_batch* b, double cfl, double* dt) {
+  if (!b || !dt) return bfail(b, SDG_E_INVALID, "sdg_batch_compute_dt: bad argument");
+  for (int p = 0; p < b->nprob; p++)
+    if (!b->has_state[p]) return bfail(b, SDG_E_STATE, "sdg_batch_compute_dt: a problem has no state");
+  for (int v = 0; v < sdg::batch::kVariants; v++) {
+    const auto& V = b->var[v];
+    if (!V.used) continue;
+    hipLaunchKernelGGL(sdg::batch::batchDtKernel, dim3(static_cast<unsigned>((V.E + sdg::batch::kThreads - 1) / sdg::batch::kThreads)),
+                       dim3(sdg::batch::kThreads), 0, b->stream, b->U, b->dvar, v, V.nb, V.nq, V.order, b->phys,
+                       b->min_edge, cfl, b->dtbuf);
+  }
+  BHIP(b, hipGetLastError());
+  std::vector<double> h(static_cast<size_t>(b->totalE));
+  std::vector<int> pid(static_cast<size_t>(b->totalE));
+  BHIP(b, hipMemcpyAsync(h.data(), b->dtbuf, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipMemcpyAsync(pid.data(), b->pid, pid.size() * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  for (int p = 0; p < b->nprob; p++) dt[p] = 1.0e308;
+  for (int64_t e = 0; e < b->totalE; e++) dt[pid[e]] = std::min(dt[pid[e]], h[e]);
+  return SDG_OK;
+}
+
+int sdg_batch_step(sdg_batch* b, const double* dt, int nsteps) {
+  if (!b || !dt || nsteps < 0) return bfail(b, SDG_E_INVALID, "sdg_batch_step: bad argument");
+  for (int p = 0; p < b->nprob; p++)
+    if (!b->has_state[p]) return bfail(b, SDG_E_STATE, "sdg_batch_step: a problem has no state");
+  if (std::memcmp(b->host_dt.data(), dt, sizeof(double) * b->nprob) != 0) {
+    std::memcpy(b->host_dt.data(), dt, sizeof(double) * b->nprob);
+    BHIP(b, hipMemcpyAsync(b->dt, b->host_dt.data(), sizeof(double) * b->nprob, hipMemcpyHostToDevice, b->stream));
+  }
+  const int S = stagesOf(b);
+  BHIP(b, hipEventRecord(b->ev0, b->stream));
+  for (int i = 0; i < nsteps; i++) {
+    const int c = b->cur;
+    if (!b->graph[c]) {
+      hipGraph_t g;
+      BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      for (int s = 0; s < S; s++) launchStage(b, s, (c + s) & 1);
+      BHIP(b, hipStreamEndCapture(b->stream, &g));
+      const hipError_t e = hipGraphInstantiate(&b->graph[c], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return bfail(b, SDG_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    BHIP(b, hipGraphLaunch(b->graph[c], b->stream));
+    b->cur = (c + S) & 1;
+  }
+  BHIP(b, hipGetLastError());
+  BHIP(b, hipEventRecord(b->ev1, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  BHIP(b, hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
+  return SDG_OK;
+}
+
+int sdg_batch_last_step_ms(const sdg_batch* b, double* ms) {
+  if (!b || !ms) return SDG_E_INVALID;
+  *ms = b->last_ms;
+  return SDG_OK;
+}
+
+// Autotuner: for every variant, the work-item size (tiles per item) that makes a stage over that
+// variant's elements fastest, then the persistent grid size; the state is saved and restored.
+int sdg_batch_tune(sdg_batch* b, double* stage_ms) {
+  if (!b) return bfail(b, SDG_E_INVALID, "sdg_batch_tune: null batch");
+  for (int p = 0; p < b->nprob; p++)
+    if (!b->has_state[p]) return bfail(b, SDG_E_STATE, "sdg_batch_tune: a problem has no state");
+  const size_t tbytes = (b->mixed ? sizeof(float) : sizeof(double)) * static_cast<size_t>(b->totalT);
+  std::vector<char> saveU(static_cast<size_t>(b->totalU) * sizeof(double)), saveUn(saveU.size()), saveT(tbytes);
+  BHIP(b, hipMemcpy(saveU.data(), b->U, saveU.size(), hipMemcpyDeviceToHost));
+  BHIP(b, hipMemcpy(saveUn.data(), b->Un, saveUn.size(), hipMemcpyDeviceToHost));
+  BHIP(b, hipMemcpy(saveT.data(), b->T[b->cur], tbytes, hipMemcpyDeviceToHost));
+  auto timeStage = [&](int reps) -> float {
+    float best = 1e30f;
+    for (int r = 0; r < reps; r++) {
+      (void)hipEventRecord(b->ev0, b->stream);
+      launchStage(b, 0, b->cur);
+      (void)hipEventRecord(b->ev1, b->stream);
+      (void)hipStreamSynchronize(b->stream);
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
+      best = std::min(best, ms);
+    }
+    return best;
+  };
+  static const int kChunks[] = {1, 2, 4, 8, 16};
+  for (int v = 0; v < sdg::batch::kVariants; v++) {
+    if (!b->var[v].used) continue;
+    float best = 1e30f;
+    int bestc = b->chunk[v];
+    for (int c : kChunks) {
+      b->chunk[v] = c;
+      if (buildItems(b, &v)) return SDG_E_DEVICE;
+      const float t = timeStage(3);
+      if (t < best) {
+        best = t;
+        bestc = c;
+      }
+    }
+    b->chunk[v] = bestc;
+  }
+  if (buildItems(b)) return SDG_E_DEVICE;
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  float best = 1e30f;
+  int bestg = b->grid;
+  for (int mult : {1, 2, 4, 8, 16}) {
+    b->grid = cus * mult;
+    const float t = timeStage(3);
+    if (t < best) {
+      best = t;
+      bestg = b->grid;
+    }
+  }
+  b->grid = bestg;
+  BHIP(b, hipGetLastError());
+  BHIP(b, hipMemcpy(b->U, saveU.data(), saveU.size(), hipMemcpyHostToDevice));
+  BHIP(b, hipMemcpy(b->Un, saveUn.data(), saveUn.size(), hipMemcpyHostToDevice));
+  BHIP(b, hipMemcpy(b->T[b->cur], saveT.data(), tbytes, hipMemcpyHostToDevice));
+  for (int i = 0; i < 2; i++) {
+    if (b->graph[i]) (void)hipGraphExecDestroy(b->graph[i]);
+    b->graph[i] = nullptr;
+  }
+  if (stage_ms) *stage_ms = best;
+  return SDG_OK;
+}
+
+int sdg_batch_get_config(const sdg_batch* b, int* chunk_tiles, int* epb, int* grid) {
+  if (!b) return SDG_E_INVALID;
+  for (int v = 0; v < sdg::batch::kVariants; v++) {
+    if (chunk_tiles) chunk_tiles[v] = b->var[v].used ? b->chunk[v] : 0;
+    if (epb) epb[v] = sdg::batch::kEpb[v];
+  }
+  if (grid) *grid = b->grid;
+  return SDG_OK;
+}
+
+}  // extern "C"

@@ -8,9 +8,13 @@ struct sdg_mesh;
 
 namespace sdg {
 struct Mesh;
+struct MixedMesh;
 void setGlobalError(const std::string& msg);
 const char* globalError();
+// the single-type mesh of a handle (nullptr for a mesh with both triangles and quadrangles)
 const Mesh* meshOf(const sdg_mesh* m);
+// the triangle/quadrangle description of a split-grid mesh (nullptr for other meshes)
+const MixedMesh* mixedOf(const sdg_mesh* m);
 }  // namespace sdg
 
 #endif  // SDG_INTERNAL_HPP_

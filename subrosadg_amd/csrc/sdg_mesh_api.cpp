@@ -9,7 +9,10 @@
 #include "sdg_internal.hpp"
 
 struct sdg_mesh {
-  sdg::Mesh mesh;
+  sdg::Mesh mesh;         // single element type (all structured meshes, uniform split grids)
+  sdg::MixedMesh mixed;   // split grids: per-element triangle / quadrangle description
+  bool single = true;
+  bool has_mixed = false;
 };
 
 namespace sdg {
@@ -18,7 +21,8 @@ thread_local std::string g_last_error;
 
 void setGlobalError(const std::string& msg) { g_last_error = msg; }
 const char* globalError() { return g_last_error.c_str(); }
-const Mesh* meshOf(const sdg_mesh* m) { return m ? &m->mesh : nullptr; }
+const Mesh* meshOf(const sdg_mesh* m) { return m && m->single ? &m->mesh : nullptr; }
+const MixedMesh* mixedOf(const sdg_mesh* m) { return m && m->has_mixed ? &m->mixed : nullptr; }
 
 }  // namespace sdg
 
@@ -61,12 +65,67 @@ int sdg_mesh_create_structured(const sdg_structured_desc* desc, sdg_mesh** out) 
   return SDG_OK;
 }
 
+int sdg_mesh_create_split(const sdg_split_desc* desc, sdg_mesh** out) {
+  if (!desc || !out) {
+    sdg::setGlobalError("sdg_mesh_create_split: null argument");
+    return SDG_E_INVALID;
+  }
+  *out = nullptr;
+  if (desc->n[0] < 2 || desc->n[1] < 2 || static_cast<int64_t>(desc->n[0]) * desc->n[1] > (int64_t(1) << 28)) {
+    sdg::setGlobalError("sdg_mesh_create_split: bad cell counts");
+    return SDG_E_INVALID;
+  }
+  sdg_mesh* m = new (std::nothrow) sdg_mesh();
+  if (!m) {
+    sdg::setGlobalError("sdg_mesh_create_split: out of host memory");
+    return SDG_E_NOMEM;
+  }
+  // split flags: Bernoulli(split_probability) per cell from a splitmix64 stream of the seed
+  std::vector<uint8_t> split(static_cast<size_t>(desc->n[0]) * desc->n[1]);
+  uint64_t state = desc->seed;
+  for (auto& f : split) {
+    state += 0x9E3779B97F4A7C15ULL;
+    uint64_t z = state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    f = static_cast<double>(z >> 11) * (1.0 / 9007199254740992.0) < desc->split_probability ? 1 : 0;
+  }
+  std::string err;
+  try {
+    if (!sdg::buildSplitMesh(desc->polynomial_order, desc->n, desc->lo, desc->hi, split, m->mixed, err)) {
+      delete m;
+      sdg::setGlobalError("sdg_mesh_create_split: " + err);
+      return SDG_E_INVALID;
+    }
+  } catch (const std::bad_alloc&) {
+    delete m;
+    sdg::setGlobalError("sdg_mesh_create_split: out of host memory");
+    return SDG_E_NOMEM;
+  }
+  m->has_mixed = true;
+  m->single = sdg::singleTypeView(m->mixed, m->mesh);
+  *out = m;
+  return SDG_OK;
+}
+
 void sdg_mesh_destroy(sdg_mesh* mesh) { delete mesh; }
 
 int sdg_mesh_get_info(const sdg_mesh* mesh, sdg_mesh_info* info) {
   if (!mesh || !info) {
     sdg::setGlobalError("sdg_mesh_get_info: null argument");
     return SDG_E_INVALID;
+  }
+  if (!mesh->single) {  // triangles and quadrangles
+    const sdg::MixedMesh& x = mesh->mixed;
+    *info = sdg_mesh_info{};
+    info->dimension = 2;
+    info->element = SDG_ELEM_MIXED_TRIANGLE_QUADRANGLE;
+    info->polynomial_order = x.order;
+    info->elements = x.elements;
+    info->faces = static_cast<int64_t>(x.faces.size());
+    info->affine = 1;
+    return SDG_OK;
   }
   const sdg::Mesh& m = mesh->mesh;
   info->dimension = m.ref.dim;
@@ -85,6 +144,18 @@ int sdg_mesh_get_info(const sdg_mesh* mesh, sdg_mesh_info* info) {
 
 int64_t sdg_mesh_field_size(const sdg_mesh* mesh, int field) {
   if (!mesh) return 0;
+  if (field == SDG_FIELD_ELEMENT_TYPE) return mesh->has_mixed ? mesh->mixed.elements : mesh->mesh.elements;
+  if (!mesh->single) {
+    const sdg::MixedMesh& x = mesh->mixed;
+    int64_t nq = 0;
+    for (int64_t e = 0; e < x.elements; e++) nq += x.refOf(e).nq;
+    switch (field) {
+      case SDG_FIELD_QUAD_COORD: return nq * 2;
+      case SDG_FIELD_JACOBIAN_DET_W: return nq;
+      case SDG_FIELD_MIN_EDGE: return x.elements;
+      default: return 0;
+    }
+  }
   const sdg::Mesh& m = mesh->mesh;
   const int64_t E = m.elements, F = static_cast<int64_t>(m.faces.size()), Fb = static_cast<int64_t>(m.bfaces.size());
   const int64_t D = m.ref.dim, nb = m.ref.nb, nq = m.ref.nq, nf = m.ref.nf, nfq = m.ref.nfq;
@@ -129,6 +200,51 @@ int sdg_mesh_export(const sdg_mesh* mesh, int field, void* out, int64_t count) {
   if (count < need) {
     sdg::setGlobalError("sdg_mesh_export: output buffer too small");
     return SDG_E_INVALID;
+  }
+  if (field == SDG_FIELD_ELEMENT_TYPE) {
+    int32_t* oi = static_cast<int32_t*>(out);
+    if (mesh->has_mixed)
+      for (int64_t e = 0; e < mesh->mixed.elements; e++) oi[e] = mesh->mixed.refOf(e).type;
+    else
+      for (int64_t e = 0; e < mesh->mesh.elements; e++) oi[e] = mesh->mesh.ref.type;
+    return SDG_OK;
+  }
+  if (!mesh->single) {
+    const sdg::MixedMesh& x = mesh->mixed;
+    double* od = static_cast<double*>(out);
+    int64_t o = 0;
+    for (int64_t e = 0; e < x.elements; e++) {
+      const sdg::RefElement& r = x.refOf(e);
+      for (int q = 0; q < r.nq; q++) {
+        if (field == SDG_FIELD_QUAD_COORD) {
+          double xq[2], jac[4];
+          // affine map of the element's vertices
+          const double* xi = &r.qpoint[static_cast<size_t>(q) * 2];
+          double N[4], G[4][2];
+          int nv = r.nvert;
+          if (r.type == SDG_ELEM_TRIANGLE) {
+            N[0] = 1.0 - xi[0] - xi[1];
+            N[1] = xi[0];
+            N[2] = xi[1];
+          } else {
+            for (int v = 0; v < 4; v++)
+              N[v] = 0.25 * (1.0 + r.vertex[v * 2] * xi[0]) * (1.0 + r.vertex[v * 2 + 1] * xi[1]);
+          }
+          (void)G;
+          (void)jac;
+          xq[0] = xq[1] = 0.0;
+          for (int v = 0; v < nv; v++)
+            for (int d = 0; d < 2; d++) xq[d] += N[v] * x.vertex[static_cast<size_t>(e) * 8 + v * 2 + d];
+          od[(o + q) * 2] = xq[0];
+          od[(o + q) * 2 + 1] = xq[1];
+        } else if (field == SDG_FIELD_JACOBIAN_DET_W) {
+          od[o + q] = x.detj[e] * r.qweight[q];
+        }
+      }
+      if (field == SDG_FIELD_MIN_EDGE) od[e] = x.min_edge[e];
+      o += r.nq;
+    }
+    return SDG_OK;
   }
   const sdg::Mesh& m = mesh->mesh;
   const sdg::RefElement& r = m.ref;

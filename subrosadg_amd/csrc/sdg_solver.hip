@@ -160,6 +160,15 @@ bool pickValu(bool mixed, Launchers& l) {
 }
 
 bool selectLaunchers(int type, int order, bool force_valu, bool mixed, Launchers& l) {
+  if (type == SDG_ELEM_TRIANGLE) {  // bases of 3..15 functions: vector kernels
+    switch (order) {
+      case 1: return pickValu<2, 3, 3, 3, 2>(mixed, l);
+      case 2: return pickValu<2, 6, 6, 3, 3>(mixed, l);
+      case 3: return pickValu<2, 10, 12, 3, 4>(mixed, l);
+      case 4: return pickValu<2, 15, 16, 3, 5>(mixed, l);
+      default: return false;
+    }
+  }
   if (type == SDG_ELEM_QUADRANGLE) {
     switch (order) {
       case 1: return pickValu<2, 4, 4, 4, 2>(mixed, l);
@@ -460,6 +469,8 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   std::string msg;
   int rc = validateProblem(problem, msg);
   if (rc != SDG_OK) return fail(nullptr, rc, "sdg_create: " + msg);
+  if (!sdg::meshOf(mesh_handle))
+    return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: meshes of mixed element types run through sdg_batch_create");
   const sdg::Mesh& mesh = *sdg::meshOf(mesh_handle);
   if (!mesh.affine) return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: curved (non-affine) elements are not supported");
   if (mesh.elements > (int64_t(1) << 30)) return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: too many elements");

@@ -187,3 +187,20 @@ def euler_box(n=6, order=2, flux=L.FLUX_ROE, boundary=((L.BC_RIEMANN_FARFIELD,) 
 
 def boundary_points(mesh):
     return mesh.field("boundary_face_coord").reshape(mesh.Fb, mesh.nfq, mesh.dim)
+
+
+def smooth_direction(xq: np.ndarray, jw: np.ndarray, rng, like: np.ndarray, kmax: int = 3) -> np.ndarray:
+    """Random smooth periodic field on [0,1]^2 (Gaussian weights on the Fourier modes
+    sin/cos(2 pi (kx x + ky y)), 0 <= kx, ky <= kmax), scaled to the quadrature L2 norm of `like`.
+    Every such direction has an O(1) component along the low modes, so directional derivatives
+    stay well above finite-difference noise."""
+    x, y = xq[..., 0], xq[..., 1]
+    d = np.zeros_like(x)
+    for kx in range(kmax + 1):
+        for ky in range(kmax + 1):
+            if kx == 0 and ky == 0:
+                continue
+            ph = 2.0 * np.pi * (kx * x + ky * y)
+            a, b = rng.standard_normal(2)
+            d += a * np.cos(ph) + b * np.sin(ph)
+    return d * np.sqrt((jw * like ** 2).sum() / (jw * d ** 2).sum())

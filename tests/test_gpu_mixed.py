@@ -55,7 +55,10 @@ def test_mixed_residual_matches_fp64(name, make):
     u0 = s.get_state()
     o = O.Oracle(prob, mesh)
     o.set_state(u0)
-    groups = P.parity_error(s.residual(), o.residual(), u0, mesh.dim, per_group=True)
+    # Each group is measured against the largest group's scale (floor 1): a nearly vanishing
+    # group (the mass residual of a nearly incompressible flow) carries the absolute FP32 noise of
+    # the fluxes that cancel in it, not a relative error of its own size.
+    groups = P.parity_error(s.residual(), o.residual(), u0, mesh.dim, floor=1.0, per_group=True)
     print(f"  parity error (density, momentum, energy) {groups}", flush=True)
     assert max(groups) <= RESIDUAL_TOL, groups
 
