@@ -131,6 +131,7 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-mesh leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the HBM-counter calibration copy of this many MiB")
     args = ap.parse_args()
@@ -159,6 +160,7 @@ def main():
     head = runs["mixed"]
 
     adjoint = None if args.no_adjoint else adjoint_leg(args)
+    batch = None if args.no_batch else batch_leg(args)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.order)
     line = {
         "metric": "DOF.stage/s mixed vs FP64 at fixed L2 error (3-D NS Taylor-Green P3)",
@@ -184,6 +186,7 @@ def main():
         "roofline": roof["mixed"],
         "roofline_fp64": roof["fp64"],
         "adjoint": adjoint,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -269,6 +272,70 @@ def adjoint_leg(args, n=64, order=3, steps=400, cfl=0.1):
            "adjoint_over_forward": (t_rec + t_bwd) / t_fwd, "reverse_over_forward": t_bwd / t_fwd,
            "snapshots": steps, "fd_check_rel_err": abs(g_adj - g_fd) / abs(g_fd)}
     print(f"[bench] adjoint leg: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def vortex_primitive(xq, eps=5.0):
+    """Shu isentropic vortex on the uniform stream rho = u = v = p = 1 (R = 1): primitive (rho, u,
+    v, T) at points xq (..., 2)."""
+    x, y = xq[..., 0], xq[..., 1]
+    r2 = x * x + y * y
+    f = np.exp(0.5 * (1.0 - r2))
+    T = 1.0 - (GAMMA - 1.0) * eps * eps / (8.0 * GAMMA * math.pi ** 2) * f * f
+    prim = np.empty(xq.shape[:-1] + (4,))
+    prim[..., 0] = T ** (1.0 / (GAMMA - 1.0))
+    prim[..., 1] = 1.0 - eps / (2 * math.pi) * f * y
+    prim[..., 2] = 1.0 + eps / (2 * math.pi) * f * x
+    prim[..., 3] = T
+    return prim
+
+
+def batch_leg(args, count=48, seed=20261202, steps=100, cfl=0.1):
+    """Batched uneven 2-D Euler: `count` isentropic-vortex problems on periodic [-5,5]^2, per
+    problem n ~ U{8..96} cells per direction, P ~ U{1..4}, each cell a quadrangle or two triangles
+    by Bernoulli(0.5) (all draws from numpy's default generator with the given seed), Lax-Friedrichs,
+    SSP-RK3 with a per-problem CFL-0.1 time step, one persistent launch per stage, whole steps
+    captured in a hipGraph, work-item and grid sizes from the autotuner. Timed in mixed precision
+    (the headline mode of this workload) and in FP64; the rate is DOF.stage/s over all problems."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(count):
+        n = int(rng.integers(8, 97))
+        order = int(rng.integers(1, 5))
+        specs.append((n, order, int(rng.integers(0, 2 ** 62))))
+    meshes = [L.Mesh.split(order, (n, n), (-5.0, -5.0), (5.0, 5.0), 0.5, sd) for n, order, sd in specs]
+    elements = sum(m.E for m in meshes)
+    out = {"workload": f"{count} 2-D Euler isentropic vortices, periodic [-5,5]^2, n ~ U{{8..96}}, P ~ U{{1..4}}, "
+                       f"Bernoulli(0.5) triangle split, Lax-Friedrichs, SSP-RK3 CFL {cfl}, {steps} steps, seed {seed}",
+           "problems": count, "elements": elements}
+    states = {}
+    for mode, prec in (("fp64", L.PREC_FP64), ("mixed", L.PREC_MIXED)):
+        prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=L.FLUX_LAX_FRIEDRICHS,
+                              time_integration=L.RK_SSPRK3, cp=3.5, cv=2.5, precision=prec)
+        b = L.Batch(prob, meshes)
+        prims = [vortex_primitive(m.field("quad_coord").reshape(-1, 2)) for m in meshes]
+        for i, pr in enumerate(prims):
+            b.set_initial_primitive(i, pr)
+        dt = b.compute_dt(cfl)
+        stage_ms = b.tune()
+        b.step(dt, 3)  # graph capture and warm-up
+        for i, pr in enumerate(prims):
+            b.set_initial_primitive(i, pr)
+        b.step(dt, steps)
+        ms = b.last_step_ms()
+        dof = sum(sz // 4 for sz in b.sizes)
+        chunk, _, grid = b.config()
+        out[mode] = {"value": dof * 3 * steps / (ms * 1e-3), "ms_per_step": ms / steps,
+                     "tuned_stage_ms": stage_ms, "grid": grid, "tiles_per_item": chunk.tolist()}
+        out["dof_per_field"] = dof
+        # algorithmic HBM bytes of a stage: read U^i and U^n, write U^{i+1} (FP64 state)
+        out[mode]["state_gb_per_s"] = 96.0 * dof * 3 * steps / (ms * 1e-3) / 1e9
+        states[mode] = [b.get_state(i) for i in range(count)]
+        b.close()
+    out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    out["mixed_l2_rel_dev_vs_fp64"] = max(group_l2(a.reshape(-1, 4), c.reshape(-1, 4), 2)
+                                          for a, c in zip(states["mixed"], states["fp64"]))
+    print(f"[bench] batch leg: {out}", file=sys.stderr, flush=True)
     return out
 
 

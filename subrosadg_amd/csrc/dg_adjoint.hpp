@@ -479,7 +479,7 @@ __global__ __launch_bounds__(kThreads) void nsAdjGatherMfma(Geo g, Phys ph, Stag
 // y = alpha x + beta y (n doubles)
 __global__ __launch_bounds__(kThreads) void axpbyKernel(double* y, const double* x, double alpha, double beta, int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (i < n) y[i] = alpha * x[i] + beta * y[i];
+  if (i < n) y[i] = beta == 0.0 ? alpha * x[i] : alpha * x[i] + beta * y[i];  // beta 0: y is not read
 }
 
 }  // namespace sdg

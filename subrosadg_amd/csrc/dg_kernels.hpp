@@ -39,6 +39,7 @@ struct MOpsT {
   const TC* lift;    // K = N_aq,  M = N_aq  : block-diagonal Vf_s Mref^-1 Vf_s^T (BR2 face lift)
   const TC* pinterp; // K = N_b,   M = N_q   : Mref^-1 V^T   (R -> gradient at volume points)
   const TC* ptrace;  // K = N_b,   M = N_aq  : Mref^-1 Vf^T  (R -> gradient traces at face points)
+  const TC* rinterp; // K = N_b,   M = N_q   : residual -> residual at volume points (monitor)
 };
 using MOps = MOpsT<double>;
 

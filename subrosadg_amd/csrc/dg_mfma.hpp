@@ -141,7 +141,7 @@ __device__ __forceinline__ void mfmaStageTail(const Geo& g, const StageArgs& a, 
     #pragma unroll 1
     for (int m0 = wave * 16; m0 < NQm; m0 += 64) {
       acc_t<TC> acc = zeroAcc<TC>();
-      tileAcc<TC, NQm>(acc, op.interp, m0, sR, 0, NBk, lane);
+      tileAcc<TC, NQm>(acc, op.rinterp, m0, sR, 0, NBk, lane);
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         double x = static_cast<double>(acc[i]);

@@ -414,6 +414,31 @@ bool buildRefElement(int type, int order, RefElement& ref, std::string& error) {
     error = "singular reference matrix";
     return false;
   }
+  // mass = L L^T (Cholesky-Banachiewicz), then L^-1 by forward substitution
+  ref.mass_chol.assign(static_cast<size_t>(nb) * nb, 0.0);
+  ref.mass_chol_inverse.assign(static_cast<size_t>(nb) * nb, 0.0);
+  double* Lc = ref.mass_chol.data();
+  for (int i = 0; i < nb; i++)
+    for (int j = 0; j <= i; j++) {
+      double acc = ref.mass[static_cast<size_t>(i) * nb + j];
+      for (int k = 0; k < j; k++) acc -= Lc[i * nb + k] * Lc[j * nb + k];
+      if (i == j) {
+        if (!(acc > 0.0)) {
+          error = "reference mass matrix is not positive definite";
+          return false;
+        }
+        Lc[i * nb + i] = std::sqrt(acc);
+      } else {
+        Lc[i * nb + j] = acc / Lc[j * nb + j];
+      }
+    }
+  double* Li = ref.mass_chol_inverse.data();
+  for (int c = 0; c < nb; c++)
+    for (int i = c; i < nb; i++) {
+      double acc = i == c ? 1.0 : 0.0;
+      for (int k = c; k < i; k++) acc -= Lc[i * nb + k] * Li[k * nb + c];
+      Li[i * nb + c] = acc / Lc[i * nb + i];
+    }
   return true;
 }
 

@@ -56,6 +56,12 @@ struct RefElement {
   std::vector<double> lsq_inverse;         // N_b x N_b        (V^T V)^-1
   std::vector<double> mass;                // N_b x N_b        sum_q V_qi V_qj w_q (reference mass)
   std::vector<double> mass_inverse;        // N_b x N_b
+  // Cholesky factor of the reference mass, mass = L L^T (lower triangular), and L^-1. The test
+  // functions psi = L^-1 phi are orthonormal on the reference element; the mixed-precision mode
+  // forms residuals against them so that FP32 rounding is not amplified by the conditioning of
+  // the mass matrix of the hierarchic basis.
+  std::vector<double> mass_chol;           // N_b x N_b  L
+  std::vector<double> mass_chol_inverse;   // N_b x N_b  L^-1
 };
 
 // Build the reference tables; returns false for an unsupported (type, order).

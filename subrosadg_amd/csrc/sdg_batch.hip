@@ -728,9 +728,31 @@ int sdg_batch_create(const sdg_problem_desc* problem, const sdg_mesh* const* mes
       for (int k = 0; k < r.nb; k++) Vt[static_cast<size_t>(k) * r.nq + q] = r.V[static_cast<size_t>(q) * r.nb + k];
     for (int s = 0; s < V.naq; s++)
       for (int k = 0; k < r.nb; k++) Vft[static_cast<size_t>(k) * V.naq + s] = r.Vf[static_cast<size_t>(s) * r.nb + k];
+    // Test-function tables and the residual-to-update map. FP64: the hierarchic basis and Mref^-1.
+    // Mixed precision: the orthonormal test functions psi = L^-1 phi (Mref = L L^T) and L^-T, the
+    // same scheme in exact arithmetic, with FP32 residual rounding unamplified by Mref's conditioning.
+    std::vector<double> dVt = r.dV, Vft2 = r.Vf, Mt = r.mass_inverse;
+    if (b->mixed) {
+      const int nb = r.nb;
+      auto li = [&](int j, int i) { return r.mass_chol_inverse[static_cast<size_t>(j) * nb + i]; };
+      for (int k = 0; k < r.nq * 2; k++)
+        for (int j = 0; j < nb; j++) {
+          double acc = 0.0;
+          for (int i = 0; i <= j; i++) acc += r.dV[static_cast<size_t>(k) * nb + i] * li(j, i);
+          dVt[static_cast<size_t>(k) * nb + j] = acc;
+        }
+      for (int k = 0; k < V.naq; k++)
+        for (int j = 0; j < nb; j++) {
+          double acc = 0.0;
+          for (int i = 0; i <= j; i++) acc += r.Vf[static_cast<size_t>(k) * nb + i] * li(j, i);
+          Vft2[static_cast<size_t>(k) * nb + j] = acc;
+        }
+      for (int k = 0; k < nb; k++)
+        for (int i = 0; i < nb; i++) Mt[static_cast<size_t>(k) * nb + i] = li(k, i);  // row k: (L^-1)_{k i}
+    }
     double *pVt, *pdV, *pVft, *pVf, *pM, *pone, *pw;
-    if (bupload(b, Vt, &pVt) || bupload(b, r.dV, &pdV) || bupload(b, Vft, &pVft) || bupload(b, r.Vf, &pVf) ||
-        bupload(b, r.mass_inverse, &pM) || bupload(b, V.one, &pone) || bupload(b, r.qweight, &pw))
+    if (bupload(b, Vt, &pVt) || bupload(b, dVt, &pdV) || bupload(b, Vft, &pVft) || bupload(b, Vft2, &pVf) ||
+        bupload(b, Mt, &pM) || bupload(b, V.one, &pone) || bupload(b, r.qweight, &pw))
       return bail(SDG_E_NOMEM);
     d = VariantDev{pVt, pdV, pVft, pVf, pM, pone, pw, V.E, V.u_base, V.t_base, V.s_base, V.e_base};
   }

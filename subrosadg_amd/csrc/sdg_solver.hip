@@ -217,7 +217,7 @@ struct sdg_ctx {
   Geo geo{};
   double* minedge = nullptr;
   double* lsq = nullptr;  // host copy
-  std::vector<double> h_lsq, h_V;
+  std::vector<double> h_lsq, h_V, h_chol;
   double *U = nullptr, *Un = nullptr, *Uf[2] = {nullptr, nullptr}, *Gf = nullptr, *Rvol = nullptr, *Hs = nullptr, *Fv = nullptr;
   double *Rout = nullptr, *Wout = nullptr, *mon = nullptr, *rel = nullptr, *dtbuf = nullptr;
   int* status = nullptr;
@@ -245,6 +245,8 @@ struct sdg_ctx {
   double adj_dt = 0.0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   double graph_dt[2] = {0.0, 0.0};
+  hipGraphExec_t adj_graph[2] = {nullptr, nullptr};  // one adjoint step, by trace-buffer parity
+  double adj_graph_dt[2] = {0.0, 0.0};
   bool use_graph = true;
   float last_ms = 0.0f;
 
@@ -309,6 +311,8 @@ void destroyGraphs(sdg_ctx* ctx) {
   for (int i = 0; i < 2; i++) {
     if (ctx->graph[i]) (void)hipGraphExecDestroy(ctx->graph[i]);
     ctx->graph[i] = nullptr;
+    if (ctx->adj_graph[i]) (void)hipGraphExecDestroy(ctx->adj_graph[i]);
+    ctx->adj_graph[i] = nullptr;
   }
 }
 
@@ -375,6 +379,34 @@ void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hi
 void enqueueStep(sdg_ctx* ctx, double dt) {
   for (int s = 0; s < ctx->stages(); s++) enqueueStage(ctx, s, dt, 0, nullptr);
   ctx->launch.monitor(ctx->stream, ctx->mon, ctx->E, ctx->rel, ctx->status);
+}
+
+// Enqueue nsteps whole steps (each replayed from a captured graph) without waiting for them.
+int launchSteps(sdg_ctx* ctx, double dt, int nsteps) {
+  for (int i = 0; i < nsteps; i++) {
+    if (ctx->use_graph) {
+      const int c = ctx->cur;
+      if (!ctx->graph[c] || ctx->graph_dt[c] != dt) {
+        if (ctx->graph[c]) (void)hipGraphExecDestroy(ctx->graph[c]);
+        ctx->graph[c] = nullptr;
+        hipGraph_t graph;
+        HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        enqueueStep(ctx, dt);
+        HIP_TRY(ctx, hipStreamEndCapture(ctx->stream, &graph));
+        hipError_t e = hipGraphInstantiate(&ctx->graph[c], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(ctx, SDG_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        ctx->graph_dt[c] = dt;
+        ctx->cur = c;  // capture advanced cur; replay below advances it for real
+      }
+      HIP_TRY(ctx, hipGraphLaunch(ctx->graph[c], ctx->stream));
+      ctx->cur = c ^ (ctx->stages() & 1);
+    } else {
+      enqueueStep(ctx, dt);
+    }
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  return SDG_OK;
 }
 
 // reference layout (E x N_b x NV, NV fastest) <-> device layout (E x NV x N_b)
@@ -691,13 +723,47 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
         upload(ctx, t_trace, &pt) || upload(ctx, t_lift, &pl) || upload(ctx, pinterp, &ppi) || upload(ctx, ptrace, &ppt))
       return bail(SDG_E_NOMEM);
     if (ctx->mixed) {
+      // FP32 tables. Residuals are formed against the orthonormal test functions psi = L^-1 phi
+      // (mass = L L^T): R_psi = L^-1 R, so the update (Mref^-1 R) = L^-T R_psi, the lifted
+      // gradient at points is V L^-T R_psi, and the monitor's R V^T is V L R_psi. In exact
+      // arithmetic this is the same scheme; in FP32 the residual's rounding then reaches the
+      // solution unamplified by the conditioning of the hierarchic basis's mass matrix.
+      const std::vector<double>& Li = r.mass_chol_inverse;
+      const std::vector<double>& Lc = r.mass_chol;
+      auto li = [&](int j, int i) { return Li[static_cast<size_t>(j) * nb + i]; };  // (L^-1)_{j i}
+      std::vector<double> m_grad = paddedOperator(nq * D, nb, [&](int j, int k) {
+        double acc = 0.0;
+        for (int i = 0; i <= j; i++) acc += r.dV[static_cast<size_t>(k) * nb + i] * li(j, i);
+        return acc;
+      });
+      std::vector<double> m_face = paddedOperator(naq, nb, [&](int j, int k) {
+        double acc = 0.0;
+        for (int i = 0; i <= j; i++) acc += Vfv(k, i) * li(j, i);
+        return acc;
+      });
+      std::vector<double> m_minv = paddedOperator(nb, nb, [&](int m, int k) { return li(k, m); });
+      std::vector<double> m_pinterp = paddedOperator(nb, nq, [&](int q, int j) {
+        double acc = 0.0;
+        for (int i = 0; i <= j; i++) acc += V(q, i) * li(j, i);
+        return acc;
+      });
+      std::vector<double> m_ptrace = paddedOperator(nb, naq, [&](int sl, int j) {
+        double acc = 0.0;
+        for (int i = 0; i <= j; i++) acc += Vfv(sl, i) * li(j, i);
+        return acc;
+      });
+      std::vector<double> m_rinterp = paddedOperator(nb, nq, [&](int q, int j) {
+        double acc = 0.0;
+        for (int i = j; i < nb; i++) acc += V(q, i) * Lc[static_cast<size_t>(i) * nb + j];
+        return acc;
+      });
       auto f32 = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
-      float *fi, *fg, *ff, *fm, *ft, *fl, *fpi, *fpt;
-      if (upload(ctx, f32(t_interp), &fi) || upload(ctx, f32(t_grad), &fg) || upload(ctx, f32(t_face), &ff) ||
-          upload(ctx, f32(t_minv), &fm) || upload(ctx, f32(t_trace), &ft) || upload(ctx, f32(t_lift), &fl) ||
-          upload(ctx, f32(pinterp), &fpi) || upload(ctx, f32(ptrace), &fpt))
+      float *fi, *fg, *ff, *fm, *ft, *fl, *fpi, *fpt, *fri;
+      if (upload(ctx, f32(t_interp), &fi) || upload(ctx, f32(m_grad), &fg) || upload(ctx, f32(m_face), &ff) ||
+          upload(ctx, f32(m_minv), &fm) || upload(ctx, f32(t_trace), &ft) || upload(ctx, f32(t_lift), &fl) ||
+          upload(ctx, f32(m_pinterp), &fpi) || upload(ctx, f32(m_ptrace), &fpt) || upload(ctx, f32(m_rinterp), &fri))
         return bail(SDG_E_NOMEM);
-      g.opf = {fi, fg, ff, fm, ft, fl, fpi, fpt};
+      g.opf = {fi, fg, ff, fm, ft, fl, fpi, fpt, fri};
     }
     if (ctx->launch.adj_volume[0]) {
       // transposes for the discrete adjoint
@@ -731,9 +797,11 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     g.op.minv = pm;
     g.op.trace = pt;
     g.op.lift = pl;
+    g.op.rinterp = pi;
   }
   ctx->minedge = dme;
   ctx->h_lsq = r.lsq_inverse;
+  ctx->h_chol = r.mass_chol;
   ctx->h_V = r.V;
   {
     // coefficients of the constant 1 by the initial-condition projection (exactly 1 on the vertex
@@ -905,6 +973,18 @@ int sdg_residual(sdg_ctx* ctx, double* residual) {
   std::vector<double> dev(static_cast<size_t>(ctx->E) * ctx->nv * ctx->nb);
   HIP_TRY(ctx, hipMemcpyAsync(dev.data(), ctx->Rout, dev.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->mixed) {  // the mixed-precision kernels form R_psi = L^-1 R; return R = L R_psi
+    const int nb = ctx->nb;
+    std::vector<double> t(nb);
+    for (size_t o = 0; o < dev.size(); o += nb) {
+      for (int i = 0; i < nb; i++) {
+        double acc = 0.0;
+        for (int j = 0; j <= i; j++) acc += ctx->h_chol[static_cast<size_t>(i) * nb + j] * dev[o + j];
+        t[i] = acc;
+      }
+      std::copy(t.begin(), t.end(), dev.begin() + static_cast<std::ptrdiff_t>(o));
+    }
+  }
   toReference(ctx, dev, residual);
   return SDG_OK;
 }
@@ -939,29 +1019,8 @@ int sdg_step(sdg_ctx* ctx, double dt, int nsteps, double* relative_error, int* n
   if (nsteps < 0 || !(dt >= 0.0)) return fail(ctx, SDG_E_INVALID, "sdg_step: bad dt or step count");
   hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-  for (int i = 0; i < nsteps; i++) {
-    if (ctx->use_graph) {
-      const int c = ctx->cur;
-      if (!ctx->graph[c] || ctx->graph_dt[c] != dt) {
-        if (ctx->graph[c]) (void)hipGraphExecDestroy(ctx->graph[c]);
-        ctx->graph[c] = nullptr;
-        hipGraph_t graph;
-        HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        enqueueStep(ctx, dt);
-        HIP_TRY(ctx, hipStreamEndCapture(ctx->stream, &graph));
-        hipError_t e = hipGraphInstantiate(&ctx->graph[c], graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) return fail(ctx, SDG_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-        ctx->graph_dt[c] = dt;
-        ctx->cur = c;  // capture advanced cur; replay below advances it for real
-      }
-      HIP_TRY(ctx, hipGraphLaunch(ctx->graph[c], ctx->stream));
-      ctx->cur = c ^ (ctx->stages() & 1);
-    } else {
-      enqueueStep(ctx, dt);
-    }
-  }
-  HIP_TRY(ctx, hipGetLastError());
+  int rc = launchSteps(ctx, dt, nsteps);
+  if (rc) return rc;
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   int st[2] = {0, 0};
   std::vector<double> rel(ctx->nv, 0.0);
@@ -1057,7 +1116,8 @@ int ensureAdjoint(sdg_ctx* ctx) {
   if (allocate(ctx, nU, &ctx->adj_lam) || allocate(ctx, nS, &ctx->adj_Hb) || allocate(ctx, nG, &ctx->adj_RtV) ||
       allocate(ctx, nU, &ctx->adj_Ua) || allocate(ctx, nS, &ctx->adj_UFf) || allocate(ctx, nGS, &ctx->adj_GFb) ||
       allocate(ctx, nS, &ctx->adj_Pown) || allocate(ctx, nS, &ctx->adj_Qnb) || allocate(ctx, nU, &ctx->adj_out) ||
-      allocate(ctx, nU, &ctx->adj_S[1]) || allocate(ctx, nU, &ctx->adj_S[2]) || allocate(ctx, nU, &ctx->adj_acc) ||
+      allocate(ctx, nU, &ctx->adj_S[0]) || allocate(ctx, nU, &ctx->adj_S[1]) || allocate(ctx, nU, &ctx->adj_S[2]) ||
+      allocate(ctx, nU, &ctx->adj_acc) ||
       allocate(ctx, nU, &ctx->adj_tmp))
     return SDG_E_NOMEM;
   ctx->adj_ready = true;
@@ -1113,13 +1173,18 @@ int loadState(sdg_ctx* ctx, const double* src) {
   return SDG_OK;
 }
 
+// k forward steps, enqueued without waiting; the NaN/step-count status is read by the caller once a
+// whole run is enqueued (recomputations repeat steps the recorded run already took).
 int advance(sdg_ctx* ctx, int k) {
   if (k <= 0) return SDG_OK;
-  int nan = 0, taken = 0;
-  const int rc = sdg_step(ctx, ctx->adj_dt, k, nullptr, &nan, &taken);
-  if (rc != SDG_OK) return rc;
-  if (nan) return fail(ctx, SDG_E_STATE, "adjoint: the residual monitor turned NaN during the forward run");
-  return SDG_OK;
+  return launchSteps(ctx, ctx->adj_dt, k);
+}
+
+// forward steps replayed inside the reverse sweep (from a checkpoint the recorded run passed)
+int recompute(sdg_ctx* ctx, int k) {
+  if (k <= 0) return SDG_OK;
+  hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
+  return launchSteps(ctx, ctx->adj_dt, k);
 }
 
 int saveState(sdg_ctx* ctx, double* dst) {
@@ -1130,26 +1195,60 @@ int saveState(sdg_ctx* ctx, double* dst) {
 // adj_lam: adjoint of U_{n+1} -> adjoint of U_n, for the step that starts from U0 (device layout).
 // Stage s maps U_s to U_{s+1} = c0 U_0 + c1 U_s + c2 dt L(U_s) (TimeIntegration.cpp:45-65,181-198),
 // so its transpose is lam_s = c1 lam_{s+1} + c2 dt L'(U_s)^T lam_{s+1}, plus c0 lam_{s+1} to U_0.
-int stepAdjoint(sdg_ctx* ctx, const double* U0) {
+// The body of one adjoint step, starting from the state in adj_S[0]: recompute the stage states,
+// then transpose the stages in reverse order. The state buffers and trace parity it leaves behind
+// are those of S - 1 forward stages. No memset (the body is captured in a graph).
+int enqueueAdjointStep(sdg_ctx* ctx) {
   const int S = ctx->stages();
-  int rc = loadState(ctx, U0);
+  int rc = loadState(ctx, ctx->adj_S[0]);
   if (rc) return rc;
   hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
   for (int s = 0; s + 1 < S; s++) {
     enqueueStage(ctx, s, ctx->adj_dt, 0, nullptr);
     if ((rc = saveState(ctx, ctx->adj_S[s + 1]))) return rc;
   }
-  HIP_TRY(ctx, hipMemsetAsync(ctx->adj_acc, 0, stateCount(ctx) * sizeof(double), ctx->stream));
   for (int s = S - 1; s >= 0; s--) {
     double c[3];
     ctx->rkCoefficients(s, c);
-    const double* Us = s == 0 ? U0 : ctx->adj_S[s];
-    if ((rc = vjpAt(ctx, Us, ctx->adj_lam, 1, ctx->adj_out))) return rc;
-    axpby(ctx, ctx->adj_acc, ctx->adj_lam, c[0], 1.0);
+    if ((rc = vjpAt(ctx, ctx->adj_S[s], ctx->adj_lam, 1, ctx->adj_out))) return rc;
+    axpby(ctx, ctx->adj_acc, ctx->adj_lam, c[0], s == S - 1 ? 0.0 : 1.0);  // beta 0: acc = c0 lam
     axpby(ctx, ctx->adj_lam, ctx->adj_out, c[2] * ctx->adj_dt, c[1]);
   }
   axpby(ctx, ctx->adj_lam, ctx->adj_acc, 1.0, 1.0);
   HIP_TRY(ctx, hipGetLastError());
+  return SDG_OK;
+}
+
+int stepAdjoint(sdg_ctx* ctx, const double* U0) {
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_S[0], U0, stateCount(ctx) * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  const int flips = (ctx->stages() - 1) & 1;
+  if (!ctx->use_graph) {
+    const int rc = enqueueAdjointStep(ctx);
+    return rc;
+  }
+  const int c = ctx->cur;
+  if (!ctx->adj_graph[c] || ctx->adj_graph_dt[c] != ctx->adj_dt) {
+    if (ctx->adj_graph[c]) (void)hipGraphExecDestroy(ctx->adj_graph[c]);
+    ctx->adj_graph[c] = nullptr;
+    hipGraph_t graph;
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueueAdjointStep(ctx);
+    hipGraph_t captured = nullptr;
+    const hipError_t ee = hipStreamEndCapture(ctx->stream, &captured);
+    if (rc) {
+      if (captured) (void)hipGraphDestroy(captured);
+      return rc;
+    }
+    if (ee != hipSuccess) return fail(ctx, SDG_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+    graph = captured;
+    const hipError_t e = hipGraphInstantiate(&ctx->adj_graph[c], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(ctx, SDG_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    ctx->adj_graph_dt[c] = ctx->adj_dt;
+    ctx->cur = c;  // capture advanced cur; the replay below advances it for real
+  }
+  HIP_TRY(ctx, hipGraphLaunch(ctx->adj_graph[c], ctx->stream));
+  ctx->cur = c ^ flips;
   return SDG_OK;
 }
 
@@ -1178,7 +1277,7 @@ int reverseSweep(sdg_ctx* ctx, int i0, int n, int s, int slot) {
   if (n == 1) return stepAdjoint(ctx, ctx->snaps[slot]);
   if (s == 0) {
     for (int jj = n - 1; jj >= 0; jj--) {
-      if ((rc = loadState(ctx, ctx->snaps[slot])) || (rc = advance(ctx, jj)) || (rc = saveState(ctx, ctx->adj_tmp)) ||
+      if ((rc = loadState(ctx, ctx->snaps[slot])) || (rc = recompute(ctx, jj)) || (rc = saveState(ctx, ctx->adj_tmp)) ||
           (rc = stepAdjoint(ctx, ctx->adj_tmp)))
         return rc;
     }
@@ -1186,7 +1285,7 @@ int reverseSweep(sdg_ctx* ctx, int i0, int n, int s, int slot) {
   }
   const int m = splitBinomial(n, s);
   if (ctx->snap_step[slot + 1] != i0 + m) {
-    if ((rc = loadState(ctx, ctx->snaps[slot])) || (rc = advance(ctx, m)) || (rc = saveState(ctx, ctx->snaps[slot + 1])))
+    if ((rc = loadState(ctx, ctx->snaps[slot])) || (rc = recompute(ctx, m)) || (rc = saveState(ctx, ctx->snaps[slot + 1])))
       return rc;
     ctx->snap_step[slot + 1] = i0 + m;
   }
@@ -1229,6 +1328,7 @@ int sdg_adjoint_forward(sdg_ctx* ctx, double dt, int nsteps, int snapshots) {
   ctx->snap_step.assign(ctx->snaps.size(), -1);
   ctx->adj_dt = dt;
   ctx->adj_nsteps = -1;
+  hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
   if ((rc = saveState(ctx, ctx->snaps[0]))) return rc;
   ctx->snap_step[0] = 0;
   // the forward run places the checkpoints of the schedule's first descent
@@ -1244,7 +1344,16 @@ int sdg_adjoint_forward(sdg_ctx* ctx, double dt, int nsteps, int snapshots) {
     s--;
   }
   if ((rc = advance(ctx, n))) return rc;
+  int st[2] = {0, 0};
+  HIP_TRY(ctx, hipMemcpyAsync(st, ctx->status, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (st[0] || st[1] != nsteps) {
+    // a NaN stop leaves the later launches as no-ops; rebuild the traces of the stopped state
+    ctx->cur = 0;
+    ctx->launch.trace(ctx->stream, ctx->geo, ctx->U, ctx->traces(0), ctx->mix);
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return fail(ctx, SDG_E_STATE, "sdg_adjoint_forward: the residual monitor turned NaN during the recorded run");
+  }
   ctx->adj_nsteps = nsteps;
   ctx->adj_nslots = nslots;
   return SDG_OK;
