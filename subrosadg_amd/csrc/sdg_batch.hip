@@ -82,14 +82,18 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 template <int NB, int NQ, int NF, int NFQ>
 struct Shape {
   static constexpr int NAQ = NF * NFQ;
-  static constexpr int MAXN = cmax(NB, cmax(NQ, NAQ));
-  static constexpr int EPB_RAW = kThreads / (NV * MAXN);
-  static constexpr int EPB = EPB_RAW < 1 ? 1 : (EPB_RAW > 16 ? 16 : EPB_RAW);
-  static constexpr int LDS_ELEMS = EPB * NV * (NB + NQ + NQ * D + NAQ + NB);
+  // elements per pass: enough that the pointwise phases (one thread per volume or face point)
+  // cover most of the workgroup, capped to keep two workgroups' panels and tables in one CU's LDS
+  static constexpr int MINP = NQ < NAQ ? NQ : NAQ;
+  static constexpr int EPB_RAW = 256 / MINP;
+  static constexpr int EPB = EPB_RAW < 1 ? 1 : (EPB_RAW > 64 ? 64 : EPB_RAW);
+  // state panel, point values (reused for the residual: NQ >= NB), volume and face fluxes
+  static constexpr int LDS_ELEMS = EPB * NV * (NB + NQ + NQ * D + NAQ);
+  static constexpr int TAB_ELEMS = NB * NQ + NQ * D * NB + NB * NAQ + NAQ * NB;
 };
 
-// numerical flux at one face point; kept out of line so that the eight tile variants share one copy
-__device__ __noinline__ void faceFluxPoint(const Phys& ph, const double* n, const double* ul, const double* ur, double* h) {
+// numerical flux at one face point
+__device__ __forceinline__ void faceFluxPoint(const Phys& ph, const double* n, const double* ul, const double* ur, double* h) {
   double wl[D + 3], wr[D + 3];
   computational<D>(ph, ul, wl);
   computational<D>(ph, ur, wr);
@@ -105,9 +109,12 @@ __device__ __forceinline__ double pointValue(const Args<TC>& a, int p, TC x, int
 }
 
 // One variant's elements [first, first + count), EPB at a time (out of line: each variant keeps
-// its own register allocation instead of the union of all eight).
+// its own register allocation instead of the union of all eight). The variant's operator tables
+// are staged in LDS in the contraction precision (tab) and the residual-to-update map in FP64 (sM)
+// when `reload` is set; the caller guarantees no thread still reads the previous tables.
 template <typename TC, int NB, int NQ, int NF, int NFQ>
-__device__ __noinline__ void eulerTile(const Args<TC>& a, const VariantDev& vd, int64_t first, int count, TC* smem) {
+__device__ __noinline__ void eulerTile(const Args<TC>& a, const VariantDev& vd, int64_t first, int count, TC* smem,
+                                       TC* tab, double* sM, bool reload) {
   using S = Shape<NB, NQ, NF, NFQ>;
   constexpr int NAQ = S::NAQ, EPB = S::EPB;
   constexpr bool MIX = sizeof(TC) == 4;
@@ -115,10 +122,26 @@ __device__ __noinline__ void eulerTile(const Args<TC>& a, const VariantDev& vd, 
   TC(*sUq)[NV][NQ] = reinterpret_cast<TC(*)[NV][NQ]>(smem + EPB * NV * NB);
   TC(*sF)[NV][NQ * D] = reinterpret_cast<TC(*)[NV][NQ * D]>(smem + EPB * NV * (NB + NQ));
   TC(*sH)[NV][NAQ] = reinterpret_cast<TC(*)[NV][NAQ]>(smem + EPB * NV * (NB + NQ + NQ * D));
-  TC(*sR)[NV][NB] = reinterpret_cast<TC(*)[NV][NB]>(smem + EPB * NV * (NB + NQ + NQ * D + NAQ));
+  static_assert(NQ >= NB, "the residual panel reuses the point-value panel");
+  TC(*sR)[NV][NB] = reinterpret_cast<TC(*)[NV][NB]>(smem + EPB * NV * NB);  // aliases sUq
+  TC* tVt = tab;                  // [NB][NQ]      basis at volume points
+  TC* tdV = tVt + NB * NQ;        // [NQ D][NB]    test-function gradients
+  TC* tVft = tdV + NQ * D * NB;   // [NB][NAQ]     basis at face points
+  TC* tVf = tVft + NB * NAQ;      // [NAQ][NB]     test functions at face points
   const int tid = threadIdx.x;
+  if (reload) {
+    for (int i = tid; i < NB * NQ; i += kThreads) tVt[i] = static_cast<TC>(vd.Vt[i]);
+    for (int i = tid; i < NQ * D * NB; i += kThreads) tdV[i] = static_cast<TC>(vd.dV[i]);
+    for (int i = tid; i < NB * NAQ; i += kThreads) tVft[i] = static_cast<TC>(vd.Vft[i]);
+    for (int i = tid; i < NAQ * NB; i += kThreads) tVf[i] = static_cast<TC>(vd.Vf[i]);
+    for (int i = tid; i < NB * NB; i += kThreads) sM[i] = vd.Minv[i];
+  }
+#pragma unroll 1
   for (int64_t c0 = first; c0 < first + count; c0 += EPB) {
     const int ne = static_cast<int>(first + count - c0 < EPB ? first + count - c0 : EPB);
+    // state panel (deviation from the problem's reference state in the mixed mode); the panel of
+    // EPB consecutive elements is one contiguous stretch of U
+    #pragma unroll 1
     for (int idx = tid; idx < EPB * NV * NB; idx += kThreads) {
       const int le = idx / (NV * NB), rem = idx % (NV * NB), v = rem / NB, b = rem % NB;
       TC x = TC(0);
@@ -131,15 +154,23 @@ __device__ __noinline__ void eulerTile(const Args<TC>& a, const VariantDev& vd, 
       sU[le][v][b] = x;
     }
     __syncthreads();
-    for (int idx = tid; idx < EPB * NV * NQ; idx += kThreads) {
-      const int le = idx / (NV * NQ), rem = idx % (NV * NQ), v = rem / NQ, q = rem % NQ;
-      TC s = TC(0);
-      #pragma unroll 4
-      for (int b = 0; b < NB; b++) s += sU[le][v][b] * static_cast<TC>(vd.Vt[b * NQ + q]);
-      sUq[le][v][q] = s;
+        // point values: one thread per (element, point), all variables (one table read per term)
+#pragma unroll 1
+    for (int idx = tid; idx < EPB * NQ; idx += kThreads) {
+      const int le = idx / NQ, q = idx % NQ;
+      TC s[NV] = {};
+#pragma unroll 5
+      for (int b = 0; b < NB; b++) {
+        const TC t = tVt[b * NQ + q];
+#pragma unroll
+        for (int v = 0; v < NV; v++) s[v] += sU[le][v][b] * t;
+      }
+#pragma unroll
+      for (int v = 0; v < NV; v++) sUq[le][v][q] = s[v];
     }
     __syncthreads();
     // volume flux (F^T (J^T)^-1 |J| w), pointwise in FP64
+    #pragma unroll 1
     for (int idx = tid; idx < EPB * NQ; idx += kThreads) {
       const int le = idx / NQ, q = idx % NQ;
       if (le >= ne) {
@@ -181,6 +212,7 @@ __device__ __noinline__ void eulerTile(const Args<TC>& a, const VariantDev& vd, 
         }
     }
     // numerical face fluxes at the element's face points
+    #pragma unroll 1
     for (int idx = tid; idx < EPB * NAQ; idx += kThreads) {
       const int le = idx / NAQ, sl = idx % NAQ;
       if (le >= ne) {
@@ -215,53 +247,72 @@ __device__ __noinline__ void eulerTile(const Args<TC>& a, const VariantDev& vd, 
       }
     }
     __syncthreads();
-    // residual R = F Phi_grad - H Phi_face
-    for (int idx = tid; idx < EPB * NV * NB; idx += kThreads) {
-      const int le = idx / (NV * NB), rem = idx % (NV * NB), v = rem / NB, i = rem % NB;
-      TC r = TC(0), hsum = TC(0);
-      #pragma unroll 4
-      for (int k = 0; k < D * NQ; k++) r += sF[le][v][k] * static_cast<TC>(vd.dV[k * NB + i]);
-      #pragma unroll 4
-      for (int k = 0; k < NAQ; k++) hsum += sH[le][v][k] * static_cast<TC>(vd.Vf[k * NB + i]);
-      sR[le][v][i] = r - hsum;
+    // residual R = F Phi_grad - H Phi_face (against the test functions of the tables)
+    #pragma unroll 1
+    for (int idx = tid; idx < EPB * NB; idx += kThreads) {
+      const int le = idx / NB, i = idx % NB;
+      TC r[NV] = {}, hsum[NV] = {};
+#pragma unroll 4
+      for (int k = 0; k < D * NQ; k++) {
+        const TC t = tdV[k * NB + i];
+#pragma unroll
+        for (int v = 0; v < NV; v++) r[v] += sF[le][v][k] * t;
+      }
+#pragma unroll 4
+      for (int k = 0; k < NAQ; k++) {
+        const TC t = tVf[k * NB + i];
+#pragma unroll
+        for (int v = 0; v < NV; v++) hsum[v] += sH[le][v][k] * t;
+      }
+      // sR aliases sUq, which the flux phase read before the barrier above; every (element,
+      // variable) row of sR is written only here
+#pragma unroll
+      for (int v = 0; v < NV; v++) sR[le][v][i] = r[v] - hsum[v];
     }
     __syncthreads();
     // RK update in FP64: U <- c1 U + c0 Un + c2 dt (R Mref^-1) / |J| (TimeIntegration.cpp:181-198)
-    for (int idx = tid; idx < EPB * NV * NB; idx += kThreads) {
-      const int le = idx / (NV * NB), rem = idx % (NV * NB), v = rem / NB, i = rem % NB;
+    #pragma unroll 1
+    for (int idx = tid; idx < EPB * NB; idx += kThreads) {
+      const int le = idx / NB, i = idx % NB;
       if (le >= ne) continue;
       const int64_t e = c0 + le, eg = vd.e_base + e;
       const int p = a.pid[eg];
-      double rm = 0.0;
-      #pragma unroll 4
-      for (int b = 0; b < NB; b++) rm += static_cast<double>(sR[le][v][b]) * vd.Minv[b * NB + i];
-      if constexpr (MIX) {
-        if (v == NV - 1) {  // energy: add back h0 R_rho
-          double rr = 0.0;
-          #pragma unroll 4
-          for (int b = 0; b < NB; b++) rr += static_cast<double>(sR[le][0][b]) * vd.Minv[b * NB + i];
-          rm += a.mix[p * 6 + 5] * rr;
-        }
+      double rm[NV] = {};
+#pragma unroll 5
+      for (int b = 0; b < NB; b++) {
+        const double t = sM[b * NB + i];
+#pragma unroll
+        for (int v = 0; v < NV; v++) rm[v] += static_cast<double>(sR[le][v][b]) * t;
       }
-      rm = rm / a.detj[eg];
-      const int64_t gi = vd.u_base + e * NV * NB + rem;
-      double u = a.U[gi] * a.c1;
-      u += a.c0 * a.Un[gi];
-      u += a.c2 * a.dt[p] * rm;
-      a.U[gi] = u;
-      if (a.last_stage) a.Un[gi] = u;
-      if constexpr (MIX) u -= a.mix[p * 6 + v] * vd.one[i];
-      sU[le][v][i] = static_cast<TC>(u);
+      if constexpr (MIX) rm[NV - 1] += a.mix[p * 6 + 5] * rm[0];  // energy: add back h0 R_rho
+      const double scale = a.c2 * a.dt[p] / a.detj[eg];
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const int64_t gi = vd.u_base + (e * NV + v) * NB + i;
+        double u = a.U[gi] * a.c1;
+        u += a.c0 * a.Un[gi];
+        u += scale * rm[v];
+        a.U[gi] = u;
+        if (a.last_stage) a.Un[gi] = u;
+        if constexpr (MIX) u -= a.mix[p * 6 + v] * vd.one[i];
+        sU[le][v][i] = static_cast<TC>(u);
+      }
     }
     __syncthreads();
-    // traces of the new state
-    for (int idx = tid; idx < EPB * NV * NAQ; idx += kThreads) {
-      const int le = idx / (NV * NAQ), rem = idx % (NV * NAQ), v = rem / NAQ, sl = rem % NAQ;
+    // traces of the new state (one contiguous stretch of the trace buffer per panel)
+    #pragma unroll 1
+    for (int idx = tid; idx < EPB * NAQ; idx += kThreads) {
+      const int le = idx / NAQ, sl = idx % NAQ;
       if (le >= ne) continue;
-      TC t = TC(0);
-      #pragma unroll 4
-      for (int b = 0; b < NB; b++) t += sU[le][v][b] * static_cast<TC>(vd.Vft[b * NAQ + sl]);
-      a.Tout[vd.t_base + (c0 + le) * NV * NAQ + rem] = t;
+      TC t[NV] = {};
+#pragma unroll 5
+      for (int b = 0; b < NB; b++) {
+        const TC w = tVft[b * NAQ + sl];
+#pragma unroll
+        for (int v = 0; v < NV; v++) t[v] += sU[le][v][b] * w;
+      }
+#pragma unroll
+      for (int v = 0; v < NV; v++) a.Tout[vd.t_base + ((c0 + le) * NV + v) * NAQ + sl] = t[v];
     }
     __syncthreads();
   }
@@ -282,6 +333,13 @@ __device__ __noinline__ void eulerTile(const Args<TC>& a, const VariantDev& vd, 
 constexpr int kMaxLdsElems = SDG_BATCH_VARIANTS(SDG_BATCH_LDS) 0 ) ) ) ) ) ) ) );
 #undef SDG_BATCH_LDS
 
+#define SDG_BATCH_TAB(c, nb, nq, nf, nfq) cmax(Shape<nb, nq, nf, nfq>::TAB_ELEMS,
+constexpr int kMaxTabElems = SDG_BATCH_VARIANTS(SDG_BATCH_TAB) 0 ) ) ) ) ) ) ) );
+#undef SDG_BATCH_TAB
+#define SDG_BATCH_NB2(c, nb, nq, nf, nfq) cmax(nb * nb,
+constexpr int kMaxNb2 = SDG_BATCH_VARIANTS(SDG_BATCH_NB2) 0 ) ) ) ) ) ) ) );
+#undef SDG_BATCH_NB2
+
 #define SDG_BATCH_EPB(c, nb, nq, nf, nfq) Shape<nb, nq, nf, nfq>::EPB,
 constexpr int kEpb[kVariants] = {SDG_BATCH_VARIANTS(SDG_BATCH_EPB)};
 #undef SDG_BATCH_EPB
@@ -290,13 +348,18 @@ constexpr int kEpb[kVariants] = {SDG_BATCH_VARIANTS(SDG_BATCH_EPB)};
 template <typename TC>
 __global__ __launch_bounds__(kThreads) void batchStageKernel(Args<TC> a) {
   __shared__ TC smem[kMaxLdsElems];
+  __shared__ TC tab[kMaxTabElems];
+  __shared__ double sM[kMaxNb2];
+  int loaded = -1;  // variant whose tables are in LDS (uniform across the workgroup)
   for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
     const WorkItem w = a.items[it];
     const VariantDev& vd = a.var[w.code];
+    const bool reload = w.code != loaded;
+    loaded = w.code;
     switch (w.code) {
 #define SDG_BATCH_CASE(c, nb, nq, nf, nfq) \
   case c:                                  \
-    eulerTile<TC, nb, nq, nf, nfq>(a, vd, w.first, w.count, smem); \
+    eulerTile<TC, nb, nq, nf, nfq>(a, vd, w.first, w.count, smem, tab, sM, reload); \
     break;
       SDG_BATCH_VARIANTS(SDG_BATCH_CASE)
 #undef SDG_BATCH_CASE
