@@ -40,7 +40,7 @@ struct AdjOps {
 
 struct AdjArgs {
   AdjOps op;
-  const double* lam;  // E x NV x N_b  adjoint seed
+  double* lam;        // E x NV x N_b  adjoint seed (updated in place when rk == 1)
   int apply_minv;     // 1: lam is the adjoint of M^-1 R (mu = Mref^-1 lam / |J|); 0: of R (mu = lam)
   double* Hb;         // E x NV x N_aq  mu traced to the face slots
   double* RtV;        // E x NG x N_b   volume part of the adjoint of R_t
@@ -49,7 +49,13 @@ struct AdjArgs {
   double* GFb;        // E x NG x N_aq  adjoint of the lifted gradient traces
   double* Pown;       // E x NV x N_aq  lifting-face adjoint to the own trace
   double* Qnb;        // E x NV x N_aq  lifting-face adjoint to the neighbour's trace of the same point
-  double* out;        // E x NV x N_b   (dR/dU)^T mu
+  double* out;        // E x NV x N_b   (dR/dU)^T mu (rk == 0)
+  // rk == 1: instead of `out`, apply the transpose of the RK stage U_{s+1} = c0 U_0 + c1 U_s +
+  // c2 dt L(U_s) in place: acc (+)= c0 lam, lam = c1 lam + c2dt VJP, and on the last transposed
+  // stage lam += acc
+  int rk, rk_first, rk_last;
+  double* acc;
+  double c0, c1, c2dt;
 };
 
 // ---- pointwise VJPs --------------------------------------------------------------------------
@@ -91,38 +97,50 @@ __device__ __forceinline__ void viscousGradientVjp(const Phys& ph, const double*
   }
 }
 
-// volume flux out[b NV + v] = sum_a (F_c - F_v)[a][v](u, g) G[a + D b]: given fbar (same layout),
-// ub = (d out / d u)^T fbar and gb = (d out / d g)^T fbar
+// Fb[a + D v] = sum_b fbar[b NV + v] G[a + D b]: the adjoint of the raw flux from that of the
+// contracted volume flux out[b NV + v] = sum_a F[a + D v] G[a + D b]
 template <int D>
-__device__ __forceinline__ void volumeFluxVjp(const Phys& ph, const double* u, const double* g, const double* G,
-                                              const double* fbar, double* ub, double* gb) {
-  constexpr int NV = D + 2, NG = NV * D;
-  using DT = Dual<NV>;
-  double Fb[D * NV];
+__device__ __forceinline__ void volumeFluxBar(const double* fbar, const double* G, double* Fb) {
+  constexpr int NV = D + 2;
   for (int aa = 0; aa < D; aa++)
     for (int v = 0; v < NV; v++) {
       double s = 0.0;
       for (int b = 0; b < D; b++) s += fbar[b * NV + v] * G[aa + D * b];
       Fb[aa + D * v] = s;
     }
-  DT uD[NV], gD[NG], fD[D * NV];
-  for (int k = 0; k < NV; k++) uD[k] = seed<NV>(u[k], k);
-  for (int r = 0; r < NG; r++) gD[r] = DT(g[r]);
-  rawVolumeFlux<D, DT, true>(ph, uD, gD, fD);
-  for (int k = 0; k < NV; k++) {
-    double s = 0.0;
-    for (int i = 0; i < D * NV; i++) s += Fb[i] * fD[i].d[k];
-    ub[k] = s;
-  }
-  viscousGradientVjp<D>(ph, u, Fb, gb);
-  for (int r = 0; r < NG; r++) gb[r] = -gb[r];
 }
 
-// One side of the face-flux VJP: derivatives with respect to the state of side SIDE (0: left,
-// 1: right) by dual numbers, and with respect to that side's gradient by linearity.
+// One direction of the volume-flux VJP: dir < NV gives ub[dir] = sum_i Fb_i dF_i/du_dir (one dual
+// seed), dir >= NV gives gb[dir - NV] = -sum_i Fb_i dF_i/dg (the viscous flux is linear in g).
+template <int D>
+__device__ __forceinline__ double volumeFluxVjpDir(const Phys& ph, const double* u, const double* g, const double* Fb,
+                                                   int dir) {
+  constexpr int NV = D + 2, NG = NV * D;
+  if (dir < NV) {
+    using DT = Dual<1>;
+    DT uD[NV], gD[NG], fD[D * NV];
+    for (int k = 0; k < NV; k++) uD[k] = k == dir ? seed<1>(u[k], 0) : DT(u[k]);
+    for (int r = 0; r < NG; r++) gD[r] = DT(g[r]);
+    rawVolumeFlux<D, DT, true>(ph, uD, gD, fD);
+    double s = 0.0;
+    for (int i = 0; i < D * NV; i++) s += Fb[i] * fD[i].d[0];
+    return s;
+  }
+  double w[D + 3], e[NG], pg[NG], fv[D * NV];
+  computational<D>(ph, u, w);
+  for (int r = 0; r < NG; r++) e[r] = r == dir - NV ? 1.0 : 0.0;
+  primitiveGradient<D>(ph, w, u, e, pg);
+  viscousRawFlux<D>(ph, w, pg, fv);
+  double s = 0.0;
+  for (int i = 0; i < D * NV; i++) s += Fb[i] * fv[i];
+  return -s;
+}
+
+// State part of one side of the face-flux VJP (dual numbers seeded on the NV components of side
+// SIDE): ub = (dt/du_SIDE)^T tb for t = H_c(ul, ur; n) - (F_v(ul, gl) + F_v(ur, gr)) . n / 2
 template <int D, int SIDE>
-__device__ __forceinline__ void faceFluxVjpSide(const Phys& ph, const double* n, const double* tb, const double* ul,
-                                                const double* ur, const double* gs, double* ub, double* gb) {
+__device__ __forceinline__ void faceStateVjp(const Phys& ph, const double* n, const double* tb, const double* ul,
+                                             const double* ur, const double* gs, double* ub) {
   constexpr int NV = D + 2, NG = NV * D;
   using DT = Dual<NV>;
   DT nD[D], uL[NV], uR[NV], wL[D + 3], wR[D + 3], h[NV];
@@ -152,24 +170,6 @@ __device__ __forceinline__ void faceFluxVjpSide(const Phys& ph, const double* n,
     }
     ub[k] = s;
   }
-  // gradient part: -(1/2) sum_{v,a} tb[v] n_a dF_v[a][v]/dg
-  double Fb[D * NV];
-  for (int v = 0; v < NV; v++)
-    for (int d = 0; d < D; d++) Fb[d + D * v] = -0.5 * tb[v] * n[d];
-  viscousGradientVjp<D>(ph, SIDE == 0 ? ul : ur, Fb, gb);
-}
-
-// face flux t = (H_c(ul, ur; n) - (F_v(ul, gl) + F_v(ur, gr)) . n / 2) jw: given tbar, the
-// adjoints of ul, ur, gl, gr
-template <int D>
-__device__ __forceinline__ void faceFluxVjp(const Phys& ph, const double* n, double jw, const double* ul,
-                                            const double* ur, const double* gl, const double* gr, const double* tbar,
-                                            double* ubl, double* ubr, double* gbl, double* gbr) {
-  constexpr int NV = D + 2;
-  double tb[NV];
-  for (int v = 0; v < NV; v++) tb[v] = tbar[v] * jw;
-  faceFluxVjpSide<D, 0>(ph, n, tb, ul, ur, gl, ubl, gbl);
-  faceFluxVjpSide<D, 1>(ph, n, tb, ul, ur, gr, ubr, gbr);
 }
 
 // ---- kernel 1: volume part ----------------------------------------------------------------------
@@ -179,8 +179,10 @@ __global__ __launch_bounds__(kThreads) void nsAdjVolumeMfma(Geo g, Phys ph, Stag
   constexpr int NV = LD::NV, NG = LD::NG, EPB = LD::EPB, NAQ = LD::NAQ;
   constexpr int NBk = LD::NBk, NBm = LD::NBm, NQm = LD::NQm, NQk = r4(NQ), NAQm = LD::NAQm;
   constexpr int NBr = LD::NBr, NDQm = r16(NQ * D);
+  constexpr int AJV = LD::AJ > 2 * NQm * 16 ? LD::AJ : 2 * NQm * 16;  // grad u_q and grad-bar_q
+  static_assert(2 * NBr >= NQm, "u-bar_q reuses the lifting work area");
   __shared__ double sUq[NQm * 16];
-  __shared__ double sAJ[LD::AJ];
+  __shared__ double sAJ[AJV];
   __shared__ double sRR[2 * NBr * 16];
   __shared__ double sFb[NDQm * 16];
   const MOps& op = g.op;
@@ -233,20 +235,26 @@ __global__ __launch_bounds__(kThreads) void nsAdjVolumeMfma(Geo g, Phys ph, Stag
     tileStore<double>(acc, sFb, m0, lane);
   }
   __syncthreads();
-  // pointwise VJP of the volume flux; u-bar_q and grad-bar_q overwrite u_q and grad u_q in place
-  for (int idx = tid; idx < EPB * NQ; idx += kThreads) {
-    const int le = idx / NQ, q = idx % NQ, e = e0 + le;
+  // pointwise VJP of the volume flux, one thread per (direction, element, point): u-bar_q into
+  // sUb (the free sRR area) and grad-bar_q into sGb (behind grad u_q in sAJ)
+  double* sUb = sRR;
+  double* sGb = sAJ + NQm * 16;
+  for (int idx = tid; idx < (NV + NG) * EPB * NQ; idx += kThreads) {
+    const int dir = idx / (EPB * NQ), rem = idx % (EPB * NQ), le = rem / NQ, q = rem % NQ, e = e0 + le;
     if (e >= g.E) continue;
-    double u[NV], gq[NG], G[D * D], fb[D * NV], ub[NV], gb[NG];
+    double u[NV], gq[NG], G[D * D], fb[D * NV], Fb[D * NV];
     for (int v = 0; v < NV; v++) u[v] = sUq[q * 16 + le * NV + v];
     for (int r = 0; r < NG; r++) gq[r] = sGq[q * 16 + le * NG + r];
     const double jw = g.detj[e] * g.wq[q];
     for (int i = 0; i < D * D; i++) G[i] = g.ginv[static_cast<int64_t>(e) * D * D + i] * jw;
     for (int b = 0; b < D; b++)
       for (int v = 0; v < NV; v++) fb[b * NV + v] = sFb[(q * D + b) * 16 + le * NV + v];
-    volumeFluxVjp<D>(ph, u, gq, G, fb, ub, gb);
-    for (int v = 0; v < NV; v++) sUq[q * 16 + le * NV + v] = ub[v];
-    for (int r = 0; r < NG; r++) sGq[q * 16 + le * NG + r] = gb[r];
+    volumeFluxBar<D>(fb, G, Fb);
+    const double x = volumeFluxVjpDir<D>(ph, u, gq, Fb, dir);
+    if (dir < NV)
+      sUb[q * 16 + le * NV + dir] = x;
+    else
+      sGb[q * 16 + le * NG + dir - NV] = x;
   }
   __syncthreads();
   // R-bar_t (volume part) = (Mref^-1 V^T)^T grad-bar / |J| -> HBM
@@ -256,7 +264,7 @@ __global__ __launch_bounds__(kThreads) void nsAdjVolumeMfma(Geo g, Phys ph, Stag
 #pragma unroll 1
   for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
     acc_t<double> acc = zeroAcc<double>();
-    tileAcc<double, NBm>(acc, j.op.pinterpT, m0, sGq, 0, NQk, lane);
+    tileAcc<double, NBm>(acc, j.op.pinterpT, m0, sGb, 0, NQk, lane);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int b = m0 + Mfma<double>::row(rr, i);
@@ -267,7 +275,7 @@ __global__ __launch_bounds__(kThreads) void nsAdjVolumeMfma(Geo g, Phys ph, Stag
 #pragma unroll 1
   for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
     acc_t<double> acc = zeroAcc<double>();
-    tileAcc<double, NBm>(acc, j.op.interpT, m0, sUq, 0, NQk, lane);
+    tileAcc<double, NBm>(acc, j.op.interpT, m0, sUb, 0, NQk, lane);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int b = m0 + Mfma<double>::row(rr, i);
@@ -280,33 +288,45 @@ __global__ __launch_bounds__(kThreads) void nsAdjVolumeMfma(Geo g, Phys ph, Stag
 template <int D, int NF, int NFQ>
 __global__ __launch_bounds__(kThreads) void nsAdjFaceKernel(Geo g, Phys ph, StageArgs a, AdjArgs j) {
   constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
+  // one thread per (part, side, face point): part 0 is the adjoint of that side's state (dual
+  // numbers seeded on its NV components), part 1 that of its gradient (the viscous flux is linear
+  // in the gradient); part and side are uniform over consecutive face points
+  const int64_t npts = static_cast<int64_t>(g.F) * NFQ;
   const int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (idx >= static_cast<int64_t>(g.F) * NFQ) return;
-  const int64_t f = idx / NFQ;
-  const int jq = static_cast<int>(idx % NFQ);
+  if (idx >= npts * 4) return;
+  const int part = static_cast<int>(idx / (2 * npts));
+  const int side = static_cast<int>((idx / npts) & 1);
+  const int64_t pt = idx % npts;
+  const int64_t f = pt / NFQ;
+  const int jq = static_cast<int>(pt % NFQ);
   const int k = g.fseq[f * NFQ + jq];
   const int64_t L = g.fpar[2 * f], R = g.fpar[2 * f + 1];
   const int slL = g.floc[2 * f] * NFQ + jq, slR = g.floc[2 * f + 1] * NFQ + k;
-  double ul[NV], ur[NV], gl[NG], gr[NG], tb[NV], n[D];
+  double ul[NV], ur[NV], gs[NG], tb[NV], n[D];
+  const double jw = g.fjw[f * NFQ + jq];
   for (int v = 0; v < NV; v++) {
     ul[v] = a.Uf_in[(L * NV + v) * NAQ + slL];
     ur[v] = a.Uf_in[(R * NV + v) * NAQ + slR];
-    tb[v] = j.Hb[(R * NV + v) * NAQ + slR] - j.Hb[(L * NV + v) * NAQ + slL];
+    tb[v] = (j.Hb[(R * NV + v) * NAQ + slR] - j.Hb[(L * NV + v) * NAQ + slL]) * jw;
   }
-  for (int r = 0; r < NG; r++) {
-    gl[r] = a.Gf[(L * NG + r) * NAQ + slL];
-    gr[r] = a.Gf[(R * NG + r) * NAQ + slR];
-  }
+  const int64_t E = side == 0 ? L : R;
+  const int sl = side == 0 ? slL : slR;
   for (int d = 0; d < D; d++) n[d] = g.fnormal[(f * NFQ + jq) * D + d];
-  double ubl[NV], ubr[NV], gbl[NG], gbr[NG];
-  faceFluxVjp<D>(ph, n, g.fjw[f * NFQ + jq], ul, ur, gl, gr, tb, ubl, ubr, gbl, gbr);
-  for (int v = 0; v < NV; v++) {
-    j.UFf[(L * NV + v) * NAQ + slL] = ubl[v];
-    j.UFf[(R * NV + v) * NAQ + slR] = ubr[v];
-  }
-  for (int r = 0; r < NG; r++) {
-    j.GFb[(L * NG + r) * NAQ + slL] = gbl[r];
-    j.GFb[(R * NG + r) * NAQ + slR] = gbr[r];
+  if (part == 0) {
+    for (int r = 0; r < NG; r++) gs[r] = a.Gf[(E * NG + r) * NAQ + sl];
+    double ub[NV];
+    if (side == 0)
+      faceStateVjp<D, 0>(ph, n, tb, ul, ur, gs, ub);
+    else
+      faceStateVjp<D, 1>(ph, n, tb, ul, ur, gs, ub);
+    for (int v = 0; v < NV; v++) j.UFf[(E * NV + v) * NAQ + sl] = ub[v];
+  } else {
+    // -(1/2) sum_{v,a} tb[v] n_a dF_v[a][v]/dg of this side
+    double Fb[D * NV], gb[NG];
+    for (int v = 0; v < NV; v++)
+      for (int d = 0; d < D; d++) Fb[d + D * v] = -0.5 * tb[v] * n[d];
+    viscousGradientVjp<D>(ph, side == 0 ? ul : ur, Fb, gb);
+    for (int r = 0; r < NG; r++) j.GFb[(E * NG + r) * NAQ + sl] = gb[r];
   }
 }
 
@@ -470,16 +490,23 @@ __global__ __launch_bounds__(kThreads) void nsAdjGatherMfma(Geo g, Phys ph, Stag
       const int b = m0 + Mfma<double>::row(rr, i);
       if (col_ok && b < NB) {
         const int64_t o = (static_cast<int64_t>(e_c) * NV + v_c) * NB + b;
-        j.out[o] = j.Ua[o] + acc[i];
+        const double vjp = j.Ua[o] + acc[i];
+        if (!j.rk) {
+          j.out[o] = vjp;
+        } else {
+          const double l = j.lam[o];
+          const double ac = j.rk_first ? j.c0 * l : j.c0 * l + j.acc[o];
+          const double ln = j.c2dt * vjp + j.c1 * l;
+          if (j.rk_last) {
+            j.lam[o] = ac + ln;
+          } else {
+            j.acc[o] = ac;
+            j.lam[o] = ln;
+          }
+        }
       }
     }
   }
-}
-
-// y = alpha x + beta y (n doubles)
-__global__ __launch_bounds__(kThreads) void axpbyKernel(double* y, const double* x, double alpha, double beta, int64_t n) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (i < n) y[i] = beta == 0.0 ? alpha * x[i] : alpha * x[i] + beta * y[i];  // beta 0: y is not read
 }
 
 }  // namespace sdg

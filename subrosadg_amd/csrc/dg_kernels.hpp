@@ -121,6 +121,12 @@ struct StageArgs {
   double c0, c1, c2dt;  // U <- c1 U + c0 Un + c2dt R Minv (TimeIntegration.cpp:181-198)
   int last_stage;
   int mode;  // 0: stage update, 1: residual only
+  // recording (matrix-core stage kernels): the updated state is also written to pool slot
+  // (step, stage + 1), or (step + 1, 0) after the last stage, with step = status[1]; slots of
+  // rec_S * state-size doubles per step, rec_nsteps steps
+  double* rec;
+  int rec_stage, rec_S, rec_nsteps;
+  int64_t rec_n;
   MixRef mix;
 };
 

@@ -185,6 +185,11 @@ __device__ __forceinline__ void mfmaStageTail(const Geo& g, const StageArgs& a, 
         u += a.c2dt * (rm[i] / idj);
         a.U[gi] = u;
         if (a.last_stage) a.Un[gi] = u;
+        if (a.rec) {
+          const int step = a.status[1] + (a.last_stage ? 1 : 0);
+          const int stage = a.last_stage ? 0 : a.rec_stage + 1;
+          if (step < a.rec_nsteps) a.rec[(static_cast<int64_t>(step) * a.rec_S + stage) * a.rec_n + gi] = u;
+        }
         un[i] = u;
       }
     }

@@ -93,7 +93,8 @@ Launchers makeLaunchers() {
                            dim3(sdg::kThreads), 0, st, g, ph, a, j);
       };
       l.adj_face = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
-        hipLaunchKernelGGL((sdg::nsAdjFaceKernel<D, NF, NFQ>), dim3(blocksFor(g.F * NFQ, sdg::kThreads)),
+        hipLaunchKernelGGL((sdg::nsAdjFaceKernel<D, NF, NFQ>),
+                           dim3(blocksFor(g.F * NFQ * 4, sdg::kThreads)),
                            dim3(sdg::kThreads), 0, st, g, ph, a, j);
       };
       l.adj_lift[0] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
@@ -247,6 +248,19 @@ struct sdg_ctx {
   double graph_dt[2] = {0.0, 0.0};
   hipGraphExec_t adj_graph[2] = {nullptr, nullptr};  // one adjoint step, by trace-buffer parity
   double adj_graph_dt[2] = {0.0, 0.0};
+  // recorded run that keeps every stage state (when the snapshot budget covers every step): the
+  // pool holds steps x stages states, written by the recording graph; the reverse sweep then
+  // needs no recomputation
+  double* adj_pool = nullptr;
+  size_t adj_pool_cap = 0;  // in doubles
+  bool adj_stored = false;
+  bool recording = false;  // stage launches also write the recording pool (graph capture only)
+  int rec_nsteps = 0;
+  hipGraphExec_t rec_graph[2] = {nullptr, nullptr};
+  double rec_graph_dt[2] = {0.0, 0.0};
+  int rec_graph_n[2] = {0, 0};
+  hipGraphExec_t adjst_graph[2] = {nullptr, nullptr};
+  double adjst_graph_dt[2] = {0.0, 0.0};
   bool use_graph = true;
   float last_ms = 0.0f;
 
@@ -313,6 +327,10 @@ void destroyGraphs(sdg_ctx* ctx) {
     ctx->graph[i] = nullptr;
     if (ctx->adj_graph[i]) (void)hipGraphExecDestroy(ctx->adj_graph[i]);
     ctx->adj_graph[i] = nullptr;
+    if (ctx->rec_graph[i]) (void)hipGraphExecDestroy(ctx->rec_graph[i]);
+    ctx->rec_graph[i] = nullptr;
+    if (ctx->adjst_graph[i]) (void)hipGraphExecDestroy(ctx->adjst_graph[i]);
+    ctx->adjst_graph[i] = nullptr;
   }
 }
 
@@ -350,6 +368,13 @@ void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hi
   a.c2dt = c[2] * dt;
   a.last_stage = (stage == ctx->stages() - 1) ? 1 : 0;
   a.mode = mode;
+  if (ctx->recording && mode == 0) {
+    a.rec = ctx->adj_pool;
+    a.rec_stage = stage;
+    a.rec_S = ctx->stages();
+    a.rec_nsteps = ctx->rec_nsteps;
+    a.rec_n = static_cast<int64_t>(ctx->E) * ctx->nv * ctx->nb;
+  }
   a.Wout = wout;
   if (mode == 1) a.status = nullptr;
   if (ev) (void)hipEventRecord(ev[0], ctx->stream);
@@ -860,6 +885,7 @@ void sdg_destroy(sdg_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   destroyGraphs(ctx);
   for (void* p : ctx->allocations) (void)hipFree(p);
+  if (ctx->adj_pool) (void)hipFree(ctx->adj_pool);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1116,24 +1142,21 @@ int ensureAdjoint(sdg_ctx* ctx) {
   if (allocate(ctx, nU, &ctx->adj_lam) || allocate(ctx, nS, &ctx->adj_Hb) || allocate(ctx, nG, &ctx->adj_RtV) ||
       allocate(ctx, nU, &ctx->adj_Ua) || allocate(ctx, nS, &ctx->adj_UFf) || allocate(ctx, nGS, &ctx->adj_GFb) ||
       allocate(ctx, nS, &ctx->adj_Pown) || allocate(ctx, nS, &ctx->adj_Qnb) || allocate(ctx, nU, &ctx->adj_out) ||
-      allocate(ctx, nU, &ctx->adj_S[0]) || allocate(ctx, nU, &ctx->adj_S[1]) || allocate(ctx, nU, &ctx->adj_S[2]) ||
-      allocate(ctx, nU, &ctx->adj_acc) ||
+      allocate(ctx, 3 * nU, &ctx->adj_S[0]) || allocate(ctx, nU, &ctx->adj_acc) ||
       allocate(ctx, nU, &ctx->adj_tmp))
     return SDG_E_NOMEM;
+  ctx->adj_S[1] = ctx->adj_S[0] + nU;  // stage states, one contiguous block
+  ctx->adj_S[2] = ctx->adj_S[0] + 2 * nU;
   ctx->adj_ready = true;
   return SDG_OK;
 }
 
 size_t stateCount(const sdg_ctx* ctx) { return static_cast<size_t>(ctx->E) * ctx->nv * ctx->nb; }
 
-void axpby(sdg_ctx* ctx, double* y, const double* x, double alpha, double beta) {
-  const int64_t n = static_cast<int64_t>(stateCount(ctx));
-  hipLaunchKernelGGL(sdg::axpbyKernel, dim3((n + sdg::kThreads - 1) / sdg::kThreads), dim3(sdg::kThreads), 0,
-                     ctx->stream, y, x, alpha, beta, n);
-}
-
 // out = (dR/dU)^T mu at state U (device layout), mu = lam, or Mref^-1 lam / |J| with apply_minv
-int vjpAt(sdg_ctx* ctx, const double* U, const double* lam, int apply_minv, double* out) {
+// With stage >= 0 the gather kernel applies the transpose of RK stage `stage` to adj_lam in place
+// (accumulating into adj_acc) instead of writing `out`.
+int vjpAt(sdg_ctx* ctx, const double* U, double* lam, int apply_minv, double* out, int stage = -1) {
   double* Uf = ctx->Uf[ctx->cur ^ 1];  // scratch: the next stage rewrites this trace buffer
   ctx->launch.trace(ctx->stream, ctx->geo, U, Uf, ctx->mix);
   StageArgs a = baseArgs(ctx);
@@ -1154,6 +1177,17 @@ int vjpAt(sdg_ctx* ctx, const double* U, const double* lam, int apply_minv, doub
   j.Pown = ctx->adj_Pown;
   j.Qnb = ctx->adj_Qnb;
   j.out = out;
+  if (stage >= 0) {
+    double c[3];
+    ctx->rkCoefficients(stage, c);
+    j.rk = 1;
+    j.rk_first = stage == ctx->stages() - 1;
+    j.rk_last = stage == 0;
+    j.acc = ctx->adj_acc;
+    j.c0 = c[0];
+    j.c1 = c[1];
+    j.c2dt = c[2] * ctx->adj_dt;
+  }
   const int vi = ctx->prob.viscous_flux == SDG_VISC_BR1 ? 0 : 1;
   ctx->launch.adj_volume[vi](ctx->stream, ctx->geo, ctx->phys, a, j);
   ctx->launch.adj_face(ctx->stream, ctx->geo, ctx->phys, a, j);
@@ -1207,48 +1241,67 @@ int enqueueAdjointStep(sdg_ctx* ctx) {
     enqueueStage(ctx, s, ctx->adj_dt, 0, nullptr);
     if ((rc = saveState(ctx, ctx->adj_S[s + 1]))) return rc;
   }
-  for (int s = S - 1; s >= 0; s--) {
-    double c[3];
-    ctx->rkCoefficients(s, c);
-    if ((rc = vjpAt(ctx, ctx->adj_S[s], ctx->adj_lam, 1, ctx->adj_out))) return rc;
-    axpby(ctx, ctx->adj_acc, ctx->adj_lam, c[0], s == S - 1 ? 0.0 : 1.0);  // beta 0: acc = c0 lam
-    axpby(ctx, ctx->adj_lam, ctx->adj_out, c[2] * ctx->adj_dt, c[1]);
-  }
-  axpby(ctx, ctx->adj_lam, ctx->adj_acc, 1.0, 1.0);
+  for (int s = S - 1; s >= 0; s--)
+    if ((rc = vjpAt(ctx, ctx->adj_S[s], ctx->adj_lam, 1, nullptr, s))) return rc;
   HIP_TRY(ctx, hipGetLastError());
+  return SDG_OK;
+}
+
+// Capture (once per trace parity and dt) and replay a graph built by `body`; the body leaves the
+// trace parity flipped by `flips`.
+template <typename Body>
+int replayGraph(sdg_ctx* ctx, hipGraphExec_t* graphs, double* graph_dt, double dt, int flips, Body body) {
+  const int c = ctx->cur;
+  if (!graphs[c] || graph_dt[c] != dt) {
+    if (graphs[c]) (void)hipGraphExecDestroy(graphs[c]);
+    graphs[c] = nullptr;
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = body();
+    hipGraph_t captured = nullptr;
+    const hipError_t ee = hipStreamEndCapture(ctx->stream, &captured);
+    if (rc) {
+      if (captured) (void)hipGraphDestroy(captured);
+      ctx->cur = c;
+      return rc;
+    }
+    if (ee != hipSuccess) return fail(ctx, SDG_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+    const hipError_t e = hipGraphInstantiate(&graphs[c], captured, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(captured);
+    if (e != hipSuccess) return fail(ctx, SDG_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    graph_dt[c] = dt;
+    ctx->cur = c;  // capture advanced cur; the replay below advances it for real
+  }
+  HIP_TRY(ctx, hipGraphLaunch(graphs[c], ctx->stream));
+  ctx->cur = c ^ flips;
   return SDG_OK;
 }
 
 int stepAdjoint(sdg_ctx* ctx, const double* U0) {
   HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_S[0], U0, stateCount(ctx) * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-  const int flips = (ctx->stages() - 1) & 1;
-  if (!ctx->use_graph) {
-    const int rc = enqueueAdjointStep(ctx);
-    return rc;
-  }
-  const int c = ctx->cur;
-  if (!ctx->adj_graph[c] || ctx->adj_graph_dt[c] != ctx->adj_dt) {
-    if (ctx->adj_graph[c]) (void)hipGraphExecDestroy(ctx->adj_graph[c]);
-    ctx->adj_graph[c] = nullptr;
-    hipGraph_t graph;
-    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueueAdjointStep(ctx);
-    hipGraph_t captured = nullptr;
-    const hipError_t ee = hipStreamEndCapture(ctx->stream, &captured);
-    if (rc) {
-      if (captured) (void)hipGraphDestroy(captured);
-      return rc;
-    }
-    if (ee != hipSuccess) return fail(ctx, SDG_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
-    graph = captured;
-    const hipError_t e = hipGraphInstantiate(&ctx->adj_graph[c], graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return fail(ctx, SDG_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    ctx->adj_graph_dt[c] = ctx->adj_dt;
-    ctx->cur = c;  // capture advanced cur; the replay below advances it for real
-  }
-  HIP_TRY(ctx, hipGraphLaunch(ctx->adj_graph[c], ctx->stream));
-  ctx->cur = c ^ flips;
+  if (!ctx->use_graph) return enqueueAdjointStep(ctx);
+  return replayGraph(ctx, ctx->adj_graph, ctx->adj_graph_dt, ctx->adj_dt, (ctx->stages() - 1) & 1,
+                     [&]() { return enqueueAdjointStep(ctx); });
+}
+
+// One adjoint step from the stage states in adj_S[0 .. S-1] (no recomputation).
+int enqueueAdjointStepStored(sdg_ctx* ctx) {
+  const int S = ctx->stages();
+  int rc;
+  for (int s = S - 1; s >= 0; s--)
+    if ((rc = vjpAt(ctx, ctx->adj_S[s], ctx->adj_lam, 1, nullptr, s))) return rc;
+  HIP_TRY(ctx, hipGetLastError());
+  return SDG_OK;
+}
+
+// One recorded forward step: every stage's updated state also goes to the pool slot of the next
+// stage input, the step index being read on the device from the status counter.
+int enqueueRecordStep(sdg_ctx* ctx, double dt, int nsteps) {
+  ctx->recording = true;
+  ctx->rec_nsteps = nsteps;
+  for (int s = 0; s < ctx->stages(); s++) enqueueStage(ctx, s, dt, 0, nullptr);
+  ctx->recording = false;
+  ctx->launch.monitor(ctx->stream, ctx->mon, ctx->E, ctx->rel, ctx->status);
+  HIP_TRY(ctx, hipGetLastError());
   return SDG_OK;
 }
 
@@ -1319,6 +1372,52 @@ int sdg_adjoint_forward(sdg_ctx* ctx, double dt, int nsteps, int snapshots) {
   if (nsteps < 0 || snapshots < 1 || !(dt >= 0.0)) return fail(ctx, SDG_E_INVALID, "sdg_adjoint_forward: bad argument");
   int rc = ensureAdjoint(ctx);
   if (rc) return rc;
+  ctx->adj_stored = false;
+  if (snapshots >= nsteps && nsteps > 0 && ctx->use_graph) {
+    // every step fits: record every stage state instead of recomputing them in the reverse sweep
+    const size_t need = static_cast<size_t>(nsteps) * ctx->stages() * stateCount(ctx);
+    if (need > ctx->adj_pool_cap) {
+      if (ctx->adj_pool) (void)hipFree(ctx->adj_pool);
+      ctx->adj_pool = nullptr;
+      ctx->adj_pool_cap = 0;
+      void* p = nullptr;
+      if (hipMalloc(&p, need * sizeof(double)) == hipSuccess) {
+        ctx->adj_pool = static_cast<double*>(p);
+        ctx->adj_pool_cap = need;
+      } else {
+        (void)hipGetLastError();  // out of device memory: fall back to the checkpointed schedule
+      }
+    }
+    if (ctx->adj_pool && need <= ctx->adj_pool_cap) {
+      ctx->adj_dt = dt;
+      ctx->adj_nsteps = -1;
+      hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
+      HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_pool, ctx->U, stateCount(ctx) * sizeof(double), hipMemcpyDeviceToDevice,
+                                  ctx->stream));  // slot (0, 0): the initial state
+      const int flips = ctx->stages() & 1;
+      for (int i = 0; i < nsteps; i++) {
+        // the graph bakes in the pool's step count; rebuild it when that changes
+        if (ctx->rec_graph_n[ctx->cur] != nsteps) ctx->rec_graph_dt[ctx->cur] = -1.0;
+        if ((rc = replayGraph(ctx, ctx->rec_graph, ctx->rec_graph_dt, dt, flips,
+                              [&]() { return enqueueRecordStep(ctx, dt, nsteps); })))
+          return rc;
+        ctx->rec_graph_n[ctx->cur ^ flips] = nsteps;
+      }
+      int st[2] = {0, 0};
+      HIP_TRY(ctx, hipMemcpyAsync(st, ctx->status, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      if (st[0] || st[1] != nsteps) {
+        ctx->cur = 0;
+        ctx->launch.trace(ctx->stream, ctx->geo, ctx->U, ctx->traces(0), ctx->mix);
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        return fail(ctx, SDG_E_STATE, "sdg_adjoint_forward: the residual monitor turned NaN during the recorded run");
+      }
+      ctx->adj_stored = true;
+      ctx->adj_nsteps = nsteps;
+      ctx->adj_nslots = nsteps;
+      return SDG_OK;
+    }
+  }
   const int nslots = std::max(1, std::min(snapshots, nsteps));
   while (static_cast<int>(ctx->snaps.size()) < nslots) {
     double* p = nullptr;
@@ -1365,10 +1464,22 @@ int sdg_adjoint_backward(sdg_ctx* ctx, const double* lambda_final, double* lambd
   std::vector<double> dev;
   toDevice(ctx, lambda_final, dev);
   HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_lam, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-  int rc = reverseSweep(ctx, 0, ctx->adj_nsteps, ctx->adj_nslots - 1, 0);
-  if (rc) return rc;
+  int rc;
+  if (ctx->adj_stored) {
+    const int S = ctx->stages();
+    const size_t n = stateCount(ctx);
+    for (int i = ctx->adj_nsteps - 1; i >= 0; i--) {
+      HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_S[0], ctx->adj_pool + static_cast<size_t>(i) * S * n, S * n * sizeof(double),
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+      if ((rc = replayGraph(ctx, ctx->adjst_graph, ctx->adjst_graph_dt, ctx->adj_dt, 0,
+                            [&]() { return enqueueAdjointStepStored(ctx); })))
+        return rc;
+    }
+  } else if ((rc = reverseSweep(ctx, 0, ctx->adj_nsteps, ctx->adj_nslots - 1, 0))) {
+    return rc;
+  }
   HIP_TRY(ctx, hipMemcpyAsync(dev.data(), ctx->adj_lam, dev.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  if ((rc = loadState(ctx, ctx->snaps[0]))) return rc;
+  if ((rc = loadState(ctx, ctx->adj_stored ? ctx->adj_pool : ctx->snaps[0]))) return rc;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   toReference(ctx, dev, lambda_initial);
   return SDG_OK;
