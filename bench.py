@@ -39,9 +39,14 @@ PEAK_HBM_GBS = 8000.0
 
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled
 # by a copy kernel of known size (fetch x2.000, write x1.000); kernel summary in
-# profiles/ns_tgv32_p3_fp64_mfma_kernels.md. Keyed by (kernel, precision mode, cells per direction,
-# order).
-PROFILED_TRAFFIC = {("nsGradMfma", "fp64", 32, 3): (190387.6 * 2.0 + 614400.2 * 1.0) * 1024}
+# profiles/ns_tgv32_p3_fp64_mixed_mfma_kernels.md. Keyed by (kernel, precision mode, cells per
+# direction, order).
+PROFILED_TRAFFIC = {("nsGradMfma", "fp64", 32, 3): (190207.1 * 2.0 + 614400.0 * 1.0) * 1024,
+                    ("nsGradMfma", "mixed", 32, 3): (144194.1 * 2.0 + 307200.0 * 1.0) * 1024}
+# the same for the batched stage kernel on the 48-problem workload (average over its launches;
+# profiles/batch48_uneven_tri_quad_euler_kernels.md)
+PROFILED_BATCH_TRAFFIC = {"fp64": (346788.6 * 2.0 + 231499.4 * 1.0) * 1024,
+                          "mixed": (312995.3 * 2.0 + 136595.1 * 1.0) * 1024}
 
 
 def tgv_setup(n: int, order: int, mach=0.1, reynolds=1600.0):
@@ -97,7 +102,7 @@ def cpu_baseline(order: int, target_s: float = 15.0):
     t0 = time.perf_counter()
     o.step(dt, 1)
     one = time.perf_counter() - t0
-    steps = max(1, min(200, int(target_s / max(one, 1e-6))))
+    steps = max(1, min(400, int(target_s / max(one, 1e-6))))
     t0 = time.perf_counter()
     o.step(dt, steps)
     t = time.perf_counter() - t0
@@ -134,9 +139,15 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-mesh leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the HBM-counter calibration copy of this many MiB")
+    ap.add_argument("--profile-leg", choices=["adjoint", "batch"], default=None,
+                    help="only run one secondary leg (for kernel profiling) and print its record")
     args = ap.parse_args()
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
+        return
+    if args.profile_leg:
+        leg = adjoint_leg(args) if args.profile_leg == "adjoint" else batch_leg(args)
+        print(json.dumps(leg), flush=True)
         return
     if args.gpus != 1:
         print("bench.py: runs on one GPU only; sharding the element partition over several GPUs "
@@ -153,7 +164,10 @@ def main():
     print(f"[bench] setup+runs {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
     # deviation of the mixed-precision state from the FP64 state after the same warmup + timed steps
     # from the same initial coefficients (the FP64 engine matches the oracle to 1e-12)
-    dev = group_l2(runs["mixed"]["state"], runs["fp64"]["state"], mesh.dim)
+    a_m, a_d = runs["mixed"]["state"], runs["fp64"]["state"]
+    dev = function_l2(mesh, a_m, a_d)
+    dev_coeff = group_l2(a_m, a_d, mesh.dim)
+    dev_all = float(np.linalg.norm(a_m - a_d) / np.linalg.norm(a_d))
     dof = mesh.E * mesh.nb
     f_grad, f_res, _ = algorithmic_counts(mesh)
     roof = {mode: roofline(r["kms"], f_grad, f_res, mesh, args, mode) for mode, r in runs.items()}
@@ -182,6 +196,10 @@ def main():
         "fp64": {"value": runs["fp64"]["value"], "ms_per_step": runs["fp64"]["ms_per_step"], "dtype": "f64"},
         "mixed_over_fp64": head["value"] / runs["fp64"]["value"],
         "mixed_l2_rel_dev_vs_fp64": dev,
+        "mixed_l2_rel_dev_definition": "max over density, momentum, energy of ||u_mixed - u_fp64||_L2(domain) / "
+                                       "||u_fp64||_L2(domain), after the warmup and timed steps",
+        "mixed_coefficient_l2_rel_dev_vs_fp64": dev_coeff,
+        "mixed_coefficient_l2_rel_dev_vs_fp64_whole_state": dev_all,
         "l2_dev_after_steps": args.warmup + args.steps,
         "roofline": roof["mixed"],
         "roofline_fp64": roof["fp64"],
@@ -330,6 +348,11 @@ def batch_leg(args, count=48, seed=20261202, steps=100, cfl=0.1):
         out["dof_per_field"] = dof
         # algorithmic HBM bytes of a stage: read U^i and U^n, write U^{i+1} (FP64 state)
         out[mode]["state_gb_per_s"] = 96.0 * dof * 3 * steps / (ms * 1e-3) / 1e9
+        if count == 48 and seed == 20261202:  # profiled HBM bytes per stage launch of this workload
+            tb = PROFILED_BATCH_TRAFFIC[mode]
+            out[mode]["profiled_traffic_mb"] = tb / 1e6
+            out[mode]["profiled_hbm_gb_per_s"] = tb * 3 * steps / (ms * 1e-3) / 1e9
+            out[mode]["profiled_hbm_frac"] = out[mode]["profiled_hbm_gb_per_s"] / PEAK_HBM_GBS
         states[mode] = [b.get_state(i) for i in range(count)]
         b.close()
     out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
@@ -337,6 +360,23 @@ def batch_leg(args, count=48, seed=20261202, steps=100, cfl=0.1):
                                           for a, c in zip(states["mixed"], states["fp64"]))
     print(f"[bench] batch leg: {out}", file=sys.stderr, flush=True)
     return out
+
+
+def function_l2(mesh, a, b):
+    """max over variable groups of the relative L2(domain) norm of the difference of two solutions
+    given as modal coefficients (reference layout E x N_b x NV): quadrature with |J| w at the
+    volume points, exact for the squared degree-P polynomials."""
+    nv = mesh.nv
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    jw = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
+    qa = np.einsum("qb,ebv->eqv", V, (a - b).reshape(mesh.E, mesh.nb, nv))
+    qb = np.einsum("qb,ebv->eqv", V, b.reshape(mesh.E, mesh.nb, nv))
+    out = []
+    for g in ([0], list(range(1, mesh.dim + 1)), [mesh.dim + 1]):
+        num = float((jw[..., None] * qa[..., g] ** 2).sum())
+        den = float((jw[..., None] * qb[..., g] ** 2).sum())
+        out.append(np.sqrt(num / den))
+    return max(out)
 
 
 def group_l2(a, b, dim):

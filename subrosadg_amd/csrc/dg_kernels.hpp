@@ -72,6 +72,9 @@ struct Geo {
   int Fb;
   MOps op;                // FP64 operator tables
   MOpsT<float> opf;       // the same tables rounded to FP32 (mixed-precision mode)
+  // mixed-precision mode: the residual-to-update map L^-T of opf.minv kept in FP64, so the update's
+  // hierarchic coefficients are formed without FP32 rounding of their (cancelling) terms
+  const double* minv_mixed64;
 };
 
 template <typename TC>

@@ -158,21 +158,29 @@ __device__ __forceinline__ void mfmaStageTail(const Geo& g, const StageArgs& a, 
       a.mon[static_cast<int64_t>(e0 + tid / NV) * NV + tid % NV] = s / NQ;
     }
   }
-  // update: U <- c1 U + c0 Un + c2dt (R Mref^-1) / |J|, in FP64 in both modes
+  // update: U <- c1 U + c0 Un + c2dt (R Mref^-1) / |J|, in FP64 in both modes; in the mixed mode
+  // the FP32 residual panel is mapped to the hierarchic coefficients by FP64 matrix cores
   #pragma unroll 1
   for (int m0 = wave * 16; m0 < NBm; m0 += 64) {
-    acc_t<TC> acc = zeroAcc<TC>();
-    tileAcc<TC, NBm>(acc, op.minv, m0, sR, 0, NBk, lane);
+    acc_t<double> acc = zeroAcc<double>();
+    if constexpr (MIX) {
+      const int kr = lane >> 4;
+#pragma unroll 4
+      for (int k = 0; k < NBk; k += 4)
+        acc = Mfma<double>::mma(g.minv_mixed64[(k + kr) * NBm + m0 + c], static_cast<double>(sR[(k + kr) * 16 + c]), acc);
+    } else {
+      tileAcc<double, NBm>(acc, op.minv, m0, sR, 0, NBk, lane);
+    }
     const double idj = col_ok ? g.detj[e] : 1.0;
     double un[4], rm[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      rm[i] = static_cast<double>(acc[i]);
+      rm[i] = acc[i];
       if constexpr (MIX) rm[i] = restoreEnergy<NV>(a, rm[i], v, lane);
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const int bi = m0 + Mfma<TC>::row(rr, i);
+      const int bi = m0 + Mfma<double>::row(rr, i);
       un[i] = 0.0;
       if (col_ok && bi < NB) {
         const int64_t gi = (static_cast<int64_t>(e) * NV + v) * NB + bi;
@@ -198,7 +206,7 @@ __device__ __forceinline__ void mfmaStageTail(const Geo& g, const StageArgs& a, 
     if (m0 < NBk) {
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int bi = m0 + Mfma<TC>::row(rr, i);
+        const int bi = m0 + Mfma<double>::row(rr, i);
         if (bi < NBk) {
           TC x = TC(0);
           if (col_ok && bi < NB) {

@@ -789,6 +789,9 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
           upload(ctx, f32(m_pinterp), &fpi) || upload(ctx, f32(m_ptrace), &fpt) || upload(ctx, f32(m_rinterp), &fri))
         return bail(SDG_E_NOMEM);
       g.opf = {fi, fg, ff, fm, ft, fl, fpi, fpt, fri};
+      double* fm64;
+      if (upload(ctx, m_minv, &fm64)) return bail(SDG_E_NOMEM);
+      g.minv_mixed64 = fm64;
     }
     if (ctx->launch.adj_volume[0]) {
       // transposes for the discrete adjoint

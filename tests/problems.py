@@ -204,3 +204,18 @@ def smooth_direction(xq: np.ndarray, jw: np.ndarray, rng, like: np.ndarray, kmax
             a, b = rng.standard_normal(2)
             d += a * np.cos(ph) + b * np.sin(ph)
     return d * np.sqrt((jw * like ** 2).sum() / (jw * d ** 2).sum())
+
+
+def function_l2(mesh, a: np.ndarray, b: np.ndarray) -> float:
+    """max over variable groups (density, momentum, energy) of ||u_a - u_b||_L2 / ||u_b||_L2 over the
+    domain for modal coefficients in the reference layout (E x N_b x NV); the volume quadrature
+    with |J| w is exact for the squared degree-P polynomials."""
+    nv, E = mesh.dim + 2, mesh.E
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    jw = mesh.field("jacobian_det_w").reshape(E, mesh.nq)
+    qa = np.einsum("qb,ebv->eqv", V, (a - b).reshape(E, mesh.nb, nv))
+    qb = np.einsum("qb,ebv->eqv", V, b.reshape(E, mesh.nb, nv))
+    out = []
+    for g in ([0], list(range(1, mesh.dim + 1)), [mesh.dim + 1]):
+        out.append(np.sqrt((jw[..., None] * qa[..., g] ** 2).sum() / (jw[..., None] * qb[..., g] ** 2).sum()))
+    return float(max(out))
