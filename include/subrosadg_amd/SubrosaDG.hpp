@@ -54,6 +54,7 @@ enum class ThermodynamicModelEnum { Constant };
 enum class EquationOfStateEnum { IdealGas, WeakCompressibleFluid };
 enum class TransportModelEnum { None, Constant, Sutherland };
 enum class TimeIntegrationEnum { ForwardEuler, HeunRK2, SSPRK3 };
+enum class PrecisionEnum { FP64, Mixed };  // not in the reference: the library's precision modes
 
 template <DimensionEnum Dimension, PolynomialOrderEnum PolynomialOrder, BoundaryTimeEnum BoundaryTimeType,
           SourceTermEnum SourceTermType>
@@ -214,6 +215,12 @@ struct System {
     iteration_end_ = iteration_range.second;
   }
   void setDeltaTime(const Real delta_time) { delta_time_ = delta_time; }
+  // Not in the reference (its SUBROSA_DG_SINGLE_PRECISION switch is uniform FP32): FP32 matrix-core
+  // contractions with FP64 state, geometry, pointwise physics and RK accumulation. Call before
+  // synchronize().
+  void setPrecision(const PrecisionEnum precision) {
+    problem_.precision = precision == PrecisionEnum::Mixed ? SDG_PREC_MIXED : SDG_PREC_FP64;
+  }
   // SystemControl.cpp:125-140: only the interval is kept (no files are written)
   void setViewConfig(const std::string& /*output_directory*/, const std::string& /*prefix*/, const int io_interval = -1) {
     io_interval_ = io_interval;
