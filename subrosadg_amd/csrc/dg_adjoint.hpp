@@ -80,21 +80,46 @@ __device__ __forceinline__ void rawVolumeFlux(const Phys& ph, const T* u, const 
   }
 }
 
-// sum_{a,v} Fb[a + D v] * (dF_v[a][v] / dg_j) for every gradient component j (F_v is linear in g)
+// gb[j] = sum_{a,v} Fb[a + D v] dF_v[a][v] / dg_j for every gradient component j: the transpose of
+// the linear map g -> F_v (primitiveGradient then viscousRawFlux at the fixed state u), applied
+// in reverse order of those two functions' operations.
 template <int D>
 __device__ __forceinline__ void viscousGradientVjp(const Phys& ph, const double* u, const double* Fb, double* gb) {
   constexpr int NV = D + 2, NG = NV * D;
   double w[D + 3];
   computational<D>(ph, u, w);
-  for (int jg = 0; jg < NG; jg++) {
-    double e[NG], pg[NG], fv[D * NV];
-    for (int r = 0; r < NG; r++) e[r] = r == jg ? 1.0 : 0.0;
-    primitiveGradient<D>(ph, w, u, e, pg);
-    viscousRawFlux<D>(ph, w, pg, fv);
-    double s = 0.0;
-    for (int i = 0; i < D * NV; i++) s += Fb[i] * fv[i];
-    gb[jg] = s;
+  const double rho = w[0];
+  const double t = w[D + 1] / ph.cv;
+  const double mu = ph.transport == 2 ? ph.mu * sutherlandRatio(t) : ph.mu;
+  const double kap = ph.transport == 2 ? ph.kappa * sutherlandRatio(t) : ph.kappa;
+  // fv[a + D (1 + b)] = tau[a + D b];  fv[a + D (D + 1)] = sum_b tau[a + D b] u_b + kap tg_a
+  double taub[D * D], deb[D];
+  for (int a = 0; a < D; a++) {
+    const double fe = Fb[a + D * (D + 1)];
+    for (int b = 0; b < D; b++) taub[a + D * b] = Fb[a + D * (1 + b)] + fe * w[1 + b];
+    deb[a] = kap * fe / ph.cv;  // tg_a = de_a / cv
   }
+  // tau = mu (vg + vg^T) - 2/3 mu tr(vg) I
+  double trb = 0.0;
+  for (int a = 0; a < D; a++) trb += taub[a + D * a];
+  double vgb[D * D];
+  for (int b = 0; b < D; b++)
+    for (int a = 0; a < D; a++)
+      vgb[a + D * b] = mu * (taub[a + D * b] + taub[b + D * a]) - (a == b ? 2.0 / 3.0 * mu * trb : 0.0);
+  for (int r = 0; r < NG; r++) gb[r] = 0.0;
+  // de_a = (g_E,a - g_rho,a etot) / rho - sum_b vg[a + D b] u_b
+  const double etot = u[D + 1] / rho;
+  for (int a = 0; a < D; a++) {
+    gb[(D + 1) * D + a] += deb[a] / rho;
+    gb[a] -= etot * deb[a] / rho;
+    for (int b = 0; b < D; b++) vgb[a + D * b] -= deb[a] * w[1 + b];
+  }
+  // vg[a + D b] = (g_m_b,a - g_rho,a u_b) / rho
+  for (int b = 0; b < D; b++)
+    for (int a = 0; a < D; a++) {
+      gb[(1 + b) * D + a] += vgb[a + D * b] / rho;
+      gb[a] -= w[1 + b] * vgb[a + D * b] / rho;
+    }
 }
 
 // Fb[a + D v] = sum_b fbar[b NV + v] G[a + D b]: the adjoint of the raw flux from that of the
@@ -110,11 +135,11 @@ __device__ __forceinline__ void volumeFluxBar(const double* fbar, const double* 
     }
 }
 
-// One direction of the volume-flux VJP: dir < NV gives ub[dir] = sum_i Fb_i dF_i/du_dir (one dual
-// seed), dir >= NV gives gb[dir - NV] = -sum_i Fb_i dF_i/dg (the viscous flux is linear in g).
+// One part of the volume-flux VJP: dir < NV returns ub[dir] = sum_i Fb_i dF_i/du_dir (one dual
+// seed); dir == NV writes gb = -(dF/dg)^T Fb (the viscous flux is linear in g) and returns 0.
 template <int D>
 __device__ __forceinline__ double volumeFluxVjpDir(const Phys& ph, const double* u, const double* g, const double* Fb,
-                                                   int dir) {
+                                                   int dir, double* gb) {
   constexpr int NV = D + 2, NG = NV * D;
   if (dir < NV) {
     using DT = Dual<1>;
@@ -126,14 +151,9 @@ __device__ __forceinline__ double volumeFluxVjpDir(const Phys& ph, const double*
     for (int i = 0; i < D * NV; i++) s += Fb[i] * fD[i].d[0];
     return s;
   }
-  double w[D + 3], e[NG], pg[NG], fv[D * NV];
-  computational<D>(ph, u, w);
-  for (int r = 0; r < NG; r++) e[r] = r == dir - NV ? 1.0 : 0.0;
-  primitiveGradient<D>(ph, w, u, e, pg);
-  viscousRawFlux<D>(ph, w, pg, fv);
-  double s = 0.0;
-  for (int i = 0; i < D * NV; i++) s += Fb[i] * fv[i];
-  return -s;
+  viscousGradientVjp<D>(ph, u, Fb, gb);
+  for (int r = 0; r < NG; r++) gb[r] = -gb[r];
+  return 0.0;
 }
 
 // State part of one side of the face-flux VJP (dual numbers seeded on the NV components of side
@@ -235,11 +255,12 @@ __global__ __launch_bounds__(kThreads) void nsAdjVolumeMfma(Geo g, Phys ph, Stag
     tileStore<double>(acc, sFb, m0, lane);
   }
   __syncthreads();
-  // pointwise VJP of the volume flux, one thread per (direction, element, point): u-bar_q into
-  // sUb (the free sRR area) and grad-bar_q into sGb (behind grad u_q in sAJ)
+  // pointwise VJP of the volume flux, one thread per (part, element, point) -- parts 0..NV-1 the
+  // state components, part NV the whole gradient: u-bar_q into sUb (the free sRR area) and
+  // grad-bar_q into sGb (behind grad u_q in sAJ)
   double* sUb = sRR;
   double* sGb = sAJ + NQm * 16;
-  for (int idx = tid; idx < (NV + NG) * EPB * NQ; idx += kThreads) {
+  for (int idx = tid; idx < (NV + 1) * EPB * NQ; idx += kThreads) {
     const int dir = idx / (EPB * NQ), rem = idx % (EPB * NQ), le = rem / NQ, q = rem % NQ, e = e0 + le;
     if (e >= g.E) continue;
     double u[NV], gq[NG], G[D * D], fb[D * NV], Fb[D * NV];
@@ -250,11 +271,13 @@ __global__ __launch_bounds__(kThreads) void nsAdjVolumeMfma(Geo g, Phys ph, Stag
     for (int b = 0; b < D; b++)
       for (int v = 0; v < NV; v++) fb[b * NV + v] = sFb[(q * D + b) * 16 + le * NV + v];
     volumeFluxBar<D>(fb, G, Fb);
-    const double x = volumeFluxVjpDir<D>(ph, u, gq, Fb, dir);
-    if (dir < NV)
+    double gbv[NG];
+    const double x = volumeFluxVjpDir<D>(ph, u, gq, Fb, dir, gbv);
+    if (dir < NV) {
       sUb[q * 16 + le * NV + dir] = x;
-    else
-      sGb[q * 16 + le * NG + dir - NV] = x;
+    } else {
+      for (int r = 0; r < NG; r++) sGb[q * 16 + le * NG + r] = gbv[r];
+    }
   }
   __syncthreads();
   // R-bar_t (volume part) = (Mref^-1 V^T)^T grad-bar / |J| -> HBM

@@ -128,8 +128,9 @@ struct StageArgs {
   // (step, stage + 1), or (step + 1, 0) after the last stage, with step = status[1]; slots of
   // rec_S * state-size doubles per step, rec_nsteps steps
   double* rec;
+  double* rec_t;  // optional: the traces of the updated state, slots of rec_S * trace-size doubles
   int rec_stage, rec_S, rec_nsteps;
-  int64_t rec_n;
+  int64_t rec_n, rec_nt;
   MixRef mix;
 };
 

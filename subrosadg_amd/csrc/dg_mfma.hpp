@@ -230,7 +230,15 @@ __device__ __forceinline__ void mfmaStageTail(const Geo& g, const StageArgs& a, 
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int sl = m0 + Mfma<TC>::row(rr, i);
-      if (col_ok && sl < NAQ) Uf_out[(static_cast<int64_t>(e) * NV + v) * NAQ + sl] = acc[i];
+      if (col_ok && sl < NAQ) {
+        const int64_t ti = (static_cast<int64_t>(e) * NV + v) * NAQ + sl;
+        Uf_out[ti] = acc[i];
+        if (a.rec_t) {
+          const int step = a.status[1] + (a.last_stage ? 1 : 0);
+          const int stage = a.last_stage ? 0 : a.rec_stage + 1;
+          if (step < a.rec_nsteps) a.rec_t[(static_cast<int64_t>(step) * a.rec_S + stage) * a.rec_nt + ti] = acc[i];
+        }
+      }
     }
   }
 }

@@ -253,6 +253,9 @@ struct sdg_ctx {
   // needs no recomputation
   double* adj_pool = nullptr;
   size_t adj_pool_cap = 0;  // in doubles
+  double* adj_tpool = nullptr;  // the stage states' traces, steps x stages x trace size
+  size_t adj_tpool_cap = 0;
+  double* adj_T = nullptr;      // the current step's stage traces (stages x trace size)
   bool adj_stored = false;
   bool recording = false;  // stage launches also write the recording pool (graph capture only)
   int rec_nsteps = 0;
@@ -370,10 +373,12 @@ void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hi
   a.mode = mode;
   if (ctx->recording && mode == 0) {
     a.rec = ctx->adj_pool;
+    a.rec_t = ctx->adj_tpool;
     a.rec_stage = stage;
     a.rec_S = ctx->stages();
     a.rec_nsteps = ctx->rec_nsteps;
     a.rec_n = static_cast<int64_t>(ctx->E) * ctx->nv * ctx->nb;
+    a.rec_nt = static_cast<int64_t>(ctx->E) * ctx->nv * ctx->naq;
   }
   a.Wout = wout;
   if (mode == 1) a.status = nullptr;
@@ -889,6 +894,7 @@ void sdg_destroy(sdg_ctx* ctx) {
   destroyGraphs(ctx);
   for (void* p : ctx->allocations) (void)hipFree(p);
   if (ctx->adj_pool) (void)hipFree(ctx->adj_pool);
+  if (ctx->adj_tpool) (void)hipFree(ctx->adj_tpool);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1159,9 +1165,13 @@ size_t stateCount(const sdg_ctx* ctx) { return static_cast<size_t>(ctx->E) * ctx
 // out = (dR/dU)^T mu at state U (device layout), mu = lam, or Mref^-1 lam / |J| with apply_minv
 // With stage >= 0 the gather kernel applies the transpose of RK stage `stage` to adj_lam in place
 // (accumulating into adj_acc) instead of writing `out`.
-int vjpAt(sdg_ctx* ctx, const double* U, double* lam, int apply_minv, double* out, int stage = -1) {
-  double* Uf = ctx->Uf[ctx->cur ^ 1];  // scratch: the next stage rewrites this trace buffer
-  ctx->launch.trace(ctx->stream, ctx->geo, U, Uf, ctx->mix);
+int vjpAt(sdg_ctx* ctx, const double* U, double* lam, int apply_minv, double* out, int stage = -1,
+          const double* traces = nullptr) {
+  double* Uf = const_cast<double*>(traces);
+  if (!Uf) {
+    Uf = ctx->Uf[ctx->cur ^ 1];  // scratch: the next stage rewrites this trace buffer
+    ctx->launch.trace(ctx->stream, ctx->geo, U, Uf, ctx->mix);
+  }
   StageArgs a = baseArgs(ctx);
   a.U = const_cast<double*>(U);
   a.Uf_in = Uf;
@@ -1289,9 +1299,10 @@ int stepAdjoint(sdg_ctx* ctx, const double* U0) {
 // One adjoint step from the stage states in adj_S[0 .. S-1] (no recomputation).
 int enqueueAdjointStepStored(sdg_ctx* ctx) {
   const int S = ctx->stages();
+  const size_t nt = static_cast<size_t>(ctx->E) * ctx->nv * ctx->naq;
   int rc;
   for (int s = S - 1; s >= 0; s--)
-    if ((rc = vjpAt(ctx, ctx->adj_S[s], ctx->adj_lam, 1, nullptr, s))) return rc;
+    if ((rc = vjpAt(ctx, ctx->adj_S[s], ctx->adj_lam, 1, nullptr, s, ctx->adj_T + s * nt))) return rc;
   HIP_TRY(ctx, hipGetLastError());
   return SDG_OK;
 }
@@ -1391,12 +1402,29 @@ int sdg_adjoint_forward(sdg_ctx* ctx, double dt, int nsteps, int snapshots) {
         (void)hipGetLastError();  // out of device memory: fall back to the checkpointed schedule
       }
     }
-    if (ctx->adj_pool && need <= ctx->adj_pool_cap) {
+    const size_t nt = static_cast<size_t>(ctx->E) * ctx->nv * ctx->naq;
+    const size_t need_t = static_cast<size_t>(nsteps) * ctx->stages() * nt;
+    if (ctx->adj_pool && need_t > ctx->adj_tpool_cap) {
+      if (ctx->adj_tpool) (void)hipFree(ctx->adj_tpool);
+      ctx->adj_tpool = nullptr;
+      ctx->adj_tpool_cap = 0;
+      void* p = nullptr;
+      if (hipMalloc(&p, need_t * sizeof(double)) == hipSuccess) {
+        ctx->adj_tpool = static_cast<double*>(p);
+        ctx->adj_tpool_cap = need_t;
+      } else {
+        (void)hipGetLastError();
+      }
+    }
+    if (ctx->adj_pool && ctx->adj_tpool && !ctx->adj_T && allocate(ctx, 3 * nt, &ctx->adj_T)) return SDG_E_NOMEM;
+    if (ctx->adj_pool && need <= ctx->adj_pool_cap && ctx->adj_tpool && need_t <= ctx->adj_tpool_cap) {
       ctx->adj_dt = dt;
       ctx->adj_nsteps = -1;
       hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
       HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_pool, ctx->U, stateCount(ctx) * sizeof(double), hipMemcpyDeviceToDevice,
-                                  ctx->stream));  // slot (0, 0): the initial state
+                                  ctx->stream));  // slot (0, 0): the initial state and its traces
+      HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_tpool, ctx->traces(ctx->cur), nt * sizeof(double), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
       const int flips = ctx->stages() & 1;
       for (int i = 0; i < nsteps; i++) {
         // the graph bakes in the pool's step count; rebuild it when that changes
@@ -1472,7 +1500,10 @@ int sdg_adjoint_backward(sdg_ctx* ctx, const double* lambda_final, double* lambd
     const int S = ctx->stages();
     const size_t n = stateCount(ctx);
     for (int i = ctx->adj_nsteps - 1; i >= 0; i--) {
+      const size_t nt = static_cast<size_t>(ctx->E) * ctx->nv * ctx->naq;
       HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_S[0], ctx->adj_pool + static_cast<size_t>(i) * S * n, S * n * sizeof(double),
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+      HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_T, ctx->adj_tpool + static_cast<size_t>(i) * S * nt, S * nt * sizeof(double),
                                   hipMemcpyDeviceToDevice, ctx->stream));
       if ((rc = replayGraph(ctx, ctx->adjst_graph, ctx->adjst_graph_dt, ctx->adj_dt, 0,
                             [&]() { return enqueueAdjointStepStored(ctx); })))

@@ -123,21 +123,26 @@ def test_adjoint_gradient_matches_finite_differences():
     del u0
 
 
-@pytest.mark.parametrize("snapshots", [1, 3, 40])
+@pytest.mark.parametrize("snapshots", [1, 3, 39])
 def test_checkpoint_schedule_does_not_change_the_adjoint(snapshots):
-    """Recomputation from binomial checkpoints is deterministic: every snapshot budget gives the
-    same adjoint as storing every step."""
+    """Recomputation from binomial checkpoints is deterministic: every snapshot budget below the
+    step count gives bit-identical adjoints, and they agree to rounding with the recorded run that
+    keeps every stage state (which reuses the forward stages' own traces instead of recomputing
+    them)."""
     steps = 40
     mesh, prob, prim, s, dt = _shear_rollout_setup(8, 3, steps)
     u0 = s.get_state()
     s.adjoint_forward(dt, steps, steps)
     _, dJ = P.kinetic_energy(mesh, s.get_state())
+    stored = s.adjoint_backward(dJ)
+    assert np.array_equal(s.get_state(), u0)  # the context is left at the recorded initial state
+    s.adjoint_forward(dt, steps, 2)
     ref = s.adjoint_backward(dJ)
     s.set_state(u0)
     s.adjoint_forward(dt, steps, snapshots)
     got = s.adjoint_backward(dJ)
     assert np.array_equal(got, ref)
-    # the context is left at the recorded initial state
+    assert np.abs(stored - ref).max() <= 1e-12 * np.abs(ref).max()
     assert np.array_equal(s.get_state(), u0)
 
 
