@@ -62,6 +62,8 @@ enum sdg_time_integration { SDG_RK_FORWARD_EULER = 0, SDG_RK_HEUN2 = 1, SDG_RK_S
 enum sdg_eos { SDG_EOS_IDEAL_GAS = 0, SDG_EOS_WEAK_COMPRESSIBLE = 1 };
 enum sdg_transport { SDG_TRANSPORT_NONE = 0, SDG_TRANSPORT_CONSTANT = 1, SDG_TRANSPORT_SUTHERLAND = 2 };
 enum sdg_precision { SDG_PREC_FP64 = 0, SDG_PREC_MIXED = 1 };
+/* InitialConditionEnum (Enum.cpp), same order: the file-based starts of sdg_raw_binary_read */
+enum sdg_initial_condition { SDG_IC_FUNCTION = 0, SDG_IC_SPECIFIC_FILE = 1, SDG_IC_LAST_STEP = 2 };
 /* BoundaryConditionEnum (Enum.cpp:92-100), same order */
 enum sdg_boundary {
   SDG_BC_RIEMANN_FARFIELD = 0,
@@ -278,6 +280,27 @@ int sdg_synchronize(sdg_ctx* ctx);
 #define SDG_KERNEL_CLASSES 3
 int sdg_time_kernels(sdg_ctx* ctx, double dt, int nsteps, double* avg_ms, int n_out);
 
+/* ---- raw state files (checkpoint / restart) ----------------------------------------------------
+ * The reference's raw format (RawBinary.cpp:42-191): an 8-byte header holding
+ * ZSTD_compressBound(payload bytes), then one zstd frame (level 1) of the payload:
+ *   per element, in mesh order: the NV x N_b coefficients (column-major, as sdg_get_state), and
+ *     for Navier-Stokes the (NV D) x N_b lifted-gradient coefficients (as sdg_get_gradient);
+ *   per boundary face: the same blocks of its parent element (RawBinary.cpp:95-123);
+ *   node_count doubles of nodal artificial viscosity (zero: no shock capturing).
+ * Replaces Solver::writeRawBinary (RawBinary.cpp:165-191) and RawBinaryCompress::write (:42-55).
+ * Navier-Stokes BR2 meshes with boundary faces are refused (SDG_E_UNSUPPORTED): their per-face
+ * gradient entries (volume part plus that face's lift) are not formed by the kernels. */
+int sdg_raw_binary_write(sdg_ctx* ctx, const char* path, int64_t node_count);
+/* Start from a raw file (InitialCondition.cpp:41-83): SDG_IC_LAST_STEP reads a file of this
+ * discretisation; SDG_IC_SPECIFIC_FILE reads a file of order P-1 and sets the higher modes to zero
+ * (the hierarchic basis of order P-1 is the first N_b(P-1) functions of order P). Gradient blocks
+ * and the trailing data are skipped. */
+int sdg_raw_binary_read(sdg_ctx* ctx, const char* path, int initial_condition);
+/* The container alone (no device needed): compress `bytes` bytes into a raw file, and read one
+ * back (payload NULL: only *bytes is set). */
+int sdg_raw_file_write(const char* path, const void* payload, int64_t bytes);
+int sdg_raw_file_read(const char* path, void* payload, int64_t capacity, int64_t* bytes);
+
 /* ---- discrete adjoint (Navier-Stokes, FP64, matrix-core element sizes) -----------------------
  * The reference has no adjoint; these entry points transpose its residual (R of sdg_residual,
  * SpatialDiscrete.cpp:194-1116) and its RK stage update (TimeIntegration.cpp:181-198).
@@ -295,8 +318,9 @@ int sdg_vjp_residual(sdg_ctx* ctx, const double* lambda, double* out);
 int sdg_adjoint_forward(sdg_ctx* ctx, double dt, int nsteps, int snapshots);
 
 /* Reverse sweep of the last sdg_adjoint_forward: given lambda_final = dJ/dU(t_N), writes
- * lambda_initial = dJ/dU(t_0). Recomputes stage states from the checkpoints; leaves the context
- * state at the recorded initial state. */
+ * lambda_initial = dJ/dU(t_0). With every step recorded, the stage states (and their traces) of
+ * the recorded run are used directly; otherwise they are recomputed from the checkpoints. Leaves
+ * the context state at the recorded initial state. */
 int sdg_adjoint_backward(sdg_ctx* ctx, const double* lambda_final, double* lambda_initial);
 
 /* ---- batched solver (no counterpart in the reference: it runs one problem per process) -------

@@ -19,6 +19,7 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <filesystem>
 #include <functional>
 #include <iostream>
 #include <stdexcept>
@@ -140,8 +141,6 @@ struct System {
                 "the GPU engine implements the compressible Euler and Navier-Stokes models");
   static_assert(SC::kShockCapturing == ShockCapturingEnum::None, "artificial viscosity is not implemented");
   static_assert(SC::kSourceTerm == SourceTermEnum::None, "source terms are not implemented");
-  static_assert(SC::kInitialCondition == InitialConditionEnum::Function,
-                "initial conditions come from InitialCondition<SC>::calculatePrimitiveFromCoordinate");
 
   InitialCondition<SC> initial_condition_;
   BoundaryCondition<SC> boundary_condition_;
@@ -154,6 +153,7 @@ struct System {
   int iteration_start_ = 0, iteration_end_ = 0;
   Real cfl_ = 0.0, delta_time_ = 0.0;
   int io_interval_ = 0;
+  std::string output_directory_, output_prefix_, initial_condition_file_;
   std::vector<std::array<Real, SC::kConservedVariableNumber>> relative_error_history_;
   std::function<void(int, const std::array<Real, SC::kConservedVariableNumber>&)> on_step_;
 
@@ -221,9 +221,29 @@ struct System {
   void setPrecision(const PrecisionEnum precision) {
     problem_.precision = precision == PrecisionEnum::Mixed ? SDG_PREC_MIXED : SDG_PREC_FP64;
   }
-  // SystemControl.cpp:125-140: only the interval is kept (no files are written)
-  void setViewConfig(const std::string& /*output_directory*/, const std::string& /*prefix*/, const int io_interval = -1) {
+  // SystemControl.cpp:124-140: raw state files <output_directory>/raw/<prefix>_<step>.zst every
+  // io_interval steps (and at the start unless restarting); -1 = only at the end. No prompt.
+  void setViewConfig(const std::string& output_directory, const std::string& prefix, const int io_interval = -1) {
+    output_directory_ = output_directory;
+    output_prefix_ = prefix;
     io_interval_ = io_interval;
+  }
+  // SystemControl.cpp:69-74: the order-(P-1) raw file of InitialConditionEnum::SpecificFile
+  template <InitialConditionEnum InitialConditionType>
+  void addInitialCondition(const std::string& initial_condition_file) {
+    static_assert(InitialConditionType == InitialConditionEnum::SpecificFile, "only SpecificFile takes a file");
+    initial_condition_file_ = initial_condition_file;
+  }
+  std::string rawPath(const int step) const {
+    return output_directory_ + "/raw/" + output_prefix_ + "_" + std::to_string(step) + ".zst";
+  }
+  // RawBinary.cpp:165-191 via the library; node count of the structured box's vertex grid
+  void writeRaw(const int step) const {
+    if (output_directory_.empty()) return;
+    std::filesystem::create_directories(output_directory_ + "/raw");
+    int64_t nodes = 1;
+    for (int d = 0; d < SC::kDimension; d++) nodes *= mesh_desc_.cells[d] + 1;
+    check(sdg_raw_binary_write(ctx_, rawPath(step).c_str(), nodes), ctx_);
   }
   // Called after every step with the step number and the residual monitor.
   void setStepCallback(std::function<void(int, const std::array<Real, SC::kConservedVariableNumber>&)> f) {
@@ -291,16 +311,25 @@ struct System {
     const int64_t nq = info.quadrature;
     std::vector<double> xq(static_cast<size_t>(sdg_mesh_field_size(mesh_, SDG_FIELD_QUAD_COORD)));
     check(sdg_mesh_export(mesh_, SDG_FIELD_QUAD_COORD, xq.data(), static_cast<int64_t>(xq.size())), nullptr);
-    std::vector<double> prim(static_cast<size_t>(info.elements * nq * NV));
-    for (int64_t p = 0; p < info.elements * nq; p++) {
-      std::array<Real, D> x;
-      for (int d = 0; d < D; d++) x[d] = xq[p * D + d];
-      const auto v = initial_condition_.calculatePrimitiveFromCoordinate(x);
-      for (int k = 0; k < NV; k++) prim[p * NV + k] = v[k];
+    if constexpr (SC::kInitialCondition == InitialConditionEnum::Function) {
+      std::vector<double> prim(static_cast<size_t>(info.elements * nq * NV));
+      for (int64_t p = 0; p < info.elements * nq; p++) {
+        std::array<Real, D> x;
+        for (int d = 0; d < D; d++) x[d] = xq[p * D + d];
+        const auto v = initial_condition_.calculatePrimitiveFromCoordinate(x);
+        for (int k = 0; k < NV; k++) prim[p * NV + k] = v[k];
+      }
+      check(sdg_set_initial_primitive(ctx_, prim.data()), ctx_);
+    } else if constexpr (SC::kInitialCondition == InitialConditionEnum::SpecificFile) {
+      check(sdg_raw_binary_read(ctx_, initial_condition_file_.c_str(), SDG_IC_SPECIFIC_FILE), ctx_);
+    } else {  // LastStep: the raw file of the first iteration (SystemControl.cpp:150-154)
+      check(sdg_raw_binary_read(ctx_, rawPath(iteration_start_).c_str(), SDG_IC_LAST_STEP), ctx_);
     }
-    check(sdg_set_initial_primitive(ctx_, prim.data()), ctx_);
-    updateBoundary(0.0);
+    updateBoundary(iteration_start_ * delta_time_);
+    // (a restart does not read the reference's error file for dt: set it with setDeltaTime)
     if (delta_time_ == 0.0) check(sdg_compute_dt(ctx_, cfl_, &delta_time_), ctx_);
+    if (SC::kInitialCondition != InitialConditionEnum::LastStep) writeRaw(0);
+    const int interval = io_interval_ > 0 ? io_interval_ : iteration_end_;
     for (int i = iteration_start_ + 1; i <= iteration_end_; i++) {
       if (SC::kBoundaryTime == BoundaryTimeEnum::TimeVarying) updateBoundary(i * delta_time_);
       std::array<Real, NV> rel{};
@@ -312,6 +341,7 @@ struct System {
         iteration_end_ = i;
         break;
       }
+      if (interval > 0 && i % interval == 0) writeRaw(i);
     }
   }
 

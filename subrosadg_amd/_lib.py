@@ -29,6 +29,7 @@ RK_FORWARD_EULER, RK_HEUN2, RK_SSPRK3 = 0, 1, 2
 EOS_IDEAL_GAS, EOS_WEAK_COMPRESSIBLE = 0, 1
 TRANSPORT_NONE, TRANSPORT_CONSTANT, TRANSPORT_SUTHERLAND = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
+IC_FUNCTION, IC_SPECIFIC_FILE, IC_LAST_STEP = 0, 1, 2
 (BC_RIEMANN_FARFIELD, BC_VELOCITY_INFLOW, BC_PRESSURE_OUTFLOW, BC_ISOTHERMAL_NONSLIP_WALL, BC_ADIABATIC_SLIP_WALL,
  BC_ADIABATIC_NONSLIP_WALL, BC_PERIODIC) = range(7)
 
@@ -156,6 +157,10 @@ SIGNATURES = {
     "sdg_batch_get_config": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(c_int)]),
     "sdg_adjoint_forward": (c_int, [c_void_p, c_double, c_int, c_int]),
     "sdg_adjoint_backward": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "sdg_raw_binary_write": (c_int, [c_void_p, c_char_p, c_int64]),
+    "sdg_raw_binary_read": (c_int, [c_void_p, c_char_p, c_int]),
+    "sdg_raw_file_write": (c_int, [c_char_p, c_void_p, c_int64]),
+    "sdg_raw_file_read": (c_int, [c_char_p, c_void_p, c_int64, POINTER(c_int64)]),
 }
 
 _lib = None
@@ -357,6 +362,14 @@ class Solver:
         self._check(self.lib.sdg_adjoint_backward(self.handle, lam.ctypes.data, out.ctypes.data))
         return out
 
+    def write_raw(self, path: str, node_count: int = 0):
+        """Raw state file in the reference's format (header + zstd frame; RawBinary.cpp:42-191)."""
+        self._check(self.lib.sdg_raw_binary_write(self.handle, os.fsencode(path), node_count))
+
+    def read_raw(self, path: str, initial_condition: int = IC_LAST_STEP):
+        """Start from a raw file: IC_LAST_STEP (same discretisation) or IC_SPECIFIC_FILE (order P-1)."""
+        self._check(self.lib.sdg_raw_binary_read(self.handle, os.fsencode(path), initial_condition))
+
     def compute_dt(self, cfl: float) -> float:
         dt = c_double()
         self._check(self.lib.sdg_compute_dt(self.handle, cfl, ctypes.byref(dt)))
@@ -465,3 +478,20 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+
+def raw_file_write(path: str, payload: bytes):
+    """Compress `payload` into a raw file (no device needed)."""
+    lib = load()
+    buf = ctypes.create_string_buffer(payload, len(payload))
+    check(lib.sdg_raw_file_write(os.fsencode(path), ctypes.cast(buf, c_void_p), len(payload)))
+
+
+def raw_file_read(path: str) -> bytes:
+    """The decompressed payload of a raw file (no device needed)."""
+    lib = load()
+    n = c_int64()
+    check(lib.sdg_raw_file_read(os.fsencode(path), None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(max(n.value, 1))
+    check(lib.sdg_raw_file_read(os.fsencode(path), ctypes.cast(buf, c_void_p), n.value, ctypes.byref(n)))
+    return buf.raw[: n.value]

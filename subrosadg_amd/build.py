@@ -15,9 +15,11 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsdg_hip.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-HOST_SOURCES = ["kit.cpp", "sdg_mesh_api.cpp"]
+HOST_SOURCES = ["kit.cpp", "sdg_mesh_api.cpp", "sdg_raw.cpp"]
+# the system zstd runtime (raw state files); only its stable C API is used
+LINK_LIBS = ["-l:libzstd.so.1"]
 HIP_SOURCES = ["sdg_solver.hip", "sdg_batch.hip"]
-EXAMPLES = ["taylor_green_3d_cns", "lid_driven_cavity_2d_cns"]
+EXAMPLES = ["taylor_green_3d_cns", "lid_driven_cavity_2d_cns", "shear_layer_2d_cns_restart"]
 ARCH = os.environ.get("SDG_OFFLOAD_ARCH", "gfx950")
 
 # Packed-FP32 VALU instructions are disabled for all device code; hipcc's host pass warns that it
@@ -56,7 +58,7 @@ def build(verbose: bool = True) -> str:
         _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)
     tmp = OUT + ".tmp"
-    _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs], verbose)
+    _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs, *LINK_LIBS], verbose)
     os.replace(tmp, OUT)
     # C++ drop-in driver example (include/subrosadg_amd/SubrosaDG.hpp) linked against the library
     root = os.path.dirname(HERE)

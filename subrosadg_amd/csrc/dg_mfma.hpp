@@ -75,6 +75,12 @@ __device__ __forceinline__ void zeroLds(TC* p, int n, int tid) {
   for (int i = tid; i < n; i += kThreads) p[i] = TC(0);
 }
 
+// FP32 deviation of a coefficient from the reference state's (one expression for every producer
+// of contraction operands, so traces computed after a stage and from a stored state agree bitwise)
+__device__ __forceinline__ float deviationOperand(double u, double uref, double one) {
+  return static_cast<float>(fma(-uref, one, u));
+}
+
 // Coefficient b of variable v of element e as a contraction operand: the FP64 state itself, or in
 // the mixed-precision mode its deviation from the reference state, rounded to FP32.
 template <typename TC>
@@ -83,7 +89,7 @@ __device__ __forceinline__ TC stateOperand(const StageArgs& a, int64_t e, int v,
   if constexpr (sizeof(TC) == 8)
     return u;
   else
-    return static_cast<float>(u - a.mix.uref[v] * a.mix.one[b]);
+    return deviationOperand(u, a.mix.uref[v], a.mix.one[b]);
 }
 
 // Conserved state at a point from a contraction result (FP64 in both modes).
@@ -211,7 +217,7 @@ __device__ __forceinline__ void mfmaStageTail(const Geo& g, const StageArgs& a, 
           TC x = TC(0);
           if (col_ok && bi < NB) {
             if constexpr (MIX)
-              x = static_cast<float>(un[i] - a.mix.uref[v] * a.mix.one[bi]);
+              x = deviationOperand(un[i], a.mix.uref[v], a.mix.one[bi]);
             else
               x = un[i];
           }
@@ -706,6 +712,46 @@ __global__ __launch_bounds__(kThreads) void nsResidualMfma(Geo g, Phys ph, Stage
     return;
   }
   mfmaStageTail<TC, D, NB, NQ, NF, NFQ, EPB>(g, a, e0, sR, sU, sRed);
+}
+
+// Traces at the element's own face points from the state in HBM, with the same operands and
+// matrix-core accumulation order as the stage tail that produces them after every stage, so a state
+// set from the host (a restart, a checkpoint) continues bit-identically. EPB elements per block.
+template <typename TC, int D, int NB, int NF, int NFQ>
+__global__ __launch_bounds__(kThreads) void mfmaTraceKernel(Geo g, const double* U, void* Uf_out, MixRef mix) {
+  constexpr int NV = D + 2, NAQ = NF * NFQ, EPB = 16 / NV;
+  constexpr int NBk = r4(NB), NAQm = r16(NAQ);
+  __shared__ TC sU[NBk * 16];
+  const MOpsT<TC>& op = opsOf<TC>(g);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c = lane & 15, rr = lane >> 4;
+  const int e0 = blockIdx.x * EPB;
+  for (int idx = tid; idx < NBk * 16; idx += kThreads) {
+    const int b = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;
+    TC x = TC(0);
+    if (b < NB && cc < EPB * NV && e < g.E) {
+      const double u = U[(static_cast<int64_t>(e) * NV + v) * NB + b];
+      if constexpr (sizeof(TC) == 8)
+        x = u;
+      else
+        x = deviationOperand(u, mix.uref[v], mix.one[b]);
+    }
+    sU[idx] = x;
+  }
+  __syncthreads();
+  const int le = c / NV, v = c % NV, e = e0 + le;
+  const bool col_ok = c < EPB * NV && e < g.E;
+  TC* out = static_cast<TC*>(Uf_out);
+#pragma unroll 1
+  for (int m0 = wave * 16; m0 < NAQm; m0 += 64) {
+    acc_t<TC> acc = zeroAcc<TC>();
+    tileAcc<TC, NAQm>(acc, op.trace, m0, sU, 0, NBk, lane);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int sl = m0 + Mfma<TC>::row(rr, i);
+      if (col_ok && sl < NAQ) out[(static_cast<int64_t>(e) * NV + v) * NAQ + sl] = acc[i];
+    }
+  }
 }
 
 }  // namespace sdg

@@ -2,7 +2,9 @@
 #ifndef SDG_INTERNAL_HPP_
 #define SDG_INTERNAL_HPP_
 
+#include <cstddef>
 #include <string>
+#include <vector>
 
 struct sdg_mesh;
 
@@ -15,6 +17,9 @@ const char* globalError();
 const Mesh* meshOf(const sdg_mesh* m);
 // the triangle/quadrangle description of a split-grid mesh (nullptr for other meshes)
 const MixedMesh* mixedOf(const sdg_mesh* m);
+// zstd raw files in the reference's layout (sdg_raw.cpp): header + one frame of the payload
+int writeRawFile(const char* path, const void* data, size_t bytes);
+int readRawFile(const char* path, std::vector<char>& out);
 }  // namespace sdg
 
 #endif  // SDG_INTERNAL_HPP_

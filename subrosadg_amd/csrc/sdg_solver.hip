@@ -128,11 +128,19 @@ Launchers makeLaunchers() {
                          0, st, g, ph, a);
     };
   }
-  l.trace = [](hipStream_t st, const Geo& g, const double* U, void* Uf, const sdg::MixRef& mix) {
-    const int64_t n = static_cast<int64_t>(g.E) * (D + 2) * NF * NFQ;
-    hipLaunchKernelGGL((sdg::traceKernel<D, NB, NQ, NF, NFQ>), dim3((n + sdg::kThreads - 1) / sdg::kThreads),
-                       dim3(sdg::kThreads), 0, st, g, U, Uf, mix);
-  };
+  if constexpr (MFMA) {
+    // the stage tail's own contraction (bit-identical traces after a host-side state change)
+    l.trace = [](hipStream_t st, const Geo& g, const double* U, void* Uf, const sdg::MixRef& mix) {
+      hipLaunchKernelGGL((sdg::mfmaTraceKernel<TC, D, NB, NF, NFQ>), dim3(blocksFor(g.E, 16 / (D + 2))),
+                         dim3(sdg::kThreads), 0, st, g, U, Uf, mix);
+    };
+  } else {
+    l.trace = [](hipStream_t st, const Geo& g, const double* U, void* Uf, const sdg::MixRef& mix) {
+      const int64_t n = static_cast<int64_t>(g.E) * (D + 2) * NF * NFQ;
+      hipLaunchKernelGGL((sdg::traceKernel<D, NB, NQ, NF, NFQ>), dim3((n + sdg::kThreads - 1) / sdg::kThreads),
+                         dim3(sdg::kThreads), 0, st, g, U, Uf, mix);
+    };
+  }
   l.dt = [](hipStream_t st, const Geo& g, const Phys& ph, const double* U, const double* me, double cfl, int order,
             double* out) {
     hipLaunchKernelGGL((sdg::dtKernel<D, NB, NQ, NF, NFQ>), dim3((g.E + sdg::kThreads - 1) / sdg::kThreads),
@@ -219,6 +227,7 @@ struct sdg_ctx {
   double* minedge = nullptr;
   double* lsq = nullptr;  // host copy
   std::vector<double> h_lsq, h_V, h_chol;
+  std::vector<int> h_bparent;  // parent element of each boundary face
   double *U = nullptr, *Un = nullptr, *Uf[2] = {nullptr, nullptr}, *Gf = nullptr, *Rvol = nullptr, *Hs = nullptr, *Fv = nullptr;
   double *Rout = nullptr, *Wout = nullptr, *mon = nullptr, *rel = nullptr, *dtbuf = nullptr;
   int* status = nullptr;
@@ -684,6 +693,7 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
       btype[bp] = fc.bc;
       bslot[bp] = fc.parent[0] * naq + fc.local[0] * nfq + j;
     }
+    ctx->h_bparent.push_back(fc.parent[0]);
   }
   ctx->Fb = Fb;
   Geo& g = ctx->geo;
@@ -1132,6 +1142,65 @@ int sdg_time_kernels(sdg_ctx* ctx, double dt, int nsteps, double* avg_ms, int n_
   for (int k = 0; k < n_out; k++) avg_ms[k] = k < SDG_KERNEL_CLASSES ? sum[k] / launches[k] : 0.0;
   if (!ctx->ns() && n_out > 1) avg_ms[1] = 0.0;
   return SDG_OK;
+}
+
+}  // extern "C"
+
+// ---- raw state files ------------------------------------------------------------------------------
+extern "C" {
+
+int sdg_raw_binary_write(sdg_ctx* ctx, const char* path, int64_t node_count) {
+  if (!ctx || !path || node_count < 0) return fail(ctx, SDG_E_INVALID, "sdg_raw_binary_write: bad argument");
+  if (!ctx->has_state) return fail(ctx, SDG_E_STATE, "sdg_raw_binary_write: no state set");
+  const bool ns = ctx->ns();
+  if (ns && ctx->prob.viscous_flux == SDG_VISC_BR2 && ctx->Fb > 0)
+    return fail(ctx, SDG_E_UNSUPPORTED,
+                "sdg_raw_binary_write: the per-face BR2 gradients of boundary faces are not formed by the kernels");
+  const size_t E = static_cast<size_t>(ctx->E), nu = static_cast<size_t>(ctx->nv) * ctx->nb,
+               ng = ns ? static_cast<size_t>(ctx->ng) * ctx->nb : 0;
+  std::vector<double> U(E * nu), G(E * ng);
+  int rc = sdg_get_state(ctx, U.data());
+  if (rc) return rc;
+  if (ns && (rc = sdg_get_gradient(ctx, G.data()))) return rc;
+  std::vector<double> payload;
+  payload.reserve((E + ctx->h_bparent.size()) * (nu + ng) + static_cast<size_t>(node_count));
+  auto block = [&](size_t e) {
+    payload.insert(payload.end(), U.begin() + static_cast<std::ptrdiff_t>(e * nu),
+                   U.begin() + static_cast<std::ptrdiff_t>((e + 1) * nu));
+    if (ns)
+      payload.insert(payload.end(), G.begin() + static_cast<std::ptrdiff_t>(e * ng),
+                     G.begin() + static_cast<std::ptrdiff_t>((e + 1) * ng));
+  };
+  for (size_t e = 0; e < E; e++) block(e);                   // RawBinary.cpp:76-90
+  for (int p : ctx->h_bparent) block(static_cast<size_t>(p));  // RawBinary.cpp:95-123, BR1 / Euler
+  payload.resize(payload.size() + static_cast<size_t>(node_count), 0.0);  // node artificial viscosity
+  rc = sdg::writeRawFile(path, payload.data(), payload.size() * sizeof(double));
+  if (rc) return fail(ctx, rc, sdg::globalError());
+  return SDG_OK;
+}
+
+int sdg_raw_binary_read(sdg_ctx* ctx, const char* path, int initial_condition) {
+  if (!ctx || !path) return fail(ctx, SDG_E_INVALID, "sdg_raw_binary_read: null argument");
+  if (initial_condition != SDG_IC_LAST_STEP && initial_condition != SDG_IC_SPECIFIC_FILE)
+    return fail(ctx, SDG_E_INVALID, "sdg_raw_binary_read: initial_condition must be LAST_STEP or SPECIFIC_FILE");
+  int nb_src = ctx->nb;
+  if (initial_condition == SDG_IC_SPECIFIC_FILE) {
+    const int q = ctx->order;  // basis size of order P - 1
+    if (q < 1) return fail(ctx, SDG_E_INVALID, "sdg_raw_binary_read: SPECIFIC_FILE needs P >= 2");
+    nb_src = ctx->type == SDG_ELEM_TRIANGLE ? q * (q + 1) / 2 : (ctx->dim == 2 ? q * q : q * q * q);
+  }
+  std::vector<char> raw;
+  int rc = sdg::readRawFile(path, raw);
+  if (rc) return fail(ctx, rc, sdg::globalError());
+  const size_t E = static_cast<size_t>(ctx->E), nv = static_cast<size_t>(ctx->nv);
+  const size_t blk = nv * nb_src + (ctx->ns() ? static_cast<size_t>(ctx->ng) * nb_src : 0);
+  if (raw.size() < E * blk * sizeof(double))
+    return fail(ctx, SDG_E_INVALID, "sdg_raw_binary_read: the file holds fewer element blocks than the mesh has elements");
+  const double* src = reinterpret_cast<const double*>(raw.data());
+  std::vector<double> coeff(E * nv * ctx->nb, 0.0);
+  for (size_t e = 0; e < E; e++)
+    std::copy(src + e * blk, src + e * blk + nv * nb_src, coeff.begin() + static_cast<std::ptrdiff_t>(e * nv * ctx->nb));
+  return sdg_set_state(ctx, coeff.data());
 }
 
 }  // extern "C"
