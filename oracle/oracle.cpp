@@ -22,6 +22,13 @@
 //   parity unpinned                -> the Exact (weakly-compressible) flux: only consistency
 //                                     and antisymmetry are checked; no independent solution
 //                                     of its Riemann problem is committed.
+//
+// The arithmetic type is ORACLE_REAL (default double). The same source is also built with
+// ORACLE_REAL=long double (x87 extended, 64-bit mantissa) into liboracle_ld.so: that build takes
+// the same FP64 operand tables and states and evaluates the same passes with 11 more mantissa
+// bits, which measures the FP64 rounding floor of a case -- how far the FP64 oracle itself is
+// from the exact-arithmetic value of the same discrete operator. Its interface takes and returns
+// arrays of ORACLE_REAL.
 #include <omp.h>
 
 #include <algorithm>
@@ -34,61 +41,67 @@
 
 #include "../include/sdg.h"
 
+#ifndef ORACLE_REAL
+#define ORACLE_REAL double
+#endif
+
 namespace {
 
-constexpr double kGamma = 1.4;        // PhysicalModel.cpp:45
-constexpr double kPrandtl = 0.71;     // PhysicalModel.cpp:91,107
-constexpr double kSutherland = 110.4 / 273.15;  // PhysicalModel.cpp:109
+using Real = ORACLE_REAL;
+
+constexpr Real kGamma = 1.4;        // PhysicalModel.cpp:45
+constexpr Real kPrandtl = 0.71;     // PhysicalModel.cpp:91,107
+constexpr Real kSutherland = 110.4 / 273.15;  // PhysicalModel.cpp:109
 
 struct Physics {
   int eos = SDG_EOS_IDEAL_GAS;
   int transport = SDG_TRANSPORT_NONE;
-  double cp = 0, cv = 0, mu = 0, kappa = 0, c0 = 0, rho0 = 0, padd = 0;
+  Real cp = 0, cv = 0, mu = 0, kappa = 0, c0 = 0, rho0 = 0, padd = 0;
 
   // ThermodynamicModel<Constant> (PhysicalModel.cpp:24-38)
-  double internalEnergyFromTemperature(double t) const { return cv * t; }
-  double temperatureFromInternalEnergy(double e) const { return e / cv; }
+  Real internalEnergyFromTemperature(Real t) const { return cv * t; }
+  Real temperatureFromInternalEnergy(Real e) const { return e / cv; }
   // EquationOfState (PhysicalModel.cpp:40-75)
-  double pressure(double rho, double e) const {
+  Real pressure(Real rho, Real e) const {
     if (eos == SDG_EOS_IDEAL_GAS) return (kGamma - 1.0) * rho * e;
     return c0 * c0 * (rho - rho0) + padd;
   }
-  double soundSpeed(double rho, double p) const {
+  Real soundSpeed(Real rho, Real p) const {
     if (eos == SDG_EOS_IDEAL_GAS) return std::sqrt(kGamma * p / rho);
     return c0;
   }
   // TransportModel (PhysicalModel.cpp:77-122)
-  double sutherlandRatio(double t) const {
+  Real sutherlandRatio(Real t) const {
     return std::sqrt(t * t * t) * (1.0 + kSutherland) / (t + kSutherland);
   }
-  double viscosity(double t) const { return transport == SDG_TRANSPORT_SUTHERLAND ? mu * sutherlandRatio(t) : mu; }
-  double conductivity(double t) const {
+  Real viscosity(Real t) const { return transport == SDG_TRANSPORT_SUTHERLAND ? mu * sutherlandRatio(t) : mu; }
+  Real conductivity(Real t) const {
     return transport == SDG_TRANSPORT_SUTHERLAND ? kappa * sutherlandRatio(t) : kappa;
   }
 };
 
 // column-major helpers: C (m x n) (+)= A (m x k) * B (k x n)
-inline void gemm(double* c, const double* a, const double* b, int m, int k, int n, bool accumulate, double alpha = 1.0) {
+inline void gemm(Real* c, const Real* a, const Real* b, int m, int k, int n, bool accumulate, Real alpha = 1.0) {
   for (int j = 0; j < n; j++) {
-    double* cj = c + static_cast<size_t>(j) * m;
+    Real* cj = c + static_cast<size_t>(j) * m;
     if (!accumulate)
       for (int i = 0; i < m; i++) cj[i] = 0.0;
     for (int l = 0; l < k; l++) {
-      const double blj = alpha * b[l + static_cast<size_t>(j) * k];
-      const double* al = a + static_cast<size_t>(l) * m;
+      const Real blj = alpha * b[l + static_cast<size_t>(j) * k];
+      const Real* al = a + static_cast<size_t>(l) * m;
       for (int i = 0; i < m; i++) cj[i] += al[i] * blj;
     }
   }
 }
 // C (m x n) (+)= A (m x k) * B^T where B is (n x k)
-inline void gemmBt(double* c, const double* a, const double* b, int m, int k, int n, bool accumulate) {
+inline void gemmBt(Real* c, const Real* a, const Real* b, int m, int k, int n, bool accumulate) {
   for (int j = 0; j < n; j++) {
-    double* cj = c + static_cast<size_t>(j) * m;
+    Real* cj = c + static_cast<size_t>(j) * m;
     if (!accumulate)
       for (int i = 0; i < m; i++) cj[i] = 0.0;
     for (int l = 0; l < k; l++) {
-      const double blj = b[j + static_cast<size_t>(l) * n];
-      const double* al = a + static_cast<size_t>(l) * m;
+      const Real blj = b[j + static_cast<size_t>(l) * n];
+      const Real* al = a + static_cast<size_t>(l) * m;
       for (int i = 0; i < m; i++) cj[i] += al[i] * blj;
     }
   }
@@ -101,25 +114,25 @@ struct Oracle {
   int equation = 0, flux = 0, viscous = 0, rk = 0;
   Physics phys;
   // operands (column-major like Eigen)
-  std::vector<double> V;    // N_q x N_b
-  std::vector<double> dV;   // (N_q D) x N_b
-  std::vector<double> Vf;   // N_aq x N_b
-  std::vector<double> lsq;  // N_b x N_b
-  std::vector<double> detjw, tiw, minv, min_edge;
+  std::vector<Real> V;    // N_q x N_b
+  std::vector<Real> dV;   // (N_q D) x N_b
+  std::vector<Real> Vf;   // N_aq x N_b
+  std::vector<Real> lsq;  // N_b x N_b
+  std::vector<Real> detjw, tiw, minv, min_edge;
   std::vector<int32_t> fparent, flocal, fseq;
-  std::vector<double> fnormal, fjw;
+  std::vector<Real> fnormal, fjw;
   // per-element solver data (SolveControl.cpp:45-174), flattened element-major
-  std::vector<double> coeff, coeff_last, vol_quad, adj_quad, residual;
-  std::vector<double> vg_coeff, vg_quad, vg_adj, vg_res;           // volume gradient
-  std::vector<double> ig_adj, ig_res, ig_coeff, g_coeff;           // interface / total gradient (NS)
+  std::vector<Real> coeff, coeff_last, vol_quad, adj_quad, residual;
+  std::vector<Real> vg_coeff, vg_quad, vg_adj, vg_res;           // volume gradient
+  std::vector<Real> ig_adj, ig_res, ig_coeff, g_coeff;           // interface / total gradient (NS)
   int nig = 0;  // interface-gradient residual blocks per element: 1 (BR1) or N_f (BR2)
-  std::vector<double> rel_err;
+  std::vector<Real> rel_err;
   bool has_state = false;
   // boundary faces (the element is the left parent): element, local face, type; per point the
   // outward normal, |J_f| w and the boundary data (boundary_dummy_variable_, computational)
   int64_t nbface = 0;
   std::vector<int32_t> bparent, blocal, btype;
-  std::vector<double> bnormal, bjw, bdummy;
+  std::vector<Real> bnormal, bjw, bdummy;
 
   size_t szC() const { return static_cast<size_t>(nv) * nb; }
   size_t szG() const { return static_cast<size_t>(nv) * dim * nb; }
@@ -127,58 +140,58 @@ struct Oracle {
 
   // --- variable conversions (VariableConvertor.cpp:315-339, 574-620) ---
   // computational: (rho, u.., e, p)
-  void computational(const double* u, double* w) const {
-    const double rho = u[0];
+  void computational(const Real* u, Real* w) const {
+    const Real rho = u[0];
     w[0] = rho;
-    double v2 = 0.0;
+    Real v2 = 0.0;
     for (int d = 0; d < dim; d++) {
       w[1 + d] = u[1 + d] / rho;
       v2 += w[1 + d] * w[1 + d];
     }
-    const double e = u[dim + 1] / rho - v2 / 2.0;
+    const Real e = u[dim + 1] / rho - v2 / 2.0;
     w[dim + 1] = e;
     w[dim + 2] = phys.pressure(rho, e);
   }
-  double velocitySquared(const double* w) const {
-    double s = 0.0;
+  Real velocitySquared(const Real* w) const {
+    Real s = 0.0;
     for (int d = 0; d < dim; d++) s += w[1 + d] * w[1 + d];
     return s;
   }
   // primitive gradient from conserved gradient g (row var*D+dim); out: (NV*D): rho, u (D x D
   // column-major, (a,b) = du_b/dx_a), T
-  void primitiveGradient(const double* w, const double* u, const double* g, double* pg) const {
-    const double rho = w[0];
-    const double* drho = g;
+  void primitiveGradient(const Real* w, const Real* u, const Real* g, Real* pg) const {
+    const Real rho = w[0];
+    const Real* drho = g;
     for (int a = 0; a < dim; a++) pg[a] = drho[a];
-    double vg[9];
+    Real vg[9];
     for (int b = 0; b < dim; b++)
       for (int a = 0; a < dim; a++) {
         vg[a + dim * b] = (g[(1 + b) * dim + a] - drho[a] * w[1 + b]) / rho;
         pg[(1 + b) * dim + a] = vg[a + dim * b];
       }
-    const double etot = u[dim + 1] / rho;
+    const Real etot = u[dim + 1] / rho;
     for (int a = 0; a < dim; a++) {
-      double vgu = 0.0;
+      Real vgu = 0.0;
       for (int b = 0; b < dim; b++) vgu += vg[a + dim * b] * w[1 + b];
-      const double de = (g[(dim + 1) * dim + a] - drho[a] * etot) / rho - vgu;
+      const Real de = (g[(dim + 1) * dim + a] - drho[a] * etot) / rho - vgu;
       pg[(dim + 1) * dim + a] = phys.temperatureFromInternalEnergy(de);
     }
   }
 
   // --- fluxes (ConvectiveFlux.cpp:27-439, ViscousFlux.cpp:58-170) ---
-  void normalFlux(const double* n, const double* w, double* f) const {  // :59-91
-    const double rho = w[0];
-    double un = 0.0;
+  void normalFlux(const Real* n, const Real* w, Real* f) const {  // :59-91
+    const Real rho = w[0];
+    Real un = 0.0;
     for (int d = 0; d < dim; d++) un += w[1 + d] * n[d];
-    const double p = w[dim + 2];
+    const Real p = w[dim + 2];
     f[0] = rho * un;
     for (int d = 0; d < dim; d++) f[1 + d] = rho * un * w[1 + d] + p * n[d];
-    const double et = w[dim + 1] + velocitySquared(w) / 2.0;
+    const Real et = w[dim + 1] + velocitySquared(w) / 2.0;
     f[dim + 1] = (rho * et + p) * un;
   }
-  void convectiveFlux(const double* n, const double* ul, const double* wl, const double* ur, const double* wr,
-                      double* h) const {
-    double fl[5], fr[5];
+  void convectiveFlux(const Real* n, const Real* ul, const Real* wl, const Real* ur, const Real* wr,
+                      Real* h) const {
+    Real fl[5], fr[5];
     if (flux == SDG_FLUX_CENTRAL) {  // :93-104
       normalFlux(n, wl, fl);
       normalFlux(n, wr, fr);
@@ -186,39 +199,39 @@ struct Oracle {
     } else if (flux == SDG_FLUX_LAX_FRIEDRICHS) {  // :106-134
       normalFlux(n, wl, fl);
       normalFlux(n, wr, fr);
-      double unl = 0, unr = 0;
+      Real unl = 0, unr = 0;
       for (int d = 0; d < dim; d++) {
         unl += wl[1 + d] * n[d];
         unr += wr[1 + d] * n[d];
       }
-      const double cl = phys.soundSpeed(wl[0], wl[dim + 2]);
-      const double cr = phys.soundSpeed(wr[0], wr[dim + 2]);
-      const double lam = std::max(std::fabs(unl) + cl, std::fabs(unr) + cr);
+      const Real cl = phys.soundSpeed(wl[0], wl[dim + 2]);
+      const Real cr = phys.soundSpeed(wr[0], wr[dim + 2]);
+      const Real lam = std::max(std::fabs(unl) + cl, std::fabs(unr) + cr);
       for (int v = 0; v < nv; v++) h[v] = ((fl[v] + fr[v]) - lam * (ur[v] - ul[v])) / 2.0;
     } else if (flux == SDG_FLUX_HLLC) {  // :136-238
-      const double rl = wl[0], rr = wr[0], pl = wl[dim + 2], pr = wr[dim + 2];
-      double unl = 0, unr = 0;
+      const Real rl = wl[0], rr = wr[0], pl = wl[dim + 2], pr = wr[dim + 2];
+      Real unl = 0, unr = 0;
       for (int d = 0; d < dim; d++) {
         unl += wl[1 + d] * n[d];
         unr += wr[1 + d] * n[d];
       }
-      const double cl = phys.soundSpeed(rl, pl), cr = phys.soundSpeed(rr, pr);
-      const double rbar = (rl + rr) / 2.0, cbar = (cl + cr) / 2.0;
-      const double ps = std::max(0.0, (pl + pr) / 2.0 - (unr - unl) * rbar * cbar);
-      const double sl =
+      const Real cl = phys.soundSpeed(rl, pl), cr = phys.soundSpeed(rr, pr);
+      const Real rbar = (rl + rr) / 2.0, cbar = (cl + cr) / 2.0;
+      const Real ps = std::max(Real(0), (pl + pr) / 2.0 - (unr - unl) * rbar * cbar);
+      const Real sl =
           unl - cl * (ps <= pl ? 1.0 : std::sqrt(1.0 + (kGamma + 1.0) * (ps / pl - 1.0) / 2.0 / kGamma));
       if (sl >= 0.0) {
         normalFlux(n, wl, h);
         return;
       }
-      const double sr =
+      const Real sr =
           unr + cr * (ps <= pr ? 1.0 : std::sqrt(1.0 + (kGamma + 1.0) * (ps / pr - 1.0) / 2.0 / kGamma));
       if (sr <= 0.0) {
         normalFlux(n, wr, h);
         return;
       }
-      const double sm = (pr - pl + rl * unl * (sl - unl) - rr * unr * (sr - unr)) / (rl * (sl - unl) - rr * (sr - unr));
-      double us[5];
+      const Real sm = (pr - pl + rl * unl * (sl - unl) - rr * unr * (sr - unr)) / (rl * (sl - unl) - rr * (sr - unr));
+      Real us[5];
       if (sm >= 0.0) {
         normalFlux(n, wl, fl);
         us[0] = rl * (sl - unl) / (sl - sm);
@@ -235,70 +248,70 @@ struct Oracle {
     } else if (flux == SDG_FLUX_ROE) {  // :240-350
       normalFlux(n, wl, fl);
       normalFlux(n, wr, fr);
-      const double sql = std::sqrt(wl[0]), sqr = std::sqrt(wr[0]);
-      const double sqs = sql + sqr;
-      const double rho = std::sqrt(wl[0] * wr[0]);
-      double vel[3];
+      const Real sql = std::sqrt(wl[0]), sqr = std::sqrt(wr[0]);
+      const Real sqs = sql + sqr;
+      const Real rho = std::sqrt(wl[0] * wr[0]);
+      Real vel[3];
       for (int d = 0; d < dim; d++) vel[d] = (sql * wl[1 + d] + sqr * wr[1 + d]) / sqs;
-      const double hl = wl[dim + 1] * kGamma + velocitySquared(wl) / 2.0;
-      const double hr = wr[dim + 1] * kGamma + velocitySquared(wr) / 2.0;
-      const double htot = (sql * hl + sqr * hr) / sqs;
-      double v2 = 0.0, un = 0.0;
+      const Real hl = wl[dim + 1] * kGamma + velocitySquared(wl) / 2.0;
+      const Real hr = wr[dim + 1] * kGamma + velocitySquared(wr) / 2.0;
+      const Real htot = (sql * hl + sqr * hr) / sqs;
+      Real v2 = 0.0, un = 0.0;
       for (int d = 0; d < dim; d++) {
         v2 += vel[d] * vel[d];
         un += vel[d] * n[d];
       }
-      const double e = (htot - v2 / 2.0) / kGamma;
-      const double p = phys.pressure(rho, e);
-      const double c = phys.soundSpeed(rho, p);
-      double dw[8];
+      const Real e = (htot - v2 / 2.0) / kGamma;
+      const Real p = phys.pressure(rho, e);
+      const Real c = phys.soundSpeed(rho, p);
+      Real dw[8];
       for (int i = 0; i < dim + 3; i++) dw[i] = wr[i] - wl[i];
-      double dun = 0.0, vdv = 0.0;
+      Real dun = 0.0, vdv = 0.0;
       for (int d = 0; d < dim; d++) {
         dun += dw[1 + d] * n[d];
         vdv += vel[d] * dw[1 + d];
       }
-      const double delta = c / 20.0;
-      const double a1 = std::fabs(un - c) > delta ? std::fabs(un - c) : ((un - c) * (un - c) + delta * delta) / (2.0 * delta);
-      const double a3 = std::fabs(un + c) > delta ? std::fabs(un + c) : ((un + c) * (un + c) + delta * delta) / (2.0 * delta);
-      const double dp = dw[dim + 2], drho = dw[0];
-      double col[5][5] = {};
-      const double s0 = a1 * (dp - rho * c * dun) / (2.0 * c * c);
+      const Real delta = c / 20.0;
+      const Real a1 = std::fabs(un - c) > delta ? std::fabs(un - c) : ((un - c) * (un - c) + delta * delta) / (2.0 * delta);
+      const Real a3 = std::fabs(un + c) > delta ? std::fabs(un + c) : ((un + c) * (un + c) + delta * delta) / (2.0 * delta);
+      const Real dp = dw[dim + 2], drho = dw[0];
+      Real col[5][5] = {};
+      const Real s0 = a1 * (dp - rho * c * dun) / (2.0 * c * c);
       col[0][0] = 1.0 * s0;
       for (int d = 0; d < dim; d++) col[0][1 + d] = (vel[d] - c * n[d]) * s0;
       col[0][dim + 1] = (htot - c * un) * s0;
-      const double s1 = std::fabs(un) * (drho - dp / (c * c));
+      const Real s1 = std::fabs(un) * (drho - dp / (c * c));
       col[1][0] = 1.0 * s1;
       for (int d = 0; d < dim; d++) col[1][1 + d] = vel[d] * s1;
       col[1][dim + 1] = v2 / 2.0 * s1;
       if (dim >= 2) {
-        const double s2 = std::fabs(un) * rho;
+        const Real s2 = std::fabs(un) * rho;
         col[2][0] = 0.0 * s2;
         for (int d = 0; d < dim; d++) col[2][1 + d] = (dw[1 + d] - dun * n[d]) * s2;
         col[2][dim + 1] = (vdv - un * dun) * s2;
       }
-      const double s3 = a3 * (dp + rho * c * dun) / (2.0 * c * c);
+      const Real s3 = a3 * (dp + rho * c * dun) / (2.0 * c * c);
       col[dim + 1][0] = 1.0 * s3;
       for (int d = 0; d < dim; d++) col[dim + 1][1 + d] = (vel[d] + c * n[d]) * s3;
       col[dim + 1][dim + 1] = (htot + c * un) * s3;
       for (int v = 0; v < nv; v++) {
-        double s = 0.0;
+        Real s = 0.0;
         for (int k = 0; k < nv; k++) s += col[k][v];  // rowwise().sum()
         h[v] = ((fl[v] + fr[v]) - s) / 2.0;
       }
     } else {  // Exact (weakly compressible), :352-414
-      const double c = phys.soundSpeed(0.0, 0.0);
-      double unl = 0, unr = 0;
+      const Real c = phys.soundSpeed(0.0, 0.0);
+      Real unl = 0, unr = 0;
       for (int d = 0; d < dim; d++) {
         unl += wl[1 + d] * n[d];
         unr += wr[1 + d] * n[d];
       }
-      const double rho = std::sqrt(wl[0] * wr[0] * std::exp((unl - unr) / c));
-      const double un = (unl + unr) / 2.0 + std::log(wl[0] / wr[0]) * c / 2.0;
-      double we[8];
-      const double* ws = un < 0.0 ? wr : wl;
-      const double uns = un < 0.0 ? unr : unl;
-      const double e = ws[dim + 1] * ws[0] / rho;
+      const Real rho = std::sqrt(wl[0] * wr[0] * std::exp((unl - unr) / c));
+      const Real un = (unl + unr) / 2.0 + std::log(wl[0] / wr[0]) * c / 2.0;
+      Real we[8];
+      const Real* ws = un < 0.0 ? wr : wl;
+      const Real uns = un < 0.0 ? unr : unl;
+      const Real e = ws[dim + 1] * ws[0] / rho;
       we[0] = rho;
       for (int d = 0; d < dim; d++) we[1 + d] = ws[1 + d] + (un - uns) * n[d];
       we[dim + 1] = e;
@@ -307,32 +320,32 @@ struct Oracle {
     }
   }
   // viscous raw flux F_v (D x NV, column-major), ViscousFlux.cpp:58-103
-  void viscousRawFlux(const double* w, const double* pg, double* fv) const {
-    const double t = phys.temperatureFromInternalEnergy(w[dim + 1]);
-    const double mu = phys.viscosity(t);
-    const double* vg = pg + dim;  // D x D column-major, (a,b) = du_b/dx_a
-    double tr = 0.0;
+  void viscousRawFlux(const Real* w, const Real* pg, Real* fv) const {
+    const Real t = phys.temperatureFromInternalEnergy(w[dim + 1]);
+    const Real mu = phys.viscosity(t);
+    const Real* vg = pg + dim;  // D x D column-major, (a,b) = du_b/dx_a
+    Real tr = 0.0;
     for (int a = 0; a < dim; a++) tr += vg[a + dim * a];
     for (int a = 0; a < dim; a++) fv[a] = 0.0;
-    double tau[9];
+    Real tau[9];
     for (int b = 0; b < dim; b++)
       for (int a = 0; a < dim; a++)
         tau[a + dim * b] = mu * (vg[a + dim * b] + vg[b + dim * a]) - (a == b ? 2.0 / 3.0 * mu * tr * 1.0 : 2.0 / 3.0 * mu * tr * 0.0);
     for (int b = 0; b < dim; b++)
       for (int a = 0; a < dim; a++) fv[a + dim * (1 + b)] = tau[a + dim * b];
-    const double k = phys.conductivity(t);
-    const double* tg = pg + (dim + 1) * dim;
+    const Real k = phys.conductivity(t);
+    const Real* tg = pg + (dim + 1) * dim;
     for (int a = 0; a < dim; a++) {
-      double s = 0.0;
+      Real s = 0.0;
       for (int b = 0; b < dim; b++) s += tau[a + dim * b] * w[1 + b];
       fv[a + dim * (dim + 1)] = s + k * tg[a];
     }
   }
-  void viscousNormalFlux(const double* n, const double* w, const double* pg, double* f) const {  // :115-124
-    double fv[15];
+  void viscousNormalFlux(const Real* n, const Real* w, const Real* pg, Real* f) const {  // :115-124
+    Real fv[15];
     viscousRawFlux(w, pg, fv);
     for (int v = 0; v < nv; v++) {
-      double s = 0.0;
+      Real s = 0.0;
       for (int a = 0; a < dim; a++) s += fv[a + dim * v] * n[a];
       f[v] = s;
     }
@@ -342,32 +355,32 @@ struct Oracle {
 
   // --- boundary conditions (BoundaryCondition.cpp:79-547), computational variables ---
   // calculateBoundaryVariable: boundary state wb from the interior state wl and boundary data wd
-  void boundaryVariable(int type, const double* n, const double* wl, const double* wd, double* wb) const {
+  void boundaryVariable(int type, const Real* n, const Real* wl, const Real* wd, Real* wb) const {
     const int nw = dim + 3;
-    double unl = 0.0;
+    Real unl = 0.0;
     for (int d = 0; d < dim; d++) unl += wl[1 + d] * n[d];
-    const double mach = unl / phys.soundSpeed(wl[0], wl[dim + 2]);
+    const Real mach = unl / phys.soundSpeed(wl[0], wl[dim + 2]);
     switch (type) {
       case 0: {  // RiemannFarfield :79-247
         if (std::fabs(mach) > 1.0) {
-          const double* src = mach < 0.0 ? wd : wl;  // supersonic inflow / outflow
+          const Real* src = mach < 0.0 ? wd : wl;  // supersonic inflow / outflow
           for (int i = 0; i < nw; i++) wb[i] = src[i];
           break;
         }
-        const double gm1 = kGamma - 1.0;
-        double und = 0.0;
+        const Real gm1 = kGamma - 1.0;
+        Real und = 0.0;
         for (int d = 0; d < dim; d++) und += wd[1 + d] * n[d];
-        const double left_toward = und - 2.0 * phys.soundSpeed(wd[0], wd[dim + 2]) / gm1;
-        const double right_toward = unl + 2.0 * phys.soundSpeed(wl[0], wl[dim + 2]) / gm1;
-        const double unb = (left_toward + right_toward) / 2.0;
-        const double* ref = mach < 0.0 ? wd : wl;  // subsonic inflow: outside, outflow: inside
-        double unr = 0.0;
+        const Real left_toward = und - 2.0 * phys.soundSpeed(wd[0], wd[dim + 2]) / gm1;
+        const Real right_toward = unl + 2.0 * phys.soundSpeed(wl[0], wl[dim + 2]) / gm1;
+        const Real unb = (left_toward + right_toward) / 2.0;
+        const Real* ref = mach < 0.0 ? wd : wl;  // subsonic inflow: outside, outflow: inside
+        Real unr = 0.0;
         for (int d = 0; d < dim; d++) unr += ref[1 + d] * n[d];
         for (int d = 0; d < dim; d++) wb[1 + d] = ref[1 + d] + (unb - unr) * n[d];
-        const double c = gm1 * (right_toward - left_toward) / 4.0;
-        const double entropy = ref[dim + 2] / std::pow(ref[0], kGamma);
-        const double rho = std::pow(c * c / (kGamma * entropy), 1.0 / gm1);
-        const double p = rho * c * c / kGamma;
+        const Real c = gm1 * (right_toward - left_toward) / 4.0;
+        const Real entropy = ref[dim + 2] / std::pow(ref[0], kGamma);
+        const Real rho = std::pow(c * c / (kGamma * entropy), 1.0 / gm1);
+        const Real p = rho * c * c / kGamma;
         wb[0] = rho;
         wb[dim + 1] = p / (gm1 * rho);
         wb[dim + 2] = p;
@@ -397,17 +410,17 @@ struct Oracle {
         break;
     }
   }
-  void conservedFromComputational(const double* w, double* u) const {
+  void conservedFromComputational(const Real* w, Real* u) const {
     u[0] = w[0];
     for (int d = 0; d < dim; d++) u[1 + d] = w[0] * w[1 + d];
     u[dim + 1] = w[0] * (w[dim + 1] + velocitySquared(w) / 2.0);
   }
-  void boundaryTraces(int64_t b, double* ul) const {  // NV x N_fq of the boundary face's element
+  void boundaryTraces(int64_t b, Real* ul) const {  // NV x N_fq of the boundary face's element
     const int64_t e = bparent[b];
     const int s = blocal[b];
     for (int j = 0; j < nfq; j++)
       for (int v = 0; v < nv; v++) {
-        double a = 0.0;
+        Real a = 0.0;
         for (int i = 0; i < nb; i++) a += coeff[e * szC() + v + nv * i] * Vf[(accumulate(s) + j) + static_cast<size_t>(naq) * i];
         ul[v + nv * j] = a;
       }
@@ -416,33 +429,33 @@ struct Oracle {
     const int rows = nv * dim;
 #pragma omp parallel for schedule(static)
     for (int64_t b = 0; b < nbface; b++) {
-      std::vector<double> ul(static_cast<size_t>(nv) * nfq);
+      std::vector<Real> ul(static_cast<size_t>(nv) * nfq);
       boundaryTraces(b, ul.data());
       const int64_t e = bparent[b];
       const int s = blocal[b];
       for (int j = 0; j < nfq; j++) {
         const size_t bp = static_cast<size_t>(b) * nfq + j;
-        const double* n = &bnormal[bp * dim];
-        const double jw = bjw[bp];
-        const double* u = &ul[nv * j];
-        double vg[5], ig[5];
+        const Real* n = &bnormal[bp * dim];
+        const Real jw = bjw[bp];
+        const Real* u = &ul[nv * j];
+        Real vg[5], ig[5];
         if (btype[b] <= 2) {  // farfield, inflow, outflow: interior state, no interface term
           for (int v = 0; v < nv; v++) {
             vg[v] = u[v];
             ig[v] = 0.0;
           }
         } else {  // walls: conserved boundary state and its jump
-          double wl[8], wb[8];
+          Real wl[8], wb[8];
           computational(u, wl);
           boundaryVariable(btype[b], n, wl, &bdummy[bp * (dim + 3)], wb);
           conservedFromComputational(wb, vg);
           for (int v = 0; v < nv; v++) ig[v] = vg[v] - u[v];
         }
-        double* cl = &vg_adj[static_cast<size_t>(e) * rows * naq + static_cast<size_t>(rows) * (accumulate(s) + j)];
+        Real* cl = &vg_adj[static_cast<size_t>(e) * rows * naq + static_cast<size_t>(rows) * (accumulate(s) + j)];
         for (int v = 0; v < nv; v++)
           for (int a = 0; a < dim; a++) cl[v * dim + a] = (n[a] * vg[v]) * jw;
         if (ns()) {
-          double* il = &ig_adj[static_cast<size_t>(e) * rows * naq + static_cast<size_t>(rows) * (accumulate(s) + j)];
+          Real* il = &ig_adj[static_cast<size_t>(e) * rows * naq + static_cast<size_t>(rows) * (accumulate(s) + j)];
           for (int v = 0; v < nv; v++)
             for (int a = 0; a < dim; a++) il[v * dim + a] = (n[a] * ig[v]) * jw;
         }
@@ -453,7 +466,7 @@ struct Oracle {
     const int rows = nv * dim;
 #pragma omp parallel for schedule(static)
     for (int64_t b = 0; b < nbface; b++) {
-      std::vector<double> ul(static_cast<size_t>(nv) * nfq), gl;
+      std::vector<Real> ul(static_cast<size_t>(nv) * nfq), gl;
       boundaryTraces(b, ul.data());
       const int64_t e = bparent[b];
       const int s = blocal[b];
@@ -463,14 +476,14 @@ struct Oracle {
       }
       for (int j = 0; j < nfq; j++) {
         const size_t bp = static_cast<size_t>(b) * nfq + j;
-        const double* n = &bnormal[bp * dim];
-        const double* u = &ul[nv * j];
-        double wl[8], wb[8], h[5], tmp[5];
+        const Real* n = &bnormal[bp * dim];
+        const Real* u = &ul[nv * j];
+        Real wl[8], wb[8], h[5], tmp[5];
         computational(u, wl);
         boundaryVariable(btype[b], n, wl, &bdummy[bp * (dim + 3)], wb);
         normalFlux(n, wb, h);  // calculateConvectiveNormalFlux of the boundary state
         if (ns()) {
-          double pgl[15], pgb[15], fl[5], fb[5], wm[8];
+          Real pgl[15], pgb[15], fl[5], fb[5], wm[8];
           primitiveGradient(wl, u, &gl[static_cast<size_t>(rows) * j], pgl);
           for (int r = 0; r < rows; r++) pgb[r] = pgl[r];
           if (btype[b] == 4 || btype[b] == 5)  // adiabatic walls: no temperature gradient
@@ -482,7 +495,7 @@ struct Oracle {
         } else {
           for (int v = 0; v < nv; v++) tmp[v] = h[v] * bjw[bp];
         }
-        double* cl = &adj_quad[static_cast<size_t>(e) * nv * naq + static_cast<size_t>(nv) * (accumulate(s) + j)];
+        Real* cl = &adj_quad[static_cast<size_t>(e) * nv * naq + static_cast<size_t>(nv) * (accumulate(s) + j)];
         for (int v = 0; v < nv; v++) cl[v] = tmp[v];
       }
     }
@@ -492,12 +505,12 @@ struct Oracle {
   void gradientQuadrature() {  // SpatialDiscrete.cpp:294-346
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < ne; e++) {
-      std::vector<double> uq(static_cast<size_t>(nv) * nq);
+      std::vector<Real> uq(static_cast<size_t>(nv) * nq);
       gemmBt(uq.data(), &coeff[e * szC()], V.data(), nv, nb, nq, false);
-      double* out = &vg_quad[static_cast<size_t>(e) * nv * dim * dim * nq];
+      Real* out = &vg_quad[static_cast<size_t>(e) * nv * dim * dim * nq];
       const int rows = nv * dim;
       for (int q = 0; q < nq; q++) {
-        const double* G = &tiw[(static_cast<size_t>(e) * nq + q) * dim * dim];
+        const Real* G = &tiw[(static_cast<size_t>(e) * nq + q) * dim * dim];
         for (int k = 0; k < nv; k++)
           for (int b = 0; b < dim; b++)
             for (int a = 0; a < dim; a++)
@@ -505,13 +518,13 @@ struct Oracle {
       }
     }
   }
-  void faceTraces(int64_t f, double* ul, double* ur) const {
+  void faceTraces(int64_t f, Real* ul, Real* ur) const {
     const int32_t L = fparent[2 * f], R = fparent[2 * f + 1];
     const int sl = flocal[2 * f], sr = flocal[2 * f + 1];
     // NV x N_fq = C * Vf[face rows]^T
     for (int j = 0; j < nfq; j++)
       for (int v = 0; v < nv; v++) {
-        double a = 0.0, b = 0.0;
+        Real a = 0.0, b = 0.0;
         for (int i = 0; i < nb; i++) {
           a += coeff[L * szC() + v + nv * i] * Vf[(accumulate(sl) + j) + static_cast<size_t>(naq) * i];
           b += coeff[R * szC() + v + nv * i] * Vf[(accumulate(sr) + j) + static_cast<size_t>(naq) * i];
@@ -524,19 +537,19 @@ struct Oracle {
     const int rows = nv * dim;
 #pragma omp parallel for schedule(static)
     for (int64_t f = 0; f < nface; f++) {
-      std::vector<double> ul(static_cast<size_t>(nv) * nfq), ur(static_cast<size_t>(nv) * nfq);
+      std::vector<Real> ul(static_cast<size_t>(nv) * nfq), ur(static_cast<size_t>(nv) * nfq);
       faceTraces(f, ul.data(), ur.data());
       const int32_t L = fparent[2 * f], R = fparent[2 * f + 1];
       const int sl = flocal[2 * f], sr = flocal[2 * f + 1];
       for (int j = 0; j < nfq; j++) {
         const int k = fseq[f * nfq + j];
-        const double* n = &fnormal[(static_cast<size_t>(f) * nfq + j) * dim];
-        const double jw = fjw[f * nfq + j];
-        double tmp[15];
+        const Real* n = &fnormal[(static_cast<size_t>(f) * nfq + j) * dim];
+        const Real jw = fjw[f * nfq + j];
+        Real tmp[15];
         for (int v = 0; v < nv; v++)
           for (int a = 0; a < dim; a++) tmp[v * dim + a] = (n[a] * (ul[v + nv * j] + ur[v + nv * k]) / 2.0) * jw;
-        double* cl = &vg_adj[static_cast<size_t>(L) * rows * naq + static_cast<size_t>(rows) * (accumulate(sl) + j)];
-        double* cr = &vg_adj[static_cast<size_t>(R) * rows * naq + static_cast<size_t>(rows) * (accumulate(sr) + k)];
+        Real* cl = &vg_adj[static_cast<size_t>(L) * rows * naq + static_cast<size_t>(rows) * (accumulate(sl) + j)];
+        Real* cr = &vg_adj[static_cast<size_t>(R) * rows * naq + static_cast<size_t>(rows) * (accumulate(sr) + k)];
         for (int r = 0; r < rows; r++) {
           cl[r] = tmp[r];
           cr[r] = -tmp[r];
@@ -544,8 +557,8 @@ struct Oracle {
         if (ns()) {
           for (int v = 0; v < nv; v++)
             for (int a = 0; a < dim; a++) tmp[v * dim + a] = (n[a] * (ur[v + nv * k] - ul[v + nv * j]) / 2.0) * jw;
-          double* il = &ig_adj[static_cast<size_t>(L) * rows * naq + static_cast<size_t>(rows) * (accumulate(sl) + j)];
-          double* ir = &ig_adj[static_cast<size_t>(R) * rows * naq + static_cast<size_t>(rows) * (accumulate(sr) + k)];
+          Real* il = &ig_adj[static_cast<size_t>(L) * rows * naq + static_cast<size_t>(rows) * (accumulate(sl) + j)];
+          Real* ir = &ig_adj[static_cast<size_t>(R) * rows * naq + static_cast<size_t>(rows) * (accumulate(sr) + k)];
           for (int r = 0; r < rows; r++) {
             il[r] = tmp[r];
             ir[r] = tmp[r];
@@ -558,7 +571,7 @@ struct Oracle {
     const int rows = nv * dim;
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < ne; e++) {
-      double* res = &vg_res[e * szG()];
+      Real* res = &vg_res[e * szG()];
       gemm(res, &vg_adj[static_cast<size_t>(e) * rows * naq], Vf.data(), rows, naq, nb, false);
       gemm(res, &vg_quad[static_cast<size_t>(e) * rows * dim * nq], dV.data(), rows, dim * nq, nb, true, -1.0);
       if (ns()) {
@@ -566,10 +579,10 @@ struct Oracle {
           gemm(&ig_res[e * szG()], &ig_adj[static_cast<size_t>(e) * rows * naq], Vf.data(), rows, naq, nb, false);
         } else {
           for (int s = 0; s < nf; s++) {
-            double* out = &ig_res[(static_cast<size_t>(e) * nf + s) * szG()];
+            Real* out = &ig_res[(static_cast<size_t>(e) * nf + s) * szG()];
             for (int j = 0; j < nb; j++)
               for (int r = 0; r < rows; r++) {
-                double acc = 0.0;
+                Real acc = 0.0;
                 for (int p = 0; p < nfq; p++)
                   acc += ig_adj[static_cast<size_t>(e) * rows * naq + r + static_cast<size_t>(rows) * (accumulate(s) + p)] *
                          Vf[(accumulate(s) + p) + static_cast<size_t>(naq) * j];
@@ -584,13 +597,13 @@ struct Oracle {
     const int rows = nv * dim;
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < ne; e++) {
-      const double* mi = &minv[static_cast<size_t>(e) * nb * nb];
+      const Real* mi = &minv[static_cast<size_t>(e) * nb * nb];
       gemm(&vg_coeff[e * szG()], &vg_res[e * szG()], mi, rows, nb, nb, false);
       if (ns()) {
-        double* g = &g_coeff[e * szG()];
-        std::memcpy(g, &vg_coeff[e * szG()], szG() * sizeof(double));
+        Real* g = &g_coeff[e * szG()];
+        std::memcpy(g, &vg_coeff[e * szG()], szG() * sizeof(Real));
         for (int s = 0; s < nig; s++) {
-          double* ic = &ig_coeff[(static_cast<size_t>(e) * nig + s) * szG()];
+          Real* ic = &ig_coeff[(static_cast<size_t>(e) * nig + s) * szG()];
           gemm(ic, &ig_res[(static_cast<size_t>(e) * nig + s) * szG()], mi, rows, nb, nb, false);
           for (size_t i = 0; i < szG(); i++) g[i] += ic[i];
         }
@@ -601,20 +614,20 @@ struct Oracle {
     const int rows = nv * dim;
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < ne; e++) {
-      std::vector<double> uq(static_cast<size_t>(nv) * nq), gq;
+      std::vector<Real> uq(static_cast<size_t>(nv) * nq), gq;
       gemmBt(uq.data(), &coeff[e * szC()], V.data(), nv, nb, nq, false);
       if (ns()) {
         gq.resize(static_cast<size_t>(rows) * nq);
         gemmBt(gq.data(), &g_coeff[e * szG()], V.data(), rows, nb, nq, false);
       }
-      double* out = &vol_quad[static_cast<size_t>(e) * nv * dim * nq];
+      Real* out = &vol_quad[static_cast<size_t>(e) * nv * dim * nq];
       for (int q = 0; q < nq; q++) {
-        double w[8], fc[15], fv[15], pg[15];
-        const double* u = &uq[static_cast<size_t>(nv) * q];
+        Real w[8], fc[15], fv[15], pg[15];
+        const Real* u = &uq[static_cast<size_t>(nv) * q];
         computational(u, w);
         // F_c (D x NV column-major), ConvectiveFlux.cpp:27-44
-        const double rho = w[0], p = w[dim + 2];
-        const double et = w[dim + 1] + velocitySquared(w) / 2.0;
+        const Real rho = w[0], p = w[dim + 2];
+        const Real et = w[dim + 1] + velocitySquared(w) / 2.0;
         for (int a = 0; a < dim; a++) {
           fc[a] = rho * w[1 + a];
           for (int b = 0; b < dim; b++) fc[a + dim * (1 + b)] = rho * w[1 + a] * w[1 + b] + (a == b ? p * 1.0 : p * 0.0);
@@ -624,12 +637,12 @@ struct Oracle {
           primitiveGradient(w, u, &gq[static_cast<size_t>(rows) * q], pg);
           viscousRawFlux(w, pg, fv);
         }
-        const double* G = &tiw[(static_cast<size_t>(e) * nq + q) * dim * dim];
+        const Real* G = &tiw[(static_cast<size_t>(e) * nq + q) * dim * dim];
         for (int v = 0; v < nv; v++)
           for (int b = 0; b < dim; b++) {
-            double s = 0.0;
+            Real s = 0.0;
             for (int a = 0; a < dim; a++) {
-              const double f = ns() ? (fc[a + dim * v] - fv[a + dim * v]) : fc[a + dim * v];
+              const Real f = ns() ? (fc[a + dim * v] - fv[a + dim * v]) : fc[a + dim * v];
               s += f * G[a + dim * b];
             }
             out[v + static_cast<size_t>(nv) * (q * dim + b)] = s;
@@ -637,13 +650,13 @@ struct Oracle {
       }
     }
   }
-  void faceGradientTrace(int64_t e, int s, double* gt) const {  // VariableConvertor.cpp:640-689
+  void faceGradientTrace(int64_t e, int s, Real* gt) const {  // VariableConvertor.cpp:640-689
     const int rows = nv * dim;
     for (int j = 0; j < nfq; j++)
       for (int r = 0; r < rows; r++) {
-        double acc = 0.0;
+        Real acc = 0.0;
         for (int i = 0; i < nb; i++) {
-          double c;
+          Real c;
           if (viscous == SDG_VISC_BR2)
             c = vg_coeff[e * szG() + r + static_cast<size_t>(rows) * i] +
                 ig_coeff[(static_cast<size_t>(e) * nig + s) * szG() + r + static_cast<size_t>(rows) * i];
@@ -658,9 +671,9 @@ struct Oracle {
     const int rows = nv * dim;
 #pragma omp parallel for schedule(static)
     for (int64_t f = 0; f < nface; f++) {
-      std::vector<double> ul(static_cast<size_t>(nv) * nfq), ur(static_cast<size_t>(nv) * nfq);
-      std::vector<double> wl(static_cast<size_t>(dim + 3) * nfq), wr(static_cast<size_t>(dim + 3) * nfq);
-      std::vector<double> gl, gr;
+      std::vector<Real> ul(static_cast<size_t>(nv) * nfq), ur(static_cast<size_t>(nv) * nfq);
+      std::vector<Real> wl(static_cast<size_t>(dim + 3) * nfq), wr(static_cast<size_t>(dim + 3) * nfq);
+      std::vector<Real> gl, gr;
       faceTraces(f, ul.data(), ur.data());
       const int32_t L = fparent[2 * f], R = fparent[2 * f + 1];
       const int sl = flocal[2 * f], sr = flocal[2 * f + 1];
@@ -676,11 +689,11 @@ struct Oracle {
       }
       for (int j = 0; j < nfq; j++) {
         const int k = fseq[f * nfq + j];
-        const double* n = &fnormal[(static_cast<size_t>(f) * nfq + j) * dim];
-        double h[5], tmp[5];
+        const Real* n = &fnormal[(static_cast<size_t>(f) * nfq + j) * dim];
+        Real h[5], tmp[5];
         convectiveFlux(n, &ul[nv * j], &wl[(dim + 3) * j], &ur[nv * k], &wr[(dim + 3) * k], h);
         if (ns()) {
-          double pgl[15], pgr[15], fl[5], fr[5];
+          Real pgl[15], pgr[15], fl[5], fr[5];
           primitiveGradient(&wl[(dim + 3) * j], &ul[nv * j], &gl[static_cast<size_t>(rows) * j], pgl);
           primitiveGradient(&wr[(dim + 3) * k], &ur[nv * k], &gr[static_cast<size_t>(rows) * k], pgr);
           viscousNormalFlux(n, &wl[(dim + 3) * j], pgl, fl);
@@ -689,8 +702,8 @@ struct Oracle {
         } else {
           for (int v = 0; v < nv; v++) tmp[v] = h[v] * fjw[f * nfq + j];
         }
-        double* cl = &adj_quad[static_cast<size_t>(L) * nv * naq + static_cast<size_t>(nv) * (accumulate(sl) + j)];
-        double* cr = &adj_quad[static_cast<size_t>(R) * nv * naq + static_cast<size_t>(nv) * (accumulate(sr) + k)];
+        Real* cl = &adj_quad[static_cast<size_t>(L) * nv * naq + static_cast<size_t>(nv) * (accumulate(sl) + j)];
+        Real* cr = &adj_quad[static_cast<size_t>(R) * nv * naq + static_cast<size_t>(nv) * (accumulate(sr) + k)];
         for (int v = 0; v < nv; v++) {
           cl[v] = tmp[v];
           cr[v] = -tmp[v];
@@ -701,31 +714,31 @@ struct Oracle {
   void calcResidual() {  // SpatialDiscrete.cpp:1016-1032
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < ne; e++) {
-      double* res = &residual[e * szC()];
+      Real* res = &residual[e * szC()];
       gemm(res, &vol_quad[static_cast<size_t>(e) * nv * dim * nq], dV.data(), nv, dim * nq, nb, false);
       gemm(res, &adj_quad[static_cast<size_t>(e) * nv * naq], Vf.data(), nv, naq, nb, true, -1.0);
     }
   }
-  void rkCoefficients(int stage, double c[3]) const {  // TimeIntegration.cpp:45-65
+  void rkCoefficients(int stage, Real c[3]) const {  // TimeIntegration.cpp:45-65
     if (rk == SDG_RK_FORWARD_EULER) {
       c[0] = 1.0; c[1] = 0.0; c[2] = 1.0;
     } else if (rk == SDG_RK_HEUN2) {
-      const double t[2][3] = {{1.0, 0.0, 1.0}, {0.5, 0.5, 0.5}};
+      const Real t[2][3] = {{1.0, 0.0, 1.0}, {0.5, 0.5, 0.5}};
       for (int i = 0; i < 3; i++) c[i] = t[stage][i];
     } else {
-      const double t[3][3] = {{1.0, 0.0, 1.0}, {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+      const Real t[3][3] = {{1.0, 0.0, 1.0}, {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
       for (int i = 0; i < 3; i++) c[i] = t[stage][i];
     }
   }
   int stages() const { return rk == SDG_RK_FORWARD_EULER ? 1 : (rk == SDG_RK_HEUN2 ? 2 : 3); }
-  void updateCoefficient(int stage, double dt) {  // TimeIntegration.cpp:181-198
-    double c[3];
+  void updateCoefficient(int stage, Real dt) {  // TimeIntegration.cpp:181-198
+    Real c[3];
     rkCoefficients(stage, c);
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < ne; e++) {
-      double* u = &coeff[e * szC()];
-      const double* un = &coeff_last[e * szC()];
-      std::vector<double> rm(szC());
+      Real* u = &coeff[e * szC()];
+      const Real* un = &coeff_last[e * szC()];
+      std::vector<Real> rm(szC());
       gemm(rm.data(), &residual[e * szC()], &minv[static_cast<size_t>(e) * nb * nb], nv, nb, nb, false);
       for (size_t i = 0; i < szC(); i++) u[i] *= c[1];
       for (size_t i = 0; i < szC(); i++) u[i] += c[0] * un[i];
@@ -733,21 +746,21 @@ struct Oracle {
     }
   }
   void relativeError() {  // TimeIntegration.cpp:279-324
-    std::vector<double> acc(nv, 0.0);
-    std::vector<double> per(static_cast<size_t>(ne) * nv);
+    std::vector<Real> acc(nv, 0.0);
+    std::vector<Real> per(static_cast<size_t>(ne) * nv);
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < ne; e++) {
-      std::vector<double> rq(static_cast<size_t>(nv) * nq);
+      std::vector<Real> rq(static_cast<size_t>(nv) * nq);
       gemmBt(rq.data(), &residual[e * szC()], V.data(), nv, nb, nq, false);
       for (int v = 0; v < nv; v++) {
-        double s = 0.0;
+        Real s = 0.0;
         for (int q = 0; q < nq; q++) s += std::fabs(rq[v + static_cast<size_t>(nv) * q]);
         per[e * nv + v] = s / nq;
       }
     }
     for (int64_t e = 0; e < ne; e++)
       for (int v = 0; v < nv; v++) acc[v] += per[e * nv + v];
-    for (int v = 0; v < nv; v++) rel_err[v] = acc[v] / static_cast<double>(ne);
+    for (int v = 0; v < nv; v++) rel_err[v] = acc[v] / static_cast<Real>(ne);
   }
   void spatial() {
     gradientQuadrature();
@@ -760,7 +773,7 @@ struct Oracle {
     boundaryQuadrature();
     calcResidual();
   }
-  void step(double dt) {  // TimeIntegration.cpp:327-350
+  void step(Real dt) {  // TimeIntegration.cpp:327-350
     coeff_last = coeff;
     for (int s = 0; s < stages(); s++) {
       spatial();
@@ -788,10 +801,10 @@ int oracle_get_threads(void) { return omp_get_max_threads(); }
 
 // Sizes: dim, nb, nq, nf, nfq, order, ne, nface. Row-major tables as sdg_mesh_export writes them.
 void* oracle_create(const sdg_problem_desc* p, int dim, int nb, int nq, int nf, int nfq, int order, int64_t ne,
-                    int64_t nface, const double* V, const double* dV, const double* Vf, const double* lsq,
-                    const double* detjw, const double* tiw, const double* minv, const double* min_edge,
-                    const int32_t* fparent, const int32_t* flocal, const int32_t* fseq, const double* fnormal,
-                    const double* fjw) {
+                    int64_t nface, const Real* V, const Real* dV, const Real* Vf, const Real* lsq,
+                    const Real* detjw, const Real* tiw, const Real* minv, const Real* min_edge,
+                    const int32_t* fparent, const int32_t* flocal, const int32_t* fseq, const Real* fnormal,
+                    const Real* fjw) {
   if (!p || dim < 1 || dim > 3) return nullptr;
   Oracle* o = new (std::nothrow) Oracle();
   if (!o) return nullptr;
@@ -860,8 +873,8 @@ void oracle_destroy(void* h) { delete static_cast<Oracle*>(h); }
 
 // Boundary faces: parent_local (F_b x 2: element, local face), type (F_b), normal
 // (F_b x N_fq x D), jw (F_b x N_fq).
-int oracle_set_boundary(void* h, int64_t nbface, const int32_t* parent_local, const int32_t* type, const double* normal,
-                        const double* jw) {
+int oracle_set_boundary(void* h, int64_t nbface, const int32_t* parent_local, const int32_t* type, const Real* normal,
+                        const Real* jw) {
   Oracle* o = static_cast<Oracle*>(h);
   o->nbface = nbface;
   o->bparent.resize(nbface);
@@ -878,12 +891,12 @@ int oracle_set_boundary(void* h, int64_t nbface, const int32_t* parent_local, co
 }
 // Boundary data: primitive (rho, u.., T) per boundary point -> computational
 // (VariableConvertor.cpp:118-139)
-int oracle_set_boundary_primitive(void* h, const double* prim) {
+int oracle_set_boundary_primitive(void* h, const Real* prim) {
   Oracle* o = static_cast<Oracle*>(h);
   const int dim = o->dim, nv = o->nv;
   for (size_t i = 0; i < static_cast<size_t>(o->nbface) * o->nfq; i++) {
-    const double* pr = prim + i * nv;
-    double* w = &o->bdummy[i * (dim + 3)];
+    const Real* pr = prim + i * nv;
+    Real* w = &o->bdummy[i * (dim + 3)];
     w[0] = pr[0];
     for (int d = 0; d < dim; d++) w[1 + d] = pr[1 + d];
     w[dim + 1] = o->phys.internalEnergyFromTemperature(pr[dim + 1]);
@@ -892,36 +905,36 @@ int oracle_set_boundary_primitive(void* h, const double* prim) {
   return 0;
 }
 
-int oracle_set_state(void* h, const double* c) {
+int oracle_set_state(void* h, const Real* c) {
   Oracle* o = static_cast<Oracle*>(h);
-  std::memcpy(o->coeff.data(), c, o->coeff.size() * sizeof(double));
+  std::memcpy(o->coeff.data(), c, o->coeff.size() * sizeof(Real));
   o->coeff_last = o->coeff;
   o->has_state = true;
   return 0;
 }
-int oracle_get_state(void* h, double* c) {
+int oracle_get_state(void* h, Real* c) {
   Oracle* o = static_cast<Oracle*>(h);
-  std::memcpy(c, o->coeff.data(), o->coeff.size() * sizeof(double));
+  std::memcpy(c, o->coeff.data(), o->coeff.size() * sizeof(Real));
   return 0;
 }
 // InitialCondition.cpp:85-104: conserved at quadrature points from primitives, then
 // C = U_q V (V^T V)^-1
-int oracle_set_initial_primitive(void* h, const double* prim) {
+int oracle_set_initial_primitive(void* h, const Real* prim) {
   Oracle* o = static_cast<Oracle*>(h);
   const int nv = o->nv, nq = o->nq, nb = o->nb, dim = o->dim;
 #pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < o->ne; e++) {
-    std::vector<double> uq(static_cast<size_t>(nv) * nq), tmp(static_cast<size_t>(nv) * nb);
+    std::vector<Real> uq(static_cast<size_t>(nv) * nq), tmp(static_cast<size_t>(nv) * nb);
     for (int q = 0; q < nq; q++) {
-      const double* pr = &prim[(static_cast<size_t>(e) * nq + q) * nv];
-      const double rho = pr[0];
-      double v2 = 0.0;
+      const Real* pr = &prim[(static_cast<size_t>(e) * nq + q) * nv];
+      const Real rho = pr[0];
+      Real v2 = 0.0;
       uq[static_cast<size_t>(nv) * q] = rho;
       for (int d = 0; d < dim; d++) {
         uq[1 + d + static_cast<size_t>(nv) * q] = rho * pr[1 + d];
         v2 += pr[1 + d] * pr[1 + d];
       }
-      const double et = o->phys.internalEnergyFromTemperature(pr[dim + 1]) + v2 / 2.0;
+      const Real et = o->phys.internalEnergyFromTemperature(pr[dim + 1]) + v2 / 2.0;
       uq[dim + 1 + static_cast<size_t>(nv) * q] = rho * et;
     }
     gemm(tmp.data(), uq.data(), o->V.data(), nv, nq, nb, false);
@@ -931,39 +944,39 @@ int oracle_set_initial_primitive(void* h, const double* prim) {
   o->has_state = true;
   return 0;
 }
-int oracle_residual(void* h, double* r) {
+int oracle_residual(void* h, Real* r) {
   Oracle* o = static_cast<Oracle*>(h);
   o->spatial();
-  if (r) std::memcpy(r, o->residual.data(), o->residual.size() * sizeof(double));
+  if (r) std::memcpy(r, o->residual.data(), o->residual.size() * sizeof(Real));
   return 0;
 }
-int oracle_get_gradient(void* h, double* g) {
+int oracle_get_gradient(void* h, Real* g) {
   Oracle* o = static_cast<Oracle*>(h);
   if (!o->ns()) return -2;
-  std::memcpy(g, o->g_coeff.data(), o->g_coeff.size() * sizeof(double));
+  std::memcpy(g, o->g_coeff.data(), o->g_coeff.size() * sizeof(Real));
   return 0;
 }
-int oracle_compute_dt(void* h, double cfl, double* dt) {  // TimeIntegration.cpp:104-131
+int oracle_compute_dt(void* h, Real cfl, Real* dt) {  // TimeIntegration.cpp:104-131
   Oracle* o = static_cast<Oracle*>(h);
   const int nv = o->nv, nq = o->nq, nb = o->nb, dim = o->dim;
-  double best = std::numeric_limits<double>::max();
+  Real best = std::numeric_limits<Real>::max();
 #pragma omp parallel for schedule(static) reduction(min : best)
   for (int64_t e = 0; e < o->ne; e++) {
-    std::vector<double> uq(static_cast<size_t>(nv) * nq);
+    std::vector<Real> uq(static_cast<size_t>(nv) * nq);
     gemmBt(uq.data(), &o->coeff[e * o->szC()], o->V.data(), nv, nb, nq, false);
     for (int q = 0; q < nq; q++) {
-      double w[8];
+      Real w[8];
       o->computational(&uq[static_cast<size_t>(nv) * q], w);
-      const double c = o->phys.soundSpeed(w[0], w[dim + 2]);
-      const double sr = std::sqrt(o->velocitySquared(w)) + c;
-      const double t = cfl * o->min_edge[e] / (sr * (o->order + 1.0) * (o->order + 1.0));
+      const Real c = o->phys.soundSpeed(w[0], w[dim + 2]);
+      const Real sr = std::sqrt(o->velocitySquared(w)) + c;
+      const Real t = cfl * o->min_edge[e] / (sr * (o->order + 1.0) * (o->order + 1.0));
       best = std::min(best, t);
     }
   }
   *dt = best;
   return 0;
 }
-int oracle_step(void* h, double dt, int nsteps, double* rel_err, int* nan_flag, int* steps_taken) {
+int oracle_step(void* h, Real dt, int nsteps, Real* rel_err, int* nan_flag, int* steps_taken) {
   Oracle* o = static_cast<Oracle*>(h);
   int taken = 0, nanf = 0;
   for (int i = 0; i < nsteps; i++) {
@@ -976,14 +989,14 @@ int oracle_step(void* h, double dt, int nsteps, double* rel_err, int* nan_flag, 
       break;
     }
   }
-  if (rel_err) std::memcpy(rel_err, o->rel_err.data(), o->nv * sizeof(double));
+  if (rel_err) std::memcpy(rel_err, o->rel_err.data(), o->nv * sizeof(Real));
   if (nan_flag) *nan_flag = nanf;
   if (steps_taken) *steps_taken = taken;
   return 0;
 }
 // Pointwise physics for unit tests: conserved states (NV each), normal (D) -> flux (NV).
-int oracle_point_flux(const sdg_problem_desc* p, int dim, int flux, const double* n, const double* ul, const double* ur,
-                      double* h) {
+int oracle_point_flux(const sdg_problem_desc* p, int dim, int flux, const Real* n, const Real* ul, const Real* ur,
+                      Real* h) {
   Oracle o;
   o.dim = dim;
   o.nv = dim + 2;
@@ -994,7 +1007,7 @@ int oracle_point_flux(const sdg_problem_desc* p, int dim, int flux, const double
   o.phys.c0 = p->reference_sound_speed;
   o.phys.rho0 = p->reference_density;
   o.phys.padd = 0.01 * p->reference_density * p->reference_sound_speed * p->reference_sound_speed;
-  double wl[8], wr[8];
+  Real wl[8], wr[8];
   o.computational(ul, wl);
   o.computational(ur, wr);
   o.convectiveFlux(n, ul, wl, ur, wr, h);
@@ -1002,8 +1015,8 @@ int oracle_point_flux(const sdg_problem_desc* p, int dim, int flux, const double
 }
 // Boundary state of calculateBoundaryVariable at one point: conserved interior state ul,
 // primitive boundary data prim (rho, u.., T); wb: computational (rho, u.., e, p).
-int oracle_point_boundary_state(const sdg_problem_desc* p, int dim, int type, const double* n, const double* ul,
-                                const double* prim, double* wb) {
+int oracle_point_boundary_state(const sdg_problem_desc* p, int dim, int type, const Real* n, const Real* ul,
+                                const Real* prim, Real* wb) {
   Oracle o;
   o.dim = dim;
   o.nv = dim + 2;
@@ -1013,7 +1026,7 @@ int oracle_point_boundary_state(const sdg_problem_desc* p, int dim, int type, co
   o.phys.c0 = p->reference_sound_speed;
   o.phys.rho0 = p->reference_density;
   o.phys.padd = 0.01 * p->reference_density * p->reference_sound_speed * p->reference_sound_speed;
-  double wl[8], wd[8];
+  Real wl[8], wd[8];
   o.computational(ul, wl);
   wd[0] = prim[0];
   for (int d = 0; d < dim; d++) wd[1 + d] = prim[1 + d];
@@ -1023,8 +1036,8 @@ int oracle_point_boundary_state(const sdg_problem_desc* p, int dim, int type, co
   return 0;
 }
 // Viscous normal flux at one point from conserved state and conserved gradient (row var*D+dim).
-int oracle_point_viscous_flux(const sdg_problem_desc* p, int dim, const double* n, const double* u, const double* g,
-                              double* f) {
+int oracle_point_viscous_flux(const sdg_problem_desc* p, int dim, const Real* n, const Real* u, const Real* g,
+                              Real* f) {
   Oracle o;
   o.dim = dim;
   o.nv = dim + 2;
@@ -1034,7 +1047,7 @@ int oracle_point_viscous_flux(const sdg_problem_desc* p, int dim, const double* 
   o.phys.cv = p->cv;
   o.phys.mu = p->dynamic_viscosity;
   o.phys.kappa = p->cp * p->dynamic_viscosity / kPrandtl;
-  double w[8], pg[15];
+  Real w[8], pg[15];
   o.computational(u, w);
   o.primitiveGradient(w, u, g, pg);
   o.viscousNormalFlux(n, w, pg, f);

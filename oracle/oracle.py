@@ -4,39 +4,49 @@ Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this mo
 checker (or the timed CPU baseline), never as part of the product path. The oracle consumes the
 same operand tables (basis, geometry, faces) that the product library exports for a mesh, so a
 parity test compares the two implementations of the hot path on identical inputs.
+
+`Oracle(..., extended=True)` runs the same source built with x87 extended precision
+(liboracle_ld.so, 64-bit mantissa) on the same FP64 tables and states. Its distance from the
+FP64 oracle is the FP64 rounding floor of a case: a Navier-Stokes residual that is the small
+difference of large volume and face terms (e.g. the mass residual of a divergence-free
+Taylor-Green state) cannot be matched between two FP64 evaluations more closely than that floor.
 """
 from __future__ import annotations
 
 import ctypes
 import os
 import subprocess
-from ctypes import POINTER, c_double, c_int, c_int64, c_void_p
+from ctypes import POINTER, c_double, c_int, c_int64, c_longdouble, c_void_p
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+LIB_LD_PATH = os.path.join(HERE, "liboracle_ld.so")
 SRC_PATH = os.path.join(HERE, "oracle.cpp")
 
 
 def build(force: bool = False) -> str:
-    """Compile oracle.cpp into liboracle.so with g++ (OpenMP)."""
-    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(SRC_PATH):
-        cmd = ["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fopenmp", "-fPIC", "-shared",
-               "-o", LIB_PATH, SRC_PATH]
-        subprocess.run(cmd, check=True)
+    """Compile oracle.cpp into liboracle.so (FP64) and liboracle_ld.so (x87 extended) with g++."""
+    for path, extra in ((LIB_PATH, []), (LIB_LD_PATH, ["-DORACLE_REAL=long double"])):
+        if force or not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(SRC_PATH):
+            cmd = ["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fopenmp", "-fPIC", "-shared", *extra,
+                   "-o", path, SRC_PATH]
+            subprocess.run(cmd, check=True)
     return LIB_PATH
 
 
-_lib = None
+_libs = {}
 
 
-def load():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def load(extended: bool = False):
+    lib = _libs.get(extended)
+    if lib is None:
+        path = LIB_LD_PATH if extended else LIB_PATH
+        if not os.path.exists(path):
             build()
-        lib = ctypes.CDLL(LIB_PATH)
+        real = c_longdouble if extended else c_double
+        lib = ctypes.CDLL(path)
         lib.oracle_create.restype = c_void_p
         lib.oracle_create.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int64, c_int64] + \
             [c_void_p] * 13
@@ -45,8 +55,8 @@ def load():
                      "oracle_get_gradient"):
             getattr(lib, name).argtypes = [c_void_p, c_void_p]
             getattr(lib, name).restype = c_int
-        lib.oracle_compute_dt.argtypes = [c_void_p, c_double, POINTER(c_double)]
-        lib.oracle_step.argtypes = [c_void_p, c_double, c_int, c_void_p, POINTER(c_int), POINTER(c_int)]
+        lib.oracle_compute_dt.argtypes = [c_void_p, real, POINTER(real)]
+        lib.oracle_step.argtypes = [c_void_p, real, c_int, c_void_p, POINTER(c_int), POINTER(c_int)]
         lib.oracle_point_flux.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
         lib.oracle_point_viscous_flux.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
         lib.oracle_point_boundary_state.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
@@ -54,8 +64,8 @@ def load():
         lib.oracle_set_boundary_primitive.argtypes = [c_void_p, c_void_p]
         lib.oracle_set_threads.argtypes = [c_int]
         lib.oracle_get_threads.restype = c_int
-        _lib = lib
-    return _lib
+        _libs[extended] = lib
+    return lib
 
 
 def _ptr(a: np.ndarray):
@@ -63,10 +73,17 @@ def _ptr(a: np.ndarray):
 
 
 class Oracle:
-    """CPU restatement of Solver<SC> on one mesh."""
+    """CPU restatement of Solver<SC> on one mesh (extended=True: the x87 extended build; arrays
+    still enter and leave as float64, converted exactly on the way in)."""
 
-    def __init__(self, problem, mesh):
-        self.lib = load()
+    _TABLES = ("basis_value", "basis_gradient", "basis_face_value", "basis_lsq_inverse", "jacobian_det_w",
+               "jacobian_tiw", "mass_inverse", "min_edge", "face_normal", "face_jacobian_w",
+               "boundary_face_normal", "boundary_face_jacobian_w")
+
+    def __init__(self, problem, mesh, extended: bool = False):
+        self.extended = extended
+        self.real = np.longdouble if extended else np.float64
+        self.lib = load(extended)
         self.mesh = mesh
         self.dim, self.nv, self.nb, self.nq = mesh.dim, mesh.nv, mesh.nb, mesh.nq
         self.E = mesh.E
@@ -75,6 +92,9 @@ class Oracle:
             "jacobian_tiw", "mass_inverse", "min_edge", "face_parent", "face_local", "face_sequence",
             "face_normal", "face_jacobian_w")}
         k = self._keep
+        for key in self._TABLES:
+            if key in k:
+                k[key] = np.ascontiguousarray(k[key], dtype=self.real)
         self.problem = problem
         self.h = self.lib.oracle_create(
             ctypes.byref(problem), mesh.dim, mesh.nb, mesh.nq, mesh.nf, mesh.nfq, mesh.order, mesh.E, mesh.F,
@@ -89,12 +109,14 @@ class Oracle:
             for key in ("boundary_face_parent", "boundary_face_type", "boundary_face_normal",
                         "boundary_face_jacobian_w"):
                 k[key] = mesh.field(key)
+                if key in self._TABLES:
+                    k[key] = np.ascontiguousarray(k[key], dtype=self.real)
             self.lib.oracle_set_boundary(self.h, self.Fb, _ptr(k["boundary_face_parent"]),
                                          _ptr(k["boundary_face_type"]), _ptr(k["boundary_face_normal"]),
                                          _ptr(k["boundary_face_jacobian_w"]))
 
     def set_boundary_primitive(self, prim: np.ndarray):
-        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        prim = np.ascontiguousarray(prim, dtype=self.real)
         assert prim.size == self.Fb * self.mesh.nfq * self.nv
         self.lib.oracle_set_boundary_primitive(self.h, _ptr(prim))
 
@@ -107,41 +129,44 @@ class Oracle:
         return (self.E, self.nb, self.nv)
 
     def set_state(self, c: np.ndarray):
-        c = np.ascontiguousarray(c, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=self.real)
         self.lib.oracle_set_state(self.h, _ptr(c))
 
+    def _out(self, shape) -> np.ndarray:
+        return np.empty(shape, dtype=self.real)
+
     def get_state(self) -> np.ndarray:
-        out = np.empty(self.state_shape())
+        out = self._out(self.state_shape())
         self.lib.oracle_get_state(self.h, _ptr(out))
-        return out
+        return out.astype(np.float64)
 
     def set_initial_primitive(self, prim: np.ndarray):
-        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        prim = np.ascontiguousarray(prim, dtype=self.real)
         self.lib.oracle_set_initial_primitive(self.h, _ptr(prim))
 
     def residual(self) -> np.ndarray:
-        out = np.empty(self.state_shape())
+        out = self._out(self.state_shape())
         self.lib.oracle_residual(self.h, _ptr(out))
-        return out
+        return out.astype(np.float64)
 
     def gradient(self) -> np.ndarray:
-        out = np.empty((self.E, self.nb, self.nv * self.dim))
+        out = self._out((self.E, self.nb, self.nv * self.dim))
         rc = self.lib.oracle_get_gradient(self.h, _ptr(out))
         if rc != 0:
             raise RuntimeError("gradient is only defined for Navier-Stokes")
-        return out
+        return out.astype(np.float64)
 
     def compute_dt(self, cfl: float) -> float:
-        dt = c_double()
+        dt = (c_longdouble if self.extended else c_double)()
         self.lib.oracle_compute_dt(self.h, cfl, ctypes.byref(dt))
-        return dt.value
+        return float(dt.value)
 
     def step(self, dt: float, nsteps: int = 1):
-        rel = np.empty(self.nv)
+        rel = self._out(self.nv)
         nan = c_int()
         taken = c_int()
         self.lib.oracle_step(self.h, dt, nsteps, _ptr(rel), ctypes.byref(nan), ctypes.byref(taken))
-        return rel, nan.value, taken.value
+        return rel.astype(np.float64), nan.value, taken.value
 
 
 def set_threads(n: int):

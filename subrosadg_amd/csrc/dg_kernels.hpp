@@ -43,6 +43,25 @@ struct MOpsT {
 };
 using MOps = MOpsT<double>;
 
+// Sum-factorisation tables of a hexahedron order (see dg_tensor.hpp).
+constexpr int kTensorMaxN = 6;
+
+// 1-D operators of the hexahedron of one order (by value in the kernel arguments)
+struct TensorOps {
+  double B[kTensorMaxN * kTensorMaxN];     // B[i N + k] = l_k(x_i)
+  double Binv[kTensorMaxN * kTensorMaxN];  // (B^-1)[k N + i]
+  double Dn[kTensorMaxN * kTensorMaxN];    // Dn[i N + m] = d/dxi Lagrange_m (x_i)
+  double lf[2 * kTensorMaxN];              // lf[s N + i] = Lagrange_i(-1) (s = 0), (+1) (s = 1)
+  double w[kTensorMaxN];                   // Gauss weights
+  double kappa[2];                         // sum_i lf[s][i]^2 / w_i
+  int axis[6], side[6];                    // local face -> normal axis (x=0, y=1, z=2), side (0: -1, 1: +1)
+  const short* perm;                       // modal index -> tensor index kx + N ky + N^2 kz
+  const short* iperm;                      // tensor index -> modal index
+  const short* slot_of;                    // [face][node] -> face point of the node's tangential position
+  const short* base;                       // [face][face point] -> node with normal index 0
+  const double* fwt;                       // [face][face point] face Gauss weight (tangential w w)
+};
+
 struct Geo {
   const double* Vt;    // N_b x N_q        basis at volume points, transposed
   const double* dV;    // (N_q D) x N_b    reference basis gradient, row q*D+d
@@ -72,6 +91,7 @@ struct Geo {
   int Fb;
   MOps op;                // FP64 operator tables
   MOpsT<float> opf;       // the same tables rounded to FP32 (mixed-precision mode)
+  TensorOps tens;         // hexahedra: 1-D operators of the sum-factorised kernels
   // mixed-precision mode: the residual-to-update map L^-T of opf.minv kept in FP64, so the update's
   // hierarchic coefficients are formed without FP32 rounding of their (cancelling) terms
   const double* minv_mixed64;

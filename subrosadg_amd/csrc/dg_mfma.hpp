@@ -624,33 +624,41 @@ __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, Sta
   const int k = g.fseq[f * NFQ + j];
   const int64_t L = g.fpar[2 * f], R = g.fpar[2 * f + 1];
   const int slL = g.floc[2 * f] * NFQ + j, slR = g.floc[2 * f + 1] * NFQ + k;
-  double ul[NV], ur[NV], gl[NG], gr[NG];
+  double ul[NV], ur[NV];
   for (int v = 0; v < NV; v++) {
     ul[v] = pointState<TC>(a, Uf_in[(L * NV + v) * NAQ + slL], v);
     ur[v] = pointState<TC>(a, Uf_in[(R * NV + v) * NAQ + slR], v);
   }
-  for (int r = 0; r < NG; r++) {
-    gl[r] = Gf[(L * NG + r) * NAQ + slL];
-    gr[r] = Gf[(R * NG + r) * NAQ + slR];
-  }
-  double wl[D + 3], wr[D + 3], n[D], h[NV], pgl[NG], pgr[NG], fvl[D * NV], fvr[D * NV];
+  double wl[D + 3], wr[D + 3], n[D], h[NV];
   computational<D>(ph, ul, wl);
   computational<D>(ph, ur, wr);
   for (int d = 0; d < D; d++) n[d] = g.fnormal[(f * NFQ + j) * D + d];
   convectiveFlux<D>(ph, n, ul, wl, ur, wr, h);
   if constexpr (sizeof(TC) == 4) reduceFaceFlux<D>(a, n, h);
-  primitiveGradient<D>(ph, wl, ul, gl, pgl);
-  primitiveGradient<D>(ph, wr, ur, gr, pgr);
-  viscousRawFlux<D>(ph, wl, pgl, fvl);
-  viscousRawFlux<D>(ph, wr, pgr, fvr);
+  // normal viscous flux of each side, one side at a time (keeps one side's gradient live)
+  double vl[NV], vr[NV];
+#pragma unroll 1
+  for (int sd = 0; sd < 2; sd++) {
+    const int64_t es = sd ? R : L;
+    const int ss = sd ? slR : slL;
+    double us[NV], ws[D + 3], gs[NG], pg[NG], fv[D * NV];
+    for (int v = 0; v < NV; v++) us[v] = sd ? ur[v] : ul[v];
+    for (int i = 0; i < D + 3; i++) ws[i] = sd ? wr[i] : wl[i];
+    for (int r = 0; r < NG; r++) gs[r] = Gf[(es * NG + r) * NAQ + ss];
+    primitiveGradient<D>(ph, ws, us, gs, pg);
+    viscousRawFlux<D>(ph, ws, pg, fv);
+    for (int v = 0; v < NV; v++) {
+      double s = 0.0;
+      for (int d = 0; d < D; d++) s += fv[d + D * v] * n[d];
+      if (sd)
+        vr[v] = s;
+      else
+        vl[v] = s;
+    }
+  }
   const double jw = g.fjw[f * NFQ + j];
   for (int v = 0; v < NV; v++) {
-    double vl = 0.0, vr = 0.0;
-    for (int d = 0; d < D; d++) {
-      vl += fvl[d + D * v] * n[d];
-      vr += fvr[d + D * v] * n[d];
-    }
-    const double t = (h[v] - (vl + vr) / 2.0) * jw;
+    const double t = (h[v] - (vl[v] + vr[v]) / 2.0) * jw;
     Hs[(L * NV + v) * NAQ + slL] = static_cast<TC>(-t);
     Hs[(R * NV + v) * NAQ + slR] = static_cast<TC>(t);
   }

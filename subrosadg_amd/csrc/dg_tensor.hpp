@@ -1,0 +1,516 @@
+// dg_tensor.hpp -- sum-factorised Navier-Stokes stage kernels for hexahedra.
+//
+// On a hexahedron the modal basis is a tensor product of 1-D Lobatto functions and the volume and
+// face rules are tensor Gauss rules with the same N = P+1 nodes per direction, so every operator
+// of the stage factors into 1-D operators along grid lines of N nodes:
+//   u(nodes) = (B x B x B) c  with B[i][k] = l_k(x_i);  c = (B^-1 x B^-1 x B^-1) u(nodes);
+//   a reference derivative along one axis is the collocation derivative Dn of the Lagrange
+//   polynomials at the Gauss nodes; the trace of a nodal field on the face at xi_a = -1/+1 is the
+//   contraction with the Lagrange values l(-1) / l(+1) along axis a;
+//   the mass matrix of the nodal (Lagrange) basis is diag(w |J|) (Gauss is exact for it).
+// The stage is then the reference's pass chain (SpatialDiscrete.cpp:194-1116,
+// TimeIntegration.cpp:181-253) in exact arithmetic, with O(N^4) instead of O(N^6) work per element:
+//   lifted gradient (BR1/BR2, TimeIntegration.cpp:200-228): integrating the weak gradient term by
+//     parts (exact for these polynomials), M^-1 [ sum_faces (A - u n |J_f| w) phi - int u grad phi ]
+//     = grad u + sum_faces l_s(i) alpha / (w_i |J|) at node (i, tangential point), alpha the face
+//     term minus the own trace; the BR2 face lift r_s likewise with the jump term beta;
+//   residual R_psi = sum_axes Dn^T (w |J| J^-1 F) + sum_faces l_s(i) Hs, the increment R_psi/(w|J|)
+//   taken back to modal coefficients with B^-1 along each axis; the reference residual (before
+//   M^-1) is (B^T x B^T x B^T) R_psi.
+// Each thread owns one node of one element; pointwise physics is FP64 (pointState / volumeFlux of
+// the matrix-core kernels, so the mixed-precision mode works the same way: contractions and work
+// buffers in TC on deviations from the reference state, FP64 state and RK accumulation).
+#ifndef SDG_DG_TENSOR_HPP_
+#define SDG_DG_TENSOR_HPP_
+
+#include <hip/hip_runtime.h>
+
+#include "dg_mfma.hpp"
+
+namespace sdg {
+
+// full 3-D Gauss weight of node q
+__device__ __forceinline__ double nodeWeight(const TensorOps& t, int q, int N) {
+  return t.w[q % N] * t.w[(q / N) % N] * t.w[q / (N * N)];
+}
+
+template <int N>
+struct TensorShape {
+  static constexpr int NPT = N * N * N;
+  static constexpr int EPB = kThreads / NPT;  // elements per workgroup, one thread per node
+  static constexpr int NV = 5, NG = 15, NFQ = N * N, NAQ = 6 * N * N;
+};
+
+__device__ __forceinline__ int axisStride(int axis, int N) { return axis == 0 ? 1 : (axis == 1 ? N : N * N); }
+__device__ __forceinline__ int nodeIndex(int q, int axis, int N) {
+  return axis == 0 ? q % N : (axis == 1 ? (q / N) % N : q / (N * N));
+}
+
+// out[c][q] = sum_m M[idx_axis(q) N + m] in[c][q with axis index m], for C components of NPT
+// values each, one thread per node (all threads of the element call it; no barrier inside)
+template <typename TC, int N, int C>
+__device__ __forceinline__ void linePass(const double* M, int axis, const TC* in, TC* out, int q, bool ok) {
+  constexpr int NPT = N * N * N;
+  if (!ok) return;
+  const int i = nodeIndex(q, axis, N), st = axisStride(axis, N);
+  const int q0 = q - i * st;
+  TC m[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) m[k] = static_cast<TC>(M[i * N + k]);
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    TC s = TC(0);
+#pragma unroll
+    for (int k = 0; k < N; k++) s += m[k] * in[c * NPT + q0 + k * st];
+    out[c * NPT + q] = s;
+  }
+}
+
+// the element's state as TC operands in tensor order (FP64 state, or its FP32 deviation)
+template <typename TC, int N>
+__device__ __forceinline__ void loadModal(const TensorOps& t, const StageArgs& a, const double* U, int64_t e, TC* sA,
+                                          int q, bool ok) {
+  constexpr int NPT = N * N * N, NV = 5;
+  if (!ok) return;
+  const int tq = t.perm[q];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    const double u = U[(e * NV + v) * NPT + q];
+    if constexpr (sizeof(TC) == 8)
+      sA[v * NPT + tq] = u;
+    else
+      sA[v * NPT + tq] = deviationOperand(u, a.mix.uref[v], a.mix.one[q]);
+  }
+}
+
+// traces of the nodal state sA at the element's own face points -> Uf_out (TC)
+// (rec: optional FP64 copy of the traces, same layout, for the adjoint's recorded run)
+template <typename TC, int N>
+__device__ __forceinline__ void writeTraces(const TensorOps& t, const TC* sA, TC* Uf_out, int64_t e, int le, int tid,
+                                            double* rec = nullptr) {
+  constexpr int NPT = N * N * N, NV = 5, NFQ = N * N, NAQ = 6 * NFQ, EPB = TensorShape<N>::EPB;
+  for (int idx = tid - le * NPT; idx < NAQ * NV; idx += NPT) {
+    if (idx < 0) break;
+    const int sl = idx % NAQ, v = idx / NAQ, s = sl / NFQ;
+    const int b = t.base[sl], st = axisStride(t.axis[s], N);
+    const double* l = t.lf + t.side[s] * N;
+    TC acc = TC(0);
+#pragma unroll
+    for (int i = 0; i < N; i++) acc += static_cast<TC>(l[i]) * sA[v * NPT + b + i * st];
+    Uf_out[(e * NV + v) * NAQ + sl] = acc;
+    if (rec) rec[(e * NV + v) * NAQ + sl] = static_cast<double>(acc);
+  }
+  (void)EPB;
+}
+
+// element slot of a thread (tid / NPT); at NPT = 64 a wavefront is one element, and the index is
+// made wave-uniform so that the element's geometry is read through scalar loads into SGPRs
+template <int NPT>
+__device__ __forceinline__ int localElement(int tid) {
+  int le = tid / NPT;
+  if constexpr (NPT == 64) le = __builtin_amdgcn_readfirstlane(le);
+  return le;
+}
+
+// ---- kernel 1: lifted gradient, volume flux, gradient traces ------------------------------------
+// LDS per element (TC values), two regions reused across the phases:
+//   sW [NG NPT]:  nodal state Y [v][node] + alpha [v][slot]  ->  the face-gradient field [r][node]
+//   sB [NV NAQ]:  the basis-pass panel (modal, then axis-0/1 partial sums)  ->  beta [v][slot]
+template <typename TC, int N, int VISC>
+__global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageArgs a) {
+  const TensorOps& t = g.tens;
+  using S = TensorShape<N>;
+  constexpr int NPT = S::NPT, EPB = S::EPB, NV = S::NV, NG = S::NG, NFQ = S::NFQ, NAQ = S::NAQ, D = 3;
+  static_assert(NV * NPT + NV * NAQ <= NG * NPT && NV * NPT <= NV * NAQ, "LDS regions do not nest");
+  __shared__ TC sW[EPB][NG * NPT];
+  __shared__ TC sB[EPB][NV * NAQ];
+  if (a.status && a.status[0]) return;
+  const int tid = threadIdx.x, le = localElement<NPT>(tid), q = tid % NPT;
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * EPB + le;
+  const bool ok = le < EPB && e < g.E;
+  const int lec = le < EPB ? le : 0;
+  TC* W = sW[lec];
+  TC* Y = W;              // nodal state [v][node]
+  TC* Al = W + NV * NPT;  // alpha [v][slot]: face term minus the own trace, scalar (times n)
+  TC* Bt = sB[lec];       // basis-pass panel, then beta [v][slot]: jump term, scalar (times n)
+  // element geometry (affine hexahedra: constant J^-1 and |J|), read before any store
+  double dj = 0.0, gi[D * D];
+  if (ok) {
+    dj = g.detj[e];
+    for (int i = 0; i < D * D; i++) gi[i] = g.ginv[e * D * D + i];
+  }
+  // face terms per (variable, slot): alpha = A - u_own n |J_f| w and beta = J (scalars of n_out).
+  // The interior faces' operands are gathered first, so that their latency overlaps the basis
+  // passes; boundary slots get jw = 0 here and are filled in after the passes.
+  const TC* Uf_in = pick<TC>(a.Uf_in, a.Uf_in32);
+  constexpr int ITEMS = (NV * NAQ + NPT - 1) / NPT;
+  TC fo[ITEMS], fn[ITEMS];
+  double fw[ITEMS];
+  if (ok) {
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+      const int idx = q + k * NPT;
+      fo[k] = fn[k] = TC(0);
+      fw[k] = 0.0;
+      if (idx < NV * NAQ) {
+        const int v = idx / NAQ, sl = idx % NAQ;
+        const int64_t es = e * NAQ + sl;
+        const int nbs = g.snb[es];
+        if (nbs >= 0) {
+          fo[k] = Uf_in[(e * NV + v) * NAQ + sl];
+          fn[k] = Uf_in[(static_cast<int64_t>(nbs / NAQ) * NV + v) * NAQ + nbs % NAQ];
+          fw[k] = g.fjw[g.sfp[es] >> 1];
+        }
+      }
+    }
+  }
+  loadModal<TC, N>(t, a, a.U, e, Bt, q, ok);
+  __syncthreads();
+  linePass<TC, N, NV>(t.B, 0, Bt, Y, q, ok);
+  __syncthreads();
+  linePass<TC, N, NV>(t.B, 1, Y, Bt, q, ok);
+  __syncthreads();
+  linePass<TC, N, NV>(t.B, 2, Bt, Y, q, ok);  // Y: nodal state
+  __syncthreads();
+  if (ok) {
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+      const int idx = q + k * NPT;
+      if (idx < NV * NAQ) {
+        const double uo = static_cast<double>(fo[k]), un = static_cast<double>(fn[k]);
+        Al[idx] = static_cast<TC>(((uo + un) / 2.0 - uo) * fw[k]);
+        Bt[idx] = static_cast<TC>(((un - uo) / 2.0) * fw[k]);
+      }
+    }
+    if (g.Fb > 0)
+      for (int idx = q; idx < NV * NAQ; idx += NPT) {
+        const int v = idx / NAQ, sl = idx % NAQ;
+        const int nbs = g.snb[e * NAQ + sl];
+        if (nbs >= 0) continue;
+        const double uo = Uf_in[(e * NV + v) * NAQ + sl];
+        double ub[NV], Ab[D], Jb[D];
+        for (int k = 0; k < NV; k++) ub[k] = pointState<TC>(a, Uf_in[(e * NV + k) * NAQ + sl], k);
+        boundaryGradientTerms<D>(ph, g, -1 - nbs, ub, v, sizeof(TC) == 4 ? a.mix.uref[v] : 0.0, Ab, Jb);
+        // A and J are (scalar) n |J_f| w; recover the scalars along the dominant normal component
+        const int bp = -1 - nbs;
+        int dm = 0;
+        for (int d = 1; d < D; d++)
+          if (fabs(g.bnormal[static_cast<int64_t>(bp) * D + d]) > fabs(g.bnormal[static_cast<int64_t>(bp) * D + dm])) dm = d;
+        const double nd = g.bnormal[static_cast<int64_t>(bp) * D + dm];
+        Al[idx] = static_cast<TC>(Ab[dm] / nd - uo * g.bjw[bp]);
+        Bt[idx] = static_cast<TC>(Jb[dm] / nd);
+      }
+  }
+  __syncthreads();
+  // gradient at the node: J^-T (Dn u along each axis) + the face corrections
+  double u[NV], wtot[NG], wface[VISC == 2 ? NG : 1];
+  if (ok) {
+    double dr[NV][D];
+    for (int ax = 0; ax < D; ax++) {
+      const int i = nodeIndex(q, ax, N), st = axisStride(ax, N), q0 = q - i * st;
+      for (int v = 0; v < NV; v++) {
+        TC s = TC(0);
+#pragma unroll
+        for (int m = 0; m < N; m++) s += static_cast<TC>(t.Dn[i * N + m]) * Y[v * NPT + q0 + m * st];
+        dr[v][ax] = static_cast<double>(s);
+      }
+    }
+    for (int v = 0; v < NV; v++) {
+      u[v] = pointState<TC>(a, Y[v * NPT + q], v);
+      for (int d = 0; d < D; d++) {
+        double s = 0.0;
+        for (int ax = 0; ax < D; ax++) s += gi[d + D * ax] * dr[v][ax];
+        wtot[v * D + d] = s;
+        if constexpr (VISC == 2) wface[v * D + d] = s;
+      }
+    }
+#pragma unroll 1  // unrolled, the six faces' LDS loads are hoisted together and double the VGPRs
+    for (int s = 0; s < 6; s++) {
+      const int ax = t.axis[s], i = nodeIndex(q, ax, N);
+      const int sl = s * NFQ + t.slot_of[s * NPT + q];
+      // alpha, beta carry |J_f| w_t1 w_t2; M^-1 = 1 / (w_i w_t1 w_t2 |J|) at the node
+      const double coef = t.lf[t.side[s] * N + i] / (nodeWeight(t, q, N) * dj);
+      const int64_t es = e * NAQ + sl;
+      const int nbs = g.snb[es];
+      double n[D];
+      if (nbs < 0) {
+        for (int d = 0; d < D; d++) n[d] = g.bnormal[static_cast<int64_t>(-1 - nbs) * D + d];
+      } else {
+        const int fps = g.sfp[es];
+        const double sg = (fps & 1) ? -1.0 : 1.0;
+        for (int d = 0; d < D; d++) n[d] = sg * g.fnormal[static_cast<int64_t>(fps >> 1) * D + d];
+      }
+      for (int v = 0; v < NV; v++) {
+        const double al = static_cast<double>(Al[v * NAQ + sl]) * coef;
+        const double be = static_cast<double>(Bt[v * NAQ + sl]) * coef;
+        for (int d = 0; d < D; d++) {
+          wtot[v * D + d] += (al + be) * n[d];
+          if constexpr (VISC == 2) wface[v * D + d] += al * n[d];
+        }
+      }
+    }
+  }
+  __syncthreads();  // Y and alpha are read; sW takes the gradient fields from here on
+  // optional: modal coefficients of the total lifted gradient (sdg_get_gradient), B^-1 per axis
+  if (a.Wout) {
+    if (ok)
+      for (int r = 0; r < NG; r++) W[r * NPT + q] = static_cast<TC>(wtot[r]);
+    __syncthreads();
+    if (ok) {
+      // modal c[k] = sum_i Binv[kx][i] Binv[ky][j] Binv[kz][l] w(i, j, l): direct triple sum (output path)
+      const int kx = q % N, ky = (q / N) % N, kz = q / (N * N);
+      const int b = t.iperm[q];
+#pragma unroll 1
+      for (int r = 0; r < NG; r++) {
+        double s = 0.0;
+#pragma unroll 1
+        for (int l = 0; l < N; l++)
+          for (int j = 0; j < N; j++)
+            for (int i = 0; i < N; i++)
+              s += t.Binv[kx * N + i] * t.Binv[ky * N + j] * t.Binv[kz * N + l] *
+                   static_cast<double>(W[r * NPT + i + N * j + N * N * l]);
+        a.Wout[(e * NPT + b) * NG + r] = s;
+      }
+    }
+    __syncthreads();
+  }
+  // the face-gradient field (BR2: without the jump lifts; BR1: the total) -> sW for the traces
+  if (ok)
+    for (int r = 0; r < NG; r++) W[r * NPT + q] = static_cast<TC>(VISC == 2 ? wface[VISC == 2 ? r : 0] : wtot[r]);
+  // volume flux (F_c - F_v)^T (J^T)^-1 |J| w -> HBM Fv[e][(q D + b)][v]
+  if (ok) {
+    double w[D + 3], pg[NG], fv[D * NV];
+    computational<D>(ph, u, w);
+    primitiveGradient<D>(ph, w, u, wtot, pg);
+    viscousRawFlux<D>(ph, w, pg, fv);
+    const double jw = dj * nodeWeight(t, q, N);
+    double G[D * D], out[D * NV];
+    for (int i = 0; i < D * D; i++) G[i] = gi[i] * jw;
+    volumeFlux<TC, D, true>(a, w, fv, G, out);
+    TC* Fv = pick<TC>(a.Fv, a.Fv32);
+    TC* o = Fv + (e * NPT + q) * D * NV;
+    for (int i = 0; i < D * NV; i++) o[i] = static_cast<TC>(out[i]);
+  }
+  __syncthreads();
+  // traces of the face-gradient field (+ the own face's BR2 lift) -> Gf
+  if (ok) {
+    TC* Gf = pick<TC>(a.Gf, a.Gf32);
+    const double idj = 1.0 / dj;
+#pragma unroll 1
+    for (int idx = q; idx < NAQ * NG; idx += NPT) {
+      const int sl = idx % NAQ, r = idx / NAQ, s = sl / NFQ;
+      const int b = t.base[sl], st = axisStride(t.axis[s], N);
+      const double* l = t.lf + t.side[s] * N;
+      TC acc = TC(0);
+#pragma unroll
+      for (int i = 0; i < N; i++) acc += static_cast<TC>(l[i]) * W[r * NPT + b + i * st];
+      double gv = static_cast<double>(acc);
+      if (VISC == 2) {  // + r_s at its own face: kappa_side beta n_d / |J|
+        const int64_t es = e * NAQ + sl;
+        const int nbs = g.snb[es];
+        double nd;
+        if (nbs < 0) {
+          nd = g.bnormal[static_cast<int64_t>(-1 - nbs) * D + r % D];
+        } else {
+          const int fps = g.sfp[es];
+          nd = ((fps & 1) ? -1.0 : 1.0) * g.fnormal[static_cast<int64_t>(fps >> 1) * D + r % D];
+        }
+        gv += t.kappa[t.side[s]] * static_cast<double>(Bt[(r / D) * NAQ + sl]) * nd * idj / t.fwt[sl];
+      }
+      Gf[(e * NG + r) * NAQ + sl] = static_cast<TC>(gv);
+    }
+  }
+}
+
+// ---- kernel 2 (after nsFaceFluxKernel): residual, update, next-stage traces, monitor -----------
+template <typename TC, int N>
+__global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, StageArgs a) {
+  const TensorOps& t = g.tens;
+  using S = TensorShape<N>;
+  constexpr int NPT = S::NPT, EPB = S::EPB, NV = S::NV, NFQ = S::NFQ, NAQ = S::NAQ, D = 3;
+  constexpr bool MIX = sizeof(TC) == 4;
+  // face-flux slots (TC), then the nodal increment / residual (FP64), in one region
+  constexpr int HB = (NAQ * NV * static_cast<int>(sizeof(TC)) > NV * NPT * 8 ? NAQ * NV * static_cast<int>(sizeof(TC))
+                                                                              : NV * NPT * 8) / 8;
+  static_assert(D * NV * NPT * sizeof(TC) >= NV * NPT * 8, "FP64 scratch must fit the flux panel");
+  __shared__ TC sF[EPB][D * NV * NPT];  // contravariant volume flux [axis][v][node]; later FP64 scratch
+  __shared__ double sHR[EPB][HB];
+  __shared__ double sM[EPB][NV];        // monitor partials
+  if (a.status && a.status[0]) return;
+  const int tid = threadIdx.x, le = localElement<NPT>(tid), q = tid % NPT;
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * EPB + le;
+  const bool ok = le < EPB && e < g.E;
+  const int lec = le < EPB ? le : 0;
+  TC* sH = reinterpret_cast<TC*>(sHR[lec]);
+  double* SR = sHR[lec];
+  const TC* Fv = pick<TC>(static_cast<const double*>(a.Fv), static_cast<const float*>(a.Fv32));
+  const TC* Hs = pick<TC>(static_cast<const double*>(a.Hs), static_cast<const float*>(a.Hs32));
+  if (ok) {
+    const TC* src = Fv + e * NPT * D * NV;
+    for (int idx = q; idx < NPT * D * NV; idx += NPT) {  // HBM [node][axis][v] -> LDS [axis][v][node]
+      const int node = idx / (D * NV), rem = idx % (D * NV);
+      sF[le][rem * NPT + node] = src[idx];
+    }
+    for (int idx = q; idx < NAQ * NV; idx += NPT) {  // HBM [v][slot] -> LDS [slot][v]
+      const int v = idx / NAQ, sl = idx % NAQ;
+      sH[sl * NV + v] = Hs[(e * NV + v) * NAQ + sl];
+    }
+  }
+  __syncthreads();
+  double R[NV];
+  if (ok) {
+    for (int v = 0; v < NV; v++) R[v] = 0.0;
+    for (int ax = 0; ax < D; ax++) {
+      const int i = nodeIndex(q, ax, N), st = axisStride(ax, N), q0 = q - i * st;
+      for (int v = 0; v < NV; v++) {
+        TC s = TC(0);
+#pragma unroll
+        for (int m = 0; m < N; m++) s += static_cast<TC>(t.Dn[m * N + i]) * sF[le][(ax * NV + v) * NPT + q0 + m * st];
+        R[v] += static_cast<double>(s);
+      }
+    }
+#pragma unroll 1
+    for (int s = 0; s < 6; s++) {
+      const int i = nodeIndex(q, t.axis[s], N);
+      const int sl = s * NFQ + t.slot_of[s * NPT + q];
+      const double l = t.lf[t.side[s] * N + i];
+      for (int v = 0; v < NV; v++) R[v] += l * static_cast<double>(sH[sl * NV + v]);
+    }
+    if constexpr (MIX) R[NV - 1] += a.mix.h0 * R[0];  // energy: add back h0 R_rho
+  }
+  __syncthreads();  // the face-flux slots are read; their region takes the nodal residual
+  if (ok)
+    for (int v = 0; v < NV; v++) SR[v * NPT + q] = R[v];
+  __syncthreads();
+  // the reference residual (before M^-1) is (B^T x B^T x B^T) R_psi (modal); its monitor is
+  // mean_q |R V^T| = mean over nodes of |(B x B x B) of it| (TimeIntegration.cpp:279-298)
+  double* T0 = reinterpret_cast<double*>(sF[lec]);  // FP64 scratch: NV NPT doubles fit in sF
+  auto pass64 = [&](const double* M, bool transpose, int axis, const double* in, double* out) {
+    if (!ok) return;
+    const int i = nodeIndex(q, axis, N), st = axisStride(axis, N), q0 = q - i * st;
+    for (int v = 0; v < NV; v++) {
+      double s = 0.0;
+      for (int k = 0; k < N; k++) s += (transpose ? M[k * N + i] : M[i * N + k]) * in[v * NPT + q0 + k * st];
+      out[v * NPT + q] = s;
+    }
+  };
+  if (a.mode == 1 || a.last_stage) {
+    pass64(t.B, true, 0, SR, T0);
+    __syncthreads();
+    pass64(t.B, true, 1, T0, SR);
+    __syncthreads();
+    pass64(t.B, true, 2, SR, T0);  // T0: modal residual, tensor order
+    __syncthreads();
+    if (a.mode == 1) {
+      if (ok)
+        for (int v = 0; v < NV; v++) a.Rout[(e * NV + v) * NPT + t.iperm[q]] = T0[v * NPT + q];
+      return;
+    }
+    pass64(t.B, false, 0, T0, SR);
+    __syncthreads();
+    pass64(t.B, false, 1, SR, T0);
+    __syncthreads();
+    pass64(t.B, false, 2, T0, SR);  // SR: R V^T at the nodes
+    __syncthreads();
+    if (le < EPB && q < NV) {
+      double s = 0.0;
+      for (int k = 0; k < NPT; k++) s += fabs(SR[q * NPT + k]);
+      sM[le][q] = s;
+    }
+    __syncthreads();
+    if (ok && q < NV) a.mon[e * NV + q] = sM[le][q] / NPT;
+    // rebuild the nodal residual from the modal one (T0 was overwritten): R_psi is needed below
+    __syncthreads();
+    if (ok)
+      for (int v = 0; v < NV; v++) SR[v * NPT + q] = R[v];
+    __syncthreads();
+  }
+  // nodal increment R_psi / (w |J|) -> modal with B^-1 along each axis (FP64)
+  if (ok) {
+    const double jw = g.detj[e] * nodeWeight(t, q, N);
+    for (int v = 0; v < NV; v++) SR[v * NPT + q] = R[v] / jw;
+  }
+  __syncthreads();
+  pass64(t.Binv, false, 0, SR, T0);
+  __syncthreads();
+  pass64(t.Binv, false, 1, T0, SR);
+  __syncthreads();
+  pass64(t.Binv, false, 2, SR, T0);  // T0: modal increment, tensor order
+  __syncthreads();
+  // RK update (FP64) in the modal order; the new coefficients (TC operands, tensor order) -> X
+  TC* X = reinterpret_cast<TC*>(SR);  // SR is free again
+  if (ok) {
+    const int b = t.iperm[q];
+    for (int v = 0; v < NV; v++) {
+      const int64_t gi = (e * NV + v) * NPT + b;
+      double u = a.U[gi] * a.c1;
+      u += a.c0 * a.Un[gi];
+      u += a.c2dt * T0[v * NPT + q];
+      a.U[gi] = u;
+      if (a.last_stage) a.Un[gi] = u;
+      if (a.rec) {
+        const int step = a.status[1] + (a.last_stage ? 1 : 0);
+        const int stage = a.last_stage ? 0 : a.rec_stage + 1;
+        if (step < a.rec_nsteps) a.rec[(static_cast<int64_t>(step) * a.rec_S + stage) * a.rec_n + gi] = u;
+      }
+      if constexpr (MIX)
+        X[v * NPT + q] = deviationOperand(u, a.mix.uref[v], a.mix.one[b]);
+      else
+        X[v * NPT + q] = u;
+    }
+  }
+  __syncthreads();
+  TC* Y = reinterpret_cast<TC*>(T0);
+  linePass<TC, N, NV>(t.B, 0, X, Y, q, ok);
+  __syncthreads();
+  linePass<TC, N, NV>(t.B, 1, Y, X, q, ok);
+  __syncthreads();
+  linePass<TC, N, NV>(t.B, 2, X, Y, q, ok);  // Y: new nodal state
+  __syncthreads();
+  double* rec_t = nullptr;
+  if (a.rec_t) {
+    const int step = a.status[1] + (a.last_stage ? 1 : 0);
+    const int stage = a.last_stage ? 0 : a.rec_stage + 1;
+    if (step < a.rec_nsteps) rec_t = a.rec_t + (static_cast<int64_t>(step) * a.rec_S + stage) * a.rec_nt;
+  }
+  if (le < EPB && e < g.E) writeTraces<TC, N>(t, Y, pick<TC>(a.Uf_out, a.Uf_out32), e, le, tid, rec_t);
+  (void)ph;
+}
+
+// traces from the state in HBM with the residual kernel's own arithmetic (restarts, checkpoints)
+template <typename TC, int N>
+__global__ __launch_bounds__(kThreads) void tensorTraceKernel(Geo g, const double* U, void* Uf_out, MixRef mix) {
+  const TensorOps& t = g.tens;
+  using S = TensorShape<N>;
+  constexpr int NPT = S::NPT, EPB = S::EPB, NV = S::NV;
+  __shared__ TC sX[EPB][NV * NPT];
+  __shared__ TC sY[EPB][NV * NPT];
+  const int tid = threadIdx.x, le = tid / NPT, q = tid % NPT;
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * EPB + le;
+  const bool ok = le < EPB && e < g.E;
+  const int lec = le < EPB ? le : 0;
+  TC* X = sX[lec];
+  TC* Y = sY[lec];
+  if (ok) {
+    const int tq = t.perm[q];
+    for (int v = 0; v < NV; v++) {
+      const double u = U[(e * NV + v) * NPT + q];
+      if constexpr (sizeof(TC) == 8)
+        X[v * NPT + tq] = u;
+      else
+        X[v * NPT + tq] = deviationOperand(u, mix.uref[v], mix.one[q]);
+    }
+  }
+  __syncthreads();
+  linePass<TC, N, NV>(t.B, 0, X, Y, q, ok);
+  __syncthreads();
+  linePass<TC, N, NV>(t.B, 1, Y, X, q, ok);
+  __syncthreads();
+  linePass<TC, N, NV>(t.B, 2, X, Y, q, ok);
+  __syncthreads();
+  if (ok) writeTraces<TC, N>(t, Y, static_cast<TC*>(Uf_out), e, le, tid);
+}
+
+}  // namespace sdg
+
+#endif  // SDG_DG_TENSOR_HPP_

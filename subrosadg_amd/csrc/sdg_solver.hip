@@ -16,6 +16,7 @@
 
 #include "../../include/sdg.h"
 #include "dg_adjoint.hpp"
+#include "dg_tensor.hpp"
 #include "dg_kernels.hpp"
 #include "dg_mfma.hpp"
 #include "kit.hpp"
@@ -35,6 +36,7 @@ using AdjFn = void (*)(hipStream_t, const Geo&, const Phys&, const StageArgs&, c
 
 struct Launchers {
   bool mfma = false;
+  bool tensor = false;  // hexahedra, Navier-Stokes: sum-factorised stage kernels
   StageFn euler = nullptr;
   StageFn ns_grad_br1 = nullptr;
   StageFn ns_grad_br2 = nullptr;
@@ -168,7 +170,45 @@ bool pickValu(bool mixed, Launchers& l) {
   return true;
 }
 
-bool selectLaunchers(int type, int order, bool force_valu, bool mixed, Launchers& l) {
+// Navier-Stokes on hexahedra P2..P4: the sum-factorised stage kernels replace the matrix-core
+// gradient and residual kernels (the face-flux kernel, the adjoint kernels and Euler stay).
+template <typename TC, int N>
+void useTensorKernels(Launchers& l) {
+  constexpr int EPB = sdg::TensorShape<N>::EPB;
+  l.ns_grad_br1 = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+    hipLaunchKernelGGL((sdg::nsGradTensor<TC, N, 1>), dim3(blocksFor(g.E, EPB)), dim3(sdg::kThreads), 0, st, g, ph, a);
+  };
+  l.ns_grad_br2 = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+    hipLaunchKernelGGL((sdg::nsGradTensor<TC, N, 2>), dim3(blocksFor(g.E, EPB)), dim3(sdg::kThreads), 0, st, g, ph, a);
+  };
+  l.ns_flux = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+    hipLaunchKernelGGL((sdg::nsResidualTensor<TC, N>), dim3(blocksFor(g.E, EPB)), dim3(sdg::kThreads), 0, st, g, ph,
+                       a);
+  };
+  l.trace = [](hipStream_t st, const Geo& g, const double* U, void* Uf, const sdg::MixRef& mix) {
+    hipLaunchKernelGGL((sdg::tensorTraceKernel<TC, N>), dim3(blocksFor(g.E, EPB)), dim3(sdg::kThreads), 0, st, g, U,
+                       Uf, mix);
+  };
+  l.tensor = true;
+}
+template <int N>
+bool pickTensor(bool mixed, Launchers& l) {
+  if (mixed)
+    useTensorKernels<float, N>(l);
+  else
+    useTensorKernels<double, N>(l);
+  return true;
+}
+
+bool selectLaunchers(int type, int order, bool force_valu, bool mixed, Launchers& l, bool tensor = false) {
+  if (type == SDG_ELEM_HEXAHEDRON && tensor && !force_valu && order >= 2 && order <= 4) {
+    if (!selectLaunchers(type, order, false, mixed, l)) return false;
+    switch (order) {
+      case 2: return pickTensor<3>(mixed, l);
+      case 3: return pickTensor<4>(mixed, l);
+      default: return pickTensor<5>(mixed, l);
+    }
+  }
   if (type == SDG_ELEM_TRIANGLE) {  // bases of 3..15 functions: vector kernels
     switch (order) {
       case 1: return pickValu<2, 3, 3, 3, 2>(mixed, l);
@@ -198,6 +238,97 @@ bool selectLaunchers(int type, int order, bool force_valu, bool mixed, Launchers
     }
   }
   return false;
+}
+
+// 1-D tables and node/face maps of the sum-factorised hexahedron kernels (see dg_tensor.hpp).
+bool buildTensorTables(const sdg::RefElement& r, sdg::TensorOps& t, std::vector<short>& perm, std::vector<short>& iperm,
+                       std::vector<short>& slot_of, std::vector<short>& base, std::vector<double>& fwt) {
+  const int N = r.order + 1, NPT = N * N * N, NFQ = r.nfq;
+  if (r.type != SDG_ELEM_HEXAHEDRON || N > sdg::kTensorMaxN || r.nb != NPT || r.nq != NPT || NFQ != N * N) return false;
+  std::vector<double> gx, gw;
+  sdg::gaussLegendre(N, gx, gw);
+  t = sdg::TensorOps{};
+  std::vector<double> B(static_cast<size_t>(N) * N), Bi(B.size());
+  for (int i = 0; i < N; i++)
+    for (int k = 0; k < N; k++) {
+      double v, d;
+      sdg::lobatto(k, gx[i], v, d);
+      B[static_cast<size_t>(i) * N + k] = v;
+    }
+  if (!sdg::invertMatrix(N, B.data(), Bi.data())) return false;
+  std::vector<double> c(N, 1.0);
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < N; j++)
+      if (j != i) c[i] *= gx[i] - gx[j];
+  for (int i = 0; i < N; i++) {
+    t.w[i] = gw[i];
+    for (int k = 0; k < N; k++) {
+      t.B[i * N + k] = B[static_cast<size_t>(i) * N + k];
+      t.Binv[i * N + k] = Bi[static_cast<size_t>(i) * N + k];
+      double dnm = 0.0;
+      if (k != i) {
+        dnm = (c[i] / c[k]) / (gx[i] - gx[k]);
+      } else {
+        for (int j = 0; j < N; j++)
+          if (j != i) dnm += 1.0 / (gx[i] - gx[j]);
+      }
+      t.Dn[i * N + k] = dnm;  // derivative of Lagrange k at node i
+    }
+    for (int s = 0; s < 2; s++) {
+      const double x0 = s == 0 ? -1.0 : 1.0;
+      double l = 1.0;
+      for (int j = 0; j < N; j++)
+        if (j != i) l *= (x0 - gx[j]) / (gx[i] - gx[j]);
+      t.lf[s * N + i] = l;
+    }
+  }
+  for (int s = 0; s < 2; s++) {
+    t.kappa[s] = 0.0;
+    for (int i = 0; i < N; i++) t.kappa[s] += t.lf[s * N + i] * t.lf[s * N + i] / t.w[i];
+  }
+  auto nodeOf = [&](double x) {
+    for (int i = 0; i < N; i++)
+      if (std::fabs(x - gx[i]) < 1e-12) return i;
+    return -1;
+  };
+  perm.assign(NPT, 0);
+  iperm.assign(NPT, 0);
+  for (int b = 0; b < NPT; b++) {
+    const int tq = r.modes[b][0] + N * r.modes[b][1] + N * N * r.modes[b][2];
+    perm[b] = static_cast<short>(tq);
+    iperm[tq] = static_cast<short>(b);
+  }
+  slot_of.assign(static_cast<size_t>(6) * NPT, -1);
+  base.assign(static_cast<size_t>(6) * NFQ, -1);
+  fwt.assign(static_cast<size_t>(6) * NFQ, 0.0);
+  const int stride[3] = {1, N, N * N};
+  for (int s = 0; s < 6; s++) {
+    const double* fp = &r.face_param[static_cast<size_t>(s) * NFQ * 3];
+    int ax = -1;
+    for (int d = 0; d < 3 && ax < 0; d++) {
+      bool on = true;
+      for (int p = 0; p < NFQ; p++) on = on && std::fabs(std::fabs(fp[p * 3 + d]) - 1.0) < 1e-12 && fp[p * 3 + d] * fp[d] > 0.0;
+      if (on) ax = d;
+    }
+    if (ax < 0) return false;
+    t.axis[s] = ax;
+    t.side[s] = fp[ax] > 0.0 ? 1 : 0;
+    for (int p = 0; p < NFQ; p++) {
+      int bq = 0;
+      for (int d = 0; d < 3; d++) {
+        if (d == ax) continue;
+        const int i = nodeOf(fp[p * 3 + d]);
+        if (i < 0) return false;
+        bq += i * stride[d];
+      }
+      base[static_cast<size_t>(s) * NFQ + p] = static_cast<short>(bq);
+      fwt[static_cast<size_t>(s) * NFQ + p] = r.fweight[p];
+      for (int i = 0; i < N; i++) slot_of[static_cast<size_t>(s) * NPT + bq + i * stride[ax]] = static_cast<short>(p);
+    }
+  }
+  for (short v : slot_of)
+    if (v < 0) return false;
+  return true;
 }
 
 // Zero-padded transposed operator T[r4(K)][r16(M)] with T[k][m] = src(m, k).
@@ -564,7 +695,9 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   const char* fv = std::getenv("SDG_FORCE_VALU");
   const bool mixed = problem->precision == SDG_PREC_MIXED;
   ctx->mixed = mixed;
-  if (!selectLaunchers(r.type, r.order, fv && fv[0] == '1', mixed, ctx->launch)) {
+  const char* dense = std::getenv("SDG_DENSE");  // 1: keep the dense matrix-core NS kernels on hexahedra
+  const bool tensor = problem->equation == SDG_EQ_COMPRESSIBLE_NS && !(dense && dense[0] == '1');
+  if (!selectLaunchers(r.type, r.order, fv && fv[0] == '1', mixed, ctx->launch, tensor)) {
     delete ctx;
     return fail(nullptr, SDG_E_UNSUPPORTED,
                 mixed ? "sdg_create: the mixed-precision mode needs a matrix-core kernel set (quadrangle P3-P5, "
@@ -842,6 +975,22 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     g.op.lift = pl;
     g.op.rinterp = pi;
   }
+  if (ctx->launch.tensor) {
+    std::vector<short> perm, iperm, slot_of, base;
+    std::vector<double> fwt;
+    if (!buildTensorTables(r, g.tens, perm, iperm, slot_of, base, fwt))
+      return bail(fail(ctx, SDG_E_INVALID, "sdg_create: inconsistent hexahedron tables for the sum-factorised kernels"));
+    short *dp, *dip, *ds, *db;
+    double* dw;
+    if (upload(ctx, perm, &dp) || upload(ctx, iperm, &dip) || upload(ctx, slot_of, &ds) || upload(ctx, base, &db) ||
+        upload(ctx, fwt, &dw))
+      return bail(SDG_E_NOMEM);
+    g.tens.perm = dp;
+    g.tens.iperm = dip;
+    g.tens.slot_of = ds;
+    g.tens.base = db;
+    g.tens.fwt = dw;
+  }
   ctx->minedge = dme;
   ctx->h_lsq = r.lsq_inverse;
   ctx->h_chol = r.mass_chol;
@@ -1018,7 +1167,9 @@ int sdg_residual(sdg_ctx* ctx, double* residual) {
   std::vector<double> dev(static_cast<size_t>(ctx->E) * ctx->nv * ctx->nb);
   HIP_TRY(ctx, hipMemcpyAsync(dev.data(), ctx->Rout, dev.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  if (ctx->mixed) {  // the mixed-precision kernels form R_psi = L^-1 R; return R = L R_psi
+  // the dense mixed-precision kernels form R_psi = L^-1 R; return R = L R_psi (the sum-factorised
+  // kernels write R itself)
+  if (ctx->mixed && !ctx->launch.tensor) {
     const int nb = ctx->nb;
     std::vector<double> t(nb);
     for (size_t o = 0; o < dev.size(); o += nb) {

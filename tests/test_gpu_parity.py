@@ -2,6 +2,14 @@
 
 FP64 bar: per-variable max |gpu - oracle| / max |oracle| <= 1e-12 for residuals, lifted
 gradients and states after several SSP-RK steps, on identical meshes and initial states.
+
+Where the FP64 oracle is itself farther than the bar from the exact-arithmetic value of the same
+discrete operator (measured with its x87 extended build on the same tables and state), no FP64
+evaluation can be held to the bar against it; there the GPU result must be at least as close to
+the extended evaluation as the bar, or as twice the FP64 oracle's own distance (fp64_parity).
+The Taylor-Green cases are such: their mass and energy residuals are differences of volume and
+face terms some 1e3-1e4 times larger (the state is divergence free), and the FP64 oracle is up to
+3e-11 (hexahedra P4) from the extended evaluation.
 """
 import numpy as np
 import pytest
@@ -19,6 +27,24 @@ TOL = 1e-12
 # hex-P3 Taylor-Green state (energy gradient: rho E ~ 179 carried through M^-1), so the bar for
 # two FP64 implementations is 5e-11.
 GRAD_TOL = 5e-11
+
+
+def fp64_parity(gpu, oracle_val, extended_val, state, dim, tol=TOL, h=None):
+    """gpu matches the FP64 oracle to tol, or (the oracle's own rounding floor being above tol)
+    lies no farther from the extended-precision evaluation than max(tol, 2 x the oracle's own
+    distance from it). Returns (error vs oracle, error vs extended, oracle's floor)."""
+    e = P.parity_error(gpu, oracle_val, state, dim, h=h)
+    ex = P.parity_error(gpu, extended_val, state, dim, h=h)
+    floor = P.parity_error(oracle_val, extended_val, state, dim, h=h)
+    print(f"  vs oracle {e:.2e}  vs extended {ex:.2e}  oracle floor {floor:.2e}", flush=True)
+    assert e <= tol or ex <= max(tol, 2.0 * floor), (e, ex, floor)
+    return e, ex, floor
+
+
+def _extended(mesh, prob, state):
+    x = O.Oracle(prob, mesh, extended=True)
+    x.set_state(state)
+    return x
 
 
 def _pair(mesh, prob, prim):
@@ -94,6 +120,8 @@ NS_CASES = [
     ("tgv-br2-hllc-p2", lambda: P.ns_tgv(n=3, order=2, flux=L.FLUX_HLLC, visc=L.VISC_BR2)),
     ("tgv-br2-hllc-p3", lambda: P.ns_tgv(n=3, order=3, flux=L.FLUX_HLLC, visc=L.VISC_BR2)),
     ("tgv-br1-lf-p1", lambda: P.ns_tgv(n=4, order=1, flux=L.FLUX_LAX_FRIEDRICHS, visc=L.VISC_BR1)),
+    ("tgv-br1-roe-p3", lambda: P.ns_tgv(n=2, order=3, flux=L.FLUX_ROE, visc=L.VISC_BR1)),
+    ("tgv-br2-lf-p4", lambda: P.ns_tgv(n=2, order=4, flux=L.FLUX_LAX_FRIEDRICHS, visc=L.VISC_BR2)),
 ]
 
 
@@ -103,29 +131,31 @@ def test_ns_residual_and_gradient_parity(name, make):
     mesh, prob, prim = make()
     o, s = _pair(mesh, prob, prim)
     u0 = o.get_state()
-    ro = o.residual()
-    go = o.gradient()
-    gg = s.gradient()
+    x = _extended(mesh, prob, u0)
+    ro, rx = o.residual(), x.residual()
+    go, gx = o.gradient(), x.gradient()
     h = float(mesh.field("min_edge").min())
-    eg = P.parity_error(gg, go, u0, mesh.dim, h=h)
-    assert eg <= GRAD_TOL, eg
-    err = P.parity_error(s.residual(), ro, u0, mesh.dim)
-    assert err <= TOL, err
+    fp64_parity(s.gradient(), go, gx, u0, mesh.dim, tol=GRAD_TOL, h=h)
+    fp64_parity(s.residual(), ro, rx, u0, mesh.dim)
 
 
-@pytest.mark.parametrize("name,make", NS_CASES[:4], ids=[c[0] for c in NS_CASES[:4]])
+STEP_CASES = NS_CASES[:4] + NS_CASES[5:]
+
+
+@pytest.mark.parametrize("name,make", STEP_CASES, ids=[c[0] for c in STEP_CASES])
 def test_ns_step_parity(name, make):
     print(f"ns steps {name}", flush=True)
     mesh, prob, prim = make()
     o, s = _pair(mesh, prob, prim)
+    x = _extended(mesh, prob, o.get_state())
     dt = o.compute_dt(0.1)
     assert abs(s.compute_dt(0.1) - dt) <= 1e-13 * dt
     ro, _, _ = o.step(dt, 5)
+    x.step(dt, 5)
     rg, nan, taken = s.step(dt, 5)
     assert nan == 0 and taken == 5
     uo = o.get_state()
-    err = P.parity_error(s.get_state(), uo, uo, mesh.dim)
-    assert err <= TOL, err
+    fp64_parity(s.get_state(), uo, x.get_state(), uo, mesh.dim)
 
 
 def test_nan_stops_stepping():
@@ -148,9 +178,12 @@ def test_nan_stops_stepping():
 @pytest.mark.parametrize("make", [lambda: P.ns_tgv(n=3, order=3), lambda: P.ns_shear(n=4, order=3, perturb=True),
                                   lambda: P.euler_vortex(n=6, order=3)])
 def test_matrix_core_and_vector_paths_agree(make, monkeypatch):
-    """The MFMA (matrix-core) stage kernels and the VALU kernels give the same residual and steps."""
+    """The MFMA (matrix-core) stage kernels and the VALU kernels give the same residual and steps
+    (hexahedra: the dense matrix-core kernels, SDG_DENSE=1)."""
     mesh, prob, prim = make()
+    monkeypatch.setenv("SDG_DENSE", "1")
     s_mfma = L.Solver(prob, mesh)
+    monkeypatch.delenv("SDG_DENSE")
     s_mfma.set_initial_primitive(prim)
     monkeypatch.setenv("SDG_FORCE_VALU", "1")
     s_valu = L.Solver(prob, mesh)
@@ -163,3 +196,30 @@ def test_matrix_core_and_vector_paths_agree(make, monkeypatch):
     s_valu.step(dt, 3)
     u1 = s_valu.get_state()
     assert P.parity_error(s_mfma.get_state(), u1, u1, mesh.dim) <= TOL
+
+
+@pytest.mark.parametrize("order,visc", [(2, L.VISC_BR2), (3, L.VISC_BR1), (3, L.VISC_BR2), (4, L.VISC_BR2)])
+def test_sum_factorised_and_dense_hexahedron_kernels_agree(order, visc, monkeypatch):
+    """Navier-Stokes on hexahedra: the sum-factorised stage kernels (default) and the dense
+    matrix-core kernels (SDG_DENSE=1) evaluate the same operator: each of residual, lifted
+    gradient and state after 3 steps of the sum-factorised path is as close to the extended
+    evaluation as the bar or twice the dense path's distance from it."""
+    mesh, prob, prim = P.ns_tgv(n=3 if order < 4 else 2, order=order, visc=visc)
+    s_sf = L.Solver(prob, mesh)
+    s_sf.set_initial_primitive(prim)
+    monkeypatch.setenv("SDG_DENSE", "1")
+    s_dn = L.Solver(prob, mesh)
+    monkeypatch.delenv("SDG_DENSE")
+    u0 = s_sf.get_state()
+    s_dn.set_state(u0)
+    x = _extended(mesh, prob, u0)
+    rx, gx = x.residual(), x.gradient()
+    h = float(mesh.field("min_edge").min())
+    fp64_parity(s_sf.gradient(), s_dn.gradient(), gx, u0, mesh.dim, tol=GRAD_TOL, h=h)
+    fp64_parity(s_sf.residual(), s_dn.residual(), rx, u0, mesh.dim)
+    dt = s_sf.compute_dt(0.1)
+    s_sf.step(dt, 3)
+    s_dn.step(dt, 3)
+    x.step(dt, 3)
+    u1 = s_dn.get_state()
+    fp64_parity(s_sf.get_state(), u1, x.get_state(), u1, mesh.dim)
