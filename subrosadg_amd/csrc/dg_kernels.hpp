@@ -734,28 +734,52 @@ __global__ __launch_bounds__(kThreads) void dtKernel(Geo g, Phys ph, const doubl
 
 // Fixed-order reduction of the monitor partials; raises the NaN stop flag when every variable is
 // NaN (SystemControl.cpp:185-191).
+constexpr int kMonitorBlocks = 256;
+
+// sum of a block's LANES-strided values, one variable per lane (LANES a multiple of NV), -> out[v]
 template <int NV>
-__global__ __launch_bounds__(kThreads) void monitorKernel(const double* mon, int E, double* rel, int* status) {
-  __shared__ double red[kThreads][NV];
-  if (status[0]) return;
+__device__ __forceinline__ void blockVariableSums(const double* x, int64_t n, double* red, double* out) {
+  constexpr int LANES = (kThreads / NV) * NV;
   const int tid = threadIdx.x;
-  double acc[NV];
-  for (int v = 0; v < NV; v++) acc[v] = 0.0;
-  const int chunk = (E + kThreads - 1) / kThreads;
-  const int lo = tid * chunk, hi = lo + chunk < E ? lo + chunk : E;
-  for (int e = lo; e < hi; e++)
-    for (int v = 0; v < NV; v++) acc[v] += mon[static_cast<int64_t>(e) * NV + v];
-  for (int v = 0; v < NV; v++) red[tid][v] = acc[v];
-  __syncthreads();
-  for (int st = kThreads / 2; st > 0; st >>= 1) {
-    if (tid < st)
-      for (int v = 0; v < NV; v++) red[tid][v] += red[tid + st][v];
-    __syncthreads();
+  double acc = 0.0;
+  if (tid < LANES) {
+#pragma unroll 4
+    for (int64_t i = tid; i < n; i += LANES) acc += x[i];
   }
-  if (tid == 0) {
+  red[tid] = acc;
+  __syncthreads();
+  if (tid < NV) {
+    double s = 0.0;
+    for (int k = tid; k < LANES; k += NV) s += red[k];
+    out[tid] = s;
+  }
+}
+
+// first pass: block b sums the monitor values of its contiguous element range -> part[b][v]
+template <int NV>
+__global__ __launch_bounds__(kThreads) void monitorPartialKernel(const double* mon, int E, double* part,
+                                                                 const int* status) {
+  __shared__ double red[kThreads];
+  if (status[0]) return;
+  const int chunk = (E + kMonitorBlocks - 1) / kMonitorBlocks;
+  const int lo = blockIdx.x * chunk, hi = lo + chunk < E ? lo + chunk : E;
+  const int64_t n = hi > lo ? static_cast<int64_t>(hi - lo) * NV : 0;
+  blockVariableSums<NV>(mon + static_cast<int64_t>(lo) * NV, n, red, part + static_cast<int64_t>(blockIdx.x) * NV);
+}
+
+// second pass: the domain mean of each variable's monitor; all-NaN stops the run
+// (SystemControl.cpp:185-191)
+template <int NV>
+__global__ __launch_bounds__(kThreads) void monitorKernel(const double* part, int E, double* rel, int* status) {
+  __shared__ double red[kThreads];
+  __shared__ double sum[NV];
+  if (status[0]) return;
+  blockVariableSums<NV>(part, static_cast<int64_t>(kMonitorBlocks) * NV, red, sum);
+  __syncthreads();
+  if (threadIdx.x == 0) {
     bool all_nan = true;
     for (int v = 0; v < NV; v++) {
-      const double r = red[0][v] / static_cast<double>(E);
+      const double r = sum[v] / static_cast<double>(E);
       rel[v] = r;
       all_nan = all_nan && isnan(r);
     }

@@ -29,17 +29,49 @@
 
 namespace sdg {
 
-// full 3-D Gauss weight of node q
-__device__ __forceinline__ double nodeWeight(const TensorOps& t, int q, int N) {
-  return t.w[q % N] * t.w[(q / N) % N] * t.w[q / (N * N)];
-}
-
 template <int N>
 struct TensorShape {
   static constexpr int NPT = N * N * N;
   static constexpr int EPB = kThreads / NPT;  // elements per workgroup, one thread per node
   static constexpr int NV = 5, NG = 15, NFQ = N * N, NAQ = 6 * N * N;
 };
+
+// the order's 1-D operators and node / face-point maps, staged into LDS once per workgroup: the
+// kernels index them per lane, which from the kernel-argument segment or HBM would be vector
+// memory loads
+template <int N>
+struct TensorLds {
+  static constexpr int NPT = N * N * N, NAQ = 6 * N * N;
+  double B[N * N], Binv[N * N], Dn[N * N], lf[2 * N], wn[NPT], fwt[NAQ], kappa[2];
+  short iperm[NPT], slot_of[6 * NPT], base[NAQ], axis[6], side[6];
+};
+
+// all threads of the workgroup; the caller's next barrier publishes the tables
+template <int N>
+__device__ __forceinline__ void stageTables(const TensorOps& t, TensorLds<N>& L) {
+  constexpr int NPT = N * N * N, NAQ = 6 * N * N;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N * N; i += kThreads) {
+    L.B[i] = t.B[i];
+    L.Binv[i] = t.Binv[i];
+    L.Dn[i] = t.Dn[i];
+  }
+  for (int i = tid; i < NPT; i += kThreads) {
+    L.wn[i] = t.w[i % N] * t.w[(i / N) % N] * t.w[i / (N * N)];
+    L.iperm[i] = t.iperm[i];
+  }
+  for (int i = tid; i < 6 * NPT; i += kThreads) L.slot_of[i] = t.slot_of[i];
+  for (int i = tid; i < NAQ; i += kThreads) {
+    L.fwt[i] = t.fwt[i];
+    L.base[i] = t.base[i];
+  }
+  if (tid < 2 * N) L.lf[tid] = t.lf[tid];
+  if (tid < 6) {
+    L.axis[tid] = static_cast<short>(t.axis[tid]);
+    L.side[tid] = static_cast<short>(t.side[tid]);
+  }
+  if (tid < 2) L.kappa[tid] = t.kappa[tid];
+}
 
 __device__ __forceinline__ int axisStride(int axis, int N) { return axis == 0 ? 1 : (axis == 1 ? N : N * N); }
 __device__ __forceinline__ int nodeIndex(int q, int axis, int N) {
@@ -86,14 +118,14 @@ __device__ __forceinline__ void loadModal(const TensorOps& t, const StageArgs& a
 // traces of the nodal state sA at the element's own face points -> Uf_out (TC)
 // (rec: optional FP64 copy of the traces, same layout, for the adjoint's recorded run)
 template <typename TC, int N>
-__device__ __forceinline__ void writeTraces(const TensorOps& t, const TC* sA, TC* Uf_out, int64_t e, int le, int tid,
+__device__ __forceinline__ void writeTraces(const TensorLds<N>& L, const TC* sA, TC* Uf_out, int64_t e, int le, int tid,
                                             double* rec = nullptr) {
   constexpr int NPT = N * N * N, NV = 5, NFQ = N * N, NAQ = 6 * NFQ, EPB = TensorShape<N>::EPB;
   for (int idx = tid - le * NPT; idx < NAQ * NV; idx += NPT) {
     if (idx < 0) break;
     const int sl = idx % NAQ, v = idx / NAQ, s = sl / NFQ;
-    const int b = t.base[sl], st = axisStride(t.axis[s], N);
-    const double* l = t.lf + t.side[s] * N;
+    const int b = L.base[sl], st = axisStride(L.axis[s], N);
+    const double* l = L.lf + L.side[s] * N;
     TC acc = TC(0);
 #pragma unroll
     for (int i = 0; i < N; i++) acc += static_cast<TC>(l[i]) * sA[v * NPT + b + i * st];
@@ -124,7 +156,9 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
   static_assert(NV * NPT + NV * NAQ <= NG * NPT && NV * NPT <= NV * NAQ, "LDS regions do not nest");
   __shared__ TC sW[EPB][NG * NPT];
   __shared__ TC sB[EPB][NV * NAQ];
+  __shared__ TensorLds<N> sT;
   if (a.status && a.status[0]) return;
+  stageTables<N>(t, sT);
   const int tid = threadIdx.x, le = localElement<NPT>(tid), q = tid % NPT;
   const int64_t e = static_cast<int64_t>(blockIdx.x) * EPB + le;
   const bool ok = le < EPB && e < g.E;
@@ -166,11 +200,11 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
   }
   loadModal<TC, N>(t, a, a.U, e, Bt, q, ok);
   __syncthreads();
-  linePass<TC, N, NV>(t.B, 0, Bt, Y, q, ok);
+  linePass<TC, N, NV>(sT.B, 0, Bt, Y, q, ok);
   __syncthreads();
-  linePass<TC, N, NV>(t.B, 1, Y, Bt, q, ok);
+  linePass<TC, N, NV>(sT.B, 1, Y, Bt, q, ok);
   __syncthreads();
-  linePass<TC, N, NV>(t.B, 2, Bt, Y, q, ok);  // Y: nodal state
+  linePass<TC, N, NV>(sT.B, 2, Bt, Y, q, ok);  // Y: nodal state
   __syncthreads();
   if (ok) {
 #pragma unroll
@@ -211,7 +245,7 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
       for (int v = 0; v < NV; v++) {
         TC s = TC(0);
 #pragma unroll
-        for (int m = 0; m < N; m++) s += static_cast<TC>(t.Dn[i * N + m]) * Y[v * NPT + q0 + m * st];
+        for (int m = 0; m < N; m++) s += static_cast<TC>(sT.Dn[i * N + m]) * Y[v * NPT + q0 + m * st];
         dr[v][ax] = static_cast<double>(s);
       }
     }
@@ -227,9 +261,9 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
 #pragma unroll 1  // unrolled, the six faces' LDS loads are hoisted together and double the VGPRs
     for (int s = 0; s < 6; s++) {
       const int ax = t.axis[s], i = nodeIndex(q, ax, N);
-      const int sl = s * NFQ + t.slot_of[s * NPT + q];
+      const int sl = s * NFQ + sT.slot_of[s * NPT + q];
       // alpha, beta carry |J_f| w_t1 w_t2; M^-1 = 1 / (w_i w_t1 w_t2 |J|) at the node
-      const double coef = t.lf[t.side[s] * N + i] / (nodeWeight(t, q, N) * dj);
+      const double coef = sT.lf[t.side[s] * N + i] / (sT.wn[q] * dj);
       const int64_t es = e * NAQ + sl;
       const int nbs = g.snb[es];
       double n[D];
@@ -259,7 +293,7 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
     if (ok) {
       // modal c[k] = sum_i Binv[kx][i] Binv[ky][j] Binv[kz][l] w(i, j, l): direct triple sum (output path)
       const int kx = q % N, ky = (q / N) % N, kz = q / (N * N);
-      const int b = t.iperm[q];
+      const int b = sT.iperm[q];
 #pragma unroll 1
       for (int r = 0; r < NG; r++) {
         double s = 0.0;
@@ -267,7 +301,7 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
         for (int l = 0; l < N; l++)
           for (int j = 0; j < N; j++)
             for (int i = 0; i < N; i++)
-              s += t.Binv[kx * N + i] * t.Binv[ky * N + j] * t.Binv[kz * N + l] *
+              s += sT.Binv[kx * N + i] * sT.Binv[ky * N + j] * sT.Binv[kz * N + l] *
                    static_cast<double>(W[r * NPT + i + N * j + N * N * l]);
         a.Wout[(e * NPT + b) * NG + r] = s;
       }
@@ -277,47 +311,54 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
   // the face-gradient field (BR2: without the jump lifts; BR1: the total) -> sW for the traces
   if (ok)
     for (int r = 0; r < NG; r++) W[r * NPT + q] = static_cast<TC>(VISC == 2 ? wface[VISC == 2 ? r : 0] : wtot[r]);
-  // volume flux (F_c - F_v)^T (J^T)^-1 |J| w -> HBM Fv[e][(q D + b)][v]
+  // volume flux (F_c - F_v)^T (J^T)^-1 |J| w -> HBM Fv[e][b NV + v][node] (component-major: the
+  // stores coalesce across the wave and the residual kernel stages it into LDS unpermuted)
   if (ok) {
     double w[D + 3], pg[NG], fv[D * NV];
     computational<D>(ph, u, w);
     primitiveGradient<D>(ph, w, u, wtot, pg);
     viscousRawFlux<D>(ph, w, pg, fv);
-    const double jw = dj * nodeWeight(t, q, N);
+    const double jw = dj * sT.wn[q];
     double G[D * D], out[D * NV];
     for (int i = 0; i < D * D; i++) G[i] = gi[i] * jw;
     volumeFlux<TC, D, true>(a, w, fv, G, out);
     TC* Fv = pick<TC>(a.Fv, a.Fv32);
-    TC* o = Fv + (e * NPT + q) * D * NV;
-    for (int i = 0; i < D * NV; i++) o[i] = static_cast<TC>(out[i]);
+    TC* o = Fv + e * NPT * D * NV + q;
+    for (int i = 0; i < D * NV; i++) o[i * NPT] = static_cast<TC>(out[i]);
   }
   __syncthreads();
-  // traces of the face-gradient field (+ the own face's BR2 lift) -> Gf
+  // traces of the face-gradient field (+ the own face's BR2 lift) -> Gf, per (variable, slot): the
+  // slot's tables, normal and beta serve the variable's D components
   if (ok) {
     TC* Gf = pick<TC>(a.Gf, a.Gf32);
     const double idj = 1.0 / dj;
 #pragma unroll 1
-    for (int idx = q; idx < NAQ * NG; idx += NPT) {
-      const int sl = idx % NAQ, r = idx / NAQ, s = sl / NFQ;
-      const int b = t.base[sl], st = axisStride(t.axis[s], N);
-      const double* l = t.lf + t.side[s] * N;
-      TC acc = TC(0);
-#pragma unroll
-      for (int i = 0; i < N; i++) acc += static_cast<TC>(l[i]) * W[r * NPT + b + i * st];
-      double gv = static_cast<double>(acc);
-      if (VISC == 2) {  // + r_s at its own face: kappa_side beta n_d / |J|
+    for (int idx = q; idx < NAQ * NV; idx += NPT) {
+      const int sl = idx % NAQ, v = idx / NAQ, s = sl / NFQ;
+      const int b = sT.base[sl], st = axisStride(sT.axis[s], N);
+      const double* l = sT.lf + sT.side[s] * N;
+      double nb[D], kb = 0.0;
+      if constexpr (VISC == 2) {  // r_s at its own face: kappa_side beta n_d / |J|
         const int64_t es = e * NAQ + sl;
         const int nbs = g.snb[es];
-        double nd;
         if (nbs < 0) {
-          nd = g.bnormal[static_cast<int64_t>(-1 - nbs) * D + r % D];
+          for (int d = 0; d < D; d++) nb[d] = g.bnormal[static_cast<int64_t>(-1 - nbs) * D + d];
         } else {
           const int fps = g.sfp[es];
-          nd = ((fps & 1) ? -1.0 : 1.0) * g.fnormal[static_cast<int64_t>(fps >> 1) * D + r % D];
+          const double sg = (fps & 1) ? -1.0 : 1.0;
+          for (int d = 0; d < D; d++) nb[d] = sg * g.fnormal[static_cast<int64_t>(fps >> 1) * D + d];
         }
-        gv += t.kappa[t.side[s]] * static_cast<double>(Bt[(r / D) * NAQ + sl]) * nd * idj / t.fwt[sl];
+        kb = sT.kappa[sT.side[s]] * static_cast<double>(Bt[v * NAQ + sl]);
       }
-      Gf[(e * NG + r) * NAQ + sl] = static_cast<TC>(gv);
+      for (int d = 0; d < D; d++) {
+        const int r = v * D + d;
+        TC acc = TC(0);
+#pragma unroll
+        for (int i = 0; i < N; i++) acc += static_cast<TC>(l[i]) * W[r * NPT + b + i * st];
+        double gv = static_cast<double>(acc);
+        if constexpr (VISC == 2) gv += kb * nb[d] * idj / sT.fwt[sl];
+        Gf[(e * NG + r) * NAQ + sl] = static_cast<TC>(gv);
+      }
     }
   }
 }
@@ -336,7 +377,9 @@ __global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, Sta
   __shared__ TC sF[EPB][D * NV * NPT];  // contravariant volume flux [axis][v][node]; later FP64 scratch
   __shared__ double sHR[EPB][HB];
   __shared__ double sM[EPB][NV];        // monitor partials
+  __shared__ TensorLds<N> sT;
   if (a.status && a.status[0]) return;
+  stageTables<N>(t, sT);
   const int tid = threadIdx.x, le = localElement<NPT>(tid), q = tid % NPT;
   const int64_t e = static_cast<int64_t>(blockIdx.x) * EPB + le;
   const bool ok = le < EPB && e < g.E;
@@ -347,10 +390,7 @@ __global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, Sta
   const TC* Hs = pick<TC>(static_cast<const double*>(a.Hs), static_cast<const float*>(a.Hs32));
   if (ok) {
     const TC* src = Fv + e * NPT * D * NV;
-    for (int idx = q; idx < NPT * D * NV; idx += NPT) {  // HBM [node][axis][v] -> LDS [axis][v][node]
-      const int node = idx / (D * NV), rem = idx % (D * NV);
-      sF[le][rem * NPT + node] = src[idx];
-    }
+    for (int idx = q; idx < NPT * D * NV; idx += NPT) sF[le][idx] = src[idx];  // [axis][v][node]
     for (int idx = q; idx < NAQ * NV; idx += NPT) {  // HBM [v][slot] -> LDS [slot][v]
       const int v = idx / NAQ, sl = idx % NAQ;
       sH[sl * NV + v] = Hs[(e * NV + v) * NAQ + sl];
@@ -365,15 +405,15 @@ __global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, Sta
       for (int v = 0; v < NV; v++) {
         TC s = TC(0);
 #pragma unroll
-        for (int m = 0; m < N; m++) s += static_cast<TC>(t.Dn[m * N + i]) * sF[le][(ax * NV + v) * NPT + q0 + m * st];
+        for (int m = 0; m < N; m++) s += static_cast<TC>(sT.Dn[m * N + i]) * sF[le][(ax * NV + v) * NPT + q0 + m * st];
         R[v] += static_cast<double>(s);
       }
     }
 #pragma unroll 1
     for (int s = 0; s < 6; s++) {
       const int i = nodeIndex(q, t.axis[s], N);
-      const int sl = s * NFQ + t.slot_of[s * NPT + q];
-      const double l = t.lf[t.side[s] * N + i];
+      const int sl = s * NFQ + sT.slot_of[s * NPT + q];
+      const double l = sT.lf[t.side[s] * N + i];
       for (int v = 0; v < NV; v++) R[v] += l * static_cast<double>(sH[sl * NV + v]);
     }
     if constexpr (MIX) R[NV - 1] += a.mix.h0 * R[0];  // energy: add back h0 R_rho
@@ -395,22 +435,22 @@ __global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, Sta
     }
   };
   if (a.mode == 1 || a.last_stage) {
-    pass64(t.B, true, 0, SR, T0);
+    pass64(sT.B, true, 0, SR, T0);
     __syncthreads();
-    pass64(t.B, true, 1, T0, SR);
+    pass64(sT.B, true, 1, T0, SR);
     __syncthreads();
-    pass64(t.B, true, 2, SR, T0);  // T0: modal residual, tensor order
+    pass64(sT.B, true, 2, SR, T0);  // T0: modal residual, tensor order
     __syncthreads();
     if (a.mode == 1) {
       if (ok)
-        for (int v = 0; v < NV; v++) a.Rout[(e * NV + v) * NPT + t.iperm[q]] = T0[v * NPT + q];
+        for (int v = 0; v < NV; v++) a.Rout[(e * NV + v) * NPT + sT.iperm[q]] = T0[v * NPT + q];
       return;
     }
-    pass64(t.B, false, 0, T0, SR);
+    pass64(sT.B, false, 0, T0, SR);
     __syncthreads();
-    pass64(t.B, false, 1, SR, T0);
+    pass64(sT.B, false, 1, SR, T0);
     __syncthreads();
-    pass64(t.B, false, 2, T0, SR);  // SR: R V^T at the nodes
+    pass64(sT.B, false, 2, T0, SR);  // SR: R V^T at the nodes
     __syncthreads();
     if (le < EPB && q < NV) {
       double s = 0.0;
@@ -427,20 +467,20 @@ __global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, Sta
   }
   // nodal increment R_psi / (w |J|) -> modal with B^-1 along each axis (FP64)
   if (ok) {
-    const double jw = g.detj[e] * nodeWeight(t, q, N);
+    const double jw = g.detj[e] * sT.wn[q];
     for (int v = 0; v < NV; v++) SR[v * NPT + q] = R[v] / jw;
   }
   __syncthreads();
-  pass64(t.Binv, false, 0, SR, T0);
+  pass64(sT.Binv, false, 0, SR, T0);
   __syncthreads();
-  pass64(t.Binv, false, 1, T0, SR);
+  pass64(sT.Binv, false, 1, T0, SR);
   __syncthreads();
-  pass64(t.Binv, false, 2, SR, T0);  // T0: modal increment, tensor order
+  pass64(sT.Binv, false, 2, SR, T0);  // T0: modal increment, tensor order
   __syncthreads();
   // RK update (FP64) in the modal order; the new coefficients (TC operands, tensor order) -> X
   TC* X = reinterpret_cast<TC*>(SR);  // SR is free again
   if (ok) {
-    const int b = t.iperm[q];
+    const int b = sT.iperm[q];
     for (int v = 0; v < NV; v++) {
       const int64_t gi = (e * NV + v) * NPT + b;
       double u = a.U[gi] * a.c1;
@@ -461,11 +501,11 @@ __global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, Sta
   }
   __syncthreads();
   TC* Y = reinterpret_cast<TC*>(T0);
-  linePass<TC, N, NV>(t.B, 0, X, Y, q, ok);
+  linePass<TC, N, NV>(sT.B, 0, X, Y, q, ok);
   __syncthreads();
-  linePass<TC, N, NV>(t.B, 1, Y, X, q, ok);
+  linePass<TC, N, NV>(sT.B, 1, Y, X, q, ok);
   __syncthreads();
-  linePass<TC, N, NV>(t.B, 2, X, Y, q, ok);  // Y: new nodal state
+  linePass<TC, N, NV>(sT.B, 2, X, Y, q, ok);  // Y: new nodal state
   __syncthreads();
   double* rec_t = nullptr;
   if (a.rec_t) {
@@ -473,7 +513,7 @@ __global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, Sta
     const int stage = a.last_stage ? 0 : a.rec_stage + 1;
     if (step < a.rec_nsteps) rec_t = a.rec_t + (static_cast<int64_t>(step) * a.rec_S + stage) * a.rec_nt;
   }
-  if (le < EPB && e < g.E) writeTraces<TC, N>(t, Y, pick<TC>(a.Uf_out, a.Uf_out32), e, le, tid, rec_t);
+  if (le < EPB && e < g.E) writeTraces<TC, N>(sT, Y, pick<TC>(a.Uf_out, a.Uf_out32), e, le, tid, rec_t);
   (void)ph;
 }
 
@@ -485,7 +525,9 @@ __global__ __launch_bounds__(kThreads) void tensorTraceKernel(Geo g, const doubl
   constexpr int NPT = S::NPT, EPB = S::EPB, NV = S::NV;
   __shared__ TC sX[EPB][NV * NPT];
   __shared__ TC sY[EPB][NV * NPT];
-  const int tid = threadIdx.x, le = tid / NPT, q = tid % NPT;
+  __shared__ TensorLds<N> sT;
+  stageTables<N>(t, sT);
+  const int tid = threadIdx.x, le = localElement<NPT>(tid), q = tid % NPT;
   const int64_t e = static_cast<int64_t>(blockIdx.x) * EPB + le;
   const bool ok = le < EPB && e < g.E;
   const int lec = le < EPB ? le : 0;
@@ -502,13 +544,13 @@ __global__ __launch_bounds__(kThreads) void tensorTraceKernel(Geo g, const doubl
     }
   }
   __syncthreads();
-  linePass<TC, N, NV>(t.B, 0, X, Y, q, ok);
+  linePass<TC, N, NV>(sT.B, 0, X, Y, q, ok);
   __syncthreads();
-  linePass<TC, N, NV>(t.B, 1, Y, X, q, ok);
+  linePass<TC, N, NV>(sT.B, 1, Y, X, q, ok);
   __syncthreads();
-  linePass<TC, N, NV>(t.B, 2, X, Y, q, ok);
+  linePass<TC, N, NV>(sT.B, 2, X, Y, q, ok);
   __syncthreads();
-  if (ok) writeTraces<TC, N>(t, Y, static_cast<TC*>(Uf_out), e, le, tid);
+  if (ok) writeTraces<TC, N>(sT, Y, static_cast<TC*>(Uf_out), e, le, tid);
 }
 
 }  // namespace sdg

@@ -149,7 +149,11 @@ Launchers makeLaunchers() {
                        dim3(sdg::kThreads), 0, st, g, ph, U, me, cfl, order, out);
   };
   l.monitor = [](hipStream_t st, const double* mon, int E, double* rel, int* status) {
-    hipLaunchKernelGGL((sdg::monitorKernel<D + 2>), dim3(1), dim3(sdg::kThreads), 0, st, mon, E, rel, status);
+    // mon holds E x NV per-element values followed by kMonitorBlocks x NV partial sums
+    double* part = const_cast<double*>(mon) + static_cast<int64_t>(E) * (D + 2);
+    hipLaunchKernelGGL((sdg::monitorPartialKernel<D + 2>), dim3(sdg::kMonitorBlocks), dim3(sdg::kThreads), 0, st, mon,
+                       E, part, status);
+    hipLaunchKernelGGL((sdg::monitorKernel<D + 2>), dim3(1), dim3(sdg::kThreads), 0, st, part, E, rel, status);
   };
   return l;
 }
@@ -1025,7 +1029,7 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
                  : (allocate(ctx, nUf, &ctx->Uf[0]) || allocate(ctx, nUf, &ctx->Uf[1])))
     return bail(SDG_E_NOMEM);
   if (allocate(ctx, nU, &ctx->U) || allocate(ctx, nU, &ctx->Un) || allocate(ctx, nU, &ctx->Rout) ||
-      allocate(ctx, static_cast<size_t>(E) * ctx->nv, &ctx->mon) || allocate(ctx, static_cast<size_t>(ctx->nv), &ctx->rel) ||
+      allocate(ctx, (static_cast<size_t>(E) + sdg::kMonitorBlocks) * ctx->nv, &ctx->mon) || allocate(ctx, static_cast<size_t>(ctx->nv), &ctx->rel) ||
       allocate(ctx, static_cast<size_t>(E), &ctx->dtbuf) || allocate(ctx, size_t(2), &ctx->status))
     return bail(SDG_E_NOMEM);
   if (ctx->ns()) {
