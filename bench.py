@@ -85,6 +85,30 @@ def algorithmic_counts(mesh, ns=True):
     return f_grad, f_res, b_alg
 
 
+def tensor_path(mesh) -> bool:
+    """True where the engine runs the sum-factorised Navier-Stokes kernels (hexahedra P2-P4)."""
+    return mesh.elem == L.ELEM_HEXAHEDRON and 2 <= mesh.order <= 4 and os.environ.get("SDG_DENSE") != "1"
+
+
+def tensor_bytes(mesh, tc: int):
+    """Algorithmic HBM bytes per element and launch of the three sum-factorised Navier-Stokes stage
+    kernels (tc: bytes of the contraction type, 8 FP64 / 4 mixed; the state is FP64 in both):
+      gradient: state U + own and neighbour face traces + volume flux Fv + gradient traces Gf
+                + geometry (|J|, J^-1, two slot maps, and half of each face point's normal and
+                weight, which the neighbour shares);
+      face flux (per element, its half of the face points): both sides' traces and gradient traces,
+                normal, weight, point map, and the two signed flux slots written;
+      residual: Fv + flux slots + U + U_n read, U + next traces written, U_n written at the last of
+                the 3 stages (1/3 per launch), |J|."""
+    nv, nq, naq, D = mesh.nv, mesh.nq, mesh.nf * mesh.nfq, mesh.dim
+    ng = nv * D
+    geo = 8 + 8 * D * D + naq * 8 + naq * (8 * D + 8) / 2
+    grad = 8 * nv * nq + 2 * nv * naq * tc + D * nv * nq * tc + ng * naq * tc + geo
+    face = naq / 2 * (2 * (nv + ng) * tc + 8 * D + 8 + 4 + 2 * nv * tc)
+    res = D * nv * nq * tc + nv * naq * tc + 3 * 8 * nv * nq + nv * naq * tc + 8 * nv * nq / 3 + 8
+    return grad, face, res
+
+
 def cpu_baseline(order: int, target_s: float = 15.0):
     """The CPU oracle (C++/OpenMP restatement of the reference) on a bounded sample of the same
     workload: an 8^3 mesh of the same element, order, physics and scheme."""
@@ -187,7 +211,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 matrix-core contractions, f64 state/geometry/physics/RK",
+        "dtype": "f32 contractions and work buffers (sum-factorised line passes), f64 state/geometry/"
+                 "physics/RK",
         "data": "synthetic",
         "config": {"workload": f"3-D compressible NS Taylor-Green vortex, periodic [0,2pi]^3, {args.n}^3 hex, "
                                f"P{args.order}, Re 1600, Ma 0.1, HLLC + BR2, SSP-RK3, CFL {args.cfl}",
@@ -410,12 +435,25 @@ def timed_run(mode, pd, mesh, prim, args):
 
 
 def roofline(kms, f_grad, f_res, mesh, args, mode):
-    """Roofline entry of the dominant kernel: algorithmic flops per launch / HIP-event duration."""
+    """Roofline entry of the dominant kernel (kms: HIP-event averages of the gradient, face-flux,
+    residual and monitor kernels). Sum-factorised path: HBM-bound, algorithmic bytes per launch /
+    duration. Dense matrix-core path: algorithmic flops per launch / duration."""
+    if tensor_path(mesh):
+        per = dict(zip(("nsGradTensor", "nsFaceFluxKernel", "nsResidualTensor"),
+                       tensor_bytes(mesh, 8 if mode == "fp64" else 4)))
+        names = list(per)
+        k = int(np.argmax(kms[:3]))
+        name, ms = names[k], kms[k]
+        achieved = per[name] * mesh.E / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS, "traffic": PROFILED_TRAFFIC.get((name, mode, args.n, args.order)),
+                "kernel": name, "algorithmic_bytes_per_element": per[name], "kernel_ms": ms,
+                "kernel_ms_all": kms.tolist(), "kernel_ms_names": names + ["monitor"]}
     peak = PEAK_FP64_TFLOPS if mode == "fp64" else PEAK_FP32_MFMA_TFLOPS
-    if kms[0] >= kms[1]:
+    if kms[0] >= kms[1] + kms[2]:
         name, ms, flops = "nsGradMfma", kms[0], f_grad
     else:
-        name, ms, flops = "nsFaceFluxKernel+nsResidualMfma", kms[1], f_res
+        name, ms, flops = "nsFaceFluxKernel+nsResidualMfma", kms[1] + kms[2], f_res
     achieved = flops * mesh.E / (ms * 1e-3) / 1e12
     return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
             "traffic": PROFILED_TRAFFIC.get((name, mode, args.n, args.order)), "kernel": name,

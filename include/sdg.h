@@ -274,10 +274,11 @@ int sdg_synchronize(sdg_ctx* ctx);
  * kernel eagerly (no graph) with HIP events around each launch on the context stream, and write
  * the average duration in milliseconds of each kernel class into avg_ms[0..n_out):
  *   [0] stage kernel 1 (Euler: the whole stage; Navier-Stokes: gradient lifting + volume flux)
- *   [1] stage kernel 2 (Navier-Stokes: face fluxes + residual + update; Euler: 0)
- *   [2] residual-monitor reduction
+ *   [1] Navier-Stokes face-flux kernel (Euler: 0)
+ *   [2] Navier-Stokes residual + update + next traces (Euler: 0)
+ *   [3] residual-monitor reduction
  * The state advances exactly as with sdg_step. */
-#define SDG_KERNEL_CLASSES 3
+#define SDG_KERNEL_CLASSES 4
 int sdg_time_kernels(sdg_ctx* ctx, double dt, int nsteps, double* avg_ms, int n_out);
 
 /* ---- raw state files (checkpoint / restart) ----------------------------------------------------

@@ -276,6 +276,7 @@ class Mesh:
         self.nfq = info.face_quadrature
         self.Fb = info.boundary_faces
         self.nv = self.dim + 2
+        self.elem = info.element
         self.mixed = info.element == ELEM_MIXED_TRIANGLE_QUADRANGLE
 
     def field(self, name: str) -> np.ndarray:
@@ -388,8 +389,8 @@ class Solver:
         return ms.value
 
     def time_kernels(self, dt: float, nsteps: int) -> np.ndarray:
-        out = np.zeros(3)
-        self._check(self.lib.sdg_time_kernels(self.handle, dt, nsteps, out.ctypes.data, 3))
+        out = np.zeros(4)
+        self._check(self.lib.sdg_time_kernels(self.handle, dt, nsteps, out.ctypes.data, 4))
         return out
 
     def synchronize(self):

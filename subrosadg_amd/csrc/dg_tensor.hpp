@@ -73,6 +73,22 @@ __device__ __forceinline__ void stageTables(const TensorOps& t, TensorLds<N>& L)
   if (tid < 2) L.kappa[tid] = t.kappa[tid];
 }
 
+// barrier between the phases of one element's work: at NPT = 64 an element is exactly one
+// wavefront that touches only its own LDS region (the shared tables are read-only by then), so
+// the lanes of the wavefront need only see each other's LDS writes, which wavefront-scope fences
+// give without waiting for the other wavefronts of the workgroup; otherwise elements straddle
+// wavefronts and the workgroup barrier is needed
+template <int NPT>
+__device__ __forceinline__ void elemSync() {
+  if constexpr (NPT == 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 __device__ __forceinline__ int axisStride(int axis, int N) { return axis == 0 ? 1 : (axis == 1 ? N : N * N); }
 __device__ __forceinline__ int nodeIndex(int q, int axis, int N) {
   return axis == 0 ? q % N : (axis == 1 ? (q / N) % N : q / (N * N));
@@ -201,11 +217,11 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
   loadModal<TC, N>(t, a, a.U, e, Bt, q, ok);
   __syncthreads();
   linePass<TC, N, NV>(sT.B, 0, Bt, Y, q, ok);
-  __syncthreads();
+  elemSync<NPT>();
   linePass<TC, N, NV>(sT.B, 1, Y, Bt, q, ok);
-  __syncthreads();
+  elemSync<NPT>();
   linePass<TC, N, NV>(sT.B, 2, Bt, Y, q, ok);  // Y: nodal state
-  __syncthreads();
+  elemSync<NPT>();
   if (ok) {
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
@@ -235,7 +251,7 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
         Bt[idx] = static_cast<TC>(Jb[dm] / nd);
       }
   }
-  __syncthreads();
+  elemSync<NPT>();
   // gradient at the node: J^-T (Dn u along each axis) + the face corrections
   double u[NV], wtot[NG], wface[VISC == 2 ? NG : 1];
   if (ok) {
@@ -284,12 +300,12 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
       }
     }
   }
-  __syncthreads();  // Y and alpha are read; sW takes the gradient fields from here on
+  elemSync<NPT>();  // Y and alpha are read; sW takes the gradient fields from here on
   // optional: modal coefficients of the total lifted gradient (sdg_get_gradient), B^-1 per axis
   if (a.Wout) {
     if (ok)
       for (int r = 0; r < NG; r++) W[r * NPT + q] = static_cast<TC>(wtot[r]);
-    __syncthreads();
+    elemSync<NPT>();
     if (ok) {
       // modal c[k] = sum_i Binv[kx][i] Binv[ky][j] Binv[kz][l] w(i, j, l): direct triple sum (output path)
       const int kx = q % N, ky = (q / N) % N, kz = q / (N * N);
@@ -306,7 +322,7 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
         a.Wout[(e * NPT + b) * NG + r] = s;
       }
     }
-    __syncthreads();
+    elemSync<NPT>();
   }
   // the face-gradient field (BR2: without the jump lifts; BR1: the total) -> sW for the traces
   if (ok)
@@ -326,7 +342,7 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
     TC* o = Fv + e * NPT * D * NV + q;
     for (int i = 0; i < D * NV; i++) o[i * NPT] = static_cast<TC>(out[i]);
   }
-  __syncthreads();
+  elemSync<NPT>();
   // traces of the face-gradient field (+ the own face's BR2 lift) -> Gf, per (variable, slot): the
   // slot's tables, normal and beta serve the variable's D components
   if (ok) {
@@ -388,7 +404,15 @@ __global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, Sta
   double* SR = sHR[lec];
   const TC* Fv = pick<TC>(static_cast<const double*>(a.Fv), static_cast<const float*>(a.Fv32));
   const TC* Hs = pick<TC>(static_cast<const double*>(a.Hs), static_cast<const float*>(a.Hs32));
+  // the state and step-start state for the update, gathered early so that their latency overlaps
+  // the residual (modal index of this thread's tensor position)
+  double pu[NV], pn[NV];
   if (ok) {
+    const int b = t.iperm[q];
+    for (int v = 0; v < NV; v++) {
+      pu[v] = a.U[(e * NV + v) * NPT + b];
+      pn[v] = a.Un[(e * NV + v) * NPT + b];
+    }
     const TC* src = Fv + e * NPT * D * NV;
     for (int idx = q; idx < NPT * D * NV; idx += NPT) sF[le][idx] = src[idx];  // [axis][v][node]
     for (int idx = q; idx < NAQ * NV; idx += NPT) {  // HBM [v][slot] -> LDS [slot][v]
@@ -418,10 +442,12 @@ __global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, Sta
     }
     if constexpr (MIX) R[NV - 1] += a.mix.h0 * R[0];  // energy: add back h0 R_rho
   }
-  __syncthreads();  // the face-flux slots are read; their region takes the nodal residual
-  if (ok)
-    for (int v = 0; v < NV; v++) SR[v * NPT + q] = R[v];
-  __syncthreads();
+  elemSync<NPT>();  // the face-flux slots are read; their region takes the nodal residual
+  const bool monitor = a.mode == 1 || a.last_stage;
+  const double jw = ok ? g.detj[e] * sT.wn[q] : 1.0;
+  if (ok)  // residual R_psi for the monitor stages, else directly the nodal increment R_psi / (w |J|)
+    for (int v = 0; v < NV; v++) SR[v * NPT + q] = monitor ? R[v] : R[v] / jw;
+  elemSync<NPT>();
   // the reference residual (before M^-1) is (B^T x B^T x B^T) R_psi (modal); its monitor is
   // mean_q |R V^T| = mean over nodes of |(B x B x B) of it| (TimeIntegration.cpp:279-298)
   double* T0 = reinterpret_cast<double*>(sF[lec]);  // FP64 scratch: NV NPT doubles fit in sF
@@ -434,57 +460,52 @@ __global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, Sta
       out[v * NPT + q] = s;
     }
   };
-  if (a.mode == 1 || a.last_stage) {
+  if (monitor) {
     pass64(sT.B, true, 0, SR, T0);
-    __syncthreads();
+    elemSync<NPT>();
     pass64(sT.B, true, 1, T0, SR);
-    __syncthreads();
+    elemSync<NPT>();
     pass64(sT.B, true, 2, SR, T0);  // T0: modal residual, tensor order
-    __syncthreads();
+    elemSync<NPT>();
     if (a.mode == 1) {
       if (ok)
         for (int v = 0; v < NV; v++) a.Rout[(e * NV + v) * NPT + sT.iperm[q]] = T0[v * NPT + q];
       return;
     }
     pass64(sT.B, false, 0, T0, SR);
-    __syncthreads();
+    elemSync<NPT>();
     pass64(sT.B, false, 1, SR, T0);
-    __syncthreads();
+    elemSync<NPT>();
     pass64(sT.B, false, 2, T0, SR);  // SR: R V^T at the nodes
-    __syncthreads();
+    elemSync<NPT>();
     if (le < EPB && q < NV) {
       double s = 0.0;
       for (int k = 0; k < NPT; k++) s += fabs(SR[q * NPT + k]);
       sM[le][q] = s;
     }
-    __syncthreads();
+    elemSync<NPT>();
     if (ok && q < NV) a.mon[e * NV + q] = sM[le][q] / NPT;
-    // rebuild the nodal residual from the modal one (T0 was overwritten): R_psi is needed below
-    __syncthreads();
+    // the nodal increment R_psi / (w |J|) from the registers (SR was overwritten)
+    elemSync<NPT>();
     if (ok)
-      for (int v = 0; v < NV; v++) SR[v * NPT + q] = R[v];
-    __syncthreads();
+      for (int v = 0; v < NV; v++) SR[v * NPT + q] = R[v] / jw;
+    elemSync<NPT>();
   }
-  // nodal increment R_psi / (w |J|) -> modal with B^-1 along each axis (FP64)
-  if (ok) {
-    const double jw = g.detj[e] * sT.wn[q];
-    for (int v = 0; v < NV; v++) SR[v * NPT + q] = R[v] / jw;
-  }
-  __syncthreads();
+  // nodal increment -> modal with B^-1 along each axis (FP64)
   pass64(sT.Binv, false, 0, SR, T0);
-  __syncthreads();
+  elemSync<NPT>();
   pass64(sT.Binv, false, 1, T0, SR);
-  __syncthreads();
+  elemSync<NPT>();
   pass64(sT.Binv, false, 2, SR, T0);  // T0: modal increment, tensor order
-  __syncthreads();
+  elemSync<NPT>();
   // RK update (FP64) in the modal order; the new coefficients (TC operands, tensor order) -> X
   TC* X = reinterpret_cast<TC*>(SR);  // SR is free again
   if (ok) {
     const int b = sT.iperm[q];
     for (int v = 0; v < NV; v++) {
       const int64_t gi = (e * NV + v) * NPT + b;
-      double u = a.U[gi] * a.c1;
-      u += a.c0 * a.Un[gi];
+      double u = pu[v] * a.c1;
+      u += a.c0 * pn[v];
       u += a.c2dt * T0[v * NPT + q];
       a.U[gi] = u;
       if (a.last_stage) a.Un[gi] = u;
@@ -499,14 +520,14 @@ __global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, Sta
         X[v * NPT + q] = u;
     }
   }
-  __syncthreads();
+  elemSync<NPT>();
   TC* Y = reinterpret_cast<TC*>(T0);
   linePass<TC, N, NV>(sT.B, 0, X, Y, q, ok);
-  __syncthreads();
+  elemSync<NPT>();
   linePass<TC, N, NV>(sT.B, 1, Y, X, q, ok);
-  __syncthreads();
+  elemSync<NPT>();
   linePass<TC, N, NV>(sT.B, 2, X, Y, q, ok);  // Y: new nodal state
-  __syncthreads();
+  elemSync<NPT>();
   double* rec_t = nullptr;
   if (a.rec_t) {
     const int step = a.status[1] + (a.last_stage ? 1 : 0);
@@ -545,11 +566,11 @@ __global__ __launch_bounds__(kThreads) void tensorTraceKernel(Geo g, const doubl
   }
   __syncthreads();
   linePass<TC, N, NV>(sT.B, 0, X, Y, q, ok);
-  __syncthreads();
+  elemSync<NPT>();
   linePass<TC, N, NV>(sT.B, 1, Y, X, q, ok);
-  __syncthreads();
+  elemSync<NPT>();
   linePass<TC, N, NV>(sT.B, 2, X, Y, q, ok);
-  __syncthreads();
+  elemSync<NPT>();
   if (ok) writeTraces<TC, N>(sT, Y, static_cast<TC*>(Uf_out), e, le, tid);
 }
 

@@ -505,7 +505,8 @@ StageArgs baseArgs(sdg_ctx* ctx) {
 }
 
 // Enqueue one stage (mode 0: update, 1: residual only). Advances ctx->cur for mode 0. With ev
-// (four events), records ev[0]/ev[1] around the first kernel and ev[2]/ev[3] around the second.
+// (six events), records ev[0]/ev[1] around the first kernel, ev[2]/ev[3] around the face-flux
+// kernel and ev[4]/ev[5] around the residual kernel (empty intervals where a path has none).
 void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hipEvent_t* ev = nullptr) {
   StageArgs a = baseArgs(ctx);
   double c[3];
@@ -537,15 +538,16 @@ void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hi
       (void)hipEventRecord(ev[2], ctx->stream);
     }
     if (ctx->launch.ns_face) ctx->launch.ns_face(ctx->stream, ctx->geo, ctx->phys, a);
+    if (ev) {
+      (void)hipEventRecord(ev[3], ctx->stream);
+      (void)hipEventRecord(ev[4], ctx->stream);
+    }
     ctx->launch.ns_flux(ctx->stream, ctx->geo, ctx->phys, a);
-    if (ev) (void)hipEventRecord(ev[3], ctx->stream);
+    if (ev) (void)hipEventRecord(ev[5], ctx->stream);
   } else {
     ctx->launch.euler(ctx->stream, ctx->geo, ctx->phys, a);
-    if (ev) {
-      (void)hipEventRecord(ev[1], ctx->stream);
-      (void)hipEventRecord(ev[2], ctx->stream);
-      (void)hipEventRecord(ev[3], ctx->stream);
-    }
+    if (ev)
+      for (int k = 1; k < 6; k++) (void)hipEventRecord(ev[k], ctx->stream);
   }
   if (mode == 0) ctx->cur ^= 1;
 }
@@ -1260,7 +1262,7 @@ int sdg_time_kernels(sdg_ctx* ctx, double dt, int nsteps, double* avg_ms, int n_
     return fail(ctx, SDG_E_STATE, "sdg_time_kernels: boundary data not set (sdg_set_boundary_primitive)");
   if (nsteps < 1 || !(dt >= 0.0)) return fail(ctx, SDG_E_INVALID, "sdg_time_kernels: bad dt or step count");
   const int st = ctx->stages();
-  const size_t per_step = static_cast<size_t>(4 * st + 2);
+  const size_t per_step = static_cast<size_t>(6 * st + 2);
   std::vector<hipEvent_t> ev(per_step * nsteps, nullptr);
   for (auto& e : ev)
     if (hipEventCreate(&e) != hipSuccess) {
@@ -1271,31 +1273,32 @@ int sdg_time_kernels(sdg_ctx* ctx, double dt, int nsteps, double* avg_ms, int n_
   hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
   for (int i = 0; i < nsteps; i++) {
     hipEvent_t* e = &ev[per_step * i];
-    for (int s = 0; s < st; s++) enqueueStage(ctx, s, dt, 0, nullptr, e + 4 * s);
-    (void)hipEventRecord(e[4 * st], ctx->stream);
+    for (int s = 0; s < st; s++) enqueueStage(ctx, s, dt, 0, nullptr, e + 6 * s);
+    (void)hipEventRecord(e[6 * st], ctx->stream);
     ctx->launch.monitor(ctx->stream, ctx->mon, ctx->E, ctx->rel, ctx->status);
-    (void)hipEventRecord(e[4 * st + 1], ctx->stream);
+    (void)hipEventRecord(e[6 * st + 1], ctx->stream);
   }
   hipError_t err = hipStreamSynchronize(ctx->stream);
-  double sum[SDG_KERNEL_CLASSES] = {0.0, 0.0, 0.0};
+  double sum[SDG_KERNEL_CLASSES] = {0.0, 0.0, 0.0, 0.0};
   for (int i = 0; i < nsteps && err == hipSuccess; i++) {
     hipEvent_t* e = &ev[per_step * i];
-    for (int s = 0; s < st; s++) {
-      float a = 0.0f, b = 0.0f;
-      (void)hipEventElapsedTime(&a, e[4 * s], e[4 * s + 1]);
-      (void)hipEventElapsedTime(&b, e[4 * s + 2], e[4 * s + 3]);
-      sum[0] += a;
-      sum[1] += b;
-    }
+    for (int s = 0; s < st; s++)
+      for (int k = 0; k < 3; k++) {
+        float x = 0.0f;
+        (void)hipEventElapsedTime(&x, e[6 * s + 2 * k], e[6 * s + 2 * k + 1]);
+        sum[k] += x;
+      }
     float m = 0.0f;
-    (void)hipEventElapsedTime(&m, e[4 * st], e[4 * st + 1]);
-    sum[2] += m;
+    (void)hipEventElapsedTime(&m, e[6 * st], e[6 * st + 1]);
+    sum[3] += m;
   }
   for (auto& x : ev) (void)hipEventDestroy(x);
   if (err != hipSuccess) return fail(ctx, SDG_E_DEVICE, std::string("sdg_time_kernels: ") + hipGetErrorString(err));
-  const double launches[SDG_KERNEL_CLASSES] = {double(nsteps) * st, double(nsteps) * st, double(nsteps)};
+  const double launches[SDG_KERNEL_CLASSES] = {double(nsteps) * st, double(nsteps) * st, double(nsteps) * st,
+                                               double(nsteps)};
   for (int k = 0; k < n_out; k++) avg_ms[k] = k < SDG_KERNEL_CLASSES ? sum[k] / launches[k] : 0.0;
-  if (!ctx->ns() && n_out > 1) avg_ms[1] = 0.0;
+  if (!ctx->ns())
+    for (int k = 1; k < n_out && k < 3; k++) avg_ms[k] = 0.0;
   return SDG_OK;
 }
 
