@@ -78,6 +78,10 @@ struct Geo {
   const int* fseqinv;     // F x N_fq       left point of right point k
   const int* snb;         // E x N_aq       neighbour slot: neighbour element * N_aq + its slot
   const int* sfp;         // E x N_aq       2 (f N_fq + p) + side: face point of the slot and side
+  // sum-factorised kernels only (else null): per element slot, the outward normal of the element
+  // at the slot's face point and the point's |J_f| w, so that they are one load away
+  const double* snrm;     // E x N_aq x D
+  const double* sjw;      // E x N_aq
   const int* fpar;        // F x 2          left, right parent element
   const int* floc;        // F x 2          local face index in left, right parent
   // boundary face points bp = b N_fq + p (a slot on a boundary face has snb = -1 - bp)

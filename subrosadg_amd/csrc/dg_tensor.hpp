@@ -209,7 +209,7 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
         if (nbs >= 0) {
           fo[k] = Uf_in[(e * NV + v) * NAQ + sl];
           fn[k] = Uf_in[(static_cast<int64_t>(nbs / NAQ) * NV + v) * NAQ + nbs % NAQ];
-          fw[k] = g.fjw[g.sfp[es] >> 1];
+          fw[k] = g.sjw[es];
         }
       }
     }
@@ -281,15 +281,8 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
       // alpha, beta carry |J_f| w_t1 w_t2; M^-1 = 1 / (w_i w_t1 w_t2 |J|) at the node
       const double coef = sT.lf[t.side[s] * N + i] / (sT.wn[q] * dj);
       const int64_t es = e * NAQ + sl;
-      const int nbs = g.snb[es];
       double n[D];
-      if (nbs < 0) {
-        for (int d = 0; d < D; d++) n[d] = g.bnormal[static_cast<int64_t>(-1 - nbs) * D + d];
-      } else {
-        const int fps = g.sfp[es];
-        const double sg = (fps & 1) ? -1.0 : 1.0;
-        for (int d = 0; d < D; d++) n[d] = sg * g.fnormal[static_cast<int64_t>(fps >> 1) * D + d];
-      }
+      for (int d = 0; d < D; d++) n[d] = g.snrm[es * D + d];
       for (int v = 0; v < NV; v++) {
         const double al = static_cast<double>(Al[v * NAQ + sl]) * coef;
         const double be = static_cast<double>(Bt[v * NAQ + sl]) * coef;
@@ -356,14 +349,7 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
       double nb[D], kb = 0.0;
       if constexpr (VISC == 2) {  // r_s at its own face: kappa_side beta n_d / |J|
         const int64_t es = e * NAQ + sl;
-        const int nbs = g.snb[es];
-        if (nbs < 0) {
-          for (int d = 0; d < D; d++) nb[d] = g.bnormal[static_cast<int64_t>(-1 - nbs) * D + d];
-        } else {
-          const int fps = g.sfp[es];
-          const double sg = (fps & 1) ? -1.0 : 1.0;
-          for (int d = 0; d < D; d++) nb[d] = sg * g.fnormal[static_cast<int64_t>(fps >> 1) * D + d];
-        }
+        for (int d = 0; d < D; d++) nb[d] = g.snrm[es * D + d];
         kb = sT.kappa[sT.side[s]] * static_cast<double>(Bt[v * NAQ + sl]);
       }
       for (int d = 0; d < D; d++) {

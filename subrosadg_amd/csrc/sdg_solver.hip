@@ -836,6 +836,25 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   }
   ctx->Fb = Fb;
   Geo& g = ctx->geo;
+  if (ctx->launch.tensor) {  // per slot normal and face weight for the sum-factorised kernels
+    std::vector<double> snrm(static_cast<size_t>(E) * naq * D), sjw(static_cast<size_t>(E) * naq);
+    for (size_t o = 0; o < sjw.size(); o++) {
+      if (snb[o] < 0) {
+        const size_t bp = static_cast<size_t>(-1 - snb[o]);
+        for (int d = 0; d < D; d++) snrm[o * D + d] = bnormal[bp * D + d];
+        sjw[o] = bjw[bp];
+      } else {
+        const size_t fp = static_cast<size_t>(sfp[o] >> 1);
+        const double sg = (sfp[o] & 1) ? -1.0 : 1.0;
+        for (int d = 0; d < D; d++) snrm[o * D + d] = sg * fnormal[fp * D + d];
+        sjw[o] = fjw[fp];
+      }
+    }
+    double *dsn, *dsj;
+    if (upload(ctx, snrm, &dsn) || upload(ctx, sjw, &dsj)) return bail(SDG_E_NOMEM);
+    g.snrm = dsn;
+    g.sjw = dsj;
+  }
   double *dVt, *ddV, *dVft, *dVf, *dMinv, *dLf, *dwq, *dginv, *ddetj, *dnormal, *djw, *dme;
   int *dseq, *dseqinv, *dfpar, *dfloc, *dsnb, *dsfp, *dbtype, *dbslot;
   double *dbnormal, *dbjw;
