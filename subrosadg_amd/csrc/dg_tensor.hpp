@@ -32,7 +32,10 @@ namespace sdg {
 template <int N>
 struct TensorShape {
   static constexpr int NPT = N * N * N;
-  static constexpr int EPB = kThreads / NPT;  // elements per workgroup, one thread per node
+  // threads per workgroup (one thread per node, as many elements as fit; one-element workgroups
+  // at N = 4 measured slower: the order tables are then staged once per element)
+  static constexpr int TPB = kThreads;
+  static constexpr int EPB = TPB / NPT;
   static constexpr int NV = 5, NG = 15, NFQ = N * N, NAQ = 6 * N * N;
 };
 
@@ -51,17 +54,18 @@ template <int N>
 __device__ __forceinline__ void stageTables(const TensorOps& t, TensorLds<N>& L) {
   constexpr int NPT = N * N * N, NAQ = 6 * N * N;
   const int tid = threadIdx.x;
-  for (int i = tid; i < N * N; i += kThreads) {
+  constexpr int TPB = TensorShape<N>::TPB;
+  for (int i = tid; i < N * N; i += TPB) {
     L.B[i] = t.B[i];
     L.Binv[i] = t.Binv[i];
     L.Dn[i] = t.Dn[i];
   }
-  for (int i = tid; i < NPT; i += kThreads) {
+  for (int i = tid; i < NPT; i += TPB) {
     L.wn[i] = t.w[i % N] * t.w[(i / N) % N] * t.w[i / (N * N)];
     L.iperm[i] = t.iperm[i];
   }
-  for (int i = tid; i < 6 * NPT; i += kThreads) L.slot_of[i] = t.slot_of[i];
-  for (int i = tid; i < NAQ; i += kThreads) {
+  for (int i = tid; i < 6 * NPT; i += TPB) L.slot_of[i] = t.slot_of[i];
+  for (int i = tid; i < NAQ; i += TPB) {
     L.fwt[i] = t.fwt[i];
     L.base[i] = t.base[i];
   }
@@ -165,7 +169,7 @@ __device__ __forceinline__ int localElement(int tid) {
 //   sW [NG NPT]:  nodal state Y [v][node] + alpha [v][slot]  ->  the face-gradient field [r][node]
 //   sB [NV NAQ]:  the basis-pass panel (modal, then axis-0/1 partial sums)  ->  beta [v][slot]
 template <typename TC, int N, int VISC>
-__global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageArgs a) {
+__global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys ph, StageArgs a) {
   const TensorOps& t = g.tens;
   using S = TensorShape<N>;
   constexpr int NPT = S::NPT, EPB = S::EPB, NV = S::NV, NG = S::NG, NFQ = S::NFQ, NAQ = S::NAQ, D = 3;
@@ -274,15 +278,24 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
         if constexpr (VISC == 2) wface[v * D + d] = s;
       }
     }
+    // the outward normal at the node's face point, loaded one face ahead of its use
+    double nx[D];
+    {
+      const int64_t es = e * NAQ + sT.slot_of[q];
+      for (int d = 0; d < D; d++) nx[d] = g.snrm[es * D + d];
+    }
 #pragma unroll 1  // unrolled, the six faces' LDS loads are hoisted together and double the VGPRs
     for (int s = 0; s < 6; s++) {
       const int ax = t.axis[s], i = nodeIndex(q, ax, N);
       const int sl = s * NFQ + sT.slot_of[s * NPT + q];
+      double n[D];
+      for (int d = 0; d < D; d++) n[d] = nx[d];
+      if (s < 5) {
+        const int64_t es = e * NAQ + (s + 1) * NFQ + sT.slot_of[(s + 1) * NPT + q];
+        for (int d = 0; d < D; d++) nx[d] = g.snrm[es * D + d];
+      }
       // alpha, beta carry |J_f| w_t1 w_t2; M^-1 = 1 / (w_i w_t1 w_t2 |J|) at the node
       const double coef = sT.lf[t.side[s] * N + i] / (sT.wn[q] * dj);
-      const int64_t es = e * NAQ + sl;
-      double n[D];
-      for (int d = 0; d < D; d++) n[d] = g.snrm[es * D + d];
       for (int v = 0; v < NV; v++) {
         const double al = static_cast<double>(Al[v * NAQ + sl]) * coef;
         const double be = static_cast<double>(Bt[v * NAQ + sl]) * coef;
@@ -341,17 +354,25 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
   if (ok) {
     TC* Gf = pick<TC>(a.Gf, a.Gf32);
     const double idj = 1.0 / dj;
+    constexpr int GI = (NAQ * NV + NPT - 1) / NPT;
+    double nx[D];  // BR2: the slot's outward normal, loaded one item ahead of its use
+    if constexpr (VISC == 2)
+      for (int d = 0; d < D; d++) nx[d] = g.snrm[(e * NAQ + q % NAQ) * D + d];
 #pragma unroll 1
-    for (int idx = q; idx < NAQ * NV; idx += NPT) {
+    for (int k = 0; k < GI; k++) {
+      const int idx = q + k * NPT;
+      double nb[D], kb = 0.0;
+      if constexpr (VISC == 2) {
+        for (int d = 0; d < D; d++) nb[d] = nx[d];
+        if (k + 1 < GI && idx + NPT < NAQ * NV)
+          for (int d = 0; d < D; d++) nx[d] = g.snrm[(e * NAQ + (idx + NPT) % NAQ) * D + d];
+      }
+      if (idx >= NAQ * NV) break;
       const int sl = idx % NAQ, v = idx / NAQ, s = sl / NFQ;
       const int b = sT.base[sl], st = axisStride(sT.axis[s], N);
       const double* l = sT.lf + sT.side[s] * N;
-      double nb[D], kb = 0.0;
-      if constexpr (VISC == 2) {  // r_s at its own face: kappa_side beta n_d / |J|
-        const int64_t es = e * NAQ + sl;
-        for (int d = 0; d < D; d++) nb[d] = g.snrm[es * D + d];
+      if constexpr (VISC == 2)  // r_s at its own face: kappa_side beta n_d / |J|
         kb = sT.kappa[sT.side[s]] * static_cast<double>(Bt[v * NAQ + sl]);
-      }
       for (int d = 0; d < D; d++) {
         const int r = v * D + d;
         TC acc = TC(0);
@@ -367,7 +388,7 @@ __global__ __launch_bounds__(kThreads) void nsGradTensor(Geo g, Phys ph, StageAr
 
 // ---- kernel 2 (after nsFaceFluxKernel): residual, update, next-stage traces, monitor -----------
 template <typename TC, int N>
-__global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, StageArgs a) {
+__global__ __launch_bounds__(TensorShape<N>::TPB) void nsResidualTensor(Geo g, Phys ph, StageArgs a) {
   const TensorOps& t = g.tens;
   using S = TensorShape<N>;
   constexpr int NPT = S::NPT, EPB = S::EPB, NV = S::NV, NFQ = S::NFQ, NAQ = S::NAQ, D = 3;
@@ -399,11 +420,24 @@ __global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, Sta
       pu[v] = a.U[(e * NV + v) * NPT + b];
       pn[v] = a.Un[(e * NV + v) * NPT + b];
     }
+    // compile-time trip counts, so that all the loads are issued before the first wait
     const TC* src = Fv + e * NPT * D * NV;
-    for (int idx = q; idx < NPT * D * NV; idx += NPT) sF[le][idx] = src[idx];  // [axis][v][node]
-    for (int idx = q; idx < NAQ * NV; idx += NPT) {  // HBM [v][slot] -> LDS [slot][v]
-      const int v = idx / NAQ, sl = idx % NAQ;
-      sH[sl * NV + v] = Hs[(e * NV + v) * NAQ + sl];
+    TC f[D * NV];
+#pragma unroll
+    for (int k = 0; k < D * NV; k++) f[k] = src[k * NPT + q];
+    constexpr int HI = (NAQ * NV + NPT - 1) / NPT;
+    TC h[HI];
+#pragma unroll
+    for (int k = 0; k < HI; k++) {
+      const int idx = q + k * NPT;
+      if (idx < NAQ * NV) h[k] = Hs[(e * NV + idx / NAQ) * NAQ + idx % NAQ];
+    }
+#pragma unroll
+    for (int k = 0; k < D * NV; k++) sF[le][k * NPT + q] = f[k];  // [axis][v][node]
+#pragma unroll
+    for (int k = 0; k < HI; k++) {  // HBM [v][slot] -> LDS [slot][v]
+      const int idx = q + k * NPT;
+      if (idx < NAQ * NV) sH[(idx % NAQ) * NV + idx / NAQ] = h[k];
     }
   }
   __syncthreads();
@@ -526,7 +560,7 @@ __global__ __launch_bounds__(kThreads) void nsResidualTensor(Geo g, Phys ph, Sta
 
 // traces from the state in HBM with the residual kernel's own arithmetic (restarts, checkpoints)
 template <typename TC, int N>
-__global__ __launch_bounds__(kThreads) void tensorTraceKernel(Geo g, const double* U, void* Uf_out, MixRef mix) {
+__global__ __launch_bounds__(TensorShape<N>::TPB) void tensorTraceKernel(Geo g, const double* U, void* Uf_out, MixRef mix) {
   const TensorOps& t = g.tens;
   using S = TensorShape<N>;
   constexpr int NPT = S::NPT, EPB = S::EPB, NV = S::NV;

@@ -178,19 +178,19 @@ bool pickValu(bool mixed, Launchers& l) {
 // gradient and residual kernels (the face-flux kernel, the adjoint kernels and Euler stay).
 template <typename TC, int N>
 void useTensorKernels(Launchers& l) {
-  constexpr int EPB = sdg::TensorShape<N>::EPB;
+  constexpr int EPB = sdg::TensorShape<N>::EPB, TPB = sdg::TensorShape<N>::TPB;
   l.ns_grad_br1 = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-    hipLaunchKernelGGL((sdg::nsGradTensor<TC, N, 1>), dim3(blocksFor(g.E, EPB)), dim3(sdg::kThreads), 0, st, g, ph, a);
+    hipLaunchKernelGGL((sdg::nsGradTensor<TC, N, 1>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a);
   };
   l.ns_grad_br2 = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-    hipLaunchKernelGGL((sdg::nsGradTensor<TC, N, 2>), dim3(blocksFor(g.E, EPB)), dim3(sdg::kThreads), 0, st, g, ph, a);
+    hipLaunchKernelGGL((sdg::nsGradTensor<TC, N, 2>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a);
   };
   l.ns_flux = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-    hipLaunchKernelGGL((sdg::nsResidualTensor<TC, N>), dim3(blocksFor(g.E, EPB)), dim3(sdg::kThreads), 0, st, g, ph,
+    hipLaunchKernelGGL((sdg::nsResidualTensor<TC, N>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph,
                        a);
   };
   l.trace = [](hipStream_t st, const Geo& g, const double* U, void* Uf, const sdg::MixRef& mix) {
-    hipLaunchKernelGGL((sdg::tensorTraceKernel<TC, N>), dim3(blocksFor(g.E, EPB)), dim3(sdg::kThreads), 0, st, g, U,
+    hipLaunchKernelGGL((sdg::tensorTraceKernel<TC, N>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, U,
                        Uf, mix);
   };
   l.tensor = true;
