@@ -49,32 +49,49 @@ struct TensorLds {
   short iperm[NPT], slot_of[6 * NPT], base[NAQ], axis[6], side[6];
 };
 
-// all threads of the workgroup; the caller's next barrier publishes the tables
+// all threads of the workgroup; the caller's next barrier publishes the tables. Every thread
+// issues its loads unconditionally (indices clamped into range) before any store, so the copies
+// cost one memory round trip.
 template <int N>
 __device__ __forceinline__ void stageTables(const TensorOps& t, TensorLds<N>& L) {
-  constexpr int NPT = N * N * N, NAQ = 6 * N * N;
+  constexpr int NPT = N * N * N, NAQ = 6 * N * N, TPB = TensorShape<N>::TPB;
+  constexpr int NS = (6 * NPT + TPB - 1) / TPB;
+  static_assert(NPT <= TPB && NAQ <= TPB, "one table entry per thread");
   const int tid = threadIdx.x;
-  constexpr int TPB = TensorShape<N>::TPB;
-  for (int i = tid; i < N * N; i += TPB) {
-    L.B[i] = t.B[i];
-    L.Binv[i] = t.Binv[i];
-    L.Dn[i] = t.Dn[i];
+  const int i2 = tid < N * N ? tid : 0, i3 = tid < NPT ? tid : 0, ia = tid < NAQ ? tid : 0;
+  const double b = t.B[i2], bi = t.Binv[i2], dn = t.Dn[i2];
+  const double w0 = t.w[i3 % N], w1 = t.w[(i3 / N) % N], w2 = t.w[i3 / (N * N)];
+  const short ip = t.iperm[i3], bs = t.base[ia];
+  const double fw = t.fwt[ia], lf = t.lf[tid < 2 * N ? tid : 0], kp = t.kappa[tid < 2 ? tid : 0];
+  const int ax = t.axis[tid < 6 ? tid : 0], sd = t.side[tid < 6 ? tid : 0];
+  short so[NS];
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    const int j = tid + k * TPB;
+    so[k] = t.slot_of[j < 6 * NPT ? j : 0];
   }
-  for (int i = tid; i < NPT; i += TPB) {
-    L.wn[i] = t.w[i % N] * t.w[(i / N) % N] * t.w[i / (N * N)];
-    L.iperm[i] = t.iperm[i];
+  if (tid < N * N) {
+    L.B[tid] = b;
+    L.Binv[tid] = bi;
+    L.Dn[tid] = dn;
   }
-  for (int i = tid; i < 6 * NPT; i += TPB) L.slot_of[i] = t.slot_of[i];
-  for (int i = tid; i < NAQ; i += TPB) {
-    L.fwt[i] = t.fwt[i];
-    L.base[i] = t.base[i];
+  if (tid < NPT) {
+    L.wn[tid] = w0 * w1 * w2;
+    L.iperm[tid] = ip;
   }
-  if (tid < 2 * N) L.lf[tid] = t.lf[tid];
+#pragma unroll
+  for (int k = 0; k < NS; k++)
+    if (tid + k * TPB < 6 * NPT) L.slot_of[tid + k * TPB] = so[k];
+  if (tid < NAQ) {
+    L.fwt[tid] = fw;
+    L.base[tid] = bs;
+  }
+  if (tid < 2 * N) L.lf[tid] = lf;
   if (tid < 6) {
-    L.axis[tid] = static_cast<short>(t.axis[tid]);
-    L.side[tid] = static_cast<short>(t.side[tid]);
+    L.axis[tid] = static_cast<short>(ax);
+    L.side[tid] = static_cast<short>(sd);
   }
-  if (tid < 2) L.kappa[tid] = t.kappa[tid];
+  if (tid < 2) L.kappa[tid] = kp;
 }
 
 // barrier between the phases of one element's work: at NPT = 64 an element is exactly one
@@ -201,21 +218,26 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
   TC fo[ITEMS], fn[ITEMS];
   double fw[ITEMS];
   if (ok) {
+    // branch-free: the neighbour slots first, then every trace and weight load at once (a slot
+    // past the end or on a boundary reads its own trace and gets weight 0)
+    int nbk[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
       const int idx = q + k * NPT;
-      fo[k] = fn[k] = TC(0);
-      fw[k] = 0.0;
-      if (idx < NV * NAQ) {
-        const int v = idx / NAQ, sl = idx % NAQ;
-        const int64_t es = e * NAQ + sl;
-        const int nbs = g.snb[es];
-        if (nbs >= 0) {
-          fo[k] = Uf_in[(e * NV + v) * NAQ + sl];
-          fn[k] = Uf_in[(static_cast<int64_t>(nbs / NAQ) * NV + v) * NAQ + nbs % NAQ];
-          fw[k] = g.sjw[es];
-        }
-      }
+      nbk[k] = g.snb[e * NAQ + (idx < NV * NAQ ? idx % NAQ : 0)];
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+      const int idx = q + k * NPT;
+      const bool in = idx < NV * NAQ;
+      const int v = in ? idx / NAQ : 0, sl = in ? idx % NAQ : 0, nbs = nbk[k];
+      const bool inner = in && nbs >= 0;
+      const int64_t own = (e * NV + v) * NAQ + sl;
+      const int64_t nbr = inner ? (static_cast<int64_t>(nbs / NAQ) * NV + v) * NAQ + nbs % NAQ : own;
+      fo[k] = Uf_in[own];
+      fn[k] = Uf_in[nbr];
+      const double jw = g.sjw[e * NAQ + sl];
+      fw[k] = inner ? jw : 0.0;
     }
   }
   loadModal<TC, N>(t, a, a.U, e, Bt, q, ok);
