@@ -419,7 +419,10 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsResidualTensor(Geo g, P
   constexpr int HB = (NAQ * NV * static_cast<int>(sizeof(TC)) > NV * NPT * 8 ? NAQ * NV * static_cast<int>(sizeof(TC))
                                                                               : NV * NPT * 8) / 8;
   static_assert(D * NV * NPT * sizeof(TC) >= NV * NPT * 8, "FP64 scratch must fit the flux panel");
-  __shared__ TC sF[EPB][D * NV * NPT];  // contravariant volume flux [axis][v][node]; later FP64 scratch
+  // per-element stride rounded to 16 bytes: the panel is reused as FP64 scratch, which must stay
+  // 8-byte aligned for every element (odd N with FP32 panels would misalign it)
+  constexpr int FS = static_cast<int>((D * NV * NPT * sizeof(TC) + 15) / 16 * 16 / sizeof(TC));
+  __shared__ __align__(16) TC sF[EPB][FS];  // contravariant volume flux [axis][v][node]; later FP64 scratch
   __shared__ double sHR[EPB][HB];
   __shared__ double sM[EPB][NV];        // monitor partials
   __shared__ TensorLds<N> sT;
