@@ -598,7 +598,10 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
 // evaluated once with its left/right roles; t = (H_c - H_v) |J_f| w goes to the left parent's slot
 // as -t and to the right parent's slot (point seq[j]) as +t, the signs with which the face term
 // enters R = R_vol + sum_slots Phi_f s.
-template <typename TC, int D, int NF, int NFQ>
+// VN: each side's normal viscous flux F_v(u, grad u) . n_out was formed by the side's gradient
+// kernel (a.Vn, FP64, with the same arithmetic as below); the face kernel then reads 2 NV values per
+// point instead of both sides' 2 NV D gradient traces (boundary points still read Gf).
+template <typename TC, int D, int NF, int NFQ, bool VN = false>
 __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, StageArgs a) {
   constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
   if (a.status && a.status[0]) return;
@@ -637,23 +640,30 @@ __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, Sta
   if constexpr (sizeof(TC) == 4) reduceFaceFlux<D>(a, n, h);
   // normal viscous flux of each side, one side at a time (keeps one side's gradient live)
   double vl[NV], vr[NV];
-#pragma unroll 1
-  for (int sd = 0; sd < 2; sd++) {
-    const int64_t es = sd ? R : L;
-    const int ss = sd ? slR : slL;
-    double us[NV], ws[D + 3], gs[NG], pg[NG], fv[D * NV];
-    for (int v = 0; v < NV; v++) us[v] = sd ? ur[v] : ul[v];
-    for (int i = 0; i < D + 3; i++) ws[i] = sd ? wr[i] : wl[i];
-    for (int r = 0; r < NG; r++) gs[r] = Gf[(es * NG + r) * NAQ + ss];
-    primitiveGradient<D>(ph, ws, us, gs, pg);
-    viscousRawFlux<D>(ph, ws, pg, fv);
+  if constexpr (VN) {  // the right side's outward normal is -n
     for (int v = 0; v < NV; v++) {
-      double s = 0.0;
-      for (int d = 0; d < D; d++) s += fv[d + D * v] * n[d];
-      if (sd)
-        vr[v] = s;
-      else
-        vl[v] = s;
+      vl[v] = a.Vn[(L * NV + v) * NAQ + slL];
+      vr[v] = -a.Vn[(R * NV + v) * NAQ + slR];
+    }
+  } else {
+#pragma unroll 1
+    for (int sd = 0; sd < 2; sd++) {
+      const int64_t es = sd ? R : L;
+      const int ss = sd ? slR : slL;
+      double us[NV], ws[D + 3], gs[NG], pg[NG], fv[D * NV];
+      for (int v = 0; v < NV; v++) us[v] = sd ? ur[v] : ul[v];
+      for (int i = 0; i < D + 3; i++) ws[i] = sd ? wr[i] : wl[i];
+      for (int r = 0; r < NG; r++) gs[r] = Gf[(es * NG + r) * NAQ + ss];
+      primitiveGradient<D>(ph, ws, us, gs, pg);
+      viscousRawFlux<D>(ph, ws, pg, fv);
+      for (int v = 0; v < NV; v++) {
+        double s = 0.0;
+        for (int d = 0; d < D; d++) s += fv[d + D * v] * n[d];
+        if (sd)
+          vr[v] = s;
+        else
+          vl[v] = s;
+      }
     }
   }
   const double jw = g.fjw[f * NFQ + j];

@@ -371,38 +371,45 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
     for (int i = 0; i < D * NV; i++) o[i * NPT] = static_cast<TC>(out[i]);
   }
   elemSync<NPT>();
-  // traces of the face-gradient field (+ the own face's BR2 lift) -> Gf, per (variable, slot): the
-  // slot's tables, normal and beta serve the variable's D components
+  // per own face point: the trace of the face-gradient field (+ the own face's BR2 lift, the
+  // gradient the face flux of the reference uses for this side), then this side's normal viscous
+  // flux F_v(u, grad u) . n_out -> Vn (FP64), so the face kernel reads NV values per side instead
+  // of NV D; a boundary point also keeps its gradient trace in Gf for the boundary flux
   if (ok) {
     TC* Gf = pick<TC>(a.Gf, a.Gf32);
     const double idj = 1.0 / dj;
-    constexpr int GI = (NAQ * NV + NPT - 1) / NPT;
-    double nx[D];  // BR2: the slot's outward normal, loaded one item ahead of its use
-    if constexpr (VISC == 2)
-      for (int d = 0; d < D; d++) nx[d] = g.snrm[(e * NAQ + q % NAQ) * D + d];
+    constexpr int SI = (NAQ + NPT - 1) / NPT;
 #pragma unroll 1
-    for (int k = 0; k < GI; k++) {
-      const int idx = q + k * NPT;
-      double nb[D], kb = 0.0;
-      if constexpr (VISC == 2) {
-        for (int d = 0; d < D; d++) nb[d] = nx[d];
-        if (k + 1 < GI && idx + NPT < NAQ * NV)
-          for (int d = 0; d < D; d++) nx[d] = g.snrm[(e * NAQ + (idx + NPT) % NAQ) * D + d];
-      }
-      if (idx >= NAQ * NV) break;
-      const int sl = idx % NAQ, v = idx / NAQ, s = sl / NFQ;
+    for (int k = 0; k < SI; k++) {
+      const int sl = q + k * NPT;
+      if (sl >= NAQ) break;
+      const int s = sl / NFQ;
       const int b = sT.base[sl], st = axisStride(sT.axis[s], N);
       const double* l = sT.lf + sT.side[s] * N;
-      if constexpr (VISC == 2)  // r_s at its own face: kappa_side beta n_d / |J|
-        kb = sT.kappa[sT.side[s]] * static_cast<double>(Bt[v * NAQ + sl]);
-      for (int d = 0; d < D; d++) {
-        const int r = v * D + d;
+      const int64_t es = e * NAQ + sl;
+      const int nbs = g.snb[es];
+      double n[D], us[NV], gs[NG];
+      for (int d = 0; d < D; d++) n[d] = g.snrm[es * D + d];
+      for (int v = 0; v < NV; v++) us[v] = pointState<TC>(a, Uf_in[(e * NV + v) * NAQ + sl], v);
+      for (int r = 0; r < NG; r++) {
         TC acc = TC(0);
 #pragma unroll
         for (int i = 0; i < N; i++) acc += static_cast<TC>(l[i]) * W[r * NPT + b + i * st];
         double gv = static_cast<double>(acc);
-        if constexpr (VISC == 2) gv += kb * nb[d] * idj / sT.fwt[sl];
-        Gf[(e * NG + r) * NAQ + sl] = static_cast<TC>(gv);
+        if constexpr (VISC == 2)  // r_s at its own face: kappa_side beta n_d / |J|
+          gv += sT.kappa[sT.side[s]] * static_cast<double>(Bt[(r / D) * NAQ + sl]) * n[r % D] * idj / sT.fwt[sl];
+        const TC gt = static_cast<TC>(gv);
+        gs[r] = static_cast<double>(gt);
+        if (nbs < 0) Gf[(e * NG + r) * NAQ + sl] = gt;
+      }
+      double ws[D + 3], pg[NG], fv[D * NV];
+      computational<D>(ph, us, ws);
+      primitiveGradient<D>(ph, ws, us, gs, pg);
+      viscousRawFlux<D>(ph, ws, pg, fv);
+      for (int v = 0; v < NV; v++) {
+        double sum = 0.0;
+        for (int d = 0; d < D; d++) sum += fv[d + D * v] * n[d];
+        a.Vn[(e * NV + v) * NAQ + sl] = sum;
       }
     }
   }

@@ -185,6 +185,12 @@ void useTensorKernels(Launchers& l) {
   l.ns_grad_br2 = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
     hipLaunchKernelGGL((sdg::nsGradTensor<TC, N, 2>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a);
   };
+  // face fluxes from the elements' own normal viscous fluxes (written by the gradient kernel)
+  l.ns_face = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+    constexpr int NFQ = N * N;
+    hipLaunchKernelGGL((sdg::nsFaceFluxKernel<TC, 3, 6, NFQ, true>), dim3(blocksFor((g.F + g.Fb) * NFQ, sdg::kThreads)),
+                       dim3(sdg::kThreads), 0, st, g, ph, a);
+  };
   l.ns_flux = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
     hipLaunchKernelGGL((sdg::nsResidualTensor<TC, N>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph,
                        a);
@@ -363,7 +369,8 @@ struct sdg_ctx {
   double* lsq = nullptr;  // host copy
   std::vector<double> h_lsq, h_V, h_chol;
   std::vector<int> h_bparent;  // parent element of each boundary face
-  double *U = nullptr, *Un = nullptr, *Uf[2] = {nullptr, nullptr}, *Gf = nullptr, *Rvol = nullptr, *Hs = nullptr, *Fv = nullptr;
+  double *U = nullptr, *Un = nullptr, *Uf[2] = {nullptr, nullptr}, *Gf = nullptr, *Rvol = nullptr, *Hs = nullptr, *Fv = nullptr,
+         *Vn = nullptr;
   double *Rout = nullptr, *Wout = nullptr, *mon = nullptr, *rel = nullptr, *dtbuf = nullptr;
   int* status = nullptr;
   // boundary faces: count and boundary data (computational variables per boundary face point)
@@ -492,6 +499,7 @@ StageArgs baseArgs(sdg_ctx* ctx) {
   a.Rout = ctx->Rout;
   a.Wout = nullptr;
   a.Hs = ctx->Hs;
+  a.Vn = ctx->Vn;
   a.Fv = ctx->Fv;
   a.Uf_in32 = ctx->Uf32[ctx->cur];
   a.Uf_out32 = ctx->Uf32[ctx->cur ^ 1];
@@ -1062,6 +1070,7 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
       if (allocate(ctx, nGf, &ctx->Gf) || allocate(ctx, nUf, &ctx->Hs)) return bail(SDG_E_NOMEM);
       if (ctx->launch.mfma ? allocate(ctx, nFv, &ctx->Fv) : allocate(ctx, nU, &ctx->Rvol)) return bail(SDG_E_NOMEM);
     }
+    if (ctx->launch.tensor && allocate(ctx, nUf, &ctx->Vn)) return bail(SDG_E_NOMEM);
   }
   hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
