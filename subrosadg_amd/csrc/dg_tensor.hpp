@@ -355,21 +355,6 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
   // the face-gradient field (BR2: without the jump lifts; BR1: the total) -> sW for the traces
   if (ok)
     for (int r = 0; r < NG; r++) W[r * NPT + q] = static_cast<TC>(VISC == 2 ? wface[VISC == 2 ? r : 0] : wtot[r]);
-  // volume flux (F_c - F_v)^T (J^T)^-1 |J| w -> HBM Fv[e][b NV + v][node] (component-major: the
-  // stores coalesce across the wave and the residual kernel stages it into LDS unpermuted)
-  if (ok) {
-    double w[D + 3], pg[NG], fv[D * NV];
-    computational<D>(ph, u, w);
-    primitiveGradient<D>(ph, w, u, wtot, pg);
-    viscousRawFlux<D>(ph, w, pg, fv);
-    const double jw = dj * sT.wn[q];
-    double G[D * D], out[D * NV];
-    for (int i = 0; i < D * D; i++) G[i] = gi[i] * jw;
-    volumeFlux<TC, D, true>(a, w, fv, G, out);
-    TC* Fv = pick<TC>(a.Fv, a.Fv32);
-    TC* o = Fv + e * NPT * D * NV + q;
-    for (int i = 0; i < D * NV; i++) o[i * NPT] = static_cast<TC>(out[i]);
-  }
   elemSync<NPT>();
   // per own face point: the trace of the face-gradient field (+ the own face's BR2 lift, the
   // gradient the face flux of the reference uses for this side), then this side's normal viscous
@@ -413,6 +398,36 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
       }
     }
   }
+  elemSync<NPT>();  // the face-gradient field is read; sW takes the volume flux
+  // volume flux (F_c - F_v)^T (J^T)^-1 |J| w at the node -> sW [b NV + v][node], then its weak
+  // divergence sum_axes Dn^T (.) -> HBM Rvol[e][v][node] (FP64): the residual kernel continues
+  // from there with the face lift
+  if (ok) {
+    double w[D + 3], pg[NG], fv[D * NV];
+    computational<D>(ph, u, w);
+    primitiveGradient<D>(ph, w, u, wtot, pg);
+    viscousRawFlux<D>(ph, w, pg, fv);
+    const double jw = dj * sT.wn[q];
+    double G[D * D], out[D * NV];
+    for (int i = 0; i < D * D; i++) G[i] = gi[i] * jw;
+    volumeFlux<TC, D, true>(a, w, fv, G, out);
+    for (int i = 0; i < D * NV; i++) W[i * NPT + q] = static_cast<TC>(out[i]);
+  }
+  elemSync<NPT>();
+  if (ok) {
+    double R[NV];
+    for (int v = 0; v < NV; v++) R[v] = 0.0;
+    for (int ax = 0; ax < D; ax++) {
+      const int i = nodeIndex(q, ax, N), st = axisStride(ax, N), q0 = q - i * st;
+      for (int v = 0; v < NV; v++) {
+        TC s = TC(0);
+#pragma unroll
+        for (int m = 0; m < N; m++) s += static_cast<TC>(sT.Dn[m * N + i]) * W[(ax * NV + v) * NPT + q0 + m * st];
+        R[v] += static_cast<double>(s);
+      }
+    }
+    for (int v = 0; v < NV; v++) a.Rvol[(e * NV + v) * NPT + q] = R[v];
+  }
 }
 
 // ---- kernel 2 (after nsFaceFluxKernel): residual, update, next-stage traces, monitor -----------
@@ -425,11 +440,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsResidualTensor(Geo g, P
   // face-flux slots (TC), then the nodal increment / residual (FP64), in one region
   constexpr int HB = (NAQ * NV * static_cast<int>(sizeof(TC)) > NV * NPT * 8 ? NAQ * NV * static_cast<int>(sizeof(TC))
                                                                               : NV * NPT * 8) / 8;
-  static_assert(D * NV * NPT * sizeof(TC) >= NV * NPT * 8, "FP64 scratch must fit the flux panel");
-  // per-element stride rounded to 16 bytes: the panel is reused as FP64 scratch, which must stay
-  // 8-byte aligned for every element (odd N with FP32 panels would misalign it)
-  constexpr int FS = static_cast<int>((D * NV * NPT * sizeof(TC) + 15) / 16 * 16 / sizeof(TC));
-  __shared__ __align__(16) TC sF[EPB][FS];  // contravariant volume flux [axis][v][node]; later FP64 scratch
+  __shared__ double sF[EPB][NV * NPT];  // FP64 scratch of the basis passes
   __shared__ double sHR[EPB][HB];
   __shared__ double sM[EPB][NV];        // monitor partials
   __shared__ TensorLds<N> sT;
@@ -441,11 +452,10 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsResidualTensor(Geo g, P
   const int lec = le < EPB ? le : 0;
   TC* sH = reinterpret_cast<TC*>(sHR[lec]);
   double* SR = sHR[lec];
-  const TC* Fv = pick<TC>(static_cast<const double*>(a.Fv), static_cast<const float*>(a.Fv32));
   const TC* Hs = pick<TC>(static_cast<const double*>(a.Hs), static_cast<const float*>(a.Hs32));
   // the state and step-start state for the update, gathered early so that their latency overlaps
   // the residual (modal index of this thread's tensor position)
-  double pu[NV], pn[NV];
+  double pu[NV], pn[NV], rv[NV];
   if (ok) {
     const int b = t.iperm[q];
     for (int v = 0; v < NV; v++) {
@@ -453,10 +463,8 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsResidualTensor(Geo g, P
       pn[v] = a.Un[(e * NV + v) * NPT + b];
     }
     // compile-time trip counts, so that all the loads are issued before the first wait
-    const TC* src = Fv + e * NPT * D * NV;
-    TC f[D * NV];
-#pragma unroll
-    for (int k = 0; k < D * NV; k++) f[k] = src[k * NPT + q];
+#pragma unroll  // the weak divergence of the volume flux at the node (gradient kernel)
+    for (int v = 0; v < NV; v++) rv[v] = a.Rvol[(e * NV + v) * NPT + q];
     constexpr int HI = (NAQ * NV + NPT - 1) / NPT;
     TC h[HI];
 #pragma unroll
@@ -464,8 +472,6 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsResidualTensor(Geo g, P
       const int idx = q + k * NPT;
       if (idx < NAQ * NV) h[k] = Hs[(e * NV + idx / NAQ) * NAQ + idx % NAQ];
     }
-#pragma unroll
-    for (int k = 0; k < D * NV; k++) sF[le][k * NPT + q] = f[k];  // [axis][v][node]
 #pragma unroll
     for (int k = 0; k < HI; k++) {  // HBM [v][slot] -> LDS [slot][v]
       const int idx = q + k * NPT;
@@ -475,16 +481,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsResidualTensor(Geo g, P
   __syncthreads();
   double R[NV];
   if (ok) {
-    for (int v = 0; v < NV; v++) R[v] = 0.0;
-    for (int ax = 0; ax < D; ax++) {
-      const int i = nodeIndex(q, ax, N), st = axisStride(ax, N), q0 = q - i * st;
-      for (int v = 0; v < NV; v++) {
-        TC s = TC(0);
-#pragma unroll
-        for (int m = 0; m < N; m++) s += static_cast<TC>(sT.Dn[m * N + i]) * sF[le][(ax * NV + v) * NPT + q0 + m * st];
-        R[v] += static_cast<double>(s);
-      }
-    }
+    for (int v = 0; v < NV; v++) R[v] = rv[v];
 #pragma unroll 1
     for (int s = 0; s < 6; s++) {
       const int i = nodeIndex(q, t.axis[s], N);
@@ -502,7 +499,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsResidualTensor(Geo g, P
   elemSync<NPT>();
   // the reference residual (before M^-1) is (B^T x B^T x B^T) R_psi (modal); its monitor is
   // mean_q |R V^T| = mean over nodes of |(B x B x B) of it| (TimeIntegration.cpp:279-298)
-  double* T0 = reinterpret_cast<double*>(sF[lec]);  // FP64 scratch: NV NPT doubles fit in sF
+  double* T0 = sF[lec];
   auto pass64 = [&](const double* M, bool transpose, int axis, const double* in, double* out) {
     if (!ok) return;
     const int i = nodeIndex(q, axis, N), st = axisStride(axis, N), q0 = q - i * st;

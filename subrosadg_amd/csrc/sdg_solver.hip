@@ -1063,14 +1063,16 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     return bail(SDG_E_NOMEM);
   if (ctx->ns()) {
     const size_t nGf = static_cast<size_t>(E) * ctx->ng * naq, nFv = static_cast<size_t>(E) * nq * D * ctx->nv;
+    const bool tensor = ctx->launch.tensor;  // sum-factorised: Vn and the weak divergence Rvol, no Fv
     if (ctx->mixed) {
-      if (allocate(ctx, nGf, &ctx->Gf32) || allocate(ctx, nUf, &ctx->Hs32) || allocate(ctx, nFv, &ctx->Fv32))
+      if (allocate(ctx, nGf, &ctx->Gf32) || allocate(ctx, nUf, &ctx->Hs32) || (!tensor && allocate(ctx, nFv, &ctx->Fv32)))
         return bail(SDG_E_NOMEM);
     } else {
       if (allocate(ctx, nGf, &ctx->Gf) || allocate(ctx, nUf, &ctx->Hs)) return bail(SDG_E_NOMEM);
-      if (ctx->launch.mfma ? allocate(ctx, nFv, &ctx->Fv) : allocate(ctx, nU, &ctx->Rvol)) return bail(SDG_E_NOMEM);
+      if (!tensor && (ctx->launch.mfma ? allocate(ctx, nFv, &ctx->Fv) : allocate(ctx, nU, &ctx->Rvol)))
+        return bail(SDG_E_NOMEM);
     }
-    if (ctx->launch.tensor && allocate(ctx, nUf, &ctx->Vn)) return bail(SDG_E_NOMEM);
+    if (tensor && (allocate(ctx, nUf, &ctx->Vn) || allocate(ctx, nU, &ctx->Rvol))) return bail(SDG_E_NOMEM);
   }
   hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
