@@ -99,20 +99,21 @@ def tensor_path(mesh) -> bool:
 
 def tensor_bytes(mesh, tc: int):
     """Algorithmic HBM bytes per element and launch of the three sum-factorised Navier-Stokes stage
-    kernels (tc: bytes of the contraction type, 8 FP64 / 4 mixed; the state is FP64 in both):
-      gradient: state U + own and neighbour face traces + volume flux Fv + gradient traces Gf
-                + geometry (|J|, J^-1, two slot maps, and half of each face point's normal and
-                weight, which the neighbour shares);
-      face flux (per element, its half of the face points): both sides' traces and gradient traces,
-                normal, weight, point map, and the two signed flux slots written;
-      residual: Fv + flux slots + U + U_n read, U + next traces written, U_n written at the last of
-                the 3 stages (1/3 per launch), |J|."""
+    kernels on a periodic mesh (tc: bytes of the contraction type, 8 FP64 / 4 mixed; the state is
+    FP64 in both):
+      gradient: state U + own and neighbour face traces + the normal viscous flux Vn at the
+                element's face points and the weak divergence Rvol of the volume flux at its
+                nodes (both written in the contraction type) + geometry (|J|, J^-1, two slot maps,
+                and half of each face point's normal and weight, which the neighbour shares);
+      face flux (per element, its half of the face points): both sides' traces and normal viscous
+                fluxes, normal, weight, point map, and the two signed flux slots written;
+      residual: Rvol + flux slots + U + U_n read, U + next traces written, U_n written at the last
+                of the 3 stages (1/3 per launch), |J|."""
     nv, nq, naq, D = mesh.nv, mesh.nq, mesh.nf * mesh.nfq, mesh.dim
-    ng = nv * D
     geo = 8 + 8 * D * D + naq * 8 + naq * (8 * D + 8) / 2
-    grad = 8 * nv * nq + 2 * nv * naq * tc + D * nv * nq * tc + ng * naq * tc + geo
-    face = naq / 2 * (2 * (nv + ng) * tc + 8 * D + 8 + 4 + 2 * nv * tc)
-    res = D * nv * nq * tc + nv * naq * tc + 3 * 8 * nv * nq + nv * naq * tc + 8 * nv * nq / 3 + 8
+    grad = 8 * nv * nq + 2 * nv * naq * tc + nv * naq * tc + nv * nq * tc + geo
+    face = naq / 2 * (4 * nv * tc + 8 * D + 8 + 4 + 2 * nv * tc)
+    res = nv * nq * tc + nv * naq * tc + 3 * 8 * nv * nq + nv * naq * tc + 8 * nv * nq / 3 + 8
     return grad, face, res
 
 

@@ -132,13 +132,14 @@ struct StageArgs {
   const double* Uf_in;
   double* Uf_out;
   double* Gf;
-  double* Rvol;
+  double* Rvol;   // (sum-factorised path: the weak divergence of the volume flux, FP32 values in
+                  // the mixed mode)
   double* Rout;   // residual output (mode 1)
   double* Wout;   // lifted total gradient output (optional)
   double* Hs;     // E x NV x N_aq signed face-flux slots (Navier-Stokes, matrix-core path)
   double* Fv;     // E x (N_q D) x NV contravariant volume flux (Navier-Stokes, matrix-core path)
   double* Vn;     // E x NV x N_aq the element's own normal viscous flux F_v . n_out at its face
-                  // points (FP64 in every mode; sum-factorised path, else null)
+                  // points (sum-factorised path, else null; holds FP32 values in the mixed mode)
   // FP32 versions of the work buffers above (mixed-precision mode; traces hold the deviation)
   const float* Uf_in32;
   float* Uf_out32;

@@ -599,8 +599,9 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
 // as -t and to the right parent's slot (point seq[j]) as +t, the signs with which the face term
 // enters R = R_vol + sum_slots Phi_f s.
 // VN: each side's normal viscous flux F_v(u, grad u) . n_out was formed by the side's gradient
-// kernel (a.Vn, FP64, with the same arithmetic as below); the face kernel then reads 2 NV values per
-// point instead of both sides' 2 NV D gradient traces (boundary points still read Gf).
+// kernel (a.Vn holding TC values, with the same arithmetic as below); the face kernel then reads
+// 2 NV values per point instead of both sides' 2 NV D gradient traces (boundary points still
+// read Gf).
 template <typename TC, int D, int NF, int NFQ, bool VN = false>
 __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, StageArgs a) {
   constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
@@ -641,9 +642,10 @@ __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, Sta
   // normal viscous flux of each side, one side at a time (keeps one side's gradient live)
   double vl[NV], vr[NV];
   if constexpr (VN) {  // the right side's outward normal is -n
+    const TC* Vn = reinterpret_cast<const TC*>(a.Vn);
     for (int v = 0; v < NV; v++) {
-      vl[v] = a.Vn[(L * NV + v) * NAQ + slL];
-      vr[v] = -a.Vn[(R * NV + v) * NAQ + slR];
+      vl[v] = static_cast<double>(Vn[(L * NV + v) * NAQ + slL]);
+      vr[v] = -static_cast<double>(Vn[(R * NV + v) * NAQ + slR]);
     }
   } else {
 #pragma unroll 1

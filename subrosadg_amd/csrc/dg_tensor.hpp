@@ -358,8 +358,8 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
   elemSync<NPT>();
   // per own face point: the trace of the face-gradient field (+ the own face's BR2 lift, the
   // gradient the face flux of the reference uses for this side), then this side's normal viscous
-  // flux F_v(u, grad u) . n_out -> Vn (FP64), so the face kernel reads NV values per side instead
-  // of NV D; a boundary point also keeps its gradient trace in Gf for the boundary flux
+  // flux F_v(u, grad u) . n_out (FP64 physics) -> Vn in TC, so the face kernel reads NV values per
+  // side instead of NV D; a boundary point also keeps its gradient trace in Gf for the boundary flux
   if (ok) {
     TC* Gf = pick<TC>(a.Gf, a.Gf32);
     const double idj = 1.0 / dj;
@@ -391,16 +391,17 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
       computational<D>(ph, us, ws);
       primitiveGradient<D>(ph, ws, us, gs, pg);
       viscousRawFlux<D>(ph, ws, pg, fv);
+      TC* Vn = reinterpret_cast<TC*>(a.Vn);  // TC values in the FP64-sized buffer
       for (int v = 0; v < NV; v++) {
         double sum = 0.0;
         for (int d = 0; d < D; d++) sum += fv[d + D * v] * n[d];
-        a.Vn[(e * NV + v) * NAQ + sl] = sum;
+        Vn[(e * NV + v) * NAQ + sl] = static_cast<TC>(sum);
       }
     }
   }
   elemSync<NPT>();  // the face-gradient field is read; sW takes the volume flux
   // volume flux (F_c - F_v)^T (J^T)^-1 |J| w at the node -> sW [b NV + v][node], then its weak
-  // divergence sum_axes Dn^T (.) -> HBM Rvol[e][v][node] (FP64): the residual kernel continues
+  // divergence sum_axes Dn^T (.) -> HBM Rvol[e][v][node] in TC: the residual kernel continues
   // from there with the face lift
   if (ok) {
     double w[D + 3], pg[NG], fv[D * NV];
@@ -426,7 +427,8 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
         R[v] += static_cast<double>(s);
       }
     }
-    for (int v = 0; v < NV; v++) a.Rvol[(e * NV + v) * NPT + q] = R[v];
+    TC* Rv = reinterpret_cast<TC*>(a.Rvol);  // TC values in the FP64-sized buffer
+    for (int v = 0; v < NV; v++) Rv[(e * NV + v) * NPT + q] = static_cast<TC>(R[v]);
   }
 }
 
@@ -464,7 +466,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsResidualTensor(Geo g, P
     }
     // compile-time trip counts, so that all the loads are issued before the first wait
 #pragma unroll  // the weak divergence of the volume flux at the node (gradient kernel)
-    for (int v = 0; v < NV; v++) rv[v] = a.Rvol[(e * NV + v) * NPT + q];
+    for (int v = 0; v < NV; v++) rv[v] = reinterpret_cast<const TC*>(a.Rvol)[(e * NV + v) * NPT + q];
     constexpr int HI = (NAQ * NV + NPT - 1) / NPT;
     TC h[HI];
 #pragma unroll
