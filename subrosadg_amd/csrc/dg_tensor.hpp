@@ -364,18 +364,32 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
     TC* Gf = pick<TC>(a.Gf, a.Gf32);
     const double idj = 1.0 / dj;
     constexpr int SI = (NAQ + NPT - 1) / NPT;
-#pragma unroll 1
+    // the slots' neighbour flags, normals and own traces, all gathered before the loop (a slot
+    // past the end reads slot 0)
+    int nbk[SI];
+    double nk[SI][D];
+    TC uk[SI][NV];
+#pragma unroll
+    for (int k = 0; k < SI; k++) {
+      const int sl = q + k * NPT < NAQ ? q + k * NPT : 0;
+      const int64_t es = e * NAQ + sl;
+      nbk[k] = g.snb[es];
+#pragma unroll
+      for (int d = 0; d < D; d++) nk[k][d] = g.snrm[es * D + d];
+#pragma unroll
+      for (int v = 0; v < NV; v++) uk[k][v] = Uf_in[(e * NV + v) * NAQ + sl];
+    }
+#pragma unroll
     for (int k = 0; k < SI; k++) {
       const int sl = q + k * NPT;
       if (sl >= NAQ) break;
       const int s = sl / NFQ;
       const int b = sT.base[sl], st = axisStride(sT.axis[s], N);
       const double* l = sT.lf + sT.side[s] * N;
-      const int64_t es = e * NAQ + sl;
-      const int nbs = g.snb[es];
+      const int nbs = nbk[k];
       double n[D], us[NV], gs[NG];
-      for (int d = 0; d < D; d++) n[d] = g.snrm[es * D + d];
-      for (int v = 0; v < NV; v++) us[v] = pointState<TC>(a, Uf_in[(e * NV + v) * NAQ + sl], v);
+      for (int d = 0; d < D; d++) n[d] = nk[k][d];
+      for (int v = 0; v < NV; v++) us[v] = pointState<TC>(a, uk[k][v], v);
       for (int r = 0; r < NG; r++) {
         TC acc = TC(0);
 #pragma unroll
