@@ -149,6 +149,18 @@ def parity_error(a: np.ndarray, b: np.ndarray, state: np.ndarray, dim: int, floo
     return float(max(errs))
 
 
+def fp64_parity(gpu, oracle_val, extended_val, state, dim, tol, h=None):
+    """gpu matches the FP64 oracle to tol, or (the oracle's own rounding floor being above tol)
+    lies no farther from the extended-precision evaluation than max(tol, 2 x the oracle's own
+    distance from it). Returns (error vs oracle, error vs extended, oracle's floor)."""
+    e = parity_error(gpu, oracle_val, state, dim, h=h)
+    ex = parity_error(gpu, extended_val, state, dim, h=h)
+    floor = parity_error(oracle_val, extended_val, state, dim, h=h)
+    print(f"  vs oracle {e:.2e}  vs extended {ex:.2e}  oracle floor {floor:.2e}", flush=True)
+    assert e <= tol or ex <= max(tol, 2.0 * floor), (e, ex, floor)
+    return e, ex, floor
+
+
 def kinetic_energy(mesh, U: np.ndarray):
     """Domain-integrated kinetic energy J = sum_e sum_q |J|w 1/2 |m|^2 / rho of modal coefficients
     U (E, N_b, NV) and its gradient dJ/dU (same shape)."""

@@ -71,7 +71,8 @@ def _pair(mesh, prob, prim, bprim):
 
 CASES = [(eq, el, order, name, bc) for name, bc in BCS for (eq, el, order) in
          [(L.EQ_EULER, L.ELEM_QUADRANGLE, 2), (L.EQ_EULER, L.ELEM_QUADRANGLE, 3),
-          (L.EQ_NS, L.ELEM_QUADRANGLE, 2), (L.EQ_NS, L.ELEM_QUADRANGLE, 3), (L.EQ_NS, L.ELEM_HEXAHEDRON, 2)]]
+          (L.EQ_NS, L.ELEM_QUADRANGLE, 2), (L.EQ_NS, L.ELEM_QUADRANGLE, 3), (L.EQ_NS, L.ELEM_HEXAHEDRON, 2),
+          (L.EQ_NS, L.ELEM_HEXAHEDRON, 3)]]
 
 
 @pytest.mark.parametrize("eq,el,order,name,bc", CASES,
@@ -81,25 +82,30 @@ def test_boundary_residual_and_step_parity(eq, el, order, name, bc):
     mesh, prob, prim, bprim = _setup(eq, el, order, bc, n=3 if el == L.ELEM_HEXAHEDRON else 4)
     o, s = _pair(mesh, prob, prim, bprim)
     u0 = o.get_state()
+    x = O.Oracle(prob, mesh, extended=True)  # FP64 rounding floor of the case
+    x.set_state(u0)
+    x.set_boundary_primitive(bprim)
     ro = o.residual()  # also fills the oracle's lifted gradient
+    rx = x.residual()
     if eq == L.EQ_NS:
         h = float(mesh.field("min_edge").min())
-        eg = P.parity_error(s.gradient(), o.gradient(), u0, mesh.dim, h=h)
-        assert eg <= GRAD_TOL, eg
-    err = P.parity_error(s.residual(), ro, u0, mesh.dim)
-    assert err <= TOL, err
+        P.fp64_parity(s.gradient(), o.gradient(), x.gradient(), u0, mesh.dim, tol=GRAD_TOL, h=h)
+    P.fp64_parity(s.residual(), ro, rx, u0, mesh.dim, tol=TOL)
     dt = o.compute_dt(0.1)
     o.step(dt, 5)
+    x.step(dt, 5)
     _, nan, taken = s.step(dt, 5)
     assert nan == 0 and taken == 5
     uo = o.get_state()
-    es = P.parity_error(s.get_state(), uo, uo, mesh.dim)
-    assert es <= TOL, es
+    P.fp64_parity(s.get_state(), uo, x.get_state(), uo, mesh.dim, tol=TOL)
 
 
+@pytest.mark.parametrize("el", [L.ELEM_QUADRANGLE, L.ELEM_HEXAHEDRON], ids=["quad-p3", "hex-p3"])
 @pytest.mark.parametrize("name,bc", BCS, ids=[b[0] for b in BCS])
-def test_boundary_mixed_precision(name, bc):
-    mesh, prob, prim, bprim = _setup(L.EQ_NS, L.ELEM_QUADRANGLE, 3, bc, precision=L.PREC_MIXED)
+def test_boundary_mixed_precision(name, bc, el):
+    """20 mixed-precision steps against the FP64 oracle (hexahedra: the sum-factorised kernels)."""
+    mesh, prob, prim, bprim = _setup(L.EQ_NS, el, 3, bc, precision=L.PREC_MIXED,
+                                     n=3 if el == L.ELEM_HEXAHEDRON else 4)
     s = L.Solver(prob, mesh)
     s.set_initial_primitive(prim)
     s.set_boundary_primitive(bprim)
@@ -111,8 +117,10 @@ def test_boundary_mixed_precision(name, bc):
     dt = o.compute_dt(0.1)
     o.step(dt, 20)
     s.step(dt, 20)
-    a, b = s.get_state().reshape(-1, 4), o.get_state().reshape(-1, 4)
-    dev = max(np.linalg.norm(a[:, g] - b[:, g]) / np.linalg.norm(b[:, g]) for g in ([0], [1, 2], [3]))
+    nv = mesh.nv
+    a, b = s.get_state().reshape(-1, nv), o.get_state().reshape(-1, nv)
+    dev = max(np.linalg.norm(a[:, g] - b[:, g]) / np.linalg.norm(b[:, g])
+              for g in ([0], list(range(1, nv - 1)), [nv - 1]))
     print(f"  mixed L2 deviation {dev:.3e}", flush=True)
     assert dev <= 1e-6, dev
 

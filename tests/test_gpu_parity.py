@@ -30,15 +30,7 @@ GRAD_TOL = 5e-11
 
 
 def fp64_parity(gpu, oracle_val, extended_val, state, dim, tol=TOL, h=None):
-    """gpu matches the FP64 oracle to tol, or (the oracle's own rounding floor being above tol)
-    lies no farther from the extended-precision evaluation than max(tol, 2 x the oracle's own
-    distance from it). Returns (error vs oracle, error vs extended, oracle's floor)."""
-    e = P.parity_error(gpu, oracle_val, state, dim, h=h)
-    ex = P.parity_error(gpu, extended_val, state, dim, h=h)
-    floor = P.parity_error(oracle_val, extended_val, state, dim, h=h)
-    print(f"  vs oracle {e:.2e}  vs extended {ex:.2e}  oracle floor {floor:.2e}", flush=True)
-    assert e <= tol or ex <= max(tol, 2.0 * floor), (e, ex, floor)
-    return e, ex, floor
+    return P.fp64_parity(gpu, oracle_val, extended_val, state, dim, tol=tol, h=h)
 
 
 def _extended(mesh, prob, state):
