@@ -44,12 +44,12 @@ PEAK_HBM_GBS = 8000.0
 # Sum-factorised kernels (the default on hexahedra): profiles/ns_tgv32_p3_fp64_mixed_tensor_kernels.md.
 PROFILED_TRAFFIC = {("nsGradMfma", "fp64", 32, 3): (190207.1 * 2.0 + 614400.0 * 1.0) * 1024,
                     ("nsGradMfma", "mixed", 32, 3): (144194.1 * 2.0 + 307200.0 * 1.0) * 1024,
-                    ("nsGradTensor", "fp64", 32, 3): (181855.4 * 2.0 + 614400.0 * 1.0) * 1024,
-                    ("nsGradTensor", "mixed", 32, 3): (139279.1 * 2.0 + 307200.0 * 1.0) * 1024,
-                    ("nsFaceFluxKernel", "fp64", 32, 3): (274280.2 * 2.0 + 122880.0 * 1.0) * 1024,
-                    ("nsFaceFluxKernel", "mixed", 32, 3): (201431.1 * 2.0 + 61646.7 * 1.0) * 1024,
-                    ("nsResidualTensor", "fp64", 32, 3): (266861.3 * 2.0 + 232533.4 * 1.0) * 1024,
-                    ("nsResidualTensor", "mixed", 32, 3): (174706.0 * 2.0 + 171093.4 * 1.0) * 1024}
+                    ("nsGradTensor", "fp64", 32, 3): (194245.8 * 2.0 + 204800.0 * 1.0) * 1024,
+                    ("nsGradTensor", "mixed", 32, 3): (145675.7 * 2.0 + 102400.0 * 1.0) * 1024,
+                    ("nsFaceFluxKernel", "fp64", 32, 3): (151386.9 * 2.0 + 122880.0 * 1.0) * 1024,
+                    ("nsFaceFluxKernel", "mixed", 32, 3): (114860.2 * 2.0 + 61509.7 * 1.0) * 1024,
+                    ("nsResidualTensor", "fp64", 32, 3): (184933.9 * 2.0 + 232533.4 * 1.0) * 1024,
+                    ("nsResidualTensor", "mixed", 32, 3): (133740.9 * 2.0 + 171093.4 * 1.0) * 1024}
 # the same for the batched stage kernel on the 48-problem workload (average over its launches;
 # profiles/batch48_uneven_tri_quad_euler_kernels.md)
 PROFILED_BATCH_TRAFFIC = {"fp64": (346788.6 * 2.0 + 231499.4 * 1.0) * 1024,
