@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SDG_ABI_VERSION 2
+#define SDG_ABI_VERSION 3
 
 /* ---- status codes ---------------------------------------------------------------------- */
 #define SDG_OK 0
@@ -62,6 +62,10 @@ enum sdg_time_integration { SDG_RK_FORWARD_EULER = 0, SDG_RK_HEUN2 = 1, SDG_RK_S
 enum sdg_eos { SDG_EOS_IDEAL_GAS = 0, SDG_EOS_WEAK_COMPRESSIBLE = 1 };
 enum sdg_transport { SDG_TRANSPORT_NONE = 0, SDG_TRANSPORT_CONSTANT = 1, SDG_TRANSPORT_SUTHERLAND = 2 };
 enum sdg_precision { SDG_PREC_FP64 = 0, SDG_PREC_MIXED = 1 };
+/* SourceTermEnum (Enum.cpp:71-74), same order. Boussinesq: rho beta (T - T_ref) g, g = 1, on the
+ * last momentum component (y in 2-D, z in 3-D), added to the residual as int S phi
+ * (SourceTerm.cpp:29-58, SpatialDiscrete.cpp:254-262,1026-1029). */
+enum sdg_source_term { SDG_SOURCE_NONE = 0, SDG_SOURCE_BOUSSINESQ = 1 };
 /* InitialConditionEnum (Enum.cpp), same order: the file-based starts of sdg_raw_binary_read */
 enum sdg_initial_condition { SDG_IC_FUNCTION = 0, SDG_IC_SPECIFIC_FILE = 1, SDG_IC_LAST_STEP = 2 };
 /* BoundaryConditionEnum (Enum.cpp:92-100), same order */
@@ -93,6 +97,11 @@ typedef struct sdg_problem_desc {
   double dynamic_viscosity;
   double reference_sound_speed; /* weakly-compressible EOS only                           */
   double reference_density;     /* weakly-compressible EOS only                           */
+  int source_term;        /* enum sdg_source_term (FP64, single problems; setSourceTerm,  */
+                          /* SystemControl.cpp:65-68)                                     */
+  int pad_;
+  double thermal_expansion_coefficient; /* Boussinesq beta                                */
+  double reference_temperature;         /* Boussinesq T_ref                               */
 } sdg_problem_desc;
 
 /* Structured mesh on an axis-aligned box: the project-side replacement for the gmsh

@@ -140,7 +140,6 @@ struct System {
                     SC::kEquationModel == EquationModelEnum::CompresibleNS,
                 "the GPU engine implements the compressible Euler and Navier-Stokes models");
   static_assert(SC::kShockCapturing == ShockCapturingEnum::None, "artificial viscosity is not implemented");
-  static_assert(SC::kSourceTerm == SourceTermEnum::None, "source terms are not implemented");
 
   InitialCondition<SC> initial_condition_;
   BoundaryCondition<SC> boundary_condition_;
@@ -167,6 +166,7 @@ struct System {
     problem_.equation_of_state = static_cast<int>(SC::kEquationOfState);
     problem_.transport = static_cast<int>(SC::kTransportModel);
     problem_.precision = SDG_PREC_FP64;
+    problem_.source_term = static_cast<int>(SC::kSourceTerm);
   }
   ~System() {
     if (ctx_) sdg_destroy(ctx_);
@@ -207,6 +207,12 @@ struct System {
   template <TransportModelEnum TransportModelType>
   void setTransportModel(const Real dynamic_viscosity) {
     problem_.dynamic_viscosity = dynamic_viscosity;
+  }
+  // SystemControl.cpp:65-68: Boussinesq thermal expansion coefficient and reference temperature
+  void setSourceTerm(const Real thermal_expansion_coefficient, const Real reference_temperature) {
+    static_assert(SC::kSourceTerm == SourceTermEnum::Boussinesq, "setSourceTerm needs SourceTermEnum::Boussinesq");
+    problem_.thermal_expansion_coefficient = thermal_expansion_coefficient;
+    problem_.reference_temperature = reference_temperature;
   }
   // SystemControl.cpp:110-121 (no interactive prompt: the range is required)
   void setTimeIntegration(const Real courant_friedrichs_lewy_number, const std::pair<int, int> iteration_range) {

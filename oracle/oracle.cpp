@@ -57,6 +57,8 @@ struct Physics {
   int eos = SDG_EOS_IDEAL_GAS;
   int transport = SDG_TRANSPORT_NONE;
   Real cp = 0, cv = 0, mu = 0, kappa = 0, c0 = 0, rho0 = 0, padd = 0;
+  int source = SDG_SOURCE_NONE;  // SourceTerm.cpp:29-58
+  Real beta = 0, tref = 0;
 
   // ThermodynamicModel<Constant> (PhysicalModel.cpp:24-38)
   Real internalEnergyFromTemperature(Real t) const { return cv * t; }
@@ -122,7 +124,7 @@ struct Oracle {
   std::vector<int32_t> fparent, flocal, fseq;
   std::vector<Real> fnormal, fjw;
   // per-element solver data (SolveControl.cpp:45-174), flattened element-major
-  std::vector<Real> coeff, coeff_last, vol_quad, adj_quad, residual;
+  std::vector<Real> coeff, coeff_last, vol_quad, adj_quad, residual, src_quad;
   std::vector<Real> vg_coeff, vg_quad, vg_adj, vg_res;           // volume gradient
   std::vector<Real> ig_adj, ig_res, ig_coeff, g_coeff;           // interface / total gradient (NS)
   int nig = 0;  // interface-gradient residual blocks per element: 1 (BR1) or N_f (BR2)
@@ -625,6 +627,10 @@ struct Oracle {
         Real w[8], fc[15], fv[15], pg[15];
         const Real* u = &uq[static_cast<size_t>(nv) * q];
         computational(u, w);
+        if (phys.source == SDG_SOURCE_BOUSSINESQ)  // rho beta (T - T_ref) g |J| w, g = 1 (SourceTerm.cpp:41-55)
+          src_quad[static_cast<size_t>(e) * nq + q] =
+              w[0] * phys.beta * (phys.temperatureFromInternalEnergy(w[dim + 1]) - phys.tref) * 1.0 *
+              detjw[static_cast<size_t>(e) * nq + q];
         // F_c (D x NV column-major), ConvectiveFlux.cpp:27-44
         const Real rho = w[0], p = w[dim + 2];
         const Real et = w[dim + 1] + velocitySquared(w) / 2.0;
@@ -717,6 +723,12 @@ struct Oracle {
       Real* res = &residual[e * szC()];
       gemm(res, &vol_quad[static_cast<size_t>(e) * nv * dim * nq], dV.data(), nv, dim * nq, nb, false);
       gemm(res, &adj_quad[static_cast<size_t>(e) * nv * naq], Vf.data(), nv, naq, nb, true, -1.0);
+      if (phys.source == SDG_SOURCE_BOUSSINESQ)  // + S V on the last momentum row (SpatialDiscrete.cpp:1026-1029)
+        for (int b = 0; b < nb; b++) {
+          Real s = 0.0;
+          for (int q = 0; q < nq; q++) s += src_quad[static_cast<size_t>(e) * nq + q] * V[q + static_cast<size_t>(nq) * b];
+          res[dim + static_cast<size_t>(nv) * b] += s;
+        }
     }
   }
   void rkCoefficients(int stage, Real c[3]) const {  // TimeIntegration.cpp:45-65
@@ -831,6 +843,9 @@ void* oracle_create(const sdg_problem_desc* p, int dim, int nb, int nq, int nf, 
   o->phys.c0 = p->reference_sound_speed;
   o->phys.rho0 = p->reference_density;
   o->phys.padd = 0.01 * p->reference_density * p->reference_sound_speed * p->reference_sound_speed;
+  o->phys.source = p->source_term;
+  o->phys.beta = p->thermal_expansion_coefficient;
+  o->phys.tref = p->reference_temperature;
   colMajorFromRowMajor(o->V, V, nq, nb);
   colMajorFromRowMajor(o->dV, dV, nq * dim, nb);
   colMajorFromRowMajor(o->Vf, Vf, nf * nfq, nb);
@@ -852,6 +867,7 @@ void* oracle_create(const sdg_problem_desc* p, int dim, int nb, int nq, int nf, 
   o->coeff.assign(ne * nv * nb, 0.0);
   o->coeff_last.assign(ne * nv * nb, 0.0);
   o->vol_quad.assign(ne * nv * dim * nq, 0.0);
+  o->src_quad.assign(ne * nq, 0.0);
   o->adj_quad.assign(ne * nv * o->naq, 0.0);
   o->residual.assign(ne * nv * nb, 0.0);
   o->vg_coeff.assign(ne * o->szG(), 0.0);

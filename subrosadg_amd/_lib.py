@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsdg_hip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # status codes
 OK, E_INVALID, E_UNSUPPORTED, E_DEVICE, E_NOMEM, E_STATE = 0, -1, -2, -3, -4, -5
@@ -29,6 +29,7 @@ RK_FORWARD_EULER, RK_HEUN2, RK_SSPRK3 = 0, 1, 2
 EOS_IDEAL_GAS, EOS_WEAK_COMPRESSIBLE = 0, 1
 TRANSPORT_NONE, TRANSPORT_CONSTANT, TRANSPORT_SUTHERLAND = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
+SOURCE_NONE, SOURCE_BOUSSINESQ = 0, 1
 IC_FUNCTION, IC_SPECIFIC_FILE, IC_LAST_STEP = 0, 1, 2
 (BC_RIEMANN_FARFIELD, BC_VELOCITY_INFLOW, BC_PRESSURE_OUTFLOW, BC_ISOTHERMAL_NONSLIP_WALL, BC_ADIABATIC_SLIP_WALL,
  BC_ADIABATIC_NONSLIP_WALL, BC_PERIODIC) = range(7)
@@ -78,6 +79,10 @@ class ProblemDesc(Structure):
         ("dynamic_viscosity", c_double),
         ("reference_sound_speed", c_double),
         ("reference_density", c_double),
+        ("source_term", c_int),
+        ("pad_", c_int),
+        ("thermal_expansion_coefficient", c_double),
+        ("reference_temperature", c_double),
     ]
 
 
@@ -214,6 +219,9 @@ def problem_desc(**kw) -> ProblemDesc:
     p.dynamic_viscosity = kw.get("dynamic_viscosity", 0.0)
     p.reference_sound_speed = kw.get("reference_sound_speed", 0.0)
     p.reference_density = kw.get("reference_density", 0.0)
+    p.source_term = kw.get("source_term", SOURCE_NONE)
+    p.thermal_expansion_coefficient = kw.get("thermal_expansion_coefficient", 0.0)
+    p.reference_temperature = kw.get("reference_temperature", 0.0)
     return p
 
 

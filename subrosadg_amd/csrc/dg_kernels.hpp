@@ -138,6 +138,8 @@ struct StageArgs {
   double* Wout;   // lifted total gradient output (optional)
   double* Hs;     // E x NV x N_aq signed face-flux slots (Navier-Stokes, matrix-core path)
   double* Fv;     // E x (N_q D) x NV contravariant volume flux (Navier-Stokes, matrix-core path)
+  const double* Rsrc;  // E x N_b modal source residual of the last momentum component (null: none;
+                       // the sum-factorised kernels form the source at their nodes instead)
   double* Vn;     // E x NV x N_aq the element's own normal viscous flux F_v . n_out at its face
                   // points (sum-factorised path, else null; holds FP32 values in the mixed mode)
   // FP32 versions of the work buffers above (mixed-precision mode; traces hold the deviation)
@@ -413,7 +415,8 @@ __global__ __launch_bounds__(kThreads) void eulerStageKernel(Geo g, Phys ph, Sta
     for (int k = 0; k < D * NQ; k++) r += sF[le][v][k] * g.dV[k * NB + i];
     double h = 0.0;
     for (int k = 0; k < NAQ; k++) h += sH[le][v][k] * g.Vf[k * NB + i];
-    sR[le][v][i] = r - h;
+    const int e = e0 + le;
+    sR[le][v][i] = (v == D && a.Rsrc && e < g.E) ? r - h + a.Rsrc[static_cast<int64_t>(e) * NB + i] : r - h;
   }
   __syncthreads();
   stageTail<D, NB, NQ, NF, NFQ, EPB>(g, a, e0, sU, sR);
@@ -689,7 +692,7 @@ __global__ __launch_bounds__(kThreads) void nsFluxKernel(Geo g, Phys ph, StageAr
     double h = 0.0;
     for (int k = 0; k < NAQ; k++) h += sH[le][v][k] * g.Vf[k * NB + i];
     const double r = e < g.E ? a.Rvol[static_cast<int64_t>(e) * NV * NB + rem] : 0.0;
-    sR[le][v][i] = r - h;
+    sR[le][v][i] = (v == D && a.Rsrc && e < g.E) ? r - h + a.Rsrc[static_cast<int64_t>(e) * NB + i] : r - h;
   }
   __syncthreads();
   stageTail<D, NB, NQ, NF, NFQ, EPB>(g, a, e0, sU, sR);
@@ -741,6 +744,39 @@ __global__ __launch_bounds__(kThreads) void dtKernel(Geo g, Phys ph, const doubl
 
 // Fixed-order reduction of the monitor partials; raises the NaN stop flag when every variable is
 // NaN (SystemControl.cpp:185-191).
+// Boussinesq source, modal: Rsrc[e][b] = sum_q rho beta (T - T_ref) g |J| w_q V[q][b] for the last
+// momentum component (SpatialDiscrete.cpp:254-262, 1026-1029), from the state U of the stage;
+// point values in LDS, one thread per point, then one per mode
+template <int D, int NB, int NQ>
+__global__ __launch_bounds__(kThreads) void sourceKernel(Geo g, Phys ph, const double* U, double* Rsrc) {
+  constexpr int NV = D + 2, NM = NQ > NB ? NQ : NB, EPB = kThreads / NM > 0 ? kThreads / NM : 1;
+  __shared__ double sS[EPB][NQ];
+  const int tid = threadIdx.x, e0 = blockIdx.x * EPB;
+  for (int idx = tid; idx < EPB * NQ; idx += kThreads) {
+    const int le = idx / NQ, q = idx % NQ, e = e0 + le;
+    double s = 0.0;
+    if (e < g.E) {
+      double u[NV], w[D + 3];
+      for (int v = 0; v < NV; v++) {
+        double x = 0.0;
+        for (int b = 0; b < NB; b++) x += U[(static_cast<int64_t>(e) * NV + v) * NB + b] * g.Vt[b * NQ + q];
+        u[v] = x;
+      }
+      computational<D>(ph, u, w);
+      s = boussinesqSource<D>(ph, w) * (g.detj[e] * g.wq[q]);
+    }
+    sS[le][q] = s;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < EPB * NB; idx += kThreads) {
+    const int le = idx / NB, b = idx % NB, e = e0 + le;
+    if (e >= g.E) continue;
+    double r = 0.0;
+    for (int q = 0; q < NQ; q++) r += sS[le][q] * g.Vt[b * NQ + q];
+    Rsrc[static_cast<int64_t>(e) * NB + b] = r;
+  }
+}
+
 constexpr int kMonitorBlocks = 256;
 
 // sum of a block's LANES-strided values, one variable per lane (LANES a multiple of NV), -> out[v]

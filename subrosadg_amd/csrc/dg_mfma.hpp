@@ -130,6 +130,19 @@ __device__ __forceinline__ double restoreEnergy(const StageArgs& a, double x, in
 // ---- shared tail: residual -> (monitor) -> update -> next traces --------------------------------
 // sR: [NBm][16] residual panel (row basis, column le*NV+v), sU: [NBk][16] state panel (FP64 mode:
 // the current state; mixed mode: scratch, the update reads the FP64 state from HBM).
+// + the modal source residual of the last momentum component (Rsrc), before the update or the
+// residual output; the FP64 kernels only (the mixed mode refuses a source term)
+template <typename TC, int D, int NB, int EPB>
+__device__ __forceinline__ void addSource(const Geo& g, const StageArgs& a, int e0, TC* sR) {
+  if (!a.Rsrc) return;
+  constexpr int NV = D + 2;
+  for (int idx = threadIdx.x; idx < NB * EPB; idx += kThreads) {
+    const int b = idx / EPB, le = idx % EPB, e = e0 + le;
+    if (e < g.E) sR[b * 16 + le * NV + D] += static_cast<TC>(a.Rsrc[static_cast<int64_t>(e) * NB + b]);
+  }
+  __syncthreads();
+}
+
 template <typename TC, int D, int NB, int NQ, int NF, int NFQ, int EPB>
 __device__ __forceinline__ void mfmaStageTail(const Geo& g, const StageArgs& a, int e0, TC* sR, TC* sU,
                                               double* sRed) {
@@ -378,6 +391,7 @@ __global__ __launch_bounds__(kThreads) void eulerStageMfma(Geo g, Phys ph, Stage
   }
   for (int idx = NBm * 16 + tid; idx < NBr * 16; idx += kThreads) sR[idx] = TC(0);
   __syncthreads();
+  addSource<TC, D, NB, EPB>(g, a, e0, sR);
   if (a.mode == 1) {
     for (int idx = tid; idx < NB * 16; idx += kThreads) {
       const int b = idx / 16, c = idx % 16, le = c / NV, v = c % NV, e = e0 + le;
@@ -720,6 +734,7 @@ __global__ __launch_bounds__(kThreads) void nsResidualMfma(Geo g, Phys ph, Stage
     tileStore<TC>(acc, sR, m0, lane);
   }
   __syncthreads();
+  addSource<TC, D, NB, EPB>(g, a, e0, sR);
   if (a.mode == 1) {
     for (int idx = tid; idx < NB * 16; idx += kThreads) {
       const int b = idx / 16, cc = idx % 16, le = cc / NV, v = cc % NV, e = e0 + le;

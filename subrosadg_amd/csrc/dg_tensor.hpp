@@ -417,12 +417,14 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
   // volume flux (F_c - F_v)^T (J^T)^-1 |J| w at the node -> sW [b NV + v][node], then its weak
   // divergence sum_axes Dn^T (.) -> HBM Rvol[e][v][node] in TC: the residual kernel continues
   // from there with the face lift
+  double src = 0.0;  // Boussinesq source at the node times w |J| (the test function is the node's)
   if (ok) {
     double w[D + 3], pg[NG], fv[D * NV];
     computational<D>(ph, u, w);
     primitiveGradient<D>(ph, w, u, wtot, pg);
     viscousRawFlux<D>(ph, w, pg, fv);
     const double jw = dj * sT.wn[q];
+    if (ph.source == kSourceBoussinesq) src = boussinesqSource<D>(ph, w) * jw;
     double G[D * D], out[D * NV];
     for (int i = 0; i < D * D; i++) G[i] = gi[i] * jw;
     volumeFlux<TC, D, true>(a, w, fv, G, out);
@@ -441,6 +443,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
         R[v] += static_cast<double>(s);
       }
     }
+    R[D] += src;
     TC* Rv = reinterpret_cast<TC*>(a.Rvol);  // TC values in the FP64-sized buffer
     for (int v = 0; v < NV; v++) Rv[(e * NV + v) * NPT + q] = static_cast<TC>(R[v]);
   }

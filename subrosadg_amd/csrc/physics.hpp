@@ -22,7 +22,19 @@ struct Phys {
   int flux;       // sdg_convective_flux
   int pad_;
   double cp, cv, mu, kappa, c0, rho0, padd;
+  int source;  // sdg_source_term
+  int pad2_;
+  double beta, tref;  // Boussinesq thermal expansion coefficient and reference temperature
 };
+
+constexpr int kSourceBoussinesq = 1;  // sdg_source_term value of the Boussinesq source
+
+// Boussinesq source on the last momentum component at a point (SourceTerm.cpp:41-55): rho beta
+// (T - T_ref) g with g = 1, T = e / c_v; w: computational variables (rho, u.., e, p)
+template <int D>
+__device__ __forceinline__ double boussinesqSource(const Phys& ph, const double* w) {
+  return w[0] * ph.beta * (w[D + 1] / ph.cv - ph.tref) * 1.0;
+}
 
 template <typename T>
 __device__ __forceinline__ T pressureOf(const Phys& ph, T rho, T e) {  // PhysicalModel.cpp:45-75

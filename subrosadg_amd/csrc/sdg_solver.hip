@@ -32,6 +32,7 @@ using StageFn = void (*)(hipStream_t, const Geo&, const Phys&, const StageArgs&)
 using TraceFn = void (*)(hipStream_t, const Geo&, const double*, void*, const sdg::MixRef&);
 using DtFn = void (*)(hipStream_t, const Geo&, const Phys&, const double*, const double*, double, int, double*);
 using MonFn = void (*)(hipStream_t, const double*, int, double*, int*);
+using SrcFn = void (*)(hipStream_t, const Geo&, const Phys&, const double*, double*);
 using AdjFn = void (*)(hipStream_t, const Geo&, const Phys&, const StageArgs&, const sdg::AdjArgs&);
 
 struct Launchers {
@@ -43,6 +44,7 @@ struct Launchers {
   StageFn ns_flux = nullptr;
   StageFn ns_face = nullptr;  // matrix-core path: face fluxes before ns_flux
   TraceFn trace = nullptr;
+  SrcFn source = nullptr;  // modal Boussinesq source (all kernel sets but the sum-factorised one)
   DtFn dt = nullptr;
   MonFn monitor = nullptr;
   // discrete adjoint (Navier-Stokes, FP64 matrix-core sizes only)
@@ -143,6 +145,11 @@ Launchers makeLaunchers() {
                          dim3(sdg::kThreads), 0, st, g, U, Uf, mix);
     };
   }
+  l.source = [](hipStream_t st, const Geo& g, const Phys& ph, const double* U, double* out) {
+    constexpr int NM = NQ > NB ? NQ : NB, EPBS = sdg::kThreads / NM > 0 ? sdg::kThreads / NM : 1;
+    hipLaunchKernelGGL((sdg::sourceKernel<D, NB, NQ>), dim3(blocksFor(g.E, EPBS)), dim3(sdg::kThreads), 0, st, g, ph,
+                       U, out);
+  };
   l.dt = [](hipStream_t st, const Geo& g, const Phys& ph, const double* U, const double* me, double cfl, int order,
             double* out) {
     hipLaunchKernelGGL((sdg::dtKernel<D, NB, NQ, NF, NFQ>), dim3((g.E + sdg::kThreads - 1) / sdg::kThreads),
@@ -370,7 +377,7 @@ struct sdg_ctx {
   std::vector<double> h_lsq, h_V, h_chol;
   std::vector<int> h_bparent;  // parent element of each boundary face
   double *U = nullptr, *Un = nullptr, *Uf[2] = {nullptr, nullptr}, *Gf = nullptr, *Rvol = nullptr, *Hs = nullptr, *Fv = nullptr,
-         *Vn = nullptr;
+         *Vn = nullptr, *Rsrc = nullptr;
   double *Rout = nullptr, *Wout = nullptr, *mon = nullptr, *rel = nullptr, *dtbuf = nullptr;
   int* status = nullptr;
   // boundary faces: count and boundary data (computational variables per boundary face point)
@@ -500,6 +507,7 @@ StageArgs baseArgs(sdg_ctx* ctx) {
   a.Wout = nullptr;
   a.Hs = ctx->Hs;
   a.Vn = ctx->Vn;
+  a.Rsrc = ctx->Rsrc;
   a.Fv = ctx->Fv;
   a.Uf_in32 = ctx->Uf32[ctx->cur];
   a.Uf_out32 = ctx->Uf32[ctx->cur ^ 1];
@@ -535,6 +543,7 @@ void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hi
   }
   a.Wout = wout;
   if (mode == 1) a.status = nullptr;
+  if (ctx->Rsrc) ctx->launch.source(ctx->stream, ctx->geo, ctx->phys, ctx->U, ctx->Rsrc);
   if (ev) (void)hipEventRecord(ev[0], ctx->stream);
   if (ctx->ns()) {
     if (ctx->prob.viscous_flux == SDG_VISC_BR1)
@@ -670,6 +679,14 @@ int validateProblem(const sdg_problem_desc* p, std::string& msg) {
     msg = "unknown precision mode";
     return SDG_E_INVALID;
   }
+  if (p->source_term != SDG_SOURCE_NONE && p->source_term != SDG_SOURCE_BOUSSINESQ) {
+    msg = "unknown source term";
+    return SDG_E_INVALID;
+  }
+  if (p->source_term != SDG_SOURCE_NONE && p->precision != SDG_PREC_FP64) {
+    msg = "the Boussinesq source term runs in the FP64 mode only";
+    return SDG_E_UNSUPPORTED;
+  }
   return SDG_OK;
 }
 
@@ -729,6 +746,9 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   ph.c0 = problem->reference_sound_speed;
   ph.rho0 = problem->reference_density;
   ph.padd = 0.01 * problem->reference_density * problem->reference_sound_speed * problem->reference_sound_speed;
+  ph.source = problem->source_term;
+  ph.beta = problem->thermal_expansion_coefficient;
+  ph.tref = problem->reference_temperature;
   const char* ng = std::getenv("SDG_NO_GRAPH");
   ctx->use_graph = !(ng && ng[0] == '1');
 
@@ -1074,6 +1094,11 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     }
     if (tensor && (allocate(ctx, nUf, &ctx->Vn) || allocate(ctx, nU, &ctx->Rvol))) return bail(SDG_E_NOMEM);
   }
+  // modal Boussinesq source for the kernel sets that take it from a buffer (the sum-factorised
+  // kernels form it at their nodes)
+  if (ctx->phys.source != SDG_SOURCE_NONE && !ctx->launch.tensor &&
+      allocate(ctx, static_cast<size_t>(E) * ctx->nb, &ctx->Rsrc))
+    return bail(SDG_E_NOMEM);
   hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
     ctx->err = "sdg_create: device initialisation failed";
@@ -1398,10 +1423,10 @@ namespace {
 
 int ensureAdjoint(sdg_ctx* ctx) {
   if (ctx->adj_ready) return SDG_OK;
-  if (!ctx->ns() || ctx->mixed || !ctx->launch.adj_volume[0] || ctx->Fb > 0)
+  if (!ctx->ns() || ctx->mixed || !ctx->launch.adj_volume[0] || ctx->Fb > 0 || ctx->phys.source != SDG_SOURCE_NONE)
     return fail(ctx, SDG_E_UNSUPPORTED,
                 "adjoint: implemented for Navier-Stokes in FP64 on periodic meshes of the matrix-core element sizes "
-                "(quadrangle P3-P5, hexahedron P2-P4)");
+                "(quadrangle P3-P5, hexahedron P2-P4), without a source term");
   const size_t E = static_cast<size_t>(ctx->E);
   const size_t nU = E * ctx->nv * ctx->nb, nS = E * ctx->nv * ctx->naq, nG = E * ctx->ng * ctx->nb,
                nGS = E * ctx->ng * ctx->naq;
