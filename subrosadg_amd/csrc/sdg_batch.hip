@@ -140,18 +140,39 @@ __device__ __noinline__ void eulerTile(const Args<TC>& a, const VariantDev& vd, 
   for (int64_t c0 = first; c0 < first + count; c0 += EPB) {
     const int ne = static_cast<int>(first + count - c0 < EPB ? first + count - c0 : EPB);
     // state panel (deviation from the problem's reference state in the mixed mode); the panel of
-    // EPB consecutive elements is one contiguous stretch of U
-    #pragma unroll 1
-    for (int idx = tid; idx < EPB * NV * NB; idx += kThreads) {
-      const int le = idx / (NV * NB), rem = idx % (NV * NB), v = rem / NB, b = rem % NB;
-      TC x = TC(0);
-      if (le < ne) {
-        const int64_t e = c0 + le;
-        double u = a.U[vd.u_base + e * NV * NB + rem];
-        if constexpr (MIX) u -= a.mix[a.pid[vd.e_base + e] * 6 + v] * vd.one[b];
-        x = static_cast<TC>(u);
+    // EPB consecutive elements is one contiguous stretch of U. Compile-time trip count: every
+    // thread issues all its loads (indices clamped into the panel) before the first wait.
+    {
+      constexpr int IT = (EPB * NV * NB + kThreads - 1) / kThreads;
+      double u[IT];
+#pragma unroll
+      for (int k = 0; k < IT; k++) {
+        const int idx = tid + k * kThreads;
+        const int le = idx / (NV * NB);
+        const int64_t e = c0 + (le < ne ? le : 0);
+        u[k] = a.U[vd.u_base + e * NV * NB + idx % (NV * NB)];
       }
-      sU[le][v][b] = x;
+      if constexpr (MIX) {
+        int pk[IT];
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+          const int le = (tid + k * kThreads) / (NV * NB);
+          pk[k] = a.pid[vd.e_base + c0 + (le < ne ? le : 0)];
+        }
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+          const int rem = (tid + k * kThreads) % (NV * NB);
+          u[k] -= a.mix[pk[k] * 6 + rem / NB] * vd.one[rem % NB];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < IT; k++) {
+        const int idx = tid + k * kThreads;
+        if (idx < EPB * NV * NB) {
+          const int le = idx / (NV * NB), rem = idx % (NV * NB);
+          sU[le][rem / NB][rem % NB] = le < ne ? static_cast<TC>(u[k]) : TC(0);
+        }
+      }
     }
     __syncthreads();
         // point values: one thread per (element, point), all variables (one table read per term)
