@@ -197,6 +197,16 @@ __device__ __forceinline__ void mfmaStageTail(const Geo& g, const StageArgs& a, 
       rm[i] = acc[i];
       if constexpr (MIX) rm[i] = restoreEnergy<NV>(a, rm[i], v, lane);
     }
+    // U_n (and, in the mixed mode, U) of the tile's four rows, loaded before any store: the
+    // compiler must assume the stores to U may alias them, which would serialise the rows
+    double pn[4], pu[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int bi = m0 + Mfma<double>::row(rr, i);
+      const int64_t gi = (static_cast<int64_t>(col_ok ? e : 0) * NV + v) * NB + (bi < NB ? bi : 0);
+      pn[i] = a.Un[gi];
+      pu[i] = MIX ? a.U[gi] : 0.0;
+    }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int bi = m0 + Mfma<double>::row(rr, i);
@@ -205,10 +215,10 @@ __device__ __forceinline__ void mfmaStageTail(const Geo& g, const StageArgs& a, 
         const int64_t gi = (static_cast<int64_t>(e) * NV + v) * NB + bi;
         double u;
         if constexpr (MIX)
-          u = a.U[gi] * a.c1;
+          u = pu[i] * a.c1;
         else
           u = sU[bi * 16 + c] * a.c1;
-        u += a.c0 * a.Un[gi];
+        u += a.c0 * pn[i];
         u += a.c2dt * (rm[i] / idj);
         a.U[gi] = u;
         if (a.last_stage) a.Un[gi] = u;

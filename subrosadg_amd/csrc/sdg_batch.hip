@@ -298,6 +298,15 @@ __device__ __noinline__ void eulerTile(const Args<TC>& a, const VariantDev& vd, 
       if (le >= ne) continue;
       const int64_t e = c0 + le, eg = vd.e_base + e;
       const int p = a.pid[eg];
+      // U and U_n of the mode, loaded before any store (U and U_n may alias as far as the compiler
+      // knows, which would otherwise serialise load -> store -> load per variable)
+      double uo[NV], uno[NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const int64_t gi = vd.u_base + (e * NV + v) * NB + i;
+        uo[v] = a.U[gi];
+        uno[v] = a.Un[gi];
+      }
       double rm[NV] = {};
 #pragma unroll 5
       for (int b = 0; b < NB; b++) {
@@ -310,8 +319,8 @@ __device__ __noinline__ void eulerTile(const Args<TC>& a, const VariantDev& vd, 
 #pragma unroll
       for (int v = 0; v < NV; v++) {
         const int64_t gi = vd.u_base + (e * NV + v) * NB + i;
-        double u = a.U[gi] * a.c1;
-        u += a.c0 * a.Un[gi];
+        double u = uo[v] * a.c1;
+        u += a.c0 * uno[v];
         u += scale * rm[v];
         a.U[gi] = u;
         if (a.last_stage) a.Un[gi] = u;
