@@ -98,3 +98,19 @@ def test_paths_without_a_source_refuse_it():
     s.set_initial_primitive(_wavy_primitive(mesh))
     with pytest.raises(L.SDGError):
         s.vjp_residual(np.ones_like(s.get_state()))
+
+
+def test_cpp_driver_thermal_cavity_example():
+    """The C++ drop-in driver with SourceTermEnum::Boussinesq and setSourceTerm on a heated cavity
+    (sum-factorised hexahedron kernels with walls and the nodal source): the run completes with a
+    finite state whose z momentum, zero initially, has grown."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "examples", "bin",
+                       "thermal_cavity_3d_cns")
+    r = subprocess.run([exe, "4", "40"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    wmax = float(r.stdout.strip().splitlines()[-1].split()[-1])
+    assert 0.0 < wmax < 1.0
