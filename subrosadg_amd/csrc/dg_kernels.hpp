@@ -746,11 +746,14 @@ __global__ __launch_bounds__(kThreads) void dtKernel(Geo g, Phys ph, const doubl
 // NaN (SystemControl.cpp:185-191).
 // Boussinesq source, modal: Rsrc[e][b] = sum_q rho beta (T - T_ref) g |J| w_q V[q][b] for the last
 // momentum component (SpatialDiscrete.cpp:254-262, 1026-1029), from the state U of the stage;
-// point values in LDS, one thread per point, then one per mode
+// point values in LDS, one thread per point, then one per mode. Li (mixed mode, else null): the
+// residuals are formed against the orthonormal test functions L^-1 phi, so Rsrc <- L^-1 Rsrc.
 template <int D, int NB, int NQ>
-__global__ __launch_bounds__(kThreads) void sourceKernel(Geo g, Phys ph, const double* U, double* Rsrc) {
+__global__ __launch_bounds__(kThreads) void sourceKernel(Geo g, Phys ph, const double* U, double* Rsrc,
+                                                         const double* Li) {
   constexpr int NV = D + 2, NM = NQ > NB ? NQ : NB, EPB = kThreads / NM > 0 ? kThreads / NM : 1;
   __shared__ double sS[EPB][NQ];
+  __shared__ double sP[EPB][NB];
   const int tid = threadIdx.x, e0 = blockIdx.x * EPB;
   for (int idx = tid; idx < EPB * NQ; idx += kThreads) {
     const int le = idx / NQ, q = idx % NQ, e = e0 + le;
@@ -770,10 +773,22 @@ __global__ __launch_bounds__(kThreads) void sourceKernel(Geo g, Phys ph, const d
   __syncthreads();
   for (int idx = tid; idx < EPB * NB; idx += kThreads) {
     const int le = idx / NB, b = idx % NB, e = e0 + le;
-    if (e >= g.E) continue;
     double r = 0.0;
     for (int q = 0; q < NQ; q++) r += sS[le][q] * g.Vt[b * NQ + q];
-    Rsrc[static_cast<int64_t>(e) * NB + b] = r;
+    if (!Li) {
+      if (e < g.E) Rsrc[static_cast<int64_t>(e) * NB + b] = r;
+    } else {
+      sP[le][b] = r;
+    }
+  }
+  if (!Li) return;
+  __syncthreads();
+  for (int idx = tid; idx < EPB * NB; idx += kThreads) {
+    const int le = idx / NB, j = idx % NB, e = e0 + le;
+    if (e >= g.E) continue;
+    double r = 0.0;
+    for (int i = 0; i <= j; i++) r += Li[j * NB + i] * sP[le][i];  // L^-1 is lower triangular
+    Rsrc[static_cast<int64_t>(e) * NB + j] = r;
   }
 }
 

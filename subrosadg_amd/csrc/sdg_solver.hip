@@ -32,7 +32,7 @@ using StageFn = void (*)(hipStream_t, const Geo&, const Phys&, const StageArgs&)
 using TraceFn = void (*)(hipStream_t, const Geo&, const double*, void*, const sdg::MixRef&);
 using DtFn = void (*)(hipStream_t, const Geo&, const Phys&, const double*, const double*, double, int, double*);
 using MonFn = void (*)(hipStream_t, const double*, int, double*, int*);
-using SrcFn = void (*)(hipStream_t, const Geo&, const Phys&, const double*, double*);
+using SrcFn = void (*)(hipStream_t, const Geo&, const Phys&, const double*, double*, const double*);
 using AdjFn = void (*)(hipStream_t, const Geo&, const Phys&, const StageArgs&, const sdg::AdjArgs&);
 
 struct Launchers {
@@ -145,10 +145,10 @@ Launchers makeLaunchers() {
                          dim3(sdg::kThreads), 0, st, g, U, Uf, mix);
     };
   }
-  l.source = [](hipStream_t st, const Geo& g, const Phys& ph, const double* U, double* out) {
+  l.source = [](hipStream_t st, const Geo& g, const Phys& ph, const double* U, double* out, const double* Li) {
     constexpr int NM = NQ > NB ? NQ : NB, EPBS = sdg::kThreads / NM > 0 ? sdg::kThreads / NM : 1;
     hipLaunchKernelGGL((sdg::sourceKernel<D, NB, NQ>), dim3(blocksFor(g.E, EPBS)), dim3(sdg::kThreads), 0, st, g, ph,
-                       U, out);
+                       U, out, Li);
   };
   l.dt = [](hipStream_t st, const Geo& g, const Phys& ph, const double* U, const double* me, double cfl, int order,
             double* out) {
@@ -377,7 +377,7 @@ struct sdg_ctx {
   std::vector<double> h_lsq, h_V, h_chol;
   std::vector<int> h_bparent;  // parent element of each boundary face
   double *U = nullptr, *Un = nullptr, *Uf[2] = {nullptr, nullptr}, *Gf = nullptr, *Rvol = nullptr, *Hs = nullptr, *Fv = nullptr,
-         *Vn = nullptr, *Rsrc = nullptr;
+         *Vn = nullptr, *Rsrc = nullptr, *src_li = nullptr;
   double *Rout = nullptr, *Wout = nullptr, *mon = nullptr, *rel = nullptr, *dtbuf = nullptr;
   int* status = nullptr;
   // boundary faces: count and boundary data (computational variables per boundary face point)
@@ -543,7 +543,7 @@ void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hi
   }
   a.Wout = wout;
   if (mode == 1) a.status = nullptr;
-  if (ctx->Rsrc) ctx->launch.source(ctx->stream, ctx->geo, ctx->phys, ctx->U, ctx->Rsrc);
+  if (ctx->Rsrc) ctx->launch.source(ctx->stream, ctx->geo, ctx->phys, ctx->U, ctx->Rsrc, ctx->src_li);
   if (ev) (void)hipEventRecord(ev[0], ctx->stream);
   if (ctx->ns()) {
     if (ctx->prob.viscous_flux == SDG_VISC_BR1)
@@ -682,10 +682,6 @@ int validateProblem(const sdg_problem_desc* p, std::string& msg) {
   if (p->source_term != SDG_SOURCE_NONE && p->source_term != SDG_SOURCE_BOUSSINESQ) {
     msg = "unknown source term";
     return SDG_E_INVALID;
-  }
-  if (p->source_term != SDG_SOURCE_NONE && p->precision != SDG_PREC_FP64) {
-    msg = "the Boussinesq source term runs in the FP64 mode only";
-    return SDG_E_UNSUPPORTED;
   }
   return SDG_OK;
 }
@@ -1096,9 +1092,10 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   }
   // modal Boussinesq source for the kernel sets that take it from a buffer (the sum-factorised
   // kernels form it at their nodes)
-  if (ctx->phys.source != SDG_SOURCE_NONE && !ctx->launch.tensor &&
-      allocate(ctx, static_cast<size_t>(E) * ctx->nb, &ctx->Rsrc))
-    return bail(SDG_E_NOMEM);
+  if (ctx->phys.source != SDG_SOURCE_NONE && !ctx->launch.tensor) {
+    if (allocate(ctx, static_cast<size_t>(E) * ctx->nb, &ctx->Rsrc)) return bail(SDG_E_NOMEM);
+    if (ctx->mixed && upload(ctx, r.mass_chol_inverse, &ctx->src_li)) return bail(SDG_E_NOMEM);
+  }
   hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
     ctx->err = "sdg_create: device initialisation failed";

@@ -1,8 +1,8 @@
 """Boussinesq source term on the GPU (SourceTerm.cpp:29-58): residual and 5-step parity with the
 oracle on every kernel family that carries it -- vector kernels (Euler and Navier-Stokes, small
 bases), matrix-core kernels (Euler and Navier-Stokes quadrangles, dense hexahedra) and the
-sum-factorised hexahedron kernels (the source formed at their nodes) -- and the refusals of the
-paths without it (mixed precision, batch, adjoint)."""
+sum-factorised hexahedron kernels (the source formed at their nodes) --, the mixed mode, and the
+refusal of the adjoint."""
 import numpy as np
 import pytest
 
@@ -89,10 +89,31 @@ def test_source_with_isothermal_walls_on_hexahedra():
     P.fp64_parity(s.get_state(), uo, x.get_state(), uo, 3, tol=TOL)
 
 
+@pytest.mark.parametrize("element", [L.ELEM_QUADRANGLE, L.ELEM_HEXAHEDRON], ids=["quad-p3", "hex-p3"])
+def test_source_in_the_mixed_mode(element):
+    """20 mixed-precision steps with the source against the FP64 oracle (quadrangles: the dense
+    matrix-core kernels with the source against the orthonormal test functions; hexahedra: the
+    sum-factorised kernels with the nodal source)."""
+    mesh, prob = _box(element, 3, 3 if element == L.ELEM_HEXAHEDRON else 4, L.EQ_NS, beta=0.7, tref=0.98,
+                      precision=L.PREC_MIXED)
+    s = L.Solver(prob, mesh)
+    s.set_initial_primitive(_wavy_primitive(mesh))
+    p64 = L.ProblemDesc.from_buffer_copy(prob)
+    p64.precision = L.PREC_FP64
+    o = O.Oracle(p64, mesh)
+    o.set_state(s.get_state())
+    dt = o.compute_dt(0.1)
+    o.step(dt, 20)
+    s.step(dt, 20)
+    nv = mesh.nv
+    a, b = s.get_state().reshape(-1, nv), o.get_state().reshape(-1, nv)
+    dev = max(np.linalg.norm(a[:, g] - b[:, g]) / np.linalg.norm(b[:, g])
+              for g in ([0], list(range(1, nv - 1)), [nv - 1]))
+    print(f"  mixed L2 deviation {dev:.3e}", flush=True)
+    assert dev <= 1e-6, dev
+
+
 def test_paths_without_a_source_refuse_it():
-    mesh, prob = _box(L.ELEM_QUADRANGLE, 3, 4, L.EQ_NS, beta=0.7, tref=0.98, precision=L.PREC_MIXED)
-    with pytest.raises(L.SDGError):
-        L.Solver(prob, mesh)
     mesh, prob = _box(L.ELEM_QUADRANGLE, 3, 4, L.EQ_NS, beta=0.7, tref=0.98)
     s = L.Solver(prob, mesh)
     s.set_initial_primitive(_wavy_primitive(mesh))
