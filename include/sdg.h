@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SDG_ABI_VERSION 3
+#define SDG_ABI_VERSION 4
 
 /* ---- status codes ---------------------------------------------------------------------- */
 #define SDG_OK 0
@@ -66,6 +66,15 @@ enum sdg_precision { SDG_PREC_FP64 = 0, SDG_PREC_MIXED = 1 };
  * last momentum component (y in 2-D, z in 3-D), added to the residual as int S phi
  * (SourceTerm.cpp:29-58, SpatialDiscrete.cpp:254-262,1026-1029). */
 enum sdg_source_term { SDG_SOURCE_NONE = 0, SDG_SOURCE_BOUSSINESQ = 1 };
+/* ShockCapturingEnum (Enum.cpp:76-79), same order. Artificial viscosity (Persson-Peraire sensor on
+ * the density, SpatialDiscrete.cpp:37-192, ViscousFlux.cpp:105-190): once per step, per element
+ * s = log10(|rho_high|^2 / |rho|^2) over the modes above order P-1, eps_e = f r / P times 0, 1 or
+ * (1 + sin(pi (s - s0) / (2 k))) / 2 below, above or inside [s0 - k, s0 + k], s0 = log10(1 / P^4)
+ * (0 for P = 1), k the empirical tolerance, f the factor, r the inner radius; the node value is
+ * the maximum over the elements that share the node, and the P1 interpolant of the node values
+ * times the volume gradient of the conserved variables is subtracted from the fluxes (central
+ * average on interior faces, the element's own value on boundary faces). Euler, FP64. */
+enum sdg_shock_capturing { SDG_SHOCK_NONE = 0, SDG_SHOCK_ARTIFICIAL_VISCOSITY = 1 };
 /* InitialConditionEnum (Enum.cpp), same order: the file-based starts of sdg_raw_binary_read */
 enum sdg_initial_condition { SDG_IC_FUNCTION = 0, SDG_IC_SPECIFIC_FILE = 1, SDG_IC_LAST_STEP = 2 };
 /* BoundaryConditionEnum (Enum.cpp:92-100), same order */
@@ -102,6 +111,11 @@ typedef struct sdg_problem_desc {
   int pad_;
   double thermal_expansion_coefficient; /* Boussinesq beta                                */
   double reference_temperature;         /* Boussinesq T_ref                               */
+  int shock_capturing;    /* enum sdg_shock_capturing (setArtificialViscosity,            */
+                          /* SystemControl.cpp:105-108)                                   */
+  int pad2_;
+  double empirical_tolerance;           /* k above                                        */
+  double artificial_viscosity_factor;   /* f above (the reference's default is 1)         */
 } sdg_problem_desc;
 
 /* Structured mesh on an axis-aligned box: the project-side replacement for the gmsh
@@ -133,6 +147,9 @@ typedef struct sdg_mesh_info {
   int face_quadrature;    /* N_fq  (points per face, Gauss order 2P+1)                    */
   int affine;             /* 1 when every element has a constant Jacobian                 */
   int64_t boundary_faces; /* boundary faces F_b                                           */
+  int64_t nodes;          /* mesh nodes (element vertices; periodic images are distinct)  */
+  int vertices;           /* vertices per element (kBasicNodeNumber)                      */
+  int pad_;
 } sdg_mesh_info;
 
 /* Operand tables a mesh exports (all FP64 unless noted). Shapes:
@@ -161,6 +178,11 @@ typedef struct sdg_mesh_info {
  *   BOUNDARY_FACE_JACOBIAN_W F_b x N_fq      |J_f| w
  *   BOUNDARY_FACE_PARENT (int32) F_b x 2     element, its local face index
  *   BOUNDARY_FACE_TYPE (int32)   F_b         enum sdg_boundary
+ *   INNER_RADIUS       E               inner_radius_ (gmsh "innerRadius", Geometry.cpp:29-42):
+ *                                      here D |element| / |boundary|
+ *   NODE_TAG (int32)   E x nvert       0-based node of each vertex (node_tag_ - 1)
+ *   NODAL_VALUE        N_q x nvert     P1 nodal functions at the volume points  BasisFunction.cpp:198-207
+ *   NODAL_FACE_VALUE   (N_f*N_fq) x nvert  ... at the face points in parent coordinates  :161-180
  */
 enum sdg_mesh_field {
   SDG_FIELD_BASIS_VALUE = 0,
@@ -189,7 +211,11 @@ enum sdg_mesh_field {
   SDG_FIELD_BOUNDARY_FACE_PARENT = 23,
   SDG_FIELD_BOUNDARY_FACE_TYPE = 24,
   SDG_FIELD_ELEMENT_TYPE = 25,    /* (int32) E: enum sdg_element per element */
-  SDG_FIELD_COUNT_ = 26
+  SDG_FIELD_INNER_RADIUS = 26,
+  SDG_FIELD_NODE_TAG = 27,
+  SDG_FIELD_NODAL_VALUE = 28,
+  SDG_FIELD_NODAL_FACE_VALUE = 29,
+  SDG_FIELD_COUNT_ = 30
 };
 
 /* Split grid (MeshModelEnum::TriangleQuadrangle meshes of the reference): a periodic structured
@@ -266,6 +292,10 @@ int sdg_set_boundary_primitive(sdg_ctx* ctx, const double* primitive_at_boundary
  * element column-major, row var*D+dim (variable_gradient_basis_function_coefficient_,
  * TimeIntegration.cpp:200-228). */
 int sdg_get_gradient(sdg_ctx* ctx, double* gradient);
+/* Artificial viscosity only: the node values of the last evaluation (node_artificial_viscosity_,
+ * SolveControl.cpp:302), mesh nodes long. They are formed at the start of every step
+ * (TimeIntegration.cpp:336-338) and of every sdg_residual call. */
+int sdg_get_artificial_viscosity(sdg_ctx* ctx, double* node_values);
 
 /* Advance nsteps explicit steps of size dt: stepSolver (TimeIntegration.cpp:327-350) repeated,
  * including the residual monitor calculateRelativeError (TimeIntegration.cpp:279-324).
@@ -296,7 +326,8 @@ int sdg_time_kernels(sdg_ctx* ctx, double dt, int nsteps, double* avg_ms, int n_
  *   per element, in mesh order: the NV x N_b coefficients (column-major, as sdg_get_state), and
  *     for Navier-Stokes the (NV D) x N_b lifted-gradient coefficients (as sdg_get_gradient);
  *   per boundary face: the same blocks of its parent element (RawBinary.cpp:95-123);
- *   node_count doubles of nodal artificial viscosity (zero: no shock capturing).
+ *   node_count doubles of nodal artificial viscosity: with shock capturing node_count must be the
+ *     mesh's node count and the values are those of the last step; without it they are zero.
  * Replaces Solver::writeRawBinary (RawBinary.cpp:165-191) and RawBinaryCompress::write (:42-55).
  * Navier-Stokes BR2 meshes with boundary faces are refused (SDG_E_UNSUPPORTED): their per-face
  * gradient entries (volume part plus that face's lift) are not formed by the kernels. */

@@ -139,7 +139,9 @@ struct System {
   static_assert(SC::kEquationModel == EquationModelEnum::CompresibleEuler ||
                     SC::kEquationModel == EquationModelEnum::CompresibleNS,
                 "the GPU engine implements the compressible Euler and Navier-Stokes models");
-  static_assert(SC::kShockCapturing == ShockCapturingEnum::None, "artificial viscosity is not implemented");
+  static_assert(SC::kShockCapturing == ShockCapturingEnum::None ||
+                    SC::kEquationModel == EquationModelEnum::CompresibleEuler,
+                "the GPU engine runs artificial viscosity with the compressible Euler model");
 
   InitialCondition<SC> initial_condition_;
   BoundaryCondition<SC> boundary_condition_;
@@ -167,6 +169,8 @@ struct System {
     problem_.transport = static_cast<int>(SC::kTransportModel);
     problem_.precision = SDG_PREC_FP64;
     problem_.source_term = static_cast<int>(SC::kSourceTerm);
+    problem_.shock_capturing = static_cast<int>(SC::kShockCapturing);
+    problem_.artificial_viscosity_factor = 1.0;
   }
   ~System() {
     if (ctx_) sdg_destroy(ctx_);
@@ -214,6 +218,13 @@ struct System {
     problem_.thermal_expansion_coefficient = thermal_expansion_coefficient;
     problem_.reference_temperature = reference_temperature;
   }
+  // SystemControl.cpp:105-108: empirical tolerance k of the sensor ramp and the viscosity factor
+  void setArtificialViscosity(const Real empirical_tolerance, const Real artificial_viscosity_factor = 1.0) {
+    static_assert(SC::kShockCapturing == ShockCapturingEnum::ArtificialViscosity,
+                  "setArtificialViscosity needs ShockCapturingEnum::ArtificialViscosity");
+    problem_.empirical_tolerance = empirical_tolerance;
+    problem_.artificial_viscosity_factor = artificial_viscosity_factor;
+  }
   // SystemControl.cpp:110-121 (no interactive prompt: the range is required)
   void setTimeIntegration(const Real courant_friedrichs_lewy_number, const std::pair<int, int> iteration_range) {
     cfl_ = courant_friedrichs_lewy_number;
@@ -243,13 +254,13 @@ struct System {
   std::string rawPath(const int step) const {
     return output_directory_ + "/raw/" + output_prefix_ + "_" + std::to_string(step) + ".zst";
   }
-  // RawBinary.cpp:165-191 via the library; node count of the structured box's vertex grid
+  // RawBinary.cpp:165-191 via the library; node count of the mesh (the vertex grid of the box)
   void writeRaw(const int step) const {
     if (output_directory_.empty()) return;
     std::filesystem::create_directories(output_directory_ + "/raw");
-    int64_t nodes = 1;
-    for (int d = 0; d < SC::kDimension; d++) nodes *= mesh_desc_.cells[d] + 1;
-    check(sdg_raw_binary_write(ctx_, rawPath(step).c_str(), nodes), ctx_);
+    sdg_mesh_info info{};
+    check(sdg_mesh_get_info(mesh_, &info), nullptr);
+    check(sdg_raw_binary_write(ctx_, rawPath(step).c_str(), info.nodes), ctx_);
   }
   // Called after every step with the step number and the residual monitor.
   void setStepCallback(std::function<void(int, const std::array<Real, SC::kConservedVariableNumber>&)> f) {

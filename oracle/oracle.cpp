@@ -135,6 +135,15 @@ struct Oracle {
   int64_t nbface = 0;
   std::vector<int32_t> bparent, blocal, btype;
   std::vector<Real> bnormal, bjw, bdummy;
+  // artificial viscosity (ShockCapturingEnum::ArtificialViscosity): node tags (0-based), inner
+  // radii, the P1 nodal functions at volume and face points (row-major N x nvert), the tolerance
+  // k and factor f of setArtificialViscosity, node values and per-element vertex values
+  bool av = false;
+  int nvert = 0;
+  int64_t nnode = 0;
+  std::vector<int32_t> node_tag;
+  std::vector<Real> inner_radius, nodal_q, nodal_f, node_av, elem_av;
+  Real av_tol = 0.0, av_factor = 1.0;
 
   size_t szC() const { return static_cast<size_t>(nv) * nb; }
   size_t szG() const { return static_cast<size_t>(nv) * dim * nb; }
@@ -476,6 +485,11 @@ struct Oracle {
         gl.resize(static_cast<size_t>(rows) * nfq);
         faceGradientTrace(e, s, gl.data());
       }
+      std::vector<Real> vl;
+      if (av) {
+        vl.resize(static_cast<size_t>(rows) * nfq);
+        faceVolumeGradientTrace(e, s, vl.data());
+      }
       for (int j = 0; j < nfq; j++) {
         const size_t bp = static_cast<size_t>(b) * nfq + j;
         const Real* n = &bnormal[bp * dim];
@@ -497,9 +511,88 @@ struct Oracle {
         } else {
           for (int v = 0; v < nv; v++) tmp[v] = h[v] * bjw[bp];
         }
+        if (av) {  // the element's own value (SpatialDiscrete.cpp:812-830)
+          Real al[5];
+          avNormalFlux(n, avAt(nodal_f, accumulate(s) + j, e), &vl[static_cast<size_t>(rows) * j], al);
+          for (int v = 0; v < nv; v++) tmp[v] -= al[v] * bjw[bp];
+        }
         Real* cl = &adj_quad[static_cast<size_t>(e) * nv * naq + static_cast<size_t>(nv) * (accumulate(s) + j)];
         for (int v = 0; v < nv; v++) cl[v] = tmp[v];
       }
+    }
+  }
+
+  // --- artificial viscosity (SpatialDiscrete.cpp:37-192) ---
+  // kArtificialViscosityTolerance (SimulationControl.cpp:892-893): log10(1/P^4) to 12 digits, 0 for P = 1
+  static Real sensorThreshold(int p) {
+    static const Real t[5] = {0.0, -1.20411998266, -1.90848501888, -2.40823996531, -2.79588001734};
+    return t[p - 1];
+  }
+  // number of basis functions of order P-1 (the hierarchic basis of order P-1 is a prefix)
+  int lowModes() const {
+    if (order == 1) return 0;
+    const int p = order - 1;
+    if (dim == 2 && nvert == 3) return (p + 1) * (p + 2) / 2;
+    return dim == 2 ? (p + 1) * (p + 1) : (p + 1) * (p + 1) * (p + 1);
+  }
+  void artificialViscosity() {
+    const Real s0 = sensorThreshold(order);
+    const int lo = lowModes();
+    std::vector<Real> eps(static_cast<size_t>(ne));
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; e++) {  // calculateElementArtificialViscosity, :37-88
+      Real num = 0.0, den = 0.0;
+      for (int q = 0; q < nq; q++) {
+        Real all = 0.0, high = 0.0;
+        for (int b = 0; b < nb; b++) all += V[q + static_cast<size_t>(nq) * b] * coeff[e * szC() + nv * b];
+        for (int b = lo; b < nb; b++) high += V[q + static_cast<size_t>(nq) * b] * coeff[e * szC() + nv * b];
+        const Real jw = detjw[static_cast<size_t>(e) * nq + q];
+        num += high * (high * jw);
+        den += all * (all * jw);
+      }
+      const Real s = std::log10(num / den);
+      const Real top = av_factor * (inner_radius[e] / static_cast<Real>(order));
+      if (s < s0 - av_tol)
+        eps[e] = 0.0;
+      else if (s > s0 + av_tol)
+        eps[e] = top;
+      else
+        eps[e] = top * (1.0 + std::sin(Real(3.14159265358979323846) * (s - s0) / (2.0 * av_tol))) / 2.0;
+    }
+    std::fill(node_av.begin(), node_av.end(), Real(0.0));  // maxElementArtificialViscosity, :90-107
+    for (int64_t e = 0; e < ne; e++)
+      for (int j = 0; j < nvert; j++) {
+        Real& n = node_av[node_tag[static_cast<size_t>(e) * nvert + j]];
+        n = std::max(n, eps[e]);
+      }
+    for (int64_t e = 0; e < ne; e++)  // storeElementArtificialViscosity, :109-121
+      for (int j = 0; j < nvert; j++)
+        elem_av[static_cast<size_t>(e) * nvert + j] = node_av[node_tag[static_cast<size_t>(e) * nvert + j]];
+  }
+  // P1 interpolant of the element's vertex values at row `row` of a nodal table
+  Real avAt(const std::vector<Real>& table, int row, int64_t e) const {
+    Real s = 0.0;
+    for (int j = 0; j < nvert; j++)
+      s += table[static_cast<size_t>(row) * nvert + j] * elem_av[static_cast<size_t>(e) * nvert + j];
+    return s;
+  }
+  // volume-gradient coefficients at the face points of side s (get<ViscousFluxEnum::None>)
+  void faceVolumeGradientTrace(int64_t e, int s, Real* gt) const {
+    const int rows = nv * dim;
+    for (int j = 0; j < nfq; j++)
+      for (int r = 0; r < rows; r++) {
+        Real acc = 0.0;
+        for (int i = 0; i < nb; i++)
+          acc += vg_coeff[e * szG() + r + static_cast<size_t>(rows) * i] * Vf[(accumulate(s) + j) + static_cast<size_t>(naq) * i];
+        gt[r + static_cast<size_t>(rows) * j] = acc;
+      }
+  }
+  // artificial viscous normal flux eps g^T n per variable (ViscousFlux.cpp:105-134)
+  void avNormalFlux(const Real* n, Real eps, const Real* g, Real* out) const {
+    for (int v = 0; v < nv; v++) {
+      Real s = 0.0;
+      for (int a = 0; a < dim; a++) s += (eps * g[v * dim + a]) * n[a];
+      out[v] = s;
     }
   }
 
@@ -622,6 +715,11 @@ struct Oracle {
         gq.resize(static_cast<size_t>(rows) * nq);
         gemmBt(gq.data(), &g_coeff[e * szG()], V.data(), rows, nb, nq, false);
       }
+      std::vector<Real> gv;
+      if (av) {
+        gv.resize(static_cast<size_t>(rows) * nq);
+        gemmBt(gv.data(), &vg_coeff[e * szG()], V.data(), rows, nb, nq, false);
+      }
       Real* out = &vol_quad[static_cast<size_t>(e) * nv * dim * nq];
       for (int q = 0; q < nq; q++) {
         Real w[8], fc[15], fv[15], pg[15];
@@ -653,6 +751,16 @@ struct Oracle {
             }
             out[v + static_cast<size_t>(nv) * (q * dim + b)] = s;
           }
+        if (av) {  // - (eps grad U)^T G (SpatialDiscrete.cpp:211-215,226-228,245-249)
+          const Real eps = avAt(nodal_q, q, e);
+          const Real* g = &gv[static_cast<size_t>(rows) * q];
+          for (int v = 0; v < nv; v++)
+            for (int b = 0; b < dim; b++) {
+              Real s = 0.0;
+              for (int a = 0; a < dim; a++) s += (eps * g[v * dim + a]) * G[a + dim * b];
+              out[v + static_cast<size_t>(nv) * (q * dim + b)] -= s;
+            }
+        }
       }
     }
   }
@@ -693,6 +801,13 @@ struct Oracle {
         faceGradientTrace(L, sl, gl.data());
         faceGradientTrace(R, sr, gr.data());
       }
+      std::vector<Real> vl, vr;
+      if (av) {
+        vl.resize(static_cast<size_t>(rows) * nfq);
+        vr.resize(static_cast<size_t>(rows) * nfq);
+        faceVolumeGradientTrace(L, sl, vl.data());
+        faceVolumeGradientTrace(R, sr, vr.data());
+      }
       for (int j = 0; j < nfq; j++) {
         const int k = fseq[f * nfq + j];
         const Real* n = &fnormal[(static_cast<size_t>(f) * nfq + j) * dim];
@@ -707,6 +822,12 @@ struct Oracle {
           for (int v = 0; v < nv; v++) tmp[v] = (h[v] - (fl[v] + fr[v]) / 2.0) * fjw[f * nfq + j];
         } else {
           for (int v = 0; v < nv; v++) tmp[v] = h[v] * fjw[f * nfq + j];
+        }
+        if (av) {  // central average of the two sides (ViscousFlux.cpp:172-187, SpatialDiscrete.cpp:714-735)
+          Real al[5], ar[5];
+          avNormalFlux(n, avAt(nodal_f, accumulate(sl) + j, L), &vl[static_cast<size_t>(rows) * j], al);
+          avNormalFlux(n, avAt(nodal_f, accumulate(sr) + k, R), &vr[static_cast<size_t>(rows) * k], ar);
+          for (int v = 0; v < nv; v++) tmp[v] -= ((al[v] + ar[v]) / 2.0) * fjw[f * nfq + j];
         }
         Real* cl = &adj_quad[static_cast<size_t>(L) * nv * naq + static_cast<size_t>(nv) * (accumulate(sl) + j)];
         Real* cr = &adj_quad[static_cast<size_t>(R) * nv * naq + static_cast<size_t>(nv) * (accumulate(sr) + k)];
@@ -787,6 +908,7 @@ struct Oracle {
   }
   void step(Real dt) {  // TimeIntegration.cpp:327-350
     coeff_last = coeff;
+    if (av) artificialViscosity();
     for (int s = 0; s < stages(); s++) {
       spatial();
       updateCoefficient(s, dt);
@@ -960,8 +1082,42 @@ int oracle_set_initial_primitive(void* h, const Real* prim) {
   o->has_state = true;
   return 0;
 }
+// Artificial viscosity: node tags (E x nvert, 0-based), inner radii (E), P1 nodal functions at
+// the volume points (N_q x nvert) and face points (N_aq x nvert), row-major; tolerance k, factor f.
+int oracle_set_artificial_viscosity(void* h, int nvert, int64_t nnode, const int32_t* tag, const Real* radius,
+                                    const Real* nodal_q, const Real* nodal_f, Real tol, Real factor) {
+  Oracle* o = static_cast<Oracle*>(h);
+  o->av = true;
+  o->nvert = nvert;
+  o->nnode = nnode;
+  o->node_tag.assign(tag, tag + o->ne * nvert);
+  o->inner_radius.assign(radius, radius + o->ne);
+  o->nodal_q.assign(nodal_q, nodal_q + static_cast<size_t>(o->nq) * nvert);
+  o->nodal_f.assign(nodal_f, nodal_f + static_cast<size_t>(o->naq) * nvert);
+  o->node_av.assign(nnode, 0.0);
+  o->elem_av.assign(static_cast<size_t>(o->ne) * nvert, 0.0);
+  o->av_tol = tol;
+  o->av_factor = factor;
+  return 0;
+}
+// Volume-gradient coefficients of the last residual evaluation, (NV D) x N_b per element
+// column-major (variable_volume_gradient_basis_function_coefficient_, TimeIntegration.cpp:205-206).
+int oracle_get_volume_gradient(void* h, Real* g) {
+  Oracle* o = static_cast<Oracle*>(h);
+  std::memcpy(g, o->vg_coeff.data(), o->vg_coeff.size() * sizeof(Real));
+  return 0;
+}
+int oracle_get_artificial_viscosity(void* h, Real* node_values) {
+  Oracle* o = static_cast<Oracle*>(h);
+  std::memcpy(node_values, o->node_av.data(), o->node_av.size() * sizeof(Real));
+  return 0;
+}
+
+// With artificial viscosity the node values are formed from the current state first, as at the
+// start of a step.
 int oracle_residual(void* h, Real* r) {
   Oracle* o = static_cast<Oracle*>(h);
+  if (o->av) o->artificialViscosity();
   o->spatial();
   if (r) std::memcpy(r, o->residual.data(), o->residual.size() * sizeof(Real));
   return 0;

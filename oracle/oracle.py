@@ -52,7 +52,7 @@ def load(extended: bool = False):
             [c_void_p] * 13
         lib.oracle_destroy.argtypes = [c_void_p]
         for name in ("oracle_set_state", "oracle_get_state", "oracle_set_initial_primitive", "oracle_residual",
-                     "oracle_get_gradient"):
+                     "oracle_get_gradient", "oracle_get_volume_gradient", "oracle_get_artificial_viscosity"):
             getattr(lib, name).argtypes = [c_void_p, c_void_p]
             getattr(lib, name).restype = c_int
         lib.oracle_compute_dt.argtypes = [c_void_p, real, POINTER(real)]
@@ -62,6 +62,8 @@ def load(extended: bool = False):
         lib.oracle_point_boundary_state.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
         lib.oracle_set_boundary.argtypes = [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]
         lib.oracle_set_boundary_primitive.argtypes = [c_void_p, c_void_p]
+        lib.oracle_set_artificial_viscosity.argtypes = [c_void_p, c_int, c_int64, c_void_p, c_void_p, c_void_p,
+                                                        c_void_p, real, real]
         lib.oracle_set_threads.argtypes = [c_int]
         lib.oracle_get_threads.restype = c_int
         _libs[extended] = lib
@@ -114,6 +116,17 @@ class Oracle:
             self.lib.oracle_set_boundary(self.h, self.Fb, _ptr(k["boundary_face_parent"]),
                                          _ptr(k["boundary_face_type"]), _ptr(k["boundary_face_normal"]),
                                          _ptr(k["boundary_face_jacobian_w"]))
+        self.av = getattr(problem, "shock_capturing", 0) == 1
+        if self.av:
+            for key in ("node_tag", "inner_radius", "nodal_value", "nodal_face_value"):
+                k[key] = mesh.field(key)
+                if k[key].dtype == np.float64:
+                    k[key] = np.ascontiguousarray(k[key], dtype=self.real)
+            self.nodes = mesh.nodes
+            self.lib.oracle_set_artificial_viscosity(
+                self.h, mesh.nvert, mesh.nodes, _ptr(k["node_tag"]), _ptr(k["inner_radius"]),
+                _ptr(k["nodal_value"]), _ptr(k["nodal_face_value"]), problem.empirical_tolerance,
+                problem.artificial_viscosity_factor)
 
     def set_boundary_primitive(self, prim: np.ndarray):
         prim = np.ascontiguousarray(prim, dtype=self.real)
@@ -154,6 +167,18 @@ class Oracle:
         rc = self.lib.oracle_get_gradient(self.h, _ptr(out))
         if rc != 0:
             raise RuntimeError("gradient is only defined for Navier-Stokes")
+        return out.astype(np.float64)
+
+    def volume_gradient(self) -> np.ndarray:
+        """Volume-gradient coefficients of the last residual or stage, (E, N_b, NV D)."""
+        out = self._out((self.E, self.nb, self.nv * self.dim))
+        self.lib.oracle_get_volume_gradient(self.h, _ptr(out))
+        return out.astype(np.float64)
+
+    def artificial_viscosity(self) -> np.ndarray:
+        """Node values of the last artificial-viscosity evaluation."""
+        out = self._out(self.nodes)
+        self.lib.oracle_get_artificial_viscosity(self.h, _ptr(out))
         return out.astype(np.float64)
 
     def compute_dt(self, cfl: float) -> float:

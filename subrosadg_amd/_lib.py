@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsdg_hip.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # status codes
 OK, E_INVALID, E_UNSUPPORTED, E_DEVICE, E_NOMEM, E_STATE = 0, -1, -2, -3, -4, -5
@@ -30,6 +30,7 @@ EOS_IDEAL_GAS, EOS_WEAK_COMPRESSIBLE = 0, 1
 TRANSPORT_NONE, TRANSPORT_CONSTANT, TRANSPORT_SUTHERLAND = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 SOURCE_NONE, SOURCE_BOUSSINESQ = 0, 1
+SHOCK_NONE, SHOCK_ARTIFICIAL_VISCOSITY = 0, 1
 IC_FUNCTION, IC_SPECIFIC_FILE, IC_LAST_STEP = 0, 1, 2
 (BC_RIEMANN_FARFIELD, BC_VELOCITY_INFLOW, BC_PRESSURE_OUTFLOW, BC_ISOTHERMAL_NONSLIP_WALL, BC_ADIABATIC_SLIP_WALL,
  BC_ADIABATIC_NONSLIP_WALL, BC_PERIODIC) = range(7)
@@ -61,6 +62,10 @@ FIELDS = {
     "boundary_face_parent": (23, np.int32),
     "boundary_face_type": (24, np.int32),
     "element_type": (25, np.int32),
+    "inner_radius": (26, np.float64),
+    "node_tag": (27, np.int32),
+    "nodal_value": (28, np.float64),
+    "nodal_face_value": (29, np.float64),
 }
 
 
@@ -83,6 +88,10 @@ class ProblemDesc(Structure):
         ("pad_", c_int),
         ("thermal_expansion_coefficient", c_double),
         ("reference_temperature", c_double),
+        ("shock_capturing", c_int),
+        ("pad2_", c_int),
+        ("empirical_tolerance", c_double),
+        ("artificial_viscosity_factor", c_double),
     ]
 
 
@@ -122,6 +131,9 @@ class MeshInfo(Structure):
         ("face_quadrature", c_int),
         ("affine", c_int),
         ("boundary_faces", c_int64),
+        ("nodes", c_int64),
+        ("vertices", c_int),
+        ("pad_", c_int),
     ]
 
 
@@ -143,6 +155,7 @@ SIGNATURES = {
     "sdg_residual": (c_int, [c_void_p, c_void_p]),
     "sdg_set_boundary_primitive": (c_int, [c_void_p, c_void_p]),
     "sdg_get_gradient": (c_int, [c_void_p, c_void_p]),
+    "sdg_get_artificial_viscosity": (c_int, [c_void_p, c_void_p]),
     "sdg_step": (c_int, [c_void_p, c_double, c_int, c_void_p, POINTER(c_int), POINTER(c_int)]),
     "sdg_last_step_ms": (c_int, [c_void_p, POINTER(c_double)]),
     "sdg_synchronize": (c_int, [c_void_p]),
@@ -222,6 +235,9 @@ def problem_desc(**kw) -> ProblemDesc:
     p.source_term = kw.get("source_term", SOURCE_NONE)
     p.thermal_expansion_coefficient = kw.get("thermal_expansion_coefficient", 0.0)
     p.reference_temperature = kw.get("reference_temperature", 0.0)
+    p.shock_capturing = kw.get("shock_capturing", SHOCK_NONE)
+    p.empirical_tolerance = kw.get("empirical_tolerance", 0.0)
+    p.artificial_viscosity_factor = kw.get("artificial_viscosity_factor", 1.0)
     return p
 
 
@@ -283,6 +299,8 @@ class Mesh:
         self.nf = info.faces_per_element
         self.nfq = info.face_quadrature
         self.Fb = info.boundary_faces
+        self.nodes = info.nodes
+        self.nvert = info.vertices
         self.nv = self.dim + 2
         self.elem = info.element
         self.mixed = info.element == ELEM_MIXED_TRIANGLE_QUADRANGLE
@@ -345,6 +363,12 @@ class Solver:
     def gradient(self) -> np.ndarray:
         out = np.empty((self.E, self.nb, self.nv * self.dim))
         self._check(self.lib.sdg_get_gradient(self.handle, out.ctypes.data))
+        return out
+
+    def artificial_viscosity(self) -> np.ndarray:
+        """Node values of the last artificial-viscosity evaluation (mesh nodes long)."""
+        out = np.empty(self.mesh.nodes)
+        self._check(self.lib.sdg_get_artificial_viscosity(self.handle, out.ctypes.data))
         return out
 
     def set_boundary_primitive(self, prim: np.ndarray):

@@ -99,7 +99,23 @@ struct Geo {
   // mixed-precision mode: the residual-to-update map L^-T of opf.minv kept in FP64, so the update's
   // hierarchic coefficients are formed without FP32 rounding of their (cancelling) terms
   const double* minv_mixed64;
+  // artificial viscosity: per element the node of each vertex and the inner radius, the P1 nodal
+  // functions at the volume and face points, and the node values of the current step
+  const int* ntag;        // E x nvert
+  const double* inrad;    // E
+  const double* nodq;     // N_q x nvert
+  const double* nodf;     // N_aq x nvert
+  const double* nodeav;   // nodes
+  int nvert;
+  int order;              // P
 };
+
+// P1 interpolant of the node values of element e at row `row` of a nodal table
+__device__ __forceinline__ double avAt(const Geo& g, const double* table, int row, int e) {
+  double s = 0.0;
+  for (int j = 0; j < g.nvert; j++) s += table[row * g.nvert + j] * g.nodeav[g.ntag[e * g.nvert + j]];
+  return s;
+}
 
 template <typename TC>
 __device__ __forceinline__ const MOpsT<TC>& opsOf(const Geo& g) {
@@ -547,12 +563,13 @@ __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageAr
     if (a.Wout && e < g.E) a.Wout[static_cast<int64_t>(e) * NG * NB + static_cast<int64_t>(i) * NG + r] = wt * idj;
   }
   __syncthreads();
-  // total gradient at volume points and lifted gradient traces at face points
+  // total gradient at volume points and lifted gradient traces at face points (VISC == 0,
+  // artificial viscosity: the volume gradient, get<ViscousFluxEnum::None>)
   for (int idx = tid; idx < EPB * NG * NQ; idx += kThreads) {
     const int le = idx / (NG * NQ), rem = idx % (NG * NQ);
     const int r = rem / NQ, q = rem % NQ;
     double s = 0.0;
-    for (int b = 0; b < NB; b++) s += sWt[le][r][b] * g.Vt[b * NQ + q];
+    for (int b = 0; b < NB; b++) s += (VISC == 0 ? sWv[le][r][b] : sWt[le][r][b]) * g.Vt[b * NQ + q];
     sR1[le][r][q] = s;
   }
   for (int idx = tid; idx < EPB * NG * NAQ; idx += kThreads) {
@@ -566,6 +583,8 @@ __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageAr
       double l = 0.0;
       for (int pp = 0; pp < NFQ; pp++) l += sJ[le][r][s * NFQ + pp] * g.Lf[(s * NFQ + pp) * NFQ + j];
       t += l / g.detj[e < g.E ? e : g.E - 1];
+    } else if (VISC == 0) {
+      for (int b = 0; b < NB; b++) t += sWv[le][r][b] * g.Vft[b * NAQ + sl];
     } else {
       for (int b = 0; b < NB; b++) t += sWt[le][r][b] * g.Vft[b * NAQ + sl];
     }
@@ -581,8 +600,10 @@ __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageAr
     for (int v = 0; v < NV; v++) u[v] = sUq[le][v][q];
     for (int r = 0; r < NG; r++) gq[r] = sR1[le][r][q];
     computational<D>(ph, u, w);
-    primitiveGradient<D>(ph, w, u, gq, pg);
-    viscousRawFlux<D>(ph, w, pg, fv);
+    if constexpr (VISC != 0) {
+      primitiveGradient<D>(ph, w, u, gq, pg);
+      viscousRawFlux<D>(ph, w, pg, fv);
+    }
     const double rho = w[0], p = w[D + 2];
     const double et = w[D + 1] + velocitySquared<D>(w) / 2.0;
     double fc[D * NV];
@@ -595,12 +616,23 @@ __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageAr
     double G[D * D];
     for (int i = 0; i < D * D; i++) G[i] = g.ginv[static_cast<int64_t>(e) * D * D + i] * jw;
     double* out = &sR2[le][0][0];
-    for (int v = 0; v < NV; v++)
-      for (int b = 0; b < D; b++) {
-        double s = 0.0;
-        for (int aa = 0; aa < D; aa++) s += (fc[aa + D * v] - fv[aa + D * v]) * G[aa + D * b];
-        out[v * (D * NQ) + q * D + b] = s;
-      }
+    if constexpr (VISC == 0) {  // F_c^T G - (eps grad U)^T G (SpatialDiscrete.cpp:211-249)
+      const double eps = avAt(g, g.nodq, q, e);
+      for (int v = 0; v < NV; v++)
+        for (int b = 0; b < D; b++) {
+          double s = 0.0, t = 0.0;
+          for (int aa = 0; aa < D; aa++) s += fc[aa + D * v] * G[aa + D * b];
+          for (int aa = 0; aa < D; aa++) t += (eps * gq[v * D + aa]) * G[aa + D * b];
+          out[v * (D * NQ) + q * D + b] = s - t;
+        }
+    } else {
+      for (int v = 0; v < NV; v++)
+        for (int b = 0; b < D; b++) {
+          double s = 0.0;
+          for (int aa = 0; aa < D; aa++) s += (fc[aa + D * v] - fv[aa + D * v]) * G[aa + D * b];
+          out[v * (D * NQ) + q * D + b] = s;
+        }
+    }
   }
   __syncthreads();
   for (int idx = tid; idx < EPB * NV * NB; idx += kThreads) {
@@ -617,7 +649,10 @@ __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageAr
 // ---- Navier-Stokes, kernel 2: face fluxes + residual + update + next traces -------------------
 // calculateAdjacencyQuadrature, calculateResidual, updateBasisFunctionCoefficient
 // (SpatialDiscrete.cpp:633-748,1016-1032; TimeIntegration.cpp:181-198)
-template <int D, int NB, int NQ, int NF, int NFQ, int EPB>
+// AV = true: Euler with artificial viscosity, where Gf holds the volume gradient; the face flux is
+// the convective one minus the central average of eps grad U . n (boundary faces: the element's
+// own value), SpatialDiscrete.cpp:714-735,812-830.
+template <int D, int NB, int NQ, int NF, int NFQ, int EPB, bool AV = false>
 __global__ __launch_bounds__(kThreads) void nsFluxKernel(Geo g, Phys ph, StageArgs a) {
   using S = Sizes<D, NB, NQ, NF, NFQ>;
   constexpr int NV = S::NV, NG = S::NG, NAQ = S::NAQ;
@@ -645,7 +680,16 @@ __global__ __launch_bounds__(kThreads) void nsFluxKernel(Geo g, Phys ph, StageAr
       double uo[NV], go[NG], t[NV];
       for (int v = 0; v < NV; v++) uo[v] = a.Uf_in[static_cast<int64_t>(e) * NV * NAQ + v * NAQ + sl];
       for (int r = 0; r < NG; r++) go[r] = a.Gf[static_cast<int64_t>(e) * NG * NAQ + r * NAQ + sl];
-      boundaryFlux<D, true>(ph, g, -1 - nbs, uo, go, false, 0.0, 0.0, t);
+      const int bp = -1 - nbs;
+      boundaryFlux<D, !AV>(ph, g, bp, uo, go, false, 0.0, 0.0, t);
+      if constexpr (AV) {
+        const double eps = avAt(g, g.nodf, sl, e), jw = g.bjw[bp];
+        for (int v = 0; v < NV; v++) {
+          double f = 0.0;
+          for (int d = 0; d < D; d++) f += (eps * go[v * D + d]) * g.bnormal[static_cast<int64_t>(bp) * D + d];
+          t[v] -= f * jw;
+        }
+      }
       for (int v = 0; v < NV; v++) sH[le][v][sl] = t[v];
       continue;
     }
@@ -669,6 +713,19 @@ __global__ __launch_bounds__(kThreads) void nsFluxKernel(Geo g, Phys ph, StageAr
     computational<D>(ph, ur, wr);
     for (int d = 0; d < D; d++) n[d] = g.fnormal[fp * D + d];
     convectiveFlux<D>(ph, n, ul, wl, ur, wr, h);
+    if constexpr (AV) {
+      const double el = avAt(g, g.nodf, lsl, static_cast<int>(lo)), er = avAt(g, g.nodf, rsl, static_cast<int>(ro));
+      const double jw = g.fjw[fp];
+      for (int v = 0; v < NV; v++) {
+        double al = 0.0, ar = 0.0;
+        for (int d = 0; d < D; d++) al += (el * gl[v * D + d]) * n[d];
+        for (int d = 0; d < D; d++) ar += (er * gr[v * D + d]) * n[d];
+        double t = h[v] * jw;
+        t -= ((al + ar) / 2.0) * jw;
+        sH[le][v][sl] = side == 0 ? t : -t;
+      }
+      continue;
+    }
     primitiveGradient<D>(ph, wl, ul, gl, pgl);
     primitiveGradient<D>(ph, wr, ur, gr, pgr);
     viscousRawFlux<D>(ph, wl, pgl, fvl);
@@ -696,6 +753,47 @@ __global__ __launch_bounds__(kThreads) void nsFluxKernel(Geo g, Phys ph, StageAr
   }
   __syncthreads();
   stageTail<D, NB, NQ, NF, NFQ, EPB>(g, a, e0, sU, sR);
+}
+
+// ---- artificial viscosity (SpatialDiscrete.cpp:37-192) ----------------------------------------
+__global__ __launch_bounds__(kThreads) void avClearKernel(double* node, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kThreads)
+    node[i] = 0.0;
+}
+
+// One thread per element: the density sensor s = log10(sum_q rho_h^2 |J|w / sum_q rho^2 |J|w) with
+// rho_h the part of the modes above order P-1, the element value eps_e, and its maximum into the
+// element's nodes (non-negative doubles order as their bit patterns, so the maximum is an integer
+// atomic on the bits).
+template <int D, int NB, int NQ>
+__global__ __launch_bounds__(kThreads) void avSensorKernel(Geo g, Phys ph, const double* U, double* node) {
+  constexpr int NV = D + 2;
+  const int e = blockIdx.x * kThreads + threadIdx.x;
+  if (e >= g.E) return;
+  const double* u = U + static_cast<int64_t>(e) * NV * NB;  // density row: the first N_b entries
+  const double dj = g.detj[e];
+  double num = 0.0, den = 0.0;
+  for (int q = 0; q < NQ; q++) {
+    double all = 0.0, high = 0.0;
+    for (int b = 0; b < NB; b++) all += g.Vt[b * NQ + q] * u[b];
+    for (int b = ph.av_low; b < NB; b++) high += g.Vt[b * NQ + q] * u[b];
+    const double jw = dj * g.wq[q];
+    num += high * (high * jw);
+    den += all * (all * jw);
+  }
+  const double s = log10(num / den);
+  const double top = ph.av_factor * (g.inrad[e] / static_cast<double>(g.order));
+  double eps;
+  if (s < ph.av_s0 - ph.av_tol)
+    eps = 0.0;
+  else if (s > ph.av_s0 + ph.av_tol)
+    eps = top;
+  else
+    eps = top * (1.0 + sin(3.14159265358979323846 * (s - ph.av_s0) / (2.0 * ph.av_tol))) / 2.0;
+  const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(eps));
+  for (int j = 0; j < g.nvert; j++)
+    atomicMax(reinterpret_cast<unsigned long long*>(node + g.ntag[e * g.nvert + j]), bits);
 }
 
 // ---- auxiliary kernels ---------------------------------------------------------------------

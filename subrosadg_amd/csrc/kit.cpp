@@ -7,6 +7,7 @@
 #include <complex>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <unordered_map>
 #include <utility>
 
@@ -439,6 +440,22 @@ bool buildRefElement(int type, int order, RefElement& ref, std::string& error) {
       for (int k = c; k < i; k++) acc -= Lc[i * nb + k] * Li[k * nb + c];
       Li[i * nb + c] = acc / Lc[i * nb + i];
     }
+  // order-1 nodal functions of the vertices: barycentric on the triangle, tensor P1 otherwise
+  auto nodal = [&](const double* xi, double* out) {
+    if (type == 2) {
+      out[0] = 1.0 - xi[0] - xi[1];
+      out[1] = xi[0];
+      out[2] = xi[1];
+      return;
+    }
+    double g[3];
+    for (int v = 0; v < ref.nvert; v++) p1Shape(dim, &ref.vertex[static_cast<size_t>(v) * dim], xi, out[v], g);
+  };
+  ref.nodal_value.assign(static_cast<size_t>(nq) * ref.nvert, 0.0);
+  for (int q = 0; q < nq; q++) nodal(&ref.qpoint[static_cast<size_t>(q) * dim], &ref.nodal_value[static_cast<size_t>(q) * ref.nvert]);
+  ref.nodal_face_value.assign(static_cast<size_t>(nf) * nfq * ref.nvert, 0.0);
+  for (int s = 0; s < nf * nfq; s++)
+    nodal(&ref.face_param[static_cast<size_t>(s) * dim], &ref.nodal_face_value[static_cast<size_t>(s) * ref.nvert]);
   return true;
 }
 
@@ -703,6 +720,7 @@ bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3]
         }
       }
   mesh.affine = true;
+  assignNodes(mesh);
   return true;
 }
 
@@ -879,7 +897,57 @@ bool singleTypeView(const MixedMesh& mm, Mesh& out) {
       out.elem_side[static_cast<size_t>(e) * nf + s] = mm.elem_side[static_cast<size_t>(e) * 4 + s];
     }
   out.affine = true;
+  assignNodes(out);
   return true;
+}
+
+void assignNodes(Mesh& mesh) {
+  const RefElement& ref = mesh.ref;
+  const int dim = ref.dim, nv = ref.nvert;
+  const int64_t E = mesh.elements;
+  // vertices coincide when their coordinates agree to a millionth of the smallest edge
+  double hmin = std::numeric_limits<double>::max();
+  for (int64_t e = 0; e < E; e++) hmin = std::min(hmin, mesh.min_edge[e]);
+  const double step = 1e-6 * hmin;
+  std::map<std::array<int64_t, 3>, int32_t> index;
+  mesh.node_tag.assign(static_cast<size_t>(E) * nv, 0);
+  for (int64_t e = 0; e < E; e++)
+    for (int v = 0; v < nv; v++) {
+      std::array<int64_t, 3> key = {0, 0, 0};
+      for (int d = 0; d < dim; d++)
+        key[d] = static_cast<int64_t>(std::llround(mesh.vertex[(static_cast<size_t>(e) * nv + v) * dim + d] / step));
+      auto it = index.find(key);
+      if (it == index.end()) it = index.emplace(key, static_cast<int32_t>(index.size())).first;
+      mesh.node_tag[static_cast<size_t>(e) * nv + v] = it->second;
+    }
+  mesh.nodes = static_cast<int64_t>(index.size());
+  // inner radius D |element| / |boundary|; reference measures: triangle 1/2, quadrangle 4, hexahedron 8
+  const double measure = ref.type == 2 ? 0.5 : (dim == 2 ? 4.0 : 8.0);
+  mesh.inner_radius.assign(E, 0.0);
+  for (int64_t e = 0; e < E; e++) {
+    const double* x = &mesh.vertex[static_cast<size_t>(e) * nv * dim];
+    double surface = 0.0;
+    for (int f = 0; f < ref.nf; f++) {
+      const int* fv = &ref.face_vertex[static_cast<size_t>(f) * ref.nfvert];
+      if (dim == 2) {
+        double s2 = 0.0;
+        for (int d = 0; d < 2; d++) {
+          const double t = x[fv[1] * 2 + d] - x[fv[0] * 2 + d];
+          s2 += t * t;
+        }
+        surface += std::sqrt(s2);
+      } else {  // planar parallelogram face a, b, c, d: |(b - a) x (d - a)|
+        double u[3], w[3];
+        for (int d = 0; d < 3; d++) {
+          u[d] = x[fv[1] * 3 + d] - x[fv[0] * 3 + d];
+          w[d] = x[fv[3] * 3 + d] - x[fv[0] * 3 + d];
+        }
+        const double c[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+        surface += std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+      }
+    }
+    mesh.inner_radius[e] = dim * (measure * mesh.detj[e]) / surface;
+  }
 }
 
 void elementGeometry(const Mesh& mesh, int64_t e, double* detjw, double* tiw, double* coord) {

@@ -62,6 +62,10 @@ struct RefElement {
   // the mass matrix of the hierarchic basis.
   std::vector<double> mass_chol;           // N_b x N_b  L
   std::vector<double> mass_chol_inverse;   // N_b x N_b  L^-1
+  // P1 Lagrange functions of the vertices (the reference's order-1 nodal basis on the basic
+  // nodes, BasisFunction.cpp:161-207): at the volume points and at the face points
+  std::vector<double> nodal_value;         // N_q x nvert
+  std::vector<double> nodal_face_value;    // (N_f*N_fq) x nvert
 };
 
 // Build the reference tables; returns false for an unsupported (type, order).
@@ -95,6 +99,14 @@ struct Mesh {
   std::vector<int32_t> elem_face;   // E x N_f
   std::vector<int8_t> elem_side;    // E x N_f
   bool affine = true;
+  // mesh nodes (the element vertices): per element the 0-based node index of each vertex, shared
+  // by the elements that meet at a point. Periodic images are distinct nodes, as gmsh keeps the
+  // nodes of a periodic side apart from their masters.
+  std::vector<int32_t> node_tag;    // E x nvert
+  int64_t nodes = 0;
+  // inner radius per element: D |element| / |boundary| (the inscribed radius of a triangle, of a
+  // square or cube, and of any polygon with an inscribed circle)
+  std::vector<double> inner_radius; // E
 };
 
 // Structured mesh on an axis-aligned box; periodic[d] != 0 makes direction d periodic, otherwise
@@ -126,6 +138,9 @@ bool buildSplitMesh(int order, const int n[2], const double lo[2], const double 
                     MixedMesh& mesh, std::string& error);
 // The mesh as a single-type Mesh; false when it holds both element types.
 bool singleTypeView(const MixedMesh& mm, Mesh& out);
+// Node tags (vertices that coincide in physical space share a node) and inner radii from the
+// element vertices and Jacobians of an affine mesh.
+void assignNodes(Mesh& mesh);
 
 // Geometry derived per element and volume point (affine elements: constant Jacobian).
 //   detjw[q]           = |J| w_q

@@ -25,6 +25,11 @@ struct Phys {
   int source;  // sdg_source_term
   int pad2_;
   double beta, tref;  // Boussinesq thermal expansion coefficient and reference temperature
+  // artificial viscosity: on/off, number of modes of order P-1, empirical tolerance k, factor f
+  // and the sensor threshold s0 (kArtificialViscosityTolerance, SimulationControl.cpp:892-897)
+  int shock;
+  int av_low;
+  double av_tol, av_factor, av_s0;
 };
 
 constexpr int kSourceBoussinesq = 1;  // sdg_source_term value of the Boussinesq source

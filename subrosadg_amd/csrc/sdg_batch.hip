@@ -655,6 +655,8 @@ int sdg_batch_create(const sdg_problem_desc* problem, const sdg_mesh* const* mes
   if (problem->abi_version != SDG_ABI_VERSION) return bfail(nullptr, SDG_E_INVALID, "sdg_batch_create: abi_version mismatch");
   if (problem->source_term != SDG_SOURCE_NONE)
     return bfail(nullptr, SDG_E_UNSUPPORTED, "sdg_batch_create: source terms are not supported by the batched solver");
+  if (problem->shock_capturing != SDG_SHOCK_NONE)
+    return bfail(nullptr, SDG_E_UNSUPPORTED, "sdg_batch_create: shock capturing is not supported by the batched solver");
   if (problem->equation != SDG_EQ_COMPRESSIBLE_EULER)
     return bfail(nullptr, SDG_E_UNSUPPORTED, "sdg_batch_create: the batched solver runs the compressible Euler model");
   if (problem->convective_flux < SDG_FLUX_CENTRAL || problem->convective_flux > SDG_FLUX_EXACT ||

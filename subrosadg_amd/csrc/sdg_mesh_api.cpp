@@ -139,6 +139,8 @@ int sdg_mesh_get_info(const sdg_mesh* mesh, sdg_mesh_info* info) {
   info->face_quadrature = m.ref.nfq;
   info->affine = m.affine ? 1 : 0;
   info->boundary_faces = static_cast<int64_t>(m.bfaces.size());
+  info->nodes = m.nodes;
+  info->vertices = m.ref.nvert;
   return SDG_OK;
 }
 
@@ -185,6 +187,10 @@ int64_t sdg_mesh_field_size(const sdg_mesh* mesh, int field) {
     case SDG_FIELD_BOUNDARY_FACE_JACOBIAN_W: return Fb * nfq;
     case SDG_FIELD_BOUNDARY_FACE_PARENT: return Fb * 2;
     case SDG_FIELD_BOUNDARY_FACE_TYPE: return Fb;
+    case SDG_FIELD_INNER_RADIUS: return E;
+    case SDG_FIELD_NODE_TAG: return E * mesh->mesh.ref.nvert;
+    case SDG_FIELD_NODAL_VALUE: return nq * mesh->mesh.ref.nvert;
+    case SDG_FIELD_NODAL_FACE_VALUE: return nf * nfq * mesh->mesh.ref.nvert;
     default: return 0;
   }
 }
@@ -325,6 +331,10 @@ int sdg_mesh_export(const sdg_mesh* mesh, int field, void* out, int64_t count) {
     case SDG_FIELD_BOUNDARY_FACE_TYPE:
       for (size_t f = 0; f < m.bfaces.size(); f++) oi[f] = m.bfaces[f].bc;
       break;
+    case SDG_FIELD_INNER_RADIUS: copy(m.inner_radius); break;
+    case SDG_FIELD_NODE_TAG: std::memcpy(oi, m.node_tag.data(), m.node_tag.size() * sizeof(int32_t)); break;
+    case SDG_FIELD_NODAL_VALUE: copy(r.nodal_value); break;
+    case SDG_FIELD_NODAL_FACE_VALUE: copy(r.nodal_face_value); break;
     default:
       sdg::setGlobalError("sdg_mesh_export: unknown field");
       return SDG_E_INVALID;

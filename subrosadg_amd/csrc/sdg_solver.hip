@@ -34,6 +34,7 @@ using DtFn = void (*)(hipStream_t, const Geo&, const Phys&, const double*, const
 using MonFn = void (*)(hipStream_t, const double*, int, double*, int*);
 using SrcFn = void (*)(hipStream_t, const Geo&, const Phys&, const double*, double*, const double*);
 using AdjFn = void (*)(hipStream_t, const Geo&, const Phys&, const StageArgs&, const sdg::AdjArgs&);
+using AvFn = void (*)(hipStream_t, const Geo&, const Phys&, const double*, double*);
 
 struct Launchers {
   bool mfma = false;
@@ -47,6 +48,11 @@ struct Launchers {
   SrcFn source = nullptr;  // modal Boussinesq source (all kernel sets but the sum-factorised one)
   DtFn dt = nullptr;
   MonFn monitor = nullptr;
+  // Euler with artificial viscosity (vector kernels): gradient pass + volume flux, face fluxes +
+  // residual + update, and the per-step sensor
+  StageFn av_grad = nullptr;
+  StageFn av_flux = nullptr;
+  AvFn av_sensor = nullptr;
   // discrete adjoint (Navier-Stokes, FP64 matrix-core sizes only)
   AdjFn adj_volume[2] = {nullptr, nullptr};  // BR1, BR2
   AdjFn adj_face = nullptr;
@@ -130,6 +136,18 @@ Launchers makeLaunchers() {
     l.ns_flux = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
       hipLaunchKernelGGL((sdg::nsFluxKernel<D, NB, NQ, NF, NFQ, EPB>), dim3(blocksFor(g.E, EPB)), dim3(sdg::kThreads),
                          0, st, g, ph, a);
+    };
+    l.av_grad = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+      hipLaunchKernelGGL((sdg::nsGradKernel<D, NB, NQ, NF, NFQ, EPB, 0>), dim3(blocksFor(g.E, EPB)),
+                         dim3(sdg::kThreads), 0, st, g, ph, a);
+    };
+    l.av_flux = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+      hipLaunchKernelGGL((sdg::nsFluxKernel<D, NB, NQ, NF, NFQ, EPB, true>), dim3(blocksFor(g.E, EPB)),
+                         dim3(sdg::kThreads), 0, st, g, ph, a);
+    };
+    l.av_sensor = [](hipStream_t st, const Geo& g, const Phys& ph, const double* U, double* node) {
+      hipLaunchKernelGGL((sdg::avSensorKernel<D, NB, NQ>), dim3(blocksFor(g.E, sdg::kThreads)), dim3(sdg::kThreads), 0,
+                         st, g, ph, U, node);
     };
   }
   if constexpr (MFMA) {
@@ -380,6 +398,10 @@ struct sdg_ctx {
          *Vn = nullptr, *Rsrc = nullptr, *src_li = nullptr;
   double *Rout = nullptr, *Wout = nullptr, *mon = nullptr, *rel = nullptr, *dtbuf = nullptr;
   int* status = nullptr;
+  // artificial viscosity: node values (written once per step by the sensor kernel)
+  bool av = false;
+  double* nodeav = nullptr;
+  int64_t nnode = 0;
   // boundary faces: count and boundary data (computational variables per boundary face point)
   int64_t Fb = 0;
   double* bdata = nullptr;
@@ -561,6 +583,16 @@ void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hi
     }
     ctx->launch.ns_flux(ctx->stream, ctx->geo, ctx->phys, a);
     if (ev) (void)hipEventRecord(ev[5], ctx->stream);
+  } else if (ctx->av) {
+    ctx->launch.av_grad(ctx->stream, ctx->geo, ctx->phys, a);
+    if (ev) {
+      (void)hipEventRecord(ev[1], ctx->stream);
+      (void)hipEventRecord(ev[2], ctx->stream);
+      (void)hipEventRecord(ev[3], ctx->stream);
+      (void)hipEventRecord(ev[4], ctx->stream);
+    }
+    ctx->launch.av_flux(ctx->stream, ctx->geo, ctx->phys, a);
+    if (ev) (void)hipEventRecord(ev[5], ctx->stream);
   } else {
     ctx->launch.euler(ctx->stream, ctx->geo, ctx->phys, a);
     if (ev)
@@ -569,7 +601,18 @@ void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hi
   if (mode == 0) ctx->cur ^= 1;
 }
 
+// Node artificial viscosity from the current state (calculateArtificialViscosity,
+// SpatialDiscrete.cpp:123-192): clear with a kernel (the launch may be captured in a graph), then
+// the per-element sensor with its maximum into the nodes.
+void enqueueArtificialViscosity(sdg_ctx* ctx) {
+  if (!ctx->av) return;
+  hipLaunchKernelGGL(sdg::avClearKernel, dim3(static_cast<unsigned>(std::min<int64_t>((ctx->nnode + sdg::kThreads - 1) / sdg::kThreads, 1024))),
+                     dim3(sdg::kThreads), 0, ctx->stream, ctx->nodeav, ctx->nnode);
+  ctx->launch.av_sensor(ctx->stream, ctx->geo, ctx->phys, ctx->U, ctx->nodeav);
+}
+
 void enqueueStep(sdg_ctx* ctx, double dt) {
+  enqueueArtificialViscosity(ctx);  // once per step, before the stages (TimeIntegration.cpp:336-338)
   for (int s = 0; s < ctx->stages(); s++) enqueueStage(ctx, s, dt, 0, nullptr);
   ctx->launch.monitor(ctx->stream, ctx->mon, ctx->E, ctx->rel, ctx->status);
 }
@@ -683,6 +726,15 @@ int validateProblem(const sdg_problem_desc* p, std::string& msg) {
     msg = "unknown source term";
     return SDG_E_INVALID;
   }
+  if (p->shock_capturing != SDG_SHOCK_NONE && p->shock_capturing != SDG_SHOCK_ARTIFICIAL_VISCOSITY) {
+    msg = "unknown shock capturing";
+    return SDG_E_INVALID;
+  }
+  if (p->shock_capturing == SDG_SHOCK_ARTIFICIAL_VISCOSITY &&
+      (!(p->empirical_tolerance > 0.0) || !(p->artificial_viscosity_factor >= 0.0))) {
+    msg = "artificial viscosity needs a positive empirical tolerance and a non-negative factor";
+    return SDG_E_INVALID;
+  }
   return SDG_OK;
 }
 
@@ -724,7 +776,12 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   ctx->mixed = mixed;
   const char* dense = std::getenv("SDG_DENSE");  // 1: keep the dense matrix-core NS kernels on hexahedra
   const bool tensor = problem->equation == SDG_EQ_COMPRESSIBLE_NS && !(dense && dense[0] == '1');
-  if (!selectLaunchers(r.type, r.order, fv && fv[0] == '1', mixed, ctx->launch, tensor)) {
+  ctx->av = problem->shock_capturing == SDG_SHOCK_ARTIFICIAL_VISCOSITY;
+  if (ctx->av && (problem->equation != SDG_EQ_COMPRESSIBLE_EULER || mixed)) {
+    delete ctx;
+    return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: artificial viscosity runs with the Euler model in FP64");
+  }
+  if (!selectLaunchers(r.type, r.order, (fv && fv[0] == '1') || ctx->av, mixed, ctx->launch, tensor)) {
     delete ctx;
     return fail(nullptr, SDG_E_UNSUPPORTED,
                 mixed ? "sdg_create: the mixed-precision mode needs a matrix-core kernel set (quadrangle P3-P5, "
@@ -745,6 +802,16 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   ph.source = problem->source_term;
   ph.beta = problem->thermal_expansion_coefficient;
   ph.tref = problem->reference_temperature;
+  ph.shock = problem->shock_capturing;
+  ph.av_tol = problem->empirical_tolerance;
+  ph.av_factor = problem->artificial_viscosity_factor;
+  {
+    // kArtificialViscosityTolerance (SimulationControl.cpp:892-893): log10(1/P^4) to 12 digits
+    static const double kThreshold[5] = {0.0, -1.20411998266, -1.90848501888, -2.40823996531, -2.79588001734};
+    ph.av_s0 = kThreshold[r.order - 1];
+    const int p = r.order - 1;  // the hierarchic basis of order P-1 is a prefix of order P's
+    ph.av_low = p == 0 ? 0 : (r.type == SDG_ELEM_TRIANGLE ? (p + 1) * (p + 2) / 2 : (r.dim == 2 ? (p + 1) * (p + 1) : (p + 1) * (p + 1) * (p + 1)));
+  }
   const char* ng = std::getenv("SDG_NO_GRAPH");
   ctx->use_graph = !(ng && ng[0] == '1');
 
@@ -916,6 +983,22 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   g.floc = dfloc;
   g.E = ctx->E;
   g.F = static_cast<int>(F);
+  g.nvert = r.nvert;
+  g.order = r.order;
+  if (ctx->av) {
+    double *dnq, *dnf, *drad;
+    int* dtag;
+    std::vector<int> tag(mesh.node_tag.begin(), mesh.node_tag.end());
+    ctx->nnode = mesh.nodes;
+    if (upload(ctx, r.nodal_value, &dnq) || upload(ctx, r.nodal_face_value, &dnf) || upload(ctx, mesh.inner_radius, &drad) ||
+        upload(ctx, tag, &dtag) || allocate(ctx, static_cast<size_t>(ctx->nnode), &ctx->nodeav))
+      return bail(SDG_E_NOMEM);
+    g.nodq = dnq;
+    g.nodf = dnf;
+    g.inrad = drad;
+    g.ntag = dtag;
+    g.nodeav = ctx->nodeav;
+  }
   {
     const int nfq_ = nfq;
     auto V = [&](int q, int b) { return r.V[static_cast<size_t>(q) * nb + b]; };
@@ -1090,6 +1173,9 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     }
     if (tensor && (allocate(ctx, nUf, &ctx->Vn) || allocate(ctx, nU, &ctx->Rvol))) return bail(SDG_E_NOMEM);
   }
+  if (ctx->av &&
+      (allocate(ctx, static_cast<size_t>(E) * ctx->ng * naq, &ctx->Gf) || allocate(ctx, nU, &ctx->Rvol)))
+    return bail(SDG_E_NOMEM);
   // modal Boussinesq source for the kernel sets that take it from a buffer (the sum-factorised
   // kernels form it at their nodes)
   if (ctx->phys.source != SDG_SOURCE_NONE && !ctx->launch.tensor) {
@@ -1220,6 +1306,7 @@ int sdg_residual(sdg_ctx* ctx, double* residual) {
   if (!ctx->has_state) return fail(ctx, SDG_E_STATE, "sdg_residual: no state set");
   if (ctx->Fb > 0 && !ctx->has_bdata)
     return fail(ctx, SDG_E_STATE, "sdg_residual: boundary data not set (sdg_set_boundary_primitive)");
+  enqueueArtificialViscosity(ctx);
   enqueueStage(ctx, -1, 0.0, 1, nullptr);
   HIP_TRY(ctx, hipGetLastError());
   std::vector<double> dev(static_cast<size_t>(ctx->E) * ctx->nv * ctx->nb);
@@ -1325,6 +1412,7 @@ int sdg_time_kernels(sdg_ctx* ctx, double dt, int nsteps, double* avg_ms, int n_
   hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
   for (int i = 0; i < nsteps; i++) {
     hipEvent_t* e = &ev[per_step * i];
+    enqueueArtificialViscosity(ctx);
     for (int s = 0; s < st; s++) enqueueStage(ctx, s, dt, 0, nullptr, e + 6 * s);
     (void)hipEventRecord(e[6 * st], ctx->stream);
     ctx->launch.monitor(ctx->stream, ctx->mon, ctx->E, ctx->rel, ctx->status);
@@ -1349,8 +1437,17 @@ int sdg_time_kernels(sdg_ctx* ctx, double dt, int nsteps, double* avg_ms, int n_
   const double launches[SDG_KERNEL_CLASSES] = {double(nsteps) * st, double(nsteps) * st, double(nsteps) * st,
                                                double(nsteps)};
   for (int k = 0; k < n_out; k++) avg_ms[k] = k < SDG_KERNEL_CLASSES ? sum[k] / launches[k] : 0.0;
-  if (!ctx->ns())
+  if (!ctx->ns() && !ctx->av)
     for (int k = 1; k < n_out && k < 3; k++) avg_ms[k] = 0.0;
+  return SDG_OK;
+}
+
+int sdg_get_artificial_viscosity(sdg_ctx* ctx, double* node_values) {
+  if (!ctx || !node_values) return fail(ctx, SDG_E_INVALID, "sdg_get_artificial_viscosity: null argument");
+  if (!ctx->av) return fail(ctx, SDG_E_UNSUPPORTED, "sdg_get_artificial_viscosity: shock capturing is off");
+  HIP_TRY(ctx, hipMemcpyAsync(node_values, ctx->nodeav, static_cast<size_t>(ctx->nnode) * sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return SDG_OK;
 }
 
@@ -1362,6 +1459,8 @@ extern "C" {
 int sdg_raw_binary_write(sdg_ctx* ctx, const char* path, int64_t node_count) {
   if (!ctx || !path || node_count < 0) return fail(ctx, SDG_E_INVALID, "sdg_raw_binary_write: bad argument");
   if (!ctx->has_state) return fail(ctx, SDG_E_STATE, "sdg_raw_binary_write: no state set");
+  if (ctx->av && node_count != ctx->nnode)
+    return fail(ctx, SDG_E_INVALID, "sdg_raw_binary_write: with artificial viscosity node_count must be the mesh's node count");
   const bool ns = ctx->ns();
   if (ns && ctx->prob.viscous_flux == SDG_VISC_BR2 && ctx->Fb > 0)
     return fail(ctx, SDG_E_UNSUPPORTED,
@@ -1384,6 +1483,9 @@ int sdg_raw_binary_write(sdg_ctx* ctx, const char* path, int64_t node_count) {
   for (size_t e = 0; e < E; e++) block(e);                   // RawBinary.cpp:76-90
   for (int p : ctx->h_bparent) block(static_cast<size_t>(p));  // RawBinary.cpp:95-123, BR1 / Euler
   payload.resize(payload.size() + static_cast<size_t>(node_count), 0.0);  // node artificial viscosity
+  if (ctx->av && node_count > 0 &&
+      (rc = sdg_get_artificial_viscosity(ctx, payload.data() + payload.size() - static_cast<size_t>(node_count))))
+    return rc;
   rc = sdg::writeRawFile(path, payload.data(), payload.size() * sizeof(double));
   if (rc) return fail(ctx, rc, sdg::globalError());
   return SDG_OK;

@@ -231,3 +231,30 @@ def function_l2(mesh, a: np.ndarray, b: np.ndarray) -> float:
     for g in ([0], list(range(1, mesh.dim + 1)), [mesh.dim + 1]):
         out.append(np.sqrt((jw[..., None] * qa[..., g] ** 2).sum() / (jw[..., None] * qb[..., g] ** 2).sum()))
     return float(max(out))
+
+
+def sod_primitive(xq: np.ndarray, x0=0.47) -> np.ndarray:
+    """Sod's states (rho, u.., T) with R = 1: rho = 1, p = 1 left of x = x0 and rho = 0.125,
+    p = 0.1 right of it (T = p / rho), at rest."""
+    left = xq[..., 0] < x0
+    prim = np.zeros(xq.shape[:-1] + (xq.shape[-1] + 2,))
+    prim[..., 0] = np.where(left, 1.0, 0.125)
+    prim[..., -1] = np.where(left, 1.0, 0.8)
+    return prim
+
+
+def euler_av(element=L.ELEM_QUADRANGLE, n=6, order=3, flux=L.FLUX_HLLC, tol=0.5, factor=1.0, boundary=None,
+             split=False):
+    """2-D (or 3-D for hexahedra) Euler with artificial viscosity (setArtificialViscosity(tol,
+    factor)) on the unit box, Sod's jump at x = 0.47 inside an element (and the periodic image at x = 0)."""
+    prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=flux, cp=3.5, cv=2.5,
+                          shock_capturing=L.SHOCK_ARTIFICIAL_VISCOSITY, empirical_tolerance=tol,
+                          artificial_viscosity_factor=factor)
+    if split:
+        mesh = L.Mesh.split(order, (n, n), (0.0, 0.0), (1.0, 1.0), 1.0, 7)
+    elif element == L.ELEM_HEXAHEDRON:
+        mesh = L.Mesh(element, order, (n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), boundary=boundary)
+    else:
+        mesh = L.Mesh(element, order, (n, n), (0.0, 0.0), (1.0, 1.0), boundary=boundary)
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, mesh.dim)
+    return mesh, prob, sod_primitive(xq)
