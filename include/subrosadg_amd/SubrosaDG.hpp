@@ -370,6 +370,17 @@ struct System {
     check(sdg_get_state(ctx_, c.data()), ctx_);
     return c;
   }
+  // Node artificial viscosity of the last step (node_artificial_viscosity_, the reference's
+  // ViewVariableEnum::ArtificialViscosity output), mesh nodes long.
+  std::vector<Real> artificialViscosity() const {
+    static_assert(SC::kShockCapturing == ShockCapturingEnum::ArtificialViscosity,
+                  "artificialViscosity needs ShockCapturingEnum::ArtificialViscosity");
+    sdg_mesh_info info{};
+    check(sdg_mesh_get_info(mesh_, &info), nullptr);
+    std::vector<Real> v(static_cast<size_t>(info.nodes));
+    check(sdg_get_artificial_viscosity(ctx_, v.data()), ctx_);
+    return v;
+  }
 };
 
 }  // namespace SubrosaDG

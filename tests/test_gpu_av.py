@@ -199,3 +199,18 @@ def test_shock_tube_against_the_exact_solution():
     print(f"  Sod L1(rho) = {l1:.4f}, rho in [{rho.min():.3f}, {rho.max():.3f}]", flush=True)
     assert rho.min() > 0.1 and rho.max() < 1.05
     assert l1 < 0.01
+
+
+def test_cpp_explosion_example():
+    """The C++ drop-in driver with ShockCapturingEnum::ArtificialViscosity and
+    setArtificialViscosity(4.0): the blast stays finite and positive, and the viscosity is on at the
+    front only."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "examples", "bin",
+                       "explosion_2d_ceuler")
+    r = subprocess.run([exe, "24", "150"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    assert "explosion: 150 steps" in r.stdout
