@@ -5,7 +5,8 @@
 // The reference meshes the circle into the element edges (gmsh); on the structured grid the circle
 // cuts through elements, where the least-squares projection of the jump would undershoot to
 // negative pressure, so the initial jump is spread over one cell (a tanh profile of width h/2).
-// Usage: explosion_2d_ceuler [cells per direction] [steps]
+// Usage: explosion_2d_ceuler [cells per direction] [steps] [output directory]
+// With an output directory: raw files every steps/2 steps and their VTU files (system.view()).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -48,6 +49,7 @@ std::array<SubrosaDG::Real, 4> SubrosaDG::BoundaryCondition<SimulationControl>::
 int main(int argc, char* argv[]) {
   const int n = argc > 1 ? std::atoi(argv[1]) : 40;
   const int steps = argc > 2 ? std::atoi(argv[2]) : 200;
+  const std::string dir = argc > 3 ? argv[3] : "";
   g_width = 1.0 / n;
   try {
     SubrosaDG::System<SimulationControl> system;
@@ -56,6 +58,7 @@ int main(int argc, char* argv[]) {
     system.setThermodynamicModel<SimulationControl::kThermodynamicModel>(2.5, 25.0 / 14.0);
     system.setArtificialViscosity(4.0);
     system.setTimeIntegration(0.1, {0, steps});
+    if (!dir.empty()) system.setViewConfig(dir, "explosion", steps > 1 ? steps / 2 : 1);
     bool finite = true;
     system.setStepCallback([&finite, steps](int i, const std::array<double, 4>& r) {
       for (double v : r) finite = finite && std::isfinite(v);
@@ -63,6 +66,7 @@ int main(int argc, char* argv[]) {
     });
     system.synchronize();
     system.solve();
+    if (!dir.empty()) system.view();
     const auto av = system.artificialViscosity();
     const auto c = system.state();
     double rho_min = 1e300;  // vertex-mode density coefficients: the density at the element corners

@@ -342,6 +342,17 @@ int sdg_raw_binary_read(sdg_ctx* ctx, const char* path, int initial_condition);
 int sdg_raw_file_write(const char* path, const void* payload, int64_t bytes);
 int sdg_raw_file_read(const char* path, void* payload, int64_t capacity, int64_t* bytes);
 
+/* ---- VTU output (host only; no device work) ---------------------------------------------------
+ * Replaces View::stepView (Paraview.cpp:280-313): `coefficients` (reference layout, E x N_b x NV,
+ * e.g. from sdg_get_state) sampled at the equispaced points of order P of every element and written
+ * to `path` as an ASCII VTK XML unstructured grid of the P^D linear sub-cells of each element (P^2
+ * triangles), points per element. Point data: Density, Velocity (3 components), Pressure,
+ * Temperature, MachNumber, and ArtificialViscosity when node_values (mesh nodes long, e.g. from
+ * sdg_get_artificial_viscosity) is not NULL; field data TMSTEP = step and TimeValue = time.
+ * The reference writes VTK Lagrange cells of order P; the sub-cells carry the same point values. */
+int sdg_write_vtu(const sdg_mesh* mesh, const sdg_problem_desc* problem, const double* coefficients,
+                  const double* node_values, int step, double time, const char* path);
+
 /* ---- discrete adjoint (Navier-Stokes, FP64, matrix-core element sizes) -----------------------
  * The reference has no adjoint; these entry points transpose its residual (R of sdg_residual,
  * SpatialDiscrete.cpp:194-1116) and its RK stage update (TimeIntegration.cpp:181-198).

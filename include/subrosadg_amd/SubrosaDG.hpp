@@ -370,6 +370,36 @@ struct System {
     check(sdg_get_state(ctx_, c.data()), ctx_);
     return c;
   }
+  // View::view (Paraview.cpp:280-313): every raw file of the run, read back, written as
+  // <output_directory>/vtu/<prefix>_<step>.vtu (density, velocity, pressure, temperature, Mach
+  // number, and the node artificial viscosity with shock capturing); the solver keeps its final state.
+  void view() {
+    if (output_directory_.empty()) throw std::runtime_error("SubrosaDG: view needs setViewConfig");
+    std::filesystem::create_directories(output_directory_ + "/vtu");
+    const std::vector<Real> final_state = state();
+    sdg_mesh_info info{};
+    check(sdg_mesh_get_info(mesh_, &info), nullptr);
+    for (int step = iteration_start_; step <= iteration_end_; step++) {
+      const std::string raw = rawPath(step);
+      if (!std::filesystem::exists(raw)) continue;
+      check(sdg_raw_binary_read(ctx_, raw.c_str(), SDG_IC_LAST_STEP), ctx_);
+      const std::vector<Real> c = state();
+      std::vector<double> payload;
+      const double* node_values = nullptr;
+      if constexpr (SC::kShockCapturing == ShockCapturingEnum::ArtificialViscosity) {
+        int64_t bytes = 0;
+        check(sdg_raw_file_read(raw.c_str(), nullptr, 0, &bytes), nullptr);
+        payload.resize(static_cast<size_t>(bytes) / sizeof(double));
+        check(sdg_raw_file_read(raw.c_str(), payload.data(), bytes, &bytes), nullptr);
+        if (payload.size() >= static_cast<size_t>(info.nodes)) node_values = payload.data() + payload.size() - info.nodes;
+      }
+      const std::string out =
+          output_directory_ + "/vtu/" + output_prefix_ + "_" + std::to_string(step) + ".vtu";
+      check(sdg_write_vtu(mesh_, &problem_, c.data(), node_values, step, step * delta_time_, out.c_str()), nullptr);
+    }
+    check(sdg_set_state(ctx_, final_state.data()), ctx_);
+  }
+
   // Node artificial viscosity of the last step (node_artificial_viscosity_, the reference's
   // ViewVariableEnum::ArtificialViscosity output), mesh nodes long.
   std::vector<Real> artificialViscosity() const {

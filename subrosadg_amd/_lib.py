@@ -178,6 +178,7 @@ SIGNATURES = {
     "sdg_raw_binary_write": (c_int, [c_void_p, c_char_p, c_int64]),
     "sdg_raw_binary_read": (c_int, [c_void_p, c_char_p, c_int]),
     "sdg_raw_file_write": (c_int, [c_char_p, c_void_p, c_int64]),
+    "sdg_write_vtu": (c_int, [c_void_p, POINTER(ProblemDesc), c_void_p, c_void_p, c_int, c_double, c_char_p]),
     "sdg_raw_file_read": (c_int, [c_char_p, c_void_p, c_int64, POINTER(c_int64)]),
 }
 
@@ -518,6 +519,17 @@ def raw_file_write(path: str, payload: bytes):
     lib = load()
     buf = ctypes.create_string_buffer(payload, len(payload))
     check(lib.sdg_raw_file_write(os.fsencode(path), ctypes.cast(buf, c_void_p), len(payload)))
+
+
+def write_vtu(mesh: "Mesh", problem: ProblemDesc, coefficients: np.ndarray, path: str, node_values=None,
+              step: int = 0, time: float = 0.0):
+    """VTK XML output of a state (host only): see sdg_write_vtu in include/sdg.h."""
+    lib = load()
+    c = np.ascontiguousarray(coefficients, dtype=np.float64)
+    assert c.size == mesh.E * mesh.nb * mesh.nv
+    nv = None if node_values is None else np.ascontiguousarray(node_values, dtype=np.float64)
+    check(lib.sdg_write_vtu(mesh.handle, ctypes.byref(problem), c.ctypes.data,
+                            None if nv is None else nv.ctypes.data, int(step), float(time), path.encode()))
 
 
 def raw_file_read(path: str) -> bytes:

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsdg_hip.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-HOST_SOURCES = ["kit.cpp", "sdg_mesh_api.cpp", "sdg_raw.cpp"]
+HOST_SOURCES = ["kit.cpp", "sdg_mesh_api.cpp", "sdg_raw.cpp", "sdg_vtu.cpp"]
 # the system zstd runtime (raw state files); only its stable C API is used
 LINK_LIBS = ["-l:libzstd.so.1"]
 HIP_SOURCES = ["sdg_solver.hip", "sdg_batch.hip"]

@@ -262,6 +262,20 @@ static void evalBasis(const RefElement& ref, const double* xi, double* value, do
   }
 }
 
+// order-1 nodal functions of the vertices: barycentric on the triangle, tensor P1 otherwise
+void nodalValues(const RefElement& ref, const double* xi, double* out) {
+  if (ref.type == 2) {
+    out[0] = 1.0 - xi[0] - xi[1];
+    out[1] = xi[0];
+    out[2] = xi[1];
+    return;
+  }
+  double g[3];
+  for (int v = 0; v < ref.nvert; v++) p1Shape(ref.dim, &ref.vertex[static_cast<size_t>(v) * ref.dim], xi, out[v], g);
+}
+
+void basisValues(const RefElement& ref, const double* xi, double* value) { evalBasis(ref, xi, value, nullptr); }
+
 bool buildRefElement(int type, int order, RefElement& ref, std::string& error) {
   if (order < 1 || order > 5) {
     error = "polynomial order must be 1..5";
@@ -440,17 +454,7 @@ bool buildRefElement(int type, int order, RefElement& ref, std::string& error) {
       for (int k = c; k < i; k++) acc -= Lc[i * nb + k] * Li[k * nb + c];
       Li[i * nb + c] = acc / Lc[i * nb + i];
     }
-  // order-1 nodal functions of the vertices: barycentric on the triangle, tensor P1 otherwise
-  auto nodal = [&](const double* xi, double* out) {
-    if (type == 2) {
-      out[0] = 1.0 - xi[0] - xi[1];
-      out[1] = xi[0];
-      out[2] = xi[1];
-      return;
-    }
-    double g[3];
-    for (int v = 0; v < ref.nvert; v++) p1Shape(dim, &ref.vertex[static_cast<size_t>(v) * dim], xi, out[v], g);
-  };
+  auto nodal = [&](const double* xi, double* out) { nodalValues(ref, xi, out); };
   ref.nodal_value.assign(static_cast<size_t>(nq) * ref.nvert, 0.0);
   for (int q = 0; q < nq; q++) nodal(&ref.qpoint[static_cast<size_t>(q) * dim], &ref.nodal_value[static_cast<size_t>(q) * ref.nvert]);
   ref.nodal_face_value.assign(static_cast<size_t>(nf) * nfq * ref.nvert, 0.0);
@@ -900,6 +904,8 @@ bool singleTypeView(const MixedMesh& mm, Mesh& out) {
   assignNodes(out);
   return true;
 }
+
+void mapReferencePoint(const Mesh& mesh, int64_t e, const double* xi, double* x) { mapPoint(mesh, e, xi, x, nullptr); }
 
 void assignNodes(Mesh& mesh) {
   const RefElement& ref = mesh.ref;

@@ -141,6 +141,12 @@ bool singleTypeView(const MixedMesh& mm, Mesh& out);
 // Node tags (vertices that coincide in physical space share a node) and inner radii from the
 // element vertices and Jacobians of an affine mesh.
 void assignNodes(Mesh& mesh);
+// Modal basis values (N_b) at a reference point.
+void basisValues(const RefElement& ref, const double* xi, double* value);
+// Order-1 nodal functions of the vertices (nvert) at a reference point.
+void nodalValues(const RefElement& ref, const double* xi, double* value);
+// Physical point of reference point xi in element e.
+void mapReferencePoint(const Mesh& mesh, int64_t e, const double* xi, double* x);
 
 // Geometry derived per element and volume point (affine elements: constant Jacobian).
 //   detjw[q]           = |J| w_q

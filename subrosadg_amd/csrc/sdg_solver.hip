@@ -993,6 +993,11 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     if (upload(ctx, r.nodal_value, &dnq) || upload(ctx, r.nodal_face_value, &dnf) || upload(ctx, mesh.inner_radius, &drad) ||
         upload(ctx, tag, &dtag) || allocate(ctx, static_cast<size_t>(ctx->nnode), &ctx->nodeav))
       return bail(SDG_E_NOMEM);
+    // zero until the first step forms them, as the reference initialises them (InitialCondition.cpp:156-157)
+    if (hipMemsetAsync(ctx->nodeav, 0, static_cast<size_t>(ctx->nnode) * sizeof(double), ctx->stream) != hipSuccess) {
+      ctx->err = "sdg_create: device initialisation failed";
+      return bail(SDG_E_DEVICE);
+    }
     g.nodq = dnq;
     g.nodf = dnf;
     g.inrad = drad;

@@ -201,7 +201,7 @@ def test_shock_tube_against_the_exact_solution():
     assert l1 < 0.01
 
 
-def test_cpp_explosion_example():
+def test_cpp_explosion_example(tmp_path):
     """The C++ drop-in driver with ShockCapturingEnum::ArtificialViscosity and
     setArtificialViscosity(4.0): the blast stays finite and positive, and the viscosity is on at the
     front only."""
@@ -210,7 +210,16 @@ def test_cpp_explosion_example():
 
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "examples", "bin",
                        "explosion_2d_ceuler")
-    r = subprocess.run([exe, "24", "150"], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "24", "150", str(tmp_path)], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stderr
     assert "explosion: 150 steps" in r.stdout
+    # raw files at 0, 75, 150 and their VTU files with the node viscosity (system.view())
+    import xml.etree.ElementTree as ET
+
+    assert sorted(os.listdir(tmp_path / "vtu")) == ["explosion_0.vtu", "explosion_150.vtu", "explosion_75.vtu"]
+    root = ET.parse(tmp_path / "vtu" / "explosion_150.vtu").getroot()
+    arrays = {d.get("Name"): d for d in root.iter("DataArray")}
+    av = np.array(arrays["ArtificialViscosity"].text.split(), dtype=np.float64)
+    rho = np.array(arrays["Density"].text.split(), dtype=np.float64)
+    assert av.max() > 0.0 and rho.min() > 0.0 and int(arrays["TMSTEP"].text) == 150
