@@ -183,6 +183,7 @@ typedef struct sdg_mesh_info {
  *   NODE_TAG (int32)   E x nvert       0-based node of each vertex (node_tag_ - 1)
  *   NODAL_VALUE        N_q x nvert     P1 nodal functions at the volume points  BasisFunction.cpp:198-207
  *   NODAL_FACE_VALUE   (N_f*N_fq) x nvert  ... at the face points in parent coordinates  :161-180
+ *   BOUNDARY_FACE_PHYSICAL (int32) F_b  gmsh physical tag of a boundary face of a mesh file (-1 otherwise)
  */
 enum sdg_mesh_field {
   SDG_FIELD_BASIS_VALUE = 0,
@@ -215,7 +216,8 @@ enum sdg_mesh_field {
   SDG_FIELD_NODE_TAG = 27,
   SDG_FIELD_NODAL_VALUE = 28,
   SDG_FIELD_NODAL_FACE_VALUE = 29,
-  SDG_FIELD_COUNT_ = 30
+  SDG_FIELD_BOUNDARY_FACE_PHYSICAL = 30,
+  SDG_FIELD_COUNT_ = 31
 };
 
 /* Split grid (MeshModelEnum::TriangleQuadrangle meshes of the reference): a periodic structured
@@ -234,6 +236,20 @@ typedef struct sdg_split_desc {
   uint64_t seed;
 } sdg_split_desc;
 
+/* Mesh file written by gmsh (MSH 4.1 ASCII): affine elements of one type -- 3-node triangles or
+ * 4-node parallelogram quadrangles in 2-D, 8-node parallelepiped hexahedra in 3-D -- with the
+ * boundary given by the elements of one dimension lower in physical groups; boundary_count pairs
+ * map a physical tag to an enum sdg_boundary type (addBoundaryCondition<T>(physical index),
+ * SystemControl.cpp:76-80). Elements of negative orientation are mirrored; $Periodic sections,
+ * curved (higher-order) and mixed element types are refused. */
+typedef struct sdg_gmsh_desc {
+  const char* path;
+  int polynomial_order;   /* P = 1..5 (triangles 1..4)                                    */
+  int boundary_count;
+  const int* physical_index;
+  const int* boundary;
+} sdg_gmsh_desc;
+
 typedef struct sdg_mesh sdg_mesh;
 typedef struct sdg_ctx sdg_ctx;
 typedef struct sdg_batch sdg_batch;
@@ -250,6 +266,9 @@ const char* sdg_last_error(const sdg_ctx* ctx);
  * mass-matrix inverses, normals, basis and quadrature tables. */
 int sdg_mesh_create_structured(const sdg_structured_desc* desc, sdg_mesh** out);
 int sdg_mesh_create_split(const sdg_split_desc* desc, sdg_mesh** out);
+/* Replaces Mesh<SC>::initializeMesh + readMeshElement (ReadControl.cpp:286-335) for a mesh file:
+ * element and face construction (Adjacency.cpp:61-183), geometry, tables as above. */
+int sdg_mesh_create_gmsh(const sdg_gmsh_desc* desc, sdg_mesh** out);
 void sdg_mesh_destroy(sdg_mesh* mesh);
 int sdg_mesh_get_info(const sdg_mesh* mesh, sdg_mesh_info* info);
 /* Number of scalar entries of an exported field (0 for an unknown field). */

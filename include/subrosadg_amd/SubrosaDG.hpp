@@ -7,9 +7,10 @@
 // System setters and solve() loop with the all-NaN stop (SystemControl.cpp:159-195), and the
 // user-specialised InitialCondition<SC>::calculatePrimitiveFromCoordinate.
 // Changed: coordinates and primitive states are std::array instead of Eigen vectors; meshes are
-// structured boxes (setMesh(StructuredMesh)) instead of gmsh files, each box side carrying a
-// physical index (or -1: periodic direction); raw-binary and VTU output are not produced (the
-// monitor history is kept in memory and can be streamed). Boundary data come from the
+// structured boxes (setMesh(StructuredMesh)), each box side carrying a physical index (or -1:
+// periodic direction), or existing gmsh MSH 4.1 files (setMesh(path)) whose boundary groups carry
+// their physical tags -- there is no mesh-generating callback, as that needs the gmsh library.
+// Raw files (setViewConfig) and VTU files (view()) are written. Boundary data come from the
 // user-specialised BoundaryCondition<SC>::calculatePrimitiveFromCoordinate(x, t, physical_index),
 // evaluated at the boundary face points at start-up and, for BoundaryTimeEnum::TimeVarying,
 // before every step (BoundaryCondition.cpp:29-50).
@@ -147,6 +148,7 @@ struct System {
   BoundaryCondition<SC> boundary_condition_;
   std::vector<std::pair<Isize, BoundaryConditionEnum>> boundary_types_;
   StructuredMesh mesh_desc_;
+  std::string mesh_file_;  // gmsh MSH 4.1 file (setMesh(path)); empty for a structured box
   bool mesh_set_ = false;
   sdg_problem_desc problem_{};
   sdg_mesh* mesh_ = nullptr;
@@ -184,7 +186,15 @@ struct System {
   }
 
   // SystemControl.cpp:55-61 (structured periodic box instead of a gmsh file)
+  // SystemControl.cpp:55-61 with an existing gmsh file (MSH 4.1 ASCII, affine elements of one type):
+  // the boundary faces keep the physical tags of their groups for addBoundaryCondition and the
+  // BoundaryCondition callback. (The reference's mesh-generating callback needs the gmsh library.)
+  void setMesh(const std::filesystem::path& mesh_file) {
+    mesh_file_ = mesh_file.string();
+    mesh_set_ = true;
+  }
   void setMesh(const StructuredMesh& mesh) {
+    mesh_file_.clear();
     mesh_desc_ = mesh;
     mesh_set_ = true;
   }
@@ -270,6 +280,20 @@ struct System {
   // SystemControl.cpp:146-157: build the mesh and the device context
   void synchronize() {
     if (!mesh_set_) throw std::runtime_error("SubrosaDG: setMesh was not called");
+    if (!mesh_file_.empty()) {
+      std::vector<int> phys, types;
+      for (const auto& b : boundary_types_) {
+        phys.push_back(static_cast<int>(b.first));
+        types.push_back(static_cast<int>(b.second));
+      }
+      sdg_gmsh_desc g{mesh_file_.c_str(), SC::kPolynomialOrder, static_cast<int>(phys.size()), phys.data(), types.data()};
+      check(sdg_mesh_create_gmsh(&g, &mesh_), nullptr);
+      sdg_mesh_info info{};
+      check(sdg_mesh_get_info(mesh_, &info), nullptr);
+      if (info.dimension != SC::kDimension) throw std::runtime_error("SubrosaDG: mesh file dimension mismatch");
+      check(sdg_create(&problem_, mesh_, &ctx_), nullptr);
+      return;
+    }
     sdg_structured_desc d{};
     d.element = SC::kDimension == 2 ? SDG_ELEM_QUADRANGLE : SDG_ELEM_HEXAHEDRON;
     d.polynomial_order = SC::kPolynomialOrder;
@@ -304,8 +328,10 @@ struct System {
     const int64_t npts = info.boundary_faces * info.face_quadrature;
     std::vector<double> xb(static_cast<size_t>(npts * D)), nb(static_cast<size_t>(npts * D)),
         prim(static_cast<size_t>(npts * NV));
+    std::vector<int32_t> phys(static_cast<size_t>(info.boundary_faces));
     check(sdg_mesh_export(mesh_, SDG_FIELD_BOUNDARY_FACE_COORD, xb.data(), npts * D), nullptr);
     check(sdg_mesh_export(mesh_, SDG_FIELD_BOUNDARY_FACE_NORMAL, nb.data(), npts * D), nullptr);
+    check(sdg_mesh_export(mesh_, SDG_FIELD_BOUNDARY_FACE_PHYSICAL, phys.data(), info.boundary_faces), nullptr);
     for (int64_t p = 0; p < npts; p++) {
       std::array<Real, D> x;
       int side = 0;
@@ -313,7 +339,8 @@ struct System {
         x[dd] = xb[p * D + dd];
         if (std::fabs(nb[p * D + dd]) > 0.5) side = 2 * dd + (nb[p * D + dd] > 0.0 ? 1 : 0);
       }
-      const auto v = boundary_condition_.calculatePrimitiveFromCoordinate(x, time, mesh_desc_.physical_index[side]);
+      const Isize physical = mesh_file_.empty() ? mesh_desc_.physical_index[side] : phys[p / info.face_quadrature];
+      const auto v = boundary_condition_.calculatePrimitiveFromCoordinate(x, time, physical);
       for (int k = 0; k < NV; k++) prim[p * NV + k] = v[k];
     }
     check(sdg_set_boundary_primitive(ctx_, prim.data()), ctx_);

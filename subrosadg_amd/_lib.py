@@ -66,6 +66,7 @@ FIELDS = {
     "node_tag": (27, np.int32),
     "nodal_value": (28, np.float64),
     "nodal_face_value": (29, np.float64),
+    "boundary_face_physical": (30, np.int32),
 }
 
 
@@ -118,6 +119,16 @@ class SplitDesc(Structure):
     ]
 
 
+class GmshDesc(Structure):
+    _fields_ = [
+        ("path", c_char_p),
+        ("polynomial_order", c_int),
+        ("boundary_count", c_int),
+        ("physical_index", POINTER(c_int)),
+        ("boundary", POINTER(c_int)),
+    ]
+
+
 class MeshInfo(Structure):
     _fields_ = [
         ("dimension", c_int),
@@ -162,6 +173,7 @@ SIGNATURES = {
     "sdg_time_kernels": (c_int, [c_void_p, c_double, c_int, c_void_p, c_int]),
     "sdg_vjp_residual": (c_int, [c_void_p, c_void_p, c_void_p]),
     "sdg_mesh_create_split": (c_int, [POINTER(SplitDesc), POINTER(c_void_p)]),
+    "sdg_mesh_create_gmsh": (c_int, [POINTER(GmshDesc), POINTER(c_void_p)]),
     "sdg_batch_create": (c_int, [POINTER(ProblemDesc), c_void_p, c_int, POINTER(c_void_p)]),
     "sdg_batch_destroy": (None, [c_void_p]),
     "sdg_batch_last_error": (c_char_p, [c_void_p]),
@@ -284,6 +296,23 @@ class Mesh:
         d.seed = int(seed)
         h = c_void_p()
         check(lib.sdg_mesh_create_split(ctypes.byref(d), ctypes.byref(h)))
+        return cls(0, order, (), (), (), _handle=h)
+
+    @classmethod
+    def gmsh(cls, path: str, order: int, boundary: dict | None = None):
+        """Mesh from a gmsh MSH 4.1 ASCII file; `boundary` maps physical tags to BC_* types."""
+        lib = load()
+        d = GmshDesc()
+        d.path = path.encode()
+        d.polynomial_order = order
+        items = sorted((boundary or {}).items())
+        ph = (c_int * max(1, len(items)))(*[int(k) for k, _ in items])
+        bc = (c_int * max(1, len(items)))(*[int(v) for _, v in items])
+        d.boundary_count = len(items)
+        d.physical_index = ph
+        d.boundary = bc
+        h = c_void_p()
+        check(lib.sdg_mesh_create_gmsh(ctypes.byref(d), ctypes.byref(h)))
         return cls(0, order, (), (), (), _handle=h)
 
     def _attach(self, h):

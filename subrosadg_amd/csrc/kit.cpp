@@ -728,6 +728,199 @@ bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3]
   return true;
 }
 
+bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double* xyz, int64_t elements,
+                               const int64_t* conn, int64_t nbface, const int64_t* bconn, const int* btype,
+                               const int* bphysical, Mesh& mesh, std::string& error) {
+  mesh = Mesh();
+  if (type != 2 && type != 3 && type != 6) {
+    error = "element type must be triangle, quadrangle or hexahedron";
+    return false;
+  }
+  if (!buildRefElement(type, order, mesh.ref, error)) return false;
+  const RefElement& ref = mesh.ref;
+  const int dim = ref.dim, nv = ref.nvert, nf = ref.nf, nfq = ref.nfq, nfv = ref.nfvert;
+  if (elements < 1 || elements > (int64_t(1) << 30)) {
+    error = "element count out of range";
+    return false;
+  }
+  const int64_t E = elements;
+  mesh.elements = E;
+  std::vector<int64_t> cv(conn, conn + E * nv);
+  for (int64_t i = 0; i < E * nv; i++)
+    if (cv[i] < 0 || cv[i] >= nnode) {
+      error = "element node index out of range";
+      return false;
+    }
+  mesh.vertex.assign(static_cast<size_t>(E) * nv * dim, 0.0);
+  mesh.jac.assign(static_cast<size_t>(E) * dim * dim, 0.0);
+  mesh.detj.assign(E, 0.0);
+  mesh.min_edge.assign(E, 0.0);
+  static const int kMirror[3][8] = {{0, 2, 1}, {0, 3, 2, 1}, {0, 3, 2, 1, 4, 7, 6, 5}};
+  const int* mirror = kMirror[type == 2 ? 0 : (type == 3 ? 1 : 2)];
+  const double xi_c[3] = {type == 2 ? 1.0 / 3.0 : 0.0, type == 2 ? 1.0 / 3.0 : 0.0, 0.0};
+  static const int kHexEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                      {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+  double xmax = 0.0;
+  for (int64_t e = 0; e < E; e++) {
+    for (int pass = 0; pass < 2; pass++) {
+      for (int v = 0; v < nv; v++)
+        for (int d = 0; d < dim; d++)
+          mesh.vertex[(static_cast<size_t>(e) * nv + v) * dim + d] = xyz[cv[static_cast<size_t>(e) * nv + v] * 3 + d];
+      double x[3];
+      mapPoint(mesh, e, xi_c, x, &mesh.jac[static_cast<size_t>(e) * dim * dim]);
+      mesh.detj[e] = det(dim, &mesh.jac[static_cast<size_t>(e) * dim * dim]);
+      if (mesh.detj[e] > 0.0) break;
+      if (pass == 1 || mesh.detj[e] == 0.0) {
+        error = "degenerate element";
+        return false;
+      }
+      int64_t tmp[8];
+      for (int v = 0; v < nv; v++) tmp[v] = cv[static_cast<size_t>(e) * nv + mirror[v]];
+      for (int v = 0; v < nv; v++) cv[static_cast<size_t>(e) * nv + v] = tmp[v];
+    }
+    // affine: the map through the centre and its Jacobian reproduces every vertex
+    double xc[3];
+    mapPoint(mesh, e, xi_c, xc, nullptr);
+    double me = std::numeric_limits<double>::max(), span = 0.0;
+    const int ne = dim == 2 ? nf : 12;
+    for (int g = 0; g < ne; g++) {
+      const int a = dim == 2 ? ref.face_vertex[g * 2] : kHexEdge[g][0];
+      const int b = dim == 2 ? ref.face_vertex[g * 2 + 1] : kHexEdge[g][1];
+      double s2 = 0.0;
+      for (int d = 0; d < dim; d++) {
+        const double t = mesh.vertex[(static_cast<size_t>(e) * nv + a) * dim + d] - mesh.vertex[(static_cast<size_t>(e) * nv + b) * dim + d];
+        s2 += t * t;
+      }
+      me = std::min(me, std::sqrt(s2));
+      span = std::max(span, std::sqrt(s2));
+    }
+    mesh.min_edge[e] = me;
+    const double* J = &mesh.jac[static_cast<size_t>(e) * dim * dim];
+    for (int v = 0; v < nv; v++) {
+      double err2 = 0.0;
+      for (int l = 0; l < dim; l++) {
+        double x = xc[l];
+        for (int k = 0; k < dim; k++) x += J[l * dim + k] * (ref.vertex[static_cast<size_t>(v) * dim + k] - xi_c[k]);
+        const double t = x - mesh.vertex[(static_cast<size_t>(e) * nv + v) * dim + l];
+        err2 += t * t;
+        xmax = std::max(xmax, std::fabs(mesh.vertex[(static_cast<size_t>(e) * nv + v) * dim + l]));
+      }
+      if (err2 > 1e-20 * span * span) {
+        error = "non-affine element (the engine needs triangles, parallelograms or parallelepipeds)";
+        return false;
+      }
+    }
+  }
+  // faces keyed by their sorted node indices
+  auto keyOf = [&](const int64_t* nodes) {
+    std::array<int64_t, 4> k = {-1, -1, -1, -1};
+    for (int v = 0; v < nfv; v++) k[v] = nodes[v];
+    std::sort(k.begin(), k.begin() + nfv);
+    return k;
+  };
+  std::map<std::array<int64_t, 4>, std::vector<std::pair<int64_t, int>>> visits;
+  for (int64_t e = 0; e < E; e++)
+    for (int s = 0; s < nf; s++) {
+      int64_t fn[4];
+      for (int v = 0; v < nfv; v++) fn[v] = cv[static_cast<size_t>(e) * nv + ref.face_vertex[s * nfv + v]];
+      visits[keyOf(fn)].push_back({e, s});
+    }
+  std::map<std::array<int64_t, 4>, std::pair<int, int>> boundary_type;  // type, physical tag
+  for (int64_t b = 0; b < nbface; b++) {
+    if (btype[b] < 0 || btype[b] > 5) {
+      error = "boundary type must be 0..5";
+      return false;
+    }
+    boundary_type[keyOf(&bconn[static_cast<size_t>(b) * nfv])] = {btype[b], bphysical ? bphysical[b] : -1};
+  }
+  mesh.elem_face.assign(static_cast<size_t>(E) * nf, -1);
+  mesh.elem_side.assign(static_cast<size_t>(E) * nf, 0);
+  std::vector<double> xl(static_cast<size_t>(nfq) * dim), xr(static_cast<size_t>(nfq) * dim);
+  const double tol = 1e-20 * (1.0 + xmax * xmax);
+  for (int64_t e = 0; e < E; e++)
+    for (int s = 0; s < nf; s++) {
+      if (mesh.elem_face[static_cast<size_t>(e) * nf + s] != -1) continue;
+      int64_t fn[4];
+      for (int v = 0; v < nfv; v++) fn[v] = cv[static_cast<size_t>(e) * nv + ref.face_vertex[s * nfv + v]];
+      const auto key = keyOf(fn);
+      const auto& list = visits[key];
+      for (int p = 0; p < nfq; p++)
+        mapPoint(mesh, e, &ref.face_param[(static_cast<size_t>(s) * nfq + p) * dim], &xl[p * dim], nullptr);
+      if (list.size() == 1) {
+        const auto it = boundary_type.find(key);
+        if (it == boundary_type.end()) {
+          error = "a boundary face has no boundary condition (no boundary element of a mapped physical group)";
+          return false;
+        }
+        Face face;
+        face.parent[0] = static_cast<int32_t>(e);
+        face.parent[1] = -1;
+        face.local[0] = s;
+        face.local[1] = -1;
+        face.bc = it->second.first;
+        face.physical = it->second.second;
+        face.seq.resize(nfq);
+        for (int p = 0; p < nfq; p++) face.seq[p] = p;
+        face.coord.assign(xl.begin(), xl.end());
+        faceNormals(mesh, e, s, face);
+        mesh.elem_face[static_cast<size_t>(e) * nf + s] = -2 - static_cast<int32_t>(mesh.bfaces.size());
+        mesh.bfaces.push_back(std::move(face));
+        continue;
+      }
+      if (list.size() != 2) {
+        error = "a face is shared by more than two elements";
+        return false;
+      }
+      const int64_t r = list[1].first;
+      const int rs = list[1].second;
+      Face face;
+      face.parent[0] = static_cast<int32_t>(e);
+      face.parent[1] = static_cast<int32_t>(r);
+      face.local[0] = s;
+      face.local[1] = rs;
+      for (int p = 0; p < nfq; p++)
+        mapPoint(mesh, r, &ref.face_param[(static_cast<size_t>(rs) * nfq + p) * dim], &xr[p * dim], nullptr);
+      face.seq.assign(nfq, -1);
+      for (int p = 0; p < nfq; p++) {
+        double best = std::numeric_limits<double>::max();
+        int bi = -1;
+        for (int qq = 0; qq < nfq; qq++) {
+          double s2 = 0.0;
+          for (int d = 0; d < dim; d++) {
+            const double t = xl[p * dim + d] - xr[qq * dim + d];
+            s2 += t * t;
+          }
+          if (s2 < best) {
+            best = s2;
+            bi = qq;
+          }
+        }
+        if (best > tol) {
+          error = "face points of the two parents do not match";
+          return false;
+        }
+        face.seq[p] = bi;
+      }
+      face.rotation = 0;
+      if (dim == 3) {
+        const int64_t first = fn[0];
+        for (int v = 0; v < nfv; v++)
+          if (cv[static_cast<size_t>(r) * nv + ref.face_vertex[rs * nfv + v]] == first) face.rotation = v;
+      }
+      face.coord.assign(xl.begin(), xl.end());
+      faceNormals(mesh, e, s, face);
+      const int32_t fid = static_cast<int32_t>(mesh.faces.size());
+      mesh.faces.push_back(std::move(face));
+      mesh.elem_face[static_cast<size_t>(e) * nf + s] = fid;
+      mesh.elem_side[static_cast<size_t>(e) * nf + s] = 0;
+      mesh.elem_face[static_cast<size_t>(r) * nf + rs] = fid;
+      mesh.elem_side[static_cast<size_t>(r) * nf + rs] = 1;
+    }
+  mesh.affine = true;
+  assignNodes(mesh);
+  return true;
+}
+
 bool buildSplitMesh(int order, const int n[2], const double lo[2], const double hi[2], const std::vector<uint8_t>& split,
                     MixedMesh& mesh, std::string& error) {
   mesh = MixedMesh();

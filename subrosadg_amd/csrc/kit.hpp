@@ -79,6 +79,7 @@ struct Face {
   int32_t local[2] = {0, 0};    // local face index in left, right
   int32_t rotation = 0;
   int32_t bc = -1;              // boundary faces: sdg_boundary type
+  int32_t physical = -1;        // boundary faces read from a mesh file: the gmsh physical tag
   std::vector<int32_t> seq;     // N_fq: right point index of left point j
   std::vector<double> normal;   // N_fq x D, outward from the left parent
   std::vector<double> jw;       // N_fq: |J_f| w
@@ -114,6 +115,17 @@ struct Mesh {
 // a message on a bad descriptor.
 bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3], const double hi[3],
                          const int periodic[3], const int bc[6], Mesh& mesh, std::string& error);
+
+// Mesh of one element type from node coordinates (nnode x 3) and element vertex lists (E x nvert,
+// 0-based node indices, gmsh vertex order), with the boundary faces given by their node lists
+// (nbface x nfvert, any order) and sdg_boundary types. Elements must be affine (triangles,
+// parallelogram quadrangles, parallelepiped hexahedra); an element with a negative Jacobian is
+// mirrored. Faces pair by their node sets and are created at the first visit in element order,
+// whose element is the left parent (Adjacency.cpp:61-183); right-parent points are matched by
+// coordinates, boundary faces take the type of their boundary element.
+bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double* xyz, int64_t elements,
+                               const int64_t* conn, int64_t nbface, const int64_t* bconn, const int* btype,
+                               const int* bphysical, Mesh& mesh, std::string& error);
 
 // Triangles and quadrangles on a periodic structured n[0] x n[1] grid: cell (i, j) (i fastest) is a
 // quadrangle, or, when split[cell] != 0, the two triangles (A, B, C), (A, C, D) of its corners

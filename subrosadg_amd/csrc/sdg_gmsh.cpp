@@ -1,0 +1,230 @@
+// sdg_gmsh.cpp -- reader of gmsh's MSH 4.1 ASCII format for affine meshes of one element type:
+// the mesh-file half of the reference's Mesh<SC>::initializeMesh / readMeshElement
+// (src/Mesh/ReadControl.cpp:286-335), which reads through the gmsh library; here the file is parsed
+// directly and handed to the connectivity builder of the kit.
+//
+// Read: $MeshFormat (4.1, ASCII), $Entities (the physical tags of the boundary curves / surfaces),
+// $Nodes, $Elements. The highest-dimensional elements are the mesh (3-node triangles or 4-node
+// quadrangles in 2-D, 8-node hexahedra in 3-D; one type); elements of one dimension lower carry the
+// boundary faces, whose type comes from the physical group of their entity through the caller's
+// map. Refused: binary files, other versions, higher-order (curved) or mixed element types, and
+// $Periodic sections (periodic meshes are built by sdg_mesh_create_structured / _split).
+#include <algorithm>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "kit.hpp"
+#include "sdg_internal.hpp"
+
+namespace sdg {
+
+namespace {
+
+// nodes per element of the first-order gmsh types used here (0: unsupported)
+int gmshNodes(int type) {
+  switch (type) {
+    case 15: return 1;  // point
+    case 1: return 2;   // 2-node line
+    case 2: return 3;   // 3-node triangle
+    case 3: return 4;   // 4-node quadrangle
+    case 5: return 8;   // 8-node hexahedron
+    default: return 0;
+  }
+}
+int gmshDim(int type) { return type == 15 ? 0 : (type == 1 ? 1 : (type == 5 ? 3 : 2)); }
+
+bool skipTo(std::istream& in, const std::string& end) {
+  std::string line;
+  while (in >> line)
+    if (line == end) return true;
+  return false;
+}
+
+}  // namespace
+
+bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_physical, Mesh& mesh,
+                  std::string& error) {
+  std::ifstream in(path);
+  if (!in) {
+    error = std::string("cannot open ") + path;
+    return false;
+  }
+  std::map<std::pair<int, int>, int> physical;  // (dim, entity) -> first physical tag
+  std::map<int64_t, int64_t> node_index;       // gmsh node tag -> 0-based index
+  std::vector<double> xyz;
+  struct Block {
+    int dim, entity, type;
+    std::vector<int64_t> nodes;  // gmsh node tags, gmshNodes(type) per element
+  };
+  std::vector<Block> blocks;
+  bool have_format = false, have_nodes = false, have_elements = false;
+  std::string tok;
+  while (in >> tok) {
+    if (tok == "$MeshFormat") {
+      std::string version;
+      int file_type = 0, data_size = 0;
+      in >> version >> file_type >> data_size;
+      if (version.rfind("4.1", 0) != 0 || file_type != 0) {
+        error = "only the MSH 4.1 ASCII format is read";
+        return false;
+      }
+      have_format = true;
+      skipTo(in, "$EndMeshFormat");
+    } else if (tok == "$Entities") {
+      int64_t count[4];
+      in >> count[0] >> count[1] >> count[2] >> count[3];
+      for (int dim = 0; dim < 4; dim++)
+        for (int64_t i = 0; i < count[dim]; i++) {
+          int tag = 0;
+          double box[6];
+          in >> tag;
+          const int nbox = dim == 0 ? 3 : 6;
+          for (int k = 0; k < nbox; k++) in >> box[k];
+          int64_t nphys = 0;
+          in >> nphys;
+          for (int64_t k = 0; k < nphys; k++) {
+            int ph = 0;
+            in >> ph;
+            if (k == 0) physical[{dim, tag}] = ph;
+          }
+          if (dim > 0) {
+            int64_t nbound = 0;
+            in >> nbound;
+            for (int64_t k = 0; k < nbound; k++) {
+              int b = 0;
+              in >> b;
+            }
+          }
+        }
+      if (!in) {
+        error = "malformed $Entities section";
+        return false;
+      }
+      skipTo(in, "$EndEntities");
+    } else if (tok == "$Nodes") {
+      int64_t nblocks = 0, nnodes = 0, mintag = 0, maxtag = 0;
+      in >> nblocks >> nnodes >> mintag >> maxtag;
+      xyz.reserve(static_cast<size_t>(nnodes) * 3);
+      for (int64_t b = 0; b < nblocks; b++) {
+        int dim = 0, entity = 0, parametric = 0;
+        int64_t n = 0;
+        in >> dim >> entity >> parametric >> n;
+        if (parametric) {
+          error = "parametric node coordinates are not read";
+          return false;
+        }
+        std::vector<int64_t> tags(static_cast<size_t>(n));
+        for (auto& t : tags) in >> t;
+        for (int64_t i = 0; i < n; i++) {
+          double x, y, z;
+          in >> x >> y >> z;
+          node_index[tags[static_cast<size_t>(i)]] = static_cast<int64_t>(xyz.size() / 3);
+          xyz.insert(xyz.end(), {x, y, z});
+        }
+      }
+      if (!in || static_cast<int64_t>(xyz.size() / 3) != nnodes) {
+        error = "malformed $Nodes section";
+        return false;
+      }
+      have_nodes = true;
+      skipTo(in, "$EndNodes");
+    } else if (tok == "$Elements") {
+      int64_t nblocks = 0, nelem = 0, mintag = 0, maxtag = 0;
+      in >> nblocks >> nelem >> mintag >> maxtag;
+      for (int64_t b = 0; b < nblocks; b++) {
+        Block blk;
+        int64_t n = 0;
+        in >> blk.dim >> blk.entity >> blk.type >> n;
+        const int nn = gmshNodes(blk.type);
+        if (nn == 0) {
+          error = "element type " + std::to_string(blk.type) + " is not read (first-order lines, triangles, "
+                  "quadrangles and hexahedra only)";
+          return false;
+        }
+        blk.nodes.resize(static_cast<size_t>(n) * nn);
+        for (int64_t i = 0; i < n; i++) {
+          int64_t etag = 0;
+          in >> etag;
+          for (int k = 0; k < nn; k++) in >> blk.nodes[static_cast<size_t>(i) * nn + k];
+        }
+        blocks.push_back(std::move(blk));
+      }
+      if (!in) {
+        error = "malformed $Elements section";
+        return false;
+      }
+      have_elements = true;
+      skipTo(in, "$EndElements");
+    } else if (tok == "$Periodic") {
+      error = "periodic meshes are not read from .msh files (use the structured or split-grid builders)";
+      return false;
+    } else if (tok.size() > 1 && tok[0] == '$' && tok.rfind("$End", 0) != 0) {
+      skipTo(in, "$End" + tok.substr(1));  // sections not needed here ($PhysicalNames, ...)
+    }
+  }
+  if (!have_format || !have_nodes || !have_elements) {
+    error = "missing $MeshFormat, $Nodes or $Elements";
+    return false;
+  }
+  int dim = 0;
+  for (const auto& b : blocks) dim = std::max(dim, gmshDim(b.type));
+  if (dim < 2) {
+    error = "no 2-D or 3-D elements";
+    return false;
+  }
+  int type = -1;
+  std::vector<int64_t> conn, bconn;
+  std::vector<int> btype, bphysical;
+  auto index = [&](int64_t tag, int64_t& out) {
+    const auto it = node_index.find(tag);
+    if (it == node_index.end()) return false;
+    out = it->second;
+    return true;
+  };
+  for (const auto& b : blocks) {
+    const int d = gmshDim(b.type), nn = gmshNodes(b.type);
+    if (d == dim) {
+      const int t = b.type == 2 ? 2 : (b.type == 3 ? 3 : 6);
+      if (type >= 0 && type != t) {
+        error = "meshes of more than one element type are not read";
+        return false;
+      }
+      type = t;
+      for (int64_t tag : b.nodes) {
+        int64_t i = 0;
+        if (!index(tag, i)) {
+          error = "element references an unknown node";
+          return false;
+        }
+        conn.push_back(i);
+      }
+    } else if (d == dim - 1) {
+      const auto ph = physical.find({d, b.entity});
+      if (ph == physical.end()) continue;  // boundary elements outside any physical group
+      const auto bc = bc_of_physical.find(ph->second);
+      if (bc == bc_of_physical.end()) continue;
+      for (size_t k = 0; k < b.nodes.size(); k += static_cast<size_t>(nn)) {
+        for (int v = 0; v < nn; v++) {
+          int64_t i = 0;
+          if (!index(b.nodes[k + static_cast<size_t>(v)], i)) {
+            error = "boundary element references an unknown node";
+            return false;
+          }
+          bconn.push_back(i);
+        }
+        btype.push_back(bc->second);
+        bphysical.push_back(ph->second);
+      }
+    }
+  }
+  const int nvert = type == 2 ? 3 : (type == 3 ? 4 : 8);
+  return buildMeshFromConnectivity(type, order, static_cast<int64_t>(xyz.size() / 3), xyz.data(),
+                                   static_cast<int64_t>(conn.size()) / nvert, conn.data(),
+                                   static_cast<int64_t>(btype.size()), bconn.data(), btype.data(), bphysical.data(),
+                                   mesh, error);
+}
+
+}  // namespace sdg

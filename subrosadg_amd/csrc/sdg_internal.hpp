@@ -3,6 +3,7 @@
 #define SDG_INTERNAL_HPP_
 
 #include <cstddef>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,9 @@ const MixedMesh* mixedOf(const sdg_mesh* m);
 // zstd raw files in the reference's layout (sdg_raw.cpp): header + one frame of the payload
 int writeRawFile(const char* path, const void* data, size_t bytes);
 int readRawFile(const char* path, std::vector<char>& out);
+// gmsh MSH 4.1 ASCII file -> single-type affine mesh; boundary types by physical tag (sdg_gmsh.cpp)
+bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_physical, Mesh& mesh,
+                  std::string& error);
 }  // namespace sdg
 
 #endif  // SDG_INTERNAL_HPP_

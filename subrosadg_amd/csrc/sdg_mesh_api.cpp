@@ -1,5 +1,6 @@
 // sdg_mesh_api.cpp -- mesh and operand-export part of the C-ABI (host only, no device work).
 #include <cstring>
+#include <map>
 #include <new>
 #include <string>
 #include <vector>
@@ -29,6 +30,35 @@ const MixedMesh* mixedOf(const sdg_mesh* m) { return m && m->has_mixed ? &m->mix
 extern "C" {
 
 int sdg_abi_version(void) { return SDG_ABI_VERSION; }
+
+int sdg_mesh_create_gmsh(const sdg_gmsh_desc* desc, sdg_mesh** out) {
+  if (!desc || !out || !desc->path || (desc->boundary_count > 0 && (!desc->physical_index || !desc->boundary))) {
+    sdg::setGlobalError("sdg_mesh_create_gmsh: null argument");
+    return SDG_E_INVALID;
+  }
+  *out = nullptr;
+  sdg_mesh* m = new (std::nothrow) sdg_mesh();
+  if (!m) {
+    sdg::setGlobalError("sdg_mesh_create_gmsh: out of host memory");
+    return SDG_E_NOMEM;
+  }
+  std::string err;
+  try {
+    std::map<int, int> bc;
+    for (int i = 0; i < desc->boundary_count; i++) bc[desc->physical_index[i]] = desc->boundary[i];
+    if (!sdg::readGmshMesh(desc->path, desc->polynomial_order, bc, m->mesh, err)) {
+      delete m;
+      sdg::setGlobalError("sdg_mesh_create_gmsh: " + err);
+      return SDG_E_INVALID;
+    }
+  } catch (const std::exception& e) {
+    delete m;
+    sdg::setGlobalError(std::string("sdg_mesh_create_gmsh: ") + e.what());
+    return SDG_E_NOMEM;
+  }
+  *out = m;
+  return SDG_OK;
+}
 
 int sdg_mesh_create_structured(const sdg_structured_desc* desc, sdg_mesh** out) {
   if (!desc || !out) {
@@ -191,13 +221,15 @@ int64_t sdg_mesh_field_size(const sdg_mesh* mesh, int field) {
     case SDG_FIELD_NODE_TAG: return E * mesh->mesh.ref.nvert;
     case SDG_FIELD_NODAL_VALUE: return nq * mesh->mesh.ref.nvert;
     case SDG_FIELD_NODAL_FACE_VALUE: return nf * nfq * mesh->mesh.ref.nvert;
+    case SDG_FIELD_BOUNDARY_FACE_PHYSICAL: return Fb;
     default: return 0;
   }
 }
 
 int sdg_mesh_export(const sdg_mesh* mesh, int field, void* out, int64_t count) {
   const int64_t need = sdg_mesh_field_size(mesh, field);
-  const bool boundary_field = field >= SDG_FIELD_BOUNDARY_FACE_COORD && field <= SDG_FIELD_BOUNDARY_FACE_TYPE;
+  const bool boundary_field = (field >= SDG_FIELD_BOUNDARY_FACE_COORD && field <= SDG_FIELD_BOUNDARY_FACE_TYPE) ||
+                              field == SDG_FIELD_BOUNDARY_FACE_PHYSICAL;
   if (mesh && boundary_field && need == 0) return SDG_OK;  // no boundary faces
   if (!mesh || !out || need == 0) {
     sdg::setGlobalError("sdg_mesh_export: null argument or unknown field");
@@ -335,6 +367,9 @@ int sdg_mesh_export(const sdg_mesh* mesh, int field, void* out, int64_t count) {
     case SDG_FIELD_NODE_TAG: std::memcpy(oi, m.node_tag.data(), m.node_tag.size() * sizeof(int32_t)); break;
     case SDG_FIELD_NODAL_VALUE: copy(r.nodal_value); break;
     case SDG_FIELD_NODAL_FACE_VALUE: copy(r.nodal_face_value); break;
+    case SDG_FIELD_BOUNDARY_FACE_PHYSICAL:
+      for (size_t f = 0; f < m.bfaces.size(); f++) oi[f] = m.bfaces[f].physical;
+      break;
     default:
       sdg::setGlobalError("sdg_mesh_export: unknown field");
       return SDG_E_INVALID;
