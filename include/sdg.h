@@ -240,8 +240,9 @@ typedef struct sdg_split_desc {
  * 4-node parallelogram quadrangles in 2-D, 8-node parallelepiped hexahedra in 3-D -- with the
  * boundary given by the elements of one dimension lower in physical groups; boundary_count pairs
  * map a physical tag to an enum sdg_boundary type (addBoundaryCondition<T>(physical index),
- * SystemControl.cpp:76-80). Elements of negative orientation are mirrored; $Periodic sections,
- * curved (higher-order) and mixed element types are refused. */
+ * SystemControl.cpp:76-80). Elements of negative orientation are mirrored; $Periodic links
+ * (translations) pair the faces on their slave nodes with the faces on the master nodes; curved
+ * (higher-order) and mixed element types are refused. */
 typedef struct sdg_gmsh_desc {
   const char* path;
   int polynomial_order;   /* P = 1..5 (triangles 1..4)                                    */

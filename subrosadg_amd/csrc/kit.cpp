@@ -730,7 +730,8 @@ bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3]
 
 bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double* xyz, int64_t elements,
                                const int64_t* conn, int64_t nbface, const int64_t* bconn, const int* btype,
-                               const int* bphysical, Mesh& mesh, std::string& error) {
+                               const int* bphysical, Mesh& mesh, std::string& error,
+                               const std::vector<std::map<int64_t, int64_t>>* periodic_links) {
   mesh = Mesh();
   if (type != 2 && type != 3 && type != 6) {
     error = "element type must be triangle, quadrangle or hexahedron";
@@ -833,6 +834,34 @@ bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double*
     }
     boundary_type[keyOf(&bconn[static_cast<size_t>(b) * nfv])] = {btype[b], bphysical ? bphysical[b] : -1};
   }
+  // periodic partners: an unpaired face on the slave nodes of a link and the unpaired face on
+  // their masters; link_of records the link for the rotation's node comparison
+  std::map<std::pair<int64_t, int>, std::pair<std::pair<int64_t, int>, int>> partner;
+  if (periodic_links) {
+    for (const auto& kv : visits) {
+      if (kv.second.size() != 1) continue;
+      const auto [e, s] = kv.second[0];
+      for (size_t l = 0; l < periodic_links->size(); l++) {
+        const auto& link = (*periodic_links)[l];
+        int64_t mapped[4];
+        bool slave = true;
+        for (int v = 0; v < nfv && slave; v++) {
+          const auto it = link.find(cv[static_cast<size_t>(e) * nv + ref.face_vertex[s * nfv + v]]);
+          slave = it != link.end();
+          if (slave) mapped[v] = it->second;
+        }
+        if (!slave) continue;
+        const auto m = visits.find(keyOf(mapped));
+        if (m == visits.end() || m->second.size() != 1) {
+          error = "a periodic face has no unpaired master face";
+          return false;
+        }
+        partner[{e, s}] = {m->second[0], static_cast<int>(l)};
+        partner[m->second[0]] = {{e, s}, static_cast<int>(l)};
+        break;
+      }
+    }
+  }
   mesh.elem_face.assign(static_cast<size_t>(E) * nf, -1);
   mesh.elem_side.assign(static_cast<size_t>(E) * nf, 0);
   std::vector<double> xl(static_cast<size_t>(nfq) * dim), xr(static_cast<size_t>(nfq) * dim);
@@ -843,7 +872,9 @@ bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double*
       int64_t fn[4];
       for (int v = 0; v < nfv; v++) fn[v] = cv[static_cast<size_t>(e) * nv + ref.face_vertex[s * nfv + v]];
       const auto key = keyOf(fn);
-      const auto& list = visits[key];
+      std::vector<std::pair<int64_t, int>> list = visits[key];
+      const auto pp = partner.find({e, s});
+      if (pp != partner.end()) list.push_back(pp->second.first);
       for (int p = 0; p < nfq; p++)
         mapPoint(mesh, e, &ref.face_param[(static_cast<size_t>(s) * nfq + p) * dim], &xl[p * dim], nullptr);
       if (list.size() == 1) {
@@ -880,6 +911,12 @@ bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double*
       face.local[1] = rs;
       for (int p = 0; p < nfq; p++)
         mapPoint(mesh, r, &ref.face_param[(static_cast<size_t>(rs) * nfq + p) * dim], &xr[p * dim], nullptr);
+      // translation between the two faces (zero unless the face is periodic): difference of the
+      // face-vertex centroids
+      double shift[3] = {0.0, 0.0, 0.0};
+      for (int v = 0; v < nfv; v++)
+        for (int d = 0; d < dim; d++)
+          shift[d] += (xyz[fn[v] * 3 + d] - xyz[cv[static_cast<size_t>(r) * nv + ref.face_vertex[rs * nfv + v]] * 3 + d]) / nfv;
       face.seq.assign(nfq, -1);
       for (int p = 0; p < nfq; p++) {
         double best = std::numeric_limits<double>::max();
@@ -887,7 +924,7 @@ bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double*
         for (int qq = 0; qq < nfq; qq++) {
           double s2 = 0.0;
           for (int d = 0; d < dim; d++) {
-            const double t = xl[p * dim + d] - xr[qq * dim + d];
+            const double t = xl[p * dim + d] - (xr[qq * dim + d] + shift[d]);
             s2 += t * t;
           }
           if (s2 < best) {
@@ -902,10 +939,16 @@ bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double*
         face.seq[p] = bi;
       }
       face.rotation = 0;
-      if (dim == 3) {
-        const int64_t first = fn[0];
+      if (dim == 3) {  // the left's first face vertex in the right's list (through the periodic map)
+        auto image = [&](int64_t n) {
+          if (pp == partner.end()) return n;
+          const auto& link = (*periodic_links)[static_cast<size_t>(pp->second.second)];
+          const auto it = link.find(n);
+          return it == link.end() ? n : it->second;
+        };
+        const int64_t first = image(fn[0]);
         for (int v = 0; v < nfv; v++)
-          if (cv[static_cast<size_t>(r) * nv + ref.face_vertex[rs * nfv + v]] == first) face.rotation = v;
+          if (image(cv[static_cast<size_t>(r) * nv + ref.face_vertex[rs * nfv + v]]) == first) face.rotation = v;
       }
       face.coord.assign(xl.begin(), xl.end());
       faceNormals(mesh, e, s, face);

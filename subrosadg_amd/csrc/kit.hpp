@@ -18,6 +18,7 @@
 
 #include <array>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -122,10 +123,15 @@ bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3]
 // parallelogram quadrangles, parallelepiped hexahedra); an element with a negative Jacobian is
 // mirrored. Faces pair by their node sets and are created at the first visit in element order,
 // whose element is the left parent (Adjacency.cpp:61-183); right-parent points are matched by
-// coordinates, boundary faces take the type of their boundary element.
+// coordinates, boundary faces take the type of their boundary element. periodic_links (optional)
+// are node maps slave -> master, one per periodic pair of boundaries: an unpaired face whose nodes
+// are all slaves of one link pairs with the unpaired face on the master nodes, and the right
+// parent's points are matched after the translation between the two faces (translational
+// periodicity, Adjacency.cpp:185-328).
 bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double* xyz, int64_t elements,
                                const int64_t* conn, int64_t nbface, const int64_t* bconn, const int* btype,
-                               const int* bphysical, Mesh& mesh, std::string& error);
+                               const int* bphysical, Mesh& mesh, std::string& error,
+                               const std::vector<std::map<int64_t, int64_t>>* periodic_links = nullptr);
 
 // Triangles and quadrangles on a periodic structured n[0] x n[1] grid: cell (i, j) (i fastest) is a
 // quadrangle, or, when split[cell] != 0, the two triangles (A, B, C), (A, C, D) of its corners

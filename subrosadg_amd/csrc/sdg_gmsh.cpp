@@ -7,8 +7,8 @@
 // $Nodes, $Elements. The highest-dimensional elements are the mesh (3-node triangles or 4-node
 // quadrangles in 2-D, 8-node hexahedra in 3-D; one type); elements of one dimension lower carry the
 // boundary faces, whose type comes from the physical group of their entity through the caller's
-// map. Refused: binary files, other versions, higher-order (curved) or mixed element types, and
-// $Periodic sections (periodic meshes are built by sdg_mesh_create_structured / _split).
+// map; $Periodic node correspondences (translations) pair the faces of periodic boundaries.
+// Refused: binary files, other versions, higher-order (curved) or mixed element types.
 #include <algorithm>
 #include <fstream>
 #include <map>
@@ -60,6 +60,7 @@ bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_p
     std::vector<int64_t> nodes;  // gmsh node tags, gmshNodes(type) per element
   };
   std::vector<Block> blocks;
+  std::vector<std::vector<std::pair<int64_t, int64_t>>> periodic_pairs;  // per link: slave, master tags
   bool have_format = false, have_nodes = false, have_elements = false;
   std::string tok;
   while (in >> tok) {
@@ -158,9 +159,30 @@ bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_p
       }
       have_elements = true;
       skipTo(in, "$EndElements");
-    } else if (tok == "$Periodic") {
-      error = "periodic meshes are not read from .msh files (use the structured or split-grid builders)";
-      return false;
+    } else if (tok == "$Periodic") {  // node correspondences of translational periodicity
+      int64_t links = 0;
+      in >> links;
+      for (int64_t l = 0; l < links; l++) {
+        int dim_l = 0, tag_l = 0, master_l = 0;
+        int64_t naffine = 0, npairs = 0;
+        in >> dim_l >> tag_l >> master_l >> naffine;
+        for (int64_t k = 0; k < naffine; k++) {
+          double a = 0.0;
+          in >> a;
+        }
+        in >> npairs;
+        periodic_pairs.emplace_back();
+        for (int64_t k = 0; k < npairs; k++) {
+          int64_t slave = 0, mast = 0;
+          in >> slave >> mast;
+          periodic_pairs.back().push_back({slave, mast});
+        }
+      }
+      if (!in) {
+        error = "malformed $Periodic section";
+        return false;
+      }
+      skipTo(in, "$EndPeriodic");
     } else if (tok.size() > 1 && tok[0] == '$' && tok.rfind("$End", 0) != 0) {
       skipTo(in, "$End" + tok.substr(1));  // sections not needed here ($PhysicalNames, ...)
     }
@@ -220,11 +242,24 @@ bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_p
       }
     }
   }
+  // one node map per periodic link (a corner node is a slave in more than one link)
+  std::vector<std::map<int64_t, int64_t>> links;
+  for (const auto& link : periodic_pairs) {
+    links.emplace_back();
+    for (const auto& pr : link) {
+      int64_t si = 0, mi = 0;
+      if (!index(pr.first, si) || !index(pr.second, mi)) {
+        error = "$Periodic references an unknown node";
+        return false;
+      }
+      if (si != mi) links.back()[si] = mi;
+    }
+  }
   const int nvert = type == 2 ? 3 : (type == 3 ? 4 : 8);
   return buildMeshFromConnectivity(type, order, static_cast<int64_t>(xyz.size() / 3), xyz.data(),
                                    static_cast<int64_t>(conn.size()) / nvert, conn.data(),
                                    static_cast<int64_t>(btype.size()), bconn.data(), btype.data(), bphysical.data(),
-                                   mesh, error);
+                                   mesh, error, links.empty() ? nullptr : &links);
 }
 
 }  // namespace sdg

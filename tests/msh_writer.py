@@ -8,10 +8,12 @@ GMSH_TYPE = {"tri": 2, "quad": 3, "hex": 5, "line": 1, "bquad": 3}
 
 
 def write_msh(path, dim: int, node_tags: np.ndarray, xyz: np.ndarray, elem_type: str, elems: np.ndarray,
-              boundary: list[tuple[int, str, np.ndarray]], periodic: bool = False, elem_type_code: int | None = None):
+              boundary: list[tuple[int, str, np.ndarray]], periodic=None, elem_type_code: int | None = None):
     """node_tags (N,), xyz (N, 3); elems (E, nvert) of node tags; boundary: (physical tag, element
-    kind, (n, nfvert) node tags) per group. elem_type_code overrides the gmsh type of the domain
-    elements (to write unsupported types)."""
+    kind, (n, nfvert) node tags) per group; periodic: the (slave tag, master tag) node pairs of one
+    $Periodic link, or a list of such lists (one link each; the affine part is written as the
+    identity, which the reader skips).
+    elem_type_code overrides the gmsh type of the domain elements (to write unsupported types)."""
     lines = ["$MeshFormat", "4.1 0 8", "$EndMeshFormat"]
     nb = len(boundary)
     counts = [0, 0, 0, 0]
@@ -42,8 +44,14 @@ def write_msh(path, dim: int, node_tags: np.ndarray, xyz: np.ndarray, elem_type:
             lines.append(f"{tag} " + " ".join(str(int(v)) for v in c))
             tag += 1
     lines.append("$EndElements")
-    if periodic:
-        lines += ["$Periodic", "0", "$EndPeriodic"]
+    if periodic is not None:
+        links = periodic if periodic and isinstance(periodic[0], list) else [periodic]
+        lines += ["$Periodic", str(len(links))]
+        for k, link in enumerate(links):
+            lines += [f"{dim - 1} {2 * k + 1} {2 * k + 2}",
+                      "16 " + " ".join("1" if i % 5 == 0 else "0" for i in range(16)), str(len(link))]
+            lines += [f"{int(a)} {int(b)}" for a, b in link]
+        lines.append("$EndPeriodic")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 

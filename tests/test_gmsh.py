@@ -2,8 +2,8 @@
 ReadControl.cpp:286-335): a file holding a structured grid in the structured builder's order gives
 bit-identical operand tables (quadrangles and hexahedra); shuffled node tags, shuffled and mirrored
 elements and cyclically rotated vertex lists give the same element integrals of the oracle residual
-(matched by centroid); an unstructured triangulation preserves the free stream; unsupported input
-is refused. GPU parity on a file mesh is in test_gpu_gmsh.py."""
+(matched by centroid); $Periodic node pairs give the structured builder's periodic tables; an
+unstructured triangulation preserves the free stream; unsupported input is refused. GPU parity on a file mesh is in test_gpu_gmsh.py."""
 import numpy as np
 import pytest
 
@@ -45,6 +45,33 @@ def test_hexahedron_file_equals_the_structured_builder(tmp_path):
     g = L.Mesh.gmsh(path, 2, {10 + k: kinds[k] for k in range(6)})
     s = L.Mesh(L.ELEM_HEXAHEDRON, 2, (n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0),
                boundary=((FAR, FAR), (SLIP, SLIP), (L.BC_ADIABATIC_NONSLIP_WALL,) * 2))
+    _same_tables(g, s)
+
+
+def test_periodic_file_equals_the_structured_builder(tmp_path):
+    """$Periodic links (x = 1 onto x = 0, y = 1 onto y = 0, the corner a slave in both) give the
+    structured builder's periodic tables bit for bit."""
+    nx, ny = 4, 3
+    tags, xyz, elems, _ = W.grid_quads(nx, ny)
+    tag = lambda i, j: 1 + i + (nx + 1) * j  # noqa: E731
+    pairs = [[(tag(nx, j), tag(0, j)) for j in range(ny + 1)], [(tag(i, ny), tag(i, 0)) for i in range(nx + 1)]]
+    path = str(tmp_path / "p.msh")
+    W.write_msh(path, 2, tags, xyz, "quad", elems, [], periodic=pairs)
+    g = L.Mesh.gmsh(path, 3, {})
+    s = L.Mesh(L.ELEM_QUADRANGLE, 3, (nx, ny), (0.0, 0.0), (1.0, 1.0))
+    _same_tables(g, s)
+
+
+def test_periodic_hexahedra_with_walls(tmp_path):
+    n = 2
+    tags, xyz, elems, sides = W.grid_hexes(n)
+    tag = lambda i, j, k: 1 + i + (n + 1) * (j + (n + 1) * k)  # noqa: E731
+    pairs = [(tag(i, j, n), tag(i, j, 0)) for j in range(n + 1) for i in range(n + 1)]  # z periodic
+    path = str(tmp_path / "ph.msh")
+    W.write_msh(path, 3, tags, xyz, "hex", elems, [(10 + k, "bquad", sides[k]) for k in range(4)], periodic=pairs)
+    g = L.Mesh.gmsh(path, 2, {10: FAR, 11: FAR, 12: SLIP, 13: SLIP})
+    s = L.Mesh(L.ELEM_HEXAHEDRON, 2, (n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0),
+               boundary=((FAR, FAR), (SLIP, SLIP), None))
     _same_tables(g, s)
 
 
@@ -105,9 +132,10 @@ def test_shuffled_and_mirrored_quadrangles_give_the_same_element_residuals(tmp_p
     assert np.allclose(rg[ig], rs[is_], rtol=1e-11, atol=1e-12 * np.abs(rs).max())
 
 
-def triangle_file(path, n=6, seed=3, bc=(1, 1, 1, 1)):
+def triangle_file(path, n=6, seed=3, bc=(1, 1, 1, 1), periodic=False):
     """Unstructured-like triangulation of the unit square: an (n+1)^2 node grid with jittered
-    interior nodes, each cell cut along a random diagonal; sides in physical groups bc."""
+    interior nodes, each cell cut along a random diagonal; sides in physical groups bc, or, with
+    periodic=True, $Periodic links x = 1 -> x = 0 and y = 1 -> y = 0 and no boundary groups."""
     tags, xyz, quads, sides = W.grid_quads(n, n)
     rng = np.random.default_rng(seed)
     inner = (xyz[:, 0] > 0) & (xyz[:, 0] < 1) & (xyz[:, 1] > 0) & (xyz[:, 1] < 1)
@@ -117,6 +145,11 @@ def triangle_file(path, n=6, seed=3, bc=(1, 1, 1, 1)):
     for q in quads:
         a, b, c, d = q
         tris += [[a, b, c], [a, c, d]] if rng.random() < 0.5 else [[a, b, d], [b, c, d]]
+    if periodic:
+        tag = lambda i, j: 1 + i + (n + 1) * j  # noqa: E731
+        links = [[(tag(n, j), tag(0, j)) for j in range(n + 1)], [(tag(i, n), tag(i, 0)) for i in range(n + 1)]]
+        W.write_msh(path, 2, tags, xyz, "tri", np.array(tris), [], periodic=links)
+        return
     W.write_msh(path, 2, tags, xyz, "tri", np.array(tris), [(bc[k], "line", sides[k]) for k in range(4)])
 
 
@@ -136,12 +169,28 @@ def test_unstructured_triangles_free_stream_and_conservation(tmp_path, order):
     assert m.nodes == 49 and np.bincount(m.field("node_tag")).min() >= 1
 
 
+def test_periodic_triangulation_free_stream_and_conservation(tmp_path):
+    path = str(tmp_path / "pt.msh")
+    triangle_file(path, periodic=True)
+    m = L.Mesh.gmsh(path, 3, {})
+    assert m.Fb == 0 and m.F == 3 * m.E // 2
+    u = (1.0, 0.3, 0.2, 1.0)
+    o = O.Oracle(_euler(), m)
+    o.set_initial_primitive(np.tile(np.asarray(u), (m.E, m.nq, 1)))
+    assert np.abs(o.residual()).max() < 1e-12
+    xq = m.field("quad_coord").reshape(m.E, m.nq, 2)
+    o.set_initial_primitive(_state(xq))
+    r = o.residual()
+    one = P.project_conserved(m, np.ones((m.E, m.nq, 4)))[:, :, 0]
+    assert np.abs(np.einsum("eb,ebv->v", one, r)).max() < 1e-12 * np.abs(r).max() * m.E
+
+
 def test_refusals(tmp_path):
     tags, xyz, elems, sides = W.grid_quads(3, 3)
     groups = [(1, "line", s) for s in sides]
     p = str(tmp_path / "per.msh")
-    W.write_msh(p, 2, tags, xyz, "quad", elems, groups, periodic=True)
-    with pytest.raises(L.SDGError, match="periodic"):
+    W.write_msh(p, 2, tags, xyz, "quad", elems, groups, periodic=[(1, 999)])
+    with pytest.raises(L.SDGError, match="unknown node"):
         L.Mesh.gmsh(p, 2, {1: FAR})
     p = str(tmp_path / "curved.msh")
     W.write_msh(p, 2, tags, xyz, "quad", elems, groups, elem_type_code=10)  # 9-node quadrangle

@@ -1,6 +1,6 @@
 """GPU parity on meshes read from gmsh files (sdg_mesh_create_gmsh): an unstructured triangulation
-with far-field and slip-wall groups, Euler with and without artificial viscosity, residual and 5
-SSP-RK3 steps against the oracle at the FP64 bar."""
+with far-field and slip-wall groups or with $Periodic links, Euler with and without artificial
+viscosity, residual and 5 SSP-RK3 steps against the oracle at the FP64 bar."""
 import numpy as np
 import pytest
 
@@ -12,11 +12,12 @@ from test_gmsh import triangle_file
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("order,av", [(2, False), (3, False), (3, True)], ids=["p2", "p3", "p3-av"])
-def test_triangle_file_mesh_parity(tmp_path, order, av):
-    print(f"gmsh triangle parity P{order} av={av}", flush=True)
+@pytest.mark.parametrize("order,av,periodic", [(2, False, False), (3, False, False), (3, True, False), (3, False, True)],
+                         ids=["p2", "p3", "p3-av", "p3-periodic"])
+def test_triangle_file_mesh_parity(tmp_path, order, av, periodic):
+    print(f"gmsh triangle parity P{order} av={av} periodic={periodic}", flush=True)
     path = str(tmp_path / "t.msh")
-    triangle_file(path, n=6, bc=(1, 1, 2, 2))
+    triangle_file(path, n=6, bc=(1, 1, 2, 2), periodic=periodic)
     mesh = L.Mesh.gmsh(path, order, {1: L.BC_RIEMANN_FARFIELD, 2: L.BC_ADIABATIC_SLIP_WALL})
     prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=L.FLUX_HLLC, cp=3.5, cv=2.5,
                           shock_capturing=L.SHOCK_ARTIFICIAL_VISCOSITY if av else L.SHOCK_NONE,
@@ -29,11 +30,13 @@ def test_triangle_file_mesh_parity(tmp_path, order, av):
     bprim = np.tile(np.array([1.0, 0.3, 0.0, 1.0]), (mesh.Fb, mesh.nfq, 1))
     s = L.Solver(prob, mesh)
     s.set_initial_primitive(prim)
-    s.set_boundary_primitive(bprim)
+    if mesh.Fb:
+        s.set_boundary_primitive(bprim)
     o, x = O.Oracle(prob, mesh), O.Oracle(prob, mesh, extended=True)
     for k in (o, x):
         k.set_state(s.get_state())
-        k.set_boundary_primitive(bprim)
+        if mesh.Fb:
+            k.set_boundary_primitive(bprim)
     u0 = o.get_state()
     P.fp64_parity(s.residual(), o.residual(), x.residual(), u0, 2, tol=1e-12)
     if av:
