@@ -140,9 +140,6 @@ struct System {
   static_assert(SC::kEquationModel == EquationModelEnum::CompresibleEuler ||
                     SC::kEquationModel == EquationModelEnum::CompresibleNS,
                 "the GPU engine implements the compressible Euler and Navier-Stokes models");
-  static_assert(SC::kShockCapturing == ShockCapturingEnum::None ||
-                    SC::kEquationModel == EquationModelEnum::CompresibleEuler,
-                "the GPU engine runs artificial viscosity with the compressible Euler model");
 
   InitialCondition<SC> initial_condition_;
   BoundaryCondition<SC> boundary_condition_;

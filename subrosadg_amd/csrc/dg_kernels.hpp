@@ -148,6 +148,7 @@ struct StageArgs {
   const double* Uf_in;
   double* Uf_out;
   double* Gf;
+  double* GfV;    // Navier-Stokes with artificial viscosity: E x (NV D) x N_aq volume-gradient traces
   double* Rvol;   // (sum-factorised path: the weak divergence of the volume flux, FP32 values in
                   // the mixed mode)
   double* Rout;   // residual output (mode 1)
@@ -447,7 +448,10 @@ __global__ __launch_bounds__(kThreads) void eulerStageKernel(Geo g, Phys ph, Sta
 //   BR1 face trace: w Phi_f^T
 // which equals the reference's per-face lifting r_s = R_w,s M^-1 (BR2, eta = 1,
 // VariableConvertor.cpp:674-688) up to rounding.
-template <int D, int NB, int NQ, int NF, int NFQ, int EPB, int VISC>
+// VISC = 0: Euler with artificial viscosity (the traces and the flux use the volume gradient);
+// VISC = 1, 2 with AV: Navier-Stokes with artificial viscosity (the volume-gradient traces go to
+// GfV, the volume flux also subtracts eps grad U with the volume gradient).
+template <int D, int NB, int NQ, int NF, int NFQ, int EPB, int VISC, bool AV = false>
 __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageArgs a) {
   using S = Sizes<D, NB, NQ, NF, NFQ>;
   constexpr int NV = S::NV, NG = S::NG, NAQ = S::NAQ;
@@ -589,6 +593,11 @@ __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageAr
       for (int b = 0; b < NB; b++) t += sWt[le][r][b] * g.Vft[b * NAQ + sl];
     }
     if (e < g.E) a.Gf[static_cast<int64_t>(e) * NG * NAQ + rem] = t;
+    if constexpr (AV && VISC != 0) {
+      double tv = 0.0;
+      for (int b = 0; b < NB; b++) tv += sWv[le][r][b] * g.Vft[b * NAQ + sl];
+      if (e < g.E) a.GfV[static_cast<int64_t>(e) * NG * NAQ + rem] = tv;
+    }
   }
   __syncthreads();
   // volume flux: (F_c - F_v)^T (J^T)^-1 |J| w (SpatialDiscrete.cpp:216-253)
@@ -625,6 +634,21 @@ __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageAr
           for (int aa = 0; aa < D; aa++) t += (eps * gq[v * D + aa]) * G[aa + D * b];
           out[v * (D * NQ) + q * D + b] = s - t;
         }
+    } else if constexpr (AV) {  // (F_c - F_v)^T G - (eps grad U)^T G with the volume gradient
+      const double eps = avAt(g, g.nodq, q, e);
+      double gv[NG];
+      for (int r = 0; r < NG; r++) {
+        double sv = 0.0;
+        for (int b = 0; b < NB; b++) sv += sWv[le][r][b] * g.Vt[b * NQ + q];
+        gv[r] = sv;
+      }
+      for (int v = 0; v < NV; v++)
+        for (int b = 0; b < D; b++) {
+          double s = 0.0, t = 0.0;
+          for (int aa = 0; aa < D; aa++) s += (fc[aa + D * v] - fv[aa + D * v]) * G[aa + D * b];
+          for (int aa = 0; aa < D; aa++) t += (eps * gv[v * D + aa]) * G[aa + D * b];
+          out[v * (D * NQ) + q * D + b] = s - t;
+        }
     } else {
       for (int v = 0; v < NV; v++)
         for (int b = 0; b < D; b++) {
@@ -649,10 +673,11 @@ __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageAr
 // ---- Navier-Stokes, kernel 2: face fluxes + residual + update + next traces -------------------
 // calculateAdjacencyQuadrature, calculateResidual, updateBasisFunctionCoefficient
 // (SpatialDiscrete.cpp:633-748,1016-1032; TimeIntegration.cpp:181-198)
-// AV = true: Euler with artificial viscosity, where Gf holds the volume gradient; the face flux is
+// AV = 1: Euler with artificial viscosity, where Gf holds the volume gradient; the face flux is
 // the convective one minus the central average of eps grad U . n (boundary faces: the element's
-// own value), SpatialDiscrete.cpp:714-735,812-830.
-template <int D, int NB, int NQ, int NF, int NFQ, int EPB, bool AV = false>
+// own value), SpatialDiscrete.cpp:714-735,812-830. AV = 2: Navier-Stokes with artificial
+// viscosity, the same term from the volume-gradient traces GfV after the viscous flux.
+template <int D, int NB, int NQ, int NF, int NFQ, int EPB, int AV = 0>
 __global__ __launch_bounds__(kThreads) void nsFluxKernel(Geo g, Phys ph, StageArgs a) {
   using S = Sizes<D, NB, NQ, NF, NFQ>;
   constexpr int NV = S::NV, NG = S::NG, NAQ = S::NAQ;
@@ -681,12 +706,14 @@ __global__ __launch_bounds__(kThreads) void nsFluxKernel(Geo g, Phys ph, StageAr
       for (int v = 0; v < NV; v++) uo[v] = a.Uf_in[static_cast<int64_t>(e) * NV * NAQ + v * NAQ + sl];
       for (int r = 0; r < NG; r++) go[r] = a.Gf[static_cast<int64_t>(e) * NG * NAQ + r * NAQ + sl];
       const int bp = -1 - nbs;
-      boundaryFlux<D, !AV>(ph, g, bp, uo, go, false, 0.0, 0.0, t);
-      if constexpr (AV) {
+      boundaryFlux<D, AV != 1>(ph, g, bp, uo, go, false, 0.0, 0.0, t);
+      if constexpr (AV != 0) {
         const double eps = avAt(g, g.nodf, sl, e), jw = g.bjw[bp];
+        const double* gsrc = AV == 2 ? a.GfV : a.Gf;
         for (int v = 0; v < NV; v++) {
           double f = 0.0;
-          for (int d = 0; d < D; d++) f += (eps * go[v * D + d]) * g.bnormal[static_cast<int64_t>(bp) * D + d];
+          for (int d = 0; d < D; d++)
+            f += (eps * gsrc[static_cast<int64_t>(e) * NG * NAQ + (v * D + d) * NAQ + sl]) * g.bnormal[static_cast<int64_t>(bp) * D + d];
           t[v] -= f * jw;
         }
       }
@@ -713,7 +740,7 @@ __global__ __launch_bounds__(kThreads) void nsFluxKernel(Geo g, Phys ph, StageAr
     computational<D>(ph, ur, wr);
     for (int d = 0; d < D; d++) n[d] = g.fnormal[fp * D + d];
     convectiveFlux<D>(ph, n, ul, wl, ur, wr, h);
-    if constexpr (AV) {
+    if constexpr (AV == 1) {
       const double el = avAt(g, g.nodf, lsl, static_cast<int>(lo)), er = avAt(g, g.nodf, rsl, static_cast<int>(ro));
       const double jw = g.fjw[fp];
       for (int v = 0; v < NV; v++) {
@@ -731,13 +758,24 @@ __global__ __launch_bounds__(kThreads) void nsFluxKernel(Geo g, Phys ph, StageAr
     viscousRawFlux<D>(ph, wl, pgl, fvl);
     viscousRawFlux<D>(ph, wr, pgr, fvr);
     const double jw = g.fjw[fp];
+    double el = 0.0, er = 0.0;
+    if constexpr (AV == 2) {
+      el = avAt(g, g.nodf, lsl, static_cast<int>(lo));
+      er = avAt(g, g.nodf, rsl, static_cast<int>(ro));
+    }
     for (int v = 0; v < NV; v++) {
       double vl = 0.0, vr = 0.0;
       for (int d = 0; d < D; d++) {
         vl += fvl[d + D * v] * n[d];
         vr += fvr[d + D * v] * n[d];
       }
-      const double t = (h[v] - (vl + vr) / 2.0) * jw;
+      double t = (h[v] - (vl + vr) / 2.0) * jw;
+      if constexpr (AV == 2) {  // minus the central average of eps grad U . n (volume gradient)
+        double al = 0.0, ar = 0.0;
+        for (int d = 0; d < D; d++) al += (el * a.GfV[lo * NG * NAQ + (v * D + d) * NAQ + lsl]) * n[d];
+        for (int d = 0; d < D; d++) ar += (er * a.GfV[ro * NG * NAQ + (v * D + d) * NAQ + rsl]) * n[d];
+        t -= ((al + ar) / 2.0) * jw;
+      }
       sH[le][v][sl] = side == 0 ? t : -t;
     }
   }

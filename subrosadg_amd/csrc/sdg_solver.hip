@@ -52,6 +52,8 @@ struct Launchers {
   // residual + update, and the per-step sensor
   StageFn av_grad = nullptr;
   StageFn av_flux = nullptr;
+  StageFn av_ns_grad[2] = {nullptr, nullptr};  // Navier-Stokes with artificial viscosity: BR1, BR2
+  StageFn av_ns_flux = nullptr;
   AvFn av_sensor = nullptr;
   // discrete adjoint (Navier-Stokes, FP64 matrix-core sizes only)
   AdjFn adj_volume[2] = {nullptr, nullptr};  // BR1, BR2
@@ -142,7 +144,19 @@ Launchers makeLaunchers() {
                          dim3(sdg::kThreads), 0, st, g, ph, a);
     };
     l.av_flux = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-      hipLaunchKernelGGL((sdg::nsFluxKernel<D, NB, NQ, NF, NFQ, EPB, true>), dim3(blocksFor(g.E, EPB)),
+      hipLaunchKernelGGL((sdg::nsFluxKernel<D, NB, NQ, NF, NFQ, EPB, 1>), dim3(blocksFor(g.E, EPB)),
+                         dim3(sdg::kThreads), 0, st, g, ph, a);
+    };
+    l.av_ns_grad[0] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+      hipLaunchKernelGGL((sdg::nsGradKernel<D, NB, NQ, NF, NFQ, EPB, 1, true>), dim3(blocksFor(g.E, EPB)),
+                         dim3(sdg::kThreads), 0, st, g, ph, a);
+    };
+    l.av_ns_grad[1] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+      hipLaunchKernelGGL((sdg::nsGradKernel<D, NB, NQ, NF, NFQ, EPB, 2, true>), dim3(blocksFor(g.E, EPB)),
+                         dim3(sdg::kThreads), 0, st, g, ph, a);
+    };
+    l.av_ns_flux = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+      hipLaunchKernelGGL((sdg::nsFluxKernel<D, NB, NQ, NF, NFQ, EPB, 2>), dim3(blocksFor(g.E, EPB)),
                          dim3(sdg::kThreads), 0, st, g, ph, a);
     };
     l.av_sensor = [](hipStream_t st, const Geo& g, const Phys& ph, const double* U, double* node) {
@@ -394,6 +408,7 @@ struct sdg_ctx {
   double* lsq = nullptr;  // host copy
   std::vector<double> h_lsq, h_V, h_chol;
   std::vector<int> h_bparent;  // parent element of each boundary face
+  double* GfV = nullptr;  // Navier-Stokes with artificial viscosity: volume-gradient traces
   double *U = nullptr, *Un = nullptr, *Uf[2] = {nullptr, nullptr}, *Gf = nullptr, *Rvol = nullptr, *Hs = nullptr, *Fv = nullptr,
          *Vn = nullptr, *Rsrc = nullptr, *src_li = nullptr;
   double *Rout = nullptr, *Wout = nullptr, *mon = nullptr, *rel = nullptr, *dtbuf = nullptr;
@@ -524,6 +539,7 @@ StageArgs baseArgs(sdg_ctx* ctx) {
   a.Uf_in = ctx->Uf[ctx->cur];
   a.Uf_out = ctx->Uf[ctx->cur ^ 1];
   a.Gf = ctx->Gf;
+  a.GfV = ctx->GfV;
   a.Rvol = ctx->Rvol;
   a.Rout = ctx->Rout;
   a.Wout = nullptr;
@@ -568,7 +584,9 @@ void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hi
   if (ctx->Rsrc) ctx->launch.source(ctx->stream, ctx->geo, ctx->phys, ctx->U, ctx->Rsrc, ctx->src_li);
   if (ev) (void)hipEventRecord(ev[0], ctx->stream);
   if (ctx->ns()) {
-    if (ctx->prob.viscous_flux == SDG_VISC_BR1)
+    if (ctx->av)
+      ctx->launch.av_ns_grad[ctx->prob.viscous_flux == SDG_VISC_BR1 ? 0 : 1](ctx->stream, ctx->geo, ctx->phys, a);
+    else if (ctx->prob.viscous_flux == SDG_VISC_BR1)
       ctx->launch.ns_grad_br1(ctx->stream, ctx->geo, ctx->phys, a);
     else
       ctx->launch.ns_grad_br2(ctx->stream, ctx->geo, ctx->phys, a);
@@ -581,7 +599,10 @@ void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hi
       (void)hipEventRecord(ev[3], ctx->stream);
       (void)hipEventRecord(ev[4], ctx->stream);
     }
-    ctx->launch.ns_flux(ctx->stream, ctx->geo, ctx->phys, a);
+    if (ctx->av)
+      ctx->launch.av_ns_flux(ctx->stream, ctx->geo, ctx->phys, a);
+    else
+      ctx->launch.ns_flux(ctx->stream, ctx->geo, ctx->phys, a);
     if (ev) (void)hipEventRecord(ev[5], ctx->stream);
   } else if (ctx->av) {
     ctx->launch.av_grad(ctx->stream, ctx->geo, ctx->phys, a);
@@ -777,11 +798,11 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   const char* dense = std::getenv("SDG_DENSE");  // 1: keep the dense matrix-core NS kernels on hexahedra
   const bool tensor = problem->equation == SDG_EQ_COMPRESSIBLE_NS && !(dense && dense[0] == '1');
   ctx->av = problem->shock_capturing == SDG_SHOCK_ARTIFICIAL_VISCOSITY;
-  if (ctx->av && (problem->equation != SDG_EQ_COMPRESSIBLE_EULER || mixed)) {
+  if (ctx->av && mixed) {
     delete ctx;
-    return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: artificial viscosity runs with the Euler model in FP64");
+    return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: artificial viscosity runs in FP64");
   }
-  if (!selectLaunchers(r.type, r.order, (fv && fv[0] == '1') || ctx->av, mixed, ctx->launch, tensor)) {
+  if (!selectLaunchers(r.type, r.order, (fv && fv[0] == '1') || ctx->av, mixed, ctx->launch, tensor && !ctx->av)) {
     delete ctx;
     return fail(nullptr, SDG_E_UNSUPPORTED,
                 mixed ? "sdg_create: the mixed-precision mode needs a matrix-core kernel set (quadrangle P3-P5, "
@@ -1178,9 +1199,10 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     }
     if (tensor && (allocate(ctx, nUf, &ctx->Vn) || allocate(ctx, nU, &ctx->Rvol))) return bail(SDG_E_NOMEM);
   }
-  if (ctx->av &&
+  if (ctx->av && !ctx->ns() &&
       (allocate(ctx, static_cast<size_t>(E) * ctx->ng * naq, &ctx->Gf) || allocate(ctx, nU, &ctx->Rvol)))
     return bail(SDG_E_NOMEM);
+  if (ctx->av && ctx->ns() && allocate(ctx, static_cast<size_t>(E) * ctx->ng * naq, &ctx->GfV)) return bail(SDG_E_NOMEM);
   // modal Boussinesq source for the kernel sets that take it from a buffer (the sum-factorised
   // kernels form it at their nodes)
   if (ctx->phys.source != SDG_SOURCE_NONE && !ctx->launch.tensor) {

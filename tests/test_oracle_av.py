@@ -208,6 +208,21 @@ def test_energy_identity_for_uniform_viscosity(mesh_kind, order):
     assert abs(lhs + eps * energy) <= 1e-10 * eps * energy
 
 
+def test_navier_stokes_keeps_the_same_viscosity_term():
+    """With Navier-Stokes the artificial term is the same linear operator of the volume gradient:
+    R_NS+AV(f) - R_NS+AV(0) equals the Euler R_AV(f) - R_AV(0) at the same state (the viscous and
+    convective parts cancel in each difference)."""
+    mesh, _, prim = P.euler_av(n=6, order=3, tol=0.5, factor=1.0)
+    C = _conserved(mesh, prim)
+    r_euler, _ = _av_residual(mesh, C, 0.5, 1.0)
+    common = dict(equation=L.EQ_NS, convective_flux=L.FLUX_HLLC, viscous_flux=L.VISC_BR2,
+                  transport=L.TRANSPORT_CONSTANT, dynamic_viscosity=1e-3, cp=3.5, cv=2.5,
+                  shock_capturing=L.SHOCK_ARTIFICIAL_VISCOSITY, empirical_tolerance=0.5)
+    r1 = _oracle_with_state(mesh, L.problem_desc(artificial_viscosity_factor=1.0, **common), C).residual()
+    r0 = _oracle_with_state(mesh, L.problem_desc(artificial_viscosity_factor=0.0, **common), C).residual()
+    assert np.abs((r1 - r0) - r_euler).max() <= 1e-11 * np.abs(r_euler).max()
+
+
 def test_conservation_with_varying_viscosity():
     mesh, prob, prim = P.euler_av(n=6, order=3, tol=0.5, factor=1.0)
     C = _conserved(mesh, prim)
