@@ -169,16 +169,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-mesh leg")
+    ap.add_argument("--no-shock", action="store_true", help="skip the artificial-viscosity leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the HBM-counter calibration copy of this many MiB")
-    ap.add_argument("--profile-leg", choices=["adjoint", "batch"], default=None,
+    ap.add_argument("--profile-leg", choices=["adjoint", "batch", "shock"], default=None,
                     help="only run one secondary leg (for kernel profiling) and print its record")
     args = ap.parse_args()
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
         return
     if args.profile_leg:
-        leg = adjoint_leg(args) if args.profile_leg == "adjoint" else batch_leg(args)
+        leg = {"adjoint": adjoint_leg, "batch": batch_leg, "shock": shock_leg}[args.profile_leg](args)
         print(json.dumps(leg), flush=True)
         return
     if args.gpus != 1:
@@ -207,6 +208,7 @@ def main():
 
     adjoint = None if args.no_adjoint else adjoint_leg(args)
     batch = None if args.no_batch else batch_leg(args)
+    shock = None if args.no_shock else shock_leg(args)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.order)
     line = {
         "metric": "DOF.stage/s mixed vs FP64 at fixed L2 error (3-D NS Taylor-Green P3)",
@@ -238,9 +240,53 @@ def main():
         "roofline_fp64": roof["fp64"],
         "adjoint": adjoint,
         "batch": batch,
+        "shock_capturing": shock,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def shock_leg(args, n=256, order=3, steps=100, cfl=0.1):
+    """Artificial-viscosity cost on the explosion case (2-D Euler, [0,2]^2 with adiabatic slip
+    walls, a circle of radius 0.4 at rho = 1, p = 1 in rho = 0.125, p = 0.1 spread over half a
+    cell, 256^2 quadrangles P3, HLLC, SSP-RK3, setArtificialViscosity(4.0)): the step with the
+    per-step sensor and the gradient passes (vector kernels) against the plain Euler step on the
+    same mesh and state."""
+    slip = L.BC_ADIABATIC_SLIP_WALL
+    mesh = L.Mesh(L.ELEM_QUADRANGLE, order, (n, n), (0.0, 0.0), (2.0, 2.0), boundary=((slip, slip), (slip, slip)))
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 2)
+    r = np.sqrt((xq[..., 0] - 1.0) ** 2 + (xq[..., 1] - 1.0) ** 2)
+    w = 0.5 * (1.0 - np.tanh((r - 0.4) / (1.0 / n)))
+    gas = 2.5 - 25.0 / 14.0
+    prim = np.zeros((mesh.E, mesh.nq, 4))
+    prim[..., 0] = 0.125 + 0.875 * w
+    prim[..., 3] = (0.1 + 0.9 * w) / (prim[..., 0] * gas)
+    out = {"workload": f"2-D Euler explosion, [0,2]^2 slip walls, {n}^2 quads, P{order}, HLLC, SSP-RK3, CFL {cfl}, "
+                       f"artificial viscosity (tolerance 4.0, factor 1)", "elements": mesh.E, "steps": steps}
+    for label, sc in (("artificial_viscosity", L.SHOCK_ARTIFICIAL_VISCOSITY), ("plain", L.SHOCK_NONE)):
+        prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=L.FLUX_HLLC, cp=2.5, cv=25.0 / 14.0,
+                              shock_capturing=sc, empirical_tolerance=4.0, artificial_viscosity_factor=1.0)
+        s = L.Solver(prob, mesh)
+        s.set_initial_primitive(prim)
+        s.set_boundary_primitive(np.zeros((mesh.Fb, mesh.nfq, 4)))
+        dt = s.compute_dt(cfl)
+        s.step(dt, 5)
+        kms = s.time_kernels(dt, 5)
+        s.synchronize()
+        t0 = time.perf_counter()
+        _, nan, taken = s.step(dt, steps)
+        s.synchronize()
+        el = (time.perf_counter() - t0) / steps
+        rec = {"ms_per_step": el * 1e3, "dof_stage_per_s": mesh.E * mesh.nb * 3 / el, "finite": bool(nan == 0),
+               "kernel_ms": [float(x) for x in kms]}
+        if sc:
+            av = s.artificial_viscosity()
+            rec["nodes_with_viscosity"] = int((av > 0).sum())
+            rec["nodes"] = int(av.size)
+        out[label] = rec
+    out["cost_over_plain"] = out["artificial_viscosity"]["ms_per_step"] / out["plain"]["ms_per_step"]
+    print(f"[bench] shock leg: {out}", file=sys.stderr, flush=True)
+    return out
 
 
 def adjoint_leg(args, n=64, order=3, steps=400, cfl=0.1):
