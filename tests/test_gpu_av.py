@@ -252,3 +252,22 @@ def test_cpp_explosion_example(tmp_path):
     av = np.array(arrays["ArtificialViscosity"].text.split(), dtype=np.float64)
     rho = np.array(arrays["Density"].text.split(), dtype=np.float64)
     assert av.max() > 0.0 and rho.min() > 0.0 and int(arrays["TMSTEP"].text) == 150
+
+
+def test_restart_with_artificial_viscosity_is_bit_identical(tmp_path):
+    """LastStep restart (InitialCondition.cpp:46-59) with shock capturing: the node values are
+    formed again from the state at the start of every step, so a run restarted from a raw file
+    continues bit for bit."""
+    mesh, prob, prim, _ = _case("quad", 3, None, 6)
+    s = L.Solver(prob, mesh)
+    s.set_initial_primitive(prim)
+    dt = s.compute_dt(0.05)
+    s.step(dt, 3)
+    path = str(tmp_path / "av_3.zst")
+    s.write_raw(path, node_count=mesh.nodes)
+    r = L.Solver(prob, mesh)
+    r.read_raw(path, L.IC_LAST_STEP)
+    s.step(dt, 4)
+    r.step(dt, 4)
+    assert np.array_equal(r.get_state(), s.get_state())
+    assert np.array_equal(r.artificial_viscosity(), s.artificial_viscosity())
