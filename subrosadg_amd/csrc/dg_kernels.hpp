@@ -855,27 +855,50 @@ __global__ __launch_bounds__(kThreads) void traceKernel(Geo g, const double* U, 
 }
 
 // CFL time step per element: min_q CFL minEdge / ((|u| + c)(P+1)^2) (TimeIntegration.cpp:104-131)
+// CFL time step per element (TimeIntegration.cpp:104-131): min over its volume points of
+// cfl minEdge / ((|u| + c) (P+1)^2). EPB elements per workgroup: the coefficient blocks are staged
+// in LDS with coalesced loads, one thread per point, then a sequential minimum per element.
+template <int D, int NB, int NQ>
+struct DtShape {
+  static constexpr int EPB = kThreads / NQ < 1 ? 1 : (kThreads / NQ > 16 ? 16 : kThreads / NQ);
+};
 template <int D, int NB, int NQ, int NF, int NFQ>
 __global__ __launch_bounds__(kThreads) void dtKernel(Geo g, Phys ph, const double* U, const double* minedge, double cfl,
                                                      int order, double* out) {
-  constexpr int NV = D + 2;
-  const int e = blockIdx.x * kThreads + threadIdx.x;
-  if (e >= g.E) return;
-  double best = 1.0e308;
-  for (int q = 0; q < NQ; q++) {
+  constexpr int NV = D + 2, EPB = DtShape<D, NB, NQ>::EPB;
+  __shared__ double sU[EPB][NV * NB];
+  __shared__ double sT[EPB][NQ];
+  const int tid = threadIdx.x;
+  const int64_t e0 = static_cast<int64_t>(blockIdx.x) * EPB;
+  for (int idx = tid; idx < EPB * NV * NB; idx += kThreads) {
+    const int le = idx / (NV * NB), k = idx % (NV * NB);
+    const int64_t e = e0 + le;
+    sU[le][k] = e < g.E ? U[e * NV * NB + k] : 1.0;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < EPB * NQ; idx += kThreads) {
+    const int le = idx / NQ, q = idx % NQ;
+    const int64_t e = e0 + le;
+    if (e >= g.E) continue;
     double u[NV], w[D + 3];
     for (int v = 0; v < NV; v++) {
       double s = 0.0;
-      for (int b = 0; b < NB; b++) s += U[static_cast<int64_t>(e) * NV * NB + v * NB + b] * g.Vt[b * NQ + q];
+      for (int b = 0; b < NB; b++) s += sU[le][v * NB + b] * g.Vt[b * NQ + q];
       u[v] = s;
     }
     computational<D>(ph, u, w);
     const double c = soundSpeedOf(ph, w[0], w[D + 2]);
     const double sr = sqrt(velocitySquared<D>(w)) + c;
-    const double t = cfl * minedge[e] / (sr * (order + 1.0) * (order + 1.0));
-    best = fmin(best, t);
+    sT[le][q] = cfl * minedge[e] / (sr * (order + 1.0) * (order + 1.0));
   }
-  out[e] = best;
+  __syncthreads();
+  for (int le = tid; le < EPB; le += kThreads) {
+    const int64_t e = e0 + le;
+    if (e >= g.E) continue;
+    double best = 1.0e308;
+    for (int q = 0; q < NQ; q++) best = fmin(best, sT[le][q]);
+    out[e] = best;
+  }
 }
 
 // Fixed-order reduction of the monitor partials; raises the NaN stop flag when every variable is

@@ -184,7 +184,7 @@ Launchers makeLaunchers() {
   };
   l.dt = [](hipStream_t st, const Geo& g, const Phys& ph, const double* U, const double* me, double cfl, int order,
             double* out) {
-    hipLaunchKernelGGL((sdg::dtKernel<D, NB, NQ, NF, NFQ>), dim3((g.E + sdg::kThreads - 1) / sdg::kThreads),
+    hipLaunchKernelGGL((sdg::dtKernel<D, NB, NQ, NF, NFQ>), dim3(blocksFor(g.E, sdg::DtShape<D, NB, NQ>::EPB)),
                        dim3(sdg::kThreads), 0, st, g, ph, U, me, cfl, order, out);
   };
   l.monitor = [](hipStream_t st, const double* mon, int E, double* rel, int* status) {
