@@ -395,6 +395,12 @@ class Solver:
         self._check(self.lib.sdg_get_gradient(self.handle, out.ctypes.data))
         return out
 
+    def write_vtu(self, path: str, step: int = 0, time: float = 0.0):
+        """VTU file of the current state (with the node artificial viscosity when shock capturing
+        is on); see sdg_write_vtu."""
+        nodes = self.artificial_viscosity() if getattr(self.problem, "shock_capturing", 0) else None
+        write_vtu(self.mesh, self.problem, self.get_state(), path, node_values=nodes, step=step, time=time)
+
     def artificial_viscosity(self) -> np.ndarray:
         """Node values of the last artificial-viscosity evaluation (mesh nodes long)."""
         out = np.empty(self.mesh.nodes)

@@ -160,6 +160,14 @@ def test_raw_file_holds_the_node_values(tmp_path):
     assert np.array_equal(data[: u.size], u.reshape(-1))
     nodes = s.artificial_viscosity()
     assert nodes.max() > 0.0 and np.array_equal(data[u.size:], nodes)
+    # the same state as a VTU file: node values at the element corners, density positive
+    import xml.etree.ElementTree as ET
+
+    vtu = str(tmp_path / "av.vtu")
+    s.write_vtu(vtu, step=2)
+    arrays = {d.get("Name"): d for d in ET.parse(vtu).getroot().iter("DataArray")}
+    av = np.array(arrays["ArtificialViscosity"].text.split(), dtype=np.float64)
+    assert np.isclose(av.max(), nodes.max(), rtol=1e-14) and int(arrays["TMSTEP"].text) == 2
 
 
 def _sod_exact(x, t, x0, gamma=1.4):
