@@ -121,7 +121,7 @@ typedef struct sdg_problem_desc {
 
 /* Structured mesh on an axis-aligned box: the project-side replacement for the gmsh
  * transfinite (+ setPeriodic) generators of the examples (e.g. periodic_2d_ceuler.cpp:63-94).
- * element: SDG_ELEM_QUADRANGLE (2-D) or SDG_ELEM_HEXAHEDRON (3-D). A direction is periodic
+ * element: SDG_ELEM_LINE (1-D), SDG_ELEM_QUADRANGLE (2-D) or SDG_ELEM_HEXAHEDRON (3-D). A direction is periodic
  * (needs at least 2 cells) unless bounded[d] != 0; the sides of a bounded direction are boundary
  * faces of type boundary[2 d] (low side) and boundary[2 d + 1] (high side), enum sdg_boundary
  * without SDG_BC_PERIODIC. A zero-initialised descriptor tail means fully periodic. */

@@ -292,7 +292,7 @@ struct System {
       return;
     }
     sdg_structured_desc d{};
-    d.element = SC::kDimension == 2 ? SDG_ELEM_QUADRANGLE : SDG_ELEM_HEXAHEDRON;
+    d.element = SC::kDimension == 1 ? SDG_ELEM_LINE : (SC::kDimension == 2 ? SDG_ELEM_QUADRANGLE : SDG_ELEM_HEXAHEDRON);
     d.polynomial_order = SC::kPolynomialOrder;
     for (int i = 0; i < 3; i++) {
       d.n[i] = mesh_desc_.cells[i];

@@ -533,6 +533,7 @@ struct Oracle {
     if (order == 1) return 0;
     const int p = order - 1;
     if (dim == 2 && nvert == 3) return (p + 1) * (p + 2) / 2;
+    if (dim == 1) return p + 1;
     return dim == 2 ? (p + 1) * (p + 1) : (p + 1) * (p + 1) * (p + 1);
   }
   void artificialViscosity() {

@@ -287,7 +287,20 @@ bool buildRefElement(int type, int order, RefElement& ref, std::string& error) {
   const int n1 = order + 1;  // points per direction: Gauss order 2P -> P+1, order 2P+1 -> P+1
   std::vector<double> gx, gw;
   gaussLegendre(n1, gx, gw);
-  if (type == 3) {  // quadrangle
+  if (type == 1) {  // line: faces are its two end points (normal -1 / +1, Geometry.cpp:102-112)
+    ref.dim = 1;
+    ref.nf = 2;
+    ref.nvert = 2;
+    ref.nfvert = 1;
+    for (int i = 0; i <= order; i++) ref.modes.push_back({i, 0, 0});
+    ref.vertex = {-1.0, 1.0};
+    ref.face_vertex = {0, 1};
+    for (int a = 0; a < n1; a++) {
+      ref.qpoint.push_back(gx[a]);
+      ref.qweight.push_back(gw[a]);
+    }
+    ref.fweight.push_back(1.0);  // one point per face; fpoint stays empty (0-D face coordinates)
+  } else if (type == 3) {  // quadrangle
     ref.dim = 2;
     ref.nf = 4;
     ref.nvert = 4;
@@ -358,7 +371,7 @@ bool buildRefElement(int type, int order, RefElement& ref, std::string& error) {
     }
     ref.modes.assign(static_cast<size_t>((order + 1) * (order + 2) / 2), {0, 0, 0});
   } else {
-    error = "unsupported element type (triangle=2, quadrangle=3 and hexahedron=6 are implemented)";
+    error = "unsupported element type (line=1, triangle=2, quadrangle=3 and hexahedron=6 are implemented)";
     return false;
   }
   // hierarchical ordering: sort modes by (max index, then slowest-to-fastest index) so that the
@@ -394,8 +407,10 @@ bool buildRefElement(int type, int order, RefElement& ref, std::string& error) {
     for (int j = 0; j < nfq; j++) {
       double xi[3] = {0, 0, 0};
       for (int v = 0; v < ref.nfvert; v++) {
-        double n, g[2];
-        if (dim == 2) {
+        double n = 1.0, g[2];
+        if (dim == 1) {
+          n = 1.0;  // the face is the vertex itself
+        } else if (dim == 2) {
           const double fv[1] = {kLineFaceVertex[v]};
           p1Shape(1, fv, &ref.fpoint[j], n, g);
         } else {
@@ -500,6 +515,11 @@ void faceNormalsRef(const RefElement& ref, const double* J, int s, Face& face) {
   const int dim = ref.dim, nfq = ref.nfq;
   face.normal.assign(static_cast<size_t>(nfq) * dim, 0.0);
   face.jw.assign(nfq, 0.0);
+  if (dim == 1) {  // end points: outward -1 at the first vertex, +1 at the second, weight 1
+    face.normal[0] = s == 0 ? -1.0 : 1.0;
+    face.jw[0] = ref.fweight[0];
+    return;
+  }
   for (int p = 0; p < nfq; p++) {
     double ts[3] = {0, 0, 0}, tt[3] = {0, 0, 0};  // d xi / d s, d xi / d t
     for (int v = 0; v < ref.nfvert; v++) {
@@ -549,7 +569,7 @@ bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3]
   if (!buildRefElement(type, order, mesh.ref, error)) return false;
   const RefElement& ref = mesh.ref;
   const int dim = ref.dim;
-  int nc[3] = {n[0], n[1], dim == 3 ? n[2] : 1};
+  int nc[3] = {n[0], dim >= 2 ? n[1] : 1, dim == 3 ? n[2] : 1};
   for (int d = 0; d < dim; d++) {
     if (nc[d] < (periodic[d] ? 2 : 1)) {
       error = "a periodic direction needs at least 2 cells, a bounded one at least 1";
@@ -592,7 +612,7 @@ bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3]
         const int64_t e = cellIndex(i, j, k);
         for (int v = 0; v < ref.nvert; v++) {
           const double* rv = &ref.vertex[static_cast<size_t>(v) * dim];
-          const int off[3] = {rv[0] > 0 ? 1 : 0, rv[1] > 0 ? 1 : 0, dim == 3 && rv[2] > 0 ? 1 : 0};
+          const int off[3] = {rv[0] > 0 ? 1 : 0, dim >= 2 && rv[1] > 0 ? 1 : 0, dim == 3 && rv[2] > 0 ? 1 : 0};
           const int idx[3] = {i + off[0], j + off[1], k + off[2]};
           for (int d = 0; d < dim; d++) mesh.vertex[(static_cast<size_t>(e) * ref.nvert + v) * dim + d] = lo[d] + idx[d] * h[d];
           vid[static_cast<size_t>(e) * ref.nvert + v] = vertexId(idx[0], idx[1], idx[2]);
@@ -615,10 +635,11 @@ bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3]
       return false;
     }
     double me = std::numeric_limits<double>::max();
-    const int ne = dim == 2 ? 4 : 12;
+    static const int kLineEdge[1][2] = {{0, 1}};
+    const int ne = dim == 1 ? 1 : (dim == 2 ? 4 : 12);
     for (int g = 0; g < ne; g++) {
-      const int a = dim == 2 ? kQuadEdge[g][0] : kHexEdge[g][0];
-      const int b = dim == 2 ? kQuadEdge[g][1] : kHexEdge[g][1];
+      const int a = dim == 1 ? kLineEdge[g][0] : (dim == 2 ? kQuadEdge[g][0] : kHexEdge[g][0]);
+      const int b = dim == 1 ? kLineEdge[g][1] : (dim == 2 ? kQuadEdge[g][1] : kHexEdge[g][1]);
       double s = 0.0;
       for (int d = 0; d < dim; d++) {
         const double t = mesh.vertex[(static_cast<size_t>(e) * ref.nvert + a) * dim + d] -
@@ -644,7 +665,8 @@ bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3]
         const int64_t e = cellIndex(i, j, k);
         for (int s = 0; s < nf; s++) {
           if (mesh.elem_face[static_cast<size_t>(e) * nf + s] != -1) continue;
-          const int* nbr = dim == 2 ? kQuadNb[s] : kHexNb[s];
+          static const int kLineNb[2][4] = {{-1, 0, 0, 1}, {1, 0, 0, 0}};
+          const int* nbr = dim == 1 ? kLineNb[s] : (dim == 2 ? kQuadNb[s] : kHexNb[s]);
           const int ijk[3] = {i + nbr[0], j + nbr[1], k + nbr[2]};
           int side_out = -1;  // boundary side 2 d + (0 low, 1 high) when the neighbour is outside
           for (int d = 0; d < dim; d++)
@@ -1164,14 +1186,16 @@ void assignNodes(Mesh& mesh) {
     }
   mesh.nodes = static_cast<int64_t>(index.size());
   // inner radius D |element| / |boundary|; reference measures: triangle 1/2, quadrangle 4, hexahedron 8
-  const double measure = ref.type == 2 ? 0.5 : (dim == 2 ? 4.0 : 8.0);
+  const double measure = ref.type == 2 ? 0.5 : (dim == 1 ? 2.0 : (dim == 2 ? 4.0 : 8.0));
   mesh.inner_radius.assign(E, 0.0);
   for (int64_t e = 0; e < E; e++) {
     const double* x = &mesh.vertex[static_cast<size_t>(e) * nv * dim];
     double surface = 0.0;
     for (int f = 0; f < ref.nf; f++) {
       const int* fv = &ref.face_vertex[static_cast<size_t>(f) * ref.nfvert];
-      if (dim == 2) {
+      if (dim == 1) {
+        surface += 1.0;  // an end point
+      } else if (dim == 2) {
         double s2 = 0.0;
         for (int d = 0; d < 2; d++) {
           const double t = x[fv[1] * 2 + d] - x[fv[0] * 2 + d];

@@ -66,8 +66,8 @@ int sdg_mesh_create_structured(const sdg_structured_desc* desc, sdg_mesh** out) 
     return SDG_E_INVALID;
   }
   *out = nullptr;
-  if (desc->element != SDG_ELEM_QUADRANGLE && desc->element != SDG_ELEM_HEXAHEDRON) {
-    sdg::setGlobalError("sdg_mesh_create_structured: element must be quadrangle or hexahedron");
+  if (desc->element != SDG_ELEM_LINE && desc->element != SDG_ELEM_QUADRANGLE && desc->element != SDG_ELEM_HEXAHEDRON) {
+    sdg::setGlobalError("sdg_mesh_create_structured: element must be line, quadrangle or hexahedron");
     return SDG_E_UNSUPPORTED;
   }
   sdg_mesh* m = new (std::nothrow) sdg_mesh();

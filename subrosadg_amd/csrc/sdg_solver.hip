@@ -258,6 +258,16 @@ bool selectLaunchers(int type, int order, bool force_valu, bool mixed, Launchers
       default: return pickTensor<5>(mixed, l);
     }
   }
+  if (type == SDG_ELEM_LINE) {  // 1-D: vector kernels (P+1 modes, P+1 Gauss points, two end-point faces)
+    switch (order) {
+      case 1: return pickValu<1, 2, 2, 2, 1>(mixed, l);
+      case 2: return pickValu<1, 3, 3, 2, 1>(mixed, l);
+      case 3: return pickValu<1, 4, 4, 2, 1>(mixed, l);
+      case 4: return pickValu<1, 5, 5, 2, 1>(mixed, l);
+      case 5: return pickValu<1, 6, 6, 2, 1>(mixed, l);
+      default: return false;
+    }
+  }
   if (type == SDG_ELEM_TRIANGLE) {  // bases of 3..15 functions: vector kernels
     switch (order) {
       case 1: return pickValu<2, 3, 3, 3, 2>(mixed, l);
@@ -831,7 +841,9 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     static const double kThreshold[5] = {0.0, -1.20411998266, -1.90848501888, -2.40823996531, -2.79588001734};
     ph.av_s0 = kThreshold[r.order - 1];
     const int p = r.order - 1;  // the hierarchic basis of order P-1 is a prefix of order P's
-    ph.av_low = p == 0 ? 0 : (r.type == SDG_ELEM_TRIANGLE ? (p + 1) * (p + 2) / 2 : (r.dim == 2 ? (p + 1) * (p + 1) : (p + 1) * (p + 1) * (p + 1)));
+    ph.av_low = p == 0 ? 0
+                : (r.type == SDG_ELEM_TRIANGLE ? (p + 1) * (p + 2) / 2
+                                               : (r.dim == 1 ? p + 1 : (r.dim == 2 ? (p + 1) * (p + 1) : (p + 1) * (p + 1) * (p + 1))));
   }
   const char* ng = std::getenv("SDG_NO_GRAPH");
   ctx->use_graph = !(ng && ng[0] == '1');
