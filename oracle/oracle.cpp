@@ -19,9 +19,16 @@
 //                                     lifted gradient of continuous polynomials, convergence
 //                                     order on a smooth Euler solution, viscous decay rate,
 //                                     SSP-RK3 on a linear ODE (tests/test_oracle_scheme.py)
-//   parity unpinned                -> the Exact (weakly-compressible) flux: only consistency
-//                                     and antisymmetry are checked; no independent solution
-//                                     of its Riemann problem is committed.
+//   closed forms                   -> BR1/BR2 face lifts of a single jump (Legendre kernel at
+//                                     the end points), CFL dt of a uniform state, the monitor
+//                                     of element-constant and linear residuals, the Exact flux's
+//                                     star state on the two isothermal rarefaction invariants,
+//                                     HLLC at symmetric pairs (zero mass and energy flux, normal
+//                                     momentum flux = p*, vacuum), Sutherland's law through the
+//                                     viscous flux (tests/test_oracle_pins.py)
+//   parity unpinned                -> the value of HLLC's star-pressure estimate away from
+//                                     symmetric pairs is the reference's formula
+//                                     (ConvectiveFlux.cpp:162-165), which no closed form fixes.
 //
 // The arithmetic type is ORACLE_REAL (default double). The same source is also built with
 // ORACLE_REAL=long double (x87 extended, 64-bit mantissa) into liboracle_ld.so: that build takes
@@ -918,6 +925,146 @@ struct Oracle {
   }
 };
 
+// Euler solver on a 2-D mesh of triangles and quadrangles: the reference holds one ElementSolver
+// per element type (Solver::triangle_, quadrangle_) and runs each pass over both, the faces
+// coupling elements of either type (TimeIntegration.cpp:279-350, SpatialDiscrete.cpp:194-266,
+// 633-748, 1016-1032). Same passes and arithmetic as Oracle's Euler path; per-type operand
+// tables, per-element blocks of each element's own size. Periodic meshes (no boundary faces).
+struct MixedOracle {
+  Oracle pt;  // pointwise physics, flux and RK tables (dim 2, NV 4)
+  struct Part {
+    int nb = 0, nq = 0, nf = 0, nfq = 0, naq = 0;
+    std::vector<Real> V, dV, Vf, lsq;  // column-major like Oracle's
+  } part[2];                           // [0] triangle, [1] quadrangle
+  int64_t ne = 0, nface = 0;
+  std::vector<int8_t> etype;
+  std::vector<int64_t> coff, qoff, aoff, moff;  // per element: coefficient, point, slot, mass offsets
+  std::vector<Real> detjw, tiw, minv, min_edge;
+  std::vector<int32_t> fparent, flocal, fseq;
+  std::vector<Real> fnormal, fjw;
+  std::vector<Real> coeff, coeff_last, vol_quad, adj_quad, residual, rel_err;
+  static constexpr int kNv = 4, kDim = 2;
+
+  const Part& of(int64_t e) const { return part[etype[e]]; }
+  void faceTrace(int64_t e, int s, Real* u) const {  // NV x N_fq = C Vf[face rows]^T
+    const Part& p = of(e);
+    for (int j = 0; j < p.nfq; j++)
+      for (int v = 0; v < kNv; v++) {
+        Real a = 0.0;
+        for (int i = 0; i < p.nb; i++) a += coeff[coff[e] + v + kNv * i] * p.Vf[(s * p.nfq + j) + static_cast<size_t>(p.naq) * i];
+        u[v + kNv * j] = a;
+      }
+  }
+  void volumeQuadrature() {
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; e++) {
+      const Part& p = of(e);
+      std::vector<Real> uq(static_cast<size_t>(kNv) * p.nq);
+      gemmBt(uq.data(), &coeff[coff[e]], p.V.data(), kNv, p.nb, p.nq, false);
+      Real* out = &vol_quad[qoff[e] * kNv * kDim];
+      for (int q = 0; q < p.nq; q++) {
+        Real w[8], fc[15];
+        pt.computational(&uq[static_cast<size_t>(kNv) * q], w);
+        const Real rho = w[0], pr = w[kDim + 2];
+        const Real et = w[kDim + 1] + pt.velocitySquared(w) / 2.0;
+        for (int a = 0; a < kDim; a++) {
+          fc[a] = rho * w[1 + a];
+          for (int b = 0; b < kDim; b++) fc[a + kDim * (1 + b)] = rho * w[1 + a] * w[1 + b] + (a == b ? pr * 1.0 : pr * 0.0);
+          fc[a + kDim * (kDim + 1)] = (rho * et + pr) * w[1 + a];
+        }
+        const Real* G = &tiw[(qoff[e] + q) * kDim * kDim];
+        for (int v = 0; v < kNv; v++)
+          for (int b = 0; b < kDim; b++) {
+            Real acc = 0.0;
+            for (int a = 0; a < kDim; a++) acc += fc[a + kDim * v] * G[a + kDim * b];
+            out[v + static_cast<size_t>(kNv) * (q * kDim + b)] = acc;
+          }
+      }
+    }
+  }
+  void adjacencyQuadrature() {
+#pragma omp parallel for schedule(static)
+    for (int64_t f = 0; f < nface; f++) {
+      const int32_t L = fparent[2 * f], R = fparent[2 * f + 1];
+      const int sl = flocal[2 * f], sr = flocal[2 * f + 1];
+      const int nfq = of(L).nfq;
+      std::vector<Real> ul(static_cast<size_t>(kNv) * nfq), ur(static_cast<size_t>(kNv) * nfq);
+      faceTrace(L, sl, ul.data());
+      faceTrace(R, sr, ur.data());
+      for (int j = 0; j < nfq; j++) {
+        const int k = fseq[f * nfq + j];
+        Real wl[8], wr[8], h[5];
+        pt.computational(&ul[kNv * j], wl);
+        pt.computational(&ur[kNv * k], wr);
+        pt.convectiveFlux(&fnormal[(static_cast<size_t>(f) * nfq + j) * kDim], &ul[kNv * j], wl, &ur[kNv * k], wr, h);
+        Real* cl = &adj_quad[(aoff[L] + sl * nfq + j) * kNv];
+        Real* cr = &adj_quad[(aoff[R] + sr * nfq + k) * kNv];
+        for (int v = 0; v < kNv; v++) {
+          const Real t = h[v] * fjw[f * nfq + j];
+          cl[v] = t;
+          cr[v] = -t;
+        }
+      }
+    }
+  }
+  void calcResidual() {
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; e++) {
+      const Part& p = of(e);
+      Real* res = &residual[coff[e]];
+      gemm(res, &vol_quad[qoff[e] * kNv * kDim], p.dV.data(), kNv, kDim * p.nq, p.nb, false);
+      gemm(res, &adj_quad[aoff[e] * kNv], p.Vf.data(), kNv, p.naq, p.nb, true, -1.0);
+    }
+  }
+  void updateCoefficient(int stage, Real dt) {
+    Real c[3];
+    pt.rkCoefficients(stage, c);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; e++) {
+      const int nb = of(e).nb;
+      const size_t sz = static_cast<size_t>(kNv) * nb;
+      std::vector<Real> rm(sz);
+      gemm(rm.data(), &residual[coff[e]], &minv[moff[e]], kNv, nb, nb, false);
+      Real* u = &coeff[coff[e]];
+      const Real* un = &coeff_last[coff[e]];
+      for (size_t i = 0; i < sz; i++) u[i] *= c[1];
+      for (size_t i = 0; i < sz; i++) u[i] += c[0] * un[i];
+      for (size_t i = 0; i < sz; i++) u[i] += c[2] * dt * rm[i];
+    }
+  }
+  void relativeError() {  // sum over both element types, divided by the element count
+    std::vector<Real> per(static_cast<size_t>(ne) * kNv);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; e++) {
+      const Part& p = of(e);
+      std::vector<Real> rq(static_cast<size_t>(kNv) * p.nq);
+      gemmBt(rq.data(), &residual[coff[e]], p.V.data(), kNv, p.nb, p.nq, false);
+      for (int v = 0; v < kNv; v++) {
+        Real acc = 0.0;
+        for (int q = 0; q < p.nq; q++) acc += std::fabs(rq[v + static_cast<size_t>(kNv) * q]);
+        per[e * kNv + v] = acc / p.nq;
+      }
+    }
+    std::vector<Real> acc(kNv, 0.0);
+    for (int64_t e = 0; e < ne; e++)
+      for (int v = 0; v < kNv; v++) acc[v] += per[e * kNv + v];
+    for (int v = 0; v < kNv; v++) rel_err[v] = acc[v] / static_cast<Real>(ne);
+  }
+  void spatial() {
+    volumeQuadrature();
+    adjacencyQuadrature();
+    calcResidual();
+  }
+  void step(Real dt) {
+    coeff_last = coeff;
+    for (int s = 0; s < pt.stages(); s++) {
+      spatial();
+      updateCoefficient(s, dt);
+    }
+    relativeError();
+  }
+};
+
 template <typename T>
 void colMajorFromRowMajor(std::vector<T>& dst, const T* src, int rows, int cols) {
   dst.resize(static_cast<size_t>(rows) * cols);
@@ -1129,6 +1276,27 @@ int oracle_get_gradient(void* h, Real* g) {
   std::memcpy(g, o->g_coeff.data(), o->g_coeff.size() * sizeof(Real));
   return 0;
 }
+// Gradient traces each element uses at its own face points in the viscous face flux of the last
+// residual evaluation (VariableConvertor.cpp:640-689: BR2 the volume gradient plus that face's
+// lift, BR1 the lifted gradient), E x N_f x N_fq x (NV D).
+int oracle_get_face_gradient(void* h, Real* out) {
+  Oracle* o = static_cast<Oracle*>(h);
+  if (!o->ns()) return -2;
+  const size_t rows = static_cast<size_t>(o->nv) * o->dim, blk = rows * o->nfq;
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < o->ne; e++)
+    for (int s = 0; s < o->nf; s++) o->faceGradientTrace(e, s, out + (static_cast<size_t>(e) * o->nf + s) * blk);
+  return 0;
+}
+// The residual monitor (TimeIntegration.cpp:279-324) of a given residual, NV x N_b per element
+// column-major; the oracle's own residual is replaced by it.
+int oracle_monitor(void* h, const Real* r, Real* rel_err) {
+  Oracle* o = static_cast<Oracle*>(h);
+  std::memcpy(o->residual.data(), r, o->residual.size() * sizeof(Real));
+  o->relativeError();
+  std::memcpy(rel_err, o->rel_err.data(), o->nv * sizeof(Real));
+  return 0;
+}
 int oracle_compute_dt(void* h, Real cfl, Real* dt) {  // TimeIntegration.cpp:104-131
   Oracle* o = static_cast<Oracle*>(h);
   const int nv = o->nv, nq = o->nq, nb = o->nb, dim = o->dim;
@@ -1167,6 +1335,149 @@ int oracle_step(void* h, Real dt, int nsteps, Real* rel_err, int* nan_flag, int*
   if (steps_taken) *steps_taken = taken;
   return 0;
 }
+// Mixed triangle / quadrangle Euler meshes. Per type t (0 triangle, 1 quadrangle): sizes
+// nb[t], nq[t], nf[t], nfq (line faces, shared) and row-major tables V[t], dV[t], Vf[t], lsq[t]
+// as sdg_mesh_export writes them for a mesh of that type; per element its type, and detjw, tiw,
+// minv blocks of its own size concatenated in element order (sdg_mesh_export of a split mesh).
+void* oracle_mixed_create(const sdg_problem_desc* p, int order, int64_t ne, int64_t nface, const int8_t* etype,
+                          const int* nb, const int* nq, const int* nf, int nfq, const Real* const* V,
+                          const Real* const* dV, const Real* const* Vf, const Real* const* lsq, const Real* detjw,
+                          const Real* tiw, const Real* minv, const Real* min_edge, const int32_t* fparent,
+                          const int32_t* flocal, const int32_t* fseq, const Real* fnormal, const Real* fjw) {
+  if (p->equation != SDG_EQ_COMPRESSIBLE_EULER) return nullptr;
+  MixedOracle* o = new (std::nothrow) MixedOracle();
+  if (!o) return nullptr;
+  o->pt.dim = 2;
+  o->pt.nv = 4;
+  o->pt.order = order;
+  o->pt.flux = p->convective_flux;
+  o->pt.rk = p->time_integration;
+  o->pt.phys.eos = p->equation_of_state;
+  o->pt.phys.cp = p->cp;
+  o->pt.phys.cv = p->cv;
+  o->pt.phys.c0 = p->reference_sound_speed;
+  o->pt.phys.rho0 = p->reference_density;
+  o->pt.phys.padd = 0.01 * p->reference_density * p->reference_sound_speed * p->reference_sound_speed;
+  for (int t = 0; t < 2; t++) {
+    MixedOracle::Part& q = o->part[t];
+    q.nb = nb[t];
+    q.nq = nq[t];
+    q.nf = nf[t];
+    q.nfq = nfq;
+    q.naq = nf[t] * nfq;
+    colMajorFromRowMajor(q.V, V[t], nq[t], nb[t]);
+    colMajorFromRowMajor(q.dV, dV[t], nq[t] * 2, nb[t]);
+    colMajorFromRowMajor(q.Vf, Vf[t], nf[t] * nfq, nb[t]);
+    colMajorFromRowMajor(q.lsq, lsq[t], nb[t], nb[t]);
+  }
+  o->ne = ne;
+  o->nface = nface;
+  o->etype.assign(etype, etype + ne);
+  o->coff.resize(ne);
+  o->qoff.resize(ne);
+  o->aoff.resize(ne);
+  o->moff.resize(ne);
+  int64_t c = 0, q = 0, a = 0, m = 0;
+  for (int64_t e = 0; e < ne; e++) {
+    const MixedOracle::Part& pe = o->of(e);
+    o->coff[e] = c;
+    o->qoff[e] = q;
+    o->aoff[e] = a;
+    o->moff[e] = m;
+    c += 4 * pe.nb;
+    q += pe.nq;
+    a += pe.naq;
+    m += static_cast<int64_t>(pe.nb) * pe.nb;
+  }
+  o->detjw.assign(detjw, detjw + q);
+  o->tiw.assign(tiw, tiw + q * 4);
+  o->minv.resize(m);
+  for (int64_t e = 0; e < ne; e++) {  // row-major blocks -> column-major
+    const int nbe = o->of(e).nb;
+    for (int r = 0; r < nbe; r++)
+      for (int k = 0; k < nbe; k++)
+        o->minv[o->moff[e] + r + static_cast<size_t>(nbe) * k] = minv[o->moff[e] + static_cast<size_t>(r) * nbe + k];
+  }
+  o->min_edge.assign(min_edge, min_edge + ne);
+  o->fparent.assign(fparent, fparent + 2 * nface);
+  o->flocal.assign(flocal, flocal + 2 * nface);
+  o->fseq.assign(fseq, fseq + nface * nfq);
+  o->fnormal.assign(fnormal, fnormal + nface * nfq * 2);
+  o->fjw.assign(fjw, fjw + nface * nfq);
+  o->coeff.assign(c, 0.0);
+  o->coeff_last.assign(c, 0.0);
+  o->residual.assign(c, 0.0);
+  o->vol_quad.assign(q * 4 * 2, 0.0);
+  o->adj_quad.assign(a * 4, 0.0);
+  o->rel_err.assign(4, 0.0);
+  return o;
+}
+void oracle_mixed_destroy(void* h) { delete static_cast<MixedOracle*>(h); }
+int oracle_mixed_set_state(void* h, const Real* c) {
+  MixedOracle* o = static_cast<MixedOracle*>(h);
+  std::memcpy(o->coeff.data(), c, o->coeff.size() * sizeof(Real));
+  o->coeff_last = o->coeff;
+  return 0;
+}
+int oracle_mixed_get_state(void* h, Real* c) {
+  MixedOracle* o = static_cast<MixedOracle*>(h);
+  std::memcpy(c, o->coeff.data(), o->coeff.size() * sizeof(Real));
+  return 0;
+}
+// InitialCondition.cpp:85-104 per element type; prim: each element's points in element order
+int oracle_mixed_set_initial_primitive(void* h, const Real* prim) {
+  MixedOracle* o = static_cast<MixedOracle*>(h);
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < o->ne; e++) {
+    const MixedOracle::Part& p = o->of(e);
+    std::vector<Real> uq(static_cast<size_t>(4) * p.nq), tmp(static_cast<size_t>(4) * p.nb);
+    for (int q = 0; q < p.nq; q++) {
+      const Real* pr = &prim[(o->qoff[e] + q) * 4];
+      const Real rho = pr[0];
+      uq[4 * q] = rho;
+      uq[4 * q + 1] = rho * pr[1];
+      uq[4 * q + 2] = rho * pr[2];
+      const Real et = o->pt.phys.internalEnergyFromTemperature(pr[3]) + (pr[1] * pr[1] + pr[2] * pr[2]) / 2.0;
+      uq[4 * q + 3] = rho * et;
+    }
+    gemm(tmp.data(), uq.data(), p.V.data(), 4, p.nq, p.nb, false);
+    gemm(&o->coeff[o->coff[e]], tmp.data(), p.lsq.data(), 4, p.nb, p.nb, false);
+  }
+  o->coeff_last = o->coeff;
+  return 0;
+}
+int oracle_mixed_residual(void* h, Real* r) {
+  MixedOracle* o = static_cast<MixedOracle*>(h);
+  o->spatial();
+  if (r) std::memcpy(r, o->residual.data(), o->residual.size() * sizeof(Real));
+  return 0;
+}
+int oracle_mixed_compute_dt(void* h, Real cfl, Real* dt) {  // TimeIntegration.cpp:104-131, both types
+  MixedOracle* o = static_cast<MixedOracle*>(h);
+  Real best = std::numeric_limits<Real>::max();
+  const int order = o->pt.order;
+#pragma omp parallel for schedule(static) reduction(min : best)
+  for (int64_t e = 0; e < o->ne; e++) {
+    const MixedOracle::Part& p = o->of(e);
+    std::vector<Real> uq(static_cast<size_t>(4) * p.nq);
+    gemmBt(uq.data(), &o->coeff[o->coff[e]], p.V.data(), 4, p.nb, p.nq, false);
+    for (int q = 0; q < p.nq; q++) {
+      Real w[8];
+      o->pt.computational(&uq[4 * q], w);
+      const Real sr = std::sqrt(o->pt.velocitySquared(w)) + o->pt.phys.soundSpeed(w[0], w[4]);
+      best = std::min(best, cfl * o->min_edge[e] / (sr * (order + 1.0) * (order + 1.0)));
+    }
+  }
+  *dt = best;
+  return 0;
+}
+int oracle_mixed_step(void* h, Real dt, int nsteps, Real* rel_err) {
+  MixedOracle* o = static_cast<MixedOracle*>(h);
+  for (int i = 0; i < nsteps; i++) o->step(dt);
+  if (rel_err) std::memcpy(rel_err, o->rel_err.data(), 4 * sizeof(Real));
+  return 0;
+}
+
 // Pointwise physics for unit tests: conserved states (NV each), normal (D) -> flux (NV).
 int oracle_point_flux(const sdg_problem_desc* p, int dim, int flux, const Real* n, const Real* ul, const Real* ur,
                       Real* h) {

@@ -55,6 +55,10 @@ def load(extended: bool = False):
                      "oracle_get_gradient", "oracle_get_volume_gradient", "oracle_get_artificial_viscosity"):
             getattr(lib, name).argtypes = [c_void_p, c_void_p]
             getattr(lib, name).restype = c_int
+        lib.oracle_get_face_gradient.argtypes = [c_void_p, c_void_p]
+        lib.oracle_get_face_gradient.restype = c_int
+        lib.oracle_monitor.argtypes = [c_void_p, c_void_p, c_void_p]
+        lib.oracle_monitor.restype = c_int
         lib.oracle_compute_dt.argtypes = [c_void_p, real, POINTER(real)]
         lib.oracle_step.argtypes = [c_void_p, real, c_int, c_void_p, POINTER(c_int), POINTER(c_int)]
         lib.oracle_point_flux.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
@@ -169,6 +173,22 @@ class Oracle:
             raise RuntimeError("gradient is only defined for Navier-Stokes")
         return out.astype(np.float64)
 
+    def face_gradient(self) -> np.ndarray:
+        """Gradient traces each element uses at its own face points in the viscous face flux of the
+        last residual (BR2: volume gradient plus that face's lift), (E, N_f, N_fq, NV D)."""
+        out = self._out((self.E, self.mesh.nf, self.mesh.nfq, self.nv * self.dim))
+        if self.lib.oracle_get_face_gradient(self.h, _ptr(out)) != 0:
+            raise RuntimeError("face gradients are only defined for Navier-Stokes")
+        return out.astype(np.float64)
+
+    def monitor(self, residual: np.ndarray) -> np.ndarray:
+        """Residual monitor of a given residual (E, N_b, NV); replaces the oracle's residual."""
+        r = np.ascontiguousarray(residual, dtype=self.real)
+        assert r.shape == self.state_shape()
+        out = self._out(self.nv)
+        self.lib.oracle_monitor(self.h, _ptr(r), _ptr(out))
+        return out.astype(np.float64)
+
     def volume_gradient(self) -> np.ndarray:
         """Volume-gradient coefficients of the last residual or stage, (E, N_b, NV D)."""
         out = self._out((self.E, self.nb, self.nv * self.dim))
@@ -232,3 +252,92 @@ def point_boundary_state(problem, dim: int, bc: int, n, ul, prim) -> np.ndarray:
     wb = np.empty(dim + 3)
     lib.oracle_point_boundary_state(ctypes.byref(problem), dim, bc, _ptr(n), _ptr(ul), _ptr(prim), _ptr(wb))
     return wb
+
+
+class MixedOracle:
+    """Euler oracle on a split grid of triangles and quadrangles (the reference's triangle_ and
+    quadrangle_ element solvers in one Solver). State layout: per element its N_b x NV block, in
+    element order (each element's own N_b), as the batched solver returns a problem's state."""
+
+    def __init__(self, problem, mesh):
+        from subrosadg_amd import _lib as L
+
+        self.lib = load()
+        lib = self.lib
+        if not hasattr(lib, "_mixed_ready"):
+            lib.oracle_mixed_create.restype = c_void_p
+            lib.oracle_mixed_create.argtypes = [c_void_p, c_int, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
+                                                c_void_p, c_int] + [c_void_p] * 13
+            lib.oracle_mixed_destroy.argtypes = [c_void_p]
+            for name in ("oracle_mixed_set_state", "oracle_mixed_get_state", "oracle_mixed_set_initial_primitive",
+                         "oracle_mixed_residual"):
+                getattr(lib, name).argtypes = [c_void_p, c_void_p]
+            lib.oracle_mixed_compute_dt.argtypes = [c_void_p, c_double, POINTER(c_double)]
+            lib.oracle_mixed_step.argtypes = [c_void_p, c_double, c_int, c_void_p]
+            lib._mixed_ready = True
+        self.mesh = mesh
+        order = mesh.order
+        refs = [L.Mesh.split(order, (2, 2), (0.0, 0.0), (1.0, 1.0), frac, 0) for frac in (1.0, 0.0)]
+        et = mesh.field("element_type")
+        self.etype = np.ascontiguousarray((et == L.ELEM_QUADRANGLE).astype(np.int8))
+        self.nb = np.array([r.nb for r in refs], dtype=np.int32)
+        nq = np.array([r.nq for r in refs], dtype=np.int32)
+        nf = np.array([r.nf for r in refs], dtype=np.int32)
+        nfq = refs[0].nfq
+        keep = {}
+        tabs = {}
+        for key in ("basis_value", "basis_gradient", "basis_face_value", "basis_lsq_inverse"):
+            arrs = [np.ascontiguousarray(r.field(key)) for r in refs]
+            keep[key] = arrs
+            tabs[key] = (c_void_p * 2)(*[a.ctypes.data for a in arrs])
+        for key in ("jacobian_det_w", "jacobian_tiw", "mass_inverse", "min_edge", "face_parent", "face_local",
+                    "face_sequence", "face_normal", "face_jacobian_w"):
+            keep[key] = mesh.field(key)
+        self._keep = (keep, tabs, nq, nf, refs)
+        self.E = mesh.E
+        self.F = keep["face_parent"].size // 2
+        self.size = int(self.nb[self.etype].sum()) * 4
+        k = keep
+        self.h = lib.oracle_mixed_create(
+            ctypes.byref(problem), order, self.E, self.F, _ptr(self.etype), _ptr(self.nb), _ptr(nq), _ptr(nf), nfq,
+            ctypes.cast(tabs["basis_value"], c_void_p), ctypes.cast(tabs["basis_gradient"], c_void_p),
+            ctypes.cast(tabs["basis_face_value"], c_void_p), ctypes.cast(tabs["basis_lsq_inverse"], c_void_p),
+            _ptr(k["jacobian_det_w"]), _ptr(k["jacobian_tiw"]), _ptr(k["mass_inverse"]), _ptr(k["min_edge"]),
+            _ptr(k["face_parent"]), _ptr(k["face_local"]), _ptr(k["face_sequence"]), _ptr(k["face_normal"]),
+            _ptr(k["face_jacobian_w"]))
+        if not self.h:
+            raise RuntimeError("oracle_mixed_create failed (Euler only)")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.oracle_mixed_destroy(self.h)
+            self.h = None
+
+    def set_state(self, c):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        assert c.size == self.size
+        self.lib.oracle_mixed_set_state(self.h, _ptr(c))
+
+    def get_state(self):
+        out = np.empty(self.size)
+        self.lib.oracle_mixed_get_state(self.h, _ptr(out))
+        return out
+
+    def set_initial_primitive(self, prim):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        self.lib.oracle_mixed_set_initial_primitive(self.h, _ptr(prim))
+
+    def residual(self):
+        out = np.empty(self.size)
+        self.lib.oracle_mixed_residual(self.h, _ptr(out))
+        return out
+
+    def compute_dt(self, cfl):
+        dt = c_double()
+        self.lib.oracle_mixed_compute_dt(self.h, cfl, ctypes.byref(dt))
+        return dt.value
+
+    def step(self, dt, nsteps=1):
+        rel = np.empty(4)
+        self.lib.oracle_mixed_step(self.h, dt, nsteps, _ptr(rel))
+        return rel

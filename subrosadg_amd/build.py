@@ -6,6 +6,7 @@ from __future__ import annotations
 
 import os
 import shutil
+from concurrent.futures import ThreadPoolExecutor
 import subprocess
 import sys
 import time
@@ -48,16 +49,17 @@ def build(verbose: bool = True) -> str:
     """Compile every source and link libsdg_hip.so; always recompiles."""
     os.makedirs(BUILD_DIR, exist_ok=True)
     hipcc = _hipcc()
-    objs = []
     t0 = time.time()
-    for src in HOST_SOURCES:
-        obj = os.path.join(BUILD_DIR, src + ".o")
-        _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
-    for src in HIP_SOURCES:
-        obj = os.path.join(BUILD_DIR, src + ".o")
-        _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+    jobs = [["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", os.path.join(BUILD_DIR, src + ".o")]
+            for src in HOST_SOURCES]
+    jobs += [[hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", os.path.join(BUILD_DIR, src + ".o")]
+             for src in HIP_SOURCES]
+    objs = [cmd[-1] for cmd in jobs]
+    # the sources compile independently; the HIP ones dominate, so they start first
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        for fut in [pool.submit(_run, cmd, verbose) for cmd in reversed(jobs)]:
+            fut.result()
     tmp = OUT + ".tmp"
     _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs, *LINK_LIBS], verbose)
     os.replace(tmp, OUT)
@@ -65,9 +67,12 @@ def build(verbose: bool = True) -> str:
     root = os.path.dirname(HERE)
     exe_dir = os.path.join(root, "examples", "bin")
     os.makedirs(exe_dir, exist_ok=True)
-    for ex in EXAMPLES:
-        _run(["g++", "-std=c++17", "-O2", "-I", os.path.join(root, "include"), os.path.join(root, "examples", ex + ".cpp"),
-              "-L", HERE, "-lsdg_hip", "-Wl,-rpath,$ORIGIN/../../subrosadg_amd", "-o", os.path.join(exe_dir, ex)], verbose)
+    ex_jobs = [["g++", "-std=c++17", "-O2", "-I", os.path.join(root, "include"), os.path.join(root, "examples", ex + ".cpp"),
+                "-L", HERE, "-lsdg_hip", "-Wl,-rpath,$ORIGIN/../../subrosadg_amd", "-o", os.path.join(exe_dir, ex)]
+               for ex in EXAMPLES]
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        for fut in [pool.submit(_run, cmd, verbose) for cmd in ex_jobs]:
+            fut.result()
     if verbose:
         print(f"built {OUT} in {time.time() - t0:.1f}s", flush=True)
     return OUT

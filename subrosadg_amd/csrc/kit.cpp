@@ -1216,30 +1216,35 @@ void assignNodes(Mesh& mesh) {
   }
 }
 
-void elementGeometry(const Mesh& mesh, int64_t e, double* detjw, double* tiw, double* coord) {
-  const RefElement& ref = mesh.ref;
+void affineGeometry(const RefElement& ref, const double* J, double dj, double* detjw, double* tiw) {
   const int dim = ref.dim;
-  const double* J = &mesh.jac[static_cast<size_t>(e) * dim * dim];
   // Jt(k,l) = J(l,k); G = inverse(Jt) |J| w, column-major: tiw[a + D b] = inv(Jt)(a,b)
   double jt[9], jti[9];
   for (int k = 0; k < dim; k++)
     for (int l = 0; l < dim; l++) jt[k * dim + l] = J[l * dim + k];
   invertMatrix(dim, jt, jti);
-  const double dj = mesh.detj[e];
   for (int q = 0; q < ref.nq; q++) {
     const double jw = dj * ref.qweight[q];
     if (detjw) detjw[q] = jw;
     if (tiw)
       for (int a = 0; a < dim; a++)
         for (int b = 0; b < dim; b++) tiw[static_cast<size_t>(q) * dim * dim + a + dim * b] = jti[a * dim + b] * jw;
-    if (coord) mapPoint(mesh, e, &ref.qpoint[static_cast<size_t>(q) * dim], &coord[static_cast<size_t>(q) * dim], nullptr);
   }
 }
 
-void elementMassInverse(const Mesh& mesh, int64_t e, double* minv) {
+void elementGeometry(const Mesh& mesh, int64_t e, double* detjw, double* tiw, double* coord) {
   const RefElement& ref = mesh.ref;
+  const int dim = ref.dim;
+  affineGeometry(ref, &mesh.jac[static_cast<size_t>(e) * dim * dim], mesh.detj[e], detjw, tiw);
+  if (coord)
+    for (int q = 0; q < ref.nq; q++)
+      mapPoint(mesh, e, &ref.qpoint[static_cast<size_t>(q) * dim], &coord[static_cast<size_t>(q) * dim], nullptr);
+}
+
+void elementMassInverse(const Mesh& mesh, int64_t e, double* minv) { affineMassInverse(mesh.ref, mesh.detj[e], minv); }
+
+void affineMassInverse(const RefElement& ref, double dj, double* minv) {
   const int nb = ref.nb;
-  const double dj = mesh.detj[e];
   std::vector<double> m(static_cast<size_t>(nb) * nb, 0.0);
   for (int i = 0; i < nb; i++)
     for (int j = 0; j < nb; j++) {

@@ -69,6 +69,14 @@ struct RefElement {
   std::vector<double> nodal_face_value;    // (N_f*N_fq) x nvert
 };
 
+// N_b of the modal basis of an element type and order (the hierarchic basis of order P-1 is a
+// prefix of order P's, so this is also where the modes above order P-1 start). Order 0: 1.
+inline int basisSize(int type, int dim, int order) {
+  const int n = order + 1;
+  if (type == 2 /* SDG_ELEM_TRIANGLE */) return n * (n + 1) / 2;
+  return dim == 1 ? n : (dim == 2 ? n * n : n * n * n);
+}
+
 // Build the reference tables; returns false for an unsupported (type, order).
 bool buildRefElement(int type, int order, RefElement& ref, std::string& error);
 
@@ -173,6 +181,10 @@ void mapReferencePoint(const Mesh& mesh, int64_t e, const double* xi, double* x)
 void elementGeometry(const Mesh& mesh, int64_t e, double* detjw, double* tiw, double* coord);
 // Inverse of the element mass matrix sum_q V_qi V_qj |J| w_q (row-major N_b x N_b).
 void elementMassInverse(const Mesh& mesh, int64_t e, double* minv);
+// The same two for an affine element given by its reference tables, Jacobian (jac[l*D+k] =
+// d x_l / d xi_k) and determinant (elements of split triangle / quadrangle grids).
+void affineGeometry(const RefElement& ref, const double* jac, double detj, double* detjw, double* tiw);
+void affineMassInverse(const RefElement& ref, double detj, double* minv);
 
 }  // namespace sdg
 

@@ -200,6 +200,7 @@ bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_p
   int type = -1;
   std::vector<int64_t> conn, bconn;
   std::vector<int> btype, bphysical;
+  std::vector<std::pair<int, int>> bnodes;  // per boundary block: nodes per element, physical group
   auto index = [&](int64_t tag, int64_t& out) {
     const auto it = node_index.find(tag);
     if (it == node_index.end()) return false;
@@ -228,6 +229,7 @@ bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_p
       if (ph == physical.end()) continue;  // boundary elements outside any physical group
       const auto bc = bc_of_physical.find(ph->second);
       if (bc == bc_of_physical.end()) continue;
+      bnodes.push_back({nn, ph->second});
       for (size_t k = 0; k < b.nodes.size(); k += static_cast<size_t>(nn)) {
         for (int v = 0; v < nn; v++) {
           int64_t i = 0;
@@ -241,6 +243,17 @@ bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_p
         bphysical.push_back(ph->second);
       }
     }
+  }
+  // boundary elements must be faces of the volume elements: 2 nodes (triangles, quadrangles) or
+  // 4 nodes (hexahedra); any other count would be read as the wrong number of face vertices
+  if (type >= 0) {
+    const int face_nodes = type == 6 ? 4 : 2;
+    for (const auto& bn : bnodes)
+      if (bn.first != face_nodes) {
+        error = "physical group " + std::to_string(bn.second) + " holds boundary elements of " +
+                std::to_string(bn.first) + " nodes; faces of the volume elements have " + std::to_string(face_nodes);
+        return false;
+      }
   }
   // one node map per periodic link (a corner node is a slave in more than one link)
   std::vector<std::map<int64_t, int64_t>> links;

@@ -179,12 +179,24 @@ int64_t sdg_mesh_field_size(const sdg_mesh* mesh, int field) {
   if (field == SDG_FIELD_ELEMENT_TYPE) return mesh->has_mixed ? mesh->mixed.elements : mesh->mesh.elements;
   if (!mesh->single) {
     const sdg::MixedMesh& x = mesh->mixed;
-    int64_t nq = 0;
-    for (int64_t e = 0; e < x.elements; e++) nq += x.refOf(e).nq;
+    int64_t nq = 0, nb2 = 0;
+    for (int64_t e = 0; e < x.elements; e++) {
+      nq += x.refOf(e).nq;
+      nb2 += static_cast<int64_t>(x.refOf(e).nb) * x.refOf(e).nb;
+    }
+    const int64_t F = static_cast<int64_t>(x.faces.size()), nfq = x.ref[0].nfq;  // line faces of both types
     switch (field) {
       case SDG_FIELD_QUAD_COORD: return nq * 2;
       case SDG_FIELD_JACOBIAN_DET_W: return nq;
+      case SDG_FIELD_JACOBIAN_TIW: return nq * 4;
+      case SDG_FIELD_MASS_INVERSE: return nb2;
       case SDG_FIELD_MIN_EDGE: return x.elements;
+      case SDG_FIELD_FACE_NORMAL: return F * nfq * 2;
+      case SDG_FIELD_FACE_JACOBIAN_W: return F * nfq;
+      case SDG_FIELD_FACE_COORD: return F * nfq * 2;
+      case SDG_FIELD_FACE_PARENT: return F * 2;
+      case SDG_FIELD_FACE_LOCAL: return F * 2;
+      case SDG_FIELD_FACE_SEQUENCE: return F * nfq;
       default: return 0;
     }
   }
@@ -250,6 +262,42 @@ int sdg_mesh_export(const sdg_mesh* mesh, int field, void* out, int64_t count) {
   if (!mesh->single) {
     const sdg::MixedMesh& x = mesh->mixed;
     double* od = static_cast<double*>(out);
+    int32_t* oi = static_cast<int32_t*>(out);
+    const size_t nfq = static_cast<size_t>(x.ref[0].nfq);
+    // per-element blocks concatenated in element order (each element's own N_q or N_b)
+    if (field == SDG_FIELD_JACOBIAN_TIW || field == SDG_FIELD_MASS_INVERSE) {
+      int64_t o = 0;
+      for (int64_t e = 0; e < x.elements; e++) {
+        const sdg::RefElement& r = x.refOf(e);
+        if (field == SDG_FIELD_JACOBIAN_TIW) {
+          sdg::affineGeometry(r, &x.jac[static_cast<size_t>(e) * 4], x.detj[e], nullptr, od + o);
+          o += static_cast<int64_t>(r.nq) * 4;
+        } else {
+          sdg::affineMassInverse(r, x.detj[e], od + o);
+          o += static_cast<int64_t>(r.nb) * r.nb;
+        }
+      }
+      return SDG_OK;
+    }
+    for (size_t f = 0; field >= SDG_FIELD_FACE_NORMAL && field <= SDG_FIELD_FACE_SEQUENCE && f < x.faces.size(); f++) {
+      const sdg::Face& fc = x.faces[f];
+      switch (field) {
+        case SDG_FIELD_FACE_NORMAL: std::memcpy(od + f * nfq * 2, fc.normal.data(), sizeof(double) * nfq * 2); break;
+        case SDG_FIELD_FACE_JACOBIAN_W: std::memcpy(od + f * nfq, fc.jw.data(), sizeof(double) * nfq); break;
+        case SDG_FIELD_FACE_COORD: std::memcpy(od + f * nfq * 2, fc.coord.data(), sizeof(double) * nfq * 2); break;
+        case SDG_FIELD_FACE_PARENT:
+          oi[2 * f] = fc.parent[0];
+          oi[2 * f + 1] = fc.parent[1];
+          break;
+        case SDG_FIELD_FACE_LOCAL:
+          oi[2 * f] = fc.local[0];
+          oi[2 * f + 1] = fc.local[1];
+          break;
+        case SDG_FIELD_FACE_SEQUENCE: std::memcpy(oi + f * nfq, fc.seq.data(), sizeof(int32_t) * nfq); break;
+        default: break;
+      }
+    }
+    if (field >= SDG_FIELD_FACE_NORMAL && field <= SDG_FIELD_FACE_SEQUENCE) return SDG_OK;
     int64_t o = 0;
     for (int64_t e = 0; e < x.elements; e++) {
       const sdg::RefElement& r = x.refOf(e);

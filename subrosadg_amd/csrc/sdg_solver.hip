@@ -841,9 +841,7 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     static const double kThreshold[5] = {0.0, -1.20411998266, -1.90848501888, -2.40823996531, -2.79588001734};
     ph.av_s0 = kThreshold[r.order - 1];
     const int p = r.order - 1;  // the hierarchic basis of order P-1 is a prefix of order P's
-    ph.av_low = p == 0 ? 0
-                : (r.type == SDG_ELEM_TRIANGLE ? (p + 1) * (p + 2) / 2
-                                               : (r.dim == 1 ? p + 1 : (r.dim == 2 ? (p + 1) * (p + 1) : (p + 1) * (p + 1) * (p + 1))));
+    ph.av_low = p == 0 ? 0 : sdg::basisSize(r.type, r.dim, p);
   }
   const char* ng = std::getenv("SDG_NO_GRAPH");
   ctx->use_graph = !(ng && ng[0] == '1');
@@ -1536,9 +1534,8 @@ int sdg_raw_binary_read(sdg_ctx* ctx, const char* path, int initial_condition) {
     return fail(ctx, SDG_E_INVALID, "sdg_raw_binary_read: initial_condition must be LAST_STEP or SPECIFIC_FILE");
   int nb_src = ctx->nb;
   if (initial_condition == SDG_IC_SPECIFIC_FILE) {
-    const int q = ctx->order;  // basis size of order P - 1
-    if (q < 1) return fail(ctx, SDG_E_INVALID, "sdg_raw_binary_read: SPECIFIC_FILE needs P >= 2");
-    nb_src = ctx->type == SDG_ELEM_TRIANGLE ? q * (q + 1) / 2 : (ctx->dim == 2 ? q * q : q * q * q);
+    if (ctx->order < 2) return fail(ctx, SDG_E_INVALID, "sdg_raw_binary_read: SPECIFIC_FILE needs P >= 2");
+    nb_src = sdg::basisSize(ctx->type, ctx->dim, ctx->order - 1);  // basis of order P - 1
   }
   std::vector<char> raw;
   int rc = sdg::readRawFile(path, raw);

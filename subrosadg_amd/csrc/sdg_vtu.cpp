@@ -49,6 +49,13 @@ void samplePattern(const sdg::RefElement& ref, std::vector<double>& xi, std::vec
     return;
   }
   auto t = [&](int i) { return -1.0 + 2.0 * i / P; };
+  if (dim == 1) {
+    for (int i = 0; i <= P; i++) xi.push_back(t(i));
+    for (int i = 0; i < P; i++) cells.insert(cells.end(), {i, i + 1});
+    cell_nodes = 2;
+    vtk_type = 3;  // VTK_LINE
+    return;
+  }
   if (dim == 2) {
     for (int j = 0; j <= P; j++)
       for (int i = 0; i <= P; i++) {

@@ -4,7 +4,7 @@ from __future__ import annotations
 
 import numpy as np
 
-GMSH_TYPE = {"tri": 2, "quad": 3, "hex": 5, "line": 1, "bquad": 3}
+GMSH_TYPE = {"tri": 2, "quad": 3, "hex": 5, "line": 1, "bquad": 3, "btri": 2}
 
 
 def write_msh(path, dim: int, node_tags: np.ndarray, xyz: np.ndarray, elem_type: str, elems: np.ndarray,

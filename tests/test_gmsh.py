@@ -208,3 +208,16 @@ def test_refusals(tmp_path):
         L.Mesh.gmsh(p, 2, {})
     with pytest.raises(L.SDGError, match="cannot open"):
         L.Mesh.gmsh(str(tmp_path / "none.msh"), 2, {1: FAR})
+
+
+def test_boundary_elements_that_are_not_faces_are_refused(tmp_path):
+    """A hexahedron file whose wall group holds triangles (3 nodes against the 4 of a hexahedron
+    face) is refused with an error naming the group, instead of reading past the node list."""
+    tags, xyz, elems, sides = W.grid_hexes(2)
+    groups = [(10 + k, "bquad", sides[k]) for k in range(5)]
+    q = sides[5]
+    tris = np.concatenate([q[:, [0, 1, 2]], q[:, [0, 2, 3]]])
+    p = str(tmp_path / "tri_wall.msh")
+    W.write_msh(p, 3, tags, xyz, "hex", elems, groups + [(15, "btri", tris)])
+    with pytest.raises(L.SDGError, match="physical group 15"):
+        L.Mesh.gmsh(p, 2, {10 + k: FAR for k in range(6)})

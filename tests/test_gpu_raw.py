@@ -72,6 +72,25 @@ def test_specific_file_prolongs_from_order_p_minus_one(tmp_path):
     assert np.all(hi[:, lo_mesh.nb:, :] == 0.0)
 
 
+@pytest.mark.parametrize("order", [3, 4])
+def test_specific_file_on_lines(tmp_path, order):
+    """A line run of order P starts from an order P-1 file (basis sizes P and P+1)."""
+    print(f"specific file on lines P{order}", flush=True)
+    prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=L.FLUX_ROE, cp=3.5, cv=2.5)
+    meshes = [L.Mesh(L.ELEM_LINE, p, (8,), (0.0,), (1.0,)) for p in (order - 1, order)]
+    x = meshes[0].field("quad_coord").reshape(meshes[0].E, meshes[0].nq, 1)[..., 0]
+    prim = np.stack([1.0 + 0.2 * np.sin(2 * np.pi * x), 0.3 + 0.0 * x, 1.0 + 0.0 * x], axis=-1)
+    s = L.Solver(prob, meshes[0])
+    s.set_initial_primitive(prim)
+    path = str(tmp_path / "line.zst")
+    s.write_raw(path)
+    r = L.Solver(prob, meshes[1])
+    r.read_raw(path, L.IC_SPECIFIC_FILE)
+    lo, hi = s.get_state(), r.get_state()
+    assert np.array_equal(hi[:, : meshes[0].nb, :], lo)
+    assert np.all(hi[:, meshes[0].nb:, :] == 0.0)
+
+
 def test_boundary_faces_carry_their_parents_blocks(tmp_path):
     mesh, prob, prim, bprim = P.euler_box(n=4, order=2)
     s = L.Solver(prob, mesh)

@@ -19,7 +19,8 @@ GAMMA, CV = 1.4, 2.5
 
 def _fields(x):
     """Conserved fields of degree <= 2 in every coordinate (rho > 0, p > 0 on the unit box)."""
-    X, Y = x[..., 0], x[..., 1]
+    X = x[..., 0]
+    Y = x[..., 1] if x.shape[-1] >= 2 else 0.5 * X
     Z = x[..., 2] if x.shape[-1] == 3 else 0.0
     rho = 1.0 + 0.2 * X + 0.1 * Y * Y + 0.05 * Z
     mom = [0.3 + 0.1 * X * Y, -0.2 + 0.1 * Y, 0.1 * X + 0.05 * Z][: x.shape[-1]]
@@ -28,6 +29,8 @@ def _fields(x):
 
 
 def _mesh(kind, order):
+    if kind == "line":
+        return L.Mesh(L.ELEM_LINE, order, (4,), (0.0,), (1.0,))
     if kind == "tri":
         return L.Mesh.split(order, (3, 3), (0.0, 0.0), (1.0, 1.0), 1.0, 2)
     if kind == "hex":
@@ -47,7 +50,8 @@ def _read(path):
     return arrays, int(piece.get("NumberOfPoints")), int(piece.get("NumberOfCells"))
 
 
-@pytest.mark.parametrize("kind,order", [("quad", 2), ("quad", 3), ("tri", 2), ("tri", 3), ("hex", 2)])
+@pytest.mark.parametrize("kind,order", [("line", 2), ("line", 4), ("quad", 2), ("quad", 3), ("tri", 2), ("tri", 3),
+                                        ("hex", 2)])
 def test_vtu_points_fields_and_cells(tmp_path, kind, order):
     m = _mesh(kind, order)
     prob = L.problem_desc(equation=L.EQ_EULER, cp=GAMMA * CV, cv=CV)
@@ -77,6 +81,8 @@ def test_vtu_points_fields_and_cells(tmp_path, kind, order):
     pts = x.reshape(m.E, per, m.dim)
     av = a["ArtificialViscosity"].reshape(m.E, per)
     corners = [0, order, per - 1] if kind == "tri" else None
+    if kind == "line":
+        corners = [0, order]
     if kind == "quad":
         corners = [0, order, per - 1, per - 1 - order]
     if kind == "hex":
@@ -87,7 +93,11 @@ def test_vtu_points_fields_and_cells(tmp_path, kind, order):
     # the sub-cells tile the elements with positive orientation
     conn = a["connectivity"].astype(np.int64).reshape(ncells, -1)
     cx = a["Points"][conn][..., : m.dim]
-    if m.dim == 2:
+    if m.dim == 1:
+        length = cx[:, 1, 0] - cx[:, 0, 0]
+        assert conn.shape[1] == 2 and np.all(length > 0.0) and math.isclose(length.sum(), 1.0, rel_tol=1e-12)
+        assert np.allclose(length, 1.0 / (m.E * order), rtol=1e-12)  # equispaced points of order P
+    elif m.dim == 2:
         xs, ys = cx[..., 0], cx[..., 1]
         area = 0.5 * (xs * np.roll(ys, -1, axis=1) - np.roll(xs, -1, axis=1) * ys).sum(1)
         assert np.all(area > 0.0) and math.isclose(area.sum(), 1.0, rel_tol=1e-12)
@@ -96,7 +106,8 @@ def test_vtu_points_fields_and_cells(tmp_path, kind, order):
         vol = np.einsum("ci,ci->c", np.cross(e1, e3), e4)
         assert np.all(vol > 0.0) and math.isclose(vol.sum(), 1.0, rel_tol=1e-12)
         assert np.allclose(cx[:, 6], cx[:, 0] + e1 + e3 + e4)  # opposite corner of a box cell
-    assert set(np.unique(a["types"]).astype(int)) == {5 if kind == "tri" else (9 if m.dim == 2 else 12)}
+    vtk = {"line": 3, "tri": 5, "quad": 9, "hex": 12}[kind]
+    assert set(np.unique(a["types"]).astype(int)) == {vtk}
 
 
 def test_vtu_without_viscosity_and_errors(tmp_path):
