@@ -1,20 +1,24 @@
-"""Benchmark: DG residual + SSP-RK3 stepping of the 3-D compressible Navier-Stokes Taylor-Green
-vortex on one MI355X, in the FP64 mode and in the mixed-precision mode.
+"""Benchmark: DG residual + SSP-RK3 stepping on one MI355X.
 
-Workload (default): periodic [0, 2 pi]^3, uniform 32^3 hexahedra, P3 (64 modes per element,
-5 variables), Re = 1600, Ma = 0.1, Pr = 0.71, HLLC + BR2, SSP-RK3, 500 timed steps. Synthetic
-initial state u = sin x cos y cos z, v = -cos x sin y cos z, w = 0,
-p = p0 + (cos 2x + cos 2y)(cos 2z + 2)/16, rho = p / (R T0).
+Headline workload (`value`): the 3-D compressible Navier-Stokes Taylor-Green vortex, periodic
+[0, 2 pi]^3, uniform 32^3 hexahedra, P3 (64 modes per element, 5 variables), Re = 1600, Ma = 0.1,
+Pr = 0.71, HLLC + BR2, SSP-RK3, CFL 0.1, with synthetic initial state u = sin x cos y cos z,
+v = -cos x sin y cos z, w = 0, p = p0 + (cos 2x + cos 2y)(cos 2z + 2)/16, rho = p / (R T0). A step is
+one SSP-RK3 time step (3 stages) of the whole mesh with the state resident in HBM.
+Metric: DOF.stage/s = E * N_b * stages * steps / wall time of the timed steps, the median of 5 runs
+of K steps each (after W warm-up steps). `value` is the mixed-precision rate (FP32 contractions and
+work buffers, FP64 state, geometry, pointwise physics and RK accumulation); the line also carries the
+FP64 rate, their ratio and the relative L2 deviation of the mixed state from the FP64 state after the
+same steps.
 
-A step is one SSP-RK3 time step (3 stages) of the whole mesh with the state resident in HBM.
-Metric: DOF.stage/s = E * N_b * stages * steps / wall time of the timed steps. `value` is the
-mixed-precision rate (FP32 matrix-core contractions, FP64 state, geometry, pointwise physics and RK
-accumulation); the line also carries the FP64 rate, their ratio and the relative L2 deviation of
-the mixed state from the FP64 state after the same steps (per variable group, the largest).
-
-Also printed: a roofline entry for the dominant kernel of each mode (HIP events on the solver's
-stream) and the CPU baseline (the oracle on a bounded sample of the same workload on the host
-cores).
+Further legs (same JSON line): the 2-D Euler isentropic vortex (32^2 quadrangles P2, Roe, 200 steps)
+with its final state against the oracle; the 64^3 hexahedron P4 Taylor-Green vortex in FP64 and
+mixed precision; the 48-problem batched uneven 2-D Euler workload; the discrete adjoint of the 2-D
+double shear layer (adjoint / forward time at full storage and at a binomial snapshot budget); the
+adjoint of the 64^3 P4 Taylor-Green vortex's kinetic-energy decay with respect to the velocity
+amplitude; artificial-viscosity shock capturing. A roofline entry for the dominant kernel (HIP events
+on the solver's stream, bytes and flops as defined below) and the CPU baseline (the oracle on the
+same 32^3 mesh on the host's cores).
 """
 from __future__ import annotations
 
@@ -22,6 +26,8 @@ import argparse
 import json
 import math
 import os
+import statistics
+import subprocess
 import sys
 import time
 
@@ -34,28 +40,30 @@ from subrosadg_amd import _lib as L  # noqa: E402
 
 GAMMA = 1.4
 PEAK_FP64_TFLOPS = 78.6        # MI355X dense FP64 (vector and matrix)
-PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
+PEAK_FP32_TFLOPS = 157.3       # MI355X dense FP32 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
+RUNS = 5                       # timed runs per configuration; the median is reported
 
-# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled
-# by a copy kernel of known size (fetch x2.000, write x1.000); kernel summary in
-# profiles/ns_tgv32_p3_fp64_mixed_mfma_kernels.md. Keyed by (kernel, precision mode, cells per
-# direction, order).
-# Sum-factorised kernels (the default on hexahedra): profiles/ns_tgv32_p3_fp64_mixed_tensor_kernels.md.
-PROFILED_TRAFFIC = {("nsGradMfma", "fp64", 32, 3): (190207.1 * 2.0 + 614400.0 * 1.0) * 1024,
-                    ("nsGradMfma", "mixed", 32, 3): (144194.1 * 2.0 + 307200.0 * 1.0) * 1024,
-                    ("nsGradTensor", "fp64", 32, 3): (194245.8 * 2.0 + 204800.0 * 1.0) * 1024,
+# HBM bytes per launch of the stage kernels on the 32^3 P3 workload, from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE passes scaled by a copy kernel of known size (fetch x2.000, write x1.000):
+# profiles/ns_tgv32_p3_tensor_counters.md. Keyed by (kernel, precision mode, cells per direction, order).
+PROFILED_TRAFFIC = {("nsGradTensor", "fp64", 32, 3): (194245.8 * 2.0 + 204800.0 * 1.0) * 1024,
                     ("nsGradTensor", "mixed", 32, 3): (145675.7 * 2.0 + 102400.0 * 1.0) * 1024,
                     ("nsFaceFluxKernel", "fp64", 32, 3): (151386.9 * 2.0 + 122880.0 * 1.0) * 1024,
                     ("nsFaceFluxKernel", "mixed", 32, 3): (114860.2 * 2.0 + 61509.7 * 1.0) * 1024,
                     ("nsResidualTensor", "fp64", 32, 3): (184933.9 * 2.0 + 232533.4 * 1.0) * 1024,
                     ("nsResidualTensor", "mixed", 32, 3): (133740.9 * 2.0 + 171093.4 * 1.0) * 1024}
-# the same for the batched stage kernel on the 48-problem workload (average over its launches;
+# the batched stage kernel on the 48-problem workload (average over its launches;
 # profiles/batch48_uneven_tri_quad_euler_kernels.md)
 PROFILED_BATCH_TRAFFIC = {"fp64": (346788.6 * 2.0 + 231499.4 * 1.0) * 1024,
                           "mixed": (312995.3 * 2.0 + 136595.1 * 1.0) * 1024}
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# ---- workloads -----------------------------------------------------------------------------------
 def tgv_setup(n: int, order: int, mach=0.1, reynolds=1600.0):
     mesh = L.Mesh(L.ELEM_HEXAHEDRON, order, (n, n, n), (0.0, 0.0, 0.0), (2 * math.pi,) * 3)
     p0 = 1.0 / (GAMMA * mach ** 2)
@@ -65,235 +73,307 @@ def tgv_setup(n: int, order: int, mach=0.1, reynolds=1600.0):
     prob = L.problem_desc(equation=L.EQ_NS, convective_flux=L.FLUX_HLLC, viscous_flux=L.VISC_BR2,
                           time_integration=L.RK_SSPRK3, transport=L.TRANSPORT_CONSTANT,
                           cp=cp, cv=cv, dynamic_viscosity=1.0 / reynolds)
+    return mesh, prob, tgv_primitive(mesh, p0, R)
+
+
+def tgv_primitive(mesh, p0, R, amplitude=1.0):
+    """Taylor-Green primitive state (rho, u, v, w, T) at the volume points; the velocity scaled by
+    `amplitude` and the pressure perturbation by its square (the incompressible pressure of the
+    scaled field), rho = p / (R T0), T0 = 1."""
     xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 3)
     x, y, z = xq[..., 0], xq[..., 1], xq[..., 2]
-    p = p0 + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0) / 16.0
+    p = p0 + amplitude ** 2 * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0) / 16.0
     prim = np.zeros((mesh.E, mesh.nq, 5))
     prim[..., 0] = p / (R * 1.0)
-    prim[..., 1] = np.sin(x) * np.cos(y) * np.cos(z)
-    prim[..., 2] = -np.cos(x) * np.sin(y) * np.cos(z)
+    prim[..., 1] = amplitude * np.sin(x) * np.cos(y) * np.cos(z)
+    prim[..., 2] = -amplitude * np.cos(x) * np.sin(y) * np.cos(z)
     prim[..., 4] = 1.0
-    return mesh, prob, prim
+    return prim
 
 
-def algorithmic_counts(mesh, ns=True):
-    """Dense reference-formulation flop counts per element-stage, split by the kernel that performs
-    each contraction, and the FP64 state bytes B_alg.
+def vortex_primitive(xq, eps=5.0):
+    """Shu isentropic vortex on the uniform stream rho = u = v = p = 1 (R = 1): primitive (rho, u,
+    v, T) at points xq (..., 2)."""
+    x, y = xq[..., 0], xq[..., 1]
+    r2 = x * x + y * y
+    f = np.exp(0.5 * (1.0 - r2))
+    T = 1.0 - (GAMMA - 1.0) * eps * eps / (8.0 * GAMMA * math.pi ** 2) * f * f
+    prim = np.empty(xq.shape[:-1] + (4,))
+    prim[..., 0] = T ** (1.0 / (GAMMA - 1.0))
+    prim[..., 1] = 1.0 - eps / (2 * math.pi) * f * y
+    prim[..., 2] = 1.0 + eps / (2 * math.pi) * f * x
+    prim[..., 3] = T
+    return prim
 
-    F_alg = 2 NV N_b (N_q + 2 N_aq + D N_q + N_b)
-          + [NS] 2 NV D N_b (D N_q + N_q + 3 N_aq + N_b (1 + N_f))
-    gradient kernel: volume interpolation and every gradient-lifting term;
-    residual kernel: weak divergence, face lift, mass inverse and next-stage traces."""
+
+# ---- algorithmic counts ------------------------------------------------------------------------------
+def algorithmic_counts(mesh, ns=True, mixed=False):
+    """Per element and stage, the dense reference formulation (each distinct contraction counted
+    once, pointwise flux flops excluded):
+      F_alg = 2 NV N_b (N_q + 2 N_aq + D N_q + N_b) + [NS] 2 NV D N_b (D N_q + N_q + 3 N_aq + N_b (1 + N_f))
+      B_alg = 8 (3 NV N_b) + [NS] 8 (2 NV D N_b) in FP64; the gradient term at 4 bytes in mixed mode
+    (read U^i and U^n, write U^{i+1}; write and read the gradient coefficients)."""
     D, nv, nb, nq = mesh.dim, mesh.nv, mesh.nb, mesh.nq
     naq, nf = mesh.nf * mesh.nfq, mesh.nf
-    f_grad = 2 * nv * nb * nq + (2 * nv * D * nb * (D * nq + nq + 3 * naq + nb * (1 + nf)) if ns else 0)
-    f_res = 2 * nv * nb * (D * nq + 2 * naq + nb)
-    b_alg = 8 * (3 * nv * nb + (2 * nv * D * nb if ns else 0))
-    return f_grad, f_res, b_alg
+    f_alg = 2 * nv * nb * (nq + 2 * naq + D * nq + nb)
+    if ns:
+        f_alg += 2 * nv * D * nb * (D * nq + nq + 3 * naq + nb * (1 + nf))
+    b_alg = 8 * 3 * nv * nb + ((4 if mixed else 8) * 2 * nv * D * nb if ns else 0)
+    return f_alg, b_alg
+
+
+def tensor_flops(order: int) -> int:
+    """Contraction flops per element and stage that the sum-factorised hexahedron kernels execute
+    (N = P+1 nodes per direction, N^3 nodes, 6 N^2 face points; pointwise physics excluded, as in
+    F_alg): modal -> nodal state, 3 line passes of N-term sums over NV components (3 NV N^4 MACs);
+    reference derivatives along 3 axes (3 NV N^4); the chain rule J^-T (9 NV N^3); the six faces'
+    lift corrections (6 NV 2 D N^3); traces of the 15 gradient components (15 6 N^2 N); the weak
+    divergence Dn^T along 3 axes after the contravariant map (3 NV N^4 + 9 NV N^3); the face lift
+    of the fluxes (6 NV N^3); back to modal coefficients, 3 B^-1 line passes (3 NV N^4); the next
+    stage's traces (NV 6 N^2 N). Two flops per MAC."""
+    N = order + 1
+    nv = 5
+    macs = 12 * nv * N ** 4 + (9 + 12 + 9 + 6) * nv * N ** 3 + 15 * 6 * N ** 3 + nv * 6 * N ** 3
+    return 2 * macs
 
 
 def tensor_path(mesh) -> bool:
     """True where the engine runs the sum-factorised Navier-Stokes kernels (hexahedra P2-P4)."""
-    return mesh.elem == L.ELEM_HEXAHEDRON and 2 <= mesh.order <= 4 and os.environ.get("SDG_DENSE") != "1"
+    return mesh.elem == L.ELEM_HEXAHEDRON and 2 <= mesh.order <= 4
 
 
-def tensor_bytes(mesh, tc: int):
-    """Algorithmic HBM bytes per element and launch of the three sum-factorised Navier-Stokes stage
-    kernels on a periodic mesh (tc: bytes of the contraction type, 8 FP64 / 4 mixed; the state is
-    FP64 in both):
-      gradient: state U + own and neighbour face traces + the normal viscous flux Vn at the
-                element's face points and the weak divergence Rvol of the volume flux at its
-                nodes (both written in the contraction type) + geometry (|J|, J^-1, two slot maps,
-                and half of each face point's normal and weight, which the neighbour shares);
-      face flux (per element, its half of the face points): both sides' traces and normal viscous
-                fluxes, normal, weight, point map, and the two signed flux slots written;
-      residual: Rvol + flux slots + U + U_n read, U + next traces written, U_n written at the last
-                of the 3 stages (1/3 per launch), |J|."""
-    nv, nq, naq, D = mesh.nv, mesh.nq, mesh.nf * mesh.nfq, mesh.dim
-    geo = 8 + 8 * D * D + naq * 8 + naq * (8 * D + 8) / 2
-    grad = 8 * nv * nq + 2 * nv * naq * tc + nv * naq * tc + nv * nq * tc + geo
-    face = naq / 2 * (4 * nv * tc + 8 * D + 8 + 4 + 2 * nv * tc)
-    res = nv * nq * tc + nv * naq * tc + 3 * 8 * nv * nq + nv * naq * tc + 8 * nv * nq / 3 + 8
-    return grad, face, res
-
-
-def cpu_baseline(order: int, target_s: float = 15.0):
-    """The CPU oracle (C++/OpenMP restatement of the reference) on a bounded sample of the same
-    workload: an 8^3 mesh of the same element, order, physics and scheme."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O  # checker library, used here only as the timed CPU baseline
-
-    cores = os.cpu_count() or 1
-    threads = max(1, min(cores - 1, 16)) if cores > 1 else 1  # reference rule: physical cores - 1
-    O.set_threads(threads)
-    n = 8
-    mesh, prob, prim = tgv_setup(n, order)
-    o = O.Oracle(prob, mesh)
-    o.set_initial_primitive(prim)
-    dt = o.compute_dt(0.1)
-    t0 = time.perf_counter()
-    o.step(dt, 1)
-    one = time.perf_counter() - t0
-    steps = max(1, min(400, int(target_s / max(one, 1e-6))))
-    t0 = time.perf_counter()
-    o.step(dt, steps)
-    t = time.perf_counter() - t0
-    value = mesh.E * mesh.nb * 3 * steps / t
-    return {"value": value, "unit": "DOF.stage/s", "cores": O.get_threads(), "kind": "port",
-            "sample": f"oracle, TGV {n}^3 hex P{order} HLLC+BR2 SSP-RK3 FP64, {steps} steps, {t:.1f} s"}
-
-
-def copy_probe(mib: int, reps: int = 5):
-    """Calibration workload for the HBM byte counters: an elementwise copy of `mib` MiB (read mib,
-    write mib per launch), run under `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` beside the solver
-    so that the counters' scale can be read off a kernel of known traffic."""
-    import torch
-
-    n = mib * (1 << 20) // 8
-    src = torch.rand(n, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(reps):
-        torch.mul(src, 1.0, out=dst)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_mib": mib, "reps": reps, "bytes_read": n * 8, "bytes_written": n * 8}))
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=32, help="elements per direction")
-    ap.add_argument("--order", type=int, default=3)
-    ap.add_argument("--cfl", type=float, default=0.1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing leg")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-mesh leg")
-    ap.add_argument("--no-shock", action="store_true", help="skip the artificial-viscosity leg")
-    ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="only run the HBM-counter calibration copy of this many MiB")
-    ap.add_argument("--profile-leg", choices=["adjoint", "batch", "shock"], default=None,
-                    help="only run one secondary leg (for kernel profiling) and print its record")
-    args = ap.parse_args()
-    if args.copy_probe_mib > 0:
-        copy_probe(args.copy_probe_mib)
-        return
-    if args.profile_leg:
-        leg = {"adjoint": adjoint_leg, "batch": batch_leg, "shock": shock_leg}[args.profile_leg](args)
-        print(json.dumps(leg), flush=True)
-        return
-    if args.gpus != 1:
-        print("bench.py: runs on one GPU only; sharding the element partition over several GPUs "
-              "(face-trace halo exchange per stage) is not implemented", file=sys.stderr)
-        sys.exit(2)
-
-    t_setup = time.perf_counter()
-    mesh, prob, prim = tgv_setup(args.n, args.order)
-    prob_mixed = L.ProblemDesc.from_buffer_copy(prob)
-    prob_mixed.precision = L.PREC_MIXED
-    runs = {}
-    for mode, pd in (("fp64", prob), ("mixed", prob_mixed)):
-        runs[mode] = timed_run(mode, pd, mesh, prim, args)
-    print(f"[bench] setup+runs {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
-    # deviation of the mixed-precision state from the FP64 state after the same warmup + timed steps
-    # from the same initial coefficients (the FP64 engine matches the oracle to 1e-12)
-    a_m, a_d = runs["mixed"]["state"], runs["fp64"]["state"]
-    dev = function_l2(mesh, a_m, a_d)
-    dev_coeff = group_l2(a_m, a_d, mesh.dim)
-    dev_all = float(np.linalg.norm(a_m - a_d) / np.linalg.norm(a_d))
-    dof = mesh.E * mesh.nb
-    f_grad, f_res, _ = algorithmic_counts(mesh)
-    roof = {mode: roofline(r["kms"], f_grad, f_res, mesh, args, mode) for mode, r in runs.items()}
-    head = runs["mixed"]
-
-    adjoint = None if args.no_adjoint else adjoint_leg(args)
-    batch = None if args.no_batch else batch_leg(args)
-    shock = None if args.no_shock else shock_leg(args)
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args.order)
-    line = {
-        "metric": "DOF.stage/s mixed vs FP64 at fixed L2 error (3-D NS Taylor-Green P3)",
-        "value": head["value"],
-        "unit": "DOF.stage/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": head["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 contractions and work buffers (sum-factorised line passes), f64 state/geometry/"
-                 "physics/RK",
-        "data": "synthetic",
-        "config": {"workload": f"3-D compressible NS Taylor-Green vortex, periodic [0,2pi]^3, {args.n}^3 hex, "
-                               f"P{args.order}, Re 1600, Ma 0.1, HLLC + BR2, SSP-RK3, CFL {args.cfl}",
-                   "elements": mesh.E, "basis": mesh.nb, "dof_per_field": dof, "precision": "mixed",
-                   "parallelism": "single GPU"},
-        "fp64": {"value": runs["fp64"]["value"], "ms_per_step": runs["fp64"]["ms_per_step"], "dtype": "f64"},
-        "mixed_over_fp64": head["value"] / runs["fp64"]["value"],
-        "mixed_l2_rel_dev_vs_fp64": dev,
-        "mixed_l2_rel_dev_definition": "max over density, momentum, energy of ||u_mixed - u_fp64||_L2(domain) / "
-                                       "||u_fp64||_L2(domain), after the warmup and timed steps",
-        "mixed_coefficient_l2_rel_dev_vs_fp64": dev_coeff,
-        "mixed_coefficient_l2_rel_dev_vs_fp64_whole_state": dev_all,
-        "l2_dev_after_steps": args.warmup + args.steps,
-        "roofline": roof["mixed"],
-        "roofline_fp64": roof["fp64"],
-        "adjoint": adjoint,
-        "batch": batch,
-        "shock_capturing": shock,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
-
-
-def shock_leg(args, n=256, order=3, steps=100, cfl=0.1):
-    """Artificial-viscosity cost on the explosion case (2-D Euler, [0,2]^2 with adiabatic slip
-    walls, a circle of radius 0.4 at rho = 1, p = 1 in rho = 0.125, p = 0.1 spread over half a
-    cell, 256^2 quadrangles P3, HLLC, SSP-RK3, setArtificialViscosity(4.0)): the step with the
-    per-step sensor and the gradient passes (vector kernels) against the plain Euler step on the
-    same mesh and state."""
-    slip = L.BC_ADIABATIC_SLIP_WALL
-    mesh = L.Mesh(L.ELEM_QUADRANGLE, order, (n, n), (0.0, 0.0), (2.0, 2.0), boundary=((slip, slip), (slip, slip)))
-    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 2)
-    r = np.sqrt((xq[..., 0] - 1.0) ** 2 + (xq[..., 1] - 1.0) ** 2)
-    w = 0.5 * (1.0 - np.tanh((r - 0.4) / (1.0 / n)))
-    gas = 2.5 - 25.0 / 14.0
-    prim = np.zeros((mesh.E, mesh.nq, 4))
-    prim[..., 0] = 0.125 + 0.875 * w
-    prim[..., 3] = (0.1 + 0.9 * w) / (prim[..., 0] * gas)
-    out = {"workload": f"2-D Euler explosion, [0,2]^2 slip walls, {n}^2 quads, P{order}, HLLC, SSP-RK3, CFL {cfl}, "
-                       f"artificial viscosity (tolerance 4.0, factor 1)", "elements": mesh.E, "steps": steps}
-    for label, sc in (("artificial_viscosity", L.SHOCK_ARTIFICIAL_VISCOSITY), ("plain", L.SHOCK_NONE)):
-        prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=L.FLUX_HLLC, cp=2.5, cv=25.0 / 14.0,
-                              shock_capturing=sc, empirical_tolerance=4.0, artificial_viscosity_factor=1.0)
-        s = L.Solver(prob, mesh)
-        s.set_initial_primitive(prim)
-        s.set_boundary_primitive(np.zeros((mesh.Fb, mesh.nfq, 4)))
-        dt = s.compute_dt(cfl)
-        s.step(dt, 5)
-        kms = s.time_kernels(dt, 5)
-        s.synchronize()
-        t0 = time.perf_counter()
-        _, nan, taken = s.step(dt, steps)
-        s.synchronize()
-        el = (time.perf_counter() - t0) / steps
-        rec = {"ms_per_step": el * 1e3, "dof_stage_per_s": mesh.E * mesh.nb * 3 / el, "finite": bool(nan == 0),
-               "kernel_ms": [float(x) for x in kms]}
-        if sc:
-            av = s.artificial_viscosity()
-            rec["nodes_with_viscosity"] = int((av > 0).sum())
-            rec["nodes"] = int(av.size)
-        out[label] = rec
-    out["cost_over_plain"] = out["artificial_viscosity"]["ms_per_step"] / out["plain"]["ms_per_step"]
-    print(f"[bench] shock leg: {out}", file=sys.stderr, flush=True)
+def roofline(kms, mesh, mode, n):
+    """Roofline entry of the dominant kernel. kms: HIP-event averages of the gradient, face-flux and
+    residual kernels (and the monitor) on the solver's stream. `achieved`: the algorithmic bytes
+    B_alg per element-stage times the elements one launch processes, over the launch's duration
+    (B_alg counts the whole stage's state traffic, so this is the rate the stage would need if the
+    kernel were all of it). Also reported: the flop side of the same count (F_alg, the dense
+    formulation) and of the flops the sum-factorised kernels execute, and the whole stage (the
+    three kernels) against B_alg."""
+    mixed = mode == "mixed"
+    f_alg, b_alg = algorithmic_counts(mesh, mixed=mixed)
+    names = ["nsGradTensor", "nsFaceFluxKernel", "nsResidualTensor"] if tensor_path(mesh) else \
+        ["nsGradMfma", "nsFaceFluxKernel", "nsResidualMfma"]
+    k = int(np.argmax(kms[:3]))
+    name, ms = names[k], float(kms[k])
+    stage_ms = float(sum(kms[:3]))
+    peak_f = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
+    achieved = b_alg * mesh.E / (ms * 1e-3) / 1e9
+    out = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+           "traffic": PROFILED_TRAFFIC.get((name, mode, n, mesh.order)), "kernel": name, "kernel_ms": ms,
+           "algorithmic_bytes_per_element_stage": b_alg, "kernel_ms_all": [float(x) for x in kms],
+           "kernel_ms_names": names + ["monitor"], "stage_ms": stage_ms,
+           "stage_hbm_frac": b_alg * mesh.E / (stage_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+           "flop_alg_per_element_stage": f_alg,
+           "flop_alg_frac_of_peak_kernel": f_alg * mesh.E / (ms * 1e-3) / 1e12 / peak_f,
+           "flop_alg_frac_of_peak_stage": f_alg * mesh.E / (stage_ms * 1e-3) / 1e12 / peak_f}
+    if tensor_path(mesh):
+        f_sf = tensor_flops(mesh.order)
+        out["flop_executed_per_element_stage"] = f_sf
+        out["flop_executed_frac_of_peak_stage"] = f_sf * mesh.E / (stage_ms * 1e-3) / 1e12 / peak_f
     return out
 
 
-def adjoint_leg(args, n=64, order=3, steps=400, cfl=0.1):
+# ---- measurement helpers -----------------------------------------------------------------------------
+def timed_steps(solver, dt, steps, runs=RUNS):
+    """`runs` timed runs of `steps` steps each (the state advances through all of them); wall time
+    per run between stream synchronisations. Returns the per-run ms/step."""
+    out = []
+    for _ in range(runs):
+        solver.synchronize()
+        t0 = time.perf_counter()
+        rel, nan, taken = solver.step(dt, steps)
+        solver.synchronize()
+        wall = time.perf_counter() - t0
+        if nan or taken != steps or not np.all(np.isfinite(rel)):
+            raise RuntimeError(f"bench: run did not complete cleanly (nan={nan}, taken={taken})")
+        out.append(1000.0 * wall / steps)
+    return out
+
+
+def function_l2(mesh, a, b):
+    """max over variable groups of the relative L2(domain) norm of the difference of two solutions
+    given as modal coefficients (reference layout E x N_b x NV): quadrature with |J| w at the
+    volume points, exact for the squared degree-P polynomials."""
+    nv = mesh.nv
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    jw = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
+    out = []
+    for g in ([0], list(range(1, mesh.dim + 1)), [mesh.dim + 1]):
+        num = den = 0.0
+        for c0 in range(0, mesh.E, 32768):  # chunks: the 64^3 P4 point values do not fit at once
+            c1 = min(mesh.E, c0 + 32768)
+            da = (a - b).reshape(mesh.E, mesh.nb, nv)[c0:c1][..., g]
+            bb = b.reshape(mesh.E, mesh.nb, nv)[c0:c1][..., g]
+            qa = np.einsum("qb,ebv->eqv", V, da)
+            qb = np.einsum("qb,ebv->eqv", V, bb)
+            num += float((jw[c0:c1, :, None] * qa ** 2).sum())
+            den += float((jw[c0:c1, :, None] * qb ** 2).sum())
+        out.append(math.sqrt(num / den))
+    return max(out)
+
+
+def group_l2(a, b, dim):
+    """max over variable groups (density, momentum vector, energy) of ||a - b||_2 / ||b||_2."""
+    nv = dim + 2
+    a2, b2 = a.reshape(-1, nv), b.reshape(-1, nv)
+    return max(float(np.linalg.norm(a2[:, g] - b2[:, g]) / np.linalg.norm(b2[:, g]))
+               for g in ([0], list(range(1, dim + 1)), [dim + 1]))
+
+
+def tgv_modes(mesh, prob, prim, args, label, runs=RUNS, kernel_steps=10):
+    """FP64 and mixed runs of one Taylor-Green configuration from the same initial coefficients:
+    per mode the median rate of `runs` runs of args.steps steps after args.warmup steps, the
+    per-kernel HIP-event averages, and the mixed state's deviation from the FP64 state."""
+    out, states, coeff0 = {}, {}, None
+    for mode in ("fp64", "mixed"):
+        pd = L.ProblemDesc.from_buffer_copy(prob)
+        pd.precision = L.PREC_MIXED if mode == "mixed" else L.PREC_FP64
+        s = L.Solver(pd, mesh)
+        if coeff0 is None:
+            s.set_initial_primitive(prim)
+            coeff0 = s.get_state()
+        else:
+            s.set_state(coeff0)
+        dt = s.compute_dt(args.cfl) if mode == "fp64" else out["fp64"]["dt"]
+        s.step(dt, args.warmup)
+        ms = timed_steps(s, dt, args.steps, runs)
+        states[mode] = s.get_state()
+        kms = s.time_kernels(dt, kernel_steps)
+        s.close()
+        med = statistics.median(ms)
+        out[mode] = {"value": mesh.E * mesh.nb * 3 / (med * 1e-3), "ms_per_step": med, "ms_per_step_runs": ms,
+                     "dt": dt, "kms": kms}
+        log(f"{label} {mode}: {med:.3f} ms/step (runs {[round(x, 3) for x in ms]}), "
+            f"{out[mode]['value']:.4e} DOF.stage/s, kernels {[round(float(x), 4) for x in kms]} ms")
+    out["mixed_l2_rel_dev_vs_fp64"] = function_l2(mesh, states["mixed"], states["fp64"])
+    out["mixed_coefficient_l2_rel_dev_vs_fp64"] = group_l2(states["mixed"], states["fp64"], mesh.dim)
+    out["steps_compared"] = args.warmup + runs * args.steps
+    return out
+
+
+# ---- legs ------------------------------------------------------------------------------------------
+def c1_leg(args, n=32, order=2, steps=200):
+    """2-D Euler isentropic vortex: periodic [-5, 5]^2, 32^2 quadrangles P2 (3x3 Gauss, the reference's
+    order-2P rule), Roe, SSP-RK3 at CFL 0.1, 200 steps, FP64; the final state against the oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    mesh = L.Mesh(L.ELEM_QUADRANGLE, order, (n, n), (-5.0, -5.0), (5.0, 5.0))
+    prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=L.FLUX_ROE, time_integration=L.RK_SSPRK3,
+                          cp=3.5, cv=2.5)
+    prim = vortex_primitive(mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 2))
+    s = L.Solver(prob, mesh)
+    s.set_initial_primitive(prim)
+    u0 = s.get_state()
+    dt = s.compute_dt(0.1)
+    s.step(dt, 5)
+    ms = []
+    for _ in range(RUNS):
+        s.set_state(u0)
+        ms += timed_steps(s, dt, steps, runs=1)
+    final = s.get_state()
+    s.close()
+    med = statistics.median(ms)
+    rec = {"workload": f"2-D Euler isentropic vortex, periodic [-5,5]^2, {n}^2 quads, P{order}, Roe, SSP-RK3 CFL 0.1, "
+                       f"{steps} steps, FP64", "value": mesh.E * mesh.nb * 3 / (med * 1e-3), "unit": "DOF.stage/s",
+           "ms_per_step": med, "ms_per_step_runs": ms, "steps": steps}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # checker only: the final state of the same run on the CPU
+        o = O.Oracle(prob, mesh)
+        o.set_state(u0)
+        o.step(dt, steps)
+        uo = o.get_state()
+        rec["final_state_max_rel_err_vs_oracle"] = float(np.abs(final - uo).max() / np.abs(uo).max())
+    log(f"C1: {rec}")
+    return rec
+
+
+def c5_leg(args, n=64, order=4):
+    """3-D NS Taylor-Green vortex on 64^3 hexahedra P4 (32.8 M DOF per field): FP64 and mixed."""
+    mesh, prob, prim = tgv_setup(n, order)
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup = args.c5_steps, 2
+    rec = tgv_modes(mesh, prob, prim, sub, "C5", runs=3, kernel_steps=2)
+    res = {"workload": f"3-D NS Taylor-Green vortex, periodic [0,2pi]^3, {n}^3 hex, P{order}, Re 1600, Ma 0.1, "
+                       f"HLLC + BR2, SSP-RK3 CFL {args.cfl}", "elements": mesh.E, "dof_per_field": mesh.E * mesh.nb}
+    for mode in ("fp64", "mixed"):
+        r = rec[mode]
+        res[mode] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "ms_per_step_runs": r["ms_per_step_runs"],
+                     "roofline": roofline(r["kms"], mesh, mode, n)}
+    res["mixed_over_fp64"] = rec["mixed"]["value"] / rec["fp64"]["value"]
+    res["mixed_l2_rel_dev_vs_fp64"] = rec["mixed_l2_rel_dev_vs_fp64"]
+    res["steps_compared"] = rec["steps_compared"]
+    log(f"C5: {res}")
+    return res
+
+
+def batch_leg(args, count=48, seed=20261202, steps=100, cfl=0.1):
+    """Batched uneven 2-D Euler: `count` isentropic-vortex problems on periodic [-5,5]^2, per
+    problem n ~ U{8..96} cells per direction, P ~ U{1..4}, each cell a quadrangle or two triangles
+    by Bernoulli(0.5) (all draws from numpy's default generator with the given seed), Lax-Friedrichs,
+    SSP-RK3 with a per-problem CFL-0.1 time step, one persistent launch per stage, whole steps
+    captured in a hipGraph, work-item and grid sizes from the autotuner. Timed in mixed precision
+    (the headline mode of this workload) and in FP64 (median of 5 runs of `steps` steps from the
+    initial state); the rate is DOF.stage/s over all problems."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import problems as P  # the workload's draws and the split-grid L2 norm
+
+    specs = P.c3_specs(count, seed)
+    meshes = [L.Mesh.split(order, (n, n), (-5.0, -5.0), (5.0, 5.0), 0.5, sd) for n, order, sd in specs]
+    elements = sum(m.E for m in meshes)
+    out = {"workload": f"{count} 2-D Euler isentropic vortices, periodic [-5,5]^2, n ~ U{{8..96}}, P ~ U{{1..4}}, "
+                       f"Bernoulli(0.5) triangle split, Lax-Friedrichs, SSP-RK3 CFL {cfl}, {steps} steps, seed {seed}",
+           "problems": count, "elements": elements}
+    states = {}
+    prims = [vortex_primitive(m.field("quad_coord").reshape(-1, 2)) for m in meshes]
+    for mode, prec in (("fp64", L.PREC_FP64), ("mixed", L.PREC_MIXED)):
+        prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=L.FLUX_LAX_FRIEDRICHS,
+                              time_integration=L.RK_SSPRK3, cp=3.5, cv=2.5, precision=prec)
+        b = L.Batch(prob, meshes)
+        for i, pr in enumerate(prims):
+            b.set_initial_primitive(i, pr)
+        u0 = [b.get_state(i) for i in range(count)]
+        dt = b.compute_dt(cfl)
+        stage_ms = b.tune()
+        b.step(dt, 3)  # graph capture and warm-up
+        ms = []
+        for _ in range(RUNS):
+            for i in range(count):
+                b.set_state(i, u0[i])
+            b.step(dt, steps)
+            ms.append(b.last_step_ms())
+        dof = sum(sz // 4 for sz in b.sizes)
+        chunk, _, grid = b.config()
+        med = statistics.median(ms)
+        out[mode] = {"value": dof * 3 * steps / (med * 1e-3), "ms_per_step": med / steps,
+                     "ms_per_step_runs": [x / steps for x in ms], "tuned_stage_ms": stage_ms, "grid": grid,
+                     "tiles_per_item": chunk.tolist()}
+        out["dof_per_field"] = dof
+        # algorithmic HBM bytes of a stage: read U^i and U^n, write U^{i+1} (FP64 state)
+        out[mode]["state_gb_per_s"] = 96.0 * dof * 3 * steps / (med * 1e-3) / 1e9
+        out[mode]["state_hbm_frac"] = out[mode]["state_gb_per_s"] / PEAK_HBM_GBS
+        if count == 48 and seed == 20261202:  # profiled HBM bytes per stage launch of this workload
+            tb = PROFILED_BATCH_TRAFFIC[mode]
+            out[mode]["profiled_traffic_mb"] = tb / 1e6
+        states[mode] = [b.get_state(i) for i in range(count)]
+        b.close()
+    out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    devs = [P.split_function_l2(m, a, c) for m, a, c in zip(meshes, states["mixed"], states["fp64"])]
+    out["mixed_l2_rel_dev_vs_fp64"] = max(devs)
+    out["mixed_l2_rel_dev_definition"] = ("max over problems and over density, momentum, energy of "
+                                          "||u_mixed - u_fp64||_L2(domain) / ||u_fp64||_L2(domain) after the steps")
+    out["mixed_coefficient_l2_rel_dev_vs_fp64"] = max(group_l2(a.reshape(-1, 4), c.reshape(-1, 4), 2)
+                                                      for a, c in zip(states["mixed"], states["fp64"]))
+    log(f"batch leg: {out}")
+    return out
+
+
+def adjoint_leg(args, n=64, order=3, steps=400, cfl=0.1, budget=32):
     """Adjoint/forward time ratio on the double shear layer (periodic [0,1]^2, 64^2 quads, P3,
     Roe + BR2, Re 1e4, Ma 0.1, 400 SSP-RK3 steps, FP64): forward = the plain 400-step run;
-    adjoint = the checkpointed forward (one snapshot per step) plus the reverse sweep that turns
-    dJ/dU(t_N) of the final kinetic energy J into dJ/dU(t_0). One gradient direction is checked
+    adjoint = the checkpointed forward plus the reverse sweep that turns dJ/dU(t_N) of the final
+    kinetic energy J into dJ/dU(t_0), with every step stored (no recomputation) and with a binomial
+    budget of 32 snapshots (recomputation from the checkpoints). One gradient direction is checked
     against central differences of J."""
     gamma, mach = 1.4, 0.1
     mesh = L.Mesh(L.ELEM_QUADRANGLE, order, (n, n), (0.0, 0.0), (1.0, 1.0))
@@ -323,22 +403,30 @@ def adjoint_leg(args, n=64, order=3, steps=400, cfl=0.1):
 
     s = L.Solver(prob, mesh)
     s.set_initial_primitive(prim)
+    u0 = s.get_state()
     dt = s.compute_dt(cfl)
     s.step(dt, 5)  # warm-up (graph capture, clocks)
-    s.set_initial_primitive(prim)
-    s.synchronize()
-    t0 = time.perf_counter()
-    s.step(dt, steps)
-    s.synchronize()
-    t_fwd = time.perf_counter() - t0
-    s.set_initial_primitive(prim)
-    t0 = time.perf_counter()
-    s.adjoint_forward(dt, steps, steps)
-    t_rec = time.perf_counter() - t0
-    J, dJ = ke(s.get_state())
-    t0 = time.perf_counter()
-    lam0 = s.adjoint_backward(dJ)
-    t_bwd = time.perf_counter() - t0
+
+    def sync_time(fn):
+        s.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        s.synchronize()
+        return time.perf_counter() - t0, r
+
+    t_fwd = statistics.median([sync_time(lambda: (s.set_state(u0), s.step(dt, steps)))[0] for _ in range(3)])
+    rec = {}
+    lam0 = None
+    for label, snaps in (("full_storage", steps), ("binomial", budget)):
+        s.set_state(u0)
+        t_rec, _ = sync_time(lambda: s.adjoint_forward(dt, steps, snaps))
+        J, dJ = ke(s.get_state())
+        t_bwd, lam = sync_time(lambda: s.adjoint_backward(dJ))
+        if lam0 is None:
+            lam0 = lam
+        rec[label] = {"snapshots": snaps, "checkpointed_forward_s": t_rec, "reverse_s": t_bwd,
+                      "adjoint_over_forward": (t_rec + t_bwd) / t_fwd, "identical_to_full_storage":
+                      bool(np.array_equal(lam, lam0))}
     # one finite-difference check along a seeded random smooth v perturbation with the L2 norm of
     # the initial v field (Fourier modes 0 <= kx, ky <= 3), central step 1e-6 of that norm
     rng = np.random.default_rng(20261202)
@@ -365,153 +453,265 @@ def adjoint_leg(args, n=64, order=3, steps=400, cfl=0.1):
     s.close()
     out = {"workload": f"2-D NS double shear layer, {n}^2 quads, P{order}, Roe + BR2, Re 1e4, Ma 0.1, "
                        f"{steps} SSP-RK3 steps, FP64, d(final kinetic energy)/d(initial state)",
-           "forward_s": t_fwd, "checkpointed_forward_s": t_rec, "reverse_s": t_bwd,
-           "adjoint_over_forward": (t_rec + t_bwd) / t_fwd, "reverse_over_forward": t_bwd / t_fwd,
-           "snapshots": steps, "fd_check_rel_err": abs(g_adj - g_fd) / abs(g_fd)}
-    print(f"[bench] adjoint leg: {out}", file=sys.stderr, flush=True)
+           "forward_s": t_fwd, "forward_dof_stage_per_s": mesh.E * mesh.nb * 3 * steps / t_fwd,
+           **rec, "adjoint_over_forward": rec["binomial"]["adjoint_over_forward"],
+           "fd_check_rel_err": abs(g_adj - g_fd) / abs(g_fd)}
+    log(f"adjoint leg: {out}")
     return out
 
 
-def vortex_primitive(xq, eps=5.0):
-    """Shu isentropic vortex on the uniform stream rho = u = v = p = 1 (R = 1): primitive (rho, u,
-    v, T) at points xq (..., 2)."""
-    x, y = xq[..., 0], xq[..., 1]
-    r2 = x * x + y * y
-    f = np.exp(0.5 * (1.0 - r2))
-    T = 1.0 - (GAMMA - 1.0) * eps * eps / (8.0 * GAMMA * math.pi ** 2) * f * f
-    prim = np.empty(xq.shape[:-1] + (4,))
-    prim[..., 0] = T ** (1.0 / (GAMMA - 1.0))
-    prim[..., 1] = 1.0 - eps / (2 * math.pi) * f * y
-    prim[..., 2] = 1.0 + eps / (2 * math.pi) * f * x
-    prim[..., 3] = T
-    return prim
+def c5_adjoint_leg(args, n=64, order=4, steps=None):
+    """Gradient of the time-integrated kinetic-energy dissipation int_0^T eps_K dt = K(0) - K(T)
+    of the 64^3 P4 Taylor-Green vortex (HLLC + BR2, FP64, fixed dt) with respect to the initial
+    velocity amplitude A (u = A sin x cos y cos z, ..., p = p0 + A^2 (...)/16): the checkpointed
+    adjoint of `steps` SSP-RK3 steps with a binomial snapshot budget, against the central finite
+    difference of the objective in A (step 1e-4)."""
+    steps = steps or args.c5_adjoint_steps
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import problems as P
 
+    mesh, prob, _ = tgv_setup(n, order)
+    p0 = 1.0 / (GAMMA * 0.01)
+    s = L.Solver(prob, mesh)
+    A = 1.0
 
-def batch_leg(args, count=48, seed=20261202, steps=100, cfl=0.1):
-    """Batched uneven 2-D Euler: `count` isentropic-vortex problems on periodic [-5,5]^2, per
-    problem n ~ U{8..96} cells per direction, P ~ U{1..4}, each cell a quadrangle or two triangles
-    by Bernoulli(0.5) (all draws from numpy's default generator with the given seed), Lax-Friedrichs,
-    SSP-RK3 with a per-problem CFL-0.1 time step, one persistent launch per stage, whole steps
-    captured in a hipGraph, work-item and grid sizes from the autotuner. Timed in mixed precision
-    (the headline mode of this workload) and in FP64; the rate is DOF.stage/s over all problems."""
-    rng = np.random.default_rng(seed)
-    specs = []
-    for _ in range(count):
-        n = int(rng.integers(8, 97))
-        order = int(rng.integers(1, 5))
-        specs.append((n, order, int(rng.integers(0, 2 ** 62))))
-    meshes = [L.Mesh.split(order, (n, n), (-5.0, -5.0), (5.0, 5.0), 0.5, sd) for n, order, sd in specs]
-    elements = sum(m.E for m in meshes)
-    out = {"workload": f"{count} 2-D Euler isentropic vortices, periodic [-5,5]^2, n ~ U{{8..96}}, P ~ U{{1..4}}, "
-                       f"Bernoulli(0.5) triangle split, Lax-Friedrichs, SSP-RK3 CFL {cfl}, {steps} steps, seed {seed}",
-           "problems": count, "elements": elements}
-    states = {}
-    for mode, prec in (("fp64", L.PREC_FP64), ("mixed", L.PREC_MIXED)):
-        prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=L.FLUX_LAX_FRIEDRICHS,
-                              time_integration=L.RK_SSPRK3, cp=3.5, cv=2.5, precision=prec)
-        b = L.Batch(prob, meshes)
-        prims = [vortex_primitive(m.field("quad_coord").reshape(-1, 2)) for m in meshes]
-        for i, pr in enumerate(prims):
-            b.set_initial_primitive(i, pr)
-        dt = b.compute_dt(cfl)
-        stage_ms = b.tune()
-        b.step(dt, 3)  # graph capture and warm-up
-        for i, pr in enumerate(prims):
-            b.set_initial_primitive(i, pr)
-        b.step(dt, steps)
-        ms = b.last_step_ms()
-        dof = sum(sz // 4 for sz in b.sizes)
-        chunk, _, grid = b.config()
-        out[mode] = {"value": dof * 3 * steps / (ms * 1e-3), "ms_per_step": ms / steps,
-                     "tuned_stage_ms": stage_ms, "grid": grid, "tiles_per_item": chunk.tolist()}
-        out["dof_per_field"] = dof
-        # algorithmic HBM bytes of a stage: read U^i and U^n, write U^{i+1} (FP64 state)
-        out[mode]["state_gb_per_s"] = 96.0 * dof * 3 * steps / (ms * 1e-3) / 1e9
-        if count == 48 and seed == 20261202:  # profiled HBM bytes per stage launch of this workload
-            tb = PROFILED_BATCH_TRAFFIC[mode]
-            out[mode]["profiled_traffic_mb"] = tb / 1e6
-            out[mode]["profiled_hbm_gb_per_s"] = tb * 3 * steps / (ms * 1e-3) / 1e9
-            out[mode]["profiled_hbm_frac"] = out[mode]["profiled_hbm_gb_per_s"] / PEAK_HBM_GBS
-        states[mode] = [b.get_state(i) for i in range(count)]
-        b.close()
-    out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
-    out["mixed_l2_rel_dev_vs_fp64"] = max(group_l2(a.reshape(-1, 4), c.reshape(-1, 4), 2)
-                                          for a, c in zip(states["mixed"], states["fp64"]))
-    print(f"[bench] batch leg: {out}", file=sys.stderr, flush=True)
-    return out
+    def u_of(amp):
+        s.set_initial_primitive(tgv_primitive(mesh, p0, p0, amp))
+        return s.get_state()
 
-
-def function_l2(mesh, a, b):
-    """max over variable groups of the relative L2(domain) norm of the difference of two solutions
-    given as modal coefficients (reference layout E x N_b x NV): quadrature with |J| w at the
-    volume points, exact for the squared degree-P polynomials."""
-    nv = mesh.nv
-    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
-    jw = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
-    qa = np.einsum("qb,ebv->eqv", V, (a - b).reshape(mesh.E, mesh.nb, nv))
-    qb = np.einsum("qb,ebv->eqv", V, b.reshape(mesh.E, mesh.nb, nv))
-    out = []
-    for g in ([0], list(range(1, mesh.dim + 1)), [mesh.dim + 1]):
-        num = float((jw[..., None] * qa[..., g] ** 2).sum())
-        den = float((jw[..., None] * qb[..., g] ** 2).sum())
-        out.append(np.sqrt(num / den))
-    return max(out)
-
-
-def group_l2(a, b, dim):
-    """max over variable groups (density, momentum vector, energy) of ||a - b||_2 / ||b||_2."""
-    nv = dim + 2
-    a2, b2 = a.reshape(-1, nv), b.reshape(-1, nv)
-    return max(float(np.linalg.norm(a2[:, g] - b2[:, g]) / np.linalg.norm(b2[:, g]))
-               for g in ([0], list(range(1, dim + 1)), [dim + 1]))
-
-
-def timed_run(mode, pd, mesh, prim, args):
-    solver = L.Solver(pd, mesh)
-    solver.set_initial_primitive(prim)
-    dt = solver.compute_dt(args.cfl)
-    rel, nan, _ = solver.step(dt, args.warmup)
-    solver.synchronize()
+    u0 = u_of(A)
+    dt = s.compute_dt(args.cfl)
+    K0, dK0 = P.kinetic_energy(mesh, u0)
+    s.synchronize()
     t0 = time.perf_counter()
-    rel, nan, taken = solver.step(dt, args.steps)
-    solver.synchronize()
-    wall = time.perf_counter() - t0
-    if nan or taken != args.steps or not np.all(np.isfinite(rel)):
-        raise RuntimeError(f"bench: {mode} run did not complete cleanly (nan={nan}, taken={taken})")
-    value = mesh.E * mesh.nb * 3 * args.steps / wall
-    print(f"[bench] {mode}: dt={dt:.4e} {args.steps} steps in {wall:.3f}s ({1000 * wall / args.steps:.3f} ms/step, "
-          f"{value:.4e} DOF.stage/s), monitor {rel.tolist()}", file=sys.stderr, flush=True)
-    state = solver.get_state()
-    # per-kernel averages with HIP events on the solver stream (advances the state further)
-    kms = solver.time_kernels(dt, max(2, min(args.steps, 10)))
-    solver.close()
-    return {"value": value, "ms_per_step": 1000.0 * wall / args.steps, "state": state, "kms": kms}
+    s.step(dt, steps)
+    s.synchronize()
+    t_fwd = time.perf_counter() - t0
+    s.set_state(u0)
+    snaps = max(2, min(steps, 8))
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.adjoint_forward(dt, steps, snaps)
+    s.synchronize()
+    t_rec = time.perf_counter() - t0
+    KT, dKT = P.kinetic_energy(mesh, s.get_state())
+    t0 = time.perf_counter()
+    lam0 = s.adjoint_backward(-dKT)
+    t_bwd = time.perf_counter() - t0
+    # dJ/dA = <dK(0)/dU0 + lambda_0, dU0/dA>, dU0/dA by central differences of the projection
+    h = 1e-4
+    du0 = (u_of(A + h) - u_of(A - h)) / (2 * h)
+    g_adj = float(((dK0 + lam0) * du0).sum())
+    eps = 1e-3
+    js = []
+    for sign in (1.0, -1.0):
+        ua = u_of(A + sign * eps)
+        ka = P.kinetic_energy(mesh, ua, gradient=False)[0]
+        s.step(dt, steps)
+        js.append(ka - P.kinetic_energy(mesh, s.get_state(), gradient=False)[0])
+    g_fd = (js[0] - js[1]) / (2 * eps)
+    s.close()
+    out = {"workload": f"3-D NS Taylor-Green vortex {n}^3 hex P{order}, HLLC + BR2, FP64, {steps} SSP-RK3 steps, "
+                       f"d(K(0) - K(T))/d(velocity amplitude)", "steps": steps, "snapshots": snaps,
+           "forward_s": t_fwd, "checkpointed_forward_s": t_rec, "reverse_s": t_bwd,
+           "adjoint_over_forward": (t_rec + t_bwd) / t_fwd, "dissipation": K0 - KT, "gradient": g_adj,
+           "gradient_fd": g_fd, "fd_check_rel_err": abs(g_adj - g_fd) / abs(g_fd)}
+    log(f"C5 adjoint leg: {out}")
+    return out
 
 
-def roofline(kms, f_grad, f_res, mesh, args, mode):
-    """Roofline entry of the dominant kernel (kms: HIP-event averages of the gradient, face-flux,
-    residual and monitor kernels). Sum-factorised path: HBM-bound, algorithmic bytes per launch /
-    duration. Dense matrix-core path: algorithmic flops per launch / duration."""
-    if tensor_path(mesh):
-        per = dict(zip(("nsGradTensor", "nsFaceFluxKernel", "nsResidualTensor"),
-                       tensor_bytes(mesh, 8 if mode == "fp64" else 4)))
-        names = list(per)
-        k = int(np.argmax(kms[:3]))
-        name, ms = names[k], kms[k]
-        achieved = per[name] * mesh.E / (ms * 1e-3) / 1e9
-        return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM_GBS, "traffic": PROFILED_TRAFFIC.get((name, mode, args.n, args.order)),
-                "kernel": name, "algorithmic_bytes_per_element": per[name], "kernel_ms": ms,
-                "kernel_ms_all": kms.tolist(), "kernel_ms_names": names + ["monitor"]}
-    peak = PEAK_FP64_TFLOPS if mode == "fp64" else PEAK_FP32_MFMA_TFLOPS
-    if kms[0] >= kms[1] + kms[2]:
-        name, ms, flops = "nsGradMfma", kms[0], f_grad
-    else:
-        name, ms, flops = "nsFaceFluxKernel+nsResidualMfma", kms[1] + kms[2], f_res
-    achieved = flops * mesh.E / (ms * 1e-3) / 1e12
-    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": PROFILED_TRAFFIC.get((name, mode, args.n, args.order)), "kernel": name,
-            "kernel_ms": ms, "kernel_ms_all": kms.tolist()}
+def shock_leg(args, n=256, order=3, steps=100, cfl=0.1):
+    """Artificial-viscosity cost on the explosion case (2-D Euler, [0,2]^2 with adiabatic slip
+    walls, a circle of radius 0.4 at rho = 1, p = 1 in rho = 0.125, p = 0.1 spread over half a
+    cell, 256^2 quadrangles P3, HLLC, SSP-RK3, setArtificialViscosity(4.0)): the step with the
+    per-step sensor and the gradient passes (vector kernels) against the plain Euler step on the
+    same mesh and state."""
+    slip = L.BC_ADIABATIC_SLIP_WALL
+    mesh = L.Mesh(L.ELEM_QUADRANGLE, order, (n, n), (0.0, 0.0), (2.0, 2.0), boundary=((slip, slip), (slip, slip)))
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 2)
+    r = np.sqrt((xq[..., 0] - 1.0) ** 2 + (xq[..., 1] - 1.0) ** 2)
+    w = 0.5 * (1.0 - np.tanh((r - 0.4) / (1.0 / n)))
+    gas = 2.5 - 25.0 / 14.0
+    prim = np.zeros((mesh.E, mesh.nq, 4))
+    prim[..., 0] = 0.125 + 0.875 * w
+    prim[..., 3] = (0.1 + 0.9 * w) / (prim[..., 0] * gas)
+    out = {"workload": f"2-D Euler explosion, [0,2]^2 slip walls, {n}^2 quads, P{order}, HLLC, SSP-RK3, CFL {cfl}, "
+                       f"artificial viscosity (tolerance 4.0, factor 1)", "elements": mesh.E, "steps": steps}
+    for label, sc in (("artificial_viscosity", L.SHOCK_ARTIFICIAL_VISCOSITY), ("plain", L.SHOCK_NONE)):
+        prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=L.FLUX_HLLC, cp=2.5, cv=25.0 / 14.0,
+                              shock_capturing=sc, empirical_tolerance=4.0, artificial_viscosity_factor=1.0)
+        s = L.Solver(prob, mesh)
+        s.set_initial_primitive(prim)
+        s.set_boundary_primitive(np.zeros((mesh.Fb, mesh.nfq, 4)))
+        dt = s.compute_dt(cfl)
+        s.step(dt, 5)
+        kms = s.time_kernels(dt, 5)
+        ms = timed_steps(s, dt, steps, runs=3)
+        med = statistics.median(ms)
+        rec = {"ms_per_step": med, "dof_stage_per_s": mesh.E * mesh.nb * 3 / (med * 1e-3),
+               "kernel_ms": [float(x) for x in kms]}
+        if sc:
+            av = s.artificial_viscosity()
+            rec["nodes_with_viscosity"] = int((av > 0).sum())
+            rec["nodes"] = int(av.size)
+        s.close()
+        out[label] = rec
+    out["cost_over_plain"] = out["artificial_viscosity"]["ms_per_step"] / out["plain"]["ms_per_step"]
+    log(f"shock leg: {out}")
+    return out
+
+
+def host_cpu():
+    """The host's CPU as lscpu reports it (model, sockets, cores, threads per core) and nproc."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(n: int, order: int, target_s: float = 12.0):
+    """The CPU oracle (C++/OpenMP restatement of the reference, the reference's per-element dense
+    pass structure) on the same workload: the 32^3 hexahedron P3 Taylor-Green mesh, FP64, timed over
+    a bounded number of steps with T = physical cores - 1 threads (the reference's rule) and with
+    all cores. Physical cores from lscpu (cores per socket x sockets)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # checker library, used here only as the timed CPU baseline
+
+    host = host_cpu()
+    try:
+        physical = int(host["Core(s) per socket"]) * int(host["Socket(s)"])
+    except (KeyError, ValueError):
+        physical = os.cpu_count() or 1
+    mesh, prob, prim = tgv_setup(n, order)
+    o = O.Oracle(prob, mesh)
+    o.set_initial_primitive(prim)
+    dt = o.compute_dt(0.1)
+    runs = {}
+    for label, threads in (("cores_minus_1", max(1, physical - 1)), ("all_cores", physical)):
+        O.set_threads(threads)
+        t0 = time.perf_counter()
+        o.step(dt, 1)
+        one = time.perf_counter() - t0
+        steps = max(1, min(50, int(target_s / max(one, 1e-6))))
+        t0 = time.perf_counter()
+        o.step(dt, steps)
+        t = time.perf_counter() - t0
+        runs[label] = {"value": mesh.E * mesh.nb * 3 * steps / t, "threads": O.get_threads(), "steps": steps,
+                       "seconds": t}
+        log(f"cpu baseline {label}: {runs[label]}")
+    main = runs["cores_minus_1"]
+    return {"value": main["value"], "unit": "DOF.stage/s", "cores": main["threads"], "kind": "port",
+            "sample": f"oracle, TGV {n}^3 hex P{order} HLLC+BR2 SSP-RK3 FP64, {main['steps']} steps, "
+                      f"{main['seconds']:.1f} s, T = physical cores - 1",
+            "all_cores": runs["all_cores"], "host": host}
+
+
+def copy_probe(mib: int, reps: int = 5):
+    """Calibration workload for the HBM byte counters: an elementwise copy of `mib` MiB (read mib,
+    write mib per launch), run under `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` beside the solver
+    so that the counters' scale can be read off a kernel of known traffic."""
+    import torch
+
+    nel = mib * (1 << 20) // 8
+    src = torch.rand(nel, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "reps": reps, "bytes_read": nel * 8, "bytes_written": nel * 8}))
+
+
+LEGS = {"c1": c1_leg, "c5": c5_leg, "batch": batch_leg, "adjoint": adjoint_leg, "c5_adjoint": c5_adjoint_leg,
+        "shock": shock_leg}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50, help="timed steps per run (5 runs, median)")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=32, help="elements per direction")
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--cfl", type=float, default=0.1)
+    ap.add_argument("--c5-steps", type=int, default=10, help="timed steps per run of the 64^3 P4 leg")
+    ap.add_argument("--c5-adjoint-steps", type=int, default=10, help="steps of the 64^3 P4 adjoint leg")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip", default="", help="comma-separated legs to skip: " + ",".join(LEGS))
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run the HBM-counter calibration copy of this many MiB")
+    ap.add_argument("--profile-leg", choices=["headline"] + list(LEGS), default=None,
+                    help="only run one leg (for kernel profiling) and print its record")
+    args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+        return
+    if args.gpus != 1:
+        print("bench.py: runs on one GPU only; sharding the element partition over several GPUs "
+              "(face-trace halo exchange per stage) is not implemented", file=sys.stderr)
+        sys.exit(2)
+    if args.profile_leg and args.profile_leg != "headline":
+        print(json.dumps(LEGS[args.profile_leg](args)), flush=True)
+        return
+
+    t_setup = time.perf_counter()
+    mesh, prob, prim = tgv_setup(args.n, args.order)
+    rec = tgv_modes(mesh, prob, prim, args, "C2")
+    log(f"C2 runs {time.perf_counter() - t_setup:.1f}s")
+    head = rec["mixed"]
+    roof = {mode: roofline(rec[mode]["kms"], mesh, mode, args.n) for mode in ("fp64", "mixed")}
+    if args.profile_leg == "headline":
+        return
+    skip = set(filter(None, args.skip.split(",")))
+    legs = {}
+    for name, fn in LEGS.items():
+        if name in skip:
+            continue
+        t0 = time.perf_counter()
+        try:
+            legs[name] = fn(args)
+        except L.SDGError as exc:  # a leg the engine refuses is reported, not fatal
+            legs[name] = {"error": str(exc)}
+        log(f"leg {name}: {time.perf_counter() - t0:.1f}s")
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.n, args.order)
+    line = {
+        "metric": "DOF.stage/s mixed vs FP64 at fixed L2 error (3-D NS Taylor-Green P3)",
+        "value": head["value"],
+        "unit": "DOF.stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 contractions and work buffers (sum-factorised line passes), f64 state/geometry/"
+                 "physics/RK",
+        "data": "synthetic",
+        "config": {"workload": f"3-D compressible NS Taylor-Green vortex, periodic [0,2pi]^3, {args.n}^3 hex, "
+                               f"P{args.order}, Re 1600, Ma 0.1, HLLC + BR2, SSP-RK3, CFL {args.cfl}",
+                   "elements": mesh.E, "basis": mesh.nb, "dof_per_field": mesh.E * mesh.nb, "precision": "mixed",
+                   "parallelism": "single GPU", "runs": RUNS, "statistic": "median"},
+        "ms_per_step_runs": head["ms_per_step_runs"],
+        "fp64": {"value": rec["fp64"]["value"], "ms_per_step": rec["fp64"]["ms_per_step"],
+                 "ms_per_step_runs": rec["fp64"]["ms_per_step_runs"], "dtype": "f64"},
+        "mixed_over_fp64": head["value"] / rec["fp64"]["value"],
+        "mixed_l2_rel_dev_vs_fp64": rec["mixed_l2_rel_dev_vs_fp64"],
+        "mixed_l2_rel_dev_definition": "max over density, momentum, energy of ||u_mixed - u_fp64||_L2(domain) / "
+                                       "||u_fp64||_L2(domain), after the warmup and timed steps",
+        "mixed_coefficient_l2_rel_dev_vs_fp64": rec["mixed_coefficient_l2_rel_dev_vs_fp64"],
+        "l2_dev_after_steps": rec["steps_compared"],
+        "roofline": roof["mixed"],
+        "roofline_fp64": roof["fp64"],
+        **legs,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -2,6 +2,7 @@
 // driver: a bounded structured box whose sides carry physical indices, isothermal no-slip walls
 // (BoundaryConditionEnum::IsoThermalNonSlipWall), and the boundary data from
 // BoundaryCondition<SC>::calculatePrimitiveFromCoordinate (wall velocity and temperature).
+// Usage: lid_driven_cavity_2d_cns [cells per direction] [steps] [output directory for the final raw state]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -48,10 +49,16 @@ int main(int argc, char* argv[]) {
   system.setThermodynamicModel<SimulationControl::kThermodynamicModel>(kGamma * cv, cv);
   system.setTransportModel<SimulationControl::kTransportModel>(1.0 / 100.0);
   system.setTimeIntegration(0.1, {0, steps});
+  if (argc > 3) system.setViewConfig(argv[3], "cavity", steps);  // final state: raw/cavity_<steps>.zst
   system.setStepCallback([steps](int i, const std::array<double, 4>& r) {
     if (i % 50 == 0 || i == steps) std::printf("step %d  monitor %.6e %.6e %.6e %.6e\n", i, r[0], r[1], r[2], r[3]);
   });
-  system.synchronize();
-  system.solve();
+  try {
+    system.synchronize();
+    system.solve();
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "%s\n", e.what());
+    return EXIT_FAILURE;
+  }
   return EXIT_SUCCESS;
 }

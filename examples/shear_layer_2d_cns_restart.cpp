@@ -2,7 +2,9 @@
 // files (setViewConfig) and the two file-based starts of the reference:
 //   1. Function initial condition, N steps, a raw file every N/2 steps (raw/shear_<step>.zst);
 //   2. InitialConditionEnum::LastStep from step N/2 to N: must end bit-identical to run 1;
-//   3. InitialConditionEnum::SpecificFile: an order P+1 run started from run 1's final order-P file.
+//   3. InitialConditionEnum::SpecificFile: an order P+1 run started from run 1's final order-P file;
+//   4. LastStep again with no setDeltaTime: the time step comes from run 1's error.txt (the time of
+//      its first step, TimeIntegration.cpp:138-146) and the run must again end bit-identical.
 // Usage: shear_layer_2d_cns_restart <output directory> [cells per direction] [steps]
 #include <cmath>
 #include <cstdio>
@@ -37,7 +39,7 @@ void configure(SubrosaDG::System<SC>& system, int n, int first, int last, double
   system.template setThermodynamicModel<SC::kThermodynamicModel>(kGamma * cv, cv);
   system.template setTransportModel<SC::kTransportModel>(1.0e-4);
   system.setTimeIntegration(0.1, {first, last});
-  system.setDeltaTime(dt);
+  if (dt > 0.0) system.setDeltaTime(dt);
   system.setViewConfig(dir, "shear", (last - first) > 1 ? (last - first) / 2 : 1);
 }
 }  // namespace
@@ -83,7 +85,18 @@ int main(int argc, char* argv[]) {
     third.synchronize();
     third.solve();
     std::printf("specific-file start at order 4: %zu coefficients, 2 steps\n", third.state().size());
-    return diff == 0.0 ? EXIT_SUCCESS : EXIT_FAILURE;
+
+    SubrosaDG::System<Restart> fourth;  // time step from the log of the first run
+    configure(fourth, n, half, steps, 0.0, dir);
+    fourth.setViewConfig(dir, "shear", 1 << 30);
+    fourth.synchronize();
+    fourth.solve();
+    const auto logged = fourth.state();
+    double diff4 = 0.0;
+    for (size_t i = 0; i < reference.size(); i++) diff4 = std::fmax(diff4, std::fabs(logged[i] - reference[i]));
+    std::printf("last-step restart with the logged time step %.6e: max |difference| = %.3e\n", fourth.deltaTime(),
+                diff4);
+    return diff == 0.0 && diff4 == 0.0 ? EXIT_SUCCESS : EXIT_FAILURE;
   } catch (const std::exception& e) {
     std::fprintf(stderr, "%s\n", e.what());
     return EXIT_FAILURE;

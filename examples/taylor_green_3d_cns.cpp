@@ -1,6 +1,7 @@
 // 3-D compressible Navier-Stokes Taylor-Green vortex (Re 1600, Ma 0.1) on the GPU engine through
 // the C++ drop-in driver: the same SimulationControl / System usage as the reference's examples
 // (e.g. examples/periodic_3d_ceuler.cpp), with a structured periodic box instead of a gmsh mesh.
+// Usage: taylor_green_3d_cns [cells per direction] [steps] [output directory for the final raw state]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -40,10 +41,16 @@ int main(int argc, char* argv[]) {
   system.setThermodynamicModel<SimulationControl::kThermodynamicModel>(kGamma * cv, cv);
   system.setTransportModel<SimulationControl::kTransportModel>(1.0 / 1600.0);
   system.setTimeIntegration(0.1, {0, steps});
+  if (argc > 3) system.setViewConfig(argv[3], "tgv", steps);  // final state: raw/tgv_<steps>.zst
   system.setStepCallback([](int i, const std::array<double, 5>& r) {
     std::printf("step %d  monitor %.6e %.6e %.6e %.6e %.6e\n", i, r[0], r[1], r[2], r[3], r[4]);
   });
-  system.synchronize();
-  system.solve();
+  try {
+    system.synchronize();
+    system.solve();
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "%s\n", e.what());
+    return EXIT_FAILURE;
+  }
   return EXIT_SUCCESS;
 }

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SDG_ABI_VERSION 4
+#define SDG_ABI_VERSION 5
 
 /* ---- status codes ---------------------------------------------------------------------- */
 #define SDG_OK 0
@@ -89,6 +89,16 @@ enum sdg_boundary {
   SDG_BC_PERIODIC = 6
 };
 
+/* Kernel family of a single-problem context. AUTO: sum-factorised Navier-Stokes kernels on
+ * hexahedra P2-P4, dense matrix-core kernels on the other matrix-core sizes (quadrangles P3-P5,
+ * hexahedra P2-P4 Euler), vector kernels elsewhere. DENSE keeps the dense matrix-core kernels on
+ * hexahedra; VECTOR runs the vector kernels for every size (FP64 only). All evaluate the same
+ * operator; the choice exists to compare them. */
+enum sdg_kernels { SDG_KERNELS_AUTO = 0, SDG_KERNELS_DENSE = 1, SDG_KERNELS_VECTOR = 2 };
+/* GRAPH: each sdg_step call replays whole steps from captured hipGraphs; EAGER: every kernel is
+ * launched on the stream as it comes (same kernels, same results). */
+enum sdg_launch { SDG_LAUNCH_GRAPH = 0, SDG_LAUNCH_EAGER = 1 };
+
 /* Runtime copy of the reference's compile-time SimulationControl
  * (SimulationControl.cpp:1197-1279) plus the System setters that carry physical constants
  * (SystemControl.cpp:79-109). gamma = 1.4 and Pr = 0.71 are constants, as in
@@ -117,6 +127,10 @@ typedef struct sdg_problem_desc {
   int pad2_;
   double empirical_tolerance;           /* k above                                        */
   double artificial_viscosity_factor;   /* f above (the reference's default is 1)         */
+  int kernels;            /* enum sdg_kernels: which kernel family runs the stages (no    */
+                          /* counterpart in the reference; AUTO unless comparing families)*/
+  int launch;             /* enum sdg_launch: whole steps replayed from hipGraphs or      */
+                          /* launched kernel by kernel (no counterpart in the reference)  */
 } sdg_problem_desc;
 
 /* Structured mesh on an axis-aligned box: the project-side replacement for the gmsh

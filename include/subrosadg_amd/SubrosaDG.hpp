@@ -20,9 +20,12 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <filesystem>
+#include <fstream>
 #include <functional>
 #include <iostream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -239,6 +242,7 @@ struct System {
     iteration_end_ = iteration_range.second;
   }
   void setDeltaTime(const Real delta_time) { delta_time_ = delta_time; }
+  Real deltaTime() const { return delta_time_; }
   // Not in the reference (its SUBROSA_DG_SINGLE_PRECISION switch is uniform FP32): FP32 matrix-core
   // contractions with FP64 state, geometry, pointwise physics and RK accumulation. Call before
   // synchronize().
@@ -269,6 +273,65 @@ struct System {
     check(sdg_mesh_get_info(mesh_, &info), nullptr);
     check(sdg_raw_binary_write(ctx_, rawPath(step).c_str(), info.nodes), ctx_);
   }
+  // The monitor log <output_directory>/error.txt (CommandLine.cpp:70-127, IOControl.cpp:318-333):
+  // a header line, the line of time 0, then one line per step of "|time|monitor...|" with each
+  // value in %.5e centred in 13 columns.
+  static std::string centred(const std::string& v) {
+    const size_t pad = v.size() < 13 ? 13 - v.size() : 0;
+    return std::string(pad / 2, ' ') + v + std::string(pad - pad / 2, ' ');
+  }
+  static std::string errorLine(const Real time, const std::array<Real, SC::kConservedVariableNumber>& err) {
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%.5e", static_cast<double>(time));
+    std::string line = "|" + centred(buf) + "|";
+    for (const Real e : err) {
+      std::snprintf(buf, sizeof(buf), "%.5e", static_cast<double>(e));
+      line += centred(buf) + "|";
+    }
+    return line;
+  }
+  static std::string errorHeader() {
+    const char* names1[] = {"rho", "rho*u", "rho*E"};
+    const char* names2[] = {"rho", "rho*u", "rho*v", "rho*E"};
+    const char* names3[] = {"rho", "rho*u", "rho*v", "rho*w", "rho*E"};
+    const char** names = SC::kDimension == 1 ? names1 : (SC::kDimension == 2 ? names2 : names3);
+    std::string line = "|" + centred("Time") + "|";
+    for (int k = 0; k < SC::kConservedVariableNumber; k++) line += centred(names[k]) + "|";
+    return line;
+  }
+  std::string errorPath() const { return output_directory_ + "/error.txt"; }
+  // Open the log for a run: a fresh file, or for LastStep the existing one cut after the line of
+  // iteration_start (CommandLine.cpp:108-127). Returns the lines kept.
+  std::vector<std::string> openErrorLog() {
+    std::vector<std::string> kept;
+    if (output_directory_.empty()) return kept;
+    std::filesystem::create_directories(output_directory_);
+    if (SC::kInitialCondition == InitialConditionEnum::LastStep) {
+      std::ifstream in(errorPath());
+      std::string line;
+      for (int i = 0; i < iteration_start_ + 2 && std::getline(in, line); i++) kept.push_back(line);
+    } else {
+      kept.push_back(errorHeader());
+      kept.push_back(errorLine(0.0, std::array<Real, SC::kConservedVariableNumber>{}));
+    }
+    std::ofstream out(errorPath(), std::ios::trunc);
+    for (const auto& l : kept) out << l << '\n';
+    return kept;
+  }
+  // Solver::calculateDeltaTime for LastStep (TimeIntegration.cpp:138-146): the time value of the
+  // third line of the log, i.e. the first step's time, is the time step of the earlier run.
+  Real deltaTimeFromErrorLog() const {
+    std::ifstream in(errorPath());
+    std::string line;
+    for (int i = 0; i < 3; i++)
+      if (!std::getline(in, line)) throw std::runtime_error("SubrosaDG: LastStep needs the earlier run's error.txt");
+    std::stringstream ss(line);
+    Real dt = 0.0;
+    ss.ignore(2) >> dt;
+    if (!(dt > 0.0)) throw std::runtime_error("SubrosaDG: no time step in " + errorPath());
+    return dt;
+  }
+
   // Called after every step with the step number and the residual monitor.
   void setStepCallback(std::function<void(int, const std::array<Real, SC::kConservedVariableNumber>&)> f) {
     on_step_ = std::move(f);
@@ -366,17 +429,31 @@ struct System {
     } else {  // LastStep: the raw file of the first iteration (SystemControl.cpp:150-154)
       check(sdg_raw_binary_read(ctx_, rawPath(iteration_start_).c_str(), SDG_IC_LAST_STEP), ctx_);
     }
+    // SystemControl.cpp:163-165 and TimeIntegration.cpp:134-146: a time step set with setDeltaTime
+    // is kept; otherwise LastStep takes the earlier run's from its log and the others compute the
+    // CFL step
+    if (delta_time_ == 0.0) {
+      if (SC::kInitialCondition == InitialConditionEnum::LastStep && !output_directory_.empty())
+        delta_time_ = deltaTimeFromErrorLog();
+      else
+        check(sdg_compute_dt(ctx_, cfl_, &delta_time_), ctx_);
+    }
     updateBoundary(iteration_start_ * delta_time_);
-    // (a restart does not read the reference's error file for dt: set it with setDeltaTime)
-    if (delta_time_ == 0.0) check(sdg_compute_dt(ctx_, cfl_, &delta_time_), ctx_);
     if (SC::kInitialCondition != InitialConditionEnum::LastStep) writeRaw(0);
+    const bool log = !output_directory_.empty();
+    if (log) openErrorLog();
+    std::ofstream error_log;
+    if (log) error_log.open(errorPath(), std::ios::app);
     const int interval = io_interval_ > 0 ? io_interval_ : iteration_end_;
     for (int i = iteration_start_ + 1; i <= iteration_end_; i++) {
-      if (SC::kBoundaryTime == BoundaryTimeEnum::TimeVarying) updateBoundary(i * delta_time_);
+      // stepSolver refreshes time-varying data at iteration_ * delta_time_, and iteration_ is the
+      // previous step's number while step i runs (TimeIntegration.cpp:334-336, SystemControl.cpp:175-177)
+      if (SC::kBoundaryTime == BoundaryTimeEnum::TimeVarying) updateBoundary((i - 1) * delta_time_);
       std::array<Real, NV> rel{};
       int nan = 0, taken = 0;
       check(sdg_step(ctx_, delta_time_, 1, rel.data(), &nan, &taken), ctx_);
       relative_error_history_.push_back(rel);
+      if (log) error_log << errorLine(static_cast<Real>(i) * delta_time_, rel) << '\n' << std::flush;
       if (on_step_) on_step_(i, rel);
       if (nan) {  // all-NaN monitor: truncate the run (SystemControl.cpp:185-191)
         iteration_end_ = i;

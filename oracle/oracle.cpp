@@ -48,9 +48,76 @@
 
 #include "../include/sdg.h"
 
+#ifdef ORACLE_TANGENT
+// Tangent-linear build (liboracle_tl.so): every Real carries a value and a directional derivative
+// (forward mode), so the same passes that evaluate R(U) also evaluate J v = dR/dU v when the state
+// is seeded with derivative v. Branches (flux cases, max / min, upwinding) follow the value, as the
+// reference's FP64 code does; the derivative is that of the branch taken. The interface arrays are
+// pairs (value, derivative) of doubles, i.e. complex128 in numpy.
+struct Tangent {
+  double v = 0.0, d = 0.0;
+  constexpr Tangent() = default;
+  constexpr Tangent(double value) : v(value) {}  // NOLINT: constants and tables enter as values
+  constexpr Tangent(double value, double der) : v(value), d(der) {}
+  Tangent& operator+=(Tangent b) { d += b.d; v += b.v; return *this; }
+  Tangent& operator-=(Tangent b) { d -= b.d; v -= b.v; return *this; }
+  Tangent& operator*=(Tangent b) { d = d * b.v + v * b.d; v *= b.v; return *this; }
+  Tangent& operator/=(Tangent b) { const double q = v / b.v; d = (d - q * b.d) / b.v; v = q; return *this; }
+};
+inline Tangent operator+(Tangent a, Tangent b) { return {a.v + b.v, a.d + b.d}; }
+inline Tangent operator-(Tangent a, Tangent b) { return {a.v - b.v, a.d - b.d}; }
+inline Tangent operator-(Tangent a) { return {-a.v, -a.d}; }
+inline Tangent operator*(Tangent a, Tangent b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+inline Tangent operator/(Tangent a, Tangent b) {
+  const double q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+inline bool operator<(Tangent a, Tangent b) { return a.v < b.v; }
+inline bool operator>(Tangent a, Tangent b) { return a.v > b.v; }
+inline bool operator<=(Tangent a, Tangent b) { return a.v <= b.v; }
+inline bool operator>=(Tangent a, Tangent b) { return a.v >= b.v; }
+inline bool operator==(Tangent a, Tangent b) { return a.v == b.v; }
+inline bool operator!=(Tangent a, Tangent b) { return a.v != b.v; }
+inline Tangent sqrt(Tangent a) {
+  const double s = std::sqrt(a.v);
+  return {s, a.d / (2.0 * s)};
+}
+inline Tangent fabs(Tangent a) { return a.v < 0.0 ? -a : a; }
+inline Tangent exp(Tangent a) {
+  const double e = std::exp(a.v);
+  return {e, e * a.d};
+}
+inline Tangent log(Tangent a) { return {std::log(a.v), a.d / a.v}; }
+inline Tangent log10(Tangent a) { return {std::log10(a.v), a.d / (a.v * 2.302585092994045684)}; }
+inline Tangent sin(Tangent a) { return {std::sin(a.v), std::cos(a.v) * a.d}; }
+inline Tangent pow(Tangent a, Tangent b) {  // a > 0
+  const double p = std::pow(a.v, b.v);
+  return {p, p * (b.v * a.d / a.v + std::log(a.v) * b.d)};
+}
+inline bool isnan(Tangent a) { return std::isnan(a.v); }
+#pragma omp declare reduction(min : Tangent : omp_out = omp_in < omp_out ? omp_in : omp_out) \
+    initializer(omp_priv = omp_orig)
+#define ORACLE_REAL Tangent
+namespace std {
+template <>
+struct numeric_limits<Tangent> {
+  static constexpr Tangent max() noexcept { return Tangent(numeric_limits<double>::max()); }
+};
+}  // namespace std
+#endif
+
 #ifndef ORACLE_REAL
 #define ORACLE_REAL double
 #endif
+
+using std::exp;
+using std::fabs;
+using std::isnan;
+using std::log;
+using std::log10;
+using std::pow;
+using std::sin;
+using std::sqrt;
 
 namespace {
 
@@ -76,12 +143,12 @@ struct Physics {
     return c0 * c0 * (rho - rho0) + padd;
   }
   Real soundSpeed(Real rho, Real p) const {
-    if (eos == SDG_EOS_IDEAL_GAS) return std::sqrt(kGamma * p / rho);
+    if (eos == SDG_EOS_IDEAL_GAS) return sqrt(kGamma * p / rho);
     return c0;
   }
   // TransportModel (PhysicalModel.cpp:77-122)
   Real sutherlandRatio(Real t) const {
-    return std::sqrt(t * t * t) * (1.0 + kSutherland) / (t + kSutherland);
+    return sqrt(t * t * t) * (1.0 + kSutherland) / (t + kSutherland);
   }
   Real viscosity(Real t) const { return transport == SDG_TRANSPORT_SUTHERLAND ? mu * sutherlandRatio(t) : mu; }
   Real conductivity(Real t) const {
@@ -224,7 +291,7 @@ struct Oracle {
       }
       const Real cl = phys.soundSpeed(wl[0], wl[dim + 2]);
       const Real cr = phys.soundSpeed(wr[0], wr[dim + 2]);
-      const Real lam = std::max(std::fabs(unl) + cl, std::fabs(unr) + cr);
+      const Real lam = std::max(fabs(unl) + cl, fabs(unr) + cr);
       for (int v = 0; v < nv; v++) h[v] = ((fl[v] + fr[v]) - lam * (ur[v] - ul[v])) / 2.0;
     } else if (flux == SDG_FLUX_HLLC) {  // :136-238
       const Real rl = wl[0], rr = wr[0], pl = wl[dim + 2], pr = wr[dim + 2];
@@ -237,13 +304,13 @@ struct Oracle {
       const Real rbar = (rl + rr) / 2.0, cbar = (cl + cr) / 2.0;
       const Real ps = std::max(Real(0), (pl + pr) / 2.0 - (unr - unl) * rbar * cbar);
       const Real sl =
-          unl - cl * (ps <= pl ? 1.0 : std::sqrt(1.0 + (kGamma + 1.0) * (ps / pl - 1.0) / 2.0 / kGamma));
+          unl - cl * (ps <= pl ? 1.0 : sqrt(1.0 + (kGamma + 1.0) * (ps / pl - 1.0) / 2.0 / kGamma));
       if (sl >= 0.0) {
         normalFlux(n, wl, h);
         return;
       }
       const Real sr =
-          unr + cr * (ps <= pr ? 1.0 : std::sqrt(1.0 + (kGamma + 1.0) * (ps / pr - 1.0) / 2.0 / kGamma));
+          unr + cr * (ps <= pr ? 1.0 : sqrt(1.0 + (kGamma + 1.0) * (ps / pr - 1.0) / 2.0 / kGamma));
       if (sr <= 0.0) {
         normalFlux(n, wr, h);
         return;
@@ -266,9 +333,9 @@ struct Oracle {
     } else if (flux == SDG_FLUX_ROE) {  // :240-350
       normalFlux(n, wl, fl);
       normalFlux(n, wr, fr);
-      const Real sql = std::sqrt(wl[0]), sqr = std::sqrt(wr[0]);
+      const Real sql = sqrt(wl[0]), sqr = sqrt(wr[0]);
       const Real sqs = sql + sqr;
-      const Real rho = std::sqrt(wl[0] * wr[0]);
+      const Real rho = sqrt(wl[0] * wr[0]);
       Real vel[3];
       for (int d = 0; d < dim; d++) vel[d] = (sql * wl[1 + d] + sqr * wr[1 + d]) / sqs;
       const Real hl = wl[dim + 1] * kGamma + velocitySquared(wl) / 2.0;
@@ -290,20 +357,20 @@ struct Oracle {
         vdv += vel[d] * dw[1 + d];
       }
       const Real delta = c / 20.0;
-      const Real a1 = std::fabs(un - c) > delta ? std::fabs(un - c) : ((un - c) * (un - c) + delta * delta) / (2.0 * delta);
-      const Real a3 = std::fabs(un + c) > delta ? std::fabs(un + c) : ((un + c) * (un + c) + delta * delta) / (2.0 * delta);
+      const Real a1 = fabs(un - c) > delta ? fabs(un - c) : ((un - c) * (un - c) + delta * delta) / (2.0 * delta);
+      const Real a3 = fabs(un + c) > delta ? fabs(un + c) : ((un + c) * (un + c) + delta * delta) / (2.0 * delta);
       const Real dp = dw[dim + 2], drho = dw[0];
       Real col[5][5] = {};
       const Real s0 = a1 * (dp - rho * c * dun) / (2.0 * c * c);
       col[0][0] = 1.0 * s0;
       for (int d = 0; d < dim; d++) col[0][1 + d] = (vel[d] - c * n[d]) * s0;
       col[0][dim + 1] = (htot - c * un) * s0;
-      const Real s1 = std::fabs(un) * (drho - dp / (c * c));
+      const Real s1 = fabs(un) * (drho - dp / (c * c));
       col[1][0] = 1.0 * s1;
       for (int d = 0; d < dim; d++) col[1][1 + d] = vel[d] * s1;
       col[1][dim + 1] = v2 / 2.0 * s1;
       if (dim >= 2) {
-        const Real s2 = std::fabs(un) * rho;
+        const Real s2 = fabs(un) * rho;
         col[2][0] = 0.0 * s2;
         for (int d = 0; d < dim; d++) col[2][1 + d] = (dw[1 + d] - dun * n[d]) * s2;
         col[2][dim + 1] = (vdv - un * dun) * s2;
@@ -324,8 +391,8 @@ struct Oracle {
         unl += wl[1 + d] * n[d];
         unr += wr[1 + d] * n[d];
       }
-      const Real rho = std::sqrt(wl[0] * wr[0] * std::exp((unl - unr) / c));
-      const Real un = (unl + unr) / 2.0 + std::log(wl[0] / wr[0]) * c / 2.0;
+      const Real rho = sqrt(wl[0] * wr[0] * exp((unl - unr) / c));
+      const Real un = (unl + unr) / 2.0 + log(wl[0] / wr[0]) * c / 2.0;
       Real we[8];
       const Real* ws = un < 0.0 ? wr : wl;
       const Real uns = un < 0.0 ? unr : unl;
@@ -380,7 +447,7 @@ struct Oracle {
     const Real mach = unl / phys.soundSpeed(wl[0], wl[dim + 2]);
     switch (type) {
       case 0: {  // RiemannFarfield :79-247
-        if (std::fabs(mach) > 1.0) {
+        if (fabs(mach) > 1.0) {
           const Real* src = mach < 0.0 ? wd : wl;  // supersonic inflow / outflow
           for (int i = 0; i < nw; i++) wb[i] = src[i];
           break;
@@ -396,8 +463,8 @@ struct Oracle {
         for (int d = 0; d < dim; d++) unr += ref[1 + d] * n[d];
         for (int d = 0; d < dim; d++) wb[1 + d] = ref[1 + d] + (unb - unr) * n[d];
         const Real c = gm1 * (right_toward - left_toward) / 4.0;
-        const Real entropy = ref[dim + 2] / std::pow(ref[0], kGamma);
-        const Real rho = std::pow(c * c / (kGamma * entropy), 1.0 / gm1);
+        const Real entropy = ref[dim + 2] / pow(ref[0], kGamma);
+        const Real rho = pow(c * c / (kGamma * entropy), 1.0 / gm1);
         const Real p = rho * c * c / kGamma;
         wb[0] = rho;
         wb[dim + 1] = p / (gm1 * rho);
@@ -558,14 +625,14 @@ struct Oracle {
         num += high * (high * jw);
         den += all * (all * jw);
       }
-      const Real s = std::log10(num / den);
+      const Real s = log10(num / den);
       const Real top = av_factor * (inner_radius[e] / static_cast<Real>(order));
       if (s < s0 - av_tol)
         eps[e] = 0.0;
       else if (s > s0 + av_tol)
         eps[e] = top;
       else
-        eps[e] = top * (1.0 + std::sin(Real(3.14159265358979323846) * (s - s0) / (2.0 * av_tol))) / 2.0;
+        eps[e] = top * (1.0 + sin(Real(3.14159265358979323846) * (s - s0) / (2.0 * av_tol))) / 2.0;
     }
     std::fill(node_av.begin(), node_av.end(), Real(0.0));  // maxElementArtificialViscosity, :90-107
     for (int64_t e = 0; e < ne; e++)
@@ -895,7 +962,7 @@ struct Oracle {
       gemmBt(rq.data(), &residual[e * szC()], V.data(), nv, nb, nq, false);
       for (int v = 0; v < nv; v++) {
         Real s = 0.0;
-        for (int q = 0; q < nq; q++) s += std::fabs(rq[v + static_cast<size_t>(nv) * q]);
+        for (int q = 0; q < nq; q++) s += fabs(rq[v + static_cast<size_t>(nv) * q]);
         per[e * nv + v] = s / nq;
       }
     }
@@ -1041,7 +1108,7 @@ struct MixedOracle {
       gemmBt(rq.data(), &residual[coff[e]], p.V.data(), kNv, p.nb, p.nq, false);
       for (int v = 0; v < kNv; v++) {
         Real acc = 0.0;
-        for (int q = 0; q < p.nq; q++) acc += std::fabs(rq[v + static_cast<size_t>(kNv) * q]);
+        for (int q = 0; q < p.nq; q++) acc += fabs(rq[v + static_cast<size_t>(kNv) * q]);
         per[e * kNv + v] = acc / p.nq;
       }
     }
@@ -1279,6 +1346,23 @@ int oracle_get_gradient(void* h, Real* g) {
 // Gradient traces each element uses at its own face points in the viscous face flux of the last
 // residual evaluation (VariableConvertor.cpp:640-689: BR2 the volume gradient plus that face's
 // lift, BR1 the lifted gradient), E x N_f x N_fq x (NV D).
+// Per-face gradient coefficients of the last residual evaluation, E x N_f x (NV D) x N_b (each block
+// column-major, row var*D+dim): BR2 the volume gradient plus that face's lift, BR1 the total
+// gradient -- the blocks RawBinary.cpp:93-110 writes for boundary faces.
+int oracle_get_face_gradient_coefficients(void* h, Real* out) {
+  Oracle* o = static_cast<Oracle*>(h);
+  if (!o->ns()) return -2;
+  const size_t blk = o->szG();
+  for (int64_t e = 0; e < o->ne; e++)
+    for (int s = 0; s < o->nf; s++) {
+      Real* dst = out + (static_cast<size_t>(e) * o->nf + s) * blk;
+      for (size_t i = 0; i < blk; i++)
+        dst[i] = o->viscous == SDG_VISC_BR2
+                     ? o->vg_coeff[e * blk + i] + o->ig_coeff[(static_cast<size_t>(e) * o->nig + s) * blk + i]
+                     : o->g_coeff[e * blk + i];
+    }
+  return 0;
+}
 int oracle_get_face_gradient(void* h, Real* out) {
   Oracle* o = static_cast<Oracle*>(h);
   if (!o->ns()) return -2;
@@ -1309,7 +1393,7 @@ int oracle_compute_dt(void* h, Real cfl, Real* dt) {  // TimeIntegration.cpp:104
       Real w[8];
       o->computational(&uq[static_cast<size_t>(nv) * q], w);
       const Real c = o->phys.soundSpeed(w[0], w[dim + 2]);
-      const Real sr = std::sqrt(o->velocitySquared(w)) + c;
+      const Real sr = sqrt(o->velocitySquared(w)) + c;
       const Real t = cfl * o->min_edge[e] / (sr * (o->order + 1.0) * (o->order + 1.0));
       best = std::min(best, t);
     }
@@ -1324,7 +1408,7 @@ int oracle_step(void* h, Real dt, int nsteps, Real* rel_err, int* nan_flag, int*
     o->step(dt);
     taken++;
     bool all_nan = true;
-    for (int v = 0; v < o->nv; v++) all_nan = all_nan && std::isnan(o->rel_err[v]);
+    for (int v = 0; v < o->nv; v++) all_nan = all_nan && isnan(o->rel_err[v]);
     if (all_nan) {  // SystemControl.cpp:185-191
       nanf = 1;
       break;
@@ -1464,7 +1548,7 @@ int oracle_mixed_compute_dt(void* h, Real cfl, Real* dt) {  // TimeIntegration.c
     for (int q = 0; q < p.nq; q++) {
       Real w[8];
       o->pt.computational(&uq[4 * q], w);
-      const Real sr = std::sqrt(o->pt.velocitySquared(w)) + o->pt.phys.soundSpeed(w[0], w[4]);
+      const Real sr = sqrt(o->pt.velocitySquared(w)) + o->pt.phys.soundSpeed(w[0], w[4]);
       best = std::min(best, cfl * o->min_edge[e] / (sr * (order + 1.0) * (order + 1.0)));
     }
   }

@@ -10,6 +10,11 @@ parity test compares the two implementations of the hot path on identical inputs
 FP64 oracle is the FP64 rounding floor of a case: a Navier-Stokes residual that is the small
 difference of large volume and face terms (e.g. the mass residual of a divergence-free
 Taylor-Green state) cannot be matched between two FP64 evaluations more closely than that floor.
+
+`Oracle(..., tangent=True)` runs the same source built with a forward-mode number type
+(liboracle_tl.so): each value carries a directional derivative, so `residual_tangent(U, v)` gives
+R(U) and the tangent-linear J v = (dR/dU) v, and `step_tangent` the same through whole RK steps.
+It is the CPU side of the adjoint's dot-product test <J v, w> = <v, J^T w>.
 """
 from __future__ import annotations
 
@@ -23,12 +28,15 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 LIB_LD_PATH = os.path.join(HERE, "liboracle_ld.so")
+LIB_TL_PATH = os.path.join(HERE, "liboracle_tl.so")
 SRC_PATH = os.path.join(HERE, "oracle.cpp")
 
 
 def build(force: bool = False) -> str:
-    """Compile oracle.cpp into liboracle.so (FP64) and liboracle_ld.so (x87 extended) with g++."""
-    for path, extra in ((LIB_PATH, []), (LIB_LD_PATH, ["-DORACLE_REAL=long double"])):
+    """Compile oracle.cpp into liboracle.so (FP64), liboracle_ld.so (x87 extended) and
+    liboracle_tl.so (FP64 tangent-linear) with g++."""
+    for path, extra in ((LIB_PATH, []), (LIB_LD_PATH, ["-DORACLE_REAL=long double"]),
+                        (LIB_TL_PATH, ["-DORACLE_TANGENT"])):
         if force or not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(SRC_PATH):
             cmd = ["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fopenmp", "-fPIC", "-shared", *extra,
                    "-o", path, SRC_PATH]
@@ -39,13 +47,19 @@ def build(force: bool = False) -> str:
 _libs = {}
 
 
-def load(extended: bool = False):
-    lib = _libs.get(extended)
+class _Tangent(ctypes.Structure):
+    """A scalar of the tangent-linear build passed by value: (value, derivative)."""
+    _fields_ = [("v", c_double), ("d", c_double)]
+
+
+def load(extended: bool = False, tangent: bool = False):
+    key = "tangent" if tangent else extended
+    lib = _libs.get(key)
     if lib is None:
-        path = LIB_LD_PATH if extended else LIB_PATH
+        path = LIB_TL_PATH if tangent else (LIB_LD_PATH if extended else LIB_PATH)
         if not os.path.exists(path):
             build()
-        real = c_longdouble if extended else c_double
+        real = _Tangent if tangent else (c_longdouble if extended else c_double)
         lib = ctypes.CDLL(path)
         lib.oracle_create.restype = c_void_p
         lib.oracle_create.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int64, c_int64] + \
@@ -57,6 +71,8 @@ def load(extended: bool = False):
             getattr(lib, name).restype = c_int
         lib.oracle_get_face_gradient.argtypes = [c_void_p, c_void_p]
         lib.oracle_get_face_gradient.restype = c_int
+        lib.oracle_get_face_gradient_coefficients.argtypes = [c_void_p, c_void_p]
+        lib.oracle_get_face_gradient_coefficients.restype = c_int
         lib.oracle_monitor.argtypes = [c_void_p, c_void_p, c_void_p]
         lib.oracle_monitor.restype = c_int
         lib.oracle_compute_dt.argtypes = [c_void_p, real, POINTER(real)]
@@ -70,7 +86,7 @@ def load(extended: bool = False):
                                                         c_void_p, real, real]
         lib.oracle_set_threads.argtypes = [c_int]
         lib.oracle_get_threads.restype = c_int
-        _libs[extended] = lib
+        _libs[key] = lib
     return lib
 
 
@@ -86,10 +102,11 @@ class Oracle:
                "jacobian_tiw", "mass_inverse", "min_edge", "face_normal", "face_jacobian_w",
                "boundary_face_normal", "boundary_face_jacobian_w")
 
-    def __init__(self, problem, mesh, extended: bool = False):
+    def __init__(self, problem, mesh, extended: bool = False, tangent: bool = False):
         self.extended = extended
-        self.real = np.longdouble if extended else np.float64
-        self.lib = load(extended)
+        self.tangent = tangent
+        self.real = np.complex128 if tangent else (np.longdouble if extended else np.float64)
+        self.lib = load(extended, tangent)
         self.mesh = mesh
         self.dim, self.nv, self.nb, self.nq = mesh.dim, mesh.nv, mesh.nb, mesh.nq
         self.E = mesh.E
@@ -181,6 +198,14 @@ class Oracle:
             raise RuntimeError("face gradients are only defined for Navier-Stokes")
         return out.astype(np.float64)
 
+    def face_gradient_coefficients(self) -> np.ndarray:
+        """Per-face gradient coefficients of the last residual, (E, N_f, N_b, NV D): BR2 the volume
+        gradient plus that face's lift, BR1 the total gradient."""
+        out = self._out((self.E, self.mesh.nf, self.nb, self.nv * self.dim))
+        if self.lib.oracle_get_face_gradient_coefficients(self.h, _ptr(out)) != 0:
+            raise RuntimeError("face gradients are only defined for Navier-Stokes")
+        return out.astype(np.float64)
+
     def monitor(self, residual: np.ndarray) -> np.ndarray:
         """Residual monitor of a given residual (E, N_b, NV); replaces the oracle's residual."""
         r = np.ascontiguousarray(residual, dtype=self.real)
@@ -205,6 +230,25 @@ class Oracle:
         dt = (c_longdouble if self.extended else c_double)()
         self.lib.oracle_compute_dt(self.h, cfl, ctypes.byref(dt))
         return float(dt.value)
+
+    def residual_tangent(self, u: np.ndarray, v: np.ndarray):
+        """(R(u), J v) with J = dR/du at u (tangent build only)."""
+        assert self.tangent
+        self.set_state(np.asarray(u, dtype=np.float64) + 1j * np.asarray(v, dtype=np.float64))
+        out = self._out(self.state_shape())
+        self.lib.oracle_residual(self.h, _ptr(out))
+        return out.real.copy(), out.imag.copy()
+
+    def step_tangent(self, u: np.ndarray, v: np.ndarray, dt: float, nsteps: int):
+        """(U_n, dU_n/du v) after nsteps steps of the fixed size dt from u (tangent build only)."""
+        assert self.tangent
+        self.set_state(np.asarray(u, dtype=np.float64) + 1j * np.asarray(v, dtype=np.float64))
+        rel = self._out(self.nv)
+        nan, taken = c_int(), c_int()
+        self.lib.oracle_step(self.h, _Tangent(dt, 0.0), nsteps, _ptr(rel), ctypes.byref(nan), ctypes.byref(taken))
+        out = self._out(self.state_shape())
+        self.lib.oracle_get_state(self.h, _ptr(out))
+        return out.real.copy(), out.imag.copy()
 
     def step(self, dt: float, nsteps: int = 1):
         rel = self._out(self.nv)

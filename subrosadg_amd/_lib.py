@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsdg_hip.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 # status codes
 OK, E_INVALID, E_UNSUPPORTED, E_DEVICE, E_NOMEM, E_STATE = 0, -1, -2, -3, -4, -5
@@ -31,6 +31,8 @@ TRANSPORT_NONE, TRANSPORT_CONSTANT, TRANSPORT_SUTHERLAND = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 SOURCE_NONE, SOURCE_BOUSSINESQ = 0, 1
 SHOCK_NONE, SHOCK_ARTIFICIAL_VISCOSITY = 0, 1
+KERNELS_AUTO, KERNELS_DENSE, KERNELS_VECTOR = 0, 1, 2
+LAUNCH_GRAPH, LAUNCH_EAGER = 0, 1
 IC_FUNCTION, IC_SPECIFIC_FILE, IC_LAST_STEP = 0, 1, 2
 (BC_RIEMANN_FARFIELD, BC_VELOCITY_INFLOW, BC_PRESSURE_OUTFLOW, BC_ISOTHERMAL_NONSLIP_WALL, BC_ADIABATIC_SLIP_WALL,
  BC_ADIABATIC_NONSLIP_WALL, BC_PERIODIC) = range(7)
@@ -93,6 +95,8 @@ class ProblemDesc(Structure):
         ("pad2_", c_int),
         ("empirical_tolerance", c_double),
         ("artificial_viscosity_factor", c_double),
+        ("kernels", c_int),
+        ("launch", c_int),
     ]
 
 
@@ -251,6 +255,8 @@ def problem_desc(**kw) -> ProblemDesc:
     p.shock_capturing = kw.get("shock_capturing", SHOCK_NONE)
     p.empirical_tolerance = kw.get("empirical_tolerance", 0.0)
     p.artificial_viscosity_factor = kw.get("artificial_viscosity_factor", 1.0)
+    p.kernels = kw.get("kernels", KERNELS_AUTO)
+    p.launch = kw.get("launch", LAUNCH_GRAPH)
     return p
 
 

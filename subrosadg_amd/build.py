@@ -21,7 +21,8 @@ HOST_SOURCES = ["kit.cpp", "sdg_mesh_api.cpp", "sdg_raw.cpp", "sdg_vtu.cpp", "sd
 LINK_LIBS = ["-l:libzstd.so.1"]
 HIP_SOURCES = ["sdg_solver.hip", "sdg_batch.hip"]
 EXAMPLES = ["taylor_green_3d_cns", "lid_driven_cavity_2d_cns", "shear_layer_2d_cns_restart", "thermal_cavity_3d_cns",
-            "explosion_2d_ceuler", "mesh_file_2d_ceuler", "sod_1d_ceuler"]
+            "explosion_2d_ceuler", "mesh_file_2d_ceuler", "sod_1d_ceuler",
+            "pulsed_inflow_2d_ceuler"]
 ARCH = os.environ.get("SDG_OFFLOAD_ARCH", "gfx950")
 
 # Packed-FP32 VALU instructions are disabled for all device code; hipcc's host pass warns that it

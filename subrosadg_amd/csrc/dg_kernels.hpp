@@ -670,6 +670,106 @@ __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageAr
   }
 }
 
+// ---- raw files of Navier-Stokes BR2 runs: the gradient blocks of boundary faces -------------
+// RawBinary.cpp:93-110 writes, for every boundary face, its parent's volume-gradient coefficients
+// plus the BR2 lift of that face, (NV D) x N_b column-major (row var*D+dim). One workgroup per
+// boundary face, FP64, from the state U: the parent's own and its neighbours' traces are formed
+// here from U, the face terms and the lifting follow nsGradKernel:
+//   w_vol = (A Phi_f - Q Phi_grad) Mref^-1 / |J|,   r_s = J_s Phi_f,s Mref^-1 / |J|.
+template <int D, int NB, int NQ, int NF, int NFQ>
+__global__ __launch_bounds__(kThreads) void br2BoundaryBlockKernel(Geo g, Phys ph, const double* U, double* out) {
+  constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
+  __shared__ double sU[NV][NB];
+  __shared__ double sUq[NV][NQ];
+  __shared__ double sA[NG][NAQ];
+  __shared__ double sJ[NG][NFQ];
+  __shared__ double sRv[NG][NB];
+  __shared__ double sRl[NG][NB];
+  const int tid = threadIdx.x;
+  const int bf = blockIdx.x;
+  const int own0 = g.bslot[static_cast<int64_t>(bf) * NFQ];  // parent * N_aq + slot of the face's point 0
+  const int e = own0 / NAQ, s = (own0 % NAQ) / NFQ;
+  for (int idx = tid; idx < NV * NB; idx += kThreads) sU[idx / NB][idx % NB] = U[static_cast<int64_t>(e) * NV * NB + idx];
+  __syncthreads();
+  for (int idx = tid; idx < NV * NQ; idx += kThreads) {
+    const int v = idx / NQ, q = idx % NQ;
+    double acc = 0.0;
+    for (int b = 0; b < NB; b++) acc += sU[v][b] * g.Vt[b * NQ + q];
+    sUq[v][q] = acc;
+  }
+  for (int sl = tid; sl < NAQ; sl += kThreads) {
+    double uo[NV];
+    for (int v = 0; v < NV; v++) {
+      double acc = 0.0;
+      for (int b = 0; b < NB; b++) acc += sU[v][b] * g.Vf[sl * NB + b];
+      uo[v] = acc;
+    }
+    const int64_t es = static_cast<int64_t>(e) * NAQ + sl;
+    const int nbs = g.snb[es];
+    const bool lifted = sl / NFQ == s;
+    if (nbs < 0) {
+      for (int v = 0; v < NV; v++) {
+        double Ab[D], Jb[D];
+        boundaryGradientTerms<D>(ph, g, -1 - nbs, uo, v, 0.0, Ab, Jb);
+        for (int d = 0; d < D; d++) {
+          sA[v * D + d][sl] = Ab[d];
+          if (lifted) sJ[v * D + d][sl % NFQ] = Jb[d];
+        }
+      }
+      continue;
+    }
+    const int fps = g.sfp[es];
+    const int side = fps & 1;
+    const int64_t fp = fps >> 1;
+    const int ne = nbs / NAQ, nsl = nbs % NAQ;
+    double n[D];
+    for (int d = 0; d < D; d++) n[d] = g.fnormal[fp * D + d];
+    const double jw = g.fjw[fp];
+    for (int v = 0; v < NV; v++) {
+      double un = 0.0;
+      for (int b = 0; b < NB; b++) un += U[(static_cast<int64_t>(ne) * NV + v) * NB + b] * g.Vf[nsl * NB + b];
+      const double ul = side == 0 ? uo[v] : un, ur = side == 0 ? un : uo[v];
+      for (int d = 0; d < D; d++) {
+        const double avg = (n[d] * (ul + ur) / 2.0) * jw;
+        sA[v * D + d][sl] = side == 0 ? avg : -avg;
+        if (lifted) sJ[v * D + d][sl % NFQ] = (n[d] * (ur - ul) / 2.0) * jw;
+      }
+    }
+  }
+  __syncthreads();
+  const double dj = g.detj[e];
+  for (int idx = tid; idx < NG * NB; idx += kThreads) {
+    const int r = idx / NB, i = idx % NB;
+    const int k = r / D, aa = r % D;
+    double fa = 0.0, fj = 0.0;
+    for (int sl = 0; sl < NAQ; sl++) fa += sA[r][sl] * g.Vf[sl * NB + i];
+    for (int j = 0; j < NFQ; j++) fj += sJ[r][j] * g.Vf[(s * NFQ + j) * NB + i];
+    double vol = 0.0;
+    for (int q = 0; q < NQ; q++) {
+      const double uk = sUq[k][q];
+      const double jw = dj * g.wq[q];
+      for (int b = 0; b < D; b++) {
+        const double G = g.ginv[static_cast<int64_t>(e) * D * D + aa + D * b] * jw;
+        vol += (uk * G) * g.dV[(q * D + b) * NB + i];
+      }
+    }
+    sRv[r][i] = fa - vol;
+    sRl[r][i] = fj;
+  }
+  __syncthreads();
+  const double idj = 1.0 / dj;
+  for (int idx = tid; idx < NG * NB; idx += kThreads) {
+    const int r = idx / NB, i = idx % NB;
+    double wv = 0.0, wl = 0.0;
+    for (int b = 0; b < NB; b++) {
+      const double m = g.Minv[b * NB + i];
+      wv += sRv[r][b] * m;
+      wl += sRl[r][b] * m;
+    }
+    out[static_cast<int64_t>(bf) * NG * NB + static_cast<int64_t>(i) * NG + r] = wv * idj + wl * idj;
+  }
+}
+
 // ---- Navier-Stokes, kernel 2: face fluxes + residual + update + next traces -------------------
 // calculateAdjacencyQuadrature, calculateResidual, updateBasisFunctionCoefficient
 // (SpatialDiscrete.cpp:633-748,1016-1032; TimeIntegration.cpp:181-198)

@@ -35,6 +35,7 @@ using MonFn = void (*)(hipStream_t, const double*, int, double*, int*);
 using SrcFn = void (*)(hipStream_t, const Geo&, const Phys&, const double*, double*, const double*);
 using AdjFn = void (*)(hipStream_t, const Geo&, const Phys&, const StageArgs&, const sdg::AdjArgs&);
 using AvFn = void (*)(hipStream_t, const Geo&, const Phys&, const double*, double*);
+using BlockFn = void (*)(hipStream_t, const Geo&, const Phys&, const double*, double*);
 
 struct Launchers {
   bool mfma = false;
@@ -55,6 +56,8 @@ struct Launchers {
   StageFn av_ns_grad[2] = {nullptr, nullptr};  // Navier-Stokes with artificial viscosity: BR1, BR2
   StageFn av_ns_flux = nullptr;
   AvFn av_sensor = nullptr;
+  // Navier-Stokes BR2 raw files: per boundary face the parent's volume gradient plus that face's lift
+  BlockFn br2_blocks = nullptr;
   // discrete adjoint (Navier-Stokes, FP64 matrix-core sizes only)
   AdjFn adj_volume[2] = {nullptr, nullptr};  // BR1, BR2
   AdjFn adj_face = nullptr;
@@ -181,6 +184,10 @@ Launchers makeLaunchers() {
     constexpr int NM = NQ > NB ? NQ : NB, EPBS = sdg::kThreads / NM > 0 ? sdg::kThreads / NM : 1;
     hipLaunchKernelGGL((sdg::sourceKernel<D, NB, NQ>), dim3(blocksFor(g.E, EPBS)), dim3(sdg::kThreads), 0, st, g, ph,
                        U, out, Li);
+  };
+  l.br2_blocks = [](hipStream_t st, const Geo& g, const Phys& ph, const double* U, double* out) {
+    hipLaunchKernelGGL((sdg::br2BoundaryBlockKernel<D, NB, NQ, NF, NFQ>), dim3(g.Fb), dim3(sdg::kThreads), 0, st, g, ph,
+                       U, out);
   };
   l.dt = [](hipStream_t st, const Geo& g, const Phys& ph, const double* U, const double* me, double cfl, int order,
             double* out) {
@@ -423,6 +430,7 @@ struct sdg_ctx {
          *Vn = nullptr, *Rsrc = nullptr, *src_li = nullptr;
   double *Rout = nullptr, *Wout = nullptr, *mon = nullptr, *rel = nullptr, *dtbuf = nullptr;
   int* status = nullptr;
+  double* Wbnd = nullptr;  // BR2 raw files: per boundary face the gradient block (allocated on first use)
   // artificial viscosity: node values (written once per step by the sensor kernel)
   bool av = false;
   double* nodeav = nullptr;
@@ -740,6 +748,14 @@ int validateProblem(const sdg_problem_desc* p, std::string& msg) {
     msg = "unknown equation of state";
     return SDG_E_INVALID;
   }
+  if (p->kernels < SDG_KERNELS_AUTO || p->kernels > SDG_KERNELS_VECTOR) {
+    msg = "unknown kernel family";
+    return SDG_E_INVALID;
+  }
+  if (p->launch != SDG_LAUNCH_GRAPH && p->launch != SDG_LAUNCH_EAGER) {
+    msg = "unknown launch mode";
+    return SDG_E_INVALID;
+  }
   if (p->equation == SDG_EQ_COMPRESSIBLE_NS && p->transport != SDG_TRANSPORT_CONSTANT &&
       p->transport != SDG_TRANSPORT_SUTHERLAND) {
     msg = "Navier-Stokes needs the constant or Sutherland transport model";
@@ -802,17 +818,20 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   ctx->type = r.type;
   ctx->E = static_cast<int>(mesh.elements);
   ctx->F = static_cast<int64_t>(mesh.faces.size());
-  const char* fv = std::getenv("SDG_FORCE_VALU");
+  const bool force_vector = problem->kernels == SDG_KERNELS_VECTOR;
   const bool mixed = problem->precision == SDG_PREC_MIXED;
   ctx->mixed = mixed;
-  const char* dense = std::getenv("SDG_DENSE");  // 1: keep the dense matrix-core NS kernels on hexahedra
-  const bool tensor = problem->equation == SDG_EQ_COMPRESSIBLE_NS && !(dense && dense[0] == '1');
+  const bool tensor = problem->equation == SDG_EQ_COMPRESSIBLE_NS && problem->kernels == SDG_KERNELS_AUTO;
   ctx->av = problem->shock_capturing == SDG_SHOCK_ARTIFICIAL_VISCOSITY;
   if (ctx->av && mixed) {
     delete ctx;
     return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: artificial viscosity runs in FP64");
   }
-  if (!selectLaunchers(r.type, r.order, (fv && fv[0] == '1') || ctx->av, mixed, ctx->launch, tensor && !ctx->av)) {
+  if (force_vector && mixed) {
+    delete ctx;
+    return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: the vector kernels run in FP64");
+  }
+  if (!selectLaunchers(r.type, r.order, force_vector || ctx->av, mixed, ctx->launch, tensor && !ctx->av)) {
     delete ctx;
     return fail(nullptr, SDG_E_UNSUPPORTED,
                 mixed ? "sdg_create: the mixed-precision mode needs a matrix-core kernel set (quadrangle P3-P5, "
@@ -843,8 +862,7 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     const int p = r.order - 1;  // the hierarchic basis of order P-1 is a prefix of order P's
     ph.av_low = p == 0 ? 0 : sdg::basisSize(r.type, r.dim, p);
   }
-  const char* ng = std::getenv("SDG_NO_GRAPH");
-  ctx->use_graph = !(ng && ng[0] == '1');
+  ctx->use_graph = problem->launch == SDG_LAUNCH_GRAPH;
 
   int dev_count = 0;
   if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1) {
@@ -862,6 +880,7 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     ctx->err = "sdg_create: stream/event creation failed";
     return bail(SDG_E_DEVICE);
   }
+
   const int D = r.dim, nb = r.nb, nq = r.nq, nf = r.nf, nfq = r.nfq, naq = nf * nfq;
   const int64_t E = mesh.elements, F = ctx->F;
   // operand tables
@@ -1499,15 +1518,25 @@ int sdg_raw_binary_write(sdg_ctx* ctx, const char* path, int64_t node_count) {
   if (ctx->av && node_count != ctx->nnode)
     return fail(ctx, SDG_E_INVALID, "sdg_raw_binary_write: with artificial viscosity node_count must be the mesh's node count");
   const bool ns = ctx->ns();
-  if (ns && ctx->prob.viscous_flux == SDG_VISC_BR2 && ctx->Fb > 0)
-    return fail(ctx, SDG_E_UNSUPPORTED,
-                "sdg_raw_binary_write: the per-face BR2 gradients of boundary faces are not formed by the kernels");
   const size_t E = static_cast<size_t>(ctx->E), nu = static_cast<size_t>(ctx->nv) * ctx->nb,
                ng = ns ? static_cast<size_t>(ctx->ng) * ctx->nb : 0;
   std::vector<double> U(E * nu), G(E * ng);
   int rc = sdg_get_state(ctx, U.data());
   if (rc) return rc;
   if (ns && (rc = sdg_get_gradient(ctx, G.data()))) return rc;
+  // BR2 with boundary faces: each boundary face's block is its parent's volume gradient plus that
+  // face's lift (RawBinary.cpp:99-107), formed by a kernel of its own
+  const bool br2_faces = ns && ctx->prob.viscous_flux == SDG_VISC_BR2 && ctx->Fb > 0;
+  std::vector<double> GB;
+  if (br2_faces) {
+    const size_t nbd = static_cast<size_t>(ctx->Fb) * ng;
+    if (!ctx->Wbnd && allocate(ctx, nbd, &ctx->Wbnd)) return SDG_E_NOMEM;
+    ctx->launch.br2_blocks(ctx->stream, ctx->geo, ctx->phys, ctx->U, ctx->Wbnd);
+    HIP_TRY(ctx, hipGetLastError());
+    GB.resize(nbd);
+    HIP_TRY(ctx, hipMemcpyAsync(GB.data(), ctx->Wbnd, nbd * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
   std::vector<double> payload;
   payload.reserve((E + ctx->h_bparent.size()) * (nu + ng) + static_cast<size_t>(node_count));
   auto block = [&](size_t e) {
@@ -1517,8 +1546,18 @@ int sdg_raw_binary_write(sdg_ctx* ctx, const char* path, int64_t node_count) {
       payload.insert(payload.end(), G.begin() + static_cast<std::ptrdiff_t>(e * ng),
                      G.begin() + static_cast<std::ptrdiff_t>((e + 1) * ng));
   };
-  for (size_t e = 0; e < E; e++) block(e);                   // RawBinary.cpp:76-90
-  for (int p : ctx->h_bparent) block(static_cast<size_t>(p));  // RawBinary.cpp:95-123, BR1 / Euler
+  for (size_t e = 0; e < E; e++) block(e);  // RawBinary.cpp:76-90
+  for (size_t bf = 0; bf < ctx->h_bparent.size(); bf++) {  // RawBinary.cpp:95-123
+    const size_t p = static_cast<size_t>(ctx->h_bparent[bf]);
+    if (!br2_faces) {
+      block(p);
+      continue;
+    }
+    payload.insert(payload.end(), U.begin() + static_cast<std::ptrdiff_t>(p * nu),
+                   U.begin() + static_cast<std::ptrdiff_t>((p + 1) * nu));
+    payload.insert(payload.end(), GB.begin() + static_cast<std::ptrdiff_t>(bf * ng),
+                   GB.begin() + static_cast<std::ptrdiff_t>((bf + 1) * ng));
+  }
   payload.resize(payload.size() + static_cast<size_t>(node_count), 0.0);  // node artificial viscosity
   if (ctx->av && node_count > 0 &&
       (rc = sdg_get_artificial_viscosity(ctx, payload.data() + payload.size() - static_cast<size_t>(node_count))))

@@ -161,20 +161,23 @@ def fp64_parity(gpu, oracle_val, extended_val, state, dim, tol, h=None):
     return e, ex, floor
 
 
-def kinetic_energy(mesh, U: np.ndarray):
+def kinetic_energy(mesh, U: np.ndarray, gradient: bool = True):
     """Domain-integrated kinetic energy J = sum_e sum_q |J|w 1/2 |m|^2 / rho of modal coefficients
-    U (E, N_b, NV) and its gradient dJ/dU (same shape)."""
-    D, nv = mesh.dim, mesh.nv
+    U (E, N_b, NV) and its gradient dJ/dU (same shape; None with gradient=False)."""
+    D = mesh.dim
     V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
     jw = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
-    uq = np.einsum("qb,ebv->eqv", V, U)
+    uq = np.matmul(V[None], U.reshape(mesh.E, mesh.nb, mesh.nv))  # (E, N_q, NV), batched products
     rho, m = uq[..., 0], uq[..., 1:1 + D]
     m2 = (m * m).sum(-1)
     J = float((jw * 0.5 * m2 / rho).sum())
+    if not gradient:
+        return J, None
     dj = np.zeros_like(uq)
     dj[..., 0] = -0.5 * m2 / rho ** 2
     dj[..., 1:1 + D] = m / rho[..., None]
-    dJ = np.einsum("eq,qb,eqv->ebv", jw, V, dj)
+    dj *= jw[..., None]
+    dJ = np.matmul(V.T[None], dj)
     return J, dJ
 
 
@@ -258,3 +261,68 @@ def euler_av(element=L.ELEM_QUADRANGLE, n=6, order=3, flux=L.FLUX_HLLC, tol=0.5,
         mesh = L.Mesh(element, order, (n, n), (0.0, 0.0), (1.0, 1.0), boundary=boundary)
     xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, mesh.dim)
     return mesh, prob, sod_primitive(xq)
+
+
+def with_kernels(prob, kernels):
+    """A copy of a problem descriptor that selects a kernel family (L.KERNELS_*)."""
+    pd = L.ProblemDesc.from_buffer_copy(prob)
+    pd.kernels = kernels
+    return pd
+
+
+def split_point_values(mesh, c):
+    """Point values of a split-grid state (per element its N_b x 4 block in element order, each
+    element's own N_b, as the batched solver and the mixed-mesh oracle store it) at the volume
+    quadrature points: a list over the element types present of (|J| w, values), shapes (n_t, N_q)
+    and (n_t, N_q, 4)."""
+    c = np.asarray(c).reshape(-1)
+    et = mesh.field("element_type")
+    jw = mesh.field("jacobian_det_w")
+    out = []
+    refs = {t: L.Mesh.split(mesh.order, (2, 2), (0, 0), (1, 1), 1.0 if t == L.ELEM_TRIANGLE else 0.0, 0)
+            for t in (L.ELEM_TRIANGLE, L.ELEM_QUADRANGLE)}
+    nbs = np.array([refs[int(t)].nb for t in (L.ELEM_TRIANGLE, L.ELEM_QUADRANGLE)])
+    nqs = np.array([refs[int(t)].nq for t in (L.ELEM_TRIANGLE, L.ELEM_QUADRANGLE)])
+    is_quad = (et == L.ELEM_QUADRANGLE).astype(np.int64)
+    coff = np.concatenate([[0], np.cumsum(nbs[is_quad] * 4)])[:-1]
+    qoff = np.concatenate([[0], np.cumsum(nqs[is_quad])])[:-1]
+    for k, t in enumerate((L.ELEM_TRIANGLE, L.ELEM_QUADRANGLE)):
+        sel = np.nonzero(is_quad == k)[0]
+        if sel.size == 0:
+            continue
+        r = refs[t]
+        V = r.field("basis_value").reshape(r.nq, r.nb)
+        C = c[coff[sel][:, None] + np.arange(r.nb * 4)[None, :]].reshape(-1, r.nb, 4)
+        J = jw[qoff[sel][:, None] + np.arange(r.nq)[None, :]]
+        out.append((J, np.einsum("qb,ebv->eqv", V, C)))
+    return out
+
+
+def split_function_l2(mesh, a, b):
+    """max over density, momentum, energy of ||a - b||_L2(domain) / ||b||_L2(domain) for split-grid
+    states."""
+    pa, pb = split_point_values(mesh, a), split_point_values(mesh, b)
+    worst = 0.0
+    for g in ([0], [1, 2], [3]):
+        num = sum(float((J[..., None] * (ua[..., g] - ub[..., g]) ** 2).sum()) for (J, ua), (_, ub) in zip(pa, pb))
+        den = sum(float((J[..., None] * ub[..., g] ** 2).sum()) for (J, ub) in pb)
+        worst = max(worst, np.sqrt(num / den))
+    return worst
+
+
+def split_integrals(mesh, c):
+    """Domain integrals of the four conserved variables of a split-grid state."""
+    return sum((J[..., None] * u).sum(axis=(0, 1)) for J, u in split_point_values(mesh, c))
+
+
+def c3_specs(count=48, seed=20261202):
+    """The batched uneven 2-D Euler workload: per problem n ~ U{8..96} cells per direction, P ~
+    U{1..4} and a split seed, all from numpy's default generator with the given seed (the draws
+    bench.py makes)."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(count):
+        n = int(rng.integers(8, 97))
+        order = int(rng.integers(1, 5))
+        specs.append((n, order, int(rng.integers(0, 2 ** 62))))
+    return specs

@@ -65,6 +65,54 @@ def test_residual_vjp_dot_product(name, make):
     assert abs(adj - fd_cpu) <= 1e-7 * abs(adj), (adj, fd_cpu)
 
 
+@pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
+def test_residual_vjp_tangent_dot_product(name, make):
+    """<J v, w> with J v from the oracle's tangent-linear build (forward-mode numbers through the
+    oracle's passes, no difference quotient) equals <v, J^T w> from the GPU adjoint kernels to
+    rounding: 1e-12 of the sum of the absolute products."""
+    print(f"vjp tangent dot product {name}", flush=True)
+    mesh, prob, prim = make()
+    s = L.Solver(prob, mesh)
+    s.set_initial_primitive(prim)
+    rng = np.random.default_rng(11)
+    u0 = s.get_state()
+    u0 = u0 + 1e-3 * _direction(rng, u0, mesh.dim)
+    s.set_state(u0)
+    w = rng.standard_normal(u0.shape)
+    v = _direction(rng, u0, mesh.dim)
+    jtw = s.vjp_residual(w)
+    _, jv = O.Oracle(prob, mesh, tangent=True).residual_tangent(u0, v)
+    lhs, rhs = float((jv * w).sum()), float((v * jtw).sum())
+    scale = float(np.abs(jv * w).sum() + np.abs(v * jtw).sum()) / 2
+    print(f"  <Jv,w> {lhs:.15e}  <v,J^T w> {rhs:.15e}  rel {abs(lhs - rhs) / abs(lhs):.2e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs, scale)
+
+
+@pytest.mark.parametrize("snapshots", [4, 12])
+def test_adjoint_steps_tangent_dot_product(snapshots):
+    """The same through 12 SSP-RK3 steps of fixed size (shear layer 8^2 quads P3, Roe + BR2):
+    <dU_n v, w> from the oracle's tangent steps equals <v, lambda_0> from the checkpointed GPU
+    adjoint sweep with lambda_n = w, at 1e-12 of the sum of the absolute products."""
+    steps = 12
+    mesh, prob, prim = P.ns_shear(n=8, order=3, perturb=True)
+    s = L.Solver(prob, mesh)
+    s.set_initial_primitive(prim)
+    dt = s.compute_dt(0.1)
+    rng = np.random.default_rng(12)
+    u0 = s.get_state()
+    u0 = u0 + 1e-3 * _direction(rng, u0, mesh.dim)
+    w = rng.standard_normal(u0.shape)
+    v = _direction(rng, u0, mesh.dim)
+    s.set_state(u0)
+    s.adjoint_forward(dt, steps, snapshots)
+    lam0 = s.adjoint_backward(w)
+    _, dun = O.Oracle(prob, mesh, tangent=True).step_tangent(u0, v, dt, steps)
+    lhs, rhs = float((dun * w).sum()), float((v * lam0).sum())
+    scale = float(np.abs(dun * w).sum() + np.abs(v * lam0).sum()) / 2
+    print(f"  <dU v,w> {lhs:.15e}  <v,lam0> {rhs:.15e}  rel {abs(lhs - rhs) / abs(lhs):.2e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs, scale)
+
+
 def _shear_rollout_setup(n, order, steps, cfl=0.1):
     mesh, prob, prim = P.ns_shear(n=n, order=order, flux=L.FLUX_ROE, visc=L.VISC_BR2)
     s = L.Solver(prob, mesh)

@@ -131,3 +131,86 @@ def test_missing_boundary_data_is_an_error():
     s.set_initial_primitive(prim)
     with pytest.raises(L.SDGError):
         s.residual()
+
+
+@pytest.mark.parametrize("eq,element,order", [(L.EQ_EULER, L.ELEM_QUADRANGLE, 3), (L.EQ_NS, L.ELEM_QUADRANGLE, 3),
+                                              (L.EQ_NS, L.ELEM_HEXAHEDRON, 2)],
+                         ids=["euler-quad-p3", "ns-quad-p3", "ns-hex-p2"])
+def test_time_varying_boundary_data(eq, element, order):
+    """BoundaryTimeEnum::TimeVarying (BoundaryCondition.cpp:29-50, TimeIntegration.cpp:334-336):
+    the boundary data are refreshed before every step from a callback of time (here a pulsed
+    inflow velocity and temperature at (i - 1) dt for step i, and a far-field stream turning in
+    time); GPU and oracle take the same data step by step and agree at the FP64 bar after 8 steps,
+    and the result differs from the run with frozen data."""
+    print(f"time-varying boundary {eq} {element} P{order}", flush=True)
+    bc = ((L.BC_VELOCITY_INFLOW, L.BC_PRESSURE_OUTFLOW), (L.BC_RIEMANN_FARFIELD,) * 2)
+    mesh, prob, prim, bprim0 = _setup(eq, element, order, bc, n=3 if element == L.ELEM_HEXAHEDRON else 4)
+    bx = P.boundary_points(mesh)
+
+    def data(t):
+        b = bprim0.copy()
+        b[..., 1] = 0.3 + 0.1 * np.sin(40.0 * t) * np.cos(math.pi * bx[..., 1])
+        b[..., 2] = 0.2 * np.cos(25.0 * t)
+        b[..., -1] += 0.05 * np.sin(30.0 * t)
+        return b
+
+    s = L.Solver(prob, mesh)
+    s.set_initial_primitive(prim)
+    o = O.Oracle(prob, mesh)
+    o.set_state(s.get_state())
+    o.set_boundary_primitive(data(0.0))
+    dt = 2.0 * o.compute_dt(0.1)
+    frozen = L.Solver(prob, mesh)
+    frozen.set_state(s.get_state())
+    frozen.set_boundary_primitive(data(0.0))
+    frozen.step(dt, 8)
+    for i in range(1, 9):
+        b = data((i - 1) * dt)
+        s.set_boundary_primitive(b)
+        o.set_boundary_primitive(b)
+        s.step(dt, 1)
+        o.step(dt, 1)
+    uo = o.get_state()
+    err = P.parity_error(s.get_state(), uo, uo, mesh.dim)
+    moved = np.abs(frozen.get_state() - uo).max() / np.abs(uo).max()
+    print(f"  parity {err:.2e}, change against frozen data {moved:.2e}", flush=True)
+    assert err <= TOL
+    assert moved > 1e-6
+
+
+def test_cpp_driver_time_varying_example(tmp_path):
+    """examples/pulsed_inflow_2d_ceuler.cpp (BoundaryTimeEnum::TimeVarying through the C++ driver):
+    its final raw state equals the oracle driven with the same callback data step by step."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "examples", "bin",
+                       "pulsed_inflow_2d_ceuler")
+    n, steps, dt = 8, 10, 2.0e-3
+    r = subprocess.run([exe, str(tmp_path), str(n), str(steps), repr(dt)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    bc = ((L.BC_VELOCITY_INFLOW, L.BC_PRESSURE_OUTFLOW), (L.BC_RIEMANN_FARFIELD,) * 2)
+    mesh, prob, prim, _ = P.euler_box(n=n, order=3, flux=L.FLUX_ROE, boundary=bc)
+    bx = P.boundary_points(mesh)
+    bn = mesh.field("boundary_face_normal").reshape(mesh.Fb, mesh.nfq, 2)
+
+    def data(t):
+        b = np.tile([1.0, 0.3, 0.2, 1.0], (mesh.Fb, mesh.nfq, 1))
+        inflow, far = bn[..., 0] < -0.5, np.abs(bn[..., 1]) > 0.5
+        b[..., 1] = np.where(inflow, 0.3 + 0.1 * np.sin(40.0 * t) * np.cos(math.pi * bx[..., 1]), b[..., 1])
+        b[..., 2] = np.where(inflow | far, 0.2 * np.cos(25.0 * t), b[..., 2])
+        b[..., 3] = np.where(inflow, 1.0 + 0.05 * np.sin(30.0 * t), b[..., 3])
+        return b
+
+    o = O.Oracle(prob, mesh)
+    o.set_initial_primitive(prim)
+    for i in range(1, steps + 1):
+        o.set_boundary_primitive(data((i - 1) * dt))
+        o.step(dt, 1)
+    uo = o.get_state()
+    payload = np.frombuffer(L.raw_file_read(str(tmp_path / "raw" / f"pulse_{steps}.zst")), dtype=np.float64)
+    got = payload[:uo.size].reshape(uo.shape)
+    err = P.parity_error(got, uo, uo, 2)
+    print(f"  driver vs oracle {err:.2e}", flush=True)
+    assert err <= TOL

@@ -104,3 +104,95 @@ def test_c2_mixed_mode_full_size():
     dev = P.function_l2(mesh, b.get_state(), a.get_state())
     print(f"  mixed L2 deviation {dev:.3e}", flush=True)
     assert dev <= 1e-6
+
+
+def test_c5_mixed_mode_full_size():
+    """64^3 hex P4 Taylor-Green: 10 steps of the mixed mode within 1e-6 (relative L2 over the domain
+    per variable group) of the FP64 engine."""
+    print("C5 mixed vs FP64, 64^3 P4", flush=True)
+    mesh, prob, prim = P.ns_tgv(n=64, order=4)
+    pm = L.ProblemDesc.from_buffer_copy(prob)
+    pm.precision = L.PREC_MIXED
+    a = L.Solver(prob, mesh)
+    a.set_initial_primitive(prim)
+    u0 = a.get_state()
+    dt = a.compute_dt(0.1)
+    a.step(dt, 10)
+    ua = a.get_state()
+    a.close()
+    b = L.Solver(pm, mesh)
+    b.set_state(u0)
+    b.step(dt, 10)
+    ub = b.get_state()
+    b.close()
+    dev = _chunked_l2(mesh, ub, ua)
+    print(f"  mixed L2 deviation {dev:.3e}", flush=True)
+    assert dev <= 1e-6
+
+
+def _chunked_l2(mesh, a, b, chunk=32768):
+    """P.function_l2 in element chunks (the point values of 64^3 P4 are 1.3 GB per array)."""
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    jw = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
+    worst = 0.0
+    for g in ([0], [1, 2, 3], [4]):
+        num = den = 0.0
+        for c0 in range(0, mesh.E, chunk):
+            c1 = min(mesh.E, c0 + chunk)
+            qa = np.einsum("qb,ebv->eqv", V, (a[c0:c1] - b[c0:c1])[..., g])
+            qb = np.einsum("qb,ebv->eqv", V, b[c0:c1][..., g])
+            num += float((jw[c0:c1, :, None] * qa ** 2).sum())
+            den += float((jw[c0:c1, :, None] * qb ** 2).sum())
+        worst = max(worst, np.sqrt(num / den))
+    return worst
+
+
+def _tgv_primitive(mesh, amplitude):
+    cp, cv = P.tgv_thermo(0.1)
+    p0 = cp - cv
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 3)
+    x, y, z = xq[..., 0], xq[..., 1], xq[..., 2]
+    p = p0 + amplitude ** 2 * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0) / 16.0
+    prim = np.zeros((mesh.E, mesh.nq, 5))
+    prim[..., 0] = p / p0
+    prim[..., 1] = amplitude * np.sin(x) * np.cos(y) * np.cos(z)
+    prim[..., 2] = -amplitude * np.cos(x) * np.sin(y) * np.cos(z)
+    prim[..., 4] = 1.0
+    return prim
+
+
+def test_c5_adjoint_amplitude_gradient_full_size():
+    """64^3 hex P4 Taylor-Green (HLLC + BR2, FP64): the gradient of the time-integrated kinetic-energy
+    dissipation K(0) - K(T) over 4 SSP-RK3 steps with respect to the initial velocity amplitude A
+    (u scaled by A, the pressure perturbation by A^2), from the checkpointed adjoint with 2 snapshots
+    (binomial recomputation), equals the central difference of the objective in A (step 1e-3) to
+    1e-5 relative."""
+    print("C5 adjoint, 64^3 P4, 4 steps", flush=True)
+    steps = 4
+    mesh, prob, _ = P.ns_tgv(n=64, order=4)
+    s = L.Solver(prob, mesh)
+
+    def u_of(amp):
+        s.set_initial_primitive(_tgv_primitive(mesh, amp))
+        return s.get_state()
+
+    u0 = u_of(1.0)
+    dt = s.compute_dt(0.1)
+    K0, dK0 = P.kinetic_energy(mesh, u0)
+    s.adjoint_forward(dt, steps, 2)
+    KT, dKT = P.kinetic_energy(mesh, s.get_state())
+    lam0 = s.adjoint_backward(-dKT)
+    h = 1e-4
+    du0 = (u_of(1.0 + h) - u_of(1.0 - h)) / (2 * h)
+    g_adj = float(((dK0 + lam0) * du0).sum())
+    eps = 1e-3
+    js = []
+    for sign in (1.0, -1.0):
+        ua = u_of(1.0 + sign * eps)
+        ka = P.kinetic_energy(mesh, ua, gradient=False)[0]
+        s.step(dt, steps)
+        js.append(ka - P.kinetic_energy(mesh, s.get_state(), gradient=False)[0])
+    g_fd = (js[0] - js[1]) / (2 * eps)
+    rel = abs(g_adj - g_fd) / abs(g_fd)
+    print(f"  dissipation {K0 - KT:.6e}: adjoint {g_adj:.10e} fd {g_fd:.10e} rel {rel:.2e}", flush=True)
+    assert rel <= 1e-5

@@ -169,17 +169,13 @@ def test_nan_stops_stepping():
 
 @pytest.mark.parametrize("make", [lambda: P.ns_tgv(n=3, order=3), lambda: P.ns_shear(n=4, order=3, perturb=True),
                                   lambda: P.euler_vortex(n=6, order=3)])
-def test_matrix_core_and_vector_paths_agree(make, monkeypatch):
+def test_matrix_core_and_vector_paths_agree(make):
     """The MFMA (matrix-core) stage kernels and the VALU kernels give the same residual and steps
-    (hexahedra: the dense matrix-core kernels, SDG_DENSE=1)."""
+    (hexahedra: the dense matrix-core kernels, KERNELS_DENSE)."""
     mesh, prob, prim = make()
-    monkeypatch.setenv("SDG_DENSE", "1")
-    s_mfma = L.Solver(prob, mesh)
-    monkeypatch.delenv("SDG_DENSE")
+    s_mfma = L.Solver(P.with_kernels(prob, L.KERNELS_DENSE), mesh)
     s_mfma.set_initial_primitive(prim)
-    monkeypatch.setenv("SDG_FORCE_VALU", "1")
-    s_valu = L.Solver(prob, mesh)
-    monkeypatch.delenv("SDG_FORCE_VALU")
+    s_valu = L.Solver(P.with_kernels(prob, L.KERNELS_VECTOR), mesh)
     s_valu.set_state(s_mfma.get_state())
     u0 = s_mfma.get_state()
     assert P.parity_error(s_mfma.residual(), s_valu.residual(), u0, mesh.dim) <= TOL
@@ -191,17 +187,15 @@ def test_matrix_core_and_vector_paths_agree(make, monkeypatch):
 
 
 @pytest.mark.parametrize("order,visc", [(2, L.VISC_BR2), (3, L.VISC_BR1), (3, L.VISC_BR2), (4, L.VISC_BR2)])
-def test_sum_factorised_and_dense_hexahedron_kernels_agree(order, visc, monkeypatch):
+def test_sum_factorised_and_dense_hexahedron_kernels_agree(order, visc):
     """Navier-Stokes on hexahedra: the sum-factorised stage kernels (default) and the dense
-    matrix-core kernels (SDG_DENSE=1) evaluate the same operator: each of residual, lifted
+    matrix-core kernels (KERNELS_DENSE) evaluate the same operator: each of residual, lifted
     gradient and state after 3 steps of the sum-factorised path is as close to the extended
     evaluation as the bar or twice the dense path's distance from it."""
     mesh, prob, prim = P.ns_tgv(n=3 if order < 4 else 2, order=order, visc=visc)
     s_sf = L.Solver(prob, mesh)
     s_sf.set_initial_primitive(prim)
-    monkeypatch.setenv("SDG_DENSE", "1")
-    s_dn = L.Solver(prob, mesh)
-    monkeypatch.delenv("SDG_DENSE")
+    s_dn = L.Solver(P.with_kernels(prob, L.KERNELS_DENSE), mesh)
     u0 = s_sf.get_state()
     s_dn.set_state(u0)
     x = _extended(mesh, prob, u0)

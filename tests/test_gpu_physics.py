@@ -22,7 +22,7 @@ def _mesh(kind, order):
     if kind == "quad":
         return L.Mesh(L.ELEM_QUADRANGLE, order, (5, 4), (0.0, 0.0), (1.0, 1.0))
     if kind == "tri":
-        return L.Mesh.split(order, (4, 4), (0.0, 0.0), (1.0, 1.0), 0.5, 7)
+        return L.Mesh.split(order, (4, 4), (0.0, 0.0), (1.0, 1.0), 1.0, 7)
     if kind == "hex":
         return L.Mesh(L.ELEM_HEXAHEDRON, order, (3, 3, 3), (0.0,) * 3, (1.0,) * 3)
     return L.Mesh(L.ELEM_LINE, order, (10,), (0.0,), (1.0,))

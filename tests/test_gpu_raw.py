@@ -5,10 +5,13 @@ an order-P run from an order-(P-1) file (SpecificFile, InitialCondition.cpp:41-8
 import numpy as np
 import pytest
 
+import oracle as O
 import problems as P
 from subrosadg_amd import _lib as L
 
 pytestmark = pytest.mark.gpu
+
+GRAD_TOL = 5e-11
 
 
 def _payload(path):
@@ -108,16 +111,57 @@ def test_boundary_faces_carry_their_parents_blocks(tmp_path):
     assert np.array_equal(blocks[mesh.E:], u.reshape(mesh.E, nu)[par])
 
 
-def test_br2_with_boundary_faces_is_refused(tmp_path):
-    mesh, prob, prim, bprim = P.euler_box(n=3, order=3)
+@pytest.mark.parametrize("element,order", [(L.ELEM_QUADRANGLE, 3), (L.ELEM_QUADRANGLE, 2), (L.ELEM_HEXAHEDRON, 2),
+                                           (L.ELEM_HEXAHEDRON, 3)])
+def test_br2_boundary_face_blocks(tmp_path, element, order):
+    """Navier-Stokes BR2 with boundary faces (far field, no-slip walls): each boundary face's block
+    is its parent's state and the parent's volume gradient plus that face's BR2 lift
+    (RawBinary.cpp:93-110), at the gradient bar against the oracle's per-face coefficients; the
+    element blocks are the state and the lifted total gradient."""
+    print(f"BR2 boundary blocks {element} P{order}", flush=True)
+    dim = 2 if element == L.ELEM_QUADRANGLE else 3
+    wall, far = L.BC_ISOTHERMAL_NONSLIP_WALL, L.BC_RIEMANN_FARFIELD
+    bcs = ((far, far), (wall, wall)) + ((None,) if dim == 3 else ())
+    n = 3
+    mesh = L.Mesh(element, order, (n,) * dim, (0.0,) * dim, (1.0,) * dim, boundary=bcs)
     cp, cv = P.tgv_thermo(0.1)
-    ns = L.problem_desc(equation=L.EQ_NS, convective_flux=L.FLUX_ROE, viscous_flux=L.VISC_BR2,
-                        transport=L.TRANSPORT_CONSTANT, cp=cp, cv=cv, dynamic_viscosity=1e-3)
-    s = L.Solver(ns, mesh)
+    prob = L.problem_desc(equation=L.EQ_NS, convective_flux=L.FLUX_ROE, viscous_flux=L.VISC_BR2,
+                          transport=L.TRANSPORT_CONSTANT, cp=cp, cv=cv, dynamic_viscosity=1e-3)
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, dim)
+    prim = np.zeros((mesh.E, mesh.nq, dim + 2))
+    prim[..., 0] = 1.0 + 0.1 * np.sin(2 * np.pi * xq[..., 0]) * np.cos(2 * np.pi * xq[..., 1])
+    prim[..., 1] = 0.3 + 0.05 * np.cos(2 * np.pi * xq[..., 1])
+    prim[..., 2] = 0.1 * np.sin(2 * np.pi * xq[..., 0])
+    prim[..., -1] = 1.0 + 0.05 * np.cos(2 * np.pi * (xq[..., 0] + xq[..., 1]))
+    bprim = np.zeros((mesh.Fb, mesh.nfq, dim + 2))
+    bprim[..., 0], bprim[..., 1], bprim[..., -1] = 1.0, 0.3, 1.0
+    s = L.Solver(prob, mesh)
     s.set_initial_primitive(prim)
     s.set_boundary_primitive(bprim)
-    with pytest.raises(L.SDGError):
-        s.write_raw(str(tmp_path / "br2.zst"))
+    path = str(tmp_path / "br2.zst")
+    s.write_raw(path)
+    u = s.get_state()
+    o = O.Oracle(prob, mesh)
+    o.set_state(u)
+    o.set_boundary_primitive(bprim)
+    o.residual()
+    fg = o.face_gradient_coefficients()
+    data = _payload(path)
+    nu, ng = mesh.nb * mesh.nv, mesh.nb * mesh.nv * dim
+    bpar = mesh.field("boundary_face_parent").reshape(-1, 2)
+    assert data.size == (mesh.E + mesh.Fb) * (nu + ng)
+    blocks = data.reshape(-1, nu + ng)
+    assert np.array_equal(blocks[: mesh.E, :nu].reshape(u.shape), u)
+    assert np.array_equal(blocks[mesh.E:, :nu], u.reshape(mesh.E, nu)[bpar[:, 0]])
+    got = blocks[mesh.E:, nu:].reshape(mesh.Fb, mesh.nb, mesh.nv * dim)
+    ref = fg[bpar[:, 0], bpar[:, 1]]
+    h = float(mesh.field("min_edge").min())
+    err = P.parity_error(got.reshape(-1, mesh.nv * dim), ref.reshape(-1, mesh.nv * dim), u.reshape(-1, mesh.nv), dim,
+                         h=h)
+    print(f"  boundary gradient blocks vs oracle {err:.2e}", flush=True)
+    assert err <= GRAD_TOL
+    # the blocks differ from the parents' total gradients (the other faces' lifts are not in them)
+    assert np.abs(got - s.gradient()[bpar[:, 0]]).max() > 1e-6 * np.abs(got).max()
 
 
 def test_cpp_driver_restart_example(tmp_path):
@@ -131,5 +175,16 @@ def test_cpp_driver_restart_example(tmp_path):
     r = subprocess.run([exe, str(tmp_path), "8", "10"], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stderr
-    assert "max |difference| = 0.000e+00" in r.stdout
+    assert r.stdout.count("max |difference| = 0.000e+00") == 2
     assert sorted(os.listdir(tmp_path / "raw")) == ["shear_0.zst", "shear_10.zst", "shear_5.zst"]
+    # the monitor log in the reference's format (CommandLine.cpp:70-127): header, time 0, one line
+    # per step; the restart without setDeltaTime took its time step from the time of step 1
+    lines = (tmp_path / "error.txt").read_text().splitlines()
+    assert lines[0] == "|    Time     |     rho     |    rho*u    |    rho*v    |    rho*E    |"
+    assert lines[1] == "|" + "|".join([" 0.00000e+00 "] * 5) + "|"
+    assert len(lines) == 12
+    for k, line in enumerate(lines[2:], start=1):
+        cells = line.split("|")[1:-1]
+        assert len(cells) == 5 and all(len(c) == 13 for c in cells)
+        assert float(cells[0]) == float(f"{k * 2.0e-5:.5e}")
+    assert "logged time step 2.000000e-05" in r.stdout

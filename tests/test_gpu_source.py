@@ -26,13 +26,10 @@ CASES = [("euler-quad-p2-valu", L.ELEM_QUADRANGLE, 2, L.EQ_EULER, False),
 
 
 @pytest.mark.parametrize("name,element,order,eq,dense", CASES, ids=[c[0] for c in CASES])
-def test_source_residual_and_step_parity(name, element, order, eq, dense, monkeypatch):
+def test_source_residual_and_step_parity(name, element, order, eq, dense):
     print(f"source parity {name}", flush=True)
     mesh, prob = _box(element, order, 3 if element == L.ELEM_HEXAHEDRON else 4, eq, beta=0.7, tref=0.98)
-    if dense:
-        monkeypatch.setenv("SDG_DENSE", "1")
-    s = L.Solver(prob, mesh)
-    monkeypatch.delenv("SDG_DENSE", raising=False)
+    s = L.Solver(P.with_kernels(prob, L.KERNELS_DENSE) if dense else prob, mesh)
     s.set_initial_primitive(_wavy_primitive(mesh))
     u0 = s.get_state()
     o = O.Oracle(prob, mesh)
