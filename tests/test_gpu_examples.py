@@ -1,7 +1,8 @@
 """The C++ drop-in driver examples without a test elsewhere (taylor_green_3d_cns, lid_driven_cavity_2d_cns):
-each run writes its final raw state, which equals, bit for bit, the same problem driven through the
-C-ABI from Python (the same library calls in the same order: projection, CFL time step, steps); that
-path is held to the oracle by the parity tests."""
+each run writes its final raw state, which equals the same problem driven through the C-ABI from
+Python (the same library calls: projection, CFL time step, steps) at the FP64 bar; the two sides
+form their constants (p0 = 1/(gamma Ma^2), c_v, c_p) in different orders, so their inputs differ in
+the last bit. The Python path is held to the oracle by the parity tests."""
 import os
 import subprocess
 
@@ -24,10 +25,13 @@ def _run(name, args, out):
     return r.stdout
 
 
-def _raw_state(path, shape):
+def _raw_state(path, shape, dim):
+    """The state blocks of a Navier-Stokes raw file: per element NV x N_b coefficients followed by
+    its (NV D) x N_b gradient coefficients (RawBinary.cpp:76-90)."""
     payload = np.frombuffer(L.raw_file_read(str(path)), dtype=np.float64)
-    n = int(np.prod(shape))
-    return payload[:n].reshape(shape)
+    E, nb, nv = shape
+    nu, ng = nb * nv, nb * nv * dim
+    return payload[:E * (nu + ng)].reshape(E, nu + ng)[:, :nu].reshape(shape)
 
 
 def test_taylor_green_example(tmp_path):
@@ -40,9 +44,10 @@ def test_taylor_green_example(tmp_path):
     dt = s.compute_dt(0.1)
     s.step(dt, steps)
     ref = s.get_state()
-    got = _raw_state(tmp_path / "raw" / f"tgv_{steps}.zst", ref.shape)
-    print(f"  max |driver - python| = {np.abs(got - ref).max():.3e}", flush=True)
-    assert np.array_equal(got, ref)
+    got = _raw_state(tmp_path / "raw" / f"tgv_{steps}.zst", ref.shape, 3)
+    err = P.parity_error(got, ref, ref, 3)
+    print(f"  driver vs python {err:.3e}", flush=True)
+    assert err <= 1e-11
 
 
 def test_lid_driven_cavity_example(tmp_path):
@@ -62,6 +67,7 @@ def test_lid_driven_cavity_example(tmp_path):
     dt = s.compute_dt(0.1)
     s.step(dt, steps)
     ref = s.get_state()
-    got = _raw_state(tmp_path / "raw" / f"cavity_{steps}.zst", ref.shape)
-    print(f"  max |driver - python| = {np.abs(got - ref).max():.3e}", flush=True)
-    assert np.array_equal(got, ref)
+    got = _raw_state(tmp_path / "raw" / f"cavity_{steps}.zst", ref.shape, 2)
+    err = P.parity_error(got, ref, ref, 2)
+    print(f"  driver vs python {err:.3e}", flush=True)
+    assert err <= 1e-11
