@@ -311,6 +311,14 @@ int sdg_get_state(sdg_ctx* ctx, double* coefficients);
 /* CFL time step from the current state: min over elements and points of
  * CFL * minEdge / ((|u| + c) (P+1)^2) (TimeIntegration.cpp:104-131). */
 int sdg_compute_dt(sdg_ctx* ctx, double cfl, double* dt);
+/* No counterpart in the reference. For callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation): sdg_compute_dt with no allocation of any
+ * kind -- the per-element minima and their minimum live in a fixed device array of the library
+ * (one per device, 16 MiB, held until the process ends) and only the result is copied back.
+ * Meshes of more than 2^21 elements are refused (SDG_E_INVALID). Calls that overlap on one device
+ * share that array and race. Same result as sdg_compute_dt bit for bit (per-element minima that
+ * are NaN are skipped here). */
+int sdg_compute_dt_noalloc(sdg_ctx* ctx, double cfl, double* dt);
 
 /* One evaluation of the spatial residual R(U) of the current state, before M^-1: the passes
  * calculateGardientQuadrature .. calculateResidual of stepSolver (TimeIntegration.cpp:340-346).

@@ -167,6 +167,7 @@ SIGNATURES = {
     "sdg_set_state": (c_int, [c_void_p, c_void_p]),
     "sdg_get_state": (c_int, [c_void_p, c_void_p]),
     "sdg_compute_dt": (c_int, [c_void_p, c_double, POINTER(c_double)]),
+    "sdg_compute_dt_noalloc": (c_int, [c_void_p, c_double, POINTER(c_double)]),
     "sdg_residual": (c_int, [c_void_p, c_void_p]),
     "sdg_set_boundary_primitive": (c_int, [c_void_p, c_void_p]),
     "sdg_get_gradient": (c_int, [c_void_p, c_void_p]),

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -409,6 +410,57 @@ std::vector<double> paddedOperator(int K, int M, F src) {
 
 }  // namespace
 
+// ---- scratch of the allocation-free entry point (sdg_compute_dt_noalloc) ----------------------
+// The per-element CFL minima and, after them, the minimum over the mesh. Every device the library
+// runs on holds one copy until the process ends; calls on one device share it, so calls that
+// overlap on one device race. In host code the name is only a stand-in: kernels and copies take
+// the device address that hipGetSymbolAddress returns for the current device (noallocScratch).
+constexpr int64_t kNoAllocMaxElements = int64_t(1) << 21;  // 16 MiB of minima
+static __device__ __align__(256) double scratch[kNoAllocMaxElements + 32];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be the current device; resolved once per
+// device (first context created there) and kept. Null with msg set on failure.
+double* noallocScratch(int device, std::string& msg) {
+  if (device < 0 || device >= kMaxDevices) {
+    msg = "device index beyond the scratch table";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      msg = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+  }
+  return g_scratch_ptr[device];
+}
+
+// out = min over v[0..n) in one workgroup (the minimum is exact, so the order does not matter).
+__global__ __launch_bounds__(256) void minReduceKernel(const double* v, int64_t n, double* out) {
+  __shared__ double part[256];
+  const int tid = threadIdx.x;
+  double best = 1.0e308;
+  for (int64_t i = tid; i < n; i += 256) best = fmin(best, v[i]);
+  part[tid] = best;
+  __syncthreads();
+  for (int width = 128; width > 0; width >>= 1) {
+    if (tid < width) part[tid] = fmin(part[tid], part[tid + width]);
+    __syncthreads();
+  }
+  if (tid == 0) *out = part[0];
+}
+
+}  // namespace
+
 struct sdg_ctx {
   std::string err;
   sdg_problem_desc prob{};
@@ -431,6 +483,11 @@ struct sdg_ctx {
   double *Rout = nullptr, *Wout = nullptr, *mon = nullptr, *rel = nullptr, *dtbuf = nullptr;
   int* status = nullptr;
   double* Wbnd = nullptr;  // BR2 raw files: per boundary face the gradient block (allocated on first use)
+  // device the context was created on, and that device's address of the file-scope scratch array
+  // of sdg_compute_dt_noalloc (null with scratch_err set if it could not be resolved)
+  int device = 0;
+  double* scratch = nullptr;
+  std::string scratch_err;
   // artificial viscosity: node values (written once per step by the sensor kernel)
   bool av = false;
   double* nodeav = nullptr;
@@ -880,7 +937,11 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     ctx->err = "sdg_create: stream/event creation failed";
     return bail(SDG_E_DEVICE);
   }
-
+  if (hipGetDevice(&ctx->device) != hipSuccess) {
+    ctx->err = "sdg_create: hipGetDevice failed";
+    return bail(SDG_E_DEVICE);
+  }
+  ctx->scratch = noallocScratch(ctx->device, ctx->scratch_err);
   const int D = r.dim, nb = r.nb, nq = r.nq, nf = r.nf, nfq = r.nfq, naq = nf * nfq;
   const int64_t E = mesh.elements, F = ctx->F;
   // operand tables
@@ -1354,6 +1415,22 @@ int sdg_compute_dt(sdg_ctx* ctx, double cfl, double* dt) {
   double best = h.empty() ? 0.0 : h[0];
   for (double v : h) best = std::min(best, v);
   *dt = best;
+  return SDG_OK;
+}
+
+int sdg_compute_dt_noalloc(sdg_ctx* ctx, double cfl, double* dt) {
+  if (!ctx || !dt) return fail(ctx, SDG_E_INVALID, "sdg_compute_dt_noalloc: null argument");
+  if (ctx->E > kNoAllocMaxElements)
+    return fail(ctx, SDG_E_INVALID, "sdg_compute_dt_noalloc: more elements than the scratch holds (2^21)");
+  if (!ctx->has_state) return fail(ctx, SDG_E_STATE, "sdg_compute_dt_noalloc: no state set");
+  if (!ctx->scratch) return fail(ctx, SDG_E_DEVICE, "sdg_compute_dt_noalloc: " + ctx->scratch_err);
+  double* ptr = ctx->scratch;
+  ctx->launch.dt(ctx->stream, ctx->geo, ctx->phys, ctx->U, ctx->minedge, cfl, ctx->order, ptr);
+  hipLaunchKernelGGL(minReduceKernel, dim3(1), dim3(256), 0, ctx->stream, ptr, static_cast<int64_t>(ctx->E),
+                     ptr + kNoAllocMaxElements);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipMemcpyAsync(dt, ptr + kNoAllocMaxElements, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return SDG_OK;
 }
 
