@@ -122,16 +122,19 @@ __device__ __forceinline__ void normalFlux(const T* n, const T* w, T* f) {
 }
 
 // numerical convective flux H(uL, uR; n): ConvectiveFlux.cpp:93-439
-template <int D, typename T>
+// FLUX >= 0 fixes the flux at compile time (the other cases are not compiled in); FLUX < 0 takes
+// it from ph.flux.
+template <int D, typename T, int FLUX = -1>
 __device__ void convectiveFlux(const Phys& ph, const T* n, const T* ul, const T* wl, const T* ur, const T* wr, T* h) {
   constexpr int NV = D + 2;
   T fl[NV], fr[NV];
-  if (ph.flux == 0) {  // Central :93-104
+  const int flux = FLUX < 0 ? ph.flux : FLUX;
+  if (flux == 0) {  // Central :93-104
     normalFlux<D>(n, wl, fl);
     normalFlux<D>(n, wr, fr);
 #pragma unroll
     for (int v = 0; v < NV; v++) h[v] = (fl[v] + fr[v]) / T(2.0);
-  } else if (ph.flux == 1) {  // Lax-Friedrichs :106-134
+  } else if (flux == 1) {  // Lax-Friedrichs :106-134
     normalFlux<D>(n, wl, fl);
     normalFlux<D>(n, wr, fr);
     T unl = T(0), unr = T(0);
@@ -145,7 +148,7 @@ __device__ void convectiveFlux(const Phys& ph, const T* n, const T* ul, const T*
     const T lam = fmax(fabs(unl) + cl, fabs(unr) + cr);
 #pragma unroll
     for (int v = 0; v < NV; v++) h[v] = ((fl[v] + fr[v]) - lam * (ur[v] - ul[v])) / T(2.0);
-  } else if (ph.flux == 2) {  // HLLC :136-238
+  } else if (flux == 2) {  // HLLC :136-238
     const T rl = wl[0], rr = wr[0], pl = wl[D + 2], pr = wr[D + 2];
     T unl = T(0), unr = T(0);
 #pragma unroll
@@ -186,7 +189,7 @@ __device__ void convectiveFlux(const Phys& ph, const T* n, const T* ul, const T*
 #pragma unroll
       for (int v = 0; v < NV; v++) h[v] = fr[v] + sr * (us[v] - ur[v]);
     }
-  } else if (ph.flux == 3) {  // Roe with Harten entropy fix :240-350
+  } else if (flux == 3) {  // Roe with Harten entropy fix :240-350
     normalFlux<D>(n, wl, fl);
     normalFlux<D>(n, wr, fr);
     const T sql = sqrt(wl[0]), sqr = sqrt(wr[0]);
