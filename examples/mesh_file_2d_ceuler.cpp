@@ -1,8 +1,10 @@
 // 2-D compressible Euler on a mesh read from a gmsh file through the C++ drop-in driver
-// (setMesh(path), the reference's System::setMesh with an existing .msh, SystemControl.cpp:55-61):
-// physical group 1 is a Riemann far field, group 2 an adiabatic slip wall; a uniform stream along
-// x through the domain must stay uniform (free-stream preservation on any affine mesh), so the
-// residual monitor stays at rounding level.
+// (setMesh(path, generate), the reference's System::setMesh, SystemControl.cpp:55-61): an existing
+// .msh is read; for a path that does not exist the callback generates a triangulation of
+// [0, 2] x [0, 1] with jittered interior nodes in memory. Physical group 1 is a Riemann far field
+// (x = 0 and x = 2), group 2 an adiabatic slip wall (y = 0 and y = 1); a uniform stream along x
+// must stay uniform (free-stream preservation on any affine mesh), so the residual monitor stays
+// at rounding level.
 // Usage: mesh_file_2d_ceuler <mesh.msh> [order: 2 or 3] [steps]
 #include <cmath>
 #include <cstdio>
@@ -52,10 +54,39 @@ std::array<SubrosaDG::Real, 4> SubrosaDG::BoundaryCondition<P3>::calculatePrimit
   return kStream;
 }
 
+// nx x ny cells of two triangles each, interior nodes moved by up to 0.2 of a cell
+void generateMesh(SubrosaDG::ArrayMesh& mesh) {
+  constexpr int nx = 10, ny = 5;
+  mesh.element = SDG_ELEM_TRIANGLE;
+  const auto id = [](int i, int j) { return static_cast<int64_t>(j * (nx + 1) + i); };
+  for (int j = 0; j <= ny; j++)
+    for (int i = 0; i <= nx; i++) {
+      double x = 2.0 * i / nx, y = 1.0 * j / ny;
+      if (i > 0 && i < nx && j > 0 && j < ny) {
+        x += 0.2 * (2.0 / nx) * std::sin(3.1 * i + 1.7 * j);
+        y += 0.2 * (1.0 / ny) * std::cos(2.3 * i - 0.9 * j);
+      }
+      mesh.nodes.push_back({x, y, 0.0});
+    }
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      mesh.elements.insert(mesh.elements.end(), {id(i, j), id(i + 1, j), id(i + 1, j + 1)});
+      mesh.elements.insert(mesh.elements.end(), {id(i, j), id(i + 1, j + 1), id(i, j + 1)});
+    }
+  for (int j = 0; j < ny; j++) {
+    mesh.boundary_faces.insert(mesh.boundary_faces.end(), {id(0, j), id(0, j + 1), id(nx, j), id(nx, j + 1)});
+    mesh.boundary_physical.insert(mesh.boundary_physical.end(), {1, 1});
+  }
+  for (int i = 0; i < nx; i++) {
+    mesh.boundary_faces.insert(mesh.boundary_faces.end(), {id(i, 0), id(i + 1, 0), id(i, ny), id(i + 1, ny)});
+    mesh.boundary_physical.insert(mesh.boundary_physical.end(), {2, 2});
+  }
+}
+
 template <typename SC>
 int run(const std::string& file, int steps) {
   SubrosaDG::System<SC> system;
-  system.setMesh(std::filesystem::path(file));
+  system.setMesh(std::filesystem::path(file), generateMesh);
   system.template addBoundaryCondition<SubrosaDG::BoundaryConditionEnum::RiemannFarfield>(1);
   system.template addBoundaryCondition<SubrosaDG::BoundaryConditionEnum::AdiabaticSlipWall>(2);
   system.template setThermodynamicModel<SC::kThermodynamicModel>(3.5, 2.5);

@@ -285,6 +285,27 @@ int sdg_mesh_create_split(const sdg_split_desc* desc, sdg_mesh** out);
 /* Replaces Mesh<SC>::initializeMesh + readMeshElement (ReadControl.cpp:286-335) for a mesh file:
  * element and face construction (Adjacency.cpp:61-183), geometry, tables as above. */
 int sdg_mesh_create_gmsh(const sdg_gmsh_desc* desc, sdg_mesh** out);
+/* A mesh given as host arrays (borrowed for the call only): what the reference reads out of the gmsh
+ * model that a setMesh(path, generate_mesh_function) callback builds (ReadControl.cpp:247-335) --
+ * node coordinates, the elements' vertex lists, the boundary faces with their types and physical
+ * tags, and translational periodic node pairs (Adjacency.cpp:185-328). Affine elements of one type;
+ * faces, geometry and tables are built as for a mesh file. */
+typedef struct sdg_array_mesh_desc {
+  int element;                      /* SDG_ELEM_TRIANGLE, SDG_ELEM_QUADRANGLE or SDG_ELEM_HEXAHEDRON */
+  int polynomial_order;             /* P = 1..5 (triangles 1..4)                                   */
+  int64_t node_count;
+  const double* nodes;              /* node_count x 3 coordinates (z = 0 in 2-D)                   */
+  int64_t element_count;
+  const int64_t* elements;          /* element_count x vertices, 0-based nodes, gmsh vertex order   */
+  int64_t boundary_face_count;
+  const int64_t* boundary_faces;    /* count x face vertices (2 in 2-D, 4 for hexahedra), any order  */
+  const int* boundary_type;         /* enum sdg_boundary per boundary face                          */
+  const int* boundary_physical;     /* physical tag per boundary face (NULL: -1)                    */
+  int periodic_link_count;          /* translational periodic pairs of boundaries                   */
+  const int64_t* periodic_link_size; /* per link: its number of (slave, master) node pairs          */
+  const int64_t* periodic_pairs;    /* all links' pairs, concatenated, 0-based nodes               */
+} sdg_array_mesh_desc;
+int sdg_mesh_create_from_arrays(const sdg_array_mesh_desc* desc, sdg_mesh** out);
 void sdg_mesh_destroy(sdg_mesh* mesh);
 int sdg_mesh_get_info(const sdg_mesh* mesh, sdg_mesh_info* info);
 /* Number of scalar entries of an exported field (0 for an unknown field). */
@@ -386,13 +407,30 @@ int sdg_raw_file_write(const char* path, const void* payload, int64_t bytes);
 int sdg_raw_file_read(const char* path, void* payload, int64_t capacity, int64_t* bytes);
 
 /* ---- VTU output (host only; no device work) ---------------------------------------------------
- * Replaces View::stepView (Paraview.cpp:280-313): `coefficients` (reference layout, E x N_b x NV,
- * e.g. from sdg_get_state) sampled at the equispaced points of order P of every element and written
- * to `path` as an ASCII VTK XML unstructured grid of the P^D linear sub-cells of each element (P^2
- * triangles), points per element. Point data: Density, Velocity (3 components), Pressure,
- * Temperature, MachNumber, and ArtificialViscosity when node_values (mesh nodes long, e.g. from
- * sdg_get_artificial_viscosity) is not NULL; field data TMSTEP = step and TimeValue = time.
- * The reference writes VTK Lagrange cells of order P; the sub-cells carry the same point values. */
+ * ViewVariableEnum (Enum.cpp:210-233), same order. */
+enum sdg_view_variable {
+  SDG_VIEW_DENSITY = 0, SDG_VIEW_VELOCITY = 1, SDG_VIEW_TEMPERATURE = 2, SDG_VIEW_PRESSURE = 3,
+  SDG_VIEW_SOUND_SPEED = 4, SDG_VIEW_MACH_NUMBER = 5, SDG_VIEW_ENTROPY = 6, SDG_VIEW_VORTICITY = 7,
+  SDG_VIEW_HEAT_FLUX = 8, SDG_VIEW_ARTIFICIAL_VISCOSITY = 9, SDG_VIEW_VELOCITY_X = 10, SDG_VIEW_VELOCITY_Y = 11,
+  SDG_VIEW_VELOCITY_Z = 12, SDG_VIEW_MACH_NUMBER_X = 13, SDG_VIEW_MACH_NUMBER_Y = 14, SDG_VIEW_MACH_NUMBER_Z = 15,
+  SDG_VIEW_VORTICITY_X = 16, SDG_VIEW_VORTICITY_Y = 17, SDG_VIEW_VORTICITY_Z = 18, SDG_VIEW_HEAT_FLUX_X = 19,
+  SDG_VIEW_HEAT_FLUX_Y = 20, SDG_VIEW_HEAT_FLUX_Z = 21
+};
+/* Replaces View::stepView for the element field (Paraview.cpp:154-313, ViewVariable::get
+ * VariableConvertor.cpp:756-882): `coefficients` (reference layout, E x N_b x NV, e.g. from
+ * sdg_get_state or a raw file) and, for Vorticity and HeatFlux, the lifted-gradient coefficients
+ * `gradient` (E x N_b x NV D, as sdg_get_gradient; may be NULL otherwise) are evaluated at each
+ * element's equispaced nodes of order P and written to `path` as an ASCII VTK XML unstructured grid
+ * of one VTK Lagrange cell per element (curve 68, triangle 69, quadrilateral 70, hexahedron 72, nodes
+ * in VTK's order), with the point data `variables[0..variable_count)` named as the reference names
+ * them; in 2-D and 3-D Velocity, MachNumber and HeatFlux (the temperature gradient, as the
+ * reference) are 3-vectors, Vorticity too in 3-D. ArtificialViscosity needs node_values (mesh nodes
+ * long). Field data TMSTEP = step and TimeValue = time. */
+int sdg_write_view(const sdg_mesh* mesh, const sdg_problem_desc* problem, const double* coefficients,
+                   const double* gradient, const double* node_values, const int* variables, int variable_count, int step,
+                   double time, const char* path);
+/* sdg_write_view with Density, Velocity, Pressure, Temperature, MachNumber and, when node_values is not
+ * NULL, ArtificialViscosity. */
 int sdg_write_vtu(const sdg_mesh* mesh, const sdg_problem_desc* problem, const double* coefficients,
                   const double* node_values, int step, double time, const char* path);
 

@@ -131,6 +131,18 @@ struct BoundaryCondition {
 // Structured box (replaces the gmsh generate-mesh callback of the examples). physical_index gives
 // the boundary physical index of each side (x low, x high, y low, y high, z low, z high); a
 // direction whose two sides are -1 is periodic.
+// A mesh in host arrays (the analogue of the gmsh model a setMesh(path, generate_mesh_function)
+// callback builds, SystemControl.cpp:55-61): node coordinates, element vertex lists (0-based, gmsh
+// vertex order), boundary faces with their physical index, and translational periodic node pairs.
+struct ArrayMesh {
+  int element = SDG_ELEM_QUADRANGLE;                    // SDG_ELEM_TRIANGLE / QUADRANGLE / HEXAHEDRON
+  std::vector<std::array<Real, 3>> nodes;
+  std::vector<int64_t> elements;                        // element_count x vertices
+  std::vector<int64_t> boundary_faces;                  // count x face vertices
+  std::vector<Isize> boundary_physical;                 // per boundary face
+  std::vector<std::vector<std::pair<int64_t, int64_t>>> periodic;  // per link: (slave, master) pairs
+};
+
 struct StructuredMesh {
   std::array<int, 3> cells{2, 2, 2};
   std::array<Real, 3> lower{0.0, 0.0, 0.0};
@@ -149,6 +161,8 @@ struct System {
   std::vector<std::pair<Isize, BoundaryConditionEnum>> boundary_types_;
   StructuredMesh mesh_desc_;
   std::string mesh_file_;  // gmsh MSH 4.1 file (setMesh(path)); empty for a structured box
+  ArrayMesh array_mesh_;   // setMesh(path, generate) without the file: the callback's arrays
+  bool use_arrays_ = false;
   bool mesh_set_ = false;
   sdg_problem_desc problem_{};
   sdg_mesh* mesh_ = nullptr;
@@ -191,11 +205,26 @@ struct System {
   // BoundaryCondition callback. (The reference's mesh-generating callback needs the gmsh library.)
   void setMesh(const std::filesystem::path& mesh_file) {
     mesh_file_ = mesh_file.string();
+    use_arrays_ = false;
     mesh_set_ = true;
   }
   void setMesh(const StructuredMesh& mesh) {
     mesh_file_.clear();
+    use_arrays_ = false;
     mesh_desc_ = mesh;
+    mesh_set_ = true;
+  }
+  // SystemControl.cpp:55-61 with a generating callback: an existing mesh file is read; otherwise the
+  // callback fills an ArrayMesh in memory (where the reference's callback drives the gmsh library).
+  void setMesh(const std::filesystem::path& mesh_file, const std::function<void(ArrayMesh&)>& generate_mesh_function) {
+    if (std::filesystem::exists(mesh_file)) {
+      setMesh(mesh_file);
+      return;
+    }
+    array_mesh_ = ArrayMesh{};
+    generate_mesh_function(array_mesh_);
+    mesh_file_.clear();
+    use_arrays_ = true;
     mesh_set_ = true;
   }
   // SystemControl.cpp:76-80: boundary type of a physical index (Periodic: nothing to record,
@@ -340,6 +369,39 @@ struct System {
   // SystemControl.cpp:146-157: build the mesh and the device context
   void synchronize() {
     if (!mesh_set_) throw std::runtime_error("SubrosaDG: setMesh was not called");
+    if (use_arrays_) {
+      const ArrayMesh& am = array_mesh_;
+      std::vector<double> xyz;
+      for (const auto& p : am.nodes) xyz.insert(xyz.end(), p.begin(), p.end());
+      std::vector<int> btype, bphys;
+      for (const Isize ph : am.boundary_physical) {
+        btype.push_back(boundaryType(ph));
+        bphys.push_back(static_cast<int>(ph));
+      }
+      std::vector<int64_t> sizes, pairs;
+      for (const auto& link : am.periodic) {
+        sizes.push_back(static_cast<int64_t>(link.size()));
+        for (const auto& pr : link) pairs.insert(pairs.end(), {pr.first, pr.second});
+      }
+      const int nvert = am.element == SDG_ELEM_TRIANGLE ? 3 : (am.element == SDG_ELEM_QUADRANGLE ? 4 : 8);
+      sdg_array_mesh_desc d{};
+      d.element = am.element;
+      d.polynomial_order = SC::kPolynomialOrder;
+      d.node_count = static_cast<int64_t>(am.nodes.size());
+      d.nodes = xyz.data();
+      d.element_count = static_cast<int64_t>(am.elements.size()) / nvert;
+      d.elements = am.elements.data();
+      d.boundary_face_count = static_cast<int64_t>(am.boundary_physical.size());
+      d.boundary_faces = am.boundary_faces.data();
+      d.boundary_type = btype.data();
+      d.boundary_physical = bphys.data();
+      d.periodic_link_count = static_cast<int>(sizes.size());
+      d.periodic_link_size = sizes.data();
+      d.periodic_pairs = pairs.data();
+      check(sdg_mesh_create_from_arrays(&d, &mesh_), nullptr);
+      check(sdg_create(&problem_, mesh_, &ctx_), nullptr);
+      return;
+    }
     if (!mesh_file_.empty()) {
       std::vector<int> phys, types;
       for (const auto& b : boundary_types_) {
@@ -399,7 +461,7 @@ struct System {
         x[dd] = xb[p * D + dd];
         if (std::fabs(nb[p * D + dd]) > 0.5) side = 2 * dd + (nb[p * D + dd] > 0.0 ? 1 : 0);
       }
-      const Isize physical = mesh_file_.empty() ? mesh_desc_.physical_index[side] : phys[p / info.face_quadrature];
+      const Isize physical = mesh_file_.empty() && !use_arrays_ ? mesh_desc_.physical_index[side] : phys[p / info.face_quadrature];
       const auto v = boundary_condition_.calculatePrimitiveFromCoordinate(x, time, physical);
       for (int k = 0; k < NV; k++) prim[p * NV + k] = v[k];
     }

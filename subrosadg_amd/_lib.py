@@ -31,6 +31,10 @@ TRANSPORT_NONE, TRANSPORT_CONSTANT, TRANSPORT_SUTHERLAND = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 SOURCE_NONE, SOURCE_BOUSSINESQ = 0, 1
 SHOCK_NONE, SHOCK_ARTIFICIAL_VISCOSITY = 0, 1
+VIEW_VARIABLES = ("Density", "Velocity", "Temperature", "Pressure", "SoundSpeed", "MachNumber", "Entropy", "Vorticity",
+                  "HeatFlux", "ArtificialViscosity", "VelocityX", "VelocityY", "VelocityZ", "MachNumberX",
+                  "MachNumberY", "MachNumberZ", "VorticityX", "VorticityY", "VorticityZ", "HeatFluxX", "HeatFluxY",
+                  "HeatFluxZ")  # enum sdg_view_variable, in order
 KERNELS_AUTO, KERNELS_DENSE, KERNELS_VECTOR = 0, 1, 2
 LAUNCH_GRAPH, LAUNCH_EAGER = 0, 1
 IC_FUNCTION, IC_SPECIFIC_FILE, IC_LAST_STEP = 0, 1, 2
@@ -123,6 +127,24 @@ class SplitDesc(Structure):
     ]
 
 
+class ArrayMeshDesc(Structure):
+    _fields_ = [
+        ("element", c_int),
+        ("polynomial_order", c_int),
+        ("node_count", c_int64),
+        ("nodes", c_void_p),
+        ("element_count", c_int64),
+        ("elements", c_void_p),
+        ("boundary_face_count", c_int64),
+        ("boundary_faces", c_void_p),
+        ("boundary_type", c_void_p),
+        ("boundary_physical", c_void_p),
+        ("periodic_link_count", c_int),
+        ("periodic_link_size", c_void_p),
+        ("periodic_pairs", c_void_p),
+    ]
+
+
 class GmshDesc(Structure):
     _fields_ = [
         ("path", c_char_p),
@@ -179,6 +201,7 @@ SIGNATURES = {
     "sdg_vjp_residual": (c_int, [c_void_p, c_void_p, c_void_p]),
     "sdg_mesh_create_split": (c_int, [POINTER(SplitDesc), POINTER(c_void_p)]),
     "sdg_mesh_create_gmsh": (c_int, [POINTER(GmshDesc), POINTER(c_void_p)]),
+    "sdg_mesh_create_from_arrays": (c_int, [POINTER(ArrayMeshDesc), POINTER(c_void_p)]),
     "sdg_batch_create": (c_int, [POINTER(ProblemDesc), c_void_p, c_int, POINTER(c_void_p)]),
     "sdg_batch_destroy": (None, [c_void_p]),
     "sdg_batch_last_error": (c_char_p, [c_void_p]),
@@ -196,6 +219,8 @@ SIGNATURES = {
     "sdg_raw_binary_read": (c_int, [c_void_p, c_char_p, c_int]),
     "sdg_raw_file_write": (c_int, [c_char_p, c_void_p, c_int64]),
     "sdg_write_vtu": (c_int, [c_void_p, POINTER(ProblemDesc), c_void_p, c_void_p, c_int, c_double, c_char_p]),
+    "sdg_write_view": (c_int, [c_void_p, POINTER(ProblemDesc), c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                               c_double, c_char_p]),
     "sdg_raw_file_read": (c_int, [c_char_p, c_void_p, c_int64, POINTER(c_int64)]),
 }
 
@@ -320,6 +345,43 @@ class Mesh:
         d.boundary = bc
         h = c_void_p()
         check(lib.sdg_mesh_create_gmsh(ctypes.byref(d), ctypes.byref(h)))
+        return cls(0, order, (), (), (), _handle=h)
+
+    @classmethod
+    def from_arrays(cls, element: int, order: int, nodes, elements, boundary_faces=None, boundary_type=None,
+                    boundary_physical=None, periodic=None):
+        """Mesh from host arrays (sdg_mesh_create_from_arrays): nodes (N, 3) or (N, 2), elements (E,
+        nvert) of 0-based nodes; boundary faces (F_b, face vertices) with BC_* types and optional
+        physical tags; periodic: a list of links, each a list of (slave, master) node pairs."""
+        lib = load()
+        xyz = np.zeros((len(nodes), 3))
+        nodes = np.asarray(nodes, dtype=np.float64)
+        xyz[:, : nodes.shape[1]] = nodes
+        conn = np.ascontiguousarray(elements, dtype=np.int64)
+        keep = [xyz, conn]
+        d = ArrayMeshDesc()
+        d.element, d.polynomial_order = element, order
+        d.node_count, d.nodes = xyz.shape[0], xyz.ctypes.data
+        d.element_count, d.elements = conn.shape[0], conn.ctypes.data
+        if boundary_faces is not None and len(boundary_faces):
+            bf = np.ascontiguousarray(boundary_faces, dtype=np.int64)
+            bt = np.ascontiguousarray(boundary_type, dtype=np.int32)
+            keep += [bf, bt]
+            d.boundary_face_count, d.boundary_faces, d.boundary_type = bf.shape[0], bf.ctypes.data, bt.ctypes.data
+            if boundary_physical is not None:
+                bp = np.ascontiguousarray(boundary_physical, dtype=np.int32)
+                keep.append(bp)
+                d.boundary_physical = bp.ctypes.data
+        if periodic:
+            sizes = np.array([len(link) for link in periodic], dtype=np.int64)
+            pairs = np.ascontiguousarray(np.concatenate([np.asarray(link, dtype=np.int64).reshape(-1, 2)
+                                                         for link in periodic]), dtype=np.int64)
+            keep += [sizes, pairs]
+            d.periodic_link_count, d.periodic_link_size, d.periodic_pairs = len(periodic), sizes.ctypes.data, \
+                pairs.ctypes.data
+        h = c_void_p()
+        check(lib.sdg_mesh_create_from_arrays(ctypes.byref(d), ctypes.byref(h)))
+        del keep
         return cls(0, order, (), (), (), _handle=h)
 
     def _attach(self, h):
@@ -572,6 +634,20 @@ def write_vtu(mesh: "Mesh", problem: ProblemDesc, coefficients: np.ndarray, path
     nv = None if node_values is None else np.ascontiguousarray(node_values, dtype=np.float64)
     check(lib.sdg_write_vtu(mesh.handle, ctypes.byref(problem), c.ctypes.data,
                             None if nv is None else nv.ctypes.data, int(step), float(time), path.encode()))
+
+
+def write_view(mesh: "Mesh", problem: ProblemDesc, coefficients: np.ndarray, path: str, variables,
+               gradient=None, node_values=None, step: int = 0, time: float = 0.0):
+    """VTK XML output of the named view variables (host only): see sdg_write_view in include/sdg.h."""
+    lib = load()
+    c = np.ascontiguousarray(coefficients, dtype=np.float64)
+    assert c.size == mesh.E * mesh.nb * mesh.nv
+    g = None if gradient is None else np.ascontiguousarray(gradient, dtype=np.float64)
+    nv = None if node_values is None else np.ascontiguousarray(node_values, dtype=np.float64)
+    ids = np.array([VIEW_VARIABLES.index(v) if isinstance(v, str) else int(v) for v in variables], dtype=np.int32)
+    check(lib.sdg_write_view(mesh.handle, ctypes.byref(problem), c.ctypes.data, None if g is None else g.ctypes.data,
+                             None if nv is None else nv.ctypes.data, ids.ctypes.data, int(ids.size), int(step),
+                             float(time), path.encode()))
 
 
 def raw_file_read(path: str) -> bytes:

@@ -60,6 +60,70 @@ int sdg_mesh_create_gmsh(const sdg_gmsh_desc* desc, sdg_mesh** out) {
   return SDG_OK;
 }
 
+int sdg_mesh_create_from_arrays(const sdg_array_mesh_desc* d, sdg_mesh** out) {
+  auto refuse = [](const std::string& msg) {
+    sdg::setGlobalError("sdg_mesh_create_from_arrays: " + msg);
+    return SDG_E_INVALID;
+  };
+  if (!d || !out) return refuse("null argument");
+  *out = nullptr;
+  const int type = d->element;
+  int nvert = 0, fvert = 0;
+  if (type == SDG_ELEM_TRIANGLE) {
+    nvert = 3, fvert = 2;
+  } else if (type == SDG_ELEM_QUADRANGLE) {
+    nvert = 4, fvert = 2;
+  } else if (type == SDG_ELEM_HEXAHEDRON) {
+    nvert = 8, fvert = 4;
+  } else {
+    sdg::setGlobalError("sdg_mesh_create_from_arrays: element must be triangle, quadrangle or hexahedron");
+    return SDG_E_UNSUPPORTED;
+  }
+  if (d->node_count < 1 || !d->nodes || d->element_count < 1 || !d->elements) return refuse("no nodes or elements");
+  if (d->boundary_face_count < 0 || (d->boundary_face_count > 0 && (!d->boundary_faces || !d->boundary_type)))
+    return refuse("boundary faces without node lists or types");
+  if (d->periodic_link_count < 0 || (d->periodic_link_count > 0 && (!d->periodic_link_size || !d->periodic_pairs)))
+    return refuse("periodic links without pairs");
+  for (int64_t i = 0; i < d->element_count * nvert; i++)
+    if (d->elements[i] < 0 || d->elements[i] >= d->node_count) return refuse("element vertex out of range");
+  for (int64_t i = 0; i < d->boundary_face_count * fvert; i++)
+    if (d->boundary_faces[i] < 0 || d->boundary_faces[i] >= d->node_count) return refuse("boundary face node out of range");
+  std::vector<std::map<int64_t, int64_t>> links(static_cast<size_t>(d->periodic_link_count));
+  int64_t off = 0;
+  for (int l = 0; l < d->periodic_link_count; l++) {
+    if (d->periodic_link_size[l] < 0) return refuse("negative periodic link size");
+    for (int64_t k = 0; k < d->periodic_link_size[l]; k++, off++) {
+      const int64_t sl = d->periodic_pairs[2 * off], ms = d->periodic_pairs[2 * off + 1];
+      if (sl < 0 || sl >= d->node_count || ms < 0 || ms >= d->node_count) return refuse("periodic node out of range");
+      if (sl != ms) links[static_cast<size_t>(l)][sl] = ms;
+    }
+  }
+  sdg_mesh* m = new (std::nothrow) sdg_mesh();
+  if (!m) {
+    sdg::setGlobalError("sdg_mesh_create_from_arrays: out of host memory");
+    return SDG_E_NOMEM;
+  }
+  std::string err;
+  try {
+    std::vector<int> phys(static_cast<size_t>(d->boundary_face_count), -1);
+    if (d->boundary_physical)
+      for (int64_t i = 0; i < d->boundary_face_count; i++) phys[static_cast<size_t>(i)] = d->boundary_physical[i];
+    if (!sdg::buildMeshFromConnectivity(type, d->polynomial_order, d->node_count, d->nodes, d->element_count,
+                                        d->elements, d->boundary_face_count, d->boundary_faces, d->boundary_type,
+                                        phys.data(), m->mesh, err, links.empty() ? nullptr : &links)) {
+      delete m;
+      sdg::setGlobalError("sdg_mesh_create_from_arrays: " + err);
+      return SDG_E_INVALID;
+    }
+  } catch (const std::exception& e) {
+    delete m;
+    sdg::setGlobalError(std::string("sdg_mesh_create_from_arrays: ") + e.what());
+    return SDG_E_NOMEM;
+  }
+  *out = m;
+  return SDG_OK;
+}
+
 int sdg_mesh_create_structured(const sdg_structured_desc* desc, sdg_mesh** out) {
   if (!desc || !out) {
     sdg::setGlobalError("sdg_mesh_create_structured: null argument");

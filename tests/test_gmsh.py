@@ -221,3 +221,43 @@ def test_boundary_elements_that_are_not_faces_are_refused(tmp_path):
     W.write_msh(p, 3, tags, xyz, "hex", elems, groups + [(15, "btri", tris)])
     with pytest.raises(L.SDGError, match="physical group 15"):
         L.Mesh.gmsh(p, 2, {10 + k: FAR for k in range(6)})
+
+
+def test_array_meshes_equal_the_structured_builder():
+    """sdg_mesh_create_from_arrays (a mesh handed over as host arrays, e.g. read out of a gmsh model)
+    gives the structured builder's tables bit for bit: quadrangles with boundaries, periodic
+    quadrangles, hexahedra with boundaries."""
+    nx, ny = 5, 3
+    tags, xyz, elems, sides = W.grid_quads(nx, ny, (0.0, 0.0), (2.0, 1.0))
+    bf = np.concatenate(sides) - 1
+    bt = [FAR] * (2 * ny) + [SLIP] * (2 * nx)
+    a = L.Mesh.from_arrays(L.ELEM_QUADRANGLE, 3, xyz, elems - 1, bf, bt, [1] * (2 * ny) + [2] * (2 * nx))
+    s = L.Mesh(L.ELEM_QUADRANGLE, 3, (nx, ny), (0.0, 0.0), (2.0, 1.0), boundary=((FAR, FAR), (SLIP, SLIP)))
+    _same_tables(a, s)
+    # the boundary faces come in element order; each keeps the physical tag it was given with
+    assert np.array_equal(a.field("boundary_face_physical"), np.where(a.field("boundary_face_type") == FAR, 1, 2))
+    # periodic in both directions through node pairs (the corner a slave in both links)
+    nx, ny = 4, 3
+    tags, xyz, elems, _ = W.grid_quads(nx, ny)
+    node = lambda i, j: i + (nx + 1) * j  # noqa: E731
+    pairs = [[(node(nx, j), node(0, j)) for j in range(ny + 1)], [(node(i, ny), node(i, 0)) for i in range(nx + 1)]]
+    a = L.Mesh.from_arrays(L.ELEM_QUADRANGLE, 2, xyz[:, :2], elems - 1, periodic=pairs)
+    _same_tables(a, L.Mesh(L.ELEM_QUADRANGLE, 2, (nx, ny), (0.0, 0.0), (1.0, 1.0)))
+    n = 2
+    tags, xyz, elems, sides = W.grid_hexes(n)
+    kinds = [FAR, FAR, SLIP, SLIP, L.BC_ADIABATIC_NONSLIP_WALL, L.BC_ADIABATIC_NONSLIP_WALL]
+    bt = np.concatenate([[kinds[k]] * len(sides[k]) for k in range(6)])
+    a = L.Mesh.from_arrays(L.ELEM_HEXAHEDRON, 2, xyz, elems - 1, np.concatenate(sides) - 1, bt)
+    s = L.Mesh(L.ELEM_HEXAHEDRON, 2, (n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0),
+               boundary=((FAR, FAR), (SLIP, SLIP), (L.BC_ADIABATIC_NONSLIP_WALL,) * 2))
+    _same_tables(a, s)
+
+
+def test_array_mesh_refusals():
+    tags, xyz, elems, sides = W.grid_quads(3, 2)
+    with pytest.raises(L.SDGError):  # vertex index out of range
+        L.Mesh.from_arrays(L.ELEM_QUADRANGLE, 2, xyz, elems)
+    with pytest.raises(L.SDGError):  # unpaired faces without boundary data
+        L.Mesh.from_arrays(L.ELEM_QUADRANGLE, 2, xyz, elems - 1)
+    with pytest.raises(L.SDGError):
+        L.Mesh.from_arrays(L.ELEM_LINE, 2, xyz, elems - 1)

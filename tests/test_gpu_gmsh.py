@@ -65,3 +65,20 @@ def test_cpp_driver_reads_a_mesh_file(tmp_path):
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stderr
     assert "mesh file: 40 steps" in r.stdout
+
+
+def test_cpp_driver_generates_a_mesh(tmp_path):
+    """setMesh(path, generate) in the C++ drop-in with a path that does not exist: the callback's
+    in-memory triangulation (jittered interior nodes, far field at x = 0 and 2, slip walls at
+    y = 0 and 1) goes through sdg_mesh_create_from_arrays and keeps a uniform stream uniform."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "examples", "bin",
+                       "mesh_file_2d_ceuler")
+    path = str(tmp_path / "absent.msh")
+    r = subprocess.run([exe, path, "2", "40"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    assert "mesh file: 40 steps" in r.stdout
+    assert not os.path.exists(path)

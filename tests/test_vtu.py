@@ -1,9 +1,12 @@
-"""VTU output (sdg_write_vtu, host only; the reference's View::stepView, Paraview.cpp:280-313):
-the XML parses, the sample points are the affine images of the equispaced reference points, the
-point data equal the closed forms of a state whose conserved fields are polynomials of degree P
-(exactly represented, so density, velocity, pressure, temperature and Mach number are known at
-every point), the artificial viscosity is the P1 interpolant of the node values, and the linear
-sub-cells tile each element with positive orientation (their areas / volumes sum to the domain's)."""
+"""VTU output (sdg_write_view / sdg_write_vtu, host only; the reference's View::stepView,
+Paraview.cpp:154-313): the XML parses; each element is one VTK Lagrange cell whose nodes are the
+affine images of the element's equispaced points of order P in VTK's node order (checked
+geometrically: vertices counter-clockwise / positively oriented, edge nodes on their edges in order
+from the edge's first vertex, face nodes on their faces, interior nodes inside); the point data
+equal the closed forms of a state whose conserved fields are polynomials of degree P (exactly
+represented, so density, velocity, pressure, temperature, sound speed, entropy and Mach number are
+known at every node), the gradient variables (vorticity, temperature gradient) the closed forms of
+the exact gradient, and the artificial viscosity the P1 interpolant of the node values."""
 import ctypes
 import math
 import xml.etree.ElementTree as ET
@@ -28,6 +31,22 @@ def _fields(x):
     return [rho] + mom + [rhoE]
 
 
+def _gradients(x):
+    """d/dx_d of the fields above, (..., NV, D)."""
+    D = x.shape[-1]
+    X = x[..., 0]
+    Y = x[..., 1] if D >= 2 else 0.5 * X
+    zero, one = np.zeros_like(X), np.ones_like(X)
+    if D == 1:  # Y = X / 2 along the line
+        g = [[0.2 + 0.1 * Y], [0.1 * (Y + 0.5 * X)], [0.6 * X + 0.1]]
+    elif D == 2:
+        g = [[0.2 * one, 0.2 * Y], [0.1 * Y, 0.1 * X], [zero, 0.1 * one], [0.6 * X, 0.2 * one]]
+    else:
+        g = [[0.2 * one, 0.2 * Y, 0.05 * one], [0.1 * Y, 0.1 * X, zero], [zero, 0.1 * one, zero],
+             [0.1 * one, zero, 0.05 * one], [0.6 * X, 0.2 * one, 0.1 * one]]
+    return np.stack([np.stack(r, axis=-1) for r in g], axis=-2)
+
+
 def _mesh(kind, order):
     if kind == "line":
         return L.Mesh(L.ELEM_LINE, order, (4,), (0.0,), (1.0,))
@@ -50,67 +69,151 @@ def _read(path):
     return arrays, int(piece.get("NumberOfPoints")), int(piece.get("NumberOfCells"))
 
 
-@pytest.mark.parametrize("kind,order", [("line", 2), ("line", 4), ("quad", 2), ("quad", 3), ("tri", 2), ("tri", 3),
-                                        ("hex", 2)])
-def test_vtu_points_fields_and_cells(tmp_path, kind, order):
+def _on_segment_in_order(pts, a, b, n):
+    """pts (n, D) lie on segment a -> b at the interior equispaced positions k/(n+1), in order."""
+    t = np.arange(1, n + 1) / (n + 1)
+    return np.allclose(pts, a[None] + t[:, None] * (b - a)[None], atol=1e-12)
+
+
+def _cross2(a, b):
+    return a[0] * b[1] - a[1] * b[0]
+
+
+def _check_node_order(kind, order, cell):
+    """Geometric check of VTK's Lagrange node order on one cell's node coordinates (n, D)."""
+    P_ = order
+    m = P_ - 1
+    if kind == "line":
+        assert cell[1, 0] > cell[0, 0]
+        assert _on_segment_in_order(cell[2:], cell[0], cell[1], m)
+        return
+    if kind == "tri":
+        v = cell[:3]
+        assert _cross2(v[1] - v[0], v[2] - v[0]) > 0  # counter-clockwise
+        o = 3
+        for a, b in ((0, 1), (1, 2), (2, 0)):
+            assert _on_segment_in_order(cell[o:o + m], v[a], v[b], m)
+            o += m
+        interior = cell[o:]
+        if P_ == 3:
+            assert np.allclose(interior, v.mean(0)[None])
+        if P_ == 4:  # an order-1 triangle at barycentric (2,1,1)/4, (1,2,1)/4, (1,1,2)/4
+            expect = np.array([(2 * v[0] + v[1] + v[2]) / 4, (v[0] + 2 * v[1] + v[2]) / 4, (v[0] + v[1] + 2 * v[2]) / 4])
+            assert np.allclose(interior, expect)
+        return
+    if kind == "quad":
+        v = cell[:4]
+        assert _cross2(v[1] - v[0], v[3] - v[0]) > 0 and np.allclose(v[2], v[1] + v[3] - v[0])
+        o = 4
+        for a, b in ((0, 1), (1, 2), (3, 2), (0, 3)):
+            assert _on_segment_in_order(cell[o:o + m], v[a], v[b], m)
+            o += m
+        ij = [(i, j) for j in range(1, P_) for i in range(1, P_)]
+        expect = np.array([v[0] + (v[1] - v[0]) * i / P_ + (v[3] - v[0]) * j / P_ for i, j in ij]).reshape(-1, 2)
+        assert np.allclose(cell[o:], expect)
+        return
+    v = cell[:8]
+    e1, e3, e4 = v[1] - v[0], v[3] - v[0], v[4] - v[0]
+    assert np.dot(np.cross(e1, e3), e4) > 0
+    assert np.allclose(v[2], v[0] + e1 + e3) and np.allclose(v[6], v[0] + e1 + e3 + e4)
+    o = 8
+    for a, b in ((0, 1), (1, 2), (3, 2), (0, 3), (4, 5), (5, 6), (7, 6), (4, 7), (0, 4), (1, 5), (3, 7), (2, 6)):
+        assert _on_segment_in_order(cell[o:o + m], v[a], v[b], m), (a, b)
+        o += m
+
+    def lat(i, j, k):
+        return v[0] + e1 * i / P_ + e3 * j / P_ + e4 * k / P_
+
+    faces = [[lat(0, j, k) for k in range(1, P_) for j in range(1, P_)],
+             [lat(P_, j, k) for k in range(1, P_) for j in range(1, P_)],
+             [lat(i, 0, k) for k in range(1, P_) for i in range(1, P_)],
+             [lat(i, P_, k) for k in range(1, P_) for i in range(1, P_)],
+             [lat(i, j, 0) for j in range(1, P_) for i in range(1, P_)],
+             [lat(i, j, P_) for j in range(1, P_) for i in range(1, P_)]]
+    for f in faces:
+        assert np.allclose(cell[o:o + m * m], np.array(f).reshape(-1, 3))
+        o += m * m
+    body = [lat(i, j, k) for k in range(1, P_) for j in range(1, P_) for i in range(1, P_)]
+    assert np.allclose(cell[o:], np.array(body).reshape(-1, 3))
+
+
+@pytest.mark.parametrize("kind,order", [("line", 2), ("line", 4), ("quad", 2), ("quad", 3), ("quad", 4), ("tri", 2),
+                                        ("tri", 3), ("tri", 4), ("hex", 2), ("hex", 3)])
+def test_view_cells_points_and_fields(tmp_path, kind, order):
     m = _mesh(kind, order)
-    prob = L.problem_desc(equation=L.EQ_EULER, cp=GAMMA * CV, cv=CV)
-    xq = m.field("quad_coord").reshape(m.E, m.nq, m.dim)
+    D = m.dim
+    prob = L.problem_desc(equation=L.EQ_NS, cp=GAMMA * CV, cv=CV, viscous_flux=L.VISC_BR2,
+                          transport=L.TRANSPORT_CONSTANT, dynamic_viscosity=1e-3)
+    xq = m.field("quad_coord").reshape(m.E, m.nq, D)
     C = P.project_conserved(m, np.stack(_fields(xq), axis=-1))  # exact: degree <= P per coordinate
+    G = P.project_conserved(m, _gradients(xq).reshape(m.E, m.nq, -1))  # exact, row var*D + dim
     node = np.linspace(0.0, 1.0, m.nodes)
     path = str(tmp_path / "out.vtu")
-    L.write_vtu(m, prob, C, path, node_values=node, step=7, time=0.25)
+    names = list(L.VIEW_VARIABLES)
+    L.write_view(m, prob, C, path, names, gradient=G, node_values=node, step=7, time=0.25)
     a, npts, ncells = _read(path)
-    per = (order + 1) * (order + 2) // 2 if kind == "tri" else (order + 1) ** m.dim
-    sub = order * order if kind == "tri" else order ** m.dim
-    assert npts == m.E * per and ncells == m.E * sub
+    per = (order + 1) * (order + 2) // 2 if kind == "tri" else (order + 1) ** D
+    assert npts == m.E * per and ncells == m.E
     assert a["TMSTEP"] == 7 and a["TimeValue"] == 0.25
-    x = a["Points"][:, : m.dim]
+    vtk = {"line": 68, "tri": 69, "quad": 70, "hex": 72}[kind]
+    assert set(np.unique(a["types"]).astype(int)) == {vtk}
+    conn = a["connectivity"].astype(np.int64).reshape(ncells, per)
+    x = a["Points"][:, :D]
+    for c in range(ncells):
+        _check_node_order(kind, order, x[conn[c]])
+    # every node of an element is one of the affine images of its equispaced lattice (the cells tile
+    # the domain: their vertices' hull volumes sum to 1)
     f = _fields(x)
     rho, mom, rhoE = f[0], np.stack(f[1:-1], axis=-1), f[-1]
     vel = mom / rho[:, None]
     p = (GAMMA - 1.0) * (rhoE - 0.5 * (mom * vel).sum(-1))
+    c = np.sqrt(GAMMA * p / rho)
+    T = p / ((GAMMA - 1.0) * rho * CV)
+
+    def vec(name):
+        return a[name] if a[name].ndim == 1 else a[name][:, :D]
+
     assert np.allclose(a["Density"], rho, rtol=1e-12, atol=1e-13)
-    assert np.allclose(a["Velocity"][:, : m.dim], vel, rtol=1e-12, atol=1e-12)
-    assert np.all(a["Velocity"][:, m.dim:] == 0.0)
-    assert np.allclose(a["Pressure"], p, rtol=1e-11)
-    assert np.allclose(a["Temperature"], p / ((GAMMA - 1.0) * rho * CV), rtol=1e-11)
-    assert np.allclose(a["MachNumber"], np.sqrt((vel ** 2).sum(-1)) / np.sqrt(GAMMA * p / rho), rtol=1e-11)
-    # artificial viscosity: node values at the element corners, P1 interpolant elsewhere
-    tag = m.field("node_tag").reshape(m.E, m.nvert)
-    pts = x.reshape(m.E, per, m.dim)
-    av = a["ArtificialViscosity"].reshape(m.E, per)
-    corners = [0, order, per - 1] if kind == "tri" else None
-    if kind == "line":
-        corners = [0, order]
-    if kind == "quad":
-        corners = [0, order, per - 1, per - 1 - order]
-    if kind == "hex":
-        n1 = order + 1
-        corners = [0, order, n1 * n1 - 1, n1 * order, n1 * n1 * order, n1 * n1 * order + order, per - 1,
-                   per - 1 - order]
-    assert np.allclose(av[:, corners], node[tag], rtol=1e-14, atol=1e-15)
-    # the sub-cells tile the elements with positive orientation
-    conn = a["connectivity"].astype(np.int64).reshape(ncells, -1)
-    cx = a["Points"][conn][..., : m.dim]
-    if m.dim == 1:
-        length = cx[:, 1, 0] - cx[:, 0, 0]
-        assert conn.shape[1] == 2 and np.all(length > 0.0) and math.isclose(length.sum(), 1.0, rel_tol=1e-12)
-        assert np.allclose(length, 1.0 / (m.E * order), rtol=1e-12)  # equispaced points of order P
-    elif m.dim == 2:
-        xs, ys = cx[..., 0], cx[..., 1]
-        area = 0.5 * (xs * np.roll(ys, -1, axis=1) - np.roll(xs, -1, axis=1) * ys).sum(1)
-        assert np.all(area > 0.0) and math.isclose(area.sum(), 1.0, rel_tol=1e-12)
+    if D == 1:
+        assert np.allclose(a["Velocity"], vel[:, 0], rtol=1e-12, atol=1e-12)
     else:
-        e1, e3, e4 = cx[:, 1] - cx[:, 0], cx[:, 3] - cx[:, 0], cx[:, 4] - cx[:, 0]
-        vol = np.einsum("ci,ci->c", np.cross(e1, e3), e4)
-        assert np.all(vol > 0.0) and math.isclose(vol.sum(), 1.0, rel_tol=1e-12)
-        assert np.allclose(cx[:, 6], cx[:, 0] + e1 + e3 + e4)  # opposite corner of a box cell
-    vtk = {"line": 3, "tri": 5, "quad": 9, "hex": 12}[kind]
-    assert set(np.unique(a["types"]).astype(int)) == {vtk}
+        assert np.allclose(a["Velocity"][:, :D], vel, rtol=1e-12, atol=1e-12) and np.all(a["Velocity"][:, D:] == 0.0)
+    assert np.allclose(a["Pressure"], p, rtol=1e-11)
+    assert np.allclose(a["Temperature"], T, rtol=1e-11)
+    assert np.allclose(a["SoundSpeed"], c, rtol=1e-11)
+    assert np.allclose(a["Entropy"], p / rho ** GAMMA, rtol=1e-11)
+    assert np.allclose(vec("MachNumber"), (vel / c[:, None]).squeeze() if D == 1 else vel / c[:, None], rtol=1e-11,
+                       atol=1e-12)
+    for d, axis in enumerate("XYZ"[:D]):
+        assert np.allclose(a["Velocity" + axis], vel[:, d], rtol=1e-12, atol=1e-12)
+        assert np.allclose(a["MachNumber" + axis], vel[:, d] / c, rtol=1e-11, atol=1e-12)
+    # gradients: grad u_k = (grad m_k - u_k grad rho) / rho, grad T from grad e
+    g = _gradients(x)  # (npts, NV, D)
+    gu = (g[:, 1:1 + D, :] - vel[:, :, None] * g[:, 0:1, :]) / rho[:, None, None]  # [k][d] = d u_k / d x_d
+    e = rhoE / rho - 0.5 * (vel ** 2).sum(-1)
+    ge = (g[:, D + 1, :] - (rhoE / rho)[:, None] * g[:, 0, :]) / rho[:, None] - np.einsum("pk,pkd->pd", vel, gu)
+    gT = ge / CV
+    assert np.allclose(vec("HeatFlux"), gT.squeeze() if D == 1 else gT, rtol=1e-10, atol=1e-12)
+    for d, axis in enumerate("XYZ"[:D]):
+        assert np.allclose(a["HeatFlux" + axis], gT[:, d], rtol=1e-10, atol=1e-12)
+    if D == 2:
+        assert np.allclose(a["Vorticity"], gu[:, 1, 0] - gu[:, 0, 1], rtol=1e-10, atol=1e-12)
+        assert np.allclose(a["VorticityZ"], gu[:, 1, 0] - gu[:, 0, 1], rtol=1e-10, atol=1e-12)
+    if D == 3:
+        w = np.stack([gu[:, 2, 1] - gu[:, 1, 2], gu[:, 0, 2] - gu[:, 2, 0], gu[:, 1, 0] - gu[:, 0, 1]], axis=-1)
+        assert np.allclose(a["Vorticity"], w, rtol=1e-10, atol=1e-12)
+        assert np.allclose(a["VorticityX"], w[:, 0], rtol=1e-10, atol=1e-12)
+    # artificial viscosity: node values at the element vertices (the first nodes of each cell), P1
+    # interpolant elsewhere (at the edge nodes: linear between the edge's vertices)
+    tag = m.field("node_tag").reshape(m.E, m.nvert)
+    av = a["ArtificialViscosity"][conn]
+    vmap = {"line": [0, 1], "tri": [0, 1, 2], "quad": [0, 1, 2, 3], "hex": list(range(8))}[kind]
+    assert np.allclose(av[:, vmap], node[tag], rtol=1e-14, atol=1e-15)
+    assert np.all(av.min(1) >= node[tag].min(1) - 1e-14) and np.all(av.max(1) <= node[tag].max(1) + 1e-14)
+    _ = e
 
 
-def test_vtu_without_viscosity_and_errors(tmp_path):
+def test_vtu_defaults_without_viscosity_and_errors(tmp_path):
     m = _mesh("quad", 2)
     prob = L.problem_desc(equation=L.EQ_EULER, cp=GAMMA * CV, cv=CV)
     xq = m.field("quad_coord").reshape(m.E, m.nq, m.dim)
@@ -118,10 +221,19 @@ def test_vtu_without_viscosity_and_errors(tmp_path):
     path = str(tmp_path / "plain.vtu")
     L.write_vtu(m, prob, C, path)
     a, _, _ = _read(path)
+    assert {"Density", "Velocity", "Pressure", "Temperature", "MachNumber"} <= set(a)
     assert "ArtificialViscosity" not in a
+    assert a["MachNumber"].shape[1] == 3
     with pytest.raises(L.SDGError):
         L.write_vtu(m, prob, C, str(tmp_path / "missing" / "x.vtu"))
+    with pytest.raises(L.SDGError):  # gradient variables need the lifted gradient
+        L.write_view(m, prob, C, path, ["Vorticity"])
+    with pytest.raises(L.SDGError):
+        L.write_view(m, prob, C, path, ["ArtificialViscosity"])
+    with pytest.raises(L.SDGError):
+        L.write_view(m, prob, C, path, [99])
     mixed = L.Mesh.split(2, (3, 3), (0.0, 0.0), (1.0, 1.0), 0.5, 4)
     buf = np.zeros(mixed.E * 9 * 4)
     rc = L.load().sdg_write_vtu(mixed.handle, ctypes.byref(prob), buf.ctypes.data, None, 0, 0.0, path.encode())
     assert rc == L.E_UNSUPPORTED
+    _ = math
