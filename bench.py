@@ -46,17 +46,18 @@ RUNS = 5                       # timed runs per configuration; the median is rep
 
 # HBM bytes per launch of the stage kernels on the 32^3 P3 workload, from rocprofv3 FETCH_SIZE /
 # WRITE_SIZE passes scaled by a copy kernel of known size (fetch x2.000, write x1.000):
-# profiles/ns_tgv32_p3_tensor_counters.md. Keyed by (kernel, precision mode, cells per direction, order).
-PROFILED_TRAFFIC = {("nsGradTensor", "fp64", 32, 3): (194245.8 * 2.0 + 204800.0 * 1.0) * 1024,
-                    ("nsGradTensor", "mixed", 32, 3): (145675.7 * 2.0 + 102400.0 * 1.0) * 1024,
-                    ("nsFaceFluxKernel", "fp64", 32, 3): (151386.9 * 2.0 + 122880.0 * 1.0) * 1024,
-                    ("nsFaceFluxKernel", "mixed", 32, 3): (114860.2 * 2.0 + 61509.7 * 1.0) * 1024,
-                    ("nsResidualTensor", "fp64", 32, 3): (184933.9 * 2.0 + 232533.4 * 1.0) * 1024,
-                    ("nsResidualTensor", "mixed", 32, 3): (133740.9 * 2.0 + 171093.4 * 1.0) * 1024}
-# the batched stage kernel on the 48-problem workload (average over its launches;
-# profiles/batch48_uneven_tri_quad_euler_kernels.md)
-PROFILED_BATCH_TRAFFIC = {"fp64": (346788.6 * 2.0 + 231499.4 * 1.0) * 1024,
-                          "mixed": (312995.3 * 2.0 + 136595.1 * 1.0) * 1024}
+# profiles/ns_tgv32_p3_fp64_mixed_tensor_kernels.md. Keyed by (kernel, precision mode, cells per direction, order).
+PROFILED_TRAFFIC = {("nsGradTensor", "fp64", 32, 3): (194351.2 * 2.0 + 204800.0 * 1.0) * 1024,
+                    ("nsGradTensor", "mixed", 32, 3): (145729.7 * 2.0 + 102400.0 * 1.0) * 1024,
+                    ("nsFaceFluxKernel", "fp64", 32, 3): (151387.7 * 2.0 + 122880.0 * 1.0) * 1024,
+                    ("nsFaceFluxKernel", "mixed", 32, 3): (114839.1 * 2.0 + 61503.2 * 1.0) * 1024,
+                    ("nsResidualTensor", "fp64", 32, 3): (184931.6 * 2.0 + 232533.4 * 1.0) * 1024,
+                    ("nsResidualTensor", "mixed", 32, 3): (133740.8 * 2.0 + 171093.3 * 1.0) * 1024}
+# the batched stage on the 48-problem workload: one launch per element variant (triangle and quadrangle,
+# P1-P4), FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) KiB per launch, summed over the 8 variants of a stage
+# (profiles/batch48_uneven_tri_quad_euler_kernels.md)
+PROFILED_BATCH_TRAFFIC = {"fp64": (568801.4) * 1024,
+                          "mixed": (494566.9) * 1024}
 
 
 def log(msg):
@@ -142,7 +143,7 @@ def tensor_path(mesh) -> bool:
     return mesh.elem == L.ELEM_HEXAHEDRON and 2 <= mesh.order <= 4
 
 
-def roofline(kms, mesh, mode, n):
+def roofline(kms, mesh, mode, n, kernels="auto"):
     """Roofline entry of the dominant kernel. kms: HIP-event averages of the gradient, face-flux and
     residual kernels (and the monitor) on the solver's stream. `achieved`: the algorithmic bytes
     B_alg per element-stage times the elements one launch processes, over the launch's duration
@@ -152,8 +153,13 @@ def roofline(kms, mesh, mode, n):
     three kernels) against B_alg."""
     mixed = mode == "mixed"
     f_alg, b_alg = algorithmic_counts(mesh, mixed=mixed)
-    names = ["nsGradTensor", "nsFaceFluxKernel", "nsResidualTensor"] if tensor_path(mesh) else \
-        ["nsGradMfma", "nsFaceFluxKernel", "nsResidualMfma"]
+    tensor = tensor_path(mesh) and kernels == "auto"
+    if kernels == "vector":  # no separate face-flux kernel: its interval is empty
+        names = ["nsGradKernel", "(none)", "nsFluxKernel"]
+    elif tensor:
+        names = ["nsGradTensor", "nsFaceFluxKernel", "nsResidualTensor"]
+    else:
+        names = ["nsGradMfma", "nsFaceFluxKernel", "nsResidualMfma"]
     k = int(np.argmax(kms[:3]))
     name, ms = names[k], float(kms[k])
     stage_ms = float(sum(kms[:3]))
@@ -167,7 +173,7 @@ def roofline(kms, mesh, mode, n):
            "flop_alg_per_element_stage": f_alg,
            "flop_alg_frac_of_peak_kernel": f_alg * mesh.E / (ms * 1e-3) / 1e12 / peak_f,
            "flop_alg_frac_of_peak_stage": f_alg * mesh.E / (stage_ms * 1e-3) / 1e12 / peak_f}
-    if tensor_path(mesh):
+    if tensor:
         f_sf = tensor_flops(mesh.order)
         out["flop_executed_per_element_stage"] = f_sf
         out["flop_executed_frac_of_peak_stage"] = f_sf * mesh.E / (stage_ms * 1e-3) / 1e12 / peak_f
@@ -221,12 +227,12 @@ def group_l2(a, b, dim):
                for g in ([0], list(range(1, dim + 1)), [dim + 1]))
 
 
-def tgv_modes(mesh, prob, prim, args, label, runs=RUNS, kernel_steps=10):
+def tgv_modes(mesh, prob, prim, args, label, runs=RUNS, kernel_steps=10, modes=("fp64", "mixed")):
     """FP64 and mixed runs of one Taylor-Green configuration from the same initial coefficients:
     per mode the median rate of `runs` runs of args.steps steps after args.warmup steps, the
     per-kernel HIP-event averages, and the mixed state's deviation from the FP64 state."""
     out, states, coeff0 = {}, {}, None
-    for mode in ("fp64", "mixed"):
+    for mode in modes:
         pd = L.ProblemDesc.from_buffer_copy(prob)
         pd.precision = L.PREC_MIXED if mode == "mixed" else L.PREC_FP64
         s = L.Solver(pd, mesh)
@@ -246,6 +252,8 @@ def tgv_modes(mesh, prob, prim, args, label, runs=RUNS, kernel_steps=10):
                      "dt": dt, "kms": kms}
         log(f"{label} {mode}: {med:.3f} ms/step (runs {[round(x, 3) for x in ms]}), "
             f"{out[mode]['value']:.4e} DOF.stage/s, kernels {[round(float(x), 4) for x in kms]} ms")
+    if "mixed" not in states:
+        return out
     out["mixed_l2_rel_dev_vs_fp64"] = function_l2(mesh, states["mixed"], states["fp64"])
     out["mixed_coefficient_l2_rel_dev_vs_fp64"] = group_l2(states["mixed"], states["fp64"], mesh.dim)
     out["steps_compared"] = args.warmup + runs * args.steps
@@ -651,7 +659,12 @@ def main():
                     help="only run the HBM-counter calibration copy of this many MiB")
     ap.add_argument("--profile-leg", choices=["headline"] + list(LEGS), default=None,
                     help="only run one leg (for kernel profiling) and print its record")
+    ap.add_argument("--kernels", choices=["auto", "dense", "vector"], default="auto",
+                    help="headline kernels: auto (sum-factorised on hexahedra P2-P4), the dense "
+                         "matrix-core kernels or the vector kernels (FP64 only)")
     args = ap.parse_args()
+    if args.kernels != "auto" and args.profile_leg != "headline":
+        ap.error("--kernels other than auto only with --profile-leg headline (the bench line is the default kernels)")
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
         return
@@ -665,12 +678,14 @@ def main():
 
     t_setup = time.perf_counter()
     mesh, prob, prim = tgv_setup(args.n, args.order)
-    rec = tgv_modes(mesh, prob, prim, args, "C2")
+    prob.kernels = {"auto": L.KERNELS_AUTO, "dense": L.KERNELS_DENSE, "vector": L.KERNELS_VECTOR}[args.kernels]
+    rec = tgv_modes(mesh, prob, prim, args, "C2", modes=("fp64",) if args.kernels == "vector" else ("fp64", "mixed"))
     log(f"C2 runs {time.perf_counter() - t_setup:.1f}s")
-    head = rec["mixed"]
-    roof = {mode: roofline(rec[mode]["kms"], mesh, mode, args.n) for mode in ("fp64", "mixed")}
+    roof = {mode: roofline(rec[mode]["kms"], mesh, mode, args.n, args.kernels) for mode in rec if mode in ("fp64", "mixed")}
     if args.profile_leg == "headline":
+        log(f"headline kernels {args.kernels}: roofline {roof}")
         return
+    head = rec["mixed"]
     skip = set(filter(None, args.skip.split(",")))
     legs = {}
     for name, fn in LEGS.items():

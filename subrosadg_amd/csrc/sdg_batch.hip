@@ -1,9 +1,10 @@
 // sdg_batch.hip -- batched solver: many independent 2-D compressible Euler problems on meshes of
-// mixed triangles and quadrangles, orders 1..4, advanced together. Every RK stage of the whole
-// batch is ONE persistent launch: workgroups stride over a list of work items (element-type and
-// order variant, first element, count), each handled by the variant's element tile with the
-// tile's shape fixed at compile time. Whole steps are captured in hipGraphs. The work-item size
-// per variant and the grid size are picked by a measuring autotuner (sdg_batch_tune).
+// mixed triangles and quadrangles, orders 1..4, advanced together. An RK stage of the whole batch
+// is one persistent launch per element variant present (element type and order): each variant's
+// kernel has its tile shape and numerical flux fixed at compile time and strides over that
+// variant's work items (first element, count); the variants' launches run side by side on forked
+// streams, which a captured step turns into parallel branches of one hipGraph. The work-item size
+// and the grid of each variant are picked by a measuring autotuner (sdg_batch_tune).
 //
 // Per stage and element this is the reference's stepSolver pass chain for the Euler model
 // (TimeIntegration.cpp:327-350, SpatialDiscrete.cpp:194-266,633-748,1016-1032): volume flux,
