@@ -46,6 +46,8 @@ enum sdg_element {
   SDG_ELEM_LINE = 1,
   SDG_ELEM_TRIANGLE = 2,
   SDG_ELEM_QUADRANGLE = 3,
+  SDG_ELEM_TETRAHEDRON = 4,
+  SDG_ELEM_PYRAMID = 5,
   SDG_ELEM_HEXAHEDRON = 6,
   SDG_ELEM_MIXED_TRIANGLE_QUADRANGLE = 100 /* mesh info of a split grid holding both types */
 };
@@ -291,14 +293,19 @@ int sdg_mesh_create_gmsh(const sdg_gmsh_desc* desc, sdg_mesh** out);
  * tags, and translational periodic node pairs (Adjacency.cpp:185-328). Affine elements of one type;
  * faces, geometry and tables are built as for a mesh file. */
 typedef struct sdg_array_mesh_desc {
-  int element;                      /* SDG_ELEM_TRIANGLE, SDG_ELEM_QUADRANGLE or SDG_ELEM_HEXAHEDRON */
-  int polynomial_order;             /* P = 1..5 (triangles 1..4)                                   */
+  int element;                      /* SDG_ELEM_TRIANGLE, _QUADRANGLE, _TETRAHEDRON, _PYRAMID or    */
+                                    /* _HEXAHEDRON                                                 */
+  int polynomial_order;             /* P = 1..5 (triangles, tetrahedra and pyramids 1..4)           */
+  int geometry_order;               /* order of the element map: 1, or the gmsh order of the node   */
+                                    /* lists (1..5 in 2-D, 1..2 in 3-D); 0 means 1                  */
   int64_t node_count;
   const double* nodes;              /* node_count x 3 coordinates (z = 0 in 2-D)                   */
   int64_t element_count;
-  const int64_t* elements;          /* element_count x vertices, 0-based nodes, gmsh vertex order   */
+  const int64_t* elements;          /* element_count x geometry nodes (vertices first), 0-based,    */
+                                    /* gmsh node order                                             */
   int64_t boundary_face_count;
-  const int64_t* boundary_faces;    /* count x face vertices (2 in 2-D, 4 for hexahedra), any order  */
+  const int64_t* boundary_faces;    /* count x face vertices (2 in 2-D, 3 for tetrahedra, 4 for     */
+                                    /* hexahedra and pyramids, a triangle padded with -1), any order */
   const int* boundary_type;         /* enum sdg_boundary per boundary face                          */
   const int* boundary_physical;     /* physical tag per boundary face (NULL: -1)                    */
   int periodic_link_count;          /* translational periodic pairs of boundaries                   */

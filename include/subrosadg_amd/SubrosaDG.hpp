@@ -135,9 +135,10 @@ struct BoundaryCondition {
 // callback builds, SystemControl.cpp:55-61): node coordinates, element vertex lists (0-based, gmsh
 // vertex order), boundary faces with their physical index, and translational periodic node pairs.
 struct ArrayMesh {
-  int element = SDG_ELEM_QUADRANGLE;                    // SDG_ELEM_TRIANGLE / QUADRANGLE / HEXAHEDRON
+  int element = SDG_ELEM_QUADRANGLE;                    // SDG_ELEM_TRIANGLE / QUADRANGLE / TETRAHEDRON / HEXAHEDRON
+  int geometry_order = 1;                               // gmsh order of the element node lists (curved elements)
   std::vector<std::array<Real, 3>> nodes;
-  std::vector<int64_t> elements;                        // element_count x vertices
+  std::vector<int64_t> elements;                        // element_count x geometry nodes, gmsh order
   std::vector<int64_t> boundary_faces;                  // count x face vertices
   std::vector<Isize> boundary_physical;                 // per boundary face
   std::vector<std::vector<std::pair<int64_t, int64_t>>> periodic;  // per link: (slave, master) pairs
@@ -383,10 +384,15 @@ struct System {
         sizes.push_back(static_cast<int64_t>(link.size()));
         for (const auto& pr : link) pairs.insert(pairs.end(), {pr.first, pr.second});
       }
-      const int nvert = am.element == SDG_ELEM_TRIANGLE ? 3 : (am.element == SDG_ELEM_QUADRANGLE ? 4 : 8);
+      const int g = am.geometry_order, g1 = g + 1;
+      const int nvert = am.element == SDG_ELEM_TRIANGLE      ? g1 * (g1 + 1) / 2
+                        : am.element == SDG_ELEM_QUADRANGLE  ? g1 * g1
+                        : am.element == SDG_ELEM_TETRAHEDRON ? g1 * (g1 + 1) * (g1 + 2) / 6
+                                                             : g1 * g1 * g1;
       sdg_array_mesh_desc d{};
       d.element = am.element;
       d.polynomial_order = SC::kPolynomialOrder;
+      d.geometry_order = g;
       d.node_count = static_cast<int64_t>(am.nodes.size());
       d.nodes = xyz.data();
       d.element_count = static_cast<int64_t>(am.elements.size()) / nvert;

@@ -607,6 +607,8 @@ struct Oracle {
     if (order == 1) return 0;
     const int p = order - 1;
     if (dim == 2 && nvert == 3) return (p + 1) * (p + 2) / 2;
+    if (dim == 3 && nvert == 4) return (p + 1) * (p + 2) * (p + 3) / 6;  // tetrahedra
+    if (dim == 3 && nvert == 5) return (p + 1) * (p + 2) * (2 * p + 3) / 6;  // pyramids
     if (dim == 1) return p + 1;
     return dim == 2 ? (p + 1) * (p + 1) : (p + 1) * (p + 1) * (p + 1);
   }

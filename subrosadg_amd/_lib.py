@@ -20,7 +20,7 @@ ABI_VERSION = 5
 OK, E_INVALID, E_UNSUPPORTED, E_DEVICE, E_NOMEM, E_STATE = 0, -1, -2, -3, -4, -5
 
 # enums (include/sdg.h)
-ELEM_LINE, ELEM_TRIANGLE, ELEM_QUADRANGLE, ELEM_HEXAHEDRON = 1, 2, 3, 6
+ELEM_LINE, ELEM_TRIANGLE, ELEM_QUADRANGLE, ELEM_TETRAHEDRON, ELEM_PYRAMID, ELEM_HEXAHEDRON = 1, 2, 3, 4, 5, 6
 ELEM_MIXED_TRIANGLE_QUADRANGLE = 100
 EQ_EULER, EQ_NS = 0, 1
 FLUX_CENTRAL, FLUX_LAX_FRIEDRICHS, FLUX_HLLC, FLUX_ROE, FLUX_EXACT = 0, 1, 2, 3, 4
@@ -131,6 +131,7 @@ class ArrayMeshDesc(Structure):
     _fields_ = [
         ("element", c_int),
         ("polynomial_order", c_int),
+        ("geometry_order", c_int),
         ("node_count", c_int64),
         ("nodes", c_void_p),
         ("element_count", c_int64),
@@ -349,10 +350,11 @@ class Mesh:
 
     @classmethod
     def from_arrays(cls, element: int, order: int, nodes, elements, boundary_faces=None, boundary_type=None,
-                    boundary_physical=None, periodic=None):
+                    boundary_physical=None, periodic=None, geometry_order=1):
         """Mesh from host arrays (sdg_mesh_create_from_arrays): nodes (N, 3) or (N, 2), elements (E,
-        nvert) of 0-based nodes; boundary faces (F_b, face vertices) with BC_* types and optional
-        physical tags; periodic: a list of links, each a list of (slave, master) node pairs."""
+        geometry nodes) of 0-based nodes in gmsh order (the vertices for geometry_order 1); boundary
+        faces (F_b, face vertices) with BC_* types and optional physical tags; periodic: a list of
+        links, each a list of (slave, master) node pairs."""
         lib = load()
         xyz = np.zeros((len(nodes), 3))
         nodes = np.asarray(nodes, dtype=np.float64)
@@ -360,7 +362,7 @@ class Mesh:
         conn = np.ascontiguousarray(elements, dtype=np.int64)
         keep = [xyz, conn]
         d = ArrayMeshDesc()
-        d.element, d.polynomial_order = element, order
+        d.element, d.polynomial_order, d.geometry_order = element, order, geometry_order
         d.node_count, d.nodes = xyz.shape[0], xyz.ctypes.data
         d.element_count, d.elements = conn.shape[0], conn.ctypes.data
         if boundary_faces is not None and len(boundary_faces):

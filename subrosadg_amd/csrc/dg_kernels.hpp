@@ -108,7 +108,25 @@ struct Geo {
   const double* nodeav;   // nodes
   int nvert;
   int order;              // P
+  // curved (non-affine) elements, else null (vector kernels only): per element and volume point
+  // (J^T)^-1 |J| w (column-major D x D) and |J| w, the element mass inverse (Geometry.cpp:88-100)
+  // and the BR2 face blocks Vf_s M_e^-1 Vf_s^T
+  const double* qG;       // E x N_q x D*D
+  const double* qjw;      // E x N_q
+  const double* minvE;    // E x N_b x N_b
+  const double* LfE;      // E x N_f x N_fq x N_fq
 };
+
+// |J| w at volume point q of element e, and the (a, b) entry (i = a + D b) of (J^T)^-1 |J| w given
+// that value: from the element's constant Jacobian, or from the per-point tables of curved elements
+template <int NQ>
+__device__ __forceinline__ double pointJw(const Geo& g, int e, int q) {
+  return g.qjw ? g.qjw[static_cast<int64_t>(e) * NQ + q] : g.detj[e] * g.wq[q];
+}
+template <int D, int NQ>
+__device__ __forceinline__ double pointG(const Geo& g, int e, int q, int i, double jw) {
+  return g.qG ? g.qG[(static_cast<int64_t>(e) * NQ + q) * D * D + i] : g.ginv[static_cast<int64_t>(e) * D * D + i] * jw;
+}
 
 // P1 interpolant of the node values of element e at row `row` of a nodal table
 __device__ __forceinline__ double avAt(const Geo& g, const double* table, int row, int e) {
@@ -305,8 +323,13 @@ __device__ __forceinline__ void stageTail(const Geo& g, const StageArgs& a, int 
     const int e = e0 + le;
     if (e >= g.E) continue;
     double rm = 0.0;
-    for (int b = 0; b < NB; b++) rm += sR[le][v][b] * g.Minv[b * NB + i];
-    rm = rm / g.detj[e];
+    if (g.minvE) {
+      const double* M = g.minvE + static_cast<int64_t>(e) * NB * NB;
+      for (int b = 0; b < NB; b++) rm += sR[le][v][b] * M[b * NB + i];
+    } else {
+      for (int b = 0; b < NB; b++) rm += sR[le][v][b] * g.Minv[b * NB + i];
+      rm = rm / g.detj[e];
+    }
     const int64_t gi = static_cast<int64_t>(e) * NV * NB + rem;
     double u = sU[le][v][i] * a.c1;
     u += a.c0 * a.Un[gi];
@@ -375,9 +398,9 @@ __global__ __launch_bounds__(kThreads) void eulerStageKernel(Geo g, Phys ph, Sta
       for (int b = 0; b < D; b++) fc[aa + D * (1 + b)] = rho * w[1 + aa] * w[1 + b] + (aa == b ? p * 1.0 : p * 0.0);
       fc[aa + D * (D + 1)] = (rho * et + p) * w[1 + aa];
     }
-    const double jw = g.detj[e] * g.wq[q];
+    const double jw = pointJw<NQ>(g, e, q);
     double G[D * D];
-    for (int i = 0; i < D * D; i++) G[i] = g.ginv[static_cast<int64_t>(e) * D * D + i] * jw;
+    for (int i = 0; i < D * D; i++) G[i] = pointG<D, NQ>(g, e, q, i, jw);
     for (int v = 0; v < NV; v++)
       for (int b = 0; b < D; b++) {
         double s = 0.0;
@@ -537,12 +560,11 @@ __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageAr
       fj += sJ[le][r][sl] * vf;
     }
     double vol = 0.0;
-    const double dj = g.detj[ec];
     for (int q = 0; q < NQ; q++) {
       const double uk = sUq[le][k][q];
-      const double jw = dj * g.wq[q];
+      const double jw = pointJw<NQ>(g, ec, q);
       for (int b = 0; b < D; b++) {
-        const double G = g.ginv[static_cast<int64_t>(ec) * D * D + aa + D * b] * jw;
+        const double G = pointG<D, NQ>(g, ec, q, aa + D * b, jw);
         vol += (uk * G) * g.dV[(q * D + b) * NB + i];
       }
     }
@@ -555,16 +577,27 @@ __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageAr
     const int le = idx / (NG * NB), rem = idx % (NG * NB);
     const int r = rem / NB, i = rem % NB;
     const int e = e0 + le;
-    const double idj = 1.0 / g.detj[e < g.E ? e : g.E - 1];
+    const int ec = e < g.E ? e : g.E - 1;
     double wv = 0.0, wt = 0.0;
-    for (int b = 0; b < NB; b++) {
-      const double m = g.Minv[b * NB + i];
-      wv += sR1[le][r][b] * m;
-      wt += sR2[le][r][b] * m;
+    if (g.minvE) {  // curved: the element's own mass inverse
+      const double* M = g.minvE + static_cast<int64_t>(ec) * NB * NB;
+      for (int b = 0; b < NB; b++) {
+        wv += sR1[le][r][b] * M[b * NB + i];
+        wt += sR2[le][r][b] * M[b * NB + i];
+      }
+    } else {
+      const double idj = 1.0 / g.detj[ec];
+      for (int b = 0; b < NB; b++) {
+        const double m = g.Minv[b * NB + i];
+        wv += sR1[le][r][b] * m;
+        wt += sR2[le][r][b] * m;
+      }
+      wv *= idj;
+      wt *= idj;
     }
-    sWv[le][r][i] = wv * idj;
-    sWt[le][r][i] = wt * idj;
-    if (a.Wout && e < g.E) a.Wout[static_cast<int64_t>(e) * NG * NB + static_cast<int64_t>(i) * NG + r] = wt * idj;
+    sWv[le][r][i] = wv;
+    sWt[le][r][i] = wt;
+    if (a.Wout && e < g.E) a.Wout[static_cast<int64_t>(e) * NG * NB + static_cast<int64_t>(i) * NG + r] = wt;
   }
   __syncthreads();
   // total gradient at volume points and lifted gradient traces at face points (VISC == 0,
@@ -584,9 +617,16 @@ __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageAr
     if (VISC == 2) {
       for (int b = 0; b < NB; b++) t += sWv[le][r][b] * g.Vft[b * NAQ + sl];
       const int s = sl / NFQ, j = sl % NFQ;
+      const int ec = e < g.E ? e : g.E - 1;
       double l = 0.0;
-      for (int pp = 0; pp < NFQ; pp++) l += sJ[le][r][s * NFQ + pp] * g.Lf[(s * NFQ + pp) * NFQ + j];
-      t += l / g.detj[e < g.E ? e : g.E - 1];
+      if (g.LfE) {  // curved: the element's own face blocks
+        const double* Lb = g.LfE + static_cast<int64_t>(ec) * NAQ * NFQ;
+        for (int pp = 0; pp < NFQ; pp++) l += sJ[le][r][s * NFQ + pp] * Lb[(s * NFQ + pp) * NFQ + j];
+        t += l;
+      } else {
+        for (int pp = 0; pp < NFQ; pp++) l += sJ[le][r][s * NFQ + pp] * g.Lf[(s * NFQ + pp) * NFQ + j];
+        t += l / g.detj[ec];
+      }
     } else if (VISC == 0) {
       for (int b = 0; b < NB; b++) t += sWv[le][r][b] * g.Vft[b * NAQ + sl];
     } else {
@@ -621,9 +661,9 @@ __global__ __launch_bounds__(kThreads) void nsGradKernel(Geo g, Phys ph, StageAr
       for (int b = 0; b < D; b++) fc[aa + D * (1 + b)] = rho * w[1 + aa] * w[1 + b] + (aa == b ? p * 1.0 : p * 0.0);
       fc[aa + D * (D + 1)] = (rho * et + p) * w[1 + aa];
     }
-    const double jw = g.detj[e] * g.wq[q];
+    const double jw = pointJw<NQ>(g, e, q);
     double G[D * D];
-    for (int i = 0; i < D * D; i++) G[i] = g.ginv[static_cast<int64_t>(e) * D * D + i] * jw;
+    for (int i = 0; i < D * D; i++) G[i] = pointG<D, NQ>(g, e, q, i, jw);
     double* out = &sR2[le][0][0];
     if constexpr (VISC == 0) {  // F_c^T G - (eps grad U)^T G (SpatialDiscrete.cpp:211-249)
       const double eps = avAt(g, g.nodq, q, e);
@@ -737,7 +777,6 @@ __global__ __launch_bounds__(kThreads) void br2BoundaryBlockKernel(Geo g, Phys p
     }
   }
   __syncthreads();
-  const double dj = g.detj[e];
   for (int idx = tid; idx < NG * NB; idx += kThreads) {
     const int r = idx / NB, i = idx % NB;
     const int k = r / D, aa = r % D;
@@ -747,9 +786,9 @@ __global__ __launch_bounds__(kThreads) void br2BoundaryBlockKernel(Geo g, Phys p
     double vol = 0.0;
     for (int q = 0; q < NQ; q++) {
       const double uk = sUq[k][q];
-      const double jw = dj * g.wq[q];
+      const double jw = pointJw<NQ>(g, e, q);
       for (int b = 0; b < D; b++) {
-        const double G = g.ginv[static_cast<int64_t>(e) * D * D + aa + D * b] * jw;
+        const double G = pointG<D, NQ>(g, e, q, aa + D * b, jw);
         vol += (uk * G) * g.dV[(q * D + b) * NB + i];
       }
     }
@@ -757,16 +796,26 @@ __global__ __launch_bounds__(kThreads) void br2BoundaryBlockKernel(Geo g, Phys p
     sRl[r][i] = fj;
   }
   __syncthreads();
-  const double idj = 1.0 / dj;
   for (int idx = tid; idx < NG * NB; idx += kThreads) {
     const int r = idx / NB, i = idx % NB;
     double wv = 0.0, wl = 0.0;
-    for (int b = 0; b < NB; b++) {
-      const double m = g.Minv[b * NB + i];
-      wv += sRv[r][b] * m;
-      wl += sRl[r][b] * m;
+    if (g.minvE) {
+      const double* M = g.minvE + static_cast<int64_t>(e) * NB * NB;
+      for (int b = 0; b < NB; b++) {
+        wv += sRv[r][b] * M[b * NB + i];
+        wl += sRl[r][b] * M[b * NB + i];
+      }
+    } else {
+      const double idj = 1.0 / g.detj[e];
+      for (int b = 0; b < NB; b++) {
+        const double m = g.Minv[b * NB + i];
+        wv += sRv[r][b] * m;
+        wl += sRl[r][b] * m;
+      }
+      wv *= idj;
+      wl *= idj;
     }
-    out[static_cast<int64_t>(bf) * NG * NB + static_cast<int64_t>(i) * NG + r] = wv * idj + wl * idj;
+    out[static_cast<int64_t>(bf) * NG * NB + static_cast<int64_t>(i) * NG + r] = wv + wl;
   }
 }
 
@@ -1025,7 +1074,7 @@ __global__ __launch_bounds__(kThreads) void sourceKernel(Geo g, Phys ph, const d
         u[v] = x;
       }
       computational<D>(ph, u, w);
-      s = boussinesqSource<D>(ph, w) * (g.detj[e] * g.wq[q]);
+      s = boussinesqSource<D>(ph, w) * pointJw<NQ>(g, e, q);
     }
     sS[le][q] = s;
   }

@@ -23,9 +23,23 @@ const double kHexVertex[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, 
 // local face vertex lists (SimulationControl.cpp:177-216)
 const int kQuadFace[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
 const int kHexFace[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {0, 4, 7, 3}, {1, 2, 6, 5}, {2, 3, 7, 6}, {4, 5, 6, 7}};
-// P1 face element vertices: line (-1),(1); quad (-1,-1),(1,-1),(1,1),(-1,1)
+// P1 face element vertices: line (-1),(1); quad (-1,-1),(1,-1),(1,1),(-1,1); triangle (0,0),(1,0),(0,1)
 const double kLineFaceVertex[2] = {-1, 1};
 const double kQuadFaceVertex[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+// tetrahedron: gmsh reference vertices and local face vertex lists (SimulationControl.cpp:193-198),
+// each face ordered so that (v1 - v0) x (v2 - v0) points out of the element
+const double kTetVertex[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+const int kTetFace[4][3] = {{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {3, 1, 2}};
+// pyramid: gmsh reference vertices (base [-1,1]^2 at z = 0, apex (0,0,1)) and local faces
+// (SimulationControl.cpp:199-204): four triangles, then the quadrangle base; -1 pads a triangle
+const double kPyrVertex[5][3] = {{-1, -1, 0}, {1, -1, 0}, {1, 1, 0}, {-1, 1, 0}, {0, 0, 1}};
+const int kPyrFace[5][4] = {{0, 1, 4, -1}, {3, 0, 4, -1}, {1, 2, 4, -1}, {2, 3, 4, -1}, {0, 3, 2, 1}};
+const int kPyrEdge[8][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 4}, {2, 4}, {3, 4}};
+// element edges (vertex pairs) of the 2-D and 3-D elements, for minimum edge lengths
+const int kTriEdge[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+const int kQuadEdge[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+const int kTetEdge[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+const int kHexEdge12[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6}, {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
 
 // Tensor-product P1 shape function of a vertex with coordinates v (entries +-1) at xi, and its
 // gradient.
@@ -91,6 +105,103 @@ void triangleRule(int order, std::vector<double>& pts, std::vector<double>& w) {
   }
 }
 
+// Symmetric rules on the reference triangle of degree 3, 5, 7, 9 with 4, 7, 13, 19 points: the face
+// rules (order 2P+1, SimulationControl.cpp:275-283) of tetrahedra P = 1..4, with the orbit types of
+// the volume rules above; parameters solved from the moment equations (Dunavant's orbit structures)
+// in 50-digit arithmetic, exactness checked in tests/test_kit_tet.py.
+const TriOrbit kTriRule3[] = {{0, 0.0, 0.0, -0.28125}, {1, 0.2, 0.0, 0.26041666666666667}};
+// degree 5: Radon's rule, a = (6 +- sqrt 15) / 21, w = (155 +- sqrt 15) / 2400, centroid 9/80
+const TriOrbit kTriRule5[] = {{0, 0.0, 0.0, 0.1125},
+                              {1, 0.47014206410511509, 0.0, 0.066197076394253090},
+                              {1, 0.10128650732345634, 0.0, 0.062969590272413576}};
+const TriOrbit kTriRule7[] = {{0, 0.0, 0.0, -0.074785022233840875},
+                              {1, 0.26034596607903983, 0.0, 0.087807628716603906},
+                              {1, 0.065130102902215812, 0.0, 0.026673617804419246},
+                              {2, 0.048690315425316412, 0.31286549600487386, 0.03855688044512857}};
+const TriOrbit kTriRule9[] = {{0, 0.0, 0.0, 0.048567898141399417},
+                              {1, 0.48968251919873763, 0.0, 0.015667350113569535},
+                              {1, 0.43708959149293664, 0.0, 0.03891377050238714},
+                              {1, 0.18820353561903273, 0.0, 0.039823869463605127},
+                              {1, 0.04472951339445271, 0.0, 0.012788837829349016},
+                              {2, 0.036838412054736284, 0.2219629891607657, 0.021641769688644689}};
+
+void expandTriRule(const TriOrbit* r, int n, std::vector<double>& pts, std::vector<double>& w) {
+  pts.clear();
+  w.clear();
+  auto add = [&](double l1, double l2, double wt) {
+    pts.push_back(l1);
+    pts.push_back(l2);
+    w.push_back(wt);
+  };
+  for (int i = 0; i < n; i++) {
+    const TriOrbit& o = r[i];
+    if (o.kind == 0) {
+      add(1.0 / 3.0, 1.0 / 3.0, o.w);
+    } else if (o.kind == 1) {
+      const double c = 1.0 - 2.0 * o.a;
+      add(o.a, o.a, o.w);
+      add(o.a, c, o.w);
+      add(c, o.a, o.w);
+    } else {
+      const double c = 1.0 - o.a - o.b;
+      const double t[6][2] = {{o.b, c}, {c, o.a}, {o.a, o.b}, {o.b, o.a}, {c, o.b}, {o.a, c}};
+      for (int k = 0; k < 6; k++) add(t[k][0], t[k][1], o.w);
+    }
+  }
+}
+
+// face rule of a tetrahedron of order P: degree 2P+1 on the reference triangle
+void triangleFaceRule(int order, std::vector<double>& pts, std::vector<double>& w) {
+  if (order == 1) expandTriRule(kTriRule3, 2, pts, w);
+  else if (order == 2) expandTriRule(kTriRule5, 3, pts, w);
+  else if (order == 3) expandTriRule(kTriRule7, 4, pts, w);
+  else expandTriRule(kTriRule9, 6, pts, w);
+}
+
+// Grundmann-Moeller rule of degree 2s+1 on the reference simplex of dimension n (the origin and
+// the unit vectors; A. Grundmann, H. M. Moeller, SIAM J. Numer. Anal. 15 (1978)): for i = 0..s and
+// every composition beta of s - i into n + 1 parts, the point with barycentric coordinates
+// (2 beta_j + 1) / (d + n - 2i), weight proportional to (-1)^i (d + n - 2i)^d / (i! (d + n - i)!),
+// d = 2s + 1; weights scaled to the simplex volume 1/n!. Volume rules of tetrahedra (s = P).
+void grundmannMoeller(int n, int s, std::vector<double>& pts, std::vector<double>& w) {
+  pts.clear();
+  w.clear();
+  const int d = 2 * s + 1;
+  auto fact = [](int k) {
+    double f = 1.0;
+    for (int i = 2; i <= k; i++) f *= i;
+    return f;
+  };
+  std::vector<int> beta(static_cast<size_t>(n) + 1, 0);
+  for (int i = 0; i <= s; i++) {
+    const int m = s - i;
+    const double den = d + n - 2 * i;
+    const double coef = (i % 2 ? -1.0 : 1.0) * std::pow(den, d) / (fact(i) * fact(d + n - i));
+    // compositions of m into n + 1 parts, beta_1..beta_n enumerated, beta_0 the rest
+    std::fill(beta.begin(), beta.end(), 0);
+    while (true) {
+      int used = 0;
+      for (int j = 1; j <= n; j++) used += beta[j];
+      if (used <= m) {
+        beta[0] = m - used;
+        for (int j = 1; j <= n; j++) pts.push_back((2.0 * beta[j] + 1.0) / den);
+        w.push_back(coef);
+      }
+      int j = 1;
+      while (j <= n) {
+        if (++beta[j] <= m) break;
+        beta[j] = 0;
+        j++;
+      }
+      if (j > n) break;
+    }
+  }
+  double sum = 0.0;
+  for (double x : w) sum += x;
+  const double vol = 1.0 / fact(n);
+  for (double& x : w) x *= vol / sum;
+}
+
 // Legendre polynomials L_0..L_n and derivatives at x (any arithmetic type)
 template <typename T>
 void legendreSeries(int n, T x, T* L, T* dL) {
@@ -131,6 +242,150 @@ void triangleBasis(int order, T xi, T eta, T* value) {
       for (int i = 0; i <= k - 3; i++) value[b++] = lam[0] * lam[1] * lam[2] * L[i] * L2[k - 3 - i];
     }
   }
+}
+
+// Hierarchic modal basis of order P on the reference tetrahedron spanning P_P, built like the
+// triangle's: the 4 barycentric vertex functions, then per degree k = 2..P the 6 edge functions
+// lambda_a lambda_b kappa_{k-2}(lambda_b - lambda_a), the face functions
+// lambda_a lambda_b lambda_c L_i(lambda_b - lambda_a) L_j(lambda_c - lambda_b) (i + j = k - 3) of
+// the 4 faces, and the bubbles lambda_0 lambda_1 lambda_2 lambda_3 L_i L_j L_l of
+// (lambda_1 - lambda_0, lambda_2 - lambda_1, lambda_3 - lambda_2) with i + j + l = k - 4:
+// 4 + 6 (P-1) + 2 (P-1)(P-2) + (P-1)(P-2)(P-3)/6 = (P+1)(P+2)(P+3)/6 functions. The edge, face and
+// bubble functions are scaled by 4, 27 and 256 (the reciprocal maxima of lambda_a lambda_b,
+// lambda_a lambda_b lambda_c and the product of all four), which keeps the reference mass matrix
+// of P4 at a condition number near 1e4 instead of 3e6. The reference takes gmsh's H1Legendre basis
+// of the same space (BasisFunction.cpp:58-73).
+template <typename T>
+void tetBasis(int order, T x, T y, T z, T* value) {
+  const T lam[4] = {T(1.0) - x - y - z, x, y, z};
+  static const int kEdge[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+  static const int kFace[4][3] = {{0, 1, 2}, {0, 1, 3}, {1, 2, 3}, {0, 2, 3}};
+  T L[8], dL[8], L2[8], dL2[8], L3[8], dL3[8];
+  int b = 0;
+  for (int v = 0; v < 4; v++) value[b++] = lam[v];
+  for (int k = 2; k <= order; k++) {
+    for (int e = 0; e < 6; e++) {
+      const T l0 = lam[kEdge[e][0]], l1 = lam[kEdge[e][1]];
+      legendreSeries(k - 1, l1 - l0, L, dL);
+      const T kappa = T(-4.0 * (2.0 * k - 1.0) / (k * (k - 1.0) * std::sqrt(2.0 * (2.0 * k - 1.0)))) * dL[k - 1];
+      value[b++] = T(4.0) * l0 * l1 * kappa;
+    }
+    if (k >= 3)
+      for (int f = 0; f < 4; f++) {
+        const T la = lam[kFace[f][0]], lb = lam[kFace[f][1]], lc = lam[kFace[f][2]];
+        legendreSeries(k - 3, lb - la, L, dL);
+        legendreSeries(k - 3, lc - lb, L2, dL2);
+        for (int i = 0; i <= k - 3; i++) value[b++] = T(27.0) * la * lb * lc * L[i] * L2[k - 3 - i];
+      }
+    if (k >= 4) {
+      legendreSeries(k - 4, lam[1] - lam[0], L, dL);
+      legendreSeries(k - 4, lam[2] - lam[1], L2, dL2);
+      legendreSeries(k - 4, lam[3] - lam[2], L3, dL3);
+      for (int i = 0; i <= k - 4; i++)
+        for (int j = 0; i + j <= k - 4; j++)
+          value[b++] = T(256.0) * lam[0] * lam[1] * lam[2] * lam[3] * L[i] * L2[j] * L3[k - 4 - i - j];
+    }
+  }
+}
+
+// Gauss-Jacobi rule of n points for the weight (1 - t)^alpha on [-1, 1] (beta = 0): the nodes are the
+// eigenvalues of the Jacobi matrix of the monic recurrence and the weights mu0 times the squared
+// first eigenvector components (Golub-Welsch), by cyclic Jacobi rotations of the small matrix.
+void gaussJacobi(int n, double alpha, std::vector<double>& x, std::vector<double>& w) {
+  std::vector<double> A(static_cast<size_t>(n) * n, 0.0), Q(static_cast<size_t>(n) * n, 0.0);
+  const double beta = 0.0;
+  for (int k = 0; k < n; k++) {
+    const double s = 2.0 * k + alpha + beta;
+    A[k * n + k] = (beta * beta - alpha * alpha) / (s * (s + 2.0));
+    if (k > 0) {
+      const double num = 4.0 * k * (k + alpha) * (k + beta) * (k + alpha + beta);
+      const double den = s * s * (s + 1.0) * (s - 1.0);
+      A[k * n + k - 1] = A[(k - 1) * n + k] = std::sqrt(num / den);
+    }
+    Q[k * n + k] = 1.0;
+  }
+  if (n >= 1) A[0] = (beta - alpha) / (alpha + beta + 2.0);
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) off += A[p * n + q] * A[p * n + q];
+    if (off < 1e-34) break;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) {
+        const double apq = A[p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        const double theta = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < n; k++) {  // A <- A G
+          const double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - sn * akq;
+          A[k * n + q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {  // A <- G^T A
+          const double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - sn * aqk;
+          A[q * n + k] = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {  // Q <- Q G
+          const double qkp = Q[k * n + p], qkq = Q[k * n + q];
+          Q[k * n + p] = c * qkp - sn * qkq;
+          Q[k * n + q] = sn * qkp + c * qkq;
+        }
+      }
+  }
+  const double mu0 = std::pow(2.0, alpha + beta + 1.0) * std::tgamma(alpha + 1.0) * std::tgamma(beta + 1.0) /
+                     std::tgamma(alpha + beta + 2.0);
+  std::vector<std::pair<double, double>> nw;
+  for (int k = 0; k < n; k++) nw.push_back({A[k * n + k], mu0 * Q[k] * Q[k]});
+  std::sort(nw.begin(), nw.end());
+  x.clear();
+  w.clear();
+  for (const auto& p : nw) {
+    x.push_back(p.first);
+    w.push_back(p.second);
+  }
+}
+
+// Jacobi polynomials P_k^(alpha, 0)(t), k = 0..n (three-term recurrence)
+template <typename T>
+void jacobiSeries(int n, double alpha, T t, T* P) {
+  P[0] = T(1.0);
+  if (n == 0) return;
+  P[1] = (T(alpha + 2.0) * t + T(alpha)) / T(2.0);
+  for (int k = 2; k <= n; k++) {
+    const double a = k + alpha;  // beta = 0
+    const double c1 = 2.0 * k * a * (2.0 * k + alpha - 2.0);
+    const double c2 = (2.0 * k + alpha - 1.0) * (2.0 * k + alpha) * (2.0 * k + alpha - 2.0);
+    const double c3 = (2.0 * k + alpha - 1.0) * alpha * alpha;
+    const double c4 = 2.0 * (k + alpha - 1.0) * (k - 1.0) * (2.0 * k + alpha);
+    P[k] = ((T(c2) * t + T(c3)) * P[k - 1] - T(c4) * P[k - 2]) / T(c1);
+  }
+}
+
+// Modal basis of order P on the reference pyramid: the space of gmsh's pyramid elements (Bergot's
+// functions, (P+1)(P+2)(2P+3)/6 of them; the reference takes gmsh's Lagrange pyramid basis,
+// BasisFunction.cpp:62-67), phi_ijk = L_i(x/(1-z)) L_j(y/(1-z)) (1-z)^m P_k^(2m+2,0)(2z-1), m = max(i, j),
+// 0 <= k <= P - m; ordered by m + k, so the order P-1 basis is a prefix. Rational in z; the rules
+// never reach the apex.
+template <typename T>
+void pyramidBasis(int order, T x, T y, T z, T* value) {
+  const T s = T(1.0) - z;
+  const T xt = x / s, yt = y / s;
+  T Lx[8], dLx[8], Ly[8], dLy[8], J[8];
+  legendreSeries(order, xt, Lx, dLx);
+  legendreSeries(order, yt, Ly, dLy);
+  int b = 0;
+  for (int d = 0; d <= order; d++)
+    for (int m = 0; m <= d; m++) {
+      const int k = d - m;
+      jacobiSeries(k, 2.0 * m + 2.0, T(2.0) * z - T(1.0), J);
+      T sm = T(1.0);
+      for (int i = 0; i < m; i++) sm = sm * s;
+      const T radial = sm * J[k];
+      for (int j = 0; j <= m; j++) value[b++] = Lx[m] * Ly[j] * radial;
+      for (int i = 0; i < m; i++) value[b++] = Lx[i] * Ly[m] * radial;
+    }
 }
 
 double det(int dim, const double* a) {  // row-major
@@ -232,6 +487,32 @@ bool invertMatrix(int n, const double* a, double* inverse) {
 
 static void evalBasis(const RefElement& ref, const double* xi, double* value, double* grad /*N_b x D*/) {
   const int dim = ref.dim;
+  if (ref.type == 5) {  // pyramid: values, and gradients by complex-step differentiation
+    pyramidBasis<double>(ref.order, xi[0], xi[1], xi[2], value);
+    if (grad) {
+      const double h = 1e-30;
+      std::vector<std::complex<double>> cv(ref.nb);
+      for (int d = 0; d < 3; d++) {
+        const std::complex<double> x(xi[0], d == 0 ? h : 0.0), y(xi[1], d == 1 ? h : 0.0), z(xi[2], d == 2 ? h : 0.0);
+        pyramidBasis<std::complex<double>>(ref.order, x, y, z, cv.data());
+        for (int b = 0; b < ref.nb; b++) grad[b * 3 + d] = cv[b].imag() / h;
+      }
+    }
+    return;
+  }
+  if (ref.type == 4) {  // tetrahedron: values, and gradients by complex-step differentiation
+    tetBasis<double>(ref.order, xi[0], xi[1], xi[2], value);
+    if (grad) {
+      const double h = 1e-30;
+      std::vector<std::complex<double>> cv(ref.nb);
+      for (int d = 0; d < 3; d++) {
+        const std::complex<double> x(xi[0], d == 0 ? h : 0.0), y(xi[1], d == 1 ? h : 0.0), z(xi[2], d == 2 ? h : 0.0);
+        tetBasis<std::complex<double>>(ref.order, x, y, z, cv.data());
+        for (int b = 0; b < ref.nb; b++) grad[b * 3 + d] = cv[b].imag() / h;
+      }
+    }
+    return;
+  }
   if (ref.type == 2) {  // triangle: values, and gradients by complex-step differentiation
     triangleBasis<double>(ref.order, xi[0], xi[1], value);
     if (grad) {
@@ -268,6 +549,22 @@ void nodalValues(const RefElement& ref, const double* xi, double* out) {
     out[0] = 1.0 - xi[0] - xi[1];
     out[1] = xi[0];
     out[2] = xi[1];
+    return;
+  }
+  if (ref.type == 5) {  // the rational P1 pyramid functions: base (1 +- x)(1 +- y)/4 - z/4 + +-x y z/(4(1 - z)), apex z
+    const double x = xi[0], y = xi[1], z = xi[2], r = z < 1.0 ? x * y * z / (1.0 - z) : 0.0;
+    for (int v = 0; v < 4; v++) {
+      const double a = kPyrVertex[v][0], b = kPyrVertex[v][1];
+      out[v] = 0.25 * ((1.0 + a * x) * (1.0 + b * y) - z + a * b * r);
+    }
+    out[4] = z;
+    return;
+  }
+  if (ref.type == 4) {
+    out[0] = 1.0 - xi[0] - xi[1] - xi[2];
+    out[1] = xi[0];
+    out[2] = xi[1];
+    out[3] = xi[2];
     return;
   }
   double g[3];
@@ -370,13 +667,65 @@ bool buildRefElement(int type, int order, RefElement& ref, std::string& error) {
       ref.fweight.push_back(gw[a]);
     }
     ref.modes.assign(static_cast<size_t>((order + 1) * (order + 2) / 2), {0, 0, 0});
+  } else if (type == 4) {  // tetrahedron
+    if (order > 4) {
+      error = "tetrahedra are implemented for P = 1..4";
+      return false;
+    }
+    ref.dim = 3;
+    ref.nf = 4;
+    ref.nvert = 4;
+    ref.nfvert = 3;
+    for (int v = 0; v < 4; v++)
+      for (int d = 0; d < 3; d++) ref.vertex.push_back(kTetVertex[v][d]);
+    for (int f = 0; f < 4; f++)
+      for (int v = 0; v < 3; v++) ref.face_vertex.push_back(kTetFace[f][v]);
+    grundmannMoeller(3, order, ref.qpoint, ref.qweight);
+    triangleFaceRule(order, ref.fpoint, ref.fweight);
+    ref.modes.assign(static_cast<size_t>((order + 1) * (order + 2) * (order + 3) / 6), {0, 0, 0});
+  } else if (type == 5) {  // pyramid
+    if (order > 4) {
+      error = "pyramids are implemented for P = 1..4";
+      return false;
+    }
+    ref.dim = 3;
+    ref.nf = 5;
+    ref.nvert = 5;
+    ref.nfvert = 4;  // the largest face: the quadrangle base
+    for (int v = 0; v < 5; v++)
+      for (int d = 0; d < 3; d++) ref.vertex.push_back(kPyrVertex[v][d]);
+    for (int f = 0; f < 5; f++)
+      for (int v = 0; v < 4; v++) ref.face_vertex.push_back(kPyrFace[f][v]);
+    // volume rule of order 2P: collapsed Gauss -- x = a (1 - z), y = b (1 - z) with Gauss-Legendre
+    // in a, b and Gauss-Jacobi (weight (1 - z)^2) in z, (P+1)^3 points (the reference's counts,
+    // SimulationControl.cpp:265); exact for the mass matrix of the space above
+    std::vector<double> jx, jw;
+    gaussJacobi(n1, 2.0, jx, jw);
+    for (int c = 0; c < n1; c++)
+      for (int b = 0; b < n1; b++)
+        for (int a = 0; a < n1; a++) {
+          const double z = 0.5 * (1.0 + jx[c]);
+          ref.qpoint.push_back(gx[a] * (1.0 - z));
+          ref.qpoint.push_back(gx[b] * (1.0 - z));
+          ref.qpoint.push_back(z);
+          ref.qweight.push_back(gw[a] * gw[b] * jw[c] / 8.0);
+        }
+    // the face rule of the record (exports): the base's Gauss rule, second coordinate fastest
+    for (int a = 0; a < n1; a++)
+      for (int b = 0; b < n1; b++) {
+        ref.fpoint.push_back(gx[a]);
+        ref.fpoint.push_back(gx[b]);
+        ref.fweight.push_back(gw[a] * gw[b]);
+      }
+    ref.modes.assign(static_cast<size_t>(basisSize(5, 3, order)), {0, 0, 0});
   } else {
-    error = "unsupported element type (line=1, triangle=2, quadrangle=3 and hexahedron=6 are implemented)";
+    error = "unsupported element type (line=1, triangle=2, quadrangle=3, tetrahedron=4, pyramid=5 and hexahedron=6 "
+            "are implemented)";
     return false;
   }
   // hierarchical ordering: sort modes by (max index, then slowest-to-fastest index) so that the
   // first N_b(P-1) functions span the order P-1 space
-  if (type != 2)
+  if (type != 2 && type != 4 && type != 5)
   std::stable_sort(ref.modes.begin(), ref.modes.end(), [](const std::array<int, 3>& a, const std::array<int, 3>& b) {
     const int ma = std::max(a[0], std::max(a[1], a[2]));
     const int mb = std::max(b[0], std::max(b[1], b[2]));
@@ -400,21 +749,46 @@ bool buildRefElement(int type, int order, RefElement& ref, std::string& error) {
       for (int d = 0; d < dim; d++) ref.dV[(static_cast<size_t>(q) * dim + d) * nb + b] = grad[b * dim + d];
     }
   }
+  // per face: vertex count, slot points and weights (pyramids: the triangle rule padded with
+  // zero-weight copies of its first point)
+  const int fd = dim > 1 ? dim - 1 : 0;
+  ref.face_nv.assign(nf, ref.nfvert);
+  ref.face_nreal.assign(nf, nfq);
+  ref.fslot_point.assign(static_cast<size_t>(nf) * nfq * fd, 0.0);
+  ref.fslot_weight.assign(static_cast<size_t>(nf) * nfq, 0.0);
+  std::vector<double> tp, tw;
+  if (type == 5) triangleFaceRule(order, tp, tw);
+  for (int f = 0; f < nf; f++) {
+    const bool tri5 = type == 5 && f < 4;
+    if (tri5) {
+      ref.face_nv[f] = 3;
+      ref.face_nreal[f] = static_cast<int>(tw.size());
+    }
+    for (int j = 0; j < nfq; j++) {
+      const int src = tri5 ? (j < static_cast<int>(tw.size()) ? j : 0) : j;
+      for (int d = 0; d < fd; d++)
+        ref.fslot_point[(static_cast<size_t>(f) * nfq + j) * fd + d] = tri5 ? tp[static_cast<size_t>(src) * 2 + d] : ref.fpoint[static_cast<size_t>(j) * fd + d];
+      ref.fslot_weight[static_cast<size_t>(f) * nfq + j] = tri5 ? (j < static_cast<int>(tw.size()) ? tw[src] : 0.0) : ref.fweight[j];
+    }
+  }
   // face points in parent coordinates: P1 map of the face rule through the face vertices
   ref.face_param.assign(static_cast<size_t>(nf) * nfq * dim, 0.0);
   ref.Vf.assign(static_cast<size_t>(nf) * nfq * nb, 0.0);
   for (int f = 0; f < nf; f++) {
     for (int j = 0; j < nfq; j++) {
       double xi[3] = {0, 0, 0};
-      for (int v = 0; v < ref.nfvert; v++) {
+      const double* fp = fd ? &ref.fslot_point[(static_cast<size_t>(f) * nfq + j) * fd] : nullptr;
+      for (int v = 0; v < ref.face_nv[f]; v++) {
         double n = 1.0, g[2];
         if (dim == 1) {
           n = 1.0;  // the face is the vertex itself
         } else if (dim == 2) {
           const double fv[1] = {kLineFaceVertex[v]};
-          p1Shape(1, fv, &ref.fpoint[j], n, g);
+          p1Shape(1, fv, fp, n, g);
+        } else if (ref.face_nv[f] == 3) {  // triangle face: barycentric P1 functions
+          n = v == 0 ? 1.0 - fp[0] - fp[1] : fp[v - 1];
         } else {
-          p1Shape(2, kQuadFaceVertex[v], &ref.fpoint[static_cast<size_t>(j) * 2], n, g);
+          p1Shape(2, kQuadFaceVertex[v], fp, n, g);
         }
         const int lv = ref.face_vertex[f * ref.nfvert + v];
         for (int d = 0; d < dim; d++) xi[d] += n * ref.vertex[lv * dim + d];
@@ -484,6 +858,20 @@ namespace {
 // and the Jacobian J(l,k) = dx_l/dxi_k
 void mapPointRef(const RefElement& ref, const double* xv_all, const double* xi, double* x, double* jac) {
   const int dim = ref.dim;
+  if (ref.type == 5) {  // pyramid with a parallelogram base: x = c + a xi + b eta + d zeta (affine)
+    const double* v = xv_all;
+    for (int l = 0; l < 3; l++) {
+      const double c = 0.5 * (v[l] + v[2 * 3 + l]);
+      const double a = 0.5 * (v[3 + l] - v[l]), b = 0.5 * (v[3 * 3 + l] - v[l]), d = v[4 * 3 + l] - c;
+      x[l] = c + a * xi[0] + b * xi[1] + d * xi[2];
+      if (jac) {
+        jac[l * 3 + 0] = a;
+        jac[l * 3 + 1] = b;
+        jac[l * 3 + 2] = d;
+      }
+    }
+    return;
+  }
   for (int d = 0; d < dim; d++) x[d] = 0.0;
   if (jac)
     for (int i = 0; i < dim * dim; i++) jac[i] = 0.0;
@@ -495,6 +883,9 @@ void mapPointRef(const RefElement& ref, const double* xv_all, const double* xi, 
       n = N[v];
       g[0] = G[v][0];
       g[1] = G[v][1];
+    } else if (ref.type == 4) {
+      n = v == 0 ? 1.0 - xi[0] - xi[1] - xi[2] : xi[v - 1];
+      for (int d = 0; d < 3; d++) g[d] = v == 0 ? -1.0 : (d == v - 1 ? 1.0 : 0.0);
     } else {
       p1Shape(dim, &ref.vertex[static_cast<size_t>(v) * dim], xi, n, g);
     }
@@ -507,11 +898,31 @@ void mapPointRef(const RefElement& ref, const double* xv_all, const double* xi, 
   }
 }
 void mapPoint(const Mesh& mesh, int64_t e, const double* xi, double* x, double* jac) {
-  mapPointRef(mesh.ref, &mesh.vertex[static_cast<size_t>(e) * mesh.ref.nvert * mesh.ref.dim], xi, x, jac);
+  if (mesh.gnode.empty()) {
+    mapPointRef(mesh.ref, &mesh.vertex[static_cast<size_t>(e) * mesh.ref.nvert * mesh.ref.dim], xi, x, jac);
+    return;
+  }
+  // curved element: x = sum_k N_k(xi) X_k, J(l, k) = sum_n X_n,l dN_n / dxi_k
+  const GeomMap& gm = mesh.geom;
+  const int dim = mesh.ref.dim, n = gm.n;
+  double N[64], dN[64 * 3];
+  geomShape(gm, xi, N, dN);
+  const double* X = &mesh.gnode[static_cast<size_t>(e) * n * dim];
+  for (int l = 0; l < dim; l++) {
+    double acc = 0.0;
+    for (int k = 0; k < n; k++) acc += N[k] * X[k * dim + l];
+    x[l] = acc;
+    if (jac)
+      for (int kk = 0; kk < dim; kk++) {
+        double j = 0.0;
+        for (int k = 0; k < n; k++) j += X[k * dim + l] * dN[k * dim + kk];
+        jac[l * dim + kk] = j;
+      }
+  }
 }
 // normals (outward from element e through its local face s) and |J_f| w at the face points, from
 // the element's affine face map (Geometry.cpp:69-86,102-168)
-void faceNormalsRef(const RefElement& ref, const double* J, int s, Face& face) {
+void faceNormalsRef(const RefElement& ref, const double* Jall, int jstride, int s, Face& face) {
   const int dim = ref.dim, nfq = ref.nfq;
   face.normal.assign(static_cast<size_t>(nfq) * dim, 0.0);
   face.jw.assign(nfq, 0.0);
@@ -520,15 +931,21 @@ void faceNormalsRef(const RefElement& ref, const double* J, int s, Face& face) {
     face.jw[0] = ref.fweight[0];
     return;
   }
+  const int fd = dim - 1;
   for (int p = 0; p < nfq; p++) {
+    const double* J = Jall + static_cast<size_t>(p) * jstride;
+    const double* fp = &ref.fslot_point[(static_cast<size_t>(s) * nfq + p) * fd];
     double ts[3] = {0, 0, 0}, tt[3] = {0, 0, 0};  // d xi / d s, d xi / d t
-    for (int v = 0; v < ref.nfvert; v++) {
+    for (int v = 0; v < ref.face_nv[s]; v++) {
       double nval, g[2];
       if (dim == 2) {
         const double fv[1] = {kLineFaceVertex[v]};
-        p1Shape(1, fv, &ref.fpoint[p], nval, g);
+        p1Shape(1, fv, fp, nval, g);
+      } else if (ref.face_nv[s] == 3) {  // triangle face: gradients of the barycentric P1 functions
+        g[0] = v == 0 ? -1.0 : (v == 1 ? 1.0 : 0.0);
+        g[1] = v == 0 ? -1.0 : (v == 2 ? 1.0 : 0.0);
       } else {
-        p1Shape(2, kQuadFaceVertex[v], &ref.fpoint[static_cast<size_t>(p) * 2], nval, g);
+        p1Shape(2, kQuadFaceVertex[v], fp, nval, g);
       }
       const int lv = ref.face_vertex[s * ref.nfvert + v];
       for (int d = 0; d < dim; d++) {
@@ -555,13 +972,184 @@ void faceNormalsRef(const RefElement& ref, const double* J, int s, Face& face) {
     for (int d = 0; d < dim; d++) norm += nv[d] * nv[d];
     norm = std::sqrt(norm);
     for (int d = 0; d < dim; d++) face.normal[static_cast<size_t>(p) * dim + d] = nv[d] / norm;
-    face.jw[p] = norm * ref.fweight[p];
+    face.jw[p] = norm * ref.fslot_weight[static_cast<size_t>(s) * nfq + p];
   }
 }
 void faceNormals(const Mesh& mesh, int64_t e, int s, Face& face) {
-  faceNormalsRef(mesh.ref, &mesh.jac[static_cast<size_t>(e) * mesh.ref.dim * mesh.ref.dim], s, face);
+  const RefElement& ref = mesh.ref;
+  const int dim = ref.dim;
+  if (mesh.gnode.empty() || dim == 1) {
+    faceNormalsRef(ref, &mesh.jac[static_cast<size_t>(e) * dim * dim], 0, s, face);
+    return;
+  }
+  // curved element: the Jacobian of the element map at each face point
+  std::vector<double> J(static_cast<size_t>(ref.nfq) * dim * dim);
+  double x[3];
+  for (int p = 0; p < ref.nfq; p++)
+    mapPoint(mesh, e, &ref.face_param[(static_cast<size_t>(s) * ref.nfq + p) * dim], x, &J[static_cast<size_t>(p) * dim * dim]);
+  faceNormalsRef(ref, J.data(), dim * dim, s, face);
 }
 }  // namespace
+
+namespace {
+// lattice points of a triangle of order g in gmsh's order: the vertices (0,0), (g,0), (0,g), the
+// interior points of the edges (0,1), (1,2), (2,0) in the edge's direction, then the interior as a
+// triangle of order g - 3 shifted by (1, 1)
+void triangleLattice(int g, std::vector<std::array<int, 3>>& out) {
+  if (g < 0) return;
+  if (g == 0) {
+    out.push_back({0, 0, 0});
+    return;
+  }
+  out.push_back({0, 0, 0});
+  out.push_back({g, 0, 0});
+  out.push_back({0, g, 0});
+  for (int k = 1; k < g; k++) out.push_back({k, 0, 0});
+  for (int k = 1; k < g; k++) out.push_back({g - k, k, 0});
+  for (int k = 1; k < g; k++) out.push_back({0, g - k, 0});
+  std::vector<std::array<int, 3>> inner;
+  triangleLattice(g - 3, inner);
+  for (const auto& q : inner) out.push_back({q[0] + 1, q[1] + 1, 0});
+}
+// the same for a quadrangle: vertices (0,0), (g,0), (g,g), (0,g), edges (0,1), (1,2), (2,3), (3,0),
+// then the interior as a quadrangle of order g - 2 shifted by (1, 1)
+void quadrangleLattice(int g, std::vector<std::array<int, 3>>& out) {
+  if (g < 0) return;
+  if (g == 0) {
+    out.push_back({0, 0, 0});
+    return;
+  }
+  out.push_back({0, 0, 0});
+  out.push_back({g, 0, 0});
+  out.push_back({g, g, 0});
+  out.push_back({0, g, 0});
+  for (int k = 1; k < g; k++) out.push_back({k, 0, 0});
+  for (int k = 1; k < g; k++) out.push_back({g, k, 0});
+  for (int k = 1; k < g; k++) out.push_back({g - k, g, 0});
+  for (int k = 1; k < g; k++) out.push_back({0, g - k, 0});
+  std::vector<std::array<int, 3>> inner;
+  quadrangleLattice(g - 2, inner);
+  for (const auto& q : inner) out.push_back({q[0] + 1, q[1] + 1, 0});
+}
+}  // namespace
+
+bool buildGeomMap(int type, int order, GeomMap& g, std::string& error) {
+  g = GeomMap();
+  g.type = type;
+  g.order = order;
+  if (order < 1) {
+    error = "geometry order must be >= 1";
+    return false;
+  }
+  std::vector<double>& x = g.node;
+  if (type == 2 || type == 3) {  // lattice nodes
+    g.dim = 2;
+    if (order > 5) {
+      error = "triangles and quadrangles of geometry order 1..5 are read";
+      return false;
+    }
+    std::vector<std::array<int, 3>> lat;
+    if (type == 2) triangleLattice(order, lat);
+    else quadrangleLattice(order, lat);
+    for (const auto& q : lat)
+      for (int d = 0; d < 2; d++) x.push_back(type == 2 ? double(q[d]) / order : -1.0 + 2.0 * q[d] / order);
+  } else if (type == 6 || type == 4) {
+    g.dim = 3;
+    if (order > 2) {
+      error = "hexahedra and tetrahedra of geometry order 1 or 2 are read";
+      return false;
+    }
+    const int nv = type == 6 ? 8 : 4;
+    for (int v = 0; v < nv; v++)
+      for (int d = 0; d < 3; d++) x.push_back(type == 6 ? kHexVertex[v][d] : kTetVertex[v][d]);
+    if (order == 2) {
+      auto mid = [&](const std::vector<int>& vs) {
+        for (int d = 0; d < 3; d++) {
+          double c = 0.0;
+          for (int v : vs) c += type == 6 ? kHexVertex[v][d] : kTetVertex[v][d];
+          x.push_back(c / vs.size());
+        }
+      };
+      if (type == 6) {  // 27-node hexahedron: 12 edge, 6 face and the centre nodes
+        static const int kE[12][2] = {{0, 1}, {0, 3}, {0, 4}, {1, 2}, {1, 5}, {2, 3},
+                                      {2, 6}, {3, 7}, {4, 5}, {4, 7}, {5, 6}, {6, 7}};
+        for (const auto& e : kE) mid({e[0], e[1]});
+        for (const auto& f : kHexFace) mid({f[0], f[1], f[2], f[3]});
+        mid({0, 1, 2, 3, 4, 5, 6, 7});
+      } else {  // 10-node tetrahedron: edge midpoints
+        static const int kE[6][2] = {{0, 1}, {1, 2}, {2, 0}, {3, 0}, {3, 2}, {3, 1}};
+        for (const auto& e : kE) mid({e[0], e[1]});
+      }
+    }
+  } else if (type == 5) {  // pyramids: first order, affine (parallelogram base) only
+    g.dim = 3;
+    if (order != 1) {
+      error = "pyramids of geometry order 1 are read";
+      return false;
+    }
+    for (int v = 0; v < 5; v++)
+      for (int d = 0; d < 3; d++) x.push_back(kPyrVertex[v][d]);
+    g.n = 5;
+    return true;  // no interpolation space: the element map is the affine one of mapPointRef
+  } else {
+    error = "curved elements are read for triangles, quadrangles, tetrahedra and hexahedra";
+    return false;
+  }
+  const int dim = g.dim;
+  g.n = static_cast<int>(x.size()) / dim;
+  // monomials: total degree <= order on simplices, degree <= order per coordinate otherwise
+  const bool simplex = type == 2 || type == 4;
+  for (int c = 0; c <= (dim == 3 ? order : 0); c++)
+    for (int b = 0; b <= order; b++)
+      for (int a = 0; a <= order; a++)
+        if (!simplex || a + b + c <= order) g.expo.push_back({a, b, c});
+  if (static_cast<int>(g.expo.size()) != g.n) {
+    error = "geometry node count does not match the interpolation space";
+    return false;
+  }
+  const int n = g.n;
+  std::vector<double> A(static_cast<size_t>(n) * n);
+  for (int i = 0; i < n; i++)
+    for (int m = 0; m < n; m++) {
+      double v = 1.0;
+      for (int d = 0; d < dim; d++) v *= std::pow(x[static_cast<size_t>(i) * dim + d], g.expo[m][d]);
+      A[static_cast<size_t>(i) * n + m] = v;
+    }
+  g.ainv.assign(static_cast<size_t>(n) * n, 0.0);
+  if (!invertMatrix(n, A.data(), g.ainv.data())) {
+    error = "singular geometry interpolation";
+    return false;
+  }
+  return true;
+}
+
+void geomShape(const GeomMap& g, const double* xi, double* N, double* dN) {
+  const int n = g.n, dim = g.dim;
+  double mono[64], dmono[64 * 3];
+  for (int m = 0; m < n; m++) {
+    double v = 1.0;
+    for (int d = 0; d < dim; d++) v *= std::pow(xi[d], g.expo[m][d]);
+    mono[m] = v;
+    for (int k = 0; k < dim; k++) {
+      const int ek = g.expo[m][k];
+      double dv = ek == 0 ? 0.0 : ek * std::pow(xi[k], ek - 1);
+      for (int d = 0; d < dim; d++)
+        if (d != k) dv *= std::pow(xi[d], g.expo[m][d]);
+      dmono[m * dim + k] = dv;
+    }
+  }
+  for (int k = 0; k < n; k++) {
+    double v = 0.0, dv[3] = {0.0, 0.0, 0.0};
+    for (int m = 0; m < n; m++) {
+      const double a = g.ainv[static_cast<size_t>(m) * n + k];
+      v += a * mono[m];
+      for (int d = 0; d < dim; d++) dv[d] += a * dmono[m * dim + d];
+    }
+    N[k] = v;
+    if (dN)
+      for (int d = 0; d < dim; d++) dN[k * dim + d] = dv[d];
+  }
+}
 
 bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3], const double hi[3],
                          const int periodic[3], const int bc[6], Mesh& mesh, std::string& error) {
@@ -753,37 +1341,60 @@ bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3]
 bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double* xyz, int64_t elements,
                                const int64_t* conn, int64_t nbface, const int64_t* bconn, const int* btype,
                                const int* bphysical, Mesh& mesh, std::string& error,
-                               const std::vector<std::map<int64_t, int64_t>>* periodic_links) {
+                               const std::vector<std::map<int64_t, int64_t>>* periodic_links, int geometry_order) {
   mesh = Mesh();
-  if (type != 2 && type != 3 && type != 6) {
-    error = "element type must be triangle, quadrangle or hexahedron";
+  if (type != 2 && type != 3 && type != 4 && type != 5 && type != 6) {
+    error = "element type must be triangle, quadrangle, tetrahedron, pyramid or hexahedron";
     return false;
   }
   if (!buildRefElement(type, order, mesh.ref, error)) return false;
+  GeomMap gm;
+  if (!buildGeomMap(type, geometry_order, gm, error)) return false;
   const RefElement& ref = mesh.ref;
-  const int dim = ref.dim, nv = ref.nvert, nf = ref.nf, nfq = ref.nfq, nfv = ref.nfvert;
+  const int dim = ref.dim, nv = ref.nvert, nf = ref.nf, nfq = ref.nfq, nfv = ref.nfvert, ngn = gm.n;
   if (elements < 1 || elements > (int64_t(1) << 30)) {
     error = "element count out of range";
     return false;
   }
   const int64_t E = elements;
   mesh.elements = E;
-  std::vector<int64_t> cv(conn, conn + E * nv);
-  for (int64_t i = 0; i < E * nv; i++)
-    if (cv[i] < 0 || cv[i] >= nnode) {
+  for (int64_t i = 0; i < E * ngn; i++)
+    if (conn[i] < 0 || conn[i] >= nnode) {
       error = "element node index out of range";
       return false;
     }
+  std::vector<int64_t> cn(conn, conn + E * ngn);  // element nodes (mirrored elements are reordered)
+  std::vector<int64_t> cv(static_cast<size_t>(E) * nv);  // vertices: the first nodes of each element
+  for (int64_t e = 0; e < E; e++)
+    for (int v = 0; v < nv; v++) cv[static_cast<size_t>(e) * nv + v] = cn[static_cast<size_t>(e) * ngn + v];
+  // mirror of an inverted element: the map with the first two reference coordinates swapped covers
+  // the same cell with the opposite orientation; node k of the mirrored list is the old node at the
+  // swapped position of node k (on the vertices: (0,2,1), (0,3,2,1), (0,2,1,3), (0,3,2,1,4,7,6,5))
+  std::vector<int> gmirror(ngn, -1);
+  for (int k = 0; k < ngn; k++) {
+    double sw[3];
+    for (int d = 0; d < dim; d++) sw[d] = gm.node[static_cast<size_t>(k) * dim + d];
+    std::swap(sw[0], sw[1]);
+    for (int j = 0; j < ngn; j++) {
+      double d2 = 0.0;
+      for (int d = 0; d < dim; d++) d2 += std::pow(gm.node[static_cast<size_t>(j) * dim + d] - sw[d], 2);
+      if (d2 < 1e-20) gmirror[k] = j;
+    }
+  }
   mesh.vertex.assign(static_cast<size_t>(E) * nv * dim, 0.0);
   mesh.jac.assign(static_cast<size_t>(E) * dim * dim, 0.0);
   mesh.detj.assign(E, 0.0);
   mesh.min_edge.assign(E, 0.0);
-  static const int kMirror[3][8] = {{0, 2, 1}, {0, 3, 2, 1}, {0, 3, 2, 1, 4, 7, 6, 5}};
-  const int* mirror = kMirror[type == 2 ? 0 : (type == 3 ? 1 : 2)];
-  const double xi_c[3] = {type == 2 ? 1.0 / 3.0 : 0.0, type == 2 ? 1.0 / 3.0 : 0.0, 0.0};
-  static const int kHexEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
-                                      {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+  const double c3 = type == 2 ? 1.0 / 3.0 : (type == 4 ? 0.25 : 0.0);
+  const double xi_c[3] = {c3, c3, type == 4 || type == 5 ? 0.25 : 0.0};
+  const int(*edges)[2] = type == 2 ? kTriEdge
+                         : type == 3 ? kQuadEdge
+                         : type == 4 ? kTetEdge
+                         : type == 5 ? kPyrEdge
+                                     : kHexEdge12;
+  const int nedge = type == 2 ? 3 : (type == 3 ? 4 : (type == 4 ? 6 : (type == 5 ? 8 : 12)));
   double xmax = 0.0;
+  bool curved = false;
   for (int64_t e = 0; e < E; e++) {
     for (int pass = 0; pass < 2; pass++) {
       for (int v = 0; v < nv; v++)
@@ -797,18 +1408,15 @@ bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double*
         error = "degenerate element";
         return false;
       }
-      int64_t tmp[8];
-      for (int v = 0; v < nv; v++) tmp[v] = cv[static_cast<size_t>(e) * nv + mirror[v]];
-      for (int v = 0; v < nv; v++) cv[static_cast<size_t>(e) * nv + v] = tmp[v];
+      std::vector<int64_t> tmp(cn.begin() + e * ngn, cn.begin() + (e + 1) * ngn);
+      for (int k = 0; k < ngn; k++) cn[static_cast<size_t>(e) * ngn + k] = tmp[static_cast<size_t>(gmirror[k])];
+      for (int v = 0; v < nv; v++) cv[static_cast<size_t>(e) * nv + v] = cn[static_cast<size_t>(e) * ngn + v];
     }
-    // affine: the map through the centre and its Jacobian reproduces every vertex
     double xc[3];
     mapPoint(mesh, e, xi_c, xc, nullptr);
     double me = std::numeric_limits<double>::max(), span = 0.0;
-    const int ne = dim == 2 ? nf : 12;
-    for (int g = 0; g < ne; g++) {
-      const int a = dim == 2 ? ref.face_vertex[g * 2] : kHexEdge[g][0];
-      const int b = dim == 2 ? ref.face_vertex[g * 2 + 1] : kHexEdge[g][1];
+    for (int g = 0; g < nedge; g++) {
+      const int a = edges[g][0], b = edges[g][1];
       double s2 = 0.0;
       for (int d = 0; d < dim; d++) {
         const double t = mesh.vertex[(static_cast<size_t>(e) * nv + a) * dim + d] - mesh.vertex[(static_cast<size_t>(e) * nv + b) * dim + d];
@@ -818,34 +1426,65 @@ bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double*
       span = std::max(span, std::sqrt(s2));
     }
     mesh.min_edge[e] = me;
+    // affine: the map through the centre and its Jacobian reproduces every geometry node
     const double* J = &mesh.jac[static_cast<size_t>(e) * dim * dim];
-    for (int v = 0; v < nv; v++) {
+    for (int k = 0; k < ngn && !curved; k++) {
+      const int64_t node = cn[static_cast<size_t>(e) * ngn + k];
       double err2 = 0.0;
       for (int l = 0; l < dim; l++) {
         double x = xc[l];
-        for (int k = 0; k < dim; k++) x += J[l * dim + k] * (ref.vertex[static_cast<size_t>(v) * dim + k] - xi_c[k]);
-        const double t = x - mesh.vertex[(static_cast<size_t>(e) * nv + v) * dim + l];
+        for (int kk = 0; kk < dim; kk++) x += J[l * dim + kk] * (gm.node[static_cast<size_t>(k) * dim + kk] - xi_c[kk]);
+        const double t = x - xyz[node * 3 + l];
         err2 += t * t;
-        xmax = std::max(xmax, std::fabs(mesh.vertex[(static_cast<size_t>(e) * nv + v) * dim + l]));
+        xmax = std::max(xmax, std::fabs(xyz[node * 3 + l]));
       }
-      if (err2 > 1e-20 * span * span) {
-        error = "non-affine element (the engine needs triangles, parallelograms or parallelepipeds)";
-        return false;
+      if (err2 > 1e-20 * span * span) curved = true;
+    }
+  }
+  if (curved && type == 5) {
+    error = "a pyramid is not affine (its base must be a parallelogram)";
+    return false;
+  }
+  if (curved) {
+    // every element through its geometry nodes (the reference's per-point Jacobians,
+    // Geometry.cpp:44-67, and per-element mass matrices, Geometry.cpp:88-100)
+    mesh.affine = false;
+    mesh.geom = gm;
+    mesh.gnode.assign(static_cast<size_t>(E) * ngn * dim, 0.0);
+    for (int64_t e = 0; e < E; e++)
+      for (int k = 0; k < ngn; k++)
+        for (int d = 0; d < dim; d++) mesh.gnode[(static_cast<size_t>(e) * ngn + k) * dim + d] = xyz[cn[static_cast<size_t>(e) * ngn + k] * 3 + d];
+    std::vector<double> jq(static_cast<size_t>(dim) * dim);
+    for (int64_t e = 0; e < E; e++) {
+      double x[3];
+      mapPoint(mesh, e, xi_c, x, &mesh.jac[static_cast<size_t>(e) * dim * dim]);
+      mesh.detj[e] = det(dim, &mesh.jac[static_cast<size_t>(e) * dim * dim]);
+      for (int q = 0; q < ref.nq; q++) {
+        mapPoint(mesh, e, &ref.qpoint[static_cast<size_t>(q) * dim], x, jq.data());
+        if (!(det(dim, jq.data()) > 0.0)) {
+          error = "a curved element has a non-positive Jacobian at a quadrature point";
+          return false;
+        }
       }
     }
   }
   // faces keyed by their sorted node indices
+  // (a face of fewer vertices than nfv -- a pyramid's triangle -- is padded with -1)
   auto keyOf = [&](const int64_t* nodes) {
     std::array<int64_t, 4> k = {-1, -1, -1, -1};
-    for (int v = 0; v < nfv; v++) k[v] = nodes[v];
+    for (int v = 0; v < nfv; v++) k[v] = nodes[v] < 0 ? -1 : nodes[v];
     std::sort(k.begin(), k.begin() + nfv);
     return k;
+  };
+  auto faceNodes = [&](int64_t e, int s, int64_t* fn) {
+    for (int v = 0; v < 4; v++) fn[v] = -1;
+    for (int v = 0; v < ref.face_nv[s]; v++) fn[v] = cv[static_cast<size_t>(e) * nv + ref.face_vertex[s * nfv + v]];
   };
   std::map<std::array<int64_t, 4>, std::vector<std::pair<int64_t, int>>> visits;
   for (int64_t e = 0; e < E; e++)
     for (int s = 0; s < nf; s++) {
       int64_t fn[4];
-      for (int v = 0; v < nfv; v++) fn[v] = cv[static_cast<size_t>(e) * nv + ref.face_vertex[s * nfv + v]];
+      faceNodes(e, s, fn);
       visits[keyOf(fn)].push_back({e, s});
     }
   std::map<std::array<int64_t, 4>, std::pair<int, int>> boundary_type;  // type, physical tag
@@ -867,7 +1506,8 @@ bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double*
         const auto& link = (*periodic_links)[l];
         int64_t mapped[4];
         bool slave = true;
-        for (int v = 0; v < nfv && slave; v++) {
+        for (int v = 0; v < 4; v++) mapped[v] = -1;
+        for (int v = 0; v < ref.face_nv[s] && slave; v++) {
           const auto it = link.find(cv[static_cast<size_t>(e) * nv + ref.face_vertex[s * nfv + v]]);
           slave = it != link.end();
           if (slave) mapped[v] = it->second;
@@ -892,7 +1532,7 @@ bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double*
     for (int s = 0; s < nf; s++) {
       if (mesh.elem_face[static_cast<size_t>(e) * nf + s] != -1) continue;
       int64_t fn[4];
-      for (int v = 0; v < nfv; v++) fn[v] = cv[static_cast<size_t>(e) * nv + ref.face_vertex[s * nfv + v]];
+      faceNodes(e, s, fn);
       const auto key = keyOf(fn);
       std::vector<std::pair<int64_t, int>> list = visits[key];
       const auto pp = partner.find({e, s});
@@ -936,14 +1576,16 @@ bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double*
       // translation between the two faces (zero unless the face is periodic): difference of the
       // face-vertex centroids
       double shift[3] = {0.0, 0.0, 0.0};
-      for (int v = 0; v < nfv; v++)
+      const int fnv = ref.face_nv[s], nreal = ref.face_nreal[s];
+      for (int v = 0; v < fnv; v++)
         for (int d = 0; d < dim; d++)
-          shift[d] += (xyz[fn[v] * 3 + d] - xyz[cv[static_cast<size_t>(r) * nv + ref.face_vertex[rs * nfv + v]] * 3 + d]) / nfv;
+          shift[d] += (xyz[fn[v] * 3 + d] - xyz[cv[static_cast<size_t>(r) * nv + ref.face_vertex[rs * nfv + v]] * 3 + d]) / fnv;
       face.seq.assign(nfq, -1);
-      for (int p = 0; p < nfq; p++) {
+      for (int p = nreal; p < nfq; p++) face.seq[p] = p;  // zero-weight padding slots
+      for (int p = 0; p < nreal; p++) {
         double best = std::numeric_limits<double>::max();
         int bi = -1;
-        for (int qq = 0; qq < nfq; qq++) {
+        for (int qq = 0; qq < nreal; qq++) {
           double s2 = 0.0;
           for (int d = 0; d < dim; d++) {
             const double t = xl[p * dim + d] - (xr[qq * dim + d] + shift[d]);
@@ -969,7 +1611,7 @@ bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double*
           return it == link.end() ? n : it->second;
         };
         const int64_t first = image(fn[0]);
-        for (int v = 0; v < nfv; v++)
+        for (int v = 0; v < ref.face_nv[rs]; v++)
           if (image(cv[static_cast<size_t>(r) * nv + ref.face_vertex[rs * nfv + v]]) == first) face.rotation = v;
       }
       face.coord.assign(xl.begin(), xl.end());
@@ -981,7 +1623,6 @@ bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double*
       mesh.elem_face[static_cast<size_t>(r) * nf + rs] = fid;
       mesh.elem_side[static_cast<size_t>(r) * nf + rs] = 1;
     }
-  mesh.affine = true;
   assignNodes(mesh);
   return true;
 }
@@ -1123,7 +1764,7 @@ bool buildSplitMesh(int order, const int n[2], const double lo[2], const double 
         face.seq[p] = bi;
       }
       face.coord.assign(xl.begin(), xl.end());
-      faceNormalsRef(ref, &mesh.jac[static_cast<size_t>(e) * 4], s, face);
+      faceNormalsRef(ref, &mesh.jac[static_cast<size_t>(e) * 4], 0, s, face);
       const int32_t fid = static_cast<int32_t>(mesh.faces.size());
       mesh.faces.push_back(std::move(face));
       mesh.elem_face[static_cast<size_t>(e) * 4 + s] = fid;
@@ -1185,11 +1826,29 @@ void assignNodes(Mesh& mesh) {
       mesh.node_tag[static_cast<size_t>(e) * nv + v] = it->second;
     }
   mesh.nodes = static_cast<int64_t>(index.size());
-  // inner radius D |element| / |boundary|; reference measures: triangle 1/2, quadrangle 4, hexahedron 8
-  const double measure = ref.type == 2 ? 0.5 : (dim == 1 ? 2.0 : (dim == 2 ? 4.0 : 8.0));
+  // inner radius D |element| / |boundary|; reference volumes: triangle 1/2, quadrangle 4,
+  // tetrahedron 1/6, pyramid 4/3, hexahedron 8. Curved elements: the volume and the face areas by
+  // quadrature.
+  const double measure = ref.type == 2   ? 0.5
+                         : ref.type == 4 ? 1.0 / 6.0
+                         : ref.type == 5 ? 4.0 / 3.0
+                                         : (dim == 1 ? 2.0 : (dim == 2 ? 4.0 : 8.0));
   mesh.inner_radius.assign(E, 0.0);
   for (int64_t e = 0; e < E; e++) {
     const double* x = &mesh.vertex[static_cast<size_t>(e) * nv * dim];
+    if (!mesh.gnode.empty()) {
+      std::vector<double> jw(static_cast<size_t>(ref.nq));
+      elementGeometry(mesh, e, jw.data(), nullptr, nullptr);
+      double volume = 0.0, surface = 0.0;
+      for (double v : jw) volume += v;
+      for (int f = 0; f < ref.nf; f++) {
+        const int32_t id = mesh.elem_face[static_cast<size_t>(e) * ref.nf + f];
+        const Face& face = id >= 0 ? mesh.faces[id] : mesh.bfaces[-2 - id];
+        for (double v : face.jw) surface += v;
+      }
+      mesh.inner_radius[e] = dim * volume / surface;
+      continue;
+    }
     double surface = 0.0;
     for (int f = 0; f < ref.nf; f++) {
       const int* fv = &ref.face_vertex[static_cast<size_t>(f) * ref.nfvert];
@@ -1202,14 +1861,15 @@ void assignNodes(Mesh& mesh) {
           s2 += t * t;
         }
         surface += std::sqrt(s2);
-      } else {  // planar parallelogram face a, b, c, d: |(b - a) x (d - a)|
+      } else {  // planar parallelogram face a, b, c, d: |(b - a) x (d - a)|; triangle a, b, c: half |(b - a) x (c - a)|
+        const bool tri = ref.face_nv[f] == 3;
         double u[3], w[3];
         for (int d = 0; d < 3; d++) {
           u[d] = x[fv[1] * 3 + d] - x[fv[0] * 3 + d];
-          w[d] = x[fv[3] * 3 + d] - x[fv[0] * 3 + d];
+          w[d] = x[fv[tri ? 2 : 3] * 3 + d] - x[fv[0] * 3 + d];
         }
         const double c[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
-        surface += std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        surface += (tri ? 0.5 : 1.0) * std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
       }
     }
     mesh.inner_radius[e] = dim * (measure * mesh.detj[e]) / surface;
@@ -1235,13 +1895,49 @@ void affineGeometry(const RefElement& ref, const double* J, double dj, double* d
 void elementGeometry(const Mesh& mesh, int64_t e, double* detjw, double* tiw, double* coord) {
   const RefElement& ref = mesh.ref;
   const int dim = ref.dim;
+  if (!mesh.gnode.empty()) {  // curved: the Jacobian at each point (Geometry.cpp:44-67)
+    double x[3], J[9], jt[9], jti[9];
+    for (int q = 0; q < ref.nq; q++) {
+      mapPoint(mesh, e, &ref.qpoint[static_cast<size_t>(q) * dim], x, J);
+      const double jw = det(dim, J) * ref.qweight[q];
+      if (detjw) detjw[q] = jw;
+      if (tiw) {
+        for (int k = 0; k < dim; k++)
+          for (int l = 0; l < dim; l++) jt[k * dim + l] = J[l * dim + k];
+        invertMatrix(dim, jt, jti);
+        for (int a = 0; a < dim; a++)
+          for (int b = 0; b < dim; b++) tiw[static_cast<size_t>(q) * dim * dim + a + dim * b] = jti[a * dim + b] * jw;
+      }
+      if (coord)
+        for (int d = 0; d < dim; d++) coord[static_cast<size_t>(q) * dim + d] = x[d];
+    }
+    return;
+  }
   affineGeometry(ref, &mesh.jac[static_cast<size_t>(e) * dim * dim], mesh.detj[e], detjw, tiw);
   if (coord)
     for (int q = 0; q < ref.nq; q++)
       mapPoint(mesh, e, &ref.qpoint[static_cast<size_t>(q) * dim], &coord[static_cast<size_t>(q) * dim], nullptr);
 }
 
-void elementMassInverse(const Mesh& mesh, int64_t e, double* minv) { affineMassInverse(mesh.ref, mesh.detj[e], minv); }
+void elementMassInverse(const Mesh& mesh, int64_t e, double* minv) {
+  if (mesh.gnode.empty()) {
+    affineMassInverse(mesh.ref, mesh.detj[e], minv);
+    return;
+  }
+  // curved: (sum_q V_qi V_qj |J_q| w_q)^-1 (Geometry.cpp:88-100)
+  const RefElement& ref = mesh.ref;
+  const int nb = ref.nb;
+  std::vector<double> jw(static_cast<size_t>(ref.nq)), m(static_cast<size_t>(nb) * nb, 0.0);
+  elementGeometry(mesh, e, jw.data(), nullptr, nullptr);
+  for (int i = 0; i < nb; i++)
+    for (int j = 0; j < nb; j++) {
+      double acc = 0.0;
+      for (int q = 0; q < ref.nq; q++)
+        acc += ref.V[static_cast<size_t>(q) * nb + i] * (ref.V[static_cast<size_t>(q) * nb + j] * jw[q]);
+      m[static_cast<size_t>(i) * nb + j] = acc;
+    }
+  invertMatrix(nb, m.data(), minv);
+}
 
 void affineMassInverse(const RefElement& ref, double dj, double* minv) {
   const int nb = ref.nb;

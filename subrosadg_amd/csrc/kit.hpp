@@ -67,6 +67,14 @@ struct RefElement {
   // nodes, BasisFunction.cpp:161-207): at the volume points and at the face points
   std::vector<double> nodal_value;         // N_q x nvert
   std::vector<double> nodal_face_value;    // (N_f*N_fq) x nvert
+  // per face: its vertex count (pyramids: 3 or 4, else nfvert), the face-rule point and weight of
+  // each of its N_fq slots, and how many slots are real points. A pyramid's triangle faces carry
+  // the triangle rule in their first slots and zero-weight copies of their first point in the
+  // rest, so that every face has the N_fq = (P+1)^2 slots of the quadrangle base.
+  std::vector<int> face_nv;                // N_f
+  std::vector<int> face_nreal;             // N_f
+  std::vector<double> fslot_point;         // N_f x N_fq x (D-1)
+  std::vector<double> fslot_weight;        // N_f x N_fq
 };
 
 // N_b of the modal basis of an element type and order (the hierarchic basis of order P-1 is a
@@ -74,6 +82,8 @@ struct RefElement {
 inline int basisSize(int type, int dim, int order) {
   const int n = order + 1;
   if (type == 2 /* SDG_ELEM_TRIANGLE */) return n * (n + 1) / 2;
+  if (type == 4 /* SDG_ELEM_TETRAHEDRON */) return n * (n + 1) * (n + 2) / 6;
+  if (type == 5 /* SDG_ELEM_PYRAMID */) return n * (n + 1) * (2 * n + 1) / 6;
   return dim == 1 ? n : (dim == 2 ? n * n : n * n * n);
 }
 
@@ -82,6 +92,20 @@ bool buildRefElement(int type, int order, RefElement& ref, std::string& error);
 
 // Dense inverse by Gauss-Jordan elimination with partial pivoting (row-major n x n).
 bool invertMatrix(int n, const double* a, double* inverse);
+
+// Element map of a gmsh element of geometry order g >= 1: the Lagrange functions through its
+// geometry nodes, in gmsh's node order (vertices, then the interior nodes of the edges, faces and
+// volume; ReadControl.cpp:286-335 reads them through gmsh, Geometry.cpp:44-67 evaluates the map).
+struct GeomMap {
+  int type = 0, dim = 0, order = 1, n = 0;  // n geometry nodes
+  std::vector<double> node;                 // n x dim reference coordinates
+  std::vector<std::array<int, 3>> expo;     // n monomial exponents spanning the map's space
+  std::vector<double> ainv;                 // n x n: N_k(xi) = sum_m ainv[m n + k] xi^expo[m]
+};
+// false with a message for an unsupported (type, geometry order)
+bool buildGeomMap(int type, int order, GeomMap& g, std::string& error);
+// shape values N (n) and reference gradients dN (n x dim, dN[k dim + d] = dN_k / dxi_d) at xi
+void geomShape(const GeomMap& g, const double* xi, double* N, double* dN);
 
 struct Face {
   int32_t parent[2] = {0, 0};   // left, right element (boundary face: right = -1)
@@ -109,6 +133,11 @@ struct Mesh {
   std::vector<int32_t> elem_face;   // E x N_f
   std::vector<int8_t> elem_side;    // E x N_f
   bool affine = true;
+  // curved (non-affine) meshes: the element map through each element's geometry nodes and their
+  // physical coordinates (E x geom.n x D); jac / detj then hold the map at the element centre.
+  // Empty for affine meshes.
+  GeomMap geom;
+  std::vector<double> gnode;
   // mesh nodes (the element vertices): per element the 0-based node index of each vertex, shared
   // by the elements that meet at a point. Periodic images are distinct nodes, as gmsh keeps the
   // nodes of a periodic side apart from their masters.
@@ -128,18 +157,23 @@ bool buildStructuredMesh(int type, int order, const int n[3], const double lo[3]
 // Mesh of one element type from node coordinates (nnode x 3) and element vertex lists (E x nvert,
 // 0-based node indices, gmsh vertex order), with the boundary faces given by their node lists
 // (nbface x nfvert, any order) and sdg_boundary types. Elements must be affine (triangles,
-// parallelogram quadrangles, parallelepiped hexahedra); an element with a negative Jacobian is
-// mirrored. Faces pair by their node sets and are created at the first visit in element order,
+// parallelogram quadrangles, parallelepiped hexahedra, tetrahedra) or curved (below); an affine element
+// with a negative Jacobian is mirrored. Faces pair by their node sets and are created at the first visit in element order,
 // whose element is the left parent (Adjacency.cpp:61-183); right-parent points are matched by
 // coordinates, boundary faces take the type of their boundary element. periodic_links (optional)
 // are node maps slave -> master, one per periodic pair of boundaries: an unpaired face whose nodes
 // are all slaves of one link pairs with the unpaired face on the master nodes, and the right
 // parent's points are matched after the translation between the two faces (translational
 // periodicity, Adjacency.cpp:185-328).
+// geometry_order > 1: conn holds the element's gmsh geometry nodes of that order (vertices first);
+// an element whose nodes do not all lie on the affine map of its vertices makes the mesh curved,
+// as does a non-parallelogram quadrangle or non-parallelepiped hexahedron (then the bilinear /
+// trilinear map of the vertices is the element map).
 bool buildMeshFromConnectivity(int type, int order, int64_t nnode, const double* xyz, int64_t elements,
                                const int64_t* conn, int64_t nbface, const int64_t* bconn, const int* btype,
                                const int* bphysical, Mesh& mesh, std::string& error,
-                               const std::vector<std::map<int64_t, int64_t>>* periodic_links = nullptr);
+                               const std::vector<std::map<int64_t, int64_t>>* periodic_links = nullptr,
+                               int geometry_order = 1);
 
 // Triangles and quadrangles on a periodic structured n[0] x n[1] grid: cell (i, j) (i fastest) is a
 // quadrangle, or, when split[cell] != 0, the two triangles (A, B, C), (A, C, D) of its corners
@@ -174,7 +208,8 @@ void nodalValues(const RefElement& ref, const double* xi, double* value);
 // Physical point of reference point xi in element e.
 void mapReferencePoint(const Mesh& mesh, int64_t e, const double* xi, double* x);
 
-// Geometry derived per element and volume point (affine elements: constant Jacobian).
+// Geometry derived per element and volume point (affine elements: constant Jacobian; curved
+// elements: the Jacobian of the element map at each point).
 //   detjw[q]           = |J| w_q
 //   tiw[q*D*D + a+D*b] = ((J^T)^-1)(a,b) |J| w_q   (column-major D x D)
 //   coord[q*D + d]     = physical point

@@ -1,14 +1,17 @@
-// sdg_gmsh.cpp -- reader of gmsh's MSH 4.1 ASCII format for affine meshes of one element type:
+// sdg_gmsh.cpp -- reader of gmsh's MSH 4.1 ASCII format for meshes of one element type:
 // the mesh-file half of the reference's Mesh<SC>::initializeMesh / readMeshElement
 // (src/Mesh/ReadControl.cpp:286-335), which reads through the gmsh library; here the file is parsed
 // directly and handed to the connectivity builder of the kit.
 //
 // Read: $MeshFormat (4.1, ASCII), $Entities (the physical tags of the boundary curves / surfaces),
-// $Nodes, $Elements. The highest-dimensional elements are the mesh (3-node triangles or 4-node
-// quadrangles in 2-D, 8-node hexahedra in 3-D; one type); elements of one dimension lower carry the
-// boundary faces, whose type comes from the physical group of their entity through the caller's
-// map; $Periodic node correspondences (translations) pair the faces of periodic boundaries.
-// Refused: binary files, other versions, higher-order (curved) or mixed element types.
+// $Nodes, $Elements. The highest-dimensional elements are the mesh (triangles or quadrangles of
+// order 1..5 in 2-D, tetrahedra or hexahedra of order 1 or 2, or first-order pyramids in 3-D; one
+// type and order); their high-order nodes define the element map, so elements may be curved.
+// Elements of one dimension
+// lower carry the boundary faces (identified by their vertices), whose type comes from the physical
+// group of their entity through the caller's map; $Periodic node correspondences (translations)
+// pair the faces of periodic boundaries. Refused: binary files, other versions, prisms, high-order
+// pyramids, serendipity elements, mixed element types.
 #include <algorithm>
 #include <fstream>
 #include <map>
@@ -23,18 +26,26 @@ namespace sdg {
 
 namespace {
 
-// nodes per element of the first-order gmsh types used here (0: unsupported)
-int gmshNodes(int type) {
-  switch (type) {
-    case 15: return 1;  // point
-    case 1: return 2;   // 2-node line
-    case 2: return 3;   // 3-node triangle
-    case 3: return 4;   // 4-node quadrangle
-    case 5: return 8;   // 8-node hexahedron
-    default: return 0;
-  }
+// The gmsh element types read here: element shape (enum sdg_element; 0 for a point), dimension,
+// geometry order, nodes and vertices (the first nodes). The reference meshes with setOrder(P),
+// so its files hold elements of order P (SimulationControl.cpp:26-31).
+struct GmshType {
+  int code, shape, dim, order, nodes, vertices;
+};
+constexpr GmshType kGmshTypes[] = {
+    {15, 0, 0, 1, 1, 1},                                                       // point
+    {1, 1, 1, 1, 2, 2},   {8, 1, 1, 2, 3, 2},   {26, 1, 1, 3, 4, 2},   {27, 1, 1, 4, 5, 2},   {28, 1, 1, 5, 6, 2},
+    {2, 2, 2, 1, 3, 3},   {9, 2, 2, 2, 6, 3},   {21, 2, 2, 3, 10, 3},  {23, 2, 2, 4, 15, 3},  {25, 2, 2, 5, 21, 3},
+    {3, 3, 2, 1, 4, 4},   {10, 3, 2, 2, 9, 4},  {36, 3, 2, 3, 16, 4},  {37, 3, 2, 4, 25, 4},  {38, 3, 2, 5, 36, 4},
+    {4, 4, 3, 1, 4, 4},   {11, 4, 3, 2, 10, 4},
+    {7, 5, 3, 1, 5, 5},
+    {5, 6, 3, 1, 8, 8},   {12, 6, 3, 2, 27, 8},
+};
+const GmshType* gmshType(int code) {
+  for (const auto& t : kGmshTypes)
+    if (t.code == code) return &t;
+  return nullptr;
 }
-int gmshDim(int type) { return type == 15 ? 0 : (type == 1 ? 1 : (type == 5 ? 3 : 2)); }
 
 bool skipTo(std::istream& in, const std::string& end) {
   std::string line;
@@ -57,7 +68,7 @@ bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_p
   std::vector<double> xyz;
   struct Block {
     int dim, entity, type;
-    std::vector<int64_t> nodes;  // gmsh node tags, gmshNodes(type) per element
+    std::vector<int64_t> nodes;  // gmsh node tags, gmshType(type)->nodes per element
   };
   std::vector<Block> blocks;
   std::vector<std::vector<std::pair<int64_t, int64_t>>> periodic_pairs;  // per link: slave, master tags
@@ -139,12 +150,14 @@ bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_p
         Block blk;
         int64_t n = 0;
         in >> blk.dim >> blk.entity >> blk.type >> n;
-        const int nn = gmshNodes(blk.type);
-        if (nn == 0) {
-          error = "element type " + std::to_string(blk.type) + " is not read (first-order lines, triangles, "
-                  "quadrangles and hexahedra only)";
+        const GmshType* gt = gmshType(blk.type);
+        if (!gt) {
+          error = "element type " + std::to_string(blk.type) + " is not read (lines, triangles and quadrangles of "
+                  "order 1..5, tetrahedra and hexahedra of order 1 or 2, first-order pyramids; no prisms or serendipity "
+                  "elements)";
           return false;
         }
+        const int nn = gt->nodes;
         blk.nodes.resize(static_cast<size_t>(n) * nn);
         for (int64_t i = 0; i < n; i++) {
           int64_t etag = 0;
@@ -192,13 +205,14 @@ bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_p
     return false;
   }
   int dim = 0;
-  for (const auto& b : blocks) dim = std::max(dim, gmshDim(b.type));
+  for (const auto& b : blocks) dim = std::max(dim, gmshType(b.type)->dim);
   if (dim < 2) {
     error = "no 2-D or 3-D elements";
     return false;
   }
-  int type = -1;
+  int type = -1, code = -1;
   std::vector<int64_t> conn, bconn;
+  std::vector<int64_t> tri_faces;  // boundary faces that are triangles (padded to 4 nodes for pyramids)
   std::vector<int> btype, bphysical;
   std::vector<std::pair<int, int>> bnodes;  // per boundary block: nodes per element, physical group
   auto index = [&](int64_t tag, int64_t& out) {
@@ -208,14 +222,15 @@ bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_p
     return true;
   };
   for (const auto& b : blocks) {
-    const int d = gmshDim(b.type), nn = gmshNodes(b.type);
+    const GmshType* gt = gmshType(b.type);
+    const int d = gt->dim, nn = gt->nodes;
     if (d == dim) {
-      const int t = b.type == 2 ? 2 : (b.type == 3 ? 3 : 6);
-      if (type >= 0 && type != t) {
-        error = "meshes of more than one element type are not read";
+      if (code >= 0 && code != b.type) {
+        error = "meshes of more than one element type or order are not read";
         return false;
       }
-      type = t;
+      type = gt->shape;
+      code = b.type;
       for (int64_t tag : b.nodes) {
         int64_t i = 0;
         if (!index(tag, i)) {
@@ -229,9 +244,10 @@ bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_p
       if (ph == physical.end()) continue;  // boundary elements outside any physical group
       const auto bc = bc_of_physical.find(ph->second);
       if (bc == bc_of_physical.end()) continue;
-      bnodes.push_back({nn, ph->second});
+      bnodes.push_back({gt->shape, ph->second});
       for (size_t k = 0; k < b.nodes.size(); k += static_cast<size_t>(nn)) {
-        for (int v = 0; v < nn; v++) {
+        if (gt->shape == 2) tri_faces.push_back(static_cast<int64_t>(btype.size()));
+        for (int v = 0; v < gt->vertices; v++) {  // a face is identified by its vertices
           int64_t i = 0;
           if (!index(b.nodes[k + static_cast<size_t>(v)], i)) {
             error = "boundary element references an unknown node";
@@ -244,14 +260,16 @@ bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_p
       }
     }
   }
-  // boundary elements must be faces of the volume elements: 2 nodes (triangles, quadrangles) or
-  // 4 nodes (hexahedra); any other count would be read as the wrong number of face vertices
+  // boundary elements must have the shape of the volume elements' faces: lines (triangles,
+  // quadrangles), triangles (tetrahedra) or quadrangles (hexahedra); another shape would be read
+  // as the wrong number of face vertices
   if (type >= 0) {
-    const int face_nodes = type == 6 ? 4 : 2;
+    const int face_shape = type == 6 ? 3 : (type == 4 ? 2 : 1);
+    static const char* kShape[] = {"points", "lines", "triangles", "quadrangles"};
     for (const auto& bn : bnodes)
-      if (bn.first != face_nodes) {
-        error = "physical group " + std::to_string(bn.second) + " holds boundary elements of " +
-                std::to_string(bn.first) + " nodes; faces of the volume elements have " + std::to_string(face_nodes);
+      if (bn.first != face_shape && !(type == 5 && (bn.first == 2 || bn.first == 3))) {
+        error = "physical group " + std::to_string(bn.second) + " holds boundary " + kShape[bn.first] +
+                "; faces of the volume elements are " + kShape[face_shape];
         return false;
       }
   }
@@ -268,11 +286,26 @@ bool readGmshMesh(const char* path, int order, const std::map<int, int>& bc_of_p
       if (si != mi) links.back()[si] = mi;
     }
   }
-  const int nvert = type == 2 ? 3 : (type == 3 ? 4 : 8);
+  if (type == 5 && !tri_faces.empty()) {  // pyramids: boundary rows of 4 nodes, triangles padded with -1
+    std::vector<int64_t> padded;
+    size_t o = 0, t = 0;
+    for (int64_t f = 0; f < static_cast<int64_t>(btype.size()); f++) {
+      const bool tri = t < tri_faces.size() && tri_faces[t] == f;
+      const int n = tri ? 3 : 4;
+      for (int v = 0; v < n; v++) padded.push_back(bconn[o + v]);
+      if (tri) {
+        padded.push_back(-1);
+        t++;
+      }
+      o += n;
+    }
+    bconn.swap(padded);
+  }
+  const GmshType* gt = gmshType(code);
   return buildMeshFromConnectivity(type, order, static_cast<int64_t>(xyz.size() / 3), xyz.data(),
-                                   static_cast<int64_t>(conn.size()) / nvert, conn.data(),
+                                   static_cast<int64_t>(conn.size()) / gt->nodes, conn.data(),
                                    static_cast<int64_t>(btype.size()), bconn.data(), btype.data(), bphysical.data(),
-                                   mesh, error, links.empty() ? nullptr : &links);
+                                   mesh, error, links.empty() ? nullptr : &links, gt->order);
 }
 
 }  // namespace sdg

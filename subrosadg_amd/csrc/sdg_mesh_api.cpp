@@ -68,26 +68,38 @@ int sdg_mesh_create_from_arrays(const sdg_array_mesh_desc* d, sdg_mesh** out) {
   if (!d || !out) return refuse("null argument");
   *out = nullptr;
   const int type = d->element;
-  int nvert = 0, fvert = 0;
-  if (type == SDG_ELEM_TRIANGLE) {
-    nvert = 3, fvert = 2;
-  } else if (type == SDG_ELEM_QUADRANGLE) {
-    nvert = 4, fvert = 2;
-  } else if (type == SDG_ELEM_HEXAHEDRON) {
-    nvert = 8, fvert = 4;
+  int fvert = 0;
+  if (type == SDG_ELEM_TRIANGLE || type == SDG_ELEM_QUADRANGLE) {
+    fvert = 2;
+  } else if (type == SDG_ELEM_TETRAHEDRON) {
+    fvert = 3;
+  } else if (type == SDG_ELEM_HEXAHEDRON || type == SDG_ELEM_PYRAMID) {
+    fvert = 4;
   } else {
-    sdg::setGlobalError("sdg_mesh_create_from_arrays: element must be triangle, quadrangle or hexahedron");
+    sdg::setGlobalError("sdg_mesh_create_from_arrays: element must be triangle, quadrangle, tetrahedron, pyramid or "
+                        "hexahedron");
     return SDG_E_UNSUPPORTED;
   }
+  const int gorder = d->geometry_order > 0 ? d->geometry_order : 1;
+  sdg::GeomMap gm;
+  std::string gerr;
+  if (!sdg::buildGeomMap(type, gorder, gm, gerr)) {
+    sdg::setGlobalError("sdg_mesh_create_from_arrays: " + gerr);
+    return SDG_E_UNSUPPORTED;
+  }
+  const int nvert = gm.n;  // nodes per element in `elements`
   if (d->node_count < 1 || !d->nodes || d->element_count < 1 || !d->elements) return refuse("no nodes or elements");
   if (d->boundary_face_count < 0 || (d->boundary_face_count > 0 && (!d->boundary_faces || !d->boundary_type)))
     return refuse("boundary faces without node lists or types");
   if (d->periodic_link_count < 0 || (d->periodic_link_count > 0 && (!d->periodic_link_size || !d->periodic_pairs)))
     return refuse("periodic links without pairs");
   for (int64_t i = 0; i < d->element_count * nvert; i++)
-    if (d->elements[i] < 0 || d->elements[i] >= d->node_count) return refuse("element vertex out of range");
-  for (int64_t i = 0; i < d->boundary_face_count * fvert; i++)
-    if (d->boundary_faces[i] < 0 || d->boundary_faces[i] >= d->node_count) return refuse("boundary face node out of range");
+    if (d->elements[i] < 0 || d->elements[i] >= d->node_count) return refuse("element node out of range");
+  for (int64_t i = 0; i < d->boundary_face_count * fvert; i++) {
+    const bool pad = type == SDG_ELEM_PYRAMID && i % fvert == 3 && d->boundary_faces[i] == -1;  // a triangle
+    if (!pad && (d->boundary_faces[i] < 0 || d->boundary_faces[i] >= d->node_count))
+      return refuse("boundary face node out of range");
+  }
   std::vector<std::map<int64_t, int64_t>> links(static_cast<size_t>(d->periodic_link_count));
   int64_t off = 0;
   for (int l = 0; l < d->periodic_link_count; l++) {
@@ -110,7 +122,7 @@ int sdg_mesh_create_from_arrays(const sdg_array_mesh_desc* d, sdg_mesh** out) {
       for (int64_t i = 0; i < d->boundary_face_count; i++) phys[static_cast<size_t>(i)] = d->boundary_physical[i];
     if (!sdg::buildMeshFromConnectivity(type, d->polynomial_order, d->node_count, d->nodes, d->element_count,
                                         d->elements, d->boundary_face_count, d->boundary_faces, d->boundary_type,
-                                        phys.data(), m->mesh, err, links.empty() ? nullptr : &links)) {
+                                        phys.data(), m->mesh, err, links.empty() ? nullptr : &links, gorder)) {
       delete m;
       sdg::setGlobalError("sdg_mesh_create_from_arrays: " + err);
       return SDG_E_INVALID;

@@ -295,6 +295,24 @@ bool selectLaunchers(int type, int order, bool force_valu, bool mixed, Launchers
       default: return false;
     }
   }
+  if (type == SDG_ELEM_TETRAHEDRON) {  // bases of 4..35 functions, Grundmann-Moeller volume rules: vector kernels
+    switch (order) {
+      case 1: return pickValu<3, 4, 5, 4, 4>(mixed, l);
+      case 2: return pickValu<3, 10, 15, 4, 7>(mixed, l);
+      case 3: return pickValu<3, 20, 35, 4, 13>(mixed, l);
+      case 4: return pickValu<3, 35, 70, 4, 19>(mixed, l);
+      default: return false;
+    }
+  }
+  if (type == SDG_ELEM_PYRAMID) {  // (P+1)^3 collapsed points, five faces of (P+1)^2 slots: vector kernels
+    switch (order) {
+      case 1: return pickValu<3, 5, 8, 5, 4>(mixed, l);
+      case 2: return pickValu<3, 14, 27, 5, 9>(mixed, l);
+      case 3: return pickValu<3, 30, 64, 5, 16>(mixed, l);
+      case 4: return pickValu<3, 55, 125, 5, 25>(mixed, l);
+      default: return false;
+    }
+  }
   if (type == SDG_ELEM_HEXAHEDRON) {
     switch (order) {
       case 1: return pickValu<3, 8, 8, 6, 4>(mixed, l);
@@ -857,7 +875,12 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   if (!sdg::meshOf(mesh_handle))
     return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: meshes of mixed element types run through sdg_batch_create");
   const sdg::Mesh& mesh = *sdg::meshOf(mesh_handle);
-  if (!mesh.affine) return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: curved (non-affine) elements are not supported");
+  const bool curved = !mesh.affine;
+  if (curved && (problem->precision == SDG_PREC_MIXED || problem->kernels == SDG_KERNELS_DENSE ||
+                 problem->shock_capturing == SDG_SHOCK_ARTIFICIAL_VISCOSITY))
+    return fail(nullptr, SDG_E_UNSUPPORTED,
+                "sdg_create: curved elements run on the FP64 vector kernels (no mixed precision, dense kernels or "
+                "artificial viscosity)");
   if (mesh.elements > (int64_t(1) << 30)) return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: too many elements");
   sdg_ctx* ctx = new (std::nothrow) sdg_ctx();
   if (!ctx) return fail(nullptr, SDG_E_NOMEM, "sdg_create: out of host memory");
@@ -875,10 +898,10 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   ctx->type = r.type;
   ctx->E = static_cast<int>(mesh.elements);
   ctx->F = static_cast<int64_t>(mesh.faces.size());
-  const bool force_vector = problem->kernels == SDG_KERNELS_VECTOR;
+  const bool force_vector = problem->kernels == SDG_KERNELS_VECTOR || curved;
   const bool mixed = problem->precision == SDG_PREC_MIXED;
   ctx->mixed = mixed;
-  const bool tensor = problem->equation == SDG_EQ_COMPRESSIBLE_NS && problem->kernels == SDG_KERNELS_AUTO;
+  const bool tensor = problem->equation == SDG_EQ_COMPRESSIBLE_NS && problem->kernels == SDG_KERNELS_AUTO && !curved;
   ctx->av = problem->shock_capturing == SDG_SHOCK_ARTIFICIAL_VISCOSITY;
   if (ctx->av && mixed) {
     delete ctx;
@@ -1096,6 +1119,36 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   g.F = static_cast<int>(F);
   g.nvert = r.nvert;
   g.order = r.order;
+  if (curved) {  // per-point geometry, element mass inverses and BR2 face blocks (Geometry.cpp:44-100)
+    std::vector<double> qG(static_cast<size_t>(E) * nq * D * D), qjw(static_cast<size_t>(E) * nq),
+        minvE(static_cast<size_t>(E) * nb * nb), LfE(static_cast<size_t>(E) * naq * nfq);
+    std::vector<double> t(nb);
+    for (int64_t e = 0; e < E; e++) {
+      sdg::elementGeometry(mesh, e, &qjw[static_cast<size_t>(e) * nq], &qG[static_cast<size_t>(e) * nq * D * D], nullptr);
+      double* M = &minvE[static_cast<size_t>(e) * nb * nb];
+      sdg::elementMassInverse(mesh, e, M);
+      for (int s = 0; s < nf; s++)
+        for (int j = 0; j < nfq; j++) {
+          for (int b = 0; b < nb; b++) {
+            double acc = 0.0;
+            for (int c = 0; c < nb; c++) acc += M[static_cast<size_t>(b) * nb + c] * r.Vf[static_cast<size_t>(s * nfq + j) * nb + c];
+            t[b] = acc;
+          }
+          for (int p = 0; p < nfq; p++) {
+            double acc = 0.0;
+            for (int b = 0; b < nb; b++) acc += r.Vf[static_cast<size_t>(s * nfq + p) * nb + b] * t[b];
+            LfE[((static_cast<size_t>(e) * nf + s) * nfq + p) * nfq + j] = acc;
+          }
+        }
+    }
+    double *dqG, *dqjw, *dminvE, *dLfE;
+    if (upload(ctx, qG, &dqG) || upload(ctx, qjw, &dqjw) || upload(ctx, minvE, &dminvE) || upload(ctx, LfE, &dLfE))
+      return bail(SDG_E_NOMEM);
+    g.qG = dqG;
+    g.qjw = dqjw;
+    g.minvE = dminvE;
+    g.LfE = dLfE;
+  }
   if (ctx->av) {
     double *dnq, *dnf, *drad;
     int* dtag;

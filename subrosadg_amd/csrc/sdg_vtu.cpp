@@ -82,6 +82,46 @@ void lagrangeNodes(const sdg::RefElement& ref, std::vector<double>& xi, int& vtk
     vtk_type = 69;  // VTK_LAGRANGE_TRIANGLE
     return;
   }
+  if (ref.type == SDG_ELEM_TETRAHEDRON) {  // reference (0,0,0), (1,0,0), (0,1,0), (0,0,1)
+    // barycentric lattice weights (a0, a1, a2, a3), sum P: vertices, the edges (0,1), (1,2), (2,0),
+    // (0,3), (1,3), (2,3) in their direction, the faces (0,1,3), (1,2,3), (2,0,3), (0,2,1) (interior
+    // points as a triangle of order P - 3 in the face's vertex order), then the interior (P = 4:
+    // the centroid)
+    auto add = [&](const int w[4]) {
+      for (int d = 1; d <= 3; d++) xi.push_back(static_cast<double>(w[d]) / P);
+    };
+    for (int v = 0; v < 4; v++) {
+      int w[4] = {0, 0, 0, 0};
+      w[v] = P;
+      add(w);
+    }
+    static const int kEdge[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+    for (const auto& ed : kEdge)
+      for (int k = 1; k < P; k++) {
+        int w[4] = {0, 0, 0, 0};
+        w[ed[0]] = P - k;
+        w[ed[1]] = k;
+        add(w);
+      }
+    static const int kFace[4][3] = {{0, 1, 3}, {1, 2, 3}, {2, 0, 3}, {0, 2, 1}};
+    std::vector<std::pair<int, int>> inner;  // (i, j) lattice of the face interior
+    if (P == 3) inner = {{1, 1}};
+    if (P == 4) inner = {{1, 1}, {2, 1}, {1, 2}};
+    for (const auto& f : kFace)
+      for (const auto& q : inner) {
+        int w[4] = {0, 0, 0, 0};
+        w[f[0]] = P - q.first - q.second;
+        w[f[1]] = q.first;
+        w[f[2]] = q.second;
+        add(w);
+      }
+    if (P == 4) {
+      const int w[4] = {1, 1, 1, 1};
+      add(w);
+    }
+    vtk_type = 71;  // VTK_LAGRANGE_TETRAHEDRON
+    return;
+  }
   if (dim == 1) {
     xi.push_back(t(0));
     xi.push_back(t(P));
@@ -197,6 +237,8 @@ int sdg_write_view(const sdg_mesh* mesh_handle, const sdg_problem_desc* problem,
   if (!(problem->cv > 0.0)) return vtuFail(SDG_E_INVALID, "specific heats must be positive");
   const sdg::Mesh& m = *mp;
   const sdg::RefElement& ref = m.ref;
+  if (ref.type == SDG_ELEM_PYRAMID)  // the Lagrange nodes include the apex, where the pyramid basis is 0/0
+    return vtuFail(SDG_E_UNSUPPORTED, "pyramids are not written");
   const int D = ref.dim, nb = ref.nb, nv = D + 2, ng = nv * D;
   for (int i = 0; i < variable_count; i++) {
     if (!variableName(variables[i])) return vtuFail(SDG_E_INVALID, "unknown view variable");

@@ -4,7 +4,8 @@ from __future__ import annotations
 
 import numpy as np
 
-GMSH_TYPE = {"tri": 2, "quad": 3, "hex": 5, "line": 1, "bquad": 3, "btri": 2}
+GMSH_TYPE = {"tri": 2, "quad": 3, "hex": 5, "line": 1, "bquad": 3, "btri": 2, "tet": 4, "tri6": 9, "quad9": 10,
+             "line3": 8, "tet10": 11, "hex27": 12, "tri10": 21, "quad16": 36}
 
 
 def write_msh(path, dim: int, node_tags: np.ndarray, xyz: np.ndarray, elem_type: str, elems: np.ndarray,

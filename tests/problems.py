@@ -326,3 +326,279 @@ def c3_specs(count=48, seed=20261202):
         order = int(rng.integers(1, 5))
         specs.append((n, order, int(rng.integers(0, 2 ** 62))))
     return specs
+
+
+# ---- tetrahedra and curved elements ------------------------------------------------------------
+KUHN = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+
+
+def tet_box(n=2, order=2, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), periodic=True, jitter=0.0, seed=5, bc=None):
+    """Tetrahedra of the Kuhn (Freudenthal) split of an n^3 grid of boxes: per box the 6 tetrahedra
+    (v, v + e_a, v + e_a + e_b, v + 1) along the permutations (a, b, c) of the axes, conforming across
+    boxes. jitter moves the interior nodes by up to that fraction of a box. periodic: the x, y, z
+    sides are linked by node pairs (one link per direction); otherwise the boundary faces get the
+    types bc[side] (sides x-, x+, y-, y+, z-, z+; physical tag = side + 1)."""
+    n1 = n + 1
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    h = (hi - lo) / n
+    idx = np.arange(n1 ** 3).reshape(n1, n1, n1)  # [i, j, k]
+    ijk = np.stack(np.meshgrid(np.arange(n1), np.arange(n1), np.arange(n1), indexing="ij"), -1).reshape(-1, 3)
+    xyz = lo + ijk * h
+    if jitter:
+        rng = np.random.default_rng(seed)
+        inner = np.all((ijk > 0) & (ijk < n), axis=1)
+        xyz[inner] += jitter * h * rng.uniform(-1, 1, (inner.sum(), 3))
+    tets = []
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                for perm in KUHN:
+                    p = np.array([i, j, k])
+                    vs = [idx[tuple(p)]]
+                    for a in perm:
+                        p = p.copy()
+                        p[a] += 1
+                        vs.append(idx[tuple(p)])
+                    tets.append(vs)
+    tets = np.array(tets, dtype=np.int64)
+    kw = {}
+    if periodic:
+        links = []
+        for d in range(3):
+            sl = np.nonzero(ijk[:, d] == n)[0]
+            m = ijk[sl].copy()
+            m[:, d] = 0
+            links.append([(int(s), int(idx[tuple(mm)])) for s, mm in zip(sl, m)])
+        kw["periodic"] = links
+    else:
+        faces = {}
+        for t in tets:
+            for f in ((0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)):
+                key = tuple(sorted(t[list(f)]))
+                faces[key] = faces.get(key, 0) + 1
+        bf, bt, bp = [], [], []
+        for key, cnt in faces.items():
+            if cnt != 1:
+                continue
+            c = ijk[list(key)]
+            for side in range(6):
+                d, v = side // 2, 0 if side % 2 == 0 else n
+                if np.all(c[:, d] == v):
+                    bf.append(key)
+                    bt.append((bc or [L.BC_RIEMANN_FARFIELD] * 6)[side])
+                    bp.append(side + 1)
+                    break
+        kw.update(boundary_faces=np.array(bf, dtype=np.int64), boundary_type=bt, boundary_physical=bp)
+    return L.Mesh.from_arrays(L.ELEM_TETRAHEDRON, order, xyz, tets, **kw)
+
+
+def curved_quad_annulus(n=(6, 3), order=3, geometry_order=2, r=(1.0, 2.0), angle=(0.0, np.pi / 2),
+                        bc=(L.BC_RIEMANN_FARFIELD, L.BC_ADIABATIC_SLIP_WALL)):
+    """Quadrangles of geometry order g on a sector of an annulus: n[0] cells along the angle, n[1]
+    along the radius; every geometry node on its exact polar position, so the cells are curved.
+    Boundary sides: the radial lines get bc[0] (tag 1), the arcs bc[1] (tag 2). Node order per cell
+    as gmsh's: vertices, edge interiors along (0,1), (1,2), (2,3), (3,0), then the interior (as a
+    cell of order g - 2)."""
+    na, nr = n
+    g = geometry_order
+    # global lattice of (na g + 1) x (nr g + 1) points
+    ta = np.linspace(angle[0], angle[1], na * g + 1)
+    tr = np.linspace(r[0], r[1], nr * g + 1)
+    A, R = np.meshgrid(ta, tr, indexing="ij")
+    xyz = np.stack([R * np.cos(A), R * np.sin(A), 0 * A], -1).reshape(-1, 3)
+    gid = np.arange((na * g + 1) * (nr * g + 1)).reshape(na * g + 1, nr * g + 1)
+    lat = quad_lattice(g)
+    cells = []
+    for i in range(na):
+        for j in range(nr):  # first lattice index radial, second angular: counter-clockwise cells
+            cells.append([gid[i * g + b, j * g + a] for a, b in lat])
+    bf, bt, bp = [], [], []
+    for j in range(nr):  # radial sides
+        for i in (0, na * g):
+            bf.append((gid[i, j * g], gid[i, (j + 1) * g]))
+            bt.append(bc[0])
+            bp.append(1)
+    for i in range(na):  # arcs
+        for j in (0, nr * g):
+            bf.append((gid[i * g, j], gid[(i + 1) * g, j]))
+            bt.append(bc[1])
+            bp.append(2)
+    return L.Mesh.from_arrays(L.ELEM_QUADRANGLE, order, xyz, np.array(cells, dtype=np.int64), bf, bt, bp,
+                              geometry_order=g)
+
+
+def quad_lattice(g):
+    """gmsh's node order of a quadrangle of order g as lattice indices (a, b), 0..g."""
+    if g == 0:
+        return [(0, 0)]
+    out = [(0, 0), (g, 0), (g, g), (0, g)]
+    out += [(k, 0) for k in range(1, g)] + [(g, k) for k in range(1, g)]
+    out += [(g - k, g) for k in range(1, g)] + [(0, g - k) for k in range(1, g)]
+    if g >= 2:
+        out += [(a + 1, b + 1) for a, b in quad_lattice(g - 2)]
+    return out
+
+
+def tri_lattice(g):
+    """gmsh's node order of a triangle of order g as lattice indices (a, b), a + b <= g."""
+    if g < 0:
+        return []
+    if g == 0:
+        return [(0, 0)]
+    out = [(0, 0), (g, 0), (0, g)]
+    out += [(k, 0) for k in range(1, g)] + [(g - k, k) for k in range(1, g)] + [(0, g - k) for k in range(1, g)]
+    out += [(a + 1, b + 1) for a, b in tri_lattice(g - 3)]
+    return out
+
+
+def lattice_mesh(element, n, g, order=2, warp=None, lo=0.0, hi=1.0, periodic=False, bc=L.BC_RIEMANN_FARFIELD):
+    """Elements of geometry order g on a structured grid of the unit square / cube: quadrangles
+    (element QUADRANGLE), pairs of triangles per cell (TRIANGLE), hexahedra (HEXAHEDRON) or Kuhn
+    tetrahedra per cell (TETRAHEDRON, g <= 2); geometry nodes in gmsh order on a lattice of
+    n g + 1 points per direction, moved by warp(xyz) -> xyz when given (curved cells). Bounded:
+    every boundary face gets bc with physical tag side + 1; periodic: one link per direction."""
+    dim = 3 if element in (L.ELEM_HEXAHEDRON, L.ELEM_TETRAHEDRON) else 2
+    m1 = n * g + 1
+    grid = np.stack(np.meshgrid(*([np.arange(m1)] * dim), indexing="ij"), -1).reshape(-1, dim)
+    gid = np.arange(m1 ** dim).reshape((m1,) * dim)
+    xyz = np.zeros((len(grid), 3))
+    xyz[:, :dim] = lo + (hi - lo) * grid / (n * g)
+    if warp is not None:
+        xyz = warp(xyz)
+    cells = []
+    hexlat = None
+    if element == L.ELEM_HEXAHEDRON:
+        if g == 1:
+            hexlat = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+        else:
+            v = [(0, 0, 0), (2, 0, 0), (2, 2, 0), (0, 2, 0), (0, 0, 2), (2, 0, 2), (2, 2, 2), (0, 2, 2)]
+            edges = [(0, 1), (0, 3), (0, 4), (1, 2), (1, 5), (2, 3), (2, 6), (3, 7), (4, 5), (4, 7), (5, 6), (6, 7)]
+            faces = [(0, 3, 2, 1), (0, 1, 5, 4), (0, 4, 7, 3), (1, 2, 6, 5), (2, 3, 7, 6), (4, 5, 6, 7)]
+            hexlat = v + [tuple((np.add(v[a], v[b]) // 2)) for a, b in edges]
+            hexlat += [tuple(np.sum([v[k] for k in f], 0) // 4) for f in faces] + [(1, 1, 1)]
+    for c in np.ndindex(*([n] * dim)):
+        base = np.array(c) * g
+        if element == L.ELEM_QUADRANGLE:
+            cells.append([gid[tuple(base + (a, b))] for a, b in quad_lattice(g)])
+        elif element == L.ELEM_TRIANGLE:
+            # (A, B, C) and (A, C, D) of the cell corners, nodes of each on the cell's lattice
+            for tri in (((0, 0), (g, 0), (g, g)), ((0, 0), (g, g), (0, g))):
+                p0, p1, p2 = (np.array(t) for t in tri)
+                cells.append([gid[tuple(base + p0 + ((p1 - p0) * a + (p2 - p0) * b) // g)] for a, b in tri_lattice(g)])
+        elif element == L.ELEM_HEXAHEDRON:
+            cells.append([gid[tuple(base + np.array(h) * g // (2 if g == 2 else 1))] for h in hexlat])
+        else:
+            for perm in KUHN:
+                path = [np.zeros(3, int)]
+                for a in perm:
+                    q = path[-1].copy()
+                    q[a] += g
+                    path.append(q)
+                vs = [gid[tuple(base + p)] for p in path]
+                if g == 2:
+                    for a, b in ((0, 1), (1, 2), (2, 0), (3, 0), (3, 2), (3, 1)):
+                        vs.append(gid[tuple(base + (path[a] + path[b]) // 2)])
+                cells.append(vs)
+    cells = np.array(cells, dtype=np.int64)
+    nv = {L.ELEM_QUADRANGLE: 4, L.ELEM_TRIANGLE: 3, L.ELEM_HEXAHEDRON: 8, L.ELEM_TETRAHEDRON: 4}[element]
+    nfv = {L.ELEM_QUADRANGLE: 2, L.ELEM_TRIANGLE: 2, L.ELEM_HEXAHEDRON: 4, L.ELEM_TETRAHEDRON: 3}[element]
+    kw = {"geometry_order": g}
+    if periodic:
+        links = []
+        for d in range(dim):
+            sl = np.nonzero(grid[:, d] == n * g)[0]
+            mm = grid[sl].copy()
+            mm[:, d] = 0
+            links.append([(int(s), int(gid[tuple(x)])) for s, x in zip(sl, mm)])
+        kw["periodic"] = links
+    else:
+        fvl = {L.ELEM_QUADRANGLE: [(0, 1), (1, 2), (2, 3), (3, 0)], L.ELEM_TRIANGLE: [(0, 1), (1, 2), (2, 0)],
+               L.ELEM_HEXAHEDRON: [(0, 3, 2, 1), (0, 1, 5, 4), (0, 4, 7, 3), (1, 2, 6, 5), (2, 3, 7, 6), (4, 5, 6, 7)],
+               L.ELEM_TETRAHEDRON: [(0, 2, 1), (0, 1, 3), (0, 3, 2), (3, 1, 2)]}[element]
+        count = {}
+        for c in cells:
+            for f in fvl:
+                key = tuple(sorted(c[list(f)]))
+                count[key] = count.get(key, 0) + 1
+        bf, bp = [], []
+        for key, cnt in count.items():
+            if cnt != 1:
+                continue
+            gc = grid[list(key)]
+            for side in range(2 * dim):
+                d, v = side // 2, 0 if side % 2 == 0 else n * g
+                if np.all(gc[:, d] == v):
+                    bf.append(key)
+                    bp.append(side + 1)
+                    break
+        kw.update(boundary_faces=np.array(bf, dtype=np.int64).reshape(-1, nfv), boundary_type=[bc] * len(bf),
+                  boundary_physical=bp)
+    del nv
+    return L.Mesh.from_arrays(element, order, xyz, cells, **kw), cells, xyz
+
+
+def bump(amplitude=0.06):
+    """A smooth warp of the unit square / cube that keeps its sides in place."""
+    def warp(xyz):
+        x = xyz.copy()
+        s = np.sin(np.pi * xyz[:, 0]) * np.sin(np.pi * xyz[:, 1])
+        x[:, 0] += amplitude * s * np.cos(0.7 * xyz[:, 1])
+        x[:, 1] += amplitude * s * np.sin(1.3 * xyz[:, 0] + 0.2)
+        if np.any(xyz[:, 2] != 0):
+            t = s * np.sin(np.pi * xyz[:, 2])
+            x[:, 0] = xyz[:, 0] + amplitude * t
+            x[:, 1] = xyz[:, 1] + amplitude * t * np.cos(xyz[:, 0])
+            x[:, 2] = xyz[:, 2] + amplitude * t * np.sin(2.0 * xyz[:, 1])
+        return x
+    return warp
+
+
+def pyramid_box(n=2, order=2, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), periodic=True, jitter=0.0, seed=3, bc=None):
+    """Pyramids: every box of an n^3 grid split into the six pyramids on its faces with a common apex
+    near the box centre (moved by up to jitter of a box; the bases stay parallelograms, so the
+    elements stay affine). Quadrangle bases pair across boxes, triangles inside a box. periodic:
+    one link per direction; otherwise the box sides get bc[side] (tag side + 1)."""
+    n1 = n + 1
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    h = (hi - lo) / n
+    ijk = np.stack(np.meshgrid(np.arange(n1), np.arange(n1), np.arange(n1), indexing="ij"), -1).reshape(-1, 3)
+    idx = np.arange(n1 ** 3).reshape(n1, n1, n1)
+    xyz = [lo + p * h for p in ijk]
+    rng = np.random.default_rng(seed)
+    # hexahedron faces (gmsh order, outward); the pyramid base is the face reversed (inward normal,
+    # i.e. counter-clockwise seen from the apex)
+    corner = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    faces = [(0, 3, 2, 1), (0, 1, 5, 4), (0, 4, 7, 3), (1, 2, 6, 5), (2, 3, 7, 6), (4, 5, 6, 7)]
+    pyrs, sides = [], []
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                c = len(xyz)
+                xyz.append(lo + (np.array([i, j, k]) + 0.5 + jitter * rng.uniform(-1, 1, 3)) * h)
+                verts = [idx[i + a, j + b, k + cc] for a, b, cc in corner]
+                for f in faces:
+                    base = [verts[f[0]], verts[f[3]], verts[f[2]], verts[f[1]]]
+                    pyrs.append(base + [c])
+    xyz = np.array(xyz)
+    pyrs = np.array(pyrs, dtype=np.int64)
+    kw = {}
+    if periodic:
+        links = []
+        for d in range(3):
+            sl = np.nonzero(ijk[:, d] == n)[0]
+            m = ijk[sl].copy()
+            m[:, d] = 0
+            links.append([(int(s), int(idx[tuple(mm)])) for s, mm in zip(sl, m)])
+        kw["periodic"] = links
+    else:
+        bf, bt, bp = [], [], []
+        for p in pyrs:
+            g = ijk[p[:4]]
+            for side in range(6):
+                d, v = side // 2, 0 if side % 2 == 0 else n
+                if np.all(g[:, d] == v):
+                    bf.append(p[:4])
+                    bt.append((bc or [L.BC_RIEMANN_FARFIELD] * 6)[side])
+                    bp.append(side + 1)
+        kw.update(boundary_faces=np.array(bf, dtype=np.int64), boundary_type=bt, boundary_physical=bp)
+    return L.Mesh.from_arrays(L.ELEM_PYRAMID, order, xyz, pyrs, **kw)

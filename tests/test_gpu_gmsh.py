@@ -82,3 +82,28 @@ def test_cpp_driver_generates_a_mesh(tmp_path):
     assert r.returncode == 0, r.stderr
     assert "mesh file: 40 steps" in r.stdout
     assert not os.path.exists(path)
+
+
+def test_cpp_driver_reads_a_curved_mesh_file(tmp_path):
+    """setMesh(path) in the C++ drop-in on a file of 9-node quadrangles (an annular sector, every
+    geometry node on its circle, boundary lines of 3 nodes, all in the far-field group): a uniform
+    stream stays uniform on the curved cells."""
+    import os
+    import subprocess
+
+    import msh_writer as W
+    from test_curved import _annulus_arrays
+
+    g, na, nr = 2, 5, 2
+    gid, xyz, cells = _annulus_arrays(g, n=(na, nr))
+    radial = np.array([[gid[i, j * g], gid[i, (j + 1) * g], gid[i, j * g + 1]] for j in range(nr) for i in (0, na * g)])
+    arcs = np.array([[gid[i * g, j], gid[(i + 1) * g, j], gid[i * g + 1, j]] for i in range(na) for j in (0, nr * g)])
+    path = str(tmp_path / "annulus.msh")
+    W.write_msh(path, 2, np.arange(1, len(xyz) + 1), xyz, "quad9", cells + 1,
+                [(1, "line3", radial + 1), (1, "line3", arcs + 1)])
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "examples", "bin",
+                       "mesh_file_2d_ceuler")
+    r = subprocess.run([exe, path, "3", "40"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    assert "mesh file: 40 steps" in r.stdout

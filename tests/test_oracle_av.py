@@ -232,3 +232,28 @@ def test_conservation_with_varying_viscosity():
     one = P.project_conserved(mesh, np.ones((mesh.E, mesh.nq, mesh.nv)))[:, :, 0]
     tot = np.einsum("eb,ebv->v", one, R)
     assert np.abs(tot).max() <= 1e-12 * np.abs(R).max() * mesh.E
+
+
+@pytest.mark.parametrize("order", [2, 3, 4])
+def test_sensor_closed_forms_on_tetrahedra(order):
+    """Tetrahedra: the modes above order P-1 start at (P)(P+1)(P+2)/6 (the order P-1 basis is a
+    prefix). No viscosity for a state of degree P-1; the full value f r / P on an element whose
+    density holds only the first mode of degree P, and also only the last one."""
+    mesh = P.tet_box(n=1, order=order, periodic=False)
+    k, f = 0.3, 2.0
+    xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 3)
+    prim = _uniform_state(mesh)
+    prim[..., 0] = 1.0 + 0.1 * (xq[..., 0] + xq[..., 1] * xq[..., 2] if order >= 3 else xq[..., 0])
+    o = _oracle_with_state(mesh, _av_problem(k, f), _conserved(mesh, prim))
+    assert np.abs(o.artificial_viscosity()).max() == 0.0
+    low = order * (order + 1) * (order + 2) // 6
+    tag = mesh.field("node_tag").reshape(mesh.E, 4)
+    radius = mesh.field("inner_radius")
+    for mode in (low, mesh.nb - 1):
+        C = _conserved(mesh, _uniform_state(mesh))
+        C[3, :, 0] = 0.0
+        C[3, mode, 0] = 0.7
+        o = _oracle_with_state(mesh, _av_problem(k, f), C)
+        want = np.zeros(mesh.nodes)
+        want[tag[3]] = f * radius[3] / order
+        assert np.allclose(o.artificial_viscosity(), want, rtol=1e-13, atol=0.0), mode

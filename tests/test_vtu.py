@@ -54,6 +54,8 @@ def _mesh(kind, order):
         return L.Mesh.split(order, (3, 3), (0.0, 0.0), (1.0, 1.0), 1.0, 2)
     if kind == "hex":
         return L.Mesh(L.ELEM_HEXAHEDRON, order, (2, 2, 2), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    if kind == "tet":
+        return P.tet_box(n=1, order=order, periodic=False)
     return L.Mesh(L.ELEM_QUADRANGLE, order, (3, 2), (0.0, 0.0), (1.0, 1.0))
 
 
@@ -101,6 +103,23 @@ def _check_node_order(kind, order, cell):
             expect = np.array([(2 * v[0] + v[1] + v[2]) / 4, (v[0] + 2 * v[1] + v[2]) / 4, (v[0] + v[1] + 2 * v[2]) / 4])
             assert np.allclose(interior, expect)
         return
+    if kind == "tet":
+        v = cell[:4]
+        assert np.dot(np.cross(v[1] - v[0], v[2] - v[0]), v[3] - v[0]) > 0
+        o = 4
+        for a, b in ((0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)):
+            assert _on_segment_in_order(cell[o:o + m], v[a], v[b], m), (a, b)
+            o += m
+        inner = {3: [(1, 1, 1)], 4: [(2, 1, 1), (1, 2, 1), (1, 1, 2)]}.get(P_, [])
+        for f in ((0, 1, 3), (1, 2, 3), (2, 0, 3), (0, 2, 1)):
+            for wts in inner:
+                assert np.allclose(cell[o], sum(w * v[k] for w, k in zip(wts, f)) / P_)
+                o += 1
+        if P_ == 4:
+            assert np.allclose(cell[o], v.mean(0))
+            o += 1
+        assert o == len(cell)
+        return
     if kind == "quad":
         v = cell[:4]
         assert _cross2(v[1] - v[0], v[3] - v[0]) > 0 and np.allclose(v[2], v[1] + v[3] - v[0])
@@ -138,7 +157,8 @@ def _check_node_order(kind, order, cell):
 
 
 @pytest.mark.parametrize("kind,order", [("line", 2), ("line", 4), ("quad", 2), ("quad", 3), ("quad", 4), ("tri", 2),
-                                        ("tri", 3), ("tri", 4), ("hex", 2), ("hex", 3)])
+                                        ("tri", 3), ("tri", 4), ("hex", 2), ("hex", 3), ("tet", 2), ("tet", 3),
+                                        ("tet", 4)])
 def test_view_cells_points_and_fields(tmp_path, kind, order):
     m = _mesh(kind, order)
     D = m.dim
@@ -152,10 +172,10 @@ def test_view_cells_points_and_fields(tmp_path, kind, order):
     names = list(L.VIEW_VARIABLES)
     L.write_view(m, prob, C, path, names, gradient=G, node_values=node, step=7, time=0.25)
     a, npts, ncells = _read(path)
-    per = (order + 1) * (order + 2) // 2 if kind == "tri" else (order + 1) ** D
+    per = {"tri": (order + 1) * (order + 2) // 2, "tet": (order + 1) * (order + 2) * (order + 3) // 6}.get(kind, (order + 1) ** D)
     assert npts == m.E * per and ncells == m.E
     assert a["TMSTEP"] == 7 and a["TimeValue"] == 0.25
-    vtk = {"line": 68, "tri": 69, "quad": 70, "hex": 72}[kind]
+    vtk = {"line": 68, "tri": 69, "quad": 70, "tet": 71, "hex": 72}[kind]
     assert set(np.unique(a["types"]).astype(int)) == {vtk}
     conn = a["connectivity"].astype(np.int64).reshape(ncells, per)
     x = a["Points"][:, :D]
@@ -173,41 +193,44 @@ def test_view_cells_points_and_fields(tmp_path, kind, order):
     def vec(name):
         return a[name] if a[name].ndim == 1 else a[name][:, :D]
 
-    assert np.allclose(a["Density"], rho, rtol=1e-12, atol=1e-13)
+    # the least-squares projection onto the P3 / P4 tetrahedron bases (condition numbers near 1e4 and
+    # 1e5) reproduces the quadratic fields to ~1e-12 instead of ~1e-14
+    k = 30.0 if kind == "tet" else 1.0
+    assert np.allclose(a["Density"], rho, rtol=k * 1e-12, atol=k * 1e-13)
     if D == 1:
-        assert np.allclose(a["Velocity"], vel[:, 0], rtol=1e-12, atol=1e-12)
+        assert np.allclose(a["Velocity"], vel[:, 0], rtol=k * 1e-12, atol=k * 1e-12)
     else:
-        assert np.allclose(a["Velocity"][:, :D], vel, rtol=1e-12, atol=1e-12) and np.all(a["Velocity"][:, D:] == 0.0)
-    assert np.allclose(a["Pressure"], p, rtol=1e-11)
-    assert np.allclose(a["Temperature"], T, rtol=1e-11)
-    assert np.allclose(a["SoundSpeed"], c, rtol=1e-11)
-    assert np.allclose(a["Entropy"], p / rho ** GAMMA, rtol=1e-11)
-    assert np.allclose(vec("MachNumber"), (vel / c[:, None]).squeeze() if D == 1 else vel / c[:, None], rtol=1e-11,
+        assert np.allclose(a["Velocity"][:, :D], vel, rtol=k * 1e-12, atol=k * 1e-12) and np.all(a["Velocity"][:, D:] == 0.0)
+    assert np.allclose(a["Pressure"], p, rtol=k * 1e-11)
+    assert np.allclose(a["Temperature"], T, rtol=k * 1e-11)
+    assert np.allclose(a["SoundSpeed"], c, rtol=k * 1e-11)
+    assert np.allclose(a["Entropy"], p / rho ** GAMMA, rtol=k * 1e-11)
+    assert np.allclose(vec("MachNumber"), (vel / c[:, None]).squeeze() if D == 1 else vel / c[:, None], rtol=k * 1e-11,
                        atol=1e-12)
     for d, axis in enumerate("XYZ"[:D]):
-        assert np.allclose(a["Velocity" + axis], vel[:, d], rtol=1e-12, atol=1e-12)
-        assert np.allclose(a["MachNumber" + axis], vel[:, d] / c, rtol=1e-11, atol=1e-12)
+        assert np.allclose(a["Velocity" + axis], vel[:, d], rtol=k * 1e-12, atol=k * 1e-12)
+        assert np.allclose(a["MachNumber" + axis], vel[:, d] / c, rtol=k * 1e-11, atol=k * 1e-12)
     # gradients: grad u_k = (grad m_k - u_k grad rho) / rho, grad T from grad e
     g = _gradients(x)  # (npts, NV, D)
     gu = (g[:, 1:1 + D, :] - vel[:, :, None] * g[:, 0:1, :]) / rho[:, None, None]  # [k][d] = d u_k / d x_d
     e = rhoE / rho - 0.5 * (vel ** 2).sum(-1)
     ge = (g[:, D + 1, :] - (rhoE / rho)[:, None] * g[:, 0, :]) / rho[:, None] - np.einsum("pk,pkd->pd", vel, gu)
     gT = ge / CV
-    assert np.allclose(vec("HeatFlux"), gT.squeeze() if D == 1 else gT, rtol=1e-10, atol=1e-12)
+    assert np.allclose(vec("HeatFlux"), gT.squeeze() if D == 1 else gT, rtol=k * 1e-10, atol=k * 1e-12)
     for d, axis in enumerate("XYZ"[:D]):
-        assert np.allclose(a["HeatFlux" + axis], gT[:, d], rtol=1e-10, atol=1e-12)
+        assert np.allclose(a["HeatFlux" + axis], gT[:, d], rtol=k * 1e-10, atol=k * 1e-12)
     if D == 2:
-        assert np.allclose(a["Vorticity"], gu[:, 1, 0] - gu[:, 0, 1], rtol=1e-10, atol=1e-12)
-        assert np.allclose(a["VorticityZ"], gu[:, 1, 0] - gu[:, 0, 1], rtol=1e-10, atol=1e-12)
+        assert np.allclose(a["Vorticity"], gu[:, 1, 0] - gu[:, 0, 1], rtol=k * 1e-10, atol=k * 1e-12)
+        assert np.allclose(a["VorticityZ"], gu[:, 1, 0] - gu[:, 0, 1], rtol=k * 1e-10, atol=k * 1e-12)
     if D == 3:
         w = np.stack([gu[:, 2, 1] - gu[:, 1, 2], gu[:, 0, 2] - gu[:, 2, 0], gu[:, 1, 0] - gu[:, 0, 1]], axis=-1)
-        assert np.allclose(a["Vorticity"], w, rtol=1e-10, atol=1e-12)
-        assert np.allclose(a["VorticityX"], w[:, 0], rtol=1e-10, atol=1e-12)
+        assert np.allclose(a["Vorticity"], w, rtol=k * 1e-10, atol=k * 1e-12)
+        assert np.allclose(a["VorticityX"], w[:, 0], rtol=k * 1e-10, atol=k * 1e-12)
     # artificial viscosity: node values at the element vertices (the first nodes of each cell), P1
     # interpolant elsewhere (at the edge nodes: linear between the edge's vertices)
     tag = m.field("node_tag").reshape(m.E, m.nvert)
     av = a["ArtificialViscosity"][conn]
-    vmap = {"line": [0, 1], "tri": [0, 1, 2], "quad": [0, 1, 2, 3], "hex": list(range(8))}[kind]
+    vmap = {"line": [0, 1], "tri": [0, 1, 2], "quad": [0, 1, 2, 3], "tet": [0, 1, 2, 3], "hex": list(range(8))}[kind]
     assert np.allclose(av[:, vmap], node[tag], rtol=1e-14, atol=1e-15)
     assert np.all(av.min(1) >= node[tag].min(1) - 1e-14) and np.all(av.max(1) <= node[tag].max(1) + 1e-14)
     _ = e
@@ -237,3 +260,11 @@ def test_vtu_defaults_without_viscosity_and_errors(tmp_path):
     rc = L.load().sdg_write_vtu(mixed.handle, ctypes.byref(prob), buf.ctypes.data, None, 0, 0.0, path.encode())
     assert rc == L.E_UNSUPPORTED
     _ = math
+
+
+def test_pyramids_are_refused(tmp_path):
+    m = P.pyramid_box(n=1, order=2, periodic=False)
+    prob = L.problem_desc(equation=L.EQ_EULER, cp=GAMMA * CV, cv=CV)
+    C = P.project_conserved(m, np.tile(np.array([1.0, 0.1, 0.0, 0.0, 2.5]), (m.E, m.nq, 1)))
+    with pytest.raises(L.SDGError, match="pyramids"):
+        L.write_view(m, prob, C, str(tmp_path / "p.vtu"), ["Density"])
