@@ -306,6 +306,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
       const int64_t es = e * NAQ + sT.slot_of[q];
       for (int d = 0; d < D; d++) nx[d] = g.snrm[es * D + d];
     }
+    const double inw = 1.0 / (sT.wn[q] * dj);  // M^-1 at the node, one reciprocal for the six faces
 #pragma unroll 1  // unrolled, the six faces' LDS loads are hoisted together and double the VGPRs
     for (int s = 0; s < 6; s++) {
       const int ax = t.axis[s], i = nodeIndex(q, ax, N);
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
         for (int d = 0; d < D; d++) nx[d] = g.snrm[es * D + d];
       }
       // alpha, beta carry |J_f| w_t1 w_t2; M^-1 = 1 / (w_i w_t1 w_t2 |J|) at the node
-      const double coef = sT.lf[t.side[s] * N + i] / (sT.wn[q] * dj);
+      const double coef = sT.lf[t.side[s] * N + i] * inw;
       for (int v = 0; v < NV; v++) {
         const double al = static_cast<double>(Al[v * NAQ + sl]) * coef;
         const double be = static_cast<double>(Bt[v * NAQ + sl]) * coef;
@@ -390,13 +391,14 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
       double n[D], us[NV], gs[NG];
       for (int d = 0; d < D; d++) n[d] = nk[k][d];
       for (int v = 0; v < NV; v++) us[v] = pointState<TC>(a, uk[k][v], v);
+      const double kf = sT.kappa[sT.side[s]] * idj / sT.fwt[sl];  // BR2 own-face lift factor of the slot
       for (int r = 0; r < NG; r++) {
         TC acc = TC(0);
 #pragma unroll
         for (int i = 0; i < N; i++) acc += static_cast<TC>(l[i]) * W[r * NPT + b + i * st];
         double gv = static_cast<double>(acc);
         if constexpr (VISC == 2)  // r_s at its own face: kappa_side beta n_d / |J|
-          gv += sT.kappa[sT.side[s]] * static_cast<double>(Bt[(r / D) * NAQ + sl]) * n[r % D] * idj / sT.fwt[sl];
+          gv += kf * static_cast<double>(Bt[(r / D) * NAQ + sl]) * n[r % D];
         const TC gt = static_cast<TC>(gv);
         gs[r] = static_cast<double>(gt);
         if (nbs < 0) Gf[(e * NG + r) * NAQ + sl] = gt;
@@ -513,8 +515,10 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsResidualTensor(Geo g, P
   elemSync<NPT>();  // the face-flux slots are read; their region takes the nodal residual
   const bool monitor = a.mode == 1 || a.last_stage;
   const double jw = ok ? g.detj[e] * sT.wn[q] : 1.0;
-  if (ok)  // residual R_psi for the monitor stages, else directly the nodal increment R_psi / (w |J|)
-    for (int v = 0; v < NV; v++) SR[v * NPT + q] = monitor ? R[v] : R[v] / jw;
+  if (ok) {  // residual R_psi for the monitor stages, else directly the nodal increment R_psi / (w |J|)
+    const double ijw = 1.0 / jw;
+    for (int v = 0; v < NV; v++) SR[v * NPT + q] = monitor ? R[v] : R[v] * ijw;
+  }
   elemSync<NPT>();
   // the reference residual (before M^-1) is (B^T x B^T x B^T) R_psi (modal); its monitor is
   // mean_q |R V^T| = mean over nodes of |(B x B x B) of it| (TimeIntegration.cpp:279-298)
@@ -555,8 +559,10 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsResidualTensor(Geo g, P
     if (ok && q < NV) a.mon[e * NV + q] = sM[le][q] / NPT;
     // the nodal increment R_psi / (w |J|) from the registers (SR was overwritten)
     elemSync<NPT>();
-    if (ok)
-      for (int v = 0; v < NV; v++) SR[v * NPT + q] = R[v] / jw;
+    if (ok) {
+      const double ijw = 1.0 / jw;
+      for (int v = 0; v < NV; v++) SR[v * NPT + q] = R[v] * ijw;
+    }
     elemSync<NPT>();
   }
   // nodal increment -> modal with B^-1 along each axis (FP64)

@@ -56,18 +56,21 @@ __device__ __forceinline__ T sutherlandRatio(T t) {  // PhysicalModel.cpp:111-11
   return sqrt(t * t * t) * (T(1.0) + T(kSutherland)) / (t + T(kSutherland));
 }
 
-// conserved -> computational (rho, u.., e, p): VariableConvertor.cpp:315-339
+// conserved -> computational (rho, u.., e, p): VariableConvertor.cpp:315-339. The divisions by rho
+// are products with one reciprocal (an FP64 division is a multi-instruction sequence around a
+// quarter-rate reciprocal; this differs from the reference's quotients by rounding only).
 template <int D, typename T>
 __device__ __forceinline__ void computational(const Phys& ph, const T* u, T* w) {
   const T rho = u[0];
+  const T ir = T(1.0) / rho;
   w[0] = rho;
   T v2 = T(0);
 #pragma unroll
   for (int d = 0; d < D; d++) {
-    w[1 + d] = u[1 + d] / rho;
+    w[1 + d] = u[1 + d] * ir;
     v2 += w[1 + d] * w[1 + d];
   }
-  const T e = u[D + 1] / rho - v2 / T(2.0);
+  const T e = u[D + 1] * ir - v2 / T(2.0);
   w[D + 1] = e;
   w[D + 2] = pressureOf(ph, rho, e);
 }
@@ -82,9 +85,11 @@ __device__ __forceinline__ T velocitySquared(const T* w) {
 // conserved gradient (row var*D+dim) -> primitive gradient: VariableConvertor.cpp:574-620.
 // pg layout: [0,D) grad rho; [D, D+D*D) velocity gradient column-major (a,b) = du_b/dx_a;
 // [(D+1)D, (D+2)D) grad T
+// (divisions by rho and c_v as products with reciprocals, as in computational)
 template <int D, typename T>
 __device__ __forceinline__ void primitiveGradient(const Phys& ph, const T* w, const T* u, const T* g, T* pg) {
   const T rho = w[0];
+  const T ir = T(1.0) / rho, icv = T(1.0) / T(ph.cv);
 #pragma unroll
   for (int a = 0; a < D; a++) pg[a] = g[a];
   T vg[D * D];
@@ -92,17 +97,17 @@ __device__ __forceinline__ void primitiveGradient(const Phys& ph, const T* w, co
   for (int b = 0; b < D; b++)
 #pragma unroll
     for (int a = 0; a < D; a++) {
-      vg[a + D * b] = (g[(1 + b) * D + a] - g[a] * w[1 + b]) / rho;
+      vg[a + D * b] = (g[(1 + b) * D + a] - g[a] * w[1 + b]) * ir;
       pg[(1 + b) * D + a] = vg[a + D * b];
     }
-  const T etot = u[D + 1] / rho;
+  const T etot = u[D + 1] * ir;
 #pragma unroll
   for (int a = 0; a < D; a++) {
     T vgu = T(0);
 #pragma unroll
     for (int b = 0; b < D; b++) vgu += vg[a + D * b] * w[1 + b];
-    const T de = (g[(D + 1) * D + a] - g[a] * etot) / rho - vgu;
-    pg[(D + 1) * D + a] = de / T(ph.cv);
+    const T de = (g[(D + 1) * D + a] - g[a] * etot) * ir - vgu;
+    pg[(D + 1) * D + a] = de * icv;
   }
 }
 
@@ -172,20 +177,22 @@ __device__ void convectiveFlux(const Phys& ph, const T* n, const T* ul, const T*
     }
     const T sm = (pr - pl + rl * unl * (sl - unl) - rr * unr * (sr - unr)) / (rl * (sl - unl) - rr * (sr - unr));
     T us[NV];
-    if (sm >= T(0)) {
+    if (sm >= T(0)) {  // the star state's divisions by (s - s_m) as products with one reciprocal
       normalFlux<D>(n, wl, fl);
-      us[0] = rl * (sl - unl) / (sl - sm);
+      const T is = T(1.0) / (sl - sm);
+      us[0] = rl * (sl - unl) * is;
 #pragma unroll
-      for (int d = 0; d < D; d++) us[1 + d] = ((sl - unl) * rl * wl[1 + d] + (ps - pl) * n[d]) / (sl - sm);
-      us[D + 1] = ((sl - unl) * rl * (wl[D + 1] + velocitySquared<D>(wl) / T(2.0)) - pl * unl + ps * sm) / (sl - sm);
+      for (int d = 0; d < D; d++) us[1 + d] = ((sl - unl) * rl * wl[1 + d] + (ps - pl) * n[d]) * is;
+      us[D + 1] = ((sl - unl) * rl * (wl[D + 1] + velocitySquared<D>(wl) / T(2.0)) - pl * unl + ps * sm) * is;
 #pragma unroll
       for (int v = 0; v < NV; v++) h[v] = fl[v] + sl * (us[v] - ul[v]);
     } else {
       normalFlux<D>(n, wr, fr);
-      us[0] = rr * (sr - unr) / (sr - sm);
+      const T is = T(1.0) / (sr - sm);
+      us[0] = rr * (sr - unr) * is;
 #pragma unroll
-      for (int d = 0; d < D; d++) us[1 + d] = ((sr - unr) * rr * wr[1 + d] + (ps - pr) * n[d]) / (sr - sm);
-      us[D + 1] = ((sr - unr) * rr * (wr[D + 1] + velocitySquared<D>(wr) / T(2.0)) - pr * unr + ps * sm) / (sr - sm);
+      for (int d = 0; d < D; d++) us[1 + d] = ((sr - unr) * rr * wr[1 + d] + (ps - pr) * n[d]) * is;
+      us[D + 1] = ((sr - unr) * rr * (wr[D + 1] + velocitySquared<D>(wr) / T(2.0)) - pr * unr + ps * sm) * is;
 #pragma unroll
       for (int v = 0; v < NV; v++) h[v] = fr[v] + sr * (us[v] - ur[v]);
     }
@@ -194,14 +201,15 @@ __device__ void convectiveFlux(const Phys& ph, const T* n, const T* ul, const T*
     normalFlux<D>(n, wr, fr);
     const T sql = sqrt(wl[0]), sqr = sqrt(wr[0]);
     const T sqs = sql + sqr;
+    const T isqs = T(1.0) / sqs;  // Roe averages: products with one reciprocal of the weight sum
     const T rho = sqrt(wl[0] * wr[0]);
     T vel[D];
 #pragma unroll
-    for (int d = 0; d < D; d++) vel[d] = (sql * wl[1 + d] + sqr * wr[1 + d]) / sqs;
+    for (int d = 0; d < D; d++) vel[d] = (sql * wl[1 + d] + sqr * wr[1 + d]) * isqs;
     const T g = T(kGamma);
     const T hl = wl[D + 1] * g + velocitySquared<D>(wl) / T(2.0);
     const T hr = wr[D + 1] * g + velocitySquared<D>(wr) / T(2.0);
-    const T htot = (sql * hl + sqr * hr) / sqs;
+    const T htot = (sql * hl + sqr * hr) * isqs;
     T v2 = T(0), un = T(0);
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -229,12 +237,13 @@ __device__ void convectiveFlux(const Phys& ph, const T* n, const T* ul, const T*
     T sum[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) sum[v] = T(0);
-    const T s0 = a1 * (dp - rho * c * dun) / (T(2.0) * c * c);
+    const T ic2 = T(1.0) / (c * c);
+    const T s0 = a1 * (dp - rho * c * dun) * (T(0.5) * ic2);
     sum[0] += T(1.0) * s0;
 #pragma unroll
     for (int d = 0; d < D; d++) sum[1 + d] += (vel[d] - c * n[d]) * s0;
     sum[D + 1] += (htot - c * un) * s0;
-    const T s1 = fabs(un) * (drho - dp / (c * c));
+    const T s1 = fabs(un) * (drho - dp * ic2);
     sum[0] += T(1.0) * s1;
 #pragma unroll
     for (int d = 0; d < D; d++) sum[1 + d] += vel[d] * s1;
@@ -247,7 +256,7 @@ __device__ void convectiveFlux(const Phys& ph, const T* n, const T* ul, const T*
       sum[D + 1] += (vdv - un * dun) * s2;
     }
     // in 3-D the fourth column stays zero (ConvectiveFlux.cpp:240-350); adding it changes nothing
-    const T s3 = a3 * (dp + rho * c * dun) / (T(2.0) * c * c);
+    const T s3 = a3 * (dp + rho * c * dun) * (T(0.5) * ic2);
     sum[0] += T(1.0) * s3;
 #pragma unroll
     for (int d = 0; d < D; d++) sum[1 + d] += (vel[d] + c * n[d]) * s3;
@@ -280,9 +289,12 @@ __device__ void convectiveFlux(const Phys& ph, const T* n, const T* ul, const T*
 // viscous raw flux F_v, D x NV column-major: ViscousFlux.cpp:58-103
 template <int D, typename T>
 __device__ __forceinline__ void viscousRawFlux(const Phys& ph, const T* w, const T* pg, T* fv) {
-  const T t = w[D + 1] / T(ph.cv);
-  const T mu = ph.transport == 2 ? T(ph.mu) * sutherlandRatio(t) : T(ph.mu);
-  const T kap = ph.transport == 2 ? T(ph.kappa) * sutherlandRatio(t) : T(ph.kappa);
+  T mu = T(ph.mu), kap = T(ph.kappa);
+  if (ph.transport == 2) {
+    const T r = sutherlandRatio(w[D + 1] / T(ph.cv));
+    mu = mu * r;
+    kap = kap * r;
+  }
   const T* vg = pg + D;
   T tr = T(0);
 #pragma unroll
