@@ -532,8 +532,8 @@ def shock_leg(args, n=256, order=3, steps=100, cfl=0.1):
     """Artificial-viscosity cost on the explosion case (2-D Euler, [0,2]^2 with adiabatic slip
     walls, a circle of radius 0.4 at rho = 1, p = 1 in rho = 0.125, p = 0.1 spread over half a
     cell, 256^2 quadrangles P3, HLLC, SSP-RK3, setArtificialViscosity(4.0)): the step with the
-    per-step sensor and the gradient passes (vector kernels) against the plain Euler step on the
-    same mesh and state."""
+    per-step sensor and the gradient passes (matrix-core kernels, FP64) against the plain Euler step
+    on the same mesh and state."""
     slip = L.BC_ADIABATIC_SLIP_WALL
     mesh = L.Mesh(L.ELEM_QUADRANGLE, order, (n, n), (0.0, 0.0), (2.0, 2.0), boundary=((slip, slip), (slip, slip)))
     xq = mesh.field("quad_coord").reshape(mesh.E, mesh.nq, 2)

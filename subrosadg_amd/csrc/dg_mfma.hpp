@@ -561,8 +561,8 @@ __device__ __forceinline__ void gradLift(const Geo& g, const Phys& ph, const Sta
       const int k0 = (s_lo * NFQ) / 4 * 4;
       const int k1 = r4((s_hi + 1) * NFQ);
       tileAcc<TC, NAQm>(acc, op.lift, m0, sJ, k0, k1 < NAQk ? k1 : NAQk, lane);
-    } else {
-      tileAcc<TC, NAQm>(acc, op.ptrace, m0, sRt, 0, NBk, lane);
+    } else {  // BR1: the total gradient; VISC == 0 (artificial viscosity): the volume gradient
+      tileAcc<TC, NAQm>(acc, op.ptrace, m0, VISC == 0 ? sRw : sRt, 0, NBk, lane);
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -576,7 +576,7 @@ __device__ __forceinline__ void gradLift(const Geo& g, const Phys& ph, const Sta
   #pragma unroll 1
   for (int m0 = wave * 16; m0 < NQm; m0 += 64) {
     acc_t<TC> acc = zeroAcc<TC>();
-    tileAcc<TC, NQm>(acc, op.pinterp, m0, sRt, 0, NBk, lane);
+    tileAcc<TC, NQm>(acc, op.pinterp, m0, VISC == 0 ? sRw : sRt, 0, NBk, lane);
 #pragma unroll
     for (int i = 0; i < 4; i++) acc[i] = static_cast<TC>(static_cast<double>(acc[i]) * idj);
     tileStore<TC>(acc, sGq, m0, lane);
@@ -606,8 +606,15 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
     for (int v = 0; v < NV; v++) u[v] = pointState<TC>(a, sUq[q * 16 + le * NV + v], v);
     for (int r = 0; r < NG; r++) gq[r] = sGq[q * 16 + le * NG + r];
     computational<D>(ph, u, w);
-    primitiveGradient<D>(ph, w, u, gq, pg);
-    viscousRawFlux<D>(ph, w, pg, fv);
+    if constexpr (VISC == 0) {  // Euler with artificial viscosity: F_v = eps grad U (volume gradient)
+      const double eps = avAt(g, g.nodq, q, e);
+      for (int v = 0; v < NV; v++)
+        for (int aa = 0; aa < D; aa++) fv[aa + D * v] = eps * gq[v * D + aa];
+      (void)pg;
+    } else {
+      primitiveGradient<D>(ph, w, u, gq, pg);
+      viscousRawFlux<D>(ph, w, pg, fv);
+    }
     const double jw = g.detj[e] * g.wq[q];
     double G[D * D], out[D * NV];
     for (int i = 0; i < D * D; i++) G[i] = g.ginv[static_cast<int64_t>(e) * D * D + i] * jw;
@@ -626,7 +633,9 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
 // kernel (a.Vn holding TC values, with the same arithmetic as below); the face kernel then reads
 // 2 NV values per point instead of both sides' 2 NV D gradient traces (boundary points still
 // read Gf).
-template <typename TC, int D, int NF, int NFQ, bool VN = false>
+// AV: Euler with artificial viscosity (FP64): Gf holds the volume-gradient traces and the viscous
+// part is the central average of eps grad U . n (SpatialDiscrete.cpp:714-735,812-830).
+template <typename TC, int D, int NF, int NFQ, bool VN = false, bool AV = false>
 __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, StageArgs a) {
   constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
   if (a.status && a.status[0]) return;
@@ -643,7 +652,15 @@ __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, Sta
     double ub[NV], gb[NG], t[NV];
     for (int v = 0; v < NV; v++) ub[v] = pointState<TC>(a, Uf_in[(e * NV + v) * NAQ + sl], v);
     for (int r = 0; r < NG; r++) gb[r] = Gf[(e * NG + r) * NAQ + sl];
-    boundaryFlux<D, true>(ph, g, bp, ub, gb, sizeof(TC) == 4, a.mix.pref, a.mix.h0, t);
+    boundaryFlux<D, !AV>(ph, g, bp, ub, gb, sizeof(TC) == 4, a.mix.pref, a.mix.h0, t);
+    if constexpr (AV) {  // minus eps grad U . n of the element's own volume gradient
+      const double eps = avAt(g, g.nodf, sl, static_cast<int>(e)), bjw = g.bjw[bp];
+      for (int v = 0; v < NV; v++) {
+        double f = 0.0;
+        for (int d = 0; d < D; d++) f += (eps * gb[v * D + d]) * g.bnormal[static_cast<int64_t>(bp) * D + d];
+        t[v] -= f * bjw;
+      }
+    }
     for (int v = 0; v < NV; v++) Hs[(e * NV + v) * NAQ + sl] = static_cast<TC>(-t[v]);
     return;
   }
@@ -665,7 +682,16 @@ __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, Sta
   if constexpr (sizeof(TC) == 4) reduceFaceFlux<D>(a, n, h);
   // normal viscous flux of each side, one side at a time (keeps one side's gradient live)
   double vl[NV], vr[NV];
-  if constexpr (VN) {  // the right side's outward normal is -n
+  if constexpr (AV) {  // eps grad U . n of each side with the left normal
+    const double el = avAt(g, g.nodf, slL, static_cast<int>(L)), er = avAt(g, g.nodf, slR, static_cast<int>(R));
+    for (int v = 0; v < NV; v++) {
+      double al = 0.0, ar = 0.0;
+      for (int d = 0; d < D; d++) al += (el * static_cast<double>(Gf[(L * NG + v * D + d) * NAQ + slL])) * n[d];
+      for (int d = 0; d < D; d++) ar += (er * static_cast<double>(Gf[(R * NG + v * D + d) * NAQ + slR])) * n[d];
+      vl[v] = al;
+      vr[v] = ar;
+    }
+  } else if constexpr (VN) {  // the right side's outward normal is -n
     const TC* Vn = reinterpret_cast<const TC*>(a.Vn);
     for (int v = 0; v < NV; v++) {
       vl[v] = static_cast<double>(Vn[(L * NV + v) * NAQ + slL]);

@@ -99,6 +99,22 @@ Launchers makeLaunchers() {
       hipLaunchKernelGGL((sdg::nsResidualMfma<TC, D, NB, NQ, NF, NFQ>), dim3(blocksFor(g.E, EPB_S)), dim3(sdg::kThreads),
                          0, st, g, ph, a);
     };
+    if constexpr (sizeof(TC) == 8) {  // Euler with artificial viscosity (FP64): volume gradient, AV fluxes
+      l.av_grad = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+        hipLaunchKernelGGL((sdg::nsGradMfma<TC, D, NB, NQ, NF, NFQ, 0>), dim3(blocksFor(g.E, EPB_G)),
+                           dim3(sdg::kThreads), 0, st, g, ph, a);
+      };
+      l.av_flux = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+        hipLaunchKernelGGL((sdg::nsFaceFluxKernel<TC, D, NF, NFQ, false, true>),
+                           dim3(blocksFor((g.F + g.Fb) * NFQ, sdg::kThreads)), dim3(sdg::kThreads), 0, st, g, ph, a);
+        hipLaunchKernelGGL((sdg::nsResidualMfma<TC, D, NB, NQ, NF, NFQ>), dim3(blocksFor(g.E, EPB_S)),
+                           dim3(sdg::kThreads), 0, st, g, ph, a);
+      };
+      l.av_sensor = [](hipStream_t st, const Geo& g, const Phys& ph, const double* U, double* node) {
+        hipLaunchKernelGGL((sdg::avSensorKernel<D, NB, NQ>), dim3(blocksFor(g.E, sdg::kThreads)), dim3(sdg::kThreads),
+                           0, st, g, ph, U, node);
+      };
+    }
     if constexpr (sizeof(TC) == 8) {
       l.adj_volume[0] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
         hipLaunchKernelGGL((sdg::nsAdjVolumeMfma<D, NB, NQ, NF, NFQ, 1>), dim3(blocksFor(g.E, EPB_G)),
@@ -911,7 +927,10 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     delete ctx;
     return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: the vector kernels run in FP64");
   }
-  if (!selectLaunchers(r.type, r.order, force_vector || ctx->av, mixed, ctx->launch, tensor && !ctx->av)) {
+  // artificial viscosity: the vector kernels, or for Euler in FP64 the matrix-core kernels where the
+  // element size has them (the Navier-Stokes terms need the vector kernels' second gradient trace)
+  const bool av_vector = ctx->av && problem->equation == SDG_EQ_COMPRESSIBLE_NS;
+  if (!selectLaunchers(r.type, r.order, force_vector || av_vector, mixed, ctx->launch, tensor && !ctx->av)) {
     delete ctx;
     return fail(nullptr, SDG_E_UNSUPPORTED,
                 mixed ? "sdg_create: the mixed-precision mode needs a matrix-core kernel set (quadrangle P3-P5, "
@@ -1343,7 +1362,9 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     if (tensor && (allocate(ctx, nUf, &ctx->Vn) || allocate(ctx, nU, &ctx->Rvol))) return bail(SDG_E_NOMEM);
   }
   if (ctx->av && !ctx->ns() &&
-      (allocate(ctx, static_cast<size_t>(E) * ctx->ng * naq, &ctx->Gf) || allocate(ctx, nU, &ctx->Rvol)))
+      (allocate(ctx, static_cast<size_t>(E) * ctx->ng * naq, &ctx->Gf) ||
+       (ctx->launch.mfma ? (allocate(ctx, nUf, &ctx->Hs) || allocate(ctx, static_cast<size_t>(E) * nq * D * ctx->nv, &ctx->Fv))
+                         : allocate(ctx, nU, &ctx->Rvol))))
     return bail(SDG_E_NOMEM);
   if (ctx->av && ctx->ns() && allocate(ctx, static_cast<size_t>(E) * ctx->ng * naq, &ctx->GfV)) return bail(SDG_E_NOMEM);
   // modal Boussinesq source for the kernel sets that take it from a buffer (the sum-factorised
