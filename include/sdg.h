@@ -463,6 +463,13 @@ int sdg_adjoint_forward(sdg_ctx* ctx, double dt, int nsteps, int snapshots);
  * the context state at the recorded initial state. */
 int sdg_adjoint_backward(sdg_ctx* ctx, const double* lambda_final, double* lambda_initial);
 
+/* sdg_adjoint_backward with both adjoints in device memory of the context's device (hipMalloc'd,
+ * or CUDA tensors of torch), in the same [E][N_b][NV] layout: no host copy and no PCIe transfer,
+ * for callers whose objective gradient is formed on the device. Returns once lambda_initial is
+ * written (stream synchronised). SDG_E_INVALID for host or other-device pointers. No counterpart
+ * in the reference (it has no adjoint). */
+int sdg_adjoint_backward_device(sdg_ctx* ctx, const double* lambda_final, double* lambda_initial);
+
 /* ---- batched solver (no counterpart in the reference: it runs one problem per process) -------
  * Many independent 2-D compressible Euler problems on split grids (triangles and quadrangles,
  * orders 1..4), advanced together: every RK stage of the whole batch is one persistent launch,

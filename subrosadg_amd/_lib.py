@@ -216,6 +216,7 @@ SIGNATURES = {
     "sdg_batch_get_config": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(c_int)]),
     "sdg_adjoint_forward": (c_int, [c_void_p, c_double, c_int, c_int]),
     "sdg_adjoint_backward": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "sdg_adjoint_backward_device": (c_int, [c_void_p, c_void_p, c_void_p]),
     "sdg_raw_binary_write": (c_int, [c_void_p, c_char_p, c_int64]),
     "sdg_raw_binary_read": (c_int, [c_void_p, c_char_p, c_int]),
     "sdg_raw_file_write": (c_int, [c_char_p, c_void_p, c_int64]),
@@ -420,6 +421,57 @@ class Mesh:
             self.handle = None
 
 
+def _hip_runtime():
+    """The HIP runtime the library links (already loaded with it, found by its soname)."""
+    load()
+    lib = ctypes.CDLL("libamdhip64.so.7")
+    lib.hipMalloc.argtypes = [POINTER(c_void_p), ctypes.c_size_t]
+    lib.hipFree.argtypes = [c_void_p]
+    lib.hipMemcpy.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, c_int]
+    for f in (lib.hipMalloc, lib.hipFree, lib.hipMemcpy):
+        f.restype = c_int
+    return lib
+
+
+class DeviceArray:
+    """A float64 array in device memory (hipMalloc on the current device), for the entry points
+    that take device pointers. Freed with the object."""
+
+    def __init__(self, shape):
+        self.shape = tuple(int(x) for x in shape)
+        self.nbytes = 8 * int(np.prod(self.shape))
+        self._hip = _hip_runtime()
+        self._ptr = c_void_p()
+        if self._hip.hipMalloc(ctypes.byref(self._ptr), max(self.nbytes, 8)) != 0:
+            raise MemoryError(f"hipMalloc of {self.nbytes} bytes failed")
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(a.shape)
+        if d._hip.hipMemcpy(d._ptr, a.ctypes.data, d.nbytes, 1) != 0:  # host to device
+            raise RuntimeError("hipMemcpy to the device failed")
+        return d
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape)
+        if self._hip.hipMemcpy(out.ctypes.data, self._ptr, self.nbytes, 2) != 0:  # device to host
+            raise RuntimeError("hipMemcpy from the device failed")
+        return out
+
+    def data_ptr(self) -> int:
+        return self._ptr.value
+
+    def __del__(self):
+        if getattr(self, "_ptr", None) is not None and self._ptr.value:
+            self._hip.hipFree(self._ptr)
+            self._ptr = None
+
+
+def _device_ptr(x) -> int:
+    return x.data_ptr()
+
+
 class Solver:
     """A device solver context (sdg_ctx) on one mesh: the drop-in for Solver<SC>."""
 
@@ -501,6 +553,18 @@ class Solver:
         out = np.empty(self.state_shape())
         self._check(self.lib.sdg_adjoint_backward(self.handle, lam.ctypes.data, out.ctypes.data))
         return out
+
+    def adjoint_backward_device(self, lam_final, lam_initial=None):
+        """adjoint_backward with both adjoints in device memory of the solver's device (float64,
+        reference layout): DeviceArray buffers or CUDA tensors. Nothing crosses to the host.
+        Returns lam_initial (a DeviceArray, or an empty_like tensor, when not given)."""
+        assert tuple(lam_final.shape) == self.state_shape()
+        if lam_initial is None:
+            lam_initial = DeviceArray(lam_final.shape) if isinstance(lam_final, DeviceArray) else \
+                lam_final.new_empty(lam_final.shape)
+        assert tuple(lam_initial.shape) == self.state_shape()
+        self._check(self.lib.sdg_adjoint_backward_device(self.handle, _device_ptr(lam_final), _device_ptr(lam_initial)))
+        return lam_initial
 
     def write_raw(self, path: str, node_count: int = 0):
         """Raw state file in the reference's format (header + zstd frame; RawBinary.cpp:42-191)."""

@@ -6,8 +6,9 @@ its backward pass runs the checkpointed discrete adjoint of the library (sdg_adj
 sdg_adjoint_backward), so any differentiable torch objective of the final state can be
 back-propagated to the initial state.
 
-The library works on host buffers at its boundary, so tensors cross to NumPy (CPU) here; the
-stepping and the adjoint run on the GPU.
+The state crosses to NumPy (CPU) for set_state / get_state; a CUDA gradient of the final state is
+handed to the reverse sweep in device memory (sdg_adjoint_backward_device), a CPU one through the
+host entry point. The stepping and the adjoint run on the GPU either way.
 """
 from __future__ import annotations
 
@@ -26,6 +27,9 @@ class DGRollout(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_out: torch.Tensor):
+        if grad_out.is_cuda:  # device-resident reverse sweep: no host round trip of the adjoints
+            lam = grad_out.detach().to(torch.float64).contiguous()
+            return ctx.solver.adjoint_backward_device(lam).to(grad_out.dtype), None, None, None, None
         lam0 = ctx.solver.adjoint_backward(grad_out.detach().cpu().numpy().astype(np.float64))
         return torch.from_numpy(lam0).to(dtype=grad_out.dtype, device=grad_out.device), None, None, None, None
 

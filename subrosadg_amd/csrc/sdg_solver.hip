@@ -17,6 +17,7 @@
 
 #include "../../include/sdg.h"
 #include "dg_adjoint.hpp"
+#include "dg_tensor_adjoint.hpp"
 #include "dg_tensor.hpp"
 #include "dg_kernels.hpp"
 #include "dg_mfma.hpp"
@@ -262,6 +263,24 @@ void useTensorKernels(Launchers& l) {
     hipLaunchKernelGGL((sdg::tensorTraceKernel<TC, N>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, U,
                        Uf, mix);
   };
+  if constexpr (sizeof(TC) == 8) {
+    // the transpose of these stage kernels (the face-flux VJP of the matrix-core set stays)
+    l.adj_volume[0] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+      hipLaunchKernelGGL((sdg::nsAdjVolumeTensor<N, 1>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
+    };
+    l.adj_volume[1] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+      hipLaunchKernelGGL((sdg::nsAdjVolumeTensor<N, 2>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
+    };
+    l.adj_lift[0] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+      hipLaunchKernelGGL((sdg::nsAdjLiftTensor<N, 1>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
+    };
+    l.adj_lift[1] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+      hipLaunchKernelGGL((sdg::nsAdjLiftTensor<N, 2>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
+    };
+    l.adj_gather = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+      hipLaunchKernelGGL((sdg::nsAdjGatherTensor<N>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
+    };
+  }
   l.tensor = true;
 }
 template <int N>
@@ -476,6 +495,23 @@ double* noallocScratch(int device, std::string& msg) {
     g_scratch_ptr[device] = static_cast<double*>(ptr);
   }
   return g_scratch_ptr[device];
+}
+
+// state-shaped arrays between the reference layout [E][N_b][NV] and the device layout [E][NV][N_b]
+// (one thread per entry, writes coalesced)
+__global__ __launch_bounds__(256) void referenceToDeviceKernel(const double* ref, double* dev, int E, int nv, int nb) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= static_cast<int64_t>(E) * nv * nb) return;
+  const int64_t e = i / (nv * nb);
+  const int r = static_cast<int>(i % (nv * nb)), v = r / nb, b = r % nb;
+  dev[i] = ref[(e * nb + b) * nv + v];
+}
+__global__ __launch_bounds__(256) void deviceToReferenceKernel(const double* dev, double* ref, int E, int nv, int nb) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= static_cast<int64_t>(E) * nv * nb) return;
+  const int64_t e = i / (nv * nb);
+  const int r = static_cast<int>(i % (nv * nb)), b = r / nv, v = r % nv;
+  ref[i] = dev[(e * nv + v) * nb + b];
 }
 
 // out = min over v[0..n) in one workgroup (the minimum is exact, so the order does not matter).
@@ -1968,6 +2004,8 @@ int reverseSweep(sdg_ctx* ctx, int i0, int n, int s, int slot) {
   return reverseSweep(ctx, i0, m, s, slot);
 }
 
+int reverseRecorded(sdg_ctx* ctx);
+
 }  // namespace
 
 extern "C" {
@@ -2102,6 +2140,51 @@ int sdg_adjoint_backward(sdg_ctx* ctx, const double* lambda_final, double* lambd
   std::vector<double> dev;
   toDevice(ctx, lambda_final, dev);
   HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_lam, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  int rc = reverseRecorded(ctx);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(dev.data(), ctx->adj_lam, dev.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if ((rc = loadState(ctx, ctx->adj_stored ? ctx->adj_pool : ctx->snaps[0]))) return rc;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  toReference(ctx, dev, lambda_initial);
+  return SDG_OK;
+}
+
+int sdg_adjoint_backward_device(sdg_ctx* ctx, const double* lambda_final, double* lambda_initial) {
+  if (!ctx || !lambda_final || !lambda_initial)
+    return fail(ctx, SDG_E_INVALID, "sdg_adjoint_backward_device: null argument");
+  if (ctx->adj_nsteps < 0)
+    return fail(ctx, SDG_E_STATE, "sdg_adjoint_backward_device: no recorded sdg_adjoint_forward run");
+  hipPointerAttribute_t at[2];
+  const void* ptrs[2] = {lambda_final, lambda_initial};
+  for (int k = 0; k < 2; k++) {
+    if (hipPointerGetAttributes(&at[k], ptrs[k]) != hipSuccess || at[k].type != hipMemoryTypeDevice ||
+        at[k].device != ctx->device) {
+      (void)hipGetLastError();
+      return fail(ctx, SDG_E_INVALID, "sdg_adjoint_backward_device: the adjoints must be device memory of the "
+                                      "context's device");
+    }
+  }
+  const int64_t n = static_cast<int64_t>(stateCount(ctx));
+  const int grid = static_cast<int>((n + 255) / 256);
+  hipLaunchKernelGGL(referenceToDeviceKernel, dim3(grid), dim3(256), 0, ctx->stream, lambda_final, ctx->adj_lam,
+                     ctx->E, ctx->nv, ctx->nb);
+  HIP_TRY(ctx, hipGetLastError());
+  int rc = reverseRecorded(ctx);
+  if (rc) return rc;
+  hipLaunchKernelGGL(deviceToReferenceKernel, dim3(grid), dim3(256), 0, ctx->stream, ctx->adj_lam, lambda_initial,
+                     ctx->E, ctx->nv, ctx->nb);
+  HIP_TRY(ctx, hipGetLastError());
+  if ((rc = loadState(ctx, ctx->adj_stored ? ctx->adj_pool : ctx->snaps[0]))) return rc;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return SDG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// the reverse sweep of the recorded run, from the adjoint of the final state in adj_lam
+int reverseRecorded(sdg_ctx* ctx) {
   int rc;
   if (ctx->adj_stored) {
     const int S = ctx->stages();
@@ -2119,11 +2202,7 @@ int sdg_adjoint_backward(sdg_ctx* ctx, const double* lambda_final, double* lambd
   } else if ((rc = reverseSweep(ctx, 0, ctx->adj_nsteps, ctx->adj_nslots - 1, 0))) {
     return rc;
   }
-  HIP_TRY(ctx, hipMemcpyAsync(dev.data(), ctx->adj_lam, dev.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  if ((rc = loadState(ctx, ctx->adj_stored ? ctx->adj_pool : ctx->snaps[0]))) return rc;
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  toReference(ctx, dev, lambda_initial);
   return SDG_OK;
 }
 
-}  // extern "C"
+}  // namespace

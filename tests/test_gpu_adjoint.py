@@ -18,7 +18,19 @@ CASES = [
     ("shear-quad-p4-lf-br2", lambda: P.ns_shear(n=4, order=4, flux=L.FLUX_LAX_FRIEDRICHS, perturb=True)),
     ("tgv-hex-p2-hllc-br2", lambda: P.ns_tgv(n=3, order=2)),
     ("tgv-hex-p3-roe-br2", lambda: P.ns_tgv(n=2, order=3, flux=L.FLUX_ROE)),
+    ("tgv-hex-p3-hllc-br1", lambda: P.ns_tgv(n=2, order=3, visc=L.VISC_BR1)),
+    ("tgv-hex-p4-lf-br1", lambda: P.ns_tgv(n=2, order=4, flux=L.FLUX_LAX_FRIEDRICHS, visc=L.VISC_BR1)),
+    ("tgv-hex-p4-hllc-br2", lambda: P.ns_tgv(n=2, order=4)),
+    ("tgv-hex-p3-roe-br2-dense", lambda: _dense(P.ns_tgv(n=2, order=3, flux=L.FLUX_ROE))),
 ]
+
+
+def _dense(case):
+    """The same case on the dense matrix-core kernels (hexahedra default to the sum-factorised
+    stage kernels and their adjoint)."""
+    mesh, prob, prim = case
+    prob.kernels = L.KERNELS_DENSE
+    return mesh, prob, prim
 
 
 def _direction(rng, u0, dim):
@@ -111,6 +123,49 @@ def test_adjoint_steps_tangent_dot_product(snapshots):
     scale = float(np.abs(dun * w).sum() + np.abs(v * lam0).sum()) / 2
     print(f"  <dU v,w> {lhs:.15e}  <v,lam0> {rhs:.15e}  rel {abs(lhs - rhs) / abs(lhs):.2e}", flush=True)
     assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs, scale)
+
+
+@pytest.mark.parametrize("snapshots", [2, 4])
+def test_adjoint_steps_tangent_dot_product_hexahedra(snapshots):
+    """The same for the sum-factorised hexahedron kernels (Taylor-Green vortex 2^3 hex P3, HLLC +
+    BR2, 4 SSP-RK3 steps): binomial recomputation (2 snapshots) and the recorded run that keeps
+    every stage state (4 snapshots)."""
+    steps = 4
+    mesh, prob, prim = P.ns_tgv(n=2, order=3)
+    s = L.Solver(prob, mesh)
+    s.set_initial_primitive(prim)
+    dt = s.compute_dt(0.1)
+    rng = np.random.default_rng(13)
+    u0 = s.get_state()
+    u0 = u0 + 1e-3 * _direction(rng, u0, mesh.dim)
+    w = rng.standard_normal(u0.shape)
+    v = _direction(rng, u0, mesh.dim)
+    s.set_state(u0)
+    s.adjoint_forward(dt, steps, snapshots)
+    lam0 = s.adjoint_backward(w)
+    _, dun = O.Oracle(prob, mesh, tangent=True).step_tangent(u0, v, dt, steps)
+    lhs, rhs = float((dun * w).sum()), float((v * lam0).sum())
+    scale = float(np.abs(dun * w).sum() + np.abs(v * lam0).sum()) / 2
+    print(f"  <dU v,w> {lhs:.15e}  <v,lam0> {rhs:.15e}  rel {abs(lhs - rhs) / abs(lhs):.2e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs, scale)
+
+
+def test_hexahedron_adjoint_schedules_agree_bitwise():
+    """Sum-factorised kernels: the recorded run (every stage state and its traces kept) and
+    binomial recomputation from 1 or 3 snapshots give bit-identical adjoints."""
+    steps = 6
+    mesh, prob, prim = P.ns_tgv(n=2, order=3)
+    s = L.Solver(prob, mesh)
+    s.set_initial_primitive(prim)
+    dt = s.compute_dt(0.1)
+    u0 = s.get_state()
+    s.adjoint_forward(dt, steps, steps)
+    _, dJ = P.kinetic_energy(mesh, s.get_state())
+    ref = s.adjoint_backward(dJ)
+    for snapshots in (1, 3):
+        s.set_state(u0)
+        s.adjoint_forward(dt, steps, snapshots)
+        assert np.array_equal(s.adjoint_backward(dJ), ref), snapshots
 
 
 def _shear_rollout_setup(n, order, steps, cfl=0.1):
@@ -211,3 +266,24 @@ def test_torch_autograd_wrapper():
     s.adjoint_forward(dt, steps, steps)
     ref = s.adjoint_backward(dJ)
     assert np.abs(u0.grad.numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_device_resident_backward_matches_host():
+    """sdg_adjoint_backward_device (adjoints in device memory) gives the host entry point's
+    adjoint bit for bit, leaves the recorded initial state and refuses host pointers."""
+    steps = 6
+    mesh, prob, prim = P.ns_tgv(n=2, order=3)
+    s = L.Solver(prob, mesh)
+    s.set_initial_primitive(prim)
+    dt = s.compute_dt(0.1)
+    u0 = s.get_state()
+    s.adjoint_forward(dt, steps, 2)
+    _, dJ = P.kinetic_energy(mesh, s.get_state())
+    ref = s.adjoint_backward(dJ)
+    s.set_state(u0)
+    s.adjoint_forward(dt, steps, 2)
+    got = s.adjoint_backward_device(L.DeviceArray.from_numpy(dJ)).numpy()
+    assert np.array_equal(got, ref)
+    assert np.array_equal(s.get_state(), u0)
+    host = np.zeros(s.state_shape())
+    assert s.lib.sdg_adjoint_backward_device(s.handle, host.ctypes.data, host.ctypes.data) == L.E_INVALID
