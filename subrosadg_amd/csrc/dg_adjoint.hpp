@@ -26,6 +26,7 @@
 #include "dg_mfma.hpp"
 #include "dual.hpp"
 #include "physics.hpp"
+#include "physics_adjoint.hpp"
 
 namespace sdg {
 
@@ -156,18 +157,20 @@ __device__ __forceinline__ double volumeFluxVjpDir(const Phys& ph, const double*
   return 0.0;
 }
 
-// State part of one side of the face-flux VJP (dual numbers seeded on the NV components of side
-// SIDE): ub = (dt/du_SIDE)^T tb for t = H_c(ul, ur; n) - (F_v(ul, gl) + F_v(ur, gr)) . n / 2
+// One component of the state part of one side of the face-flux VJP (a dual number seeded on
+// component k of side SIDE): sum_v tb[v] d t_v / du_SIDE[k] for t = H_c(ul, ur; n) -
+// (F_v(ul, gl) + F_v(ur, gr)) . n / 2. One seed per evaluation keeps the flux's registers those of
+// a single-direction dual number (the NV-direction form spilled to scratch in three dimensions).
 template <int D, int SIDE>
-__device__ __forceinline__ void faceStateVjp(const Phys& ph, const double* n, const double* tb, const double* ul,
-                                             const double* ur, const double* gs, double* ub) {
+__device__ __forceinline__ double faceStateVjpDir(const Phys& ph, const double* n, const double* tb, const double* ul,
+                                                  const double* ur, const double* gs, int k) {
   constexpr int NV = D + 2, NG = NV * D;
-  using DT = Dual<NV>;
+  using DT = Dual<1>;
   DT nD[D], uL[NV], uR[NV], wL[D + 3], wR[D + 3], h[NV];
   for (int d = 0; d < D; d++) nD[d] = DT(n[d]);
-  for (int k = 0; k < NV; k++) {
-    uL[k] = SIDE == 0 ? seed<NV>(ul[k], k) : DT(ul[k]);
-    uR[k] = SIDE == 1 ? seed<NV>(ur[k], k) : DT(ur[k]);
+  for (int c = 0; c < NV; c++) {
+    uL[c] = (SIDE == 0 && c == k) ? seed<1>(ul[c], 0) : DT(ul[c]);
+    uR[c] = (SIDE == 1 && c == k) ? seed<1>(ur[c], 0) : DT(ur[c]);
   }
   computational<D>(ph, uL, wL);
   computational<D>(ph, uR, wR);
@@ -181,15 +184,13 @@ __device__ __forceinline__ void faceStateVjp(const Phys& ph, const double* n, co
     primitiveGradient<D>(ph, wR, uR, gD, pg);
     viscousRawFlux<D>(ph, wR, pg, fv);
   }
-  for (int k = 0; k < NV; k++) {
-    double s = 0.0;
-    for (int v = 0; v < NV; v++) {
-      double dv = 0.0;
-      for (int d = 0; d < D; d++) dv += fv[d + D * v].d[k] * n[d];
-      s += tb[v] * (h[v].d[k] - dv / 2.0);
-    }
-    ub[k] = s;
+  double s = 0.0;
+  for (int v = 0; v < NV; v++) {
+    double dv = 0.0;
+    for (int d = 0; d < D; d++) dv += fv[d + D * v].d[0] * n[d];
+    s += tb[v] * (h[v].d[0] - dv / 2.0);
   }
+  return s;
 }
 
 // ---- kernel 1: volume part ----------------------------------------------------------------------
@@ -255,13 +256,13 @@ __global__ __launch_bounds__(kThreads) void nsAdjVolumeMfma(Geo g, Phys ph, Stag
     tileStore<double>(acc, sFb, m0, lane);
   }
   __syncthreads();
-  // pointwise VJP of the volume flux, one thread per (part, element, point) -- parts 0..NV-1 the
-  // state components, part NV the whole gradient: u-bar_q into sUb (the free sRR area) and
+  // pointwise VJP of the volume flux, one thread per (part, element, point) -- part 0 the state
+  // (one reverse-mode pass), part 1 the gradient: u-bar_q into sUb (the free sRR area) and
   // grad-bar_q into sGb (behind grad u_q in sAJ)
   double* sUb = sRR;
   double* sGb = sAJ + NQm * 16;
-  for (int idx = tid; idx < (NV + 1) * EPB * NQ; idx += kThreads) {
-    const int dir = idx / (EPB * NQ), rem = idx % (EPB * NQ), le = rem / NQ, q = rem % NQ, e = e0 + le;
+  for (int idx = tid; idx < 2 * EPB * NQ; idx += kThreads) {
+    const int part = idx / (EPB * NQ), rem = idx % (EPB * NQ), le = rem / NQ, q = rem % NQ, e = e0 + le;
     if (e >= g.E) continue;
     double u[NV], gq[NG], G[D * D], fb[D * NV], Fb[D * NV];
     for (int v = 0; v < NV; v++) u[v] = sUq[q * 16 + le * NV + v];
@@ -271,11 +272,13 @@ __global__ __launch_bounds__(kThreads) void nsAdjVolumeMfma(Geo g, Phys ph, Stag
     for (int b = 0; b < D; b++)
       for (int v = 0; v < NV; v++) fb[b * NV + v] = sFb[(q * D + b) * 16 + le * NV + v];
     volumeFluxBar<D>(fb, G, Fb);
-    double gbv[NG];
-    const double x = volumeFluxVjpDir<D>(ph, u, gq, Fb, dir, gbv);
-    if (dir < NV) {
-      sUb[q * 16 + le * NV + dir] = x;
-    } else {
+    if (part == 0) {  // the state (reverse mode)
+      double ub[NV];
+      volumeFluxStateVjp<D>(ph, u, gq, Fb, ub);
+      for (int v = 0; v < NV; v++) sUb[q * 16 + le * NV + v] = ub[v];
+    } else {  // the gradient (linear)
+      double gbv[NG];
+      volumeFluxVjpDir<D>(ph, u, gq, Fb, NV, gbv);
       for (int r = 0; r < NG; r++) sGb[q * 16 + le * NG + r] = gbv[r];
     }
   }
@@ -311,12 +314,12 @@ __global__ __launch_bounds__(kThreads) void nsAdjVolumeMfma(Geo g, Phys ph, Stag
 template <int D, int NF, int NFQ>
 __global__ __launch_bounds__(kThreads) void nsAdjFaceKernel(Geo g, Phys ph, StageArgs a, AdjArgs j) {
   constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
-  // one thread per (part, side, face point): part 0 is the adjoint of that side's state (dual
-  // numbers seeded on its NV components), part 1 that of its gradient (the viscous flux is linear
-  // in the gradient); part and side are uniform over consecutive face points
+  // one thread per (part, side, face point): parts 0 .. NV-1 are the adjoints of one component of
+  // that side's state (a dual number seeded on it), part NV that of its gradient (the viscous flux
+  // is linear in the gradient); part and side are uniform over consecutive face points
   const int64_t npts = static_cast<int64_t>(g.F) * NFQ;
   const int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (idx >= npts * 4) return;
+  if (idx >= npts * 2 * (NV + 1)) return;
   const int part = static_cast<int>(idx / (2 * npts));
   const int side = static_cast<int>((idx / npts) & 1);
   const int64_t pt = idx % npts;
@@ -335,14 +338,11 @@ __global__ __launch_bounds__(kThreads) void nsAdjFaceKernel(Geo g, Phys ph, Stag
   const int64_t E = side == 0 ? L : R;
   const int sl = side == 0 ? slL : slR;
   for (int d = 0; d < D; d++) n[d] = g.fnormal[(f * NFQ + jq) * D + d];
-  if (part == 0) {
+  if (part < NV) {
     for (int r = 0; r < NG; r++) gs[r] = a.Gf[(E * NG + r) * NAQ + sl];
-    double ub[NV];
-    if (side == 0)
-      faceStateVjp<D, 0>(ph, n, tb, ul, ur, gs, ub);
-    else
-      faceStateVjp<D, 1>(ph, n, tb, ul, ur, gs, ub);
-    for (int v = 0; v < NV; v++) j.UFf[(E * NV + v) * NAQ + sl] = ub[v];
+    const double ub = side == 0 ? faceStateVjpDir<D, 0>(ph, n, tb, ul, ur, gs, part)
+                                : faceStateVjpDir<D, 1>(ph, n, tb, ul, ur, gs, part);
+    j.UFf[(E * NV + part) * NAQ + sl] = ub;
   } else {
     // -(1/2) sum_{v,a} tb[v] n_a dF_v[a][v]/dg of this side
     double Fb[D * NV], gb[NG];
@@ -350,6 +350,59 @@ __global__ __launch_bounds__(kThreads) void nsAdjFaceKernel(Geo g, Phys ph, Stag
       for (int d = 0; d < D; d++) Fb[d + D * v] = -0.5 * tb[v] * n[d];
     viscousGradientVjp<D>(ph, side == 0 ? ul : ur, Fb, gb);
     for (int r = 0; r < NG; r++) j.GFb[(E * NG + r) * NAQ + sl] = gb[r];
+  }
+}
+
+// ---- kernel 2, reverse mode (Central, Lax-Friedrichs, HLLC): one thread per face point ---------
+// The numerical flux and both sides' viscous fluxes are differentiated in one reverse pass
+// (physics_adjoint.hpp) instead of one dual-number pass per state component and side.
+template <int D, int NF, int NFQ>
+__global__ __launch_bounds__(kThreads) void nsAdjFaceRevKernel(Geo g, Phys ph, StageArgs a, AdjArgs j) {
+  constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
+  const int64_t pt = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (pt >= static_cast<int64_t>(g.F) * NFQ) return;
+  const int64_t f = pt / NFQ;
+  const int jq = static_cast<int>(pt % NFQ);
+  const int k = g.fseq[f * NFQ + jq];
+  const int64_t L = g.fpar[2 * f], R = g.fpar[2 * f + 1];
+  const int slL = g.floc[2 * f] * NFQ + jq, slR = g.floc[2 * f + 1] * NFQ + k;
+  double ul[NV], ur[NV], tb[NV], n[D];
+  const double jw = g.fjw[f * NFQ + jq];
+  for (int v = 0; v < NV; v++) {
+    ul[v] = a.Uf_in[(L * NV + v) * NAQ + slL];
+    ur[v] = a.Uf_in[(R * NV + v) * NAQ + slR];
+    tb[v] = (j.Hb[(R * NV + v) * NAQ + slR] - j.Hb[(L * NV + v) * NAQ + slL]) * jw;
+  }
+  for (int d = 0; d < D; d++) n[d] = g.fnormal[(f * NFQ + jq) * D + d];
+  double wl[D + 3], wr[D + 3], wbl[D + 3], wbr[D + 3], ubl[NV], ubr[NV];
+  computational<D>(ph, ul, wl);
+  computational<D>(ph, ur, wr);
+  for (int i = 0; i < D + 3; i++) wbl[i] = wbr[i] = 0.0;
+  for (int v = 0; v < NV; v++) ubl[v] = ubr[v] = 0.0;
+  convectiveFluxVjp<D>(ph, n, ul, wl, ur, wr, tb, wbl, wbr, ubl, ubr);
+  // t = H_c - (F_v(u_L, g_L) + F_v(u_R, g_R)) . n / 2: each side's F_v has the adjoint -tb n / 2
+  double Fvb[D * NV];
+  for (int v = 0; v < NV; v++)
+    for (int d = 0; d < D; d++) Fvb[d + D * v] = -0.5 * tb[v] * n[d];
+#pragma unroll 1
+  for (int sd = 0; sd < 2; sd++) {
+    const int64_t E = sd ? R : L;
+    const int sl = sd ? slR : slL;
+    double gs[NG], gb[NG], x[NV];
+    for (int r = 0; r < NG; r++) gs[r] = a.Gf[(E * NG + r) * NAQ + sl];
+    if (sd == 0) {
+      viscousStateVjp<D>(ph, ul, wl, gs, Fvb, wbl);
+      viscousGradientVjp<D>(ph, ul, Fvb, gb);
+      computationalVjp<D>(ph, ul, wl, wbl, x);
+      for (int v = 0; v < NV; v++) x[v] += ubl[v];
+    } else {
+      viscousStateVjp<D>(ph, ur, wr, gs, Fvb, wbr);
+      viscousGradientVjp<D>(ph, ur, Fvb, gb);
+      computationalVjp<D>(ph, ur, wr, wbr, x);
+      for (int v = 0; v < NV; v++) x[v] += ubr[v];
+    }
+    for (int r = 0; r < NG; r++) j.GFb[(E * NG + r) * NAQ + sl] = gb[r];
+    for (int v = 0; v < NV; v++) j.UFf[(E * NV + v) * NAQ + sl] = x[v];
   }
 }
 

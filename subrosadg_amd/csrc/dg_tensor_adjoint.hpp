@@ -17,7 +17,8 @@
 //   nsAdjVolumeTensor  recompute Y, alpha, beta, g_tot, g_face and gs (-> Gf); mu = the adjoint of
 //                      R at the nodes; H-bar = lf-traces of mu (-> Hb); the pointwise volume-flux
 //                      VJP of W-bar_a = Dn_a mu -> u-bar (-> Ua) and g_tot-bar (-> RtV)
-//   nsAdjFaceKernel    (dg_adjoint.hpp, unchanged) face-flux VJP -> UFf, GFb
+//   nsAdjFaceRevKernel (dg_adjoint.hpp, or nsAdjFaceKernel for the Roe and Exact fluxes) face-flux
+//                      VJP -> UFf, GFb
 //   nsAdjLiftTensor    gs-bar -> g_face-bar (lf transposes) and beta-bar (own-face lift); the
 //                      transpose of the gradient: Y-bar += Dn^T J^-1 (g_tot-bar + g_face-bar)
 //                      (-> Ua), alpha-bar + beta-bar per slot -> own and neighbour trace adjoints
@@ -55,16 +56,19 @@ __device__ __forceinline__ void linePassT(const double* M, int axis, const doubl
 }
 
 // ---- kernel 1: forward recomputation, mu, H-bar, volume-flux VJP --------------------------------
-// LDS per element, two regions as in nsGradTensor: sW [NG NPT] = Y + alpha, then the face-gradient
-// field, then mu (+ pass scratch); sB [NV NAQ] = the basis-pass panel, then beta, then lam
+// LDS per element: sW [NG NPT] = Y + alpha (+ pass scratch), then the face-gradient field;
+// sB [NV NAQ] = the basis-pass panel, then beta; sM [NV NPT] = lam, then mu. The volume-flux VJP
+// follows the gradient directly, so that the node's state and total gradient die before the
+// face-gradient phase.
 template <int N, int VISC>
-__global__ __launch_bounds__(TensorShape<N>::TPB) void nsAdjVolumeTensor(Geo g, Phys ph, StageArgs a, AdjArgs j) {
+__global__ __launch_bounds__(TensorShape<N>::TPB, 2) void nsAdjVolumeTensor(Geo g, Phys ph, StageArgs a, AdjArgs j) {
   const TensorOps& t = g.tens;
   using S = TensorShape<N>;
   constexpr int NPT = S::NPT, EPB = S::EPB, NV = S::NV, NG = S::NG, NFQ = S::NFQ, NAQ = S::NAQ, D = 3;
   static_assert(NV * NPT + NV * NAQ <= NG * NPT && NV * NPT <= NV * NAQ, "LDS regions do not nest");
   __shared__ double sW[EPB][NG * NPT];
   __shared__ double sB[EPB][NV * NAQ];
+  __shared__ double sM[EPB][NV * NPT];
   __shared__ TensorLds<N> sT;
   stageTables<N>(t, sT);
   const int tid = threadIdx.x, le = localElement<NPT>(tid), q = tid % NPT;
@@ -75,12 +79,9 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsAdjVolumeTensor(Geo g, 
   double* Y = W;
   double* Al = W + NV * NPT;
   double* Bt = sB[lec];
-  double dj = 1.0, gi[D * D];
-  for (int i = 0; i < D * D; i++) gi[i] = 0.0;
-  if (ok) {
-    dj = g.detj[e];
-    for (int i = 0; i < D * D; i++) gi[i] = g.ginv[e * D * D + i];
-  }
+  double* Mu = sM[lec];
+  double dj = 1.0;
+  if (ok) dj = g.detj[e];
   constexpr int ITEMS = (NV * NAQ + NPT - 1) / NPT;
   double fo[ITEMS], fn[ITEMS], fw[ITEMS];
   if (ok) {
@@ -99,12 +100,35 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsAdjVolumeTensor(Geo g, 
     }
   }
   loadModal<double, N>(t, a, a.U, e, Bt, q, ok);
+  if (ok) {
+    const int tq = t.perm[q];
+    for (int v = 0; v < NV; v++) Mu[v * NPT + tq] = j.lam[(e * NV + v) * NPT + q];
+  }
   __syncthreads();
   linePass<double, N, NV>(sT.B, 0, Bt, Y, q, ok);
   elemSync<NPT>();
   linePass<double, N, NV>(sT.B, 1, Y, Bt, q, ok);
   elemSync<NPT>();
   linePass<double, N, NV>(sT.B, 2, Bt, Y, q, ok);
+  elemSync<NPT>();
+  // mu: the adjoint of the nodal residual R (Bt and the alpha region as pass scratch)
+  if (j.apply_minv) {  // increment = (B^-1)^3 (R / (w |J|)): mu = ((B^-1)^T)^3 lam / (w |J|)
+    linePassT<N, NV>(sT.Binv, 0, Mu, Bt, q, ok);
+    elemSync<NPT>();
+    linePassT<N, NV>(sT.Binv, 1, Bt, Al, q, ok);
+    elemSync<NPT>();
+    linePassT<N, NV>(sT.Binv, 2, Al, Mu, q, ok);
+    if (ok) {
+      const double ijw = 1.0 / (sT.wn[q] * dj);
+      for (int v = 0; v < NV; v++) Mu[v * NPT + q] *= ijw;
+    }
+  } else {  // reference residual (B^T)^3 R: mu = B^3 lam
+    linePass<double, N, NV>(sT.B, 0, Mu, Bt, q, ok);
+    elemSync<NPT>();
+    linePass<double, N, NV>(sT.B, 1, Bt, Al, q, ok);
+    elemSync<NPT>();
+    linePass<double, N, NV>(sT.B, 2, Al, Mu, q, ok);
+  }
   elemSync<NPT>();
   if (ok) {
 #pragma unroll
@@ -118,27 +142,30 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsAdjVolumeTensor(Geo g, 
   }
   elemSync<NPT>();
   // gradients at the node (the forward kernel's arithmetic)
-  double u[NV], gtot[NG], gface[VISC == 2 ? NG : 1];
+  double gi[D * D], u[NV], gtot[NG], gface[NG];
+  for (int i = 0; i < D * D; i++) gi[i] = ok ? g.ginv[e * D * D + i] : 0.0;
   for (int v = 0; v < NV; v++) u[v] = 0.0;
-  for (int r = 0; r < NG; r++) gtot[r] = 0.0;
+  for (int r = 0; r < NG; r++) gtot[r] = gface[r] = 0.0;
   if (ok) {
-    double dr[NV][D];
-    for (int ax = 0; ax < D; ax++) {
-      const int i = nodeIndex(q, ax, N), st = axisStride(ax, N), q0 = q - i * st;
-      for (int v = 0; v < NV; v++) {
-        double s = 0.0;
+    {
+      double dr[NV][D];
+      for (int ax = 0; ax < D; ax++) {
+        const int i = nodeIndex(q, ax, N), st = axisStride(ax, N), q0 = q - i * st;
+        for (int v = 0; v < NV; v++) {
+          double s = 0.0;
 #pragma unroll
-        for (int m = 0; m < N; m++) s += sT.Dn[i * N + m] * Y[v * NPT + q0 + m * st];
-        dr[v][ax] = s;
+          for (int m = 0; m < N; m++) s += sT.Dn[i * N + m] * Y[v * NPT + q0 + m * st];
+          dr[v][ax] = s;
+        }
       }
-    }
-    for (int v = 0; v < NV; v++) {
-      u[v] = Y[v * NPT + q];
-      for (int d = 0; d < D; d++) {
-        double s = 0.0;
-        for (int ax = 0; ax < D; ax++) s += gi[d + D * ax] * dr[v][ax];
-        gtot[v * D + d] = s;
-        if constexpr (VISC == 2) gface[v * D + d] = s;
+      for (int v = 0; v < NV; v++) {
+        u[v] = Y[v * NPT + q];
+        for (int d = 0; d < D; d++) {
+          double s = 0.0;
+          for (int ax = 0; ax < D; ax++) s += gi[d + D * ax] * dr[v][ax];
+          gtot[v * D + d] = s;
+          gface[v * D + d] = s;
+        }
       }
     }
     const double inw = 1.0 / (sT.wn[q] * dj);
@@ -153,15 +180,48 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsAdjVolumeTensor(Geo g, 
         const double al = Al[v * NAQ + sl] * coef, be = Bt[v * NAQ + sl] * coef;
         for (int d = 0; d < D; d++) {
           gtot[v * D + d] += (al + be) * n[d];
-          if constexpr (VISC == 2) gface[v * D + d] += al * n[d];
+          gface[v * D + d] += (VISC == 2 ? al : al + be) * n[d];
         }
       }
     }
   }
-  elemSync<NPT>();  // Y and alpha are read
+  elemSync<NPT>();  // Y and alpha are read: sW takes the face-gradient field (BR2: without the jump
+                    // lifts; BR1: the total gradient)
   if (ok)
-    for (int r = 0; r < NG; r++) W[r * NPT + q] = VISC == 2 ? gface[VISC == 2 ? r : 0] : gtot[r];
-  elemSync<NPT>();
+    for (int r = 0; r < NG; r++) W[r * NPT + q] = gface[r];
+  // volume flux: W-bar_a = Dn_a mu (transpose of Rvol = sum_a Dn_a^T W_a), then the pointwise VJP
+  if (ok) {
+    double fbar[D * NV];
+    for (int ax = 0; ax < D; ax++) {
+      const int i = nodeIndex(q, ax, N), st = axisStride(ax, N), q0 = q - i * st;
+      for (int v = 0; v < NV; v++) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; k++) s += sT.Dn[i * N + k] * Mu[v * NPT + q0 + k * st];
+        fbar[ax * NV + v] = s;
+      }
+    }
+    const double jw = dj * sT.wn[q];
+    double G[D * D], Fb[D * NV], gb[NG], ub[NV];
+    for (int i = 0; i < D * D; i++) G[i] = gi[i] * jw;
+    volumeFluxBar<D>(fbar, G, Fb);
+    volumeFluxStateVjp<D>(ph, u, gtot, Fb, ub);
+    for (int v = 0; v < NV; v++) j.Ua[(e * NV + v) * NPT + q] = ub[v];
+    volumeFluxVjpDir<D>(ph, u, gtot, Fb, NV, gb);
+    for (int r = 0; r < NG; r++) j.RtV[(e * NG + r) * NPT + q] = gb[r];
+  }
+  // H-bar: R += lf_s(i) H_s at the nodes of the slot's line
+  if (ok)
+    for (int idx = q; idx < NV * NAQ; idx += NPT) {
+      const int sl = idx % NAQ, v = idx / NAQ, s = sl / NFQ;
+      const int b = sT.base[sl], st = axisStride(sT.axis[s], N);
+      const double* l = sT.lf + sT.side[s] * N;
+      double acc = 0.0;
+#pragma unroll
+      for (int i = 0; i < N; i++) acc += l[i] * Mu[v * NPT + b + i * st];
+      j.Hb[(e * NV + v) * NAQ + sl] = acc;
+    }
+  elemSync<NPT>();  // the face-gradient field is complete
   // gs per own slot: the face-gradient trace plus (BR2) the own face's lift -> Gf (FP64), the
   // gradient the face-flux VJP differentiates the viscous flux at
   if (ok) {
@@ -181,66 +241,6 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsAdjVolumeTensor(Geo g, 
         a.Gf[(e * NG + r) * NAQ + sl] = acc;
       }
     }
-  }
-  elemSync<NPT>();  // the face-gradient field and beta are read
-  // mu: the adjoint of the nodal residual R
-  double* Mu = W;
-  double* Ps = W + NV * NPT;  // pass scratch behind mu (NV NPT <= NG NPT - NV NPT)
-  if (ok) {
-    const int tq = t.perm[q];
-    for (int v = 0; v < NV; v++) Bt[v * NPT + tq] = j.lam[(e * NV + v) * NPT + q];
-  }
-  elemSync<NPT>();
-  if (j.apply_minv) {  // increment = (B^-1)^3 (R / (w |J|)): mu = ((B^-1)^T)^3 lam / (w |J|)
-    linePassT<N, NV>(sT.Binv, 0, Bt, Mu, q, ok);
-    elemSync<NPT>();
-    linePassT<N, NV>(sT.Binv, 1, Mu, Ps, q, ok);
-    elemSync<NPT>();
-    linePassT<N, NV>(sT.Binv, 2, Ps, Mu, q, ok);
-    if (ok) {
-      const double ijw = 1.0 / (sT.wn[q] * dj);
-      for (int v = 0; v < NV; v++) Mu[v * NPT + q] *= ijw;
-    }
-  } else {  // reference residual (B^T)^3 R: mu = B^3 lam
-    linePass<double, N, NV>(sT.B, 0, Bt, Mu, q, ok);
-    elemSync<NPT>();
-    linePass<double, N, NV>(sT.B, 1, Mu, Ps, q, ok);
-    elemSync<NPT>();
-    linePass<double, N, NV>(sT.B, 2, Ps, Mu, q, ok);
-  }
-  elemSync<NPT>();
-  // H-bar: R += lf_s(i) H_s at the nodes of the slot's line
-  if (ok)
-    for (int idx = q; idx < NV * NAQ; idx += NPT) {
-      const int sl = idx % NAQ, v = idx / NAQ, s = sl / NFQ;
-      const int b = sT.base[sl], st = axisStride(sT.axis[s], N);
-      const double* l = sT.lf + sT.side[s] * N;
-      double acc = 0.0;
-#pragma unroll
-      for (int i = 0; i < N; i++) acc += l[i] * Mu[v * NPT + b + i * st];
-      j.Hb[(e * NV + v) * NAQ + sl] = acc;
-    }
-  // volume flux: W-bar_a = Dn_a mu (transpose of Rvol = sum_a Dn_a^T W_a), then the pointwise VJP
-  if (ok) {
-    double fbar[D * NV];
-    for (int ax = 0; ax < D; ax++) {
-      const int i = nodeIndex(q, ax, N), st = axisStride(ax, N), q0 = q - i * st;
-      for (int v = 0; v < NV; v++) {
-        double s = 0.0;
-#pragma unroll
-        for (int k = 0; k < N; k++) s += sT.Dn[i * N + k] * Mu[v * NPT + q0 + k * st];
-        fbar[ax * NV + v] = s;
-      }
-    }
-    const double jw = dj * sT.wn[q];
-    double G[D * D], Fb[D * NV], gb[NG];
-    for (int i = 0; i < D * D; i++) G[i] = gi[i] * jw;
-    volumeFluxBar<D>(fbar, G, Fb);
-#pragma unroll 1
-    for (int dir = 0; dir < NV; dir++)
-      j.Ua[(e * NV + dir) * NPT + q] = volumeFluxVjpDir<D>(ph, u, gtot, Fb, dir, gb);
-    volumeFluxVjpDir<D>(ph, u, gtot, Fb, NV, gb);
-    for (int r = 0; r < NG; r++) j.RtV[(e * NG + r) * NPT + q] = gb[r];
   }
 }
 

@@ -126,9 +126,13 @@ Launchers makeLaunchers() {
                            dim3(sdg::kThreads), 0, st, g, ph, a, j);
       };
       l.adj_face = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
-        hipLaunchKernelGGL((sdg::nsAdjFaceKernel<D, NF, NFQ>),
-                           dim3(blocksFor(g.F * NFQ * 4, sdg::kThreads)),
-                           dim3(sdg::kThreads), 0, st, g, ph, a, j);
+        if (ph.flux <= SDG_FLUX_HLLC)  // reverse mode, one thread per face point
+          hipLaunchKernelGGL((sdg::nsAdjFaceRevKernel<D, NF, NFQ>), dim3(blocksFor(g.F * NFQ, sdg::kThreads)),
+                             dim3(sdg::kThreads), 0, st, g, ph, a, j);
+        else  // dual numbers, one thread per (state component or gradient, side, face point)
+          hipLaunchKernelGGL((sdg::nsAdjFaceKernel<D, NF, NFQ>),
+                             dim3(blocksFor(g.F * NFQ * 2 * (D + 3), sdg::kThreads)),
+                             dim3(sdg::kThreads), 0, st, g, ph, a, j);
       };
       l.adj_lift[0] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
         hipLaunchKernelGGL((sdg::nsAdjLiftMfma<D, NB, NQ, NF, NFQ, 1>), dim3(blocksFor(g.E, EPB_G)),
