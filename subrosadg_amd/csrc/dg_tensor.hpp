@@ -17,9 +17,11 @@
 //   residual R_psi = sum_axes Dn^T (w |J| J^-1 F) + sum_faces l_s(i) Hs, the increment R_psi/(w|J|)
 //   taken back to modal coefficients with B^-1 along each axis; the reference residual (before
 //   M^-1) is (B^T x B^T x B^T) R_psi.
-// Each thread owns one node of one element; pointwise physics is FP64 (pointState / volumeFlux of
-// the matrix-core kernels, so the mixed-precision mode works the same way: contractions and work
-// buffers in TC on deviations from the reference state, FP64 state and RK accumulation).
+// Each thread owns one node of one element. FP64 mode: FP64 throughout. Mixed-precision mode:
+// contractions and work buffers in TC = FP32 on deviations from the reference state, the
+// conversion to computational variables (the low-Mach pressure recovery) and the flux deviations
+// p - pref, rho H - h0 rho in FP64, the primitive gradient, viscous and volume fluxes in FP32
+// (volumeFlux32, normalViscous32), FP64 state and RK accumulation.
 #ifndef SDG_DG_TENSOR_HPP_
 #define SDG_DG_TENSOR_HPP_
 
@@ -179,6 +181,69 @@ __device__ __forceinline__ int localElement(int tid) {
   int le = tid / NPT;
   if constexpr (NPT == 64) le = __builtin_amdgcn_readfirstlane(le);
   return le;
+}
+
+// Mixed-precision mode, pointwise physics in FP32 after the FP64 pressure recovery: w =
+// computational(u) stays FP64 (p = (gamma - 1)(rho E - rho |v|^2 / 2) cancels two digits at
+// Ma 0.1), so do the deviations p - pref and rho H - h0 rho of the momentum and energy fluxes;
+// the primitive gradient, the viscous flux and the contraction with (J^T)^-1 |J| w are FP32.
+// out[b NV + v] = sum_a (F_c - F_v)[a][v] G[a][b] as volumeFlux.
+template <int D>
+__device__ __forceinline__ void volumeFlux32(const Phys& ph, const StageArgs& a, const double* u, const double* w,
+                                             const double* g, const double* G, float* out) {
+  constexpr int NV = D + 2, NG = NV * D;
+  float wf[D + 3], uf[NV], gf[NG], pg[NG], fv[D * NV], fc[D * NV];
+#pragma unroll
+  for (int i = 0; i < D + 2; i++) wf[i] = static_cast<float>(w[i]);
+  wf[D + 2] = 0.0f;  // the viscous flux does not read p
+#pragma unroll
+  for (int v = 0; v < NV; v++) uf[v] = static_cast<float>(u[v]);
+#pragma unroll
+  for (int r = 0; r < NG; r++) gf[r] = static_cast<float>(g[r]);
+  primitiveGradient<D>(ph, wf, uf, gf, pg);
+  viscousRawFlux<D>(ph, wf, pg, fv);
+  const double rho = w[0], et = w[D + 1] + velocitySquared<D>(w) / 2.0;
+  const float pdev = static_cast<float>(w[D + 2] - a.mix.pref);
+  const float hdev = static_cast<float>(rho * et + w[D + 2] - a.mix.h0 * rho);
+  const float rf = wf[0];
+#pragma unroll
+  for (int aa = 0; aa < D; aa++) {
+    fc[aa] = rf * wf[1 + aa];
+#pragma unroll
+    for (int b = 0; b < D; b++) fc[aa + D * (1 + b)] = rf * wf[1 + aa] * wf[1 + b] + (aa == b ? pdev : 0.0f);
+    fc[aa + D * (D + 1)] = hdev * wf[1 + aa];
+  }
+#pragma unroll
+  for (int b = 0; b < D; b++)
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      float s = 0.0f;
+#pragma unroll
+      for (int aa = 0; aa < D; aa++) s += (fc[aa + D * v] - fv[aa + D * v]) * static_cast<float>(G[aa + D * b]);
+      out[b * NV + v] = s;
+    }
+}
+
+// Mixed-precision mode: the normal viscous flux F_v(u, g) . n in FP32 (w = computational(u) FP64)
+template <int D>
+__device__ __forceinline__ void normalViscous32(const Phys& ph, const double* u, const double* w, const float* g,
+                                                const double* n, float* out) {
+  constexpr int NV = D + 2, NG = NV * D;
+  float wf[D + 3], uf[NV], pg[NG], fv[D * NV];
+#pragma unroll
+  for (int i = 0; i < D + 2; i++) wf[i] = static_cast<float>(w[i]);
+  wf[D + 2] = 0.0f;
+#pragma unroll
+  for (int v = 0; v < NV; v++) uf[v] = static_cast<float>(u[v]);
+  primitiveGradient<D>(ph, wf, uf, g, pg);
+  viscousRawFlux<D>(ph, wf, pg, fv);
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    float s = 0.0f;
+#pragma unroll
+    for (int d = 0; d < D; d++) s += fv[d + D * v] * static_cast<float>(n[d]);
+    out[v] = s;
+  }
 }
 
 // ---- kernel 1: lifted gradient, volume flux, gradient traces ------------------------------------
@@ -392,6 +457,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
       for (int d = 0; d < D; d++) n[d] = nk[k][d];
       for (int v = 0; v < NV; v++) us[v] = pointState<TC>(a, uk[k][v], v);
       const double kf = sT.kappa[sT.side[s]] * idj / sT.fwt[sl];  // BR2 own-face lift factor of the slot
+      TC gt[NG];
       for (int r = 0; r < NG; r++) {
         TC acc = TC(0);
 #pragma unroll
@@ -399,19 +465,26 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
         double gv = static_cast<double>(acc);
         if constexpr (VISC == 2)  // r_s at its own face: kappa_side beta n_d / |J|
           gv += kf * static_cast<double>(Bt[(r / D) * NAQ + sl]) * n[r % D];
-        const TC gt = static_cast<TC>(gv);
-        gs[r] = static_cast<double>(gt);
-        if (nbs < 0) Gf[(e * NG + r) * NAQ + sl] = gt;
+        gt[r] = static_cast<TC>(gv);
+        gs[r] = static_cast<double>(gt[r]);
+        if (nbs < 0) Gf[(e * NG + r) * NAQ + sl] = gt[r];
       }
-      double ws[D + 3], pg[NG], fv[D * NV];
+      double ws[D + 3];
       computational<D>(ph, us, ws);
-      primitiveGradient<D>(ph, ws, us, gs, pg);
-      viscousRawFlux<D>(ph, ws, pg, fv);
       TC* Vn = reinterpret_cast<TC*>(a.Vn);  // TC values in the FP64-sized buffer
-      for (int v = 0; v < NV; v++) {
-        double sum = 0.0;
-        for (int d = 0; d < D; d++) sum += fv[d + D * v] * n[d];
-        Vn[(e * NV + v) * NAQ + sl] = static_cast<TC>(sum);
+      if constexpr (sizeof(TC) == 4) {
+        float vn[NV];
+        normalViscous32<D>(ph, us, ws, gt, n, vn);
+        for (int v = 0; v < NV; v++) Vn[(e * NV + v) * NAQ + sl] = vn[v];
+      } else {
+        double pg[NG], fv[D * NV];
+        primitiveGradient<D>(ph, ws, us, gs, pg);
+        viscousRawFlux<D>(ph, ws, pg, fv);
+        for (int v = 0; v < NV; v++) {
+          double sum = 0.0;
+          for (int d = 0; d < D; d++) sum += fv[d + D * v] * n[d];
+          Vn[(e * NV + v) * NAQ + sl] = static_cast<TC>(sum);
+        }
       }
     }
   }
@@ -421,16 +494,23 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
   // from there with the face lift
   double src = 0.0;  // Boussinesq source at the node times w |J| (the test function is the node's)
   if (ok) {
-    double w[D + 3], pg[NG], fv[D * NV];
+    double w[D + 3];
     computational<D>(ph, u, w);
-    primitiveGradient<D>(ph, w, u, wtot, pg);
-    viscousRawFlux<D>(ph, w, pg, fv);
     const double jw = dj * sT.wn[q];
     if (ph.source == kSourceBoussinesq) src = boussinesqSource<D>(ph, w) * jw;
-    double G[D * D], out[D * NV];
+    double G[D * D];
     for (int i = 0; i < D * D; i++) G[i] = gi[i] * jw;
-    volumeFlux<TC, D, true>(a, w, fv, G, out);
-    for (int i = 0; i < D * NV; i++) W[i * NPT + q] = static_cast<TC>(out[i]);
+    if constexpr (sizeof(TC) == 4) {
+      float out[D * NV];
+      volumeFlux32<D>(ph, a, u, w, wtot, G, out);
+      for (int i = 0; i < D * NV; i++) W[i * NPT + q] = out[i];
+    } else {
+      double pg[NG], fv[D * NV], out[D * NV];
+      primitiveGradient<D>(ph, w, u, wtot, pg);
+      viscousRawFlux<D>(ph, w, pg, fv);
+      volumeFlux<TC, D, true>(a, w, fv, G, out);
+      for (int i = 0; i < D * NV; i++) W[i * NPT + q] = static_cast<TC>(out[i]);
+    }
   }
   elemSync<NPT>();
   if (ok) {
