@@ -441,10 +441,11 @@ int sdg_write_view(const sdg_mesh* mesh, const sdg_problem_desc* problem, const 
 int sdg_write_vtu(const sdg_mesh* mesh, const sdg_problem_desc* problem, const double* coefficients,
                   const double* node_values, int step, double time, const char* path);
 
-/* ---- discrete adjoint (Navier-Stokes, FP64, matrix-core element sizes) -----------------------
+/* ---- discrete adjoint (Navier-Stokes, FP64, periodic meshes: quadrangles P3-P5, hexahedra P2-P4)
  * The reference has no adjoint; these entry points transpose its residual (R of sdg_residual,
- * SpatialDiscrete.cpp:194-1116) and its RK stage update (TimeIntegration.cpp:181-198).
- * All arrays are in the reference layout (E x N_b x NV, NV fastest). */
+ * SpatialDiscrete.cpp:194-1116) and its RK stage update (TimeIntegration.cpp:181-198), each as the
+ * transpose of the kernel family that runs the forward stages (sum-factorised on hexahedra by
+ * default). All arrays are in the reference layout (E x N_b x NV, NV fastest). */
 
 /* out = (dR/dU)^T lambda at the current state. */
 int sdg_vjp_residual(sdg_ctx* ctx, const double* lambda, double* out);

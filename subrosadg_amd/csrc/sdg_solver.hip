@@ -60,7 +60,7 @@ struct Launchers {
   AvFn av_sensor = nullptr;
   // Navier-Stokes BR2 raw files: per boundary face the parent's volume gradient plus that face's lift
   BlockFn br2_blocks = nullptr;
-  // discrete adjoint (Navier-Stokes, FP64 matrix-core sizes only)
+  // discrete adjoint (Navier-Stokes, FP64: the dense matrix-core set, or the sum-factorised set on hexahedra)
   AdjFn adj_volume[2] = {nullptr, nullptr};  // BR1, BR2
   AdjFn adj_face = nullptr;
   AdjFn adj_lift[2] = {nullptr, nullptr};
