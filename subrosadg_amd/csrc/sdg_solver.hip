@@ -2010,6 +2010,10 @@ int reverseSweep(sdg_ctx* ctx, int i0, int n, int s, int slot) {
 
 int reverseRecorded(sdg_ctx* ctx);
 
+// state size (doubles) from which the stored reverse sweep launches its kernels on the recording
+// pools directly instead of replaying a graph over copies of each step's states
+constexpr size_t kDirectAdjointStates = size_t(1) << 22;
+
 }  // namespace
 
 extern "C" {
@@ -2193,8 +2197,19 @@ int reverseRecorded(sdg_ctx* ctx) {
   if (ctx->adj_stored) {
     const int S = ctx->stages();
     const size_t n = stateCount(ctx);
+    const size_t nt = static_cast<size_t>(ctx->E) * ctx->nv * ctx->naq;
+    if (n >= kDirectAdjointStates) {
+      // large meshes: the stage kernels read the recorded states and traces in the pools directly
+      // (no per-step copies into the graph's fixed buffers; a launch is short against the kernels)
+      for (int i = ctx->adj_nsteps - 1; i >= 0; i--)
+        for (int st = S - 1; st >= 0; st--) {
+          const size_t slot = static_cast<size_t>(i) * S + st;
+          if ((rc = vjpAt(ctx, ctx->adj_pool + slot * n, ctx->adj_lam, 1, nullptr, st, ctx->adj_tpool + slot * nt)))
+            return rc;
+        }
+      return SDG_OK;
+    }
     for (int i = ctx->adj_nsteps - 1; i >= 0; i--) {
-      const size_t nt = static_cast<size_t>(ctx->E) * ctx->nv * ctx->naq;
       HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_S[0], ctx->adj_pool + static_cast<size_t>(i) * S * n, S * n * sizeof(double),
                                   hipMemcpyDeviceToDevice, ctx->stream));
       HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_T, ctx->adj_tpool + static_cast<size_t>(i) * S * nt, S * nt * sizeof(double),
