@@ -190,17 +190,15 @@ __device__ __forceinline__ int localElement(int tid) {
 // out[b NV + v] = sum_a (F_c - F_v)[a][v] G[a][b] as volumeFlux.
 template <int D>
 __device__ __forceinline__ void volumeFlux32(const Phys& ph, const StageArgs& a, const double* u, const double* w,
-                                             const double* g, const double* G, float* out) {
+                                             const float* g, const double* G, float* out) {
   constexpr int NV = D + 2, NG = NV * D;
-  float wf[D + 3], uf[NV], gf[NG], pg[NG], fv[D * NV], fc[D * NV];
+  float wf[D + 3], uf[NV], pg[NG], fv[D * NV], fc[D * NV];
 #pragma unroll
   for (int i = 0; i < D + 2; i++) wf[i] = static_cast<float>(w[i]);
   wf[D + 2] = 0.0f;  // the viscous flux does not read p
 #pragma unroll
   for (int v = 0; v < NV; v++) uf[v] = static_cast<float>(u[v]);
-#pragma unroll
-  for (int r = 0; r < NG; r++) gf[r] = static_cast<float>(g[r]);
-  primitiveGradient<D>(ph, wf, uf, gf, pg);
+  primitiveGradient<D>(ph, wf, uf, g, pg);
   viscousRawFlux<D>(ph, wf, pg, fv);
   const double rho = w[0], et = w[D + 1] + velocitySquared<D>(w) / 2.0;
   const float pdev = static_cast<float>(w[D + 2] - a.mix.pref);
@@ -343,24 +341,28 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
       }
   }
   elemSync<NPT>();
-  // gradient at the node: J^-T (Dn u along each axis) + the face corrections
-  double u[NV], wtot[NG], wface[VISC == 2 ? NG : 1];
+  // gradient at the node: J^-T (Dn u along each axis) + the face corrections, assembled in the
+  // contraction type (the mixed mode's lifted gradient is an FP32 quantity throughout)
+  double u[NV];
+  TC wtot[NG], wface[VISC == 2 ? NG : 1];
   if (ok) {
-    double dr[NV][D];
+    TC dr[NV][D];
     for (int ax = 0; ax < D; ax++) {
       const int i = nodeIndex(q, ax, N), st = axisStride(ax, N), q0 = q - i * st;
       for (int v = 0; v < NV; v++) {
         TC s = TC(0);
 #pragma unroll
         for (int m = 0; m < N; m++) s += static_cast<TC>(sT.Dn[i * N + m]) * Y[v * NPT + q0 + m * st];
-        dr[v][ax] = static_cast<double>(s);
+        dr[v][ax] = s;
       }
     }
+    TC git[D * D];
+    for (int i = 0; i < D * D; i++) git[i] = static_cast<TC>(gi[i]);
     for (int v = 0; v < NV; v++) {
       u[v] = pointState<TC>(a, Y[v * NPT + q], v);
       for (int d = 0; d < D; d++) {
-        double s = 0.0;
-        for (int ax = 0; ax < D; ax++) s += gi[d + D * ax] * dr[v][ax];
+        TC s = TC(0);
+        for (int ax = 0; ax < D; ax++) s += git[d + D * ax] * dr[v][ax];
         wtot[v * D + d] = s;
         if constexpr (VISC == 2) wface[v * D + d] = s;
       }
@@ -376,17 +378,17 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
     for (int s = 0; s < 6; s++) {
       const int ax = t.axis[s], i = nodeIndex(q, ax, N);
       const int sl = s * NFQ + sT.slot_of[s * NPT + q];
-      double n[D];
-      for (int d = 0; d < D; d++) n[d] = nx[d];
+      TC n[D];
+      for (int d = 0; d < D; d++) n[d] = static_cast<TC>(nx[d]);
       if (s < 5) {
         const int64_t es = e * NAQ + (s + 1) * NFQ + sT.slot_of[(s + 1) * NPT + q];
         for (int d = 0; d < D; d++) nx[d] = g.snrm[es * D + d];
       }
       // alpha, beta carry |J_f| w_t1 w_t2; M^-1 = 1 / (w_i w_t1 w_t2 |J|) at the node
-      const double coef = sT.lf[t.side[s] * N + i] * inw;
+      const TC coef = static_cast<TC>(sT.lf[t.side[s] * N + i] * inw);
       for (int v = 0; v < NV; v++) {
-        const double al = static_cast<double>(Al[v * NAQ + sl]) * coef;
-        const double be = static_cast<double>(Bt[v * NAQ + sl]) * coef;
+        const TC al = Al[v * NAQ + sl] * coef;
+        const TC be = Bt[v * NAQ + sl] * coef;
         for (int d = 0; d < D; d++) {
           wtot[v * D + d] += (al + be) * n[d];
           if constexpr (VISC == 2) wface[v * D + d] += al * n[d];
