@@ -47,12 +47,12 @@ RUNS = 5                       # timed runs per configuration; the median is rep
 # HBM bytes per launch of the stage kernels on the 32^3 P3 workload, from rocprofv3 FETCH_SIZE /
 # WRITE_SIZE passes scaled by a copy kernel of known size (fetch x2.000, write x1.000):
 # profiles/ns_tgv32_p3_fp64_mixed_tensor_kernels.md. Keyed by (kernel, precision mode, cells per direction, order).
-PROFILED_TRAFFIC = {("nsGradTensor", "fp64", 32, 3): (194351.2 * 2.0 + 204800.0 * 1.0) * 1024,
-                    ("nsGradTensor", "mixed", 32, 3): (145729.7 * 2.0 + 102400.0 * 1.0) * 1024,
-                    ("nsFaceFluxKernel", "fp64", 32, 3): (151387.7 * 2.0 + 122880.0 * 1.0) * 1024,
-                    ("nsFaceFluxKernel", "mixed", 32, 3): (114839.1 * 2.0 + 61503.2 * 1.0) * 1024,
+PROFILED_TRAFFIC = {("nsGradTensor", "fp64", 32, 3): (194707.8 * 2.0 + 204800.0 * 1.0) * 1024,
+                    ("nsGradTensor", "mixed", 32, 3): (146875.7 * 2.0 + 102400.0 * 1.0) * 1024,
+                    ("nsFaceFluxKernel", "fp64", 32, 3): (151378.2 * 2.0 + 122880.0 * 1.0) * 1024,
+                    ("nsFaceFluxKernel", "mixed", 32, 3): (114859.4 * 2.0 + 61514.2 * 1.0) * 1024,
                     ("nsResidualTensor", "fp64", 32, 3): (184931.6 * 2.0 + 232533.4 * 1.0) * 1024,
-                    ("nsResidualTensor", "mixed", 32, 3): (133740.8 * 2.0 + 171093.3 * 1.0) * 1024}
+                    ("nsResidualTensor", "mixed", 32, 3): (133738.0 * 2.0 + 171093.4 * 1.0) * 1024}
 # the batched stage on the 48-problem workload: one launch per element variant (triangle and quadrangle,
 # P1-P4), FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) KiB per launch, summed over the 8 variants of a stage
 # (profiles/batch48_uneven_tri_quad_euler_kernels.md)
