@@ -48,8 +48,23 @@ template <int N>
 struct TensorLds {
   static constexpr int NPT = N * N * N, NAQ = 6 * N * N;
   double B[N * N], Binv[N * N], Dn[N * N], lf[2 * N], wn[NPT], fwt[NAQ], kappa[2];
+  float Bf[N * N], Dnf[N * N], lff[2 * N];  // FP32 copies for the mixed mode's contractions
   short iperm[NPT], slot_of[6 * NPT], base[NAQ], axis[6], side[6];
 };
+
+// the contraction-type copy of a staged table
+template <typename TC, int N>
+__device__ __forceinline__ const TC* tabB(const TensorLds<N>& L) {
+  if constexpr (sizeof(TC) == 8) return L.B; else return L.Bf;
+}
+template <typename TC, int N>
+__device__ __forceinline__ const TC* tabDn(const TensorLds<N>& L) {
+  if constexpr (sizeof(TC) == 8) return L.Dn; else return L.Dnf;
+}
+template <typename TC, int N>
+__device__ __forceinline__ const TC* tabLf(const TensorLds<N>& L) {
+  if constexpr (sizeof(TC) == 8) return L.lf; else return L.lff;
+}
 
 // all threads of the workgroup; the caller's next barrier publishes the tables. Every thread
 // issues its loads unconditionally (indices clamped into range) before any store, so the copies
@@ -76,6 +91,8 @@ __device__ __forceinline__ void stageTables(const TensorOps& t, TensorLds<N>& L)
     L.B[tid] = b;
     L.Binv[tid] = bi;
     L.Dn[tid] = dn;
+    L.Bf[tid] = static_cast<float>(b);
+    L.Dnf[tid] = static_cast<float>(dn);
   }
   if (tid < NPT) {
     L.wn[tid] = w0 * w1 * w2;
@@ -88,7 +105,10 @@ __device__ __forceinline__ void stageTables(const TensorOps& t, TensorLds<N>& L)
     L.fwt[tid] = fw;
     L.base[tid] = bs;
   }
-  if (tid < 2 * N) L.lf[tid] = lf;
+  if (tid < 2 * N) {
+    L.lf[tid] = lf;
+    L.lff[tid] = static_cast<float>(lf);
+  }
   if (tid < 6) {
     L.axis[tid] = static_cast<short>(ax);
     L.side[tid] = static_cast<short>(sd);
@@ -120,14 +140,14 @@ __device__ __forceinline__ int nodeIndex(int q, int axis, int N) {
 // out[c][q] = sum_m M[idx_axis(q) N + m] in[c][q with axis index m], for C components of NPT
 // values each, one thread per node (all threads of the element call it; no barrier inside)
 template <typename TC, int N, int C>
-__device__ __forceinline__ void linePass(const double* M, int axis, const TC* in, TC* out, int q, bool ok) {
+__device__ __forceinline__ void linePass(const TC* M, int axis, const TC* in, TC* out, int q, bool ok) {
   constexpr int NPT = N * N * N;
   if (!ok) return;
   const int i = nodeIndex(q, axis, N), st = axisStride(axis, N);
   const int q0 = q - i * st;
   TC m[N];
 #pragma unroll
-  for (int k = 0; k < N; k++) m[k] = static_cast<TC>(M[i * N + k]);
+  for (int k = 0; k < N; k++) m[k] = M[i * N + k];
 #pragma unroll
   for (int c = 0; c < C; c++) {
     TC s = TC(0);
@@ -164,10 +184,10 @@ __device__ __forceinline__ void writeTraces(const TensorLds<N>& L, const TC* sA,
     if (idx < 0) break;
     const int sl = idx % NAQ, v = idx / NAQ, s = sl / NFQ;
     const int b = L.base[sl], st = axisStride(L.axis[s], N);
-    const double* l = L.lf + L.side[s] * N;
+    const TC* l = tabLf<TC, N>(L) + L.side[s] * N;
     TC acc = TC(0);
 #pragma unroll
-    for (int i = 0; i < N; i++) acc += static_cast<TC>(l[i]) * sA[v * NPT + b + i * st];
+    for (int i = 0; i < N; i++) acc += l[i] * sA[v * NPT + b + i * st];
     Uf_out[(e * NV + v) * NAQ + sl] = acc;
     if (rec) rec[(e * NV + v) * NAQ + sl] = static_cast<double>(acc);
   }
@@ -305,20 +325,22 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
   }
   loadModal<TC, N>(t, a, a.U, e, Bt, q, ok);
   __syncthreads();
-  linePass<TC, N, NV>(sT.B, 0, Bt, Y, q, ok);
+  linePass<TC, N, NV>(tabB<TC, N>(sT), 0, Bt, Y, q, ok);
   elemSync<NPT>();
-  linePass<TC, N, NV>(sT.B, 1, Y, Bt, q, ok);
+  linePass<TC, N, NV>(tabB<TC, N>(sT), 1, Y, Bt, q, ok);
   elemSync<NPT>();
-  linePass<TC, N, NV>(sT.B, 2, Bt, Y, q, ok);  // Y: nodal state
+  linePass<TC, N, NV>(tabB<TC, N>(sT), 2, Bt, Y, q, ok);  // Y: nodal state
   elemSync<NPT>();
   if (ok) {
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
       const int idx = q + k * NPT;
       if (idx < NV * NAQ) {
-        const double uo = static_cast<double>(fo[k]), un = static_cast<double>(fn[k]);
-        Al[idx] = static_cast<TC>(((uo + un) / 2.0 - uo) * fw[k]);
-        Bt[idx] = static_cast<TC>(((un - uo) / 2.0) * fw[k]);
+        // in the contraction type (the FP64 mode's arithmetic is FP64; the mixed mode's traces
+        // are FP32 deviations already)
+        const TC uo = fo[k], un = fn[k], w = static_cast<TC>(fw[k]);
+        Al[idx] = ((uo + un) / TC(2) - uo) * w;
+        Bt[idx] = ((un - uo) / TC(2)) * w;
       }
     }
     if (g.Fb > 0)
@@ -352,7 +374,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
       for (int v = 0; v < NV; v++) {
         TC s = TC(0);
 #pragma unroll
-        for (int m = 0; m < N; m++) s += static_cast<TC>(sT.Dn[i * N + m]) * Y[v * NPT + q0 + m * st];
+        for (int m = 0; m < N; m++) s += tabDn<TC, N>(sT)[i * N + m] * Y[v * NPT + q0 + m * st];
         dr[v][ax] = s;
       }
     }
@@ -453,21 +475,27 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
       if (sl >= NAQ) break;
       const int s = sl / NFQ;
       const int b = sT.base[sl], st = axisStride(sT.axis[s], N);
-      const double* l = sT.lf + sT.side[s] * N;
       const int nbs = nbk[k];
       double n[D], us[NV], gs[NG];
       for (int d = 0; d < D; d++) n[d] = nk[k][d];
       for (int v = 0; v < NV; v++) us[v] = pointState<TC>(a, uk[k][v], v);
-      const double kf = sT.kappa[sT.side[s]] * idj / sT.fwt[sl];  // BR2 own-face lift factor of the slot
+      // BR2 own-face lift factor of the slot, times the normal (in the contraction type)
+      const double kf = sT.kappa[sT.side[s]] * idj / sT.fwt[sl];
+      TC kn[D];
+      for (int d = 0; d < D; d++) kn[d] = static_cast<TC>(kf * n[d]);
+      const TC* lt = tabLf<TC, N>(sT) + sT.side[s] * N;
       TC gt[NG];
       for (int r = 0; r < NG; r++) {
         TC acc = TC(0);
 #pragma unroll
-        for (int i = 0; i < N; i++) acc += static_cast<TC>(l[i]) * W[r * NPT + b + i * st];
-        double gv = static_cast<double>(acc);
-        if constexpr (VISC == 2)  // r_s at its own face: kappa_side beta n_d / |J|
-          gv += kf * static_cast<double>(Bt[(r / D) * NAQ + sl]) * n[r % D];
-        gt[r] = static_cast<TC>(gv);
+        for (int i = 0; i < N; i++) acc += lt[i] * W[r * NPT + b + i * st];
+        if constexpr (VISC == 2) {  // r_s at its own face: kappa_side beta n_d / |J|
+          if constexpr (sizeof(TC) == 8)
+            acc += kf * Bt[(r / D) * NAQ + sl] * n[r % D];
+          else
+            acc += Bt[(r / D) * NAQ + sl] * kn[r % D];
+        }
+        gt[r] = acc;
         gs[r] = static_cast<double>(gt[r]);
         if (nbs < 0) Gf[(e * NG + r) * NAQ + sl] = gt[r];
       }
@@ -516,20 +544,20 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
   }
   elemSync<NPT>();
   if (ok) {
-    double R[NV];
-    for (int v = 0; v < NV; v++) R[v] = 0.0;
+    TC R[NV];
+    for (int v = 0; v < NV; v++) R[v] = TC(0);
     for (int ax = 0; ax < D; ax++) {
       const int i = nodeIndex(q, ax, N), st = axisStride(ax, N), q0 = q - i * st;
       for (int v = 0; v < NV; v++) {
         TC s = TC(0);
 #pragma unroll
-        for (int m = 0; m < N; m++) s += static_cast<TC>(sT.Dn[m * N + i]) * W[(ax * NV + v) * NPT + q0 + m * st];
-        R[v] += static_cast<double>(s);
+        for (int m = 0; m < N; m++) s += tabDn<TC, N>(sT)[m * N + i] * W[(ax * NV + v) * NPT + q0 + m * st];
+        R[v] += s;
       }
     }
-    R[D] += src;
+    R[D] += static_cast<TC>(src);
     TC* Rv = reinterpret_cast<TC*>(a.Rvol);  // TC values in the FP64-sized buffer
-    for (int v = 0; v < NV; v++) Rv[(e * NV + v) * NPT + q] = static_cast<TC>(R[v]);
+    for (int v = 0; v < NV; v++) Rv[(e * NV + v) * NPT + q] = R[v];
   }
 }
 
@@ -678,11 +706,11 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsResidualTensor(Geo g, P
   }
   elemSync<NPT>();
   TC* Y = reinterpret_cast<TC*>(T0);
-  linePass<TC, N, NV>(sT.B, 0, X, Y, q, ok);
+  linePass<TC, N, NV>(tabB<TC, N>(sT), 0, X, Y, q, ok);
   elemSync<NPT>();
-  linePass<TC, N, NV>(sT.B, 1, Y, X, q, ok);
+  linePass<TC, N, NV>(tabB<TC, N>(sT), 1, Y, X, q, ok);
   elemSync<NPT>();
-  linePass<TC, N, NV>(sT.B, 2, X, Y, q, ok);  // Y: new nodal state
+  linePass<TC, N, NV>(tabB<TC, N>(sT), 2, X, Y, q, ok);  // Y: new nodal state
   elemSync<NPT>();
   double* rec_t = nullptr;
   if (a.rec_t) {
@@ -721,11 +749,11 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void tensorTraceKernel(Geo g, 
     }
   }
   __syncthreads();
-  linePass<TC, N, NV>(sT.B, 0, X, Y, q, ok);
+  linePass<TC, N, NV>(tabB<TC, N>(sT), 0, X, Y, q, ok);
   elemSync<NPT>();
-  linePass<TC, N, NV>(sT.B, 1, Y, X, q, ok);
+  linePass<TC, N, NV>(tabB<TC, N>(sT), 1, Y, X, q, ok);
   elemSync<NPT>();
-  linePass<TC, N, NV>(sT.B, 2, X, Y, q, ok);
+  linePass<TC, N, NV>(tabB<TC, N>(sT), 2, X, Y, q, ok);
   elemSync<NPT>();
   if (ok) writeTraces<TC, N>(sT, Y, static_cast<TC*>(Uf_out), e, le, tid);
 }
