@@ -353,7 +353,7 @@ __global__ __launch_bounds__(kThreads) void nsAdjFaceKernel(Geo g, Phys ph, Stag
   }
 }
 
-// ---- kernel 2, reverse mode (Central, Lax-Friedrichs, HLLC): one thread per face point ---------
+// ---- kernel 2, reverse mode (Central, Lax-Friedrichs, HLLC, Roe): one thread per face point ----
 // The numerical flux and both sides' viscous fluxes are differentiated in one reverse pass
 // (physics_adjoint.hpp) instead of one dual-number pass per state component and side.
 template <int D, int NF, int NFQ>

@@ -17,7 +17,7 @@
 //   nsAdjVolumeTensor  recompute Y, alpha, beta, g_tot, g_face and gs (-> Gf); mu = the adjoint of
 //                      R at the nodes; H-bar = lf-traces of mu (-> Hb); the pointwise volume-flux
 //                      VJP of W-bar_a = Dn_a mu -> u-bar (-> Ua) and g_tot-bar (-> RtV)
-//   nsAdjFaceRevKernel (dg_adjoint.hpp, or nsAdjFaceKernel for the Roe and Exact fluxes) face-flux
+//   nsAdjFaceRevKernel (dg_adjoint.hpp, or nsAdjFaceKernel for the Exact flux) face-flux
 //                      VJP -> UFf, GFb
 //   nsAdjLiftTensor    gs-bar -> g_face-bar (lf transposes) and beta-bar (own-face lift); the
 //                      transpose of the gradient: Y-bar += Dn^T J^-1 (g_tot-bar + g_face-bar)

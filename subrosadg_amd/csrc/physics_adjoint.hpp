@@ -139,10 +139,213 @@ __device__ __forceinline__ void hllcStarVjp(const double* n, const double* u, co
   unb -= ab;
 }
 
-// numerical convective flux h = H(u_L, u_R; n) (ConvectiveFlux.cpp:93-238, the Central,
-// Lax-Friedrichs and HLLC cases): wbl/wbr += adjoints of the computational variables, ubl/ubr +=
-// adjoints of the conserved states where H uses them directly. Returns false for the fluxes without a
-// reverse form here (Roe, Exact), leaving the outputs untouched.
+// Roe flux with Harten's entropy fix (ConvectiveFlux.cpp:240-350, restated in physics.hpp):
+// wbl/wbr += adjoints of the computational variables for h = ((F_L + F_R) - sum_k s_k r_k) / 2
+template <int D>
+__device__ void roeFluxVjp(const Phys& ph, const double* n, const double* wl, const double* wr, const double* hb,
+                           double* wbl, double* wbr) {
+  constexpr int NV = D + 2;
+  double fb[NV], sb[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    fb[v] = hb[v] / 2.0;
+    sb[v] = -hb[v] / 2.0;  // adjoint of the dissipation sum
+  }
+  normalFluxVjp<D>(n, wl, fb, wbl);
+  normalFluxVjp<D>(n, wr, fb, wbr);
+  // forward recomputation of the Roe state and the wave strengths
+  const double g = kGamma;
+  const double sql = sqrt(wl[0]), sqr = sqrt(wr[0]);
+  const double sqs = sql + sqr, isqs = 1.0 / sqs;
+  const double rho = sqrt(wl[0] * wr[0]);
+  double vel[D], v2l = 0.0, v2r = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    vel[d] = (sql * wl[1 + d] + sqr * wr[1 + d]) * isqs;
+    v2l += wl[1 + d] * wl[1 + d];
+    v2r += wr[1 + d] * wr[1 + d];
+  }
+  const double hl = wl[D + 1] * g + v2l / 2.0, hr = wr[D + 1] * g + v2r / 2.0;
+  const double htot = (sql * hl + sqr * hr) * isqs;
+  double v2 = 0.0, un = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    v2 += vel[d] * vel[d];
+    un += vel[d] * n[d];
+  }
+  const double e = (htot - v2 / 2.0) / g;
+  const double p = pressureOf(ph, rho, e);
+  const double c = soundSpeedOf(ph, rho, p);
+  double dw[D + 3];
+#pragma unroll
+  for (int i = 0; i < D + 3; i++) dw[i] = wr[i] - wl[i];
+  double dun = 0.0, vdv = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    dun += dw[1 + d] * n[d];
+    vdv += vel[d] * dw[1 + d];
+  }
+  const double delta = c / 20.0;
+  const double xm = un - c, xp = un + c;
+  const bool am = fabs(xm) > delta, ap = fabs(xp) > delta;
+  const double a1 = am ? fabs(xm) : (xm * xm + delta * delta) / (2.0 * delta);
+  const double a3 = ap ? fabs(xp) : (xp * xp + delta * delta) / (2.0 * delta);
+  const double dp = dw[D + 2], drho = dw[0];
+  const double ic2 = 1.0 / (c * c);
+  const double X0 = dp - rho * c * dun, X3 = dp + rho * c * dun;
+  const double s0 = a1 * X0 * (0.5 * ic2);
+  const double aun = fabs(un), sgn = un < 0.0 ? -1.0 : 1.0;
+  const double s1 = aun * (drho - dp * ic2);
+  const double s2 = aun * rho;
+  const double s3 = a3 * X3 * (0.5 * ic2);
+  // reverse
+  double velb[D], rhob = 0.0, cb = 0.0, unb = 0.0, htotb = 0.0, v2b = 0.0, dunb = 0.0, vdvb = 0.0, dpb = 0.0,
+         drhob = 0.0, ic2b = 0.0, a1b = 0.0, a3b = 0.0, dwb[D + 3];
+#pragma unroll
+  for (int d = 0; d < D; d++) velb[d] = 0.0;
+#pragma unroll
+  for (int i = 0; i < D + 3; i++) dwb[i] = 0.0;
+  const double se = sb[D + 1];
+  // column 3: (1, vel + c n, htot + c un) s3
+  double s3b = sb[0] + se * (htot + c * un);
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    s3b += sb[1 + d] * (vel[d] + c * n[d]);
+    velb[d] += sb[1 + d] * s3;
+    cb += sb[1 + d] * n[d] * s3;
+  }
+  htotb += se * s3;
+  cb += se * un * s3;
+  unb += se * c * s3;
+  // column 2 (D >= 2): (0, dv - dun n, vdv - un dun) s2
+  double s2b = 0.0;
+  if (D >= 2) {
+    s2b += se * (vdv - un * dun);
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      s2b += sb[1 + d] * (dw[1 + d] - dun * n[d]);
+      dwb[1 + d] += sb[1 + d] * s2;
+      dunb -= sb[1 + d] * n[d] * s2;
+    }
+    vdvb += se * s2;
+    unb -= se * dun * s2;
+    dunb -= se * un * s2;
+  }
+  // column 1: (1, vel, |vel|^2 / 2) s1
+  double s1b = sb[0] + se * v2 / 2.0;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    s1b += sb[1 + d] * vel[d];
+    velb[d] += sb[1 + d] * s1;
+  }
+  v2b += se * s1 / 2.0;
+  // column 0: (1, vel - c n, htot - c un) s0
+  double s0b = sb[0] + se * (htot - c * un);
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    s0b += sb[1 + d] * (vel[d] - c * n[d]);
+    velb[d] += sb[1 + d] * s0;
+    cb -= sb[1 + d] * n[d] * s0;
+  }
+  htotb += se * s0;
+  cb -= se * un * s0;
+  unb -= se * c * s0;
+  // wave strengths
+  a3b += s3b * X3 * 0.5 * ic2;
+  const double X3b = s3b * a3 * 0.5 * ic2;
+  ic2b += s3b * a3 * X3 * 0.5;
+  dpb += X3b;
+  rhob += X3b * c * dun;
+  cb += X3b * rho * dun;
+  dunb += X3b * rho * c;
+  unb += s2b * rho * sgn;
+  rhob += s2b * aun;
+  unb += s1b * (drho - dp * ic2) * sgn;
+  drhob += s1b * aun;
+  dpb -= s1b * aun * ic2;
+  ic2b -= s1b * aun * dp;
+  a1b += s0b * X0 * 0.5 * ic2;
+  const double X0b = s0b * a1 * 0.5 * ic2;
+  ic2b += s0b * a1 * X0 * 0.5;
+  dpb += X0b;
+  rhob -= X0b * c * dun;
+  cb -= X0b * rho * dun;
+  dunb -= X0b * rho * c;
+  cb += ic2b * (-2.0 * ic2 / c);  // ic2 = 1 / c^2
+  // entropy-fixed eigenvalues |un -+ c|
+  double deltab = 0.0;
+  if (ap) {
+    const double t = xp < 0.0 ? -a3b : a3b;
+    unb += t;
+    cb += t;
+  } else {
+    unb += a3b * xp / delta;
+    cb += a3b * xp / delta;
+    deltab += a3b * (delta * delta - xp * xp) / (2.0 * delta * delta);
+  }
+  if (am) {
+    const double t = xm < 0.0 ? -a1b : a1b;
+    unb += t;
+    cb -= t;
+  } else {
+    unb += a1b * xm / delta;
+    cb -= a1b * xm / delta;
+    deltab += a1b * (delta * delta - xm * xm) / (2.0 * delta * delta);
+  }
+  cb += deltab / 20.0;
+  // jumps
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    dwb[1 + d] += dunb * n[d] + vdvb * vel[d];
+    velb[d] += vdvb * dw[1 + d];
+  }
+  dwb[D + 2] += dpb;
+  dwb[0] += drhob;
+  // c(rho, p), p(rho, e), e = (htot - |vel|^2 / 2) / gamma
+  double pb = 0.0, eb = 0.0;
+  soundSpeedVjp(ph, rho, p, c, cb, rhob, pb);
+  if (ph.eos == 0) {
+    rhob += pb * (kGamma - 1.0) * e;
+    eb += pb * (kGamma - 1.0) * rho;
+  } else {
+    rhob += pb * ph.c0 * ph.c0;
+  }
+  htotb += eb / g;
+  v2b -= eb / (2.0 * g);
+#pragma unroll
+  for (int d = 0; d < D; d++) velb[d] += unb * n[d] + 2.0 * v2b * vel[d];
+  // Roe averages
+  const double isqs2 = isqs * isqs;
+  double sqlb = htotb * hl * isqs, sqrb = htotb * hr * isqs;
+  const double hlb = htotb * sql * isqs, hrb = htotb * sqr * isqs;
+  double isqsb = htotb * (sql * hl + sqr * hr);
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    sqlb += velb[d] * wl[1 + d] * isqs;
+    sqrb += velb[d] * wr[1 + d] * isqs;
+    wbl[1 + d] += velb[d] * sql * isqs + hlb * wl[1 + d];
+    wbr[1 + d] += velb[d] * sqr * isqs + hrb * wr[1 + d];
+    isqsb += velb[d] * (sql * wl[1 + d] + sqr * wr[1 + d]);
+  }
+  wbl[D + 1] += hlb * g;
+  wbr[D + 1] += hrb * g;
+  const double sqsb = -isqsb * isqs2;
+  sqlb += sqsb;
+  sqrb += sqsb;
+  wbl[0] += rhob * wr[0] / (2.0 * rho) + sqlb / (2.0 * sql);
+  wbr[0] += rhob * wl[0] / (2.0 * rho) + sqrb / (2.0 * sqr);
+  // dw = wr - wl
+#pragma unroll
+  for (int i = 0; i < D + 3; i++) {
+    wbr[i] += dwb[i];
+    wbl[i] -= dwb[i];
+  }
+}
+
+// numerical convective flux h = H(u_L, u_R; n) (ConvectiveFlux.cpp:93-350, the Central,
+// Lax-Friedrichs, HLLC and Roe cases): wbl/wbr += adjoints of the computational variables, ubl/ubr +=
+// adjoints of the conserved states where H uses them directly. Returns false for the flux without a
+// reverse form here (Exact), leaving the outputs untouched.
 template <int D>
 __device__ bool convectiveFluxVjp(const Phys& ph, const double* n, const double* ul, const double* wl,
                                   const double* ur, const double* wr, const double* hb, double* wbl, double* wbr,
@@ -193,6 +396,10 @@ __device__ bool convectiveFluxVjp(const Phys& ph, const double* n, const double*
       side(wl, wbl, unl, cl);
     else
       side(wr, wbr, unr, cr);
+    return true;
+  }
+  if (ph.flux == 3) {
+    roeFluxVjp<D>(ph, n, wl, wr, hb, wbl, wbr);
     return true;
   }
   if (ph.flux != 2) return false;

@@ -126,7 +126,7 @@ Launchers makeLaunchers() {
                            dim3(sdg::kThreads), 0, st, g, ph, a, j);
       };
       l.adj_face = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
-        if (ph.flux <= SDG_FLUX_HLLC)  // reverse mode, one thread per face point
+        if (ph.flux <= SDG_FLUX_ROE)  // reverse mode, one thread per face point
           hipLaunchKernelGGL((sdg::nsAdjFaceRevKernel<D, NF, NFQ>), dim3(blocksFor(g.F * NFQ, sdg::kThreads)),
                              dim3(sdg::kThreads), 0, st, g, ph, a, j);
         else  // dual numbers, one thread per (state component or gradient, side, face point)
