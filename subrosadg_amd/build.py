@@ -68,7 +68,7 @@ def build(verbose: bool = True) -> str:
     root = os.path.dirname(HERE)
     exe_dir = os.path.join(root, "examples", "bin")
     os.makedirs(exe_dir, exist_ok=True)
-    ex_jobs = [["g++", "-std=c++17", "-O2", "-I", os.path.join(root, "include"), os.path.join(root, "examples", ex + ".cpp"),
+    ex_jobs = [["g++", "-std=c++23", "-O2", "-I", os.path.join(root, "include"), os.path.join(root, "examples", ex + ".cpp"),
                 "-L", HERE, "-lsdg_hip", "-Wl,-rpath,$ORIGIN/../../subrosadg_amd", "-o", os.path.join(exe_dir, ex)]
                for ex in EXAMPLES]
     with ThreadPoolExecutor(max_workers=workers) as pool:
