@@ -468,6 +468,20 @@ def adjoint_leg(args, n=64, order=3, steps=400, cfl=0.1, budget=32):
     return out
 
 
+def mixed_desc(prob):
+    q = L.ProblemDesc.from_buffer_copy(prob)
+    q.precision = L.PREC_MIXED
+    return q
+
+
+def sync_time_of(solver, fn):
+    solver.synchronize()
+    t0 = time.perf_counter()
+    r = fn()
+    solver.synchronize()
+    return time.perf_counter() - t0, r
+
+
 def c5_adjoint_leg(args, n=64, order=4, steps=None):
     """Gradient of the time-integrated kinetic-energy dissipation int_0^T eps_K dt = K(0) - K(T)
     of the 64^3 P4 Taylor-Green vortex (HLLC + BR2, FP64, fixed dt) with respect to the initial
@@ -545,11 +559,37 @@ def c5_adjoint_leg(args, n=64, order=4, steps=None):
         js.append(ka - P.kinetic_energy(mesh, s.get_state(), gradient=False)[0])
     g_fd = (js[0] - js[1]) / (2 * eps)
     s.close()
+    # the same in a mixed-precision context: FP32 contractions and fluxes in the recorded and
+    # recomputed forward runs, the FP64 adjoint kernels on FP64 traces of the stored FP64 states
+    sm = L.Solver(mixed_desc(prob), mesh)
+    sm.set_initial_primitive(tgv_primitive(mesh, p0, p0, A))
+    sm.step(dt, 1)
+    fwm = []
+    for _ in range(3):
+        sm.set_state(u0)
+        fwm.append(sync_time_of(sm, lambda: sm.step(dt, steps))[0])
+    t_fwd_m = statistics.median(fwm)
+    sm.set_state(u0)
+    t_rec_m, _ = sync_time_of(sm, lambda: sm.adjoint_forward(dt, steps, steps))
+    _, dKTm = P.kinetic_energy(mesh, sm.get_state())
+    lam_dev = L.DeviceArray.from_numpy(-dKTm)
+    out_dev = L.DeviceArray(lam_dev.shape)
+    t_bwd_m, _ = sync_time_of(sm, lambda: sm.adjoint_backward_device(lam_dev, out_dev))
+    g_adj_m = float(((dK0 + out_dev.numpy()) * du0).sum())
+    del lam_dev, out_dev
+    sm.close()
+    mixed = {"forward_s": t_fwd_m, "forward_dof_stage_per_s": mesh.E * mesh.nb * 3 * steps / t_fwd_m,
+             "full_storage": {"snapshots": steps, "checkpointed_forward_s": t_rec_m, "reverse_s": t_bwd_m,
+                              "adjoint_over_forward": (t_rec_m + t_bwd_m) / t_fwd_m,
+                              "adjoint_over_fp64_forward": (t_rec_m + t_bwd_m) / t_fwd},
+             "gradient": g_adj_m, "rel_diff_vs_fp64_gradient": abs(g_adj_m - g_adj) / abs(g_adj),
+             "fd_check_rel_err": abs(g_adj_m - g_fd) / abs(g_fd)}
     out = {"workload": f"3-D NS Taylor-Green vortex {n}^3 hex P{order}, HLLC + BR2, FP64, {steps} SSP-RK3 steps, "
                        f"d(K(0) - K(T))/d(velocity amplitude)", "steps": steps, "forward_s": t_fwd,
            "forward_dof_stage_per_s": mesh.E * mesh.nb * 3 * steps / t_fwd, **rec,
            "adjoint_over_forward": rec["binomial"]["adjoint_over_forward"], "dissipation": K0 - KT,
-           "gradient": g_adj, "gradient_fd": g_fd, "fd_check_rel_err": abs(g_adj - g_fd) / abs(g_fd)}
+           "gradient": g_adj, "gradient_fd": g_fd, "fd_check_rel_err": abs(g_adj - g_fd) / abs(g_fd),
+           "mixed": mixed}
     log(f"C5 adjoint leg: {out}")
     return out
 

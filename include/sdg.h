@@ -441,7 +441,9 @@ int sdg_write_view(const sdg_mesh* mesh, const sdg_problem_desc* problem, const 
 int sdg_write_vtu(const sdg_mesh* mesh, const sdg_problem_desc* problem, const double* coefficients,
                   const double* node_values, int step, double time, const char* path);
 
-/* ---- discrete adjoint (Navier-Stokes, FP64, periodic meshes: quadrangles P3-P5, hexahedra P2-P4)
+/* ---- discrete adjoint (Navier-Stokes, periodic meshes: quadrangles P3-P5 and hexahedra P2-P4 in FP64,
+ * hexahedra P2-P4 in mixed precision: the mixed run is recorded and recomputed as it ran, and the
+ * FP64 residual is differentiated at its states)
  * The reference has no adjoint; these entry points transpose its residual (R of sdg_residual,
  * SpatialDiscrete.cpp:194-1116) and its RK stage update (TimeIntegration.cpp:181-198), each as the
  * transpose of the kernel family that runs the forward stages (sum-factorised on hexahedra by

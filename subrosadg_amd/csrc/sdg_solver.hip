@@ -48,6 +48,7 @@ struct Launchers {
   StageFn ns_flux = nullptr;
   StageFn ns_face = nullptr;  // matrix-core path: face fluxes before ns_flux
   TraceFn trace = nullptr;
+  TraceFn trace64 = nullptr;  // FP64 traces of the FP64 state (the adjoint of mixed-precision contexts)
   SrcFn source = nullptr;  // modal Boussinesq source (all kernel sets but the sum-factorised one)
   DtFn dt = nullptr;
   MonFn monitor = nullptr;
@@ -267,24 +268,35 @@ void useTensorKernels(Launchers& l) {
     hipLaunchKernelGGL((sdg::tensorTraceKernel<TC, N>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, U,
                        Uf, mix);
   };
-  if constexpr (sizeof(TC) == 8) {
-    // the transpose of these stage kernels (the face-flux VJP of the matrix-core set stays)
-    l.adj_volume[0] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
-      hipLaunchKernelGGL((sdg::nsAdjVolumeTensor<N, 1>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
-    };
-    l.adj_volume[1] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
-      hipLaunchKernelGGL((sdg::nsAdjVolumeTensor<N, 2>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
-    };
-    l.adj_lift[0] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
-      hipLaunchKernelGGL((sdg::nsAdjLiftTensor<N, 1>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
-    };
-    l.adj_lift[1] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
-      hipLaunchKernelGGL((sdg::nsAdjLiftTensor<N, 2>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
-    };
-    l.adj_gather = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
-      hipLaunchKernelGGL((sdg::nsAdjGatherTensor<N>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
-    };
-  }
+  // the transpose of these stage kernels, FP64 in both precision modes: a mixed-precision context's
+  // adjoint differentiates the FP64 residual at the states of its run, on FP64 traces (trace64)
+  l.trace64 = [](hipStream_t st, const Geo& g, const double* U, void* Uf, const sdg::MixRef& mix) {
+    hipLaunchKernelGGL((sdg::tensorTraceKernel<double, N>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, U, Uf, mix);
+  };
+  l.adj_volume[0] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+    hipLaunchKernelGGL((sdg::nsAdjVolumeTensor<N, 1>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
+  };
+  l.adj_volume[1] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+    hipLaunchKernelGGL((sdg::nsAdjVolumeTensor<N, 2>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
+  };
+  l.adj_face = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+    constexpr int NFQ = N * N;
+    if (ph.flux <= SDG_FLUX_ROE)
+      hipLaunchKernelGGL((sdg::nsAdjFaceRevKernel<3, 6, NFQ>), dim3(blocksFor(g.F * NFQ, sdg::kThreads)),
+                         dim3(sdg::kThreads), 0, st, g, ph, a, j);
+    else
+      hipLaunchKernelGGL((sdg::nsAdjFaceKernel<3, 6, NFQ>), dim3(blocksFor(g.F * NFQ * 2 * 6, sdg::kThreads)),
+                         dim3(sdg::kThreads), 0, st, g, ph, a, j);
+  };
+  l.adj_lift[0] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+    hipLaunchKernelGGL((sdg::nsAdjLiftTensor<N, 1>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
+  };
+  l.adj_lift[1] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+    hipLaunchKernelGGL((sdg::nsAdjLiftTensor<N, 2>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
+  };
+  l.adj_gather = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a, const sdg::AdjArgs& j) {
+    hipLaunchKernelGGL((sdg::nsAdjGatherTensor<N>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a, j);
+  };
   l.tensor = true;
 }
 template <int N>
@@ -584,6 +596,7 @@ struct sdg_ctx {
   double *adj_lam = nullptr, *adj_Hb = nullptr, *adj_RtV = nullptr, *adj_Ua = nullptr, *adj_UFf = nullptr;
   double *adj_GFb = nullptr, *adj_Pown = nullptr, *adj_Qnb = nullptr, *adj_out = nullptr;
   double *adj_S[3] = {nullptr, nullptr, nullptr}, *adj_acc = nullptr, *adj_tmp = nullptr;
+  double *adj_Uf = nullptr, *adj_Gf = nullptr;  // mixed-precision contexts: FP64 traces and gradient traces
   std::vector<double*> snaps;   // checkpoint states (device layout)
   std::vector<int> snap_step;   // step index held by each snapshot, -1 when free
   int adj_nsteps = -1, adj_nslots = 0;
@@ -721,7 +734,7 @@ void enqueueStage(sdg_ctx* ctx, int stage, double dt, int mode, double* wout, hi
   a.mode = mode;
   if (ctx->recording && mode == 0) {
     a.rec = ctx->adj_pool;
-    a.rec_t = ctx->adj_tpool;
+    a.rec_t = ctx->mixed ? nullptr : ctx->adj_tpool;  // a mixed run's traces are FP32 deviations
     a.rec_stage = stage;
     a.rec_S = ctx->stages();
     a.rec_nsteps = ctx->rec_nsteps;
@@ -1788,10 +1801,11 @@ namespace {
 
 int ensureAdjoint(sdg_ctx* ctx) {
   if (ctx->adj_ready) return SDG_OK;
-  if (!ctx->ns() || ctx->mixed || !ctx->launch.adj_volume[0] || ctx->Fb > 0 || ctx->phys.source != SDG_SOURCE_NONE)
+  if (!ctx->ns() || (ctx->mixed && !ctx->launch.tensor) || !ctx->launch.adj_volume[0] || ctx->Fb > 0 ||
+      ctx->phys.source != SDG_SOURCE_NONE)
     return fail(ctx, SDG_E_UNSUPPORTED,
-                "adjoint: implemented for Navier-Stokes in FP64 on periodic meshes of the matrix-core element sizes "
-                "(quadrangle P3-P5, hexahedron P2-P4), without a source term");
+                "adjoint: implemented for Navier-Stokes on periodic meshes of the matrix-core element sizes "
+                "(quadrangle P3-P5, hexahedron P2-P4; mixed precision on hexahedra P2-P4), without a source term");
   const size_t E = static_cast<size_t>(ctx->E);
   const size_t nU = E * ctx->nv * ctx->nb, nS = E * ctx->nv * ctx->naq, nG = E * ctx->ng * ctx->nb,
                nGS = E * ctx->ng * ctx->naq;
@@ -1801,6 +1815,7 @@ int ensureAdjoint(sdg_ctx* ctx) {
       allocate(ctx, 3 * nU, &ctx->adj_S[0]) || allocate(ctx, nU, &ctx->adj_acc) ||
       allocate(ctx, nU, &ctx->adj_tmp))
     return SDG_E_NOMEM;
+  if (ctx->mixed && (allocate(ctx, nS, &ctx->adj_Uf) || allocate(ctx, nGS, &ctx->adj_Gf))) return SDG_E_NOMEM;
   ctx->adj_S[1] = ctx->adj_S[0] + nU;  // stage states, one contiguous block
   ctx->adj_S[2] = ctx->adj_S[0] + 2 * nU;
   ctx->adj_ready = true;
@@ -1815,13 +1830,17 @@ size_t stateCount(const sdg_ctx* ctx) { return static_cast<size_t>(ctx->E) * ctx
 int vjpAt(sdg_ctx* ctx, const double* U, double* lam, int apply_minv, double* out, int stage = -1,
           const double* traces = nullptr) {
   double* Uf = const_cast<double*>(traces);
-  if (!Uf) {
+  if (ctx->mixed) {  // FP64 traces of the FP64 state (the run's own traces are FP32 deviations)
+    Uf = ctx->adj_Uf;
+    ctx->launch.trace64(ctx->stream, ctx->geo, U, Uf, ctx->mix);
+  } else if (!Uf) {
     Uf = ctx->Uf[ctx->cur ^ 1];  // scratch: the next stage rewrites this trace buffer
     ctx->launch.trace(ctx->stream, ctx->geo, U, Uf, ctx->mix);
   }
   StageArgs a = baseArgs(ctx);
   a.U = const_cast<double*>(U);
   a.Uf_in = Uf;
+  if (ctx->mixed) a.Gf = ctx->adj_Gf;
   a.Wout = nullptr;
   a.status = nullptr;
   a.mode = 1;
@@ -1949,7 +1968,8 @@ int enqueueAdjointStepStored(sdg_ctx* ctx) {
   const size_t nt = static_cast<size_t>(ctx->E) * ctx->nv * ctx->naq;
   int rc;
   for (int s = S - 1; s >= 0; s--)
-    if ((rc = vjpAt(ctx, ctx->adj_S[s], ctx->adj_lam, 1, nullptr, s, ctx->adj_T + s * nt))) return rc;
+    if ((rc = vjpAt(ctx, ctx->adj_S[s], ctx->adj_lam, 1, nullptr, s, ctx->mixed ? nullptr : ctx->adj_T + s * nt)))
+      return rc;
   HIP_TRY(ctx, hipGetLastError());
   return SDG_OK;
 }
@@ -2056,8 +2076,11 @@ int sdg_adjoint_forward(sdg_ctx* ctx, double dt, int nsteps, int snapshots) {
       }
     }
     const size_t nt = static_cast<size_t>(ctx->E) * ctx->nv * ctx->naq;
-    const size_t need_t = static_cast<size_t>(nsteps) * ctx->stages() * nt;
-    if (ctx->adj_pool && need_t > ctx->adj_tpool_cap) {
+    // FP64 mode: the stage states' traces are kept too; a mixed-precision run's are FP32 deviations,
+    // and its adjoint recomputes FP64 traces from the states
+    const bool keep_t = !ctx->mixed;
+    const size_t need_t = keep_t ? static_cast<size_t>(nsteps) * ctx->stages() * nt : 0;
+    if (keep_t && ctx->adj_pool && need_t > ctx->adj_tpool_cap) {
       if (ctx->adj_tpool) (void)hipFree(ctx->adj_tpool);
       ctx->adj_tpool = nullptr;
       ctx->adj_tpool_cap = 0;
@@ -2069,15 +2092,17 @@ int sdg_adjoint_forward(sdg_ctx* ctx, double dt, int nsteps, int snapshots) {
         (void)hipGetLastError();
       }
     }
-    if (ctx->adj_pool && ctx->adj_tpool && !ctx->adj_T && allocate(ctx, 3 * nt, &ctx->adj_T)) return SDG_E_NOMEM;
-    if (ctx->adj_pool && need <= ctx->adj_pool_cap && ctx->adj_tpool && need_t <= ctx->adj_tpool_cap) {
+    if (keep_t && ctx->adj_pool && ctx->adj_tpool && !ctx->adj_T && allocate(ctx, 3 * nt, &ctx->adj_T))
+      return SDG_E_NOMEM;
+    if (ctx->adj_pool && need <= ctx->adj_pool_cap && (!keep_t || (ctx->adj_tpool && need_t <= ctx->adj_tpool_cap))) {
       ctx->adj_dt = dt;
       ctx->adj_nsteps = -1;
       hipLaunchKernelGGL(sdg::resetStatusKernel, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
       HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_pool, ctx->U, stateCount(ctx) * sizeof(double), hipMemcpyDeviceToDevice,
                                   ctx->stream));  // slot (0, 0): the initial state and its traces
-      HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_tpool, ctx->traces(ctx->cur), nt * sizeof(double), hipMemcpyDeviceToDevice,
-                                  ctx->stream));
+      if (keep_t)
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_tpool, ctx->traces(ctx->cur), nt * sizeof(double),
+                                    hipMemcpyDeviceToDevice, ctx->stream));
       const int flips = ctx->stages() & 1;
       for (int i = 0; i < nsteps; i++) {
         // the graph bakes in the pool's step count; rebuild it when that changes
@@ -2204,16 +2229,17 @@ int reverseRecorded(sdg_ctx* ctx) {
       for (int i = ctx->adj_nsteps - 1; i >= 0; i--)
         for (int st = S - 1; st >= 0; st--) {
           const size_t slot = static_cast<size_t>(i) * S + st;
-          if ((rc = vjpAt(ctx, ctx->adj_pool + slot * n, ctx->adj_lam, 1, nullptr, st, ctx->adj_tpool + slot * nt)))
-            return rc;
+          const double* tr = ctx->mixed ? nullptr : ctx->adj_tpool + slot * nt;
+          if ((rc = vjpAt(ctx, ctx->adj_pool + slot * n, ctx->adj_lam, 1, nullptr, st, tr))) return rc;
         }
       return SDG_OK;
     }
     for (int i = ctx->adj_nsteps - 1; i >= 0; i--) {
       HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_S[0], ctx->adj_pool + static_cast<size_t>(i) * S * n, S * n * sizeof(double),
                                   hipMemcpyDeviceToDevice, ctx->stream));
-      HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_T, ctx->adj_tpool + static_cast<size_t>(i) * S * nt, S * nt * sizeof(double),
-                                  hipMemcpyDeviceToDevice, ctx->stream));
+      if (!ctx->mixed)
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->adj_T, ctx->adj_tpool + static_cast<size_t>(i) * S * nt,
+                                    S * nt * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
       if ((rc = replayGraph(ctx, ctx->adjst_graph, ctx->adjst_graph_dt, ctx->adj_dt, 0,
                             [&]() { return enqueueAdjointStepStored(ctx); })))
         return rc;
