@@ -589,8 +589,9 @@ __device__ __forceinline__ void volumeFluxStateVjp(const Phys& ph, const double*
   computational<D>(ph, u, w);
 #pragma unroll
   for (int i = 0; i < D + 3; i++) wb[i] = 0.0;
-  // F_c[a][.] is the normal flux with n = e_a
-#pragma unroll
+  // F_c[a][.] is the normal flux with n = e_a (a rolled loop: unrolled, the three passes' temporaries
+  // push the volume kernel at P4 further past two wavefronts' registers)
+#pragma unroll 1
   for (int a = 0; a < D; a++) {
     double n[D], fb[NV];
 #pragma unroll
