@@ -110,6 +110,11 @@ struct numeric_limits<Tangent> {
 #define ORACLE_REAL double
 #endif
 
+// loops over fewer items than this run on the calling thread: for small meshes the per-pass
+// fork/join of the thread team costs more than the work (results do not depend on it: the loops
+// are over independent elements or faces, and the reductions are exact minima)
+constexpr int64_t kOmpMinItems = 256;
+
 using std::exp;
 using std::fabs;
 using std::isnan;
@@ -512,7 +517,7 @@ struct Oracle {
   }
   void boundaryGradientQuadrature() {  // SpatialDiscrete.cpp:913-968 with calculateBoundaryGradientVariable
     const int rows = nv * dim;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(nbface >= kOmpMinItems)
     for (int64_t b = 0; b < nbface; b++) {
       std::vector<Real> ul(static_cast<size_t>(nv) * nfq);
       boundaryTraces(b, ul.data());
@@ -549,7 +554,7 @@ struct Oracle {
   }
   void boundaryQuadrature() {  // SpatialDiscrete.cpp:750-842
     const int rows = nv * dim;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(nbface >= kOmpMinItems)
     for (int64_t b = 0; b < nbface; b++) {
       std::vector<Real> ul(static_cast<size_t>(nv) * nfq), gl;
       boundaryTraces(b, ul.data());
@@ -616,7 +621,7 @@ struct Oracle {
     const Real s0 = sensorThreshold(order);
     const int lo = lowModes();
     std::vector<Real> eps(static_cast<size_t>(ne));
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(ne >= kOmpMinItems)
     for (int64_t e = 0; e < ne; e++) {  // calculateElementArtificialViscosity, :37-88
       Real num = 0.0, den = 0.0;
       for (int q = 0; q < nq; q++) {
@@ -675,7 +680,7 @@ struct Oracle {
 
   // --- passes of stepSolver ---
   void gradientQuadrature() {  // SpatialDiscrete.cpp:294-346
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(ne >= kOmpMinItems)
     for (int64_t e = 0; e < ne; e++) {
       std::vector<Real> uq(static_cast<size_t>(nv) * nq);
       gemmBt(uq.data(), &coeff[e * szC()], V.data(), nv, nb, nq, false);
@@ -707,7 +712,7 @@ struct Oracle {
   }
   void adjacencyGradientQuadrature() {  // SpatialDiscrete.cpp:844-911
     const int rows = nv * dim;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(nface >= kOmpMinItems)
     for (int64_t f = 0; f < nface; f++) {
       std::vector<Real> ul(static_cast<size_t>(nv) * nfq), ur(static_cast<size_t>(nv) * nfq);
       faceTraces(f, ul.data(), ur.data());
@@ -741,7 +746,7 @@ struct Oracle {
   }
   void gradientResidual() {  // SpatialDiscrete.cpp:1034-1068
     const int rows = nv * dim;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(ne >= kOmpMinItems)
     for (int64_t e = 0; e < ne; e++) {
       Real* res = &vg_res[e * szG()];
       gemm(res, &vg_adj[static_cast<size_t>(e) * rows * naq], Vf.data(), rows, naq, nb, false);
@@ -767,7 +772,7 @@ struct Oracle {
   }
   void updateGradientCoefficient() {  // TimeIntegration.cpp:200-228
     const int rows = nv * dim;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(ne >= kOmpMinItems)
     for (int64_t e = 0; e < ne; e++) {
       const Real* mi = &minv[static_cast<size_t>(e) * nb * nb];
       gemm(&vg_coeff[e * szG()], &vg_res[e * szG()], mi, rows, nb, nb, false);
@@ -784,7 +789,7 @@ struct Oracle {
   }
   void volumeQuadrature() {  // SpatialDiscrete.cpp:194-266
     const int rows = nv * dim;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(ne >= kOmpMinItems)
     for (int64_t e = 0; e < ne; e++) {
       std::vector<Real> uq(static_cast<size_t>(nv) * nq), gq;
       gemmBt(uq.data(), &coeff[e * szC()], V.data(), nv, nb, nq, false);
@@ -860,7 +865,7 @@ struct Oracle {
   }
   void adjacencyQuadrature() {  // SpatialDiscrete.cpp:633-748
     const int rows = nv * dim;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(nface >= kOmpMinItems)
     for (int64_t f = 0; f < nface; f++) {
       std::vector<Real> ul(static_cast<size_t>(nv) * nfq), ur(static_cast<size_t>(nv) * nfq);
       std::vector<Real> wl(static_cast<size_t>(dim + 3) * nfq), wr(static_cast<size_t>(dim + 3) * nfq);
@@ -916,7 +921,7 @@ struct Oracle {
     }
   }
   void calcResidual() {  // SpatialDiscrete.cpp:1016-1032
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(ne >= kOmpMinItems)
     for (int64_t e = 0; e < ne; e++) {
       Real* res = &residual[e * szC()];
       gemm(res, &vol_quad[static_cast<size_t>(e) * nv * dim * nq], dV.data(), nv, dim * nq, nb, false);
@@ -944,7 +949,7 @@ struct Oracle {
   void updateCoefficient(int stage, Real dt) {  // TimeIntegration.cpp:181-198
     Real c[3];
     rkCoefficients(stage, c);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(ne >= kOmpMinItems)
     for (int64_t e = 0; e < ne; e++) {
       Real* u = &coeff[e * szC()];
       const Real* un = &coeff_last[e * szC()];
@@ -958,7 +963,7 @@ struct Oracle {
   void relativeError() {  // TimeIntegration.cpp:279-324
     std::vector<Real> acc(nv, 0.0);
     std::vector<Real> per(static_cast<size_t>(ne) * nv);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(ne >= kOmpMinItems)
     for (int64_t e = 0; e < ne; e++) {
       std::vector<Real> rq(static_cast<size_t>(nv) * nq);
       gemmBt(rq.data(), &residual[e * szC()], V.data(), nv, nb, nq, false);
@@ -1025,7 +1030,7 @@ struct MixedOracle {
       }
   }
   void volumeQuadrature() {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(ne >= kOmpMinItems)
     for (int64_t e = 0; e < ne; e++) {
       const Part& p = of(e);
       std::vector<Real> uq(static_cast<size_t>(kNv) * p.nq);
@@ -1052,7 +1057,7 @@ struct MixedOracle {
     }
   }
   void adjacencyQuadrature() {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(nface >= kOmpMinItems)
     for (int64_t f = 0; f < nface; f++) {
       const int32_t L = fparent[2 * f], R = fparent[2 * f + 1];
       const int sl = flocal[2 * f], sr = flocal[2 * f + 1];
@@ -1077,7 +1082,7 @@ struct MixedOracle {
     }
   }
   void calcResidual() {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(ne >= kOmpMinItems)
     for (int64_t e = 0; e < ne; e++) {
       const Part& p = of(e);
       Real* res = &residual[coff[e]];
@@ -1088,7 +1093,7 @@ struct MixedOracle {
   void updateCoefficient(int stage, Real dt) {
     Real c[3];
     pt.rkCoefficients(stage, c);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(ne >= kOmpMinItems)
     for (int64_t e = 0; e < ne; e++) {
       const int nb = of(e).nb;
       const size_t sz = static_cast<size_t>(kNv) * nb;
@@ -1103,7 +1108,7 @@ struct MixedOracle {
   }
   void relativeError() {  // sum over both element types, divided by the element count
     std::vector<Real> per(static_cast<size_t>(ne) * kNv);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(ne >= kOmpMinItems)
     for (int64_t e = 0; e < ne; e++) {
       const Part& p = of(e);
       std::vector<Real> rq(static_cast<size_t>(kNv) * p.nq);
@@ -1277,7 +1282,7 @@ int oracle_get_state(void* h, Real* c) {
 int oracle_set_initial_primitive(void* h, const Real* prim) {
   Oracle* o = static_cast<Oracle*>(h);
   const int nv = o->nv, nq = o->nq, nb = o->nb, dim = o->dim;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(o->ne >= kOmpMinItems)
   for (int64_t e = 0; e < o->ne; e++) {
     std::vector<Real> uq(static_cast<size_t>(nv) * nq), tmp(static_cast<size_t>(nv) * nb);
     for (int q = 0; q < nq; q++) {
@@ -1369,7 +1374,7 @@ int oracle_get_face_gradient(void* h, Real* out) {
   Oracle* o = static_cast<Oracle*>(h);
   if (!o->ns()) return -2;
   const size_t rows = static_cast<size_t>(o->nv) * o->dim, blk = rows * o->nfq;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(o->ne >= kOmpMinItems)
   for (int64_t e = 0; e < o->ne; e++)
     for (int s = 0; s < o->nf; s++) o->faceGradientTrace(e, s, out + (static_cast<size_t>(e) * o->nf + s) * blk);
   return 0;
@@ -1387,7 +1392,7 @@ int oracle_compute_dt(void* h, Real cfl, Real* dt) {  // TimeIntegration.cpp:104
   Oracle* o = static_cast<Oracle*>(h);
   const int nv = o->nv, nq = o->nq, nb = o->nb, dim = o->dim;
   Real best = std::numeric_limits<Real>::max();
-#pragma omp parallel for schedule(static) reduction(min : best)
+#pragma omp parallel for schedule(static) reduction(min : best) if(o->ne >= kOmpMinItems)
   for (int64_t e = 0; e < o->ne; e++) {
     std::vector<Real> uq(static_cast<size_t>(nv) * nq);
     gemmBt(uq.data(), &o->coeff[e * o->szC()], o->V.data(), nv, nb, nq, false);
@@ -1513,7 +1518,7 @@ int oracle_mixed_get_state(void* h, Real* c) {
 // InitialCondition.cpp:85-104 per element type; prim: each element's points in element order
 int oracle_mixed_set_initial_primitive(void* h, const Real* prim) {
   MixedOracle* o = static_cast<MixedOracle*>(h);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(o->ne >= kOmpMinItems)
   for (int64_t e = 0; e < o->ne; e++) {
     const MixedOracle::Part& p = o->of(e);
     std::vector<Real> uq(static_cast<size_t>(4) * p.nq), tmp(static_cast<size_t>(4) * p.nb);
@@ -1542,7 +1547,7 @@ int oracle_mixed_compute_dt(void* h, Real cfl, Real* dt) {  // TimeIntegration.c
   MixedOracle* o = static_cast<MixedOracle*>(h);
   Real best = std::numeric_limits<Real>::max();
   const int order = o->pt.order;
-#pragma omp parallel for schedule(static) reduction(min : best)
+#pragma omp parallel for schedule(static) reduction(min : best) if(o->ne >= kOmpMinItems)
   for (int64_t e = 0; e < o->ne; e++) {
     const MixedOracle::Part& p = o->of(e);
     std::vector<Real> uq(static_cast<size_t>(4) * p.nq);
