@@ -611,9 +611,15 @@ def shock_leg(args, n=256, order=3, steps=100, cfl=0.1):
     prim[..., 3] = (0.1 + 0.9 * w) / (prim[..., 0] * gas)
     out = {"workload": f"2-D Euler explosion, [0,2]^2 slip walls, {n}^2 quads, P{order}, HLLC, SSP-RK3, CFL {cfl}, "
                        f"artificial viscosity (tolerance 4.0, factor 1)", "elements": mesh.E, "steps": steps}
-    for label, sc in (("artificial_viscosity", L.SHOCK_ARTIFICIAL_VISCOSITY), ("plain", L.SHOCK_NONE)):
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    jwq = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
+    finals = {}
+    for label, sc, prec in (("artificial_viscosity", L.SHOCK_ARTIFICIAL_VISCOSITY, L.PREC_FP64),
+                            ("artificial_viscosity_mixed", L.SHOCK_ARTIFICIAL_VISCOSITY, L.PREC_MIXED),
+                            ("plain", L.SHOCK_NONE, L.PREC_FP64)):
         prob = L.problem_desc(equation=L.EQ_EULER, convective_flux=L.FLUX_HLLC, cp=2.5, cv=25.0 / 14.0,
-                              shock_capturing=sc, empirical_tolerance=4.0, artificial_viscosity_factor=1.0)
+                              shock_capturing=sc, empirical_tolerance=4.0, artificial_viscosity_factor=1.0,
+                              precision=prec)
         s = L.Solver(prob, mesh)
         s.set_initial_primitive(prim)
         s.set_boundary_primitive(np.zeros((mesh.Fb, mesh.nfq, 4)))
@@ -628,6 +634,7 @@ def shock_leg(args, n=256, order=3, steps=100, cfl=0.1):
         med = statistics.median(ms)
         rec = {"ms_per_step": med, "dof_stage_per_s": mesh.E * mesh.nb * 3 / (med * 1e-3),
                "kernel_ms": [float(x) for x in kms]}
+        finals[label] = np.einsum("qb,ebv->eqv", V, s.get_state())
         if sc:
             av = s.artificial_viscosity()
             rec["nodes_with_viscosity"] = int((av > 0).sum())
@@ -635,6 +642,13 @@ def shock_leg(args, n=256, order=3, steps=100, cfl=0.1):
         s.close()
         out[label] = rec
     out["cost_over_plain"] = out["artificial_viscosity"]["ms_per_step"] / out["plain"]["ms_per_step"]
+    # the mixed run against the FP64 run after the same steps: relative L2 over the domain per group
+    a, b = finals["artificial_viscosity_mixed"], finals["artificial_viscosity"]
+    out["artificial_viscosity_mixed"]["mixed_over_fp64"] = (out["artificial_viscosity"]["ms_per_step"]
+                                                            / out["artificial_viscosity_mixed"]["ms_per_step"])
+    out["artificial_viscosity_mixed"]["l2_rel_dev_vs_fp64"] = max(
+        float(np.sqrt((jwq[..., None] * (a[..., g] - b[..., g]) ** 2).sum() / (jwq[..., None] * b[..., g] ** 2).sum()))
+        for g in ([0], [1, 2], [3]))
     log(f"shock leg: {out}")
     return out
 

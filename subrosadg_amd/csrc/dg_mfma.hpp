@@ -610,6 +610,8 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
       const double eps = avAt(g, g.nodq, q, e);
       for (int v = 0; v < NV; v++)
         for (int aa = 0; aa < D; aa++) fv[aa + D * v] = eps * gq[v * D + aa];
+      if constexpr (sizeof(TC) == 4)  // mixed: the energy row reduced by h0 times the density row
+        for (int aa = 0; aa < D; aa++) fv[aa + D * (NV - 1)] -= a.mix.h0 * fv[aa];
       (void)pg;
     } else {
       primitiveGradient<D>(ph, w, u, gq, pg);
@@ -633,8 +635,9 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
 // kernel (a.Vn holding TC values, with the same arithmetic as below); the face kernel then reads
 // 2 NV values per point instead of both sides' 2 NV D gradient traces (boundary points still
 // read Gf).
-// AV: Euler with artificial viscosity (FP64): Gf holds the volume-gradient traces and the viscous
-// part is the central average of eps grad U . n (SpatialDiscrete.cpp:714-735,812-830).
+// AV: Euler with artificial viscosity: Gf holds the volume-gradient traces and the viscous part is
+// the central average of eps grad U . n (SpatialDiscrete.cpp:714-735,812-830); in the mixed mode
+// its energy row is reduced by h0 times its density row like the convective part.
 template <typename TC, int D, int NF, int NFQ, bool VN = false, bool AV = false>
 __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, StageArgs a) {
   constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
@@ -655,11 +658,13 @@ __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, Sta
     boundaryFlux<D, !AV>(ph, g, bp, ub, gb, sizeof(TC) == 4, a.mix.pref, a.mix.h0, t);
     if constexpr (AV) {  // minus eps grad U . n of the element's own volume gradient
       const double eps = avAt(g, g.nodf, sl, static_cast<int>(e)), bjw = g.bjw[bp];
+      double f[NV];
       for (int v = 0; v < NV; v++) {
-        double f = 0.0;
-        for (int d = 0; d < D; d++) f += (eps * gb[v * D + d]) * g.bnormal[static_cast<int64_t>(bp) * D + d];
-        t[v] -= f * bjw;
+        f[v] = 0.0;
+        for (int d = 0; d < D; d++) f[v] += (eps * gb[v * D + d]) * g.bnormal[static_cast<int64_t>(bp) * D + d];
       }
+      if constexpr (sizeof(TC) == 4) f[NV - 1] -= a.mix.h0 * f[0];  // mixed: energy reduced by h0 mass
+      for (int v = 0; v < NV; v++) t[v] -= f[v] * bjw;
     }
     for (int v = 0; v < NV; v++) Hs[(e * NV + v) * NAQ + sl] = static_cast<TC>(-t[v]);
     return;
@@ -690,6 +695,10 @@ __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, Sta
       for (int d = 0; d < D; d++) ar += (er * static_cast<double>(Gf[(R * NG + v * D + d) * NAQ + slR])) * n[d];
       vl[v] = al;
       vr[v] = ar;
+    }
+    if constexpr (sizeof(TC) == 4) {  // mixed: the energy rows reduced by h0 times the density rows
+      vl[NV - 1] -= a.mix.h0 * vl[0];
+      vr[NV - 1] -= a.mix.h0 * vr[0];
     }
   } else if constexpr (VN) {  // the right side's outward normal is -n
     const TC* Vn = reinterpret_cast<const TC*>(a.Vn);

@@ -101,7 +101,7 @@ Launchers makeLaunchers() {
       hipLaunchKernelGGL((sdg::nsResidualMfma<TC, D, NB, NQ, NF, NFQ>), dim3(blocksFor(g.E, EPB_S)), dim3(sdg::kThreads),
                          0, st, g, ph, a);
     };
-    if constexpr (sizeof(TC) == 8) {  // Euler with artificial viscosity (FP64): volume gradient, AV fluxes
+    {  // Euler with artificial viscosity: volume gradient, AV fluxes (both precision modes)
       l.av_grad = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
         hipLaunchKernelGGL((sdg::nsGradMfma<TC, D, NB, NQ, NF, NFQ, 0>), dim3(blocksFor(g.E, EPB_G)),
                            dim3(sdg::kThreads), 0, st, g, ph, a);
@@ -972,9 +972,10 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   ctx->mixed = mixed;
   const bool tensor = problem->equation == SDG_EQ_COMPRESSIBLE_NS && problem->kernels == SDG_KERNELS_AUTO && !curved;
   ctx->av = problem->shock_capturing == SDG_SHOCK_ARTIFICIAL_VISCOSITY;
-  if (ctx->av && mixed) {
+  if (ctx->av && mixed && problem->equation != SDG_EQ_COMPRESSIBLE_EULER) {
     delete ctx;
-    return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: artificial viscosity runs in FP64");
+    return fail(nullptr, SDG_E_UNSUPPORTED,
+                "sdg_create: artificial viscosity with Navier-Stokes runs in FP64 (Euler also in mixed precision)");
   }
   if (force_vector && mixed) {
     delete ctx;
@@ -1414,10 +1415,15 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     }
     if (tensor && (allocate(ctx, nUf, &ctx->Vn) || allocate(ctx, nU, &ctx->Rvol))) return bail(SDG_E_NOMEM);
   }
-  if (ctx->av && !ctx->ns() &&
+  if (ctx->av && !ctx->ns() && !ctx->mixed &&
       (allocate(ctx, static_cast<size_t>(E) * ctx->ng * naq, &ctx->Gf) ||
        (ctx->launch.mfma ? (allocate(ctx, nUf, &ctx->Hs) || allocate(ctx, static_cast<size_t>(E) * nq * D * ctx->nv, &ctx->Fv))
                          : allocate(ctx, nU, &ctx->Rvol))))
+    return bail(SDG_E_NOMEM);
+  // mixed-precision Euler with artificial viscosity (matrix-core kernels): the FP32 work buffers
+  if (ctx->av && !ctx->ns() && ctx->mixed &&
+      (allocate(ctx, static_cast<size_t>(E) * ctx->ng * naq, &ctx->Gf32) || allocate(ctx, nUf, &ctx->Hs32) ||
+       allocate(ctx, static_cast<size_t>(E) * nq * D * ctx->nv, &ctx->Fv32)))
     return bail(SDG_E_NOMEM);
   if (ctx->av && ctx->ns() && allocate(ctx, static_cast<size_t>(E) * ctx->ng * naq, &ctx->GfV)) return bail(SDG_E_NOMEM);
   // modal Boussinesq source for the kernel sets that take it from a buffer (the sum-factorised

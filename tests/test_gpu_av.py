@@ -130,8 +130,9 @@ def test_zero_factor_matches_the_plain_euler_kernels(kind, order):
 def test_refusals(tmp_path):
     mesh = L.Mesh(L.ELEM_QUADRANGLE, 3, (4, 4), (0.0, 0.0), (1.0, 1.0))
     common = dict(cp=3.5, cv=2.5, shock_capturing=L.SHOCK_ARTIFICIAL_VISCOSITY, empirical_tolerance=0.5)
-    mixed = L.problem_desc(equation=L.EQ_EULER, precision=L.PREC_MIXED, **common)
-    with pytest.raises(L.SDGError):
+    mixed = L.problem_desc(equation=L.EQ_NS, viscous_flux=L.VISC_BR2, transport=L.TRANSPORT_CONSTANT,
+                           dynamic_viscosity=1e-3, precision=L.PREC_MIXED, **common)
+    with pytest.raises(L.SDGError):  # Navier-Stokes with artificial viscosity runs in FP64
         L.Solver(mixed, mesh)
     bad = L.problem_desc(equation=L.EQ_EULER, cp=3.5, cv=2.5, shock_capturing=L.SHOCK_ARTIFICIAL_VISCOSITY,
                          empirical_tolerance=0.0)
@@ -279,3 +280,47 @@ def test_restart_with_artificial_viscosity_is_bit_identical(tmp_path):
     r.step(dt, 4)
     assert np.array_equal(r.get_state(), s.get_state())
     assert np.array_equal(r.artificial_viscosity(), s.artificial_viscosity())
+
+
+MIXED_CASES = [("quad", 3, None, 6), ("quad", 4, BOX, 4), ("hex", 3, None, 3)]
+
+
+@pytest.mark.parametrize("kind,order,bc,n", MIXED_CASES,
+                         ids=[f"{c[0]}-p{c[1]}-{'periodic' if c[2] is None else 'bounded'}" for c in MIXED_CASES])
+def test_mixed_precision_artificial_viscosity(kind, order, bc, n):
+    """Euler with artificial viscosity in the mixed-precision mode (matrix-core kernels: FP32
+    contractions on deviations, the artificial-viscosity energy rows reduced by h0 times the density
+    rows like the convective ones, the sensor on the FP64 state): the residual agrees with the FP64
+    oracle's to FP32 contraction accuracy, and 5 steps (the horizon of the FP64 parity test above on
+    these sharp initial states) stay within 1e-6 relative L2 per variable group of the FP64
+    engine."""
+    print(f"mixed av {kind} P{order} bc={bc is not None} n={n}", flush=True)
+    mesh, prob, prim, bprim = _case(kind, order, bc, n)
+    q = L.ProblemDesc.from_buffer_copy(prob)
+    q.precision = L.PREC_MIXED
+    o, s64 = _pair(mesh, prob, prim, bprim)
+    sm = L.Solver(q, mesh)
+    sm.set_initial_primitive(prim)
+    if bprim is not None:
+        sm.set_boundary_primitive(bprim)
+    r_ref = o.residual()
+    r_mix = sm.residual()
+    nv = mesh.dim + 2
+    groups = ([0], list(range(1, mesh.dim + 1)), [mesh.dim + 1])
+    for gi in groups:
+        err = np.abs(r_mix[..., gi] - r_ref[..., gi]).max() / np.abs(r_ref[..., gi]).max()
+        assert err <= 2e-5, (gi, err)
+    dt = o.compute_dt(0.05)  # the step size of the FP64 parity test above
+    for label, s in (("fp64", s64), ("mixed", sm)):
+        _, nan, taken = s.step(dt, 5)
+        assert nan == 0 and taken == 5, label
+    # relative L2 deviation of the solution over the domain, per variable group (the modal
+    # coefficients' Euclidean norm would weight the small high Lobatto modes far beyond their share)
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    jw = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
+    ua = np.einsum("qb,ebv->eqv", V, sm.get_state())
+    ub = np.einsum("qb,ebv->eqv", V, s64.get_state())
+    dev = max(np.sqrt((jw[..., None] * (ua[..., g] - ub[..., g]) ** 2).sum() / (jw[..., None] * ub[..., g] ** 2).sum())
+              for g in groups)
+    print(f"  mixed vs FP64 after 5 steps: {dev:.2e}", flush=True)
+    assert dev <= 1e-6, dev
