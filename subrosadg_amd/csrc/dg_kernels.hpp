@@ -60,6 +60,8 @@ struct TensorOps {
   const short* slot_of;                    // [face][node] -> face point of the node's tangential position
   const short* base;                       // [face][face point] -> node with normal index 0
   const double* fwt;                       // [face][face point] face Gauss weight (tangential w w)
+  const double* nodt;                      // artificial viscosity: [node][vertex] P1 nodal functions at the
+                                           // tensor-ordered Gauss nodes (else null)
 };
 
 struct Geo {

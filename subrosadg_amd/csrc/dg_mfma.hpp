@@ -637,7 +637,9 @@ __global__ __launch_bounds__(kThreads) void nsGradMfma(Geo g, Phys ph, StageArgs
 // read Gf).
 // AV: Euler with artificial viscosity: Gf holds the volume-gradient traces and the viscous part is
 // the central average of eps grad U . n (SpatialDiscrete.cpp:714-735,812-830); in the mixed mode
-// its energy row is reduced by h0 times its density row like the convective part.
+// its energy row is reduced by h0 times its density row like the convective part. VN with AV
+// (Navier-Stokes, sum-factorised kernels): each side's Vn already holds (F_v + eps grad U) . n_out;
+// boundary points take eps grad U from the volume-gradient traces GfV.
 template <typename TC, int D, int NF, int NFQ, bool VN = false, bool AV = false>
 __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, StageArgs a) {
   constexpr int NV = D + 2, NG = NV * D, NAQ = NF * NFQ;
@@ -655,13 +657,18 @@ __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, Sta
     double ub[NV], gb[NG], t[NV];
     for (int v = 0; v < NV; v++) ub[v] = pointState<TC>(a, Uf_in[(e * NV + v) * NAQ + sl], v);
     for (int r = 0; r < NG; r++) gb[r] = Gf[(e * NG + r) * NAQ + sl];
-    boundaryFlux<D, !AV>(ph, g, bp, ub, gb, sizeof(TC) == 4, a.mix.pref, a.mix.h0, t);
+    // Euler with AV: no viscous flux; Navier-Stokes (VN) with AV: the viscous flux from Gf and the
+    // artificial term from the volume-gradient traces GfV
+    boundaryFlux<D, VN || !AV>(ph, g, bp, ub, gb, sizeof(TC) == 4, a.mix.pref, a.mix.h0, t);
     if constexpr (AV) {  // minus eps grad U . n of the element's own volume gradient
       const double eps = avAt(g, g.nodf, sl, static_cast<int>(e)), bjw = g.bjw[bp];
       double f[NV];
       for (int v = 0; v < NV; v++) {
         f[v] = 0.0;
-        for (int d = 0; d < D; d++) f[v] += (eps * gb[v * D + d]) * g.bnormal[static_cast<int64_t>(bp) * D + d];
+        for (int d = 0; d < D; d++) {
+          const double gv = VN ? a.GfV[(e * NG + v * D + d) * NAQ + sl] : gb[v * D + d];
+          f[v] += (eps * gv) * g.bnormal[static_cast<int64_t>(bp) * D + d];
+        }
       }
       if constexpr (sizeof(TC) == 4) f[NV - 1] -= a.mix.h0 * f[0];  // mixed: energy reduced by h0 mass
       for (int v = 0; v < NV; v++) t[v] -= f[v] * bjw;
@@ -687,7 +694,7 @@ __global__ __launch_bounds__(kThreads) void nsFaceFluxKernel(Geo g, Phys ph, Sta
   if constexpr (sizeof(TC) == 4) reduceFaceFlux<D>(a, n, h);
   // normal viscous flux of each side, one side at a time (keeps one side's gradient live)
   double vl[NV], vr[NV];
-  if constexpr (AV) {  // eps grad U . n of each side with the left normal
+  if constexpr (AV && !VN) {  // Euler: eps grad U . n of each side with the left normal
     const double el = avAt(g, g.nodf, slL, static_cast<int>(L)), er = avAt(g, g.nodf, slR, static_cast<int>(R));
     for (int v = 0; v < NV; v++) {
       double al = 0.0, ar = 0.0;

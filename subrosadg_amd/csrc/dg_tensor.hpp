@@ -268,8 +268,13 @@ __device__ __forceinline__ void normalViscous32(const Phys& ph, const double* u,
 // LDS per element (TC values), two regions reused across the phases:
 //   sW [NG NPT]:  nodal state Y [v][node] + alpha [v][slot]  ->  the face-gradient field [r][node]
 //   sB [NV NAQ]:  the basis-pass panel (modal, then axis-0/1 partial sums)  ->  beta [v][slot]
-template <typename TC, int N, int VISC>
+// AV (FP64, BR2): artificial viscosity eps grad U with grad U the volume gradient (the BR2
+// face-gradient field: the volume gradient of the reference, SpatialDiscrete.cpp:844-911) added to
+// the volume flux and to each side's normal viscous flux (ViscousFlux.cpp:105-190); boundary points
+// keep its trace in GfV for the boundary flux
+template <typename TC, int N, int VISC, bool AV = false>
 __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys ph, StageArgs a) {
+  static_assert(!AV || (VISC == 2 && sizeof(TC) == 8), "artificial viscosity: BR2 in FP64");
   const TensorOps& t = g.tens;
   using S = TensorShape<N>;
   constexpr int NPT = S::NPT, EPB = S::EPB, NV = S::NV, NG = S::NG, NFQ = S::NFQ, NAQ = S::NAQ, D = 3;
@@ -510,6 +515,16 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
         double pg[NG], fv[D * NV];
         primitiveGradient<D>(ph, ws, us, gs, pg);
         viscousRawFlux<D>(ph, ws, pg, fv);
+        if constexpr (AV) {  // + eps grad U with the volume-gradient trace at the slot (without the lift)
+          const double eps = avAt(g, g.nodf, sl, static_cast<int>(e));
+          for (int r = 0; r < NG; r++) {
+            double tr = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; i++) tr += lt[i] * W[r * NPT + b + i * st];
+            fv[(r % D) + D * (r / D)] += eps * tr;
+            if (nbs < 0) a.GfV[(e * NG + r) * NAQ + sl] = tr;
+          }
+        }
         for (int v = 0; v < NV; v++) {
           double sum = 0.0;
           for (int d = 0; d < D; d++) sum += fv[d + D * v] * n[d];
@@ -538,6 +553,11 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
       double pg[NG], fv[D * NV], out[D * NV];
       primitiveGradient<D>(ph, w, u, wtot, pg);
       viscousRawFlux<D>(ph, w, pg, fv);
+      if constexpr (AV) {  // + eps grad U, the volume gradient (the BR2 face-gradient field) at the node
+        double eps = 0.0;
+        for (int j = 0; j < g.nvert; j++) eps += t.nodt[q * g.nvert + j] * g.nodeav[g.ntag[e * g.nvert + j]];
+        for (int r = 0; r < NG; r++) fv[(r % D) + D * (r / D)] += eps * wface[r];
+      }
       volumeFlux<TC, D, true>(a, w, fv, G, out);
       for (int i = 0; i < D * NV; i++) W[i * NPT + q] = static_cast<TC>(out[i]);
     }

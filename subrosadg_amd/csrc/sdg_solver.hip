@@ -299,6 +299,31 @@ void useTensorKernels(Launchers& l) {
   };
   l.tensor = true;
 }
+// Navier-Stokes (BR2, FP64) with artificial viscosity on the sum-factorised kernels: the gradient
+// kernel adds eps grad U to the volume flux and to each side's normal viscous flux, boundary points
+// keep the volume-gradient trace (GfV) for the boundary flux
+template <int N>
+void useTensorAvKernels(Launchers& l) {
+  constexpr int EPB = sdg::TensorShape<N>::EPB, TPB = sdg::TensorShape<N>::TPB;
+  l.av_ns_grad[1] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+    hipLaunchKernelGGL((sdg::nsGradTensor<double, N, 2, true>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a);
+  };
+  l.ns_face = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+    constexpr int NFQ = N * N;
+    hipLaunchKernelGGL((sdg::nsFaceFluxKernel<double, 3, 6, NFQ, true, true>),
+                       dim3(blocksFor((g.F + g.Fb) * NFQ, sdg::kThreads)), dim3(sdg::kThreads), 0, st, g, ph, a);
+  };
+  l.av_ns_flux = l.ns_flux;
+}
+bool selectTensorAv(int order, Launchers& l) {
+  switch (order) {
+    case 2: useTensorAvKernels<3>(l); return true;
+    case 3: useTensorAvKernels<4>(l); return true;
+    case 4: useTensorAvKernels<5>(l); return true;
+    default: return false;
+  }
+}
+
 template <int N>
 bool pickTensor(bool mixed, Launchers& l) {
   if (mixed)
@@ -983,13 +1008,23 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   }
   // artificial viscosity: the vector kernels, or for Euler in FP64 the matrix-core kernels where the
   // element size has them (the Navier-Stokes terms need the vector kernels' second gradient trace)
-  const bool av_vector = ctx->av && problem->equation == SDG_EQ_COMPRESSIBLE_NS;
-  if (!selectLaunchers(r.type, r.order, force_vector || av_vector, mixed, ctx->launch, tensor && !ctx->av)) {
+  // Navier-Stokes with artificial viscosity: the sum-factorised kernels for BR2 in FP64 on hexahedra
+  // P2-P4 (the face-gradient field they trace is the reference's volume gradient), else the vector
+  // kernels
+  const bool tensor_av = ctx->av && tensor && !mixed && problem->viscous_flux == SDG_VISC_BR2 &&
+                         r.type == SDG_ELEM_HEXAHEDRON && r.order >= 2 && r.order <= 4;
+  const bool av_vector = ctx->av && problem->equation == SDG_EQ_COMPRESSIBLE_NS && !tensor_av;
+  if (!selectLaunchers(r.type, r.order, force_vector || av_vector, mixed, ctx->launch,
+                       tensor && (!ctx->av || tensor_av))) {
     delete ctx;
     return fail(nullptr, SDG_E_UNSUPPORTED,
                 mixed ? "sdg_create: the mixed-precision mode needs a matrix-core kernel set (quadrangle P3-P5, "
                              "hexahedron P2-P4)"
                            : "sdg_create: no kernel instantiation for this element type and order");
+  }
+  if (tensor_av && !selectTensorAv(r.order, ctx->launch)) {
+    delete ctx;
+    return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: no sum-factorised artificial-viscosity kernels for this order");
   }
   Phys& ph = ctx->phys;
   ph.eos = problem->equation_of_state;
@@ -1364,6 +1399,18 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
     g.tens.slot_of = ds;
     g.tens.base = db;
     g.tens.fwt = dw;
+    if (ctx->av) {  // P1 nodal functions at the tensor-ordered nodes (node artificial viscosity)
+      const int N = r.order + 1;
+      std::vector<double> gx, gw, nodt(static_cast<size_t>(N) * N * N * r.nvert);
+      sdg::gaussLegendre(N, gx, gw);
+      for (int q = 0; q < N * N * N; q++) {
+        const double xi[3] = {gx[q % N], gx[(q / N) % N], gx[q / (N * N)]};
+        sdg::nodalValues(r, xi, &nodt[static_cast<size_t>(q) * r.nvert]);
+      }
+      double* dn;
+      if (upload(ctx, nodt, &dn)) return bail(SDG_E_NOMEM);
+      g.tens.nodt = dn;
+    }
   }
   ctx->minedge = dme;
   ctx->h_lsq = r.lsq_inverse;
@@ -1808,10 +1855,11 @@ namespace {
 int ensureAdjoint(sdg_ctx* ctx) {
   if (ctx->adj_ready) return SDG_OK;
   if (!ctx->ns() || (ctx->mixed && !ctx->launch.tensor) || !ctx->launch.adj_volume[0] || ctx->Fb > 0 ||
-      ctx->phys.source != SDG_SOURCE_NONE)
+      ctx->phys.source != SDG_SOURCE_NONE || ctx->av)
     return fail(ctx, SDG_E_UNSUPPORTED,
                 "adjoint: implemented for Navier-Stokes on periodic meshes of the matrix-core element sizes "
-                "(quadrangle P3-P5, hexahedron P2-P4; mixed precision on hexahedra P2-P4), without a source term");
+                "(quadrangle P3-P5, hexahedron P2-P4; mixed precision on hexahedra P2-P4), without a source term or "
+                "artificial viscosity");
   const size_t E = static_cast<size_t>(ctx->E);
   const size_t nU = E * ctx->nv * ctx->nb, nS = E * ctx->nv * ctx->naq, nG = E * ctx->ng * ctx->nb,
                nGS = E * ctx->ng * ctx->naq;

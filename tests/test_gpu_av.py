@@ -74,8 +74,11 @@ def test_residual_nodes_and_steps(kind, order, bc, n):
     assert np.abs(s.artificial_viscosity() - o.artificial_viscosity()).max() <= 1e-12 * no.max()
 
 
+# hexahedra P2-P4 with BR2 run the sum-factorised kernels (the bounded case exercises the boundary
+# points' volume-gradient traces), the other cases the vector kernels
 NS_CASES = [("quad", 3, L.VISC_BR1, None), ("quad", 3, L.VISC_BR2, None), ("quad", 2, L.VISC_BR1, BOX),
-            ("hex", 2, L.VISC_BR2, None), ("hex", 2, L.VISC_BR1, HEXBOX)]
+            ("hex", 2, L.VISC_BR2, None), ("hex", 2, L.VISC_BR1, HEXBOX), ("hex", 3, L.VISC_BR2, HEXBOX),
+            ("hex", 3, L.VISC_BR2, None)]
 
 
 @pytest.mark.parametrize("kind,order,visc,bc", NS_CASES,
