@@ -268,13 +268,13 @@ __device__ __forceinline__ void normalViscous32(const Phys& ph, const double* u,
 // LDS per element (TC values), two regions reused across the phases:
 //   sW [NG NPT]:  nodal state Y [v][node] + alpha [v][slot]  ->  the face-gradient field [r][node]
 //   sB [NV NAQ]:  the basis-pass panel (modal, then axis-0/1 partial sums)  ->  beta [v][slot]
-// AV (FP64, BR2): artificial viscosity eps grad U with grad U the volume gradient (the BR2
-// face-gradient field: the volume gradient of the reference, SpatialDiscrete.cpp:844-911) added to
-// the volume flux and to each side's normal viscous flux (ViscousFlux.cpp:105-190); boundary points
-// keep its trace in GfV for the boundary flux
+// AV (FP64): artificial viscosity eps grad U with grad U the volume gradient (the gradient with the
+// faces' average traces, SpatialDiscrete.cpp:844-911: BR2's face-gradient field, the alpha lifts
+// alone under BR1) added to the volume flux and to each side's normal viscous flux
+// (ViscousFlux.cpp:105-190); boundary points keep its trace in GfV for the boundary flux
 template <typename TC, int N, int VISC, bool AV = false>
 __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys ph, StageArgs a) {
-  static_assert(!AV || (VISC == 2 && sizeof(TC) == 8), "artificial viscosity: BR2 in FP64");
+  static_assert(!AV || ((VISC == 1 || VISC == 2) && sizeof(TC) == 8), "artificial viscosity: BR1 / BR2 in FP64");
   const TensorOps& t = g.tens;
   using S = TensorShape<N>;
   constexpr int NPT = S::NPT, EPB = S::EPB, NV = S::NV, NG = S::NG, NFQ = S::NFQ, NAQ = S::NAQ, D = 3;
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
   // gradient at the node: J^-T (Dn u along each axis) + the face corrections, assembled in the
   // contraction type (the mixed mode's lifted gradient is an FP32 quantity throughout)
   double u[NV];
-  TC wtot[NG], wface[VISC == 2 ? NG : 1];
+  TC wtot[NG], wface[VISC == 2 || AV ? NG : 1];  // wface: the volume gradient (alpha lifts only)
   if (ok) {
     TC dr[NV][D];
     for (int ax = 0; ax < D; ax++) {
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
         TC s = TC(0);
         for (int ax = 0; ax < D; ax++) s += git[d + D * ax] * dr[v][ax];
         wtot[v * D + d] = s;
-        if constexpr (VISC == 2) wface[v * D + d] = s;
+        if constexpr (VISC == 2 || AV) wface[v * D + d] = s;
       }
     }
     // the outward normal at the node's face point, loaded one face ahead of its use
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
         const TC be = Bt[v * NAQ + sl] * coef;
         for (int d = 0; d < D; d++) {
           wtot[v * D + d] += (al + be) * n[d];
-          if constexpr (VISC == 2) wface[v * D + d] += al * n[d];
+          if constexpr (VISC == 2 || AV) wface[v * D + d] += al * n[d];
         }
       }
     }
@@ -443,6 +443,40 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
               s += sT.Binv[kx * N + i] * sT.Binv[ky * N + j] * sT.Binv[kz * N + l] *
                    static_cast<double>(W[r * NPT + i + N * j + N * N * l]);
         a.Wout[(e * NPT + b) * NG + r] = s;
+      }
+    }
+    elemSync<NPT>();
+  }
+  // BR1 with artificial viscosity: the AV term of each own slot's normal flux takes the volume
+  // gradient's trace, which is not the field BR1's viscous flux traces, so it goes first: eps times
+  // the trace . n per slot (NV values kept in registers) and, at boundary slots, the trace itself
+  constexpr int SI0 = (NAQ + NPT - 1) / NPT;
+  double avn[AV && VISC == 1 ? SI0 : 1][NV];
+  if constexpr (AV && VISC == 1) {
+    if (ok)
+      for (int r = 0; r < NG; r++) W[r * NPT + q] = wface[r];
+    elemSync<NPT>();
+    if (ok) {
+#pragma unroll
+      for (int k = 0; k < SI0; k++) {
+        const int sl = q + k * NPT;
+        if (sl >= NAQ) break;
+        const int s = sl / NFQ;
+        const int b = sT.base[sl], st = axisStride(sT.axis[s], N);
+        const int64_t es = e * NAQ + sl;
+        const int nbs = g.snb[es];
+        double n[D];
+        for (int d = 0; d < D; d++) n[d] = g.snrm[es * D + d];
+        const TC* lt = tabLf<TC, N>(sT) + sT.side[s] * N;
+        const double eps = avAt(g, g.nodf, sl, static_cast<int>(e));
+        for (int v = 0; v < NV; v++) avn[k][v] = 0.0;
+        for (int r = 0; r < NG; r++) {
+          double tr = 0.0;
+#pragma unroll
+          for (int i = 0; i < N; i++) tr += lt[i] * W[r * NPT + b + i * st];
+          avn[k][r / D] += eps * tr * n[r % D];
+          if (nbs < 0) a.GfV[(e * NG + r) * NAQ + sl] = tr;
+        }
       }
     }
     elemSync<NPT>();
@@ -515,7 +549,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
         double pg[NG], fv[D * NV];
         primitiveGradient<D>(ph, ws, us, gs, pg);
         viscousRawFlux<D>(ph, ws, pg, fv);
-        if constexpr (AV) {  // + eps grad U with the volume-gradient trace at the slot (without the lift)
+        if constexpr (AV && VISC == 2) {  // + eps grad U with the volume-gradient trace at the slot (without the lift)
           const double eps = avAt(g, g.nodf, sl, static_cast<int>(e));
           for (int r = 0; r < NG; r++) {
             double tr = 0.0;
@@ -528,6 +562,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
         for (int v = 0; v < NV; v++) {
           double sum = 0.0;
           for (int d = 0; d < D; d++) sum += fv[d + D * v] * n[d];
+          if constexpr (AV && VISC == 1) sum += avn[k][v];
           Vn[(e * NV + v) * NAQ + sl] = static_cast<TC>(sum);
         }
       }
@@ -553,7 +588,7 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
       double pg[NG], fv[D * NV], out[D * NV];
       primitiveGradient<D>(ph, w, u, wtot, pg);
       viscousRawFlux<D>(ph, w, pg, fv);
-      if constexpr (AV) {  // + eps grad U, the volume gradient (the BR2 face-gradient field) at the node
+      if constexpr (AV) {  // + eps grad U, the volume gradient (alpha lifts only) at the node
         double eps = 0.0;
         for (int j = 0; j < g.nvert; j++) eps += t.nodt[q * g.nvert + j] * g.nodeav[g.ntag[e * g.nvert + j]];
         for (int r = 0; r < NG; r++) fv[(r % D) + D * (r / D)] += eps * wface[r];

@@ -299,12 +299,15 @@ void useTensorKernels(Launchers& l) {
   };
   l.tensor = true;
 }
-// Navier-Stokes (BR2, FP64) with artificial viscosity on the sum-factorised kernels: the gradient
-// kernel adds eps grad U to the volume flux and to each side's normal viscous flux, boundary points
-// keep the volume-gradient trace (GfV) for the boundary flux
+// Navier-Stokes (BR1 or BR2, FP64) with artificial viscosity on the sum-factorised kernels: the
+// gradient kernel adds eps grad U (the volume gradient) to the volume flux and to each side's normal
+// viscous flux, boundary points keep the volume-gradient trace (GfV) for the boundary flux
 template <int N>
 void useTensorAvKernels(Launchers& l) {
   constexpr int EPB = sdg::TensorShape<N>::EPB, TPB = sdg::TensorShape<N>::TPB;
+  l.av_ns_grad[0] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
+    hipLaunchKernelGGL((sdg::nsGradTensor<double, N, 1, true>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a);
+  };
   l.av_ns_grad[1] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
     hipLaunchKernelGGL((sdg::nsGradTensor<double, N, 2, true>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a);
   };
@@ -1008,10 +1011,10 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   }
   // artificial viscosity: the vector kernels, or for Euler in FP64 the matrix-core kernels where the
   // element size has them (the Navier-Stokes terms need the vector kernels' second gradient trace)
-  // Navier-Stokes with artificial viscosity: the sum-factorised kernels for BR2 in FP64 on hexahedra
-  // P2-P4 (the face-gradient field they trace is the reference's volume gradient), else the vector
-  // kernels
-  const bool tensor_av = ctx->av && tensor && !mixed && problem->viscous_flux == SDG_VISC_BR2 &&
+  // Navier-Stokes with artificial viscosity: the sum-factorised kernels for BR1 and BR2 in FP64 on
+  // hexahedra P2-P4, else the vector kernels
+  const bool tensor_av = ctx->av && tensor && !mixed &&
+                         (problem->viscous_flux == SDG_VISC_BR1 || problem->viscous_flux == SDG_VISC_BR2) &&
                          r.type == SDG_ELEM_HEXAHEDRON && r.order >= 2 && r.order <= 4;
   const bool av_vector = ctx->av && problem->equation == SDG_EQ_COMPRESSIBLE_NS && !tensor_av;
   if (!selectLaunchers(r.type, r.order, force_vector || av_vector, mixed, ctx->launch,

@@ -74,11 +74,12 @@ def test_residual_nodes_and_steps(kind, order, bc, n):
     assert np.abs(s.artificial_viscosity() - o.artificial_viscosity()).max() <= 1e-12 * no.max()
 
 
-# hexahedra P2-P4 with BR2 run the sum-factorised kernels (the bounded case exercises the boundary
-# points' volume-gradient traces), the other cases the vector kernels
+# hexahedra P2-P4 run the sum-factorised kernels (the bounded cases exercise the boundary points'
+# volume-gradient traces), the other cases the vector kernels
 NS_CASES = [("quad", 3, L.VISC_BR1, None), ("quad", 3, L.VISC_BR2, None), ("quad", 2, L.VISC_BR1, BOX),
             ("hex", 2, L.VISC_BR2, None), ("hex", 2, L.VISC_BR1, HEXBOX), ("hex", 3, L.VISC_BR2, HEXBOX),
-            ("hex", 3, L.VISC_BR2, None), ("hex", 2, L.VISC_BR2, HEXBOX)]
+            ("hex", 3, L.VISC_BR2, None), ("hex", 2, L.VISC_BR2, HEXBOX), ("hex", 3, L.VISC_BR1, None),
+            ("hex", 3, L.VISC_BR1, HEXBOX)]
 
 
 @pytest.mark.parametrize("kind,order,visc,bc", NS_CASES,
@@ -109,37 +110,53 @@ def test_navier_stokes_with_artificial_viscosity(kind, order, visc, bc):
     P.fp64_parity(s.get_state(), uo, x.get_state(), uo, mesh.dim, tol=TOL)
 
 
-@pytest.mark.parametrize("order", [2, 3])
-def test_navier_stokes_av_sum_factorised_matches_vector_kernels(order):
-    """Hexahedra with BR2: the sum-factorised NS + AV kernels against the vector kernels on the same
-    bounded case (node values, residual, 5 steps), two routes to the same discrete operator."""
-    print(f"ns+av sum-factorised vs vector hex P{order}", flush=True)
+@pytest.mark.parametrize("order,visc", [(2, L.VISC_BR2), (3, L.VISC_BR2), (4, L.VISC_BR2), (2, L.VISC_BR1),
+                                        (3, L.VISC_BR1), (4, L.VISC_BR1)])
+def test_navier_stokes_av_sum_factorised_matches_vector_kernels(order, visc):
+    """Hexahedra: the sum-factorised NS + AV kernels and the vector kernels on the same bounded case,
+    two routes to the same discrete operator: node values and residual against each other at 1e-12,
+    residual and 5 steps of each against the oracle at the FP64 bar (the extended-precision build
+    arbitrates where the FP64 oracle's own rounding is above it). At P4 the projection of Sod's jump
+    overshoots to negative density, so P4 takes a milder jump (rho 1 -> 0.8, T 1 -> 0.9) inside and
+    at the boundary. On this P4 state the sensor amplifies rounding: after the steps the FP64
+    oracle's own distance from the extended build is 0.7-1.4e-11 (4.5e-13 for the residual), and
+    both kernel routes lie 1.8-2.3e-11 from it, so the P4 steps are held to 1e-10."""
+    print(f"ns+av sum-factorised vs vector hex P{order} visc={visc}", flush=True)
+    steps, step_tol = 5, (TOL if order < 4 else 1e-10)
     mesh, _, prim, bprim = _case("hex", order, HEXBOX, 3)
-    res, states, nodes = [], [], []
+    if order == 4:
+        for pr in (prim, bprim):
+            mild = pr[..., 0] < 0.5
+            pr[..., 0] = np.where(mild, 0.8, pr[..., 0])
+            pr[..., -1] = np.where(mild, 0.9, pr[..., -1])
+    prob = L.problem_desc(equation=L.EQ_NS, convective_flux=L.FLUX_HLLC, viscous_flux=visc,
+                          transport=L.TRANSPORT_CONSTANT, dynamic_viscosity=1e-3, cp=3.5, cv=2.5,
+                          shock_capturing=L.SHOCK_ARTIFICIAL_VISCOSITY, empirical_tolerance=0.5)
+    o, s0 = _pair(mesh, prob, prim, bprim)
+    x, _ = _pair(mesh, prob, prim, bprim, extended=True)
+    x.set_state(o.get_state())
+    u0 = o.get_state()
+    dt = o.compute_dt(0.05)
+    ro, rx = o.residual(), x.residual()
+    o.step(dt, steps)
+    x.step(dt, steps)
+    uo, ux = o.get_state(), x.get_state()
+    s0.close()
+    res, nodes = [], []
     for kern in (L.KERNELS_AUTO, L.KERNELS_VECTOR):
-        prob = L.problem_desc(equation=L.EQ_NS, convective_flux=L.FLUX_HLLC, viscous_flux=L.VISC_BR2,
-                              transport=L.TRANSPORT_CONSTANT, dynamic_viscosity=1e-3, cp=3.5, cv=2.5,
-                              shock_capturing=L.SHOCK_ARTIFICIAL_VISCOSITY, empirical_tolerance=0.5)
-        prob.kernels = kern
-        s = L.Solver(prob, mesh)
-        s.set_initial_primitive(prim)
-        if bprim is not None:
-            s.set_boundary_primitive(bprim)
-        if kern == L.KERNELS_AUTO:
-            u0 = s.get_state()
-            dt = s.compute_dt(0.05)
-        else:
-            s.set_state(u0)
+        s = L.Solver(P.with_kernels(prob, kern), mesh)
+        s.set_state(u0)
+        s.set_boundary_primitive(bprim)
         res.append(s.residual())
         nodes.append(s.artificial_viscosity())
-        _, nan, taken = s.step(dt, 5)
-        assert nan == 0 and taken == 5
-        states.append(s.get_state())
+        P.fp64_parity(res[-1], ro, rx, u0, mesh.dim, tol=TOL)
+        _, nan, taken = s.step(dt, steps)
+        assert nan == 0 and taken == steps
+        P.fp64_parity(s.get_state(), uo, ux, uo, mesh.dim, tol=step_tol)
         s.close()
     assert nodes[0].max() > 0.0
     assert np.abs(nodes[0] - nodes[1]).max() <= 1e-13 * nodes[1].max()
     assert P.parity_error(res[0], res[1], u0, mesh.dim) <= 1e-12
-    assert P.parity_error(states[0], states[1], u0, mesh.dim) <= 1e-12
 
 
 @pytest.mark.parametrize("kind,order", [("quad", 3), ("quad", 4), ("hex", 2)])
