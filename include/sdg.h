@@ -76,7 +76,8 @@ enum sdg_source_term { SDG_SOURCE_NONE = 0, SDG_SOURCE_BOUSSINESQ = 1 };
  * the maximum over the elements that share the node, and the P1 interpolant of the node values
  * times the volume gradient of the conserved variables is subtracted from the fluxes (central
  * average on interior faces, the element's own value on boundary faces). Euler and Navier-Stokes
- * (FP64, single problems; with Navier-Stokes the viscous flux keeps its own lifted gradient). */
+ * (single problems; with Navier-Stokes the viscous flux keeps its own lifted gradient), in FP64 and,
+ * for Euler on the matrix-core sizes and Navier-Stokes on hexahedra P2-P4, in the mixed mode. */
 enum sdg_shock_capturing { SDG_SHOCK_NONE = 0, SDG_SHOCK_ARTIFICIAL_VISCOSITY = 1 };
 /* InitialConditionEnum (Enum.cpp), same order: the file-based starts of sdg_raw_binary_read */
 enum sdg_initial_condition { SDG_IC_FUNCTION = 0, SDG_IC_SPECIFIC_FILE = 1, SDG_IC_LAST_STEP = 2 };

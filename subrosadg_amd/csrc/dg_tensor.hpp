@@ -208,9 +208,12 @@ __device__ __forceinline__ int localElement(int tid) {
 // Ma 0.1), so do the deviations p - pref and rho H - h0 rho of the momentum and energy fluxes;
 // the primitive gradient, the viscous flux and the contraction with (J^T)^-1 |J| w are FP32.
 // out[b NV + v] = sum_a (F_c - F_v)[a][v] G[a][b] as volumeFlux.
+// avg (optional): the artificial term eps grad U per gradient component, added to F_v (its energy
+// row reduced by h0 times its density row like the convective one)
 template <int D>
 __device__ __forceinline__ void volumeFlux32(const Phys& ph, const StageArgs& a, const double* u, const double* w,
-                                             const float* g, const double* G, float* out) {
+                                             const float* g, const double* G, float* out,
+                                             const float* avg = nullptr) {
   constexpr int NV = D + 2, NG = NV * D;
   float wf[D + 3], uf[NV], pg[NG], fv[D * NV], fc[D * NV];
 #pragma unroll
@@ -220,6 +223,9 @@ __device__ __forceinline__ void volumeFlux32(const Phys& ph, const StageArgs& a,
   for (int v = 0; v < NV; v++) uf[v] = static_cast<float>(u[v]);
   primitiveGradient<D>(ph, wf, uf, g, pg);
   viscousRawFlux<D>(ph, wf, pg, fv);
+  if (avg)
+#pragma unroll
+    for (int r = 0; r < NG; r++) fv[(r % D) + D * (r / D)] += avg[r];
   const double rho = w[0], et = w[D + 1] + velocitySquared<D>(w) / 2.0;
   const float pdev = static_cast<float>(w[D + 2] - a.mix.pref);
   const float hdev = static_cast<float>(rho * et + w[D + 2] - a.mix.h0 * rho);
@@ -274,7 +280,7 @@ __device__ __forceinline__ void normalViscous32(const Phys& ph, const double* u,
 // (ViscousFlux.cpp:105-190); boundary points keep its trace in GfV for the boundary flux
 template <typename TC, int N, int VISC, bool AV = false>
 __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys ph, StageArgs a) {
-  static_assert(!AV || ((VISC == 1 || VISC == 2) && sizeof(TC) == 8), "artificial viscosity: BR1 / BR2 in FP64");
+  static_assert(!AV || VISC == 1 || VISC == 2, "artificial viscosity: BR1 / BR2");
   const TensorOps& t = g.tens;
   using S = TensorShape<N>;
   constexpr int NPT = S::NPT, EPB = S::EPB, NV = S::NV, NG = S::NG, NFQ = S::NFQ, NAQ = S::NAQ, D = 3;
@@ -544,6 +550,24 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
       if constexpr (sizeof(TC) == 4) {
         float vn[NV];
         normalViscous32<D>(ph, us, ws, gt, n, vn);
+        if constexpr (AV) {  // + eps grad U . n_out of the volume gradient, energy reduced by h0 mass
+          double av[NV];
+          if constexpr (VISC == 2) {
+            const double eps = avAt(g, g.nodf, sl, static_cast<int>(e));
+            for (int v = 0; v < NV; v++) av[v] = 0.0;
+            for (int r = 0; r < NG; r++) {
+              double tr = 0.0;
+#pragma unroll
+              for (int i = 0; i < N; i++) tr += static_cast<double>(lt[i] * W[r * NPT + b + i * st]);
+              av[r / D] += eps * tr * n[r % D];
+              if (nbs < 0) a.GfV[(e * NG + r) * NAQ + sl] = tr;
+            }
+          } else {
+            for (int v = 0; v < NV; v++) av[v] = avn[k][v];
+          }
+          av[NV - 1] -= a.mix.h0 * av[0];
+          for (int v = 0; v < NV; v++) vn[v] = static_cast<float>(static_cast<double>(vn[v]) + av[v]);
+        }
         for (int v = 0; v < NV; v++) Vn[(e * NV + v) * NAQ + sl] = vn[v];
       } else {
         double pg[NG], fv[D * NV];
@@ -582,7 +606,19 @@ __global__ __launch_bounds__(TensorShape<N>::TPB) void nsGradTensor(Geo g, Phys 
     for (int i = 0; i < D * D; i++) G[i] = gi[i] * jw;
     if constexpr (sizeof(TC) == 4) {
       float out[D * NV];
-      volumeFlux32<D>(ph, a, u, w, wtot, G, out);
+      if constexpr (AV) {  // eps grad U of the volume gradient, energy row reduced by h0 mass
+        double eps = 0.0;
+        for (int j = 0; j < g.nvert; j++) eps += t.nodt[q * g.nvert + j] * g.nodeav[g.ntag[e * g.nvert + j]];
+        float avg[NG];
+        for (int r = 0; r < NG; r++) {
+          double gv = static_cast<double>(wface[r]);
+          if (r / D == NV - 1) gv -= a.mix.h0 * static_cast<double>(wface[r % D]);
+          avg[r] = static_cast<float>(eps * gv);
+        }
+        volumeFlux32<D>(ph, a, u, w, wtot, G, out, avg);
+      } else {
+        volumeFlux32<D>(ph, a, u, w, wtot, G, out);
+      }
       for (int i = 0; i < D * NV; i++) W[i * NPT + q] = out[i];
     } else {
       double pg[NG], fv[D * NV], out[D * NV];

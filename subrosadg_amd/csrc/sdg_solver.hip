@@ -299,32 +299,36 @@ void useTensorKernels(Launchers& l) {
   };
   l.tensor = true;
 }
-// Navier-Stokes (BR1 or BR2, FP64) with artificial viscosity on the sum-factorised kernels: the
+// Navier-Stokes (BR1 or BR2, FP64 or mixed) with artificial viscosity on the sum-factorised kernels: the
 // gradient kernel adds eps grad U (the volume gradient) to the volume flux and to each side's normal
 // viscous flux, boundary points keep the volume-gradient trace (GfV) for the boundary flux
-template <int N>
+template <typename TC, int N>
 void useTensorAvKernels(Launchers& l) {
   constexpr int EPB = sdg::TensorShape<N>::EPB, TPB = sdg::TensorShape<N>::TPB;
   l.av_ns_grad[0] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-    hipLaunchKernelGGL((sdg::nsGradTensor<double, N, 1, true>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a);
+    hipLaunchKernelGGL((sdg::nsGradTensor<TC, N, 1, true>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a);
   };
   l.av_ns_grad[1] = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
-    hipLaunchKernelGGL((sdg::nsGradTensor<double, N, 2, true>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a);
+    hipLaunchKernelGGL((sdg::nsGradTensor<TC, N, 2, true>), dim3(blocksFor(g.E, EPB)), dim3(TPB), 0, st, g, ph, a);
   };
   l.ns_face = [](hipStream_t st, const Geo& g, const Phys& ph, const StageArgs& a) {
     constexpr int NFQ = N * N;
-    hipLaunchKernelGGL((sdg::nsFaceFluxKernel<double, 3, 6, NFQ, true, true>),
+    hipLaunchKernelGGL((sdg::nsFaceFluxKernel<TC, 3, 6, NFQ, true, true>),
                        dim3(blocksFor((g.F + g.Fb) * NFQ, sdg::kThreads)), dim3(sdg::kThreads), 0, st, g, ph, a);
   };
   l.av_ns_flux = l.ns_flux;
 }
-bool selectTensorAv(int order, Launchers& l) {
+template <typename TC>
+bool selectTensorAvT(int order, Launchers& l) {
   switch (order) {
-    case 2: useTensorAvKernels<3>(l); return true;
-    case 3: useTensorAvKernels<4>(l); return true;
-    case 4: useTensorAvKernels<5>(l); return true;
+    case 2: useTensorAvKernels<TC, 3>(l); return true;
+    case 3: useTensorAvKernels<TC, 4>(l); return true;
+    case 4: useTensorAvKernels<TC, 5>(l); return true;
     default: return false;
   }
+}
+bool selectTensorAv(int order, bool mixed, Launchers& l) {
+  return mixed ? selectTensorAvT<float>(order, l) : selectTensorAvT<double>(order, l);
 }
 
 template <int N>
@@ -1000,10 +1004,13 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   ctx->mixed = mixed;
   const bool tensor = problem->equation == SDG_EQ_COMPRESSIBLE_NS && problem->kernels == SDG_KERNELS_AUTO && !curved;
   ctx->av = problem->shock_capturing == SDG_SHOCK_ARTIFICIAL_VISCOSITY;
-  if (ctx->av && mixed && problem->equation != SDG_EQ_COMPRESSIBLE_EULER) {
+  const bool tensor_size = r.type == SDG_ELEM_HEXAHEDRON && r.order >= 2 && r.order <= 4 &&
+                           (problem->viscous_flux == SDG_VISC_BR1 || problem->viscous_flux == SDG_VISC_BR2);
+  if (ctx->av && mixed && problem->equation != SDG_EQ_COMPRESSIBLE_EULER && !(tensor && tensor_size)) {
     delete ctx;
     return fail(nullptr, SDG_E_UNSUPPORTED,
-                "sdg_create: artificial viscosity with Navier-Stokes runs in FP64 (Euler also in mixed precision)");
+                "sdg_create: artificial viscosity with Navier-Stokes in mixed precision runs on the sum-factorised "
+                "kernels (hexahedra P2-P4, BR1 / BR2); elsewhere it runs in FP64");
   }
   if (force_vector && mixed) {
     delete ctx;
@@ -1011,11 +1018,9 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
   }
   // artificial viscosity: the vector kernels, or for Euler in FP64 the matrix-core kernels where the
   // element size has them (the Navier-Stokes terms need the vector kernels' second gradient trace)
-  // Navier-Stokes with artificial viscosity: the sum-factorised kernels for BR1 and BR2 in FP64 on
-  // hexahedra P2-P4, else the vector kernels
-  const bool tensor_av = ctx->av && tensor && !mixed &&
-                         (problem->viscous_flux == SDG_VISC_BR1 || problem->viscous_flux == SDG_VISC_BR2) &&
-                         r.type == SDG_ELEM_HEXAHEDRON && r.order >= 2 && r.order <= 4;
+  // Navier-Stokes with artificial viscosity: the sum-factorised kernels for BR1 and BR2 on hexahedra
+  // P2-P4 (FP64 or mixed), else the FP64 vector kernels
+  const bool tensor_av = ctx->av && tensor && tensor_size;
   const bool av_vector = ctx->av && problem->equation == SDG_EQ_COMPRESSIBLE_NS && !tensor_av;
   if (!selectLaunchers(r.type, r.order, force_vector || av_vector, mixed, ctx->launch,
                        tensor && (!ctx->av || tensor_av))) {
@@ -1025,7 +1030,7 @@ int sdg_create(const sdg_problem_desc* problem, const sdg_mesh* mesh_handle, sdg
                              "hexahedron P2-P4)"
                            : "sdg_create: no kernel instantiation for this element type and order");
   }
-  if (tensor_av && !selectTensorAv(r.order, ctx->launch)) {
+  if (tensor_av && !selectTensorAv(r.order, mixed, ctx->launch)) {
     delete ctx;
     return fail(nullptr, SDG_E_UNSUPPORTED, "sdg_create: no sum-factorised artificial-viscosity kernels for this order");
   }

@@ -185,7 +185,7 @@ def test_refusals(tmp_path):
     common = dict(cp=3.5, cv=2.5, shock_capturing=L.SHOCK_ARTIFICIAL_VISCOSITY, empirical_tolerance=0.5)
     mixed = L.problem_desc(equation=L.EQ_NS, viscous_flux=L.VISC_BR2, transport=L.TRANSPORT_CONSTANT,
                            dynamic_viscosity=1e-3, precision=L.PREC_MIXED, **common)
-    with pytest.raises(L.SDGError):  # Navier-Stokes with artificial viscosity runs in FP64
+    with pytest.raises(L.SDGError):  # NS + AV in mixed precision: sum-factorised hexahedra P2-P4 only
         L.Solver(mixed, mesh)
     bad = L.problem_desc(equation=L.EQ_EULER, cp=3.5, cv=2.5, shock_capturing=L.SHOCK_ARTIFICIAL_VISCOSITY,
                          empirical_tolerance=0.0)
@@ -369,6 +369,55 @@ def test_mixed_precision_artificial_viscosity(kind, order, bc, n):
         assert nan == 0 and taken == 5, label
     # relative L2 deviation of the solution over the domain, per variable group (the modal
     # coefficients' Euclidean norm would weight the small high Lobatto modes far beyond their share)
+    V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
+    jw = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
+    ua = np.einsum("qb,ebv->eqv", V, sm.get_state())
+    ub = np.einsum("qb,ebv->eqv", V, s64.get_state())
+    dev = max(np.sqrt((jw[..., None] * (ua[..., g] - ub[..., g]) ** 2).sum() / (jw[..., None] * ub[..., g] ** 2).sum())
+              for g in groups)
+    print(f"  mixed vs FP64 after 5 steps: {dev:.2e}", flush=True)
+    assert dev <= 1e-6, dev
+
+
+NS_MIXED_CASES = [(3, L.VISC_BR2, None), (3, L.VISC_BR1, HEXBOX), (2, L.VISC_BR2, HEXBOX), (4, L.VISC_BR1, None)]
+
+
+@pytest.mark.parametrize("order,visc,bc", NS_MIXED_CASES,
+                         ids=[f"hex-p{c[0]}-br{c[1]}-{'periodic' if c[2] is None else 'bounded'}" for c in NS_MIXED_CASES])
+def test_mixed_precision_navier_stokes_artificial_viscosity(order, visc, bc):
+    """Navier-Stokes with artificial viscosity in the mixed-precision mode (sum-factorised hexahedron
+    kernels: FP32 artificial terms on the volume gradient, their energy rows reduced by h0 times the
+    density rows like the convective ones, the sensor on the FP64 state): 5 steps stay within 1e-6
+    relative L2 per variable group of the FP64 engine. P4 takes the milder jump of the P4 kernel
+    cross-check (the projection of Sod's jump overshoots to negative density there)."""
+    print(f"mixed ns+av hex P{order} visc={visc} bc={bc is not None}", flush=True)
+    mesh, _, prim, bprim = _case("hex", order, bc, 3)
+    if order == 4:
+        for pr in (prim, bprim):
+            if pr is None:
+                continue
+            mild = pr[..., 0] < 0.5
+            pr[..., 0] = np.where(mild, 0.8, pr[..., 0])
+            pr[..., -1] = np.where(mild, 0.9, pr[..., -1])
+    prob = L.problem_desc(equation=L.EQ_NS, convective_flux=L.FLUX_HLLC, viscous_flux=visc,
+                          transport=L.TRANSPORT_CONSTANT, dynamic_viscosity=1e-3, cp=3.5, cv=2.5,
+                          shock_capturing=L.SHOCK_ARTIFICIAL_VISCOSITY, empirical_tolerance=0.5)
+    q = L.ProblemDesc.from_buffer_copy(prob)
+    q.precision = L.PREC_MIXED
+    runs = []
+    for pd in (prob, q):
+        s = L.Solver(pd, mesh)
+        s.set_initial_primitive(prim)
+        if bprim is not None:
+            s.set_boundary_primitive(bprim)
+        runs.append(s)
+    s64, sm = runs
+    dt = s64.compute_dt(0.05)
+    for label, s in (("fp64", s64), ("mixed", sm)):
+        _, nan, taken = s.step(dt, 5)
+        assert nan == 0 and taken == 5, label
+    assert sm.artificial_viscosity().max() > 0.0  # the sensor of the last step
+    groups = ([0], list(range(1, mesh.dim + 1)), [mesh.dim + 1])
     V = mesh.field("basis_value").reshape(mesh.nq, mesh.nb)
     jw = mesh.field("jacobian_det_w").reshape(mesh.E, mesh.nq)
     ua = np.einsum("qb,ebv->eqv", V, sm.get_state())
